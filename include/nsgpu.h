@@ -1,0 +1,235 @@
+/*
+ * nsgpu.h - C ABI of the MI355X (gfx950) grid ray-casting surface builder.
+ *
+ * This is the drop-in boundary for NanoShaper's grid ray caster.  Every entry
+ * point takes plain pointers and sizes; no C++ or torch types cross it.  The
+ * host side of the reference (config parsing, atom loading, the SES patch
+ * construction) runs on the CPU and hands the patch arrays over with
+ * nsgpu_set_patches(); everything from there to the maps, the cavity labels and
+ * the marching-cubes mesh runs in HIP kernels on the device.
+ *
+ * Citations name the reference interface each entry point replaces
+ * (paths relative to the reference's src/ directory).
+ *
+ * Return convention: 0 on success, a negative NSGPU_E* code on failure; the
+ * message of the last failure is available from nsgpu_last_error().
+ */
+#ifndef NSGPU_H
+#define NSGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NSGPU_ABI_VERSION 1
+
+/* error codes */
+#define NSGPU_OK 0
+#define NSGPU_E_ARG -1      /* invalid argument (null pointer, bad size)        */
+#define NSGPU_E_DEVICE -2   /* HIP runtime error                                 */
+#define NSGPU_E_STATE -3    /* call order violated (e.g. get_surf before grid)   */
+#define NSGPU_E_NOMEM -4    /* device or host allocation failed                  */
+#define NSGPU_E_BUILD -5    /* host patch construction failed                    */
+
+/* DelPhi map codes (DelphiShared.h:21-33, ConnollySurface.cpp:214). */
+#define NSGPU_STATUS_INSIDE -1
+#define NSGPU_STATUS_TEMP_OUT 2
+#define NSGPU_STATUS_OUT 3
+#define NSGPU_STATUS_FIRST_CAV 4
+
+/* ------------------------------------------------------------------------ */
+/* Grid descriptor: the result of DelPhiShared::buildGrid                    */
+/* (DelphiShared.cpp:1038-1201).  Node i sits at x[i] = xmin + i*side.       */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int32_t nx, ny, nz;
+    int32_t pad_;
+    double xmin, ymin, zmin;
+    double xmax, ymax, zmax;
+    double scale;  /* grid points per Angstrom                  */
+    double side;   /* 1/scale                                   */
+    double hside;  /* side/2                                    */
+    double A;      /* side*side, pixel area of one ray           */
+    double rmaxdim;
+    double baricenter[3];
+} nsgpu_grid;
+
+/* ------------------------------------------------------------------------ */
+/* Patch set (SoA).  One entry per SES cell of ConnollySurface              */
+/* (ConnollySurface.h:31-167).  Each patch is a primitive (sphere or torus)  */
+/* plus a feasibility region; a ray/primitive hit is kept when the hit point */
+/* is feasible (ConnollySurface::isFeasible, ConnollySurface.cpp:4769-5021). */
+/*                                                                          */
+/* All clipping tests are normalised to "value > 0 (or < 0) rejects":       */
+/*   POINT        data[off + 4c .. +3] = clipping sphere (cx,cy,cz,r^2);    */
+/*                hit rejected when |p-c|^2 - r^2 < 0.                     */
+/*   FACET_REG/   data[off + 4c .. +3] = plane (a,b,c,d);                  */
+/*   FACET_SING   hit rejected when a*x+b*y+c*z+d > 0.  Singular facets     */
+/*                carry the 4 trimming planes, regular facets planes 1..3,  */
+/*                both followed by their self-intersection planes with the  */
+/*                sign already applied.                                    */
+/*   TORUS_REG/   data[off .. off+13] = torus centre (3), inverse frame     */
+/*   TORUS_SING   rotation (9, row major), major radius R, self-            */
+/*                intersection radius (<0 = none); then cnt pairs of planes */
+/*                (8 doubles per pair) with the sign already applied;       */
+/*                flags bit c = pair c is "acute" (AND) rather than OR.     */
+/*                Singular tori have cnt = 0.                               */
+/* bsph[4*p..] is the bounding sphere intersected first: the probe sphere   */
+/* (facets), the torus clipping sphere (tori) or the atom (points).          */
+/* The torus minor radius and the facet sphere radius are the probe radius. */
+/* ------------------------------------------------------------------------ */
+#define NSGPU_PATCH_POINT 0
+#define NSGPU_PATCH_FACET_REG 1
+#define NSGPU_PATCH_FACET_SING 2
+#define NSGPU_PATCH_TORUS_REG 3
+#define NSGPU_PATCH_TORUS_SING 4
+#define NSGPU_TORUS_HDR 14
+
+typedef struct {
+    int32_t n;           /* number of patches                           */
+    int32_t n_data;      /* doubles in data[]                            */
+    const int32_t *kind; /* [n]   NSGPU_PATCH_*                          */
+    const int32_t *off;  /* [n]   offset into data[]                     */
+    const int32_t *cnt;  /* [n]   clip items (spheres/planes/plane pairs) */
+    const int32_t *flags;/* [n]   torus: acute bit per plane pair         */
+    const int32_t *atom; /* [n*3] atom ids (point: id,-1,-1; torus: i,j,-1; facet: i,j,k) */
+    const double *bsph;  /* [n*4] bounding sphere                        */
+    const double *data;  /* [n_data] pool                                */
+} nsgpu_patches;
+
+/* ------------------------------------------------------------------------ */
+/* Surface::getSurf options (Surface.h:890) and results.                    */
+/* ------------------------------------------------------------------------ */
+#define NSGPU_PREC_FP64 0  /* reference arithmetic throughout                         */
+#define NSGPU_PREC_MIXED 1 /* FP32 candidate cull, FP64 roots/feasibility/depths     */
+
+typedef struct {
+    int32_t build_epsmap;           /* DelPhiShared::buildEpsMap (epsmap + idebmap)      */
+    int32_t build_status;           /* DelPhiShared::buildStatus (status map)            */
+    int32_t accurate_triangulation; /* Surface::accurateTriangulation (edge-ray panels)  */
+    int32_t fill_cavities;          /* getSurf(..., fillCav, cav_vol)                    */
+    double cav_vol;                 /* Conditional_Volume_Filling_Value                  */
+    int32_t inside;                 /* epsmap inside code (5 for ses)                    */
+    int32_t precision;              /* NSGPU_PREC_*                                      */
+} nsgpu_surf_opts;
+
+typedef struct {
+    double volume;         /* Surface::getSurf *surf_volume (Surface.cpp:1677-1684)   */
+    int64_t total_rays;    /* rays traced over all panels and phases                  */
+    int64_t odd_rays;      /* rays with an odd hit count (dropped, as the reference)  */
+    int64_t total_hits;    /* feasible ray/patch hits                                 */
+    int64_t candidates;    /* ray/patch pairs tested after bounding-box culling       */
+    int32_t num_cavities;  /* getCavitiesWithBilevelStatusMap() return value          */
+    int32_t max_hits_per_ray;
+} nsgpu_surf_result;
+
+typedef struct {
+    int64_t num_vertices;
+    int64_t num_triangles;
+    int64_t num_analytic_vertices; /* vertices from edge-ray hits (assembleVerticesList) */
+    double area;                   /* triangulateSurface return value                  */
+} nsgpu_mesh_info;
+
+typedef struct nsgpu_ctx nsgpu_ctx;
+
+/* -------------------- context --------------------------------------------- */
+/* One context = one DelPhiShared grid + one Surface on one device.           */
+/* Calls on one context are not thread safe; distinct contexts are.           */
+int nsgpu_create(int device, nsgpu_ctx **out);
+void nsgpu_destroy(nsgpu_ctx *ctx);
+const char *nsgpu_last_error(const nsgpu_ctx *ctx);
+int nsgpu_abi_version(void);
+
+/* -------------------- host stages that stay on the CPU --------------------- */
+/* DelPhiShared::buildGrid(scale, perfil) (DelphiShared.cpp:1038-1201),
+ * bounds from DelPhiShared::getBounds (DelphiShared.cpp:775-797).
+ * xyzr: n*4 doubles (x, y, z, radius). */
+int nsgpu_build_grid(const double *xyzr, int32_t n, double scale, double perfil,
+                     nsgpu_grid *out);
+
+/* Atom jitter of ConnollySurface::buildConnollyCGAL (ConnollySurface.cpp:2484-2489)
+ * with the Park-Miller generator randnum() (tools.cpp:64-84).  *seed is the
+ * generator state (the reference starts at SEED = 1234, tools.h:1827) and is
+ * advanced in place. */
+void nsgpu_jitter(double *xyzr, int32_t n, double amplitude, int32_t *seed);
+
+/* SES patch construction, the host stage ConnollySurface::build
+ * (ConnollySurface.cpp:2326-3316) emits: point, facet and edge cells with
+ * their clipping geometry.  Built here without CGAL by free-probe enumeration
+ * (see DESIGN.md).  The patch set is owned by the library. */
+typedef struct nsgpu_patch_set nsgpu_patch_set;
+int nsgpu_ses_build(const double *xyzr, int32_t n, double probe_radius,
+                    nsgpu_patch_set **out);
+void nsgpu_patch_set_view(const nsgpu_patch_set *ps, nsgpu_patches *view);
+void nsgpu_patch_set_free(nsgpu_patch_set *ps);
+/* counts[5] = number of patches of each NSGPU_PATCH_* kind */
+void nsgpu_patch_set_counts(const nsgpu_patch_set *ps, int64_t counts[5]);
+
+/* -------------------- device hot path ------------------------------------- */
+/* Grid upload: replaces the DelPhiShared arrays the reference colours in place. */
+int nsgpu_set_grid(nsgpu_ctx *ctx, const nsgpu_grid *grid);
+
+/* Patch upload (device copy of the SoA); probe_radius is Surface::probe_radius. */
+int nsgpu_set_patches(nsgpu_ctx *ctx, const nsgpu_patches *p, double probe_radius);
+
+/* Surface::getSurf(&vol, optimizeGrids, fillCav, cavVol) (Surface.cpp:398-1689):
+ * ray-patch intersection over the 3 grid panels (+3 edge panels when
+ * accurate_triangulation), per-ray hit sort, epsmap/idebmap/status fill,
+ * corner bitmap and vertex records, cavity labelling and conditional filling
+ * (getCavitiesWithBilevelStatusMap Surface.cpp:1994-2126, fillCavities
+ * :2355-2487), analytic vertex assembly (assembleVerticesList :7415-7660).
+ * Maps stay on the device until read back. */
+int nsgpu_get_surf(nsgpu_ctx *ctx, const nsgpu_surf_opts *opts, nsgpu_surf_result *res);
+
+/* Surface::triangulateSurface(outputMesh, buildAtomsMap, iso=0) (Surface.cpp:9909-10520):
+ * classify cubes from the corner bitmap, add the missing edge midpoints,
+ * emit triangles (reversed winding, as the reference), sum the area. */
+int nsgpu_triangulate(nsgpu_ctx *ctx, nsgpu_mesh_info *info);
+
+/* Wait for all work queued on the context's stream. */
+int nsgpu_synchronize(nsgpu_ctx *ctx);
+
+/* Stage timing with HIP events on the context's stream (no counterpart in the
+ * reference, whose stages print wall-clock times, e.g. Surface.cpp:1093-1096).
+ * When on, get_surf/triangulate record one event pair per stage; stage_times
+ * writes up to n durations in ms in the order intersect, sort, fill, corners,
+ * cavities, vertices, marching cubes, and returns the number of stages. */
+int nsgpu_set_profiling(nsgpu_ctx *ctx, int on);
+int nsgpu_stage_times(nsgpu_ctx *ctx, double *ms, int n);
+
+/* -------------------- read back (caller-owned buffers, DelPhi layouts) ----- */
+/* epsmap int32 [((k*ny+j)*nx+i)*3+l]; idebmap uint8 [(k*ny+j)*nx+i], 1 = out;
+ * status int32 [(k*ny+j)*nx+i]; corner bits uint32 [ceil(nx*ny*nz/32)],
+ * bit (flat & 31) of word (flat >> 5), 1 = outside (tools.h:1324-1413). */
+int nsgpu_read_epsmap(nsgpu_ctx *ctx, int32_t *out);
+int nsgpu_read_idebmap(nsgpu_ctx *ctx, uint8_t *out);
+int nsgpu_read_status(nsgpu_ctx *ctx, int32_t *out);
+int nsgpu_read_corner_bits(nsgpu_ctx *ctx, uint32_t *out);
+/* cavity sizes in A^3 (DelPhiShared::cavitiesSize) and filled flags. */
+int nsgpu_read_cavities(nsgpu_ctx *ctx, double *sizes, uint8_t *filled);
+/* mesh: verts [num_vertices*3], tris [num_triangles*3] */
+int nsgpu_read_mesh(nsgpu_ctx *ctx, double *verts, int32_t *tris);
+/* Sorted hits of every ray of one panel slot (0..2 grid rays, 3..5 edge rays):
+ * counts[N*N] and t[N*N*max_hits], ray (m,n) at index n*N+m, N = max(nx,ny,nz).
+ * max_hits must be >= nsgpu_surf_result.max_hits_per_ray. */
+int nsgpu_read_ray_hits(nsgpu_ctx *ctx, int32_t slot, int32_t max_hits, int32_t *counts,
+                        double *t);
+
+/* -------------------- synthetic inputs ------------------------------------ */
+/* Seeded self-avoiding chain: 1.5 A steps, >= 3.0 A separation between atoms
+ * at least 3 bonds apart, centres confined to a sphere of radius R, radii
+ * C/N/O/S = 1.70/1.55/1.52/1.80 A with probabilities .62/.17/.19/.02.
+ * The confinement radius grows by 2 % whenever the walk jams; the radius
+ * finally used is written to *radius_used.  Writes n*4 doubles (x,y,z,r).
+ * Returns the number of atoms placed. */
+int32_t nsgpu_generate_chain(int32_t n, double radius, uint64_t seed, double *xyzr,
+                             double *radius_used);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NSGPU_H */

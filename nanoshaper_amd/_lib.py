@@ -1,0 +1,98 @@
+"""ctypes binding of libnsgpu.so (the C ABI declared in include/nsgpu.h).
+
+The library is loaded from the package directory only; a missing library is
+an error, never a fallback.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnsgpu.so")
+
+c_int32_p = ctypes.POINTER(ctypes.c_int32)
+c_double_p = ctypes.POINTER(ctypes.c_double)
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32), ("pad_", ctypes.c_int32),
+                ("xmin", ctypes.c_double), ("ymin", ctypes.c_double), ("zmin", ctypes.c_double),
+                ("xmax", ctypes.c_double), ("ymax", ctypes.c_double), ("zmax", ctypes.c_double),
+                ("scale", ctypes.c_double), ("side", ctypes.c_double), ("hside", ctypes.c_double),
+                ("A", ctypes.c_double), ("rmaxdim", ctypes.c_double), ("baricenter", ctypes.c_double * 3)]
+
+
+class Patches(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("n_data", ctypes.c_int32),
+                ("kind", c_int32_p), ("off", c_int32_p), ("cnt", c_int32_p), ("flags", c_int32_p),
+                ("atom", c_int32_p), ("bsph", c_double_p), ("data", c_double_p)]
+
+
+class SurfOpts(ctypes.Structure):
+    _fields_ = [("build_epsmap", ctypes.c_int32), ("build_status", ctypes.c_int32),
+                ("accurate_triangulation", ctypes.c_int32), ("fill_cavities", ctypes.c_int32),
+                ("cav_vol", ctypes.c_double), ("inside", ctypes.c_int32), ("precision", ctypes.c_int32)]
+
+
+class SurfResult(ctypes.Structure):
+    _fields_ = [("volume", ctypes.c_double), ("total_rays", ctypes.c_int64), ("odd_rays", ctypes.c_int64),
+                ("total_hits", ctypes.c_int64), ("candidates", ctypes.c_int64), ("num_cavities", ctypes.c_int32),
+                ("max_hits_per_ray", ctypes.c_int32)]
+
+
+class MeshInfo(ctypes.Structure):
+    _fields_ = [("num_vertices", ctypes.c_int64), ("num_triangles", ctypes.c_int64),
+                ("num_analytic_vertices", ctypes.c_int64), ("area", ctypes.c_double)]
+
+
+VP = ctypes.c_void_p
+
+# name -> (restype, argtypes); every symbol declared in include/nsgpu.h
+SIGNATURES = {
+    "nsgpu_abi_version": (ctypes.c_int, []),
+    "nsgpu_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(VP)]),
+    "nsgpu_destroy": (None, [VP]),
+    "nsgpu_last_error": (ctypes.c_char_p, [VP]),
+    "nsgpu_build_grid": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.POINTER(Grid)]),
+    "nsgpu_jitter": (None, [VP, ctypes.c_int32, ctypes.c_double, c_int32_p]),
+    "nsgpu_ses_build": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_double, ctypes.POINTER(VP)]),
+    "nsgpu_patch_set_view": (None, [VP, ctypes.POINTER(Patches)]),
+    "nsgpu_patch_set_free": (None, [VP]),
+    "nsgpu_patch_set_counts": (None, [VP, VP]),
+    "nsgpu_set_grid": (ctypes.c_int, [VP, ctypes.POINTER(Grid)]),
+    "nsgpu_set_patches": (ctypes.c_int, [VP, ctypes.POINTER(Patches), ctypes.c_double]),
+    "nsgpu_get_surf": (ctypes.c_int, [VP, ctypes.POINTER(SurfOpts), ctypes.POINTER(SurfResult)]),
+    "nsgpu_triangulate": (ctypes.c_int, [VP, ctypes.POINTER(MeshInfo)]),
+    "nsgpu_synchronize": (ctypes.c_int, [VP]),
+    "nsgpu_set_profiling": (ctypes.c_int, [VP, ctypes.c_int]),
+    "nsgpu_stage_times": (ctypes.c_int, [VP, VP, ctypes.c_int]),
+    "nsgpu_read_epsmap": (ctypes.c_int, [VP, VP]),
+    "nsgpu_read_idebmap": (ctypes.c_int, [VP, VP]),
+    "nsgpu_read_status": (ctypes.c_int, [VP, VP]),
+    "nsgpu_read_corner_bits": (ctypes.c_int, [VP, VP]),
+    "nsgpu_read_cavities": (ctypes.c_int, [VP, VP, VP]),
+    "nsgpu_read_mesh": (ctypes.c_int, [VP, VP, VP]),
+    "nsgpu_read_ray_hits": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_int32, VP, VP]),
+    "nsgpu_generate_chain": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_double, ctypes.c_uint64, VP,
+                                              ctypes.POINTER(ctypes.c_double)]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libnsgpu.so from the package directory (raises if it is absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libnsgpu.so is not built: run __graft_entry__.build() or "
+                           "python nanoshaper_amd/build.py (looked in %s)" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.nsgpu_abi_version() != 1:
+        raise RuntimeError("libnsgpu ABI version mismatch")
+    _lib = lib
+    return lib

@@ -1,0 +1,649 @@
+// C ABI of the device hot path: one context = one grid + one patch set on one
+// device, driving the kernels of kernels/*.hip on a private HIP stream.
+// Device buffers are grown on demand and reused across calls.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../kernels/launch.hpp"
+#include "nsgpu.h"
+
+
+using namespace nsg;
+
+namespace {
+// grow-only device buffer
+struct DBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap && p) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t b = std::max<size_t>(bytes, 256);
+        hipError_t e = hipMalloc(&p, b);
+        if (e == hipSuccess) cap = b;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T>
+    T *as() const { return (T *)p; }
+};
+
+enum Stage { ST_INTERSECT, ST_SORT, ST_FILL, ST_CORNERS, ST_CAV, ST_VERTS, ST_MC, ST_COUNT };
+}  // namespace
+
+struct nsgpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // grid
+    bool have_grid = false;
+    nsgpu_grid grid{};
+    DBuf gx, gy, gz;
+    // patches
+    bool have_patches = false;
+    int npatch = 0;
+    double probe = 1.4;
+    DBuf pkind, poff, pcnt, pflags, pbsph, pdata;
+    // rays
+    int K = 16;
+    DBuf rcnt, rt, rvol, partial, stats, cand;
+    int nslots = 0, panels[2] = {0, 0};
+    // maps
+    DBuf eps, ideb, status, bits;
+    // cavities
+    DBuf parent, nroots, rootlist, bkey, ccount, clabel, cfilled;
+    std::vector<double> cav_size;
+    std::vector<uint8_t> cav_filled;
+    // vertices and mesh
+    DBuf idx, vcount, voffs, scan_scratch, verts, tris, tarea;
+    int64_t n_analytic = 0, nverts = 0, ntris = 0;
+    double area = 0;
+    bool have_surf = false, have_vertices = false, have_mesh = false;
+    nsgpu_surf_opts opts{};
+    nsgpu_surf_result res{};
+    // stage timing
+    bool profile = false;
+    hipEvent_t ev[2 * ST_COUNT] = {};
+    double stage_ms[ST_COUNT] = {};
+};
+
+#define HIP_TRY(ctx, expr)                                                                           \
+    do {                                                                                             \
+        hipError_t e_ = (expr);                                                                      \
+        if (e_ != hipSuccess) {                                                                      \
+            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);                          \
+            return NSGPU_E_DEVICE;                                                                   \
+        }                                                                                            \
+    } while (0)
+
+static int fail(nsgpu_ctx *c, int code, const char *msg) {
+    c->err = msg;
+    return code;
+}
+
+static DevGrid dev_grid(const nsgpu_ctx *c) {
+    DevGrid g;
+    g.nx = c->grid.nx;
+    g.ny = c->grid.ny;
+    g.nz = c->grid.nz;
+    g.N = std::max(g.nx, std::max(g.ny, g.nz));
+    g.xmin = c->grid.xmin;
+    g.ymin = c->grid.ymin;
+    g.zmin = c->grid.zmin;
+    g.xmax = c->grid.xmax;
+    g.ymax = c->grid.ymax;
+    g.zmax = c->grid.zmax;
+    g.scale = c->grid.scale;
+    g.side = c->grid.side;
+    g.hside = c->grid.hside;
+    g.x = c->gx.as<double>();
+    g.y = c->gy.as<double>();
+    g.z = c->gz.as<double>();
+    return g;
+}
+
+static DevRays dev_rays(const nsgpu_ctx *c) {
+    DevRays r;
+    r.N = std::max(c->grid.nx, std::max(c->grid.ny, c->grid.nz));
+    r.K = c->K;
+    r.nslots = c->nslots;
+    r.panels0 = c->panels[0];
+    r.cnt = c->rcnt.as<int>();
+    r.t = c->rt.as<double>();
+    r.rvol = c->rvol.as<double>();
+    return r;
+}
+
+static void mark(nsgpu_ctx *c, int stage, int end) {
+    if (c->profile) (void)hipEventRecord(c->ev[2 * stage + end], c->stream);
+}
+
+extern "C" {
+
+int nsgpu_create(int device, nsgpu_ctx **out) {
+    if (!out) return NSGPU_E_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return NSGPU_E_DEVICE;
+    if (device < 0 || device >= ndev) return NSGPU_E_ARG;
+    if (hipSetDevice(device) != hipSuccess) return NSGPU_E_DEVICE;
+    auto *c = new nsgpu_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return NSGPU_E_DEVICE;
+    }
+    for (auto &e : c->ev) (void)hipEventCreate(&e);
+    mc_upload_tables(c->stream);
+    *out = c;
+    return NSGPU_OK;
+}
+
+void nsgpu_destroy(nsgpu_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    DBuf *all[] = {&c->gx, &c->gy, &c->gz, &c->pkind, &c->poff, &c->pcnt, &c->pflags, &c->pbsph, &c->pdata,
+                   &c->rcnt, &c->rt, &c->rvol, &c->partial, &c->stats, &c->cand, &c->eps, &c->ideb, &c->status,
+                   &c->bits, &c->parent, &c->nroots, &c->rootlist, &c->bkey, &c->ccount, &c->clabel, &c->cfilled,
+                   &c->idx, &c->vcount, &c->voffs, &c->scan_scratch, &c->verts, &c->tris, &c->tarea};
+    for (DBuf *b : all) b->release();
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *nsgpu_last_error(const nsgpu_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int nsgpu_set_profiling(nsgpu_ctx *c, int on) {
+    if (!c) return NSGPU_E_ARG;
+    c->profile = on != 0;
+    return NSGPU_OK;
+}
+
+int nsgpu_stage_times(nsgpu_ctx *c, double *ms, int n) {
+    if (!c || !ms) return NSGPU_E_ARG;
+    (void)hipSetDevice(c->device);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (int s = 0; s < ST_COUNT && s < n; ++s) {
+        float t = 0;
+        if (hipEventElapsedTime(&t, c->ev[2 * s], c->ev[2 * s + 1]) != hipSuccess) t = 0;
+        ms[s] = t;
+    }
+    return ST_COUNT;
+}
+
+int nsgpu_set_grid(nsgpu_ctx *c, const nsgpu_grid *g) {
+    if (!c || !g || g->nx <= 1 || g->ny <= 1 || g->nz <= 1) return NSGPU_E_ARG;
+    if ((int64_t)g->nx * g->ny * g->nz >= (int64_t)1 << 31)
+        return fail(c, NSGPU_E_ARG, "grid exceeds 2^31 nodes");
+    HIP_TRY(c, hipSetDevice(c->device));
+    c->grid = *g;
+    std::vector<double> x(g->nx), y(g->ny), z(g->nz);
+    // DelphiShared.cpp:1135-1142
+    for (int i = 0; i < g->nx; i++) x[i] = g->xmin + i * g->side;
+    for (int i = 0; i < g->ny; i++) y[i] = g->ymin + i * g->side;
+    for (int i = 0; i < g->nz; i++) z[i] = g->zmin + i * g->side;
+    HIP_TRY(c, c->gx.ensure(x.size() * 8));
+    HIP_TRY(c, c->gy.ensure(y.size() * 8));
+    HIP_TRY(c, c->gz.ensure(z.size() * 8));
+    HIP_TRY(c, hipMemcpyAsync(c->gx.p, x.data(), x.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->gy.p, y.data(), y.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->gz.p, z.data(), z.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->have_grid = true;
+    c->have_surf = c->have_vertices = c->have_mesh = false;
+    return NSGPU_OK;
+}
+
+int nsgpu_set_patches(nsgpu_ctx *c, const nsgpu_patches *p, double probe_radius) {
+    if (!c || !p || p->n < 0 || p->n_data < 0 || !(probe_radius > 0)) return NSGPU_E_ARG;
+    if (p->n > 0 && (!p->kind || !p->off || !p->cnt || !p->flags || !p->bsph || (!p->data && p->n_data)))
+        return NSGPU_E_ARG;
+    for (int i = 0; i < p->n; ++i) {
+        const int k = p->kind[i];
+        if (k < NSGPU_PATCH_POINT || k > NSGPU_PATCH_TORUS_SING) return fail(c, NSGPU_E_ARG, "bad patch kind");
+        const int64_t need = (k >= NSGPU_PATCH_TORUS_REG) ? NSGPU_TORUS_HDR + 8LL * p->cnt[i] : 4LL * p->cnt[i];
+        if (p->off[i] < 0 || p->cnt[i] < 0 || (int64_t)p->off[i] + need > p->n_data)
+            return fail(c, NSGPU_E_ARG, "patch data out of range");
+        if (k == NSGPU_PATCH_TORUS_REG && (p->cnt[i] < 1 || p->cnt[i] > 31))
+            return fail(c, NSGPU_E_ARG, "regular torus needs 1..31 plane pairs");
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    const size_t n = (size_t)p->n;
+    HIP_TRY(c, c->pkind.ensure(n * 4));
+    HIP_TRY(c, c->poff.ensure(n * 4));
+    HIP_TRY(c, c->pcnt.ensure(n * 4));
+    HIP_TRY(c, c->pflags.ensure(n * 4));
+    HIP_TRY(c, c->pbsph.ensure(n * 32));
+    HIP_TRY(c, c->pdata.ensure((size_t)p->n_data * 8));
+    if (n) {
+        HIP_TRY(c, hipMemcpyAsync(c->pkind.p, p->kind, n * 4, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->poff.p, p->off, n * 4, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->pcnt.p, p->cnt, n * 4, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->pflags.p, p->flags, n * 4, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->pbsph.p, p->bsph, n * 32, hipMemcpyHostToDevice, c->stream));
+    }
+    if (p->n_data)
+        HIP_TRY(c, hipMemcpyAsync(c->pdata.p, p->data, (size_t)p->n_data * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->npatch = p->n;
+    c->probe = probe_radius;
+    c->have_patches = true;
+    c->have_surf = c->have_vertices = c->have_mesh = false;
+    return NSGPU_OK;
+}
+
+int nsgpu_synchronize(nsgpu_ctx *c) {
+    if (!c) return NSGPU_E_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return NSGPU_OK;
+}
+
+// ray tracing and sort; grows the per-ray capacity and repeats when a ray overflows
+static int trace(nsgpu_ctx *c) {
+    const DevGrid g = dev_grid(c);
+    const int64_t N = g.N;
+    const int64_t nrays = N * N * c->nslots;
+    DevPatches P;
+    P.n = c->npatch;
+    P.kind = c->pkind.as<int>();
+    P.off = c->poff.as<int>();
+    P.cnt = c->pcnt.as<int>();
+    P.flags = c->pflags.as<int>();
+    P.bsph = c->pbsph.as<double>();
+    P.data = c->pdata.as<double>();
+    HIP_TRY(c, c->rcnt.ensure(nrays * 4));
+    HIP_TRY(c, c->rvol.ensure(nrays * 8));
+    HIP_TRY(c, c->stats.ensure(sizeof(RayStats)));
+    HIP_TRY(c, c->cand.ensure(8));
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        HIP_TRY(c, c->rt.ensure((size_t)nrays * c->K * 8));
+        const DevRays R = dev_rays(c);
+        HIP_TRY(c, hipMemsetAsync(c->rcnt.p, 0, nrays * 4, c->stream));
+        HIP_TRY(c, hipMemsetAsync(c->stats.p, 0, sizeof(RayStats), c->stream));
+        HIP_TRY(c, hipMemsetAsync(c->cand.p, 0, 8, c->stream));
+        mark(c, ST_INTERSECT, 0);
+        launch_intersect(c->stream, g, P, R, c->probe, c->cand.as<unsigned long long>());
+        HIP_TRY(c, hipGetLastError());
+        mark(c, ST_INTERSECT, 1);
+        mark(c, ST_SORT, 0);
+        launch_sort(c->stream, g, R, c->stats.as<RayStats>());
+        HIP_TRY(c, hipGetLastError());
+        mark(c, ST_SORT, 1);
+        RayStats st;
+        unsigned long long cand = 0;
+        HIP_TRY(c, hipMemcpyAsync(&st, c->stats.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(&cand, c->cand.p, 8, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        c->res.total_hits = (int64_t)st.hits;
+        c->res.odd_rays = (int64_t)st.odd;
+        c->res.max_hits_per_ray = st.max_cnt;
+        c->res.candidates = (int64_t)cand;
+        if (st.max_cnt <= c->K) return NSGPU_OK;
+        int k = c->K;
+        while (k < st.max_cnt) k *= 2;
+        c->K = k;
+    }
+    return fail(c, NSGPU_E_STATE, "ray hit capacity did not converge");
+}
+
+static int cavities(nsgpu_ctx *c) {
+    const DevGrid g = dev_grid(c);
+    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
+    HIP_TRY(c, c->parent.ensure(nvox * 4));
+    HIP_TRY(c, c->nroots.ensure(4));
+    HIP_TRY(c, c->rootlist.ensure(nvox * 4));
+    HIP_TRY(c, hipMemsetAsync(c->nroots.p, 0, 4, c->stream));
+    launch_ccl(c->stream, g, c->status.as<int>(), c->parent.as<int>(), c->nroots.as<int>(), c->rootlist.as<int>());
+    HIP_TRY(c, hipGetLastError());
+    int nroots = 0;
+    HIP_TRY(c, hipMemcpyAsync(&nroots, c->nroots.p, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->cav_size.clear();
+    c->cav_filled.clear();
+    c->res.num_cavities = 0;
+    if (nroots == 0) return NSGPU_OK;
+    HIP_TRY(c, c->bkey.ensure((size_t)nroots * 4));
+    HIP_TRY(c, c->ccount.ensure((size_t)nroots * 4));
+    HIP_TRY(c, hipMemsetAsync(c->bkey.p, 0xff, (size_t)nroots * 4, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->ccount.p, 0, (size_t)nroots * 4, c->stream));
+    launch_ccl_keys(c->stream, g, c->parent.as<int>(), c->bkey.as<unsigned int>(), c->ccount.as<int>());
+    HIP_TRY(c, hipGetLastError());
+    std::vector<int> roots(nroots), count(nroots);
+    std::vector<unsigned int> bkey(nroots);
+    HIP_TRY(c, hipMemcpyAsync(roots.data(), c->rootlist.p, (size_t)nroots * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(bkey.data(), c->bkey.p, (size_t)nroots * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(count.data(), c->ccount.p, (size_t)nroots * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    // replay the alternating scan of getCavitiesWithBilevelStatusMap: even
+    // passes take the component met first scanning k upwards (smallest flat
+    // index = the root), odd passes the first one scanning k downwards
+    std::vector<int> byF(nroots), byB(nroots);
+    for (int s = 0; s < nroots; ++s) byF[s] = byB[s] = s;
+    std::sort(byF.begin(), byF.end(), [&](int a, int b) { return roots[a] < roots[b]; });
+    std::sort(byB.begin(), byB.end(), [&](int a, int b) { return bkey[a] < bkey[b]; });
+    std::vector<int> label(nroots, 0);
+    std::vector<char> done(nroots, 0);
+    size_t pf = 0, pb = 0;
+    int id = NSGPU_STATUS_TEMP_OUT;
+    for (int t = 0; t < nroots; ++t) {
+        int pick;
+        if (t % 2 == 0) {
+            while (done[byF[pf]]) ++pf;
+            pick = byF[pf];
+        } else {
+            while (done[byB[pb]]) ++pb;
+            pick = byB[pb];
+        }
+        done[pick] = 1;
+        label[pick] = ++id;
+    }
+    const double vol = c->opts.cav_vol < 0 ? 11.4 : c->opts.cav_vol;
+    const double cube = c->grid.side * c->grid.side * c->grid.side;
+    const int ncav = id - NSGPU_STATUS_OUT;
+    c->cav_size.assign(std::max(ncav, 0), 0.0);
+    c->cav_filled.assign(std::max(ncav, 0), 0);
+    std::vector<uint8_t> filled(nroots, 0);
+    for (int s = 0; s < nroots; ++s) {
+        if (label[s] < NSGPU_STATUS_FIRST_CAV) continue;
+        const double v = count[s] * cube;
+        c->cav_size[label[s] - NSGPU_STATUS_FIRST_CAV] = v;
+        filled[s] = v <= vol;
+        c->cav_filled[label[s] - NSGPU_STATUS_FIRST_CAV] = filled[s];
+    }
+    c->res.num_cavities = std::max(ncav, 0);
+    HIP_TRY(c, c->clabel.ensure((size_t)nroots * 4));
+    HIP_TRY(c, c->cfilled.ensure((size_t)nroots));
+    HIP_TRY(c, hipMemcpyAsync(c->clabel.p, label.data(), (size_t)nroots * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->cfilled.p, filled.data(), (size_t)nroots, hipMemcpyHostToDevice, c->stream));
+    launch_ccl_label(c->stream, g, c->parent.as<int>(), c->clabel.as<int>(), c->cfilled.as<uint8_t>(),
+                     c->opts.build_epsmap, c->opts.accurate_triangulation, c->opts.inside, c->status.as<int>(),
+                     c->eps.as<int>(), c->ideb.as<uint8_t>(), c->bits.as<uint32_t>());
+    HIP_TRY(c, hipGetLastError());
+    // the host vectors above must outlive the copies
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return NSGPU_OK;
+}
+
+static int read_total(nsgpu_ctx *c, const int *counts, const int *offs, int64_t n, int64_t *total) {
+    if (n <= 0) {
+        *total = 0;
+        return NSGPU_OK;
+    }
+    int a = 0, b = 0;
+    HIP_TRY(c, hipMemcpyAsync(&a, counts + n - 1, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(&b, offs + n - 1, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    *total = (int64_t)a + b;
+    return NSGPU_OK;
+}
+
+static int assemble(nsgpu_ctx *c, const int eslots[3]) {
+    const DevGrid g = dev_grid(c);
+    const DevRays R = dev_rays(c);
+    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
+    const int64_t nrays = 3 * (int64_t)R.N * R.N;
+    const int64_t ncubes = mc_num_cubes(g);
+    const int64_t nmax = std::max(nrays, ncubes);
+    HIP_TRY(c, c->idx.ensure((size_t)nvox * 3 * 4));
+    HIP_TRY(c, c->vcount.ensure((size_t)nmax * 4));
+    HIP_TRY(c, c->voffs.ensure((size_t)nmax * 4));
+    HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(nmax) * 4));
+    HIP_TRY(c, hipMemsetAsync(c->idx.p, 0xff, (size_t)nvox * 3 * 4, c->stream));
+    launch_vtx_count(c->stream, g, R, eslots, c->bits.as<uint32_t>(), c->vcount.as<int>());
+    launch_scan(c->stream, c->vcount.as<int>(), nrays, c->voffs.as<int>(), c->scan_scratch.as<int>());
+    HIP_TRY(c, hipGetLastError());
+    int64_t nv = 0;
+    int rc = read_total(c, c->vcount.as<int>(), c->voffs.as<int>(), nrays, &nv);
+    if (rc) return rc;
+    HIP_TRY(c, c->verts.ensure((size_t)std::max<int64_t>(nv, 1) * 24));
+    launch_vtx_write(c->stream, g, R, eslots, c->bits.as<uint32_t>(), c->voffs.as<int>(), c->verts.as<double>(),
+                     c->idx.as<int>());
+    HIP_TRY(c, hipGetLastError());
+    c->n_analytic = nv;
+    c->nverts = nv;
+    return NSGPU_OK;
+}
+
+int nsgpu_get_surf(nsgpu_ctx *c, const nsgpu_surf_opts *o, nsgpu_surf_result *res) {
+    if (!c || !o) return NSGPU_E_ARG;
+    if (!c->have_grid || !c->have_patches) return fail(c, NSGPU_E_STATE, "set_grid and set_patches first");
+    if (o->precision != NSGPU_PREC_FP64 && o->precision != NSGPU_PREC_MIXED)
+        return fail(c, NSGPU_E_ARG, "unknown precision mode");
+    HIP_TRY(c, hipSetDevice(c->device));
+    c->opts = *o;
+    std::memset(&c->res, 0, sizeof c->res);
+    c->have_surf = c->have_vertices = c->have_mesh = false;
+    // panels rule, Surface.cpp:652-659
+    c->panels[0] = 3;
+    c->panels[1] = 3;
+    if (!o->build_status && !o->build_epsmap) c->panels[0] = 0;
+    else if (o->build_status && !o->build_epsmap) c->panels[0] = 1;
+    if (!o->accurate_triangulation) c->panels[1] = 0;
+    c->nslots = c->panels[0] + c->panels[1];
+    if (c->nslots == 0) return fail(c, NSGPU_E_ARG, "nothing to compute (no epsmap, status or triangulation)");
+    const DevGrid g = dev_grid(c);
+    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
+    const int64_t nwords = (nvox + 31) / 32;
+    // intersections
+    int rc = trace(c);
+    if (rc) return rc;
+    const DevRays R = dev_rays(c);
+    FillOpts fo;
+    fo.build_eps = o->build_epsmap;
+    fo.build_status = o->build_status;
+    fo.inside = o->inside;
+    fo.slot_x = c->panels[0] >= 1 ? 0 : -1;
+    fo.slot_z = (c->panels[0] >= 3 && o->build_epsmap) ? 1 : -1;
+    fo.slot_y = (c->panels[0] >= 3 && o->build_epsmap) ? 2 : -1;
+    fo.eslot_x = c->panels[0] + 0;
+    fo.eslot_z = c->panels[0] + 1;
+    fo.eslot_y = c->panels[0] + 2;
+    // maps (buffers always exist so that later stages can index them)
+    HIP_TRY(c, c->eps.ensure((size_t)nvox * 12));
+    HIP_TRY(c, c->ideb.ensure((size_t)nvox));
+    HIP_TRY(c, c->status.ensure((size_t)nvox * 4));
+    HIP_TRY(c, c->bits.ensure((size_t)nwords * 4));
+    mark(c, ST_FILL, 0);
+    if (c->panels[0] > 0) launch_fill_maps(c->stream, g, R, fo, c->eps.as<int>(), c->ideb.as<uint8_t>(), c->status.as<int>());
+    HIP_TRY(c, hipGetLastError());
+    mark(c, ST_FILL, 1);
+    mark(c, ST_CORNERS, 0);
+    if (o->accurate_triangulation) launch_fill_corners(c->stream, g, R, fo, c->bits.as<uint32_t>());
+    HIP_TRY(c, hipGetLastError());
+    mark(c, ST_CORNERS, 1);
+    // volume: A * sum of segment lengths / (panels x phases traced), Surface.cpp:1677-1684
+    {
+        const int64_t NN = (int64_t)R.N * R.N;
+        int fill_panels = 3;
+        if (!o->build_status && !o->build_epsmap) fill_panels = 0;
+        else if (o->build_status && !o->build_epsmap) fill_panels = 1;
+        if (o->accurate_triangulation) fill_panels = 3;
+        int flag_sum = 0;
+        std::vector<int> vslots;
+        for (int p = 0; p < fill_panels; ++p) {
+            if ((o->build_status && p == 0) || o->build_epsmap) {
+                ++flag_sum;
+                if (p < c->panels[0]) vslots.push_back(p);
+            }
+            if (o->accurate_triangulation) {
+                ++flag_sum;
+                vslots.push_back(c->panels[0] + p);
+            }
+        }
+        const int64_t per = sum_partials_count(NN);
+        HIP_TRY(c, c->partial.ensure((size_t)per * c->nslots * 8));
+        for (int s : vslots) launch_sum_blocks(c->stream, R.rvol + s * NN, NN, c->partial.as<double>() + s * per);
+        HIP_TRY(c, hipGetLastError());
+        std::vector<double> part((size_t)per * c->nslots, 0.0);
+        HIP_TRY(c, hipMemcpyAsync(part.data(), c->partial.p, part.size() * 8, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        double sum = 0;
+        for (int s : vslots)
+            for (int64_t q = 0; q < per; ++q) sum += part[(size_t)s * per + q];
+        c->res.volume = flag_sum ? c->grid.A * sum / flag_sum : 0.0;
+        int64_t traced = 0;
+        for (int p = 0; p < fill_panels; ++p) {
+            if ((o->build_status && p == 0) || o->build_epsmap) traced += NN;
+            if (o->accurate_triangulation) traced += NN;
+        }
+        c->res.total_rays = traced;
+    }
+    mark(c, ST_CAV, 0);
+    if (o->fill_cavities && o->build_status) {
+        rc = cavities(c);
+        if (rc) return rc;
+    }
+    mark(c, ST_CAV, 1);
+    mark(c, ST_VERTS, 0);
+    if (o->accurate_triangulation) {
+        const int es[3] = {fo.eslot_x, fo.eslot_z, fo.eslot_y};
+        rc = assemble(c, es);
+        if (rc) return rc;
+        c->have_vertices = true;
+    }
+    mark(c, ST_VERTS, 1);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->have_surf = true;
+    if (res) *res = c->res;
+    return NSGPU_OK;
+}
+
+int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
+    if (!c) return NSGPU_E_ARG;
+    if (!c->have_vertices) return fail(c, NSGPU_E_STATE, "get_surf with accurate_triangulation first");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const DevGrid g = dev_grid(c);
+    const int64_t ncubes = mc_num_cubes(g);
+    mark(c, ST_MC, 0);
+    c->nverts = c->n_analytic;
+    // missing vertices of edges 5, 6, 10
+    launch_mc_vcount(c->stream, g, c->bits.as<uint32_t>(), c->idx.as<int>(), c->vcount.as<int>());
+    launch_scan(c->stream, c->vcount.as<int>(), ncubes, c->voffs.as<int>(), c->scan_scratch.as<int>());
+    HIP_TRY(c, hipGetLastError());
+    int64_t nadd = 0;
+    int rc = read_total(c, c->vcount.as<int>(), c->voffs.as<int>(), ncubes, &nadd);
+    if (rc) return rc;
+    const int64_t nv = c->n_analytic + nadd;
+    if (nv >= ((int64_t)1 << 31)) return fail(c, NSGPU_E_NOMEM, "too many vertices");
+    if ((size_t)nv * 24 > c->verts.cap) {
+        // grow while keeping the analytic vertices
+        DBuf nb;
+        HIP_TRY(c, nb.ensure((size_t)nv * 24));
+        if (c->n_analytic)
+            HIP_TRY(c, hipMemcpyAsync(nb.p, c->verts.p, (size_t)c->n_analytic * 24, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        c->verts.release();
+        c->verts = nb;
+    }
+    launch_mc_vwrite(c->stream, g, c->bits.as<uint32_t>(), c->idx.as<int>(), c->voffs.as<int>(), (int)c->n_analytic,
+                     c->verts.as<double>());
+    HIP_TRY(c, hipGetLastError());
+    c->nverts = nv;
+    // triangles
+    launch_mc_tcount(c->stream, g, c->bits.as<uint32_t>(), c->idx.as<int>(), c->vcount.as<int>());
+    launch_scan(c->stream, c->vcount.as<int>(), ncubes, c->voffs.as<int>(), c->scan_scratch.as<int>());
+    HIP_TRY(c, hipGetLastError());
+    int64_t nt = 0;
+    rc = read_total(c, c->vcount.as<int>(), c->voffs.as<int>(), ncubes, &nt);
+    if (rc) return rc;
+    HIP_TRY(c, c->tris.ensure((size_t)std::max<int64_t>(nt, 1) * 12));
+    HIP_TRY(c, c->tarea.ensure((size_t)std::max<int64_t>(nt, 1) * 8));
+    launch_mc_twrite(c->stream, g, c->bits.as<uint32_t>(), c->idx.as<int>(), c->voffs.as<int>(),
+                     c->verts.as<double>(), c->tris.as<int>(), c->tarea.as<double>());
+    HIP_TRY(c, hipGetLastError());
+    const int64_t per = sum_partials_count(nt);
+    HIP_TRY(c, c->partial.ensure((size_t)std::max<int64_t>(per, 1) * 8));
+    launch_sum_blocks(c->stream, c->tarea.as<double>(), nt, c->partial.as<double>());
+    HIP_TRY(c, hipGetLastError());
+    mark(c, ST_MC, 1);
+    std::vector<double> part((size_t)per, 0.0);
+    if (per) HIP_TRY(c, hipMemcpyAsync(part.data(), c->partial.p, (size_t)per * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    double area = 0;
+    for (double v : part) area += v;
+    c->ntris = nt;
+    c->area = area;
+    c->have_mesh = true;
+    if (info) {
+        info->num_vertices = c->nverts;
+        info->num_triangles = c->ntris;
+        info->num_analytic_vertices = c->n_analytic;
+        info->area = area;
+    }
+    return NSGPU_OK;
+}
+
+static int read_buf(nsgpu_ctx *c, const DBuf &b, void *out, size_t bytes) {
+    if (!c || !out) return NSGPU_E_ARG;
+    if (!c->have_surf) return fail(c, NSGPU_E_STATE, "no surface computed");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (bytes) HIP_TRY(c, hipMemcpyAsync(out, b.p, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return NSGPU_OK;
+}
+
+static int64_t nvox_of(const nsgpu_ctx *c) { return (int64_t)c->grid.nx * c->grid.ny * c->grid.nz; }
+
+int nsgpu_read_epsmap(nsgpu_ctx *c, int32_t *out) { return read_buf(c, c->eps, out, (size_t)nvox_of(c) * 12); }
+int nsgpu_read_idebmap(nsgpu_ctx *c, uint8_t *out) { return read_buf(c, c->ideb, out, (size_t)nvox_of(c)); }
+int nsgpu_read_status(nsgpu_ctx *c, int32_t *out) { return read_buf(c, c->status, out, (size_t)nvox_of(c) * 4); }
+int nsgpu_read_corner_bits(nsgpu_ctx *c, uint32_t *out) {
+    return read_buf(c, c->bits, out, (size_t)((nvox_of(c) + 31) / 32) * 4);
+}
+
+int nsgpu_read_cavities(nsgpu_ctx *c, double *sizes, uint8_t *filled) {
+    if (!c) return NSGPU_E_ARG;
+    for (size_t q = 0; q < c->cav_size.size(); ++q) {
+        if (sizes) sizes[q] = c->cav_size[q];
+        if (filled) filled[q] = c->cav_filled[q];
+    }
+    return (int)c->cav_size.size();
+}
+
+int nsgpu_read_mesh(nsgpu_ctx *c, double *verts, int32_t *tris) {
+    if (!c) return NSGPU_E_ARG;
+    if (!c->have_mesh) return fail(c, NSGPU_E_STATE, "no mesh computed");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (verts && c->nverts)
+        HIP_TRY(c, hipMemcpyAsync(verts, c->verts.p, (size_t)c->nverts * 24, hipMemcpyDeviceToHost, c->stream));
+    if (tris && c->ntris)
+        HIP_TRY(c, hipMemcpyAsync(tris, c->tris.p, (size_t)c->ntris * 12, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return NSGPU_OK;
+}
+
+int nsgpu_read_ray_hits(nsgpu_ctx *c, int32_t slot, int32_t max_hits, int32_t *counts, double *t) {
+    if (!c || !counts || !t || max_hits <= 0) return NSGPU_E_ARG;
+    if (!c->have_surf) return fail(c, NSGPU_E_STATE, "no surface computed");
+    if (slot < 0 || slot >= c->nslots) return fail(c, NSGPU_E_ARG, "slot out of range");
+    if (max_hits < c->res.max_hits_per_ray) return fail(c, NSGPU_E_ARG, "max_hits below the longest ray");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const int64_t N = std::max(c->grid.nx, std::max(c->grid.ny, c->grid.nz));
+    const int64_t NN = N * N;
+    std::vector<double> buf((size_t)NN * c->K);
+    HIP_TRY(c, hipMemcpyAsync(counts, c->rcnt.as<int>() + slot * NN, NN * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(buf.data(), c->rt.as<double>() + slot * NN * c->K, buf.size() * 8,
+                              hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (int64_t r = 0; r < NN; ++r)
+        for (int h = 0; h < counts[r] && h < max_hits; ++h) t[r * max_hits + h] = buf[(size_t)r * c->K + h];
+    return NSGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,49 @@
+// Device-side types and helpers shared by the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nsgpu.h"
+
+namespace nsg {
+
+constexpr double kEpsInt = 1e-8;  // Surface.h:286 EPS_INT
+constexpr double kDelta = 1e-10;  // Surface.h:290 delta_accurate_triangulation
+
+// Grid as the kernels see it; node coordinates come from host-computed
+// arrays so that device and host agree bit for bit.
+struct DevGrid {
+    int nx, ny, nz, N;
+    double xmin, ymin, zmin, xmax, ymax, zmax;
+    double scale, side, hside;
+    const double *x, *y, *z;
+};
+
+struct DevPatches {
+    int n;
+    const int *kind, *off, *cnt, *flags;
+    const double *bsph;  // 4 per patch
+    const double *data;
+};
+
+// Ray hit storage: ray r of slot s keeps up to K depths at hits[(s*NN+r)*K].
+struct DevRays {
+    int N;            // rays per panel side (max(nx,ny,nz))
+    int K;            // capacity per ray
+    int nslots;       // panels[0] + panels[1]
+    int panels0;      // grid-ray panels
+    int *cnt;         // [nslots*N*N]
+    double *t;        // [nslots*N*N*K]
+    double *rvol;     // [nslots*N*N] segment length per ray
+};
+
+// rintp, tools.cpp:43-46
+__device__ __forceinline__ double rintp(double x) { return floor(x + 0.5); }
+
+__device__ __forceinline__ int64_t flat(const DevGrid &g, int i, int j, int k) {
+    return ((int64_t)k * g.ny + j) * g.nx + i;
+}
+
+}  // namespace nsg
+
+#define NSG_CHECK_LAUNCH() (void)hipGetLastError()

@@ -1,0 +1,42 @@
+// Host-side launch interface of the kernels (defined in kernels/*.hip).
+#pragma once
+#include "common.hpp"
+
+namespace nsg {
+
+struct RayStats {
+    unsigned long long hits, odd;
+    int max_cnt;
+};
+
+struct FillOpts {
+    int build_eps, build_status, inside;
+    int slot_x, slot_z, slot_y;     // grid-ray slots of panels 0 (x), 1 (z), 2 (y); -1 = not traced
+    int eslot_x, eslot_z, eslot_y;  // edge-ray slots
+};
+
+void launch_intersect(hipStream_t, const DevGrid &, const DevPatches &, const DevRays &, double,
+                      unsigned long long *);
+void launch_sort(hipStream_t, const DevGrid &, const DevRays &, RayStats *);
+void launch_fill_maps(hipStream_t, const DevGrid &, const DevRays &, const FillOpts &, int *, uint8_t *, int *);
+void launch_fill_corners(hipStream_t, const DevGrid &, const DevRays &, const FillOpts &, uint32_t *);
+int64_t sum_partials_count(int64_t n);
+void launch_sum_blocks(hipStream_t, const double *, int64_t, double *);
+void launch_ccl(hipStream_t, const DevGrid &, const int *, int *, int *, int *);
+void launch_ccl_keys(hipStream_t, const DevGrid &, const int *, unsigned int *, int *);
+void launch_ccl_label(hipStream_t, const DevGrid &, const int *, const int *, const uint8_t *, int, int, int, int *,
+                      int *, uint8_t *, uint32_t *);
+int64_t scan_scratch_ints(int64_t n);
+void launch_scan(hipStream_t, const int *, int64_t, int *, int *);
+void mc_upload_tables(hipStream_t);
+void launch_vtx_count(hipStream_t, const DevGrid &, const DevRays &, const int[3], const uint32_t *, int *);
+void launch_vtx_write(hipStream_t, const DevGrid &, const DevRays &, const int[3], const uint32_t *, const int *,
+                      double *, int *);
+int64_t mc_num_cubes(const DevGrid &);
+void launch_mc_vcount(hipStream_t, const DevGrid &, const uint32_t *, const int *, int *);
+void launch_mc_vwrite(hipStream_t, const DevGrid &, const uint32_t *, int *, const int *, int, double *);
+void launch_mc_tcount(hipStream_t, const DevGrid &, const uint32_t *, const int *, int *);
+void launch_mc_twrite(hipStream_t, const DevGrid &, const uint32_t *, const int *, const int *, const double *, int *,
+                      double *);
+
+}  // namespace nsg

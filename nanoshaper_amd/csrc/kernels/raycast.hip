@@ -1,0 +1,360 @@
+// Ray/patch intersection over the grid and edge-ray panels, and the per-ray
+// sort.  One 64-lane workgroup owns one (patch, panel slot): the patch record
+// is loaded once (wave-uniform, scalar path), the lanes stride over the pixel
+// rectangle of the patch's bounding sphere, and every feasible depth is
+// appended to its ray's slot list.  The FP64 path evaluates the reference's
+// expressions in the reference's order (the file is built with
+// -ffp-contract=off), so roots, feasibility and rounding decisions match the
+// CPU code (ConnollySurface::getPatchIntersectionData, ConnollySurface.cpp:
+// 4319-4591; raySphere/rayTorus/quarticEqSolutions, tools.cpp:350-1099).
+#include "launch.hpp"
+
+namespace nsg {
+
+// raySphere, tools.cpp:833-862, for an axis-aligned direction.  The products
+// with the two zero direction components are exact zeros and drop out.
+__device__ __forceinline__ bool ray_sphere(const double o[3], const double dir[3], const double c[3], double r,
+                                           double &t1, double &t2) {
+    const double A = dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2];
+    const double tx = o[0] - c[0], ty = o[1] - c[1], tz = o[2] - c[2];
+    const double B = tx * dir[0] + ty * dir[1] + tz * dir[2];
+    const double C = tx * tx + ty * ty + tz * tz - r * r;
+    double det = B * B - A * C;
+    if (det < 0.) return false;
+    det = sqrt(det);
+    t1 = (-B - det) / A;
+    t2 = (-B + det) / A;
+    if (t2 < t1) {
+        const double tt = t1;
+        t1 = t2;
+        t2 = tt;
+    }
+    return true;
+}
+
+// realCubicSolution, tools.cpp:350-393
+__device__ double cubic_root(double b, double c, double d) {
+    const double p = -(1.0 / 3.0) * b * b + c;
+    const double q = (2. / 27.) * b * b * b - (1. / 3.) * b * c + d;
+    const double D3p = (1. / 27.) * p * p * p;
+    const double D3 = D3p + 0.25 * q * q;
+    if (fabs(c) < b * b * 1.E-6 && fabs(d) < fabs(b) * 1.E-6) return c / b - b - d / (b * b);
+    if (fabs(d / c) < 4.E-4) {
+        const double ratio = d / c;
+        if (D3 <= 0.) return 0.5 * (ratio - b + sqrt((b - ratio) * (b - ratio) - 4. * (c - ratio * (b - ratio))));
+        const double eps = -ratio * ratio * (b - ratio) / (3. * ratio * ratio - 2. * b * ratio + c);
+        return eps - ratio;
+    }
+    if (D3 >= 0.) {
+        const double radD3 = sqrt(D3);
+        const double A = cbrt(radD3 - 0.5 * q);
+        const double psu3A = cbrt(radD3 + 0.5 * q);
+        return A - psu3A - (1. / 3.) * b;
+    }
+    const double Aarg = acos(-0.5 * q / sqrt(-D3p));
+    return 2. * sqrt(-(1. / 3.) * p) * cos((1. / 3.) * Aarg) - (1. / 3.) * b;
+}
+
+// quarticEqSolutions, tools.cpp:525-641: Ferrari's depressed quartic with the
+// resolvent cubic, the discriminant-based root count and the reciprocal
+// variant for large coefficients.
+__device__ int quartic_roots(double roots[4], double b, double c, double d, double e) {
+    const double b_2 = 0.5 * b, b_4 = 0.25 * b;
+    double p = c - 1.5 * (b_2 * b_2);
+    double q = d + b_2 * (b_2 * b_2 - c);
+    double r = e + b_4 * (b_4 * (c - 3.0 * b_4 * b_4) - d);
+    double f = p * p - 4. * r;
+    double delta = 4. * f * (4. * r * (p * p) - p * (q * q) - 16. * (r * r)) + (q * q) * (128. * p * r - 27. * (q * q));
+    bool change_variant = false;
+    if (e != 0. && (fabs(p) > 3.E+3 || fabs(r) > 3.E+3 || fabs(q) > 3.E+3 || fabs(delta) > 1.E+14)) {
+        const double dsu2e = 0.5 * d / e;
+        change_variant = true;
+        p = -1.5 * dsu2e * dsu2e + c / e;
+        q = dsu2e * (dsu2e * dsu2e - c / e) + b / e;
+        r = dsu2e * (-3. / 16. * (dsu2e * dsu2e * dsu2e) + 0.25 * dsu2e * (c / e) - 0.5 * b / e) + 1. / e;
+        f = p * p - 4. * r;
+        delta = 4. * f * (4. * r * (p * p) - p * (q * q) - 16. * (r * r)) + (q * q) * (128. * p * r - 27. * (q * q));
+    }
+    int num;
+    if (delta < 0. || (delta == 0. && (p >= 0. || f < 0.)))
+        num = 2;
+    else if ((p < 0. && f > 0.) || (delta == 0. && p == 0. && r == 0.))
+        num = 4;
+    else
+        return 0;
+    const double m = cubic_root(p, 0.25 * (p * p) - r, (-0.125) * (q * q));
+    if (m <= 0) return 0;
+    double coeff, p1, p2, p3 = 0, smax;
+    if (m < 1.E-14) {
+        p1 = p + p;
+        p2 = (q * q) / f;
+        p3 = 2. * sqrt(f);
+        smax = p3 - p1 - p2;
+        coeff = 2. * fabs(q) / f;
+    } else {
+        p1 = -2. * (p + m);
+        p2 = sqrt(2. * (q * q) / m);
+        smax = p1 + p2;
+        coeff = sqrt(2. * m);
+    }
+    roots[0] = 0.5 * (((q >= 0.) ? -coeff : coeff) - sqrt(smax));
+    roots[1] = 0.5 * (((q >= 0.) ? -coeff : coeff) + sqrt(smax));
+    if (num == 4) {
+        const double smin = (m < 1.E-14) ? -p1 - p2 - p3 : p1 - p2;
+        roots[2] = 0.5 * (((q >= 0.) ? coeff : -coeff) - sqrt(smin));
+        roots[3] = 0.5 * (((q >= 0.) ? coeff : -coeff) + sqrt(smin));
+    }
+    if (change_variant) {
+        for (int i = 0; i < num; i++) roots[i] = 1. / (roots[i] - 0.25 * d / e);
+    } else {
+        for (int i = 0; i < num; i++) roots[i] -= 0.25 * b;
+    }
+    return num;
+}
+
+// rayTorus, tools.cpp:907-1099 (analytical).  t[0] is the clipping-sphere
+// entry; the ray is moved there, brought into the torus frame and the quartic
+// in the distance along the (unit, axis-aligned) direction is solved; roots
+// behind the entry or outside the clipping sphere are dropped.
+__device__ int ray_torus(const double *td, const double sc[3], double probe, double radius, int vc,
+                         const double orig[3], double ray_dir, double t[4]) {
+    double o1[3] = {orig[0], orig[1], orig[2]};
+    o1[vc] += t[0] * ray_dir;
+    const double tx = o1[0] - td[0], ty = o1[1] - td[1], tz = o1[2] - td[2];
+    const double *iv = td + 3;
+    double pp[3];
+    for (int i = 0; i < 3; i++) pp[i] = iv[3 * i + 0] * tx + iv[3 * i + 1] * ty + iv[3 * i + 2] * tz;
+    const double r2 = probe * probe;
+    const double R2 = td[12] * td[12];
+    const double pd = pp[0] * iv[0 + vc] + pp[1] * iv[3 + vc] + pp[2] * iv[6 + vc];
+    const double p2 = pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2];
+    const double coeff = p2 - r2 - R2;
+    const double uz = iv[6 + vc];
+    const double B = 4 * pd;
+    const double C = 2 * coeff + 4 * pd * pd + 4 * R2 * uz * uz;
+    const double D = 4 * pd * coeff + 8 * R2 * pp[2] * uz;
+    const double E = coeff * coeff - 4 * R2 * (r2 - pp[2] * pp[2]);
+    double roots[4];
+    const int nr = quartic_roots(roots, B, C, D, E);
+    const double t0 = t[0];
+    int ni = 0;
+    for (int i = 0; i < nr; i++) {
+        if (roots[i] < 0) continue;
+        double pt[3] = {o1[0], o1[1], o1[2]};
+        pt[vc] += roots[i];
+        const double dd = (sc[0] - pt[0]) * (sc[0] - pt[0]) + (sc[1] - pt[1]) * (sc[1] - pt[1]) +
+                          (sc[2] - pt[2]) * (sc[2] - pt[2]);
+        if (dd < radius * radius) t[ni++] = t0 + roots[i] / ray_dir;
+    }
+    return ni;
+}
+
+// ConnollySurface::isFeasible, ConnollySurface.cpp:4769-5021, on the SoA form
+// (plane signs applied when the patches were built).
+__device__ bool feasible(int kind, const double *d, int cnt, int flags, const double p[3]) {
+    if (kind == NSGPU_PATCH_FACET_REG || kind == NSGPU_PATCH_FACET_SING) {
+        for (int i = 0; i < cnt; i++) {
+            const double *pl = d + 4 * i;
+            const double tst = (pl[0] * p[0] + pl[1] * p[1] + pl[2] * p[2]) + pl[3];
+            if (tst > 0) return false;
+        }
+        return true;
+    }
+    if (kind == NSGPU_PATCH_POINT) {
+        for (int i = 0; i < cnt; i++) {
+            const double *s = d + 4 * i;
+            double dd = (s[0] - p[0]) * (s[0] - p[0]) + (s[1] - p[1]) * (s[1] - p[1]) + (s[2] - p[2]) * (s[2] - p[2]);
+            dd -= s[3];
+            if (dd < 0) return false;
+        }
+        return true;
+    }
+    const double r = d[13];
+    if (r >= 0.) {
+        const double dd = (d[0] - p[0]) * (d[0] - p[0]) + (d[1] - p[1]) * (d[1] - p[1]) + (d[2] - p[2]) * (d[2] - p[2]);
+        if (dd < r * r) return false;
+    }
+    if (kind == NSGPU_PATCH_TORUS_SING) return true;
+    const double *pl = d + NSGPU_TORUS_HDR;
+    for (int i = 0; i < cnt; i++, pl += 8) {
+        const double t1 = (pl[0] * p[0] + pl[1] * p[1] + pl[2] * p[2]) + pl[3];
+        const double t2 = (pl[4] * p[0] + pl[5] * p[1] + pl[6] * p[2]) + pl[7];
+        if ((flags >> i) & 1) {
+            if (t1 > 0 || t2 > 0) continue;
+            return true;
+        }
+        if (t1 <= 0 || t2 <= 0) return true;
+    }
+    return false;
+}
+
+__device__ __forceinline__ void push_hit(const DevRays &R, int slot, int64_t ray, double t) {
+    const int64_t r = (int64_t)slot * R.N * R.N + ray;
+    const int pos = atomicAdd(&R.cnt[r], 1);
+    if (pos < R.K) R.t[r * R.K + pos] = t;
+}
+
+// grid: (numPatches, nslots); block: 64 lanes (one wave)
+__global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRays R, double probe,
+                                                  unsigned long long *cand_total) {
+    const int it = blockIdx.x;
+    const int slot = blockIdx.y;
+    const int phase = slot < R.panels0 ? 0 : 1;
+    const int panel = phase == 0 ? slot : slot - R.panels0;
+    const int kind = P.kind[it];
+    const double sc[3] = {P.bsph[4 * it], P.bsph[4 * it + 1], P.bsph[4 * it + 2]};
+    const double radius = P.bsph[4 * it + 3];
+    const double *pd = P.data + P.off[it];
+    const int pcnt = P.cnt[it], pflags = P.flags[it];
+    const bool torus = kind == NSGPU_PATCH_TORUS_REG || kind == NSGPU_PATCH_TORUS_SING;
+    const double down[3] = {sc[0] - radius, sc[1] - radius, sc[2] - radius};
+    const double up[3] = {sc[0] + radius, sc[1] + radius, sc[2] + radius};
+    // CELL_CULLING (ConnollySurface.cpp:4378-4391)
+    if (down[0] > g.xmax + kEpsInt || down[1] > g.ymax + kEpsInt || down[2] > g.zmax + kEpsInt) return;
+    const double lowpad = g.hside - kDelta + kEpsInt;
+    if (up[0] < g.xmin - lowpad || up[1] < g.ymin - lowpad || up[2] < g.zmin - lowpad) return;
+    const double gmin[3] = {g.xmin, g.ymin, g.zmin};
+    const int gn[3] = {g.nx, g.ny, g.nz};
+    int64_t is[3], ie[3];
+    for (int d = 0; d < 3; ++d) {
+        is[d] = (int64_t)rintp(fmax(0., down[d] - gmin[d]) * g.scale);
+        ie[d] = (int64_t)rintp(fmax(0., up[d] - gmin[d]) * g.scale);
+        if (ie[d] >= gn[d]) ie[d] = gn[d] - 1;
+    }
+    const int64_t pix[3] = {1 + ie[0] - is[0], 1 + ie[1] - is[1], 1 + ie[2] - is[2]};
+    const double delta = (phase == 0) ? 0. : kDelta - g.hside;
+    int fd, ld, vc;
+    const double *ga;   // axis of the ray
+    const double *gf, *gl;
+    if (panel == 0) { fd = 1; ld = 2; vc = 0; ga = g.x; gf = g.y; gl = g.z; }
+    else if (panel == 1) { fd = 0; ld = 1; vc = 2; ga = g.z; gf = g.x; gl = g.y; }
+    else { fd = 0; ld = 2; vc = 1; ga = g.y; gf = g.x; gl = g.z; }
+    const int na = gn[vc];
+    const double ray_dir = ga[na - 1] - ga[0];
+    double dir[3] = {0., 0., 0.};
+    dir[vc] = ray_dir;
+    const int64_t total = pix[fd] * pix[ld];
+    unsigned int cand = 0;
+    for (int64_t rp = threadIdx.x; rp < total; rp += 64) {
+        const int64_t nl = rp / pix[fd];
+        const int64_t m = rp - nl * pix[fd] + is[fd];
+        const int64_t n = nl + is[ld];
+        double pa[3];
+        pa[vc] = ga[0] + delta;
+        pa[fd] = gf[m] + delta;
+        if (pa[fd] < down[fd] || pa[fd] > up[fd]) continue;
+        pa[ld] = gl[n] + delta;
+        if (pa[ld] < down[ld] || pa[ld] > up[ld]) continue;
+        ++cand;
+        // EQ_CULLING (ConnollySurface.cpp:4496-4502)
+        const double dx = sc[fd] - pa[fd], dy = sc[ld] - pa[ld];
+        if (dx * dx + dy * dy > radius * radius) continue;
+        double t[4];
+        if (!ray_sphere(pa, dir, sc, radius, t[0], t[1])) continue;
+        const int ni = torus ? ray_torus(pd, sc, probe, radius, vc, pa, ray_dir, t) : 2;
+        for (int i = 0; i < ni; i++) {
+            double ip[3] = {pa[0], pa[1], pa[2]};
+            ip[vc] += t[i] * ray_dir;
+            if (!feasible(kind, pd, pcnt, pflags, ip)) continue;
+            push_hit(R, slot, n * R.N + m, t[i]);
+        }
+    }
+    // candidate count: wave reduction, one atomic per workgroup
+    for (int off = 32; off > 0; off >>= 1) cand += __shfl_down(cand, off, 64);
+    if (threadIdx.x == 0 && cand) atomicAdd(cand_total, (unsigned long long)cand);
+}
+
+// Per-ray insertion sort of the depths (Surface::reorderPatchIntersections,
+// Surface.cpp:5080-5095), plus the ray's segment-length integral of
+// setVerticesAndGridsWithIntersectionData (Surface.cpp:5265-5290, :5425-5440)
+// and the statistics of the ray lists.
+__global__ void __launch_bounds__(256) k_sort(DevGrid g, DevRays R, RayStats *st) {
+    const int64_t nper = (int64_t)R.N * R.N;
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = nper * R.nslots;
+    int c = 0;
+    unsigned int odd = 0;
+    if (r < total) {
+        c = R.cnt[r];
+        const int n = c < R.K ? c : R.K;
+        double *T = R.t + r * R.K;
+        for (int a = 1; a < n; ++a) {
+            const double v = T[a];
+            int b = a - 1;
+            while (b >= 0 && T[b] > v) {
+                T[b + 1] = T[b];
+                --b;
+            }
+            T[b + 1] = v;
+        }
+        const int slot = (int)(r / nper);
+        const int phase = slot < R.panels0 ? 0 : 1;
+        const int panel = phase == 0 ? slot : slot - R.panels0;
+        const double *ga = panel == 0 ? g.x : (panel == 1 ? g.z : g.y);
+        const int na = panel == 0 ? g.nx : (panel == 1 ? g.nz : g.ny);
+        double pa, dir;
+        if (phase == 0) {
+            pa = ga[0];
+            dir = ga[na - 1] - pa;
+        } else {
+            const double delta = kDelta - g.hside;
+            pa = ga[0] + delta;
+            dir = (ga[na - 1] + delta) - pa;
+        }
+        double vol = 0.0;
+        if (c <= R.K && (c & 1) == 0) {
+            for (int i = 0; i < c; i += 2)
+                if (fabs(T[i + 1] - T[i]) >= kEpsInt) {
+                    const double lim1 = pa + dir * T[i];
+                    const double lim2 = pa + dir * T[i + 1];
+                    vol += lim2 - lim1;
+                }
+        }
+        R.rvol[r] = vol;
+        odd = (c & 1);
+    }
+    // block reduction of the statistics
+    __shared__ unsigned long long sh[256 / 64][2];
+    __shared__ int shm[256 / 64];
+    unsigned long long h = (unsigned long long)c, o = odd;
+    int mx = c;
+    for (int off = 32; off > 0; off >>= 1) {
+        h += __shfl_down(h, off, 64);
+        o += __shfl_down(o, off, 64);
+        mx = max(mx, __shfl_down(mx, off, 64));
+    }
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    if (l == 0) {
+        sh[w][0] = h;
+        sh[w][1] = o;
+        shm[w] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        h = sh[0][0];
+        o = sh[0][1];
+        mx = shm[0];
+        for (int q = 1; q < 256 / 64; ++q) {
+            h += sh[q][0];
+            o += sh[q][1];
+            mx = max(mx, shm[q]);
+        }
+        atomicAdd(&st->hits, h);
+        atomicAdd(&st->odd, o);
+        atomicMax(&st->max_cnt, mx);
+    }
+}
+
+void launch_intersect(hipStream_t s, const DevGrid &g, const DevPatches &P, const DevRays &R, double probe,
+                      unsigned long long *cand) {
+    if (P.n == 0 || R.nslots == 0) return;
+    dim3 grid((unsigned)P.n, (unsigned)R.nslots);
+    hipLaunchKernelGGL(k_intersect, grid, dim3(64), 0, s, g, P, R, probe, cand);
+}
+
+void launch_sort(hipStream_t s, const DevGrid &g, const DevRays &R, RayStats *st) {
+    const int64_t total = (int64_t)R.N * R.N * R.nslots;
+    hipLaunchKernelGGL(k_sort, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, g, R, st);
+}
+
+}  // namespace nsg

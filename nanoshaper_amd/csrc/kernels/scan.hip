@@ -1,0 +1,78 @@
+// Exclusive prefix sum of int32 counts (reduce, scan of block sums, add),
+// used for the ordered compaction of vertices and triangles.  Blocks of 256
+// lanes cover 1024 items; block sums are scanned recursively.
+#include "launch.hpp"
+
+namespace nsg {
+
+constexpr int kScanItems = 1024;
+
+__device__ __forceinline__ int block_excl_scan(int v, int *sh, int &total) {
+    // inclusive scan within waves, then across the 4 waves
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) sh[w] = x;
+    __syncthreads();
+    int base = 0;
+    for (int q = 0; q < w; ++q) base += sh[q];
+    total = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return base + x - v;
+}
+
+__global__ void __launch_bounds__(256) k_scan_reduce(const int *in, int64_t n, int *bsum) {
+    __shared__ int sh[4];
+    const int64_t base = (int64_t)blockIdx.x * kScanItems + threadIdx.x * 4;
+    int v = 0;
+    for (int q = 0; q < 4; ++q)
+        if (base + q < n) v += in[base + q];
+    int total;
+    block_excl_scan(v, sh, total);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(256) k_scan_down(const int *in, int64_t n, const int *boff, int *out) {
+    __shared__ int sh[4];
+    const int64_t base = (int64_t)blockIdx.x * kScanItems + threadIdx.x * 4;
+    int v[4], s = 0;
+    for (int q = 0; q < 4; ++q) {
+        v[q] = (base + q < n) ? in[base + q] : 0;
+        s += v[q];
+    }
+    int total;
+    int run = block_excl_scan(s, sh, total) + (boff ? boff[blockIdx.x] : 0);
+    for (int q = 0; q < 4; ++q) {
+        if (base + q < n) out[base + q] = run;
+        run += v[q];
+    }
+}
+
+// scratch must hold scan_scratch_ints(n) ints; out[n] receives the total when
+// out has n+1 entries (with_total)
+int64_t scan_scratch_ints(int64_t n) {
+    int64_t s = 0;
+    while (n > kScanItems) {
+        n = (n + kScanItems - 1) / kScanItems;
+        s += 2 * n;
+    }
+    return s + 2;
+}
+
+void launch_scan(hipStream_t st, const int *in, int64_t n, int *out, int *scratch) {
+    if (n <= 0) return;
+    const int64_t nb = (n + kScanItems - 1) / kScanItems;
+    if (nb == 1) {
+        hipLaunchKernelGGL(k_scan_down, dim3(1), dim3(256), 0, st, in, n, (const int *)nullptr, out);
+        return;
+    }
+    int *bsum = scratch, *boff = scratch + nb;
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(256), 0, st, in, n, bsum);
+    launch_scan(st, bsum, nb, boff, scratch + 2 * nb);
+    hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(256), 0, st, in, n, boff, out);
+}
+
+}  // namespace nsg

@@ -1,0 +1,291 @@
+"""Host-side mirror of NanoShaper's surface API over the C ABI.
+
+* ``build_grid``        DelPhiShared::buildGrid (DelphiShared.cpp:1038-1201)
+* ``jitter``            ConnollySurface.cpp:2484-2489 with randnum() (tools.cpp:64-84)
+* ``ses_patches``       ConnollySurface::build (host stage, ConnollySurface.cpp:2326-3316)
+* ``DeviceSurface``     Surface::getSurf / triangulateSurface on the GPU
+* ``NanoShaper``        the NS::NanoShaper facade (nanoshaper.h:294-424)
+
+Arrays cross the boundary as NumPy buffers in the DelPhi layouts.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import Grid, MeshInfo, Patches, SurfOpts, SurfResult
+
+PATCH_KINDS = ("point", "facet_regular", "facet_singular", "torus_regular", "torus_singular")
+STATUS_INSIDE, STATUS_TEMP_OUT, STATUS_OUT, STATUS_FIRST_CAV = -1, 2, 3, 4
+STAGES = ("intersect", "sort", "fill", "corners", "cavities", "vertices", "marching_cubes")
+
+
+class NsgpuError(RuntimeError):
+    pass
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(rc, ctx=None, what=""):
+    if rc != 0:
+        lib = _lib.load()
+        msg = lib.nsgpu_last_error(ctx).decode() if ctx else ""
+        raise NsgpuError("%s failed (%d): %s" % (what, rc, msg))
+
+
+def as_xyzr(atoms):
+    a = np.ascontiguousarray(np.asarray(atoms, dtype=np.float64))
+    if a.ndim != 2 or a.shape[1] != 4 or a.shape[0] == 0:
+        raise ValueError("atoms must be an (n, 4) array of x, y, z, radius")
+    return a
+
+
+def build_grid(xyzr, scale=2.0, perfil=80.0):
+    """DelPhiShared::buildGrid: cubic odd grid centred on the atom bounding box."""
+    lib = _lib.load()
+    a = as_xyzr(xyzr)
+    g = Grid()
+    _check(lib.nsgpu_build_grid(_ptr(a), a.shape[0], float(scale), float(perfil), ctypes.byref(g)), None,
+           "build_grid")
+    return g
+
+
+def grid_coords(g):
+    """Node coordinates x[i] = xmin + i*side (DelphiShared.cpp:1135-1142)."""
+    return (g.xmin + np.arange(g.nx) * g.side, g.ymin + np.arange(g.ny) * g.side,
+            g.zmin + np.arange(g.nz) * g.side)
+
+
+def jitter(xyzr, amplitude=1e-4, seed=1234):
+    """Return (jittered copy, new generator state)."""
+    lib = _lib.load()
+    a = as_xyzr(xyzr).copy()
+    s = ctypes.c_int32(seed)
+    lib.nsgpu_jitter(_ptr(a), a.shape[0], float(amplitude), ctypes.byref(s))
+    return a, s.value
+
+
+def generate_chain(n, radius, seed=0):
+    """Seeded self-avoiding chain; returns (xyzr, confinement radius used)."""
+    lib = _lib.load()
+    out = np.zeros((n, 4))
+    ru = ctypes.c_double()
+    got = lib.nsgpu_generate_chain(int(n), float(radius), int(seed), _ptr(out), ctypes.byref(ru))
+    return out[:got].copy(), ru.value
+
+
+class PatchSet:
+    """SES patches built on the host (library-owned), with NumPy views."""
+
+    def __init__(self, handle, probe):
+        self._lib = _lib.load()
+        self._h = handle
+        self.probe = probe
+        self.view = Patches()
+        self._lib.nsgpu_patch_set_view(self._h, ctypes.byref(self.view))
+        n, nd = self.view.n, self.view.n_data
+        self.kind = np.ctypeslib.as_array(self.view.kind, (n,)) if n else np.zeros(0, np.int32)
+        self.off = np.ctypeslib.as_array(self.view.off, (n,)) if n else np.zeros(0, np.int32)
+        self.cnt = np.ctypeslib.as_array(self.view.cnt, (n,)) if n else np.zeros(0, np.int32)
+        self.flags = np.ctypeslib.as_array(self.view.flags, (n,)) if n else np.zeros(0, np.int32)
+        self.atom = np.ctypeslib.as_array(self.view.atom, (n, 3)) if n else np.zeros((0, 3), np.int32)
+        self.bsph = np.ctypeslib.as_array(self.view.bsph, (n, 4)) if n else np.zeros((0, 4))
+        self.data = np.ctypeslib.as_array(self.view.data, (nd,)) if nd else np.zeros(0)
+
+    def counts(self):
+        c = np.zeros(5, np.int64)
+        self._lib.nsgpu_patch_set_counts(self._h, _ptr(c))
+        return dict(zip(PATCH_KINDS, c.tolist()))
+
+    def __len__(self):
+        return self.view.n
+
+    def close(self):
+        if self._h:
+            self._lib.nsgpu_patch_set_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ses_patches(xyzr, probe=1.4):
+    """Build the SES patch set of (already jittered) atoms."""
+    lib = _lib.load()
+    a = as_xyzr(xyzr)
+    h = ctypes.c_void_p()
+    rc = lib.nsgpu_ses_build(_ptr(a), a.shape[0], float(probe), ctypes.byref(h))
+    if rc != 0:
+        if h:
+            lib.nsgpu_patch_set_free(h)
+        raise NsgpuError("ses_build failed (%d)" % rc)
+    return PatchSet(h, float(probe))
+
+
+def surf_opts(build_epsmap=True, build_status=True, accurate_triangulation=True, fill_cavities=True,
+              cav_vol=11.4, inside=5, precision=0):
+    return SurfOpts(int(build_epsmap), int(build_status), int(accurate_triangulation), int(fill_cavities),
+                    float(cav_vol), int(inside), int(precision))
+
+
+def _result_dict(r):
+    return {f: getattr(r, f) for f, _ in SurfResult._fields_}
+
+
+class DeviceSurface:
+    """One device context: grid + patches -> maps, cavities and mesh on the GPU."""
+
+    def __init__(self, device=0):
+        self._lib = _lib.load()
+        h = ctypes.c_void_p()
+        rc = self._lib.nsgpu_create(int(device), ctypes.byref(h))
+        if rc != 0:
+            raise NsgpuError("nsgpu_create failed (%d): no usable HIP device %d" % (rc, device))
+        self._h = h
+        self.grid = None
+        self.result = None
+        self.mesh_info = None
+
+    def _chk(self, rc, what):
+        _check(rc, self._h, what)
+
+    def set_grid(self, g):
+        self._chk(self._lib.nsgpu_set_grid(self._h, ctypes.byref(g)), "set_grid")
+        self.grid = g
+
+    def set_patches(self, ps):
+        self._chk(self._lib.nsgpu_set_patches(self._h, ctypes.byref(ps.view), ps.probe), "set_patches")
+
+    def set_profiling(self, on=True):
+        self._chk(self._lib.nsgpu_set_profiling(self._h, int(on)), "set_profiling")
+
+    def stage_times(self):
+        ms = np.zeros(len(STAGES))
+        self._lib.nsgpu_stage_times(self._h, _ptr(ms), len(STAGES))
+        return dict(zip(STAGES, ms.tolist()))
+
+    def get_surf(self, **kw):
+        o = surf_opts(**kw)
+        r = SurfResult()
+        self._chk(self._lib.nsgpu_get_surf(self._h, ctypes.byref(o), ctypes.byref(r)), "get_surf")
+        self.result = _result_dict(r)
+        return self.result
+
+    def triangulate(self):
+        m = MeshInfo()
+        self._chk(self._lib.nsgpu_triangulate(self._h, ctypes.byref(m)), "triangulate")
+        self.mesh_info = {f: getattr(m, f) for f, _ in MeshInfo._fields_}
+        return self.mesh_info
+
+    def synchronize(self):
+        self._chk(self._lib.nsgpu_synchronize(self._h), "synchronize")
+
+    def _shape(self):
+        g = self.grid
+        return g.nz, g.ny, g.nx
+
+    def epsmap(self):
+        out = np.empty(self._shape() + (3,), np.int32)
+        self._chk(self._lib.nsgpu_read_epsmap(self._h, _ptr(out)), "read_epsmap")
+        return out
+
+    def idebmap(self):
+        out = np.empty(self._shape(), np.uint8)
+        self._chk(self._lib.nsgpu_read_idebmap(self._h, _ptr(out)), "read_idebmap")
+        return out
+
+    def status(self):
+        out = np.empty(self._shape(), np.int32)
+        self._chk(self._lib.nsgpu_read_status(self._h, _ptr(out)), "read_status")
+        return out
+
+    def corner_bits(self):
+        nz, ny, nx = self._shape()
+        out = np.empty((nx * ny * nz + 31) // 32, np.uint32)
+        self._chk(self._lib.nsgpu_read_corner_bits(self._h, _ptr(out)), "read_corner_bits")
+        return out
+
+    def cavities(self):
+        n = self.result["num_cavities"] if self.result else 0
+        sizes = np.zeros(max(n, 1))
+        filled = np.zeros(max(n, 1), np.uint8)
+        self._lib.nsgpu_read_cavities(self._h, _ptr(sizes), _ptr(filled))
+        return sizes[:n], filled[:n].astype(bool)
+
+    def mesh(self):
+        m = self.mesh_info
+        v = np.empty((m["num_vertices"], 3))
+        t = np.empty((m["num_triangles"], 3), np.int32)
+        self._chk(self._lib.nsgpu_read_mesh(self._h, _ptr(v), _ptr(t)), "read_mesh")
+        return v, t
+
+    def ray_hits(self, slot):
+        g = self.grid
+        N = max(g.nx, g.ny, g.nz)
+        maxh = max(1, self.result["max_hits_per_ray"])
+        counts = np.empty(N * N, np.int32)
+        t = np.zeros((N * N, maxh))
+        self._chk(self._lib.nsgpu_read_ray_hits(self._h, int(slot), maxh, _ptr(counts), _ptr(t)), "read_ray_hits")
+        return counts, t
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.nsgpu_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class NanoShaper:
+    """Mirror of NS::NanoShaper (nanoshaper.h:294-424) for the "ses" surface.
+
+    ``build_analytical_surface`` runs the host stages (grid, jitter, patches),
+    ``colour_grid`` runs Surface::getSurf on the device and returns the
+    volume, ``triangulate`` returns the mesh area.
+    """
+
+    def __init__(self, atoms, surface_type="ses", probe=1.4, scale=2.0, perfil=80.0, device=0,
+                 rand_displacement=1e-4, fill_cavities=True, cav_vol=11.4, accurate_triangulation=True):
+        if surface_type != "ses":
+            raise NsgpuError("surface type %r is not available; only 'ses' is built" % surface_type)
+        self.atoms = as_xyzr(atoms)
+        self.probe, self.scale, self.perfil = probe, scale, perfil
+        self.rand_displacement = rand_displacement
+        self.opts = dict(fill_cavities=fill_cavities, cav_vol=cav_vol,
+                         accurate_triangulation=accurate_triangulation)
+        self.device = device
+        self.surface = None
+
+    def build_analytical_surface(self):
+        self.grid = build_grid(self.atoms, self.scale, self.perfil)
+        self.jittered, _ = jitter(self.atoms, self.rand_displacement, 1234)
+        self.patches = ses_patches(self.jittered, self.probe)
+        self.surface = DeviceSurface(self.device)
+        self.surface.set_grid(self.grid)
+        self.surface.set_patches(self.patches)
+        return True
+
+    def colour_grid(self):
+        r = self.surface.get_surf(**self.opts)
+        return r["volume"]
+
+    def triangulate(self):
+        return self.surface.triangulate()["area"]
+
+    def get_grid_point_status(self, ix, iy, iz):
+        """+1 outside, -1 inside (NS::NanoShaper::getGridPointStatus)."""
+        st = self.surface.status()
+        return -1 if st[iz, iy, ix] == STATUS_INSIDE else 1
+
+    def get_grid_size(self):
+        g = self.grid
+        return g.nx, g.ny, g.nz

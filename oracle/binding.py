@@ -1,0 +1,152 @@
+"""ctypes binding of the oracle library (oracle/libns_oracle.so).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, never by the product package.  The interface
+mirrors nanoshaper_amd.DeviceSurface so results can be compared field by field.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libns_oracle.so")
+VP = ctypes.c_void_p
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    from nanoshaper_amd._lib import Grid, MeshInfo, Patches, SurfOpts, SurfResult
+    lib = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "orc_create": (VP, [ctypes.POINTER(Grid), ctypes.POINTER(Patches), ctypes.c_double]),
+        "orc_free": (None, [VP]),
+        "orc_get_surf": (ctypes.c_int, [VP, ctypes.POINTER(SurfOpts), ctypes.POINTER(SurfResult)]),
+        "orc_triangulate": (ctypes.c_int, [VP, ctypes.POINTER(MeshInfo)]),
+        "orc_read_epsmap": (None, [VP, VP]),
+        "orc_read_idebmap": (None, [VP, VP]),
+        "orc_read_status": (None, [VP, VP]),
+        "orc_read_corner_bits": (None, [VP, VP]),
+        "orc_read_cavities": (None, [VP, VP, VP]),
+        "orc_read_mesh": (None, [VP, VP, VP]),
+        "orc_read_ray_hits": (ctypes.c_int, [VP, ctypes.c_int, ctypes.c_int, VP, VP]),
+        "orc_ray_sphere": (ctypes.c_int, [VP, VP, VP, ctypes.c_double, VP, VP]),
+        "orc_cubic": (ctypes.c_double, [ctypes.c_double] * 3),
+        "orc_quartic": (None, [VP, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, VP]),
+        "orc_ray_torus": (ctypes.c_int, [VP, VP, VP, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                         VP, VP, VP, VP]),
+        "orc_is_feasible": (ctypes.c_int, [ctypes.POINTER(Patches), ctypes.c_int, VP]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class OracleSurface:
+    """CPU restatement of Surface::getSurf / triangulateSurface on one patch set."""
+
+    def __init__(self, grid, patchset):
+        from nanoshaper_amd._lib import Grid  # noqa: F401
+        self._lib = load()
+        self.grid = grid
+        self._h = self._lib.orc_create(ctypes.byref(grid), ctypes.byref(patchset.view), patchset.probe)
+        self.result = None
+        self.mesh_info = None
+
+    def get_surf(self, **kw):
+        from nanoshaper_amd._lib import SurfResult
+        from nanoshaper_amd.surface import surf_opts
+        o = surf_opts(**kw)
+        r = SurfResult()
+        rc = self._lib.orc_get_surf(self._h, ctypes.byref(o), ctypes.byref(r))
+        if rc != 0:
+            raise RuntimeError("oracle get_surf failed")
+        self.result = {f: getattr(r, f) for f, _ in SurfResult._fields_}
+        return self.result
+
+    def triangulate(self):
+        from nanoshaper_amd._lib import MeshInfo
+        m = MeshInfo()
+        if self._lib.orc_triangulate(self._h, ctypes.byref(m)) != 0:
+            raise RuntimeError("oracle triangulate needs accurate_triangulation")
+        self.mesh_info = {f: getattr(m, f) for f, _ in MeshInfo._fields_}
+        return self.mesh_info
+
+    def _shape(self):
+        g = self.grid
+        return g.nz, g.ny, g.nx
+
+    def epsmap(self):
+        out = np.empty(self._shape() + (3,), np.int32)
+        self._lib.orc_read_epsmap(self._h, _ptr(out))
+        return out
+
+    def idebmap(self):
+        out = np.empty(self._shape(), np.uint8)
+        self._lib.orc_read_idebmap(self._h, _ptr(out))
+        return out
+
+    def status(self):
+        out = np.empty(self._shape(), np.int32)
+        self._lib.orc_read_status(self._h, _ptr(out))
+        return out
+
+    def corner_bits(self):
+        nz, ny, nx = self._shape()
+        out = np.empty((nx * ny * nz + 31) // 32, np.uint32)
+        self._lib.orc_read_corner_bits(self._h, _ptr(out))
+        return out
+
+    def cavities(self):
+        n = self.result["num_cavities"]
+        sizes = np.zeros(max(n, 1))
+        filled = np.zeros(max(n, 1), np.uint8)
+        self._lib.orc_read_cavities(self._h, _ptr(sizes), _ptr(filled))
+        return sizes[:n], filled[:n].astype(bool)
+
+    def mesh(self):
+        m = self.mesh_info
+        v = np.empty((m["num_vertices"], 3))
+        t = np.empty((m["num_triangles"], 3), np.int32)
+        self._lib.orc_read_mesh(self._h, _ptr(v), _ptr(t))
+        return v, t
+
+    def ray_hits(self, slot):
+        g = self.grid
+        N = max(g.nx, g.ny, g.nz)
+        maxh = max(1, self.result["max_hits_per_ray"])
+        counts = np.empty(N * N, np.int32)
+        t = np.zeros((N * N, maxh))
+        self._lib.orc_read_ray_hits(self._h, int(slot), maxh, _ptr(counts), _ptr(t))
+        return counts, t
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.orc_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
