@@ -1,0 +1,176 @@
+"""Benchmark of the grid ray-casting SES builder on one MI355X.
+
+Workload: 5,000-atom seeded chain, probe 1.4 A, grid 4.0 points/A (perfil 80),
+epsmap + idebmap + status, 3 grid-ray + 3 edge-ray panels, cavity detection
+and filling, analytic marching cubes.  A step is one Surface::getSurf plus
+triangulateSurface over the device-resident patch set; host stages (grid,
+jitter, patch construction) run once before timing.
+
+Prints one JSON line: rays/s (all six panels) as the value, with voxels/s,
+stage times, the roofline of the dominant kernel and the CPU baseline (the
+oracle, single threaded, on the same workload) beside it.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+import nanoshaper_amd as ns  # noqa: E402
+
+# MI355X published peaks
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+# FP64 operations per path of the ray/patch candidate set, counted statically
+# in the intersection kernel (add/sub/mul/div/sqrt/cbrt/acos/cos = 1 op):
+# bounding-box candidate: ray origin (3) + disc cull (6); disc pass: ray/sphere
+# up to the discriminant (23); sphere hit: roots (5); torus: frame change and
+# quartic coefficients (55) + resolvent cubic and Ferrari roots (80) + root
+# checks (25); tested root: hit point (2); feasibility item: one plane or
+# clipping-sphere test (6).
+OPS = {"candidates": 9, "disc_pass": 23, "sphere_hits": 5, "torus_quartics": 160, "roots_tested": 2,
+       "feas_items": 6}
+
+CONFIGS = {
+    # name: atoms, confinement radius, grid scale, perfil
+    "c2": (5000, 28.0, 4.0, 80.0),
+    "c1": (330, 12.0, 2.0, 90.0),
+    "c5": (2000, 21.0, 3.0, 80.0),
+}
+
+
+def build_inputs(cfg):
+    n, radius, scale, perfil = CONFIGS[cfg]
+    t0 = time.perf_counter()
+    xyzr, r_used = ns.generate_chain(n, radius, seed=0)
+    grid = ns.build_grid(xyzr, scale, perfil)
+    jit, _ = ns.jitter(xyzr)
+    ps = ns.ses_patches(jit, 1.4)
+    return xyzr, r_used, grid, ps, time.perf_counter() - t0
+
+
+def cpu_baseline(grid, ps, budget_s):
+    """Oracle (single-threaded C restatement) on the full workload, repeated
+    while the budget lasts (at least once)."""
+    from oracle.binding import OracleSurface
+    orc = OracleSurface(grid, ps)
+    times, rays = [], 0
+    t_all = time.perf_counter()
+    while True:
+        t = time.perf_counter()
+        r = orc.get_surf()
+        orc.triangulate()
+        times.append(time.perf_counter() - t)
+        rays = r["total_rays"]
+        if time.perf_counter() - t_all + times[-1] > budget_s:
+            break
+    orc.close()
+    sec = float(np.median(times))
+    return {"value": rays / sec, "unit": "rays/s", "cores": 1, "kind": "port",
+            "sample": "full workload, %d run(s), median %.3f s per step" % (len(times), sec)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    xyzr, r_used, grid, ps, host_s = build_inputs(args.config)
+    print("inputs: %d atoms (R %.2f A), grid %d^3, %d patches %s, host build %.3f s" %
+          (len(xyzr), r_used, grid.nx, len(ps), ps.counts(), host_s), file=sys.stderr, flush=True)
+    dev = ns.DeviceSurface(0)
+    dev.set_grid(grid)
+    dev.set_patches(ps)
+    opts = dict(precision=args.precision)
+    for _ in range(args.warmup):
+        dev.get_surf(**opts)
+        dev.triangulate()
+    dev.synchronize()
+    print("warmup done", file=sys.stderr, flush=True)
+
+    dev.set_profiling(True)
+    stage_sum = {s: 0.0 for s in ns.STAGES}
+    dev.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = dev.get_surf(**opts)
+        m = dev.triangulate()
+        st = dev.stage_times()
+        for k, v in st.items():
+            stage_sum[k] += v
+    dev.synchronize()
+    elapsed = time.perf_counter() - t0
+    dev.set_profiling(False)
+    stage_avg = {k: v / args.steps for k, v in stage_sum.items()}
+    ms_per_step = elapsed * 1e3 / args.steps
+    nvox = grid.nx * grid.ny * grid.nz
+    rays = r["total_rays"]
+
+    # roofline of the dominant kernel (the ray/patch intersection)
+    pc = dev.intersect_path_counts(**opts)
+    flops = sum(OPS[k] * pc[k] for k in OPS)
+    t_int = stage_avg["intersect"] * 1e-3
+    achieved = flops / t_int / 1e12 if t_int > 0 else 0.0
+    roofline = {"kernel": "k_intersect", "bound": "fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                "flops_per_launch": flops, "path_counts": pc}
+    fill_bytes = 17.0 * nvox
+    t_fill = stage_avg["fill"] * 1e-3
+    fill_roof = {"kernel": "k_fill_maps", "bound": "hbm", "bytes_per_launch": fill_bytes,
+                 "achieved_GBs": fill_bytes / t_fill / 1e9 if t_fill > 0 else 0.0, "peak_GBs": PEAK_HBM_GBS}
+    fill_roof["frac"] = fill_roof["achieved_GBs"] / PEAK_HBM_GBS
+
+    cpu = None
+    if not args.no_cpu:
+        print("cpu baseline ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(grid, ps, args.cpu_budget)
+    value = rays * args.steps / elapsed
+    out = {
+        "metric": "SES rays/s & voxels/s at 1e-6 A hit err; batched frames/s; fwd+bwd ms",
+        "value": value,
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == 0 else "f32-cull/f64",
+        "data": "synthetic",
+        "config": {"workload": "plain SES %s: %d-atom seeded chain (R %.2f A), probe 1.4 A, grid %.1f/A "
+                               "perfil %.0f -> %d^3, maps+cavities+MC, 6 panels" %
+                               (args.config, len(xyzr), r_used, CONFIGS[args.config][2], CONFIGS[args.config][3],
+                                grid.nx),
+                   "atoms": len(xyzr), "grid": grid.nx, "patches": len(ps), "rays_per_step": rays,
+                   "voxels": nvox},
+        "voxels_per_s": nvox * args.steps / elapsed,
+        "stage_ms": stage_avg,
+        "roofline": roofline,
+        "roofline_fill": fill_roof,
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "result": {"volume": r["volume"], "area": m["area"], "cavities": r["num_cavities"],
+                   "triangles": m["num_triangles"], "hits": r["total_hits"], "odd_rays": r["odd_rays"]},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

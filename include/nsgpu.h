@@ -200,6 +200,13 @@ int nsgpu_synchronize(nsgpu_ctx *ctx);
 int nsgpu_set_profiling(nsgpu_ctx *ctx, int on);
 int nsgpu_stage_times(nsgpu_ctx *ctx, double *ms, int n);
 
+/* Path counts of the ray/patch candidate set for the roofline (no
+ * counterpart in the reference).  Runs a counting copy of the intersection
+ * kernel that writes no hits; counts[8] receives: bounding-box candidates,
+ * disc-cull passes, ray/sphere hits, torus quartics, roots tested,
+ * feasible hits, feasibility items tested, feasibility calls. */
+int nsgpu_intersect_path_counts(nsgpu_ctx *ctx, const nsgpu_surf_opts *opts, int64_t *counts, int n);
+
 /* -------------------- read back (caller-owned buffers, DelPhi layouts) ----- */
 /* epsmap int32 [((k*ny+j)*nx+i)*3+l]; idebmap uint8 [(k*ny+j)*nx+i], 1 = out;
  * status int32 [(k*ny+j)*nx+i]; corner bits uint32 [ceil(nx*ny*nz/32)],

@@ -65,6 +65,7 @@ SIGNATURES = {
     "nsgpu_synchronize": (ctypes.c_int, [VP]),
     "nsgpu_set_profiling": (ctypes.c_int, [VP, ctypes.c_int]),
     "nsgpu_stage_times": (ctypes.c_int, [VP, VP, ctypes.c_int]),
+    "nsgpu_intersect_path_counts": (ctypes.c_int, [VP, ctypes.POINTER(SurfOpts), VP, ctypes.c_int]),
     "nsgpu_read_epsmap": (ctypes.c_int, [VP, VP]),
     "nsgpu_read_idebmap": (ctypes.c_int, [VP, VP]),
     "nsgpu_read_status": (ctypes.c_int, [VP, VP]),

@@ -247,6 +247,39 @@ int nsgpu_set_patches(nsgpu_ctx *c, const nsgpu_patches *p, double probe_radius)
     return NSGPU_OK;
 }
 
+int nsgpu_intersect_path_counts(nsgpu_ctx *c, const nsgpu_surf_opts *o, int64_t *counts, int n) {
+    if (!c || !o || !counts || n < 8) return NSGPU_E_ARG;
+    if (!c->have_grid || !c->have_patches) return fail(c, NSGPU_E_STATE, "set_grid and set_patches first");
+    HIP_TRY(c, hipSetDevice(c->device));
+    int panels[2] = {3, 3};
+    if (!o->build_status && !o->build_epsmap) panels[0] = 0;
+    else if (o->build_status && !o->build_epsmap) panels[0] = 1;
+    if (!o->accurate_triangulation) panels[1] = 0;
+    const DevGrid g = dev_grid(c);
+    DevRays R = dev_rays(c);
+    R.nslots = panels[0] + panels[1];
+    R.panels0 = panels[0];
+    DevPatches P;
+    P.n = c->npatch;
+    P.kind = c->pkind.as<int>();
+    P.off = c->poff.as<int>();
+    P.cnt = c->pcnt.as<int>();
+    P.flags = c->pflags.as<int>();
+    P.bsph = c->pbsph.as<double>();
+    P.data = c->pdata.as<double>();
+    DBuf cnt;
+    HIP_TRY(c, cnt.ensure(8 * 8));
+    HIP_TRY(c, hipMemsetAsync(cnt.p, 0, 64, c->stream));
+    launch_intersect_count(c->stream, g, P, R, c->probe, cnt.as<unsigned long long>());
+    HIP_TRY(c, hipGetLastError());
+    unsigned long long h[8];
+    HIP_TRY(c, hipMemcpyAsync(h, cnt.p, 64, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    cnt.release();
+    for (int q = 0; q < 8; ++q) counts[q] = (int64_t)h[q];
+    return NSGPU_OK;
+}
+
 int nsgpu_synchronize(nsgpu_ctx *c) {
     if (!c) return NSGPU_E_ARG;
     HIP_TRY(c, hipSetDevice(c->device));

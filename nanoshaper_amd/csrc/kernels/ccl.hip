@@ -1,7 +1,7 @@
 // Cavity detection as connected-component labelling of the TEMP_OUT nodes
-// (6-connectivity) with a lock-free union-find: links always go from the
-// larger root to the smaller, so each component's root is its smallest flat
-// index, i.e. the first node the forward scan of
+// (6-connectivity) with a lock-free union-find over the x-runs of TEMP_OUT
+// nodes: links always go from the larger root to the smaller, so each
+// component's root is its smallest flat index, i.e. the first node the forward scan of
 // getCavitiesWithBilevelStatusMap (Surface.cpp:1994-2126) meets.  The first
 // node of the backward-z scan is recorded per component as a second key; the
 // host replays the alternating scan order over the components to give each
@@ -41,28 +41,60 @@ __device__ void unite(int *P, int a, int b) {
     }
 }
 
-__global__ void __launch_bounds__(256) k_ccl_init(const int *status, int *P, int64_t nvox) {
-    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (f < nvox) P[f] = status[f] == NSGPU_STATUS_TEMP_OUT ? (int)f : -1;
+// Runs of TEMP_OUT nodes along x are the union-find nodes: one wave walks a
+// row in 64-node chunks, finds run starts with a ballot and points every node
+// at the first node of its run (the run's smallest flat index).
+__global__ void __launch_bounds__(256) k_ccl_rows(DevGrid g, const int *status, int *P) {
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= (int64_t)g.ny * g.nz) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t base = row * g.nx;
+    int carry = 0;  // run start (x index) of a run entering the chunk from the left
+    for (int c0 = 0; c0 < g.nx; c0 += 64) {
+        const int i = c0 + lane;
+        const bool in = i < g.nx;
+        const bool t = in && status[base + i] == NSGPU_STATUS_TEMP_OUT;
+        const unsigned long long stop = __ballot(in && !t);
+        const unsigned long long below = stop & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
+        const int start = below ? c0 + (63 - __clzll(below)) + 1 : carry;
+        if (in) P[base + i] = t ? (int)(base + start) : -1;
+        // the run reaching the end of this chunk continues into the next one
+        const bool last_t = __shfl(t ? 1 : 0, 63, 64) != 0;
+        const int last_start = __shfl(start, 63, 64);
+        carry = last_t ? last_start : c0 + 64;
+    }
 }
 
-__global__ void __launch_bounds__(256) k_ccl_merge(DevGrid g, int *P) {
+// Links between runs of neighbouring rows (+y) and planes (+z), made once per
+// overlap interval: at a node whose left neighbour does not also overlap.
+__global__ void __launch_bounds__(256) k_ccl_link(DevGrid g, const int *status, int *P) {
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
     const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nvox || ld(P + f) < 0) return;
+    if (f >= nvox || status[f] != NSGPU_STATUS_TEMP_OUT) return;
     const int i = (int)(f % g.nx);
     const int j = (int)((f / g.nx) % g.ny);
     const int k = (int)(f / ((int64_t)g.nx * g.ny));
     const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
-    // skip the link when the neighbour already has the same parent (the common case)
-    if (i + 1 < g.nx && ld(P + f + 1) >= 0 && ld(P + f + 1) != ld(P + f)) unite(P, (int)f, (int)(f + 1));
-    if (j + 1 < g.ny && ld(P + f + sy) >= 0) unite(P, (int)f, (int)(f + sy));
-    if (k + 1 < g.nz && ld(P + f + sz) >= 0) unite(P, (int)f, (int)(f + sz));
+    const bool left = i > 0 && status[f - 1] == NSGPU_STATUS_TEMP_OUT;
+    if (j + 1 < g.ny && status[f + sy] == NSGPU_STATUS_TEMP_OUT &&
+        !(left && status[f - 1 + sy] == NSGPU_STATUS_TEMP_OUT))
+        unite(P, ld(P + f), ld(P + f + sy));
+    if (k + 1 < g.nz && status[f + sz] == NSGPU_STATUS_TEMP_OUT &&
+        !(left && status[f - 1 + sz] == NSGPU_STATUS_TEMP_OUT))
+        unite(P, ld(P + f), ld(P + f + sz));
 }
 
+// Every node is pointed at its root.  The walk only reads: a compressing walk
+// could overwrite a node another lane has just flattened with a stale
+// ancestor.  Each lane writes its own entry only, so after the kernel every
+// entry is the root.
 __global__ void __launch_bounds__(256) k_ccl_flatten(int *P, int64_t nvox) {
     const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (f < nvox && ld(P + f) >= 0) st(P + f, find_root(P, (int)f));
+    if (f >= nvox) return;
+    int x = ld(P + f);
+    if (x < 0) return;
+    for (int p = ld(P + x); p != x; p = ld(P + x)) x = p;
+    st(P + f, x);
 }
 
 // roots get a compact slot, stored in place as -(slot+2)
@@ -81,17 +113,43 @@ __device__ __forceinline__ int slot_of(const int *P, int64_t f) {
     return p < -1 ? -(p + 2) : -(P[p] + 2);
 }
 
+// Size and backward-scan key of every component.  The component of node 0
+// (when node 0 is TEMP_OUT) is the first one any scan meets, so it is always
+// labelled first and neither key is needed for it: it is skipped, which
+// removes the only heavily contended address (the outside region).  Waves
+// whose lanes all belong to one component combine before the atomics.
 __global__ void __launch_bounds__(256) k_ccl_keys(DevGrid g, const int *P, unsigned int *bkey, int *count) {
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
     const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nvox) return;
-    const int s = slot_of(P, f);
-    if (s < 0) return;
-    const int64_t plane = (int64_t)g.nx * g.ny;
-    const int64_t k = f / plane;
-    const unsigned int rank = (unsigned int)((g.nz - 1 - k) * plane + (f - k * plane));
-    atomicMin(bkey + s, rank);
-    atomicAdd(count + s, 1);
+    const int skip = slot_of(P, 0);
+    int s = -1;
+    unsigned int rank = 0xffffffffu;
+    if (f < nvox) {
+        s = slot_of(P, f);
+        if (s == skip) s = -1;
+        const int64_t plane = (int64_t)g.nx * g.ny;
+        const int64_t k = f / plane;
+        rank = (unsigned int)((g.nz - 1 - k) * plane + (f - k * plane));
+    }
+    const bool valid = s >= 0;
+    const unsigned long long act = __ballot(valid);
+    if (act == 0) return;
+    const int leader = __ffsll((long long)act) - 1;
+    const int s0 = __shfl(s, leader, 64);
+    if (__ballot(valid && s == s0) == act) {
+        unsigned int r = valid ? rank : 0xffffffffu;
+        for (int off = 32; off > 0; off >>= 1) r = min(r, (unsigned int)__shfl_down((int)r, off, 64));
+        r = (unsigned int)__shfl((int)r, 0, 64);
+        if ((int)(threadIdx.x & 63) == leader) {
+            atomicMin(bkey + s0, r);
+            atomicAdd(count + s0, __popcll(act));
+        }
+        return;
+    }
+    if (valid) {
+        atomicMin(bkey + s, rank);
+        atomicAdd(count + s, 1);
+    }
 }
 
 // final labels and fillCavities (Surface.cpp:2355-2487) for the filled ones
@@ -133,8 +191,9 @@ static inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 void launch_ccl(hipStream_t s, const DevGrid &g, const int *status, int *P, int *nroots, int *rootlist) {
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
-    hipLaunchKernelGGL(k_ccl_init, dim3(nblk(nvox)), dim3(256), 0, s, status, P, nvox);
-    hipLaunchKernelGGL(k_ccl_merge, dim3(nblk(nvox)), dim3(256), 0, s, g, P);
+    const int64_t rows = (int64_t)g.ny * g.nz;
+    hipLaunchKernelGGL(k_ccl_rows, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, s, g, status, P);
+    hipLaunchKernelGGL(k_ccl_link, dim3(nblk(nvox)), dim3(256), 0, s, g, status, P);
     hipLaunchKernelGGL(k_ccl_flatten, dim3(nblk(nvox)), dim3(256), 0, s, P, nvox);
     hipLaunchKernelGGL(k_ccl_roots, dim3(nblk(nvox)), dim3(256), 0, s, P, nvox, nroots, rootlist);
 }

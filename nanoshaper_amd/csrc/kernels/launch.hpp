@@ -17,6 +17,8 @@ struct FillOpts {
 
 void launch_intersect(hipStream_t, const DevGrid &, const DevPatches &, const DevRays &, double,
                       unsigned long long *);
+void launch_intersect_count(hipStream_t, const DevGrid &, const DevPatches &, const DevRays &, double,
+                            unsigned long long *);
 void launch_sort(hipStream_t, const DevGrid &, const DevRays &, RayStats *);
 void launch_fill_maps(hipStream_t, const DevGrid &, const DevRays &, const FillOpts &, int *, uint8_t *, int *);
 void launch_fill_corners(hipStream_t, const DevGrid &, const DevRays &, const FillOpts &, uint32_t *);

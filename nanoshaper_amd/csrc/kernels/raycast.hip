@@ -194,7 +194,11 @@ __device__ __forceinline__ void push_hit(const DevRays &R, int slot, int64_t ray
     if (pos < R.K) R.t[r * R.K + pos] = t;
 }
 
+// Path counters of the counting variant (see intersect_path_counts).
+enum { PC_CAND, PC_DISC, PC_SPHERE, PC_TORUS, PC_ROOTS, PC_HITS, PC_FEAS_ITEMS, PC_FEAS_CALLS, PC_N };
+
 // grid: (numPatches, nslots); block: 64 lanes (one wave)
+template <bool kCount>
 __global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRays R, double probe,
                                                   unsigned long long *cand_total) {
     const int it = blockIdx.x;
@@ -235,6 +239,7 @@ __global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRa
     dir[vc] = ray_dir;
     const int64_t total = pix[fd] * pix[ld];
     unsigned int cand = 0;
+    unsigned int pc[PC_N] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int64_t rp = threadIdx.x; rp < total; rp += 64) {
         const int64_t nl = rp / pix[fd];
         const int64_t m = rp - nl * pix[fd] + is[fd];
@@ -249,15 +254,37 @@ __global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRa
         // EQ_CULLING (ConnollySurface.cpp:4496-4502)
         const double dx = sc[fd] - pa[fd], dy = sc[ld] - pa[ld];
         if (dx * dx + dy * dy > radius * radius) continue;
+        if (kCount) ++pc[PC_DISC];
         double t[4];
         if (!ray_sphere(pa, dir, sc, radius, t[0], t[1])) continue;
-        const int ni = torus ? ray_torus(pd, sc, probe, radius, vc, pa, ray_dir, t) : 2;
+        if (kCount) ++pc[PC_SPHERE];
+        int ni = 2;
+        if (torus) {
+            ni = ray_torus(pd, sc, probe, radius, vc, pa, ray_dir, t);
+            if (kCount) ++pc[PC_TORUS];
+        }
         for (int i = 0; i < ni; i++) {
             double ip[3] = {pa[0], pa[1], pa[2]};
             ip[vc] += t[i] * ray_dir;
-            if (!feasible(kind, pd, pcnt, pflags, ip)) continue;
-            push_hit(R, slot, n * R.N + m, t[i]);
+            const bool ok = feasible(kind, pd, pcnt, pflags, ip);
+            if (kCount) {
+                ++pc[PC_ROOTS];
+                ++pc[PC_FEAS_CALLS];
+                pc[PC_FEAS_ITEMS] += (kind >= NSGPU_PATCH_TORUS_REG) ? 2 * pcnt + 1 : pcnt;
+                pc[PC_HITS] += ok;
+            }
+            if (!ok) continue;
+            if (!kCount) push_hit(R, slot, n * R.N + m, t[i]);
         }
+    }
+    if (kCount) {
+        pc[PC_CAND] = cand;
+        for (int q = 0; q < PC_N; ++q) {
+            unsigned long long v = pc[q];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+            if (threadIdx.x == 0 && v) atomicAdd(cand_total + q, v);
+        }
+        return;
     }
     // candidate count: wave reduction, one atomic per workgroup
     for (int off = 32; off > 0; off >>= 1) cand += __shfl_down(cand, off, 64);
@@ -349,7 +376,16 @@ void launch_intersect(hipStream_t s, const DevGrid &g, const DevPatches &P, cons
                       unsigned long long *cand) {
     if (P.n == 0 || R.nslots == 0) return;
     dim3 grid((unsigned)P.n, (unsigned)R.nslots);
-    hipLaunchKernelGGL(k_intersect, grid, dim3(64), 0, s, g, P, R, probe, cand);
+    hipLaunchKernelGGL(k_intersect<false>, grid, dim3(64), 0, s, g, P, R, probe, cand);
+}
+
+// Counting variant: same candidate walk, no hit output; counters[PC_N] get
+// the number of times each FP64 path of the reference's candidate set runs.
+void launch_intersect_count(hipStream_t s, const DevGrid &g, const DevPatches &P, const DevRays &R, double probe,
+                            unsigned long long *counters) {
+    if (P.n == 0 || R.nslots == 0) return;
+    dim3 grid((unsigned)P.n, (unsigned)R.nslots);
+    hipLaunchKernelGGL(k_intersect<true>, grid, dim3(64), 0, s, g, P, R, probe, counters);
 }
 
 void launch_sort(hipStream_t s, const DevGrid &g, const DevRays &R, RayStats *st) {

@@ -169,6 +169,14 @@ class DeviceSurface:
         self._lib.nsgpu_stage_times(self._h, _ptr(ms), len(STAGES))
         return dict(zip(STAGES, ms.tolist()))
 
+    def intersect_path_counts(self, **kw):
+        """Counts of each FP64 path of the ray/patch candidate set (no hits written)."""
+        o = surf_opts(**kw)
+        c = np.zeros(8, np.int64)
+        self._chk(self._lib.nsgpu_intersect_path_counts(self._h, ctypes.byref(o), _ptr(c), 8), "path_counts")
+        return dict(zip(("candidates", "disc_pass", "sphere_hits", "torus_quartics", "roots_tested",
+                         "feasible_hits", "feas_items", "feas_calls"), c.tolist()))
+
     def get_surf(self, **kw):
         o = surf_opts(**kw)
         r = SurfResult()
