@@ -1,0 +1,257 @@
+// nsgpu_surface.hpp - header-only C++ plugin layer over the C ABI (nsgpu.h).
+//
+// Mirrors the reference's surface plugin interface so that code written
+// against Surface / SurfaceFactory / ConfigFile / DelPhiShared keeps its shape:
+//   ConfigFile            ConfigFile/ConfigFile.h:59-269 ("key = value", '#' comments)
+//   DelPhiShared          DelphiShared.h:43-216 (atoms, grid, caller-visible maps)
+//   Surface               Surface.h:835-979 (build, getSurf, triangulateSurface)
+//   ConnollySurface "ses" ConnollySurface.h:682-692 (factory registration)
+//   SurfaceFactory        SurfaceFactory.h:15-65 (name -> instantiator, key "Surface")
+// Everything on the device goes through the extern "C" entry points; this
+// header adds no symbols to libnsgpu.so.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "nsgpu.h"
+
+namespace nsgpu {
+
+// key = value configuration, as the reference's ConfigFile
+class ConfigFile {
+   public:
+    ConfigFile() = default;
+    explicit ConfigFile(const std::string &path) {
+        std::ifstream in(path);
+        if (!in) throw std::runtime_error("cannot open config " + path);
+        std::string line;
+        while (std::getline(in, line)) parse_line(line);
+    }
+    void parse_line(std::string line) {
+        const size_t hash = line.find('#');
+        if (hash != std::string::npos) line.erase(hash);
+        const size_t eq = line.find('=');
+        if (eq == std::string::npos) return;
+        std::string k = trim(line.substr(0, eq)), v = trim(line.substr(eq + 1));
+        if (!k.empty()) kv_[k] = v;
+    }
+    bool keyExists(const std::string &k) const { return kv_.count(k) != 0; }
+    template <class T>
+    T read(const std::string &k, const T &def) const {
+        auto it = kv_.find(k);
+        if (it == kv_.end()) return def;
+        return convert<T>(it->second);
+    }
+    template <class T>
+    void add(const std::string &k, const T &v) {
+        std::ostringstream os;
+        os << v;
+        kv_[k] = os.str();
+    }
+
+   private:
+    static std::string trim(const std::string &s) {
+        const size_t a = s.find_first_not_of(" \t\r\n"), b = s.find_last_not_of(" \t\r\n");
+        return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
+    }
+    template <class T>
+    static T convert(const std::string &s) {
+        T v{};
+        std::istringstream is(s);
+        is >> v;
+        return v;
+    }
+    std::map<std::string, std::string> kv_;
+};
+
+template <>
+inline bool ConfigFile::convert<bool>(const std::string &s) {
+    std::string l;
+    for (char c : s) l += (char)std::tolower((unsigned char)c);
+    return l == "true" || l == "1" || l == "yes";
+}
+template <>
+inline std::string ConfigFile::convert<std::string>(const std::string &s) { return s; }
+
+// Atoms, grid and the maps the caller reads after getSurf.
+class DelPhiShared {
+   public:
+    DelPhiShared(bool buildEpsMap = true, bool buildStatus = true) : buildEpsMap(buildEpsMap), buildStatus(buildStatus) {}
+    // xyzr: n*4 doubles
+    void loadAtoms(const std::vector<double> &xyzr) {
+        if (xyzr.empty() || xyzr.size() % 4) throw std::runtime_error("atoms must be x,y,z,r quadruples");
+        atoms = xyzr;
+    }
+    bool buildGrid(double scale_, double perfil_) {
+        return nsgpu_build_grid(atoms.data(), (int32_t)(atoms.size() / 4), scale_, perfil_, &grid) == NSGPU_OK;
+    }
+    int64_t numNodes() const { return (int64_t)grid.nx * grid.ny * grid.nz; }
+
+    bool buildEpsMap, buildStatus;
+    std::vector<double> atoms;
+    nsgpu_grid grid{};
+    std::vector<int32_t> epsmap;   // [((k*ny+j)*nx+i)*3+l]
+    std::vector<uint8_t> idebmap;  // 1 = out
+    std::vector<int32_t> status;   // -1 in, 3 out, >=4 cavity
+    std::vector<double> cavitiesSize;
+    std::vector<uint8_t> cavitiesFlag;
+};
+
+// Base class of the surfaces; the ray caster is the device path.
+class Surface {
+   public:
+    Surface(ConfigFile *cf, DelPhiShared *ds) : delphi(ds) {
+        if (cf) {
+            inside = cf->read<int>("Inside_code", inside);
+            accurateTriangulation = cf->read<bool>("Accurate_Triangulation", accurateTriangulation);
+            probe_radius = cf->read<double>("Probe_Radius", probe_radius);
+            device = cf->read<int>("Device", device);
+        }
+    }
+    virtual ~Surface() {
+        if (ctx_) nsgpu_destroy(ctx_);
+    }
+    Surface(const Surface &) = delete;
+    Surface &operator=(const Surface &) = delete;
+
+    virtual bool build() = 0;
+    virtual int getNumPatches() = 0;
+    virtual bool isPatchBasedRayTracingSupported() { return true; }
+    virtual const nsgpu_patches *patches() = 0;
+
+    // Surface::getSurf(&vol, optimizeGrids, fillCav, cavVol): colours the
+    // DelPhi maps (read back into delphi) and returns true on success.
+    bool getSurf(double *surf_volume, bool /*optimize_grids*/, bool fillCav, double cav_vol) {
+        if (!delphi || !ensure_device()) return false;
+        nsgpu_surf_opts o{};
+        o.build_epsmap = delphi->buildEpsMap;
+        o.build_status = delphi->buildStatus;
+        o.accurate_triangulation = accurateTriangulation;
+        o.fill_cavities = fillCav;
+        o.cav_vol = cav_vol;
+        o.inside = inside;
+        o.precision = precision;
+        if (nsgpu_get_surf(ctx_, &o, &last_) != NSGPU_OK) return false;
+        if (surf_volume) *surf_volume = last_.volume;
+        const int64_t nv = delphi->numNodes();
+        if (delphi->buildEpsMap) {
+            delphi->epsmap.resize(nv * 3);
+            delphi->idebmap.resize(nv);
+            if (nsgpu_read_epsmap(ctx_, delphi->epsmap.data()) || nsgpu_read_idebmap(ctx_, delphi->idebmap.data()))
+                return false;
+        }
+        if (delphi->buildStatus) {
+            delphi->status.resize(nv);
+            if (nsgpu_read_status(ctx_, delphi->status.data())) return false;
+            delphi->cavitiesSize.resize(last_.num_cavities);
+            delphi->cavitiesFlag.resize(last_.num_cavities);
+            nsgpu_read_cavities(ctx_, delphi->cavitiesSize.data(), delphi->cavitiesFlag.data());
+        }
+        return true;
+    }
+
+    // Surface::triangulateSurface: returns the area, fills vertList/triList.
+    double triangulateSurface(bool outputMesh = true, bool /*buildAtomsMapHere*/ = false, double /*iso*/ = 0.0) {
+        nsgpu_mesh_info m{};
+        if (!ctx_ || nsgpu_triangulate(ctx_, &m) != NSGPU_OK) return 0.0;
+        if (outputMesh) {
+            vertList.resize(m.num_vertices * 3);
+            triList.resize(m.num_triangles * 3);
+            nsgpu_read_mesh(ctx_, vertList.data(), triList.data());
+        }
+        return m.area;
+    }
+
+    const char *lastError() const { return ctx_ ? nsgpu_last_error(ctx_) : "no device context"; }
+    const nsgpu_surf_result &lastResult() const { return last_; }
+
+    int inside = 5;
+    bool accurateTriangulation = true;
+    double probe_radius = 1.4;
+    double randDisplacement = 1e-4;
+    int precision = NSGPU_PREC_FP64;
+    int device = 0;
+    std::vector<double> vertList;
+    std::vector<int32_t> triList;
+
+   protected:
+    bool ensure_device() {
+        if (!ctx_ && nsgpu_create(device, &ctx_) != NSGPU_OK) return false;
+        if (!uploaded_) {
+            if (nsgpu_set_grid(ctx_, &delphi->grid) != NSGPU_OK) return false;
+            const nsgpu_patches *p = patches();
+            if (!p || nsgpu_set_patches(ctx_, p, probe_radius) != NSGPU_OK) return false;
+            uploaded_ = true;
+        }
+        return true;
+    }
+    DelPhiShared *delphi;
+    nsgpu_ctx *ctx_ = nullptr;
+    bool uploaded_ = false;
+    nsgpu_surf_result last_{};
+};
+
+// "ses": host patch construction (jitter + free-probe cells), device ray casting
+class ConnollySurface : public Surface {
+   public:
+    ConnollySurface(ConfigFile *cf, DelPhiShared *ds) : Surface(cf, ds) {}
+    ~ConnollySurface() override {
+        if (ps_) nsgpu_patch_set_free(ps_);
+    }
+    bool build() override {
+        if (!delphi || delphi->atoms.empty()) return false;
+        jittered_ = delphi->atoms;
+        int32_t seed = 1234;  // SEED, tools.h:1827
+        nsgpu_jitter(jittered_.data(), (int32_t)(jittered_.size() / 4), randDisplacement, &seed);
+        if (ps_) nsgpu_patch_set_free(ps_);
+        ps_ = nullptr;
+        if (nsgpu_ses_build(jittered_.data(), (int32_t)(jittered_.size() / 4), probe_radius, &ps_) != NSGPU_OK)
+            return false;
+        nsgpu_patch_set_view(ps_, &view_);
+        uploaded_ = false;
+        return true;
+    }
+    int getNumPatches() override { return ps_ ? view_.n : 0; }
+    const nsgpu_patches *patches() override { return ps_ ? &view_ : nullptr; }
+
+   private:
+    std::vector<double> jittered_;
+    nsgpu_patch_set *ps_ = nullptr;
+    nsgpu_patches view_{};
+};
+
+typedef Surface *(*surface_instantiator)(ConfigFile *, DelPhiShared *);
+
+class SurfaceFactory {
+   public:
+    void add(const std::string &name, surface_instantiator fn) { reg_[name] = fn; }
+    // reads key "Surface"; NULL when the name is unknown (SurfaceFactory.h:29-38)
+    Surface *create(ConfigFile *conf, DelPhiShared *ds) {
+        const std::string name = conf ? conf->read<std::string>("Surface", std::string("ses")) : "ses";
+        auto it = reg_.find(name);
+        return it == reg_.end() ? nullptr : it->second(conf, ds);
+    }
+    bool has(const std::string &name) const { return reg_.count(name) != 0; }
+
+   private:
+    std::map<std::string, surface_instantiator> reg_;
+};
+
+inline SurfaceFactory &surfaceFactory() {
+    static SurfaceFactory f;
+    static bool init = [] {
+        f.add("ses", [](ConfigFile *c, DelPhiShared *d) -> Surface * { return new ConnollySurface(c, d); });
+        return true;
+    }();
+    (void)init;
+    return f;
+}
+
+}  // namespace nsgpu
