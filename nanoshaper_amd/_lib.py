@@ -7,7 +7,9 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libnsgpu.so")
+# NSGPU_LIBRARY selects another build of the same library (e.g. an
+# instrumented one); there is no fallback when the file is missing.
+LIB_PATH = os.environ.get("NSGPU_LIBRARY") or os.path.join(HERE, "libnsgpu.so")
 
 c_int32_p = ctypes.POINTER(ctypes.c_int32)
 c_double_p = ctypes.POINTER(ctypes.c_double)

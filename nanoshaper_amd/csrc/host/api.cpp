@@ -76,7 +76,7 @@ struct nsgpu_ctx {
     // stage timing
     bool profile = false;
     hipEvent_t ev[2 * ST_COUNT] = {};
-    double stage_ms[ST_COUNT] = {};
+    bool recorded[ST_COUNT] = {};  // both events of the stage recorded since the last query
 };
 
 #define HIP_TRY(ctx, expr)                                                                           \
@@ -127,7 +127,9 @@ static DevRays dev_rays(const nsgpu_ctx *c) {
 }
 
 static void mark(nsgpu_ctx *c, int stage, int end) {
-    if (c->profile) (void)hipEventRecord(c->ev[2 * stage + end], c->stream);
+    if (!c->profile) return;
+    if (end == 0) c->recorded[stage] = false;
+    if (hipEventRecord(c->ev[2 * stage + end], c->stream) == hipSuccess && end == 1) c->recorded[stage] = true;
 }
 
 extern "C" {
@@ -180,9 +182,12 @@ int nsgpu_stage_times(nsgpu_ctx *c, double *ms, int n) {
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     for (int s = 0; s < ST_COUNT && s < n; ++s) {
         float t = 0;
-        if (hipEventElapsedTime(&t, c->ev[2 * s], c->ev[2 * s + 1]) != hipSuccess) t = 0;
+        // stages that did not run in the last call report 0
+        if (!c->recorded[s] || hipEventElapsedTime(&t, c->ev[2 * s], c->ev[2 * s + 1]) != hipSuccess) t = 0;
         ms[s] = t;
+        c->recorded[s] = false;
     }
+    (void)hipGetLastError();  // a failed query must not surface in a later call
     return ST_COUNT;
 }
 
@@ -267,16 +272,22 @@ int nsgpu_intersect_path_counts(nsgpu_ctx *c, const nsgpu_surf_opts *o, int64_t 
     P.flags = c->pflags.as<int>();
     P.bsph = c->pbsph.as<double>();
     P.data = c->pdata.as<double>();
-    DBuf cnt;
-    HIP_TRY(c, cnt.ensure(8 * 8));
-    HIP_TRY(c, hipMemsetAsync(cnt.p, 0, 64, c->stream));
-    launch_intersect_count(c->stream, g, P, R, c->probe, cnt.as<unsigned long long>());
+    const int64_t rows = intersect_workgroups(P, R);
+    DBuf part, out;
+    HIP_TRY(c, part.ensure((size_t)std::max<int64_t>(rows, 1) * kPathCounters * 8));
+    HIP_TRY(c, out.ensure(kPathCounters * 8));
+    HIP_TRY(c, hipMemsetAsync(part.p, 0, (size_t)std::max<int64_t>(rows, 1) * kPathCounters * 8, c->stream));
+    HIP_TRY(c, hipMemsetAsync(out.p, 0, kPathCounters * 8, c->stream));
+    launch_intersect_count(c->stream, g, P, R, c->probe, part.as<unsigned long long>());
+    if (rows) launch_reduce_cols(c->stream, part.as<unsigned long long>(), rows, kPathCounters, 0u,
+                                 out.as<unsigned long long>());
     HIP_TRY(c, hipGetLastError());
-    unsigned long long h[8];
-    HIP_TRY(c, hipMemcpyAsync(h, cnt.p, 64, hipMemcpyDeviceToHost, c->stream));
+    unsigned long long h[kPathCounters];
+    HIP_TRY(c, hipMemcpyAsync(h, out.p, sizeof h, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    cnt.release();
-    for (int q = 0; q < 8; ++q) counts[q] = (int64_t)h[q];
+    part.release();
+    out.release();
+    for (int q = 0; q < kPathCounters; ++q) counts[q] = (int64_t)h[q];
     return NSGPU_OK;
 }
 
@@ -302,34 +313,37 @@ static int trace(nsgpu_ctx *c) {
     P.data = c->pdata.as<double>();
     HIP_TRY(c, c->rcnt.ensure(nrays * 4));
     HIP_TRY(c, c->rvol.ensure(nrays * 8));
-    HIP_TRY(c, c->stats.ensure(sizeof(RayStats)));
-    HIP_TRY(c, c->cand.ensure(8));
+    // per-workgroup partials: candidates of the intersection, (hits, odd, longest) of the sort
+    const DevRays R0 = dev_rays(c);
+    const int64_t irows = std::max<int64_t>(intersect_workgroups(P, R0), 1), srows = sort_blocks(R0);
+    HIP_TRY(c, c->cand.ensure((size_t)irows * 8));
+    HIP_TRY(c, c->stats.ensure((size_t)srows * 3 * 8 + 4 * 8));
+    unsigned long long *spart = c->stats.as<unsigned long long>() + 4, *sout = c->stats.as<unsigned long long>();
     for (int attempt = 0; attempt < 4; ++attempt) {
         HIP_TRY(c, c->rt.ensure((size_t)nrays * c->K * 8));
         const DevRays R = dev_rays(c);
         HIP_TRY(c, hipMemsetAsync(c->rcnt.p, 0, nrays * 4, c->stream));
-        HIP_TRY(c, hipMemsetAsync(c->stats.p, 0, sizeof(RayStats), c->stream));
-        HIP_TRY(c, hipMemsetAsync(c->cand.p, 0, 8, c->stream));
+        HIP_TRY(c, hipMemsetAsync(c->cand.p, 0, (size_t)irows * 8, c->stream));
         mark(c, ST_INTERSECT, 0);
         launch_intersect(c->stream, g, P, R, c->probe, c->cand.as<unsigned long long>());
         HIP_TRY(c, hipGetLastError());
         mark(c, ST_INTERSECT, 1);
         mark(c, ST_SORT, 0);
-        launch_sort(c->stream, g, R, c->stats.as<RayStats>());
+        launch_sort(c->stream, g, R, spart);
+        launch_reduce_cols(c->stream, spart, srows, 3, 1u << 2, sout);
+        launch_reduce_cols(c->stream, c->cand.as<unsigned long long>(), irows, 1, 0u, sout + 3);
         HIP_TRY(c, hipGetLastError());
         mark(c, ST_SORT, 1);
-        RayStats st;
-        unsigned long long cand = 0;
-        HIP_TRY(c, hipMemcpyAsync(&st, c->stats.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(c, hipMemcpyAsync(&cand, c->cand.p, 8, hipMemcpyDeviceToHost, c->stream));
+        unsigned long long st[4];
+        HIP_TRY(c, hipMemcpyAsync(st, sout, sizeof st, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
-        c->res.total_hits = (int64_t)st.hits;
-        c->res.odd_rays = (int64_t)st.odd;
-        c->res.max_hits_per_ray = st.max_cnt;
-        c->res.candidates = (int64_t)cand;
-        if (st.max_cnt <= c->K) return NSGPU_OK;
+        c->res.total_hits = (int64_t)st[0];
+        c->res.odd_rays = (int64_t)st[1];
+        c->res.max_hits_per_ray = (int)st[2];
+        c->res.candidates = (int64_t)st[3];
+        if ((int)st[2] <= c->K) return NSGPU_OK;
         int k = c->K;
-        while (k < st.max_cnt) k *= 2;
+        while (k < (int)st[2]) k *= 2;
         c->K = k;
     }
     return fail(c, NSGPU_E_STATE, "ray hit capacity did not converge");

@@ -597,6 +597,21 @@ extern "C" int nsgpu_ses_build(const double *xyzr, int32_t n, double probe_radiu
     };
     for (int a = 0; a < n; ++a) {
         if (!exposed[a]) continue;
+        // Feasibility of a point-cell hit is the AND of all clipping spheres, so
+        // their order only changes how soon a buried hit is rejected: the
+        // spheres removing the largest caps go first.  The cap of neighbour b
+        // on the inflated sphere of a has cos(half angle) =
+        // (R_a^2 + d^2 - R_b^2) / (2 R_a d); smaller means larger.
+        auto cap_cos = [&](int q) {
+            const auto &e = B.edges[q];
+            const int b = (e.i == a) ? e.j : e.i;
+            const double *pa = xyzr + 4 * a, *pb = xyzr + 4 * b;
+            const double d = std::sqrt((pa[0] - pb[0]) * (pa[0] - pb[0]) + (pa[1] - pb[1]) * (pa[1] - pb[1]) +
+                                       (pa[2] - pb[2]) * (pa[2] - pb[2]));
+            return (B.R[a] * B.R[a] + d * d - B.R[b] * B.R[b]) / (2 * B.R[a] * d);
+        };
+        std::stable_sort(atom_edges[a].begin(), atom_edges[a].end(),
+                         [&](int p, int q) { return cap_cos(p) < cap_cos(q); });
         push_patch(NSGPU_PATCH_POINT, xyzr + 4 * a, xyzr[4 * a + 3], (int)atom_edges[a].size(), 0, a, -1, -1);
         for (int q : atom_edges[a]) {
             const auto &e = B.edges[q];

@@ -4,22 +4,22 @@
 
 namespace nsg {
 
-struct RayStats {
-    unsigned long long hits, odd;
-    int max_cnt;
-};
-
 struct FillOpts {
     int build_eps, build_status, inside;
     int slot_x, slot_z, slot_y;     // grid-ray slots of panels 0 (x), 1 (z), 2 (y); -1 = not traced
     int eslot_x, eslot_z, eslot_y;  // edge-ray slots
 };
 
+constexpr int kPathCounters = 8;
+int64_t intersect_workgroups(const DevPatches &, const DevRays &);
+int64_t sort_blocks(const DevRays &);
 void launch_intersect(hipStream_t, const DevGrid &, const DevPatches &, const DevRays &, double,
-                      unsigned long long *);
+                      unsigned long long *part);
 void launch_intersect_count(hipStream_t, const DevGrid &, const DevPatches &, const DevRays &, double,
-                            unsigned long long *);
-void launch_sort(hipStream_t, const DevGrid &, const DevRays &, RayStats *);
+                            unsigned long long *part);
+void launch_sort(hipStream_t, const DevGrid &, const DevRays &, unsigned long long *part);
+void launch_reduce_cols(hipStream_t, const unsigned long long *in, int64_t nrows, int ncols, unsigned int max_mask,
+                        unsigned long long *out);
 void launch_fill_maps(hipStream_t, const DevGrid &, const DevRays &, const FillOpts &, int *, uint8_t *, int *);
 void launch_fill_corners(hipStream_t, const DevGrid &, const DevRays &, const FillOpts &, uint32_t *);
 int64_t sum_partials_count(int64_t n);

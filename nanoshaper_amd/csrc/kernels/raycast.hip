@@ -198,9 +198,13 @@ __device__ __forceinline__ void push_hit(const DevRays &R, int slot, int64_t ray
 enum { PC_CAND, PC_DISC, PC_SPHERE, PC_TORUS, PC_ROOTS, PC_HITS, PC_FEAS_ITEMS, PC_FEAS_CALLS, PC_N };
 
 // grid: (numPatches, nslots); block: 64 lanes (one wave)
+// Per-workgroup counts go to the workgroup's own row of 'part' (zeroed before
+// the launch) and are summed by k_reduce_cols: one atomic per workgroup on a
+// shared address would serialise 10^5 workgroups at the L2.
 template <bool kCount>
 __global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRays R, double probe,
-                                                  unsigned long long *cand_total) {
+                                                  unsigned long long *part) {
+    const int64_t wg = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
     const int it = blockIdx.x;
     const int slot = blockIdx.y;
     const int phase = slot < R.panels0 ? 0 : 1;
@@ -282,20 +286,21 @@ __global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRa
         for (int q = 0; q < PC_N; ++q) {
             unsigned long long v = pc[q];
             for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-            if (threadIdx.x == 0 && v) atomicAdd(cand_total + q, v);
+            if (threadIdx.x == 0 && v) part[wg * PC_N + q] = v;
         }
         return;
     }
     // candidate count: wave reduction, one atomic per workgroup
     for (int off = 32; off > 0; off >>= 1) cand += __shfl_down(cand, off, 64);
-    if (threadIdx.x == 0 && cand) atomicAdd(cand_total, (unsigned long long)cand);
+    if (threadIdx.x == 0 && cand) part[wg] = cand;
 }
 
 // Per-ray insertion sort of the depths (Surface::reorderPatchIntersections,
 // Surface.cpp:5080-5095), plus the ray's segment-length integral of
 // setVerticesAndGridsWithIntersectionData (Surface.cpp:5265-5290, :5425-5440)
 // and the statistics of the ray lists.
-__global__ void __launch_bounds__(256) k_sort(DevGrid g, DevRays R, RayStats *st) {
+// part: 3 values per block (hits, odd rays, longest list), reduced by k_reduce_cols
+__global__ void __launch_bounds__(256) k_sort(DevGrid g, DevRays R, unsigned long long *part) {
     const int64_t nper = (int64_t)R.N * R.N;
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t total = nper * R.nslots;
@@ -366,31 +371,64 @@ __global__ void __launch_bounds__(256) k_sort(DevGrid g, DevRays R, RayStats *st
             o += sh[q][1];
             mx = max(mx, shm[q]);
         }
-        atomicAdd(&st->hits, h);
-        atomicAdd(&st->odd, o);
-        atomicMax(&st->max_cnt, mx);
+        part[3 * (int64_t)blockIdx.x + 0] = h;
+        part[3 * (int64_t)blockIdx.x + 1] = o;
+        part[3 * (int64_t)blockIdx.x + 2] = (unsigned long long)mx;
     }
 }
 
+// out[c] = sum (or max when bit c of max_mask is set) over rows of in[row*ncols + c]
+__global__ void __launch_bounds__(1024) k_reduce_cols(const unsigned long long *in, int64_t nrows, int ncols,
+                                                      unsigned int max_mask, unsigned long long *out) {
+    __shared__ unsigned long long sh[1024];
+    const int c = blockIdx.x;
+    const bool is_max = (max_mask >> c) & 1u;
+    unsigned long long acc = 0;
+    for (int64_t r = threadIdx.x; r < nrows; r += 1024) {
+        const unsigned long long v = in[r * ncols + c];
+        acc = is_max ? (v > acc ? v : acc) : acc + v;
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = 512; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            const unsigned long long a = sh[threadIdx.x], b = sh[threadIdx.x + st];
+            sh[threadIdx.x] = is_max ? (a > b ? a : b) : a + b;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = sh[0];
+}
+
+int64_t intersect_workgroups(const DevPatches &P, const DevRays &R) { return (int64_t)P.n * R.nslots; }
+int64_t sort_blocks(const DevRays &R) { return ((int64_t)R.N * R.N * R.nslots + 255) / 256; }
+
+// part: intersect_workgroups() rows, zeroed by the caller
 void launch_intersect(hipStream_t s, const DevGrid &g, const DevPatches &P, const DevRays &R, double probe,
-                      unsigned long long *cand) {
+                      unsigned long long *part) {
     if (P.n == 0 || R.nslots == 0) return;
     dim3 grid((unsigned)P.n, (unsigned)R.nslots);
-    hipLaunchKernelGGL(k_intersect<false>, grid, dim3(64), 0, s, g, P, R, probe, cand);
+    hipLaunchKernelGGL(k_intersect<false>, grid, dim3(64), 0, s, g, P, R, probe, part);
 }
 
-// Counting variant: same candidate walk, no hit output; counters[PC_N] get
-// the number of times each FP64 path of the reference's candidate set runs.
+// Counting variant: same candidate walk, no hit output; part gets PC_N
+// counters per workgroup (zeroed by the caller): how often each FP64 path of
+// the reference's candidate set runs.
 void launch_intersect_count(hipStream_t s, const DevGrid &g, const DevPatches &P, const DevRays &R, double probe,
-                            unsigned long long *counters) {
+                            unsigned long long *part) {
     if (P.n == 0 || R.nslots == 0) return;
     dim3 grid((unsigned)P.n, (unsigned)R.nslots);
-    hipLaunchKernelGGL(k_intersect<true>, grid, dim3(64), 0, s, g, P, R, probe, counters);
+    hipLaunchKernelGGL(k_intersect<true>, grid, dim3(64), 0, s, g, P, R, probe, part);
 }
 
-void launch_sort(hipStream_t s, const DevGrid &g, const DevRays &R, RayStats *st) {
-    const int64_t total = (int64_t)R.N * R.N * R.nslots;
-    hipLaunchKernelGGL(k_sort, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, g, R, st);
+// part: sort_blocks() rows of 3
+void launch_sort(hipStream_t s, const DevGrid &g, const DevRays &R, unsigned long long *part) {
+    hipLaunchKernelGGL(k_sort, dim3((unsigned)sort_blocks(R)), dim3(256), 0, s, g, R, part);
+}
+
+void launch_reduce_cols(hipStream_t s, const unsigned long long *in, int64_t nrows, int ncols, unsigned int max_mask,
+                        unsigned long long *out) {
+    hipLaunchKernelGGL(k_reduce_cols, dim3((unsigned)ncols), dim3(1024), 0, s, in, nrows, ncols, max_mask, out);
 }
 
 }  // namespace nsg
