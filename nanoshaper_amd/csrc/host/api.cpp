@@ -68,6 +68,7 @@ struct nsgpu_ctx {
     std::vector<uint8_t> cav_filled;
     // vertices and mesh
     DBuf idx, vcount, voffs, scan_scratch, verts, tris, tarea;
+    DBuf segmask, anode, acase, anv, ant, avoff, atoff;
     int64_t n_analytic = 0, nverts = 0, ntris = 0;
     double area = 0;
     bool have_surf = false, have_vertices = false, have_mesh = false;
@@ -160,7 +161,8 @@ void nsgpu_destroy(nsgpu_ctx *c) {
     DBuf *all[] = {&c->gx, &c->gy, &c->gz, &c->pkind, &c->poff, &c->pcnt, &c->pflags, &c->pbsph, &c->pdata,
                    &c->rcnt, &c->rt, &c->rvol, &c->partial, &c->stats, &c->cand, &c->eps, &c->ideb, &c->status,
                    &c->bits, &c->parent, &c->nroots, &c->rootlist, &c->bkey, &c->ccount, &c->clabel, &c->cfilled,
-                   &c->idx, &c->vcount, &c->voffs, &c->scan_scratch, &c->verts, &c->tris, &c->tarea};
+                   &c->idx, &c->vcount, &c->voffs, &c->scan_scratch, &c->verts, &c->tris, &c->tarea,
+                   &c->segmask, &c->anode, &c->acase, &c->anv, &c->ant, &c->avoff, &c->atoff};
     for (DBuf *b : all) b->release();
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -445,8 +447,7 @@ static int assemble(nsgpu_ctx *c, const int eslots[3]) {
     const DevRays R = dev_rays(c);
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
     const int64_t nrays = 3 * (int64_t)R.N * R.N;
-    const int64_t ncubes = mc_num_cubes(g);
-    const int64_t nmax = std::max(nrays, ncubes);
+    const int64_t nmax = nrays;
     HIP_TRY(c, c->idx.ensure((size_t)nvox * 3 * 4));
     HIP_TRY(c, c->vcount.ensure((size_t)nmax * 4));
     HIP_TRY(c, c->voffs.ensure((size_t)nmax * 4));
@@ -576,16 +577,47 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
     if (!c->have_vertices) return fail(c, NSGPU_E_STATE, "get_surf with accurate_triangulation first");
     HIP_TRY(c, hipSetDevice(c->device));
     const DevGrid g = dev_grid(c);
-    const int64_t ncubes = mc_num_cubes(g);
+    const int64_t nseg = mc_num_segments(g);
     mark(c, ST_MC, 0);
     c->nverts = c->n_analytic;
-    // missing vertices of edges 5, 6, 10
-    launch_mc_vcount(c->stream, g, c->bits.as<uint32_t>(), c->idx.as<int>(), c->vcount.as<int>());
-    launch_scan(c->stream, c->vcount.as<int>(), ncubes, c->voffs.as<int>(), c->scan_scratch.as<int>());
+    // 1. classify 32-cube row segments, compact the active cubes in cube order
+    HIP_TRY(c, c->segmask.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
+    HIP_TRY(c, c->vcount.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
+    HIP_TRY(c, c->voffs.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
+    HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(std::max<int64_t>(nseg, 1)) * 4));
+    launch_mc_classify(c->stream, g, c->bits.as<uint32_t>(), c->segmask.as<uint32_t>(), c->vcount.as<int>());
+    launch_scan(c->stream, c->vcount.as<int>(), nseg, c->voffs.as<int>(), c->scan_scratch.as<int>());
     HIP_TRY(c, hipGetLastError());
-    int64_t nadd = 0;
-    int rc = read_total(c, c->vcount.as<int>(), c->voffs.as<int>(), ncubes, &nadd);
+    int64_t nact = 0;
+    int rc = read_total(c, c->vcount.as<int>(), c->voffs.as<int>(), nseg, &nact);
     if (rc) return rc;
+    const size_t na = (size_t)std::max<int64_t>(nact, 1);
+    HIP_TRY(c, c->anode.ensure(na * 4));
+    HIP_TRY(c, c->acase.ensure(na));
+    HIP_TRY(c, c->anv.ensure(na * 4));
+    HIP_TRY(c, c->ant.ensure(na * 4));
+    HIP_TRY(c, c->avoff.ensure(na * 4));
+    HIP_TRY(c, c->atoff.ensure(na * 4));
+    HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(std::max<int64_t>(std::max<int64_t>(nseg, nact), 1)) * 4));
+    launch_mc_compact(c->stream, g, c->bits.as<uint32_t>(), c->segmask.as<uint32_t>(), c->voffs.as<int>(),
+                      c->anode.as<int>(), c->acase.as<uint8_t>());
+    // 2. per active cube: mid-point vertices added and triangles emitted
+    launch_mc_counts(c->stream, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->idx.as<int>(),
+                     c->anv.as<int>(), c->ant.as<int>());
+    launch_scan(c->stream, c->anv.as<int>(), nact, c->avoff.as<int>(), c->scan_scratch.as<int>());
+    launch_scan(c->stream, c->ant.as<int>(), nact, c->atoff.as<int>(), c->scan_scratch.as<int>());
+    HIP_TRY(c, hipGetLastError());
+    int64_t nadd = 0, nt = 0;
+    if (nact) {
+        int tail[4];
+        HIP_TRY(c, hipMemcpyAsync(tail + 0, c->anv.as<int>() + nact - 1, 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(tail + 1, c->avoff.as<int>() + nact - 1, 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(tail + 2, c->ant.as<int>() + nact - 1, 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(tail + 3, c->atoff.as<int>() + nact - 1, 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        nadd = (int64_t)tail[0] + tail[1];
+        nt = (int64_t)tail[2] + tail[3];
+    }
     const int64_t nv = c->n_analytic + nadd;
     if (nv >= ((int64_t)1 << 31)) return fail(c, NSGPU_E_NOMEM, "too many vertices");
     if ((size_t)nv * 24 > c->verts.cap) {
@@ -598,22 +630,15 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
         c->verts.release();
         c->verts = nb;
     }
-    launch_mc_vwrite(c->stream, g, c->bits.as<uint32_t>(), c->idx.as<int>(), c->voffs.as<int>(), (int)c->n_analytic,
-                     c->verts.as<double>());
-    HIP_TRY(c, hipGetLastError());
-    c->nverts = nv;
-    // triangles
-    launch_mc_tcount(c->stream, g, c->bits.as<uint32_t>(), c->idx.as<int>(), c->vcount.as<int>());
-    launch_scan(c->stream, c->vcount.as<int>(), ncubes, c->voffs.as<int>(), c->scan_scratch.as<int>());
-    HIP_TRY(c, hipGetLastError());
-    int64_t nt = 0;
-    rc = read_total(c, c->vcount.as<int>(), c->voffs.as<int>(), ncubes, &nt);
-    if (rc) return rc;
+    // 3. write the mid points, then the triangles that may reference them
+    launch_mc_vwrite(c->stream, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->idx.as<int>(),
+                     c->avoff.as<int>(), (int)c->n_analytic, c->verts.as<double>());
     HIP_TRY(c, c->tris.ensure((size_t)std::max<int64_t>(nt, 1) * 12));
     HIP_TRY(c, c->tarea.ensure((size_t)std::max<int64_t>(nt, 1) * 8));
-    launch_mc_twrite(c->stream, g, c->bits.as<uint32_t>(), c->idx.as<int>(), c->voffs.as<int>(),
-                     c->verts.as<double>(), c->tris.as<int>(), c->tarea.as<double>());
+    launch_mc_twrite(c->stream, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->idx.as<int>(),
+                     c->atoff.as<int>(), c->verts.as<double>(), c->tris.as<int>(), c->tarea.as<double>());
     HIP_TRY(c, hipGetLastError());
+    c->nverts = nv;
     const int64_t per = sum_partials_count(nt);
     HIP_TRY(c, c->partial.ensure((size_t)std::max<int64_t>(per, 1) * 8));
     launch_sum_blocks(c->stream, c->tarea.as<double>(), nt, c->partial.as<double>());

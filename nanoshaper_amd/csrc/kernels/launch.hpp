@@ -34,11 +34,15 @@ void mc_upload_tables(hipStream_t);
 void launch_vtx_count(hipStream_t, const DevGrid &, const DevRays &, const int[3], const uint32_t *, int *);
 void launch_vtx_write(hipStream_t, const DevGrid &, const DevRays &, const int[3], const uint32_t *, const int *,
                       double *, int *);
-int64_t mc_num_cubes(const DevGrid &);
-void launch_mc_vcount(hipStream_t, const DevGrid &, const uint32_t *, const int *, int *);
-void launch_mc_vwrite(hipStream_t, const DevGrid &, const uint32_t *, int *, const int *, int, double *);
-void launch_mc_tcount(hipStream_t, const DevGrid &, const uint32_t *, const int *, int *);
-void launch_mc_twrite(hipStream_t, const DevGrid &, const uint32_t *, const int *, const int *, const double *, int *,
-                      double *);
+int64_t mc_num_segments(const DevGrid &);
+void launch_mc_classify(hipStream_t, const DevGrid &, const uint32_t *bits, uint32_t *segmask, int *segcnt);
+void launch_mc_compact(hipStream_t, const DevGrid &, const uint32_t *bits, const uint32_t *segmask, const int *segoff,
+                       int *anode, uint8_t *acase);
+void launch_mc_counts(hipStream_t, const DevGrid &, int nact, const int *anode, const uint8_t *acase, const int *idx,
+                      int *nv, int *nt);
+void launch_mc_vwrite(hipStream_t, const DevGrid &, int nact, const int *anode, const uint8_t *acase, int *idx,
+                      const int *voff, int base, double *verts);
+void launch_mc_twrite(hipStream_t, const DevGrid &, int nact, const int *anode, const uint8_t *acase, const int *idx,
+                      const int *toff, const double *verts, int *tris, double *tarea);
 
 }  // namespace nsg

@@ -13,6 +13,7 @@
 // of an active cube get a mid-point vertex when no analytic one exists
 // (getVertices :8002-8327); triangles come from the case table with missing
 // vertices skipped and are written reversed (triangulationKernel :9075-9318).
+// Vertex and triangle order is cube order, as a single-threaded run.
 #include "launch.hpp"
 #include "../common/mc_cases.h"
 
@@ -125,18 +126,12 @@ __global__ void __launch_bounds__(256) k_vtx_write(DevGrid g, DevRays R, EdgeSlo
 }
 
 // ---------------------------------------------------------------- marching cubes
-__device__ __forceinline__ int cube_case(const uint32_t *bits, int64_t f, int64_t sy, int64_t sz, int64_t nvox) {
-    int c = 0;
-    c |= (!bit_at(bits, f, nvox)) << 0;
-    c |= (!bit_at(bits, f + 1, nvox)) << 1;
-    c |= (!bit_at(bits, f + 1 + sy, nvox)) << 2;
-    c |= (!bit_at(bits, f + sy, nvox)) << 3;
-    c |= (!bit_at(bits, f + sz, nvox)) << 4;
-    c |= (!bit_at(bits, f + 1 + sz, nvox)) << 5;
-    c |= (!bit_at(bits, f + 1 + sy + sz, nvox)) << 6;
-    c |= (!bit_at(bits, f + sy + sz, nvox)) << 7;
-    return c;
-}
+// Cubes are classified 32 at a time: a lane owns a row segment (32 cubes
+// along x at fixed j,k), reads a 64-bit window of corner bits from each of the
+// four corner rows (j,k), (j+1,k), (j,k+1), (j+1,k+1), and finds the active
+// cubes (neither all inside nor all outside) with bitwise operations.  Active
+// cubes are compacted in (k, j, i) order; the vertex and triangle passes then
+// visit only them.
 
 __constant__ uint64_t c_tri_cases[256];
 
@@ -148,7 +143,7 @@ __device__ __forceinline__ bool edge_on(int c, int e) {
     return ((c >> a) & 1) != ((c >> b) & 1);
 }
 
-// edge e of cube (i,j,k) -> vertex index slot: direction and node offset (getTriangles)
+// edge e of the cube at node f -> vertex index slot: direction and node offset (getTriangles)
 __device__ __forceinline__ int64_t edge_slot(int e, int64_t f, int64_t sy, int64_t sz, int64_t nvox) {
     switch (e) {
         case 0: return 0 * nvox + f;
@@ -166,45 +161,152 @@ __device__ __forceinline__ int64_t edge_slot(int e, int64_t f, int64_t sy, int64
     }
 }
 
-struct CubeIter {
-    int64_t ncubes;
-    int cx, cy;  // cube counts along x and y
+// Whether the cube that owns edge e (the cube where it is edge 5, 6 or 10)
+// lies in the cube range, i.e. whether getVertices adds a mid point for it
+// when no analytic vertex exists.  Owners sit at -1 in some of i, j, k.
+__device__ __forceinline__ bool owner_in_range(int e, int i, int j, int k) {
+    switch (e) {
+        case 5: case 6: case 10: return true;
+        case 7: return i > 1;             // Y(i, j, k+1)   = edge 5 of (i-1, j, k)
+        case 1: return k > 1;             // Y(i+1, j, k)   = edge 5 of (i, j, k-1)
+        case 3: return i > 1 && k > 1;    // Y(i, j, k)     = edge 5 of (i-1, j, k-1)
+        case 4: return j > 1;             // X(i, j, k+1)   = edge 6 of (i, j-1, k)
+        case 2: return k > 1;             // X(i, j+1, k)   = edge 6 of (i, j, k-1)
+        case 0: return j > 1 && k > 1;    // X(i, j, k)     = edge 6 of (i, j-1, k-1)
+        case 9: return j > 1;             // Z(i+1, j, k)   = edge 10 of (i, j-1, k)
+        case 11: return i > 1;            // Z(i, j+1, k)   = edge 10 of (i-1, j, k)
+        default: return i > 1 && j > 1;  // 8: Z(i, j, k) = edge 10 of (i-1, j-1, k)
+    }
+}
+
+// 64 corner bits starting at flat node f (bits past the grid read as outside)
+__device__ __forceinline__ uint64_t bit_window(const uint32_t *bits, int64_t f, int64_t nwords) {
+    const int64_t w = f >> 5;
+    const uint32_t s = (uint32_t)(f & 31);
+    const uint64_t lo = w < nwords ? bits[w] : 0xffffffffu;
+    const uint64_t hi = w + 1 < nwords ? bits[w + 1] : 0xffffffffu;
+    return ((hi << 32) | lo) >> s | (s ? (~0ull << (64 - s)) : 0ull);
+}
+
+struct SegIter {
+    int nseg_row;  // segments per row
+    int cx, cy, cz;
+    int64_t nseg;
 };
 
-__device__ __forceinline__ int64_t cube_node(const DevGrid &g, const CubeIter &ci, int64_t q) {
-    const int i = 1 + (int)(q % ci.cx);
-    const int j = 1 + (int)((q / ci.cx) % ci.cy);
-    const int k = 1 + (int)(q / ((int64_t)ci.cx * ci.cy));
-    return ((int64_t)k * g.ny + j) * g.nx + i;
+__device__ __forceinline__ void seg_decode(const SegIter &si, int64_t s, int &i0, int &j, int &k, int &len) {
+    const int sr = (int)(s % si.nseg_row);
+    const int64_t row = s / si.nseg_row;
+    j = 1 + (int)(row % si.cy);
+    k = 1 + (int)(row / si.cy);
+    i0 = 1 + 32 * sr;
+    len = min(32, si.cx - 32 * sr);
 }
 
-__global__ void __launch_bounds__(256) k_mc_vcount(DevGrid g, CubeIter ci, const uint32_t *bits, const int *idx,
-                                                   int *counts) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= ci.ncubes) return;
-    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (int64_t)g.nx * g.ny;
-    const int64_t f = cube_node(g, ci, q);
-    const int c = cube_case(bits, f, sy, sz, nvox);
-    int n = 0;
-    if (c != 0 && c != 255) {
-        n += edge_on(c, 5) && idx[edge_slot(5, f, sy, sz, nvox)] == -1;
-        n += edge_on(c, 6) && idx[edge_slot(6, f, sy, sz, nvox)] == -1;
-        n += edge_on(c, 10) && idx[edge_slot(10, f, sy, sz, nvox)] == -1;
+// active-cube mask of a segment; rows A=(j,k) B=(j+1,k) C=(j,k+1) D=(j+1,k+1)
+__device__ __forceinline__ uint32_t seg_active(const DevGrid &g, const uint32_t *bits, int i0, int j, int k, int len,
+                                               uint64_t rows[4]) {
+    const int64_t nwords = ((int64_t)g.nx * g.ny * g.nz + 31) >> 5;
+    const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
+    const int64_t f = ((int64_t)k * g.ny + j) * g.nx + i0;
+    rows[0] = bit_window(bits, f, nwords);
+    rows[1] = bit_window(bits, f + sy, nwords);
+    rows[2] = bit_window(bits, f + sz, nwords);
+    rows[3] = bit_window(bits, f + sy + sz, nwords);
+    uint64_t out = ~0ull, in = ~0ull;  // bit b: all eight corners of cube i0+b outside / inside
+    for (int r = 0; r < 4; ++r) {
+        out &= rows[r] & (rows[r] >> 1);
+        in &= ~rows[r] & (~rows[r] >> 1);
     }
-    counts[q] = n;
+    const uint32_t valid = len >= 32 ? 0xffffffffu : ((1u << len) - 1u);
+    return (uint32_t)(~(out | in)) & valid;
 }
 
-__global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, CubeIter ci, const uint32_t *bits, int *idx,
-                                                   const int *offs, int base, double *verts) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= ci.ncubes) return;
+__device__ __forceinline__ int seg_case(const uint64_t rows[4], int b) {
+    // corner q inside (bit 0) -> case bit q; corners 0..7 as in classifyCube
+    const uint64_t A = rows[0] >> b, B = rows[1] >> b, C = rows[2] >> b, D = rows[3] >> b;
+    return (int)(((~A) & 1) | (((~A >> 1) & 1) << 1) | (((~B >> 1) & 1) << 2) | (((~B) & 1) << 3) |
+                 (((~C) & 1) << 4) | (((~C >> 1) & 1) << 5) | (((~D >> 1) & 1) << 6) | (((~D) & 1) << 7));
+}
+
+__global__ void __launch_bounds__(256) k_mc_classify(DevGrid g, SegIter si, const uint32_t *bits, uint32_t *segmask,
+                                                     int *segcnt) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= si.nseg) return;
+    int i0, j, k, len;
+    seg_decode(si, s, i0, j, k, len);
+    uint64_t rows[4];
+    const uint32_t m = seg_active(g, bits, i0, j, k, len, rows);
+    segmask[s] = m;
+    segcnt[s] = __popc(m);
+}
+
+// active list in cube order: node index and case of each active cube
+__global__ void __launch_bounds__(256) k_mc_compact(DevGrid g, SegIter si, const uint32_t *bits,
+                                                    const uint32_t *segmask, const int *segoff, int *anode,
+                                                    uint8_t *acase) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= si.nseg) return;
+    uint32_t m = segmask[s];
+    if (!m) return;
+    int i0, j, k, len;
+    seg_decode(si, s, i0, j, k, len);
+    const int64_t nwords = ((int64_t)g.nx * g.ny * g.nz + 31) >> 5;
+    const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
+    const int64_t f = ((int64_t)k * g.ny + j) * g.nx + i0;
+    const uint64_t rows[4] = {bit_window(bits, f, nwords), bit_window(bits, f + sy, nwords),
+                              bit_window(bits, f + sz, nwords), bit_window(bits, f + sy + sz, nwords)};
+    int o = segoff[s];
+    while (m) {
+        const int b = __ffs(m) - 1;
+        m &= m - 1;
+        anode[o] = (int)(f + b);
+        acase[o] = (uint8_t)seg_case(rows, b);
+        ++o;
+    }
+}
+
+// per active cube: missing mid-point vertices it adds (edges 5, 6, 10 without
+// an analytic vertex) and triangles it emits (all three vertices present after
+// the mid points are added)
+__global__ void __launch_bounds__(256) k_mc_counts(DevGrid g, int nact, const int *anode, const uint8_t *acase,
+                                                   const int *idx, int *nv, int *nt) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nact) return;
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (int64_t)g.nx * g.ny;
-    const int64_t f = cube_node(g, ci, q);
-    const int c = cube_case(bits, f, sy, sz, nvox);
-    if (c == 0 || c == 255) return;
+    const int64_t f = anode[q];
+    const int c = acase[q];
     const int i = (int)(f % g.nx), j = (int)((f / g.nx) % g.ny), k = (int)(f / sz);
-    int v = base + offs[q];
-    // the three slots are owned by this cube only, so reading them here is race free
+    int v = 0;
+    v += edge_on(c, 5) && idx[edge_slot(5, f, sy, sz, nvox)] == -1;
+    v += edge_on(c, 6) && idx[edge_slot(6, f, sy, sz, nvox)] == -1;
+    v += edge_on(c, 10) && idx[edge_slot(10, f, sy, sz, nvox)] == -1;
+    nv[q] = v;
+    const uint64_t w = c_tri_cases[c];
+    int t = 0;
+    for (int tr = 0; tr < 5; ++tr) {
+        bool ok = true;
+        int e0 = (int)((w >> (12 * tr)) & 15);
+        if (e0 == 15) break;
+        for (int u = 0; u < 3; ++u) {
+            const int e = (int)((w >> (12 * tr + 4 * u)) & 15);
+            ok &= idx[edge_slot(e, f, sy, sz, nvox)] != -1 || owner_in_range(e, i, j, k);
+        }
+        t += ok;
+    }
+    nt[q] = t;
+}
+
+__global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const int *anode, const uint8_t *acase,
+                                                   int *idx, const int *voff, int base, double *verts) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nact) return;
+    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (int64_t)g.nx * g.ny;
+    const int64_t f = anode[q];
+    const int c = acase[q];
+    const int i = (int)(f % g.nx), j = (int)((f / g.nx) % g.ny), k = (int)(f / sz);
+    int v = base + voff[q];
+    // the slots of edges 5, 6, 10 belong to this cube only: no other lane writes them
     const int64_t s5 = edge_slot(5, f, sy, sz, nvox), s6 = edge_slot(6, f, sy, sz, nvox),
                   s10 = edge_slot(10, f, sy, sz, nvox);
     const bool m5 = edge_on(c, 5) && idx[s5] == -1;
@@ -230,15 +332,15 @@ __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, CubeIter ci, const
     }
 }
 
-template <bool kWrite>
-__device__ int cube_tris(const DevGrid &g, const CubeIter &ci, const uint32_t *bits, const int *idx, int64_t q,
-                         const double *verts, int *tris, double *tarea, int base) {
+__global__ void __launch_bounds__(256) k_mc_twrite(DevGrid g, int nact, const int *anode, const uint8_t *acase,
+                                                   const int *idx, const int *toff, const double *verts, int *tris,
+                                                   double *tarea) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nact) return;
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (int64_t)g.nx * g.ny;
-    const int64_t f = cube_node(g, ci, q);
-    const int c = cube_case(bits, f, sy, sz, nvox);
-    if (c == 0 || c == 255) return 0;
-    const uint64_t w = c_tri_cases[c];
-    int n = 0;
+    const int64_t f = anode[q];
+    const uint64_t w = c_tri_cases[acase[q]];
+    int o = toff[q];
     for (int t = 0; t < 5; ++t) {
         const int e0 = (int)((w >> (12 * t)) & 15), e1 = (int)((w >> (12 * t + 4)) & 15),
                   e2 = (int)((w >> (12 * t + 8)) & 15);
@@ -247,38 +349,20 @@ __device__ int cube_tris(const DevGrid &g, const CubeIter &ci, const uint32_t *b
         const int v1 = idx[edge_slot(e1, f, sy, sz, nvox)];
         const int v2 = idx[edge_slot(e2, f, sy, sz, nvox)];
         if (v0 == -1 || v1 == -1 || v2 == -1) continue;
-        if (kWrite) {
-            const int o = base + n;
-            tris[3 * (int64_t)o + 0] = v2;
-            tris[3 * (int64_t)o + 1] = v1;
-            tris[3 * (int64_t)o + 2] = v0;
-            const double *a0 = verts + 3 * (int64_t)v0, *a1 = verts + 3 * (int64_t)v1, *a2 = verts + 3 * (int64_t)v2;
-            const double a = sqrt((a0[0] - a1[0]) * (a0[0] - a1[0]) + (a0[1] - a1[1]) * (a0[1] - a1[1]) +
-                                  (a0[2] - a1[2]) * (a0[2] - a1[2]));
-            const double b = sqrt((a0[0] - a2[0]) * (a0[0] - a2[0]) + (a0[1] - a2[1]) * (a0[1] - a2[1]) +
-                                  (a0[2] - a2[2]) * (a0[2] - a2[2]));
-            const double cc = sqrt((a1[0] - a2[0]) * (a1[0] - a2[0]) + (a1[1] - a2[1]) * (a1[1] - a2[1]) +
-                                   (a1[2] - a2[2]) * (a1[2] - a2[2]));
-            const double ttt = (a + b + cc) * (b + cc - a) * (cc + a - b) * (a + b - cc);
-            tarea[o] = ttt > 0 ? 0.25 * sqrt(ttt) : 0.0;
-        }
-        ++n;
+        tris[3 * (int64_t)o + 0] = v2;
+        tris[3 * (int64_t)o + 1] = v1;
+        tris[3 * (int64_t)o + 2] = v0;
+        const double *a0 = verts + 3 * (int64_t)v0, *a1 = verts + 3 * (int64_t)v1, *a2 = verts + 3 * (int64_t)v2;
+        const double a = sqrt((a0[0] - a1[0]) * (a0[0] - a1[0]) + (a0[1] - a1[1]) * (a0[1] - a1[1]) +
+                              (a0[2] - a1[2]) * (a0[2] - a1[2]));
+        const double b = sqrt((a0[0] - a2[0]) * (a0[0] - a2[0]) + (a0[1] - a2[1]) * (a0[1] - a2[1]) +
+                              (a0[2] - a2[2]) * (a0[2] - a2[2]));
+        const double cc = sqrt((a1[0] - a2[0]) * (a1[0] - a2[0]) + (a1[1] - a2[1]) * (a1[1] - a2[1]) +
+                               (a1[2] - a2[2]) * (a1[2] - a2[2]));
+        const double ttt = (a + b + cc) * (b + cc - a) * (cc + a - b) * (a + b - cc);
+        tarea[o] = ttt > 0 ? 0.25 * sqrt(ttt) : 0.0;
+        ++o;
     }
-    return n;
-}
-
-__global__ void __launch_bounds__(256) k_mc_tcount(DevGrid g, CubeIter ci, const uint32_t *bits, const int *idx,
-                                                   int *counts) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= ci.ncubes) return;
-    counts[q] = cube_tris<false>(g, ci, bits, idx, q, nullptr, nullptr, nullptr, 0);
-}
-
-__global__ void __launch_bounds__(256) k_mc_twrite(DevGrid g, CubeIter ci, const uint32_t *bits, const int *idx,
-                                                   const int *offs, const double *verts, int *tris, double *tarea) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= ci.ncubes) return;
-    cube_tris<true>(g, ci, bits, idx, q, verts, tris, tarea, offs[q]);
 }
 
 static inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
@@ -301,40 +385,48 @@ void launch_vtx_write(hipStream_t s, const DevGrid &g, const DevRays &R, const i
     hipLaunchKernelGGL(k_vtx_write, dim3(nblk(n)), dim3(256), 0, s, g, R, es, bits, offs, verts, idx);
 }
 
-static CubeIter cube_iter(const DevGrid &g) {
-    CubeIter ci;
-    ci.cx = g.nx > 2 ? g.nx - 2 : 0;
-    ci.cy = g.ny > 2 ? g.ny - 2 : 0;
-    const int cz = g.nz > 2 ? g.nz - 2 : 0;
-    ci.ncubes = (int64_t)ci.cx * ci.cy * cz;
-    return ci;
+static SegIter seg_iter(const DevGrid &g) {
+    SegIter si;
+    si.cx = g.nx > 2 ? g.nx - 2 : 0;
+    si.cy = g.ny > 2 ? g.ny - 2 : 0;
+    si.cz = g.nz > 2 ? g.nz - 2 : 0;
+    si.nseg_row = (si.cx + 31) / 32;
+    si.nseg = (int64_t)si.nseg_row * si.cy * si.cz;
+    return si;
 }
 
-int64_t mc_num_cubes(const DevGrid &g) { return cube_iter(g).ncubes; }
+int64_t mc_num_segments(const DevGrid &g) { return seg_iter(g).nseg; }
 
-void launch_mc_vcount(hipStream_t s, const DevGrid &g, const uint32_t *bits, const int *idx, int *counts) {
-    const CubeIter ci = cube_iter(g);
-    if (ci.ncubes) hipLaunchKernelGGL(k_mc_vcount, dim3(nblk(ci.ncubes)), dim3(256), 0, s, g, ci, bits, idx, counts);
+void launch_mc_classify(hipStream_t s, const DevGrid &g, const uint32_t *bits, uint32_t *segmask, int *segcnt) {
+    const SegIter si = seg_iter(g);
+    if (si.nseg) hipLaunchKernelGGL(k_mc_classify, dim3(nblk(si.nseg)), dim3(256), 0, s, g, si, bits, segmask, segcnt);
 }
 
-void launch_mc_vwrite(hipStream_t s, const DevGrid &g, const uint32_t *bits, int *idx, const int *offs, int base,
-                      double *verts) {
-    const CubeIter ci = cube_iter(g);
-    if (ci.ncubes)
-        hipLaunchKernelGGL(k_mc_vwrite, dim3(nblk(ci.ncubes)), dim3(256), 0, s, g, ci, bits, idx, offs, base, verts);
+void launch_mc_compact(hipStream_t s, const DevGrid &g, const uint32_t *bits, const uint32_t *segmask,
+                       const int *segoff, int *anode, uint8_t *acase) {
+    const SegIter si = seg_iter(g);
+    if (si.nseg)
+        hipLaunchKernelGGL(k_mc_compact, dim3(nblk(si.nseg)), dim3(256), 0, s, g, si, bits, segmask, segoff, anode,
+                           acase);
 }
 
-void launch_mc_tcount(hipStream_t s, const DevGrid &g, const uint32_t *bits, const int *idx, int *counts) {
-    const CubeIter ci = cube_iter(g);
-    if (ci.ncubes) hipLaunchKernelGGL(k_mc_tcount, dim3(nblk(ci.ncubes)), dim3(256), 0, s, g, ci, bits, idx, counts);
+void launch_mc_counts(hipStream_t s, const DevGrid &g, int nact, const int *anode, const uint8_t *acase,
+                      const int *idx, int *nv, int *nt) {
+    if (nact) hipLaunchKernelGGL(k_mc_counts, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, idx, nv, nt);
 }
 
-void launch_mc_twrite(hipStream_t s, const DevGrid &g, const uint32_t *bits, const int *idx, const int *offs,
-                      const double *verts, int *tris, double *tarea) {
-    const CubeIter ci = cube_iter(g);
-    if (ci.ncubes)
-        hipLaunchKernelGGL(k_mc_twrite, dim3(nblk(ci.ncubes)), dim3(256), 0, s, g, ci, bits, idx, offs, verts, tris,
-                           tarea);
+void launch_mc_vwrite(hipStream_t s, const DevGrid &g, int nact, const int *anode, const uint8_t *acase, int *idx,
+                      const int *voff, int base, double *verts) {
+    if (nact)
+        hipLaunchKernelGGL(k_mc_vwrite, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, idx, voff, base,
+                           verts);
+}
+
+void launch_mc_twrite(hipStream_t s, const DevGrid &g, int nact, const int *anode, const uint8_t *acase,
+                      const int *idx, const int *toff, const double *verts, int *tris, double *tarea) {
+    if (nact)
+        hipLaunchKernelGGL(k_mc_twrite, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, idx, toff, verts,
+                           tris, tarea);
 }
 
 }  // namespace nsg
