@@ -76,8 +76,26 @@ def cpu_baseline(grid, ps, budget_s):
             "sample": "full workload, %d run(s), median %.3f s per step" % (len(times), sec)}
 
 
+def copy_reference(nbytes, repeats=5):
+    """Plain device-to-device copy of known size: the calibration kernel for
+    the FETCH_SIZE / WRITE_SIZE counters (a copy reads and writes nbytes)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    for _ in range(repeats):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_reference_bytes": nbytes, "repeats": repeats}))
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--copy-ref", type=int, default=0,
+                    help="only run a device-to-device copy of this many bytes (counter calibration)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -86,6 +104,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", type=int, default=0)
     args = ap.parse_args()
+    if args.copy_ref:
+        copy_reference(args.copy_ref)
+        return
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
@@ -127,8 +148,16 @@ def main():
     flops = sum(OPS[k] * pc[k] for k in OPS)
     t_int = stage_avg["intersect"] * 1e-3
     achieved = flops / t_int / 1e12 if t_int > 0 else 0.0
+    # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs
+    # (calibrated against a copy kernel) committed under profiles/
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "ses_c2_5000atoms_traffic.json")
+    if args.config == "c2" and args.precision == 0 and os.path.exists(tfile):
+        t = json.load(open(tfile)).get("kernels", {}).get("nsg::k_intersect<false>")
+        if t:
+            traffic = t["fetch_bytes"] + t["write_bytes"]
     roofline = {"kernel": "k_intersect", "bound": "fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                 "flops_per_launch": flops, "path_counts": pc}
     fill_bytes = 17.0 * nvox
     t_fill = stage_avg["fill"] * 1e-3
