@@ -58,7 +58,7 @@ struct nsgpu_ctx {
     DBuf pkind, poff, pcnt, pflags, pbsph, pdata;
     // rays
     int K = 16;
-    DBuf rcnt, rt, rvol, partial, stats, cand;
+    DBuf rcnt, rt, rvol, partial, stats, cand, seg, segst;
     int nslots = 0, panels[2] = {0, 0};
     // maps
     DBuf eps, ideb, status, bits;
@@ -124,6 +124,8 @@ static DevRays dev_rays(const nsgpu_ctx *c) {
     r.cnt = c->rcnt.as<int>();
     r.t = c->rt.as<double>();
     r.rvol = c->rvol.as<double>();
+    r.seg = c->seg.as<int>();
+    r.seg_st = c->segst.as<int>();
     return r;
 }
 
@@ -159,7 +161,7 @@ void nsgpu_destroy(nsgpu_ctx *c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     DBuf *all[] = {&c->gx, &c->gy, &c->gz, &c->pkind, &c->poff, &c->pcnt, &c->pflags, &c->pbsph, &c->pdata,
-                   &c->rcnt, &c->rt, &c->rvol, &c->partial, &c->stats, &c->cand, &c->eps, &c->ideb, &c->status,
+                   &c->rcnt, &c->rt, &c->rvol, &c->partial, &c->stats, &c->cand, &c->seg, &c->segst, &c->eps, &c->ideb, &c->status,
                    &c->bits, &c->parent, &c->nroots, &c->rootlist, &c->bkey, &c->ccount, &c->clabel, &c->cfilled,
                    &c->idx, &c->vcount, &c->voffs, &c->scan_scratch, &c->verts, &c->tris, &c->tarea,
                    &c->segmask, &c->anode, &c->acase, &c->anv, &c->ant, &c->avoff, &c->atoff};
@@ -315,6 +317,8 @@ static int trace(nsgpu_ctx *c) {
     P.data = c->pdata.as<double>();
     HIP_TRY(c, c->rcnt.ensure(nrays * 4));
     HIP_TRY(c, c->rvol.ensure(nrays * 8));
+    HIP_TRY(c, c->seg.ensure((size_t)nrays * kSegWords * 4));
+    HIP_TRY(c, c->segst.ensure((size_t)N * N * kSegWords * 4));
     // per-workgroup partials: candidates of the intersection, (hits, odd, longest) of the sort
     const DevRays R0 = dev_rays(c);
     const int64_t irows = std::max<int64_t>(intersect_workgroups(P, R0), 1), srows = sort_blocks(R0);
@@ -508,11 +512,15 @@ int nsgpu_get_surf(nsgpu_ctx *c, const nsgpu_surf_opts *o, nsgpu_surf_result *re
     HIP_TRY(c, c->status.ensure((size_t)nvox * 4));
     HIP_TRY(c, c->bits.ensure((size_t)nwords * 4));
     mark(c, ST_FILL, 0);
-    if (c->panels[0] > 0) launch_fill_maps(c->stream, g, R, fo, c->eps.as<int>(), c->ideb.as<uint8_t>(), c->status.as<int>());
+    if (c->panels[0] > 0)
+        launch_fill_tiled(c->stream, g, R, fo, c->eps.as<int>(), c->ideb.as<uint8_t>(), c->status.as<int>());
     HIP_TRY(c, hipGetLastError());
     mark(c, ST_FILL, 1);
     mark(c, ST_CORNERS, 0);
-    if (o->accurate_triangulation) launch_fill_corners(c->stream, g, R, fo, c->bits.as<uint32_t>());
+    if (o->accurate_triangulation) {
+        HIP_TRY(c, hipMemsetAsync(c->bits.p, 0xff, (size_t)nwords * 4, c->stream));
+        launch_corners_tiled(c->stream, g, R, fo, c->bits.as<uint32_t>());
+    }
     HIP_TRY(c, hipGetLastError());
     mark(c, ST_CORNERS, 1);
     // volume: A * sum of segment lengths / (panels x phases traced), Surface.cpp:1677-1684

@@ -35,7 +35,14 @@ struct DevRays {
     int *cnt;         // [nslots*N*N]
     double *t;        // [nslots*N*N*K]
     double *rvol;     // [nslots*N*N] segment length per ray
+    int *seg;         // [nslots*N*N*8] interval record per ray (see kSegWords)
+    int *seg_st;      // [N*N*8] status/idebmap intervals of the x grid rays
 };
+
+// Interval record of a ray: word 0 = number of node intervals (-1: more than
+// kSegMax, evaluate from the depths), words 1..kSegMax = lo | hi << 16.
+constexpr int kSegWords = 8;
+constexpr int kSegMax = kSegWords - 1;
 
 // rintp, tools.cpp:43-46
 __device__ __forceinline__ double rintp(double x) { return floor(x + 0.5); }

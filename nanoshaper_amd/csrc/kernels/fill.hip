@@ -116,6 +116,154 @@ __global__ void __launch_bounds__(256) k_fill_corners(DevGrid g, DevRays R, Fill
     if (lane == 0 && w0 + 1 < nwords) bits[w0 + 1] = (uint32_t)(outside >> 32);
 }
 
+// ------------------------------------------------------------- tiled fill
+// A workgroup owns a 64 x 8 x 8 (i, j, k) tile of nodes and stages the interval
+// records of the rays through the tile in LDS once: 8 x 64 z rays (i, j),
+// 8 x 64 y rays (i, k) and 8 x 8 x rays (j, k).  Each wave then walks (j, k)
+// pairs; its 64 lanes take 64 consecutive i, test integer intervals and write
+// epsmap, idebmap and status for those nodes once, coalesced.
+constexpr int TI = 64, TJ = 8, TK = 8;
+
+__device__ __forceinline__ bool rec_has(const int *rec, int i) {
+    const int n = rec[0];
+    for (int q = 0; q < n; ++q) {
+        const unsigned int w = (unsigned int)rec[1 + q];
+        if (i >= (int)(w & 0xffffu) && i <= (int)(w >> 16)) return true;
+    }
+    return false;
+}
+
+__device__ __forceinline__ void stage_records(int *dst, const int *src, int slot, int64_t NN, int N, int outer0,
+                                              int nouter, int inner0, int ninner, int outer_lim, int inner_lim) {
+    // dst[o][i][q] <- record of ray (outer0+o)*N + inner0+i in slot; 0 outside the grid
+    const int total = nouter * ninner * kSegWords;
+    for (int w = threadIdx.x; w < total; w += blockDim.x) {
+        const int o = w / (ninner * kSegWords), rem = w - o * (ninner * kSegWords);
+        const int ii = rem / kSegWords, q = rem - ii * kSegWords;
+        const int oo = outer0 + o, ix = inner0 + ii;
+        int v = 0;
+        if (slot >= 0 && oo < outer_lim && ix < inner_lim)
+            v = src[((int64_t)slot * NN + (int64_t)oo * N + ix) * kSegWords + q];
+        dst[w] = v;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_fill_tiled(DevGrid g, DevRays R, FillOpts o, int *epsmap, uint8_t *idebmap,
+                                                    int *status) {
+    __shared__ int zrec[TJ * TI * kSegWords];
+    __shared__ int yrec[TK * TI * kSegWords];
+    __shared__ int xrec[TK * TJ * kSegWords];
+    __shared__ int xst[TK * TJ * kSegWords];
+    const int i0 = blockIdx.x * TI, j0 = blockIdx.y * TJ, k0 = blockIdx.z * TK;
+    const int64_t NN = (int64_t)R.N * R.N;
+    stage_records(zrec, R.seg, o.build_eps ? o.slot_z : -1, NN, R.N, j0, TJ, i0, TI, g.ny, g.nx);
+    stage_records(yrec, R.seg, o.build_eps ? o.slot_y : -1, NN, R.N, k0, TK, i0, TI, g.nz, g.nx);
+    stage_records(xrec, R.seg, o.slot_x, NN, R.N, k0, TK, j0, TJ, g.nz, g.ny);
+    stage_records(xst, R.seg_st, o.slot_x >= 0 ? 0 : -1, 0, R.N, k0, TK, j0, TJ, g.nz, g.ny);
+    __syncthreads();
+    const int ii = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = i0 + ii;
+    if (i >= g.nx) return;
+    for (int p = wv; p < TJ * TK; p += 4) {
+        const int jj = p % TJ, kk = p / TJ;
+        const int j = j0 + jj, k = k0 + kk;
+        if (j >= g.ny || k >= g.nz) continue;
+        bool e0 = false, e1 = false, e2 = false, in = false;
+        if (o.slot_x >= 0) {
+            const int *xe = xrec + (kk * TJ + jj) * kSegWords, *xs = xst + (kk * TJ + jj) * kSegWords;
+            if (xe[0] >= 0 && xs[0] >= 0) {
+                e0 = rec_has(xe, i);
+                in = rec_has(xs, i);
+            } else {
+                grid_ray<true, true>(R, o.slot_x * NN + (int64_t)k * R.N + j, g.x, g.nx, g.xmin, g.scale, g.hside, i,
+                                     e0, in);
+            }
+        }
+        const int64_t f = ((int64_t)k * g.ny + j) * g.nx + i;
+        if (o.build_eps) {
+            bool dummy = false;
+            if (o.slot_z >= 0) {
+                const int *zr = zrec + (jj * TI + ii) * kSegWords;
+                if (zr[0] >= 0) e2 = rec_has(zr, k);
+                else grid_ray<false, false>(R, o.slot_z * NN + (int64_t)j * R.N + i, g.z, g.nz, g.zmin, g.scale,
+                                            g.hside, k, e2, dummy);
+            }
+            if (o.slot_y >= 0) {
+                const int *yr = yrec + (kk * TI + ii) * kSegWords;
+                if (yr[0] >= 0) e1 = rec_has(yr, j);
+                else grid_ray<false, false>(R, o.slot_y * NN + (int64_t)k * R.N + i, g.y, g.ny, g.ymin, g.scale,
+                                            g.hside, j, e1, dummy);
+            }
+            int *e = epsmap + 3 * f;
+            e[0] = e0 ? o.inside : 0;
+            e[1] = e1 ? o.inside : 0;
+            e[2] = e2 ? o.inside : 0;
+            idebmap[f] = in ? 0 : 1;
+        }
+        if (o.build_status) status[f] = in ? NSGPU_STATUS_INSIDE : NSGPU_STATUS_TEMP_OUT;
+    }
+}
+
+// Corner bitmap from the edge rays, same tiling; bits start at 1 (outside)
+// and each 64-node row segment clears its inside bits with at most three
+// atomicAnd on the words it overlaps.
+__global__ void __launch_bounds__(256) k_corners_tiled(DevGrid g, DevRays R, FillOpts o, uint32_t *bits) {
+    __shared__ int zrec[TJ * TI * kSegWords];
+    __shared__ int yrec[TK * TI * kSegWords];
+    __shared__ int xrec[TK * TJ * kSegWords];
+    const int i0 = blockIdx.x * TI, j0 = blockIdx.y * TJ, k0 = blockIdx.z * TK;
+    const int64_t NN = (int64_t)R.N * R.N;
+    stage_records(zrec, R.seg, o.eslot_z, NN, R.N, j0, TJ, i0, TI, g.ny, g.nx);
+    stage_records(yrec, R.seg, o.eslot_y, NN, R.N, k0, TK, i0, TI, g.nz, g.nx);
+    stage_records(xrec, R.seg, o.eslot_x, NN, R.N, k0, TK, j0, TJ, g.nz, g.ny);
+    __syncthreads();
+    const int ii = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = i0 + ii;
+    for (int p = wv; p < TJ * TK; p += 4) {
+        const int jj = p % TJ, kk = p / TJ;
+        const int j = j0 + jj, k = k0 + kk;
+        if (j >= g.ny || k >= g.nz) continue;
+        bool inside = false;
+        if (i < g.nx) {
+            const int *xr = xrec + (kk * TJ + jj) * kSegWords;
+            const int *zr = zrec + (jj * TI + ii) * kSegWords;
+            const int *yr = yrec + (kk * TI + ii) * kSegWords;
+            inside = (xr[0] >= 0 ? rec_has(xr, i)
+                                 : edge_ray(R, o.eslot_x * NN + (int64_t)k * R.N + j, g.x, g.nx, g.xmin, g.scale,
+                                            g.hside, i)) ||
+                     (zr[0] >= 0 ? rec_has(zr, k)
+                                 : edge_ray(R, o.eslot_z * NN + (int64_t)j * R.N + i, g.z, g.nz, g.zmin, g.scale,
+                                            g.hside, k)) ||
+                     (yr[0] >= 0 ? rec_has(yr, j)
+                                 : edge_ray(R, o.eslot_y * NN + (int64_t)k * R.N + i, g.y, g.ny, g.ymin, g.scale,
+                                            g.hside, j));
+        }
+        const unsigned long long m = __ballot(inside);
+        if (ii == 0 && m) {
+            const int64_t f0 = ((int64_t)k * g.ny + j) * g.nx + i0;
+            const int64_t w0 = f0 >> 5;
+            const unsigned int s = (unsigned int)(f0 & 31);
+            const uint32_t a = (uint32_t)(m << s);
+            const uint32_t b = (uint32_t)((s ? (m >> (32 - s)) : (m >> 32)) & 0xffffffffull);
+            const uint32_t c = s ? (uint32_t)(m >> (64 - s)) : 0u;
+            if (a) atomicAnd(bits + w0, ~a);
+            if (b) atomicAnd(bits + w0 + 1, ~b);
+            if (c) atomicAnd(bits + w0 + 2, ~c);
+        }
+    }
+}
+
+void launch_fill_tiled(hipStream_t s, const DevGrid &g, const DevRays &R, const FillOpts &o, int *eps, uint8_t *ideb,
+                       int *status) {
+    dim3 grid((unsigned)((g.nx + TI - 1) / TI), (unsigned)((g.ny + TJ - 1) / TJ), (unsigned)((g.nz + TK - 1) / TK));
+    hipLaunchKernelGGL(k_fill_tiled, grid, dim3(256), 0, s, g, R, o, eps, ideb, status);
+}
+
+void launch_corners_tiled(hipStream_t s, const DevGrid &g, const DevRays &R, const FillOpts &o, uint32_t *bits) {
+    dim3 grid((unsigned)((g.nx + TI - 1) / TI), (unsigned)((g.ny + TJ - 1) / TJ), (unsigned)((g.nz + TK - 1) / TK));
+    hipLaunchKernelGGL(k_corners_tiled, grid, dim3(256), 0, s, g, R, o, bits);
+}
+
 // Deterministic two-level sum of the per-ray segment lengths of each panel.
 __global__ void __launch_bounds__(256) k_sum_blocks(const double *v, int64_t n, double *partial) {
     __shared__ double sh[256];

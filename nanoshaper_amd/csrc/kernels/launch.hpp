@@ -22,6 +22,8 @@ void launch_reduce_cols(hipStream_t, const unsigned long long *in, int64_t nrows
                         unsigned long long *out);
 void launch_fill_maps(hipStream_t, const DevGrid &, const DevRays &, const FillOpts &, int *, uint8_t *, int *);
 void launch_fill_corners(hipStream_t, const DevGrid &, const DevRays &, const FillOpts &, uint32_t *);
+void launch_fill_tiled(hipStream_t, const DevGrid &, const DevRays &, const FillOpts &, int *, uint8_t *, int *);
+void launch_corners_tiled(hipStream_t, const DevGrid &, const DevRays &, const FillOpts &, uint32_t *);
 int64_t sum_partials_count(int64_t n);
 void launch_sum_blocks(hipStream_t, const double *, int64_t, double *);
 void launch_ccl(hipStream_t, const DevGrid &, const int *, int *, int *, int *);

@@ -334,14 +334,49 @@ __global__ void __launch_bounds__(256) k_sort(DevGrid g, DevRays R, unsigned lon
             dir = (ga[na - 1] + delta) - pa;
         }
         double vol = 0.0;
+        const double gmin = panel == 0 ? g.xmin : (panel == 1 ? g.zmin : g.ymin);
+        // node intervals of the map fill, from the same double comparisons
+        // setVerticesAndGridsWithIntersectionData makes per node
+        // (Surface.cpp:5427-5520, :5636-5650, :5736-5750, :5990-6008)
+        int ne = 0, ns = 0;
+        int *E = R.seg + r * kSegWords;
+        int *S = (phase == 0 && panel == 0) ? R.seg_st + (r - (int64_t)slot * nper) * kSegWords : nullptr;
+        auto push = [](int *rec, int &n, int lo, int hi) {
+            if (lo > hi) return;
+            if (n >= 0 && n < kSegMax) rec[1 + n] = (lo & 0xffff) | (hi << 16);
+            n = (n >= 0 && n < kSegMax) ? n + 1 : -1;
+        };
         if (c <= R.K && (c & 1) == 0) {
             for (int i = 0; i < c; i += 2)
                 if (fabs(T[i + 1] - T[i]) >= kEpsInt) {
                     const double lim1 = pa + dir * T[i];
                     const double lim2 = pa + dir * T[i + 1];
                     vol += lim2 - lim1;
+                    const int a = (int)rintp((lim1 - gmin) * g.scale);
+                    const int b = (int)rintp((lim2 - gmin) * g.scale);
+                    if (phase == 1) {
+                        // corners a < n <= b
+                        if (b > a) push(E, ne, max(a + 1, 0), min(b, na - 1));
+                        continue;
+                    }
+                    const int i1 = a < 0 ? 0 : (a >= na ? na - 1 : a);
+                    const int i2 = b < 0 ? 0 : (b >= na ? na - 1 : b);
+                    if (i2 < i1) continue;
+                    // panel 0 covers [i1, i2], panels 1 and 2 the half-open [i1, i2)
+                    const int last = panel == 0 ? i2 : i2 - 1;
+                    int ie = last;
+                    while (ie >= i1 && !(ga[ie] <= lim2 - g.hside)) --ie;
+                    push(E, ne, i1, ie);
+                    if (S) {
+                        int is = i1, it = i2;
+                        while (is <= i2 && !(ga[is] >= lim1)) ++is;
+                        while (it >= is && !(ga[it] <= lim2)) --it;
+                        push(S, ns, is, it);
+                    }
                 }
         }
+        E[0] = ne;
+        if (S) S[0] = ns;
         R.rvol[r] = vol;
         odd = (c & 1);
     }
