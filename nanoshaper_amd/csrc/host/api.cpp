@@ -53,9 +53,9 @@ struct nsgpu_ctx {
     DBuf gx, gy, gz;
     // patches
     bool have_patches = false;
-    int npatch = 0;
+    int npatch = 0, npatch_plain = 0;
     double probe = 1.4;
-    DBuf pkind, poff, pcnt, pflags, pbsph, pdata;
+    DBuf pkind, poff, pcnt, pflags, pbsph, pdata, porder;
     // rays
     int K = 16;
     DBuf rcnt, rt, rvol, partial, stats, cand, seg, segst;
@@ -160,7 +160,7 @@ void nsgpu_destroy(nsgpu_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    DBuf *all[] = {&c->gx, &c->gy, &c->gz, &c->pkind, &c->poff, &c->pcnt, &c->pflags, &c->pbsph, &c->pdata,
+    DBuf *all[] = {&c->gx, &c->gy, &c->gz, &c->pkind, &c->poff, &c->pcnt, &c->pflags, &c->pbsph, &c->pdata, &c->porder,
                    &c->rcnt, &c->rt, &c->rvol, &c->partial, &c->stats, &c->cand, &c->seg, &c->segst, &c->eps, &c->ideb, &c->status,
                    &c->bits, &c->parent, &c->nroots, &c->rootlist, &c->bkey, &c->ccount, &c->clabel, &c->cfilled,
                    &c->idx, &c->vcount, &c->voffs, &c->scan_scratch, &c->verts, &c->tris, &c->tarea,
@@ -248,8 +248,19 @@ int nsgpu_set_patches(nsgpu_ctx *c, const nsgpu_patches *p, double probe_radius)
     }
     if (p->n_data)
         HIP_TRY(c, hipMemcpyAsync(c->pdata.p, p->data, (size_t)p->n_data * 8, hipMemcpyHostToDevice, c->stream));
+    // patch ids by class: spheres and facets first, then tori
+    std::vector<int> order;
+    order.reserve(n);
+    for (int i = 0; i < p->n; ++i)
+        if (p->kind[i] < NSGPU_PATCH_TORUS_REG) order.push_back(i);
+    const int n_plain = (int)order.size();
+    for (int i = 0; i < p->n; ++i)
+        if (p->kind[i] >= NSGPU_PATCH_TORUS_REG) order.push_back(i);
+    HIP_TRY(c, c->porder.ensure(std::max<size_t>(n, 1) * 4));
+    if (n) HIP_TRY(c, hipMemcpyAsync(c->porder.p, order.data(), n * 4, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->npatch = p->n;
+    c->npatch_plain = n_plain;
     c->probe = probe_radius;
     c->have_patches = true;
     c->have_surf = c->have_vertices = c->have_mesh = false;
@@ -276,6 +287,8 @@ int nsgpu_intersect_path_counts(nsgpu_ctx *c, const nsgpu_surf_opts *o, int64_t 
     P.flags = c->pflags.as<int>();
     P.bsph = c->pbsph.as<double>();
     P.data = c->pdata.as<double>();
+    P.order = c->porder.as<int>();
+    P.n_plain = c->npatch_plain;
     const int64_t rows = intersect_workgroups(P, R);
     DBuf part, out;
     HIP_TRY(c, part.ensure((size_t)std::max<int64_t>(rows, 1) * kPathCounters * 8));
@@ -315,6 +328,8 @@ static int trace(nsgpu_ctx *c) {
     P.flags = c->pflags.as<int>();
     P.bsph = c->pbsph.as<double>();
     P.data = c->pdata.as<double>();
+    P.order = c->porder.as<int>();
+    P.n_plain = c->npatch_plain;
     HIP_TRY(c, c->rcnt.ensure(nrays * 4));
     HIP_TRY(c, c->rvol.ensure(nrays * 8));
     HIP_TRY(c, c->seg.ensure((size_t)nrays * kSegWords * 4));
@@ -331,7 +346,8 @@ static int trace(nsgpu_ctx *c) {
         HIP_TRY(c, hipMemsetAsync(c->rcnt.p, 0, nrays * 4, c->stream));
         HIP_TRY(c, hipMemsetAsync(c->cand.p, 0, (size_t)irows * 8, c->stream));
         mark(c, ST_INTERSECT, 0);
-        launch_intersect(c->stream, g, P, R, c->probe, c->cand.as<unsigned long long>());
+        launch_intersect(c->stream, g, P, R, c->probe, c->cand.as<unsigned long long>(),
+                         c->opts.precision == NSGPU_PREC_MIXED);
         HIP_TRY(c, hipGetLastError());
         mark(c, ST_INTERSECT, 1);
         mark(c, ST_SORT, 0);

@@ -24,6 +24,11 @@ struct DevPatches {
     const int *kind, *off, *cnt, *flags;
     const double *bsph;  // 4 per patch
     const double *data;
+    // patch ids ordered by class: [0, n_plain) spheres and facets, then tori;
+    // the two classes run as separate launches (the torus path needs far more
+    // registers than the plain one)
+    const int *order;
+    int n_plain;
 };
 
 // Ray hit storage: ray r of slot s keeps up to K depths at hits[(s*NN+r)*K].

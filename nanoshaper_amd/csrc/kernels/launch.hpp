@@ -14,7 +14,7 @@ constexpr int kPathCounters = 8;
 int64_t intersect_workgroups(const DevPatches &, const DevRays &);
 int64_t sort_blocks(const DevRays &);
 void launch_intersect(hipStream_t, const DevGrid &, const DevPatches &, const DevRays &, double,
-                      unsigned long long *part);
+                      unsigned long long *part, bool mixed);
 void launch_intersect_count(hipStream_t, const DevGrid &, const DevPatches &, const DevRays &, double,
                             unsigned long long *part);
 void launch_sort(hipStream_t, const DevGrid &, const DevRays &, unsigned long long *part);

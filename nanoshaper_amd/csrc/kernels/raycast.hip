@@ -150,25 +150,45 @@ __device__ int ray_torus(const double *td, const double sc[3], double probe, dou
 }
 
 // ConnollySurface::isFeasible, ConnollySurface.cpp:4769-5021, on the SoA form
-// (plane signs applied when the patches were built).
-__device__ bool feasible(int kind, const double *d, int cnt, int flags, const double p[3]) {
-    if (kind == NSGPU_PATCH_FACET_REG || kind == NSGPU_PATCH_FACET_SING) {
-        for (int i = 0; i < cnt; i++) {
-            const double *pl = d + 4 * i;
-            const double tst = (pl[0] * p[0] + pl[1] * p[1] + pl[2] * p[2]) + pl[3];
-            if (tst > 0) return false;
-        }
-        return true;
-    }
+// (plane signs applied when the patches were built).  Every item is the
+// reference's FP64 expression; items are taken four at a time (the tail
+// repeats the last item), so the wave issues the uniform loads of a group
+// together and branches once per group.  Each test is a pure function of the
+// point, so evaluating a group's extra items does not change the result.
+__device__ __forceinline__ bool plane_out(const double *q, const double p[3]) {
+    return (q[0] * p[0] + q[1] * p[1] + q[2] * p[2]) + q[3] > 0;
+}
+__device__ __forceinline__ bool in_clip_sphere(const double *q, const double p[3]) {
+    double dd = (q[0] - p[0]) * (q[0] - p[0]) + (q[1] - p[1]) * (q[1] - p[1]) + (q[2] - p[2]) * (q[2] - p[2]);
+    dd -= q[3];
+    return dd < 0;
+}
+
+__device__ bool feasible_plain(int kind, const double *d, int cnt, const double p[3]) {
     if (kind == NSGPU_PATCH_POINT) {
-        for (int i = 0; i < cnt; i++) {
-            const double *s = d + 4 * i;
-            double dd = (s[0] - p[0]) * (s[0] - p[0]) + (s[1] - p[1]) * (s[1] - p[1]) + (s[2] - p[2]) * (s[2] - p[2]);
-            dd -= s[3];
-            if (dd < 0) return false;
+        for (int i = 0; i < cnt; i += 4) {
+            const int j1 = min(i + 1, cnt - 1), j2 = min(i + 2, cnt - 1), j3 = min(i + 3, cnt - 1);
+            if (in_clip_sphere(d + 4 * i, p) | in_clip_sphere(d + 4 * j1, p) | in_clip_sphere(d + 4 * j2, p) |
+                in_clip_sphere(d + 4 * j3, p))
+                return false;
         }
         return true;
     }
+    for (int i = 0; i < cnt; i += 4) {
+        const int j1 = min(i + 1, cnt - 1), j2 = min(i + 2, cnt - 1), j3 = min(i + 3, cnt - 1);
+        if (plane_out(d + 4 * i, p) | plane_out(d + 4 * j1, p) | plane_out(d + 4 * j2, p) | plane_out(d + 4 * j3, p))
+            return false;
+    }
+    return true;
+}
+
+// torus: outside the self-intersection sphere, then any visible plane pair
+// (acute pair: both planes; obtuse pair: either plane)
+__device__ __forceinline__ bool pair_ok(const double *pl, int acute, const double p[3]) {
+    const bool le1 = !plane_out(pl, p), le2 = !plane_out(pl + 4, p);
+    return acute ? (le1 & le2) : (le1 | le2);
+}
+__device__ bool feasible_torus(int kind, const double *d, int cnt, int flags, const double p[3]) {
     const double r = d[13];
     if (r >= 0.) {
         const double dd = (d[0] - p[0]) * (d[0] - p[0]) + (d[1] - p[1]) * (d[1] - p[1]) + (d[2] - p[2]) * (d[2] - p[2]);
@@ -176,16 +196,194 @@ __device__ bool feasible(int kind, const double *d, int cnt, int flags, const do
     }
     if (kind == NSGPU_PATCH_TORUS_SING) return true;
     const double *pl = d + NSGPU_TORUS_HDR;
-    for (int i = 0; i < cnt; i++, pl += 8) {
-        const double t1 = (pl[0] * p[0] + pl[1] * p[1] + pl[2] * p[2]) + pl[3];
-        const double t2 = (pl[4] * p[0] + pl[5] * p[1] + pl[6] * p[2]) + pl[7];
-        if ((flags >> i) & 1) {
-            if (t1 > 0 || t2 > 0) continue;
-            return true;
-        }
-        if (t1 <= 0 || t2 <= 0) return true;
+    for (int i = 0; i < cnt; i += 2) {
+        const int j = min(i + 1, cnt - 1);
+        if (pair_ok(pl + 8 * i, (flags >> i) & 1, p) | pair_ok(pl + 8 * j, (flags >> j) & 1, p)) return true;
     }
     return false;
+}
+
+// FP64 closed forms as out-of-line calls for the mixed solver's rare
+// fallbacks, so that their registers do not count against the common path
+__device__ __attribute__((noinline)) double cubic_root_fallback(double b, double c, double d) {
+    return cubic_root(b, c, d);
+}
+__device__ __attribute__((noinline)) int quartic_roots_fallback(double roots[4], double b, double c, double d,
+                                                                double e) {
+    return quartic_roots(roots, b, c, d, e);
+}
+
+// Mixed precision, resolvent cubic of the quartic: the branch decisions of
+// realCubicSolution are taken in FP64 exactly as in cubic_root; where the
+// FP64 path would evaluate cbrt or acos/cos, the closed form is evaluated in
+// FP32 (hardware transcendentals) as a seed and refined by Newton's method in
+// FP64 on x^3 + b x^2 + c x + d.  A seed whose refinement does not settle
+// within two steps to 1e-12 relative falls back to the FP64 closed form.
+__device__ double cubic_root_mixed(double b, double c, double d) {
+    const double p = -(1.0 / 3.0) * b * b + c;
+    const double q = (2. / 27.) * b * b * b - (1. / 3.) * b * c + d;
+    const double D3p = (1. / 27.) * p * p * p;
+    const double D3 = D3p + 0.25 * q * q;
+    if (fabs(c) < b * b * 1.E-6 && fabs(d) < fabs(b) * 1.E-6) return c / b - b - d / (b * b);
+    if (fabs(d / c) < 4.E-4) return cubic_root_fallback(b, c, d);
+    float x32;
+    if (D3 >= 0.) {
+        const float radD3 = sqrtf((float)D3), hq = (float)(0.5 * q);
+        x32 = cbrtf(radD3 - hq) - cbrtf(radD3 + hq) - (float)((1. / 3.) * b);
+    } else {
+        const float Aarg = acosf((float)(-0.5 * q / sqrt(-D3p)));
+        x32 = 2.f * sqrtf((float)(-(1. / 3.) * p)) * cosf((1.f / 3.f) * Aarg) - (float)((1. / 3.) * b);
+    }
+    double x = x32;
+    for (int it = 0; it < 2; ++it) {
+        const double g = ((x + b) * x + c) * x + d;
+        const double gp = (3. * x + 2. * b) * x + c;
+        if (gp == 0.) return cubic_root_fallback(b, c, d);
+        x -= g / gp;
+    }
+    const double g = ((x + b) * x + c) * x + d;
+    const double gp = (3. * x + 2. * b) * x + c;
+    if (!(fabs(g) <= 1e-12 * fabs(gp) * (1. + fabs(x)))) return cubic_root_fallback(b, c, d);
+    return x;
+}
+
+// relative size of the quartic's discriminant below which a root pair counts
+// as near-double
+constexpr double kNearTangent = 1e-8;
+
+// quarticEqSolutions with the mixed resolvent root: the classification, the
+// variant switch and the root formulas are those of quartic_roots in FP64;
+// only the resolvent root m comes from cubic_root_mixed, except for
+// near-double roots, which take quartic_roots whole.  A root m close to
+// the m <= 0 or m < 1e-14 thresholds is recomputed by the FP64 closed form so
+// that the root count and formula branch are the reference's.
+__device__ int quartic_roots_mixed(double roots[4], double b, double c, double d, double e) {
+    const double b_2 = 0.5 * b, b_4 = 0.25 * b;
+    double p = c - 1.5 * (b_2 * b_2);
+    double q = d + b_2 * (b_2 * b_2 - c);
+    double r = e + b_4 * (b_4 * (c - 3.0 * b_4 * b_4) - d);
+    double f = p * p - 4. * r;
+    double delta = 4. * f * (4. * r * (p * p) - p * (q * q) - 16. * (r * r)) + (q * q) * (128. * p * r - 27. * (q * q));
+    bool change_variant = false;
+    if (e != 0. && (fabs(p) > 3.E+3 || fabs(r) > 3.E+3 || fabs(q) > 3.E+3 || fabs(delta) > 1.E+14)) {
+        const double dsu2e = 0.5 * d / e;
+        change_variant = true;
+        p = -1.5 * dsu2e * dsu2e + c / e;
+        q = dsu2e * (dsu2e * dsu2e - c / e) + b / e;
+        r = dsu2e * (-3. / 16. * (dsu2e * dsu2e * dsu2e) + 0.25 * dsu2e * (c / e) - 0.5 * b / e) + 1. / e;
+        f = p * p - 4. * r;
+        delta = 4. * f * (4. * r * (p * p) - p * (q * q) - 16. * (r * r)) + (q * q) * (128. * p * r - 27. * (q * q));
+    }
+    // near-multiple roots (grazing rays): the discriminant is small against
+    // its own terms and the resolvent cubic shares the near-double root; the
+    // reference's FP64 closed form decides these rays
+    const double dmag = fabs(4. * f) * (fabs(4. * r * (p * p)) + fabs(p * (q * q)) + 16. * (r * r)) +
+                        (q * q) * (fabs(128. * p * r) + 27. * (q * q));
+    if (fabs(delta) <= kNearTangent * dmag) return quartic_roots_fallback(roots, b, c, d, e);
+    int num;
+    if (delta < 0. || (delta == 0. && (p >= 0. || f < 0.)))
+        num = 2;
+    else if ((p < 0. && f > 0.) || (delta == 0. && p == 0. && r == 0.))
+        num = 4;
+    else
+        return 0;
+    const double cb = p, cc = 0.25 * (p * p) - r, cd = (-0.125) * (q * q);
+    double m = cubic_root_mixed(cb, cc, cd);
+    if (m <= 1e-9 * (1. + fabs(cb))) m = cubic_root_fallback(cb, cc, cd);
+    if (m <= 0) return 0;
+    double coeff, p1, p2, p3 = 0, smax;
+    if (m < 1.E-14) {
+        p1 = p + p;
+        p2 = (q * q) / f;
+        p3 = 2. * sqrt(f);
+        smax = p3 - p1 - p2;
+        coeff = 2. * fabs(q) / f;
+    } else {
+        p1 = -2. * (p + m);
+        p2 = sqrt(2. * (q * q) / m);
+        smax = p1 + p2;
+        coeff = sqrt(2. * m);
+    }
+    roots[0] = 0.5 * (((q >= 0.) ? -coeff : coeff) - sqrt(smax));
+    roots[1] = 0.5 * (((q >= 0.) ? -coeff : coeff) + sqrt(smax));
+    if (num == 4) {
+        const double smin = (m < 1.E-14) ? -p1 - p2 - p3 : p1 - p2;
+        roots[2] = 0.5 * (((q >= 0.) ? coeff : -coeff) - sqrt(smin));
+        roots[3] = 0.5 * (((q >= 0.) ? coeff : -coeff) + sqrt(smin));
+    }
+    if (change_variant) {
+        for (int i = 0; i < num; i++) roots[i] = 1. / (roots[i] - 0.25 * d / e);
+    } else {
+        for (int i = 0; i < num; i++) roots[i] -= 0.25 * b;
+    }
+    return num;
+}
+
+// Bracketed Newton polish of a root s of x^4 + B x^3 + C x^2 + D x + E (the
+// closed form is off by up to ~1e-5 A on grazing rays of a 5,000-atom SES).
+// A step is taken only when it is a refinement (at most kPolishStep, in A
+// along the ray), the quartic changes sign across a bracket of twice the step
+// around s, and it lowers |q|; clustered pairs closer than the bracket keep s.
+constexpr double kPolishStep = 1e-4;
+
+__device__ __forceinline__ double polish_quartic_root(double s, double B, double C, double D, double E) {
+    auto q = [&](double x) { return (((x + B) * x + C) * x + D) * x + E; };
+    auto dq = [&](double x) { return ((4 * x + 3 * B) * x + 2 * C) * x + D; };
+    for (int it = 0; it < 3; ++it) {
+        const double f = q(s), fp = dq(s);
+        if (f == 0 || fp == 0) break;
+        const double step = f / fp;
+        if (fabs(step) <= 1e-12 * (1. + fabs(s))) {  // converged: a step this small cannot change roots
+            s -= step;
+            break;
+        }
+        if (!(fabs(step) <= kPolishStep)) break;
+        const double h = 2 * fabs(step) + 1e-12;
+        if (q(s - h) * q(s + h) > 0) break;
+        const double sn = s - step;
+        if (fabs(q(sn)) >= fabs(f)) break;
+        s = sn;
+    }
+    return s;
+}
+
+// rayTorus, mixed precision: the quartic's resolvent root from an FP32 seed
+// refined in FP64 and every quartic root Newton-polished in FP64; the frame
+// change, coefficients and clipping are those of ray_torus.
+__device__ int ray_torus_mixed(const double *td, const double sc[3], double probe, double radius, int vc,
+                               const double orig[3], double ray_dir, double t[4]) {
+    double o1[3] = {orig[0], orig[1], orig[2]};
+    o1[vc] += t[0] * ray_dir;
+    const double tx = o1[0] - td[0], ty = o1[1] - td[1], tz = o1[2] - td[2];
+    const double *iv = td + 3;
+    double pp[3];
+    for (int i = 0; i < 3; i++) pp[i] = iv[3 * i + 0] * tx + iv[3 * i + 1] * ty + iv[3 * i + 2] * tz;
+    const double r2 = probe * probe;
+    const double R2 = td[12] * td[12];
+    const double pd = pp[0] * iv[0 + vc] + pp[1] * iv[3 + vc] + pp[2] * iv[6 + vc];
+    const double p2 = pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2];
+    const double coeff = p2 - r2 - R2;
+    const double uz = iv[6 + vc];
+    const double B = 4 * pd;
+    const double C = 2 * coeff + 4 * pd * pd + 4 * R2 * uz * uz;
+    const double D = 4 * pd * coeff + 8 * R2 * pp[2] * uz;
+    const double E = coeff * coeff - 4 * R2 * (r2 - pp[2] * pp[2]);
+    double roots[4];
+    const int nr = quartic_roots_mixed(roots, B, C, D, E);
+    const double t0 = t[0];
+    int ni = 0;
+    for (int i = 0; i < nr; i++) {
+        double rt = roots[i];
+        if (rt < 0) continue;
+        rt = polish_quartic_root(rt, B, C, D, E);
+        if (rt < 0) continue;
+        double pt[3] = {o1[0], o1[1], o1[2]};
+        pt[vc] += rt;
+        const double dd = (sc[0] - pt[0]) * (sc[0] - pt[0]) + (sc[1] - pt[1]) * (sc[1] - pt[1]) +
+                          (sc[2] - pt[2]) * (sc[2] - pt[2]);
+        if (dd < radius * radius) t[ni++] = t0 + rt / ray_dir;
+    }
+    return ni;
 }
 
 __device__ __forceinline__ void push_hit(const DevRays &R, int slot, int64_t ray, double t) {
@@ -201,11 +399,15 @@ enum { PC_CAND, PC_DISC, PC_SPHERE, PC_TORUS, PC_ROOTS, PC_HITS, PC_FEAS_ITEMS, 
 // Per-workgroup counts go to the workgroup's own row of 'part' (zeroed before
 // the launch) and are summed by k_reduce_cols: one atomic per workgroup on a
 // shared address would serialise 10^5 workgroups at the L2.
-template <bool kCount>
+// kTorus selects the class of patches (P.order): the plain launch covers
+// spheres and facets, the torus launch the tori.  kMixed: torus roots from the
+// mixed-precision solver.  kCount: instead of hits, per-workgroup path counts.
+template <bool kCount, bool kMixed, bool kTorus>
 __global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRays R, double probe,
                                                   unsigned long long *part) {
-    const int64_t wg = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
-    const int it = blockIdx.x;
+    const int first = kTorus ? P.n_plain : 0;
+    const int64_t wg = (int64_t)blockIdx.y * P.n + first + blockIdx.x;
+    const int it = P.order[first + blockIdx.x];
     const int slot = blockIdx.y;
     const int phase = slot < R.panels0 ? 0 : 1;
     const int panel = phase == 0 ? slot : slot - R.panels0;
@@ -214,7 +416,6 @@ __global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRa
     const double radius = P.bsph[4 * it + 3];
     const double *pd = P.data + P.off[it];
     const int pcnt = P.cnt[it], pflags = P.flags[it];
-    const bool torus = kind == NSGPU_PATCH_TORUS_REG || kind == NSGPU_PATCH_TORUS_SING;
     const double down[3] = {sc[0] - radius, sc[1] - radius, sc[2] - radius};
     const double up[3] = {sc[0] + radius, sc[1] + radius, sc[2] + radius};
     // CELL_CULLING (ConnollySurface.cpp:4378-4391)
@@ -263,18 +464,19 @@ __global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRa
         if (!ray_sphere(pa, dir, sc, radius, t[0], t[1])) continue;
         if (kCount) ++pc[PC_SPHERE];
         int ni = 2;
-        if (torus) {
-            ni = ray_torus(pd, sc, probe, radius, vc, pa, ray_dir, t);
+        if (kTorus) {
+            ni = kMixed ? ray_torus_mixed(pd, sc, probe, radius, vc, pa, ray_dir, t)
+                        : ray_torus(pd, sc, probe, radius, vc, pa, ray_dir, t);
             if (kCount) ++pc[PC_TORUS];
         }
         for (int i = 0; i < ni; i++) {
             double ip[3] = {pa[0], pa[1], pa[2]};
             ip[vc] += t[i] * ray_dir;
-            const bool ok = feasible(kind, pd, pcnt, pflags, ip);
+            const bool ok = kTorus ? feasible_torus(kind, pd, pcnt, pflags, ip) : feasible_plain(kind, pd, pcnt, ip);
             if (kCount) {
                 ++pc[PC_ROOTS];
                 ++pc[PC_FEAS_CALLS];
-                pc[PC_FEAS_ITEMS] += (kind >= NSGPU_PATCH_TORUS_REG) ? 2 * pcnt + 1 : pcnt;
+                pc[PC_FEAS_ITEMS] += kTorus ? 2 * pcnt + 1 : pcnt;
                 pc[PC_HITS] += ok;
             }
             if (!ok) continue;
@@ -290,10 +492,11 @@ __global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRa
         }
         return;
     }
-    // candidate count: wave reduction, one atomic per workgroup
+    // candidate count: wave reduction, one store per workgroup
     for (int off = 32; off > 0; off >>= 1) cand += __shfl_down(cand, off, 64);
     if (threadIdx.x == 0 && cand) part[wg] = cand;
 }
+
 
 // Per-ray insertion sort of the depths (Surface::reorderPatchIntersections,
 // Surface.cpp:5080-5095), plus the ray's segment-length integral of
@@ -438,22 +641,32 @@ __global__ void __launch_bounds__(1024) k_reduce_cols(const unsigned long long *
 int64_t intersect_workgroups(const DevPatches &P, const DevRays &R) { return (int64_t)P.n * R.nslots; }
 int64_t sort_blocks(const DevRays &R) { return ((int64_t)R.N * R.N * R.nslots + 255) / 256; }
 
-// part: intersect_workgroups() rows, zeroed by the caller
-void launch_intersect(hipStream_t s, const DevGrid &g, const DevPatches &P, const DevRays &R, double probe,
-                      unsigned long long *part) {
+// part: intersect_workgroups() rows, zeroed by the caller.  Two launches: the
+// plain patches, then the tori (they need about twice the registers).
+template <bool kCount, bool kMixed>
+static void launch_classes(hipStream_t s, const DevGrid &g, const DevPatches &P, const DevRays &R, double probe,
+                           unsigned long long *part) {
     if (P.n == 0 || R.nslots == 0) return;
-    dim3 grid((unsigned)P.n, (unsigned)R.nslots);
-    hipLaunchKernelGGL(k_intersect<false>, grid, dim3(64), 0, s, g, P, R, probe, part);
+    if (P.n_plain > 0)
+        hipLaunchKernelGGL((k_intersect<kCount, kMixed, false>), dim3((unsigned)P.n_plain, (unsigned)R.nslots),
+                           dim3(64), 0, s, g, P, R, probe, part);
+    if (P.n > P.n_plain)
+        hipLaunchKernelGGL((k_intersect<kCount, kMixed, true>), dim3((unsigned)(P.n - P.n_plain), (unsigned)R.nslots),
+                           dim3(64), 0, s, g, P, R, probe, part);
 }
-
+void launch_intersect(hipStream_t s, const DevGrid &g, const DevPatches &P, const DevRays &R, double probe,
+                      unsigned long long *part, bool mixed) {
+    if (mixed)
+        launch_classes<false, true>(s, g, P, R, probe, part);
+    else
+        launch_classes<false, false>(s, g, P, R, probe, part);
+}
 // Counting variant: same candidate walk, no hit output; part gets PC_N
 // counters per workgroup (zeroed by the caller): how often each FP64 path of
 // the reference's candidate set runs.
 void launch_intersect_count(hipStream_t s, const DevGrid &g, const DevPatches &P, const DevRays &R, double probe,
                             unsigned long long *part) {
-    if (P.n == 0 || R.nslots == 0) return;
-    dim3 grid((unsigned)P.n, (unsigned)R.nslots);
-    hipLaunchKernelGGL(k_intersect<true>, grid, dim3(64), 0, s, g, P, R, probe, part);
+    launch_classes<true, false>(s, g, P, R, probe, part);
 }
 
 // part: sort_blocks() rows of 3

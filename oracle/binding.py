@@ -49,6 +49,7 @@ def load():
         "orc_ray_torus": (ctypes.c_int, [VP, VP, VP, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                          VP, VP, VP, VP]),
         "orc_is_feasible": (ctypes.c_int, [ctypes.POINTER(Patches), ctypes.c_int, VP]),
+        "orc_set_root_refinement": (None, [ctypes.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -65,10 +66,14 @@ def _ptr(a):
 class OracleSurface:
     """CPU restatement of Surface::getSurf / triangulateSurface on one patch set."""
 
-    def __init__(self, grid, patchset):
+    def __init__(self, grid, patchset, refine_roots=False):
+        """refine_roots: torus roots refined to the true root of the computed
+        quartic (extended-precision Newton), the yardstick of the device's
+        mixed-precision mode; False restates the reference exactly."""
         from nanoshaper_amd._lib import Grid  # noqa: F401
         self._lib = load()
         self.grid = grid
+        self.refine_roots = bool(refine_roots)
         self._h = self._lib.orc_create(ctypes.byref(grid), ctypes.byref(patchset.view), patchset.probe)
         self.result = None
         self.mesh_info = None
@@ -78,7 +83,11 @@ class OracleSurface:
         from nanoshaper_amd.surface import surf_opts
         o = surf_opts(**kw)
         r = SurfResult()
-        rc = self._lib.orc_get_surf(self._h, ctypes.byref(o), ctypes.byref(r))
+        self._lib.orc_set_root_refinement(int(self.refine_roots))
+        try:
+            rc = self._lib.orc_get_surf(self._h, ctypes.byref(o), ctypes.byref(r))
+        finally:
+            self._lib.orc_set_root_refinement(0)
         if rc != 0:
             raise RuntimeError("oracle get_surf failed")
         self.result = {f: getattr(r, f) for f, _ in SurfResult._fields_}

@@ -150,6 +150,36 @@ void orc_quartic(double roots[4], double b, double c, double d, double e, int *n
     }
 }
 
+/* Not part of the reference: when set, every torus root is refined by
+ * Newton's method in extended precision on the same double coefficients.
+ * That gives the true root of the computed quartic, the yardstick for the
+ * device's mixed-precision mode, whose roots are Newton-polished. */
+static int g_refine_roots = 0;
+void orc_set_root_refinement(int on) { g_refine_roots = on; }
+
+static double orc_refine_root(double s, double B, double C, double D, double E) {
+    /* Newton's method in extended precision from the closed-form root; the
+     * result is the iterate of smallest |q| within 1e-3 A of s (s itself when
+     * no iterate lowers |q|, e.g. a complex pair near the real axis) */
+    long double x = s, best = s;
+#define ORC_Q(y) (((((y) + B) * (y) + C) * (y) + D) * (y) + E)
+    long double fbest = fabsl(ORC_Q(x));
+    for (int it = 0; it < 100 && fbest > 0; it++) {
+        const long double f = ORC_Q(x);
+        const long double fp = ((4 * x + 3 * B) * x + 2 * C) * x + D;
+        if (fp == 0) break;
+        x -= f / fp;
+        if (fabsl(x - (long double)s) > 1e-3L) break;
+        const long double fx = fabsl(ORC_Q(x));
+        if (fx < fbest) {
+            fbest = fx;
+            best = x;
+        }
+    }
+#undef ORC_Q
+    return (double)best;
+}
+
 /* rayTorus, tools.cpp:907-1099 (analytical branch).  t[0] holds the entry of
  * the clipping sphere on input; returns the hits inside the clipping sphere. */
 int orc_ray_torus(const double *invrot /*3x3 row major*/, const double *torus_center,
@@ -183,6 +213,10 @@ int orc_ray_torus(const double *invrot /*3x3 row major*/, const double *torus_ce
     double t0 = t[0];
     for (int i = 0; i < numroots; i++) {
         if (roots[i] < 0) continue;
+        if (g_refine_roots) {
+            roots[i] = orc_refine_root(roots[i], B, C, D, E);
+            if (roots[i] < 0) continue;
+        }
         double point[3], dd;
         point[0] = orig1[0];
         point[1] = orig1[1];

@@ -150,6 +150,59 @@ def test_ray_torus_matches_implicit_quartic(orc):
     assert checked > 200
 
 
+def test_root_refinement_lowers_torus_residual(orc):
+    # the extended-precision refinement (the mixed-precision yardstick) must
+    # never raise the implicit-torus residual of a hit, must lower it on some,
+    # and may move a root by at most 1e-3 A
+    rng = np.random.default_rng(11)
+    ld = np.longdouble
+    lowered = compared = 0
+    for _ in range(600):
+        rot = _random_rotation(rng)
+        invrot = np.ascontiguousarray(rot.T)
+        center = rng.normal(size=3)
+        R = rng.uniform(0.5, 3.0)
+        r = 1.4
+        panel = int(rng.integers(3))
+        vc = (0, 2, 1)[panel]
+        o = center + rng.normal(size=3) * 2
+        o[vc] = center[vc] - 10
+        d = np.zeros(3)
+        d[vc] = 1.0
+        sc = center.copy()
+        srad = R + r + 0.5
+        t1, t2 = np.zeros(1), np.zeros(1)
+        if not orc.orc_ray_sphere(_ptr(o), _ptr(d), _ptr(sc), srad, _ptr(t1), _ptr(t2)):
+            continue
+        hits = []
+        for refine in (0, 1):
+            t = np.zeros(4)
+            t[0] = t1[0]
+            n = ctypes.c_int()
+            orc.orc_set_root_refinement(refine)
+            orc.orc_ray_torus(_ptr(invrot.ravel()), _ptr(center), _ptr(sc), r, R, srad, panel, _ptr(o), _ptr(d),
+                              _ptr(t), ctypes.byref(n))
+            hits.append(np.sort(t[:n.value]))
+        orc.orc_set_root_refinement(0)
+        if len(hits[0]) != len(hits[1]) or len(hits[0]) == 0:
+            assert len(hits[0]) == len(hits[1])
+            continue
+        assert np.abs(hits[0] - hits[1]).max() <= 1e-3
+
+        def residual(tv):
+            p = np.array([ld(x) for x in o]) + np.array([ld(x) for x in d]) * ld(tv)
+            q = np.array([[ld(v) for v in row] for row in invrot]) @ (p - np.array([ld(x) for x in center]))
+            s2 = q @ q + ld(R) * ld(R) - ld(r) * ld(r)
+            return abs(s2 * s2 - 4 * ld(R) * ld(R) * (q[0] * q[0] + q[1] * q[1]))
+
+        for a, b in zip(hits[0], hits[1]):
+            fa, fb = residual(a), residual(b)
+            assert fb <= fa * (1 + 1e-9) + ld(1e-12)
+            lowered += fb < fa
+            compared += 1
+    assert compared > 400 and lowered > compared // 10
+
+
 # --------------------------------------------------------------- generator / grid
 def _park_miller_py(seed, n):
     s = seed
