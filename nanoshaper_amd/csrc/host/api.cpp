@@ -63,7 +63,7 @@ struct nsgpu_ctx {
     // maps
     DBuf eps, ideb, status, bits;
     // cavities
-    DBuf parent, nroots, rootlist, bkey, ccount, clabel, cfilled;
+    DBuf crx0, crx1, cpar, crowcnt, croff, nroots, rootlist, bkey, ccount, clabel, cfilled;
     std::vector<double> cav_size;
     std::vector<uint8_t> cav_filled;
     // vertices and mesh
@@ -162,7 +162,7 @@ void nsgpu_destroy(nsgpu_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     DBuf *all[] = {&c->gx, &c->gy, &c->gz, &c->pkind, &c->poff, &c->pcnt, &c->pflags, &c->pbsph, &c->pdata, &c->porder,
                    &c->rcnt, &c->rt, &c->rvol, &c->partial, &c->stats, &c->cand, &c->seg, &c->segst, &c->eps, &c->ideb, &c->status,
-                   &c->bits, &c->parent, &c->nroots, &c->rootlist, &c->bkey, &c->ccount, &c->clabel, &c->cfilled,
+                   &c->bits, &c->crx0, &c->crx1, &c->cpar, &c->crowcnt, &c->croff, &c->nroots, &c->rootlist, &c->bkey, &c->ccount, &c->clabel, &c->cfilled,
                    &c->idx, &c->vcount, &c->voffs, &c->scan_scratch, &c->verts, &c->tris, &c->tarea,
                    &c->segmask, &c->anode, &c->acase, &c->anv, &c->ant, &c->avoff, &c->atoff};
     for (DBuf *b : all) b->release();
@@ -373,12 +373,22 @@ static int trace(nsgpu_ctx *c) {
 
 static int cavities(nsgpu_ctx *c) {
     const DevGrid g = dev_grid(c);
-    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
-    HIP_TRY(c, c->parent.ensure(nvox * 4));
+    const int64_t rows = (int64_t)g.ny * g.nz;
+    const int64_t slots = rows * ccl_row_capacity(g);
+    if (slots > INT32_MAX) return fail(c, NSGPU_E_ARG, "grid too large for 32-bit run ids");
+    HIP_TRY(c, c->crx0.ensure(slots * 4));
+    HIP_TRY(c, c->crx1.ensure(slots * 4));
+    HIP_TRY(c, c->cpar.ensure(slots * 4));
+    HIP_TRY(c, c->crowcnt.ensure(rows * 4));
+    HIP_TRY(c, c->croff.ensure(rows * 4));
+    HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(rows) * 4));
     HIP_TRY(c, c->nroots.ensure(4));
-    HIP_TRY(c, c->rootlist.ensure(nvox * 4));
+    HIP_TRY(c, c->rootlist.ensure(slots * 4));
     HIP_TRY(c, hipMemsetAsync(c->nroots.p, 0, 4, c->stream));
-    launch_ccl(c->stream, g, c->status.as<int>(), c->parent.as<int>(), c->nroots.as<int>(), c->rootlist.as<int>());
+    const CclRuns runs{c->crx0.as<int>(), c->crx1.as<int>(), c->cpar.as<int>(), c->crowcnt.as<int>(),
+                       c->croff.as<int>()};
+    launch_ccl(c->stream, g, c->status.as<int>(), runs, c->scan_scratch.as<int>(), c->nroots.as<int>(),
+               c->rootlist.as<int>());
     HIP_TRY(c, hipGetLastError());
     int nroots = 0;
     HIP_TRY(c, hipMemcpyAsync(&nroots, c->nroots.p, 4, hipMemcpyDeviceToHost, c->stream));
@@ -391,7 +401,7 @@ static int cavities(nsgpu_ctx *c) {
     HIP_TRY(c, c->ccount.ensure((size_t)nroots * 4));
     HIP_TRY(c, hipMemsetAsync(c->bkey.p, 0xff, (size_t)nroots * 4, c->stream));
     HIP_TRY(c, hipMemsetAsync(c->ccount.p, 0, (size_t)nroots * 4, c->stream));
-    launch_ccl_keys(c->stream, g, c->parent.as<int>(), c->bkey.as<unsigned int>(), c->ccount.as<int>());
+    launch_ccl_keys(c->stream, g, runs, c->bkey.as<unsigned int>(), c->ccount.as<int>());
     HIP_TRY(c, hipGetLastError());
     std::vector<int> roots(nroots), count(nroots);
     std::vector<unsigned int> bkey(nroots);
@@ -440,7 +450,7 @@ static int cavities(nsgpu_ctx *c) {
     HIP_TRY(c, c->cfilled.ensure((size_t)nroots));
     HIP_TRY(c, hipMemcpyAsync(c->clabel.p, label.data(), (size_t)nroots * 4, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->cfilled.p, filled.data(), (size_t)nroots, hipMemcpyHostToDevice, c->stream));
-    launch_ccl_label(c->stream, g, c->parent.as<int>(), c->clabel.as<int>(), c->cfilled.as<uint8_t>(),
+    launch_ccl_label(c->stream, g, runs, c->clabel.as<int>(), c->cfilled.as<uint8_t>(),
                      c->opts.build_epsmap, c->opts.accurate_triangulation, c->opts.inside, c->status.as<int>(),
                      c->eps.as<int>(), c->ideb.as<uint8_t>(), c->bits.as<uint32_t>());
     HIP_TRY(c, hipGetLastError());

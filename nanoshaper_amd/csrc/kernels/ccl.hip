@@ -1,7 +1,20 @@
 // Cavity detection as connected-component labelling of the TEMP_OUT nodes
-// (6-connectivity) with a lock-free union-find over the x-runs of TEMP_OUT
-// nodes: links always go from the larger root to the smaller, so each
-// component's root is its smallest flat index, i.e. the first node the forward scan of
+// (6-connectivity).  The union-find nodes are the x-runs of TEMP_OUT nodes,
+// not the nodes: a 325^3 grid has ~34 M nodes but only a few hundred thousand
+// runs, so only the row scan and the final label pass touch the status map.
+//
+// Run storage: row r = k*ny + j owns the slots [r*cap, r*cap + rowcnt[r]) of
+// rx0/rx1 (first and last x of each run, in x order), cap = ceil(nx/2) (the
+// most runs a row can hold).  The union-find runs on compact run ids
+// c = roff[r] + q (roff: exclusive scan of rowcnt), which grow with the flat
+// index of the run's first node.  Unions are made in two levels: inside each
+// z-plane in LDS (one workgroup per plane, +y links), then across planes (+z
+// links) on the plane roots in global memory, where the outside region is one
+// component per plane and the cross-plane links are few distinct pairs.
+//
+// Links always go from the larger root to the smaller, so each component's
+// root is its run of smallest id, whose first node is the component's
+// smallest flat index: the first node the forward scan of
 // getCavitiesWithBilevelStatusMap (Surface.cpp:1994-2126) meets.  The first
 // node of the backward-z scan is recorded per component as a second key; the
 // host replays the alternating scan order over the components to give each
@@ -41,174 +54,307 @@ __device__ void unite(int *P, int a, int b) {
     }
 }
 
-// Runs of TEMP_OUT nodes along x are the union-find nodes: one wave walks a
-// row in 64-node chunks, finds run starts with a ballot and points every node
-// at the first node of its run (the run's smallest flat index).
-__global__ void __launch_bounds__(256) k_ccl_rows(DevGrid g, const int *status, int *P) {
+// One wave per row: the TEMP_OUT bits of up to eight 64-node chunks are
+// loaded together as ballots, then run starts and ends are wave-uniform bit
+// operations.  The k-th start and the k-th end of a row belong to the same run.
+__global__ void __launch_bounds__(256) k_ccl_runs(DevGrid g, const int *status, int cap, int *rx0, int *rx1,
+                                                  int *rowcnt) {
     const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= (int64_t)g.ny * g.nz) return;
     const int lane = threadIdx.x & 63;
     const int64_t base = row * g.nx;
-    int carry = 0;  // run start (x index) of a run entering the chunk from the left
-    for (int c0 = 0; c0 < g.nx; c0 += 64) {
-        const int i = c0 + lane;
-        const bool in = i < g.nx;
-        const bool t = in && status[base + i] == NSGPU_STATUS_TEMP_OUT;
-        const unsigned long long stop = __ballot(in && !t);
-        const unsigned long long below = stop & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
-        const int start = below ? c0 + (63 - __clzll(below)) + 1 : carry;
-        if (in) P[base + i] = t ? (int)(base + start) : -1;
-        // the run reaching the end of this chunk continues into the next one
-        const bool last_t = __shfl(t ? 1 : 0, 63, 64) != 0;
-        const int last_start = __shfl(start, 63, 64);
-        carry = last_t ? last_start : c0 + 64;
+    const int64_t slot0 = row * cap;
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int ns = 0, ne = 0;
+    unsigned long long prev = 0;  // last node of the previous chunk, in bit 0
+    for (int g0 = 0; g0 < g.nx; g0 += 64 * 8) {
+        unsigned long long m[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            const int i = g0 + 64 * c + lane;
+            m[c] = __ballot(i < g.nx && (c < 8 || lane == 0) && status[base + i] == NSGPU_STATUS_TEMP_OUT);
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const unsigned long long t = m[c];
+            const unsigned long long starts = t & ~((t << 1) | prev);
+            const unsigned long long ends = t & ~((t >> 1) | (m[c + 1] << 63));
+            prev = t >> 63;
+            const int i = g0 + 64 * c + lane;
+            if ((starts >> lane) & 1) rx0[slot0 + ns + __popcll(starts & below)] = i;
+            if ((ends >> lane) & 1) rx1[slot0 + ne + __popcll(ends & below)] = i;
+            ns += __popcll(starts);
+            ne += __popcll(ends);
+        }
+    }
+    if (lane == 0) rowcnt[row] = ns;
+}
+
+// root in LDS with path halving (the LDS twin of find_root)
+__device__ int find_root_lds(int *P, int x) {
+    int p = __hip_atomic_load(P + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    while (p != x) {
+        const int gp = __hip_atomic_load(P + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (gp != p) __hip_atomic_store(P + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        x = p;
+        p = gp;
+    }
+    return x;
+}
+
+__device__ void unite_lds(int *P, int a, int b) {
+    for (;;) {
+        a = find_root_lds(P, a);
+        b = find_root_lds(P, b);
+        if (a == b) return;
+        if (a < b) {
+            const int t = a;
+            a = b;
+            b = t;
+        }
+        const int old = atomicCAS(P + a, a, b);
+        if (old == a) return;
+        a = old;
     }
 }
 
-// Links between runs of neighbouring rows (+y) and planes (+z), made once per
-// overlap interval: at a node whose left neighbour does not also overlap.
-__global__ void __launch_bounds__(256) k_ccl_link(DevGrid g, const int *status, int *P) {
-    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
-    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nvox || status[f] != NSGPU_STATUS_TEMP_OUT) return;
-    const int i = (int)(f % g.nx);
-    const int j = (int)((f / g.nx) % g.ny);
-    const int k = (int)(f / ((int64_t)g.nx * g.ny));
-    const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
-    const bool left = i > 0 && status[f - 1] == NSGPU_STATUS_TEMP_OUT;
-    if (j + 1 < g.ny && status[f + sy] == NSGPU_STATUS_TEMP_OUT &&
-        !(left && status[f - 1 + sy] == NSGPU_STATUS_TEMP_OUT))
-        unite(P, ld(P + f), ld(P + f + sy));
-    if (k + 1 < g.nz && status[f + sz] == NSGPU_STATUS_TEMP_OUT &&
-        !(left && status[f - 1 + sz] == NSGPU_STATUS_TEMP_OUT))
-        unite(P, ld(P + f), ld(P + f + sz));
-}
-
-// Every node is pointed at its root.  The walk only reads: a compressing walk
-// could overwrite a node another lane has just flattened with a stale
-// ancestor.  Each lane writes its own entry only, so after the kernel every
-// entry is the root.
-__global__ void __launch_bounds__(256) k_ccl_flatten(int *P, int64_t nvox) {
-    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nvox) return;
-    int x = ld(P + f);
-    if (x < 0) return;
-    for (int p = ld(P + x); p != x; p = ld(P + x)) x = p;
-    st(P + f, x);
-}
-
-// roots get a compact slot, stored in place as -(slot+2)
-__global__ void __launch_bounds__(256) k_ccl_roots(int *P, int64_t nvox, int *nroots, int *rootlist) {
-    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (f < nvox && P[f] == (int)f) {
-        const int s = atomicAdd(nroots, 1);
-        rootlist[s] = (int)f;
-        P[f] = -(s + 2);
+// merge walk of two sorted run lists; f(a, b) for every overlapping pair
+template <class F>
+__device__ __forceinline__ void overlaps(const int *rx0, const int *rx1, int64_t sa, int na, int64_t sb, int nb, F f) {
+    int a = 0, b = 0;
+    while (a < na && b < nb) {
+        const int a0 = rx0[sa + a], a1 = rx1[sa + a], b0 = rx0[sb + b], b1 = rx1[sb + b];
+        if (a1 < b0) {
+            ++a;
+        } else if (b1 < a0) {
+            ++b;
+        } else {
+            f(a, b);
+            if (a1 < b1)
+                ++a;
+            else
+                ++b;
+        }
     }
 }
 
-__device__ __forceinline__ int slot_of(const int *P, int64_t f) {
-    const int p = P[f];
-    if (p == -1) return -1;
-    return p < -1 ? -(p + 2) : -(P[p] + 2);
+constexpr int kPlaneRuns = 16384;  // runs of one z-plane united in LDS
+
+// One workgroup per z-plane: +y unions of the plane's runs, then every run
+// of the plane points at its plane-level root.  A plane with more runs than
+// the LDS holds unites in global memory instead.
+__global__ void __launch_bounds__(1024) k_ccl_plane(DevGrid g, int cap, const int *rx0, const int *rx1,
+                                                    const int *rowcnt, const int *roff, int *R) {
+    __shared__ int lp[kPlaneRuns];
+    const int64_t row0 = (int64_t)blockIdx.x * g.ny, rowl = row0 + g.ny - 1;
+    const int base = roff[row0];
+    const int T = roff[rowl] + rowcnt[rowl] - base;
+    if (T == 0) return;
+    const bool in_lds = T <= kPlaneRuns;
+    for (int l = threadIdx.x; l < T; l += blockDim.x) {
+        if (in_lds)
+            lp[l] = l;
+        else
+            st(R + base + l, base + l);
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j + 1 < g.ny; j += blockDim.x) {
+        const int64_t ra = row0 + j, rb = ra + 1;
+        const int la = roff[ra] - base, lb = roff[rb] - base;
+        overlaps(rx0, rx1, ra * cap, rowcnt[ra], rb * cap, rowcnt[rb], [&](int a, int b) {
+            if (in_lds)
+                unite_lds(lp, la + a, lb + b);
+            else
+                unite(R, base + la + a, base + lb + b);
+        });
+    }
+    __syncthreads();
+    // read-only walks; each thread then writes its own entries, always a root
+    for (int l = threadIdx.x; l < T; l += blockDim.x) {
+        if (in_lds) {
+            int x = lp[l];
+            for (int p = lp[x]; p != x; p = lp[x]) x = p;
+            R[base + l] = base + x;
+        } else {
+            int x = ld(R + base + l);
+            for (int p = ld(R + x); p != x; p = ld(R + x)) x = p;
+            st(R + base + l, x);
+        }
+    }
 }
 
-// Size and backward-scan key of every component.  The component of node 0
-// (when node 0 is TEMP_OUT) is the first one any scan meets, so it is always
-// labelled first and neither key is needed for it: it is skipped, which
-// removes the only heavily contended address (the outside region).  Waves
-// whose lanes all belong to one component combine before the atomics.
-__global__ void __launch_bounds__(256) k_ccl_keys(DevGrid g, const int *P, unsigned int *bkey, int *count) {
-    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
-    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int skip = slot_of(P, 0);
-    int s = -1;
-    unsigned int rank = 0xffffffffu;
-    if (f < nvox) {
-        s = slot_of(P, f);
-        if (s == skip) s = -1;
-        const int64_t plane = (int64_t)g.nx * g.ny;
-        const int64_t k = f / plane;
-        rank = (unsigned int)((g.nz - 1 - k) * plane + (f - k * plane));
-    }
-    const bool valid = s >= 0;
-    const unsigned long long act = __ballot(valid);
+// +z links, one thread per row.  After the plane pass every run's entry is
+// its plane root, so the overlaps of a wave's rows mostly repeat one root
+// pair (the outside region of plane k and of plane k+1): the wave's first
+// lane with an overlap unites its first pair for the whole wave, and every
+// lane skips that pair and repeats of its own previous pair.
+__global__ void __launch_bounds__(256) k_ccl_zlink(DevGrid g, int cap, const int *rx0, const int *rx1,
+                                                   const int *rowcnt, const int *roff, int *R) {
+    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = row < (int64_t)g.ny * (g.nz - 1);
+    const int64_t other = row + g.ny;
+    const int ca = valid ? roff[row] : 0, cb = valid ? roff[other] : 0;
+    const int na = valid ? rowcnt[row] : 0, nb = valid ? rowcnt[other] : 0;
+    // first overlapping pair of this row
+    int fa = -1, fb = -1;
+    overlaps(rx0, rx1, row * cap, na, other * cap, nb, [&](int a, int b) {
+        if (fa < 0) {
+            fa = ld(R + ca + a);
+            fb = ld(R + cb + b);
+        }
+    });
+    const unsigned long long act = __ballot(fa >= 0);
     if (act == 0) return;
     const int leader = __ffsll((long long)act) - 1;
-    const int s0 = __shfl(s, leader, 64);
-    if (__ballot(valid && s == s0) == act) {
-        unsigned int r = valid ? rank : 0xffffffffu;
-        for (int off = 32; off > 0; off >>= 1) r = min(r, (unsigned int)__shfl_down((int)r, off, 64));
-        r = (unsigned int)__shfl((int)r, 0, 64);
-        if ((int)(threadIdx.x & 63) == leader) {
-            atomicMin(bkey + s0, r);
-            atomicAdd(count + s0, __popcll(act));
-        }
-        return;
-    }
-    if (valid) {
-        atomicMin(bkey + s, rank);
-        atomicAdd(count + s, 1);
+    const int wa = __shfl(fa, leader, 64), wb = __shfl(fb, leader, 64);
+    if ((int)(threadIdx.x & 63) == leader) unite(R, wa, wb);
+    if (fa < 0) return;
+    int last_a = wa, last_b = wb;
+    overlaps(rx0, rx1, row * cap, na, other * cap, nb, [&](int a, int b) {
+        const int pa = ld(R + ca + a), pb = ld(R + cb + b);
+        if ((pa == last_a && pb == last_b) || (pa == wa && pb == wb)) return;
+        last_a = pa;
+        last_b = pb;
+        unite(R, pa, pb);
+    });
+}
+
+// Every run is pointed at its root: read-only walks, each thread writes its
+// own row's entries, always with a root.
+__global__ void __launch_bounds__(256) k_ccl_flatten(int64_t rows, const int *rowcnt, const int *roff, int *R) {
+    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= rows) return;
+    const int n = rowcnt[row], c0 = roff[row];
+    for (int q = 0; q < n; ++q) {
+        int x = ld(R + c0 + q);
+        for (int p = ld(R + x); p != x; p = ld(R + x)) x = p;
+        st(R + c0 + q, x);
     }
 }
 
-// final labels and fillCavities (Surface.cpp:2355-2487) for the filled ones
-__global__ void __launch_bounds__(256) k_ccl_label(DevGrid g, const int *P, const int *label, const uint8_t *filled,
-                                                   int build_eps, int accurate, int inside, int *status, int *epsmap,
-                                                   uint8_t *idebmap, uint32_t *bits) {
-    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
-    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nvox) return;
-    const int s = slot_of(P, f);
-    if (s < 0) return;
-    status[f] = label[s];
-    if (!filled[s]) return;
-    const int i = (int)(f % g.nx);
-    const int j = (int)((f / g.nx) % g.ny);
-    const int k = (int)(f / ((int64_t)g.nx * g.ny));
-    const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
-    if (build_eps) {
-        epsmap[3 * f + 0] = inside;
-        epsmap[3 * f + 1] = inside;
-        epsmap[3 * f + 2] = inside;
-        if (i > 0) epsmap[3 * (f - 1) + 0] = inside;
-        if (j > 0) epsmap[3 * (f - sy) + 1] = inside;
-        if (k > 0) epsmap[3 * (f - sz) + 2] = inside;
-        idebmap[f] = 0;
+// roots get a compact slot, stored in place as -(slot+2); rootlist holds the
+// root's first node (the component's smallest flat index)
+__global__ void __launch_bounds__(256) k_ccl_roots(DevGrid g, int cap, const int *rx0, const int *rowcnt,
+                                                   const int *roff, int *R, int *nroots, int *rootlist) {
+    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= (int64_t)g.ny * g.nz) return;
+    const int n = rowcnt[row], c0 = roff[row];
+    for (int q = 0; q < n; ++q) {
+        if (R[c0 + q] != c0 + q) continue;
+        const int s = atomicAdd(nroots, 1);
+        rootlist[s] = (int)(row * g.nx + rx0[row * cap + q]);
+        R[c0 + q] = -(s + 2);
     }
-    if (accurate) {
-        for (int dk = 0; dk < 2; ++dk)
-            for (int dj = 0; dj < 2; ++dj)
-                for (int di = 0; di < 2; ++di) {
-                    if (i + di >= g.nx || j + dj >= g.ny || k + dk >= g.nz) continue;
-                    const int64_t q = f + di + dj * sy + dk * sz;
-                    atomicAnd(bits + (q >> 5), ~(1u << (uint32_t)(q & 31)));
-                }
+}
+
+__device__ __forceinline__ int slot_of(const int *R, int c) {
+    const int p = R[c];
+    return p < -1 ? -(p + 2) : -(R[p] + 2);
+}
+
+// Size and backward-scan key of every component, per run: the run's first
+// node has its smallest backward rank (k reversed, then j, then i).  The
+// component of node 0 (when node 0 is TEMP_OUT) is the first one any scan
+// meets, so it is always labelled first and neither key is needed for it: it
+// is skipped, which removes the only heavily contended address.
+__global__ void __launch_bounds__(256) k_ccl_keys(DevGrid g, int cap, const int *rx0, const int *rx1,
+                                                  const int *rowcnt, const int *roff, const int *R, unsigned int *bkey,
+                                                  int *count) {
+    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= (int64_t)g.ny * g.nz) return;
+    const int skip = (rowcnt[0] > 0 && rx0[0] == 0) ? slot_of(R, 0) : -1;
+    const int n = rowcnt[row], c0 = roff[row];
+    const int j = (int)(row % g.ny), k = (int)(row / g.ny);
+    const int64_t plane = (int64_t)g.nx * g.ny;
+    for (int q = 0; q < n; ++q) {
+        const int s = slot_of(R, c0 + q);
+        if (s == skip) continue;
+        const int64_t id = row * cap + q;
+        const unsigned int rank = (unsigned int)((g.nz - 1 - k) * plane + (int64_t)j * g.nx + rx0[id]);
+        atomicMin(bkey + s, rank);
+        atomicAdd(count + s, rx1[id] - rx0[id] + 1);
+    }
+}
+
+// Final labels and fillCavities (Surface.cpp:2355-2487) for the filled
+// components: one wave per row, lanes stride over each run's nodes.
+__global__ void __launch_bounds__(256) k_ccl_label(DevGrid g, int cap, const int *rx0, const int *rx1,
+                                                   const int *rowcnt, const int *roff, const int *R, const int *label,
+                                                   const uint8_t *filled, int build_eps, int accurate, int inside,
+                                                   int *status, int *epsmap, uint8_t *idebmap, uint32_t *bits) {
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= (int64_t)g.ny * g.nz) return;
+    const int lane = threadIdx.x & 63;
+    const int n = rowcnt[row], c0 = roff[row];
+    const int j = (int)(row % g.ny), k = (int)(row / g.ny);
+    const int64_t base = row * g.nx;
+    const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
+    for (int q = 0; q < n; ++q) {
+        const int s = slot_of(R, c0 + q);
+        const int lab = label[s];
+        const bool fill = filled[s] != 0;
+        const int64_t id = row * cap + q;
+        const int x1 = rx1[id];
+        for (int i = rx0[id] + lane; i <= x1; i += 64) {
+            const int64_t f = base + i;
+            status[f] = lab;
+            if (!fill) continue;
+            if (build_eps) {
+                epsmap[3 * f + 0] = inside;
+                epsmap[3 * f + 1] = inside;
+                epsmap[3 * f + 2] = inside;
+                if (i > 0) epsmap[3 * (f - 1) + 0] = inside;
+                if (j > 0) epsmap[3 * (f - sy) + 1] = inside;
+                if (k > 0) epsmap[3 * (f - sz) + 2] = inside;
+                idebmap[f] = 0;
+            }
+            if (accurate) {
+                for (int dk = 0; dk < 2; ++dk)
+                    for (int dj = 0; dj < 2; ++dj)
+                        for (int di = 0; di < 2; ++di) {
+                            if (i + di >= g.nx || j + dj >= g.ny || k + dk >= g.nz) continue;
+                            const int64_t qn = f + di + dj * sy + dk * sz;
+                            atomicAnd(bits + (qn >> 5), ~(1u << (uint32_t)(qn & 31)));
+                        }
+            }
+        }
     }
 }
 
 static inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 
-void launch_ccl(hipStream_t s, const DevGrid &g, const int *status, int *P, int *nroots, int *rootlist) {
-    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
+int ccl_row_capacity(const DevGrid &g) { return (g.nx + 1) / 2; }
+
+void launch_ccl(hipStream_t s, const DevGrid &g, const int *status, const CclRuns &c, int *scan_scratch,
+                int *nroots, int *rootlist) {
     const int64_t rows = (int64_t)g.ny * g.nz;
-    hipLaunchKernelGGL(k_ccl_rows, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, s, g, status, P);
-    hipLaunchKernelGGL(k_ccl_link, dim3(nblk(nvox)), dim3(256), 0, s, g, status, P);
-    hipLaunchKernelGGL(k_ccl_flatten, dim3(nblk(nvox)), dim3(256), 0, s, P, nvox);
-    hipLaunchKernelGGL(k_ccl_roots, dim3(nblk(nvox)), dim3(256), 0, s, P, nvox, nroots, rootlist);
+    const int cap = ccl_row_capacity(g);
+    hipLaunchKernelGGL(k_ccl_runs, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, s, g, status, cap, c.rx0, c.rx1,
+                       c.rowcnt);
+    launch_scan(s, c.rowcnt, rows, c.roff, scan_scratch);
+    hipLaunchKernelGGL(k_ccl_plane, dim3((unsigned)g.nz), dim3(1024), 0, s, g, cap, c.rx0, c.rx1, c.rowcnt, c.roff,
+                       c.R);
+    if (g.nz > 1)
+        hipLaunchKernelGGL(k_ccl_zlink, dim3(nblk((int64_t)g.ny * (g.nz - 1))), dim3(256), 0, s, g, cap, c.rx0, c.rx1,
+                           c.rowcnt, c.roff, c.R);
+    hipLaunchKernelGGL(k_ccl_flatten, dim3(nblk(rows)), dim3(256), 0, s, rows, c.rowcnt, c.roff, c.R);
+    hipLaunchKernelGGL(k_ccl_roots, dim3(nblk(rows)), dim3(256), 0, s, g, cap, c.rx0, c.rowcnt, c.roff, c.R, nroots,
+                       rootlist);
 }
 
-void launch_ccl_keys(hipStream_t s, const DevGrid &g, const int *P, unsigned int *bkey, int *count) {
-    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
-    hipLaunchKernelGGL(k_ccl_keys, dim3(nblk(nvox)), dim3(256), 0, s, g, P, bkey, count);
+void launch_ccl_keys(hipStream_t s, const DevGrid &g, const CclRuns &c, unsigned int *bkey, int *count) {
+    const int64_t rows = (int64_t)g.ny * g.nz;
+    hipLaunchKernelGGL(k_ccl_keys, dim3(nblk(rows)), dim3(256), 0, s, g, ccl_row_capacity(g), c.rx0, c.rx1, c.rowcnt,
+                       c.roff, c.R, bkey, count);
 }
 
-void launch_ccl_label(hipStream_t s, const DevGrid &g, const int *P, const int *label, const uint8_t *filled,
+void launch_ccl_label(hipStream_t s, const DevGrid &g, const CclRuns &c, const int *label, const uint8_t *filled,
                       int build_eps, int accurate, int inside, int *status, int *eps, uint8_t *ideb,
                       uint32_t *bits) {
-    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
-    hipLaunchKernelGGL(k_ccl_label, dim3(nblk(nvox)), dim3(256), 0, s, g, P, label, filled, build_eps, accurate,
-                       inside, status, eps, ideb, bits);
+    const int64_t rows = (int64_t)g.ny * g.nz;
+    hipLaunchKernelGGL(k_ccl_label, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, s, g, ccl_row_capacity(g), c.rx0,
+                       c.rx1, c.rowcnt, c.roff, c.R, label, filled, build_eps, accurate, inside, status, eps, ideb,
+                       bits);
 }
 
 }  // namespace nsg

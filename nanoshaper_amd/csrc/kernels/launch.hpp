@@ -26,10 +26,18 @@ void launch_fill_tiled(hipStream_t, const DevGrid &, const DevRays &, const Fill
 void launch_corners_tiled(hipStream_t, const DevGrid &, const DevRays &, const FillOpts &, uint32_t *);
 int64_t sum_partials_count(int64_t n);
 void launch_sum_blocks(hipStream_t, const double *, int64_t, double *);
-void launch_ccl(hipStream_t, const DevGrid &, const int *, int *, int *, int *);
-void launch_ccl_keys(hipStream_t, const DevGrid &, const int *, unsigned int *, int *);
-void launch_ccl_label(hipStream_t, const DevGrid &, const int *, const int *, const uint8_t *, int, int, int, int *,
-                      int *, uint8_t *, uint32_t *);
+// x-runs of TEMP_OUT nodes: rows * ccl_row_capacity() slots each of rx0, rx1
+// and R (R is indexed by compact run id), one count and one offset per row
+struct CclRuns {
+    int *rx0, *rx1, *R, *rowcnt, *roff;
+};
+int ccl_row_capacity(const DevGrid &);
+// scan_scratch: scan_scratch_ints(rows) ints
+void launch_ccl(hipStream_t, const DevGrid &, const int *status, const CclRuns &, int *scan_scratch, int *nroots,
+                int *rootlist);
+void launch_ccl_keys(hipStream_t, const DevGrid &, const CclRuns &, unsigned int *bkey, int *count);
+void launch_ccl_label(hipStream_t, const DevGrid &, const CclRuns &, const int *label, const uint8_t *filled, int,
+                      int, int, int *, int *, uint8_t *, uint32_t *);
 int64_t scan_scratch_ints(int64_t n);
 void launch_scan(hipStream_t, const int *, int64_t, int *, int *);
 void mc_upload_tables(hipStream_t);
