@@ -196,7 +196,8 @@ int nsgpu_synchronize(nsgpu_ctx *ctx);
  * reference, whose stages print wall-clock times, e.g. Surface.cpp:1093-1096).
  * When on, get_surf/triangulate record one event pair per stage; stage_times
  * writes up to n durations in ms in the order intersect, sort, fill, corners,
- * cavities, vertices, marching cubes, and returns the number of stages. */
+ * cavities, vertices, marching cubes, adjoint, and returns the number of
+ * stages. */
 int nsgpu_set_profiling(nsgpu_ctx *ctx, int on);
 int nsgpu_stage_times(nsgpu_ctx *ctx, double *ms, int n);
 
@@ -206,6 +207,22 @@ int nsgpu_stage_times(nsgpu_ctx *ctx, double *ms, int n);
  * disc-cull passes, ray/sphere hits, torus quartics, roots tested,
  * feasible hits, feasibility items tested, feasibility calls. */
 int nsgpu_intersect_path_counts(nsgpu_ctx *ctx, const nsgpu_surf_opts *opts, int64_t *counts, int n);
+
+/* -------------------- differentiation (no counterpart in the reference) ---- */
+/* Atoms (x, y, z, r per atom, after jitter) the current patch set was built
+ * from; the patch set must carry atom ids (nsgpu_patches.atom). */
+int nsgpu_set_atoms(nsgpu_ctx *ctx, const double *xyzr, int32_t n);
+
+/* Enclosed volume and area estimate with their gradients with respect to the
+ * atoms: traces the three grid-ray panels (the volume is getSurf's with the
+ * epsmap and no edge rays, Surface.cpp:1677-1684), then differentiates every
+ * kept hit through the implicit equation of its patch's sphere or torus.
+ * area = A * sum over the hits of 1/|n|_1 (n the unit normal).  dvolume and
+ * darea (either may be NULL) receive n_atoms*4 derivatives d/d(x, y, z, r);
+ * res (may be NULL) the ray statistics of the trace.  Leaves no maps (call
+ * get_surf again for them); nsgpu_read_ray_hits reads its three panels. */
+int nsgpu_surface_gradient(nsgpu_ctx *ctx, double *volume, double *area, double *dvolume, double *darea,
+                           nsgpu_surf_result *res);
 
 /* -------------------- read back (caller-owned buffers, DelPhi layouts) ----- */
 /* epsmap int32 [((k*ny+j)*nx+i)*3+l]; idebmap uint8 [(k*ny+j)*nx+i], 1 = out;
@@ -221,7 +238,8 @@ int nsgpu_read_cavities(nsgpu_ctx *ctx, double *sizes, uint8_t *filled);
 int nsgpu_read_mesh(nsgpu_ctx *ctx, double *verts, int32_t *tris);
 /* Sorted hits of every ray of one panel slot (0..2 grid rays, 3..5 edge rays):
  * counts[N*N] and t[N*N*max_hits], ray (m,n) at index n*N+m, N = max(nx,ny,nz).
- * max_hits must be >= nsgpu_surf_result.max_hits_per_ray. */
+ * max_hits must be >= nsgpu_surf_result.max_hits_per_ray.  Reads the rays of
+ * the last get_surf or surface_gradient. */
 int nsgpu_read_ray_hits(nsgpu_ctx *ctx, int32_t slot, int32_t max_hits, int32_t *counts,
                         double *t);
 

@@ -75,6 +75,9 @@ SIGNATURES = {
     "nsgpu_read_cavities": (ctypes.c_int, [VP, VP, VP]),
     "nsgpu_read_mesh": (ctypes.c_int, [VP, VP, VP]),
     "nsgpu_read_ray_hits": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_int32, VP, VP]),
+    "nsgpu_set_atoms": (ctypes.c_int, [VP, VP, ctypes.c_int32]),
+    "nsgpu_surface_gradient": (ctypes.c_int, [VP, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                              VP, VP, ctypes.POINTER(SurfResult)]),
     "nsgpu_generate_chain": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_double, ctypes.c_uint64, VP,
                                               ctypes.POINTER(ctypes.c_double)]),
 }

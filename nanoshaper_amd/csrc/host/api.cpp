@@ -40,7 +40,7 @@ struct DBuf {
     T *as() const { return (T *)p; }
 };
 
-enum Stage { ST_INTERSECT, ST_SORT, ST_FILL, ST_CORNERS, ST_CAV, ST_VERTS, ST_MC, ST_COUNT };
+enum Stage { ST_INTERSECT, ST_SORT, ST_FILL, ST_CORNERS, ST_CAV, ST_VERTS, ST_MC, ST_ADJOINT, ST_COUNT };
 }  // namespace
 
 struct nsgpu_ctx {
@@ -55,7 +55,12 @@ struct nsgpu_ctx {
     bool have_patches = false;
     int npatch = 0, npatch_plain = 0;
     double probe = 1.4;
-    DBuf pkind, poff, pcnt, pflags, pbsph, pdata, porder;
+    DBuf pkind, poff, pcnt, pflags, pbsph, pdata, porder, patom;
+    int patch_max_atom = -1;  // largest atom id the patches name (-1: no atom ids given)
+    // atoms of the patch set (adjoint)
+    DBuf atoms, rpid, gradv, grada, asums;
+    int natoms = 0;
+    bool record_pid = false;
     // rays
     int K = 16;
     DBuf rcnt, rt, rvol, partial, stats, cand, seg, segst;
@@ -71,6 +76,7 @@ struct nsgpu_ctx {
     DBuf segmask, anode, acase, anv, ant, avoff, atoff;
     int64_t n_analytic = 0, nverts = 0, ntris = 0;
     double area = 0;
+    bool have_trace = false;  // sorted ray lists of the last get_surf / surface_gradient
     bool have_surf = false, have_vertices = false, have_mesh = false;
     nsgpu_surf_opts opts{};
     nsgpu_surf_result res{};
@@ -126,6 +132,7 @@ static DevRays dev_rays(const nsgpu_ctx *c) {
     r.rvol = c->rvol.as<double>();
     r.seg = c->seg.as<int>();
     r.seg_st = c->segst.as<int>();
+    r.pid = c->record_pid ? c->rpid.as<int>() : nullptr;
     return r;
 }
 
@@ -160,7 +167,7 @@ void nsgpu_destroy(nsgpu_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    DBuf *all[] = {&c->gx, &c->gy, &c->gz, &c->pkind, &c->poff, &c->pcnt, &c->pflags, &c->pbsph, &c->pdata, &c->porder,
+    DBuf *all[] = {&c->gx, &c->gy, &c->gz, &c->pkind, &c->poff, &c->pcnt, &c->pflags, &c->pbsph, &c->pdata, &c->porder, &c->patom, &c->atoms, &c->rpid, &c->gradv, &c->grada, &c->asums,
                    &c->rcnt, &c->rt, &c->rvol, &c->partial, &c->stats, &c->cand, &c->seg, &c->segst, &c->eps, &c->ideb, &c->status,
                    &c->bits, &c->crx0, &c->crx1, &c->cpar, &c->crowcnt, &c->croff, &c->nroots, &c->rootlist, &c->bkey, &c->ccount, &c->clabel, &c->cfilled,
                    &c->idx, &c->vcount, &c->voffs, &c->scan_scratch, &c->verts, &c->tris, &c->tarea,
@@ -214,7 +221,7 @@ int nsgpu_set_grid(nsgpu_ctx *c, const nsgpu_grid *g) {
     HIP_TRY(c, hipMemcpyAsync(c->gz.p, z.data(), z.size() * 8, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->have_grid = true;
-    c->have_surf = c->have_vertices = c->have_mesh = false;
+    c->have_trace = c->have_surf = c->have_vertices = c->have_mesh = false;
     return NSGPU_OK;
 }
 
@@ -258,12 +265,21 @@ int nsgpu_set_patches(nsgpu_ctx *c, const nsgpu_patches *p, double probe_radius)
         if (p->kind[i] >= NSGPU_PATCH_TORUS_REG) order.push_back(i);
     HIP_TRY(c, c->porder.ensure(std::max<size_t>(n, 1) * 4));
     if (n) HIP_TRY(c, hipMemcpyAsync(c->porder.p, order.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+    // atom ids (optional: only the adjoint needs them)
+    c->patch_max_atom = -1;
+    HIP_TRY(c, c->patom.ensure(std::max<size_t>(n, 1) * 12));
+    if (n && p->atom) {
+        int mx = 0;
+        for (size_t i = 0; i < 3 * n; ++i) mx = std::max(mx, p->atom[i]);
+        c->patch_max_atom = mx;
+        HIP_TRY(c, hipMemcpyAsync(c->patom.p, p->atom, n * 12, hipMemcpyHostToDevice, c->stream));
+    }
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->npatch = p->n;
     c->npatch_plain = n_plain;
     c->probe = probe_radius;
     c->have_patches = true;
-    c->have_surf = c->have_vertices = c->have_mesh = false;
+    c->have_trace = c->have_surf = c->have_vertices = c->have_mesh = false;
     return NSGPU_OK;
 }
 
@@ -289,6 +305,7 @@ int nsgpu_intersect_path_counts(nsgpu_ctx *c, const nsgpu_surf_opts *o, int64_t 
     P.data = c->pdata.as<double>();
     P.order = c->porder.as<int>();
     P.n_plain = c->npatch_plain;
+    P.atom = c->patom.as<int>();
     const int64_t rows = intersect_workgroups(P, R);
     DBuf part, out;
     HIP_TRY(c, part.ensure((size_t)std::max<int64_t>(rows, 1) * kPathCounters * 8));
@@ -330,6 +347,7 @@ static int trace(nsgpu_ctx *c) {
     P.data = c->pdata.as<double>();
     P.order = c->porder.as<int>();
     P.n_plain = c->npatch_plain;
+    P.atom = c->patom.as<int>();
     HIP_TRY(c, c->rcnt.ensure(nrays * 4));
     HIP_TRY(c, c->rvol.ensure(nrays * 8));
     HIP_TRY(c, c->seg.ensure((size_t)nrays * kSegWords * 4));
@@ -342,6 +360,7 @@ static int trace(nsgpu_ctx *c) {
     unsigned long long *spart = c->stats.as<unsigned long long>() + 4, *sout = c->stats.as<unsigned long long>();
     for (int attempt = 0; attempt < 4; ++attempt) {
         HIP_TRY(c, c->rt.ensure((size_t)nrays * c->K * 8));
+        if (c->record_pid) HIP_TRY(c, c->rpid.ensure((size_t)nrays * c->K * 4));
         const DevRays R = dev_rays(c);
         HIP_TRY(c, hipMemsetAsync(c->rcnt.p, 0, nrays * 4, c->stream));
         HIP_TRY(c, hipMemsetAsync(c->cand.p, 0, (size_t)irows * 8, c->stream));
@@ -363,7 +382,10 @@ static int trace(nsgpu_ctx *c) {
         c->res.odd_rays = (int64_t)st[1];
         c->res.max_hits_per_ray = (int)st[2];
         c->res.candidates = (int64_t)st[3];
-        if ((int)st[2] <= c->K) return NSGPU_OK;
+        if ((int)st[2] <= c->K) {
+            c->have_trace = true;
+            return NSGPU_OK;
+        }
         int k = c->K;
         while (k < (int)st[2]) k *= 2;
         c->K = k;
@@ -506,7 +528,7 @@ int nsgpu_get_surf(nsgpu_ctx *c, const nsgpu_surf_opts *o, nsgpu_surf_result *re
     HIP_TRY(c, hipSetDevice(c->device));
     c->opts = *o;
     std::memset(&c->res, 0, sizeof c->res);
-    c->have_surf = c->have_vertices = c->have_mesh = false;
+    c->have_trace = c->have_surf = c->have_vertices = c->have_mesh = false;
     // panels rule, Surface.cpp:652-659
     c->panels[0] = 3;
     c->panels[1] = 3;
@@ -736,7 +758,7 @@ int nsgpu_read_mesh(nsgpu_ctx *c, double *verts, int32_t *tris) {
 
 int nsgpu_read_ray_hits(nsgpu_ctx *c, int32_t slot, int32_t max_hits, int32_t *counts, double *t) {
     if (!c || !counts || !t || max_hits <= 0) return NSGPU_E_ARG;
-    if (!c->have_surf) return fail(c, NSGPU_E_STATE, "no surface computed");
+    if (!c->have_trace) return fail(c, NSGPU_E_STATE, "no rays traced");
     if (slot < 0 || slot >= c->nslots) return fail(c, NSGPU_E_ARG, "slot out of range");
     if (max_hits < c->res.max_hits_per_ray) return fail(c, NSGPU_E_ARG, "max_hits below the longest ray");
     HIP_TRY(c, hipSetDevice(c->device));
@@ -749,6 +771,88 @@ int nsgpu_read_ray_hits(nsgpu_ctx *c, int32_t slot, int32_t max_hits, int32_t *c
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     for (int64_t r = 0; r < NN; ++r)
         for (int h = 0; h < counts[r] && h < max_hits; ++h) t[r * max_hits + h] = buf[(size_t)r * c->K + h];
+    return NSGPU_OK;
+}
+
+int nsgpu_set_atoms(nsgpu_ctx *c, const double *xyzr, int32_t n) {
+    if (!c || n < 0 || (n > 0 && !xyzr)) return NSGPU_E_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, c->atoms.ensure(std::max<size_t>((size_t)n, 1) * 32));
+    if (n) HIP_TRY(c, hipMemcpyAsync(c->atoms.p, xyzr, (size_t)n * 32, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->natoms = n;
+    return NSGPU_OK;
+}
+
+int nsgpu_surface_gradient(nsgpu_ctx *c, double *volume, double *area, double *dvolume, double *darea,
+                           nsgpu_surf_result *res) {
+    if (!c || !volume || !area) return NSGPU_E_ARG;
+    if (!c->have_grid || !c->have_patches) return fail(c, NSGPU_E_STATE, "set_grid and set_patches first");
+    if (c->npatch > 0 && c->patch_max_atom < 0) return fail(c, NSGPU_E_STATE, "the patch set carries no atom ids");
+    if (c->patch_max_atom >= c->natoms) return fail(c, NSGPU_E_STATE, "set_atoms with the patch set's atoms first");
+    HIP_TRY(c, hipSetDevice(c->device));
+    // the three grid-ray panels only (the volume of getSurf with the epsmap)
+    c->opts = nsgpu_surf_opts{1, 0, 0, 0, 0.0, 5, NSGPU_PREC_FP64};
+    std::memset(&c->res, 0, sizeof c->res);
+    c->have_trace = c->have_surf = c->have_vertices = c->have_mesh = false;
+    c->panels[0] = 3;
+    c->panels[1] = 0;
+    c->nslots = 3;
+    c->record_pid = true;
+    const int rc = trace(c);
+    c->record_pid = false;
+    if (rc) return rc;
+    const DevGrid g = dev_grid(c);
+    DevRays R = dev_rays(c);
+    R.pid = c->rpid.as<int>();
+    DevPatches P;
+    P.n = c->npatch;
+    P.kind = c->pkind.as<int>();
+    P.off = c->poff.as<int>();
+    P.cnt = c->pcnt.as<int>();
+    P.flags = c->pflags.as<int>();
+    P.bsph = c->pbsph.as<double>();
+    P.data = c->pdata.as<double>();
+    P.order = c->porder.as<int>();
+    P.n_plain = c->npatch_plain;
+    P.atom = c->patom.as<int>();
+    const size_t ng = std::max<size_t>((size_t)c->natoms, 1) * 4;
+    const int64_t nb = adjoint_blocks(R);
+    HIP_TRY(c, c->gradv.ensure(ng * 8));
+    HIP_TRY(c, c->grada.ensure(ng * 8));
+    HIP_TRY(c, c->asums.ensure((size_t)std::max<int64_t>(nb, 1) * 16));
+    HIP_TRY(c, hipMemsetAsync(c->gradv.p, 0, ng * 8, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->grada.p, 0, ng * 8, c->stream));
+    AdjointArgs a;
+    a.atoms = c->atoms.as<double>();
+    a.probe = c->probe;
+    a.area_px = c->grid.A;
+    a.flag = 3.0;
+    a.grad_v = c->gradv.as<double>();
+    a.grad_a = c->grada.as<double>();
+    a.sums = c->asums.as<double>();
+    mark(c, ST_ADJOINT, 0);
+    launch_adjoint(c->stream, g, P, R, a);
+    HIP_TRY(c, hipGetLastError());
+    mark(c, ST_ADJOINT, 1);
+    std::vector<double> sums((size_t)nb * 2);
+    if (nb)
+        HIP_TRY(c, hipMemcpyAsync(sums.data(), c->asums.p, sums.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    if (dvolume && c->natoms)
+        HIP_TRY(c, hipMemcpyAsync(dvolume, c->gradv.p, (size_t)c->natoms * 32, hipMemcpyDeviceToHost, c->stream));
+    if (darea && c->natoms)
+        HIP_TRY(c, hipMemcpyAsync(darea, c->grada.p, (size_t)c->natoms * 32, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    double v = 0., s = 0.;
+    for (int64_t b = 0; b < nb; ++b) {
+        v += sums[2 * b];
+        s += sums[2 * b + 1];
+    }
+    *volume = c->grid.A * v / 3.0;
+    *area = c->grid.A * s;
+    c->res.volume = *volume;
+    c->res.total_rays = 3LL * R.N * R.N;
+    if (res) *res = c->res;
     return NSGPU_OK;
 }
 

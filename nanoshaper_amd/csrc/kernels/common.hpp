@@ -29,6 +29,7 @@ struct DevPatches {
     // registers than the plain one)
     const int *order;
     int n_plain;
+    const int *atom;  // 3 per patch: the atoms the patch is built on (-1 unused)
 };
 
 // Ray hit storage: ray r of slot s keeps up to K depths at hits[(s*NN+r)*K].
@@ -42,6 +43,7 @@ struct DevRays {
     double *rvol;     // [nslots*N*N] segment length per ray
     int *seg;         // [nslots*N*N*8] interval record per ray (see kSegWords)
     int *seg_st;      // [N*N*8] status/idebmap intervals of the x grid rays
+    int *pid;         // [nslots*N*N*K] patch of each depth (sorted with it), or null
 };
 
 // Interval record of a ray: word 0 = number of node intervals (-1: more than

@@ -38,6 +38,17 @@ void launch_ccl(hipStream_t, const DevGrid &, const int *status, const CclRuns &
 void launch_ccl_keys(hipStream_t, const DevGrid &, const CclRuns &, unsigned int *bkey, int *count);
 void launch_ccl_label(hipStream_t, const DevGrid &, const CclRuns &, const int *label, const uint8_t *filled, int,
                       int, int, int *, int *, uint8_t *, uint32_t *);
+// adjoint of the three-panel volume and of the area estimate
+struct AdjointArgs {
+    const double *atoms;  // x, y, z, r per atom
+    double probe;
+    double area_px;  // A = side^2
+    double flag;     // panels in the volume sum (3)
+    double *grad_v, *grad_a;  // 4 per atom, accumulated (zeroed by the caller)
+    double *sums;             // 2 per block: volume and area partials
+};
+int64_t adjoint_blocks(const DevRays &);
+void launch_adjoint(hipStream_t, const DevGrid &, const DevPatches &, const DevRays &, const AdjointArgs &);
 int64_t scan_scratch_ints(int64_t n);
 void launch_scan(hipStream_t, const int *, int64_t, int *, int *);
 void mc_upload_tables(hipStream_t);

@@ -386,10 +386,13 @@ __device__ int ray_torus_mixed(const double *td, const double sc[3], double prob
     return ni;
 }
 
-__device__ __forceinline__ void push_hit(const DevRays &R, int slot, int64_t ray, double t) {
+__device__ __forceinline__ void push_hit(const DevRays &R, int slot, int64_t ray, double t, int patch) {
     const int64_t r = (int64_t)slot * R.N * R.N + ray;
     const int pos = atomicAdd(&R.cnt[r], 1);
-    if (pos < R.K) R.t[r * R.K + pos] = t;
+    if (pos < R.K) {
+        R.t[r * R.K + pos] = t;
+        if (R.pid) R.pid[r * R.K + pos] = patch;
+    }
 }
 
 // Path counters of the counting variant (see intersect_path_counts).
@@ -480,7 +483,7 @@ __global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRa
                 pc[PC_HITS] += ok;
             }
             if (!ok) continue;
-            if (!kCount) push_hit(R, slot, n * R.N + m, t[i]);
+            if (!kCount) push_hit(R, slot, n * R.N + m, t[i], it);
         }
     }
     if (kCount) {
@@ -513,14 +516,30 @@ __global__ void __launch_bounds__(256) k_sort(DevGrid g, DevRays R, unsigned lon
         c = R.cnt[r];
         const int n = c < R.K ? c : R.K;
         double *T = R.t + r * R.K;
-        for (int a = 1; a < n; ++a) {
-            const double v = T[a];
-            int b = a - 1;
-            while (b >= 0 && T[b] > v) {
-                T[b + 1] = T[b];
-                --b;
+        if (R.pid) {
+            int *I = R.pid + r * R.K;
+            for (int a = 1; a < n; ++a) {
+                const double v = T[a];
+                const int id = I[a];
+                int b = a - 1;
+                while (b >= 0 && T[b] > v) {
+                    T[b + 1] = T[b];
+                    I[b + 1] = I[b];
+                    --b;
+                }
+                T[b + 1] = v;
+                I[b + 1] = id;
             }
-            T[b + 1] = v;
+        } else {
+            for (int a = 1; a < n; ++a) {
+                const double v = T[a];
+                int b = a - 1;
+                while (b >= 0 && T[b] > v) {
+                    T[b + 1] = T[b];
+                    --b;
+                }
+                T[b + 1] = v;
+            }
         }
         const int slot = (int)(r / nper);
         const int phase = slot < R.panels0 ? 0 : 1;

@@ -17,7 +17,7 @@ from ._lib import Grid, MeshInfo, Patches, SurfOpts, SurfResult
 
 PATCH_KINDS = ("point", "facet_regular", "facet_singular", "torus_regular", "torus_singular")
 STATUS_INSIDE, STATUS_TEMP_OUT, STATUS_OUT, STATUS_FIRST_CAV = -1, 2, 3, 4
-STAGES = ("intersect", "sort", "fill", "corners", "cavities", "vertices", "marching_cubes")
+STAGES = ("intersect", "sort", "fill", "corners", "cavities", "vertices", "marching_cubes", "adjoint")
 
 
 class NsgpuError(RuntimeError):
@@ -160,6 +160,25 @@ class DeviceSurface:
 
     def set_patches(self, ps):
         self._chk(self._lib.nsgpu_set_patches(self._h, ctypes.byref(ps.view), ps.probe), "set_patches")
+
+    def set_atoms(self, xyzr):
+        """Atoms (after jitter) the current patch set was built from (adjoint)."""
+        a = np.ascontiguousarray(as_xyzr(xyzr), dtype=np.float64)
+        self._chk(self._lib.nsgpu_set_atoms(self._h, _ptr(a), a.shape[0]), "set_atoms")
+        self._natoms = a.shape[0]
+
+    def surface_gradient(self):
+        """Three-panel volume and area estimate with their gradients:
+        {'volume', 'area', 'dvolume' (n, 4), 'darea' (n, 4)}, d/d(x, y, z, r)."""
+        n = getattr(self, "_natoms", 0)
+        v, s = ctypes.c_double(), ctypes.c_double()
+        dv = np.zeros((max(n, 1), 4))
+        da = np.zeros((max(n, 1), 4))
+        r = SurfResult()
+        self._chk(self._lib.nsgpu_surface_gradient(self._h, ctypes.byref(v), ctypes.byref(s), _ptr(dv), _ptr(da),
+                                                   ctypes.byref(r)), "surface_gradient")
+        self.result = _result_dict(r)
+        return {"volume": v.value, "area": s.value, "dvolume": dv[:n], "darea": da[:n]}
 
     def set_profiling(self, on=True):
         self._chk(self._lib.nsgpu_set_profiling(self._h, int(on)), "set_profiling")
