@@ -243,6 +243,37 @@ int nsgpu_read_mesh(nsgpu_ctx *ctx, double *verts, int32_t *tris);
 int nsgpu_read_ray_hits(nsgpu_ctx *ctx, int32_t slot, int32_t max_hits, int32_t *counts,
                         double *t);
 
+/* -------------------- batches (no counterpart in the reference) ----------- */
+/* Many frames (molecules or conformations, uneven sizes), one grid and patch
+ * set each, computed by one graph launch.  A batch owns one context per frame;
+ * nsgpu_batch_frame gives it to the nsgpu_read_* calls. */
+typedef struct nsgpu_batch nsgpu_batch;
+
+int nsgpu_batch_create(int device, int32_t n_frames, nsgpu_batch **out);
+void nsgpu_batch_destroy(nsgpu_batch *batch);
+const char *nsgpu_batch_last_error(const nsgpu_batch *batch);
+nsgpu_ctx *nsgpu_batch_frame(nsgpu_batch *batch, int32_t frame);
+/* nsgpu_set_grid + nsgpu_set_patches of one frame (drops the captured graph). */
+int nsgpu_batch_set_frame(nsgpu_batch *batch, int32_t frame, const nsgpu_grid *grid,
+                          const nsgpu_patches *patches, double probe_radius);
+/* Runs every frame once eagerly (sizing its buffers and ray lists), then
+ * captures all frames' getSurf work into one graph: frames in decreasing
+ * grid size, dealt round-robin onto `chains` concurrent chains.  chains <= 0
+ * picks the chain count by timing launches of 1, 2, 4, ... 64 chains.
+ * Maps and cavities only: opts->accurate_triangulation must be 0. */
+int nsgpu_batch_prepare(nsgpu_batch *batch, const nsgpu_surf_opts *opts, int32_t chains);
+int32_t nsgpu_batch_chains(const nsgpu_batch *batch);
+/* One launch of the graph.  latency_ms[n_frames] (may be NULL) receives each
+ * frame's completion time after the launch (device clock); *redone (may be
+ * NULL) the frames that were finished on the host path (ray lists outgrown,
+ * which also drops the graph, or more cavity components than the device
+ * numbering holds). */
+int nsgpu_batch_run(nsgpu_batch *batch, double *latency_ms, int32_t *redone);
+/* The per-frame launch loop over the same frames: nsgpu_get_surf on each in
+ * turn; latency_ms[n_frames] (may be NULL) receives each call's wall time. */
+int nsgpu_batch_run_loop(nsgpu_batch *batch, double *latency_ms);
+int nsgpu_batch_result(nsgpu_batch *batch, int32_t frame, nsgpu_surf_result *res);
+
 /* -------------------- synthetic inputs ------------------------------------ */
 /* Seeded self-avoiding chain: 1.5 A steps, >= 3.0 A separation between atoms
  * at least 3 bonds apart, centres confined to a sphere of radius R, radii

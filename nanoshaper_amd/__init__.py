@@ -3,10 +3,10 @@
 The device path lives in libnsgpu.so (C ABI: include/nsgpu.h); this package
 is its host-side mirror.
 """
-from .surface import (PATCH_KINDS, STAGES, STATUS_FIRST_CAV, STATUS_INSIDE, STATUS_OUT, STATUS_TEMP_OUT,
+from .surface import (PATCH_KINDS, STAGES, STATUS_FIRST_CAV, STATUS_INSIDE, STATUS_OUT, STATUS_TEMP_OUT, Batch,
                       DeviceSurface, NanoShaper, NsgpuError, PatchSet, as_xyzr, build_grid, generate_chain,
-                      grid_coords, jitter, ses_patches, surf_opts)
+                      grid_coords, jitter, perturbed_frames, ses_patches, surf_opts)
 
-__all__ = ["PATCH_KINDS", "STAGES", "STATUS_FIRST_CAV", "STATUS_INSIDE", "STATUS_OUT", "STATUS_TEMP_OUT",
-           "DeviceSurface", "NanoShaper", "NsgpuError", "PatchSet", "as_xyzr", "build_grid", "generate_chain",
+__all__ = ["PATCH_KINDS", "STAGES", "STATUS_FIRST_CAV", "STATUS_INSIDE", "STATUS_OUT", "STATUS_TEMP_OUT", "Batch",
+           "DeviceSurface", "perturbed_frames", "NanoShaper", "NsgpuError", "PatchSet", "as_xyzr", "build_grid", "generate_chain",
            "grid_coords", "jitter", "ses_patches", "surf_opts"]

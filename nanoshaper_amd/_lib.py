@@ -78,6 +78,17 @@ SIGNATURES = {
     "nsgpu_set_atoms": (ctypes.c_int, [VP, VP, ctypes.c_int32]),
     "nsgpu_surface_gradient": (ctypes.c_int, [VP, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                               VP, VP, ctypes.POINTER(SurfResult)]),
+    "nsgpu_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int32, ctypes.POINTER(VP)]),
+    "nsgpu_batch_destroy": (None, [VP]),
+    "nsgpu_batch_last_error": (ctypes.c_char_p, [VP]),
+    "nsgpu_batch_frame": (VP, [VP, ctypes.c_int32]),
+    "nsgpu_batch_set_frame": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.POINTER(Grid), ctypes.POINTER(Patches),
+                                             ctypes.c_double]),
+    "nsgpu_batch_prepare": (ctypes.c_int, [VP, ctypes.POINTER(SurfOpts), ctypes.c_int32]),
+    "nsgpu_batch_chains": (ctypes.c_int32, [VP]),
+    "nsgpu_batch_run": (ctypes.c_int, [VP, VP, VP]),
+    "nsgpu_batch_run_loop": (ctypes.c_int, [VP, VP]),
+    "nsgpu_batch_result": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.POINTER(SurfResult)]),
     "nsgpu_generate_chain": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_double, ctypes.c_uint64, VP,
                                               ctypes.POINTER(ctypes.c_double)]),
 }

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -17,12 +18,16 @@
 using namespace nsg;
 
 namespace {
+// set while a graph is being captured: buffers must already be large enough
+thread_local bool g_no_alloc = false;
+
 // grow-only device buffer
 struct DBuf {
     void *p = nullptr;
     size_t cap = 0;
     hipError_t ensure(size_t bytes) {
         if (bytes <= cap && p) return hipSuccess;
+        if (g_no_alloc) return hipErrorOutOfMemory;
         if (p) (void)hipFree(p);
         p = nullptr;
         cap = 0;
@@ -41,6 +46,19 @@ struct DBuf {
 };
 
 enum Stage { ST_INTERSECT, ST_SORT, ST_FILL, ST_CORNERS, ST_CAV, ST_VERTS, ST_MC, ST_ADJOINT, ST_COUNT };
+
+// device result block of one getSurf, 64-bit words: hits, odd rays, longest
+// ray, candidates, volume (a double), cavity replay (int done, int count)
+enum { RES_HITS, RES_ODD, RES_MAX, RES_CAND, RES_VOL, RES_CAV, RES_WORDS = 8 };
+struct HostResults {
+    unsigned long long u[RES_WORDS];
+    double volume() const {
+        double v;
+        std::memcpy(&v, &u[RES_VOL], 8);
+        return v;
+    }
+    const int *cav_info() const { return reinterpret_cast<const int *>(&u[RES_CAV]); }
+};
 }  // namespace
 
 struct nsgpu_ctx {
@@ -68,7 +86,12 @@ struct nsgpu_ctx {
     // maps
     DBuf eps, ideb, status, bits;
     // cavities
-    DBuf crx0, crx1, cpar, crowcnt, croff, nroots, rootlist, bkey, ccount, clabel, cfilled;
+    DBuf crx0, crx1, cpar, crowcnt, croff, nroots, rootlist, bkey, ccount, clabel, cfilled, dcavsize, dcavfilled,
+        hostdone;
+    // result block (device) and its pinned host mirror
+    DBuf dres;
+    HostResults *hres = nullptr;
+    bool capturing = false;  // enqueueing into a graph: no stage events
     std::vector<double> cav_size;
     std::vector<uint8_t> cav_filled;
     // vertices and mesh
@@ -136,10 +159,10 @@ static DevRays dev_rays(const nsgpu_ctx *c) {
     return r;
 }
 
-static void mark(nsgpu_ctx *c, int stage, int end) {
-    if (!c->profile) return;
+static void mark(nsgpu_ctx *c, int stage, int end, hipStream_t st) {
+    if (!c->profile || c->capturing) return;
     if (end == 0) c->recorded[stage] = false;
-    if (hipEventRecord(c->ev[2 * stage + end], c->stream) == hipSuccess && end == 1) c->recorded[stage] = true;
+    if (hipEventRecord(c->ev[2 * stage + end], st) == hipSuccess && end == 1) c->recorded[stage] = true;
 }
 
 extern "C" {
@@ -158,6 +181,14 @@ int nsgpu_create(int device, nsgpu_ctx **out) {
         return NSGPU_E_DEVICE;
     }
     for (auto &e : c->ev) (void)hipEventCreate(&e);
+    if (hipHostMalloc((void **)&c->hres, sizeof(HostResults), hipHostMallocDefault) != hipSuccess ||
+        c->dres.ensure(sizeof(HostResults)) != hipSuccess) {
+        if (c->hres) (void)hipHostFree(c->hres);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return NSGPU_E_DEVICE;
+    }
+    std::memset(c->hres, 0, sizeof(HostResults));
     mc_upload_tables(c->stream);
     *out = c;
     return NSGPU_OK;
@@ -169,12 +200,13 @@ void nsgpu_destroy(nsgpu_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     DBuf *all[] = {&c->gx, &c->gy, &c->gz, &c->pkind, &c->poff, &c->pcnt, &c->pflags, &c->pbsph, &c->pdata, &c->porder, &c->patom, &c->atoms, &c->rpid, &c->gradv, &c->grada, &c->asums,
                    &c->rcnt, &c->rt, &c->rvol, &c->partial, &c->stats, &c->cand, &c->seg, &c->segst, &c->eps, &c->ideb, &c->status,
-                   &c->bits, &c->crx0, &c->crx1, &c->cpar, &c->crowcnt, &c->croff, &c->nroots, &c->rootlist, &c->bkey, &c->ccount, &c->clabel, &c->cfilled,
+                   &c->bits, &c->crx0, &c->crx1, &c->cpar, &c->crowcnt, &c->croff, &c->nroots, &c->rootlist, &c->bkey, &c->ccount, &c->clabel, &c->cfilled, &c->dcavsize, &c->dcavfilled, &c->hostdone, &c->dres,
                    &c->idx, &c->vcount, &c->voffs, &c->scan_scratch, &c->verts, &c->tris, &c->tarea,
                    &c->segmask, &c->anode, &c->acase, &c->anv, &c->ant, &c->avoff, &c->atoff};
     for (DBuf *b : all) b->release();
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    if (c->hres) (void)hipHostFree(c->hres);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -333,7 +365,15 @@ int nsgpu_synchronize(nsgpu_ctx *c) {
 }
 
 // ray tracing and sort; grows the per-ray capacity and repeats when a ray overflows
-static int trace(nsgpu_ctx *c) {
+// ---------------------------------------------------------------------------
+// getSurf as a host-synchronisation-free sequence on one stream: every size
+// that the device decides (hits per ray, cavity count, volume) stays on the
+// device until the single copy of the result block at the end, so the same
+// sequence runs eagerly (get_surf) and inside a captured graph (batches).
+// Clears are fill kernels, not memset nodes.
+
+// ray lists: intersection and per-ray sort; statistics into c->dres[0..3]
+static int enqueue_trace(nsgpu_ctx *c, hipStream_t st) {
     const DevGrid g = dev_grid(c);
     const int64_t N = g.N;
     const int64_t nrays = N * N * c->nslots;
@@ -352,48 +392,76 @@ static int trace(nsgpu_ctx *c) {
     HIP_TRY(c, c->rvol.ensure(nrays * 8));
     HIP_TRY(c, c->seg.ensure((size_t)nrays * kSegWords * 4));
     HIP_TRY(c, c->segst.ensure((size_t)N * N * kSegWords * 4));
+    HIP_TRY(c, c->rt.ensure((size_t)nrays * c->K * 8));
+    if (c->record_pid) HIP_TRY(c, c->rpid.ensure((size_t)nrays * c->K * 4));
+    const DevRays R = dev_rays(c);
     // per-workgroup partials: candidates of the intersection, (hits, odd, longest) of the sort
-    const DevRays R0 = dev_rays(c);
-    const int64_t irows = std::max<int64_t>(intersect_workgroups(P, R0), 1), srows = sort_blocks(R0);
+    const int64_t irows = std::max<int64_t>(intersect_workgroups(P, R), 1), srows = sort_blocks(R);
     HIP_TRY(c, c->cand.ensure((size_t)irows * 8));
-    HIP_TRY(c, c->stats.ensure((size_t)srows * 3 * 8 + 4 * 8));
-    unsigned long long *spart = c->stats.as<unsigned long long>() + 4, *sout = c->stats.as<unsigned long long>();
+    HIP_TRY(c, c->stats.ensure((size_t)srows * 3 * 8));
+    unsigned long long *spart = c->stats.as<unsigned long long>(), *sout = c->dres.as<unsigned long long>();
+    launch_fill32(st, c->rcnt.p, 0u, nrays);
+    launch_fill32(st, c->cand.p, 0u, irows * 2);
+    mark(c, ST_INTERSECT, 0, st);
+    launch_intersect(st, g, P, R, c->probe, c->cand.as<unsigned long long>(), c->opts.precision == NSGPU_PREC_MIXED);
+    HIP_TRY(c, hipGetLastError());
+    mark(c, ST_INTERSECT, 1, st);
+    mark(c, ST_SORT, 0, st);
+    launch_sort(st, g, R, spart);
+    launch_reduce_cols(st, spart, srows, 3, 1u << 2, sout + RES_HITS);
+    launch_reduce_cols(st, c->cand.as<unsigned long long>(), irows, 1, 0u, sout + RES_CAND);
+    HIP_TRY(c, hipGetLastError());
+    mark(c, ST_SORT, 1, st);
+    return NSGPU_OK;
+}
+
+// copy the result block to the pinned host mirror
+static int enqueue_results(nsgpu_ctx *c, hipStream_t st) {
+    HIP_TRY(c, hipMemcpyAsync(c->hres, c->dres.p, sizeof(HostResults), hipMemcpyDeviceToHost, st));
+    return NSGPU_OK;
+}
+
+static void take_trace_stats(nsgpu_ctx *c) {
+    c->res.total_hits = (int64_t)c->hres->u[RES_HITS];
+    c->res.odd_rays = (int64_t)c->hres->u[RES_ODD];
+    c->res.max_hits_per_ray = (int)c->hres->u[RES_MAX];
+    c->res.candidates = (int64_t)c->hres->u[RES_CAND];
+}
+
+// grows the per-ray capacity when the last trace overflowed it; true if it did
+static bool regrow(nsgpu_ctx *c) {
+    const int longest = (int)c->hres->u[RES_MAX];
+    if (longest <= c->K) return false;
+    int k = c->K;
+    while (k < longest) k *= 2;
+    c->K = k;
+    return true;
+}
+
+// eager trace (adjoint path): enqueue, wait, grow the ray lists until they hold every hit
+static int trace(nsgpu_ctx *c) {
     for (int attempt = 0; attempt < 4; ++attempt) {
-        HIP_TRY(c, c->rt.ensure((size_t)nrays * c->K * 8));
-        if (c->record_pid) HIP_TRY(c, c->rpid.ensure((size_t)nrays * c->K * 4));
-        const DevRays R = dev_rays(c);
-        HIP_TRY(c, hipMemsetAsync(c->rcnt.p, 0, nrays * 4, c->stream));
-        HIP_TRY(c, hipMemsetAsync(c->cand.p, 0, (size_t)irows * 8, c->stream));
-        mark(c, ST_INTERSECT, 0);
-        launch_intersect(c->stream, g, P, R, c->probe, c->cand.as<unsigned long long>(),
-                         c->opts.precision == NSGPU_PREC_MIXED);
-        HIP_TRY(c, hipGetLastError());
-        mark(c, ST_INTERSECT, 1);
-        mark(c, ST_SORT, 0);
-        launch_sort(c->stream, g, R, spart);
-        launch_reduce_cols(c->stream, spart, srows, 3, 1u << 2, sout);
-        launch_reduce_cols(c->stream, c->cand.as<unsigned long long>(), irows, 1, 0u, sout + 3);
-        HIP_TRY(c, hipGetLastError());
-        mark(c, ST_SORT, 1);
-        unsigned long long st[4];
-        HIP_TRY(c, hipMemcpyAsync(st, sout, sizeof st, hipMemcpyDeviceToHost, c->stream));
+        int rc = enqueue_trace(c, c->stream);
+        if (rc) return rc;
+        rc = enqueue_results(c, c->stream);
+        if (rc) return rc;
         HIP_TRY(c, hipStreamSynchronize(c->stream));
-        c->res.total_hits = (int64_t)st[0];
-        c->res.odd_rays = (int64_t)st[1];
-        c->res.max_hits_per_ray = (int)st[2];
-        c->res.candidates = (int64_t)st[3];
-        if ((int)st[2] <= c->K) {
-            c->have_trace = true;
-            return NSGPU_OK;
-        }
-        int k = c->K;
-        while (k < (int)st[2]) k *= 2;
-        c->K = k;
+        if (regrow(c)) continue;
+        take_trace_stats(c);
+        c->have_trace = true;
+        return NSGPU_OK;
     }
     return fail(c, NSGPU_E_STATE, "ray hit capacity did not converge");
 }
 
-static int cavities(nsgpu_ctx *c) {
+static CclRuns ccl_runs(nsgpu_ctx *c) {
+    return CclRuns{c->crx0.as<int>(), c->crx1.as<int>(), c->cpar.as<int>(), c->crowcnt.as<int>(),
+                   c->croff.as<int>()};
+}
+
+// cavity detection, numbering and filling (getCavitiesWithBilevelStatusMap +
+// fillCavities) for up to kReplayMax components on the device
+static int enqueue_cavities(nsgpu_ctx *c, hipStream_t st) {
     const DevGrid g = dev_grid(c);
     const int64_t rows = (int64_t)g.ny * g.nz;
     const int64_t slots = rows * ccl_row_capacity(g);
@@ -406,24 +474,49 @@ static int cavities(nsgpu_ctx *c) {
     HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(rows) * 4));
     HIP_TRY(c, c->nroots.ensure(4));
     HIP_TRY(c, c->rootlist.ensure(slots * 4));
-    HIP_TRY(c, hipMemsetAsync(c->nroots.p, 0, 4, c->stream));
-    const CclRuns runs{c->crx0.as<int>(), c->crx1.as<int>(), c->cpar.as<int>(), c->crowcnt.as<int>(),
-                       c->croff.as<int>()};
-    launch_ccl(c->stream, g, c->status.as<int>(), runs, c->scan_scratch.as<int>(), c->nroots.as<int>(),
+    HIP_TRY(c, c->bkey.ensure((size_t)kReplayMax * 4));
+    HIP_TRY(c, c->ccount.ensure((size_t)kReplayMax * 4));
+    HIP_TRY(c, c->clabel.ensure((size_t)kReplayMax * 4));
+    HIP_TRY(c, c->cfilled.ensure((size_t)kReplayMax));
+    HIP_TRY(c, c->dcavsize.ensure((size_t)kReplayMax * 8));
+    HIP_TRY(c, c->dcavfilled.ensure((size_t)kReplayMax));
+    launch_fill32(st, c->nroots.p, 0u, 1);
+    const CclRuns runs = ccl_runs(c);
+    launch_ccl(st, g, c->status.as<int>(), runs, c->scan_scratch.as<int>(), c->nroots.as<int>(),
                c->rootlist.as<int>());
+    launch_ccl_clear_keys(st, c->nroots.as<int>(), c->bkey.as<unsigned int>(), c->ccount.as<int>(), kReplayMax);
+    launch_ccl_keys(st, g, runs, c->bkey.as<unsigned int>(), c->ccount.as<int>(), kReplayMax);
+    CavityReplay r;
+    r.nroots = c->nroots.as<int>();
+    r.rootlist = c->rootlist.as<int>();
+    r.bkey = c->bkey.as<unsigned int>();
+    r.count = c->ccount.as<int>();
+    r.cube = c->grid.side * c->grid.side * c->grid.side;
+    r.vol = c->opts.cav_vol < 0 ? 11.4 : c->opts.cav_vol;
+    r.label = c->clabel.as<int>();
+    r.filled = c->cfilled.as<uint8_t>();
+    r.cav_size = c->dcavsize.as<double>();
+    r.cav_filled = c->dcavfilled.as<uint8_t>();
+    r.info = (int *)(c->dres.as<unsigned long long>() + RES_CAV);
+    launch_ccl_replay(st, r);
+    launch_ccl_label(st, g, runs, c->clabel.as<int>(), c->cfilled.as<uint8_t>(), r.info, c->opts.build_epsmap,
+                     c->opts.accurate_triangulation, c->opts.inside, c->status.as<int>(), c->eps.as<int>(),
+                     c->ideb.as<uint8_t>(), c->bits.as<uint32_t>());
     HIP_TRY(c, hipGetLastError());
+    return NSGPU_OK;
+}
+
+// host numbering when the component count exceeds the device replay
+static int cavities_on_host(nsgpu_ctx *c) {
+    const DevGrid g = dev_grid(c);
     int nroots = 0;
     HIP_TRY(c, hipMemcpyAsync(&nroots, c->nroots.p, 4, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    c->cav_size.clear();
-    c->cav_filled.clear();
-    c->res.num_cavities = 0;
-    if (nroots == 0) return NSGPU_OK;
     HIP_TRY(c, c->bkey.ensure((size_t)nroots * 4));
     HIP_TRY(c, c->ccount.ensure((size_t)nroots * 4));
-    HIP_TRY(c, hipMemsetAsync(c->bkey.p, 0xff, (size_t)nroots * 4, c->stream));
-    HIP_TRY(c, hipMemsetAsync(c->ccount.p, 0, (size_t)nroots * 4, c->stream));
-    launch_ccl_keys(c->stream, g, runs, c->bkey.as<unsigned int>(), c->ccount.as<int>());
+    launch_ccl_clear_keys(c->stream, c->nroots.as<int>(), c->bkey.as<unsigned int>(), c->ccount.as<int>(), nroots);
+    const CclRuns runs = ccl_runs(c);
+    launch_ccl_keys(c->stream, g, runs, c->bkey.as<unsigned int>(), c->ccount.as<int>(), nroots);
     HIP_TRY(c, hipGetLastError());
     std::vector<int> roots(nroots), count(nroots);
     std::vector<unsigned int> bkey(nroots);
@@ -456,9 +549,9 @@ static int cavities(nsgpu_ctx *c) {
     }
     const double vol = c->opts.cav_vol < 0 ? 11.4 : c->opts.cav_vol;
     const double cube = c->grid.side * c->grid.side * c->grid.side;
-    const int ncav = id - NSGPU_STATUS_OUT;
-    c->cav_size.assign(std::max(ncav, 0), 0.0);
-    c->cav_filled.assign(std::max(ncav, 0), 0);
+    const int ncav = std::max(id - NSGPU_STATUS_OUT, 0);
+    c->cav_size.assign(ncav, 0.0);
+    c->cav_filled.assign(ncav, 0);
     std::vector<uint8_t> filled(nroots, 0);
     for (int s = 0; s < nroots; ++s) {
         if (label[s] < NSGPU_STATUS_FIRST_CAV) continue;
@@ -467,17 +560,160 @@ static int cavities(nsgpu_ctx *c) {
         filled[s] = v <= vol;
         c->cav_filled[label[s] - NSGPU_STATUS_FIRST_CAV] = filled[s];
     }
-    c->res.num_cavities = std::max(ncav, 0);
+    c->res.num_cavities = ncav;
+    const int one = 1;
     HIP_TRY(c, c->clabel.ensure((size_t)nroots * 4));
     HIP_TRY(c, c->cfilled.ensure((size_t)nroots));
+    HIP_TRY(c, c->hostdone.ensure(4));
     HIP_TRY(c, hipMemcpyAsync(c->clabel.p, label.data(), (size_t)nroots * 4, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->cfilled.p, filled.data(), (size_t)nroots, hipMemcpyHostToDevice, c->stream));
-    launch_ccl_label(c->stream, g, runs, c->clabel.as<int>(), c->cfilled.as<uint8_t>(),
+    HIP_TRY(c, hipMemcpyAsync(c->hostdone.p, &one, 4, hipMemcpyHostToDevice, c->stream));
+    launch_ccl_label(c->stream, g, runs, c->clabel.as<int>(), c->cfilled.as<uint8_t>(), c->hostdone.as<int>(),
                      c->opts.build_epsmap, c->opts.accurate_triangulation, c->opts.inside, c->status.as<int>(),
                      c->eps.as<int>(), c->ideb.as<uint8_t>(), c->bits.as<uint32_t>());
     HIP_TRY(c, hipGetLastError());
     // the host vectors above must outlive the copies
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return NSGPU_OK;
+}
+
+// panel rule of getSurf (Surface.cpp:652-659) and the per-call state
+static int begin_surf(nsgpu_ctx *c, const nsgpu_surf_opts *o) {
+    if (!c->have_grid || !c->have_patches) return fail(c, NSGPU_E_STATE, "set_grid and set_patches first");
+    if (o->precision != NSGPU_PREC_FP64 && o->precision != NSGPU_PREC_MIXED)
+        return fail(c, NSGPU_E_ARG, "unknown precision mode");
+    c->opts = *o;
+    std::memset(&c->res, 0, sizeof c->res);
+    c->have_trace = c->have_surf = c->have_vertices = c->have_mesh = false;
+    c->panels[0] = 3;
+    c->panels[1] = 3;
+    if (!o->build_status && !o->build_epsmap) c->panels[0] = 0;
+    else if (o->build_status && !o->build_epsmap) c->panels[0] = 1;
+    if (!o->accurate_triangulation) c->panels[1] = 0;
+    c->nslots = c->panels[0] + c->panels[1];
+    if (c->nslots == 0) return fail(c, NSGPU_E_ARG, "nothing to compute (no epsmap, status or triangulation)");
+    return NSGPU_OK;
+}
+
+static FillOpts fill_opts(const nsgpu_ctx *c) {
+    const nsgpu_surf_opts *o = &c->opts;
+    FillOpts fo;
+    fo.build_eps = o->build_epsmap;
+    fo.build_status = o->build_status;
+    fo.inside = o->inside;
+    fo.slot_x = c->panels[0] >= 1 ? 0 : -1;
+    fo.slot_z = (c->panels[0] >= 3 && o->build_epsmap) ? 1 : -1;
+    fo.slot_y = (c->panels[0] >= 3 && o->build_epsmap) ? 2 : -1;
+    fo.eslot_x = c->panels[0] + 0;
+    fo.eslot_z = c->panels[0] + 1;
+    fo.eslot_y = c->panels[0] + 2;
+    return fo;
+}
+
+// slots in the volume sum and their count (Surface.cpp:1677-1684), rays traced
+static VolumeSlots volume_slots(const nsgpu_ctx *c, int64_t *traced) {
+    const nsgpu_surf_opts *o = &c->opts;
+    const int64_t N = std::max(c->grid.nx, std::max(c->grid.ny, c->grid.nz));
+    int fill_panels = 3;
+    if (!o->build_status && !o->build_epsmap) fill_panels = 0;
+    else if (o->build_status && !o->build_epsmap) fill_panels = 1;
+    if (o->accurate_triangulation) fill_panels = 3;
+    VolumeSlots vs{};
+    vs.area_px = c->grid.A;
+    *traced = 0;
+    for (int p = 0; p < fill_panels; ++p) {
+        if ((o->build_status && p == 0) || o->build_epsmap) {
+            ++vs.flag_sum;
+            *traced += N * N;
+            if (p < c->panels[0]) vs.slot[vs.n++] = p;
+        }
+        if (o->accurate_triangulation) {
+            ++vs.flag_sum;
+            *traced += N * N;
+            vs.slot[vs.n++] = c->panels[0] + p;
+        }
+    }
+    return vs;
+}
+
+// everything of getSurf up to the cavities, then the result block copy
+static int enqueue_surf(nsgpu_ctx *c, hipStream_t st) {
+    int rc = enqueue_trace(c, st);
+    if (rc) return rc;
+    const DevGrid g = dev_grid(c);
+    const DevRays R = dev_rays(c);
+    const nsgpu_surf_opts *o = &c->opts;
+    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
+    const int64_t nwords = (nvox + 31) / 32;
+    const FillOpts fo = fill_opts(c);
+    // maps (buffers always exist so that later stages can index them)
+    HIP_TRY(c, c->eps.ensure((size_t)nvox * 12));
+    HIP_TRY(c, c->ideb.ensure((size_t)nvox));
+    HIP_TRY(c, c->status.ensure((size_t)nvox * 4));
+    HIP_TRY(c, c->bits.ensure((size_t)nwords * 4));
+    mark(c, ST_FILL, 0, st);
+    if (c->panels[0] > 0)
+        launch_fill_tiled(st, g, R, fo, c->eps.as<int>(), c->ideb.as<uint8_t>(), c->status.as<int>());
+    HIP_TRY(c, hipGetLastError());
+    mark(c, ST_FILL, 1, st);
+    mark(c, ST_CORNERS, 0, st);
+    if (o->accurate_triangulation) {
+        launch_fill32(st, c->bits.p, 0xffffffffu, nwords);
+        launch_corners_tiled(st, g, R, fo, c->bits.as<uint32_t>());
+    }
+    HIP_TRY(c, hipGetLastError());
+    mark(c, ST_CORNERS, 1, st);
+    // volume: A * sum of segment lengths / (panels x phases traced)
+    {
+        const int64_t NN = (int64_t)R.N * R.N;
+        int64_t traced = 0;
+        const VolumeSlots vs = volume_slots(c, &traced);
+        const int64_t per = sum_partials_count(NN);
+        HIP_TRY(c, c->partial.ensure((size_t)per * c->nslots * 8));
+        for (int k = 0; k < vs.n; ++k)
+            launch_sum_blocks(st, R.rvol + vs.slot[k] * NN, NN, c->partial.as<double>() + vs.slot[k] * per);
+        launch_volume(st, c->partial.as<double>(), per, vs, (double *)(c->dres.as<unsigned long long>() + RES_VOL));
+        HIP_TRY(c, hipGetLastError());
+    }
+    mark(c, ST_CAV, 0, st);
+    if (o->fill_cavities && o->build_status) {
+        rc = enqueue_cavities(c, st);
+        if (rc) return rc;
+    }
+    mark(c, ST_CAV, 1, st);
+    return enqueue_results(c, st);
+}
+
+// host side of a completed enqueue_surf: statistics, volume, cavities
+static int finish_surf(nsgpu_ctx *c) {
+    take_trace_stats(c);
+    c->have_trace = true;
+    int64_t traced = 0;
+    (void)volume_slots(c, &traced);
+    c->res.total_rays = traced;
+    c->res.volume = c->hres->volume();
+    c->cav_size.clear();
+    c->cav_filled.clear();
+    c->res.num_cavities = 0;
+    if (c->opts.fill_cavities && c->opts.build_status) {
+        const int *info = c->hres->cav_info();
+        if (info[0] == 0) {
+            const int rc = cavities_on_host(c);
+            if (rc) return rc;
+        } else {
+            const int ncav = info[1];
+            c->res.num_cavities = ncav;
+            c->cav_size.resize(ncav);
+            c->cav_filled.resize(ncav);
+            if (ncav) {
+                HIP_TRY(c, hipMemcpyAsync(c->cav_size.data(), c->dcavsize.p, (size_t)ncav * 8, hipMemcpyDeviceToHost,
+                                          c->stream));
+                HIP_TRY(c, hipMemcpyAsync(c->cav_filled.data(), c->dcavfilled.p, (size_t)ncav, hipMemcpyDeviceToHost,
+                                          c->stream));
+                HIP_TRY(c, hipStreamSynchronize(c->stream));
+            }
+        }
+    }
     return NSGPU_OK;
 }
 
@@ -504,7 +740,7 @@ static int assemble(nsgpu_ctx *c, const int eslots[3]) {
     HIP_TRY(c, c->vcount.ensure((size_t)nmax * 4));
     HIP_TRY(c, c->voffs.ensure((size_t)nmax * 4));
     HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(nmax) * 4));
-    HIP_TRY(c, hipMemsetAsync(c->idx.p, 0xff, (size_t)nvox * 3 * 4, c->stream));
+    launch_fill32(c->stream, c->idx.p, 0xffffffffu, nvox * 3);
     launch_vtx_count(c->stream, g, R, eslots, c->bits.as<uint32_t>(), c->vcount.as<int>());
     launch_scan(c->stream, c->vcount.as<int>(), nrays, c->voffs.as<int>(), c->scan_scratch.as<int>());
     HIP_TRY(c, hipGetLastError());
@@ -522,106 +758,27 @@ static int assemble(nsgpu_ctx *c, const int eslots[3]) {
 
 int nsgpu_get_surf(nsgpu_ctx *c, const nsgpu_surf_opts *o, nsgpu_surf_result *res) {
     if (!c || !o) return NSGPU_E_ARG;
-    if (!c->have_grid || !c->have_patches) return fail(c, NSGPU_E_STATE, "set_grid and set_patches first");
-    if (o->precision != NSGPU_PREC_FP64 && o->precision != NSGPU_PREC_MIXED)
-        return fail(c, NSGPU_E_ARG, "unknown precision mode");
-    HIP_TRY(c, hipSetDevice(c->device));
-    c->opts = *o;
-    std::memset(&c->res, 0, sizeof c->res);
-    c->have_trace = c->have_surf = c->have_vertices = c->have_mesh = false;
-    // panels rule, Surface.cpp:652-659
-    c->panels[0] = 3;
-    c->panels[1] = 3;
-    if (!o->build_status && !o->build_epsmap) c->panels[0] = 0;
-    else if (o->build_status && !o->build_epsmap) c->panels[0] = 1;
-    if (!o->accurate_triangulation) c->panels[1] = 0;
-    c->nslots = c->panels[0] + c->panels[1];
-    if (c->nslots == 0) return fail(c, NSGPU_E_ARG, "nothing to compute (no epsmap, status or triangulation)");
-    const DevGrid g = dev_grid(c);
-    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
-    const int64_t nwords = (nvox + 31) / 32;
-    // intersections
-    int rc = trace(c);
+    int rc = begin_surf(c, o);
     if (rc) return rc;
-    const DevRays R = dev_rays(c);
-    FillOpts fo;
-    fo.build_eps = o->build_epsmap;
-    fo.build_status = o->build_status;
-    fo.inside = o->inside;
-    fo.slot_x = c->panels[0] >= 1 ? 0 : -1;
-    fo.slot_z = (c->panels[0] >= 3 && o->build_epsmap) ? 1 : -1;
-    fo.slot_y = (c->panels[0] >= 3 && o->build_epsmap) ? 2 : -1;
-    fo.eslot_x = c->panels[0] + 0;
-    fo.eslot_z = c->panels[0] + 1;
-    fo.eslot_y = c->panels[0] + 2;
-    // maps (buffers always exist so that later stages can index them)
-    HIP_TRY(c, c->eps.ensure((size_t)nvox * 12));
-    HIP_TRY(c, c->ideb.ensure((size_t)nvox));
-    HIP_TRY(c, c->status.ensure((size_t)nvox * 4));
-    HIP_TRY(c, c->bits.ensure((size_t)nwords * 4));
-    mark(c, ST_FILL, 0);
-    if (c->panels[0] > 0)
-        launch_fill_tiled(c->stream, g, R, fo, c->eps.as<int>(), c->ideb.as<uint8_t>(), c->status.as<int>());
-    HIP_TRY(c, hipGetLastError());
-    mark(c, ST_FILL, 1);
-    mark(c, ST_CORNERS, 0);
-    if (o->accurate_triangulation) {
-        HIP_TRY(c, hipMemsetAsync(c->bits.p, 0xff, (size_t)nwords * 4, c->stream));
-        launch_corners_tiled(c->stream, g, R, fo, c->bits.as<uint32_t>());
-    }
-    HIP_TRY(c, hipGetLastError());
-    mark(c, ST_CORNERS, 1);
-    // volume: A * sum of segment lengths / (panels x phases traced), Surface.cpp:1677-1684
-    {
-        const int64_t NN = (int64_t)R.N * R.N;
-        int fill_panels = 3;
-        if (!o->build_status && !o->build_epsmap) fill_panels = 0;
-        else if (o->build_status && !o->build_epsmap) fill_panels = 1;
-        if (o->accurate_triangulation) fill_panels = 3;
-        int flag_sum = 0;
-        std::vector<int> vslots;
-        for (int p = 0; p < fill_panels; ++p) {
-            if ((o->build_status && p == 0) || o->build_epsmap) {
-                ++flag_sum;
-                if (p < c->panels[0]) vslots.push_back(p);
-            }
-            if (o->accurate_triangulation) {
-                ++flag_sum;
-                vslots.push_back(c->panels[0] + p);
-            }
-        }
-        const int64_t per = sum_partials_count(NN);
-        HIP_TRY(c, c->partial.ensure((size_t)per * c->nslots * 8));
-        for (int s : vslots) launch_sum_blocks(c->stream, R.rvol + s * NN, NN, c->partial.as<double>() + s * per);
-        HIP_TRY(c, hipGetLastError());
-        std::vector<double> part((size_t)per * c->nslots, 0.0);
-        HIP_TRY(c, hipMemcpyAsync(part.data(), c->partial.p, part.size() * 8, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(c, hipStreamSynchronize(c->stream));
-        double sum = 0;
-        for (int s : vslots)
-            for (int64_t q = 0; q < per; ++q) sum += part[(size_t)s * per + q];
-        c->res.volume = flag_sum ? c->grid.A * sum / flag_sum : 0.0;
-        int64_t traced = 0;
-        for (int p = 0; p < fill_panels; ++p) {
-            if ((o->build_status && p == 0) || o->build_epsmap) traced += NN;
-            if (o->accurate_triangulation) traced += NN;
-        }
-        c->res.total_rays = traced;
-    }
-    mark(c, ST_CAV, 0);
-    if (o->fill_cavities && o->build_status) {
-        rc = cavities(c);
+    HIP_TRY(c, hipSetDevice(c->device));
+    for (int attempt = 0;; ++attempt) {
+        if (attempt == 4) return fail(c, NSGPU_E_STATE, "ray hit capacity did not converge");
+        rc = enqueue_surf(c, c->stream);
         if (rc) return rc;
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        if (!regrow(c)) break;
     }
-    mark(c, ST_CAV, 1);
-    mark(c, ST_VERTS, 0);
+    rc = finish_surf(c);
+    if (rc) return rc;
+    mark(c, ST_VERTS, 0, c->stream);
     if (o->accurate_triangulation) {
+        const FillOpts fo = fill_opts(c);
         const int es[3] = {fo.eslot_x, fo.eslot_z, fo.eslot_y};
         rc = assemble(c, es);
         if (rc) return rc;
         c->have_vertices = true;
     }
-    mark(c, ST_VERTS, 1);
+    mark(c, ST_VERTS, 1, c->stream);
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->have_surf = true;
     if (res) *res = c->res;
@@ -634,7 +791,7 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
     HIP_TRY(c, hipSetDevice(c->device));
     const DevGrid g = dev_grid(c);
     const int64_t nseg = mc_num_segments(g);
-    mark(c, ST_MC, 0);
+    mark(c, ST_MC, 0, c->stream);
     c->nverts = c->n_analytic;
     // 1. classify 32-cube row segments, compact the active cubes in cube order
     HIP_TRY(c, c->segmask.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
@@ -699,7 +856,7 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
     HIP_TRY(c, c->partial.ensure((size_t)std::max<int64_t>(per, 1) * 8));
     launch_sum_blocks(c->stream, c->tarea.as<double>(), nt, c->partial.as<double>());
     HIP_TRY(c, hipGetLastError());
-    mark(c, ST_MC, 1);
+    mark(c, ST_MC, 1, c->stream);
     std::vector<double> part((size_t)per, 0.0);
     if (per) HIP_TRY(c, hipMemcpyAsync(part.data(), c->partial.p, (size_t)per * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -831,10 +988,10 @@ int nsgpu_surface_gradient(nsgpu_ctx *c, double *volume, double *area, double *d
     a.grad_v = c->gradv.as<double>();
     a.grad_a = c->grada.as<double>();
     a.sums = c->asums.as<double>();
-    mark(c, ST_ADJOINT, 0);
+    mark(c, ST_ADJOINT, 0, c->stream);
     launch_adjoint(c->stream, g, P, R, a);
     HIP_TRY(c, hipGetLastError());
-    mark(c, ST_ADJOINT, 1);
+    mark(c, ST_ADJOINT, 1, c->stream);
     std::vector<double> sums((size_t)nb * 2);
     if (nb)
         HIP_TRY(c, hipMemcpyAsync(sums.data(), c->asums.p, sums.size() * 8, hipMemcpyDeviceToHost, c->stream));
@@ -853,6 +1010,259 @@ int nsgpu_surface_gradient(nsgpu_ctx *c, double *volume, double *area, double *d
     c->res.volume = *volume;
     c->res.total_rays = 3LL * R.N * R.N;
     if (res) *res = c->res;
+    return NSGPU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batches: many frames (grid + patch set each), one context per frame, all
+// frames' getSurf sequences captured into one graph.  The frames run in
+// `chains` concurrent chains (largest frames first, dealt round-robin), so
+// small frames fill the device together; the chain count is picked by timing
+// (nsgpu_batch_prepare with chains <= 0).
+
+struct nsgpu_batch {
+    int device = 0;
+    std::vector<nsgpu_ctx *> frames;
+    hipStream_t s0 = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int chains = 0;
+    nsgpu_surf_opts opts{};
+    bool prepared = false;
+    DBuf stamps;  // frame completion clocks, then the start clock
+    double clock_khz = 0;
+    std::string err;
+};
+
+static int bfail(nsgpu_batch *b, int code, const std::string &msg) {
+    b->err = msg;
+    return code;
+}
+
+#define BHIP_TRY(b, expr)                                                                              \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) return bfail((b), NSGPU_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static void drop_graph(nsgpu_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->prepared = false;
+}
+
+int nsgpu_batch_create(int device, int32_t n_frames, nsgpu_batch **out) {
+    if (!out || n_frames <= 0) return NSGPU_E_ARG;
+    *out = nullptr;
+    auto *b = new nsgpu_batch();
+    b->device = device;
+    for (int f = 0; f < n_frames; ++f) {
+        nsgpu_ctx *c = nullptr;
+        const int rc = nsgpu_create(device, &c);
+        if (rc) {
+            nsgpu_batch_destroy(b);
+            return rc;
+        }
+        b->frames.push_back(c);
+        hipEvent_t e;
+        (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
+        b->join.push_back(e);
+    }
+    int khz = 0;
+    if (hipStreamCreateWithFlags(&b->s0, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) != hipSuccess ||
+        b->stamps.ensure((size_t)(n_frames + 1) * 8) != hipSuccess) {
+        nsgpu_batch_destroy(b);
+        return NSGPU_E_DEVICE;
+    }
+    b->clock_khz = khz;
+    *out = b;
+    return NSGPU_OK;
+}
+
+void nsgpu_batch_destroy(nsgpu_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->s0) (void)hipStreamSynchronize(b->s0);
+    drop_graph(b);
+    for (nsgpu_ctx *c : b->frames) nsgpu_destroy(c);
+    for (hipEvent_t e : b->join)
+        if (e) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->s0) (void)hipStreamDestroy(b->s0);
+    b->stamps.release();
+    delete b;
+}
+
+const char *nsgpu_batch_last_error(const nsgpu_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+nsgpu_ctx *nsgpu_batch_frame(nsgpu_batch *b, int32_t f) {
+    if (!b || f < 0 || f >= (int)b->frames.size()) return nullptr;
+    return b->frames[f];
+}
+
+int nsgpu_batch_set_frame(nsgpu_batch *b, int32_t f, const nsgpu_grid *grid, const nsgpu_patches *p, double probe) {
+    if (!b || f < 0 || f >= (int)b->frames.size() || !grid || !p) return NSGPU_E_ARG;
+    nsgpu_ctx *c = b->frames[f];
+    int rc = nsgpu_set_grid(c, grid);
+    if (!rc) rc = nsgpu_set_patches(c, p, probe);
+    if (rc) return bfail(b, rc, std::string("frame ") + std::to_string(f) + ": " + c->err);
+    drop_graph(b);
+    return NSGPU_OK;
+}
+
+static int64_t frame_nodes(const nsgpu_ctx *c) { return (int64_t)c->grid.nx * c->grid.ny * c->grid.nz; }
+
+// capture every frame's getSurf into b->graph with `chains` concurrent chains
+static int capture(nsgpu_batch *b, int chains) {
+    drop_graph(b);
+    const int n = (int)b->frames.size();
+    std::vector<int> order(n);
+    for (int f = 0; f < n; ++f) order[f] = f;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return frame_nodes(b->frames[x]) > frame_nodes(b->frames[y]); });
+    unsigned long long *stamps = b->stamps.as<unsigned long long>();
+    BHIP_TRY(b, hipStreamBeginCapture(b->s0, hipStreamCaptureModeRelaxed));
+    g_no_alloc = true;
+    int rc = NSGPU_OK;
+    launch_stamp(b->s0, stamps + n);
+    if (hipEventRecord(b->fork, b->s0) != hipSuccess) rc = bfail(b, NSGPU_E_DEVICE, "fork event");
+    for (int pos = 0; pos < n && !rc; ++pos) {
+        const int f = order[pos];
+        nsgpu_ctx *c = b->frames[f];
+        const hipEvent_t dep = pos < chains ? b->fork : b->join[order[pos - chains]];
+        if (hipStreamWaitEvent(c->stream, dep, 0) != hipSuccess) {
+            rc = bfail(b, NSGPU_E_DEVICE, "chain wait");
+            break;
+        }
+        c->capturing = true;
+        rc = begin_surf(c, &b->opts);
+        if (!rc) rc = enqueue_surf(c, c->stream);
+        c->capturing = false;
+        if (rc) {
+            rc = bfail(b, rc, std::string("frame ") + std::to_string(f) + ": " + c->err);
+            break;
+        }
+        launch_stamp(c->stream, stamps + f);
+        if (hipEventRecord(b->join[f], c->stream) != hipSuccess) rc = bfail(b, NSGPU_E_DEVICE, "join event");
+    }
+    for (int f = 0; f < n && !rc; ++f)
+        if (hipStreamWaitEvent(b->s0, b->join[f], 0) != hipSuccess) rc = bfail(b, NSGPU_E_DEVICE, "join wait");
+    g_no_alloc = false;
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->s0, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return bfail(b, NSGPU_E_DEVICE, std::string("capture: ") + hipGetErrorString(e));
+    b->graph = graph;
+    BHIP_TRY(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->chains = chains;
+    b->prepared = true;
+    return NSGPU_OK;
+}
+
+static int launch_graph(nsgpu_batch *b, double *ms) {
+    const auto t0 = std::chrono::steady_clock::now();
+    BHIP_TRY(b, hipGraphLaunch(b->exec, b->s0));
+    BHIP_TRY(b, hipStreamSynchronize(b->s0));
+    if (ms) *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return NSGPU_OK;
+}
+
+int nsgpu_batch_prepare(nsgpu_batch *b, const nsgpu_surf_opts *opts, int32_t chains) {
+    if (!b || !opts) return NSGPU_E_ARG;
+    if (opts->accurate_triangulation)
+        return bfail(b, NSGPU_E_ARG, "batches compute maps and cavities only (accurate_triangulation = 0)");
+    BHIP_TRY(b, hipSetDevice(b->device));
+    b->opts = *opts;
+    drop_graph(b);
+    // eager pass: sizes every buffer and each frame's ray capacity
+    for (size_t f = 0; f < b->frames.size(); ++f) {
+        const int rc = nsgpu_get_surf(b->frames[f], opts, nullptr);
+        if (rc) return bfail(b, rc, std::string("frame ") + std::to_string(f) + ": " + b->frames[f]->err);
+    }
+    const int n = (int)b->frames.size();
+    if (chains > 0) return capture(b, std::min(chains, n));
+    // autotune: the chain count with the shortest of two launches
+    int best = 1;
+    double best_ms = 1e30;
+    for (int ch = 1;; ch *= 2) {
+        const int c = std::min(ch, n);
+        int rc = capture(b, c);
+        if (rc) return rc;
+        double t = 1e30;
+        for (int rep = 0; rep < 2; ++rep) {
+            double ms = 0;
+            rc = launch_graph(b, &ms);
+            if (rc) return rc;
+            t = std::min(t, ms);
+        }
+        if (t < best_ms) {
+            best_ms = t;
+            best = c;
+        }
+        if (c == n || ch >= 64) break;
+    }
+    return capture(b, best);
+}
+
+int32_t nsgpu_batch_chains(const nsgpu_batch *b) { return b ? b->chains : 0; }
+
+int nsgpu_batch_run(nsgpu_batch *b, double *latency_ms, int32_t *redone) {
+    if (!b) return NSGPU_E_ARG;
+    if (!b->prepared) return bfail(b, NSGPU_E_STATE, "nsgpu_batch_prepare first");
+    BHIP_TRY(b, hipSetDevice(b->device));
+    const int n = (int)b->frames.size();
+    int rc = launch_graph(b, nullptr);
+    if (rc) return rc;
+    std::vector<unsigned long long> st(n + 1);
+    BHIP_TRY(b, hipMemcpy(st.data(), b->stamps.p, (size_t)(n + 1) * 8, hipMemcpyDeviceToHost));
+    if (latency_ms)
+        for (int f = 0; f < n; ++f) latency_ms[f] = (double)(st[f] - st[n]) / b->clock_khz;
+    int again = 0;
+    bool regrown = false;
+    for (int f = 0; f < n; ++f) {
+        nsgpu_ctx *c = b->frames[f];
+        if (regrow(c)) {
+            // the frame's rays outgrew their lists: redo it eagerly, capture again next time
+            regrown = true;
+            ++again;
+            rc = nsgpu_get_surf(c, &b->opts, nullptr);
+        } else {
+            rc = finish_surf(c);
+            c->have_surf = true;
+            if (c->opts.fill_cavities && c->opts.build_status && c->hres->cav_info()[0] == 0) ++again;
+        }
+        if (rc) return bfail(b, rc, std::string("frame ") + std::to_string(f) + ": " + c->err);
+    }
+    if (regrown) drop_graph(b);
+    if (redone) *redone = again;
+    return NSGPU_OK;
+}
+
+int nsgpu_batch_run_loop(nsgpu_batch *b, double *latency_ms) {
+    if (!b) return NSGPU_E_ARG;
+    for (size_t f = 0; f < b->frames.size(); ++f) {
+        const auto t0 = std::chrono::steady_clock::now();
+        const int rc = nsgpu_get_surf(b->frames[f], &b->opts, nullptr);
+        if (rc) return bfail(b, rc, std::string("frame ") + std::to_string(f) + ": " + b->frames[f]->err);
+        if (latency_ms)
+            latency_ms[f] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return NSGPU_OK;
+}
+
+int nsgpu_batch_result(nsgpu_batch *b, int32_t f, nsgpu_surf_result *res) {
+    if (!b || !res || f < 0 || f >= (int)b->frames.size()) return NSGPU_E_ARG;
+    if (!b->frames[f]->have_surf) return bfail(b, NSGPU_E_STATE, "frame not computed");
+    *res = b->frames[f]->res;
     return NSGPU_OK;
 }
 

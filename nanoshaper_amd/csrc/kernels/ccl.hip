@@ -259,7 +259,7 @@ __device__ __forceinline__ int slot_of(const int *R, int c) {
 // is skipped, which removes the only heavily contended address.
 __global__ void __launch_bounds__(256) k_ccl_keys(DevGrid g, int cap, const int *rx0, const int *rx1,
                                                   const int *rowcnt, const int *roff, const int *R, unsigned int *bkey,
-                                                  int *count) {
+                                                  int *count, int capacity) {
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= (int64_t)g.ny * g.nz) return;
     const int skip = (rowcnt[0] > 0 && rx0[0] == 0) ? slot_of(R, 0) : -1;
@@ -268,7 +268,7 @@ __global__ void __launch_bounds__(256) k_ccl_keys(DevGrid g, int cap, const int 
     const int64_t plane = (int64_t)g.nx * g.ny;
     for (int q = 0; q < n; ++q) {
         const int s = slot_of(R, c0 + q);
-        if (s == skip) continue;
+        if (s == skip || s >= capacity) continue;  // beyond capacity: the replay falls back to the host
         const int64_t id = row * cap + q;
         const unsigned int rank = (unsigned int)((g.nz - 1 - k) * plane + (int64_t)j * g.nx + rx0[id]);
         atomicMin(bkey + s, rank);
@@ -280,10 +280,11 @@ __global__ void __launch_bounds__(256) k_ccl_keys(DevGrid g, int cap, const int 
 // components: one wave per row, lanes stride over each run's nodes.
 __global__ void __launch_bounds__(256) k_ccl_label(DevGrid g, int cap, const int *rx0, const int *rx1,
                                                    const int *rowcnt, const int *roff, const int *R, const int *label,
-                                                   const uint8_t *filled, int build_eps, int accurate, int inside,
-                                                   int *status, int *epsmap, uint8_t *idebmap, uint32_t *bits) {
+                                                   const uint8_t *filled, const int *done, int build_eps, int accurate,
+                                                   int inside, int *status, int *epsmap, uint8_t *idebmap,
+                                                   uint32_t *bits) {
     const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (row >= (int64_t)g.ny * g.nz) return;
+    if (row >= (int64_t)g.ny * g.nz || *done == 0) return;
     const int lane = threadIdx.x & 63;
     const int n = rowcnt[row], c0 = roff[row];
     const int j = (int)(row % g.ny), k = (int)(row / g.ny);
@@ -342,19 +343,129 @@ void launch_ccl(hipStream_t s, const DevGrid &g, const int *status, const CclRun
                        rootlist);
 }
 
-void launch_ccl_keys(hipStream_t s, const DevGrid &g, const CclRuns &c, unsigned int *bkey, int *count) {
+void launch_ccl_keys(hipStream_t s, const DevGrid &g, const CclRuns &c, unsigned int *bkey, int *count,
+                     int capacity) {
     const int64_t rows = (int64_t)g.ny * g.nz;
     hipLaunchKernelGGL(k_ccl_keys, dim3(nblk(rows)), dim3(256), 0, s, g, ccl_row_capacity(g), c.rx0, c.rx1, c.rowcnt,
-                       c.roff, c.R, bkey, count);
+                       c.roff, c.R, bkey, count, capacity);
 }
 
 void launch_ccl_label(hipStream_t s, const DevGrid &g, const CclRuns &c, const int *label, const uint8_t *filled,
-                      int build_eps, int accurate, int inside, int *status, int *eps, uint8_t *ideb,
+                      const int *done, int build_eps, int accurate, int inside, int *status, int *eps, uint8_t *ideb,
                       uint32_t *bits) {
     const int64_t rows = (int64_t)g.ny * g.nz;
     hipLaunchKernelGGL(k_ccl_label, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, s, g, ccl_row_capacity(g), c.rx0,
-                       c.rx1, c.rowcnt, c.roff, c.R, label, filled, build_eps, accurate, inside, status, eps, ideb,
-                       bits);
+                       c.rx1, c.rowcnt, c.roff, c.R, label, filled, done, build_eps, accurate, inside, status, eps,
+                       ideb, bits);
+}
+
+// keys of the *nroots components start at (no key, size 0)
+__global__ void __launch_bounds__(256) k_ccl_clear_keys(const int *nroots, unsigned int *bkey, int *count,
+                                                        int64_t capacity) {
+    const int64_t n = min((int64_t)*nroots, capacity);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        bkey[i] = 0xffffffffu;
+        count[i] = 0;
+    }
+}
+
+void launch_ccl_clear_keys(hipStream_t s, const int *nroots, unsigned int *bkey, int *count, int64_t capacity) {
+    hipLaunchKernelGGL(k_ccl_clear_keys, dim3(256), dim3(256), 0, s, nroots, bkey, count, capacity);
+}
+
+// bitonic sort of n (power of two, <= kReplayMax) keys in LDS, ascending
+__device__ void bitonic_sort(unsigned long long *k, int n) {
+    for (int size = 2; size <= n; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    const bool up = (i & size) == 0;
+                    const unsigned long long a = k[i], b = k[j];
+                    if ((a > b) == up) {
+                        k[i] = b;
+                        k[j] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+}
+
+// The alternating scan of getCavitiesWithBilevelStatusMap (Surface.cpp:
+// 1994-2126) over the components: even passes take the not yet numbered
+// component met first scanning k upwards (smallest root = smallest flat
+// index), odd passes the one met first scanning k downwards (smallest
+// backward key).  Ids count up from 3 (the first component is the outside);
+// then fillCavities' sizes and filled flags (Surface.cpp:2355-2487).  One
+// workgroup; more than kReplayMax components leave info[0] = 0 for the host.
+__global__ void __launch_bounds__(1024) k_ccl_replay(CavityReplay r) {
+    __shared__ unsigned long long keys[kReplayMax];
+    __shared__ unsigned short orderF[kReplayMax], orderB[kReplayMax];
+    __shared__ unsigned char done[kReplayMax];
+    const int n = *r.nroots;
+    if (n > kReplayMax) {
+        if (threadIdx.x == 0) {
+            r.info[0] = 0;
+            r.info[1] = 0;
+        }
+        return;
+    }
+    int np = 1;
+    while (np < n) np <<= 1;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int i = threadIdx.x; i < np; i += blockDim.x) {
+            unsigned long long key = ~0ull;
+            if (i < n) key = ((unsigned long long)(pass == 0 ? (unsigned int)r.rootlist[i] : r.bkey[i]) << 32) | i;
+            keys[i] = key;
+        }
+        __syncthreads();
+        bitonic_sort(keys, np);
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            const unsigned short slot = (unsigned short)(keys[i] & 0xffffu);
+            if (pass == 0)
+                orderF[i] = slot;
+            else
+                orderB[i] = slot;
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) done[i] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int pf = 0, pb = 0, id = NSGPU_STATUS_TEMP_OUT;
+        for (int t = 0; t < n; ++t) {
+            int pick;
+            if ((t & 1) == 0) {
+                while (done[orderF[pf]]) ++pf;
+                pick = orderF[pf];
+            } else {
+                while (done[orderB[pb]]) ++pb;
+                pick = orderB[pb];
+            }
+            done[pick] = 1;
+            r.label[pick] = ++id;
+        }
+        r.info[1] = id - NSGPU_STATUS_OUT > 0 ? id - NSGPU_STATUS_OUT : 0;
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < n; s += blockDim.x) {
+        const int lab = r.label[s];
+        uint8_t f = 0;
+        if (lab >= NSGPU_STATUS_FIRST_CAV) {
+            const double v = r.count[s] * r.cube;
+            f = v <= r.vol;
+            r.cav_size[lab - NSGPU_STATUS_FIRST_CAV] = v;
+            r.cav_filled[lab - NSGPU_STATUS_FIRST_CAV] = f;
+        }
+        r.filled[s] = f;
+    }
+    if (threadIdx.x == 0) r.info[0] = 1;
+}
+
+void launch_ccl_replay(hipStream_t s, const CavityReplay &r) {
+    hipLaunchKernelGGL(k_ccl_replay, dim3(1), dim3(1024), 0, s, r);
 }
 
 }  // namespace nsg

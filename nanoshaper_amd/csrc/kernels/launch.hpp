@@ -35,9 +35,11 @@ int ccl_row_capacity(const DevGrid &);
 // scan_scratch: scan_scratch_ints(rows) ints
 void launch_ccl(hipStream_t, const DevGrid &, const int *status, const CclRuns &, int *scan_scratch, int *nroots,
                 int *rootlist);
-void launch_ccl_keys(hipStream_t, const DevGrid &, const CclRuns &, unsigned int *bkey, int *count);
-void launch_ccl_label(hipStream_t, const DevGrid &, const CclRuns &, const int *label, const uint8_t *filled, int,
-                      int, int, int *, int *, uint8_t *, uint32_t *);
+// components with slot >= capacity are skipped
+void launch_ccl_keys(hipStream_t, const DevGrid &, const CclRuns &, unsigned int *bkey, int *count, int capacity);
+// done: the replay's info[0]; the pass does nothing when it is 0
+void launch_ccl_label(hipStream_t, const DevGrid &, const CclRuns &, const int *label, const uint8_t *filled,
+                      const int *done, int, int, int, int *, int *, uint8_t *, uint32_t *);
 // adjoint of the three-panel volume and of the area estimate
 struct AdjointArgs {
     const double *atoms;  // x, y, z, r per atom
@@ -49,6 +51,31 @@ struct AdjointArgs {
 };
 int64_t adjoint_blocks(const DevRays &);
 void launch_adjoint(hipStream_t, const DevGrid &, const DevPatches &, const DevRays &, const AdjointArgs &);
+// fills (graph-safe clears) and the volume reduction
+void launch_fill32(hipStream_t, void *p, uint32_t v, int64_t n_words);
+struct VolumeSlots {
+    int n, flag_sum;
+    int slot[6];
+    double area_px;
+};
+void launch_volume(hipStream_t, const double *partial, int64_t per, const VolumeSlots &, double *out);
+// writes the device wall clock
+void launch_stamp(hipStream_t, unsigned long long *out);
+// on-device replay of the cavity numbering (up to kReplayMax components)
+constexpr int kReplayMax = 4096;
+struct CavityReplay {
+    const int *nroots, *rootlist;
+    const unsigned int *bkey;
+    const int *count;
+    double cube, vol;  // node volume side^3, filling threshold
+    int *label;        // per component
+    uint8_t *filled;   // per component
+    double *cav_size;  // per cavity label - 4
+    uint8_t *cav_filled;
+    int *info;  // [0] replay done (0: too many components), [1] number of cavities
+};
+void launch_ccl_clear_keys(hipStream_t, const int *nroots, unsigned int *bkey, int *count, int64_t capacity);
+void launch_ccl_replay(hipStream_t, const CavityReplay &);
 int64_t scan_scratch_ints(int64_t n);
 void launch_scan(hipStream_t, const int *, int64_t, int *, int *);
 void mc_upload_tables(hipStream_t);

@@ -151,6 +151,18 @@ class DeviceSurface:
         self.result = None
         self.mesh_info = None
 
+    @classmethod
+    def _view(cls, handle, grid):
+        """A non-owning view of a context owned elsewhere (a batch frame)."""
+        d = cls.__new__(cls)
+        d._lib = _lib.load()
+        d._h = ctypes.c_void_p(handle)
+        d._owned = False
+        d.grid = grid
+        d.result = None
+        d.mesh_info = None
+        return d
+
     def _chk(self, rc, what):
         _check(rc, self._h, what)
 
@@ -262,7 +274,8 @@ class DeviceSurface:
 
     def close(self):
         if getattr(self, "_h", None):
-            self._lib.nsgpu_destroy(self._h)
+            if getattr(self, "_owned", True):
+                self._lib.nsgpu_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -316,3 +329,87 @@ class NanoShaper:
     def get_grid_size(self):
         g = self.grid
         return g.nx, g.ny, g.nz
+
+
+class Batch:
+    """Frames of uneven size (one grid and patch set each) computed by one
+    captured graph launch; maps and cavities per frame (no triangulation)."""
+
+    def __init__(self, n_frames, device=0):
+        self._lib = _lib.load()
+        h = ctypes.c_void_p()
+        rc = self._lib.nsgpu_batch_create(int(device), int(n_frames), ctypes.byref(h))
+        if rc != 0:
+            raise NsgpuError("nsgpu_batch_create failed (%d)" % rc)
+        self._h = h
+        self.n = int(n_frames)
+        self.grids = [None] * self.n
+        self._keep = [None] * self.n
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise NsgpuError("%s failed (%d): %s" % (what, rc, self._lib.nsgpu_batch_last_error(self._h).decode()))
+
+    def set_frame(self, f, grid, patchset):
+        self._chk(self._lib.nsgpu_batch_set_frame(self._h, int(f), ctypes.byref(grid), ctypes.byref(patchset.view),
+                                                  patchset.probe), "batch_set_frame")
+        self.grids[f] = grid
+        self._keep[f] = patchset
+
+    def prepare(self, chains=0, **kw):
+        """Size every frame and capture the graph; chains <= 0 autotunes.
+        Returns the chain count used."""
+        kw.setdefault("accurate_triangulation", False)
+        o = surf_opts(**kw)
+        self._chk(self._lib.nsgpu_batch_prepare(self._h, ctypes.byref(o), int(chains)), "batch_prepare")
+        return int(self._lib.nsgpu_batch_chains(self._h))
+
+    def run(self):
+        """One graph launch: (per-frame completion ms after launch, frames redone on the host path)."""
+        lat = np.zeros(self.n)
+        redone = ctypes.c_int32()
+        self._chk(self._lib.nsgpu_batch_run(self._h, _ptr(lat), ctypes.byref(redone)), "batch_run")
+        return lat, redone.value
+
+    def run_loop(self):
+        """The per-frame launch loop over the same frames: per-frame wall ms."""
+        lat = np.zeros(self.n)
+        self._chk(self._lib.nsgpu_batch_run_loop(self._h, _ptr(lat)), "batch_run_loop")
+        return lat
+
+    def result(self, f):
+        r = SurfResult()
+        self._chk(self._lib.nsgpu_batch_result(self._h, int(f), ctypes.byref(r)), "batch_result")
+        return _result_dict(r)
+
+    def frame(self, f):
+        """DeviceSurface view of frame f (maps, status, cavities)."""
+        h = self._lib.nsgpu_batch_frame(self._h, int(f))
+        d = DeviceSurface._view(h, self.grids[f])
+        d.result = self.result(f)
+        return d
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.nsgpu_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def perturbed_frames(n_frames, n_min=500, n_max=8000, sigma=0.3, seed=0):
+    """Frames of uneven size: atom counts log-uniform in [n_min, n_max], each
+    a seeded chain (confinement radius 1.65 n^(1/3) A) perturbed by per-atom
+    Gaussian displacements of standard deviation sigma (A)."""
+    rng = np.random.default_rng(seed)
+    frames = []
+    for f in range(n_frames):
+        n = int(round(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
+        xyzr, _ = generate_chain(n, 1.65 * n ** (1.0 / 3.0), seed=seed * 100003 + f)
+        xyzr[:, :3] += rng.normal(0.0, sigma, size=(len(xyzr), 3))
+        frames.append(xyzr)
+    return frames

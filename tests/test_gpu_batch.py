@@ -1,0 +1,80 @@
+"""GPU: batches of uneven frames in one captured graph launch give exactly
+the per-frame get_surf results, on every replay."""
+import numpy as np
+import pytest
+
+import nanoshaper_amd as ns
+
+pytestmark = pytest.mark.gpu
+
+OPTS = dict(build_epsmap=True, build_status=True, accurate_triangulation=False, fill_cavities=True)
+
+
+def _inputs(frames, scale=2.0):
+    out = []
+    for xyzr in frames:
+        grid = ns.build_grid(xyzr, scale, 80.0)
+        jit, _ = ns.jitter(xyzr)
+        out.append((grid, ns.ses_patches(jit, 1.4)))
+    return out
+
+
+def _reference(inputs):
+    ref = []
+    for grid, ps in inputs:
+        dev = ns.DeviceSurface(0)
+        dev.set_grid(grid)
+        dev.set_patches(ps)
+        r = dev.get_surf(**OPTS)
+        ref.append((r, dev.epsmap(), dev.idebmap(), dev.status(), dev.cavities()))
+        dev.close()
+    return ref
+
+
+def _check(batch, ref):
+    for f, (r, eps, ideb, st, cav) in enumerate(ref):
+        print("frame %d" % f, flush=True)
+        rb = batch.result(f)
+        for k in ("total_rays", "odd_rays", "total_hits", "candidates", "num_cavities", "max_hits_per_ray", "volume"):
+            assert rb[k] == r[k], (f, k, rb[k], r[k])
+        v = batch.frame(f)
+        assert np.array_equal(v.epsmap(), eps)
+        assert np.array_equal(v.idebmap(), ideb)
+        assert np.array_equal(v.status(), st)
+        sa, fa = v.cavities()
+        assert np.array_equal(sa, cav[0]) and np.array_equal(fa, cav[1])
+
+
+@pytest.mark.parametrize("chains", [0, 2])
+def test_batch_graph_matches_per_frame_get_surf(device_ok, chains):
+    frames = ns.perturbed_frames(6, n_min=150, n_max=1500, seed=3)
+    inputs = _inputs(frames)
+    ref = _reference(inputs)
+    b = ns.Batch(len(inputs))
+    for f, (grid, ps) in enumerate(inputs):
+        b.set_frame(f, grid, ps)
+    used = b.prepare(chains=chains, **OPTS)
+    assert used >= 1
+    # several replays with eager work in between: every replay is checked
+    for rep in range(3):
+        lat, redone = b.run()
+        assert redone == 0
+        assert (lat > 0).all()
+        _check(b, ref)
+        b.run_loop()
+        _check(b, ref)
+    b.close()
+
+
+def test_batch_frame_with_cavities(device_ok):
+    # a frame with buried cavities keeps the reference's labels in the graph
+    xyzr, _ = ns.generate_chain(2000, 21.0, seed=0)
+    inputs = _inputs([xyzr], scale=3.0)
+    ref = _reference(inputs)
+    assert ref[0][0]["num_cavities"] > 0
+    b = ns.Batch(1)
+    b.set_frame(0, *inputs[0])
+    b.prepare(chains=1, **OPTS)
+    b.run()
+    _check(b, ref)
+    b.close()
