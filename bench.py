@@ -8,7 +8,14 @@ jitter, patch construction) run once before timing.
 
 Prints one JSON line: rays/s (all six panels) as the value, with voxels/s,
 stage times, the roofline of the dominant kernel and the CPU baseline (the
-oracle, single threaded, on the same workload) beside it.
+oracle, single threaded, on the same workload) beside it.  Three extensions
+of the plain path are measured alongside:
+* "mixed_precision": the same step with FP32-seeded, FP64-refined torus roots;
+* "batched": 256 frames of 500-8000 atoms (log-uniform, sigma 0.3 A
+  perturbations), 2 points/A, maps + cavities, one captured graph launch vs
+  the per-frame launch loop;
+* "differentiable": 2,000 atoms at 3 points/A, three-panel volume and area
+  estimate with their gradients for all atoms (trace + adjoint kernel).
 """
 import argparse
 import json
@@ -36,6 +43,9 @@ PEAK_HBM_GBS = 8000.0
 # clipping-sphere test (6).
 OPS = {"candidates": 9, "disc_pass": 23, "sphere_hits": 5, "torus_quartics": 160, "roots_tested": 2,
        "feas_items": 6}
+
+# the two launches of the intersection stage (spheres and facets; tori)
+INTERSECT_KERNELS = ("nsg::k_intersect<false, false, false>", "nsg::k_intersect<false, false, true>")
 
 CONFIGS = {
     # name: atoms, confinement radius, grid scale, perfil
@@ -76,6 +86,100 @@ def cpu_baseline(grid, ps, budget_s):
             "sample": "full workload, %d run(s), median %.3f s per step" % (len(times), sec)}
 
 
+def bench_mixed(dev, steps, warmup):
+    """C2 step with the mixed-precision torus roots."""
+    for _ in range(warmup):
+        dev.get_surf(precision=1)
+        dev.triangulate()
+    dev.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dev.get_surf(precision=1)
+        dev.triangulate()
+    dev.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def bench_batched(n_frames=256, repeats=3):
+    """Uneven frames, one captured graph launch vs the per-frame loop."""
+    t0 = time.perf_counter()
+    frames = ns.perturbed_frames(n_frames, seed=0)
+    inputs = []
+    for x in frames:
+        g = ns.build_grid(x, 2.0, 80.0)
+        j, _ = ns.jitter(x)
+        inputs.append((g, ns.ses_patches(j, 1.4)))
+    host_s = time.perf_counter() - t0
+    b = ns.Batch(n_frames)
+    for f, (g, ps) in enumerate(inputs):
+        b.set_frame(f, g, ps)
+    chains = b.prepare(chains=0)
+    best, lat_best, redone = 1e30, None, 0
+    for _ in range(repeats):
+        t = time.perf_counter()
+        lat, redone = b.run()
+        t = time.perf_counter() - t
+        if t < best:
+            best, lat_best = t, lat
+    loop_best, loop_lat = 1e30, None
+    for _ in range(repeats):
+        t = time.perf_counter()
+        lat = b.run_loop()
+        t = time.perf_counter() - t
+        if t < loop_best:
+            loop_best, loop_lat = t, lat
+    nvox = sum(g.nx * g.ny * g.nz for g, _ in inputs)
+    b.close()
+    return {"workload": "%d frames, %d-%d atoms (log-uniform), sigma 0.3 A, probe 1.4 A, 2.0 points/A, "
+                        "epsmap+idebmap+status+cavities, 3 panels" % (n_frames, min(map(len, frames)),
+                                                                       max(map(len, frames))),
+            "frames_per_s": n_frames / best, "ms_per_batch": best * 1e3,
+            "p50_ms": float(np.percentile(lat_best, 50)), "p99_ms": float(np.percentile(lat_best, 99)),
+            "latency": "completion of each frame after the launch (device clock)",
+            "loop_frames_per_s": n_frames / loop_best, "loop_p50_ms": float(np.percentile(loop_lat, 50)),
+            "loop_p99_ms": float(np.percentile(loop_lat, 99)), "speedup_vs_loop": loop_best / best,
+            "chains": chains, "frames_redone": redone, "voxels": nvox, "host_build_s": host_s}
+
+
+def bench_differentiable(steps=10):
+    """2,000 atoms at 3 points/A: forward + adjoint, device part and through autograd."""
+    xyzr, _ = ns.generate_chain(2000, 21.0, seed=0)
+    grid = ns.build_grid(xyzr, 3.0, 80.0)
+    jit, _ = ns.jitter(xyzr)
+    dev = ns.DeviceSurface(0)
+    dev.set_grid(grid)
+    dev.set_patches(ns.ses_patches(jit, 1.4))
+    dev.set_atoms(jit)
+    out = dev.surface_gradient()
+    dev.set_profiling(True)
+    t0 = time.perf_counter()
+    st = {"intersect": 0.0, "sort": 0.0, "adjoint": 0.0}
+    for _ in range(steps):
+        out = dev.surface_gradient()
+        for k, v in dev.stage_times().items():
+            if k in st:
+                st[k] += v / steps
+    dev_ms = (time.perf_counter() - t0) * 1e3 / steps
+    dev.set_profiling(False)
+    res = {"workload": "2000-atom chain, probe 1.4 A, 3.0 points/A (%d^3), FP64; volume and area estimate, "
+                       "gradients for %d coordinates + %d radii" % (grid.nx, 3 * len(jit), len(jit)),
+           "fwd_bwd_ms": dev_ms, "stage_ms": st, "volume": out["volume"], "area": out["area"]}
+    try:
+        import torch
+        from nanoshaper_amd.autograd import surface_volume_area
+        x = torch.tensor(jit, dtype=torch.float64, requires_grad=True)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            x.grad = None
+            v, a = surface_volume_area(x, grid, 1.4, dev)
+            (v + a).backward()
+        res["autograd_fwd_bwd_ms_incl_host_patches"] = (time.perf_counter() - t0) * 1e3 / 3
+    except ImportError:
+        pass
+    dev.close()
+    return res
+
+
 def copy_reference(nbytes, repeats=5):
     """Plain device-to-device copy of known size: the calibration kernel for
     the FETCH_SIZE / WRITE_SIZE counters (a copy reads and writes nbytes)."""
@@ -103,6 +207,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", type=int, default=0)
+    ap.add_argument("--no-extras", action="store_true", help="skip the mixed, batched and differentiable lines")
     args = ap.parse_args()
     if args.copy_ref:
         copy_reference(args.copy_ref)
@@ -153,17 +258,34 @@ def main():
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "ses_c2_5000atoms_traffic.json")
     if args.config == "c2" and args.precision == 0 and os.path.exists(tfile):
-        t = json.load(open(tfile)).get("kernels", {}).get("nsg::k_intersect<false>")
-        if t:
-            traffic = t["fetch_bytes"] + t["write_bytes"]
-    roofline = {"kernel": "k_intersect", "bound": "fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+        kern = json.load(open(tfile)).get("kernels", {})
+        parts = [kern.get(k) for k in INTERSECT_KERNELS]
+        if all(parts):
+            traffic = sum(t["fetch_bytes"] + t["write_bytes"] for t in parts)
+    roofline = {"kernel": "k_intersect (plain and torus launches, one stage)", "bound": "fp64",
+                "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                 "flops_per_launch": flops, "path_counts": pc}
     fill_bytes = 17.0 * nvox
     t_fill = stage_avg["fill"] * 1e-3
-    fill_roof = {"kernel": "k_fill_maps", "bound": "hbm", "bytes_per_launch": fill_bytes,
+    fill_roof = {"kernel": "k_fill_tiled", "bound": "hbm", "bytes_per_launch": fill_bytes,
                  "achieved_GBs": fill_bytes / t_fill / 1e9 if t_fill > 0 else 0.0, "peak_GBs": PEAK_HBM_GBS}
     fill_roof["frac"] = fill_roof["achieved_GBs"] / PEAK_HBM_GBS
+
+    extras = {}
+    if not args.no_extras and args.config == "c2":
+        print("mixed precision ...", file=sys.stderr, flush=True)
+        mixed_ms = bench_mixed(dev, args.steps, args.warmup)
+        extras["mixed_precision"] = {"ms_per_step": mixed_ms, "value": rays / (mixed_ms * 1e-3),
+                                     "speedup_vs_fp64": ms_per_step / mixed_ms,
+                                     "hit_error": "<= 1e-6 A against the true roots of the same quartics "
+                                                  "(tests/test_gpu_parity.py::test_mixed_precision_matches_oracle)"}
+    dev.close()
+    if not args.no_extras:
+        print("batched frames ...", file=sys.stderr, flush=True)
+        extras["batched"] = bench_batched()
+        print("differentiable ...", file=sys.stderr, flush=True)
+        extras["differentiable"] = bench_differentiable()
 
     cpu = None
     if not args.no_cpu:
@@ -198,6 +320,7 @@ def main():
         "result": {"volume": r["volume"], "area": m["area"], "cavities": r["num_cavities"],
                    "triangles": m["num_triangles"], "hits": r["total_hits"], "odd_rays": r["odd_rays"]},
     }
+    out.update(extras)
     print(json.dumps(out))
 
 

@@ -94,6 +94,7 @@ struct nsgpu_ctx {
     bool capturing = false;  // enqueueing into a graph: no stage events
     std::vector<double> cav_size;
     std::vector<uint8_t> cav_filled;
+    bool cav_on_device = false;  // cav_size/cav_filled not yet copied from the device
     // vertices and mesh
     DBuf idx, vcount, voffs, scan_scratch, verts, tris, tarea;
     DBuf segmask, anode, acase, anv, ant, avoff, atoff;
@@ -694,6 +695,7 @@ static int finish_surf(nsgpu_ctx *c) {
     c->res.volume = c->hres->volume();
     c->cav_size.clear();
     c->cav_filled.clear();
+    c->cav_on_device = false;
     c->res.num_cavities = 0;
     if (c->opts.fill_cavities && c->opts.build_status) {
         const int *info = c->hres->cav_info();
@@ -701,17 +703,9 @@ static int finish_surf(nsgpu_ctx *c) {
             const int rc = cavities_on_host(c);
             if (rc) return rc;
         } else {
-            const int ncav = info[1];
-            c->res.num_cavities = ncav;
-            c->cav_size.resize(ncav);
-            c->cav_filled.resize(ncav);
-            if (ncav) {
-                HIP_TRY(c, hipMemcpyAsync(c->cav_size.data(), c->dcavsize.p, (size_t)ncav * 8, hipMemcpyDeviceToHost,
-                                          c->stream));
-                HIP_TRY(c, hipMemcpyAsync(c->cav_filled.data(), c->dcavfilled.p, (size_t)ncav, hipMemcpyDeviceToHost,
-                                          c->stream));
-                HIP_TRY(c, hipStreamSynchronize(c->stream));
-            }
+            // sizes and flags stay on the device until nsgpu_read_cavities asks
+            c->res.num_cavities = info[1];
+            c->cav_on_device = true;
         }
     }
     return NSGPU_OK;
@@ -894,6 +888,20 @@ int nsgpu_read_corner_bits(nsgpu_ctx *c, uint32_t *out) {
 
 int nsgpu_read_cavities(nsgpu_ctx *c, double *sizes, uint8_t *filled) {
     if (!c) return NSGPU_E_ARG;
+    if (c->cav_on_device) {
+        const int ncav = c->res.num_cavities;
+        c->cav_size.resize(ncav);
+        c->cav_filled.resize(ncav);
+        if (ncav) {
+            HIP_TRY(c, hipSetDevice(c->device));
+            HIP_TRY(c, hipMemcpyAsync(c->cav_size.data(), c->dcavsize.p, (size_t)ncav * 8, hipMemcpyDeviceToHost,
+                                      c->stream));
+            HIP_TRY(c, hipMemcpyAsync(c->cav_filled.data(), c->dcavfilled.p, (size_t)ncav, hipMemcpyDeviceToHost,
+                                      c->stream));
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+        }
+        c->cav_on_device = false;
+    }
     for (size_t q = 0; q < c->cav_size.size(); ++q) {
         if (sizes) sizes[q] = c->cav_size[q];
         if (filled) filled[q] = c->cav_filled[q];
