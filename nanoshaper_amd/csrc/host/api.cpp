@@ -97,6 +97,10 @@ struct nsgpu_ctx {
     bool cav_on_device = false;  // cav_size/cav_filled not yet copied from the device
     // vertices and mesh
     DBuf idx, vcount, voffs, scan_scratch, verts, tris, tarea;
+    // idx holds -1 everywhere except at the vslot[0 .. idx_dirty) slots written
+    // last, for grids of idx_nvox nodes (-1: state unknown, fill it)
+    DBuf vslot;
+    int64_t idx_nvox = -1, idx_dirty = 0;
     DBuf segmask, anode, acase, anv, ant, avoff, atoff;
     int64_t n_analytic = 0, nverts = 0, ntris = 0;
     double area = 0;
@@ -202,7 +206,7 @@ void nsgpu_destroy(nsgpu_ctx *c) {
     DBuf *all[] = {&c->gx, &c->gy, &c->gz, &c->pkind, &c->poff, &c->pcnt, &c->pflags, &c->pbsph, &c->pdata, &c->porder, &c->patom, &c->atoms, &c->rpid, &c->gradv, &c->grada, &c->asums,
                    &c->rcnt, &c->rt, &c->rvol, &c->partial, &c->stats, &c->cand, &c->seg, &c->segst, &c->eps, &c->ideb, &c->status,
                    &c->bits, &c->crx0, &c->crx1, &c->cpar, &c->crowcnt, &c->croff, &c->nroots, &c->rootlist, &c->bkey, &c->ccount, &c->clabel, &c->cfilled, &c->dcavsize, &c->dcavfilled, &c->hostdone, &c->dres,
-                   &c->idx, &c->vcount, &c->voffs, &c->scan_scratch, &c->verts, &c->tris, &c->tarea,
+                   &c->idx, &c->vslot, &c->vcount, &c->voffs, &c->scan_scratch, &c->verts, &c->tris, &c->tarea,
                    &c->segmask, &c->anode, &c->acase, &c->anv, &c->ant, &c->avoff, &c->atoff};
     for (DBuf *b : all) b->release();
     for (auto &e : c->ev)
@@ -730,11 +734,19 @@ static int assemble(nsgpu_ctx *c, const int eslots[3]) {
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
     const int64_t nrays = 3 * (int64_t)R.N * R.N;
     const int64_t nmax = nrays;
+    const void *old_idx = c->idx.p;
     HIP_TRY(c, c->idx.ensure((size_t)nvox * 3 * 4));
     HIP_TRY(c, c->vcount.ensure((size_t)nmax * 4));
     HIP_TRY(c, c->voffs.ensure((size_t)nmax * 4));
     HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(nmax) * 4));
-    launch_fill32(c->stream, c->idx.p, 0xffffffffu, nvox * 3);
+    // vertex index slots back to -1: only the ones the last vertices used,
+    // unless the buffer is new, the grid changed or slots exceed int32
+    if (c->idx.p != old_idx || c->idx_nvox != nvox || 3 * nvox > INT32_MAX)
+        launch_fill32(c->stream, c->idx.p, 0xffffffffu, nvox * 3);
+    else
+        launch_idx_reset(c->stream, c->idx.as<int>(), c->vslot.as<int>(), c->idx_dirty);
+    c->idx_nvox = nvox;
+    c->idx_dirty = 0;
     launch_vtx_count(c->stream, g, R, eslots, c->bits.as<uint32_t>(), c->vcount.as<int>());
     launch_scan(c->stream, c->vcount.as<int>(), nrays, c->voffs.as<int>(), c->scan_scratch.as<int>());
     HIP_TRY(c, hipGetLastError());
@@ -742,9 +754,11 @@ static int assemble(nsgpu_ctx *c, const int eslots[3]) {
     int rc = read_total(c, c->vcount.as<int>(), c->voffs.as<int>(), nrays, &nv);
     if (rc) return rc;
     HIP_TRY(c, c->verts.ensure((size_t)std::max<int64_t>(nv, 1) * 24));
+    HIP_TRY(c, c->vslot.ensure((size_t)std::max<int64_t>(nv, 1) * 4));
     launch_vtx_write(c->stream, g, R, eslots, c->bits.as<uint32_t>(), c->voffs.as<int>(), c->verts.as<double>(),
-                     c->idx.as<int>());
+                     c->idx.as<int>(), c->vslot.as<int>());
     HIP_TRY(c, hipGetLastError());
+    c->idx_dirty = nv;
     c->n_analytic = nv;
     c->nverts = nv;
     return NSGPU_OK;
@@ -837,9 +851,19 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
         c->verts.release();
         c->verts = nb;
     }
+    if ((size_t)nv * 4 > c->vslot.cap) {
+        DBuf nb;
+        HIP_TRY(c, nb.ensure((size_t)nv * 4));
+        if (c->n_analytic)
+            HIP_TRY(c, hipMemcpyAsync(nb.p, c->vslot.p, (size_t)c->n_analytic * 4, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        c->vslot.release();
+        c->vslot = nb;
+    }
     // 3. write the mid points, then the triangles that may reference them
     launch_mc_vwrite(c->stream, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->idx.as<int>(),
-                     c->avoff.as<int>(), (int)c->n_analytic, c->verts.as<double>());
+                     c->avoff.as<int>(), (int)c->n_analytic, c->verts.as<double>(), c->vslot.as<int>());
+    c->idx_dirty = nv;
     HIP_TRY(c, c->tris.ensure((size_t)std::max<int64_t>(nt, 1) * 12));
     HIP_TRY(c, c->tarea.ensure((size_t)std::max<int64_t>(nt, 1) * 8));
     launch_mc_twrite(c->stream, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->idx.as<int>(),

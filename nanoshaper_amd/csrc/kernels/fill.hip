@@ -124,42 +124,68 @@ __global__ void __launch_bounds__(256) k_fill_corners(DevGrid g, DevRays R, Fill
 // epsmap, idebmap and status for those nodes once, coalesced.
 constexpr int TI = 64, TJ = 8, TK = 8;
 
+// word q of a record at rec[q * stride]
+template <int kStride>
 __device__ __forceinline__ bool rec_has(const int *rec, int i) {
     const int n = rec[0];
     for (int q = 0; q < n; ++q) {
-        const unsigned int w = (unsigned int)rec[1 + q];
+        const unsigned int w = (unsigned int)rec[(1 + q) * kStride];
         if (i >= (int)(w & 0xffffu) && i <= (int)(w >> 16)) return true;
     }
     return false;
 }
 
+// LDS copy of the records of the rays (outer0+o)*N + inner0+i of one slot
+// (zeros outside the grid), read with two 16-byte loads per record.
+// kTransposed: word q of ray (o, i) at dst[(o*8 + q)*ninner + i], so that 64
+// lanes reading 64 consecutive rays hit 64 consecutive words; otherwise
+// dst[(o*ninner + i)*8 + q] (records read wave-uniformly).
+template <bool kTransposed>
 __device__ __forceinline__ void stage_records(int *dst, const int *src, int slot, int64_t NN, int N, int outer0,
                                               int nouter, int inner0, int ninner, int outer_lim, int inner_lim) {
-    // dst[o][i][q] <- record of ray (outer0+o)*N + inner0+i in slot; 0 outside the grid
-    const int total = nouter * ninner * kSegWords;
+    static_assert(kSegWords == 8, "records are two int4");
+    const int total = nouter * ninner * 2;
+    const int4 *s4 = reinterpret_cast<const int4 *>(src);
     for (int w = threadIdx.x; w < total; w += blockDim.x) {
-        const int o = w / (ninner * kSegWords), rem = w - o * (ninner * kSegWords);
-        const int ii = rem / kSegWords, q = rem - ii * kSegWords;
+        int o, ii, h;
+        if (kTransposed) {  // consecutive threads take consecutive rays
+            o = w / (2 * ninner);
+            const int rem = w - o * 2 * ninner;
+            h = rem / ninner;
+            ii = rem - h * ninner;
+        } else {
+            o = w / (ninner * 2);
+            const int rem = w - o * (ninner * 2);
+            ii = rem >> 1;
+            h = rem & 1;
+        }
         const int oo = outer0 + o, ix = inner0 + ii;
-        int v = 0;
-        if (slot >= 0 && oo < outer_lim && ix < inner_lim)
-            v = src[((int64_t)slot * NN + (int64_t)oo * N + ix) * kSegWords + q];
-        dst[w] = v;
+        int4 v = make_int4(0, 0, 0, 0);
+        if (slot >= 0 && oo < outer_lim && ix < inner_lim) v = s4[((int64_t)slot * NN + (int64_t)oo * N + ix) * 2 + h];
+        if (kTransposed) {
+            int *d = dst + (o * 8 + 4 * h) * ninner + ii;
+            d[0] = v.x;
+            d[ninner] = v.y;
+            d[2 * ninner] = v.z;
+            d[3 * ninner] = v.w;
+        } else {
+            reinterpret_cast<int4 *>(dst)[(o * ninner + ii) * 2 + h] = v;
+        }
     }
 }
 
 __global__ void __launch_bounds__(256) k_fill_tiled(DevGrid g, DevRays R, FillOpts o, int *epsmap, uint8_t *idebmap,
                                                     int *status) {
-    __shared__ int zrec[TJ * TI * kSegWords];
-    __shared__ int yrec[TK * TI * kSegWords];
-    __shared__ int xrec[TK * TJ * kSegWords];
-    __shared__ int xst[TK * TJ * kSegWords];
+    __shared__ __align__(16) int zrec[TJ * TI * kSegWords];
+    __shared__ __align__(16) int yrec[TK * TI * kSegWords];
+    __shared__ __align__(16) int xrec[TK * TJ * kSegWords];
+    __shared__ __align__(16) int xst[TK * TJ * kSegWords];
     const int i0 = blockIdx.x * TI, j0 = blockIdx.y * TJ, k0 = blockIdx.z * TK;
     const int64_t NN = (int64_t)R.N * R.N;
-    stage_records(zrec, R.seg, o.build_eps ? o.slot_z : -1, NN, R.N, j0, TJ, i0, TI, g.ny, g.nx);
-    stage_records(yrec, R.seg, o.build_eps ? o.slot_y : -1, NN, R.N, k0, TK, i0, TI, g.nz, g.nx);
-    stage_records(xrec, R.seg, o.slot_x, NN, R.N, k0, TK, j0, TJ, g.nz, g.ny);
-    stage_records(xst, R.seg_st, o.slot_x >= 0 ? 0 : -1, 0, R.N, k0, TK, j0, TJ, g.nz, g.ny);
+    stage_records<true>(zrec, R.seg, o.build_eps ? o.slot_z : -1, NN, R.N, j0, TJ, i0, TI, g.ny, g.nx);
+    stage_records<true>(yrec, R.seg, o.build_eps ? o.slot_y : -1, NN, R.N, k0, TK, i0, TI, g.nz, g.nx);
+    stage_records<false>(xrec, R.seg, o.slot_x, NN, R.N, k0, TK, j0, TJ, g.nz, g.ny);
+    stage_records<false>(xst, R.seg_st, o.slot_x >= 0 ? 0 : -1, 0, R.N, k0, TK, j0, TJ, g.nz, g.ny);
     __syncthreads();
     const int ii = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int i = i0 + ii;
@@ -172,8 +198,8 @@ __global__ void __launch_bounds__(256) k_fill_tiled(DevGrid g, DevRays R, FillOp
         if (o.slot_x >= 0) {
             const int *xe = xrec + (kk * TJ + jj) * kSegWords, *xs = xst + (kk * TJ + jj) * kSegWords;
             if (xe[0] >= 0 && xs[0] >= 0) {
-                e0 = rec_has(xe, i);
-                in = rec_has(xs, i);
+                e0 = rec_has<1>(xe, i);
+                in = rec_has<1>(xs, i);
             } else {
                 grid_ray<true, true>(R, o.slot_x * NN + (int64_t)k * R.N + j, g.x, g.nx, g.xmin, g.scale, g.hside, i,
                                      e0, in);
@@ -183,14 +209,14 @@ __global__ void __launch_bounds__(256) k_fill_tiled(DevGrid g, DevRays R, FillOp
         if (o.build_eps) {
             bool dummy = false;
             if (o.slot_z >= 0) {
-                const int *zr = zrec + (jj * TI + ii) * kSegWords;
-                if (zr[0] >= 0) e2 = rec_has(zr, k);
+                const int *zr = zrec + jj * 8 * TI + ii;
+                if (zr[0] >= 0) e2 = rec_has<TI>(zr, k);
                 else grid_ray<false, false>(R, o.slot_z * NN + (int64_t)j * R.N + i, g.z, g.nz, g.zmin, g.scale,
                                             g.hside, k, e2, dummy);
             }
             if (o.slot_y >= 0) {
-                const int *yr = yrec + (kk * TI + ii) * kSegWords;
-                if (yr[0] >= 0) e1 = rec_has(yr, j);
+                const int *yr = yrec + kk * 8 * TI + ii;
+                if (yr[0] >= 0) e1 = rec_has<TI>(yr, j);
                 else grid_ray<false, false>(R, o.slot_y * NN + (int64_t)k * R.N + i, g.y, g.ny, g.ymin, g.scale,
                                             g.hside, j, e1, dummy);
             }
@@ -208,14 +234,14 @@ __global__ void __launch_bounds__(256) k_fill_tiled(DevGrid g, DevRays R, FillOp
 // and each 64-node row segment clears its inside bits with at most three
 // atomicAnd on the words it overlaps.
 __global__ void __launch_bounds__(256) k_corners_tiled(DevGrid g, DevRays R, FillOpts o, uint32_t *bits) {
-    __shared__ int zrec[TJ * TI * kSegWords];
-    __shared__ int yrec[TK * TI * kSegWords];
-    __shared__ int xrec[TK * TJ * kSegWords];
+    __shared__ __align__(16) int zrec[TJ * TI * kSegWords];
+    __shared__ __align__(16) int yrec[TK * TI * kSegWords];
+    __shared__ __align__(16) int xrec[TK * TJ * kSegWords];
     const int i0 = blockIdx.x * TI, j0 = blockIdx.y * TJ, k0 = blockIdx.z * TK;
     const int64_t NN = (int64_t)R.N * R.N;
-    stage_records(zrec, R.seg, o.eslot_z, NN, R.N, j0, TJ, i0, TI, g.ny, g.nx);
-    stage_records(yrec, R.seg, o.eslot_y, NN, R.N, k0, TK, i0, TI, g.nz, g.nx);
-    stage_records(xrec, R.seg, o.eslot_x, NN, R.N, k0, TK, j0, TJ, g.nz, g.ny);
+    stage_records<true>(zrec, R.seg, o.eslot_z, NN, R.N, j0, TJ, i0, TI, g.ny, g.nx);
+    stage_records<true>(yrec, R.seg, o.eslot_y, NN, R.N, k0, TK, i0, TI, g.nz, g.nx);
+    stage_records<false>(xrec, R.seg, o.eslot_x, NN, R.N, k0, TK, j0, TJ, g.nz, g.ny);
     __syncthreads();
     const int ii = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int i = i0 + ii;
@@ -226,15 +252,15 @@ __global__ void __launch_bounds__(256) k_corners_tiled(DevGrid g, DevRays R, Fil
         bool inside = false;
         if (i < g.nx) {
             const int *xr = xrec + (kk * TJ + jj) * kSegWords;
-            const int *zr = zrec + (jj * TI + ii) * kSegWords;
-            const int *yr = yrec + (kk * TI + ii) * kSegWords;
-            inside = (xr[0] >= 0 ? rec_has(xr, i)
+            const int *zr = zrec + jj * 8 * TI + ii;
+            const int *yr = yrec + kk * 8 * TI + ii;
+            inside = (xr[0] >= 0 ? rec_has<1>(xr, i)
                                  : edge_ray(R, o.eslot_x * NN + (int64_t)k * R.N + j, g.x, g.nx, g.xmin, g.scale,
                                             g.hside, i)) ||
-                     (zr[0] >= 0 ? rec_has(zr, k)
+                     (zr[0] >= 0 ? rec_has<TI>(zr, k)
                                  : edge_ray(R, o.eslot_z * NN + (int64_t)j * R.N + i, g.z, g.nz, g.zmin, g.scale,
                                             g.hside, k)) ||
-                     (yr[0] >= 0 ? rec_has(yr, j)
+                     (yr[0] >= 0 ? rec_has<TI>(yr, j)
                                  : edge_ray(R, o.eslot_y * NN + (int64_t)k * R.N + i, g.y, g.ny, g.ymin, g.scale,
                                             g.hside, j));
         }

@@ -80,8 +80,10 @@ int64_t scan_scratch_ints(int64_t n);
 void launch_scan(hipStream_t, const int *, int64_t, int *, int *);
 void mc_upload_tables(hipStream_t);
 void launch_vtx_count(hipStream_t, const DevGrid &, const DevRays &, const int[3], const uint32_t *, int *);
+// vslot[v] receives the index slot of vertex v (for launch_idx_reset)
 void launch_vtx_write(hipStream_t, const DevGrid &, const DevRays &, const int[3], const uint32_t *, const int *,
-                      double *, int *);
+                      double *, int *, int *vslot);
+void launch_idx_reset(hipStream_t, int *idx, const int *vslot, int64_t n);
 int64_t mc_num_segments(const DevGrid &);
 void launch_mc_classify(hipStream_t, const DevGrid &, const uint32_t *bits, uint32_t *segmask, int *segcnt);
 void launch_mc_compact(hipStream_t, const DevGrid &, const uint32_t *bits, const uint32_t *segmask, const int *segoff,
@@ -89,7 +91,7 @@ void launch_mc_compact(hipStream_t, const DevGrid &, const uint32_t *bits, const
 void launch_mc_counts(hipStream_t, const DevGrid &, int nact, const int *anode, const uint8_t *acase, const int *idx,
                       int *nv, int *nt);
 void launch_mc_vwrite(hipStream_t, const DevGrid &, int nact, const int *anode, const uint8_t *acase, int *idx,
-                      const int *voff, int base, double *verts);
+                      const int *voff, int base, double *verts, int *vslot);
 void launch_mc_twrite(hipStream_t, const DevGrid &, int nact, const int *anode, const uint8_t *acase, const int *idx,
                       const int *toff, const double *verts, int *tris, double *tarea);
 

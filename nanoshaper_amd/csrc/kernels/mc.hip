@@ -108,7 +108,7 @@ __global__ void __launch_bounds__(256) k_vtx_count(DevGrid g, DevRays R, EdgeSlo
 }
 
 __global__ void __launch_bounds__(256) k_vtx_write(DevGrid g, DevRays R, EdgeSlots es, const uint32_t *bits,
-                                                   const int *offs, double *verts, int *idx) {
+                                                   const int *offs, double *verts, int *idx, int *vslot) {
     const int64_t NN = (int64_t)R.N * R.N;
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= 3 * NN) return;
@@ -121,7 +121,8 @@ __global__ void __launch_bounds__(256) k_vtx_write(DevGrid g, DevRays R, EdgeSlo
                       verts[3 * (int64_t)v + 0] = px;
                       verts[3 * (int64_t)v + 1] = py;
                       verts[3 * (int64_t)v + 2] = pz;
-                      idx[d * nvox + f] = v;
+                      idx[3 * f + d] = v;
+                      vslot[v] = (int)(3 * f + d);
                   });
 }
 
@@ -143,21 +144,23 @@ __device__ __forceinline__ bool edge_on(int c, int e) {
     return ((c >> a) & 1) != ((c >> b) & 1);
 }
 
-// edge e of the cube at node f -> vertex index slot: direction and node offset (getTriangles)
-__device__ __forceinline__ int64_t edge_slot(int e, int64_t f, int64_t sy, int64_t sz, int64_t nvox) {
+// edge e of the cube at node f -> vertex index slot (getTriangles): the three
+// edge directions of a node are interleaved, idx[3*node + direction], so the
+// twelve edges of a cube lie in four short runs of memory
+__device__ __forceinline__ int64_t edge_slot(int e, int64_t f, int64_t sy, int64_t sz, int64_t) {
     switch (e) {
-        case 0: return 0 * nvox + f;
-        case 1: return 1 * nvox + f + 1;
-        case 2: return 0 * nvox + f + sy;
-        case 3: return 1 * nvox + f;
-        case 4: return 0 * nvox + f + sz;
-        case 5: return 1 * nvox + f + 1 + sz;
-        case 6: return 0 * nvox + f + sy + sz;
-        case 7: return 1 * nvox + f + sz;
-        case 8: return 2 * nvox + f;
-        case 9: return 2 * nvox + f + 1;
-        case 10: return 2 * nvox + f + 1 + sy;
-        default: return 2 * nvox + f + sy;
+        case 0: return 3 * f + 0;
+        case 1: return 3 * (f + 1) + 1;
+        case 2: return 3 * (f + sy) + 0;
+        case 3: return 3 * f + 1;
+        case 4: return 3 * (f + sz) + 0;
+        case 5: return 3 * (f + 1 + sz) + 1;
+        case 6: return 3 * (f + sy + sz) + 0;
+        case 7: return 3 * (f + sz) + 1;
+        case 8: return 3 * f + 2;
+        case 9: return 3 * (f + 1) + 2;
+        case 10: return 3 * (f + 1 + sy) + 2;
+        default: return 3 * (f + sy) + 2;
     }
 }
 
@@ -298,7 +301,7 @@ __global__ void __launch_bounds__(256) k_mc_counts(DevGrid g, int nact, const in
 }
 
 __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const int *anode, const uint8_t *acase,
-                                                   int *idx, const int *voff, int base, double *verts) {
+                                                   int *idx, const int *voff, int base, double *verts, int *vslot) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nact) return;
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (int64_t)g.nx * g.ny;
@@ -316,18 +319,21 @@ __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const in
         verts[3 * (int64_t)v] = g.x[i + 1] - g.hside;
         verts[3 * (int64_t)v + 1] = g.y[j];
         verts[3 * (int64_t)v + 2] = g.z[k + 1] - g.hside;
+        vslot[v] = (int)s5;
         idx[s5] = v++;
     }
     if (m6) {
         verts[3 * (int64_t)v] = g.x[i];
         verts[3 * (int64_t)v + 1] = g.y[j + 1] - g.hside;
         verts[3 * (int64_t)v + 2] = g.z[k + 1] - g.hside;
+        vslot[v] = (int)s6;
         idx[s6] = v++;
     }
     if (m10) {
         verts[3 * (int64_t)v] = g.x[i + 1] - g.hside;
         verts[3 * (int64_t)v + 1] = g.y[j + 1] - g.hside;
         verts[3 * (int64_t)v + 2] = g.z[k];
+        vslot[v] = (int)s10;
         idx[s10] = v++;
     }
 }
@@ -379,10 +385,20 @@ void launch_vtx_count(hipStream_t s, const DevGrid &g, const DevRays &R, const i
 }
 
 void launch_vtx_write(hipStream_t s, const DevGrid &g, const DevRays &R, const int eslots[3], const uint32_t *bits,
-                      const int *offs, double *verts, int *idx) {
+                      const int *offs, double *verts, int *idx, int *vslot) {
     EdgeSlots es{{eslots[0], eslots[1], eslots[2]}};
     const int64_t n = 3 * (int64_t)R.N * R.N;
-    hipLaunchKernelGGL(k_vtx_write, dim3(nblk(n)), dim3(256), 0, s, g, R, es, bits, offs, verts, idx);
+    hipLaunchKernelGGL(k_vtx_write, dim3(nblk(n)), dim3(256), 0, s, g, R, es, bits, offs, verts, idx, vslot);
+}
+
+// back to "no vertex" for the slots the last vertices were written to
+__global__ void __launch_bounds__(256) k_idx_reset(int *idx, const int *vslot, int64_t n) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) idx[vslot[v]] = -1;
+}
+
+void launch_idx_reset(hipStream_t s, int *idx, const int *vslot, int64_t n) {
+    if (n > 0) hipLaunchKernelGGL(k_idx_reset, dim3(nblk(n)), dim3(256), 0, s, idx, vslot, n);
 }
 
 static SegIter seg_iter(const DevGrid &g) {
@@ -416,10 +432,10 @@ void launch_mc_counts(hipStream_t s, const DevGrid &g, int nact, const int *anod
 }
 
 void launch_mc_vwrite(hipStream_t s, const DevGrid &g, int nact, const int *anode, const uint8_t *acase, int *idx,
-                      const int *voff, int base, double *verts) {
+                      const int *voff, int base, double *verts, int *vslot) {
     if (nact)
         hipLaunchKernelGGL(k_mc_vwrite, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, idx, voff, base,
-                           verts);
+                           verts, vslot);
 }
 
 void launch_mc_twrite(hipStream_t s, const DevGrid &g, int nact, const int *anode, const uint8_t *acase,
