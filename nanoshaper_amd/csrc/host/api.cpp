@@ -81,7 +81,7 @@ struct nsgpu_ctx {
     bool record_pid = false;
     // rays
     int K = 16;
-    DBuf rcnt, rt, rvol, partial, stats, cand, seg, segst;
+    DBuf rcnt, rt, rvol, partial, stats, cand, cov, covst;
     int nslots = 0, panels[2] = {0, 0};
     // maps
     DBuf eps, ideb, status, bits;
@@ -158,8 +158,9 @@ static DevRays dev_rays(const nsgpu_ctx *c) {
     r.cnt = c->rcnt.as<int>();
     r.t = c->rt.as<double>();
     r.rvol = c->rvol.as<double>();
-    r.seg = c->seg.as<int>();
-    r.seg_st = c->segst.as<int>();
+    r.W = (r.N + 63) / 64;
+    r.cov = c->cov.as<unsigned long long>();
+    r.cov_st = c->covst.as<unsigned long long>();
     r.pid = c->record_pid ? c->rpid.as<int>() : nullptr;
     return r;
 }
@@ -204,7 +205,7 @@ void nsgpu_destroy(nsgpu_ctx *c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     DBuf *all[] = {&c->gx, &c->gy, &c->gz, &c->pkind, &c->poff, &c->pcnt, &c->pflags, &c->pbsph, &c->pdata, &c->porder, &c->patom, &c->atoms, &c->rpid, &c->gradv, &c->grada, &c->asums,
-                   &c->rcnt, &c->rt, &c->rvol, &c->partial, &c->stats, &c->cand, &c->seg, &c->segst, &c->eps, &c->ideb, &c->status,
+                   &c->rcnt, &c->rt, &c->rvol, &c->partial, &c->stats, &c->cand, &c->cov, &c->covst, &c->eps, &c->ideb, &c->status,
                    &c->bits, &c->crx0, &c->crx1, &c->cpar, &c->crowcnt, &c->croff, &c->nroots, &c->rootlist, &c->bkey, &c->ccount, &c->clabel, &c->cfilled, &c->dcavsize, &c->dcavfilled, &c->hostdone, &c->dres,
                    &c->idx, &c->vslot, &c->vcount, &c->voffs, &c->scan_scratch, &c->verts, &c->tris, &c->tarea,
                    &c->segmask, &c->anode, &c->acase, &c->anv, &c->ant, &c->avoff, &c->atoff};
@@ -395,8 +396,9 @@ static int enqueue_trace(nsgpu_ctx *c, hipStream_t st) {
     P.atom = c->patom.as<int>();
     HIP_TRY(c, c->rcnt.ensure(nrays * 4));
     HIP_TRY(c, c->rvol.ensure(nrays * 8));
-    HIP_TRY(c, c->seg.ensure((size_t)nrays * kSegWords * 4));
-    HIP_TRY(c, c->segst.ensure((size_t)N * N * kSegWords * 4));
+    const int64_t W = (N + 63) / 64;
+    HIP_TRY(c, c->cov.ensure((size_t)nrays * W * 8));
+    HIP_TRY(c, c->covst.ensure((size_t)N * N * W * 8));
     HIP_TRY(c, c->rt.ensure((size_t)nrays * c->K * 8));
     if (c->record_pid) HIP_TRY(c, c->rpid.ensure((size_t)nrays * c->K * 4));
     const DevRays R = dev_rays(c);
@@ -658,13 +660,13 @@ static int enqueue_surf(nsgpu_ctx *c, hipStream_t st) {
     HIP_TRY(c, c->bits.ensure((size_t)nwords * 4));
     mark(c, ST_FILL, 0, st);
     if (c->panels[0] > 0)
-        launch_fill_tiled(st, g, R, fo, c->eps.as<int>(), c->ideb.as<uint8_t>(), c->status.as<int>());
+        launch_fill_bits(st, g, R, fo, c->eps.as<int>(), c->ideb.as<uint8_t>(), c->status.as<int>());
     HIP_TRY(c, hipGetLastError());
     mark(c, ST_FILL, 1, st);
     mark(c, ST_CORNERS, 0, st);
     if (o->accurate_triangulation) {
         launch_fill32(st, c->bits.p, 0xffffffffu, nwords);
-        launch_corners_tiled(st, g, R, fo, c->bits.as<uint32_t>());
+        launch_corners_bits(st, g, R, fo, c->bits.as<uint32_t>());
     }
     HIP_TRY(c, hipGetLastError());
     mark(c, ST_CORNERS, 1, st);

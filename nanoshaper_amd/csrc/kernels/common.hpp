@@ -41,15 +41,16 @@ struct DevRays {
     int *cnt;         // [nslots*N*N]
     double *t;        // [nslots*N*N*K]
     double *rvol;     // [nslots*N*N] segment length per ray
-    int *seg;         // [nslots*N*N*8] interval record per ray (see kSegWords)
-    int *seg_st;      // [N*N*8] status/idebmap intervals of the x grid rays
+    // Coverage bit planes written by the sort: bit n of a ray's plane is set
+    // when the ray's kept segments cover node n of its axis (the map-fill rule
+    // of its panel and phase).  W = ceil(N/64) words per ray; word w of ray r
+    // of slot s at cov[(s*W + w)*N*N + r], so consecutive rays are
+    // consecutive words.  cov_st: the status/idebmap plane of the x grid rays.
+    int W;
+    unsigned long long *cov;     // [nslots*W*N*N]
+    unsigned long long *cov_st;  // [W*N*N]
     int *pid;         // [nslots*N*N*K] patch of each depth (sorted with it), or null
 };
-
-// Interval record of a ray: word 0 = number of node intervals (-1: more than
-// kSegMax, evaluate from the depths), words 1..kSegMax = lo | hi << 16.
-constexpr int kSegWords = 8;
-constexpr int kSegMax = kSegWords - 1;
 
 // rintp, tools.cpp:43-46
 __device__ __forceinline__ double rintp(double x) { return floor(x + 0.5); }

@@ -20,10 +20,9 @@ void launch_intersect_count(hipStream_t, const DevGrid &, const DevPatches &, co
 void launch_sort(hipStream_t, const DevGrid &, const DevRays &, unsigned long long *part);
 void launch_reduce_cols(hipStream_t, const unsigned long long *in, int64_t nrows, int ncols, unsigned int max_mask,
                         unsigned long long *out);
-void launch_fill_maps(hipStream_t, const DevGrid &, const DevRays &, const FillOpts &, int *, uint8_t *, int *);
-void launch_fill_corners(hipStream_t, const DevGrid &, const DevRays &, const FillOpts &, uint32_t *);
-void launch_fill_tiled(hipStream_t, const DevGrid &, const DevRays &, const FillOpts &, int *, uint8_t *, int *);
-void launch_corners_tiled(hipStream_t, const DevGrid &, const DevRays &, const FillOpts &, uint32_t *);
+void launch_fill_bits(hipStream_t, const DevGrid &, const DevRays &, const FillOpts &, int *, uint8_t *, int *);
+// bits must start at all ones
+void launch_corners_bits(hipStream_t, const DevGrid &, const DevRays &, const FillOpts &, uint32_t *);
 int64_t sum_partials_count(int64_t n);
 void launch_sum_blocks(hipStream_t, const double *, int64_t, double *);
 // x-runs of TEMP_OUT nodes: rows * ccl_row_capacity() slots each of rx0, rx1

@@ -501,6 +501,39 @@ __global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRa
 }
 
 
+// Writes one ray's coverage plane: node intervals arrive in ascending,
+// disjoint order (the segments of a sorted ray), and every one of the W words
+// is stored once, in order.
+struct PlaneWriter {
+    unsigned long long *base;  // word w at base[w * stride]
+    int64_t stride;
+    int W, cw;
+    unsigned long long acc;
+    __device__ PlaneWriter(unsigned long long *b, int64_t s, int w) : base(b), stride(s), W(w), cw(0), acc(0ull) {}
+    __device__ void add(int lo, int hi) {
+        if (lo < cw * 64) lo = cw * 64;  // never reached with disjoint ascending intervals
+        while (lo <= hi) {
+            const int w = lo >> 6;
+            while (cw < w) {
+                base[cw * stride] = acc;
+                acc = 0ull;
+                ++cw;
+            }
+            const int top = min(hi, (w << 6) + 63);
+            const int n = top - lo + 1;
+            acc |= (n >= 64 ? ~0ull : ((1ull << n) - 1ull)) << (lo & 63);
+            lo = top + 1;
+        }
+    }
+    __device__ void finish() {
+        while (cw < W) {
+            base[cw * stride] = acc;
+            acc = 0ull;
+            ++cw;
+        }
+    }
+};
+
 // Per-ray insertion sort of the depths (Surface::reorderPatchIntersections,
 // Surface.cpp:5080-5095), plus the ray's segment-length integral of
 // setVerticesAndGridsWithIntersectionData (Surface.cpp:5265-5290, :5425-5440)
@@ -559,14 +592,14 @@ __global__ void __launch_bounds__(256) k_sort(DevGrid g, DevRays R, unsigned lon
         const double gmin = panel == 0 ? g.xmin : (panel == 1 ? g.zmin : g.ymin);
         // node intervals of the map fill, from the same double comparisons
         // setVerticesAndGridsWithIntersectionData makes per node
-        // (Surface.cpp:5427-5520, :5636-5650, :5736-5750, :5990-6008)
-        int ne = 0, ns = 0;
-        int *E = R.seg + r * kSegWords;
-        int *S = (phase == 0 && panel == 0) ? R.seg_st + (r - (int64_t)slot * nper) * kSegWords : nullptr;
-        auto push = [](int *rec, int &n, int lo, int hi) {
-            if (lo > hi) return;
-            if (n >= 0 && n < kSegMax) rec[1 + n] = (lo & 0xffff) | (hi << 16);
-            n = (n >= 0 && n < kSegMax) ? n + 1 : -1;
+        // (Surface.cpp:5427-5520, :5636-5650, :5736-5750, :5990-6008),
+        // written as coverage bits
+        const int64_t ray = r - (int64_t)slot * nper;
+        PlaneWriter E(R.cov + (int64_t)slot * R.W * nper + ray, nper, R.W);
+        const bool st = phase == 0 && panel == 0;
+        PlaneWriter S(st ? R.cov_st + ray : nullptr, nper, st ? R.W : 0);
+        auto push = [](PlaneWriter &w, int lo, int hi) {
+            if (lo <= hi) w.add(lo, hi);
         };
         if (c <= R.K && (c & 1) == 0) {
             for (int i = 0; i < c; i += 2)
@@ -578,7 +611,7 @@ __global__ void __launch_bounds__(256) k_sort(DevGrid g, DevRays R, unsigned lon
                     const int b = (int)rintp((lim2 - gmin) * g.scale);
                     if (phase == 1) {
                         // corners a < n <= b
-                        if (b > a) push(E, ne, max(a + 1, 0), min(b, na - 1));
+                        if (b > a) push(E, max(a + 1, 0), min(b, na - 1));
                         continue;
                     }
                     const int i1 = a < 0 ? 0 : (a >= na ? na - 1 : a);
@@ -588,17 +621,17 @@ __global__ void __launch_bounds__(256) k_sort(DevGrid g, DevRays R, unsigned lon
                     const int last = panel == 0 ? i2 : i2 - 1;
                     int ie = last;
                     while (ie >= i1 && !(ga[ie] <= lim2 - g.hside)) --ie;
-                    push(E, ne, i1, ie);
-                    if (S) {
+                    push(E, i1, ie);
+                    if (st) {
                         int is = i1, it = i2;
                         while (is <= i2 && !(ga[is] >= lim1)) ++is;
                         while (it >= is && !(ga[it] <= lim2)) --it;
-                        push(S, ns, is, it);
+                        push(S, is, it);
                     }
                 }
         }
-        E[0] = ne;
-        if (S) S[0] = ns;
+        E.finish();
+        S.finish();
         R.rvol[r] = vol;
         odd = (c & 1);
     }
