@@ -489,8 +489,7 @@ static int enqueue_cavities(nsgpu_ctx *c, hipStream_t st) {
     HIP_TRY(c, c->dcavfilled.ensure((size_t)kReplayMax));
     launch_fill32(st, c->nroots.p, 0u, 1);
     const CclRuns runs = ccl_runs(c);
-    launch_ccl(st, g, c->status.as<int>(), runs, c->scan_scratch.as<int>(), c->nroots.as<int>(),
-               c->rootlist.as<int>());
+    launch_ccl(st, g, dev_rays(c), runs, c->scan_scratch.as<int>(), c->nroots.as<int>(), c->rootlist.as<int>());
     launch_ccl_clear_keys(st, c->nroots.as<int>(), c->bkey.as<unsigned int>(), c->ccount.as<int>(), kReplayMax);
     launch_ccl_keys(st, g, runs, c->bkey.as<unsigned int>(), c->ccount.as<int>(), kReplayMax);
     CavityReplay r;

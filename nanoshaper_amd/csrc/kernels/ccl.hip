@@ -54,40 +54,44 @@ __device__ void unite(int *P, int a, int b) {
     }
 }
 
-// One wave per row: the TEMP_OUT bits of up to eight 64-node chunks are
-// loaded together as ballots, then run starts and ends are wave-uniform bit
-// operations.  The k-th start and the k-th end of a row belong to the same run.
-__global__ void __launch_bounds__(256) k_ccl_runs(DevGrid g, const int *status, int cap, int *rx0, int *rx1,
-                                                  int *rowcnt) {
-    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+// One thread per row: the row's TEMP_OUT nodes are the zero bits of the x
+// grid ray's status plane (the fill wrote status = inside exactly where that
+// plane is set), so run starts and ends come from 64-node words with bit
+// operations and the status map itself is not read.  The k-th start and the
+// k-th end of a row belong to the same run.
+__global__ void __launch_bounds__(256) k_ccl_runs(DevGrid g, const unsigned long long *inside, int N, int cap,
+                                                  int *rx0, int *rx1, int *rowcnt) {
+    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= (int64_t)g.ny * g.nz) return;
-    const int lane = threadIdx.x & 63;
-    const int64_t base = row * g.nx;
+    const int j = (int)(row % g.ny), k = (int)(row / g.ny);
+    const int64_t NN = (int64_t)N * N, ray = (int64_t)k * N + j;
+    const int nw = (g.nx + 63) / 64;
     const int64_t slot0 = row * cap;
-    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    auto temp_out = [&](int w) -> unsigned long long {  // TEMP_OUT bits of word w (0 past the row)
+        if (w >= nw) return 0ull;
+        unsigned long long t = ~inside[(int64_t)w * NN + ray];
+        const int valid = g.nx - 64 * w;
+        if (valid < 64) t &= (1ull << valid) - 1ull;
+        return t;
+    };
     int ns = 0, ne = 0;
-    unsigned long long prev = 0;  // last node of the previous chunk, in bit 0
-    for (int g0 = 0; g0 < g.nx; g0 += 64 * 8) {
-        unsigned long long m[9];
-#pragma unroll
-        for (int c = 0; c < 9; ++c) {
-            const int i = g0 + 64 * c + lane;
-            m[c] = __ballot(i < g.nx && (c < 8 || lane == 0) && status[base + i] == NSGPU_STATUS_TEMP_OUT);
+    unsigned long long prev = 0ull, t = temp_out(0);
+    for (int w = 0; w < nw; ++w) {
+        const unsigned long long next = temp_out(w + 1);
+        unsigned long long starts = t & ~((t << 1) | prev);
+        unsigned long long ends = t & ~((t >> 1) | (next << 63));
+        while (starts) {
+            rx0[slot0 + ns++] = 64 * w + __builtin_ctzll(starts);
+            starts &= starts - 1ull;
         }
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const unsigned long long t = m[c];
-            const unsigned long long starts = t & ~((t << 1) | prev);
-            const unsigned long long ends = t & ~((t >> 1) | (m[c + 1] << 63));
-            prev = t >> 63;
-            const int i = g0 + 64 * c + lane;
-            if ((starts >> lane) & 1) rx0[slot0 + ns + __popcll(starts & below)] = i;
-            if ((ends >> lane) & 1) rx1[slot0 + ne + __popcll(ends & below)] = i;
-            ns += __popcll(starts);
-            ne += __popcll(ends);
+        while (ends) {
+            rx1[slot0 + ne++] = 64 * w + __builtin_ctzll(ends);
+            ends &= ends - 1ull;
         }
+        prev = t >> 63;
+        t = next;
     }
-    if (lane == 0) rowcnt[row] = ns;
+    rowcnt[row] = ns;
 }
 
 // root in LDS with path halving (the LDS twin of find_root)
@@ -326,12 +330,11 @@ static inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 int ccl_row_capacity(const DevGrid &g) { return (g.nx + 1) / 2; }
 
-void launch_ccl(hipStream_t s, const DevGrid &g, const int *status, const CclRuns &c, int *scan_scratch,
-                int *nroots, int *rootlist) {
+void launch_ccl(hipStream_t s, const DevGrid &g, const DevRays &R, const CclRuns &c, int *scan_scratch, int *nroots,
+                int *rootlist) {
     const int64_t rows = (int64_t)g.ny * g.nz;
     const int cap = ccl_row_capacity(g);
-    hipLaunchKernelGGL(k_ccl_runs, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, s, g, status, cap, c.rx0, c.rx1,
-                       c.rowcnt);
+    hipLaunchKernelGGL(k_ccl_runs, dim3(nblk(rows)), dim3(256), 0, s, g, R.cov_st, R.N, cap, c.rx0, c.rx1, c.rowcnt);
     launch_scan(s, c.rowcnt, rows, c.roff, scan_scratch);
     hipLaunchKernelGGL(k_ccl_plane, dim3((unsigned)g.nz), dim3(1024), 0, s, g, cap, c.rx0, c.rx1, c.rowcnt, c.roff,
                        c.R);

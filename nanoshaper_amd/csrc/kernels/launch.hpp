@@ -31,8 +31,8 @@ struct CclRuns {
     int *rx0, *rx1, *R, *rowcnt, *roff;
 };
 int ccl_row_capacity(const DevGrid &);
-// scan_scratch: scan_scratch_ints(rows) ints
-void launch_ccl(hipStream_t, const DevGrid &, const int *status, const CclRuns &, int *scan_scratch, int *nroots,
+// runs from the x grid rays' status plane (R.cov_st); scan_scratch: scan_scratch_ints(rows) ints
+void launch_ccl(hipStream_t, const DevGrid &, const DevRays &R, const CclRuns &, int *scan_scratch, int *nroots,
                 int *rootlist);
 // components with slot >= capacity are skipped
 void launch_ccl_keys(hipStream_t, const DevGrid &, const CclRuns &, unsigned int *bkey, int *count, int capacity);
