@@ -268,7 +268,7 @@ def main():
                 "flops_per_launch": flops, "path_counts": pc}
     fill_bytes = 17.0 * nvox
     t_fill = stage_avg["fill"] * 1e-3
-    fill_roof = {"kernel": "k_fill_tiled", "bound": "hbm", "bytes_per_launch": fill_bytes,
+    fill_roof = {"kernel": "k_fill_bits", "bound": "hbm", "bytes_per_launch": fill_bytes,
                  "achieved_GBs": fill_bytes / t_fill / 1e9 if t_fill > 0 else 0.0, "peak_GBs": PEAK_HBM_GBS}
     fill_roof["frac"] = fill_roof["achieved_GBs"] / PEAK_HBM_GBS
 

@@ -7,6 +7,8 @@
 // -ffp-contract=off), so roots, feasibility and rounding decisions match the
 // CPU code (ConnollySurface::getPatchIntersectionData, ConnollySurface.cpp:
 // 4319-4591; raySphere/rayTorus/quarticEqSolutions, tools.cpp:350-1099).
+#include <algorithm>
+
 #include "launch.hpp"
 
 namespace nsg {
@@ -668,26 +670,37 @@ __global__ void __launch_bounds__(256) k_sort(DevGrid g, DevRays R, unsigned lon
 }
 
 // out[c] = sum (or max when bit c of max_mask is set) over rows of in[row*ncols + c]
+// Column sums (or maxima, columns in max_mask) of a row-major table of
+// per-workgroup partials: blockIdx.y splits the rows, each block combines its
+// share and adds it to out[c] with one atomic (out is cleared first).
 __global__ void __launch_bounds__(1024) k_reduce_cols(const unsigned long long *in, int64_t nrows, int ncols,
                                                       unsigned int max_mask, unsigned long long *out) {
-    __shared__ unsigned long long sh[1024];
+    __shared__ unsigned long long sh[1024 / 64];
     const int c = blockIdx.x;
     const bool is_max = (max_mask >> c) & 1u;
+    const int64_t per = (nrows + gridDim.y - 1) / gridDim.y;
+    const int64_t r0 = (int64_t)blockIdx.y * per, r1 = min(nrows, r0 + per);
     unsigned long long acc = 0;
-    for (int64_t r = threadIdx.x; r < nrows; r += 1024) {
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += 1024) {
         const unsigned long long v = in[r * ncols + c];
         acc = is_max ? (v > acc ? v : acc) : acc + v;
     }
-    sh[threadIdx.x] = acc;
-    __syncthreads();
-    for (int st = 512; st > 0; st >>= 1) {
-        if (threadIdx.x < st) {
-            const unsigned long long a = sh[threadIdx.x], b = sh[threadIdx.x + st];
-            sh[threadIdx.x] = is_max ? (a > b ? a : b) : a + b;
-        }
-        __syncthreads();
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_down(acc, off, 64);
+        acc = is_max ? (o > acc ? o : acc) : acc + o;
     }
-    if (threadIdx.x == 0) out[c] = sh[0];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        acc = sh[0];
+        for (int q = 1; q < 1024 / 64; ++q) acc = is_max ? (sh[q] > acc ? sh[q] : acc) : acc + sh[q];
+        if (acc) {
+            if (is_max)
+                atomicMax(out + c, acc);
+            else
+                atomicAdd(out + c, acc);
+        }
+    }
 }
 
 int64_t intersect_workgroups(const DevPatches &P, const DevRays &R) { return (int64_t)P.n * R.nslots; }
@@ -728,7 +741,10 @@ void launch_sort(hipStream_t s, const DevGrid &g, const DevRays &R, unsigned lon
 
 void launch_reduce_cols(hipStream_t s, const unsigned long long *in, int64_t nrows, int ncols, unsigned int max_mask,
                         unsigned long long *out) {
-    hipLaunchKernelGGL(k_reduce_cols, dim3((unsigned)ncols), dim3(1024), 0, s, in, nrows, ncols, max_mask, out);
+    launch_fill32(s, out, 0u, 2 * (int64_t)ncols);
+    const int64_t parts = std::min<int64_t>(std::max<int64_t>((nrows + 16383) / 16384, 1), 256);
+    hipLaunchKernelGGL(k_reduce_cols, dim3((unsigned)ncols, (unsigned)parts), dim3(1024), 0, s, in, nrows, ncols,
+                       max_mask, out);
 }
 
 }  // namespace nsg
