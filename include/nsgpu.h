@@ -236,6 +236,17 @@ int nsgpu_read_corner_bits(nsgpu_ctx *ctx, uint32_t *out);
 int nsgpu_read_cavities(nsgpu_ctx *ctx, double *sizes, uint8_t *filled);
 /* mesh: verts [num_vertices*3], tris [num_triangles*3] */
 int nsgpu_read_mesh(nsgpu_ctx *ctx, double *verts, int32_t *tris);
+/* Vertex normals of the last mesh (Compute_Vertex_Normals), normals
+ * [num_vertices*3]: analytic vertices get the analytic normal of the patch
+ * at their intersection (ConnollySurface::getNormal, ConnollySurface.cpp:
+ * 5736-5819), mid-point vertices the normalised sum of their triangles'
+ * plane3points normals in triangle order (approximateNormals, Surface.cpp:
+ * 10588-10737). */
+int nsgpu_read_normals(nsgpu_ctx *ctx, double *normals);
+/* Compute_Vertex_Normals: when on, get_surf keeps the patch of every depth so
+ * that nsgpu_read_normals can follow the mesh (off by default, as in the
+ * reference; it costs the sort a second permuted array). */
+int nsgpu_set_vertex_normals(nsgpu_ctx *ctx, int on);
 /* Sorted hits of every ray of one panel slot (0..2 grid rays, 3..5 edge rays):
  * counts[N*N] and t[N*N*max_hits], ray (m,n) at index n*N+m, N = max(nx,ny,nz).
  * max_hits must be >= nsgpu_surf_result.max_hits_per_ray.  Reads the rays of

@@ -112,6 +112,7 @@ class Surface {
             inside = cf->read<int>("Inside_code", inside);
             accurateTriangulation = cf->read<bool>("Accurate_Triangulation", accurateTriangulation);
             probe_radius = cf->read<double>("Probe_Radius", probe_radius);
+            computeNormals = cf->read<bool>("Compute_Vertex_Normals", computeNormals);
             device = cf->read<int>("Device", device);
         }
     }
@@ -138,6 +139,7 @@ class Surface {
         o.cav_vol = cav_vol;
         o.inside = inside;
         o.precision = precision;
+        if (nsgpu_set_vertex_normals(ctx_, computeNormals) != NSGPU_OK) return false;
         if (nsgpu_get_surf(ctx_, &o, &last_) != NSGPU_OK) return false;
         if (surf_volume) *surf_volume = last_.volume;
         const int64_t nv = delphi->numNodes();
@@ -165,6 +167,10 @@ class Surface {
             vertList.resize(m.num_vertices * 3);
             triList.resize(m.num_triangles * 3);
             nsgpu_read_mesh(ctx_, vertList.data(), triList.data());
+            if (computeNormals) {
+                normList.resize(m.num_vertices * 3);
+                nsgpu_read_normals(ctx_, normList.data());
+            }
         }
         return m.area;
     }
@@ -177,9 +183,11 @@ class Surface {
     double probe_radius = 1.4;
     double randDisplacement = 1e-4;
     int precision = NSGPU_PREC_FP64;
+    bool computeNormals = false;  // Compute_Vertex_Normals
     int device = 0;
     std::vector<double> vertList;
     std::vector<int32_t> triList;
+    std::vector<double> normList;  // filled when computeNormals
 
    protected:
     bool ensure_device() {

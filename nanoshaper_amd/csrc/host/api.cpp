@@ -101,8 +101,13 @@ struct nsgpu_ctx {
     // last, for grids of idx_nvox nodes (-1: state unknown, fill it)
     DBuf vslot;
     int64_t idx_nvox = -1, idx_dirty = 0;
+    // vertex provenance for the normals: depth of each analytic vertex, cube
+    // and edge of each mid point
+    DBuf vhit, vorig, normals;
+    bool hits_have_patches = false;  // the ray lists of the last trace carry patch ids
+    bool want_normals = false;       // keep patch ids for the vertex normals (Compute_Vertex_Normals)
     DBuf segmask, anode, acase, anv, ant, avoff, atoff;
-    int64_t n_analytic = 0, nverts = 0, ntris = 0;
+    int64_t n_analytic = 0, nverts = 0, ntris = 0, nact = 0;
     double area = 0;
     bool have_trace = false;  // sorted ray lists of the last get_surf / surface_gradient
     bool have_surf = false, have_vertices = false, have_mesh = false;
@@ -207,7 +212,7 @@ void nsgpu_destroy(nsgpu_ctx *c) {
     DBuf *all[] = {&c->gx, &c->gy, &c->gz, &c->pkind, &c->poff, &c->pcnt, &c->pflags, &c->pbsph, &c->pdata, &c->porder, &c->patom, &c->atoms, &c->rpid, &c->gradv, &c->grada, &c->asums,
                    &c->rcnt, &c->rt, &c->rvol, &c->partial, &c->stats, &c->cand, &c->cov, &c->covst, &c->eps, &c->ideb, &c->status,
                    &c->bits, &c->crx0, &c->crx1, &c->cpar, &c->crowcnt, &c->croff, &c->nroots, &c->rootlist, &c->bkey, &c->ccount, &c->clabel, &c->cfilled, &c->dcavsize, &c->dcavfilled, &c->hostdone, &c->dres,
-                   &c->idx, &c->vslot, &c->vcount, &c->voffs, &c->scan_scratch, &c->verts, &c->tris, &c->tarea,
+                   &c->idx, &c->vslot, &c->vhit, &c->vorig, &c->normals, &c->vcount, &c->voffs, &c->scan_scratch, &c->verts, &c->tris, &c->tarea,
                    &c->segmask, &c->anode, &c->acase, &c->anv, &c->ant, &c->avoff, &c->atoff};
     for (DBuf *b : all) b->release();
     for (auto &e : c->ev)
@@ -756,8 +761,9 @@ static int assemble(nsgpu_ctx *c, const int eslots[3]) {
     if (rc) return rc;
     HIP_TRY(c, c->verts.ensure((size_t)std::max<int64_t>(nv, 1) * 24));
     HIP_TRY(c, c->vslot.ensure((size_t)std::max<int64_t>(nv, 1) * 4));
+    HIP_TRY(c, c->vhit.ensure((size_t)std::max<int64_t>(nv, 1) * 8));
     launch_vtx_write(c->stream, g, R, eslots, c->bits.as<uint32_t>(), c->voffs.as<int>(), c->verts.as<double>(),
-                     c->idx.as<int>(), c->vslot.as<int>());
+                     c->idx.as<int>(), c->vslot.as<int>(), c->vhit.as<int64_t>());
     HIP_TRY(c, hipGetLastError());
     c->idx_dirty = nv;
     c->n_analytic = nv;
@@ -770,13 +776,23 @@ int nsgpu_get_surf(nsgpu_ctx *c, const nsgpu_surf_opts *o, nsgpu_surf_result *re
     int rc = begin_surf(c, o);
     if (rc) return rc;
     HIP_TRY(c, hipSetDevice(c->device));
+    // the edge rays' depths keep their patch ids for the vertex normals
+    c->record_pid = o->accurate_triangulation != 0 && c->want_normals;
     for (int attempt = 0;; ++attempt) {
-        if (attempt == 4) return fail(c, NSGPU_E_STATE, "ray hit capacity did not converge");
+        if (attempt == 4) {
+            c->record_pid = false;
+            return fail(c, NSGPU_E_STATE, "ray hit capacity did not converge");
+        }
         rc = enqueue_surf(c, c->stream);
-        if (rc) return rc;
+        if (rc) {
+            c->record_pid = false;
+            return rc;
+        }
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         if (!regrow(c)) break;
     }
+    c->hits_have_patches = c->record_pid;
+    c->record_pid = false;
     rc = finish_surf(c);
     if (rc) return rc;
     mark(c, ST_VERTS, 0, c->stream);
@@ -862,8 +878,11 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
         c->vslot = nb;
     }
     // 3. write the mid points, then the triangles that may reference them
+    HIP_TRY(c, c->vorig.ensure((size_t)std::max<int64_t>(nadd, 1) * 4));
     launch_mc_vwrite(c->stream, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->idx.as<int>(),
-                     c->avoff.as<int>(), (int)c->n_analytic, c->verts.as<double>(), c->vslot.as<int>());
+                     c->avoff.as<int>(), (int)c->n_analytic, c->verts.as<double>(), c->vslot.as<int>(),
+                     c->vorig.as<int>());
+    c->nact = nact;
     c->idx_dirty = nv;
     HIP_TRY(c, c->tris.ensure((size_t)std::max<int64_t>(nt, 1) * 12));
     HIP_TRY(c, c->tarea.ensure((size_t)std::max<int64_t>(nt, 1) * 8));
@@ -1043,6 +1062,55 @@ int nsgpu_surface_gradient(nsgpu_ctx *c, double *volume, double *area, double *d
     c->res.volume = *volume;
     c->res.total_rays = 3LL * R.N * R.N;
     if (res) *res = c->res;
+    return NSGPU_OK;
+}
+
+int nsgpu_set_vertex_normals(nsgpu_ctx *c, int on) {
+    if (!c) return NSGPU_E_ARG;
+    c->want_normals = on != 0;
+    return NSGPU_OK;
+}
+
+int nsgpu_read_normals(nsgpu_ctx *c, double *normals) {
+    if (!c || !normals) return NSGPU_E_ARG;
+    if (!c->have_mesh) return fail(c, NSGPU_E_STATE, "no mesh computed");
+    if (!c->hits_have_patches)
+        return fail(c, NSGPU_E_STATE, "vertex normals need nsgpu_set_vertex_normals(ctx, 1) before get_surf");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const DevGrid g = dev_grid(c);
+    DevRays R = dev_rays(c);
+    R.pid = c->rpid.as<int>();
+    DevPatches P;
+    P.n = c->npatch;
+    P.kind = c->pkind.as<int>();
+    P.off = c->poff.as<int>();
+    P.cnt = c->pcnt.as<int>();
+    P.flags = c->pflags.as<int>();
+    P.bsph = c->pbsph.as<double>();
+    P.data = c->pdata.as<double>();
+    P.order = c->porder.as<int>();
+    P.n_plain = c->npatch_plain;
+    P.atom = c->patom.as<int>();
+    HIP_TRY(c, c->normals.ensure((size_t)std::max<int64_t>(c->nverts, 1) * 24));
+    NormalArgs a;
+    a.verts = c->verts.as<double>();
+    a.vhit = c->vhit.as<int64_t>();
+    a.vorig = c->vorig.as<int>();
+    a.anode = c->anode.as<int>();
+    a.ant = c->ant.as<int>();
+    a.atoff = c->atoff.as<int>();
+    a.tris = c->tris.as<int>();
+    a.n_analytic = c->n_analytic;
+    a.nverts = c->nverts;
+    a.nact = (int)c->nact;
+    a.eslot0 = c->panels[0];
+    a.probe = c->probe;
+    a.normals = c->normals.as<double>();
+    launch_normals(c->stream, g, P, R, a);
+    HIP_TRY(c, hipGetLastError());
+    if (c->nverts)
+        HIP_TRY(c, hipMemcpyAsync(normals, c->normals.p, (size_t)c->nverts * 24, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
     return NSGPU_OK;
 }
 

@@ -79,9 +79,10 @@ int64_t scan_scratch_ints(int64_t n);
 void launch_scan(hipStream_t, const int *, int64_t, int *, int *);
 void mc_upload_tables(hipStream_t);
 void launch_vtx_count(hipStream_t, const DevGrid &, const DevRays &, const int[3], const uint32_t *, int *);
-// vslot[v] receives the index slot of vertex v (for launch_idx_reset)
+// vslot[v] receives the index slot of vertex v (for launch_idx_reset), vhit[v]
+// the index of its depth in R.t / R.pid
 void launch_vtx_write(hipStream_t, const DevGrid &, const DevRays &, const int[3], const uint32_t *, const int *,
-                      double *, int *, int *vslot);
+                      double *, int *, int *vslot, int64_t *vhit);
 void launch_idx_reset(hipStream_t, int *idx, const int *vslot, int64_t n);
 int64_t mc_num_segments(const DevGrid &);
 void launch_mc_classify(hipStream_t, const DevGrid &, const uint32_t *bits, uint32_t *segmask, int *segcnt);
@@ -89,8 +90,23 @@ void launch_mc_compact(hipStream_t, const DevGrid &, const uint32_t *bits, const
                        int *anode, uint8_t *acase);
 void launch_mc_counts(hipStream_t, const DevGrid &, int nact, const int *anode, const uint8_t *acase, const int *idx,
                       int *nv, int *nt);
+// vorig[v - base] = 4 * (active cube) + (0, 1, 2 for edge 5, 6, 10) of each mid point
 void launch_mc_vwrite(hipStream_t, const DevGrid &, int nact, const int *anode, const uint8_t *acase, int *idx,
-                      const int *voff, int base, double *verts, int *vslot);
+                      const int *voff, int base, double *verts, int *vslot, int *vorig);
+// vertex normals: analytic at the depths the vertices come from (the ray
+// lists must carry patch ids), mid points from their adjacent triangles
+struct NormalArgs {
+    const double *verts;
+    const int64_t *vhit;
+    const int *vorig;
+    const int *anode, *ant, *atoff, *tris;
+    int64_t n_analytic, nverts;
+    int nact;
+    int eslot0;  // first edge-ray slot
+    double probe;
+    double *normals;
+};
+void launch_normals(hipStream_t, const DevGrid &, const DevPatches &, const DevRays &, const NormalArgs &);
 void launch_mc_twrite(hipStream_t, const DevGrid &, int nact, const int *anode, const uint8_t *acase, const int *idx,
                       const int *toff, const double *verts, int *tris, double *tarea);
 

@@ -90,7 +90,7 @@ __device__ int walk_edge_ray(const DevGrid &g, const DevRays &R, const uint32_t 
             if (panel == 0) { px = lim[h]; py = p0; pz = p1; }
             else if (panel == 1) { px = p0; py = p1; pz = lim[h]; }
             else { px = p0; py = lim[h]; pz = p1; }
-            emit(count, d, f, px, py, pz);
+            emit(count, d, f, px, py, pz, q + h);
             ++count;
         }
     }
@@ -104,11 +104,12 @@ __global__ void __launch_bounds__(256) k_vtx_count(DevGrid g, DevRays R, EdgeSlo
     if (r >= 3 * NN) return;
     const int panel = (int)(r / NN);
     counts[r] = walk_edge_ray(g, R, bits, panel, es.s[panel], r - panel * NN,
-                              [](int, int, int64_t, double, double, double) {});
+                              [](int, int, int64_t, double, double, double, int) {});
 }
 
 __global__ void __launch_bounds__(256) k_vtx_write(DevGrid g, DevRays R, EdgeSlots es, const uint32_t *bits,
-                                                   const int *offs, double *verts, int *idx, int *vslot) {
+                                                   const int *offs, double *verts, int *idx, int *vslot,
+                                                   int64_t *vhit) {
     const int64_t NN = (int64_t)R.N * R.N;
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= 3 * NN) return;
@@ -116,13 +117,15 @@ __global__ void __launch_bounds__(256) k_vtx_write(DevGrid g, DevRays R, EdgeSlo
     const int base = offs[r];
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
     walk_edge_ray(g, R, bits, panel, es.s[panel], r - panel * NN,
-                  [&](int q, int d, int64_t f, double px, double py, double pz) {
+                  [&](int q, int d, int64_t f, double px, double py, double pz, int hit) {
                       const int v = base + q;
                       verts[3 * (int64_t)v + 0] = px;
                       verts[3 * (int64_t)v + 1] = py;
                       verts[3 * (int64_t)v + 2] = pz;
                       idx[3 * f + d] = v;
                       vslot[v] = (int)(3 * f + d);
+                      // the depth this vertex comes from, for its analytic normal
+                      vhit[v] = ((int64_t)es.s[panel] * NN + (r - panel * NN)) * R.K + hit;
                   });
 }
 
@@ -301,7 +304,8 @@ __global__ void __launch_bounds__(256) k_mc_counts(DevGrid g, int nact, const in
 }
 
 __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const int *anode, const uint8_t *acase,
-                                                   int *idx, const int *voff, int base, double *verts, int *vslot) {
+                                                   int *idx, const int *voff, int base, double *verts, int *vslot,
+                                                   int *vorig) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nact) return;
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (int64_t)g.nx * g.ny;
@@ -320,6 +324,7 @@ __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const in
         verts[3 * (int64_t)v + 1] = g.y[j];
         verts[3 * (int64_t)v + 2] = g.z[k + 1] - g.hside;
         vslot[v] = (int)s5;
+        vorig[v - base] = 4 * q + 0;
         idx[s5] = v++;
     }
     if (m6) {
@@ -327,6 +332,7 @@ __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const in
         verts[3 * (int64_t)v + 1] = g.y[j + 1] - g.hside;
         verts[3 * (int64_t)v + 2] = g.z[k + 1] - g.hside;
         vslot[v] = (int)s6;
+        vorig[v - base] = 4 * q + 1;
         idx[s6] = v++;
     }
     if (m10) {
@@ -334,6 +340,7 @@ __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const in
         verts[3 * (int64_t)v + 1] = g.y[j + 1] - g.hside;
         verts[3 * (int64_t)v + 2] = g.z[k];
         vslot[v] = (int)s10;
+        vorig[v - base] = 4 * q + 2;
         idx[s10] = v++;
     }
 }
@@ -385,10 +392,10 @@ void launch_vtx_count(hipStream_t s, const DevGrid &g, const DevRays &R, const i
 }
 
 void launch_vtx_write(hipStream_t s, const DevGrid &g, const DevRays &R, const int eslots[3], const uint32_t *bits,
-                      const int *offs, double *verts, int *idx, int *vslot) {
+                      const int *offs, double *verts, int *idx, int *vslot, int64_t *vhit) {
     EdgeSlots es{{eslots[0], eslots[1], eslots[2]}};
     const int64_t n = 3 * (int64_t)R.N * R.N;
-    hipLaunchKernelGGL(k_vtx_write, dim3(nblk(n)), dim3(256), 0, s, g, R, es, bits, offs, verts, idx, vslot);
+    hipLaunchKernelGGL(k_vtx_write, dim3(nblk(n)), dim3(256), 0, s, g, R, es, bits, offs, verts, idx, vslot, vhit);
 }
 
 // back to "no vertex" for the slots the last vertices were written to
@@ -432,10 +439,10 @@ void launch_mc_counts(hipStream_t s, const DevGrid &g, int nact, const int *anod
 }
 
 void launch_mc_vwrite(hipStream_t s, const DevGrid &g, int nact, const int *anode, const uint8_t *acase, int *idx,
-                      const int *voff, int base, double *verts, int *vslot) {
+                      const int *voff, int base, double *verts, int *vslot, int *vorig) {
     if (nact)
         hipLaunchKernelGGL(k_mc_vwrite, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, idx, voff, base,
-                           verts, vslot);
+                           verts, vslot, vorig);
 }
 
 void launch_mc_twrite(hipStream_t s, const DevGrid &g, int nact, const int *anode, const uint8_t *acase,

@@ -263,6 +263,17 @@ class DeviceSurface:
         self._chk(self._lib.nsgpu_read_mesh(self._h, _ptr(v), _ptr(t)), "read_mesh")
         return v, t
 
+    def set_vertex_normals(self, on=True):
+        """Compute_Vertex_Normals: keep what normals() needs from the next get_surf on."""
+        self._chk(self._lib.nsgpu_set_vertex_normals(self._h, int(on)), "set_vertex_normals")
+
+    def normals(self):
+        """(num_vertices, 3) vertex normals of the last mesh (set_vertex_normals first)."""
+        m = self.mesh_info
+        out = np.empty((m["num_vertices"], 3))
+        self._chk(self._lib.nsgpu_read_normals(self._h, _ptr(out)), "read_normals")
+        return out
+
     def ray_hits(self, slot):
         g = self.grid
         N = max(g.nx, g.ny, g.nz)

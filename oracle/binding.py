@@ -42,6 +42,7 @@ def load():
         "orc_read_corner_bits": (None, [VP, VP]),
         "orc_read_cavities": (None, [VP, VP, VP]),
         "orc_read_mesh": (None, [VP, VP, VP]),
+        "orc_read_normals": (None, [VP, VP]),
         "orc_read_ray_hits": (ctypes.c_int, [VP, ctypes.c_int, ctypes.c_int, VP, VP]),
         "orc_ray_sphere": (ctypes.c_int, [VP, VP, VP, ctypes.c_double, VP, VP]),
         "orc_cubic": (ctypes.c_double, [ctypes.c_double] * 3),
@@ -139,6 +140,11 @@ class OracleSurface:
         t = np.empty((m["num_triangles"], 3), np.int32)
         self._lib.orc_read_mesh(self._h, _ptr(v), _ptr(t))
         return v, t
+
+    def normals(self):
+        out = np.empty((self.mesh_info["num_vertices"], 3))
+        self._lib.orc_read_normals(self._h, _ptr(out))
+        return out
 
     def ray_hits(self, slot):
         g = self.grid

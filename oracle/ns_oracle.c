@@ -285,6 +285,7 @@ int orc_is_feasible(const nsgpu_patches *P, int it, const double *point) {
 typedef struct {
     int32_t ix, iy, iz, dir; /* edge index and direction (0 x, 1 y, 2 z) */
     double p[3];
+    double n[3]; /* analytic normal of the depth the record comes from */
 } orc_vrec;
 
 typedef struct orc_surface {
@@ -300,6 +301,7 @@ typedef struct orc_surface {
     int64_t npix;
     int64_t *pix_off;
     double *pix_t;
+    int32_t *pix_patch; /* patch of each depth */
     /* maps */
     int32_t *epsmap;
     uint8_t *idebmap;
@@ -315,7 +317,7 @@ typedef struct orc_surface {
     double *cav_size;
     /* mesh */
     int32_t *idx[3]; /* vertex index per (edge dir, node) */
-    double *verts;
+    double *verts, *normals; /* normals: analytic, then approximated for mid points */
     int64_t nverts, cap_verts, n_analytic;
     int32_t *tris;
     int64_t ntris, cap_tris;
@@ -368,6 +370,8 @@ orc_surface *orc_create(const nsgpu_grid *g, const nsgpu_patches *p, double prob
 static void free_results(orc_surface *s) {
     free(s->pix_off);
     free(s->pix_t);
+    free(s->pix_patch);
+    free(s->normals);
     free(s->epsmap);
     free(s->idebmap);
     free(s->status);
@@ -381,6 +385,8 @@ static void free_results(orc_surface *s) {
     free(s->tris);
     s->pix_off = NULL;
     s->pix_t = NULL;
+    s->pix_patch = NULL;
+    s->normals = NULL;
     s->epsmap = NULL;
     s->idebmap = NULL;
     s->status = NULL;
@@ -415,15 +421,16 @@ void orc_free(orc_surface *s) {
 typedef struct {
     int64_t pix;
     double t;
+    int32_t patch;
 } orc_hit;
 
 static int cmp_pix(const void *a, const void *b) {
     const orc_hit *x = (const orc_hit *)a, *y = (const orc_hit *)b;
     return (x->pix > y->pix) - (x->pix < y->pix);
 }
-static int cmp_t(const void *a, const void *b) {
-    double x = *(const double *)a, y = *(const double *)b;
-    return (x > y) - (x < y);
+static int cmp_hit_t(const void *a, const void *b) {
+    const orc_hit *x = (const orc_hit *)a, *y = (const orc_hit *)b;
+    return (x->t > y->t) - (x->t < y->t);
 }
 
 static void patch_intersections(orc_surface *s) {
@@ -529,6 +536,7 @@ static void patch_intersections(orc_surface *s) {
                         }
                         hits[nh].pix = (n * N_MAX + m) * numPanels + slot;
                         hits[nh].t = t[i];
+                        hits[nh].patch = it;
                         nh++;
                     }
                 }
@@ -539,15 +547,19 @@ static void patch_intersections(orc_surface *s) {
     s->npix = N_MAX * N_MAX * numPanels;
     s->pix_off = (int64_t *)xcalloc(s->npix + 1, sizeof(int64_t));
     s->pix_t = (double *)xcalloc(nh, sizeof(double));
+    s->pix_patch = (int32_t *)xcalloc(nh, sizeof(int32_t));
     if (nh) qsort(hits, nh, sizeof(orc_hit), cmp_pix);
     for (int64_t h = 0; h < nh; h++) s->pix_off[hits[h].pix + 1]++;
     for (int64_t p = 0; p < s->npix; p++) s->pix_off[p + 1] += s->pix_off[p];
-    for (int64_t h = 0; h < nh; h++) s->pix_t[h] = hits[h].t;
     int maxh = 0;
     for (int64_t p = 0; p < s->npix; p++) {
         int64_t c = s->pix_off[p + 1] - s->pix_off[p];
-        if (c > 1) qsort(s->pix_t + s->pix_off[p], c, sizeof(double), cmp_t);
+        if (c > 1) qsort(hits + s->pix_off[p], c, sizeof(orc_hit), cmp_hit_t);
         if (c > maxh) maxh = (int)c;
+    }
+    for (int64_t h = 0; h < nh; h++) {
+        s->pix_t[h] = hits[h].t;
+        s->pix_patch[h] = hits[h].patch;
     }
     free(hits);
     s->res.total_hits = nh;
@@ -574,7 +586,51 @@ static int bit_get(const orc_surface *s, int64_t i, int64_t j, int64_t k) {
     return (s->bits[f >> 5] >> (uint32_t)(f & 31)) & 1u;
 }
 
-static void push_vrec(orc_surface *s, int ix, int iy, int iz, int dir, double px, double py, double pz) {
+/* ConnollySurface::getNormal (ConnollySurface.cpp:5736-5766) and
+ * getNormalToTorus (:5794-5819) at the point y of patch it; the torus frame
+ * is orthonormal, so its de-rotation uses the transpose of invrot */
+static void patch_normal(const orc_surface *s, int it, const double y[3], double n[3]) {
+    const nsgpu_patches *P = &s->P;
+    const int kind = P->kind[it];
+    const double *c = P->bsph + 4 * (size_t)it;
+    if (kind == NSGPU_PATCH_FACET_REG || kind == NSGPU_PATCH_FACET_SING) {
+        for (int q = 0; q < 3; q++) n[q] = (c[q] - y[q]) / s->probe;
+    } else if (kind == NSGPU_PATCH_POINT) {
+        for (int q = 0; q < 3; q++) n[q] = (y[q] - c[q]) / c[3];
+    } else {
+        const double *td = P->data + P->off[it], *iv = td + 3;
+        double t[3] = {y[0] - td[0], y[1] - td[1], y[2] - td[2]}, C[3];
+        for (int i = 0; i < 3; i++) C[i] = iv[3 * i + 0] * t[0] + iv[3 * i + 1] * t[1] + iv[3 * i + 2] * t[2];
+        double d = C[0] * C[0] + C[1] * C[1];
+        double rm1 = td[12] / sqrt(d) - 1.;
+        d = sqrt(d * rm1 * rm1 + C[2] * C[2]);
+        C[0] *= rm1 / d;
+        C[1] *= rm1 / d;
+        C[2] /= -d;
+        for (int i = 0; i < 3; i++) n[i] = iv[0 + i] * C[0] + iv[3 + i] * C[1] + iv[6 + i] * C[2];
+    }
+}
+
+/* normal of depth h of an edge ray of panel `panel` through (m, n): at the
+ * intersection point as getPatchIntersectionData formed it */
+static void depth_normal(const orc_surface *s, int panel, int64_t m, int64_t n, int64_t h, double nrm[3]) {
+    const double delta = ORC_DELTA - s->g.hside;
+    double y[3];
+    if (panel == 0) {
+        y[0] = s->x[0] + delta; y[1] = s->y[m] + delta; y[2] = s->z[n] + delta;
+        y[0] += s->pix_t[h] * (s->x[s->NX - 1] - s->x[0]);
+    } else if (panel == 1) {
+        y[2] = s->z[0] + delta; y[0] = s->x[m] + delta; y[1] = s->y[n] + delta;
+        y[2] += s->pix_t[h] * (s->z[s->NZ - 1] - s->z[0]);
+    } else {
+        y[1] = s->y[0] + delta; y[0] = s->x[m] + delta; y[2] = s->z[n] + delta;
+        y[1] += s->pix_t[h] * (s->y[s->NY - 1] - s->y[0]);
+    }
+    patch_normal(s, s->pix_patch[h], y, nrm);
+}
+
+static void push_vrec(orc_surface *s, int ix, int iy, int iz, int dir, double px, double py, double pz,
+                      const double nrm[3]) {
     if (s->nvrec == s->cap_vrec) {
         s->cap_vrec = s->cap_vrec ? 2 * s->cap_vrec : 4096;
         s->vrec = (orc_vrec *)realloc(s->vrec, s->cap_vrec * sizeof(orc_vrec));
@@ -587,6 +643,7 @@ static void push_vrec(orc_surface *s, int ix, int iy, int iz, int dir, double px
     v->p[0] = px;
     v->p[1] = py;
     v->p[2] = pz;
+    memcpy(v->n, nrm, sizeof v->n);
 }
 
 /* NEW_INTERSECTION_FILTERING pairing (Surface.cpp:5265-5285): even counts
@@ -727,8 +784,11 @@ static void fill_panel(orc_surface *s, int panel, double *volPanel, int *flags) 
                         if (xb <= xa) continue;
                         for (int i = xa + 1; i <= xb; i++)
                             if (i >= 0 && i < NX) bit_clear(s, i, m, n);
-                        push_vrec(s, xa, m, n, 0, lim1, pa[1], pa[2]);
-                        push_vrec(s, xb, m, n, 0, lim2, pa[1], pa[2]);
+                        double n1[3], n2[3];
+                        depth_normal(s, 0, m, n, s->pix_off[id] + pairs[q], n1);
+                        depth_normal(s, 0, m, n, s->pix_off[id] + pairs[q] + 1, n2);
+                        push_vrec(s, xa, m, n, 0, lim1, pa[1], pa[2], n1);
+                        push_vrec(s, xb, m, n, 0, lim2, pa[1], pa[2], n2);
                     }
                 } else if (panel == 1) {
                     dir = pb[2] - pa[2];
@@ -741,8 +801,11 @@ static void fill_panel(orc_surface *s, int panel, double *volPanel, int *flags) 
                         if (zb <= za) continue;
                         for (int k = za + 1; k <= zb; k++)
                             if (k >= 0 && k < NZ) bit_clear(s, m, n, k);
-                        push_vrec(s, m, n, za, 2, pa[0], pa[1], lim1);
-                        push_vrec(s, m, n, zb, 2, pa[0], pa[1], lim2);
+                        double n1[3], n2[3];
+                        depth_normal(s, 1, m, n, s->pix_off[id] + pairs[q], n1);
+                        depth_normal(s, 1, m, n, s->pix_off[id] + pairs[q] + 1, n2);
+                        push_vrec(s, m, n, za, 2, pa[0], pa[1], lim1, n1);
+                        push_vrec(s, m, n, zb, 2, pa[0], pa[1], lim2, n2);
                     }
                 } else {
                     dir = pb[1] - pa[1];
@@ -755,8 +818,11 @@ static void fill_panel(orc_surface *s, int panel, double *volPanel, int *flags) 
                         if (yb <= ya) continue;
                         for (int j = ya + 1; j <= yb; j++)
                             if (j >= 0 && j < NY) bit_clear(s, m, j, n);
-                        push_vrec(s, m, ya, n, 1, pa[0], lim1, pa[2]);
-                        push_vrec(s, m, yb, n, 1, pa[0], lim2, pa[2]);
+                        double n1[3], n2[3];
+                        depth_normal(s, 2, m, n, s->pix_off[id] + pairs[q], n1);
+                        depth_normal(s, 2, m, n, s->pix_off[id] + pairs[q] + 1, n2);
+                        push_vrec(s, m, ya, n, 1, pa[0], lim1, pa[2], n1);
+                        push_vrec(s, m, yb, n, 1, pa[0], lim2, pa[2], n2);
                     }
                 }
                 volPanel[panel] += vol;
@@ -914,12 +980,15 @@ static void fill_cavities(orc_surface *s, double vol) {
 
 /* assembleVerticesList, Surface.cpp:7415-7660: a record is dropped when the
  * corners of its edge agree; otherwise the first record of an edge wins. */
-static void push_vert(orc_surface *s, const double *p) {
+static void push_vert(orc_surface *s, const double *p, const double *nrm) {
+    static const double zero[3] = {0., 0., 0.};
     if (s->nverts == s->cap_verts) {
         s->cap_verts = s->cap_verts ? 2 * s->cap_verts : 4096;
         s->verts = (double *)realloc(s->verts, s->cap_verts * 3 * sizeof(double));
+        s->normals = (double *)realloc(s->normals, s->cap_verts * 3 * sizeof(double));
     }
     memcpy(s->verts + 3 * s->nverts, p, 3 * sizeof(double));
+    memcpy(s->normals + 3 * s->nverts, nrm ? nrm : zero, 3 * sizeof(double));
     s->nverts++;
 }
 
@@ -938,7 +1007,7 @@ static void assemble_vertices(orc_surface *s) {
         int32_t *slot = &s->idx[v->dir][FLAT(v->ix, v->iy, v->iz)];
         if (*slot == -1) {
             *slot = (int32_t)s->nverts;
-            push_vert(s, v->p);
+            push_vert(s, v->p, v->n);
         }
     }
     s->n_analytic = s->nverts;
@@ -1031,17 +1100,17 @@ int orc_triangulate(orc_surface *s, nsgpu_mesh_info *info) {
                 if (edge_active(c, 5) && s->idx[1][FLAT(i + 1, j, k + 1)] == -1) {
                     double p[3] = {s->x[i + 1] - s->g.hside, s->y[j], s->z[k + 1] - s->g.hside};
                     s->idx[1][FLAT(i + 1, j, k + 1)] = (int32_t)s->nverts;
-                    push_vert(s, p);
+                    push_vert(s, p, NULL);
                 }
                 if (edge_active(c, 6) && s->idx[0][FLAT(i, j + 1, k + 1)] == -1) {
                     double p[3] = {s->x[i], s->y[j + 1] - s->g.hside, s->z[k + 1] - s->g.hside};
                     s->idx[0][FLAT(i, j + 1, k + 1)] = (int32_t)s->nverts;
-                    push_vert(s, p);
+                    push_vert(s, p, NULL);
                 }
                 if (edge_active(c, 10) && s->idx[2][FLAT(i + 1, j + 1, k)] == -1) {
                     double p[3] = {s->x[i + 1] - s->g.hside, s->y[j + 1] - s->g.hside, s->z[k]};
                     s->idx[2][FLAT(i + 1, j + 1, k)] = (int32_t)s->nverts;
-                    push_vert(s, p);
+                    push_vert(s, p, NULL);
                 }
             }
     /* edge -> (direction, node offset) of getTriangles */
@@ -1082,6 +1151,33 @@ int orc_triangulate(orc_surface *s, nsgpu_mesh_info *info) {
                 }
             }
     s->area = area;
+    /* approximateNormals(appNormals, true) (Surface.cpp:10588-10737): every
+     * vertex with a zero normal (the mid points) gets the normalised sum of
+     * plane3points (tools.cpp:258-274, unnormalised) over its triangles, in
+     * triangle order */
+    for (int64_t t = 0; t < s->ntris; t++) {
+        const int32_t *tv = s->tris + 3 * t;
+        const double *p1 = s->verts + 3 * (int64_t)tv[0], *p2 = s->verts + 3 * (int64_t)tv[1],
+                     *p3 = s->verts + 3 * (int64_t)tv[2];
+        double w[3];
+        w[0] = p1[1] * (p2[2] - p3[2]) + p2[1] * (p3[2] - p1[2]) + p3[1] * (p1[2] - p2[2]);
+        w[1] = p1[2] * (p2[0] - p3[0]) + p2[2] * (p3[0] - p1[0]) + p3[2] * (p1[0] - p2[0]);
+        w[2] = p1[0] * (p2[1] - p3[1]) + p2[0] * (p3[1] - p1[1]) + p3[0] * (p1[1] - p2[1]);
+        for (int q = 0; q < 3; q++)
+            if (tv[q] >= s->n_analytic) {
+                double *nm = s->normals + 3 * (int64_t)tv[q];
+                nm[0] += w[0];
+                nm[1] += w[1];
+                nm[2] += w[2];
+            }
+    }
+    for (int64_t v = s->n_analytic; v < s->nverts; v++) {
+        double *nm = s->normals + 3 * v;
+        const double norm = sqrt(nm[0] * nm[0] + nm[1] * nm[1] + nm[2] * nm[2]);
+        nm[0] /= norm;
+        nm[1] /= norm;
+        nm[2] /= norm;
+    }
     if (info) {
         info->num_vertices = s->nverts;
         info->num_triangles = s->ntris;
@@ -1097,6 +1193,7 @@ int orc_triangulate(orc_surface *s, nsgpu_mesh_info *info) {
 void orc_read_epsmap(const orc_surface *s, int32_t *o) { memcpy(o, s->epsmap, s->nvox * 3 * sizeof(int32_t)); }
 void orc_read_idebmap(const orc_surface *s, uint8_t *o) { memcpy(o, s->idebmap, s->nvox); }
 void orc_read_status(const orc_surface *s, int32_t *o) { memcpy(o, s->status, s->nvox * sizeof(int32_t)); }
+void orc_read_normals(const orc_surface *s, double *o) { memcpy(o, s->normals, s->nverts * 3 * sizeof(double)); }
 void orc_read_corner_bits(const orc_surface *s, uint32_t *o) { memcpy(o, s->bits, ((s->nvox + 31) / 32) * sizeof(uint32_t)); }
 void orc_read_cavities(const orc_surface *s, double *sizes, uint8_t *filled) {
     for (int c = 0; c < s->ncav; c++) {

@@ -2,6 +2,7 @@
 // surface (main.cpp:748-790, normalMode :1253-1320): config -> DelPhiShared ->
 // surfaceFactory().create -> build -> getSurf -> triangulateSurface.
 // argv[1] = "host" runs the host stages only, "gpu" the whole path.
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -16,6 +17,7 @@ int main(int argc, char **argv) {
     cf.parse_line("Grid_perfil = 90.0");
     cf.parse_line("Probe_Radius = 1.4");
     cf.parse_line("Accurate_Triangulation = true");
+    cf.parse_line("Compute_Vertex_Normals = true");
     std::vector<double> xyzr(330 * 4);
     double radius_used = 0;
     const int n = nsgpu_generate_chain(330, 12.0, 0, xyzr.data(), &radius_used);
@@ -37,6 +39,11 @@ int main(int argc, char **argv) {
         return 6;
     }
     const double area = surf->triangulateSurface(true);
+    if (surf->normList.size() != surf->vertList.size()) return 7;
+    for (size_t q = 0; q + 2 < surf->normList.size(); q += 3) {
+        const double *nv = &surf->normList[q];
+        if (std::fabs(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2] - 1.0) > 1e-9) return 8;
+    }
     long inside = 0;
     for (int v : ds.status) inside += (v == -1);
     std::printf("volume %.9f area %.9f triangles %zu inside %ld cavities %zu\n", vol, area, surf->triList.size() / 3,

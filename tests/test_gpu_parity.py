@@ -26,6 +26,7 @@ def _surfaces(n, radius, scale, perfil, seed=0, probe=1.4):
     dev = ns.DeviceSurface(0)
     dev.set_grid(grid)
     dev.set_patches(ps)
+    dev.set_vertex_normals(True)
     return grid, ps, dev, OracleSurface(grid, ps)
 
 
@@ -70,6 +71,10 @@ def _compare(grid, dev, orc, opts, mesh=True, rel=1e-12, vtx_tol=1e-9):
             vb, tb = orc.mesh()
             assert np.array_equal(ta, tb)
             assert np.abs(va - vb).max() <= vtx_tol
+            # vertex normals: analytic ones from the same patch formulas,
+            # mid points from the same triangles
+            na, nb = dev.normals(), orc.normals()
+            assert np.abs(na - nb).max() <= max(1e-9, 100 * vtx_tol)
     return r
 
 

@@ -203,6 +203,34 @@ def test_root_refinement_lowers_torus_residual(orc):
     assert compared > 400 and lowered > compared // 10
 
 
+def test_vertex_normals_follow_the_surface():
+    # analytic normals of a sphere are radial; on a molecule they are unit
+    # vectors that agree with the orientation of the triangles around them
+    import nanoshaper_amd as ns
+    from oracle.binding import OracleSurface
+    a = np.array([[0.3, -0.2, 0.1, 1.7]])
+    o = OracleSurface(ns.build_grid(a, 3.0, 80.0), ns.ses_patches(a, 1.4))
+    o.get_surf()
+    o.triangulate()
+    v, _ = o.mesh()
+    radial = (v - a[0, :3]) / 1.7
+    assert np.abs(o.normals() - radial).max() < 1e-12
+    x, _ = ns.generate_chain(40, 6.0, seed=0)
+    j, _ = ns.jitter(x)
+    o = OracleSurface(ns.build_grid(x, 2.0, 80.0), ns.ses_patches(j, 1.4))
+    o.get_surf()
+    o.triangulate()
+    v, t = o.mesh()
+    n = o.normals()
+    assert np.abs(np.linalg.norm(n, axis=1) - 1).max() < 1e-12
+    acc = np.zeros_like(v)
+    w = np.cross(v[t[:, 1]] - v[t[:, 0]], v[t[:, 2]] - v[t[:, 0]])
+    for q in range(3):
+        np.add.at(acc, t[:, q], w)
+    acc /= np.linalg.norm(acc, axis=1)[:, None]
+    assert np.median((n * acc).sum(1)) > 0.95
+
+
 # --------------------------------------------------------------- generator / grid
 def _park_miller_py(seed, n):
     s = seed
