@@ -118,3 +118,30 @@ def test_point_cells_are_exposed_atoms(patches):
     # every atom on a torus or facet has a point cell
     on_surface = set(ps.atom[ps.kind >= 1, :].ravel()) - {-1}
     assert on_surface <= set(pts.tolist())
+
+
+def test_writers_follow_the_reference_formats(tmp_path):
+    from nanoshaper_amd import io
+    ideb = np.zeros((2, 3, 4), np.uint8)  # (nz, ny, nx)
+    ideb[1, 2, 3] = 1
+    p = tmp_path / "m.idebmap.txt"
+    io.save_idebmap(str(p), ideb)
+    text = p.read_text()
+    # i outermost (4 blocks), j lines, k values: node (i=3, j=2, k=1) is the last value
+    blocks = text.split("\n\n")
+    assert len([b for b in blocks if b.strip()]) == 4
+    assert blocks[3].splitlines()[2] == "0 1 "
+    eps = np.zeros((2, 3, 4, 3), np.int32)
+    eps[0, 1, 2, 1] = 5
+    paths = io.save_epsmaps(str(tmp_path / "m"), eps)
+    assert [pth.rsplit(".", 2)[-2] for pth in paths] == ["epsmapx", "epsmapy", "epsmapz"]
+    assert open(paths[1]).read().split("\n\n")[2].splitlines()[1] == "5 0 "
+    v = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.25, 0.0]])
+    t = np.array([[0, 1, 2]])
+    n = np.array([[0.0, 0.0, 1.0]] * 3)
+    io.save_off(str(tmp_path / "m.off"), v, t, normals=n, revert=True)
+    lines = (tmp_path / "m.off").read_text().splitlines()
+    assert lines[0] == "OFF+N" and lines[2] == "3 1 0" and lines[5] == "0.000 1.250 0.000 0.000 0.000 1.000"
+    assert lines[-1] == "3 2 1 0"
+    vv, tt, nn = io.load_off(str(tmp_path / "m.off"))
+    assert np.allclose(vv, v) and np.array_equal(tt, t[:, ::-1]) and np.allclose(nn, n)
