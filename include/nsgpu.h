@@ -103,8 +103,10 @@ typedef struct {
 /* ------------------------------------------------------------------------ */
 /* Surface::getSurf options (Surface.h:890) and results.                    */
 /* ------------------------------------------------------------------------ */
-#define NSGPU_PREC_FP64 0  /* reference arithmetic throughout                         */
-#define NSGPU_PREC_MIXED 1 /* FP32 candidate cull, FP64 roots/feasibility/depths     */
+#define NSGPU_PREC_FP64 0  /* reference arithmetic throughout (closed-form quartic)      */
+#define NSGPU_PREC_MIXED 1 /* torus roots: FP32-seeded resolvent, FP64 Newton polish   */
+#define NSGPU_PREC_STURM 2 /* torus roots by Sturm bisection, the reference's path for  */
+                           /* Analytical_Ray_Vs_Torus_Intersection = false              */
 
 typedef struct {
     int32_t build_epsmap;           /* DelPhiShared::buildEpsMap (epsmap + idebmap)      */

@@ -415,7 +415,10 @@ static int enqueue_trace(nsgpu_ctx *c, hipStream_t st) {
     launch_fill32(st, c->rcnt.p, 0u, nrays);
     launch_fill32(st, c->cand.p, 0u, irows * 2);
     mark(c, ST_INTERSECT, 0, st);
-    launch_intersect(st, g, P, R, c->probe, c->cand.as<unsigned long long>(), c->opts.precision == NSGPU_PREC_MIXED);
+    launch_intersect(st, g, P, R, c->probe, c->cand.as<unsigned long long>(),
+                     c->opts.precision == NSGPU_PREC_MIXED   ? kRootsMixed
+                     : c->opts.precision == NSGPU_PREC_STURM ? kRootsSturm
+                                                             : kRootsClosed);
     HIP_TRY(c, hipGetLastError());
     mark(c, ST_INTERSECT, 1, st);
     mark(c, ST_SORT, 0, st);
@@ -591,7 +594,7 @@ static int cavities_on_host(nsgpu_ctx *c) {
 // panel rule of getSurf (Surface.cpp:652-659) and the per-call state
 static int begin_surf(nsgpu_ctx *c, const nsgpu_surf_opts *o) {
     if (!c->have_grid || !c->have_patches) return fail(c, NSGPU_E_STATE, "set_grid and set_patches first");
-    if (o->precision != NSGPU_PREC_FP64 && o->precision != NSGPU_PREC_MIXED)
+    if (o->precision != NSGPU_PREC_FP64 && o->precision != NSGPU_PREC_MIXED && o->precision != NSGPU_PREC_STURM)
         return fail(c, NSGPU_E_ARG, "unknown precision mode");
     c->opts = *o;
     std::memset(&c->res, 0, sizeof c->res);

@@ -13,8 +13,10 @@ struct FillOpts {
 constexpr int kPathCounters = 8;
 int64_t intersect_workgroups(const DevPatches &, const DevRays &);
 int64_t sort_blocks(const DevRays &);
+// torus root solvers of the intersection kernel
+enum { kRootsClosed = 0, kRootsMixed = 1, kRootsSturm = 2 };
 void launch_intersect(hipStream_t, const DevGrid &, const DevPatches &, const DevRays &, double,
-                      unsigned long long *part, bool mixed);
+                      unsigned long long *part, int roots);
 void launch_intersect_count(hipStream_t, const DevGrid &, const DevPatches &, const DevRays &, double,
                             unsigned long long *part);
 void launch_sort(hipStream_t, const DevGrid &, const DevRays &, unsigned long long *part);

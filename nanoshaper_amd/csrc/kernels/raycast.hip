@@ -388,6 +388,257 @@ __device__ int ray_torus_mixed(const double *td, const double sc[3], double prob
     return ni;
 }
 
+// ---------------------------------------------------------------------------
+// Sturm-sequence roots of the torus quartic, the reference's path when
+// Analytical_Ray_Vs_Torus_Intersection is false: getRealRootsSturm
+// (tools.cpp:199-255) over sturm/sturm.cpp:20-281 and sturm/util.cpp:17-112
+// (RELERROR 1e-7, MAXPOW 32, MAXIT 800, SMALL_ENOUGH 1e-12, solve.h:12-24).
+// The recursive bisection runs from an explicit stack; its tasks write
+// disjoint root ranges, so the order they run in does not change the roots.
+namespace sturm {
+constexpr double kRelErr = 1.0e-7, kSmall = 1.0e-12;
+constexpr int kMaxPow = 32, kMaxIt = 800;
+
+struct Poly {
+    int ord;
+    double c[5];
+};
+
+__device__ __forceinline__ double eval(const Poly &p, double x) {
+    double f = p.c[p.ord];
+    for (int i = p.ord - 1; i >= 0; --i) f = x * f + p.c[i];
+    return f;
+}
+
+__device__ int modp(const Poly &u, const Poly &v, Poly &r) {
+    for (int i = 0; i <= u.ord; ++i) r.c[i] = u.c[i];
+    if (v.c[v.ord] < 0.0) {
+        for (int k = u.ord - v.ord - 1; k >= 0; k -= 2) r.c[k] = -r.c[k];
+        for (int k = u.ord - v.ord; k >= 0; --k)
+            for (int j = v.ord + k - 1; j >= k; --j) r.c[j] = -r.c[j] - r.c[v.ord + k] * v.c[j - k];
+    } else {
+        for (int k = u.ord - v.ord; k >= 0; --k)
+            for (int j = v.ord + k - 1; j >= k; --j) r.c[j] -= r.c[v.ord + k] * v.c[j - k];
+    }
+    int k = v.ord - 1;
+    while (k >= 0 && fabs(r.c[k]) < kSmall) {
+        r.c[k] = 0.0;
+        --k;
+    }
+    r.ord = k < 0 ? 0 : k;
+    return r.ord;
+}
+
+__device__ int build(Poly *s) {
+    s[0].ord = 4;
+    s[1].ord = 3;
+    double f = fabs(s[0].c[4] * 4);
+    for (int i = 1; i <= 4; ++i) s[1].c[i - 1] = s[0].c[i] * i / f;
+    int sp = 2;
+    for (; modp(s[sp - 2], s[sp - 1], s[sp]); ++sp) {
+        f = -fabs(s[sp].c[s[sp].ord]);
+        for (int i = s[sp].ord; i >= 0; --i) s[sp].c[i] /= f;
+    }
+    s[sp].c[0] = -s[sp].c[0];
+    return sp;
+}
+
+__device__ int changes(int np, const Poly *s, double a) {
+    int n = 0;
+    double lf = eval(s[0], a);
+    for (int q = 1; q <= np; ++q) {
+        const double f = eval(s[q], a);
+        if (lf == 0.0 || lf * f < 0) ++n;
+        lf = f;
+    }
+    return n;
+}
+
+// modified regula falsi on [a, b]
+__device__ bool modrf(const Poly &p, double a, double b, double &val) {
+    double fa = p.c[p.ord], fb = fa;
+    for (int i = p.ord - 1; i >= 0; --i) {
+        fa = a * fa + p.c[i];
+        fb = b * fb + p.c[i];
+    }
+    if (fa * fb > 0.0) return false;
+    if (fabs(fa) < kRelErr) {
+        val = a;
+        return true;
+    }
+    if (fabs(fb) < kRelErr) {
+        val = b;
+        return true;
+    }
+    double lfx = fa;
+    for (int its = 0; its < kMaxIt; ++its) {
+        const double x = (fb * a - fa * b) / (fb - fa);
+        const double fx = eval(p, x);
+        if (fabs(x) > kRelErr) {
+            if (fabs(fx / x) < kRelErr) {
+                val = x;
+                return true;
+            }
+        } else if (fabs(fx) < kRelErr) {
+            val = x;
+            return true;
+        }
+        if ((fa * fx) < 0) {
+            b = x;
+            fb = fx;
+            if ((lfx * fx) > 0) fa /= 2;
+        } else {
+            a = x;
+            fa = fx;
+            if ((lfx * fx) > 0) fb /= 2;
+        }
+        lfx = fx;
+    }
+    return false;
+}
+
+struct Task {
+    double min, max;
+    int atmin, atmax, first;
+};
+
+__device__ void bisect(int np, const Poly *s, double min0, double max0, int atmin0, int atmax0, double *roots) {
+    Task st[8];
+    int top = 0;
+    st[top++] = Task{min0, max0, atmin0, atmax0, 0};
+    while (top > 0) {
+        Task t = st[--top];
+        double min = t.min, max = t.max, mid = 0;
+        int its;
+        if (t.atmin - t.atmax == 1) {
+            if (modrf(s[0], min, max, roots[t.first])) continue;
+            bool done = false;
+            for (its = 0; its < kMaxIt; ++its) {
+                mid = (min + max) / 2;
+                const int atmid = changes(np, s, mid);
+                if (atmid < t.atmax || atmid > t.atmin) {
+                    done = true;
+                    break;
+                }
+                if (fabs(mid) > kRelErr) {
+                    if (fabs((max - min) / mid) < kRelErr) {
+                        roots[t.first] = mid;
+                        done = true;
+                        break;
+                    }
+                } else if (fabs(max - min) < kRelErr) {
+                    roots[t.first] = mid;
+                    done = true;
+                    break;
+                }
+                if (t.atmin - atmid == 0) min = mid;
+                else max = mid;
+            }
+            if (!done) roots[t.first] = mid;
+            continue;
+        }
+        for (its = 0; its < kMaxIt; ++its) {
+            mid = (min + max) / 2;
+            const int atmid = changes(np, s, mid);
+            if (atmid < t.atmax || atmid > t.atmin) break;
+            const int n1 = t.atmin - atmid, n2 = atmid - t.atmax;
+            if (n1 != 0 && n2 != 0) {
+                if (top < 7) {
+                    st[top++] = Task{mid, max, atmid, t.atmax, t.first + n1};
+                    st[top++] = Task{min, mid, t.atmin, atmid, t.first};
+                }
+                break;
+            }
+            if (n1 == 0) min = mid;
+            else max = mid;
+        }
+        if (its == kMaxIt)
+            for (int q = t.atmax; q < t.atmin; ++q) roots[t.first + q - t.atmax] = mid;
+    }
+}
+
+// roots of x^4 + B x^3 + C x^2 + D x + E, ascending; a root the bisection
+// gives up on stays NaN (the reference leaves it unwritten) and is dropped
+__device__ int quartic(double roots[4], double B, double C, double D, double E) {
+    Poly s[5];
+    s[0].c[0] = E;
+    s[0].c[1] = D;
+    s[0].c[2] = C;
+    s[0].c[3] = B;
+    s[0].c[4] = 1.0;
+    for (int i = 0; i < 4; ++i) roots[i] = __longlong_as_double(0x7ff8000000000000ll);
+    const int np = build(s);
+    // numroots: sign changes at -inf and +inf
+    int atpos = 0, atneg = 0;
+    double lf = s[0].c[s[0].ord];
+    for (int q = 1; q <= np; ++q) {
+        const double f = s[q].c[s[q].ord];
+        if (lf == 0.0 || lf * f < 0) ++atpos;
+        lf = f;
+    }
+    lf = (s[0].ord & 1) ? -s[0].c[s[0].ord] : s[0].c[s[0].ord];
+    for (int q = 1; q <= np; ++q) {
+        const double f = (s[q].ord & 1) ? -s[q].c[s[q].ord] : s[q].c[s[q].ord];
+        if (lf == 0.0 || lf * f < 0) ++atneg;
+        lf = f;
+    }
+    int atmin = atneg, atmax = atpos;
+    if (atmin - atmax == 0) return 0;
+    double min = -1.0;
+    int nc = changes(np, s, min);
+    for (int i = 0; nc != atmin && i != kMaxPow; ++i) {
+        min *= 10.0;
+        nc = changes(np, s, min);
+    }
+    if (nc != atmin) atmin = nc;
+    double max = 1.0;
+    nc = changes(np, s, max);
+    for (int i = 0; nc != atmax && i != kMaxPow; ++i) {
+        max *= 10.0;
+        nc = changes(np, s, max);
+    }
+    if (nc != atmax) atmax = nc;
+    const int n = atmin - atmax;
+    if (n <= 0 || n > 4) return n <= 0 ? 0 : 4;
+    bisect(np, s, min, max, atmin, atmax, roots);
+    return n;
+}
+}  // namespace sturm
+
+// rayTorus with the Sturm roots; frame change, coefficients and clipping as ray_torus
+__device__ int ray_torus_sturm(const double *td, const double sc[3], double probe, double radius, int vc,
+                               const double orig[3], double ray_dir, double t[4]) {
+    double o1[3] = {orig[0], orig[1], orig[2]};
+    o1[vc] += t[0] * ray_dir;
+    const double tx = o1[0] - td[0], ty = o1[1] - td[1], tz = o1[2] - td[2];
+    const double *iv = td + 3;
+    double pp[3];
+    for (int i = 0; i < 3; i++) pp[i] = iv[3 * i + 0] * tx + iv[3 * i + 1] * ty + iv[3 * i + 2] * tz;
+    const double r2 = probe * probe;
+    const double R2 = td[12] * td[12];
+    const double pd = pp[0] * iv[0 + vc] + pp[1] * iv[3 + vc] + pp[2] * iv[6 + vc];
+    const double p2 = pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2];
+    const double coeff = p2 - r2 - R2;
+    const double uz = iv[6 + vc];
+    const double B = 4 * pd;
+    const double C = 2 * coeff + 4 * pd * pd + 4 * R2 * uz * uz;
+    const double D = 4 * pd * coeff + 8 * R2 * pp[2] * uz;
+    const double E = coeff * coeff - 4 * R2 * (r2 - pp[2] * pp[2]);
+    double roots[4];
+    const int nr = sturm::quartic(roots, B, C, D, E);
+    const double t0 = t[0];
+    int ni = 0;
+    for (int i = 0; i < nr; i++) {
+        if (roots[i] < 0) continue;
+        double pt[3] = {o1[0], o1[1], o1[2]};
+        pt[vc] += roots[i];
+        const double dd = (sc[0] - pt[0]) * (sc[0] - pt[0]) + (sc[1] - pt[1]) * (sc[1] - pt[1]) +
+                          (sc[2] - pt[2]) * (sc[2] - pt[2]);
+        if (dd < radius * radius) t[ni++] = t0 + roots[i] / ray_dir;
+    }
+    return ni;
+}
+
 __device__ __forceinline__ void push_hit(const DevRays &R, int slot, int64_t ray, double t, int patch) {
     const int64_t r = (int64_t)slot * R.N * R.N + ray;
     const int pos = atomicAdd(&R.cnt[r], 1);
@@ -405,9 +656,10 @@ enum { PC_CAND, PC_DISC, PC_SPHERE, PC_TORUS, PC_ROOTS, PC_HITS, PC_FEAS_ITEMS, 
 // the launch) and are summed by k_reduce_cols: one atomic per workgroup on a
 // shared address would serialise 10^5 workgroups at the L2.
 // kTorus selects the class of patches (P.order): the plain launch covers
-// spheres and facets, the torus launch the tori.  kMixed: torus roots from the
-// mixed-precision solver.  kCount: instead of hits, per-workgroup path counts.
-template <bool kCount, bool kMixed, bool kTorus>
+// spheres and facets, the torus launch the tori.  kRoots: torus roots from the
+// closed form (kRootsClosed), the mixed-precision solver (kRootsMixed) or Sturm
+// bisection (kRootsSturm).  kCount: instead of hits, per-workgroup path counts.
+template <bool kCount, int kRoots, bool kTorus>
 __global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRays R, double probe,
                                                   unsigned long long *part) {
     const int first = kTorus ? P.n_plain : 0;
@@ -470,8 +722,12 @@ __global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRa
         if (kCount) ++pc[PC_SPHERE];
         int ni = 2;
         if (kTorus) {
-            ni = kMixed ? ray_torus_mixed(pd, sc, probe, radius, vc, pa, ray_dir, t)
-                        : ray_torus(pd, sc, probe, radius, vc, pa, ray_dir, t);
+            if (kRoots == kRootsMixed)
+                ni = ray_torus_mixed(pd, sc, probe, radius, vc, pa, ray_dir, t);
+            else if (kRoots == kRootsSturm)
+                ni = ray_torus_sturm(pd, sc, probe, radius, vc, pa, ray_dir, t);
+            else
+                ni = ray_torus(pd, sc, probe, radius, vc, pa, ray_dir, t);
             if (kCount) ++pc[PC_TORUS];
         }
         for (int i = 0; i < ni; i++) {
@@ -708,30 +964,32 @@ int64_t sort_blocks(const DevRays &R) { return ((int64_t)R.N * R.N * R.nslots + 
 
 // part: intersect_workgroups() rows, zeroed by the caller.  Two launches: the
 // plain patches, then the tori (they need about twice the registers).
-template <bool kCount, bool kMixed>
+template <bool kCount, int kRoots>
 static void launch_classes(hipStream_t s, const DevGrid &g, const DevPatches &P, const DevRays &R, double probe,
                            unsigned long long *part) {
     if (P.n == 0 || R.nslots == 0) return;
     if (P.n_plain > 0)
-        hipLaunchKernelGGL((k_intersect<kCount, kMixed, false>), dim3((unsigned)P.n_plain, (unsigned)R.nslots),
+        hipLaunchKernelGGL((k_intersect<kCount, kRoots, false>), dim3((unsigned)P.n_plain, (unsigned)R.nslots),
                            dim3(64), 0, s, g, P, R, probe, part);
     if (P.n > P.n_plain)
-        hipLaunchKernelGGL((k_intersect<kCount, kMixed, true>), dim3((unsigned)(P.n - P.n_plain), (unsigned)R.nslots),
+        hipLaunchKernelGGL((k_intersect<kCount, kRoots, true>), dim3((unsigned)(P.n - P.n_plain), (unsigned)R.nslots),
                            dim3(64), 0, s, g, P, R, probe, part);
 }
 void launch_intersect(hipStream_t s, const DevGrid &g, const DevPatches &P, const DevRays &R, double probe,
-                      unsigned long long *part, bool mixed) {
-    if (mixed)
-        launch_classes<false, true>(s, g, P, R, probe, part);
+                      unsigned long long *part, int roots) {
+    if (roots == kRootsMixed)
+        launch_classes<false, kRootsMixed>(s, g, P, R, probe, part);
+    else if (roots == kRootsSturm)
+        launch_classes<false, kRootsSturm>(s, g, P, R, probe, part);
     else
-        launch_classes<false, false>(s, g, P, R, probe, part);
+        launch_classes<false, kRootsClosed>(s, g, P, R, probe, part);
 }
 // Counting variant: same candidate walk, no hit output; part gets PC_N
 // counters per workgroup (zeroed by the caller): how often each FP64 path of
 // the reference's candidate set runs.
 void launch_intersect_count(hipStream_t s, const DevGrid &g, const DevPatches &P, const DevRays &R, double probe,
                             unsigned long long *part) {
-    launch_classes<true, false>(s, g, P, R, probe, part);
+    launch_classes<true, kRootsClosed>(s, g, P, R, probe, part);
 }
 
 // part: sort_blocks() rows of 3

@@ -51,6 +51,7 @@ def load():
                                          VP, VP, VP, VP]),
         "orc_is_feasible": (ctypes.c_int, [ctypes.POINTER(Patches), ctypes.c_int, VP]),
         "orc_set_root_refinement": (None, [ctypes.c_int]),
+        "orc_sturm_quartic": (ctypes.c_int, [VP, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -150,11 +150,219 @@ void orc_quartic(double roots[4], double b, double c, double d, double e, int *n
     }
 }
 
+/* ------------------------------------------------------------------------- */
+/* Sturm-sequence roots, the path of Analytical_Ray_Vs_Torus_Intersection =   */
+/* false: getRealRootsSturm (tools.cpp:199-255) over sturm/sturm.cpp:20-281    */
+/* and sturm/util.cpp:17-112 (RELERROR 1e-7, MAXPOW 32, MAXIT 800,             */
+/* SMALL_ENOUGH 1e-12, solve.h:12-24).                                        */
+/* ------------------------------------------------------------------------- */
+#define ORC_RELERROR 1.0e-7
+#define ORC_MAXPOW 32
+#define ORC_MAXIT 800
+#define ORC_SMALL_ENOUGH 1.0e-12
+typedef struct {
+    int ord;
+    double coef[5];
+} orc_poly;
+
+static double orc_evalpoly(int ord, const double *coef, double x) {
+    double f = coef[ord];
+    for (int i = ord - 1; i >= 0; i--) f = x * f + coef[i];
+    return f;
+}
+
+/* remainder of u / v into r (modp) */
+static int orc_modp(const orc_poly *u, const orc_poly *v, orc_poly *r) {
+    for (int i = 0; i <= u->ord; i++) r->coef[i] = u->coef[i];
+    if (v->coef[v->ord] < 0.0) {
+        for (int k = u->ord - v->ord - 1; k >= 0; k -= 2) r->coef[k] = -r->coef[k];
+        for (int k = u->ord - v->ord; k >= 0; k--)
+            for (int j = v->ord + k - 1; j >= k; j--) r->coef[j] = -r->coef[j] - r->coef[v->ord + k] * v->coef[j - k];
+    } else {
+        for (int k = u->ord - v->ord; k >= 0; k--)
+            for (int j = v->ord + k - 1; j >= k; j--) r->coef[j] -= r->coef[v->ord + k] * v->coef[j - k];
+    }
+    int k = v->ord - 1;
+    while (k >= 0 && fabs(r->coef[k]) < ORC_SMALL_ENOUGH) {
+        r->coef[k] = 0.0;
+        k--;
+    }
+    r->ord = (k < 0) ? 0 : k;
+    return r->ord;
+}
+
+static int orc_buildsturm(int ord, orc_poly *sseq) {
+    sseq[0].ord = ord;
+    sseq[1].ord = ord - 1;
+    double f = fabs(sseq[0].coef[ord] * ord);
+    for (int i = 1; i <= ord; i++) sseq[1].coef[i - 1] = sseq[0].coef[i] * i / f;
+    int sp = 2;
+    for (; orc_modp(&sseq[sp - 2], &sseq[sp - 1], &sseq[sp]); sp++) {
+        f = -fabs(sseq[sp].coef[sseq[sp].ord]);
+        for (int i = sseq[sp].ord; i >= 0; i--) sseq[sp].coef[i] /= f;
+    }
+    sseq[sp].coef[0] = -sseq[sp].coef[0];
+    return sp;
+}
+
+static int orc_numroots(int np, const orc_poly *sseq, int *atneg, int *atpos) {
+    int atposinf = 0, atneginf = 0;
+    double lf = sseq[0].coef[sseq[0].ord], f;
+    for (int s = 1; s <= np; s++) {
+        f = sseq[s].coef[sseq[s].ord];
+        if (lf == 0.0 || lf * f < 0) atposinf++;
+        lf = f;
+    }
+    lf = (sseq[0].ord & 1) ? -sseq[0].coef[sseq[0].ord] : sseq[0].coef[sseq[0].ord];
+    for (int s = 1; s <= np; s++) {
+        f = (sseq[s].ord & 1) ? -sseq[s].coef[sseq[s].ord] : sseq[s].coef[sseq[s].ord];
+        if (lf == 0.0 || lf * f < 0) atneginf++;
+        lf = f;
+    }
+    *atneg = atneginf;
+    *atpos = atposinf;
+    return atneginf - atposinf;
+}
+
+static int orc_numchanges(int np, const orc_poly *sseq, double a) {
+    int changes = 0;
+    double lf = orc_evalpoly(sseq[0].ord, sseq[0].coef, a);
+    for (int s = 1; s <= np; s++) {
+        double f = orc_evalpoly(sseq[s].ord, sseq[s].coef, a);
+        if (lf == 0.0 || lf * f < 0) changes++;
+        lf = f;
+    }
+    return changes;
+}
+
+/* modified regula falsi on [a, b] */
+static int orc_modrf(int ord, const double *coef, double a, double b, double *val) {
+    double fa, fb, x, fx, lfx;
+    fb = fa = coef[ord];
+    for (int i = ord - 1; i >= 0; i--) {
+        fa = a * fa + coef[i];
+        fb = b * fb + coef[i];
+    }
+    if (fa * fb > 0.0) return 0;
+    if (fabs(fa) < ORC_RELERROR) {
+        *val = a;
+        return 1;
+    }
+    if (fabs(fb) < ORC_RELERROR) {
+        *val = b;
+        return 1;
+    }
+    lfx = fa;
+    for (int its = 0; its < ORC_MAXIT; its++) {
+        x = (fb * a - fa * b) / (fb - fa);
+        fx = coef[ord];
+        for (int i = ord - 1; i >= 0; i--) fx = x * fx + coef[i];
+        if (fabs(x) > ORC_RELERROR) {
+            if (fabs(fx / x) < ORC_RELERROR) {
+                *val = x;
+                return 1;
+            }
+        } else if (fabs(fx) < ORC_RELERROR) {
+            *val = x;
+            return 1;
+        }
+        if ((fa * fx) < 0) {
+            b = x;
+            fb = fx;
+            if ((lfx * fx) > 0) fa /= 2;
+        } else {
+            a = x;
+            fa = fx;
+            if ((lfx * fx) > 0) fb /= 2;
+        }
+        lfx = fx;
+    }
+    return 0;
+}
+
+static void orc_sbisect(int np, const orc_poly *sseq, double min, double max, int atmin, int atmax, double *roots) {
+    double mid = 0;
+    int n1 = 0, n2 = 0, its, atmid;
+    if (atmin - atmax == 1) {
+        if (orc_modrf(sseq->ord, sseq->coef, min, max, &roots[0])) return;
+        for (its = 0; its < ORC_MAXIT; its++) {
+            mid = (min + max) / 2;
+            atmid = orc_numchanges(np, sseq, mid);
+            if ((atmid < atmax) || (atmid > atmin)) return;
+            if (fabs(mid) > ORC_RELERROR) {
+                if (fabs((max - min) / mid) < ORC_RELERROR) {
+                    roots[0] = mid;
+                    return;
+                }
+            } else if (fabs(max - min) < ORC_RELERROR) {
+                roots[0] = mid;
+                return;
+            }
+            if ((atmin - atmid) == 0) min = mid;
+            else max = mid;
+        }
+        roots[0] = mid;
+        return;
+    }
+    for (its = 0; its < ORC_MAXIT; its++) {
+        mid = (min + max) / 2;
+        atmid = orc_numchanges(np, sseq, mid);
+        if ((atmid < atmax) || (atmid > atmin)) return;
+        n1 = atmin - atmid;
+        n2 = atmid - atmax;
+        if (n1 != 0 && n2 != 0) {
+            orc_sbisect(np, sseq, min, mid, atmin, atmid, roots);
+            orc_sbisect(np, sseq, mid, max, atmid, atmax, &roots[n1]);
+            break;
+        }
+        if (n1 == 0) min = mid;
+        else max = mid;
+    }
+    if (its == ORC_MAXIT)
+        for (n1 = atmax; n1 < atmin; n1++) roots[n1 - atmax] = mid;
+}
+
+/* roots of x^4 + B x^3 + C x^2 + D x + E; a root the reference leaves
+ * unwritten (its bisection gives up) is NaN here and is dropped by the
+ * caller's filters */
+int orc_sturm_quartic(double roots[4], double B, double C, double D, double E) {
+    orc_poly sseq[5];
+    int atmin, atmax, nchanges;
+    sseq[0].coef[0] = E;
+    sseq[0].coef[1] = D;
+    sseq[0].coef[2] = C;
+    sseq[0].coef[3] = B;
+    sseq[0].coef[4] = 1.0;
+    for (int i = 0; i < 4; i++) roots[i] = NAN;
+    int np = orc_buildsturm(4, sseq);
+    int nroots = orc_numroots(np, sseq, &atmin, &atmax);
+    if (nroots == 0) return 0;
+    double min = -1.0;
+    nchanges = orc_numchanges(np, sseq, min);
+    for (int i = 0; nchanges != atmin && i != ORC_MAXPOW; i++) {
+        min *= 10.0;
+        nchanges = orc_numchanges(np, sseq, min);
+    }
+    if (nchanges != atmin) atmin = nchanges;
+    double max = 1.0;
+    nchanges = orc_numchanges(np, sseq, max);
+    for (int i = 0; nchanges != atmax && i != ORC_MAXPOW; i++) {
+        max *= 10.0;
+        nchanges = orc_numchanges(np, sseq, max);
+    }
+    if (nchanges != atmax) atmax = nchanges;
+    nroots = atmin - atmax;
+    orc_sbisect(np, sseq, min, max, atmin, atmax, roots);
+    return nroots;
+}
+
 /* Not part of the reference: when set, every torus root is refined by
  * Newton's method in extended precision on the same double coefficients.
  * That gives the true root of the computed quartic, the yardstick for the
  * device's mixed-precision mode, whose roots are Newton-polished. */
 static int g_refine_roots = 0;
+/* Analytical_Ray_Vs_Torus_Intersection = false: torus roots by Sturm bisection */
+static int g_sturm_roots = 0;
 void orc_set_root_refinement(int on) { g_refine_roots = on; }
 
 static double orc_refine_root(double s, double B, double C, double D, double E) {
@@ -207,7 +415,10 @@ int orc_ray_torus(const double *invrot /*3x3 row major*/, const double *torus_ce
     double D = 4 * pd * coeff + 8 * R2 * pp[2] * invrot[6 + vc];
     double E = coeff * coeff - 4 * R2 * (r2 - pp[2] * pp[2]);
     int numroots;
-    orc_quartic(roots, B, C, D, E, &numroots);
+    if (g_sturm_roots)
+        numroots = orc_sturm_quartic(roots, B, C, D, E);
+    else
+        orc_quartic(roots, B, C, D, E, &numroots);
     *numInt = 0;
     if (numroots == 0) return 0;
     double t0 = t[0];
@@ -1020,6 +1231,7 @@ int orc_get_surf(orc_surface *s, const nsgpu_surf_opts *o, nsgpu_surf_result *ou
     free_results(s);
     memset(&s->res, 0, sizeof(s->res));
     s->o = *o;
+    g_sturm_roots = o->precision == NSGPU_PREC_STURM;
     /* panels rule, Surface.cpp:652-659 */
     s->panels[0] = 3;
     s->panels[1] = 3;
@@ -1039,6 +1251,7 @@ int orc_get_surf(orc_surface *s, const nsgpu_surf_opts *o, nsgpu_surf_result *ou
     memset(s->bits, 0xff, nwords * sizeof(uint32_t));
 
     patch_intersections(s);
+    g_sturm_roots = 0;
 
     /* numPanels of the fill loop, Surface.cpp:836-842 */
     int fillPanels = 3;

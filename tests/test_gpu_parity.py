@@ -217,3 +217,14 @@ def _torus_distance(grid, ps, slot, ray, t):
         gr = 4 * s2 * p - 8 * R * R * np.array([p[0], p[1], ld(0)])
         best = min(best, float(abs(f) / np.sqrt(gr @ gr)))
     return best
+
+
+@pytest.mark.parametrize("n,radius,scale,perfil,seed", [
+    (330, 12.0, 2.0, 90.0, 0),
+    (800, 16.0, 3.0, 80.0, 5),
+])
+def test_sturm_roots_match_oracle(device_ok, n, radius, scale, perfil, seed):
+    # Analytical_Ray_Vs_Torus_Intersection = false: torus roots by Sturm
+    # bisection, restated in the kernel and in the oracle
+    grid, ps, dev, orc = _surfaces(n, radius, scale, perfil, seed)
+    _compare(grid, dev, orc, dict(precision=2))

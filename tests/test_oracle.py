@@ -150,6 +150,28 @@ def test_ray_torus_matches_implicit_quartic(orc):
     assert checked > 200
 
 
+def test_sturm_quartic_roots(orc):
+    # getRealRootsSturm: distinct real roots in ascending order to RELERROR
+    r = np.zeros(4)
+    n = orc.orc_sturm_quartic(_ptr(r), 0.0, -10.0, 0.0, 9.0)  # (x^2 - 1)(x^2 - 9)
+    assert n == 4 and np.allclose(r, [-3, -1, 1, 3], atol=1e-6)
+    assert orc.orc_sturm_quartic(_ptr(r), 0.0, 2.0, 0.0, 1.0) == 0  # (x^2 + 1)^2
+    rng = np.random.default_rng(7)
+    checked = 0
+    for _ in range(300):
+        real = np.sort(rng.uniform(-6, 6, size=int(rng.integers(0, 3)) * 2))
+        cplx = rng.uniform(-3, 3, size=2 - len(real) // 2) + 1j * rng.uniform(0.5, 2, size=2 - len(real) // 2)
+        allr = list(real) + [z for c in cplx for z in (c, np.conj(c))]
+        poly = np.real(np.poly(allr))  # monic, highest first
+        if len(real) > 1 and np.min(np.diff(real)) < 1e-3:
+            continue
+        n = orc.orc_sturm_quartic(_ptr(r), poly[1], poly[2], poly[3], poly[4])
+        assert n == len(real)
+        assert np.all(np.abs(r[:n] - real) <= 1e-6 * np.maximum(1, np.abs(real)))
+        checked += 1
+    assert checked > 250
+
+
 def test_root_refinement_lowers_torus_residual(orc):
     # the extended-precision refinement (the mixed-precision yardstick) must
     # never raise the implicit-torus residual of a hit, must lower it on some,
