@@ -113,6 +113,7 @@ class Surface {
             accurateTriangulation = cf->read<bool>("Accurate_Triangulation", accurateTriangulation);
             probe_radius = cf->read<double>("Probe_Radius", probe_radius);
             computeNormals = cf->read<bool>("Compute_Vertex_Normals", computeNormals);
+            if (!cf->read<bool>("Analytical_Ray_Vs_Torus_Intersection", true)) precision = NSGPU_PREC_STURM;
             device = cf->read<int>("Device", device);
         }
     }
