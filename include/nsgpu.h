@@ -210,6 +210,41 @@ int nsgpu_stage_times(nsgpu_ctx *ctx, double *ms, int n);
  * feasible hits, feasibility items tested, feasibility calls. */
 int nsgpu_intersect_path_counts(nsgpu_ctx *ctx, const nsgpu_surf_opts *opts, int64_t *counts, int n);
 
+/* -------------------- triangle-mesh surfaces ------------------------------- */
+/* MeshSurface::load (MeshSurface.cpp:2177-2207; loadOFF :2350-2479 reads the
+ * coordinates as floats) as the context's surface: get_surf then casts the
+ * rays against the triangles (intersect_triangle, :1487-1553, |det| < 1e-5
+ * rejected) instead of SES patches; everything after the ray lists is the
+ * same.  ray_based selects Patch_Based_Algorithm = false (the mode blobby
+ * surfaces require, main.cpp:1154-1161): rays test the triangles of their
+ * cell of the auxiliary 2D grid of at most aux_grid_dim cells per side
+ * (Max_mesh_auxiliary_grid_2d_size, default 100; preProcessPanel :338-575,
+ * getRayIntersection :1756-1855), edge rays with direction pb - pa;
+ * otherwise getPatchIntersectionData over the triangles (:1264-1484).
+ * verts [nv*3], tris [nt*3] (vertex indices).  Needs the grid first. */
+#define NSGPU_MESH_PATCH_BASED 0
+#define NSGPU_MESH_RAY_BASED 1
+int nsgpu_set_mesh(nsgpu_ctx *ctx, const double *verts, int32_t nv, const int32_t *tris, int32_t nt,
+                   int32_t ray_based, int32_t aux_grid_dim);
+
+/* BlobbySurface::build (BlobbySurface.cpp:83-229): the field
+ * sum_atoms exp(B (|p - c|^2 / r^2 - 1)) at the nodes p = x - hside of each
+ * atom's box of +-(int)(cutoff/side + 0.5) nodes, marching cubes at iso 1.0
+ * with interpolated vertices (getVertices, Surface.cpp:8143-8240), one
+ * smoothing pass (smoothSurface :13089-13240), the OFF text round trip
+ * ("%.3f" written, read back as floats), installed as a ray-based mesh
+ * surface.  B >= 0 is replaced by -2.5 (setBlobbyness); the reference's cutoff
+ * is 6 A (BlobbySurface.h:19).  info (may be NULL) receives the size of the
+ * blobby mesh and the area of its marching cubes (before smoothing). */
+int nsgpu_blobby_build(nsgpu_ctx *ctx, const double *xyzr, int32_t n, double B, double cutoff,
+                       int32_t aux_grid_dim, nsgpu_mesh_info *info);
+/* Size and contents of the mesh surface; raw = 1 reads the blobby build's
+ * smoothed vertices before the text round trip.  The field of the last
+ * blobby build: one double per node, flat index (k*ny+j)*nx+i. */
+int nsgpu_surface_mesh_size(nsgpu_ctx *ctx, int64_t *nv, int64_t *nt);
+int nsgpu_read_surface_mesh(nsgpu_ctx *ctx, int32_t raw, double *verts, int32_t *tris);
+int nsgpu_read_blobby_field(nsgpu_ctx *ctx, double *field);
+
 /* -------------------- differentiation (no counterpart in the reference) ---- */
 /* Atoms (x, y, z, r per atom, after jitter) the current patch set was built
  * from; the patch set must carry atom ids (nsgpu_patches.atom). */

@@ -77,6 +77,12 @@ SIGNATURES = {
     "nsgpu_read_normals": (ctypes.c_int, [VP, VP]),
     "nsgpu_set_vertex_normals": (ctypes.c_int, [VP, ctypes.c_int]),
     "nsgpu_read_ray_hits": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_int32, VP, VP]),
+    "nsgpu_set_mesh": (ctypes.c_int, [VP, VP, ctypes.c_int32, VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "nsgpu_blobby_build": (ctypes.c_int, [VP, VP, ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_int32,
+                                          ctypes.POINTER(MeshInfo)]),
+    "nsgpu_surface_mesh_size": (ctypes.c_int, [VP, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
+    "nsgpu_read_surface_mesh": (ctypes.c_int, [VP, ctypes.c_int32, VP, VP]),
+    "nsgpu_read_blobby_field": (ctypes.c_int, [VP, VP]),
     "nsgpu_set_atoms": (ctypes.c_int, [VP, VP, ctypes.c_int32]),
     "nsgpu_surface_gradient": (ctypes.c_int, [VP, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                               VP, VP, ctypes.POINTER(SurfResult)]),

@@ -107,6 +107,17 @@ struct nsgpu_ctx {
     bool hits_have_patches = false;  // the ray lists of the last trace carry patch ids
     bool want_normals = false;       // keep patch ids for the vertex normals (Compute_Vertex_Normals)
     DBuf segmask, anode, acase, anv, ant, avoff, atoff;
+    // triangle-mesh surface (MeshSurface, BlobbySurface): vertices as loaded,
+    // triangles, ray ranges of the auxiliary 2D grid; the blobby build also
+    // keeps its field, its smoothed vertices before the OFF text round trip and
+    // the scratch of the field and smoothing passes
+    bool surface_is_mesh = false;
+    int mesh_ray_based = 1;
+    int64_t mnv = 0, mnt = 0;
+    DBuf mverts, mtris, mrr, mraw, field, fbits, batoms, bbox, bcount, boff, bcur, blist, flag;
+    bool have_field = false, have_raw = false;
+    double m_min2d[3] = {0, 0, 0}, m_scale2d = 1;
+    int m_n2d = 1, m_aux_dim = 100;
     int64_t n_analytic = 0, nverts = 0, ntris = 0, nact = 0;
     double area = 0;
     bool have_trace = false;  // sorted ray lists of the last get_surf / surface_gradient
@@ -213,7 +224,9 @@ void nsgpu_destroy(nsgpu_ctx *c) {
                    &c->rcnt, &c->rt, &c->rvol, &c->partial, &c->stats, &c->cand, &c->cov, &c->covst, &c->eps, &c->ideb, &c->status,
                    &c->bits, &c->crx0, &c->crx1, &c->cpar, &c->crowcnt, &c->croff, &c->nroots, &c->rootlist, &c->bkey, &c->ccount, &c->clabel, &c->cfilled, &c->dcavsize, &c->dcavfilled, &c->hostdone, &c->dres,
                    &c->idx, &c->vslot, &c->vhit, &c->vorig, &c->normals, &c->vcount, &c->voffs, &c->scan_scratch, &c->verts, &c->tris, &c->tarea,
-                   &c->segmask, &c->anode, &c->acase, &c->anv, &c->ant, &c->avoff, &c->atoff};
+                   &c->segmask, &c->anode, &c->acase, &c->anv, &c->ant, &c->avoff, &c->atoff, &c->mverts, &c->mtris,
+                   &c->mrr, &c->mraw, &c->field, &c->fbits, &c->batoms, &c->bbox, &c->bcount, &c->boff, &c->bcur,
+                   &c->blist, &c->flag};
     for (DBuf *b : all) b->release();
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -245,6 +258,8 @@ int nsgpu_stage_times(nsgpu_ctx *c, double *ms, int n) {
     return ST_COUNT;
 }
 
+static int setup_aux_grid(nsgpu_ctx *c, int aux_dim, int ray_based);
+
 int nsgpu_set_grid(nsgpu_ctx *c, const nsgpu_grid *g) {
     if (!c || !g || g->nx <= 1 || g->ny <= 1 || g->nz <= 1) return NSGPU_E_ARG;
     if ((int64_t)g->nx * g->ny * g->nz >= (int64_t)1 << 31)
@@ -265,6 +280,8 @@ int nsgpu_set_grid(nsgpu_ctx *c, const nsgpu_grid *g) {
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->have_grid = true;
     c->have_trace = c->have_surf = c->have_vertices = c->have_mesh = false;
+    c->have_field = false;
+    if (c->surface_is_mesh) return setup_aux_grid(c, c->m_aux_dim, c->mesh_ray_based);
     return NSGPU_OK;
 }
 
@@ -322,6 +339,7 @@ int nsgpu_set_patches(nsgpu_ctx *c, const nsgpu_patches *p, double probe_radius)
     c->npatch_plain = n_plain;
     c->probe = probe_radius;
     c->have_patches = true;
+    c->surface_is_mesh = false;
     c->have_trace = c->have_surf = c->have_vertices = c->have_mesh = false;
     return NSGPU_OK;
 }
@@ -329,6 +347,7 @@ int nsgpu_set_patches(nsgpu_ctx *c, const nsgpu_patches *p, double probe_radius)
 int nsgpu_intersect_path_counts(nsgpu_ctx *c, const nsgpu_surf_opts *o, int64_t *counts, int n) {
     if (!c || !o || !counts || n < 8) return NSGPU_E_ARG;
     if (!c->have_grid || !c->have_patches) return fail(c, NSGPU_E_STATE, "set_grid and set_patches first");
+    if (c->surface_is_mesh) return fail(c, NSGPU_E_STATE, "path counts need an SES patch set");
     HIP_TRY(c, hipSetDevice(c->device));
     int panels[2] = {3, 3};
     if (!o->build_status && !o->build_epsmap) panels[0] = 0;
@@ -384,6 +403,19 @@ int nsgpu_synchronize(nsgpu_ctx *c) {
 // Clears are fill kernels, not memset nodes.
 
 // ray lists: intersection and per-ray sort; statistics into c->dres[0..3]
+static MeshArgs mesh_args(const nsgpu_ctx *c) {
+    MeshArgs a;
+    a.v = c->mverts.as<double>();
+    a.tri = c->mtris.as<int>();
+    a.nt = (int)c->mnt;
+    a.ray_based = c->mesh_ray_based;
+    for (int d = 0; d < 3; ++d) a.min2d[d] = c->m_min2d[d];
+    a.scale2d = c->m_scale2d;
+    a.n2d = c->m_n2d;
+    a.rr = c->mrr.as<int>();
+    return a;
+}
+
 static int enqueue_trace(nsgpu_ctx *c, hipStream_t st) {
     const DevGrid g = dev_grid(c);
     const int64_t N = g.N;
@@ -408,17 +440,21 @@ static int enqueue_trace(nsgpu_ctx *c, hipStream_t st) {
     if (c->record_pid) HIP_TRY(c, c->rpid.ensure((size_t)nrays * c->K * 4));
     const DevRays R = dev_rays(c);
     // per-workgroup partials: candidates of the intersection, (hits, odd, longest) of the sort
-    const int64_t irows = std::max<int64_t>(intersect_workgroups(P, R), 1), srows = sort_blocks(R);
+    const int64_t irows = std::max<int64_t>(c->surface_is_mesh ? mesh_workgroups(c->mnt, R) : intersect_workgroups(P, R), 1),
+                  srows = sort_blocks(R);
     HIP_TRY(c, c->cand.ensure((size_t)irows * 8));
     HIP_TRY(c, c->stats.ensure((size_t)srows * 3 * 8));
     unsigned long long *spart = c->stats.as<unsigned long long>(), *sout = c->dres.as<unsigned long long>();
     launch_fill32(st, c->rcnt.p, 0u, nrays);
     launch_fill32(st, c->cand.p, 0u, irows * 2);
     mark(c, ST_INTERSECT, 0, st);
-    launch_intersect(st, g, P, R, c->probe, c->cand.as<unsigned long long>(),
-                     c->opts.precision == NSGPU_PREC_MIXED   ? kRootsMixed
-                     : c->opts.precision == NSGPU_PREC_STURM ? kRootsSturm
-                                                             : kRootsClosed);
+    if (c->surface_is_mesh)
+        launch_intersect_mesh(st, g, R, mesh_args(c), c->cand.as<unsigned long long>());
+    else
+        launch_intersect(st, g, P, R, c->probe, c->cand.as<unsigned long long>(),
+                         c->opts.precision == NSGPU_PREC_MIXED   ? kRootsMixed
+                         : c->opts.precision == NSGPU_PREC_STURM ? kRootsSturm
+                                                                 : kRootsClosed);
     HIP_TRY(c, hipGetLastError());
     mark(c, ST_INTERSECT, 1, st);
     mark(c, ST_SORT, 0, st);
@@ -594,6 +630,8 @@ static int cavities_on_host(nsgpu_ctx *c) {
 // panel rule of getSurf (Surface.cpp:652-659) and the per-call state
 static int begin_surf(nsgpu_ctx *c, const nsgpu_surf_opts *o) {
     if (!c->have_grid || !c->have_patches) return fail(c, NSGPU_E_STATE, "set_grid and set_patches first");
+    if (c->surface_is_mesh && c->want_normals && o->accurate_triangulation)
+        return fail(c, NSGPU_E_STATE, "vertex normals need an SES patch set");
     if (o->precision != NSGPU_PREC_FP64 && o->precision != NSGPU_PREC_MIXED && o->precision != NSGPU_PREC_STURM)
         return fail(c, NSGPU_E_ARG, "unknown precision mode");
     c->opts = *o;
@@ -926,6 +964,287 @@ static int read_buf(nsgpu_ctx *c, const DBuf &b, void *out, size_t bytes) {
 
 static int64_t nvox_of(const nsgpu_ctx *c) { return (int64_t)c->grid.nx * c->grid.ny * c->grid.nz; }
 
+// ---------------------------------------------------------------------------
+// Triangle-mesh surfaces: MeshSurface (a loaded mesh) and BlobbySurface (the
+// blobby field triangulated, smoothed and reloaded as a mesh).
+
+// Auxiliary 2D grid of MeshSurface::preProcessPanel (MeshSurface.cpp:338-384):
+// cells of gridMul x gridMul rays, gridMul the smallest odd factor that brings
+// the grid to at most aux_dim cells.  A ray tests the triangles registered in
+// its cell (getRayIntersection :1779-1796), so the rays that test a triangle
+// are those whose cell lies in the triangle's cell range; per axis and phase
+// the rays of a cell range form one interval, tabulated here.
+static int setup_aux_grid(nsgpu_ctx *c, int aux_dim, int ray_based) {
+    const nsgpu_grid &G = c->grid;
+    int64_t igrid = G.nx;
+    int gridMul = 1;
+    while (igrid > aux_dim) {
+        gridMul += 2;
+        int64_t digrid = G.nx;
+        while (true) {
+            const int64_t fixedPoint = (digrid + (gridMul - 1)) / gridMul;
+            igrid = fixedPoint * gridMul;
+            if (igrid % 2 == 0) {
+                digrid = igrid + 1;
+            } else {
+                igrid = fixedPoint;
+                break;
+            }
+        }
+    }
+    const double scale2d = G.scale / ((double)gridMul);
+    const double side2d = ((double)gridMul) / G.scale;
+    for (int d = 0; d < 3; ++d) c->m_min2d[d] = G.baricenter[d] - (igrid - 1) * 0.5 * side2d;
+    c->m_scale2d = scale2d;
+    c->m_n2d = (int)igrid;
+    c->m_aux_dim = aux_dim;
+    const int n2d = (int)igrid;
+    std::vector<int> rr((size_t)3 * 2 * 2 * n2d);
+    const double mins[3] = {G.xmin, G.ymin, G.zmin};
+    const int ns[3] = {G.nx, G.ny, G.nz};
+    for (int d = 0; d < 3; ++d)
+        for (int ph = 0; ph < 2; ++ph) {
+            const double delta = ph ? 1e-10 - G.hside : 0.;
+            std::vector<int64_t> cell(ns[d]);
+            for (int m = 0; m < ns[d]; ++m) {
+                const double coord = mins[d] + m * G.side;  // the grid coordinates (DelphiShared.cpp:1135-1142)
+                cell[m] = (int64_t)std::floor(((coord + delta) - c->m_min2d[d]) * scale2d + 0.5);
+                // the reference indexes outside its cell table there
+                if (ray_based && (cell[m] < 0 || cell[m] >= n2d))
+                    return fail(c, NSGPU_E_ARG, "a grid ray falls outside the auxiliary 2D grid");
+            }
+            int *lo = rr.data() + (size_t)((d * 2 + ph) * 2) * n2d, *hi = lo + n2d;
+            for (int q = 0; q < n2d; ++q) {
+                int a = 0;
+                while (a < ns[d] && cell[a] < q) ++a;
+                int b = ns[d] - 1;
+                while (b >= 0 && cell[b] > q) --b;
+                lo[q] = a;
+                hi[q] = b;
+            }
+        }
+    HIP_TRY(c, c->mrr.ensure(rr.size() * 4));
+    HIP_TRY(c, hipMemcpyAsync(c->mrr.p, rr.data(), rr.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return NSGPU_OK;
+}
+
+static void install_mesh(nsgpu_ctx *c, int64_t nv, int64_t nt, int ray_based) {
+    c->mnv = nv;
+    c->mnt = nt;
+    c->mesh_ray_based = ray_based != 0;
+    c->surface_is_mesh = true;
+    c->have_patches = true;
+    c->npatch = c->npatch_plain = 0;
+    c->patch_max_atom = -1;
+    c->have_trace = c->have_surf = c->have_vertices = c->have_mesh = false;
+}
+
+int nsgpu_set_mesh(nsgpu_ctx *c, const double *verts, int32_t nv, const int32_t *tris, int32_t nt, int32_t ray_based,
+                   int32_t aux_grid_dim) {
+    if (!c || nv < 0 || nt < 0 || (nv && !verts) || (nt && !tris) || aux_grid_dim < 1) return NSGPU_E_ARG;
+    if (!c->have_grid) return fail(c, NSGPU_E_STATE, "set_grid first");
+    for (int64_t q = 0; q < 3 * (int64_t)nt; ++q)
+        if (tris[q] < 0 || tris[q] >= nv) return fail(c, NSGPU_E_ARG, "triangle vertex index out of range");
+    for (int64_t q = 0; q < 3 * (int64_t)nv; ++q)
+        if (!std::isfinite(verts[q])) return fail(c, NSGPU_E_ARG, "non-finite vertex coordinate");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, c->mverts.ensure((size_t)std::max(nv, 1) * 24));
+    HIP_TRY(c, c->mtris.ensure((size_t)std::max(nt, 1) * 12));
+    if (nv) HIP_TRY(c, hipMemcpyAsync(c->mverts.p, verts, (size_t)nv * 24, hipMemcpyHostToDevice, c->stream));
+    if (nt) HIP_TRY(c, hipMemcpyAsync(c->mtris.p, tris, (size_t)nt * 12, hipMemcpyHostToDevice, c->stream));
+    int rc = setup_aux_grid(c, aux_grid_dim, ray_based);
+    if (rc) return rc;
+    install_mesh(c, nv, nt, ray_based);
+    c->have_raw = false;
+    return NSGPU_OK;
+}
+
+int nsgpu_blobby_build(nsgpu_ctx *c, const double *xyzr, int32_t n, double B, double cutoff, int32_t aux_grid_dim,
+                       nsgpu_mesh_info *info) {
+    if (!c || n < 0 || (n && !xyzr) || !(cutoff > 0) || aux_grid_dim < 1) return NSGPU_E_ARG;
+    if (!c->have_grid) return fail(c, NSGPU_E_STATE, "set_grid first");
+    if (!(B < 0)) B = -2.5;  // BlobbySurface::setBlobbyness (BlobbySurface.cpp:46-54)
+    HIP_TRY(c, hipSetDevice(c->device));
+    const DevGrid g = dev_grid(c);
+    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
+    hipStream_t st = c->stream;
+    // 1. field: atom/brick pairs, brick lists, per-brick ordered sums
+    BlobbyArgs a;
+    a.n = n;
+    a.B = B;
+    a.ng = (int)(cutoff / c->grid.side + 0.5);
+    blobby_bricks(g, &a.nbx, &a.nby, &a.nbz);
+    const int64_t nb = (int64_t)a.nbx * a.nby * a.nbz;
+    HIP_TRY(c, c->batoms.ensure((size_t)std::max(n, 1) * 32));
+    if (n) HIP_TRY(c, hipMemcpyAsync(c->batoms.p, xyzr, (size_t)n * 32, hipMemcpyHostToDevice, st));
+    HIP_TRY(c, c->bbox.ensure((size_t)std::max(n, 1) * 24));
+    HIP_TRY(c, c->bcount.ensure((size_t)nb * 4));
+    HIP_TRY(c, c->boff.ensure((size_t)nb * 4));
+    HIP_TRY(c, c->bcur.ensure((size_t)nb * 4));
+    HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(nb) * 4));
+    HIP_TRY(c, c->field.ensure((size_t)nvox * 8));
+    HIP_TRY(c, c->flag.ensure(8));
+    a.xyzr = c->batoms.as<double>();
+    a.box = c->bbox.as<int>();
+    a.bcount = c->bcount.as<int>();
+    a.boff = c->boff.as<int>();
+    a.bcur = c->bcur.as<int>();
+    a.field = c->field.as<double>();
+    a.overflow = c->flag.as<int>();
+    launch_fill32(st, c->bcount.p, 0u, nb);
+    launch_fill32(st, c->bcur.p, 0u, nb);
+    launch_fill32(st, c->flag.p, 0u, 2);
+    launch_blobby_pairs(st, g, a, 0);
+    launch_scan(st, a.bcount, nb, a.boff, c->scan_scratch.as<int>());
+    HIP_TRY(c, hipGetLastError());
+    int64_t npairs = 0;
+    int rc = read_total(c, a.bcount, a.boff, nb, &npairs);
+    if (rc) return rc;
+    if (npairs >= INT32_MAX) return fail(c, NSGPU_E_NOMEM, "too many atom/brick pairs");
+    HIP_TRY(c, c->blist.ensure((size_t)std::max<int64_t>(npairs, 1) * 4));
+    a.blist = c->blist.as<int>();
+    launch_blobby_pairs(st, g, a, 1);
+    launch_blobby_field(st, g, a);
+    HIP_TRY(c, hipGetLastError());
+    int over = 0;
+    HIP_TRY(c, hipMemcpyAsync(&over, c->flag.p, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(c, hipStreamSynchronize(st));
+    if (over) return fail(c, NSGPU_E_ARG, "too many atoms reach one 8^3-node brick of the blobby field");
+    c->have_field = true;
+    // 2. marching cubes on the field at iso 1.0 (triangulateSurface(false, false, 1.0), BlobbySurface.cpp:186)
+    HIP_TRY(c, c->fbits.ensure((size_t)((nvox + 31) / 32) * 4));
+    launch_field_bits(st, a.field, nvox, 1.0, c->fbits.as<uint32_t>());
+    const void *old_idx = c->idx.p;
+    HIP_TRY(c, c->idx.ensure((size_t)nvox * 3 * 4));
+    if (c->idx.p != old_idx || c->idx_nvox != nvox || 3 * nvox > INT32_MAX)
+        launch_fill32(st, c->idx.p, 0xffffffffu, nvox * 3);
+    else
+        launch_idx_reset(st, c->idx.as<int>(), c->vslot.as<int>(), c->idx_dirty);
+    c->idx_nvox = nvox;
+    c->idx_dirty = 0;
+    c->have_trace = c->have_surf = c->have_vertices = c->have_mesh = false;
+    const int64_t nseg = mc_num_segments(g);
+    HIP_TRY(c, c->segmask.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
+    HIP_TRY(c, c->vcount.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
+    HIP_TRY(c, c->voffs.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
+    HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(std::max<int64_t>(nseg, 1)) * 4));
+    launch_mc_classify(st, g, c->fbits.as<uint32_t>(), c->segmask.as<uint32_t>(), c->vcount.as<int>());
+    launch_scan(st, c->vcount.as<int>(), nseg, c->voffs.as<int>(), c->scan_scratch.as<int>());
+    HIP_TRY(c, hipGetLastError());
+    int64_t nact = 0;
+    rc = read_total(c, c->vcount.as<int>(), c->voffs.as<int>(), nseg, &nact);
+    if (rc) return rc;
+    const size_t na = (size_t)std::max<int64_t>(nact, 1);
+    HIP_TRY(c, c->anode.ensure(na * 4));
+    HIP_TRY(c, c->acase.ensure(na));
+    HIP_TRY(c, c->anv.ensure(na * 4));
+    HIP_TRY(c, c->ant.ensure(na * 4));
+    HIP_TRY(c, c->avoff.ensure(na * 4));
+    HIP_TRY(c, c->atoff.ensure(na * 4));
+    HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(std::max<int64_t>(std::max<int64_t>(nseg, nact), 1)) * 4));
+    launch_mc_compact(st, g, c->fbits.as<uint32_t>(), c->segmask.as<uint32_t>(), c->voffs.as<int>(), c->anode.as<int>(),
+                      c->acase.as<uint8_t>());
+    launch_mc_counts(st, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->idx.as<int>(), c->anv.as<int>(),
+                     c->ant.as<int>());
+    launch_scan(st, c->anv.as<int>(), nact, c->avoff.as<int>(), c->scan_scratch.as<int>());
+    launch_scan(st, c->ant.as<int>(), nact, c->atoff.as<int>(), c->scan_scratch.as<int>());
+    HIP_TRY(c, hipGetLastError());
+    int64_t nv = 0, nt = 0;
+    rc = read_total(c, c->anv.as<int>(), c->avoff.as<int>(), nact, &nv);
+    if (rc) return rc;
+    rc = read_total(c, c->ant.as<int>(), c->atoff.as<int>(), nact, &nt);
+    if (rc) return rc;
+    if (nv >= INT32_MAX / 3 || nt >= INT32_MAX / 3) return fail(c, NSGPU_E_NOMEM, "blobby mesh too large");
+    HIP_TRY(c, c->verts.ensure((size_t)std::max<int64_t>(nv, 1) * 24));
+    HIP_TRY(c, c->vslot.ensure((size_t)std::max<int64_t>(nv, 1) * 4));
+    HIP_TRY(c, c->vorig.ensure((size_t)std::max<int64_t>(nv, 1) * 4));
+    launch_mc_vwrite(st, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->idx.as<int>(), c->avoff.as<int>(),
+                     0, c->verts.as<double>(), c->vslot.as<int>(), c->vorig.as<int>(), a.field, 1.0);
+    c->idx_dirty = nv;
+    HIP_TRY(c, c->tris.ensure((size_t)std::max<int64_t>(nt, 1) * 12));
+    HIP_TRY(c, c->tarea.ensure((size_t)std::max<int64_t>(nt, 1) * 8));
+    launch_mc_twrite(st, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->idx.as<int>(), c->atoff.as<int>(),
+                     c->verts.as<double>(), c->tris.as<int>(), c->tarea.as<double>(), 0);
+    const int64_t per = sum_partials_count(nt);
+    HIP_TRY(c, c->partial.ensure((size_t)std::max<int64_t>(per, 1) * 8));
+    launch_sum_blocks(st, c->tarea.as<double>(), nt, c->partial.as<double>());
+    HIP_TRY(c, hipGetLastError());
+    // 3. one smoothing pass (smoothSurface, BlobbySurface.cpp:188): per-vertex
+    // incidences in triangle order
+    HIP_TRY(c, c->bcount.ensure((size_t)std::max<int64_t>(nv, 1) * 4));
+    HIP_TRY(c, c->boff.ensure((size_t)std::max<int64_t>(nv, 1) * 4));
+    HIP_TRY(c, c->bcur.ensure((size_t)std::max<int64_t>(nv, 1) * 4));
+    HIP_TRY(c, c->blist.ensure((size_t)std::max<int64_t>(3 * nt, 1) * 4));
+    HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(std::max<int64_t>(nv, 1)) * 4));
+    HIP_TRY(c, c->mraw.ensure((size_t)std::max<int64_t>(nv, 1) * 24));
+    HIP_TRY(c, c->mverts.ensure((size_t)std::max<int64_t>(nv, 1) * 24));
+    HIP_TRY(c, c->mtris.ensure((size_t)std::max<int64_t>(nt, 1) * 12));
+    launch_fill32(st, c->bcount.p, 0u, nv);
+    launch_fill32(st, c->bcur.p, 0u, nv);
+    launch_fill32(st, c->flag.p, 0u, 2);
+    launch_inc_pairs(st, c->tris.as<int>(), nt, c->bcount.as<int>(), nullptr, nullptr, nullptr, 0);
+    launch_scan(st, c->bcount.as<int>(), nv, c->boff.as<int>(), c->scan_scratch.as<int>());
+    launch_inc_pairs(st, c->tris.as<int>(), nt, c->bcount.as<int>(), c->boff.as<int>(), c->bcur.as<int>(),
+                     c->blist.as<int>(), 1);
+    launch_smooth(st, c->verts.as<double>(), c->tris.as<int>(), nv, c->bcount.as<int>(), c->boff.as<int>(),
+                  c->blist.as<int>(), c->mraw.as<double>(), c->flag.as<int>());
+    // 4. the OFF text round trip ("%.3f" written, "%f" read into floats) and the mesh
+    launch_quantize(st, c->mraw.as<double>(), 3 * nv, c->mverts.as<double>());
+    if (nt) HIP_TRY(c, hipMemcpyAsync(c->mtris.p, c->tris.p, (size_t)nt * 12, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(c, hipGetLastError());
+    std::vector<double> part((size_t)per, 0.0);
+    if (per) HIP_TRY(c, hipMemcpyAsync(part.data(), c->partial.p, (size_t)per * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(c, hipMemcpyAsync(&over, c->flag.p, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(c, hipStreamSynchronize(st));
+    if (over) return fail(c, NSGPU_E_STATE, "a blobby mesh vertex is shared by too many triangles");
+    double area = 0;
+    for (double v : part) area += v;
+    rc = setup_aux_grid(c, aux_grid_dim, 1);
+    if (rc) return rc;
+    install_mesh(c, nv, nt, 1);
+    c->have_raw = true;
+    c->have_field = true;
+    if (info) {
+        info->num_vertices = nv;
+        info->num_triangles = nt;
+        info->num_analytic_vertices = 0;
+        info->area = area;
+    }
+    return NSGPU_OK;
+}
+
+int nsgpu_surface_mesh_size(nsgpu_ctx *c, int64_t *nv, int64_t *nt) {
+    if (!c || !nv || !nt) return NSGPU_E_ARG;
+    if (!c->surface_is_mesh) return fail(c, NSGPU_E_STATE, "no mesh surface");
+    *nv = c->mnv;
+    *nt = c->mnt;
+    return NSGPU_OK;
+}
+
+int nsgpu_read_surface_mesh(nsgpu_ctx *c, int32_t raw, double *verts, int32_t *tris) {
+    if (!c) return NSGPU_E_ARG;
+    if (!c->surface_is_mesh) return fail(c, NSGPU_E_STATE, "no mesh surface");
+    if (raw && !c->have_raw) return fail(c, NSGPU_E_STATE, "raw vertices exist for a blobby build only");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (verts && c->mnv)
+        HIP_TRY(c, hipMemcpyAsync(verts, (raw ? c->mraw : c->mverts).p, (size_t)c->mnv * 24, hipMemcpyDeviceToHost,
+                                  c->stream));
+    if (tris && c->mnt)
+        HIP_TRY(c, hipMemcpyAsync(tris, c->mtris.p, (size_t)c->mnt * 12, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return NSGPU_OK;
+}
+
+int nsgpu_read_blobby_field(nsgpu_ctx *c, double *field) {
+    if (!c || !field) return NSGPU_E_ARG;
+    if (!c->have_field) return fail(c, NSGPU_E_STATE, "no blobby field for this grid");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(field, c->field.p, (size_t)nvox_of(c) * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return NSGPU_OK;
+}
+
 int nsgpu_read_epsmap(nsgpu_ctx *c, int32_t *out) { return read_buf(c, c->eps, out, (size_t)nvox_of(c) * 12); }
 int nsgpu_read_idebmap(nsgpu_ctx *c, uint8_t *out) { return read_buf(c, c->ideb, out, (size_t)nvox_of(c)); }
 int nsgpu_read_status(nsgpu_ctx *c, int32_t *out) { return read_buf(c, c->status, out, (size_t)nvox_of(c) * 4); }
@@ -1000,6 +1319,7 @@ int nsgpu_surface_gradient(nsgpu_ctx *c, double *volume, double *area, double *d
                            nsgpu_surf_result *res) {
     if (!c || !volume || !area) return NSGPU_E_ARG;
     if (!c->have_grid || !c->have_patches) return fail(c, NSGPU_E_STATE, "set_grid and set_patches first");
+    if (c->surface_is_mesh) return fail(c, NSGPU_E_STATE, "gradients need an SES patch set");
     if (c->npatch > 0 && c->patch_max_atom < 0) return fail(c, NSGPU_E_STATE, "the patch set carries no atom ids");
     if (c->patch_max_atom >= c->natoms) return fail(c, NSGPU_E_STATE, "set_atoms with the patch set's atoms first");
     HIP_TRY(c, hipSetDevice(c->device));

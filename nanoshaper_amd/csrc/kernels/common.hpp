@@ -55,6 +55,17 @@ struct DevRays {
 // rintp, tools.cpp:43-46
 __device__ __forceinline__ double rintp(double x) { return floor(x + 0.5); }
 
+// appends depth t of patch `patch` to ray `ray` of slot `slot`; a full list
+// only counts (the host regrows K from the longest count and traces again)
+__device__ __forceinline__ void push_hit(const DevRays &R, int slot, int64_t ray, double t, int patch) {
+    const int64_t r = (int64_t)slot * R.N * R.N + ray;
+    const int pos = atomicAdd(&R.cnt[r], 1);
+    if (pos < R.K) {
+        R.t[r * R.K + pos] = t;
+        if (R.pid) R.pid[r * R.K + pos] = patch;
+    }
+}
+
 __device__ __forceinline__ int64_t flat(const DevGrid &g, int i, int j, int k) {
     return ((int64_t)k * g.ny + j) * g.nx + i;
 }

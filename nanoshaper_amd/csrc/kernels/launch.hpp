@@ -93,8 +93,11 @@ void launch_mc_compact(hipStream_t, const DevGrid &, const uint32_t *bits, const
 void launch_mc_counts(hipStream_t, const DevGrid &, int nact, const int *anode, const uint8_t *acase, const int *idx,
                       int *nv, int *nt);
 // vorig[v - base] = 4 * (active cube) + (0, 1, 2 for edge 5, 6, 10) of each mid point
+// field == nullptr: edge mid points; else vertexInterp at iso on the scalar
+// field (one double per node)
 void launch_mc_vwrite(hipStream_t, const DevGrid &, int nact, const int *anode, const uint8_t *acase, int *idx,
-                      const int *voff, int base, double *verts, int *vslot, int *vorig);
+                      const int *voff, int base, double *verts, int *vslot, int *vorig, const double *field = nullptr,
+                      double iso = 0.);
 // vertex normals: analytic at the depths the vertices come from (the ray
 // lists must carry patch ids), mid points from their adjacent triangles
 struct NormalArgs {
@@ -109,7 +112,49 @@ struct NormalArgs {
     double *normals;
 };
 void launch_normals(hipStream_t, const DevGrid &, const DevPatches &, const DevRays &, const NormalArgs &);
+// reverse: triangles written (t2, t1, t0) as for the analytic corner bitmap;
+// the scalar-field marching cubes keeps (t0, t1, t2)
 void launch_mc_twrite(hipStream_t, const DevGrid &, int nact, const int *anode, const uint8_t *acase, const int *idx,
-                      const int *toff, const double *verts, int *tris, double *tarea);
+                      const int *toff, const double *verts, int *tris, double *tarea, int reverse = 1);
+
+// blobby scalar field (BlobbySurface::build)
+struct BlobbyArgs {
+    const double *xyzr;  // 4 per atom
+    int n;
+    double B;
+    int ng;                  // box half-width in nodes, (int)(cutoff / side + 0.5)
+    int nbx, nby, nbz;       // bricks of 8^3 nodes
+    int *box;                // 6 per atom
+    int *bcount, *boff, *bcur, *blist;
+    double *field;           // one per node
+    int *overflow;           // largest brick list when one exceeds blobby_brick_cap()
+};
+void blobby_bricks(const DevGrid &, int *nbx, int *nby, int *nbz);
+int blobby_brick_cap();
+void launch_blobby_pairs(hipStream_t, const DevGrid &, const BlobbyArgs &, int scatter);
+void launch_blobby_field(hipStream_t, const DevGrid &, const BlobbyArgs &);
+void launch_field_bits(hipStream_t, const double *field, int64_t nvox, double iso, uint32_t *bits);
+// smoothing pass and the OFF text round trip of the blobby mesh
+int smooth_inc_cap();
+void launch_inc_pairs(hipStream_t, const int *tris, int64_t nt, int *vcnt, const int *voff, int *vcur, int *inc,
+                      int scatter);
+void launch_smooth(hipStream_t, const double *vin, const int *tris, int64_t nv, const int *vcnt, const int *voff,
+                   const int *inc, double *vout, int *overflow);
+void launch_quantize(hipStream_t, const double *in, int64_t n, double *out);
+// triangle-mesh ray casting
+struct MeshArgs {
+    const double *v;  // 3 per vertex
+    const int *tri;   // 3 per triangle
+    int nt;
+    int ray_based;    // 1: auxiliary-grid candidates (Patch_Based_Algorithm = false), 0: bounding boxes
+    double min2d[3];  // auxiliary grid (preProcessPanel)
+    double scale2d;
+    int n2d;
+    // ray ranges of the auxiliary cells: [((d*2 + phase)*2 + 0)*n2d + c] first ray
+    // along axis d with cell >= c, [...+ 1)*n2d + c] last ray with cell <= c
+    const int *rr;
+};
+int64_t mesh_workgroups(int64_t nt, const DevRays &);
+void launch_intersect_mesh(hipStream_t, const DevGrid &, const DevRays &, const MeshArgs &, unsigned long long *part);
 
 }  // namespace nsg

@@ -303,9 +303,18 @@ __global__ void __launch_bounds__(256) k_mc_counts(DevGrid g, int nact, const in
     nt[q] = t;
 }
 
+// vertexInterp (Surface.cpp:14516-14523): p1 + mu (p2 - p1), mu = (iso - v1) / (v2 - v1)
+__device__ __forceinline__ void vertex_interp(double iso, const double p1[3], const double p2[3], double v1, double v2,
+                                              double *out) {
+    const double mu = (iso - v1) / (v2 - v1);
+    out[0] = p1[0] + mu * (p2[0] - p1[0]);
+    out[1] = p1[1] + mu * (p2[1] - p1[1]);
+    out[2] = p1[2] + mu * (p2[2] - p1[2]);
+}
+
 __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const int *anode, const uint8_t *acase,
                                                    int *idx, const int *voff, int base, double *verts, int *vslot,
-                                                   int *vorig) {
+                                                   int *vorig, const double *field, double iso) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nact) return;
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (int64_t)g.nx * g.ny;
@@ -319,6 +328,34 @@ __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const in
     const bool m5 = edge_on(c, 5) && idx[s5] == -1;
     const bool m6 = edge_on(c, 6) && idx[s6] == -1;
     const bool m10 = edge_on(c, 10) && idx[s10] == -1;
+    if (field) {
+        // corner positions x_i +- hside (the mulTable corners, tools.h:154-162)
+        // and field values of the cube (getVertices :8143-8169)
+        const double xm = g.x[i] - g.hside, xp = g.x[i] + g.hside, ym = g.y[j] - g.hside, yp = g.y[j] + g.hside,
+                     zm = g.z[k] - g.hside, zp = g.z[k] + g.hside;
+        const double c2[3] = {xp, yp, zm}, c5[3] = {xp, ym, zp}, c6[3] = {xp, yp, zp}, c7[3] = {xm, yp, zp};
+        const double f2 = field[f + 1 + sy], f5 = field[f + 1 + sz], f6 = field[f + 1 + sy + sz],
+                     f7 = field[f + sy + sz];
+        if (m5) {
+            vertex_interp(iso, c5, c6, f5, f6, verts + 3 * (int64_t)v);
+            vslot[v] = (int)s5;
+            vorig[v - base] = 4 * q + 0;
+            idx[s5] = v++;
+        }
+        if (m6) {
+            vertex_interp(iso, c6, c7, f6, f7, verts + 3 * (int64_t)v);
+            vslot[v] = (int)s6;
+            vorig[v - base] = 4 * q + 1;
+            idx[s6] = v++;
+        }
+        if (m10) {
+            vertex_interp(iso, c2, c6, f2, f6, verts + 3 * (int64_t)v);
+            vslot[v] = (int)s10;
+            vorig[v - base] = 4 * q + 2;
+            idx[s10] = v++;
+        }
+        return;
+    }
     if (m5) {
         verts[3 * (int64_t)v] = g.x[i + 1] - g.hside;
         verts[3 * (int64_t)v + 1] = g.y[j];
@@ -347,7 +384,7 @@ __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const in
 
 __global__ void __launch_bounds__(256) k_mc_twrite(DevGrid g, int nact, const int *anode, const uint8_t *acase,
                                                    const int *idx, const int *toff, const double *verts, int *tris,
-                                                   double *tarea) {
+                                                   double *tarea, int reverse) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nact) return;
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (int64_t)g.nx * g.ny;
@@ -362,9 +399,9 @@ __global__ void __launch_bounds__(256) k_mc_twrite(DevGrid g, int nact, const in
         const int v1 = idx[edge_slot(e1, f, sy, sz, nvox)];
         const int v2 = idx[edge_slot(e2, f, sy, sz, nvox)];
         if (v0 == -1 || v1 == -1 || v2 == -1) continue;
-        tris[3 * (int64_t)o + 0] = v2;
+        tris[3 * (int64_t)o + 0] = reverse ? v2 : v0;
         tris[3 * (int64_t)o + 1] = v1;
-        tris[3 * (int64_t)o + 2] = v0;
+        tris[3 * (int64_t)o + 2] = reverse ? v0 : v2;
         const double *a0 = verts + 3 * (int64_t)v0, *a1 = verts + 3 * (int64_t)v1, *a2 = verts + 3 * (int64_t)v2;
         const double a = sqrt((a0[0] - a1[0]) * (a0[0] - a1[0]) + (a0[1] - a1[1]) * (a0[1] - a1[1]) +
                               (a0[2] - a1[2]) * (a0[2] - a1[2]));
@@ -439,17 +476,18 @@ void launch_mc_counts(hipStream_t s, const DevGrid &g, int nact, const int *anod
 }
 
 void launch_mc_vwrite(hipStream_t s, const DevGrid &g, int nact, const int *anode, const uint8_t *acase, int *idx,
-                      const int *voff, int base, double *verts, int *vslot, int *vorig) {
+                      const int *voff, int base, double *verts, int *vslot, int *vorig, const double *field,
+                      double iso) {
     if (nact)
         hipLaunchKernelGGL(k_mc_vwrite, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, idx, voff, base,
-                           verts, vslot, vorig);
+                           verts, vslot, vorig, field, iso);
 }
 
 void launch_mc_twrite(hipStream_t s, const DevGrid &g, int nact, const int *anode, const uint8_t *acase,
-                      const int *idx, const int *toff, const double *verts, int *tris, double *tarea) {
+                      const int *idx, const int *toff, const double *verts, int *tris, double *tarea, int reverse) {
     if (nact)
         hipLaunchKernelGGL(k_mc_twrite, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, idx, toff, verts,
-                           tris, tarea);
+                           tris, tarea, reverse);
 }
 
 }  // namespace nsg

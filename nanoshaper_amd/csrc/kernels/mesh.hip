@@ -1,0 +1,395 @@
+// Triangle-mesh surfaces: the blobby build (scalar field, marching cubes on
+// the field, one smoothing pass, the OFF text round trip) and the ray/triangle
+// intersection that replaces the patch intersection for meshes.
+//
+// BlobbySurface::build (BlobbySurface.cpp:83-229) sums
+//   phi(node) = sum over atoms of exp(B (|p - c|^2 / r^2 - 1)),  p = node - hside,
+// over the nodes of each atom's box [ix - ng, ix + ng) (clamped to the grid),
+// atom after atom.  Here every brick of 8^3 nodes gathers the atoms whose
+// boxes overlap it, sorts them by index in LDS and sums them in that order,
+// so each node adds the same terms in the same order as the reference.
+//
+// MeshSurface ray casting (Möller-Trumbore, MeshSurface.cpp:1487-1553): one
+// lane per (triangle, panel slot) walks the rays that test the triangle:
+//   ray-based (Patch_Based_Algorithm = false, the blobby path): the rays whose
+//     auxiliary 2D cell (preProcessPanel :338-575, getRayIntersection
+//     :1756-1855) lies in the triangle's cell range; direction pb - pa;
+//   patch-based (getPatchIntersectionData :1264-1484): the rays of the
+//     triangle's grid rectangle inside its bounding box; direction
+//     x[N-1] - x[0].
+#include "launch.hpp"
+
+namespace nsg {
+
+static inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+// ------------------------------------------------------------ blobby field
+constexpr int kBrick = 8;         // nodes per brick side
+constexpr int kBrickCap = 2048;   // atoms per brick list (LDS)
+
+// box [s, e) of nodes per axis of atom a (BlobbySurface.cpp:125-155)
+__device__ __forceinline__ void atom_box(const DevGrid &g, const double *xyzr, int ng, int b[6]) {
+    const int ix = (int)rintp((xyzr[0] - g.xmin) * g.scale);
+    const int iy = (int)rintp((xyzr[1] - g.ymin) * g.scale);
+    const int iz = (int)rintp((xyzr[2] - g.zmin) * g.scale);
+    b[0] = max(0, ix - ng);
+    b[1] = min(g.nx, ix + ng);
+    b[2] = max(0, iy - ng);
+    b[3] = min(g.ny, iy + ng);
+    b[4] = max(0, iz - ng);
+    b[5] = min(g.nz, iz + ng);
+}
+
+// per atom: its box (6 ints) and, per overlapped brick, a count or a list entry
+__global__ void __launch_bounds__(256) k_blobby_pairs(DevGrid g, BlobbyArgs a, int scatter) {
+    const int at = blockIdx.x * blockDim.x + threadIdx.x;
+    if (at >= a.n) return;
+    int b[6];
+    atom_box(g, a.xyzr + 4 * (int64_t)at, a.ng, b);
+    if (!scatter)
+        for (int q = 0; q < 6; ++q) a.box[6 * at + q] = b[q];
+    if (b[0] >= b[1] || b[2] >= b[3] || b[4] >= b[5]) return;
+    const int bx0 = b[0] / kBrick, bx1 = (b[1] - 1) / kBrick;
+    const int by0 = b[2] / kBrick, by1 = (b[3] - 1) / kBrick;
+    const int bz0 = b[4] / kBrick, bz1 = (b[5] - 1) / kBrick;
+    for (int bz = bz0; bz <= bz1; ++bz)
+        for (int by = by0; by <= by1; ++by)
+            for (int bx = bx0; bx <= bx1; ++bx) {
+                const int br = (bz * a.nby + by) * a.nbx + bx;
+                if (!scatter) {
+                    atomicAdd(&a.bcount[br], 1);
+                } else {
+                    const int pos = a.boff[br] + atomicAdd(&a.bcur[br], 1);
+                    a.blist[pos] = at;
+                }
+            }
+}
+
+// one workgroup per brick, two nodes per lane
+__global__ void __launch_bounds__(256) k_blobby_field(DevGrid g, BlobbyArgs a) {
+    __shared__ int list[kBrickCap];
+    const int br = blockIdx.x;
+    const int c = a.bcount[br];
+    if (c > kBrickCap) {
+        if (threadIdx.x == 0) atomicMax(a.overflow, c);
+        return;
+    }
+    int np = 1;
+    while (np < c) np <<= 1;
+    for (int q = threadIdx.x; q < np; q += blockDim.x) list[q] = q < c ? a.blist[a.boff[br] + q] : 0x7fffffff;
+    __syncthreads();
+    // bitonic sort of the atom ids
+    for (int k = 2; k <= np; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int q = threadIdx.x; q < np; q += blockDim.x) {
+                const int o = q ^ j;
+                if (o > q) {
+                    const int x = list[q], y = list[o];
+                    const bool up = (q & k) == 0;
+                    if ((x > y) == up) {
+                        list[q] = y;
+                        list[o] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    const int bx = br % a.nbx, by = (br / a.nbx) % a.nby, bz = br / (a.nbx * a.nby);
+    const int li = threadIdx.x & 7, lj = (threadIdx.x >> 3) & 7, lk = threadIdx.x >> 6;  // lk 0..3
+    const int ii = bx * kBrick + li, jj = by * kBrick + lj;
+    const int kk0 = bz * kBrick + lk, kk1 = kk0 + 4;
+    const bool inx = ii < g.nx && jj < g.ny;
+    const bool v0 = inx && kk0 < g.nz, v1 = inx && kk1 < g.nz;
+    double p[3] = {0., 0., 0.}, pz1 = 0.;
+    if (inx) {
+        p[0] = g.x[ii] - g.hside;
+        p[1] = g.y[jj] - g.hside;
+    }
+    if (v0) p[2] = g.z[kk0] - g.hside;
+    if (v1) pz1 = g.z[kk1] - g.hside;
+    double f0 = 0., f1 = 0.;
+    for (int q = 0; q < c; ++q) {
+        const int at = __builtin_amdgcn_readfirstlane(list[q]);
+        const int *b = a.box + 6 * at;
+        const double *pos = a.xyzr + 4 * (int64_t)at;
+        const bool inxy = ii >= b[0] && ii < b[1] && jj >= b[2] && jj < b[3];
+        const double r = pos[3];
+        const double r2 = r * r;
+        if (v0 && inxy && kk0 >= b[4] && kk0 < b[5]) {
+            const double d2 = (p[0] - pos[0]) * (p[0] - pos[0]) + (p[1] - pos[1]) * (p[1] - pos[1]) +
+                              (p[2] - pos[2]) * (p[2] - pos[2]);
+            f0 += exp(a.B * (d2 / r2 - 1));
+        }
+        if (v1 && inxy && kk1 >= b[4] && kk1 < b[5]) {
+            const double d2 = (p[0] - pos[0]) * (p[0] - pos[0]) + (p[1] - pos[1]) * (p[1] - pos[1]) +
+                              (pz1 - pos[2]) * (pz1 - pos[2]);
+            f1 += exp(a.B * (d2 / r2 - 1));
+        }
+    }
+    if (v0) a.field[flat(g, ii, jj, kk0)] = f0;
+    if (v1) a.field[flat(g, ii, jj, kk1)] = f1;
+}
+
+// corner bits of the scalar marching cubes: bit 0 where phi < iso (the case
+// bit classifyCube sets, Surface.cpp:14526-14547), 1 elsewhere and past the grid
+__global__ void __launch_bounds__(256) k_field_bits(const double *field, int64_t nvox, double iso, uint32_t *bits) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= (nvox + 31) >> 5) return;
+    uint32_t m = 0;
+    for (int b = 0; b < 32; ++b) {
+        const int64_t f = (w << 5) + b;
+        if (f >= nvox || !(field[f] < iso)) m |= 1u << b;
+    }
+    bits[w] = m;
+}
+
+void blobby_bricks(const DevGrid &g, int *nbx, int *nby, int *nbz) {
+    *nbx = (g.nx + kBrick - 1) / kBrick;
+    *nby = (g.ny + kBrick - 1) / kBrick;
+    *nbz = (g.nz + kBrick - 1) / kBrick;
+}
+int blobby_brick_cap() { return kBrickCap; }
+
+void launch_blobby_pairs(hipStream_t s, const DevGrid &g, const BlobbyArgs &a, int scatter) {
+    if (a.n > 0) hipLaunchKernelGGL(k_blobby_pairs, dim3(nblk(a.n)), dim3(256), 0, s, g, a, scatter);
+}
+void launch_blobby_field(hipStream_t s, const DevGrid &g, const BlobbyArgs &a) {
+    const int64_t nb = (int64_t)a.nbx * a.nby * a.nbz;
+    if (nb > 0) hipLaunchKernelGGL(k_blobby_field, dim3((unsigned)nb), dim3(256), 0, s, g, a);
+}
+void launch_field_bits(hipStream_t s, const double *field, int64_t nvox, double iso, uint32_t *bits) {
+    const int64_t nw = (nvox + 31) >> 5;
+    if (nw > 0) hipLaunchKernelGGL(k_field_bits, dim3(nblk(nw)), dim3(256), 0, s, field, nvox, iso, bits);
+}
+
+// ------------------------------------------------------------ smoothing
+// Surface::smoothSurface (Surface.cpp:13089-13240): every triangle adds the
+// sum of its other two vertices to each of its vertices, in triangle order;
+// then v <- 0.5 (v + sum / (2 * triangles of v)).  The incidences of a vertex
+// are collected (3 t + position), sorted, and summed in triangle order.
+constexpr int kIncCap = 32;
+
+__global__ void __launch_bounds__(256) k_inc_pairs(const int *tris, int64_t nt, int *vcnt, const int *voff, int *vcur,
+                                                   int *inc, int scatter) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    for (int q = 0; q < 3; ++q) {
+        const int v = tris[3 * t + q];
+        if (!scatter) {
+            atomicAdd(&vcnt[v], 1);
+        } else {
+            const int pos = voff[v] + atomicAdd(&vcur[v], 1);
+            inc[pos] = (int)(3 * t + q);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_smooth(const double *vin, const int *tris, int64_t nv, const int *vcnt,
+                                                const int *voff, const int *inc, double *vout, int *overflow) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    const int c = vcnt[v];
+    if (c > kIncCap) {
+        atomicMax(overflow, c);
+        return;
+    }
+    int e[kIncCap];
+    for (int q = 0; q < c; ++q) {
+        const int x = inc[voff[v] + q];
+        int p = q;
+        while (p > 0 && e[p - 1] > x) {
+            e[p] = e[p - 1];
+            --p;
+        }
+        e[p] = x;
+    }
+    double s0 = 0., s1 = 0., s2 = 0.;
+    for (int q = 0; q < c; ++q) {
+        const int t = e[q] / 3, pos = e[q] - 3 * t;
+        const int *tv = tris + 3 * (int64_t)t;
+        // the other two vertices in the reference's order: (v2, v3), (v1, v3), (v1, v2)
+        const int oa = pos == 0 ? tv[1] : tv[0];
+        const int ob = pos == 2 ? tv[1] : tv[2];
+        const double *A = vin + 3 * (int64_t)oa, *B = vin + 3 * (int64_t)ob;
+        s0 += A[0] + B[0];
+        s1 += A[1] + B[1];
+        s2 += A[2] + B[2];
+    }
+    const double scale = 1.0 / (double)(2 * c);
+    const double *V = vin + 3 * v;
+    vout[3 * v + 0] = 0.5 * (V[0] + s0 * scale);
+    vout[3 * v + 1] = 0.5 * (V[1] + s1 * scale);
+    vout[3 * v + 2] = 0.5 * (V[2] + s2 * scale);
+}
+
+// "%.3f" (round half to even on the exact binary value, as printf) read back
+// with sscanf("%f") into a float (MeshSurface::loadOFF, MeshSurface.cpp:
+// 2436-2446): the nearest float to n / 1000, n the printed thousandths
+__device__ double off_round_trip(double x) {
+    if (isnan(x)) return x;
+    double n = rint(x * 1000.0);
+    // x*1000 against n +- 0.5 exactly: fma rounds once, so its sign is exact
+    const double hi = fma(x, 1000.0, -(n + 0.5));
+    const double lo = fma(x, 1000.0, -(n - 0.5));
+    if (hi > 0. || (hi == 0. && fmod(n + 1., 2.) == 0.)) n += 1.;
+    else if (lo < 0. || (lo == 0. && fmod(n - 1., 2.) == 0.)) n -= 1.;
+    if (n == 0.) return signbit(x) ? -0.0 : 0.0;  // "-0.000" reads as -0.0f
+    // nearest float to the decimal n/1000: the float of the nearest double,
+    // corrected by comparing the exact distances of it and its neighbours
+    // (f*1000 has at most 34 significant bits, so f*1000 - n is exact)
+    const float f0 = (float)(n / 1000.0);
+    float best = f0;
+    double bd = fabs((double)f0 * 1000.0 - n);
+    const float cand[2] = {nextafterf(f0, -INFINITY), nextafterf(f0, INFINITY)};
+    for (int q = 0; q < 2; ++q) {
+        const double d = fabs((double)cand[q] * 1000.0 - n);
+        if (d < bd) {
+            bd = d;
+            best = cand[q];
+        }
+    }
+    return (double)best;
+}
+
+__global__ void __launch_bounds__(256) k_quantize(const double *in, int64_t n, double *out) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) out[q] = off_round_trip(in[q]);
+}
+
+int smooth_inc_cap() { return kIncCap; }
+
+void launch_inc_pairs(hipStream_t s, const int *tris, int64_t nt, int *vcnt, const int *voff, int *vcur, int *inc,
+                      int scatter) {
+    if (nt > 0) hipLaunchKernelGGL(k_inc_pairs, dim3(nblk(nt)), dim3(256), 0, s, tris, nt, vcnt, voff, vcur, inc, scatter);
+}
+void launch_smooth(hipStream_t s, const double *vin, const int *tris, int64_t nv, const int *vcnt, const int *voff,
+                   const int *inc, double *vout, int *overflow) {
+    if (nv > 0)
+        hipLaunchKernelGGL(k_smooth, dim3(nblk(nv)), dim3(256), 0, s, vin, tris, nv, vcnt, voff, inc, vout, overflow);
+}
+void launch_quantize(hipStream_t s, const double *in, int64_t n, double *out) {
+    if (n > 0) hipLaunchKernelGGL(k_quantize, dim3(nblk(n)), dim3(256), 0, s, in, n, out);
+}
+
+// ------------------------------------------------------------ ray/triangle
+// intersect_triangle, the non-culling branch (MeshSurface.cpp:1525-1550), EPS 1e-5 (MeshSurface.h:21)
+__device__ __forceinline__ bool ray_triangle(const double o[3], const double d[3], const double *v0, const double *v1,
+                                             const double *v2, double &t) {
+    const double e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
+    const double e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+    const double pv[3] = {d[1] * e2[2] - d[2] * e2[1], d[2] * e2[0] - d[0] * e2[2], d[0] * e2[1] - d[1] * e2[0]};
+    const double det = e1[0] * pv[0] + e1[1] * pv[1] + e1[2] * pv[2];
+    if (det > -1e-5 && det < 1e-5) return false;
+    const double inv = 1.0 / det;
+    const double tv[3] = {o[0] - v0[0], o[1] - v0[1], o[2] - v0[2]};
+    const double u = (tv[0] * pv[0] + tv[1] * pv[1] + tv[2] * pv[2]) * inv;
+    if (u < 0.0 || u > 1.0) return false;
+    const double qv[3] = {tv[1] * e1[2] - tv[2] * e1[1], tv[2] * e1[0] - tv[0] * e1[2], tv[0] * e1[1] - tv[1] * e1[0]};
+    const double v = (d[0] * qv[0] + d[1] * qv[1] + d[2] * qv[2]) * inv;
+    if (v < 0.0 || u + v > 1.0) return false;
+    t = (e2[0] * qv[0] + e2[1] * qv[1] + e2[2] * qv[2]) * inv;
+    return true;
+}
+
+__device__ __forceinline__ const double *axis_coords(const DevGrid &g, int d) { return d == 0 ? g.x : (d == 1 ? g.y : g.z); }
+__device__ __forceinline__ int axis_n(const DevGrid &g, int d) { return d == 0 ? g.nx : (d == 1 ? g.ny : g.nz); }
+__device__ __forceinline__ double axis_min(const DevGrid &g, int d) { return d == 0 ? g.xmin : (d == 1 ? g.ymin : g.zmin); }
+__device__ __forceinline__ double axis_max(const DevGrid &g, int d) { return d == 0 ? g.xmax : (d == 1 ? g.ymax : g.zmax); }
+
+// grid: (ceil(nt / 64), nslots); per-workgroup candidate counts to part
+template <bool kRayBased>
+__global__ void __launch_bounds__(64) k_intersect_mesh(DevGrid g, DevRays R, MeshArgs a, unsigned long long *part) {
+    const int slot = blockIdx.y;
+    const int64_t it = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    const int phase = slot >= R.panels0;
+    const int panel = phase ? slot - R.panels0 : slot;
+    unsigned long long cand = 0;
+    if (it < a.nt) {
+        int first, last, vc;
+        if (panel == 0) { first = 1; last = 2; vc = 0; }
+        else if (panel == 1) { first = 0; last = 1; vc = 2; }
+        else { first = 0; last = 2; vc = 1; }
+        const int *tv = a.tri + 3 * it;
+        const double *p0 = a.v + 3 * (int64_t)tv[0], *p1 = a.v + 3 * (int64_t)tv[1], *p2 = a.v + 3 * (int64_t)tv[2];
+        double down[3], up[3];
+        for (int q = 0; q < 3; ++q) {
+            down[q] = INFINITY;
+            up[q] = -INFINITY;
+        }
+        const double *pp[3] = {p0, p1, p2};
+        for (int pi = 0; pi < 3; ++pi)
+            for (int q = 0; q < 3; ++q) {
+                down[q] = down[q] < pp[pi][q] ? down[q] : pp[pi][q];
+                up[q] = up[q] > pp[pi][q] ? up[q] : pp[pi][q];
+            }
+        const double delta = phase ? kDelta - g.hside : 0.;
+        const double *cf = axis_coords(g, first), *cl = axis_coords(g, last), *cv = axis_coords(g, vc);
+        const int nvc = axis_n(g, vc);
+        double pa[3], dir[3] = {0., 0., 0.};
+        pa[vc] = cv[0] + delta;
+        dir[vc] = kRayBased ? (cv[nvc - 1] + delta) - pa[vc] : cv[nvc - 1] - cv[0];
+        int m0, m1, n0, n1;
+        bool skip = false;
+        if (kRayBased) {
+            int64_t cs[2], ce[2];
+            const int dd[2] = {first, last};
+            for (int q = 0; q < 2; ++q) {
+                cs[q] = (int64_t)rintp(fmax(0., down[dd[q]] - a.min2d[dd[q]]) * a.scale2d);
+                ce[q] = (int64_t)rintp(fmax(0., up[dd[q]] - a.min2d[dd[q]]) * a.scale2d);
+                if (cs[q] >= a.n2d) cs[q] = a.n2d - 1;
+                if (ce[q] >= a.n2d) ce[q] = a.n2d - 1;
+            }
+            const int *rf = a.rr + (int64_t)((first * 2 + phase) * 2) * a.n2d;
+            const int *rl = a.rr + (int64_t)((last * 2 + phase) * 2) * a.n2d;
+            m0 = rf[cs[0]];
+            m1 = rf[a.n2d + ce[0]];
+            n0 = rl[cs[1]];
+            n1 = rl[a.n2d + ce[1]];
+        } else {
+            // CELL_CULLING (MeshSurface.cpp:1318-1331)
+            for (int q = 0; q < 3; ++q) {
+                skip |= down[q] > axis_max(g, q) + kEpsInt;
+                skip |= up[q] < axis_min(g, q) - (g.hside - kDelta + kEpsInt);
+            }
+            const int dd[2] = {first, last};
+            int64_t s[2], e[2];
+            for (int q = 0; q < 2; ++q) {
+                s[q] = (int64_t)rintp(fmax(0., down[dd[q]] - axis_min(g, dd[q])) * g.scale);
+                e[q] = (int64_t)rintp(fmax(0., up[dd[q]] - axis_min(g, dd[q])) * g.scale);
+                if (e[q] >= axis_n(g, dd[q])) e[q] = axis_n(g, dd[q]) - 1;
+            }
+            m0 = (int)s[0];
+            m1 = (int)e[0];
+            n0 = (int)s[1];
+            n1 = (int)e[1];
+        }
+        if (!skip)
+            for (int n = n0; n <= n1; ++n) {
+                pa[last] = cl[n] + delta;
+                if (!kRayBased && (pa[last] < down[last] || pa[last] > up[last])) continue;
+                for (int m = m0; m <= m1; ++m) {
+                    pa[first] = cf[m] + delta;
+                    if (!kRayBased && (pa[first] < down[first] || pa[first] > up[first])) continue;
+                    ++cand;
+                    double t;
+                    if (ray_triangle(pa, dir, p0, p1, p2, t)) push_hit(R, slot, (int64_t)n * R.N + m, t, (int)it);
+                }
+            }
+    }
+    for (int off = 32; off > 0; off >>= 1) cand += __shfl_down(cand, off, 64);
+    if (threadIdx.x == 0 && cand) part[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = cand;
+}
+
+int64_t mesh_workgroups(int64_t nt, const DevRays &R) { return (int64_t)nblk(nt, 64) * R.nslots; }
+
+void launch_intersect_mesh(hipStream_t s, const DevGrid &g, const DevRays &R, const MeshArgs &a,
+                           unsigned long long *part) {
+    if (a.nt <= 0 || R.nslots <= 0) return;
+    const dim3 grid(nblk(a.nt, 64), (unsigned)R.nslots);
+    if (a.ray_based)
+        hipLaunchKernelGGL(k_intersect_mesh<true>, grid, dim3(64), 0, s, g, R, a, part);
+    else
+        hipLaunchKernelGGL(k_intersect_mesh<false>, grid, dim3(64), 0, s, g, R, a, part);
+}
+
+}  // namespace nsg

@@ -639,15 +639,6 @@ __device__ int ray_torus_sturm(const double *td, const double sc[3], double prob
     return ni;
 }
 
-__device__ __forceinline__ void push_hit(const DevRays &R, int slot, int64_t ray, double t, int patch) {
-    const int64_t r = (int64_t)slot * R.N * R.N + ray;
-    const int pos = atomicAdd(&R.cnt[r], 1);
-    if (pos < R.K) {
-        R.t[r * R.K + pos] = t;
-        if (R.pid) R.pid[r * R.K + pos] = patch;
-    }
-}
-
 // Path counters of the counting variant (see intersect_path_counts).
 enum { PC_CAND, PC_DISC, PC_SPHERE, PC_TORUS, PC_ROOTS, PC_HITS, PC_FEAS_ITEMS, PC_FEAS_CALLS, PC_N };
 

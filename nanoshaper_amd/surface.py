@@ -173,6 +173,43 @@ class DeviceSurface:
     def set_patches(self, ps):
         self._chk(self._lib.nsgpu_set_patches(self._h, ctypes.byref(ps.view), ps.probe), "set_patches")
 
+    def set_mesh(self, verts, tris, ray_based=True, aux_grid_dim=100):
+        """A triangle mesh as the surface (MeshSurface::load): the rays are cast
+        against its triangles.  ray_based: Patch_Based_Algorithm = false (the
+        auxiliary 2D grid of at most aux_grid_dim cells per side,
+        Max_mesh_auxiliary_grid_2d_size)."""
+        v = np.ascontiguousarray(verts, dtype=np.float64).reshape(-1, 3)
+        t = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1, 3)
+        self._chk(self._lib.nsgpu_set_mesh(self._h, _ptr(v), v.shape[0], _ptr(t), t.shape[0], int(bool(ray_based)),
+                                           int(aux_grid_dim)), "set_mesh")
+
+    def blobby_build(self, xyzr, blobbyness=-2.5, cutoff=6.0, aux_grid_dim=100):
+        """BlobbySurface::build on the device: field, marching cubes at iso 1,
+        one smoothing pass and the OFF text round trip; the mesh becomes the
+        surface (ray-based).  Returns the blobby mesh info (area before
+        smoothing)."""
+        a = np.ascontiguousarray(as_xyzr(xyzr), dtype=np.float64)
+        m = MeshInfo()
+        self._chk(self._lib.nsgpu_blobby_build(self._h, _ptr(a), a.shape[0], float(blobbyness), float(cutoff),
+                                               int(aux_grid_dim), ctypes.byref(m)), "blobby_build")
+        return {f: getattr(m, f) for f, _ in MeshInfo._fields_}
+
+    def surface_mesh(self, raw=False):
+        """(verts, tris) of the mesh surface; raw: the blobby build's smoothed
+        vertices before the text round trip."""
+        nv, nt = ctypes.c_int64(), ctypes.c_int64()
+        self._chk(self._lib.nsgpu_surface_mesh_size(self._h, ctypes.byref(nv), ctypes.byref(nt)), "mesh_size")
+        v = np.empty((nv.value, 3))
+        t = np.empty((nt.value, 3), np.int32)
+        self._chk(self._lib.nsgpu_read_surface_mesh(self._h, int(bool(raw)), _ptr(v), _ptr(t)), "read_surface_mesh")
+        return v, t
+
+    def blobby_field(self):
+        g = self.grid
+        out = np.empty((g.nz, g.ny, g.nx))
+        self._chk(self._lib.nsgpu_read_blobby_field(self._h, _ptr(out)), "read_blobby_field")
+        return out
+
     def set_atoms(self, xyzr):
         """Atoms (after jitter) the current patch set was built from (adjoint)."""
         a = np.ascontiguousarray(as_xyzr(xyzr), dtype=np.float64)

@@ -52,6 +52,14 @@ def load():
         "orc_is_feasible": (ctypes.c_int, [ctypes.POINTER(Patches), ctypes.c_int, VP]),
         "orc_set_root_refinement": (None, [ctypes.c_int]),
         "orc_sturm_quartic": (ctypes.c_int, [VP, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+        "orc_create_mesh": (VP, [ctypes.POINTER(Grid), VP, ctypes.c_int64, VP, ctypes.c_int64, ctypes.c_int,
+                                 ctypes.c_int]),
+        "orc_ray_triangle": (ctypes.c_int, [VP, VP, VP, VP, VP, VP]),
+        "orc_blobby_build": (ctypes.c_int, [ctypes.POINTER(Grid), VP, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                            ctypes.POINTER(VP), ctypes.POINTER(VP), ctypes.POINTER(VP),
+                                            ctypes.POINTER(VP), ctypes.POINTER(ctypes.c_int64),
+                                            ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]),
+        "orc_free_buf": (None, [VP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -65,8 +73,58 @@ def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
+def blobby_build(grid, xyzr, blobbyness=-2.5, cutoff=6.0):
+    """BlobbySurface::build restated: {'field' (nz, ny, nx), 'raw' and 'verts'
+    (nv, 3): smoothed vertices before and after the OFF text round trip,
+    'tris' (nt, 3), 'area' of the marching cubes before smoothing}."""
+    lib = load()
+    a = np.ascontiguousarray(xyzr, dtype=np.float64).reshape(-1, 4)
+    f, raw, v, t = VP(), VP(), VP(), VP()
+    nv, nt, area = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_double()
+    lib.orc_blobby_build(ctypes.byref(grid), _ptr(a), a.shape[0], float(blobbyness), float(cutoff), ctypes.byref(f),
+                         ctypes.byref(raw), ctypes.byref(v), ctypes.byref(t), ctypes.byref(nv), ctypes.byref(nt),
+                         ctypes.byref(area))
+
+    def take(p, n, dt):
+        out = np.empty(n, dt)
+        if n:
+            ctypes.memmove(out.ctypes.data, p.value, out.nbytes)
+        lib.orc_free_buf(p)
+        return out
+
+    field = take(f, grid.nx * grid.ny * grid.nz, np.float64).reshape(grid.nz, grid.ny, grid.nx)
+    return {"field": field, "raw": take(raw, 3 * nv.value, np.float64).reshape(-1, 3),
+            "verts": take(v, 3 * nv.value, np.float64).reshape(-1, 3),
+            "tris": take(t, 3 * nt.value, np.int32).reshape(-1, 3), "area": area.value}
+
+
+def ray_triangle(orig, d, v0, v1, v2):
+    """MeshSurface::intersect_triangle: t or None."""
+    lib = load()
+    arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (orig, d, v0, v1, v2)]
+    t = np.zeros(1)
+    ok = lib.orc_ray_triangle(*[_ptr(x) for x in arrs], _ptr(t))
+    return float(t[0]) if ok else None
+
+
 class OracleSurface:
     """CPU restatement of Surface::getSurf / triangulateSurface on one patch set."""
+
+    @classmethod
+    def from_mesh(cls, grid, verts, tris, ray_based=True, aux_grid_dim=100):
+        """The same on a triangle mesh (MeshSurface)."""
+        o = cls.__new__(cls)
+        o._lib = load()
+        o.grid = grid
+        o.refine_roots = False
+        v = np.ascontiguousarray(verts, dtype=np.float64).reshape(-1, 3)
+        t = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1, 3)
+        o._keep = (v, t)
+        o._h = o._lib.orc_create_mesh(ctypes.byref(grid), _ptr(v), v.shape[0], _ptr(t), t.shape[0],
+                                      int(bool(ray_based)), int(aux_grid_dim))
+        o.result = None
+        o.mesh_info = None
+        return o
 
     def __init__(self, grid, patchset, refine_roots=False):
         """refine_roots: torus roots refined to the true root of the computed

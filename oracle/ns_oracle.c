@@ -15,6 +15,7 @@
  */
 #include <math.h>
 #include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -534,6 +535,11 @@ typedef struct orc_surface {
     int64_t ntris, cap_tris;
     double area;
     nsgpu_surf_result res;
+    /* triangle-mesh surface (MeshSurface) instead of the patch set */
+    int is_mesh, ray_based, aux_dim;
+    double *mv;  /* 3 per vertex */
+    int32_t *mt; /* 3 per triangle */
+    int64_t mnv, mnt;
 } orc_surface;
 
 static void *xcalloc(size_t n, size_t s) { return calloc(n ? n : 1, s); }
@@ -615,6 +621,8 @@ static void free_results(orc_surface *s) {
 void orc_free(orc_surface *s) {
     if (!s) return;
     free_results(s);
+    free(s->mv);
+    free(s->mt);
     free((void *)s->P.kind);
     free((void *)s->P.bsph);
     free((void *)s->P.data);
@@ -644,6 +652,236 @@ static int cmp_hit_t(const void *a, const void *b) {
     return (x->t > y->t) - (x->t < y->t);
 }
 
+
+/* ------------------------------------------------------------------------- */
+/* triangle meshes: MeshSurface ray casting                                  */
+/* ------------------------------------------------------------------------- */
+
+/* MeshSurface::intersect_triangle, non-culling branch (MeshSurface.cpp:
+ * 1487-1553), EPS = 1e-5 (MeshSurface.h:21); SUB/CROSS/DOT of globals.h:220-268 */
+int orc_ray_triangle(const double *orig, const double *dir, const double *v0, const double *v1, const double *v2,
+                     double *t) {
+    double edge1[3], edge2[3], tvec[3], pvec[3], qvec[3];
+    for (int q = 0; q < 3; q++) {
+        edge1[q] = v1[q] - v0[q];
+        edge2[q] = v2[q] - v0[q];
+    }
+    pvec[0] = dir[1] * edge2[2] - dir[2] * edge2[1];
+    pvec[1] = dir[2] * edge2[0] - dir[0] * edge2[2];
+    pvec[2] = dir[0] * edge2[1] - dir[1] * edge2[0];
+    double det = edge1[0] * pvec[0] + edge1[1] * pvec[1] + edge1[2] * pvec[2];
+    if (det > -1e-5 && det < 1e-5) return 0;
+    double inv_det = 1.0 / det;
+    for (int q = 0; q < 3; q++) tvec[q] = orig[q] - v0[q];
+    double u = (tvec[0] * pvec[0] + tvec[1] * pvec[1] + tvec[2] * pvec[2]) * inv_det;
+    if (u < 0.0 || u > 1.0) return 0;
+    qvec[0] = tvec[1] * edge1[2] - tvec[2] * edge1[1];
+    qvec[1] = tvec[2] * edge1[0] - tvec[0] * edge1[2];
+    qvec[2] = tvec[0] * edge1[1] - tvec[1] * edge1[0];
+    double v = (dir[0] * qvec[0] + dir[1] * qvec[1] + dir[2] * qvec[2]) * inv_det;
+    if (v < 0.0 || u + v > 1.0) return 0;
+    *t = (edge2[0] * qvec[0] + edge2[1] * qvec[1] + edge2[2] * qvec[2]) * inv_det;
+    return 1;
+}
+
+static void push_hit_rec(orc_hit **hits, int64_t *nh, int64_t *cap, int64_t pix, double t, int32_t patch) {
+    if (*nh == *cap) {
+        *cap = *cap ? 2 * *cap : 1 << 16;
+        *hits = (orc_hit *)realloc(*hits, *cap * sizeof(orc_hit));
+    }
+    (*hits)[*nh].pix = pix;
+    (*hits)[*nh].t = t;
+    (*hits)[*nh].patch = patch;
+    (*nh)++;
+}
+
+static void tri_bbox(const orc_surface *s, int64_t it, double down[3], double up[3]) {
+    for (int q = 0; q < 3; q++) {
+        down[q] = INFINITY;
+        up[q] = -INFINITY;
+    }
+    for (int pind = 0; pind < 3; pind++) {
+        const double *p = s->mv + 3 * (int64_t)s->mt[3 * it + pind];
+        for (int q = 0; q < 3; q++) {
+            down[q] = down[q] < p[q] ? down[q] : p[q]; /* MIN, globals.h:301 */
+            up[q] = up[q] > p[q] ? up[q] : p[q];       /* MAX, globals.h:304 */
+        }
+    }
+}
+
+static const double *axis_of(const orc_surface *s, int d) { return d == 0 ? s->x : (d == 1 ? s->y : s->z); }
+static int64_t n_of(const orc_surface *s, int d) { return d == 0 ? s->NX : (d == 1 ? s->NY : s->NZ); }
+
+/* the auxiliary 2D grid of MeshSurface::preProcessPanel (MeshSurface.cpp:348-384) */
+static void aux_grid(const orc_surface *s, double min2d[3], double *scale2d, int64_t *n2d) {
+    int64_t igrid = s->NX;
+    int gridMul = 1;
+    while (igrid > s->aux_dim) {
+        gridMul += 2;
+        int64_t digrid = s->NX;
+        while (1) {
+            int64_t fixedPoint = (digrid + (gridMul - 1)) / gridMul;
+            igrid = fixedPoint * gridMul;
+            if (igrid % 2 == 0)
+                digrid = igrid + 1;
+            else {
+                igrid = fixedPoint;
+                break;
+            }
+        }
+    }
+    *scale2d = s->g.scale / ((double)gridMul);
+    double side_2d = ((double)gridMul) / s->g.scale;
+    for (int d = 0; d < 3; d++) min2d[d] = s->g.baricenter[d] - (igrid - 1) * 0.5 * side_2d;
+    *n2d = igrid;
+}
+
+static void mesh_intersections(orc_surface *s, orc_hit **hits, int64_t *nh, int64_t *cap, int64_t *cand) {
+    const int64_t N_MAX = s->N;
+    const int numPanels = s->numPanels;
+    const double hside = s->g.hside;
+    const double gmin[3] = {s->g.xmin, s->g.ymin, s->g.zmin}, gmax[3] = {s->g.xmax, s->g.ymax, s->g.zmax};
+    if (s->ray_based) {
+        /* intersectWithRayBasedAlgorithm (Surface.cpp:6257-7410) with
+         * MeshSurface::getRayIntersection (MeshSurface.cpp:1756-1855) */
+        double min2d[3], scale2d;
+        int64_t n2d;
+        aux_grid(s, min2d, &scale2d, &n2d);
+        for (int panel = 0; panel < 3; panel++) {
+            int first, last, vc;
+            if (panel == 0) { first = 1; last = 2; vc = 0; }
+            else if (panel == 1) { first = 0; last = 1; vc = 2; }
+            else { first = 0; last = 2; vc = 1; }
+            /* gridTriangleMap2D[last cell * n2d + first cell]: triangles in order */
+            int64_t ncell = n2d * n2d;
+            int64_t *coff = (int64_t *)xcalloc(ncell + 1, sizeof(int64_t));
+            int64_t *cfill = (int64_t *)xcalloc(ncell, sizeof(int64_t));
+            for (int pass = 0; pass < 2; pass++) {
+                int32_t *lst = NULL;
+                if (pass == 1) {
+                    for (int64_t q = 0; q < ncell; q++) coff[q + 1] += coff[q];
+                    lst = (int32_t *)xcalloc(coff[ncell], sizeof(int32_t));
+                }
+                for (int64_t it = 0; it < s->mnt; it++) {
+                    double down[3], up[3];
+                    tri_bbox(s, it, down, up);
+                    int64_t st[3], en[3];
+                    for (int d = 0; d < 3; d++) {
+                        st[d] = (int64_t)rintp(fmax(0., down[d] - min2d[d]) * scale2d);
+                        en[d] = (int64_t)rintp(fmax(0., up[d] - min2d[d]) * scale2d);
+                        if (st[d] >= n2d) st[d] = n2d - 1;
+                        if (en[d] >= n2d) en[d] = n2d - 1;
+                    }
+                    for (int64_t b = st[last]; b <= en[last]; b++)
+                        for (int64_t a = st[first]; a <= en[first]; a++) {
+                            if (pass == 0) coff[b * n2d + a + 1]++;
+                            else lst[coff[b * n2d + a] + cfill[b * n2d + a]++] = (int32_t)it;
+                        }
+                }
+                if (pass == 1) {
+                    for (int int_phase = 0; int_phase < 2; int_phase++) {
+                        if (panel >= s->panels[int_phase]) continue;
+                        const int slot = s->panels[0] * int_phase + panel;
+                        const double delta = int_phase == 0 ? 0. : ORC_DELTA - hside;
+                        const double *cf = axis_of(s, first), *cl = axis_of(s, last), *cv = axis_of(s, vc);
+                        double pa[3], pb[3], dir[3] = {0., 0., 0.};
+                        pa[vc] = cv[0] + delta;
+                        pb[vc] = cv[n_of(s, vc) - 1] + delta;
+                        dir[vc] = pb[vc] - pa[vc];
+                        for (int64_t n = 0; n < n_of(s, last); n++)
+                            for (int64_t m = 0; m < n_of(s, first); m++) {
+                                pa[first] = cf[m] + delta;
+                                pa[last] = cl[n] + delta;
+                                int64_t i1 = (int64_t)rintp((pa[first] - min2d[first]) * scale2d);
+                                int64_t i2 = (int64_t)rintp((pa[last] - min2d[last]) * scale2d);
+                                /* the reference reads outside its table there; such
+                                 * rays test nothing here (the library refuses the grid) */
+                                if (i1 < 0 || i1 >= n2d || i2 < 0 || i2 >= n2d) continue;
+                                const int64_t cell = i2 * n2d + i1;
+                                for (int64_t q = coff[cell]; q < coff[cell + 1]; q++) {
+                                    const int64_t it = lst[q];
+                                    const int32_t *tv = s->mt + 3 * it;
+                                    double t;
+                                    (*cand)++;
+                                    if (orc_ray_triangle(pa, dir, s->mv + 3 * (int64_t)tv[0], s->mv + 3 * (int64_t)tv[1],
+                                                         s->mv + 3 * (int64_t)tv[2], &t))
+                                        push_hit_rec(hits, nh, cap, (n * N_MAX + m) * numPanels + slot, t, (int32_t)it);
+                                }
+                            }
+                    }
+                    free(lst);
+                }
+            }
+            free(coff);
+            free(cfill);
+        }
+        return;
+    }
+    /* MeshSurface::getPatchIntersectionData (MeshSurface.cpp:1264-1484) */
+    for (int64_t it = 0; it < s->mnt; it++) {
+        double down[3], up[3];
+        tri_bbox(s, it, down, up);
+        int culled = 0;
+        for (int d = 0; d < 3; d++) {
+            if (down[d] > gmax[d] + ORC_EPS_INT) culled = 1;
+            if (up[d] < gmin[d] - (hside - ORC_DELTA + ORC_EPS_INT)) culled = 1;
+        }
+        if (culled) continue;
+        int64_t i_start[3], i_end[3];
+        for (int d = 0; d < 3; d++) {
+            i_start[d] = (int64_t)rintp(fmax(0., down[d] - gmin[d]) * s->g.scale);
+            i_end[d] = (int64_t)rintp(fmax(0., up[d] - gmin[d]) * s->g.scale);
+            if (i_end[d] >= n_of(s, d)) i_end[d] = n_of(s, d) - 1;
+        }
+        const int32_t *tv = s->mt + 3 * it;
+        for (int int_phase = 0; int_phase < 2; int_phase++) {
+            if (s->panels[int_phase] == 0) continue;
+            const double delta = int_phase == 0 ? 0. : ORC_DELTA - hside;
+            for (int panel = 0; panel < s->panels[int_phase]; panel++) {
+                int first, last, vc;
+                if (panel == 0) { first = 1; last = 2; vc = 0; }
+                else if (panel == 1) { first = 0; last = 1; vc = 2; }
+                else { first = 0; last = 2; vc = 1; }
+                const double *cf = axis_of(s, first), *cl = axis_of(s, last), *cv = axis_of(s, vc);
+                double pa[3], dir[3] = {0., 0., 0.};
+                pa[vc] = cv[0] + delta;
+                dir[vc] = cv[n_of(s, vc) - 1] - cv[0];
+                const int slot = s->panels[0] * int_phase + panel;
+                for (int64_t n = i_start[last]; n <= i_end[last]; n++)
+                    for (int64_t m = i_start[first]; m <= i_end[first]; m++) {
+                        pa[first] = cf[m] + delta;
+                        if (pa[first] < down[first] || pa[first] > up[first]) continue;
+                        pa[last] = cl[n] + delta;
+                        if (pa[last] < down[last] || pa[last] > up[last]) continue;
+                        double t;
+                        (*cand)++;
+                        if (orc_ray_triangle(pa, dir, s->mv + 3 * (int64_t)tv[0], s->mv + 3 * (int64_t)tv[1],
+                                             s->mv + 3 * (int64_t)tv[2], &t))
+                            push_hit_rec(hits, nh, cap, (n * N_MAX + m) * numPanels + slot, t, (int32_t)it);
+                    }
+            }
+        }
+    }
+}
+
+orc_surface *orc_create_mesh(const nsgpu_grid *g, const double *verts, int64_t nv, const int32_t *tris, int64_t nt,
+                             int ray_based, int aux_dim) {
+    int32_t dz[3] = {0, 0, 0};
+    double dd[4] = {0, 0, 0, 0};
+    nsgpu_patches empty = {0, 0, dz, dz, dz, dz, dz, dd, dd};
+    orc_surface *s = orc_create(g, &empty, 1.4);
+    s->is_mesh = 1;
+    s->ray_based = ray_based;
+    s->aux_dim = aux_dim;
+    s->mnv = nv;
+    s->mnt = nt;
+    s->mv = (double *)xcalloc(nv * 3, sizeof(double));
+    s->mt = (int32_t *)xcalloc(nt * 3, sizeof(int32_t));
+    memcpy(s->mv, verts, nv * 3 * sizeof(double));
+    memcpy(s->mt, tris, nt * 3 * sizeof(int32_t));
+    return s;
+}
+
 static void patch_intersections(orc_surface *s) {
     const nsgpu_grid *g = &s->g;
     const nsgpu_patches *P = &s->P;
@@ -653,7 +891,8 @@ static void patch_intersections(orc_surface *s) {
     orc_hit *hits = NULL;
     int64_t nh = 0, cap = 0, cand = 0;
     double pa[3] = {0, 0, 0};
-    for (int it = 0; it < P->n; it++) {
+    if (s->is_mesh) mesh_intersections(s, &hits, &nh, &cap, &cand);
+    for (int it = 0; it < (s->is_mesh ? 0 : P->n); it++) {
         const int kind = P->kind[it];
         const double *sphere_center = P->bsph + 4 * (size_t)it;
         const double radius = P->bsph[4 * (size_t)it + 3];
@@ -825,6 +1064,10 @@ static void patch_normal(const orc_surface *s, int it, const double y[3], double
 /* normal of depth h of an edge ray of panel `panel` through (m, n): at the
  * intersection point as getPatchIntersectionData formed it */
 static void depth_normal(const orc_surface *s, int panel, int64_t m, int64_t n, int64_t h, double nrm[3]) {
+    if (s->is_mesh) { /* no analytic normals on meshes */
+        nrm[0] = nrm[1] = nrm[2] = 0.;
+        return;
+    }
     const double delta = ORC_DELTA - s->g.hside;
     double y[3];
     if (panel == 0) {
@@ -1400,6 +1643,193 @@ int orc_triangulate(orc_surface *s, nsgpu_mesh_info *info) {
     return 0;
 }
 
+
+/* ------------------------------------------------------------------------- */
+/* BlobbySurface::build (BlobbySurface.cpp:83-229)                           */
+/* ------------------------------------------------------------------------- */
+
+/* vertexInterp, Surface.cpp:14516-14523 (ADD_MUL, globals.h:283-286) */
+static void vertex_interp(double isolevel, const double *p1, const double *p2, double valp1, double valp2, double *p) {
+    double mu = (isolevel - valp1) / (valp2 - valp1);
+    double temp[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+    p[0] = p1[0] + mu * temp[0];
+    p[1] = p1[1] + mu * temp[1];
+    p[2] = p1[2] + mu * temp[2];
+}
+
+/* mulTable, tools.h:154-162, as the reference has it (entry 3 repeats entry
+ * 0; only corners 2, 5, 6 and 7 are interpolated on) */
+static const int kMulTable[8][3] = {{0, 2, 4}, {1, 2, 4}, {1, 3, 4}, {0, 2, 4},
+                                    {0, 2, 5}, {1, 2, 5}, {1, 3, 5}, {0, 3, 5}};
+
+/* The blobby surface: field, marching cubes at iso 1.0 on it
+ * (triangulateSurface(false, false, 1.0), the scalar-field branches of
+ * getVertices :8143-8240 and triangulationKernel :9223-9303, triangles not
+ * reversed), one smoothing pass (smoothSurface :13089-13240), then the OFF
+ * text round trip ("%.3f" written, Surface.cpp:10961; "%f" read into
+ * floats, MeshSurface.cpp:2436-2446).  Outputs are malloc'd: field [nvox],
+ * raw (smoothed) and verts (reloaded) [nv*3], tris [nt*3]. */
+int orc_blobby_build(const nsgpu_grid *g, const double *xyzr, int n, double B, double cutoff, double **field_out,
+                     double **raw_out, double **verts_out, int32_t **tris_out, int64_t *nv_out, int64_t *nt_out,
+                     double *area_out) {
+    const int64_t NX = g->nx, NY = g->ny, NZ = g->nz, nvox = NX * NY * NZ;
+    double *x = (double *)xcalloc(NX, sizeof(double)), *y = (double *)xcalloc(NY, sizeof(double)),
+           *z = (double *)xcalloc(NZ, sizeof(double));
+    for (int64_t i = 0; i < NX; i++) x[i] = g->xmin + i * g->side;
+    for (int64_t i = 0; i < NY; i++) y[i] = g->ymin + i * g->side;
+    for (int64_t i = 0; i < NZ; i++) z[i] = g->zmin + i * g->side;
+    if (B >= 0) B = -2.5; /* setBlobbyness, BlobbySurface.cpp:46-54 */
+    /* the field, atom by atom (BlobbySurface.cpp:87-156) */
+    int numgrid = (int)(cutoff / g->side + 0.5);
+    double *F = (double *)xcalloc(nvox, sizeof(double));
+    for (int i = 0; i < n; i++) {
+        const double *pos = xyzr + 4 * (int64_t)i;
+        double r = pos[3];
+        double r2 = r * r;
+        int ix = (int)rintp((pos[0] - g->xmin) * g->scale);
+        int iy = (int)rintp((pos[1] - g->ymin) * g->scale);
+        int iz = (int)rintp((pos[2] - g->zmin) * g->scale);
+        int start_x = ix - numgrid > 0 ? ix - numgrid : 0;
+        int start_y = iy - numgrid > 0 ? iy - numgrid : 0;
+        int start_z = iz - numgrid > 0 ? iz - numgrid : 0;
+        int end_x = NX < ix + numgrid ? (int)NX : ix + numgrid;
+        int end_y = NY < iy + numgrid ? (int)NY : iy + numgrid;
+        int end_z = NZ < iz + numgrid ? (int)NZ : iz + numgrid;
+        for (int kk = start_z; kk < end_z; kk++)
+            for (int jj = start_y; jj < end_y; jj++)
+                for (int ii = start_x; ii < end_x; ii++) {
+                    double p[3] = {x[ii] - g->hside, y[jj] - g->hside, z[kk] - g->hside};
+                    double dist2 = (p[0] - pos[0]) * (p[0] - pos[0]) + (p[1] - pos[1]) * (p[1] - pos[1]) +
+                                   (p[2] - pos[2]) * (p[2] - pos[2]);
+                    F[(kk * NY + jj) * NX + ii] += exp(B * (dist2 / r2 - 1));
+                }
+    }
+#define FV(i, j, k) F[((int64_t)(k) * NY + (j)) * NX + (i)]
+    const double iso = 1.0;
+    int32_t *idx[3];
+    for (int d = 0; d < 3; d++) {
+        idx[d] = (int32_t *)malloc(nvox * sizeof(int32_t));
+        for (int64_t f = 0; f < nvox; f++) idx[d][f] = -1;
+    }
+    double *V = NULL;
+    int64_t nv = 0, capv = 0;
+    /* getVertices: every vertex is interpolated */
+    for (int64_t k = 1; k < NZ - 1; k++)
+        for (int64_t j = 1; j < NY - 1; j++)
+            for (int64_t i = 1; i < NX - 1; i++) {
+                double votes[8] = {FV(i, j, k), FV(i + 1, j, k), FV(i + 1, j + 1, k), FV(i, j + 1, k),
+                                   FV(i, j, k + 1), FV(i + 1, j, k + 1), FV(i + 1, j + 1, k + 1), FV(i, j + 1, k + 1)};
+                double table[6] = {x[i] - g->hside, x[i] + g->hside, y[j] - g->hside,
+                                   y[j] + g->hside, z[k] - g->hside, z[k] + g->hside};
+                double positions[8][3];
+                for (int v = 0; v < 8; v++)
+                    for (int q = 0; q < 3; q++) positions[v][q] = table[kMulTable[v][q]];
+                int c = 0;
+                for (int v = 0; v < 8; v++)
+                    if (votes[v] < iso) c |= 1 << v;
+                if (c == 0 || c == 255) continue;
+                const int e_list[3] = {5, 6, 10}, d_list[3] = {1, 0, 2};
+                const int64_t node[3] = {((k + 1) * NY + j) * NX + i + 1, ((k + 1) * NY + j + 1) * NX + i,
+                                         (k * NY + j + 1) * NX + i + 1};
+                const int pa[3] = {5, 6, 2}, pb[3] = {6, 7, 6};
+                for (int q = 0; q < 3; q++) {
+                    if (!edge_active(c, e_list[q]) || idx[d_list[q]][node[q]] != -1) continue;
+                    if (nv == capv) {
+                        capv = capv ? 2 * capv : 4096;
+                        V = (double *)realloc(V, capv * 3 * sizeof(double));
+                    }
+                    vertex_interp(iso, positions[pa[q]], positions[pb[q]], votes[pa[q]], votes[pb[q]], V + 3 * nv);
+                    idx[d_list[q]][node[q]] = (int32_t)nv++;
+                }
+            }
+    /* triangulationKernel: triangles of getTriangles in (t0, t1, t2) order */
+    static const int edir[12] = {0, 1, 0, 1, 0, 1, 0, 1, 2, 2, 2, 2};
+    static const int eoff[12][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 1}, {1, 0, 1},
+                                    {0, 1, 1}, {0, 0, 1}, {0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}};
+    int32_t *T = NULL;
+    int64_t nt = 0, capt = 0;
+    double area = 0;
+    for (int64_t k = 1; k < NZ - 1; k++)
+        for (int64_t j = 1; j < NY - 1; j++)
+            for (int64_t i = 1; i < NX - 1; i++) {
+                double votes[8] = {FV(i, j, k), FV(i + 1, j, k), FV(i + 1, j + 1, k), FV(i, j + 1, k),
+                                   FV(i, j, k + 1), FV(i + 1, j, k + 1), FV(i + 1, j + 1, k + 1), FV(i, j + 1, k + 1)};
+                int c = 0;
+                for (int v = 0; v < 8; v++)
+                    if (votes[v] < iso) c |= 1 << v;
+                if (c == 0 || c == 255) continue;
+                int vl[12];
+                for (int e = 0; e < 12; e++)
+                    vl[e] = edge_active(c, e) ? idx[edir[e]][((k + eoff[e][2]) * NY + j + eoff[e][1]) * NX + i + eoff[e][0]] : -1;
+                uint64_t w = kMcTriCases[c];
+                for (int q = 0; q < 5; q++) {
+                    int e0 = (int)((w >> (12 * q)) & 15), e1 = (int)((w >> (12 * q + 4)) & 15),
+                        e2 = (int)((w >> (12 * q + 8)) & 15);
+                    if (e0 == 15) break;
+                    if (vl[e0] == -1 || vl[e1] == -1 || vl[e2] == -1) continue;
+                    const double *v0 = V + 3 * (int64_t)vl[e0], *v1 = V + 3 * (int64_t)vl[e1], *v2 = V + 3 * (int64_t)vl[e2];
+                    double a = sqrt((v0[0] - v1[0]) * (v0[0] - v1[0]) + (v0[1] - v1[1]) * (v0[1] - v1[1]) + (v0[2] - v1[2]) * (v0[2] - v1[2]));
+                    double b = sqrt((v0[0] - v2[0]) * (v0[0] - v2[0]) + (v0[1] - v2[1]) * (v0[1] - v2[1]) + (v0[2] - v2[2]) * (v0[2] - v2[2]));
+                    double cc = sqrt((v1[0] - v2[0]) * (v1[0] - v2[0]) + (v1[1] - v2[1]) * (v1[1] - v2[1]) + (v1[2] - v2[2]) * (v1[2] - v2[2]));
+                    double ttt = (a + b + cc) * (b + cc - a) * (cc + a - b) * (a + b - cc);
+                    if (ttt > 0) area += 0.25 * sqrt(ttt);
+                    if (nt == capt) {
+                        capt = capt ? 2 * capt : 4096;
+                        T = (int32_t *)realloc(T, capt * 3 * sizeof(int32_t));
+                    }
+                    T[3 * nt + 0] = vl[e0];
+                    T[3 * nt + 1] = vl[e1];
+                    T[3 * nt + 2] = vl[e2];
+                    nt++;
+                }
+            }
+#undef FV
+    /* smoothSurface (USE_OPTIMIZED_VERTICES_BUFFERING branch, Surface.cpp:13172-13215) */
+    double *temp = (double *)xcalloc(nv * 3, sizeof(double));
+    int *num = (int *)xcalloc(nv, sizeof(int));
+    for (int64_t i = 0; i < nt; i++) {
+        int v1 = T[i * 3 + 0], v2 = T[i * 3 + 1], v3 = T[i * 3 + 2];
+        for (int q = 0; q < 3; q++) {
+            temp[v1 * 3 + q] += V[v2 * 3 + q] + V[v3 * 3 + q];
+            temp[v2 * 3 + q] += V[v1 * 3 + q] + V[v3 * 3 + q];
+            temp[v3 * 3 + q] += V[v1 * 3 + q] + V[v2 * 3 + q];
+        }
+        num[v1] += 2;
+        num[v2] += 2;
+        num[v3] += 2;
+    }
+    double *raw = (double *)xcalloc(nv * 3, sizeof(double));
+    for (int64_t i = 0; i < nv; i++) {
+        double scale = 1.0 / (double)num[i];
+        for (int q = 0; q < 3; q++) raw[i * 3 + q] = 0.5 * (V[i * 3 + q] + temp[i * 3 + q] * scale);
+    }
+    /* saveMesh's "%.3f" and loadOFF's sscanf("%f") into a float */
+    double *verts = (double *)xcalloc(nv * 3, sizeof(double));
+    for (int64_t q = 0; q < 3 * nv; q++) {
+        char buf[64];
+        snprintf(buf, sizeof buf, "%.3f", raw[q]);
+        float f = 0;
+        sscanf(buf, "%f", &f);
+        verts[q] = f;
+    }
+    for (int d = 0; d < 3; d++) free(idx[d]);
+    free(temp);
+    free(num);
+    free(V);
+    free(x);
+    free(y);
+    free(z);
+    if (field_out) *field_out = F; else free(F);
+    *raw_out = raw;
+    *verts_out = verts;
+    *tris_out = T;
+    *nv_out = nv;
+    *nt_out = nt;
+    *area_out = area;
+    return 0;
+}
+
+void orc_free_buf(void *p) { free(p); }
 /* ------------------------------------------------------------------------- */
 /* read back                                                                 */
 /* ------------------------------------------------------------------------- */
