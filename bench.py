@@ -15,7 +15,9 @@ of the plain path are measured alongside:
   perturbations), 2 points/A, maps + cavities, one captured graph launch vs
   the per-frame launch loop;
 * "differentiable": 2,000 atoms at 3 points/A, three-panel volume and area
-  estimate with their gradients for all atoms (trace + adjoint kernel).
+  estimate with their gradients for all atoms (trace + adjoint kernel);
+* "blobby": the blobby surface of 20,000 atoms at 2 points/A, built and ray
+  cast as a mesh on the device, with the oracle's time for the same work.
 """
 import argparse
 import json
@@ -180,6 +182,61 @@ def bench_differentiable(steps=10):
     return res
 
 
+def bench_blobby(steps=5, cpu=True):
+    """Blobby surface (B = -2.5, cutoff 6 A): 20,000 atoms (compacted radius
+    48 A), 2.0 points/A, perfil 80.  A step is BlobbySurface::build (field,
+    marching cubes, smoothing, OFF round trip) plus getSurf on the mesh
+    (ray-based, maps + cavities + 3 edge panels) and triangulateSurface."""
+    xyzr, r_used = ns.generate_chain(20000, 48.0, seed=0)
+    grid = ns.build_grid(xyzr, 2.0, 80.0)
+    dev = ns.DeviceSurface(0)
+    dev.set_grid(grid)
+    opts = dict(build_epsmap=True, build_status=True, accurate_triangulation=True, fill_cavities=True, inside=5)
+    info = dev.blobby_build(xyzr)
+    r = dev.get_surf(**opts)
+    m = dev.triangulate()
+    dev.synchronize()
+    tb = ts = tt = 0.0
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        info = dev.blobby_build(xyzr)
+        dev.synchronize()
+        t1 = time.perf_counter()
+        r = dev.get_surf(**opts)
+        t2 = time.perf_counter()
+        m = dev.triangulate()
+        dev.synchronize()
+        t3 = time.perf_counter()
+        tb += (t1 - t0) / steps
+        ts += (t2 - t1) / steps
+        tt += (t3 - t2) / steps
+    dev.close()
+    nvox = grid.nx * grid.ny * grid.nz
+    step = tb + ts + tt
+    res = {"workload": "blobby B -2.5, cutoff 6 A: %d-atom chain (R %.1f A), 2.0 points/A perfil 80 -> %d^3; "
+                       "build + getSurf (ray-based, maps + cavities + 6 panels) + triangulate"
+                       % (len(xyzr), r_used, grid.nx),
+           "ms_per_step": step * 1e3, "build_ms": tb * 1e3, "get_surf_ms": ts * 1e3, "triangulate_ms": tt * 1e3,
+           "rays_per_s": r["total_rays"] / step, "voxels_per_s": nvox / step,
+           "blobby_mesh": {"vertices": info["num_vertices"], "triangles": info["num_triangles"]},
+           "result": {"volume": r["volume"], "hits": r["total_hits"], "odd_rays": r["odd_rays"],
+                      "cavities": r["num_cavities"], "triangles": m["num_triangles"], "area": m["area"]}}
+    if cpu:
+        # the oracle (single thread) on the same workload, once
+        from oracle import binding
+        t0 = time.perf_counter()
+        b = binding.blobby_build(grid, xyzr)
+        o = binding.OracleSurface.from_mesh(grid, b["verts"], b["tris"], ray_based=True)
+        o.get_surf(**opts)
+        o.triangulate()
+        cpu_s = time.perf_counter() - t0
+        o.close()
+        res["cpu_baseline"] = {"seconds_per_step": cpu_s, "cores": 1, "kind": "port",
+                               "sample": "full workload, 1 run"}
+        res["speedup_vs_cpu"] = cpu_s / step
+    return res
+
+
 def copy_reference(nbytes, repeats=5):
     """Plain device-to-device copy of known size: the calibration kernel for
     the FETCH_SIZE / WRITE_SIZE counters (a copy reads and writes nbytes)."""
@@ -207,10 +264,15 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", type=int, default=0)
-    ap.add_argument("--no-extras", action="store_true", help="skip the mixed, batched and differentiable lines")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the mixed, batched, differentiable and blobby lines")
+    ap.add_argument("--blobby-only", action="store_true", help="only the blobby line (profiling)")
     args = ap.parse_args()
     if args.copy_ref:
         copy_reference(args.copy_ref)
+        return
+    if args.blobby_only:
+        print(json.dumps({"blobby": bench_blobby(args.steps, cpu=not args.no_cpu)}))
         return
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -286,6 +348,8 @@ def main():
         extras["batched"] = bench_batched()
         print("differentiable ...", file=sys.stderr, flush=True)
         extras["differentiable"] = bench_differentiable()
+        print("blobby ...", file=sys.stderr, flush=True)
+        extras["blobby"] = bench_blobby(cpu=not args.no_cpu)
 
     cpu = None
     if not args.no_cpu:

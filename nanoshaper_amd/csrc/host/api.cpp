@@ -1078,7 +1078,7 @@ int nsgpu_blobby_build(nsgpu_ctx *c, const double *xyzr, int32_t n, double B, do
     const int64_t nb = (int64_t)a.nbx * a.nby * a.nbz;
     HIP_TRY(c, c->batoms.ensure((size_t)std::max(n, 1) * 32));
     if (n) HIP_TRY(c, hipMemcpyAsync(c->batoms.p, xyzr, (size_t)n * 32, hipMemcpyHostToDevice, st));
-    HIP_TRY(c, c->bbox.ensure((size_t)std::max(n, 1) * 24));
+    HIP_TRY(c, c->bbox.ensure((size_t)std::max(n, 1) * 32));
     HIP_TRY(c, c->bcount.ensure((size_t)nb * 4));
     HIP_TRY(c, c->boff.ensure((size_t)nb * 4));
     HIP_TRY(c, c->bcur.ensure((size_t)nb * 4));
@@ -1170,34 +1170,28 @@ int nsgpu_blobby_build(nsgpu_ctx *c, const double *xyzr, int32_t n, double B, do
     HIP_TRY(c, c->partial.ensure((size_t)std::max<int64_t>(per, 1) * 8));
     launch_sum_blocks(st, c->tarea.as<double>(), nt, c->partial.as<double>());
     HIP_TRY(c, hipGetLastError());
-    // 3. one smoothing pass (smoothSurface, BlobbySurface.cpp:188): per-vertex
-    // incidences in triangle order
-    HIP_TRY(c, c->bcount.ensure((size_t)std::max<int64_t>(nv, 1) * 4));
-    HIP_TRY(c, c->boff.ensure((size_t)std::max<int64_t>(nv, 1) * 4));
-    HIP_TRY(c, c->bcur.ensure((size_t)std::max<int64_t>(nv, 1) * 4));
-    HIP_TRY(c, c->blist.ensure((size_t)std::max<int64_t>(3 * nt, 1) * 4));
-    HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(std::max<int64_t>(nv, 1)) * 4));
+    // 3. one smoothing pass (smoothSurface, BlobbySurface.cpp:188)
     HIP_TRY(c, c->mraw.ensure((size_t)std::max<int64_t>(nv, 1) * 24));
     HIP_TRY(c, c->mverts.ensure((size_t)std::max<int64_t>(nv, 1) * 24));
     HIP_TRY(c, c->mtris.ensure((size_t)std::max<int64_t>(nt, 1) * 12));
-    launch_fill32(st, c->bcount.p, 0u, nv);
-    launch_fill32(st, c->bcur.p, 0u, nv);
-    launch_fill32(st, c->flag.p, 0u, 2);
-    launch_inc_pairs(st, c->tris.as<int>(), nt, c->bcount.as<int>(), nullptr, nullptr, nullptr, 0);
-    launch_scan(st, c->bcount.as<int>(), nv, c->boff.as<int>(), c->scan_scratch.as<int>());
-    launch_inc_pairs(st, c->tris.as<int>(), nt, c->bcount.as<int>(), c->boff.as<int>(), c->bcur.as<int>(),
-                     c->blist.as<int>(), 1);
-    launch_smooth(st, c->verts.as<double>(), c->tris.as<int>(), nv, c->bcount.as<int>(), c->boff.as<int>(),
-                  c->blist.as<int>(), c->mraw.as<double>(), c->flag.as<int>());
+    SmoothArgs sm;
+    sm.vin = c->verts.as<double>();
+    sm.vslot = c->vslot.as<int>();
+    sm.nv = nv;
+    sm.anode = c->anode.as<int>();
+    sm.ant = c->ant.as<int>();
+    sm.atoff = c->atoff.as<int>();
+    sm.tris = c->tris.as<int>();
+    sm.nact = (int)nact;
+    sm.vout = c->mraw.as<double>();
+    launch_smooth(st, g, sm);
     // 4. the OFF text round trip ("%.3f" written, "%f" read into floats) and the mesh
     launch_quantize(st, c->mraw.as<double>(), 3 * nv, c->mverts.as<double>());
     if (nt) HIP_TRY(c, hipMemcpyAsync(c->mtris.p, c->tris.p, (size_t)nt * 12, hipMemcpyDeviceToDevice, st));
     HIP_TRY(c, hipGetLastError());
     std::vector<double> part((size_t)per, 0.0);
     if (per) HIP_TRY(c, hipMemcpyAsync(part.data(), c->partial.p, (size_t)per * 8, hipMemcpyDeviceToHost, st));
-    HIP_TRY(c, hipMemcpyAsync(&over, c->flag.p, 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(c, hipStreamSynchronize(st));
-    if (over) return fail(c, NSGPU_E_STATE, "a blobby mesh vertex is shared by too many triangles");
     double area = 0;
     for (double v : part) area += v;
     rc = setup_aux_grid(c, aux_grid_dim, 1);

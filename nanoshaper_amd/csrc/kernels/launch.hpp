@@ -124,7 +124,7 @@ struct BlobbyArgs {
     double B;
     int ng;                  // box half-width in nodes, (int)(cutoff / side + 0.5)
     int nbx, nby, nbz;       // bricks of 8^3 nodes
-    int *box;                // 6 per atom
+    int *box;                // 8 per atom: box [s, e) per axis, then r^2 (a double)
     int *bcount, *boff, *bcur, *blist;
     double *field;           // one per node
     int *overflow;           // largest brick list when one exceeds blobby_brick_cap()
@@ -135,11 +135,15 @@ void launch_blobby_pairs(hipStream_t, const DevGrid &, const BlobbyArgs &, int s
 void launch_blobby_field(hipStream_t, const DevGrid &, const BlobbyArgs &);
 void launch_field_bits(hipStream_t, const double *field, int64_t nvox, double iso, uint32_t *bits);
 // smoothing pass and the OFF text round trip of the blobby mesh
-int smooth_inc_cap();
-void launch_inc_pairs(hipStream_t, const int *tris, int64_t nt, int *vcnt, const int *voff, int *vcur, int *inc,
-                      int scatter);
-void launch_smooth(hipStream_t, const double *vin, const int *tris, int64_t nv, const int *vcnt, const int *voff,
-                   const int *inc, double *vout, int *overflow);
+struct SmoothArgs {
+    const double *vin;  // marching-cubes vertices
+    const int *vslot;   // edge slot 3 * node + direction of each vertex
+    int64_t nv;
+    const int *anode, *ant, *atoff, *tris;  // active cubes, their triangle counts and offsets, triangles
+    int nact;
+    double *vout;
+};
+void launch_smooth(hipStream_t, const DevGrid &, const SmoothArgs &);
 void launch_quantize(hipStream_t, const double *in, int64_t n, double *out);
 // triangle-mesh ray casting
 struct MeshArgs {

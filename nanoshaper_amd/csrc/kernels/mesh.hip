@@ -6,11 +6,11 @@
 //   phi(node) = sum over atoms of exp(B (|p - c|^2 / r^2 - 1)),  p = node - hside,
 // over the nodes of each atom's box [ix - ng, ix + ng) (clamped to the grid),
 // atom after atom.  Here every brick of 8^3 nodes gathers the atoms whose
-// boxes overlap it, sorts them by index in LDS and sums them in that order,
+// boxes overlap it, orders them by index in LDS and sums them in that order,
 // so each node adds the same terms in the same order as the reference.
 //
 // MeshSurface ray casting (Möller-Trumbore, MeshSurface.cpp:1487-1553): one
-// lane per (triangle, panel slot) walks the rays that test the triangle:
+// lane per triangle walks, panel slot after panel slot, the rays that test it:
 //   ray-based (Patch_Based_Algorithm = false, the blobby path): the rays whose
 //     auxiliary 2D cell (preProcessPanel :338-575, getRayIntersection
 //     :1756-1855) lies in the triangle's cell range; direction pb - pa;
@@ -25,7 +25,7 @@ static inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b -
 
 // ------------------------------------------------------------ blobby field
 constexpr int kBrick = 8;         // nodes per brick side
-constexpr int kBrickCap = 2048;   // atoms per brick list (LDS)
+constexpr int kBrickCap = 1024;   // atoms per brick list (LDS); a dense protein reaches ~400
 
 // box [s, e) of nodes per axis of atom a (BlobbySurface.cpp:125-155)
 __device__ __forceinline__ void atom_box(const DevGrid &g, const double *xyzr, int ng, int b[6]) {
@@ -46,8 +46,12 @@ __global__ void __launch_bounds__(256) k_blobby_pairs(DevGrid g, BlobbyArgs a, i
     if (at >= a.n) return;
     int b[6];
     atom_box(g, a.xyzr + 4 * (int64_t)at, a.ng, b);
-    if (!scatter)
-        for (int q = 0; q < 6; ++q) a.box[6 * at + q] = b[q];
+    if (!scatter) {
+        // the atom's record for the field pass: box and r^2 in 32 bytes
+        for (int q = 0; q < 6; ++q) a.box[8 * at + q] = b[q];
+        const double r = a.xyzr[4 * (int64_t)at + 3];
+        *(double *)(a.box + 8 * at + 6) = r * r;
+    }
     if (b[0] >= b[1] || b[2] >= b[3] || b[4] >= b[5]) return;
     const int bx0 = b[0] / kBrick, bx1 = (b[1] - 1) / kBrick;
     const int by0 = b[2] / kBrick, by1 = (b[3] - 1) / kBrick;
@@ -65,8 +69,13 @@ __global__ void __launch_bounds__(256) k_blobby_pairs(DevGrid g, BlobbyArgs a, i
             }
 }
 
-// one workgroup per brick, two nodes per lane
-__global__ void __launch_bounds__(256) k_blobby_field(DevGrid g, BlobbyArgs a) {
+// One workgroup of two waves per brick.  The brick's atom list is put in
+// index order by ranking (ids are distinct: rank = number of smaller ids),
+// then a lane sums the atoms for one (i, j) column and four z nodes; the
+// partial dx^2 + dy^2 is shared by the four nodes (the reference's DIST2 adds
+// the three squares left to right).
+__global__ void __launch_bounds__(128) k_blobby_field(DevGrid g, BlobbyArgs a) {
+    __shared__ int raw[kBrickCap];
     __shared__ int list[kBrickCap];
     const int br = blockIdx.x;
     const int c = a.bcount[br];
@@ -74,73 +83,76 @@ __global__ void __launch_bounds__(256) k_blobby_field(DevGrid g, BlobbyArgs a) {
         if (threadIdx.x == 0) atomicMax(a.overflow, c);
         return;
     }
-    int np = 1;
-    while (np < c) np <<= 1;
-    for (int q = threadIdx.x; q < np; q += blockDim.x) list[q] = q < c ? a.blist[a.boff[br] + q] : 0x7fffffff;
+    for (int q = threadIdx.x; q < c; q += blockDim.x) raw[q] = a.blist[a.boff[br] + q];
     __syncthreads();
-    // bitonic sort of the atom ids
-    for (int k = 2; k <= np; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int q = threadIdx.x; q < np; q += blockDim.x) {
-                const int o = q ^ j;
-                if (o > q) {
-                    const int x = list[q], y = list[o];
-                    const bool up = (q & k) == 0;
-                    if ((x > y) == up) {
-                        list[q] = y;
-                        list[o] = x;
-                    }
-                }
-            }
-            __syncthreads();
-        }
+    for (int q = threadIdx.x; q < c; q += blockDim.x) {
+        const int x = raw[q];
+        int rank = 0;
+        for (int u = 0; u < c; ++u) rank += raw[u] < x;
+        list[rank] = x;
+    }
+    __syncthreads();
     const int bx = br % a.nbx, by = (br / a.nbx) % a.nby, bz = br / (a.nbx * a.nby);
-    const int li = threadIdx.x & 7, lj = (threadIdx.x >> 3) & 7, lk = threadIdx.x >> 6;  // lk 0..3
-    const int ii = bx * kBrick + li, jj = by * kBrick + lj;
-    const int kk0 = bz * kBrick + lk, kk1 = kk0 + 4;
-    const bool inx = ii < g.nx && jj < g.ny;
-    const bool v0 = inx && kk0 < g.nz, v1 = inx && kk1 < g.nz;
-    double p[3] = {0., 0., 0.}, pz1 = 0.;
-    if (inx) {
-        p[0] = g.x[ii] - g.hside;
-        p[1] = g.y[jj] - g.hside;
+    const int li = threadIdx.x & 7, lj = (threadIdx.x >> 3) & 7, lk = threadIdx.x >> 6;  // lk 0..1
+    const int ii = bx * kBrick + li, jj = by * kBrick + lj, kb = bz * kBrick + 4 * lk;
+    const bool inxy = ii < g.nx && jj < g.ny;
+    double px = 0., py = 0., pz[4];
+    if (inxy) {
+        px = g.x[ii] - g.hside;
+        py = g.y[jj] - g.hside;
     }
-    if (v0) p[2] = g.z[kk0] - g.hside;
-    if (v1) pz1 = g.z[kk1] - g.hside;
-    double f0 = 0., f1 = 0.;
-    for (int q = 0; q < c; ++q) {
+    for (int u = 0; u < 4; ++u) pz[u] = kb + u < g.nz ? g.z[kb + u] - g.hside : 0.;
+    double f[4] = {0., 0., 0., 0.};
+    // the next atom's record and centre are loaded while the current one is summed
+    struct Rec {
+        int4 b03;
+        int2 b45;
+        double r2, x, y, z;
+    };
+    auto load = [&](int q) {
         const int at = __builtin_amdgcn_readfirstlane(list[q]);
-        const int *b = a.box + 6 * at;
+        const int *bp = a.box + 8 * at;
         const double *pos = a.xyzr + 4 * (int64_t)at;
-        const bool inxy = ii >= b[0] && ii < b[1] && jj >= b[2] && jj < b[3];
-        const double r = pos[3];
-        const double r2 = r * r;
-        if (v0 && inxy && kk0 >= b[4] && kk0 < b[5]) {
-            const double d2 = (p[0] - pos[0]) * (p[0] - pos[0]) + (p[1] - pos[1]) * (p[1] - pos[1]) +
-                              (p[2] - pos[2]) * (p[2] - pos[2]);
-            f0 += exp(a.B * (d2 / r2 - 1));
-        }
-        if (v1 && inxy && kk1 >= b[4] && kk1 < b[5]) {
-            const double d2 = (p[0] - pos[0]) * (p[0] - pos[0]) + (p[1] - pos[1]) * (p[1] - pos[1]) +
-                              (pz1 - pos[2]) * (pz1 - pos[2]);
-            f1 += exp(a.B * (d2 / r2 - 1));
-        }
+        Rec r;
+        r.b03 = *(const int4 *)bp;
+        r.b45 = *(const int2 *)(bp + 4);
+        r.r2 = *(const double *)(bp + 6);
+        r.x = pos[0];
+        r.y = pos[1];
+        r.z = pos[2];
+        return r;
+    };
+    Rec nx = c > 0 ? load(0) : Rec{};
+    for (int q = 0; q < c; ++q) {
+        const Rec cur = nx;
+        if (q + 1 < c) nx = load(q + 1);
+        const bool in = inxy & (ii >= cur.b03.x) & (ii < cur.b03.y) & (jj >= cur.b03.z) & (jj < cur.b03.w);
+        if (!in) continue;
+        const double dxy = (px - cur.x) * (px - cur.x) + (py - cur.y) * (py - cur.y);
+        for (int u = 0; u < 4; ++u)
+            if ((kb + u >= cur.b45.x) & (kb + u < cur.b45.y)) {
+                const double d2 = dxy + (pz[u] - cur.z) * (pz[u] - cur.z);
+                f[u] += exp(a.B * (d2 / cur.r2 - 1));
+            }
     }
-    if (v0) a.field[flat(g, ii, jj, kk0)] = f0;
-    if (v1) a.field[flat(g, ii, jj, kk1)] = f1;
+    if (inxy)
+        for (int u = 0; u < 4; ++u)
+            if (kb + u < g.nz) a.field[flat(g, ii, jj, kb + u)] = f[u];
 }
 
 // corner bits of the scalar marching cubes: bit 0 where phi < iso (the case
-// bit classifyCube sets, Surface.cpp:14526-14547), 1 elsewhere and past the grid
+// bit classifyCube sets, Surface.cpp:14526-14547), 1 elsewhere and past the
+// grid; one lane per node, a wave covers 64 consecutive nodes = two words
 __global__ void __launch_bounds__(256) k_field_bits(const double *field, int64_t nvox, double iso, uint32_t *bits) {
-    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= (nvox + 31) >> 5) return;
-    uint32_t m = 0;
-    for (int b = 0; b < 32; ++b) {
-        const int64_t f = (w << 5) + b;
-        if (f >= nvox || !(field[f] < iso)) m |= 1u << b;
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool out = f >= nvox || !(field[f] < iso);
+    const uint64_t m = __ballot(out);
+    const int64_t w = (f >> 6) << 1;
+    const int64_t nw = (nvox + 31) >> 5;
+    if ((threadIdx.x & 63) == 0) {
+        if (w < nw) bits[w] = (uint32_t)m;
+        if (w + 1 < nw) bits[w + 1] = (uint32_t)(m >> 32);
     }
-    bits[w] = m;
 }
 
 void blobby_bricks(const DevGrid &g, int *nbx, int *nby, int *nbz) {
@@ -155,71 +167,71 @@ void launch_blobby_pairs(hipStream_t s, const DevGrid &g, const BlobbyArgs &a, i
 }
 void launch_blobby_field(hipStream_t s, const DevGrid &g, const BlobbyArgs &a) {
     const int64_t nb = (int64_t)a.nbx * a.nby * a.nbz;
-    if (nb > 0) hipLaunchKernelGGL(k_blobby_field, dim3((unsigned)nb), dim3(256), 0, s, g, a);
+    if (nb > 0) hipLaunchKernelGGL(k_blobby_field, dim3((unsigned)nb), dim3(128), 0, s, g, a);
 }
 void launch_field_bits(hipStream_t s, const double *field, int64_t nvox, double iso, uint32_t *bits) {
-    const int64_t nw = (nvox + 31) >> 5;
-    if (nw > 0) hipLaunchKernelGGL(k_field_bits, dim3(nblk(nw)), dim3(256), 0, s, field, nvox, iso, bits);
+    if (nvox > 0) hipLaunchKernelGGL(k_field_bits, dim3(nblk(nvox)), dim3(256), 0, s, field, nvox, iso, bits);
 }
 
 // ------------------------------------------------------------ smoothing
 // Surface::smoothSurface (Surface.cpp:13089-13240): every triangle adds the
 // sum of its other two vertices to each of its vertices, in triangle order;
-// then v <- 0.5 (v + sum / (2 * triangles of v)).  The incidences of a vertex
-// are collected (3 t + position), sorted, and summed in triangle order.
-constexpr int kIncCap = 32;
+// then v <- 0.5 (v + sum / (2 * triangles of v)).  A marching-cubes vertex
+// lies on one grid edge and only the (at most four) cubes around that edge
+// emit triangles using it; their triangles are contiguous runs of the
+// triangle list in cube order, so visiting the cubes in increasing node order
+// visits the vertex's triangles in triangle order, with no incidence lists.
 
-__global__ void __launch_bounds__(256) k_inc_pairs(const int *tris, int64_t nt, int *vcnt, const int *voff, int *vcur,
-                                                   int *inc, int scatter) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nt) return;
-    for (int q = 0; q < 3; ++q) {
-        const int v = tris[3 * t + q];
-        if (!scatter) {
-            atomicAdd(&vcnt[v], 1);
-        } else {
-            const int pos = voff[v] + atomicAdd(&vcur[v], 1);
-            inc[pos] = (int)(3 * t + q);
-        }
+// index of active cube `key` in the sorted active-cube list, or -1
+__device__ __forceinline__ int active_index(const int *anode, int nact, int key) {
+    int lo = 0, hi = nact;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (anode[mid] < key) lo = mid + 1;
+        else hi = mid;
     }
+    return (lo < nact && anode[lo] == key) ? lo : -1;
 }
 
-__global__ void __launch_bounds__(256) k_smooth(const double *vin, const int *tris, int64_t nv, const int *vcnt,
-                                                const int *voff, const int *inc, double *vout, int *overflow) {
+__global__ void __launch_bounds__(256) k_smooth(DevGrid g, SmoothArgs a) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nv) return;
-    const int c = vcnt[v];
-    if (c > kIncCap) {
-        atomicMax(overflow, c);
-        return;
-    }
-    int e[kIncCap];
-    for (int q = 0; q < c; ++q) {
-        const int x = inc[voff[v] + q];
-        int p = q;
-        while (p > 0 && e[p - 1] > x) {
-            e[p] = e[p - 1];
-            --p;
-        }
-        e[p] = x;
-    }
+    if (v >= a.nv) return;
+    const int slot = a.vslot[v];
+    const int f = slot / 3, dir = slot - 3 * f;
+    const int sy = g.nx, sz = g.nx * g.ny;
+    // the cubes around the edge (node f, direction dir), increasing node index
+    int cubes[4];
+    if (dir == 0) { cubes[0] = f - sy - sz; cubes[1] = f - sz; cubes[2] = f - sy; cubes[3] = f; }
+    else if (dir == 1) { cubes[0] = f - 1 - sz; cubes[1] = f - sz; cubes[2] = f - 1; cubes[3] = f; }
+    else { cubes[0] = f - 1 - sy; cubes[1] = f - sy; cubes[2] = f - 1; cubes[3] = f; }
     double s0 = 0., s1 = 0., s2 = 0.;
-    for (int q = 0; q < c; ++q) {
-        const int t = e[q] / 3, pos = e[q] - 3 * t;
-        const int *tv = tris + 3 * (int64_t)t;
-        // the other two vertices in the reference's order: (v2, v3), (v1, v3), (v1, v2)
-        const int oa = pos == 0 ? tv[1] : tv[0];
-        const int ob = pos == 2 ? tv[1] : tv[2];
-        const double *A = vin + 3 * (int64_t)oa, *B = vin + 3 * (int64_t)ob;
-        s0 += A[0] + B[0];
-        s1 += A[1] + B[1];
-        s2 += A[2] + B[2];
+    int c = 0;
+    for (int q = 0; q < 4; ++q) {
+        if (cubes[q] < 0) continue;
+        const int ci = active_index(a.anode, a.nact, cubes[q]);
+        if (ci < 0) continue;
+        for (int t = a.atoff[ci]; t < a.atoff[ci] + a.ant[ci]; ++t) {
+            const int *tv = a.tris + 3 * (int64_t)t;
+            int pos = -1;
+            if (tv[0] == v) pos = 0;
+            else if (tv[1] == v) pos = 1;
+            else if (tv[2] == v) pos = 2;
+            if (pos < 0) continue;
+            // the other two vertices in the reference's order: (v2, v3), (v1, v3), (v1, v2)
+            const int oa = pos == 0 ? tv[1] : tv[0];
+            const int ob = pos == 2 ? tv[1] : tv[2];
+            const double *A = a.vin + 3 * (int64_t)oa, *B = a.vin + 3 * (int64_t)ob;
+            s0 += A[0] + B[0];
+            s1 += A[1] + B[1];
+            s2 += A[2] + B[2];
+            ++c;
+        }
     }
     const double scale = 1.0 / (double)(2 * c);
-    const double *V = vin + 3 * v;
-    vout[3 * v + 0] = 0.5 * (V[0] + s0 * scale);
-    vout[3 * v + 1] = 0.5 * (V[1] + s1 * scale);
-    vout[3 * v + 2] = 0.5 * (V[2] + s2 * scale);
+    const double *V = a.vin + 3 * v;
+    a.vout[3 * v + 0] = 0.5 * (V[0] + s0 * scale);
+    a.vout[3 * v + 1] = 0.5 * (V[1] + s1 * scale);
+    a.vout[3 * v + 2] = 0.5 * (V[2] + s2 * scale);
 }
 
 // "%.3f" (round half to even on the exact binary value, as printf) read back
@@ -256,16 +268,8 @@ __global__ void __launch_bounds__(256) k_quantize(const double *in, int64_t n, d
     if (q < n) out[q] = off_round_trip(in[q]);
 }
 
-int smooth_inc_cap() { return kIncCap; }
-
-void launch_inc_pairs(hipStream_t s, const int *tris, int64_t nt, int *vcnt, const int *voff, int *vcur, int *inc,
-                      int scatter) {
-    if (nt > 0) hipLaunchKernelGGL(k_inc_pairs, dim3(nblk(nt)), dim3(256), 0, s, tris, nt, vcnt, voff, vcur, inc, scatter);
-}
-void launch_smooth(hipStream_t s, const double *vin, const int *tris, int64_t nv, const int *vcnt, const int *voff,
-                   const int *inc, double *vout, int *overflow) {
-    if (nv > 0)
-        hipLaunchKernelGGL(k_smooth, dim3(nblk(nv)), dim3(256), 0, s, vin, tris, nv, vcnt, voff, inc, vout, overflow);
+void launch_smooth(hipStream_t s, const DevGrid &g, const SmoothArgs &a) {
+    if (a.nv > 0) hipLaunchKernelGGL(k_smooth, dim3(nblk(a.nv)), dim3(256), 0, s, g, a);
 }
 void launch_quantize(hipStream_t s, const double *in, int64_t n, double *out) {
     if (n > 0) hipLaunchKernelGGL(k_quantize, dim3(nblk(n)), dim3(256), 0, s, in, n, out);
@@ -296,19 +300,24 @@ __device__ __forceinline__ int axis_n(const DevGrid &g, int d) { return d == 0 ?
 __device__ __forceinline__ double axis_min(const DevGrid &g, int d) { return d == 0 ? g.xmin : (d == 1 ? g.ymin : g.zmin); }
 __device__ __forceinline__ double axis_max(const DevGrid &g, int d) { return d == 0 ? g.xmax : (d == 1 ? g.ymax : g.zmax); }
 
-// grid: (ceil(nt / 64), nslots); per-workgroup candidate counts to part
+// rays of axis d (coordinates c[m] + delta, c[m] = cmin + m side) whose
+// coordinate may lie in [lo, hi]: a superset by one ray on each side (the
+// caller compares exactly), clipped to [m0, m1]
+__device__ __forceinline__ void ray_span(double lo, double hi, double cmin, double delta, double scale, int m0, int m1,
+                                         int &a, int &b) {
+    const double ea = fmin(fmax(floor((lo - delta - cmin) * scale), -2.), 2e9);
+    const double eb = fmin(fmax(floor((hi - delta - cmin) * scale), -2.), 2e9);
+    a = max(m0, (int)ea - 1);
+    b = min(m1, (int)eb + 1);
+}
+
+// grid: ceil(nt / 64) workgroups of one wave; a lane takes one triangle
+// through every panel slot; per-workgroup candidate counts to part
 template <bool kRayBased>
 __global__ void __launch_bounds__(64) k_intersect_mesh(DevGrid g, DevRays R, MeshArgs a, unsigned long long *part) {
-    const int slot = blockIdx.y;
     const int64_t it = (int64_t)blockIdx.x * 64 + threadIdx.x;
-    const int phase = slot >= R.panels0;
-    const int panel = phase ? slot - R.panels0 : slot;
     unsigned long long cand = 0;
     if (it < a.nt) {
-        int first, last, vc;
-        if (panel == 0) { first = 1; last = 2; vc = 0; }
-        else if (panel == 1) { first = 0; last = 1; vc = 2; }
-        else { first = 0; last = 2; vc = 1; }
         const int *tv = a.tri + 3 * it;
         const double *p0 = a.v + 3 * (int64_t)tv[0], *p1 = a.v + 3 * (int64_t)tv[1], *p2 = a.v + 3 * (int64_t)tv[2];
         double down[3], up[3];
@@ -322,74 +331,101 @@ __global__ void __launch_bounds__(64) k_intersect_mesh(DevGrid g, DevRays R, Mes
                 down[q] = down[q] < pp[pi][q] ? down[q] : pp[pi][q];
                 up[q] = up[q] > pp[pi][q] ? up[q] : pp[pi][q];
             }
-        const double delta = phase ? kDelta - g.hside : 0.;
-        const double *cf = axis_coords(g, first), *cl = axis_coords(g, last), *cv = axis_coords(g, vc);
-        const int nvc = axis_n(g, vc);
-        double pa[3], dir[3] = {0., 0., 0.};
-        pa[vc] = cv[0] + delta;
-        dir[vc] = kRayBased ? (cv[nvc - 1] + delta) - pa[vc] : cv[nvc - 1] - cv[0];
-        int m0, m1, n0, n1;
-        bool skip = false;
-        if (kRayBased) {
-            int64_t cs[2], ce[2];
-            const int dd[2] = {first, last};
-            for (int q = 0; q < 2; ++q) {
-                cs[q] = (int64_t)rintp(fmax(0., down[dd[q]] - a.min2d[dd[q]]) * a.scale2d);
-                ce[q] = (int64_t)rintp(fmax(0., up[dd[q]] - a.min2d[dd[q]]) * a.scale2d);
-                if (cs[q] >= a.n2d) cs[q] = a.n2d - 1;
-                if (ce[q] >= a.n2d) ce[q] = a.n2d - 1;
-            }
-            const int *rf = a.rr + (int64_t)((first * 2 + phase) * 2) * a.n2d;
-            const int *rl = a.rr + (int64_t)((last * 2 + phase) * 2) * a.n2d;
-            m0 = rf[cs[0]];
-            m1 = rf[a.n2d + ce[0]];
-            n0 = rl[cs[1]];
-            n1 = rl[a.n2d + ce[1]];
-        } else {
-            // CELL_CULLING (MeshSurface.cpp:1318-1331)
+        bool culled = false;
+        if (!kRayBased)  // CELL_CULLING (MeshSurface.cpp:1318-1331)
             for (int q = 0; q < 3; ++q) {
-                skip |= down[q] > axis_max(g, q) + kEpsInt;
-                skip |= up[q] < axis_min(g, q) - (g.hside - kDelta + kEpsInt);
+                culled |= down[q] > axis_max(g, q) + kEpsInt;
+                culled |= up[q] < axis_min(g, q) - (g.hside - kDelta + kEpsInt);
             }
-            const int dd[2] = {first, last};
-            int64_t s[2], e[2];
-            for (int q = 0; q < 2; ++q) {
-                s[q] = (int64_t)rintp(fmax(0., down[dd[q]] - axis_min(g, dd[q])) * g.scale);
-                e[q] = (int64_t)rintp(fmax(0., up[dd[q]] - axis_min(g, dd[q])) * g.scale);
-                if (e[q] >= axis_n(g, dd[q])) e[q] = axis_n(g, dd[q]) - 1;
-            }
-            m0 = (int)s[0];
-            m1 = (int)e[0];
-            n0 = (int)s[1];
-            n1 = (int)e[1];
-        }
-        if (!skip)
-            for (int n = n0; n <= n1; ++n) {
-                pa[last] = cl[n] + delta;
-                if (!kRayBased && (pa[last] < down[last] || pa[last] > up[last])) continue;
-                for (int m = m0; m <= m1; ++m) {
-                    pa[first] = cf[m] + delta;
-                    if (!kRayBased && (pa[first] < down[first] || pa[first] > up[first])) continue;
-                    ++cand;
-                    double t;
-                    if (ray_triangle(pa, dir, p0, p1, p2, t)) push_hit(R, slot, (int64_t)n * R.N + m, t, (int)it);
+        for (int slot = 0; slot < R.nslots && !culled; ++slot) {
+            const int phase = slot >= R.panels0;
+            const int panel = phase ? slot - R.panels0 : slot;
+            int first, last, vc;
+            if (panel == 0) { first = 1; last = 2; vc = 0; }
+            else if (panel == 1) { first = 0; last = 1; vc = 2; }
+            else { first = 0; last = 2; vc = 1; }
+            const double delta = phase ? kDelta - g.hside : 0.;
+            const double *cf = axis_coords(g, first), *cl = axis_coords(g, last), *cv = axis_coords(g, vc);
+            const int nvc = axis_n(g, vc);
+            double pa[3], dir[3] = {0., 0., 0.};
+            pa[vc] = cv[0] + delta;
+            dir[vc] = kRayBased ? (cv[nvc - 1] + delta) - pa[vc] : cv[nvc - 1] - cv[0];
+            int m0, m1, n0, n1;
+            if (kRayBased) {
+                int64_t cs[2], ce[2];
+                const int dd[2] = {first, last};
+                for (int q = 0; q < 2; ++q) {
+                    cs[q] = (int64_t)rintp(fmax(0., down[dd[q]] - a.min2d[dd[q]]) * a.scale2d);
+                    ce[q] = (int64_t)rintp(fmax(0., up[dd[q]] - a.min2d[dd[q]]) * a.scale2d);
+                    if (cs[q] >= a.n2d) cs[q] = a.n2d - 1;
+                    if (ce[q] >= a.n2d) ce[q] = a.n2d - 1;
+                }
+                const int *rf = a.rr + (int64_t)((first * 2 + phase) * 2) * a.n2d;
+                const int *rl = a.rr + (int64_t)((last * 2 + phase) * 2) * a.n2d;
+                m0 = rf[cs[0]];
+                m1 = rf[a.n2d + ce[0]];
+                n0 = rl[cs[1]];
+                n1 = rl[a.n2d + ce[1]];
+                // every ray of the cell range is a candidate of the reference;
+                // only those within 1e-9 A of the bounding box can pass the
+                // barycentric tests (a ray d outside the box is d outside the
+                // triangle's projection, and u, v carry rounding errors of
+                // order 1e-16 |o - v0|, about 1e-14 A here)
+                if (m1 >= m0 && n1 >= n0) cand += (unsigned long long)(m1 - m0 + 1) * (n1 - n0 + 1);
+                const double mg = 1e-9;
+                ray_span(down[first] - mg, up[first] + mg, axis_min(g, first), delta, g.scale, m0, m1, m0, m1);
+                ray_span(down[last] - mg, up[last] + mg, axis_min(g, last), delta, g.scale, n0, n1, n0, n1);
+                for (int n = n0; n <= n1; ++n) {
+                    pa[last] = cl[n] + delta;
+                    if (pa[last] < down[last] - mg || pa[last] > up[last] + mg) continue;
+                    for (int m = m0; m <= m1; ++m) {
+                        pa[first] = cf[m] + delta;
+                        if (pa[first] < down[first] - mg || pa[first] > up[first] + mg) continue;
+                        double t;
+                        if (ray_triangle(pa, dir, p0, p1, p2, t))
+                            push_hit(R, slot, (int64_t)n * R.N + m, t, (int)it);
+                    }
+                }
+            } else {
+                const int dd[2] = {first, last};
+                int64_t s[2], e[2];
+                for (int q = 0; q < 2; ++q) {
+                    s[q] = (int64_t)rintp(fmax(0., down[dd[q]] - axis_min(g, dd[q])) * g.scale);
+                    e[q] = (int64_t)rintp(fmax(0., up[dd[q]] - axis_min(g, dd[q])) * g.scale);
+                    if (e[q] >= axis_n(g, dd[q])) e[q] = axis_n(g, dd[q]) - 1;
+                }
+                m0 = (int)s[0];
+                m1 = (int)e[0];
+                n0 = (int)s[1];
+                n1 = (int)e[1];
+                for (int n = n0; n <= n1; ++n) {
+                    pa[last] = cl[n] + delta;
+                    if (pa[last] < down[last] || pa[last] > up[last]) continue;
+                    for (int m = m0; m <= m1; ++m) {
+                        pa[first] = cf[m] + delta;
+                        if (pa[first] < down[first] || pa[first] > up[first]) continue;
+                        ++cand;
+                        double t;
+                        if (ray_triangle(pa, dir, p0, p1, p2, t))
+                            push_hit(R, slot, (int64_t)n * R.N + m, t, (int)it);
+                    }
                 }
             }
+        }
     }
     for (int off = 32; off > 0; off >>= 1) cand += __shfl_down(cand, off, 64);
-    if (threadIdx.x == 0 && cand) part[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = cand;
+    if (threadIdx.x == 0 && cand) part[blockIdx.x] = cand;
 }
 
-int64_t mesh_workgroups(int64_t nt, const DevRays &R) { return (int64_t)nblk(nt, 64) * R.nslots; }
+int64_t mesh_workgroups(int64_t nt, const DevRays &) { return (int64_t)nblk(nt, 64); }
 
 void launch_intersect_mesh(hipStream_t s, const DevGrid &g, const DevRays &R, const MeshArgs &a,
                            unsigned long long *part) {
     if (a.nt <= 0 || R.nslots <= 0) return;
-    const dim3 grid(nblk(a.nt, 64), (unsigned)R.nslots);
     if (a.ray_based)
-        hipLaunchKernelGGL(k_intersect_mesh<true>, grid, dim3(64), 0, s, g, R, a, part);
+        hipLaunchKernelGGL(k_intersect_mesh<true>, dim3(nblk(a.nt, 64)), dim3(64), 0, s, g, R, a, part);
     else
-        hipLaunchKernelGGL(k_intersect_mesh<false>, grid, dim3(64), 0, s, g, R, a, part);
+        hipLaunchKernelGGL(k_intersect_mesh<false>, dim3(nblk(a.nt, 64)), dim3(64), 0, s, g, R, a, part);
 }
 
 }  // namespace nsg
