@@ -6,6 +6,8 @@
 //   DelPhiShared          DelphiShared.h:43-216 (atoms, grid, caller-visible maps)
 //   Surface               Surface.h:835-979 (build, getSurf, triangulateSurface)
 //   ConnollySurface "ses" ConnollySurface.h:682-692 (factory registration)
+//   MeshSurface "mesh"    MeshSurface.h:211-221, MeshSurface.cpp:176-248 (OFF input)
+//   BlobbySurface "blobby" BlobbySurface.h:68-79, BlobbySurface.cpp:83-229
 //   SurfaceFactory        SurfaceFactory.h:15-65 (name -> instantiator, key "Surface")
 // Everything on the device goes through the extern "C" entry points; this
 // header adds no symbols to libnsgpu.so.
@@ -13,6 +15,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <fstream>
 #include <map>
 #include <sstream>
@@ -115,6 +118,8 @@ class Surface {
             computeNormals = cf->read<bool>("Compute_Vertex_Normals", computeNormals);
             if (!cf->read<bool>("Analytical_Ray_Vs_Torus_Intersection", true)) precision = NSGPU_PREC_STURM;
             device = cf->read<int>("Device", device);
+            patchBased = cf->read<bool>("Patch_Based_Algorithm", patchBased);
+            auxGrid2D = cf->read<int>("Max_mesh_auxiliary_grid_2d_size", auxGrid2D);
         }
     }
     virtual ~Surface() {
@@ -176,7 +181,10 @@ class Surface {
         return m.area;
     }
 
-    const char *lastError() const { return ctx_ ? nsgpu_last_error(ctx_) : "no device context"; }
+    const char *lastError() const {
+        if (!err_.empty()) return err_.c_str();
+        return ctx_ ? nsgpu_last_error(ctx_) : "no device context";
+    }
     const nsgpu_surf_result &lastResult() const { return last_; }
 
     int inside = 5;
@@ -186,22 +194,29 @@ class Surface {
     int precision = NSGPU_PREC_FP64;
     bool computeNormals = false;  // Compute_Vertex_Normals
     int device = 0;
+    bool patchBased = true;  // Patch_Based_Algorithm (triangle meshes; ses and skin are always patch based)
+    int auxGrid2D = 100;     // Max_mesh_auxiliary_grid_2d_size
     std::vector<double> vertList;
     std::vector<int32_t> triList;
     std::vector<double> normList;  // filled when computeNormals
 
    protected:
+    // the surface's device representation (patches by default)
+    virtual bool upload() {
+        const nsgpu_patches *p = patches();
+        return p && nsgpu_set_patches(ctx_, p, probe_radius) == NSGPU_OK;
+    }
     bool ensure_device() {
         if (!ctx_ && nsgpu_create(device, &ctx_) != NSGPU_OK) return false;
         if (!uploaded_) {
             if (nsgpu_set_grid(ctx_, &delphi->grid) != NSGPU_OK) return false;
-            const nsgpu_patches *p = patches();
-            if (!p || nsgpu_set_patches(ctx_, p, probe_radius) != NSGPU_OK) return false;
+            if (!upload()) return false;
             uploaded_ = true;
         }
         return true;
     }
     DelPhiShared *delphi;
+    std::string err_;
     nsgpu_ctx *ctx_ = nullptr;
     bool uploaded_ = false;
     nsgpu_surf_result last_{};
@@ -236,6 +251,107 @@ class ConnollySurface : public Surface {
     nsgpu_patches view_{};
 };
 
+// "mesh": a triangle mesh read from an OFF file (MeshSurface::loadOFF,
+// MeshSurface.cpp:2350-2479: coordinates read as floats), cast on the device
+// with the ray-based (Patch_Based_Algorithm = false) or patch-based rule.
+class MeshSurface : public Surface {
+   public:
+    MeshSurface(ConfigFile *cf, DelPhiShared *ds) : Surface(cf, ds) {
+        if (cf) fileName = cf->read<std::string>("Surface_File_Name", fileName);
+        computeNormals = false;  // vertex normals need analytic patches on the device
+    }
+    bool build() override { return loadOFF(fileName); }
+    bool loadOFF(const std::string &fn) {
+        std::ifstream in(fn);
+        if (!in) {
+            err_ = "cannot read " + fn;
+            return false;
+        }
+        std::string line;
+        bool tag = false;
+        long nv = -1, nt = -1;
+        while (std::getline(in, line)) {
+            if (line.size() <= 1) continue;
+            if (!tag && line[0] == 'O') {
+                tag = true;
+                continue;
+            }
+            if (line[0] == '#') continue;
+            if (tag) {
+                std::istringstream h(line);
+                h >> nv >> nt;
+                break;
+            }
+        }
+        if (!tag || nv <= 0 || nt <= 0) {
+            err_ = "cannot read the OFF header of " + fn;
+            return false;
+        }
+        verts.clear();
+        tris.clear();
+        while ((long)verts.size() < 3 * nv && std::getline(in, line)) {
+            if (line.size() <= 1) continue;
+            float a = 0, b = 0, c = 0;
+            std::sscanf(line.c_str(), "%f %f %f", &a, &b, &c);
+            verts.insert(verts.end(), {(double)a, (double)b, (double)c});
+        }
+        while ((long)tris.size() < 3 * nt && std::getline(in, line)) {
+            if (line.size() <= 1) continue;
+            int k = 0, a = 0, b = 0, c = 0;
+            std::sscanf(line.c_str(), "%d %d %d %d", &k, &a, &b, &c);
+            if (k != 3) {
+                err_ = "non triangular mesh";
+                return false;
+            }
+            tris.insert(tris.end(), {a, b, c});
+        }
+        uploaded_ = false;
+        return (long)verts.size() == 3 * nv && (long)tris.size() == 3 * nt;
+    }
+    int getNumPatches() override { return (int)(tris.size() / 3); }
+    const nsgpu_patches *patches() override { return nullptr; }
+    std::string fileName = "mesh.off";
+    std::vector<double> verts;
+    std::vector<int32_t> tris;
+
+   protected:
+    bool upload() override {
+        return nsgpu_set_mesh(ctx_, verts.data(), (int32_t)(verts.size() / 3), tris.data(), (int32_t)(tris.size() / 3),
+                              patchBased ? NSGPU_MESH_PATCH_BASED : NSGPU_MESH_RAY_BASED, auxGrid2D) == NSGPU_OK;
+    }
+};
+
+// "blobby": the blobby mesh built on the device from the atoms (Blobbyness,
+// cutoff 6 A), then cast as a ray-based mesh.  As in the reference, it needs
+// Patch_Based_Algorithm = false (main.cpp:1154-1161).
+class BlobbySurface : public Surface {
+   public:
+    BlobbySurface(ConfigFile *cf, DelPhiShared *ds) : Surface(cf, ds) {
+        if (cf) B = cf->read<double>("Blobbyness", B);
+        if (B >= 0) B = -2.5;  // setBlobbyness, BlobbySurface.cpp:46-54
+        computeNormals = false;
+    }
+    bool build() override {
+        if (patchBased) {
+            err_ = "This surface type does not support patch based ray-tracing: set Patch_Based_Algorithm = false";
+            return false;
+        }
+        if (!delphi || delphi->atoms.empty()) return false;
+        if (!ctx_ && nsgpu_create(device, &ctx_) != NSGPU_OK) return false;
+        if (nsgpu_set_grid(ctx_, &delphi->grid) != NSGPU_OK) return false;
+        if (nsgpu_blobby_build(ctx_, delphi->atoms.data(), (int32_t)(delphi->atoms.size() / 4), B, cutoff,
+                               auxGrid2D, &info) != NSGPU_OK)
+            return false;
+        uploaded_ = true;
+        return true;
+    }
+    int getNumPatches() override { return 0; }  // BlobbySurface.cpp:71-74
+    const nsgpu_patches *patches() override { return nullptr; }
+    double B = -2.5;
+    double cutoff = 6.0;  // DEFAULT_CUTOFF, BlobbySurface.h:19
+    nsgpu_mesh_info info{};
+};
+
 typedef Surface *(*surface_instantiator)(ConfigFile *, DelPhiShared *);
 
 class SurfaceFactory {
@@ -257,6 +373,8 @@ inline SurfaceFactory &surfaceFactory() {
     static SurfaceFactory f;
     static bool init = [] {
         f.add("ses", [](ConfigFile *c, DelPhiShared *d) -> Surface * { return new ConnollySurface(c, d); });
+        f.add("mesh", [](ConfigFile *c, DelPhiShared *d) -> Surface * { return new MeshSurface(c, d); });
+        f.add("blobby", [](ConfigFile *c, DelPhiShared *d) -> Surface * { return new BlobbySurface(c, d); });
         return true;
     }();
     (void)init;

@@ -56,8 +56,10 @@ def save_off(path, verts, tris, normals=None, revert=False, comment="# File crea
         f.write("".join("3 %d %d %d\n" % tuple(r) for r in t))
 
 
-def load_off(path):
-    """(verts, tris, normals or None) of an OFF / OFF+N file written by save_off."""
+def load_off(path, as_float32=False):
+    """(verts, tris, normals or None) of an OFF / OFF+N file written by save_off.
+    as_float32: coordinates rounded to float as MeshSurface::loadOFF reads them
+    (sscanf "%f", MeshSurface.cpp:2436-2446), the input of a "mesh" surface."""
     with open(path) as f:
         head = f.readline().strip()
         line = f.readline()
@@ -67,4 +69,10 @@ def load_off(path):
         rows = np.array([[float(x) for x in f.readline().split()] for _ in range(nv)]).reshape(nv, -1)
         tris = np.array([[int(x) for x in f.readline().split()[1:4]] for _ in range(nt)], dtype=np.int64).reshape(nt, 3)
     normals = rows[:, 3:6] if head == "OFF+N" else None
-    return rows[:, :3], tris, normals
+    verts = rows[:, :3]
+    if as_float32:
+        # float(str) is the nearest double; the nearest float to the decimal
+        # text is the float nearest that double except at exact float
+        # midpoints, which a 3-decimal text cannot be
+        verts = np.array([[float(np.float32(x)) for x in r] for r in verts]).reshape(-1, 3)
+    return verts, tris, normals

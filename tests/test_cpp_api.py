@@ -44,3 +44,32 @@ def test_cpp_client_full_path_matches_python(client, device_ok):
     assert vol == pytest.approx(r["volume"], rel=1e-12)
     assert area == pytest.approx(m["area"], rel=1e-12)
     assert int(line[7]) == int((dev.status() == -1).sum())
+
+
+@pytest.mark.gpu
+def test_cpp_client_blobby_and_mesh(client, device_ok, tmp_path):
+    import numpy as np
+
+    import nanoshaper_amd as ns
+    from nanoshaper_amd.io import save_off
+    xyzr, _ = ns.generate_chain(330, 12.0, seed=0)
+    g = ns.build_grid(xyzr, 2.0, 80.0)
+    dev = ns.DeviceSurface(0)
+    dev.set_grid(g)
+    dev.blobby_build(xyzr)
+    r = dev.get_surf(inside=5)
+    m = dev.triangulate()
+    out = subprocess.run([client, "blobby"], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    f = out.stdout.split()
+    assert float(f[1]) == pytest.approx(r["volume"], rel=1e-12)
+    assert float(f[3]) == pytest.approx(m["area"], rel=1e-12)
+    assert int(f[7]) == int((dev.status() == -1).sum())
+    # the blobby mesh written as OFF and loaded as a "mesh" surface: the same
+    # numbers (the file holds the same 3-decimal coordinates)
+    v, t = dev.surface_mesh()
+    off = str(tmp_path / "blobby.off")
+    save_off(off, v, t)
+    out = subprocess.run([client, "mesh", off], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert float(out.stdout.split()[1]) == pytest.approx(r["volume"], rel=1e-12)
