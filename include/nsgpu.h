@@ -87,6 +87,19 @@ typedef struct {
 #define NSGPU_PATCH_TORUS_REG 3
 #define NSGPU_PATCH_TORUS_SING 4
 #define NSGPU_TORUS_HDR 14
+/* Skin mixed cells (SkinSurface.h:18-101), quadric patches:              */
+/*   data[off .. off+6)   bounding box (down x, y, z, up x, y, z) of the    */
+/*                        cell's points (the atom's sphere for SKIN_POINT); */
+/*   data[off+6 .. +16)   quadric Q in the reference's order q11 q22 q33    */
+/*                        q44 q12 q23 q34 q13 q24 q14 (SkinSurface.cpp:4508);*/
+/*   then cnt planes (a,b,c,d): a hit is rejected when a x+b y+c z+d > 0    */
+/*   (the point cell's planes stored negated, so one rule serves all).      */
+/* bsph: the atom (centre, radius) of a SKIN_POINT cell, zeros otherwise.   */
+#define NSGPU_PATCH_SKIN_POINT 5 /* Del0 x Vor3: sphere of the atom        */
+#define NSGPU_PATCH_SKIN_EDGE 6  /* Del1 x Vor2: hyperboloid               */
+#define NSGPU_PATCH_SKIN_FACET 7 /* Del2 x Vor1: hyperboloid               */
+#define NSGPU_PATCH_SKIN_TETRA 8 /* Del3 x Vor0: sphere                    */
+#define NSGPU_SKIN_HDR 16
 
 typedef struct {
     int32_t n;           /* number of patches                           */
@@ -169,6 +182,21 @@ void nsgpu_patch_set_view(const nsgpu_patch_set *ps, nsgpu_patches *view);
 void nsgpu_patch_set_free(nsgpu_patch_set *ps);
 /* counts[5] = number of patches of each NSGPU_PATCH_* kind */
 void nsgpu_patch_set_counts(const nsgpu_patch_set *ps, int64_t counts[5]);
+
+/* Skin mixed complex, the host stage SkinSurface::buildSkinCGAL
+ * (SkinSurface.cpp:1271-1811) minus the triangulation itself: CGAL's
+ * Regular_triangulation_3 in the reference, any regular triangulator here.
+ * nsgpu_skin_points writes the (n + 6) weighted points to triangulate: the
+ * atoms (x, y, z, r^2/s) and six dummy points of weight -1 (:1290-1345).
+ * nsgpu_skin_build takes the finite tetrahedra of their regular
+ * triangulation (tets [ntets*4] point indices) with their neighbours (nbrs
+ * [ntets*4], the tetrahedron opposite vertex k, -1 on the hull) and emits the
+ * mixed cells (BuildDelaunayTetraCells/PointCells/EdgeCells/FacetCells,
+ * :348-1210) as a patch set of NSGPU_PATCH_SKIN_* kinds.  xyzr are the
+ * atoms after jitter; s is Skin_Surface_Parameter (0 < s < 1). */
+int nsgpu_skin_points(const double *xyzr, int32_t n, double s, double *wpts);
+int nsgpu_skin_build(const double *xyzr, int32_t n, double s, const int32_t *tets, const int32_t *nbrs,
+                     int32_t ntets, nsgpu_patch_set **out);
 
 /* -------------------- device hot path ------------------------------------- */
 /* Grid upload: replaces the DelPhiShared arrays the reference colours in place. */

@@ -57,6 +57,9 @@ SIGNATURES = {
     "nsgpu_build_grid": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.POINTER(Grid)]),
     "nsgpu_jitter": (None, [VP, ctypes.c_int32, ctypes.c_double, c_int32_p]),
     "nsgpu_ses_build": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_double, ctypes.POINTER(VP)]),
+    "nsgpu_skin_points": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_double, VP]),
+    "nsgpu_skin_build": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_double, VP, VP, ctypes.c_int32,
+                                        ctypes.POINTER(VP)]),
     "nsgpu_patch_set_view": (None, [VP, ctypes.POINTER(Patches)]),
     "nsgpu_patch_set_free": (None, [VP]),
     "nsgpu_patch_set_counts": (None, [VP, VP]),

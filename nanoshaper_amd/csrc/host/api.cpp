@@ -71,7 +71,7 @@ struct nsgpu_ctx {
     DBuf gx, gy, gz;
     // patches
     bool have_patches = false;
-    int npatch = 0, npatch_plain = 0;
+    int npatch = 0, npatch_plain = 0, npatch_skin = 0;
     double probe = 1.4;
     DBuf pkind, poff, pcnt, pflags, pbsph, pdata, porder, patom;
     int patch_max_atom = -1;  // largest atom id the patches name (-1: no atom ids given)
@@ -291,8 +291,10 @@ int nsgpu_set_patches(nsgpu_ctx *c, const nsgpu_patches *p, double probe_radius)
         return NSGPU_E_ARG;
     for (int i = 0; i < p->n; ++i) {
         const int k = p->kind[i];
-        if (k < NSGPU_PATCH_POINT || k > NSGPU_PATCH_TORUS_SING) return fail(c, NSGPU_E_ARG, "bad patch kind");
-        const int64_t need = (k >= NSGPU_PATCH_TORUS_REG) ? NSGPU_TORUS_HDR + 8LL * p->cnt[i] : 4LL * p->cnt[i];
+        if (k < NSGPU_PATCH_POINT || k > NSGPU_PATCH_SKIN_TETRA) return fail(c, NSGPU_E_ARG, "bad patch kind");
+        const int64_t need = (k >= NSGPU_PATCH_SKIN_POINT)  ? NSGPU_SKIN_HDR + 4LL * p->cnt[i]
+                             : (k >= NSGPU_PATCH_TORUS_REG) ? NSGPU_TORUS_HDR + 8LL * p->cnt[i]
+                                                            : 4LL * p->cnt[i];
         if (p->off[i] < 0 || p->cnt[i] < 0 || (int64_t)p->off[i] + need > p->n_data)
             return fail(c, NSGPU_E_ARG, "patch data out of range");
         if (k == NSGPU_PATCH_TORUS_REG && (p->cnt[i] < 1 || p->cnt[i] > 31))
@@ -315,14 +317,17 @@ int nsgpu_set_patches(nsgpu_ctx *c, const nsgpu_patches *p, double probe_radius)
     }
     if (p->n_data)
         HIP_TRY(c, hipMemcpyAsync(c->pdata.p, p->data, (size_t)p->n_data * 8, hipMemcpyHostToDevice, c->stream));
-    // patch ids by class: spheres and facets first, then tori
+    // patch ids by class: spheres and facets first, then tori, then skin cells
     std::vector<int> order;
     order.reserve(n);
     for (int i = 0; i < p->n; ++i)
         if (p->kind[i] < NSGPU_PATCH_TORUS_REG) order.push_back(i);
     const int n_plain = (int)order.size();
     for (int i = 0; i < p->n; ++i)
-        if (p->kind[i] >= NSGPU_PATCH_TORUS_REG) order.push_back(i);
+        if (p->kind[i] == NSGPU_PATCH_TORUS_REG || p->kind[i] == NSGPU_PATCH_TORUS_SING) order.push_back(i);
+    const int n_skin = p->n - (int)order.size();
+    for (int i = 0; i < p->n; ++i)
+        if (p->kind[i] >= NSGPU_PATCH_SKIN_POINT) order.push_back(i);
     HIP_TRY(c, c->porder.ensure(std::max<size_t>(n, 1) * 4));
     if (n) HIP_TRY(c, hipMemcpyAsync(c->porder.p, order.data(), n * 4, hipMemcpyHostToDevice, c->stream));
     // atom ids (optional: only the adjoint needs them)
@@ -337,6 +342,7 @@ int nsgpu_set_patches(nsgpu_ctx *c, const nsgpu_patches *p, double probe_radius)
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->npatch = p->n;
     c->npatch_plain = n_plain;
+    c->npatch_skin = n_skin;
     c->probe = probe_radius;
     c->have_patches = true;
     c->surface_is_mesh = false;
@@ -347,7 +353,7 @@ int nsgpu_set_patches(nsgpu_ctx *c, const nsgpu_patches *p, double probe_radius)
 int nsgpu_intersect_path_counts(nsgpu_ctx *c, const nsgpu_surf_opts *o, int64_t *counts, int n) {
     if (!c || !o || !counts || n < 8) return NSGPU_E_ARG;
     if (!c->have_grid || !c->have_patches) return fail(c, NSGPU_E_STATE, "set_grid and set_patches first");
-    if (c->surface_is_mesh) return fail(c, NSGPU_E_STATE, "path counts need an SES patch set");
+    if (c->surface_is_mesh || c->npatch_skin) return fail(c, NSGPU_E_STATE, "path counts need an SES patch set");
     HIP_TRY(c, hipSetDevice(c->device));
     int panels[2] = {3, 3};
     if (!o->build_status && !o->build_epsmap) panels[0] = 0;
@@ -367,6 +373,7 @@ int nsgpu_intersect_path_counts(nsgpu_ctx *c, const nsgpu_surf_opts *o, int64_t 
     P.data = c->pdata.as<double>();
     P.order = c->porder.as<int>();
     P.n_plain = c->npatch_plain;
+    P.n_skin = c->npatch_skin;
     P.atom = c->patom.as<int>();
     const int64_t rows = intersect_workgroups(P, R);
     DBuf part, out;
@@ -430,6 +437,7 @@ static int enqueue_trace(nsgpu_ctx *c, hipStream_t st) {
     P.data = c->pdata.as<double>();
     P.order = c->porder.as<int>();
     P.n_plain = c->npatch_plain;
+    P.n_skin = c->npatch_skin;
     P.atom = c->patom.as<int>();
     HIP_TRY(c, c->rcnt.ensure(nrays * 4));
     HIP_TRY(c, c->rvol.ensure(nrays * 8));
@@ -630,7 +638,7 @@ static int cavities_on_host(nsgpu_ctx *c) {
 // panel rule of getSurf (Surface.cpp:652-659) and the per-call state
 static int begin_surf(nsgpu_ctx *c, const nsgpu_surf_opts *o) {
     if (!c->have_grid || !c->have_patches) return fail(c, NSGPU_E_STATE, "set_grid and set_patches first");
-    if (c->surface_is_mesh && c->want_normals && o->accurate_triangulation)
+    if ((c->surface_is_mesh || c->npatch_skin) && c->want_normals && o->accurate_triangulation)
         return fail(c, NSGPU_E_STATE, "vertex normals need an SES patch set");
     if (o->precision != NSGPU_PREC_FP64 && o->precision != NSGPU_PREC_MIXED && o->precision != NSGPU_PREC_STURM)
         return fail(c, NSGPU_E_ARG, "unknown precision mode");
@@ -1035,7 +1043,7 @@ static void install_mesh(nsgpu_ctx *c, int64_t nv, int64_t nt, int ray_based) {
     c->mesh_ray_based = ray_based != 0;
     c->surface_is_mesh = true;
     c->have_patches = true;
-    c->npatch = c->npatch_plain = 0;
+    c->npatch = c->npatch_plain = c->npatch_skin = 0;
     c->patch_max_atom = -1;
     c->have_trace = c->have_surf = c->have_vertices = c->have_mesh = false;
 }
@@ -1313,7 +1321,7 @@ int nsgpu_surface_gradient(nsgpu_ctx *c, double *volume, double *area, double *d
                            nsgpu_surf_result *res) {
     if (!c || !volume || !area) return NSGPU_E_ARG;
     if (!c->have_grid || !c->have_patches) return fail(c, NSGPU_E_STATE, "set_grid and set_patches first");
-    if (c->surface_is_mesh) return fail(c, NSGPU_E_STATE, "gradients need an SES patch set");
+    if (c->surface_is_mesh || c->npatch_skin) return fail(c, NSGPU_E_STATE, "gradients need an SES patch set");
     if (c->npatch > 0 && c->patch_max_atom < 0) return fail(c, NSGPU_E_STATE, "the patch set carries no atom ids");
     if (c->patch_max_atom >= c->natoms) return fail(c, NSGPU_E_STATE, "set_atoms with the patch set's atoms first");
     HIP_TRY(c, hipSetDevice(c->device));
@@ -1341,6 +1349,7 @@ int nsgpu_surface_gradient(nsgpu_ctx *c, double *volume, double *area, double *d
     P.data = c->pdata.as<double>();
     P.order = c->porder.as<int>();
     P.n_plain = c->npatch_plain;
+    P.n_skin = c->npatch_skin;
     P.atom = c->patom.as<int>();
     const size_t ng = std::max<size_t>((size_t)c->natoms, 1) * 4;
     const int64_t nb = adjoint_blocks(R);
@@ -1407,6 +1416,7 @@ int nsgpu_read_normals(nsgpu_ctx *c, double *normals) {
     P.data = c->pdata.as<double>();
     P.order = c->porder.as<int>();
     P.n_plain = c->npatch_plain;
+    P.n_skin = c->npatch_skin;
     P.atom = c->patom.as<int>();
     HIP_TRY(c, c->normals.ensure((size_t)std::max<int64_t>(c->nverts, 1) * 24));
     NormalArgs a;

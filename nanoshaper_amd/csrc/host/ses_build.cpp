@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "nsgpu.h"
+#include "patch_set.hpp"
 
 namespace {
 
@@ -544,12 +545,6 @@ struct Builder {
 
 }  // namespace
 
-struct nsgpu_patch_set {
-    std::vector<int32_t> kind, off, cnt, flags, atom;
-    std::vector<double> bsph, data;
-    int64_t counts[5] = {0, 0, 0, 0, 0};
-    int errors = 0;
-};
 
 extern "C" int nsgpu_ses_build(const double *xyzr, int32_t n, double probe_radius, nsgpu_patch_set **out) {
     if (!xyzr || n <= 0 || !out || !(probe_radius > 0)) return NSGPU_E_ARG;

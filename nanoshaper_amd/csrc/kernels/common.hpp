@@ -24,11 +24,12 @@ struct DevPatches {
     const int *kind, *off, *cnt, *flags;
     const double *bsph;  // 4 per patch
     const double *data;
-    // patch ids ordered by class: [0, n_plain) spheres and facets, then tori;
-    // the two classes run as separate launches (the torus path needs far more
-    // registers than the plain one)
+    // patch ids ordered by class: [0, n_plain) spheres and facets, then tori,
+    // then skin cells; each class runs as its own launch (the torus path needs
+    // far more registers than the plain one)
     const int *order;
     int n_plain;
+    int n_skin;  // skin cells, last in order (after the tori)
     const int *atom;  // 3 per patch: the atoms the patch is built on (-1 unused)
 };
 

@@ -19,6 +19,8 @@ void launch_intersect(hipStream_t, const DevGrid &, const DevPatches &, const De
                       unsigned long long *part, int roots);
 void launch_intersect_count(hipStream_t, const DevGrid &, const DevPatches &, const DevRays &, double,
                             unsigned long long *part);
+// skin cells: order[n - n_skin, n), same per-workgroup candidate rows
+void launch_intersect_skin(hipStream_t, const DevGrid &, const DevPatches &, const DevRays &, unsigned long long *part);
 void launch_sort(hipStream_t, const DevGrid &, const DevRays &, unsigned long long *part);
 void launch_reduce_cols(hipStream_t, const unsigned long long *in, int64_t nrows, int ncols, unsigned int max_mask,
                         unsigned long long *out);

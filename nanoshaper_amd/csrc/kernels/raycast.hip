@@ -962,9 +962,11 @@ static void launch_classes(hipStream_t s, const DevGrid &g, const DevPatches &P,
     if (P.n_plain > 0)
         hipLaunchKernelGGL((k_intersect<kCount, kRoots, false>), dim3((unsigned)P.n_plain, (unsigned)R.nslots),
                            dim3(64), 0, s, g, P, R, probe, part);
-    if (P.n > P.n_plain)
-        hipLaunchKernelGGL((k_intersect<kCount, kRoots, true>), dim3((unsigned)(P.n - P.n_plain), (unsigned)R.nslots),
-                           dim3(64), 0, s, g, P, R, probe, part);
+    if (P.n - P.n_skin > P.n_plain)
+        hipLaunchKernelGGL((k_intersect<kCount, kRoots, true>),
+                           dim3((unsigned)(P.n - P.n_skin - P.n_plain), (unsigned)R.nslots), dim3(64), 0, s, g, P, R,
+                           probe, part);
+    if (!kCount && P.n_skin > 0) launch_intersect_skin(s, g, P, R, part);
 }
 void launch_intersect(hipStream_t s, const DevGrid &g, const DevPatches &P, const DevRays &R, double probe,
                       unsigned long long *part, int roots) {

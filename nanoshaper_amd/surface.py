@@ -127,6 +127,48 @@ def ses_patches(xyzr, probe=1.4):
     return PatchSet(h, float(probe))
 
 
+SKIN_KINDS = ("skin_point", "skin_edge", "skin_facet", "skin_tetra")
+
+
+def regular_triangulation(wpts):
+    """Finite tetrahedra of the regular (weighted Delaunay) triangulation of
+    weighted points (x, y, z, w) and their neighbours (-1 on the hull): the
+    lower convex hull of the points lifted to (x, y, z, |p|^2 - w), by Qhull.
+    Coordinates are centred first (the triangulation does not move)."""
+    from scipy.spatial import ConvexHull
+    p = np.asarray(wpts, dtype=np.float64).reshape(-1, 4)
+    xyz = p[:, :3] - p[:, :3].mean(axis=0)
+    lifted = np.column_stack([xyz, (xyz * xyz).sum(axis=1) - p[:, 3]])
+    hull = ConvexHull(lifted)
+    lower = hull.equations[:, 3] < 0
+    idx = np.full(len(lower), -1, np.int64)
+    idx[lower] = np.arange(int(lower.sum()))
+    tets = np.ascontiguousarray(hull.simplices[lower], dtype=np.int32)
+    nb = hull.neighbors[lower]
+    nbrs = np.ascontiguousarray(np.where(nb >= 0, idx[np.maximum(nb, 0)], -1), dtype=np.int32)
+    return tets, nbrs
+
+
+def skin_patches(xyzr, s=0.45):
+    """Skin mixed cells of (already jittered) atoms with shrink factor s
+    (Skin_Surface_Parameter): the weighted points and dummies of the
+    reference, their regular triangulation (Qhull), then the cells."""
+    lib = _lib.load()
+    a = as_xyzr(xyzr)
+    n = a.shape[0]
+    w = np.zeros(((n + 6), 4))
+    if lib.nsgpu_skin_points(_ptr(a), n, float(s), _ptr(w)) != 0:
+        raise NsgpuError("skin_points failed")
+    tets, nbrs = regular_triangulation(w)
+    h = ctypes.c_void_p()
+    rc = lib.nsgpu_skin_build(_ptr(a), n, float(s), _ptr(tets), _ptr(nbrs), tets.shape[0], ctypes.byref(h))
+    if rc != 0:
+        if h:
+            lib.nsgpu_patch_set_free(h)
+        raise NsgpuError("skin_build failed (%d)" % rc)
+    return PatchSet(h, 1.4)
+
+
 def surf_opts(build_epsmap=True, build_status=True, accurate_triangulation=True, fill_cavities=True,
               cav_vol=11.4, inside=5, precision=0):
     return SurfOpts(int(build_epsmap), int(build_status), int(accurate_triangulation), int(fill_cavities),

@@ -864,6 +864,127 @@ static void mesh_intersections(orc_surface *s, orc_hit **hits, int64_t *nh, int6
     }
 }
 
+
+/* ------------------------------------------------------------------------- */
+/* skin mixed cells: SkinSurface::getPatchIntersectionData                   */
+/* (SkinSurface.cpp:3422-4188, RAY_VS_PATCH_TESTS_CULLING off)                */
+/* ------------------------------------------------------------------------- */
+
+/* rayQuadricIntersectionX/Y/Z (SkinSurface.cpp:4632-4708): unit-axis ray */
+int orc_ray_quadric(const double *Q, const double *o, int axis, double t[2]) {
+    double q44 = Q[3], q33 = Q[2], q22 = Q[1], q11 = Q[0], q34 = Q[6], q24 = Q[8], q23 = Q[5], q14 = Q[9],
+           q13 = Q[7], q12 = Q[4];
+    double qa, b;
+    if (axis == 0) {
+        if (q11 == 0) return 0;
+        qa = q11;
+        b = q14 + o[2] * q13 + o[1] * q12 + o[0] * q11;
+    } else if (axis == 1) {
+        if (q22 == 0) return 0;
+        qa = q22;
+        b = q24 + o[2] * q23 + o[1] * q22 + o[0] * q12;
+    } else {
+        if (q33 == 0) return 0;
+        qa = q33;
+        b = q34 + o[2] * q33 + o[1] * q23 + o[0] * q13;
+    }
+    double c = q44 + (o[2] + o[2]) * q34 + (o[2] * o[2]) * q33 + (o[1] + o[1]) * q24 + (2 * o[1] * o[2]) * q23 +
+               (o[1] * o[1]) * q22 + (o[0] + o[0]) * q14 + (2 * o[0] * o[2]) * q13 + (2 * o[0] * o[1]) * q12 +
+               (o[0] * o[0]) * q11;
+    double delta = b * b - qa * c;
+    if (delta < 0) return 0;
+    delta = sqrt(delta);
+    t[0] = (-b - delta) / qa;
+    t[1] = (-b + delta) / qa;
+    return 1;
+}
+
+/* SkinSurface::isFeasible (:4374-4447) with every plane stored rejecting > 0 */
+static int skin_feasible(const double *pl, int cnt, const double *p) {
+    for (int q = 0; q < cnt; q++) {
+        const double *w = pl + 4 * q;
+        if (w[0] * p[0] + w[1] * p[1] + w[2] * p[2] + w[3] > 0) return 0;
+    }
+    return 1;
+}
+
+static void skin_cell_hits(orc_surface *s, int it, orc_hit **hits, int64_t *nh, int64_t *cap, int64_t *cand) {
+    const nsgpu_grid *g = &s->g;
+    const nsgpu_patches *P = &s->P;
+    const int64_t NX = s->NX, NY = s->NY, NZ = s->NZ, N_MAX = s->N;
+    const double *pd = P->data + P->off[it], *Q = pd + 6, *pl = pd + NSGPU_SKIN_HDR;
+    const int cnt = P->cnt[it], kind = P->kind[it];
+    const double *sphere_center = P->bsph + 4 * (size_t)it;
+    const double radius = P->bsph[4 * (size_t)it + 3];
+    const double downx = pd[0], downy = pd[1], downz = pd[2], upx = pd[3], upy = pd[4], upz = pd[5];
+    const double hside = g->hside;
+    if (downx > g->xmax + ORC_EPS_INT) return;
+    if (downy > g->ymax + ORC_EPS_INT) return;
+    if (downz > g->zmax + ORC_EPS_INT) return;
+    if (upx < g->xmin - (hside - ORC_DELTA + ORC_EPS_INT)) return;
+    if (upy < g->ymin - (hside - ORC_DELTA + ORC_EPS_INT)) return;
+    if (upz < g->zmin - (hside - ORC_DELTA + ORC_EPS_INT)) return;
+    int64_t i_start[3], i_end[3];
+    i_start[0] = (int64_t)rintp(fmax(0., (downx - g->xmin)) * g->scale);
+    i_start[1] = (int64_t)rintp(fmax(0., (downy - g->ymin)) * g->scale);
+    i_start[2] = (int64_t)rintp(fmax(0., (downz - g->zmin)) * g->scale);
+    i_end[0] = (int64_t)rintp(fmax(0., upx - g->xmin) * g->scale);
+    i_end[1] = (int64_t)rintp(fmax(0., upy - g->ymin) * g->scale);
+    i_end[2] = (int64_t)rintp(fmax(0., upz - g->zmin) * g->scale);
+    if (i_end[0] >= NX) i_end[0] = NX - 1;
+    if (i_end[1] >= NY) i_end[1] = NY - 1;
+    if (i_end[2] >= NZ) i_end[2] = NZ - 1;
+    int64_t pixels[3] = {1 + i_end[0] - i_start[0], 1 + i_end[1] - i_start[1], 1 + i_end[2] - i_start[2]};
+    const double down[3] = {downx, downy, downz}, up[3] = {upx, upy, upz};
+    for (int int_phase = 0; int_phase < 2; int_phase++) {
+        if (s->panels[int_phase] == 0) continue;
+        double delta = (int_phase == 0) ? 0. : ORC_DELTA - hside;
+        for (int panel = 0; panel < s->panels[int_phase]; panel++) {
+            double pa[3], ray_dir;
+            int first_dim, last_dim, vc;
+            if (panel == 0) {
+                first_dim = 1, last_dim = 2, vc = 0;
+                pa[0] = g->xmin + delta;
+                ray_dir = g->xmax - g->xmin;
+            } else if (panel == 1) {
+                first_dim = 0, last_dim = 1, vc = 2;
+                pa[2] = g->zmin + delta;
+                ray_dir = g->zmax - g->zmin;
+            } else {
+                first_dim = 0, last_dim = 2, vc = 1;
+                pa[1] = g->ymin + delta;
+                ray_dir = g->ymax - g->ymin;
+            }
+            const int slot = s->panels[0] * int_phase + panel;
+            const double *cf = axis_of(s, first_dim), *cl = axis_of(s, last_dim);
+            for (int64_t rp = 0; rp < pixels[first_dim] * pixels[last_dim]; rp++) {
+                int64_t n = rp / pixels[first_dim];
+                int64_t m = rp - n * pixels[first_dim];
+                n += i_start[last_dim];
+                m += i_start[first_dim];
+                pa[first_dim] = cf[m] + delta;
+                if (pa[first_dim] < down[first_dim] || pa[first_dim] > up[first_dim]) continue;
+                pa[last_dim] = cl[n] + delta;
+                if (pa[last_dim] < down[last_dim] || pa[last_dim] > up[last_dim]) continue;
+                (*cand)++;
+                if (kind == NSGPU_PATCH_SKIN_POINT) { /* EQ_CULLING */
+                    double dx = sphere_center[first_dim] - pa[first_dim];
+                    double dy = sphere_center[last_dim] - pa[last_dim];
+                    if (dx * dx + dy * dy > radius * radius) continue;
+                }
+                double t[2];
+                if (!orc_ray_quadric(Q, pa, vc, t)) continue;
+                for (int i = 0; i < 2; i++) {
+                    double intPoint[3] = {pa[0], pa[1], pa[2]};
+                    intPoint[vc] += t[i];
+                    if (!skin_feasible(pl, cnt, intPoint)) continue;
+                    push_hit_rec(hits, nh, cap, (n * N_MAX + m) * s->numPanels + slot, t[i] / ray_dir, it);
+                }
+            }
+        }
+    }
+}
+
 orc_surface *orc_create_mesh(const nsgpu_grid *g, const double *verts, int64_t nv, const int32_t *tris, int64_t nt,
                              int ray_based, int aux_dim) {
     int32_t dz[3] = {0, 0, 0};
@@ -894,6 +1015,10 @@ static void patch_intersections(orc_surface *s) {
     if (s->is_mesh) mesh_intersections(s, &hits, &nh, &cap, &cand);
     for (int it = 0; it < (s->is_mesh ? 0 : P->n); it++) {
         const int kind = P->kind[it];
+        if (kind >= NSGPU_PATCH_SKIN_POINT) {
+            skin_cell_hits(s, it, &hits, &nh, &cap, &cand);
+            continue;
+        }
         const double *sphere_center = P->bsph + 4 * (size_t)it;
         const double radius = P->bsph[4 * (size_t)it + 3];
         const int doTorus = (kind == NSGPU_PATCH_TORUS_REG || kind == NSGPU_PATCH_TORUS_SING);
