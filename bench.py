@@ -17,7 +17,9 @@ of the plain path are measured alongside:
 * "differentiable": 2,000 atoms at 3 points/A, three-panel volume and area
   estimate with their gradients for all atoms (trace + adjoint kernel);
 * "blobby": the blobby surface of 20,000 atoms at 2 points/A, built and ray
-  cast as a mesh on the device, with the oracle's time for the same work.
+  cast as a mesh on the device, with the oracle's time for the same work;
+* "skin": the skin surface (s = 0.45) of the same atoms, its mixed cells
+  built on the host and cast on the device, with the oracle's time.
 """
 import argparse
 import json
@@ -237,6 +239,57 @@ def bench_blobby(steps=5, cpu=True):
     return res
 
 
+def bench_skin(steps=10, cpu=True):
+    """Skin surface (s = 0.45): 20,000 atoms (compacted radius 48 A), 2.0
+    points/A, perfil 80.  Host: jitter, regular triangulation and mixed
+    cells, once.  A step is getSurf on the cells (maps + cavities + 6 panels)
+    plus triangulateSurface."""
+    xyzr, r_used = ns.generate_chain(20000, 48.0, seed=0)
+    grid = ns.build_grid(xyzr, 2.0, 80.0)
+    t0 = time.perf_counter()
+    jit, _ = ns.jitter(xyzr)
+    ps = ns.skin_patches(jit, 0.45)
+    host_s = time.perf_counter() - t0
+    dev = ns.DeviceSurface(0)
+    dev.set_grid(grid)
+    dev.set_patches(ps)
+    opts = dict(build_epsmap=True, build_status=True, accurate_triangulation=True, fill_cavities=True, inside=5)
+    for _ in range(2):
+        r = dev.get_surf(**opts)
+        m = dev.triangulate()
+    dev.set_profiling(True)
+    st = {k: 0.0 for k in ns.STAGES}
+    dev.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = dev.get_surf(**opts)
+        m = dev.triangulate()
+        for k, v in dev.stage_times().items():
+            st[k] += v / steps
+    dev.synchronize()
+    step = (time.perf_counter() - t0) / steps
+    dev.close()
+    kinds = np.bincount(ps.kind, minlength=9)[5:].tolist()
+    res = {"workload": "skin s 0.45: %d-atom chain (R %.1f A), 2.0 points/A perfil 80 -> %d^3; getSurf (maps + "
+                       "cavities + 6 panels) + triangulate" % (len(xyzr), r_used, grid.nx),
+           "ms_per_step": step * 1e3, "rays_per_s": r["total_rays"] / step,
+           "voxels_per_s": grid.nx * grid.ny * grid.nz / step, "stage_ms": st,
+           "cells": dict(zip(ns.surface.SKIN_KINDS, kinds)), "host_build_s": host_s,
+           "result": {"volume": r["volume"], "hits": r["total_hits"], "odd_rays": r["odd_rays"],
+                      "cavities": r["num_cavities"], "triangles": m["num_triangles"], "area": m["area"]}}
+    if cpu:
+        from oracle import binding
+        o = binding.OracleSurface(grid, ps)
+        t0 = time.perf_counter()
+        o.get_surf(**opts)
+        o.triangulate()
+        cpu_s = time.perf_counter() - t0
+        o.close()
+        res["cpu_baseline"] = {"seconds_per_step": cpu_s, "cores": 1, "kind": "port", "sample": "full workload, 1 run"}
+        res["speedup_vs_cpu"] = cpu_s / step
+    return res
+
+
 def copy_reference(nbytes, repeats=5):
     """Plain device-to-device copy of known size: the calibration kernel for
     the FETCH_SIZE / WRITE_SIZE counters (a copy reads and writes nbytes)."""
@@ -267,12 +320,16 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the mixed, batched, differentiable and blobby lines")
     ap.add_argument("--blobby-only", action="store_true", help="only the blobby line (profiling)")
+    ap.add_argument("--skin-only", action="store_true", help="only the skin line (profiling)")
     args = ap.parse_args()
     if args.copy_ref:
         copy_reference(args.copy_ref)
         return
     if args.blobby_only:
         print(json.dumps({"blobby": bench_blobby(args.steps, cpu=not args.no_cpu)}))
+        return
+    if args.skin_only:
+        print(json.dumps({"skin": bench_skin(args.steps, cpu=not args.no_cpu)}))
         return
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -350,6 +407,8 @@ def main():
         extras["differentiable"] = bench_differentiable()
         print("blobby ...", file=sys.stderr, flush=True)
         extras["blobby"] = bench_blobby(cpu=not args.no_cpu)
+        print("skin ...", file=sys.stderr, flush=True)
+        extras["skin"] = bench_skin(cpu=not args.no_cpu)
 
     cpu = None
     if not args.no_cpu:

@@ -1,8 +1,8 @@
 // Ray casting of the skin's mixed cells (SkinSurface::getPatchIntersectionData,
 // SkinSurface.cpp:3422-4188, with RAY_VS_PATCH_TESTS_CULLING off).
 //
-// One 64-lane workgroup per (cell, panel slot), lanes striding over the
-// pixel rectangle of the cell's bounding box, as for the SES patches.  Per
+// Per (cell, panel slot), the pixels of the rectangle of the cell's bounding
+// box are shared out over sixteen lanes (or a wave, for large cells).  Per
 // pixel inside the box: the disc cull of a point cell's atom (EQ_CULLING),
 // the unit-axis ray/quadric roots (rayQuadricIntersectionX/Y/Z,
 // :4632-4708), then the cell's planes at each root (isFeasible, :4374-4447).
@@ -14,7 +14,7 @@ namespace nsg {
 
 // unit-axis ray o + t e_axis against the quadric; the coefficient order and
 // the expressions are the reference's
-__device__ __forceinline__ bool ray_quadric(const double *Q, const double o[3], int axis, double t[2]) {
+__device__ __forceinline__ bool ray_quadric(const double *Q, const double o[3], const int axis, double t[2]) {
     const double q44 = Q[3], q33 = Q[2], q22 = Q[1], q11 = Q[0], q34 = Q[6], q24 = Q[8], q23 = Q[5], q14 = Q[9],
                  q13 = Q[7], q12 = Q[4];
     double qa, b;
@@ -48,68 +48,134 @@ __device__ __forceinline__ bool skin_feasible(const double *pl, int cnt, const d
     return true;
 }
 
+// The axes of a panel: rays along vc, pixels over (fd, ld) (Surface.cpp:652-659)
+template <int PANEL>
+struct PanelAxes {
+    static constexpr int fd = PANEL == 0 ? 1 : 0;
+    static constexpr int ld = PANEL == 1 ? 1 : 2;
+    static constexpr int vc = PANEL == 0 ? 0 : (PANEL == 1 ? 2 : 1);
+};
+
+// A cell's rectangle for one panel slot.
+struct SkinRect {
+    int it, kind, cnt;
+    int is0, is1, pf, total;
+    double down[3], up[3];
+};
+
+template <int PANEL>
+__device__ __forceinline__ SkinRect skin_rect(const DevGrid &g, const DevPatches &P, int it) {
+    using A = PanelAxes<PANEL>;
+    SkinRect c;
+    c.it = it;
+    c.kind = P.kind[it];
+    c.cnt = P.cnt[it];
+    const double *pd = P.data + P.off[it];
+    for (int q = 0; q < 3; ++q) {
+        c.down[q] = pd[q];
+        c.up[q] = pd[3 + q];
+    }
+    // CELL_CULLING (SkinSurface.cpp:3487-3500)
+    const double lowpad = g.hside - kDelta + kEpsInt;
+    const bool culled = c.down[0] > g.xmax + kEpsInt || c.down[1] > g.ymax + kEpsInt || c.down[2] > g.zmax + kEpsInt ||
+                        c.up[0] < g.xmin - lowpad || c.up[1] < g.ymin - lowpad || c.up[2] < g.zmin - lowpad;
+    const double gmin[3] = {g.xmin, g.ymin, g.zmin};
+    const int gn[3] = {g.nx, g.ny, g.nz};
+    int64_t is0 = (int64_t)rintp(fmax(0., c.down[A::fd] - gmin[A::fd]) * g.scale);
+    int64_t ie0 = (int64_t)rintp(fmax(0., c.up[A::fd] - gmin[A::fd]) * g.scale);
+    int64_t is1 = (int64_t)rintp(fmax(0., c.down[A::ld] - gmin[A::ld]) * g.scale);
+    int64_t ie1 = (int64_t)rintp(fmax(0., c.up[A::ld] - gmin[A::ld]) * g.scale);
+    if (ie0 >= gn[A::fd]) ie0 = gn[A::fd] - 1;
+    if (ie1 >= gn[A::ld]) ie1 = gn[A::ld] - 1;
+    const int64_t pf = 1 + ie0 - is0, pl = 1 + ie1 - is1;
+    c.is0 = (int)min(is0, (int64_t)gn[A::fd]);
+    c.is1 = (int)min(is1, (int64_t)gn[A::ld]);
+    c.pf = (int)max(pf, (int64_t)1);
+    c.total = (!culled && pf > 0 && pl > 0) ? (int)(pf * pl) : 0;
+    return c;
+}
+
+// pixels lane, lane + stride, ... of the rectangle; returns the candidates
+template <int PANEL>
+__device__ __forceinline__ unsigned int skin_rect_rays(const DevGrid &g, const DevPatches &P, const DevRays &R,
+                                                       const SkinRect &c, int slot, double delta, int lane,
+                                                       int stride) {
+    using A = PanelAxes<PANEL>;
+    const double *gf = A::fd == 0 ? g.x : g.y;
+    const double *gl = A::ld == 1 ? g.y : g.z;
+    const double gmin[3] = {g.xmin, g.ymin, g.zmin}, gmax[3] = {g.xmax, g.ymax, g.zmax};
+    const double ray_dir = gmax[A::vc] - gmin[A::vc];
+    const double *pd = P.data + P.off[c.it];
+    const double *sp = P.bsph + 4 * c.it;
+    const double sfd = sp[A::fd], sld = sp[A::ld], srad = sp[3];
+    unsigned int cand = 0;
+    for (int rp = lane; rp < c.total; rp += stride) {
+        const int nl = rp / c.pf;
+        const int m = rp - nl * c.pf + c.is0;
+        const int n = nl + c.is1;
+        double pa[3];
+        pa[A::vc] = gmin[A::vc] + delta;
+        pa[A::fd] = gf[m] + delta;
+        if (pa[A::fd] < c.down[A::fd] || pa[A::fd] > c.up[A::fd]) continue;
+        pa[A::ld] = gl[n] + delta;
+        if (pa[A::ld] < c.down[A::ld] || pa[A::ld] > c.up[A::ld]) continue;
+        ++cand;
+        if (c.kind == NSGPU_PATCH_SKIN_POINT) {  // EQ_CULLING (:3602-3608)
+            const double dx = sfd - pa[A::fd], dy = sld - pa[A::ld];
+            if (dx * dx + dy * dy > srad * srad) continue;
+        }
+        double t[2];
+        if (!ray_quadric(pd + 6, pa, A::vc, t)) continue;
+        for (int i = 0; i < 2; ++i) {
+            double ip[3] = {pa[0], pa[1], pa[2]};
+            ip[A::vc] += t[i];
+            if (!skin_feasible(pd + NSGPU_SKIN_HDR, c.cnt, ip)) continue;
+            push_hit(R, slot, (int64_t)n * R.N + m, t[i] / ray_dir, c.it);
+        }
+    }
+    return cand;
+}
+
+// grid: (ceil(n_skin / 4), nslots).  A wave takes four cells, sixteen lanes
+// each, so the lanes reading a cell's quadric and planes share their
+// addresses.  Rectangles above kWaveCell pixels (edge and facet cells next to
+// the molecule's outside span tens of A) are walked by the whole wave after.
+constexpr int kWaveCell = 256;
+
+template <int PANEL>
+__device__ __forceinline__ unsigned int skin_slot(const DevGrid &g, const DevPatches &P, const DevRays &R, int slot,
+                                                  double delta) {
+    const int first = P.n - P.n_skin;
+    const int lane = threadIdx.x, grp = lane >> 4, gl16 = lane & 15;
+    const int ci = blockIdx.x * 4 + grp;
+    unsigned int cand = 0;
+    bool big = false;
+    if (ci < P.n_skin) {
+        const SkinRect c = skin_rect<PANEL>(g, P, P.order[first + ci]);
+        big = c.total > kWaveCell;
+        if (!big) cand += skin_rect_rays<PANEL>(g, P, R, c, slot, delta, gl16, 16);
+    }
+    uint64_t mask = __ballot(big && gl16 == 0);
+    while (mask) {
+        const int b = __ffsll((unsigned long long)mask) - 1;
+        mask &= mask - 1;
+        const SkinRect c = skin_rect<PANEL>(g, P, P.order[first + blockIdx.x * 4 + (b >> 4)]);
+        cand += skin_rect_rays<PANEL>(g, P, R, c, slot, delta, lane, 64);
+    }
+    return cand;
+}
+
 __global__ void __launch_bounds__(64) k_intersect_skin(DevGrid g, DevPatches P, DevRays R, unsigned long long *part) {
     const int first = P.n - P.n_skin;
     const int64_t wg = (int64_t)blockIdx.y * P.n + first + blockIdx.x;
-    const int it = P.order[first + blockIdx.x];
     const int slot = blockIdx.y;
     const int phase = slot < R.panels0 ? 0 : 1;
     const int panel = phase == 0 ? slot : slot - R.panels0;
-    const int kind = P.kind[it];
-    const double *pd = P.data + P.off[it];
-    const double *Q = pd + 6;
-    const double *pl = pd + NSGPU_SKIN_HDR;
-    const int cnt = P.cnt[it];
-    const double down[3] = {pd[0], pd[1], pd[2]};
-    const double up[3] = {pd[3], pd[4], pd[5]};
-    const double sc[3] = {P.bsph[4 * it], P.bsph[4 * it + 1], P.bsph[4 * it + 2]};
-    const double radius = P.bsph[4 * it + 3];
-    // CELL_CULLING (SkinSurface.cpp:3487-3500)
-    if (down[0] > g.xmax + kEpsInt || down[1] > g.ymax + kEpsInt || down[2] > g.zmax + kEpsInt) return;
-    const double lowpad = g.hside - kDelta + kEpsInt;
-    if (up[0] < g.xmin - lowpad || up[1] < g.ymin - lowpad || up[2] < g.zmin - lowpad) return;
-    const double gmin[3] = {g.xmin, g.ymin, g.zmin}, gmax[3] = {g.xmax, g.ymax, g.zmax};
-    const int gn[3] = {g.nx, g.ny, g.nz};
-    int64_t is[3], ie[3];
-    for (int d = 0; d < 3; ++d) {
-        is[d] = (int64_t)rintp(fmax(0., down[d] - gmin[d]) * g.scale);
-        ie[d] = (int64_t)rintp(fmax(0., up[d] - gmin[d]) * g.scale);
-        if (ie[d] >= gn[d]) ie[d] = gn[d] - 1;
-    }
-    const int64_t pix[3] = {1 + ie[0] - is[0], 1 + ie[1] - is[1], 1 + ie[2] - is[2]};
     const double delta = (phase == 0) ? 0. : kDelta - g.hside;
-    int fd, ld, vc;
-    const double *gf, *gl;
-    if (panel == 0) { fd = 1; ld = 2; vc = 0; gf = g.y; gl = g.z; }
-    else if (panel == 1) { fd = 0; ld = 1; vc = 2; gf = g.x; gl = g.y; }
-    else { fd = 0; ld = 2; vc = 1; gf = g.x; gl = g.z; }
-    const double ray_dir = gmax[vc] - gmin[vc];
-    const int64_t total = pix[fd] * pix[ld];
-    unsigned int cand = 0;
-    for (int64_t rp = threadIdx.x; rp < total; rp += 64) {
-        const int64_t nl = rp / pix[fd];
-        const int64_t m = rp - nl * pix[fd] + is[fd];
-        const int64_t n = nl + is[ld];
-        double pa[3];
-        pa[vc] = gmin[vc] + delta;
-        pa[fd] = gf[m] + delta;
-        if (pa[fd] < down[fd] || pa[fd] > up[fd]) continue;
-        pa[ld] = gl[n] + delta;
-        if (pa[ld] < down[ld] || pa[ld] > up[ld]) continue;
-        ++cand;
-        if (kind == NSGPU_PATCH_SKIN_POINT) {  // EQ_CULLING (:3602-3608)
-            const double dx = sc[fd] - pa[fd], dy = sc[ld] - pa[ld];
-            if (dx * dx + dy * dy > radius * radius) continue;
-        }
-        double t[2];
-        if (!ray_quadric(Q, pa, vc, t)) continue;
-        for (int i = 0; i < 2; ++i) {
-            double ip[3] = {pa[0], pa[1], pa[2]};
-            ip[vc] += t[i];
-            if (!skin_feasible(pl, cnt, ip)) continue;
-            push_hit(R, slot, n * R.N + m, t[i] / ray_dir, it);
-        }
-    }
+    unsigned int cand;
+    if (panel == 0) cand = skin_slot<0>(g, P, R, slot, delta);
+    else if (panel == 1) cand = skin_slot<1>(g, P, R, slot, delta);
+    else cand = skin_slot<2>(g, P, R, slot, delta);
     for (int off = 32; off > 0; off >>= 1) cand += __shfl_down(cand, off, 64);
     if (threadIdx.x == 0 && cand) part[wg] = cand;
 }
@@ -117,7 +183,8 @@ __global__ void __launch_bounds__(64) k_intersect_skin(DevGrid g, DevPatches P, 
 void launch_intersect_skin(hipStream_t s, const DevGrid &g, const DevPatches &P, const DevRays &R,
                            unsigned long long *part) {
     if (P.n_skin <= 0 || R.nslots <= 0) return;
-    hipLaunchKernelGGL(k_intersect_skin, dim3((unsigned)P.n_skin, (unsigned)R.nslots), dim3(64), 0, s, g, P, R, part);
+    hipLaunchKernelGGL(k_intersect_skin, dim3((unsigned)((P.n_skin + 3) / 4), (unsigned)R.nslots), dim3(64), 0, s,
+                       g, P, R, part);
 }
 
 }  // namespace nsg
