@@ -145,3 +145,18 @@ def test_writers_follow_the_reference_formats(tmp_path):
     assert lines[-1] == "3 2 1 0"
     vv, tt, nn = io.load_off(str(tmp_path / "m.off"))
     assert np.allclose(vv, v) and np.array_equal(tt, t[:, ::-1]) and np.allclose(nn, n)
+
+
+def test_bench_intersect_kernels_name_the_profiled_kernels():
+    # the roofline's traffic comes from the committed counter summary: the
+    # bench must name the intersection kernels as rocprof lists them
+    import importlib.util
+    import json
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(root, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    kern = json.load(open(os.path.join(root, "profiles", "ses_c2_5000atoms_traffic.json")))["kernels"]
+    for k in bench.INTERSECT_KERNELS:
+        assert k in kern, k
