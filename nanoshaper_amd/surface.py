@@ -376,18 +376,28 @@ class DeviceSurface:
 
 
 class NanoShaper:
-    """Mirror of NS::NanoShaper (nanoshaper.h:294-424) for the "ses" surface.
+    """Mirror of NS::NanoShaper (nanoshaper.h:294-424).
 
-    ``build_analytical_surface`` runs the host stages (grid, jitter, patches),
-    ``colour_grid`` runs Surface::getSurf on the device and returns the
-    volume, ``triangulate`` returns the mesh area.
+    ``surface_type`` "ses" (parameter: Probe_Radius) or "skin" (parameter:
+    Skin_Surface_Parameter), the two types the reference's constructor takes
+    (nanoshaper.cpp:74-90); "blobby" (parameter: Blobbyness) is added here.
+    ``build_analytical_surface`` runs the host stages (grid, jitter, patches or
+    the blobby build), ``colour_grid`` runs Surface::getSurf on the device and
+    returns the volume, ``triangulate`` returns the mesh area.
     """
 
-    def __init__(self, atoms, surface_type="ses", probe=1.4, scale=2.0, perfil=80.0, device=0,
-                 rand_displacement=1e-4, fill_cavities=True, cav_vol=11.4, accurate_triangulation=True):
-        if surface_type != "ses":
-            raise NsgpuError("surface type %r is not available; only 'ses' is built" % surface_type)
+    DEFAULT_PARAMETER = {"ses": 1.4, "skin": 0.45, "blobby": -2.5}
+
+    def __init__(self, atoms, surface_type="ses", parameter=None, scale=2.0, perfil=80.0, device=0,
+                 rand_displacement=1e-4, fill_cavities=True, cav_vol=11.4, accurate_triangulation=True, probe=None):
+        if surface_type not in self.DEFAULT_PARAMETER:
+            raise NsgpuError("Unknown surface type %r" % surface_type)
+        if parameter is None:
+            parameter = probe if (probe is not None and surface_type == "ses") else self.DEFAULT_PARAMETER[surface_type]
+        self.surface_type = surface_type
+        self.parameter = float(parameter)
         self.atoms = as_xyzr(atoms)
+        probe = self.parameter if surface_type == "ses" else 1.4
         self.probe, self.scale, self.perfil = probe, scale, perfil
         self.rand_displacement = rand_displacement
         self.opts = dict(fill_cavities=fill_cavities, cav_vol=cav_vol,
@@ -397,10 +407,17 @@ class NanoShaper:
 
     def build_analytical_surface(self):
         self.grid = build_grid(self.atoms, self.scale, self.perfil)
-        self.jittered, _ = jitter(self.atoms, self.rand_displacement, 1234)
-        self.patches = ses_patches(self.jittered, self.probe)
         self.surface = DeviceSurface(self.device)
         self.surface.set_grid(self.grid)
+        if self.surface_type == "blobby":
+            # BlobbySurface::build: no jitter, the atoms as given
+            self.surface.blobby_build(self.atoms, self.parameter)
+            return True
+        self.jittered, _ = jitter(self.atoms, self.rand_displacement, 1234)
+        if self.surface_type == "skin":
+            self.patches = skin_patches(self.jittered, self.parameter)
+        else:
+            self.patches = ses_patches(self.jittered, self.probe)
         self.surface.set_patches(self.patches)
         return True
 

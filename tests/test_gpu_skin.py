@@ -88,3 +88,15 @@ def test_skin_refuses_normals_and_gradients(device_ok):
         dev.get_surf(**OPTS)
     with pytest.raises(ns.NsgpuError):
         dev.surface_gradient()
+
+
+@pytest.mark.parametrize("surface_type", ["ses", "skin", "blobby"])
+def test_facade_surface_types(device_ok, surface_type):
+    xyzr, _ = ns.generate_chain(200, 10.0, seed=5)
+    shaper = ns.NanoShaper(xyzr, surface_type, scale=2.0, perfil=80.0)
+    shaper.build_analytical_surface()
+    vol = shaper.colour_grid()
+    area = shaper.triangulate()
+    print("facade", surface_type, vol, area, flush=True)
+    assert vol > 0 and area > 0
+    assert shaper.get_grid_point_status(0, 0, 0) == 1

@@ -41,7 +41,9 @@ def test_argument_errors_without_device():
     with pytest.raises(ValueError):
         ns.build_grid(np.zeros((3, 3)))
     with pytest.raises(ns.NsgpuError):
-        ns.NanoShaper(a, surface_type="skin")
+        ns.NanoShaper(a, surface_type="mesh")  # the facade takes ses, skin and blobby
+    sk = ns.NanoShaper(a, surface_type="skin")
+    assert sk.parameter == 0.45 and ns.NanoShaper(a, "ses", probe=1.2).probe == 1.2
 
 
 def test_generator_is_deterministic_and_self_avoiding():
