@@ -16,33 +16,75 @@ __device__ __forceinline__ bool cov_bit(const unsigned long long *plane, int64_t
     return (plane[(int64_t)(n >> 6) * NN + ray] >> (n & 63)) & 1ull;
 }
 
-// A workgroup covers 64 (i) x 4 (j) nodes of one k plane.  The x-ray words
-// are wave-uniform; the z- and y-ray words of 64 consecutive i are 64
-// consecutive words of their planes.
+// A lane takes kFillNodes consecutive nodes of the flat order, so its epsmap
+// faces (12 bytes a node), idebmap bytes and status words are aligned 16-byte
+// stores; the x-ray words are shared by the lanes of a row and the z- and
+// y-ray words of consecutive i are consecutive words of their planes.
+constexpr int kFillNodes = 4;  // 8 was slower (0.25 vs 0.22 ms at 325^3)
+
 __global__ void __launch_bounds__(256) k_fill_bits(DevGrid g, DevRays R, FillOpts o, int *epsmap, uint8_t *idebmap,
                                                    int *status) {
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int k = blockIdx.z;
-    if (i >= g.nx || j >= g.ny) return;
+    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
+    const int64_t f0 = kFillNodes * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (f0 >= nvox) return;
     const int64_t NN = (int64_t)R.N * R.N, SW = (int64_t)R.W * NN;
-    bool e0 = false, in = false;
-    if (o.slot_x >= 0) {
-        const int64_t ray = (int64_t)k * R.N + j;
-        e0 = cov_bit(R.cov + o.slot_x * SW, NN, ray, i);
-        in = cov_bit(R.cov_st, NN, ray, i);
+    const int64_t plane = (int64_t)g.nx * g.ny;
+    int k = (int)(f0 / plane);
+    const int64_t rem = f0 - (int64_t)k * plane;
+    int j = (int)(rem / g.nx);
+    int i = (int)(rem - (int64_t)j * g.nx);
+    int eps[3 * kFillNodes], st[kFillNodes];
+    uint8_t ideb[kFillNodes];
+    const int nq = (int)min((int64_t)kFillNodes, nvox - f0);
+    for (int q = 0; q < kFillNodes; ++q) {
+        bool e0 = false, in = false, e1 = false, e2 = false;
+        if (q < nq) {
+            if (o.slot_x >= 0) {
+                const int64_t ray = (int64_t)k * R.N + j;
+                e0 = cov_bit(R.cov + o.slot_x * SW, NN, ray, i);
+                in = cov_bit(R.cov_st, NN, ray, i);
+            }
+            if (o.build_eps) {
+                e2 = o.slot_z >= 0 && cov_bit(R.cov + o.slot_z * SW, NN, (int64_t)j * R.N + i, k);
+                e1 = o.slot_y >= 0 && cov_bit(R.cov + o.slot_y * SW, NN, (int64_t)k * R.N + i, j);
+            }
+        }
+        eps[3 * q + 0] = e0 ? o.inside : 0;
+        eps[3 * q + 1] = e1 ? o.inside : 0;
+        eps[3 * q + 2] = e2 ? o.inside : 0;
+        ideb[q] = in ? 0 : 1;
+        st[q] = in ? NSGPU_STATUS_INSIDE : NSGPU_STATUS_TEMP_OUT;
+        if (++i == g.nx) {
+            i = 0;
+            if (++j == g.ny) {
+                j = 0;
+                ++k;
+            }
+        }
     }
-    const int64_t f = ((int64_t)k * g.ny + j) * g.nx + i;
-    if (o.build_eps) {
-        const bool e2 = o.slot_z >= 0 && cov_bit(R.cov + o.slot_z * SW, NN, (int64_t)j * R.N + i, k);
-        const bool e1 = o.slot_y >= 0 && cov_bit(R.cov + o.slot_y * SW, NN, (int64_t)k * R.N + i, j);
-        int *e = epsmap + 3 * f;
-        e[0] = e0 ? o.inside : 0;
-        e[1] = e1 ? o.inside : 0;
-        e[2] = e2 ? o.inside : 0;
-        idebmap[f] = in ? 0 : 1;
+    if (nq == kFillNodes) {
+        if (o.build_eps) {
+            int4 *e = reinterpret_cast<int4 *>(epsmap + 3 * f0);
+            for (int v = 0; v < 3 * kFillNodes / 4; ++v)
+                e[v] = make_int4(eps[4 * v], eps[4 * v + 1], eps[4 * v + 2], eps[4 * v + 3]);
+            uint32_t *iw = reinterpret_cast<uint32_t *>(idebmap + f0);
+            for (int v = 0; v < kFillNodes / 4; ++v)
+                iw[v] = (uint32_t)ideb[4 * v] | ((uint32_t)ideb[4 * v + 1] << 8) | ((uint32_t)ideb[4 * v + 2] << 16) |
+                        ((uint32_t)ideb[4 * v + 3] << 24);
+        }
+        if (o.build_status) {
+            int4 *sv = reinterpret_cast<int4 *>(status + f0);
+            for (int v = 0; v < kFillNodes / 4; ++v) sv[v] = make_int4(st[4 * v], st[4 * v + 1], st[4 * v + 2], st[4 * v + 3]);
+        }
+    } else {
+        for (int q = 0; q < nq; ++q) {
+            if (o.build_eps) {
+                for (int l = 0; l < 3; ++l) epsmap[3 * (f0 + q) + l] = eps[3 * q + l];
+                idebmap[f0 + q] = ideb[q];
+            }
+            if (o.build_status) status[f0 + q] = st[q];
+        }
     }
-    if (o.build_status) status[f] = in ? NSGPU_STATUS_INSIDE : NSGPU_STATUS_TEMP_OUT;
 }
 
 // Corner bitmap (1 = outside) from the edge-ray planes: bits start at 1 and
@@ -74,8 +116,9 @@ __global__ void __launch_bounds__(256) k_corners_bits(DevGrid g, DevRays R, Fill
 
 void launch_fill_bits(hipStream_t s, const DevGrid &g, const DevRays &R, const FillOpts &o, int *eps, uint8_t *ideb,
                       int *status) {
-    dim3 grid((unsigned)((g.nx + 63) / 64), (unsigned)((g.ny + 3) / 4), (unsigned)g.nz);
-    hipLaunchKernelGGL(k_fill_bits, grid, dim3(256), 0, s, g, R, o, eps, ideb, status);
+    const int64_t groups = ((int64_t)g.nx * g.ny * g.nz + kFillNodes - 1) / kFillNodes;
+    hipLaunchKernelGGL(k_fill_bits, dim3((unsigned)((groups + 255) / 256)), dim3(256), 0, s, g, R, o, eps, ideb,
+                       status);
 }
 
 void launch_corners_bits(hipStream_t s, const DevGrid &g, const DevRays &R, const FillOpts &o, uint32_t *bits) {
