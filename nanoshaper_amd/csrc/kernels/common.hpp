@@ -10,6 +10,14 @@ namespace nsg {
 constexpr double kEpsInt = 1e-8;  // Surface.h:286 EPS_INT
 constexpr double kDelta = 1e-10;  // Surface.h:290 delta_accurate_triangulation
 
+// The axes of a panel: rays along vc, pixels over (fd, ld) (Surface.cpp:652-659)
+template <int PANEL>
+struct PanelAxes {
+    static constexpr int fd = PANEL == 0 ? 1 : 0;
+    static constexpr int ld = PANEL == 1 ? 1 : 2;
+    static constexpr int vc = PANEL == 0 ? 0 : (PANEL == 1 ? 2 : 1);
+};
+
 // Grid as the kernels see it; node coordinates come from host-computed
 // arrays so that device and host agree bit for bit.
 struct DevGrid {

@@ -8,6 +8,7 @@
 // CPU code (ConnollySurface::getPatchIntersectionData, ConnollySurface.cpp:
 // 4319-4591; raySphere/rayTorus/quarticEqSolutions, tools.cpp:350-1099).
 #include <algorithm>
+#include <type_traits>
 
 #include "launch.hpp"
 
@@ -118,7 +119,7 @@ __device__ int quartic_roots(double roots[4], double b, double c, double d, doub
 // entry; the ray is moved there, brought into the torus frame and the quartic
 // in the distance along the (unit, axis-aligned) direction is solved; roots
 // behind the entry or outside the clipping sphere are dropped.
-__device__ int ray_torus(const double *td, const double sc[3], double probe, double radius, int vc,
+__device__ __forceinline__ int ray_torus(const double *td, const double sc[3], double probe, double radius, int vc,
                          const double orig[3], double ray_dir, double t[4]) {
     double o1[3] = {orig[0], orig[1], orig[2]};
     o1[vc] += t[0] * ray_dir;
@@ -352,7 +353,7 @@ __device__ __forceinline__ double polish_quartic_root(double s, double B, double
 // rayTorus, mixed precision: the quartic's resolvent root from an FP32 seed
 // refined in FP64 and every quartic root Newton-polished in FP64; the frame
 // change, coefficients and clipping are those of ray_torus.
-__device__ int ray_torus_mixed(const double *td, const double sc[3], double probe, double radius, int vc,
+__device__ __forceinline__ int ray_torus_mixed(const double *td, const double sc[3], double probe, double radius, int vc,
                                const double orig[3], double ray_dir, double t[4]) {
     double o1[3] = {orig[0], orig[1], orig[2]};
     o1[vc] += t[0] * ray_dir;
@@ -606,7 +607,7 @@ __device__ int quartic(double roots[4], double B, double C, double D, double E) 
 }  // namespace sturm
 
 // rayTorus with the Sturm roots; frame change, coefficients and clipping as ray_torus
-__device__ int ray_torus_sturm(const double *td, const double sc[3], double probe, double radius, int vc,
+__device__ __forceinline__ int ray_torus_sturm(const double *td, const double sc[3], double probe, double radius, int vc,
                                const double orig[3], double ray_dir, double t[4]) {
     double o1[3] = {orig[0], orig[1], orig[2]};
     o1[vc] += t[0] * ray_dir;
@@ -680,61 +681,114 @@ __global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRa
     }
     const int64_t pix[3] = {1 + ie[0] - is[0], 1 + ie[1] - is[1], 1 + ie[2] - is[2]};
     const double delta = (phase == 0) ? 0. : kDelta - g.hside;
-    int fd, ld, vc;
-    const double *ga;   // axis of the ray
-    const double *gf, *gl;
-    if (panel == 0) { fd = 1; ld = 2; vc = 0; ga = g.x; gf = g.y; gl = g.z; }
-    else if (panel == 1) { fd = 0; ld = 1; vc = 2; ga = g.z; gf = g.x; gl = g.y; }
-    else { fd = 0; ld = 2; vc = 1; ga = g.y; gf = g.x; gl = g.z; }
-    const int na = gn[vc];
-    const double ray_dir = ga[na - 1] - ga[0];
-    double dir[3] = {0., 0., 0.};
-    dir[vc] = ray_dir;
-    const int64_t total = pix[fd] * pix[ld];
     unsigned int cand = 0;
     unsigned int pc[PC_N] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int64_t rp = threadIdx.x; rp < total; rp += 64) {
-        const int64_t nl = rp / pix[fd];
-        const int64_t m = rp - nl * pix[fd] + is[fd];
-        const int64_t n = nl + is[ld];
-        double pa[3];
-        pa[vc] = ga[0] + delta;
-        pa[fd] = gf[m] + delta;
-        if (pa[fd] < down[fd] || pa[fd] > up[fd]) continue;
-        pa[ld] = gl[n] + delta;
-        if (pa[ld] < down[ld] || pa[ld] > up[ld]) continue;
-        ++cand;
-        // EQ_CULLING (ConnollySurface.cpp:4496-4502)
-        const double dx = sc[fd] - pa[fd], dy = sc[ld] - pa[ld];
-        if (dx * dx + dy * dy > radius * radius) continue;
-        if (kCount) ++pc[PC_DISC];
-        double t[4];
-        if (!ray_sphere(pa, dir, sc, radius, t[0], t[1])) continue;
-        if (kCount) ++pc[PC_SPHERE];
-        int ni = 2;
-        if (kTorus) {
-            if (kRoots == kRootsMixed)
-                ni = ray_torus_mixed(pd, sc, probe, radius, vc, pa, ray_dir, t);
-            else if (kRoots == kRootsSturm)
-                ni = ray_torus_sturm(pd, sc, probe, radius, vc, pa, ray_dir, t);
-            else
-                ni = ray_torus(pd, sc, probe, radius, vc, pa, ray_dir, t);
-            if (kCount) ++pc[PC_TORUS];
-        }
-        for (int i = 0; i < ni; i++) {
-            double ip[3] = {pa[0], pa[1], pa[2]};
-            ip[vc] += t[i] * ray_dir;
-            const bool ok = kTorus ? feasible_torus(kind, pd, pcnt, pflags, ip) : feasible_plain(kind, pd, pcnt, ip);
-            if (kCount) {
-                ++pc[PC_ROOTS];
-                ++pc[PC_FEAS_CALLS];
-                pc[PC_FEAS_ITEMS] += kTorus ? 2 * pcnt + 1 : pcnt;
-                pc[PC_HITS] += ok;
+    // The pixels that reach the sphere test are queued in LDS and taken 64 at
+    // a time, so that the roots and the feasibility tests (most of the work)
+    // run on full waves: the disc cull and the rectangle's last partial row of
+    // lanes would otherwise leave a fifth to a half of the lanes idle there.
+    __shared__ int qm[128], qn[128];
+    // One instance per panel, so that every axis index is a constant and the
+    // ray origin, hit point and frame rows stay in registers.
+    auto walk = [&](auto panel_c) {
+        constexpr int fd = PanelAxes<decltype(panel_c)::value>::fd;
+        constexpr int ld = PanelAxes<decltype(panel_c)::value>::ld;
+        constexpr int vc = PanelAxes<decltype(panel_c)::value>::vc;
+        const double *ga = vc == 0 ? g.x : (vc == 1 ? g.y : g.z);  // axis of the ray
+        const double *gf = fd == 0 ? g.x : g.y;
+        const double *gl = ld == 1 ? g.y : g.z;
+        const int na = gn[vc];
+        const double ray_dir = ga[na - 1] - ga[0];
+        double dir[3] = {0., 0., 0.};
+        dir[vc] = ray_dir;
+        const int64_t total = pix[fd] * pix[ld];
+        // the origin of the ray through pixel (m, n)
+        auto origin = [&](int m, int n, double pa[3]) {
+            pa[vc] = ga[0] + delta;
+            pa[fd] = gf[m] + delta;
+            pa[ld] = gl[n] + delta;
+        };
+        auto roots = [&](int m, int n) {
+            double pa[3], t[4];
+            origin(m, n, pa);
+            ray_sphere(pa, dir, sc, radius, t[0], t[1]);  // true: it passed in the cull
+            int ni = 2;
+            if (kTorus) {
+                if (kRoots == kRootsMixed)
+                    ni = ray_torus_mixed(pd, sc, probe, radius, vc, pa, ray_dir, t);
+                else if (kRoots == kRootsSturm)
+                    ni = ray_torus_sturm(pd, sc, probe, radius, vc, pa, ray_dir, t);
+                else
+                    ni = ray_torus(pd, sc, probe, radius, vc, pa, ray_dir, t);
+                if (kCount) ++pc[PC_TORUS];
             }
-            if (!ok) continue;
-            if (!kCount) push_hit(R, slot, n * R.N + m, t[i], it);
+            for (int i = 0; i < ni; i++) {
+                double ip[3] = {pa[0], pa[1], pa[2]};
+                ip[vc] += t[i] * ray_dir;
+                const bool ok =
+                    kTorus ? feasible_torus(kind, pd, pcnt, pflags, ip) : feasible_plain(kind, pd, pcnt, ip);
+                if (kCount) {
+                    ++pc[PC_ROOTS];
+                    ++pc[PC_FEAS_CALLS];
+                    pc[PC_FEAS_ITEMS] += kTorus ? 2 * pcnt + 1 : pcnt;
+                    pc[PC_HITS] += ok;
+                }
+                if (!ok) continue;
+                if (!kCount) push_hit(R, slot, (int64_t)n * R.N + m, t[i], it);
+            }
+        };
+        // lane walk over the rectangle without a division per pixel: pixel
+        // rp = row * pf + col advances by 64 = step_r rows + step_c columns
+        const int64_t pf = pix[fd];
+        const int step_r = (int)(64 / pf), step_c = (int)(64 % pf);
+        int col = (int)(threadIdx.x % pf), row = (int)(threadIdx.x / pf);
+        int queued = 0;
+        for (int64_t base = 0; base < total; base += 64) {
+            bool act = false;
+            const int m = col + (int)is[fd], n = row + (int)is[ld];
+            if (base + threadIdx.x < total) {
+                double pa[3];
+                origin(m, n, pa);
+                // bounding box, EQ_CULLING (ConnollySurface.cpp:4496-4502), bounding sphere
+                if (!(pa[fd] < down[fd] || pa[fd] > up[fd] || pa[ld] < down[ld] || pa[ld] > up[ld])) {
+                    ++cand;
+                    const double dx = sc[fd] - pa[fd], dy = sc[ld] - pa[ld];
+                    if (!(dx * dx + dy * dy > radius * radius)) {
+                        if (kCount) ++pc[PC_DISC];
+                        double t0, t1;
+                        act = ray_sphere(pa, dir, sc, radius, t0, t1);
+                        if (kCount && act) ++pc[PC_SPHERE];
+                    }
+                }
+            }
+            col += step_c;
+            row += step_r;
+            if (col >= pf) {
+                col -= (int)pf;
+                ++row;
+            }
+            const uint64_t mask = __ballot(act);
+            if (act) {
+                const int at = queued + __popcll(mask & ((1ull << threadIdx.x) - 1ull));
+                qm[at] = m;
+                qn[at] = n;
+            }
+            queued += __popcll(mask);
+            if (queued >= 64) {
+                __syncthreads();
+                queued -= 64;
+                roots(qm[queued + threadIdx.x], qn[queued + threadIdx.x]);
+                __syncthreads();
+            }
         }
-    }
+        if (queued > 0) {
+            __syncthreads();
+            if ((int)threadIdx.x < queued) roots(qm[threadIdx.x], qn[threadIdx.x]);
+        }
+    };
+    if (panel == 0) walk(std::integral_constant<int, 0>());
+    else if (panel == 1) walk(std::integral_constant<int, 1>());
+    else walk(std::integral_constant<int, 2>());
     if (kCount) {
         pc[PC_CAND] = cand;
         for (int q = 0; q < PC_N; ++q) {

@@ -48,14 +48,6 @@ __device__ __forceinline__ bool skin_feasible(const double *pl, int cnt, const d
     return true;
 }
 
-// The axes of a panel: rays along vc, pixels over (fd, ld) (Surface.cpp:652-659)
-template <int PANEL>
-struct PanelAxes {
-    static constexpr int fd = PANEL == 0 ? 1 : 0;
-    static constexpr int ld = PANEL == 1 ? 1 : 2;
-    static constexpr int vc = PANEL == 0 ? 0 : (PANEL == 1 ? 2 : 1);
-};
-
 // A cell's rectangle for one panel slot.
 struct SkinRect {
     int it, kind, cnt;
