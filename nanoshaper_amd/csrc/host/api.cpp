@@ -730,8 +730,7 @@ static int enqueue_surf(nsgpu_ctx *c, hipStream_t st) {
         const VolumeSlots vs = volume_slots(c, &traced);
         const int64_t per = sum_partials_count(NN);
         HIP_TRY(c, c->partial.ensure((size_t)per * c->nslots * 8));
-        for (int k = 0; k < vs.n; ++k)
-            launch_sum_blocks(st, R.rvol + vs.slot[k] * NN, NN, c->partial.as<double>() + vs.slot[k] * per);
+        launch_sum_slots(st, R.rvol, NN, vs, c->partial.as<double>());
         launch_volume(st, c->partial.as<double>(), per, vs, (double *)(c->dres.as<unsigned long long>() + RES_VOL));
         HIP_TRY(c, hipGetLastError());
     }

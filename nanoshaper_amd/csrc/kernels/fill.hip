@@ -88,29 +88,64 @@ __global__ void __launch_bounds__(256) k_fill_bits(DevGrid g, DevRays R, FillOpt
 }
 
 // Corner bitmap (1 = outside) from the edge-ray planes: bits start at 1 and
-// each wave's 64-node row segment clears its inside bits with at most three
+// each 64-node row segment clears its inside bits with at most three
 // atomicAnd on the 32-bit words it overlaps.
+//
+// A block takes a tile of 64 (i) x 16 (j) x 64 (k) nodes.  Node (i, j, k) is
+// bit k & 63 of z-ray word (j, i) and bit j & 63 of y-ray word (k, i), so
+// the tile's 16 x 64 z words and 64 x 64 y words are read once into LDS and
+// each serves every row of the tile that needs it; read per row, they were
+// fetched 64 times each.
+constexpr int kCornJ = 16;
+
 __global__ void __launch_bounds__(256) k_corners_bits(DevGrid g, DevRays R, FillOpts o, uint32_t *bits) {
-    const int lane = threadIdx.x & 63;
-    const int i0 = blockIdx.x * 64, i = i0 + lane;
-    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int k = blockIdx.z;
-    if (j >= g.ny) return;  // whole wave
+    __shared__ unsigned long long zw[kCornJ][64], yw[64][64], xw[64][kCornJ];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i0 = blockIdx.x * 64, j0 = blockIdx.y * kCornJ, k0 = blockIdx.z * 64;
     const int64_t NN = (int64_t)R.N * R.N, SW = (int64_t)R.W * NN;
-    const bool inside = i < g.nx && (cov_bit(R.cov + o.eslot_x * SW, NN, (int64_t)k * R.N + j, i) ||
-                                     cov_bit(R.cov + o.eslot_z * SW, NN, (int64_t)j * R.N + i, k) ||
-                                     cov_bit(R.cov + o.eslot_y * SW, NN, (int64_t)k * R.N + i, j));
-    const unsigned long long m = __ballot(inside);
-    if (lane == 0 && m) {
-        const int64_t f0 = ((int64_t)k * g.ny + j) * g.nx + i0;
-        const int64_t w0 = f0 >> 5;
-        const unsigned int s = (unsigned int)(f0 & 31);
-        const uint32_t a = (uint32_t)(m << s);
-        const uint32_t b = (uint32_t)((s ? (m >> (32 - s)) : (m >> 32)) & 0xffffffffull);
-        const uint32_t c = s ? (uint32_t)(m >> (64 - s)) : 0u;
-        if (a) atomicAnd(bits + w0, ~a);
-        if (b) atomicAnd(bits + w0 + 1, ~b);
-        if (c) atomicAnd(bits + w0 + 2, ~c);
+    const unsigned long long *px = R.cov + o.eslot_x * SW + (int64_t)(i0 >> 6) * NN;  // ray k N + j, bit i & 63
+    const unsigned long long *pz = R.cov + o.eslot_z * SW + (int64_t)(k0 >> 6) * NN;  // ray j N + i, bit k & 63
+    const unsigned long long *py = R.cov + o.eslot_y * SW + (int64_t)(j0 >> 6) * NN;  // ray k N + i, bit j & 63
+    const int i = i0 + lane;
+    for (int jj = wave; jj < kCornJ; jj += 4) {
+        const int j = j0 + jj;
+        zw[jj][lane] = (i < g.nx && j < g.ny) ? pz[(int64_t)j * R.N + i] : 0ull;
+    }
+    for (int kk = wave; kk < 64; kk += 4) {
+        const int k = k0 + kk;
+        yw[kk][lane] = (i < g.nx && k < g.nz) ? py[(int64_t)k * R.N + i] : 0ull;
+    }
+    for (int q = threadIdx.x; q < 64 * kCornJ; q += 256) {
+        const int kk = q / kCornJ, jj = q % kCornJ, k = k0 + kk, j = j0 + jj;
+        xw[kk][jj] = (k < g.nz && j < g.ny) ? px[(int64_t)k * R.N + j] : 0ull;
+    }
+    __syncthreads();
+    const int jbit = j0 & 63;
+    // eight rows at a time: eight independent ballots, then lane q clears the
+    // words of row q, so the rows' LDS reads and atomics overlap
+    for (int r0 = wave * 8; r0 < 64 * kCornJ; r0 += 32) {
+        unsigned long long mine = 0ull;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int r = r0 + q, kk = r / kCornJ, jj = r % kCornJ;
+            const bool inside = i < g.nx && k0 + kk < g.nz && j0 + jj < g.ny &&
+                                (((xw[kk][jj] >> lane) | (zw[jj][lane] >> kk) | (yw[kk][lane] >> (jbit + jj))) & 1ull);
+            const unsigned long long m = __ballot(inside);
+            if (lane == q) mine = m;
+        }
+        if (lane < 8 && mine) {
+            const int r = r0 + lane, k = k0 + r / kCornJ, j = j0 + r % kCornJ;
+            const unsigned long long m = mine;
+            const int64_t f0 = ((int64_t)k * g.ny + j) * g.nx + i0;
+            const int64_t w0 = f0 >> 5;
+            const unsigned int s = (unsigned int)(f0 & 31);
+            const uint32_t a = (uint32_t)(m << s);
+            const uint32_t b = (uint32_t)((s ? (m >> (32 - s)) : (m >> 32)) & 0xffffffffull);
+            const uint32_t c = s ? (uint32_t)(m >> (64 - s)) : 0u;
+            if (a) atomicAnd(bits + w0, ~a);
+            if (b) atomicAnd(bits + w0 + 1, ~b);
+            if (c) atomicAnd(bits + w0 + 2, ~c);
+        }
     }
 }
 
@@ -122,15 +157,16 @@ void launch_fill_bits(hipStream_t s, const DevGrid &g, const DevRays &R, const F
 }
 
 void launch_corners_bits(hipStream_t s, const DevGrid &g, const DevRays &R, const FillOpts &o, uint32_t *bits) {
-    dim3 grid((unsigned)((g.nx + 63) / 64), (unsigned)((g.ny + 3) / 4), (unsigned)g.nz);
+    dim3 grid((unsigned)((g.nx + 63) / 64), (unsigned)((g.ny + kCornJ - 1) / kCornJ), (unsigned)((g.nz + 63) / 64));
     hipLaunchKernelGGL(k_corners_bits, grid, dim3(256), 0, s, g, R, o, bits);
 }
 
-// Deterministic two-level sum of the per-ray segment lengths of each panel.
-__global__ void __launch_bounds__(256) k_sum_blocks(const double *v, int64_t n, double *partial) {
+// Deterministic two-level sum: block b adds elements [4096 b, 4096 b + 4096)
+// of v in a fixed order and writes one partial.
+__device__ __forceinline__ void sum_block(const double *v, int64_t n, double *partial, int64_t b) {
     __shared__ double sh[256];
     double acc = 0;
-    const int64_t base = (int64_t)blockIdx.x * 4096;
+    const int64_t base = b * 4096;
     for (int q = 0; q < 16; ++q) {
         const int64_t idx = base + q * 256 + threadIdx.x;
         if (idx < n) acc += v[idx];
@@ -141,7 +177,19 @@ __global__ void __launch_bounds__(256) k_sum_blocks(const double *v, int64_t n, 
         if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+    if (threadIdx.x == 0) partial[b] = sh[0];
+}
+
+__global__ void __launch_bounds__(256) k_sum_blocks(const double *v, int64_t n, double *partial) {
+    sum_block(v, n, partial, blockIdx.x);
+}
+
+// the per-ray segment lengths of every traced slot in one launch:
+// grid (blocks per slot, slots)
+__global__ void __launch_bounds__(256) k_sum_slots(const double *v, int64_t n, VolumeSlots vs, double *partial,
+                                                   int64_t per) {
+    const int64_t slot = vs.slot[blockIdx.y];
+    sum_block(v + slot * n, n, partial + slot * per, blockIdx.x);
 }
 
 int64_t sum_partials_count(int64_t n) { return (n + 4095) / 4096; }
@@ -149,6 +197,12 @@ int64_t sum_partials_count(int64_t n) { return (n + 4095) / 4096; }
 void launch_sum_blocks(hipStream_t s, const double *v, int64_t n, double *partial) {
     const int64_t nb = sum_partials_count(n);
     if (nb > 0) hipLaunchKernelGGL(k_sum_blocks, dim3((unsigned)nb), dim3(256), 0, s, v, n, partial);
+}
+
+void launch_sum_slots(hipStream_t s, const double *v, int64_t n, const VolumeSlots &vs, double *partial) {
+    const int64_t nb = sum_partials_count(n);
+    if (nb > 0 && vs.n > 0)
+        hipLaunchKernelGGL(k_sum_slots, dim3((unsigned)nb, (unsigned)vs.n), dim3(256), 0, s, v, n, vs, partial, nb);
 }
 
 }  // namespace nsg

@@ -62,6 +62,8 @@ struct VolumeSlots {
     double area_px;
 };
 void launch_volume(hipStream_t, const double *partial, int64_t per, const VolumeSlots &, double *out);
+// k_sum_blocks over each slot vs.slot[k] of v (slot s at v + s * n, partials at partial + s * per)
+void launch_sum_slots(hipStream_t, const double *v, int64_t n, const VolumeSlots &vs, double *partial);
 // writes the device wall clock
 void launch_stamp(hipStream_t, unsigned long long *out);
 // on-device replay of the cavity numbering (up to kReplayMax components)
