@@ -91,6 +91,10 @@ struct nsgpu_ctx {
     // result block (device) and its pinned host mirror
     DBuf dres;
     HostResults *hres = nullptr;
+    // pinned staging for the small reads between stages (totals, partial
+    // sums): copies into pageable memory are staged and serialised
+    void *hpin = nullptr;
+    size_t hpin_cap = 0;
     bool capturing = false;  // enqueueing into a graph: no stage events
     std::vector<double> cav_size;
     std::vector<uint8_t> cav_filled;
@@ -231,6 +235,7 @@ void nsgpu_destroy(nsgpu_ctx *c) {
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->hres) (void)hipHostFree(c->hres);
+    if (c->hpin) (void)hipHostFree(c->hpin);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -769,16 +774,59 @@ static int finish_surf(nsgpu_ctx *c) {
     return NSGPU_OK;
 }
 
-static int read_total(nsgpu_ctx *c, const int *counts, const int *offs, int64_t n, int64_t *total) {
+// The pinned staging buffer with room for `bytes`.  Called only when no copy
+// into it is pending (every use ends with a stream synchronisation).
+static int pinned(nsgpu_ctx *c, size_t bytes, void **out) {
+    if (bytes > c->hpin_cap) {
+        if (c->hpin) (void)hipHostFree(c->hpin);
+        c->hpin = nullptr;
+        c->hpin_cap = 0;
+        const size_t cap = std::max<size_t>(bytes, 4096);
+        HIP_TRY(c, hipHostMalloc(&c->hpin, cap, hipHostMallocDefault));
+        c->hpin_cap = cap;
+    }
+    *out = c->hpin;
+    return NSGPU_OK;
+}
+
+// count[n-1] + offs[n-1] of an exclusive scan, for each (counts, offs) pair:
+// the copies go to pinned memory, so they queue behind one synchronisation
+static int read_totals(nsgpu_ctx *c, int npairs, const int *const *counts, const int *const *offs, int64_t n,
+                       int64_t *total) {
     if (n <= 0) {
-        *total = 0;
+        for (int q = 0; q < npairs; ++q) total[q] = 0;
         return NSGPU_OK;
     }
-    int a = 0, b = 0;
-    HIP_TRY(c, hipMemcpyAsync(&a, counts + n - 1, 4, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(&b, offs + n - 1, 4, hipMemcpyDeviceToHost, c->stream));
+    void *hp = nullptr;
+    int rc = pinned(c, (size_t)npairs * 8, &hp);
+    if (rc) return rc;
+    int *h = static_cast<int *>(hp);
+    for (int q = 0; q < npairs; ++q) {
+        HIP_TRY(c, hipMemcpyAsync(h + 2 * q, counts[q] + n - 1, 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(h + 2 * q + 1, offs[q] + n - 1, 4, hipMemcpyDeviceToHost, c->stream));
+    }
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    *total = (int64_t)a + b;
+    for (int q = 0; q < npairs; ++q) total[q] = (int64_t)h[2 * q] + h[2 * q + 1];
+    return NSGPU_OK;
+}
+
+static int read_total(nsgpu_ctx *c, const int *counts, const int *offs, int64_t n, int64_t *total) {
+    return read_totals(c, 1, &counts, &offs, n, total);
+}
+
+// sum of per partial sums, in order, after one synchronisation
+static int read_partial_sum(nsgpu_ctx *c, const double *partial, int64_t per, hipStream_t st, double *sum) {
+    *sum = 0;
+    if (per <= 0) return NSGPU_OK;
+    void *hp = nullptr;
+    int rc = pinned(c, (size_t)per * 8, &hp);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpyAsync(hp, partial, (size_t)per * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(c, hipStreamSynchronize(st));
+    const double *h = static_cast<const double *>(hp);
+    double acc = 0;
+    for (int64_t q = 0; q < per; ++q) acc += h[q];
+    *sum = acc;
     return NSGPU_OK;
 }
 
@@ -893,17 +941,14 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
     launch_scan(c->stream, c->anv.as<int>(), nact, c->avoff.as<int>(), c->scan_scratch.as<int>());
     launch_scan(c->stream, c->ant.as<int>(), nact, c->atoff.as<int>(), c->scan_scratch.as<int>());
     HIP_TRY(c, hipGetLastError());
-    int64_t nadd = 0, nt = 0;
-    if (nact) {
-        int tail[4];
-        HIP_TRY(c, hipMemcpyAsync(tail + 0, c->anv.as<int>() + nact - 1, 4, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(c, hipMemcpyAsync(tail + 1, c->avoff.as<int>() + nact - 1, 4, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(c, hipMemcpyAsync(tail + 2, c->ant.as<int>() + nact - 1, 4, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(c, hipMemcpyAsync(tail + 3, c->atoff.as<int>() + nact - 1, 4, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(c, hipStreamSynchronize(c->stream));
-        nadd = (int64_t)tail[0] + tail[1];
-        nt = (int64_t)tail[2] + tail[3];
+    int64_t tot[2] = {0, 0};  // mid-point vertices, triangles
+    {
+        const int *cnts[2] = {c->anv.as<int>(), c->ant.as<int>()};
+        const int *offs[2] = {c->avoff.as<int>(), c->atoff.as<int>()};
+        rc = read_totals(c, 2, cnts, offs, nact, tot);
+        if (rc) return rc;
     }
+    const int64_t nadd = tot[0], nt = tot[1];
     const int64_t nv = c->n_analytic + nadd;
     if (nv >= ((int64_t)1 << 31)) return fail(c, NSGPU_E_NOMEM, "too many vertices");
     if ((size_t)nv * 24 > c->verts.cap) {
@@ -943,11 +988,9 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
     launch_sum_blocks(c->stream, c->tarea.as<double>(), nt, c->partial.as<double>());
     HIP_TRY(c, hipGetLastError());
     mark(c, ST_MC, 1, c->stream);
-    std::vector<double> part((size_t)per, 0.0);
-    if (per) HIP_TRY(c, hipMemcpyAsync(part.data(), c->partial.p, (size_t)per * 8, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
     double area = 0;
-    for (double v : part) area += v;
+    rc = read_partial_sum(c, c->partial.as<double>(), per, c->stream, &area);
+    if (rc) return rc;
     c->ntris = nt;
     c->area = area;
     c->have_mesh = true;
@@ -1157,11 +1200,14 @@ int nsgpu_blobby_build(nsgpu_ctx *c, const double *xyzr, int32_t n, double B, do
     launch_scan(st, c->anv.as<int>(), nact, c->avoff.as<int>(), c->scan_scratch.as<int>());
     launch_scan(st, c->ant.as<int>(), nact, c->atoff.as<int>(), c->scan_scratch.as<int>());
     HIP_TRY(c, hipGetLastError());
-    int64_t nv = 0, nt = 0;
-    rc = read_total(c, c->anv.as<int>(), c->avoff.as<int>(), nact, &nv);
-    if (rc) return rc;
-    rc = read_total(c, c->ant.as<int>(), c->atoff.as<int>(), nact, &nt);
-    if (rc) return rc;
+    int64_t tot[2] = {0, 0};  // vertices, triangles
+    {
+        const int *cnts[2] = {c->anv.as<int>(), c->ant.as<int>()};
+        const int *offs[2] = {c->avoff.as<int>(), c->atoff.as<int>()};
+        rc = read_totals(c, 2, cnts, offs, nact, tot);
+        if (rc) return rc;
+    }
+    const int64_t nv = tot[0], nt = tot[1];
     if (nv >= INT32_MAX / 3 || nt >= INT32_MAX / 3) return fail(c, NSGPU_E_NOMEM, "blobby mesh too large");
     HIP_TRY(c, c->verts.ensure((size_t)std::max<int64_t>(nv, 1) * 24));
     HIP_TRY(c, c->vslot.ensure((size_t)std::max<int64_t>(nv, 1) * 4));
@@ -1196,11 +1242,9 @@ int nsgpu_blobby_build(nsgpu_ctx *c, const double *xyzr, int32_t n, double B, do
     launch_quantize(st, c->mraw.as<double>(), 3 * nv, c->mverts.as<double>());
     if (nt) HIP_TRY(c, hipMemcpyAsync(c->mtris.p, c->tris.p, (size_t)nt * 12, hipMemcpyDeviceToDevice, st));
     HIP_TRY(c, hipGetLastError());
-    std::vector<double> part((size_t)per, 0.0);
-    if (per) HIP_TRY(c, hipMemcpyAsync(part.data(), c->partial.p, (size_t)per * 8, hipMemcpyDeviceToHost, st));
-    HIP_TRY(c, hipStreamSynchronize(st));
     double area = 0;
-    for (double v : part) area += v;
+    rc = read_partial_sum(c, c->partial.as<double>(), per, st, &area);
+    if (rc) return rc;
     rc = setup_aux_grid(c, aux_grid_dim, 1);
     if (rc) return rc;
     install_mesh(c, nv, nt, 1);
