@@ -105,6 +105,11 @@ struct nsgpu_ctx {
     // last, for grids of idx_nvox nodes (-1: state unknown, fill it)
     DBuf vslot;
     int64_t idx_nvox = -1, idx_dirty = 0;
+    // the analytic vertex count of the last assemble is still on the device
+    // (dnv[0]); n_analytic, nverts and idx_dirty wait for resolve_nv
+    DBuf dnv;
+    bool nv_pending = false;
+    int64_t nv_cap = 0;
     // vertex provenance for the normals: depth of each analytic vertex, cube
     // and edge of each mid point
     DBuf vhit, vorig, normals;
@@ -226,7 +231,7 @@ void nsgpu_destroy(nsgpu_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     DBuf *all[] = {&c->gx, &c->gy, &c->gz, &c->pkind, &c->poff, &c->pcnt, &c->pflags, &c->pbsph, &c->pdata, &c->porder, &c->patom, &c->atoms, &c->rpid, &c->gradv, &c->grada, &c->asums,
                    &c->rcnt, &c->rt, &c->rvol, &c->partial, &c->stats, &c->cand, &c->cov, &c->covst, &c->eps, &c->ideb, &c->status,
-                   &c->bits, &c->crx0, &c->crx1, &c->cpar, &c->crowcnt, &c->croff, &c->nroots, &c->rootlist, &c->bkey, &c->ccount, &c->clabel, &c->cfilled, &c->dcavsize, &c->dcavfilled, &c->hostdone, &c->dres,
+                   &c->bits, &c->crx0, &c->crx1, &c->cpar, &c->crowcnt, &c->croff, &c->nroots, &c->rootlist, &c->bkey, &c->ccount, &c->clabel, &c->cfilled, &c->dcavsize, &c->dcavfilled, &c->hostdone, &c->dres, &c->dnv,
                    &c->idx, &c->vslot, &c->vhit, &c->vorig, &c->normals, &c->vcount, &c->voffs, &c->scan_scratch, &c->verts, &c->tris, &c->tarea,
                    &c->segmask, &c->anode, &c->acase, &c->anv, &c->ant, &c->avoff, &c->atoff, &c->mverts, &c->mtris,
                    &c->mrr, &c->mraw, &c->field, &c->fbits, &c->batoms, &c->bbox, &c->bcount, &c->boff, &c->bcur,
@@ -791,27 +796,44 @@ static int pinned(nsgpu_ctx *c, size_t bytes, void **out) {
 
 // count[n-1] + offs[n-1] of an exclusive scan, for each (counts, offs) pair:
 // the copies go to pinned memory, so they queue behind one synchronisation
-static int read_totals(nsgpu_ctx *c, int npairs, const int *const *counts, const int *const *offs, int64_t n,
+static int read_totals(nsgpu_ctx *c, int npairs, const int *const *counts, const int *const *offs, const int64_t *n,
                        int64_t *total) {
-    if (n <= 0) {
-        for (int q = 0; q < npairs; ++q) total[q] = 0;
-        return NSGPU_OK;
-    }
     void *hp = nullptr;
     int rc = pinned(c, (size_t)npairs * 8, &hp);
     if (rc) return rc;
     int *h = static_cast<int *>(hp);
+    bool any = false;
     for (int q = 0; q < npairs; ++q) {
-        HIP_TRY(c, hipMemcpyAsync(h + 2 * q, counts[q] + n - 1, 4, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(c, hipMemcpyAsync(h + 2 * q + 1, offs[q] + n - 1, 4, hipMemcpyDeviceToHost, c->stream));
+        h[2 * q] = h[2 * q + 1] = 0;
+        if (n[q] <= 0) continue;
+        HIP_TRY(c, hipMemcpyAsync(h + 2 * q, counts[q] + n[q] - 1, 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(h + 2 * q + 1, offs[q] + n[q] - 1, 4, hipMemcpyDeviceToHost, c->stream));
+        any = true;
     }
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (any) HIP_TRY(c, hipStreamSynchronize(c->stream));
     for (int q = 0; q < npairs; ++q) total[q] = (int64_t)h[2 * q] + h[2 * q + 1];
     return NSGPU_OK;
 }
 
 static int read_total(nsgpu_ctx *c, const int *counts, const int *offs, int64_t n, int64_t *total) {
-    return read_totals(c, 1, &counts, &offs, n, total);
+    return read_totals(c, 1, &counts, &offs, &n, total);
+}
+
+// the analytic vertex count once it is known on the host
+static int set_nv(nsgpu_ctx *c, int64_t nv) {
+    c->nv_pending = false;
+    if (nv > c->nv_cap) return fail(c, NSGPU_E_STATE, "vertex count above its bound");
+    c->idx_dirty = nv;
+    c->n_analytic = nv;
+    c->nverts = nv;
+    return NSGPU_OK;
+}
+
+static int resolve_nv(nsgpu_ctx *c) {
+    if (!c->nv_pending) return NSGPU_OK;
+    int64_t nv = 0;
+    const int rc = read_total(c, c->dnv.as<int>(), c->dnv.as<int>() + 1, 1, &nv);
+    return rc ? rc : set_nv(c, nv);
 }
 
 // sum of per partial sums, in order, after one synchronisation
@@ -836,6 +858,8 @@ static int assemble(nsgpu_ctx *c, const int eslots[3]) {
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
     const int64_t nrays = 3 * (int64_t)R.N * R.N;
     const int64_t nmax = nrays;
+    int rc = resolve_nv(c);  // idx_dirty of the previous vertices
+    if (rc) return rc;
     const void *old_idx = c->idx.p;
     HIP_TRY(c, c->idx.ensure((size_t)nvox * 3 * 4));
     HIP_TRY(c, c->vcount.ensure((size_t)nmax * 4));
@@ -851,19 +875,22 @@ static int assemble(nsgpu_ctx *c, const int eslots[3]) {
     c->idx_dirty = 0;
     launch_vtx_count(c->stream, g, R, eslots, c->bits.as<uint32_t>(), c->vcount.as<int>());
     launch_scan(c->stream, c->vcount.as<int>(), nrays, c->voffs.as<int>(), c->scan_scratch.as<int>());
+    HIP_TRY(c, c->dnv.ensure(8));
+    launch_scan_total(c->stream, c->vcount.as<int>(), c->voffs.as<int>(), nrays, c->dnv.as<int>());
     HIP_TRY(c, hipGetLastError());
-    int64_t nv = 0;
-    int rc = read_total(c, c->vcount.as<int>(), c->voffs.as<int>(), nrays, &nv);
-    if (rc) return rc;
-    HIP_TRY(c, c->verts.ensure((size_t)std::max<int64_t>(nv, 1) * 24));
-    HIP_TRY(c, c->vslot.ensure((size_t)std::max<int64_t>(nv, 1) * 4));
-    HIP_TRY(c, c->vhit.ensure((size_t)std::max<int64_t>(nv, 1) * 8));
+    // Every vertex comes from a depth of an edge ray (at most one per depth),
+    // so the getSurf hit count bounds the total: the buffers are sized by it
+    // and the count itself is read later, with the next read the host makes.
+    const int64_t cap = std::max<int64_t>(c->res.total_hits, 1);
+    HIP_TRY(c, c->verts.ensure((size_t)cap * 24));
+    HIP_TRY(c, c->vslot.ensure((size_t)cap * 4));
+    HIP_TRY(c, c->vhit.ensure((size_t)cap * 8));
     launch_vtx_write(c->stream, g, R, eslots, c->bits.as<uint32_t>(), c->voffs.as<int>(), c->verts.as<double>(),
-                     c->idx.as<int>(), c->vslot.as<int>(), c->vhit.as<int64_t>());
+                     c->idx.as<int>(), c->vslot.as<int>(), c->vhit.as<int64_t>(), cap);
     HIP_TRY(c, hipGetLastError());
-    c->idx_dirty = nv;
-    c->n_analytic = nv;
-    c->nverts = nv;
+    c->idx_dirty = 0;
+    c->nv_cap = cap;
+    c->nv_pending = true;
     return NSGPU_OK;
 }
 
@@ -900,7 +927,6 @@ int nsgpu_get_surf(nsgpu_ctx *c, const nsgpu_surf_opts *o, nsgpu_surf_result *re
         c->have_vertices = true;
     }
     mark(c, ST_VERTS, 1, c->stream);
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->have_surf = true;
     if (res) *res = c->res;
     return NSGPU_OK;
@@ -913,7 +939,6 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
     const DevGrid g = dev_grid(c);
     const int64_t nseg = mc_num_segments(g);
     mark(c, ST_MC, 0, c->stream);
-    c->nverts = c->n_analytic;
     // 1. classify 32-cube row segments, compact the active cubes in cube order
     HIP_TRY(c, c->segmask.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
     HIP_TRY(c, c->vcount.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
@@ -922,9 +947,22 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
     launch_mc_classify(c->stream, g, c->bits.as<uint32_t>(), c->segmask.as<uint32_t>(), c->vcount.as<int>());
     launch_scan(c->stream, c->vcount.as<int>(), nseg, c->voffs.as<int>(), c->scan_scratch.as<int>());
     HIP_TRY(c, hipGetLastError());
-    int64_t nact = 0;
-    int rc = read_total(c, c->vcount.as<int>(), c->voffs.as<int>(), nseg, &nact);
-    if (rc) return rc;
+    // the active-cube count, with the analytic vertex count if still pending
+    int64_t tot[2] = {0, 0};
+    int rc = NSGPU_OK;
+    {
+        const int *cnts[2] = {c->vcount.as<int>(), c->dnv.as<int>()};
+        const int *offs[2] = {c->voffs.as<int>(), c->dnv.as<int>() + 1};
+        const int64_t ns[2] = {nseg, c->nv_pending ? 1 : 0};
+        rc = read_totals(c, 2, cnts, offs, ns, tot);
+        if (rc) return rc;
+        if (c->nv_pending) {
+            rc = set_nv(c, tot[1]);
+            if (rc) return rc;
+        }
+    }
+    const int64_t nact = tot[0];
+    c->nverts = c->n_analytic;
     const size_t na = (size_t)std::max<int64_t>(nact, 1);
     HIP_TRY(c, c->anode.ensure(na * 4));
     HIP_TRY(c, c->acase.ensure(na));
@@ -941,11 +979,11 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
     launch_scan(c->stream, c->anv.as<int>(), nact, c->avoff.as<int>(), c->scan_scratch.as<int>());
     launch_scan(c->stream, c->ant.as<int>(), nact, c->atoff.as<int>(), c->scan_scratch.as<int>());
     HIP_TRY(c, hipGetLastError());
-    int64_t tot[2] = {0, 0};  // mid-point vertices, triangles
     {
         const int *cnts[2] = {c->anv.as<int>(), c->ant.as<int>()};
         const int *offs[2] = {c->avoff.as<int>(), c->atoff.as<int>()};
-        rc = read_totals(c, 2, cnts, offs, nact, tot);
+        const int64_t ns[2] = {nact, nact};
+        rc = read_totals(c, 2, cnts, offs, ns, tot);  // mid-point vertices, triangles
         if (rc) return rc;
     }
     const int64_t nadd = tot[0], nt = tot[1];
@@ -1165,6 +1203,8 @@ int nsgpu_blobby_build(nsgpu_ctx *c, const double *xyzr, int32_t n, double B, do
     // 2. marching cubes on the field at iso 1.0 (triangulateSurface(false, false, 1.0), BlobbySurface.cpp:186)
     HIP_TRY(c, c->fbits.ensure((size_t)((nvox + 31) / 32) * 4));
     launch_field_bits(st, a.field, nvox, 1.0, c->fbits.as<uint32_t>());
+    rc = resolve_nv(c);  // idx_dirty of the previous vertices
+    if (rc) return rc;
     const void *old_idx = c->idx.p;
     HIP_TRY(c, c->idx.ensure((size_t)nvox * 3 * 4));
     if (c->idx.p != old_idx || c->idx_nvox != nvox || 3 * nvox > INT32_MAX)
@@ -1204,7 +1244,8 @@ int nsgpu_blobby_build(nsgpu_ctx *c, const double *xyzr, int32_t n, double B, do
     {
         const int *cnts[2] = {c->anv.as<int>(), c->ant.as<int>()};
         const int *offs[2] = {c->avoff.as<int>(), c->atoff.as<int>()};
-        rc = read_totals(c, 2, cnts, offs, nact, tot);
+        const int64_t ns[2] = {nact, nact};
+        rc = read_totals(c, 2, cnts, offs, ns, tot);
         if (rc) return rc;
     }
     const int64_t nv = tot[0], nt = tot[1];

@@ -83,12 +83,14 @@ void launch_ccl_clear_keys(hipStream_t, const int *nroots, unsigned int *bkey, i
 void launch_ccl_replay(hipStream_t, const CavityReplay &);
 int64_t scan_scratch_ints(int64_t n);
 void launch_scan(hipStream_t, const int *, int64_t, int *, int *);
+void launch_scan_total(hipStream_t, const int *counts, const int *offs, int64_t n, int *out);
 void mc_upload_tables(hipStream_t);
 void launch_vtx_count(hipStream_t, const DevGrid &, const DevRays &, const int[3], const uint32_t *, int *);
 // vslot[v] receives the index slot of vertex v (for launch_idx_reset), vhit[v]
 // the index of its depth in R.t / R.pid
+// vertices v >= vcap are dropped (vcap bounds the total; the host checks it)
 void launch_vtx_write(hipStream_t, const DevGrid &, const DevRays &, const int[3], const uint32_t *, const int *,
-                      double *, int *, int *vslot, int64_t *vhit);
+                      double *, int *, int *vslot, int64_t *vhit, int64_t vcap);
 void launch_idx_reset(hipStream_t, int *idx, const int *vslot, int64_t n);
 int64_t mc_num_segments(const DevGrid &);
 void launch_mc_classify(hipStream_t, const DevGrid &, const uint32_t *bits, uint32_t *segmask, int *segcnt);

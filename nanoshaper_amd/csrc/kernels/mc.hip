@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(256) k_vtx_count(DevGrid g, DevRays R, EdgeSlo
 
 __global__ void __launch_bounds__(256) k_vtx_write(DevGrid g, DevRays R, EdgeSlots es, const uint32_t *bits,
                                                    const int *offs, double *verts, int *idx, int *vslot,
-                                                   int64_t *vhit) {
+                                                   int64_t *vhit, int64_t vcap) {
     const int64_t NN = (int64_t)R.N * R.N;
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= 3 * NN) return;
@@ -119,6 +119,7 @@ __global__ void __launch_bounds__(256) k_vtx_write(DevGrid g, DevRays R, EdgeSlo
     walk_edge_ray(g, R, bits, panel, es.s[panel], r - panel * NN,
                   [&](int q, int d, int64_t f, double px, double py, double pz, int hit) {
                       const int v = base + q;
+                      if (v >= vcap) return;
                       verts[3 * (int64_t)v + 0] = px;
                       verts[3 * (int64_t)v + 1] = py;
                       verts[3 * (int64_t)v + 2] = pz;
@@ -429,10 +430,11 @@ void launch_vtx_count(hipStream_t s, const DevGrid &g, const DevRays &R, const i
 }
 
 void launch_vtx_write(hipStream_t s, const DevGrid &g, const DevRays &R, const int eslots[3], const uint32_t *bits,
-                      const int *offs, double *verts, int *idx, int *vslot, int64_t *vhit) {
+                      const int *offs, double *verts, int *idx, int *vslot, int64_t *vhit, int64_t vcap) {
     EdgeSlots es{{eslots[0], eslots[1], eslots[2]}};
     const int64_t n = 3 * (int64_t)R.N * R.N;
-    hipLaunchKernelGGL(k_vtx_write, dim3(nblk(n)), dim3(256), 0, s, g, R, es, bits, offs, verts, idx, vslot, vhit);
+    hipLaunchKernelGGL(k_vtx_write, dim3(nblk(n)), dim3(256), 0, s, g, R, es, bits, offs, verts, idx, vslot, vhit,
+                       vcap);
 }
 
 // back to "no vertex" for the slots the last vertices were written to

@@ -75,4 +75,16 @@ void launch_scan(hipStream_t st, const int *in, int64_t n, int *out, int *scratc
     hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(256), 0, st, in, n, boff, out);
 }
 
+// out[0] = counts[n-1] + offs[n-1] (the total of an exclusive scan), out[1] = 0:
+// the total stays on the device for a later batched read
+__global__ void k_scan_total(const int *counts, const int *offs, int64_t n, int *out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    out[0] = n > 0 ? counts[n - 1] + offs[n - 1] : 0;
+    out[1] = 0;
+}
+
+void launch_scan_total(hipStream_t st, const int *counts, const int *offs, int64_t n, int *out) {
+    hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, st, counts, offs, n, out);
+}
+
 }  // namespace nsg
