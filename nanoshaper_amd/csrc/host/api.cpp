@@ -979,29 +979,25 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
     launch_scan(c->stream, c->anv.as<int>(), nact, c->avoff.as<int>(), c->scan_scratch.as<int>());
     launch_scan(c->stream, c->ant.as<int>(), nact, c->atoff.as<int>(), c->scan_scratch.as<int>());
     HIP_TRY(c, hipGetLastError());
-    {
-        const int *cnts[2] = {c->anv.as<int>(), c->ant.as<int>()};
-        const int *offs[2] = {c->avoff.as<int>(), c->atoff.as<int>()};
-        const int64_t ns[2] = {nact, nact};
-        rc = read_totals(c, 2, cnts, offs, ns, tot);  // mid-point vertices, triangles
-        if (rc) return rc;
-    }
-    const int64_t nadd = tot[0], nt = tot[1];
-    const int64_t nv = c->n_analytic + nadd;
-    if (nv >= ((int64_t)1 << 31)) return fail(c, NSGPU_E_NOMEM, "too many vertices");
-    if ((size_t)nv * 24 > c->verts.cap) {
+    // A cube adds at most three mid points (edges 5, 6, 10) and emits at most
+    // five triangles: the buffers are sized by those bounds, so the totals
+    // are read once, at the end, with the area's partial sums.
+    const int64_t nadd_cap = 3 * nact, nt_cap = 5 * nact;
+    const int64_t nv_cap = c->n_analytic + nadd_cap;
+    if (nv_cap >= ((int64_t)1 << 31) || nt_cap >= ((int64_t)1 << 31)) return fail(c, NSGPU_E_NOMEM, "too many vertices");
+    if ((size_t)nv_cap * 24 > c->verts.cap) {
         // grow while keeping the analytic vertices
         DBuf nb;
-        HIP_TRY(c, nb.ensure((size_t)nv * 24));
+        HIP_TRY(c, nb.ensure((size_t)nv_cap * 24));
         if (c->n_analytic)
             HIP_TRY(c, hipMemcpyAsync(nb.p, c->verts.p, (size_t)c->n_analytic * 24, hipMemcpyDeviceToDevice, c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         c->verts.release();
         c->verts = nb;
     }
-    if ((size_t)nv * 4 > c->vslot.cap) {
+    if ((size_t)nv_cap * 4 > c->vslot.cap) {
         DBuf nb;
-        HIP_TRY(c, nb.ensure((size_t)nv * 4));
+        HIP_TRY(c, nb.ensure((size_t)nv_cap * 4));
         if (c->n_analytic)
             HIP_TRY(c, hipMemcpyAsync(nb.p, c->vslot.p, (size_t)c->n_analytic * 4, hipMemcpyDeviceToDevice, c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -1009,26 +1005,45 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
         c->vslot = nb;
     }
     // 3. write the mid points, then the triangles that may reference them
-    HIP_TRY(c, c->vorig.ensure((size_t)std::max<int64_t>(nadd, 1) * 4));
+    HIP_TRY(c, c->vorig.ensure((size_t)std::max<int64_t>(nadd_cap, 1) * 4));
     launch_mc_vwrite(c->stream, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->idx.as<int>(),
                      c->avoff.as<int>(), (int)c->n_analytic, c->verts.as<double>(), c->vslot.as<int>(),
                      c->vorig.as<int>());
     c->nact = nact;
-    c->idx_dirty = nv;
-    HIP_TRY(c, c->tris.ensure((size_t)std::max<int64_t>(nt, 1) * 12));
-    HIP_TRY(c, c->tarea.ensure((size_t)std::max<int64_t>(nt, 1) * 8));
+    HIP_TRY(c, c->tris.ensure((size_t)std::max<int64_t>(nt_cap, 1) * 12));
+    HIP_TRY(c, c->tarea.ensure((size_t)std::max<int64_t>(nt_cap, 1) * 8));
     launch_mc_twrite(c->stream, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->idx.as<int>(),
                      c->atoff.as<int>(), c->verts.as<double>(), c->tris.as<int>(), c->tarea.as<double>());
     HIP_TRY(c, hipGetLastError());
-    c->nverts = nv;
-    const int64_t per = sum_partials_count(nt);
+    const int64_t per = sum_partials_count(nt_cap);
     HIP_TRY(c, c->partial.ensure((size_t)std::max<int64_t>(per, 1) * 8));
-    launch_sum_blocks(c->stream, c->tarea.as<double>(), nt, c->partial.as<double>());
+    if (nact)
+        launch_sum_blocks_scan(c->stream, c->tarea.as<double>(), nt_cap, c->ant.as<int>() + nact - 1,
+                               c->atoff.as<int>() + nact - 1, c->partial.as<double>());
     HIP_TRY(c, hipGetLastError());
     mark(c, ST_MC, 1, c->stream);
-    double area = 0;
-    rc = read_partial_sum(c, c->partial.as<double>(), per, c->stream, &area);
+    // one read: the mid-point and triangle totals, then the area partials
+    void *hp = nullptr;
+    rc = pinned(c, 16 + (size_t)per * 8, &hp);
     if (rc) return rc;
+    int *tail = static_cast<int *>(hp);
+    const double *hpart = reinterpret_cast<const double *>(static_cast<char *>(hp) + 16);
+    tail[0] = tail[1] = tail[2] = tail[3] = 0;
+    if (nact) {
+        const int *src[4] = {c->anv.as<int>(), c->avoff.as<int>(), c->ant.as<int>(), c->atoff.as<int>()};
+        for (int q = 0; q < 4; ++q)
+            HIP_TRY(c, hipMemcpyAsync(tail + q, src[q] + nact - 1, 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(static_cast<char *>(hp) + 16, c->partial.p, (size_t)per * 8, hipMemcpyDeviceToHost,
+                                  c->stream));
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const int64_t nadd = (int64_t)tail[0] + tail[1], nt = (int64_t)tail[2] + tail[3];
+    double area = 0;
+    if (nact)
+        for (int64_t q = 0; q < per; ++q) area += hpart[q];  // partials past the last triangle are 0
+    const int64_t nv = c->n_analytic + nadd;
+    c->idx_dirty = nv;
+    c->nverts = nv;
     c->ntris = nt;
     c->area = area;
     c->have_mesh = true;

@@ -192,11 +192,23 @@ __global__ void __launch_bounds__(256) k_sum_slots(const double *v, int64_t n, V
     sum_block(v + slot * n, n, partial + slot * per, blockIdx.x);
 }
 
+// the same sums with n = *cnt + *off (an exclusive scan's total, on the device)
+__global__ void __launch_bounds__(256) k_sum_blocks_scan(const double *v, const int *cnt, const int *off,
+                                                         double *partial) {
+    sum_block(v, (int64_t)*cnt + *off, partial, blockIdx.x);
+}
+
 int64_t sum_partials_count(int64_t n) { return (n + 4095) / 4096; }
 
 void launch_sum_blocks(hipStream_t s, const double *v, int64_t n, double *partial) {
     const int64_t nb = sum_partials_count(n);
     if (nb > 0) hipLaunchKernelGGL(k_sum_blocks, dim3((unsigned)nb), dim3(256), 0, s, v, n, partial);
+}
+
+void launch_sum_blocks_scan(hipStream_t s, const double *v, int64_t ncap, const int *cnt, const int *off,
+                            double *partial) {
+    const int64_t nb = sum_partials_count(ncap);
+    if (nb > 0) hipLaunchKernelGGL(k_sum_blocks_scan, dim3((unsigned)nb), dim3(256), 0, s, v, cnt, off, partial);
 }
 
 void launch_sum_slots(hipStream_t s, const double *v, int64_t n, const VolumeSlots &vs, double *partial) {

@@ -29,6 +29,9 @@ void launch_fill_bits(hipStream_t, const DevGrid &, const DevRays &, const FillO
 void launch_corners_bits(hipStream_t, const DevGrid &, const DevRays &, const FillOpts &, uint32_t *);
 int64_t sum_partials_count(int64_t n);
 void launch_sum_blocks(hipStream_t, const double *, int64_t, double *);
+// k_sum_blocks over n = *cnt + *off <= ncap elements (partials past n are 0)
+void launch_sum_blocks_scan(hipStream_t, const double *v, int64_t ncap, const int *cnt, const int *off,
+                            double *partial);
 // x-runs of TEMP_OUT nodes: rows * ccl_row_capacity() slots each of rx0, rx1
 // and R (R is indexed by compact run id), one count and one offset per row
 struct CclRuns {
