@@ -677,6 +677,7 @@ static FillOpts fill_opts(const nsgpu_ctx *c) {
     fo.eslot_x = c->panels[0] + 0;
     fo.eslot_z = c->panels[0] + 1;
     fo.eslot_y = c->panels[0] + 2;
+    fo.out_code = (o->fill_cavities && o->build_status) ? NSGPU_STATUS_OUT : NSGPU_STATUS_TEMP_OUT;
     return fo;
 }
 

@@ -281,7 +281,10 @@ __global__ void __launch_bounds__(256) k_ccl_keys(DevGrid g, int cap, const int 
 }
 
 // Final labels and fillCavities (Surface.cpp:2355-2487) for the filled
-// components: one wave per row, lanes stride over each run's nodes.
+// components: one wave per row, lanes stride over each run's nodes.  The
+// map fill already wrote NSGPU_STATUS_OUT at every TEMP_OUT node, the label
+// of the first component (the outside region whenever node 0 is outside it),
+// so only the nodes of the other components are written here.
 __global__ void __launch_bounds__(256) k_ccl_label(DevGrid g, int cap, const int *rx0, const int *rx1,
                                                    const int *rowcnt, const int *roff, const int *R, const int *label,
                                                    const uint8_t *filled, const int *done, int build_eps, int accurate,
@@ -297,6 +300,7 @@ __global__ void __launch_bounds__(256) k_ccl_label(DevGrid g, int cap, const int
     for (int q = 0; q < n; ++q) {
         const int s = slot_of(R, c0 + q);
         const int lab = label[s];
+        if (lab == NSGPU_STATUS_OUT) continue;  // never filled: not a cavity
         const bool fill = filled[s] != 0;
         const int64_t id = row * cap + q;
         const int x1 = rx1[id];

@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(256) k_fill_bits(DevGrid g, DevRays R, FillOpt
         eps[3 * q + 1] = e1 ? o.inside : 0;
         eps[3 * q + 2] = e2 ? o.inside : 0;
         ideb[q] = in ? 0 : 1;
-        st[q] = in ? NSGPU_STATUS_INSIDE : NSGPU_STATUS_TEMP_OUT;
+        st[q] = in ? NSGPU_STATUS_INSIDE : o.out_code;
         if (++i == g.nx) {
             i = 0;
             if (++j == g.ny) {

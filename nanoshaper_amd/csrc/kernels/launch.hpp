@@ -8,6 +8,11 @@ struct FillOpts {
     int build_eps, build_status, inside;
     int slot_x, slot_z, slot_y;     // grid-ray slots of panels 0 (x), 1 (z), 2 (y); -1 = not traced
     int eslot_x, eslot_z, eslot_y;  // edge-ray slots
+    // status of a node outside the surface: NSGPU_STATUS_OUT when the cavity
+    // pass follows (its first component, the one with the smallest flat
+    // index, is numbered NSGPU_STATUS_OUT, and only the other components'
+    // nodes are relabelled), else NSGPU_STATUS_TEMP_OUT
+    int out_code;
 };
 
 constexpr int kPathCounters = 8;
