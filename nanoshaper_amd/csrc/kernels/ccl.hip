@@ -3,9 +3,11 @@
 // not the nodes: a 325^3 grid has ~34 M nodes but only a few hundred thousand
 // runs, so only the row scan and the final label pass touch the status map.
 //
-// Run storage: row r = k*ny + j owns the slots [r*cap, r*cap + rowcnt[r]) of
-// rx0/rx1 (first and last x of each run, in x order), cap = ceil(nx/2) (the
-// most runs a row can hold).  The union-find runs on compact run ids
+// Run storage: run q of row r = k*ny + j (q < rowcnt[r], in x order) is slot
+// r + q*rows of rx0/rx1 (its first and last x), rows = ny*nz; a row holds at
+// most ceil(nx/2) runs.  Run-major slots put the q-th runs of consecutive
+// rows in consecutive words, so the kernels' threads (one per row) read them
+// coalesced.  The union-find runs on compact run ids
 // c = roff[r] + q (roff: exclusive scan of rowcnt), which grow with the flat
 // index of the run's first node.  Unions are made in two levels: inside each
 // z-plane in LDS (one workgroup per plane, +y links), then across planes (+z
@@ -59,14 +61,13 @@ __device__ void unite(int *P, int a, int b) {
 // plane is set), so run starts and ends come from 64-node words with bit
 // operations and the status map itself is not read.  The k-th start and the
 // k-th end of a row belong to the same run.
-__global__ void __launch_bounds__(256) k_ccl_runs(DevGrid g, const unsigned long long *inside, int N, int cap,
+__global__ void __launch_bounds__(256) k_ccl_runs(DevGrid g, const unsigned long long *inside, int N, int64_t rs,
                                                   int *rx0, int *rx1, int *rowcnt) {
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= (int64_t)g.ny * g.nz) return;
     const int j = (int)(row % g.ny), k = (int)(row / g.ny);
     const int64_t NN = (int64_t)N * N, ray = (int64_t)k * N + j;
     const int nw = (g.nx + 63) / 64;
-    const int64_t slot0 = row * cap;
     auto temp_out = [&](int w) -> unsigned long long {  // TEMP_OUT bits of word w (0 past the row)
         if (w >= nw) return 0ull;
         unsigned long long t = ~inside[(int64_t)w * NN + ray];
@@ -81,11 +82,11 @@ __global__ void __launch_bounds__(256) k_ccl_runs(DevGrid g, const unsigned long
         unsigned long long starts = t & ~((t << 1) | prev);
         unsigned long long ends = t & ~((t >> 1) | (next << 63));
         while (starts) {
-            rx0[slot0 + ns++] = 64 * w + __builtin_ctzll(starts);
+            rx0[row + rs * ns++] = 64 * w + __builtin_ctzll(starts);
             starts &= starts - 1ull;
         }
         while (ends) {
-            rx1[slot0 + ne++] = 64 * w + __builtin_ctzll(ends);
+            rx1[row + rs * ne++] = 64 * w + __builtin_ctzll(ends);
             ends &= ends - 1ull;
         }
         prev = t >> 63;
@@ -122,12 +123,15 @@ __device__ void unite_lds(int *P, int a, int b) {
     }
 }
 
-// merge walk of two sorted run lists; f(a, b) for every overlapping pair
+// merge walk of the sorted run lists of rows ra and rb; f(a, b) for every
+// overlapping pair
 template <class F>
-__device__ __forceinline__ void overlaps(const int *rx0, const int *rx1, int64_t sa, int na, int64_t sb, int nb, F f) {
+__device__ __forceinline__ void overlaps(const int *rx0, const int *rx1, int64_t rs, int64_t ra, int na, int64_t rb,
+                                         int nb, F f) {
     int a = 0, b = 0;
     while (a < na && b < nb) {
-        const int a0 = rx0[sa + a], a1 = rx1[sa + a], b0 = rx0[sb + b], b1 = rx1[sb + b];
+        const int64_t sa = ra + rs * a, sb = rb + rs * b;
+        const int a0 = rx0[sa], a1 = rx1[sa], b0 = rx0[sb], b1 = rx1[sb];
         if (a1 < b0) {
             ++a;
         } else if (b1 < a0) {
@@ -147,7 +151,7 @@ constexpr int kPlaneRuns = 16384;  // runs of one z-plane united in LDS
 // One workgroup per z-plane: +y unions of the plane's runs, then every run
 // of the plane points at its plane-level root.  A plane with more runs than
 // the LDS holds unites in global memory instead.
-__global__ void __launch_bounds__(1024) k_ccl_plane(DevGrid g, int cap, const int *rx0, const int *rx1,
+__global__ void __launch_bounds__(1024) k_ccl_plane(DevGrid g, int64_t rs, const int *rx0, const int *rx1,
                                                     const int *rowcnt, const int *roff, int *R) {
     __shared__ int lp[kPlaneRuns];
     const int64_t row0 = (int64_t)blockIdx.x * g.ny, rowl = row0 + g.ny - 1;
@@ -165,7 +169,7 @@ __global__ void __launch_bounds__(1024) k_ccl_plane(DevGrid g, int cap, const in
     for (int j = threadIdx.x; j + 1 < g.ny; j += blockDim.x) {
         const int64_t ra = row0 + j, rb = ra + 1;
         const int la = roff[ra] - base, lb = roff[rb] - base;
-        overlaps(rx0, rx1, ra * cap, rowcnt[ra], rb * cap, rowcnt[rb], [&](int a, int b) {
+        overlaps(rx0, rx1, rs, ra, rowcnt[ra], rb, rowcnt[rb], [&](int a, int b) {
             if (in_lds)
                 unite_lds(lp, la + a, lb + b);
             else
@@ -192,7 +196,7 @@ __global__ void __launch_bounds__(1024) k_ccl_plane(DevGrid g, int cap, const in
 // pair (the outside region of plane k and of plane k+1): the wave's first
 // lane with an overlap unites its first pair for the whole wave, and every
 // lane skips that pair and repeats of its own previous pair.
-__global__ void __launch_bounds__(256) k_ccl_zlink(DevGrid g, int cap, const int *rx0, const int *rx1,
+__global__ void __launch_bounds__(256) k_ccl_zlink(DevGrid g, int64_t rs, const int *rx0, const int *rx1,
                                                    const int *rowcnt, const int *roff, int *R) {
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = row < (int64_t)g.ny * (g.nz - 1);
@@ -201,7 +205,7 @@ __global__ void __launch_bounds__(256) k_ccl_zlink(DevGrid g, int cap, const int
     const int na = valid ? rowcnt[row] : 0, nb = valid ? rowcnt[other] : 0;
     // first overlapping pair of this row
     int fa = -1, fb = -1;
-    overlaps(rx0, rx1, row * cap, na, other * cap, nb, [&](int a, int b) {
+    overlaps(rx0, rx1, rs, row, na, other, nb, [&](int a, int b) {
         if (fa < 0) {
             fa = ld(R + ca + a);
             fb = ld(R + cb + b);
@@ -214,7 +218,7 @@ __global__ void __launch_bounds__(256) k_ccl_zlink(DevGrid g, int cap, const int
     if ((int)(threadIdx.x & 63) == leader) unite(R, wa, wb);
     if (fa < 0) return;
     int last_a = wa, last_b = wb;
-    overlaps(rx0, rx1, row * cap, na, other * cap, nb, [&](int a, int b) {
+    overlaps(rx0, rx1, rs, row, na, other, nb, [&](int a, int b) {
         const int pa = ld(R + ca + a), pb = ld(R + cb + b);
         if ((pa == last_a && pb == last_b) || (pa == wa && pb == wb)) return;
         last_a = pa;
@@ -238,7 +242,7 @@ __global__ void __launch_bounds__(256) k_ccl_flatten(int64_t rows, const int *ro
 
 // roots get a compact slot, stored in place as -(slot+2); rootlist holds the
 // root's first node (the component's smallest flat index)
-__global__ void __launch_bounds__(256) k_ccl_roots(DevGrid g, int cap, const int *rx0, const int *rowcnt,
+__global__ void __launch_bounds__(256) k_ccl_roots(DevGrid g, int64_t rs, const int *rx0, const int *rowcnt,
                                                    const int *roff, int *R, int *nroots, int *rootlist) {
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= (int64_t)g.ny * g.nz) return;
@@ -246,7 +250,7 @@ __global__ void __launch_bounds__(256) k_ccl_roots(DevGrid g, int cap, const int
     for (int q = 0; q < n; ++q) {
         if (R[c0 + q] != c0 + q) continue;
         const int s = atomicAdd(nroots, 1);
-        rootlist[s] = (int)(row * g.nx + rx0[row * cap + q]);
+        rootlist[s] = (int)(row * g.nx + rx0[row + rs * q]);
         R[c0 + q] = -(s + 2);
     }
 }
@@ -261,7 +265,7 @@ __device__ __forceinline__ int slot_of(const int *R, int c) {
 // component of node 0 (when node 0 is TEMP_OUT) is the first one any scan
 // meets, so it is always labelled first and neither key is needed for it: it
 // is skipped, which removes the only heavily contended address.
-__global__ void __launch_bounds__(256) k_ccl_keys(DevGrid g, int cap, const int *rx0, const int *rx1,
+__global__ void __launch_bounds__(256) k_ccl_keys(DevGrid g, int64_t rs, const int *rx0, const int *rx1,
                                                   const int *rowcnt, const int *roff, const int *R, unsigned int *bkey,
                                                   int *count, int capacity) {
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -273,7 +277,7 @@ __global__ void __launch_bounds__(256) k_ccl_keys(DevGrid g, int cap, const int 
     for (int q = 0; q < n; ++q) {
         const int s = slot_of(R, c0 + q);
         if (s == skip || s >= capacity) continue;  // beyond capacity: the replay falls back to the host
-        const int64_t id = row * cap + q;
+        const int64_t id = row + rs * q;
         const unsigned int rank = (unsigned int)((g.nz - 1 - k) * plane + (int64_t)j * g.nx + rx0[id]);
         atomicMin(bkey + s, rank);
         atomicAdd(count + s, rx1[id] - rx0[id] + 1);
@@ -285,7 +289,7 @@ __global__ void __launch_bounds__(256) k_ccl_keys(DevGrid g, int cap, const int 
 // map fill already wrote NSGPU_STATUS_OUT at every TEMP_OUT node, the label
 // of the first component (the outside region whenever node 0 is outside it),
 // so only the nodes of the other components are written here.
-__global__ void __launch_bounds__(256) k_ccl_label(DevGrid g, int cap, const int *rx0, const int *rx1,
+__global__ void __launch_bounds__(256) k_ccl_label(DevGrid g, int64_t rs, const int *rx0, const int *rx1,
                                                    const int *rowcnt, const int *roff, const int *R, const int *label,
                                                    const uint8_t *filled, const int *done, int build_eps, int accurate,
                                                    int inside, int *status, int *epsmap, uint8_t *idebmap,
@@ -302,7 +306,7 @@ __global__ void __launch_bounds__(256) k_ccl_label(DevGrid g, int cap, const int
         const int lab = label[s];
         if (lab == NSGPU_STATUS_OUT) continue;  // never filled: not a cavity
         const bool fill = filled[s] != 0;
-        const int64_t id = row * cap + q;
+        const int64_t id = row + rs * q;
         const int x1 = rx1[id];
         for (int i = rx0[id] + lane; i <= x1; i += 64) {
             const int64_t f = base + i;
@@ -337,23 +341,22 @@ int ccl_row_capacity(const DevGrid &g) { return (g.nx + 1) / 2; }
 void launch_ccl(hipStream_t s, const DevGrid &g, const DevRays &R, const CclRuns &c, int *scan_scratch, int *nroots,
                 int *rootlist) {
     const int64_t rows = (int64_t)g.ny * g.nz;
-    const int cap = ccl_row_capacity(g);
-    hipLaunchKernelGGL(k_ccl_runs, dim3(nblk(rows)), dim3(256), 0, s, g, R.cov_st, R.N, cap, c.rx0, c.rx1, c.rowcnt);
+    hipLaunchKernelGGL(k_ccl_runs, dim3(nblk(rows)), dim3(256), 0, s, g, R.cov_st, R.N, rows, c.rx0, c.rx1, c.rowcnt);
     launch_scan(s, c.rowcnt, rows, c.roff, scan_scratch);
-    hipLaunchKernelGGL(k_ccl_plane, dim3((unsigned)g.nz), dim3(1024), 0, s, g, cap, c.rx0, c.rx1, c.rowcnt, c.roff,
+    hipLaunchKernelGGL(k_ccl_plane, dim3((unsigned)g.nz), dim3(1024), 0, s, g, rows, c.rx0, c.rx1, c.rowcnt, c.roff,
                        c.R);
     if (g.nz > 1)
-        hipLaunchKernelGGL(k_ccl_zlink, dim3(nblk((int64_t)g.ny * (g.nz - 1))), dim3(256), 0, s, g, cap, c.rx0, c.rx1,
+        hipLaunchKernelGGL(k_ccl_zlink, dim3(nblk((int64_t)g.ny * (g.nz - 1))), dim3(256), 0, s, g, rows, c.rx0, c.rx1,
                            c.rowcnt, c.roff, c.R);
     hipLaunchKernelGGL(k_ccl_flatten, dim3(nblk(rows)), dim3(256), 0, s, rows, c.rowcnt, c.roff, c.R);
-    hipLaunchKernelGGL(k_ccl_roots, dim3(nblk(rows)), dim3(256), 0, s, g, cap, c.rx0, c.rowcnt, c.roff, c.R, nroots,
+    hipLaunchKernelGGL(k_ccl_roots, dim3(nblk(rows)), dim3(256), 0, s, g, rows, c.rx0, c.rowcnt, c.roff, c.R, nroots,
                        rootlist);
 }
 
 void launch_ccl_keys(hipStream_t s, const DevGrid &g, const CclRuns &c, unsigned int *bkey, int *count,
                      int capacity) {
     const int64_t rows = (int64_t)g.ny * g.nz;
-    hipLaunchKernelGGL(k_ccl_keys, dim3(nblk(rows)), dim3(256), 0, s, g, ccl_row_capacity(g), c.rx0, c.rx1, c.rowcnt,
+    hipLaunchKernelGGL(k_ccl_keys, dim3(nblk(rows)), dim3(256), 0, s, g, rows, c.rx0, c.rx1, c.rowcnt,
                        c.roff, c.R, bkey, count, capacity);
 }
 
@@ -361,7 +364,7 @@ void launch_ccl_label(hipStream_t s, const DevGrid &g, const CclRuns &c, const i
                       const int *done, int build_eps, int accurate, int inside, int *status, int *eps, uint8_t *ideb,
                       uint32_t *bits) {
     const int64_t rows = (int64_t)g.ny * g.nz;
-    hipLaunchKernelGGL(k_ccl_label, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, s, g, ccl_row_capacity(g), c.rx0,
+    hipLaunchKernelGGL(k_ccl_label, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, s, g, rows, c.rx0,
                        c.rx1, c.rowcnt, c.roff, c.R, label, filled, done, build_eps, accurate, inside, status, eps,
                        ideb, bits);
 }
