@@ -95,6 +95,7 @@ struct nsgpu_ctx {
     // sums): copies into pageable memory are staged and serialised
     void *hpin = nullptr;
     size_t hpin_cap = 0;
+    DBuf dgather;  // device side of those reads (launch_gather)
     bool capturing = false;  // enqueueing into a graph: no stage events
     std::vector<double> cav_size;
     std::vector<uint8_t> cav_filled;
@@ -231,7 +232,7 @@ void nsgpu_destroy(nsgpu_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     DBuf *all[] = {&c->gx, &c->gy, &c->gz, &c->pkind, &c->poff, &c->pcnt, &c->pflags, &c->pbsph, &c->pdata, &c->porder, &c->patom, &c->atoms, &c->rpid, &c->gradv, &c->grada, &c->asums,
                    &c->rcnt, &c->rt, &c->rvol, &c->partial, &c->stats, &c->cand, &c->cov, &c->covst, &c->eps, &c->ideb, &c->status,
-                   &c->bits, &c->crx0, &c->crx1, &c->cpar, &c->crowcnt, &c->croff, &c->nroots, &c->rootlist, &c->bkey, &c->ccount, &c->clabel, &c->cfilled, &c->dcavsize, &c->dcavfilled, &c->hostdone, &c->dres, &c->dnv,
+                   &c->bits, &c->crx0, &c->crx1, &c->cpar, &c->crowcnt, &c->croff, &c->nroots, &c->rootlist, &c->bkey, &c->ccount, &c->clabel, &c->cfilled, &c->dcavsize, &c->dcavfilled, &c->hostdone, &c->dres, &c->dnv, &c->dgather,
                    &c->idx, &c->vslot, &c->vhit, &c->vorig, &c->normals, &c->vcount, &c->voffs, &c->scan_scratch, &c->verts, &c->tris, &c->tarea,
                    &c->segmask, &c->anode, &c->acase, &c->anv, &c->ant, &c->avoff, &c->atoff, &c->mverts, &c->mtris,
                    &c->mrr, &c->mraw, &c->field, &c->fbits, &c->batoms, &c->bbox, &c->bcount, &c->boff, &c->bcur,
@@ -797,22 +798,42 @@ static int pinned(nsgpu_ctx *c, size_t bytes, void **out) {
 
 // count[n-1] + offs[n-1] of an exclusive scan, for each (counts, offs) pair:
 // the copies go to pinned memory, so they queue behind one synchronisation
+// Device scalars (up to 8 ints) and nd doubles to the host: one gather
+// kernel, one copy into the pinned buffer, one synchronisation.  On return
+// *ints points at the ints and *dbl at the doubles.
+static int read_gathered(nsgpu_ctx *c, hipStream_t st, const GatherSrc &g, const int **ints, const double **dbl) {
+    const size_t bytes = 32 + (size_t)g.nd * 8;
+    void *hp = nullptr;
+    int rc = pinned(c, bytes, &hp);
+    if (rc) return rc;
+    HIP_TRY(c, c->dgather.ensure(bytes));
+    launch_gather(st, g, c->dgather.as<int>());
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipMemcpyAsync(hp, c->dgather.p, bytes, hipMemcpyDeviceToHost, st));
+    HIP_TRY(c, hipStreamSynchronize(st));
+    *ints = static_cast<const int *>(hp);
+    *dbl = reinterpret_cast<const double *>(static_cast<const char *>(hp) + 32);
+    return NSGPU_OK;
+}
+
 static int read_totals(nsgpu_ctx *c, int npairs, const int *const *counts, const int *const *offs, const int64_t *n,
                        int64_t *total) {
-    void *hp = nullptr;
-    int rc = pinned(c, (size_t)npairs * 8, &hp);
-    if (rc) return rc;
-    int *h = static_cast<int *>(hp);
-    bool any = false;
-    for (int q = 0; q < npairs; ++q) {
-        h[2 * q] = h[2 * q + 1] = 0;
+    GatherSrc gs{};
+    int at[4] = {-1, -1, -1, -1};
+    for (int q = 0; q < npairs && q < 4; ++q) {
+        total[q] = 0;
         if (n[q] <= 0) continue;
-        HIP_TRY(c, hipMemcpyAsync(h + 2 * q, counts[q] + n[q] - 1, 4, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(c, hipMemcpyAsync(h + 2 * q + 1, offs[q] + n[q] - 1, 4, hipMemcpyDeviceToHost, c->stream));
-        any = true;
+        at[q] = gs.n;
+        gs.src[gs.n++] = counts[q] + n[q] - 1;
+        gs.src[gs.n++] = offs[q] + n[q] - 1;
     }
-    if (any) HIP_TRY(c, hipStreamSynchronize(c->stream));
-    for (int q = 0; q < npairs; ++q) total[q] = (int64_t)h[2 * q] + h[2 * q + 1];
+    if (gs.n == 0) return NSGPU_OK;
+    const int *h = nullptr;
+    const double *unused = nullptr;
+    const int rc = read_gathered(c, c->stream, gs, &h, &unused);
+    if (rc) return rc;
+    for (int q = 0; q < npairs && q < 4; ++q)
+        if (at[q] >= 0) total[q] = (int64_t)h[at[q]] + h[at[q] + 1];
     return NSGPU_OK;
 }
 
@@ -1024,24 +1045,27 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
     HIP_TRY(c, hipGetLastError());
     mark(c, ST_MC, 1, c->stream);
     // one read: the mid-point and triangle totals, then the area partials
-    void *hp = nullptr;
-    rc = pinned(c, 16 + (size_t)per * 8, &hp);
-    if (rc) return rc;
-    int *tail = static_cast<int *>(hp);
-    const double *hpart = reinterpret_cast<const double *>(static_cast<char *>(hp) + 16);
-    tail[0] = tail[1] = tail[2] = tail[3] = 0;
-    if (nact) {
-        const int *src[4] = {c->anv.as<int>(), c->avoff.as<int>(), c->ant.as<int>(), c->atoff.as<int>()};
-        for (int q = 0; q < 4; ++q)
-            HIP_TRY(c, hipMemcpyAsync(tail + q, src[q] + nact - 1, 4, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(c, hipMemcpyAsync(static_cast<char *>(hp) + 16, c->partial.p, (size_t)per * 8, hipMemcpyDeviceToHost,
-                                  c->stream));
-    }
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    const int64_t nadd = (int64_t)tail[0] + tail[1], nt = (int64_t)tail[2] + tail[3];
+    int64_t nadd = 0, nt = 0;
     double area = 0;
-    if (nact)
+    if (nact) {
+        GatherSrc gs{};
+        gs.src[0] = c->anv.as<int>() + nact - 1;
+        gs.src[1] = c->avoff.as<int>() + nact - 1;
+        gs.src[2] = c->ant.as<int>() + nact - 1;
+        gs.src[3] = c->atoff.as<int>() + nact - 1;
+        gs.n = 4;
+        gs.dsrc = c->partial.as<double>();
+        gs.nd = per;
+        const int *tail = nullptr;
+        const double *hpart = nullptr;
+        rc = read_gathered(c, c->stream, gs, &tail, &hpart);
+        if (rc) return rc;
+        nadd = (int64_t)tail[0] + tail[1];
+        nt = (int64_t)tail[2] + tail[3];
         for (int64_t q = 0; q < per; ++q) area += hpart[q];  // partials past the last triangle are 0
+    } else {
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
     const int64_t nv = c->n_analytic + nadd;
     c->idx_dirty = nv;
     c->nverts = nv;

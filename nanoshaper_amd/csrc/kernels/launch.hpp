@@ -92,6 +92,15 @@ void launch_ccl_replay(hipStream_t, const CavityReplay &);
 int64_t scan_scratch_ints(int64_t n);
 void launch_scan(hipStream_t, const int *, int64_t, int *, int *);
 void launch_scan_total(hipStream_t, const int *counts, const int *offs, int64_t n, int *out);
+// up to 8 device ints and nd doubles gathered into one device buffer
+// (ints at dst[0..8), doubles from dst + 8), for a single copy to the host
+struct GatherSrc {
+    const int *src[8];
+    int n;
+    const double *dsrc;
+    int64_t nd;
+};
+void launch_gather(hipStream_t, const GatherSrc &, int *dst);
 void mc_upload_tables(hipStream_t);
 void launch_vtx_count(hipStream_t, const DevGrid &, const DevRays &, const int[3], const uint32_t *, int *);
 // vslot[v] receives the index slot of vertex v (for launch_idx_reset), vhit[v]

@@ -51,6 +51,33 @@ __global__ void __launch_bounds__(256) k_scan_down(const int *in, int64_t n, con
     }
 }
 
+// As k_scan_down, with the block's offset summed from the block sums of the
+// blocks before it (nb <= kFusedBlocks), which saves the scan of the block
+// sums as a launch of its own.
+constexpr int64_t kFusedBlocks = 4096;
+
+__global__ void __launch_bounds__(256) k_scan_down_fused(const int *in, int64_t n, const int *bsum, int *out) {
+    __shared__ int sh[4], red[4];
+    int pre = 0;
+    for (int q = threadIdx.x; q < (int)blockIdx.x; q += 256) pre += bsum[q];
+    for (int off = 32; off > 0; off >>= 1) pre += __shfl_down(pre, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = pre;
+    __syncthreads();
+    const int boff = red[0] + red[1] + red[2] + red[3];
+    const int64_t base = (int64_t)blockIdx.x * kScanItems + threadIdx.x * 4;
+    int v[4], sum = 0;
+    for (int q = 0; q < 4; ++q) {
+        v[q] = (base + q < n) ? in[base + q] : 0;
+        sum += v[q];
+    }
+    int total;
+    int run = block_excl_scan(sum, sh, total) + boff;
+    for (int q = 0; q < 4; ++q) {
+        if (base + q < n) out[base + q] = run;
+        run += v[q];
+    }
+}
+
 // scratch must hold scan_scratch_ints(n) ints; out[n] receives the total when
 // out has n+1 entries (with_total)
 int64_t scan_scratch_ints(int64_t n) {
@@ -71,6 +98,10 @@ void launch_scan(hipStream_t st, const int *in, int64_t n, int *out, int *scratc
     }
     int *bsum = scratch, *boff = scratch + nb;
     hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(256), 0, st, in, n, bsum);
+    if (nb <= kFusedBlocks) {
+        hipLaunchKernelGGL(k_scan_down_fused, dim3((unsigned)nb), dim3(256), 0, st, in, n, (const int *)bsum, out);
+        return;
+    }
     launch_scan(st, bsum, nb, boff, scratch + 2 * nb);
     hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(256), 0, st, in, n, boff, out);
 }
@@ -85,6 +116,21 @@ __global__ void k_scan_total(const int *counts, const int *offs, int64_t n, int 
 
 void launch_scan_total(hipStream_t st, const int *counts, const int *offs, int64_t n, int *out) {
     hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, st, counts, offs, n, out);
+}
+
+// dst[0..n) = *src[q] (the scalars the host reads between stages), then
+// dst + 8 (as doubles) = dsrc[0..nd): one copy to the host takes them all
+__global__ void __launch_bounds__(256) k_gather(GatherSrc g, int *dst) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < g.n) dst[t] = *g.src[t];
+    double *dd = reinterpret_cast<double *>(dst + 8);
+    for (int64_t q = t; q < g.nd; q += (int64_t)gridDim.x * blockDim.x) dd[q] = g.dsrc[q];
+}
+
+void launch_gather(hipStream_t st, const GatherSrc &g, int *dst) {
+    const int64_t work = g.nd > 8 ? g.nd : 8;
+    const unsigned nb = (unsigned)((work + 255) / 256 < 1024 ? (work + 255) / 256 : 1024);
+    hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, st, g, dst);
 }
 
 }  // namespace nsg
