@@ -29,30 +29,32 @@ struct EdgeSlots {
 };
 
 // Walk the vertex records of one edge ray.  emit(q, dir, ix, iy, iz, px, py, pz)
-// is called for every kept record in order; returns the count.
-template <class Emit>
-__device__ int walk_edge_ray(const DevGrid &g, const DevRays &R, const uint32_t *bits, int panel, int slot,
-                             int64_t ray, Emit emit) {
+// is called for every kept record in order; returns the count.  One instance
+// per panel (0: x rays, 1: z rays, 2: y rays), so the grid arrays and extents
+// it reads are fixed at compile time.
+template <int PANEL, class Emit>
+__device__ int walk_edge_ray_p(const DevGrid &g, const DevRays &R, const uint32_t *bits, int slot, int64_t ray,
+                               Emit emit) {
     const int c = R.cnt[(int64_t)slot * R.N * R.N + ray];
     if (c == 0 || (c & 1) || c > R.K) return 0;
     const double *T = R.t + ((int64_t)slot * R.N * R.N + ray) * R.K;
     const int n = (int)(ray / R.N), m = (int)(ray % R.N);
     const double delta = kDelta - g.hside;
-    const double *ga = panel == 0 ? g.x : (panel == 1 ? g.z : g.y);
-    const int na = panel == 0 ? g.nx : (panel == 1 ? g.nz : g.ny);
-    const double gmin = panel == 0 ? g.xmin : (panel == 1 ? g.zmin : g.ymin);
+    const double *ga = PANEL == 0 ? g.x : (PANEL == 1 ? g.z : g.y);
+    const int na = PANEL == 0 ? g.nx : (PANEL == 1 ? g.nz : g.ny);
+    const double gmin = PANEL == 0 ? g.xmin : (PANEL == 1 ? g.zmin : g.ymin);
     const double pa = ga[0] + delta;
     const double dir = (ga[na - 1] + delta) - pa;
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
     // fixed coordinates of the ray and the flat-index stride along it
     double p0, p1;
     int64_t stride;
-    if (panel == 0) {
+    if (PANEL == 0) {
         if (m >= g.ny || n >= g.nz) return 0;
         p0 = g.y[m] + delta;
         p1 = g.z[n] + delta;
         stride = 1;
-    } else if (panel == 1) {
+    } else if (PANEL == 1) {
         if (m >= g.nx || n >= g.ny) return 0;
         p0 = g.x[m] + delta;
         p1 = g.y[n] + delta;
@@ -63,38 +65,47 @@ __device__ int walk_edge_ray(const DevGrid &g, const DevRays &R, const uint32_t 
         p1 = g.z[n] + delta;
         stride = g.nx;
     }
-    int claimed[2 * 16];
-    int nclaimed = 0, count = 0;
+    // The depths are sorted and dir > 0, so the segment ends visited below come
+    // in non-decreasing node order: a node already given a vertex can only be
+    // the last one given one.
+    int last = -1, count = 0;
     for (int q = 0; q < c; q += 2) {
         const double t0 = T[q], t1 = T[q + 1];
         if (!(fabs(t1 - t0) >= kEpsInt)) continue;
-        const double lim[2] = {pa + dir * t0, pa + dir * t1};
-        const int a = (int)rintp((lim[0] - gmin) * g.scale);
-        const int b = (int)rintp((lim[1] - gmin) * g.scale);
+        const double lim0 = pa + dir * t0, lim1 = pa + dir * t1;
+        const int a = (int)rintp((lim0 - gmin) * g.scale);
+        const int b = (int)rintp((lim1 - gmin) * g.scale);
         if (b <= a) continue;
-        const int e[2] = {a, b};
+#pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int ia = e[h];
+            const int ia = h == 0 ? a : b;
+            const double lim = h == 0 ? lim0 : lim1;
             if (ia < 0 || ia >= na) continue;
             int ix, iy, iz, d;
-            if (panel == 0) { ix = ia; iy = m; iz = n; d = 0; }
-            else if (panel == 1) { ix = m; iy = n; iz = ia; d = 2; }
+            if (PANEL == 0) { ix = ia; iy = m; iz = n; d = 0; }
+            else if (PANEL == 1) { ix = m; iy = n; iz = ia; d = 2; }
             else { ix = m; iy = ia; iz = n; d = 1; }
             const int64_t f = ((int64_t)iz * g.ny + iy) * g.nx + ix;
             if (bit_at(bits, f, nvox) == bit_at(bits, f + stride, nvox)) continue;
-            bool dup = false;
-            for (int u = 0; u < nclaimed; ++u) dup |= claimed[u] == ia;
-            if (dup) continue;
-            if (nclaimed < 32) claimed[nclaimed++] = ia;
+            if (ia == last) continue;
+            last = ia;
             double px, py, pz;
-            if (panel == 0) { px = lim[h]; py = p0; pz = p1; }
-            else if (panel == 1) { px = p0; py = p1; pz = lim[h]; }
-            else { px = p0; py = lim[h]; pz = p1; }
+            if (PANEL == 0) { px = lim; py = p0; pz = p1; }
+            else if (PANEL == 1) { px = p0; py = p1; pz = lim; }
+            else { px = p0; py = lim; pz = p1; }
             emit(count, d, f, px, py, pz, q + h);
             ++count;
         }
     }
     return count;
+}
+
+template <class Emit>
+__device__ __forceinline__ int walk_edge_ray(const DevGrid &g, const DevRays &R, const uint32_t *bits, int panel,
+                                             int slot, int64_t ray, Emit emit) {
+    if (panel == 0) return walk_edge_ray_p<0>(g, R, bits, slot, ray, emit);
+    if (panel == 1) return walk_edge_ray_p<1>(g, R, bits, slot, ray, emit);
+    return walk_edge_ray_p<2>(g, R, bits, slot, ray, emit);
 }
 
 __global__ void __launch_bounds__(256) k_vtx_count(DevGrid g, DevRays R, EdgeSlots es, const uint32_t *bits,
