@@ -48,12 +48,14 @@ struct HitSurface {
     double minv[3][3];     // inverse of the facet tangency matrix, rows p - c_m
 };
 
-// f and grad_x f at x; parameter k (atom k/4, component k%4) carries the
-// derivative seed
-__device__ void surface_field(const HitSurface &h, double rp, int k, const D1 x[3], D1 &f, D1 grad[3]) {
-    const int ks = k >> 2, kq = k & 3;
+// f and grad_x f at x; parameter (atom ks, component kq: x, y, z, radius)
+// carries the derivative seed.  Inlined into loops over constant ks, kq, so
+// every array index below is a constant.
+__device__ __forceinline__ void surface_field(const HitSurface &h, double rp, int ks, int kq, const D1 x[3], D1 &f,
+                                             D1 grad[3]) {
     D1 c[3][3], r[3];
-    for (int s = 0; s < h.nat; ++s) {
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
         for (int q = 0; q < 3; ++q) c[s][q] = {h.c[s][q], (s == ks && q == kq) ? 1. : 0.};
         r[s] = {h.r[s], (s == ks && kq == 3) ? 1. : 0.};
     }
@@ -104,17 +106,19 @@ __device__ __forceinline__ void inv3(const double m[3][3], double o[3][3]) {
 }
 
 // One hit: adds its volume and area derivatives to the atoms' gradients and
-// returns its area term 1/|n|_1.
-__device__ double hit_adjoint(const DevPatches &P, const AdjointArgs &a, int pid, const double x0[3], int vc,
-                              double L, double sgn) {
+// returns its area term 1/|n|_1.  VC is the ray axis.
+template <int VC>
+__device__ __forceinline__ double hit_adjoint(const DevPatches &P, const AdjointArgs &a, int pid, const double x0[3],
+                                              double L, double sgn) {
     HitSurface h;
     h.kind = P.kind[pid];
     h.nat = h.kind == NSGPU_PATCH_POINT ? 1 : (h.kind <= NSGPU_PATCH_FACET_SING ? 3 : 2);
-    for (int s = 0; s < h.nat; ++s) {
-        const int id = P.atom[3 * pid + s];
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        const int id = s < h.nat ? P.atom[3 * pid + s] : 0;
         h.atom[s] = id;
-        for (int q = 0; q < 3; ++q) h.c[s][q] = a.atoms[4 * id + q];
-        h.r[s] = a.atoms[4 * id + 3];
+        for (int q = 0; q < 3; ++q) h.c[s][q] = s < h.nat ? a.atoms[4 * id + q] : 0.;
+        h.r[s] = s < h.nat ? a.atoms[4 * id + 3] : 0.;
     }
     if (h.kind == NSGPU_PATCH_FACET_REG || h.kind == NSGPU_PATCH_FACET_SING) {
         double m[3][3];
@@ -125,25 +129,59 @@ __device__ double hit_adjoint(const DevPatches &P, const AdjointArgs &a, int pid
     }
     const double vscale = sgn * a.area_px * L / a.flag;
     double inv_v = 0.;
-    for (int k = 0; k < 4 * h.nat; ++k) {
-        // depth derivative at the fixed point: dt = -(df/dk) / (grad f . e L)
-        D1 x[3] = {{x0[0], 0.}, {x0[1], 0.}, {x0[2], 0.}};
-        D1 f, gr[3];
-        surface_field(h, a.probe, k, x, f, gr);
-        const double ft = gr[vc].v * L;
-        const double dt = (ft != 0.) ? -f.d / ft : 0.;
-        // normal at the moved point
-        x[vc].d = L * dt;
-        surface_field(h, a.probe, k, x, f, gr);
-        const D1 gn = dsqrt(dot3(gr, gr));
-        const D1 l1 = dabs(gr[0] / gn) + dabs(gr[1] / gn) + dabs(gr[2] / gn);
-        const D1 inv = D1{1., 0.} / l1;
-        inv_v = inv.v;
-        const int idx = 4 * h.atom[k >> 2] + (k & 3);
-        atomicAdd(a.grad_v + idx, vscale * dt);
-        atomicAdd(a.grad_a + idx, a.area_px * inv.d);
+#pragma unroll
+    for (int ks = 0; ks < 3; ++ks) {
+        if (ks >= h.nat) break;
+#pragma unroll
+        for (int kq = 0; kq < 4; ++kq) {
+            // depth derivative at the fixed point: dt = -(df/dk) / (grad f . e L)
+            D1 x[3] = {{x0[0], 0.}, {x0[1], 0.}, {x0[2], 0.}};
+            D1 f, gr[3];
+            surface_field(h, a.probe, ks, kq, x, f, gr);
+            const double ft = gr[VC].v * L;
+            const double dt = (ft != 0.) ? -f.d / ft : 0.;
+            // normal at the moved point
+            x[VC].d = L * dt;
+            surface_field(h, a.probe, ks, kq, x, f, gr);
+            const D1 gn = dsqrt(dot3(gr, gr));
+            const D1 l1 = dabs(gr[0] / gn) + dabs(gr[1] / gn) + dabs(gr[2] / gn);
+            const D1 inv = D1{1., 0.} / l1;
+            inv_v = inv.v;
+            const int idx = 4 * h.atom[ks] + kq;
+            atomicAdd(a.grad_v + idx, vscale * dt);
+            atomicAdd(a.grad_a + idx, a.area_px * inv.d);
+        }
     }
     return inv_v;
+}
+
+// the kept segments of one grid ray of panel PANEL: volume and area terms
+template <int PANEL>
+__device__ __forceinline__ void ray_adjoint(const DevGrid &g, const DevPatches &P, const DevRays &R,
+                                            const AdjointArgs &a, int64_t r, int64_t ray, int c, double &vsum,
+                                            double &asum) {
+    constexpr int fd = PanelAxes<PANEL>::fd, ld = PanelAxes<PANEL>::ld, vc = PanelAxes<PANEL>::vc;
+    const double *ga = vc == 0 ? g.x : (vc == 1 ? g.y : g.z);
+    const double *gf = fd == 0 ? g.x : g.y;
+    const double *gl = ld == 1 ? g.y : g.z;
+    const int na = vc == 0 ? g.nx : (vc == 1 ? g.ny : g.nz);
+    const int m = (int)(ray % R.N), n = (int)(ray / R.N);
+    const double L = ga[na - 1] - ga[0];
+    double pa[3];
+    pa[vc] = ga[0];
+    pa[fd] = gf[m];
+    pa[ld] = gl[n];
+    const double *T = R.t + r * R.K;
+    const int *I = R.pid + r * R.K;
+    for (int i = 0; i < c; i += 2) {
+        if (fabs(T[i + 1] - T[i]) < kEpsInt) continue;
+        vsum += (pa[vc] + L * T[i + 1]) - (pa[vc] + L * T[i]);
+        for (int e = 0; e < 2; ++e) {
+            double x[3] = {pa[0], pa[1], pa[2]};
+            x[vc] += L * T[i + e];
+            asum += hit_adjoint<vc>(P, a, I[i + e], x, L, e ? 1. : -1.);
+        }
+    }
 }
 
 __global__ void __launch_bounds__(256) k_adjoint(DevGrid g, DevPatches P, DevRays R, AdjointArgs a) {
@@ -156,29 +194,9 @@ __global__ void __launch_bounds__(256) k_adjoint(DevGrid g, DevPatches P, DevRay
         if (c > 0 && c <= R.K && (c & 1) == 0) {
             const int panel = (int)(r / nper);
             const int64_t ray = r - panel * nper;
-            int fd, ld, vc;
-            const double *ga, *gf, *gl;
-            int na;
-            if (panel == 0) { fd = 1; ld = 2; vc = 0; ga = g.x; gf = g.y; gl = g.z; na = g.nx; }
-            else if (panel == 1) { fd = 0; ld = 1; vc = 2; ga = g.z; gf = g.x; gl = g.y; na = g.nz; }
-            else { fd = 0; ld = 2; vc = 1; ga = g.y; gf = g.x; gl = g.z; na = g.ny; }
-            const int m = (int)(ray % R.N), n = (int)(ray / R.N);
-            const double L = ga[na - 1] - ga[0];
-            double pa[3];
-            pa[vc] = ga[0];
-            pa[fd] = gf[m];
-            pa[ld] = gl[n];
-            const double *T = R.t + r * R.K;
-            const int *I = R.pid + r * R.K;
-            for (int i = 0; i < c; i += 2) {
-                if (fabs(T[i + 1] - T[i]) < kEpsInt) continue;
-                vsum += (pa[vc] + L * T[i + 1]) - (pa[vc] + L * T[i]);
-                for (int e = 0; e < 2; ++e) {
-                    double x[3] = {pa[0], pa[1], pa[2]};
-                    x[vc] += L * T[i + e];
-                    asum += hit_adjoint(P, a, I[i + e], x, vc, L, e ? 1. : -1.);
-                }
-            }
+            if (panel == 0) ray_adjoint<0>(g, P, R, a, r, ray, c, vsum, asum);
+            else if (panel == 1) ray_adjoint<1>(g, P, R, a, r, ray, c, vsum, asum);
+            else ray_adjoint<2>(g, P, R, a, r, ray, c, vsum, asum);
         }
     }
     for (int off = 32; off > 0; off >>= 1) {

@@ -17,6 +17,8 @@
 //   patch-based (getPatchIntersectionData :1264-1484): the rays of the
 //     triangle's grid rectangle inside its bounding box; direction
 //     x[N-1] - x[0].
+#include <type_traits>
+
 #include "launch.hpp"
 
 namespace nsg {
@@ -340,10 +342,12 @@ __global__ void __launch_bounds__(64) k_intersect_mesh(DevGrid g, DevRays R, Mes
         for (int slot = 0; slot < R.nslots && !culled; ++slot) {
             const int phase = slot >= R.panels0;
             const int panel = phase ? slot - R.panels0 : slot;
-            int first, last, vc;
-            if (panel == 0) { first = 1; last = 2; vc = 0; }
-            else if (panel == 1) { first = 0; last = 1; vc = 2; }
-            else { first = 0; last = 2; vc = 1; }
+            // one instance per panel: the axes are constants, so the ray origin
+            // and the box stay in registers
+            auto walk = [&](auto panel_c) {
+            constexpr int first = PanelAxes<decltype(panel_c)::value>::fd;
+            constexpr int last = PanelAxes<decltype(panel_c)::value>::ld;
+            constexpr int vc = PanelAxes<decltype(panel_c)::value>::vc;
             const double delta = phase ? kDelta - g.hside : 0.;
             const double *cf = axis_coords(g, first), *cl = axis_coords(g, last), *cv = axis_coords(g, vc);
             const int nvc = axis_n(g, vc);
@@ -411,6 +415,10 @@ __global__ void __launch_bounds__(64) k_intersect_mesh(DevGrid g, DevRays R, Mes
                     }
                 }
             }
+            };
+            if (panel == 0) walk(std::integral_constant<int, 0>());
+            else if (panel == 1) walk(std::integral_constant<int, 1>());
+            else walk(std::integral_constant<int, 2>());
         }
     }
     for (int off = 32; off > 0; off >>= 1) cand += __shfl_down(cand, off, 64);
