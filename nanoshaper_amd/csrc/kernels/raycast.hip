@@ -837,6 +837,68 @@ struct PlaneWriter {
     }
 };
 
+// Ascending sort of a ray's n <= NS depths (and patch ids) in registers: a
+// bitonic network over NS slots padded with +inf, every index a constant.
+// A pair is exchanged only when the first is strictly greater, as in the
+// insertion sort, so the depths come out in the same order.
+template <int NS, bool kPid>
+__device__ __forceinline__ void sort_small(double *T, int *I, int n) {
+    double v[NS];
+    int id[NS];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+        v[q] = q < n ? T[q] : __longlong_as_double(0x7ff0000000000000ll);
+        id[q] = (kPid && q < n) ? I[q] : 0;
+    }
+#pragma unroll
+    for (int k = 2; k <= NS; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < NS; ++i) {
+                const int l = i ^ j;
+                if (l <= i) continue;
+                const bool up = (i & k) == 0;
+                const int lo = up ? i : l, hi = up ? l : i;  // after the exchange v[lo] <= v[hi]
+                if (v[lo] > v[hi]) {
+                    const double t = v[lo];
+                    v[lo] = v[hi];
+                    v[hi] = t;
+                    if (kPid) {
+                        const int u = id[lo];
+                        id[lo] = id[hi];
+                        id[hi] = u;
+                    }
+                }
+            }
+#pragma unroll
+    for (int q = 0; q < NS; ++q)
+        if (q < n) {
+            T[q] = v[q];
+            if (kPid) I[q] = id[q];
+        }
+}
+
+template <bool kPid>
+__device__ __forceinline__ void sort_depths(double *T, int *I, int n) {
+    if (n <= 1) return;
+    if (n <= 4) return sort_small<4, kPid>(T, I, n);
+    if (n <= 8) return sort_small<8, kPid>(T, I, n);
+    if (n <= 16) return sort_small<16, kPid>(T, I, n);
+    for (int a = 1; a < n; ++a) {  // longer lists: insertion sort in place
+        const double v = T[a];
+        const int id = kPid ? I[a] : 0;
+        int b = a - 1;
+        while (b >= 0 && T[b] > v) {
+            T[b + 1] = T[b];
+            if (kPid) I[b + 1] = I[b];
+            --b;
+        }
+        T[b + 1] = v;
+        if (kPid) I[b + 1] = id;
+    }
+}
+
 // Per-ray insertion sort of the depths (Surface::reorderPatchIntersections,
 // Surface.cpp:5080-5095), plus the ray's segment-length integral of
 // setVerticesAndGridsWithIntersectionData (Surface.cpp:5265-5290, :5425-5440)
@@ -852,31 +914,10 @@ __global__ void __launch_bounds__(256) k_sort(DevGrid g, DevRays R, unsigned lon
         c = R.cnt[r];
         const int n = c < R.K ? c : R.K;
         double *T = R.t + r * R.K;
-        if (R.pid) {
-            int *I = R.pid + r * R.K;
-            for (int a = 1; a < n; ++a) {
-                const double v = T[a];
-                const int id = I[a];
-                int b = a - 1;
-                while (b >= 0 && T[b] > v) {
-                    T[b + 1] = T[b];
-                    I[b + 1] = I[b];
-                    --b;
-                }
-                T[b + 1] = v;
-                I[b + 1] = id;
-            }
-        } else {
-            for (int a = 1; a < n; ++a) {
-                const double v = T[a];
-                int b = a - 1;
-                while (b >= 0 && T[b] > v) {
-                    T[b + 1] = T[b];
-                    --b;
-                }
-                T[b + 1] = v;
-            }
-        }
+        if (R.pid)
+            sort_depths<true>(T, R.pid + r * R.K, n);
+        else
+            sort_depths<false>(T, nullptr, n);
         const int slot = (int)(r / nper);
         const int phase = slot < R.panels0 ? 0 : 1;
         const int panel = phase == 0 ? slot : slot - R.panels0;
