@@ -96,6 +96,7 @@ struct nsgpu_ctx {
     void *hpin = nullptr;
     size_t hpin_cap = 0;
     DBuf dgather;  // device side of those reads (launch_gather)
+    DBuf aedge;    // per active cube: its twelve edge vertex indices
     bool capturing = false;  // enqueueing into a graph: no stage events
     std::vector<double> cav_size;
     std::vector<uint8_t> cav_filled;
@@ -232,7 +233,7 @@ void nsgpu_destroy(nsgpu_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     DBuf *all[] = {&c->gx, &c->gy, &c->gz, &c->pkind, &c->poff, &c->pcnt, &c->pflags, &c->pbsph, &c->pdata, &c->porder, &c->patom, &c->atoms, &c->rpid, &c->gradv, &c->grada, &c->asums,
                    &c->rcnt, &c->rt, &c->rvol, &c->partial, &c->stats, &c->cand, &c->cov, &c->covst, &c->eps, &c->ideb, &c->status,
-                   &c->bits, &c->crx0, &c->crx1, &c->cpar, &c->crowcnt, &c->croff, &c->nroots, &c->rootlist, &c->bkey, &c->ccount, &c->clabel, &c->cfilled, &c->dcavsize, &c->dcavfilled, &c->hostdone, &c->dres, &c->dnv, &c->dgather,
+                   &c->bits, &c->crx0, &c->crx1, &c->cpar, &c->crowcnt, &c->croff, &c->nroots, &c->rootlist, &c->bkey, &c->ccount, &c->clabel, &c->cfilled, &c->dcavsize, &c->dcavfilled, &c->hostdone, &c->dres, &c->dnv, &c->dgather, &c->aedge,
                    &c->idx, &c->vslot, &c->vhit, &c->vorig, &c->normals, &c->vcount, &c->voffs, &c->scan_scratch, &c->verts, &c->tris, &c->tarea,
                    &c->segmask, &c->anode, &c->acase, &c->anv, &c->ant, &c->avoff, &c->atoff, &c->mverts, &c->mtris,
                    &c->mrr, &c->mraw, &c->field, &c->fbits, &c->batoms, &c->bbox, &c->bcount, &c->boff, &c->bcur,
@@ -996,8 +997,9 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
     launch_mc_compact(c->stream, g, c->bits.as<uint32_t>(), c->segmask.as<uint32_t>(), c->voffs.as<int>(),
                       c->anode.as<int>(), c->acase.as<uint8_t>());
     // 2. per active cube: mid-point vertices added and triangles emitted
+    HIP_TRY(c, c->aedge.ensure(na * 48));
     launch_mc_counts(c->stream, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->idx.as<int>(),
-                     c->anv.as<int>(), c->ant.as<int>());
+                     c->anv.as<int>(), c->ant.as<int>(), c->aedge.as<int>());
     launch_scan(c->stream, c->anv.as<int>(), nact, c->avoff.as<int>(), c->scan_scratch.as<int>());
     launch_scan(c->stream, c->ant.as<int>(), nact, c->atoff.as<int>(), c->scan_scratch.as<int>());
     HIP_TRY(c, hipGetLastError());
@@ -1028,14 +1030,15 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
     }
     // 3. write the mid points, then the triangles that may reference them
     HIP_TRY(c, c->vorig.ensure((size_t)std::max<int64_t>(nadd_cap, 1) * 4));
-    launch_mc_vwrite(c->stream, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->idx.as<int>(),
-                     c->avoff.as<int>(), (int)c->n_analytic, c->verts.as<double>(), c->vslot.as<int>(),
-                     c->vorig.as<int>());
+    launch_mc_vwrite(c->stream, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->aedge.as<int>(),
+                     c->idx.as<int>(), c->avoff.as<int>(), (int)c->n_analytic, c->verts.as<double>(),
+                     c->vslot.as<int>(), c->vorig.as<int>());
     c->nact = nact;
     HIP_TRY(c, c->tris.ensure((size_t)std::max<int64_t>(nt_cap, 1) * 12));
     HIP_TRY(c, c->tarea.ensure((size_t)std::max<int64_t>(nt_cap, 1) * 8));
-    launch_mc_twrite(c->stream, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->idx.as<int>(),
-                     c->atoff.as<int>(), c->verts.as<double>(), c->tris.as<int>(), c->tarea.as<double>());
+    launch_mc_twrite(c->stream, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->aedge.as<int>(),
+                     c->idx.as<int>(), c->atoff.as<int>(), c->verts.as<double>(), c->tris.as<int>(),
+                     c->tarea.as<double>());
     HIP_TRY(c, hipGetLastError());
     const int64_t per = sum_partials_count(nt_cap);
     HIP_TRY(c, c->partial.ensure((size_t)std::max<int64_t>(per, 1) * 8));
@@ -1275,8 +1278,9 @@ int nsgpu_blobby_build(nsgpu_ctx *c, const double *xyzr, int32_t n, double B, do
     HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(std::max<int64_t>(std::max<int64_t>(nseg, nact), 1)) * 4));
     launch_mc_compact(st, g, c->fbits.as<uint32_t>(), c->segmask.as<uint32_t>(), c->voffs.as<int>(), c->anode.as<int>(),
                       c->acase.as<uint8_t>());
+    HIP_TRY(c, c->aedge.ensure(na * 48));
     launch_mc_counts(st, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->idx.as<int>(), c->anv.as<int>(),
-                     c->ant.as<int>());
+                     c->ant.as<int>(), c->aedge.as<int>());
     launch_scan(st, c->anv.as<int>(), nact, c->avoff.as<int>(), c->scan_scratch.as<int>());
     launch_scan(st, c->ant.as<int>(), nact, c->atoff.as<int>(), c->scan_scratch.as<int>());
     HIP_TRY(c, hipGetLastError());
@@ -1293,13 +1297,13 @@ int nsgpu_blobby_build(nsgpu_ctx *c, const double *xyzr, int32_t n, double B, do
     HIP_TRY(c, c->verts.ensure((size_t)std::max<int64_t>(nv, 1) * 24));
     HIP_TRY(c, c->vslot.ensure((size_t)std::max<int64_t>(nv, 1) * 4));
     HIP_TRY(c, c->vorig.ensure((size_t)std::max<int64_t>(nv, 1) * 4));
-    launch_mc_vwrite(st, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->idx.as<int>(), c->avoff.as<int>(),
-                     0, c->verts.as<double>(), c->vslot.as<int>(), c->vorig.as<int>(), a.field, 1.0);
+    launch_mc_vwrite(st, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->aedge.as<int>(), c->idx.as<int>(),
+                     c->avoff.as<int>(), 0, c->verts.as<double>(), c->vslot.as<int>(), c->vorig.as<int>(), a.field, 1.0);
     c->idx_dirty = nv;
     HIP_TRY(c, c->tris.ensure((size_t)std::max<int64_t>(nt, 1) * 12));
     HIP_TRY(c, c->tarea.ensure((size_t)std::max<int64_t>(nt, 1) * 8));
-    launch_mc_twrite(st, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->idx.as<int>(), c->atoff.as<int>(),
-                     c->verts.as<double>(), c->tris.as<int>(), c->tarea.as<double>(), 0);
+    launch_mc_twrite(st, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->aedge.as<int>(), c->idx.as<int>(),
+                     c->atoff.as<int>(), c->verts.as<double>(), c->tris.as<int>(), c->tarea.as<double>(), 0);
     const int64_t per = sum_partials_count(nt);
     HIP_TRY(c, c->partial.ensure((size_t)std::max<int64_t>(per, 1) * 8));
     launch_sum_blocks(st, c->tarea.as<double>(), nt, c->partial.as<double>());

@@ -113,14 +113,16 @@ int64_t mc_num_segments(const DevGrid &);
 void launch_mc_classify(hipStream_t, const DevGrid &, const uint32_t *bits, uint32_t *segmask, int *segcnt);
 void launch_mc_compact(hipStream_t, const DevGrid &, const uint32_t *bits, const uint32_t *segmask, const int *segoff,
                        int *anode, uint8_t *acase);
+// aedge[12 q + e]: the vertex index of edge e of active cube q (-1: none yet),
+// read once here from idx so that the later passes read it coalesced
 void launch_mc_counts(hipStream_t, const DevGrid &, int nact, const int *anode, const uint8_t *acase, const int *idx,
-                      int *nv, int *nt);
+                      int *nv, int *nt, int *aedge);
 // vorig[v - base] = 4 * (active cube) + (0, 1, 2 for edge 5, 6, 10) of each mid point
 // field == nullptr: edge mid points; else vertexInterp at iso on the scalar
 // field (one double per node)
-void launch_mc_vwrite(hipStream_t, const DevGrid &, int nact, const int *anode, const uint8_t *acase, int *idx,
-                      const int *voff, int base, double *verts, int *vslot, int *vorig, const double *field = nullptr,
-                      double iso = 0.);
+void launch_mc_vwrite(hipStream_t, const DevGrid &, int nact, const int *anode, const uint8_t *acase,
+                      const int *aedge, int *idx, const int *voff, int base, double *verts, int *vslot, int *vorig,
+                      const double *field = nullptr, double iso = 0.);
 // vertex normals: analytic at the depths the vertices come from (the ray
 // lists must carry patch ids), mid points from their adjacent triangles
 struct NormalArgs {
@@ -137,8 +139,9 @@ struct NormalArgs {
 void launch_normals(hipStream_t, const DevGrid &, const DevPatches &, const DevRays &, const NormalArgs &);
 // reverse: triangles written (t2, t1, t0) as for the analytic corner bitmap;
 // the scalar-field marching cubes keeps (t0, t1, t2)
-void launch_mc_twrite(hipStream_t, const DevGrid &, int nact, const int *anode, const uint8_t *acase, const int *idx,
-                      const int *toff, const double *verts, int *tris, double *tarea, int reverse = 1);
+void launch_mc_twrite(hipStream_t, const DevGrid &, int nact, const int *anode, const uint8_t *acase,
+                      const int *aedge, const int *idx, const int *toff, const double *verts, int *tris, double *tarea,
+                      int reverse = 1);
 
 // blobby scalar field (BlobbySurface::build)
 struct BlobbyArgs {

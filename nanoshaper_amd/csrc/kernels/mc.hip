@@ -288,17 +288,29 @@ __global__ void __launch_bounds__(256) k_mc_compact(DevGrid g, SegIter si, const
 // an analytic vertex) and triangles it emits (all three vertices present after
 // the mid points are added)
 __global__ void __launch_bounds__(256) k_mc_counts(DevGrid g, int nact, const int *anode, const uint8_t *acase,
-                                                   const int *idx, int *nv, int *nt) {
+                                                   const int *idx, int *nv, int *nt, int *aedge) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nact) return;
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (int64_t)g.nx * g.ny;
     const int64_t f = anode[q];
     const int c = acase[q];
     const int i = (int)(f % g.nx), j = (int)((f / g.nx) % g.ny), k = (int)(f / sz);
+    // the twelve edge slots of an active cube (1 <= i, j, k <= n - 2) are in the grid
+    int ids[12];
+    unsigned have = 0;  // bit e: edge e has a vertex
+#pragma unroll
+    for (int e = 0; e < 12; ++e) {
+        ids[e] = idx[edge_slot(e, f, sy, sz, nvox)];
+        have |= (ids[e] != -1 ? 1u : 0u) << e;
+    }
+    int4 *rec = reinterpret_cast<int4 *>(aedge + 12 * (int64_t)q);
+    rec[0] = make_int4(ids[0], ids[1], ids[2], ids[3]);
+    rec[1] = make_int4(ids[4], ids[5], ids[6], ids[7]);
+    rec[2] = make_int4(ids[8], ids[9], ids[10], ids[11]);
     int v = 0;
-    v += edge_on(c, 5) && idx[edge_slot(5, f, sy, sz, nvox)] == -1;
-    v += edge_on(c, 6) && idx[edge_slot(6, f, sy, sz, nvox)] == -1;
-    v += edge_on(c, 10) && idx[edge_slot(10, f, sy, sz, nvox)] == -1;
+    v += edge_on(c, 5) && ids[5] == -1;
+    v += edge_on(c, 6) && ids[6] == -1;
+    v += edge_on(c, 10) && ids[10] == -1;
     nv[q] = v;
     const uint64_t w = c_tri_cases[c];
     int t = 0;
@@ -308,7 +320,7 @@ __global__ void __launch_bounds__(256) k_mc_counts(DevGrid g, int nact, const in
         if (e0 == 15) break;
         for (int u = 0; u < 3; ++u) {
             const int e = (int)((w >> (12 * tr + 4 * u)) & 15);
-            ok &= idx[edge_slot(e, f, sy, sz, nvox)] != -1 || owner_in_range(e, i, j, k);
+            ok &= ((have >> e) & 1u) || owner_in_range(e, i, j, k);
         }
         t += ok;
     }
@@ -325,8 +337,9 @@ __device__ __forceinline__ void vertex_interp(double iso, const double p1[3], co
 }
 
 __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const int *anode, const uint8_t *acase,
-                                                   int *idx, const int *voff, int base, double *verts, int *vslot,
-                                                   int *vorig, const double *field, double iso) {
+                                                   const int *aedge, int *idx, const int *voff, int base,
+                                                   double *verts, int *vslot, int *vorig, const double *field,
+                                                   double iso) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nact) return;
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (int64_t)g.nx * g.ny;
@@ -337,9 +350,10 @@ __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const in
     // the slots of edges 5, 6, 10 belong to this cube only: no other lane writes them
     const int64_t s5 = edge_slot(5, f, sy, sz, nvox), s6 = edge_slot(6, f, sy, sz, nvox),
                   s10 = edge_slot(10, f, sy, sz, nvox);
-    const bool m5 = edge_on(c, 5) && idx[s5] == -1;
-    const bool m6 = edge_on(c, 6) && idx[s6] == -1;
-    const bool m10 = edge_on(c, 10) && idx[s10] == -1;
+    const int *rec = aedge + 12 * (int64_t)q;
+    const bool m5 = edge_on(c, 5) && rec[5] == -1;
+    const bool m6 = edge_on(c, 6) && rec[6] == -1;
+    const bool m10 = edge_on(c, 10) && rec[10] == -1;
     if (field) {
         // corner positions x_i +- hside (the mulTable corners, tools.h:154-162)
         // and field values of the cube (getVertices :8143-8169)
@@ -394,9 +408,18 @@ __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const in
     }
 }
 
+// Edge vertex indices from the cube's record; an edge still at -1 there may
+// have received a mid point from the cube that owns it (k_mc_vwrite), so
+// only those are looked up in idx again.
+__device__ __forceinline__ int edge_vertex(const int *rec, const int *idx, int e, int64_t f, int64_t sy, int64_t sz,
+                                           int64_t nvox) {
+    const int v = rec[e];
+    return v != -1 ? v : idx[edge_slot(e, f, sy, sz, nvox)];
+}
+
 __global__ void __launch_bounds__(256) k_mc_twrite(DevGrid g, int nact, const int *anode, const uint8_t *acase,
-                                                   const int *idx, const int *toff, const double *verts, int *tris,
-                                                   double *tarea, int reverse) {
+                                                   const int *aedge, const int *idx, const int *toff,
+                                                   const double *verts, int *tris, double *tarea, int reverse) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nact) return;
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (int64_t)g.nx * g.ny;
@@ -407,9 +430,10 @@ __global__ void __launch_bounds__(256) k_mc_twrite(DevGrid g, int nact, const in
         const int e0 = (int)((w >> (12 * t)) & 15), e1 = (int)((w >> (12 * t + 4)) & 15),
                   e2 = (int)((w >> (12 * t + 8)) & 15);
         if (e0 == 15) break;
-        const int v0 = idx[edge_slot(e0, f, sy, sz, nvox)];
-        const int v1 = idx[edge_slot(e1, f, sy, sz, nvox)];
-        const int v2 = idx[edge_slot(e2, f, sy, sz, nvox)];
+        const int *rec = aedge + 12 * (int64_t)q;
+        const int v0 = edge_vertex(rec, idx, e0, f, sy, sz, nvox);
+        const int v1 = edge_vertex(rec, idx, e1, f, sy, sz, nvox);
+        const int v2 = edge_vertex(rec, idx, e2, f, sy, sz, nvox);
         if (v0 == -1 || v1 == -1 || v2 == -1) continue;
         tris[3 * (int64_t)o + 0] = reverse ? v2 : v0;
         tris[3 * (int64_t)o + 1] = v1;
@@ -484,23 +508,25 @@ void launch_mc_compact(hipStream_t s, const DevGrid &g, const uint32_t *bits, co
 }
 
 void launch_mc_counts(hipStream_t s, const DevGrid &g, int nact, const int *anode, const uint8_t *acase,
-                      const int *idx, int *nv, int *nt) {
-    if (nact) hipLaunchKernelGGL(k_mc_counts, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, idx, nv, nt);
+                      const int *idx, int *nv, int *nt, int *aedge) {
+    if (nact)
+        hipLaunchKernelGGL(k_mc_counts, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, idx, nv, nt, aedge);
 }
 
-void launch_mc_vwrite(hipStream_t s, const DevGrid &g, int nact, const int *anode, const uint8_t *acase, int *idx,
-                      const int *voff, int base, double *verts, int *vslot, int *vorig, const double *field,
-                      double iso) {
+void launch_mc_vwrite(hipStream_t s, const DevGrid &g, int nact, const int *anode, const uint8_t *acase,
+                      const int *aedge, int *idx, const int *voff, int base, double *verts, int *vslot, int *vorig,
+                      const double *field, double iso) {
     if (nact)
-        hipLaunchKernelGGL(k_mc_vwrite, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, idx, voff, base,
-                           verts, vslot, vorig, field, iso);
+        hipLaunchKernelGGL(k_mc_vwrite, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, aedge, idx, voff,
+                           base, verts, vslot, vorig, field, iso);
 }
 
 void launch_mc_twrite(hipStream_t s, const DevGrid &g, int nact, const int *anode, const uint8_t *acase,
-                      const int *idx, const int *toff, const double *verts, int *tris, double *tarea, int reverse) {
+                      const int *aedge, const int *idx, const int *toff, const double *verts, int *tris,
+                      double *tarea, int reverse) {
     if (nact)
-        hipLaunchKernelGGL(k_mc_twrite, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, idx, toff, verts,
-                           tris, tarea, reverse);
+        hipLaunchKernelGGL(k_mc_twrite, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, aedge, idx, toff,
+                           verts, tris, tarea, reverse);
 }
 
 }  // namespace nsg
