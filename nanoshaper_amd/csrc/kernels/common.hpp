@@ -75,6 +75,15 @@ __device__ __forceinline__ void push_hit(const DevRays &R, int slot, int64_t ray
     }
 }
 
+// Workgroups are dealt to the eight XCDs (each with its own L2) round robin:
+// block b runs on XCD b % 8.  This maps the blocks of one XCD onto one
+// contiguous range of [0, nb), so that neighbouring work items, which share
+// cache lines, meet in the same L2.  A bijection on [0, nb).
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7, x = b & 7, y = b >> 3;
+    return x < r ? x * (q + 1) + y : r * (q + 1) + (x - r) * q + y;
+}
+
 __device__ __forceinline__ int64_t flat(const DevGrid &g, int i, int j, int k) {
     return ((int64_t)k * g.ny + j) * g.nx + i;
 }

@@ -25,7 +25,7 @@ constexpr int kFillNodes = 4;  // 8 was slower (0.25 vs 0.22 ms at 325^3)
 __global__ void __launch_bounds__(256) k_fill_bits(DevGrid g, DevRays R, FillOpts o, int *epsmap, uint8_t *idebmap,
                                                    int *status) {
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
-    const int64_t f0 = kFillNodes * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    const int64_t f0 = kFillNodes * ((int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
     if (f0 >= nvox) return;
     const int64_t NN = (int64_t)R.N * R.N, SW = (int64_t)R.W * NN;
     const int64_t plane = (int64_t)g.nx * g.ny;

@@ -289,7 +289,7 @@ __global__ void __launch_bounds__(256) k_mc_compact(DevGrid g, SegIter si, const
 // the mid points are added)
 __global__ void __launch_bounds__(256) k_mc_counts(DevGrid g, int nact, const int *anode, const uint8_t *acase,
                                                    const int *idx, int *nv, int *nt, int *aedge) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (q >= nact) return;
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (int64_t)g.nx * g.ny;
     const int64_t f = anode[q];
@@ -340,7 +340,7 @@ __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const in
                                                    const int *aedge, int *idx, const int *voff, int base,
                                                    double *verts, int *vslot, int *vorig, const double *field,
                                                    double iso) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (q >= nact) return;
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (int64_t)g.nx * g.ny;
     const int64_t f = anode[q];
@@ -420,7 +420,7 @@ __device__ __forceinline__ int edge_vertex(const int *rec, const int *idx, int e
 __global__ void __launch_bounds__(256) k_mc_twrite(DevGrid g, int nact, const int *anode, const uint8_t *acase,
                                                    const int *aedge, const int *idx, const int *toff,
                                                    const double *verts, int *tris, double *tarea, int reverse) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (q >= nact) return;
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (int64_t)g.nx * g.ny;
     const int64_t f = anode[q];
