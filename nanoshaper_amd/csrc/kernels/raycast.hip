@@ -260,7 +260,7 @@ constexpr double kNearTangent = 1e-8;
 // near-double roots, which take quartic_roots whole.  A root m close to
 // the m <= 0 or m < 1e-14 thresholds is recomputed by the FP64 closed form so
 // that the root count and formula branch are the reference's.
-__device__ int quartic_roots_mixed(double roots[4], double b, double c, double d, double e) {
+__device__ __forceinline__ int quartic_roots_mixed(double roots[4], double b, double c, double d, double e) {
     const double b_2 = 0.5 * b, b_4 = 0.25 * b;
     double p = c - 1.5 * (b_2 * b_2);
     double q = d + b_2 * (b_2 * b_2 - c);
@@ -282,7 +282,15 @@ __device__ int quartic_roots_mixed(double roots[4], double b, double c, double d
     // reference's FP64 closed form decides these rays
     const double dmag = fabs(4. * f) * (fabs(4. * r * (p * p)) + fabs(p * (q * q)) + 16. * (r * r)) +
                         (q * q) * (fabs(128. * p * r) + 27. * (q * q));
-    if (fabs(delta) <= kNearTangent * dmag) return quartic_roots_fallback(roots, b, c, d, e);
+    if (fabs(delta) <= kNearTangent * dmag) {
+        // the out-of-line FP64 solver writes to an array of its own, so that
+        // roots itself never needs an address and stays in registers
+        double fb[4];
+        const int nf = quartic_roots_fallback(fb, b, c, d, e);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) roots[i] = fb[i];
+        return nf;
+    }
     int num;
     if (delta < 0. || (delta == 0. && (p >= 0. || f < 0.)))
         num = 2;
@@ -315,9 +323,13 @@ __device__ int quartic_roots_mixed(double roots[4], double b, double c, double d
         roots[3] = 0.5 * (((q >= 0.) ? coeff : -coeff) + sqrt(smin));
     }
     if (change_variant) {
-        for (int i = 0; i < num; i++) roots[i] = 1. / (roots[i] - 0.25 * d / e);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if (i < num) roots[i] = 1. / (roots[i] - 0.25 * d / e);
     } else {
-        for (int i = 0; i < num; i++) roots[i] -= 0.25 * b;
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if (i < num) roots[i] -= 0.25 * b;
     }
     return num;
 }
@@ -375,7 +387,9 @@ __device__ __forceinline__ int ray_torus_mixed(const double *td, const double sc
     const int nr = quartic_roots_mixed(roots, B, C, D, E);
     const double t0 = t[0];
     int ni = 0;
-    for (int i = 0; i < nr; i++) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        if (i >= nr) break;
         double rt = roots[i];
         if (rt < 0) continue;
         rt = polish_quartic_root(rt, B, C, D, E);
