@@ -901,10 +901,12 @@ static int assemble(nsgpu_ctx *c, const int eslots[3]) {
     HIP_TRY(c, c->dnv.ensure(8));
     launch_scan_total(c->stream, c->vcount.as<int>(), c->voffs.as<int>(), nrays, c->dnv.as<int>());
     HIP_TRY(c, hipGetLastError());
-    // Every vertex comes from a depth of an edge ray (at most one per depth),
-    // so the getSurf hit count bounds the total: the buffers are sized by it
-    // and the count itself is read later, with the next read the host makes.
-    const int64_t cap = std::max<int64_t>(c->res.total_hits, 1);
+    // Every vertex comes from a depth of an edge ray (at most one per depth,
+    // at most K per ray), so the edge rays' list capacity bounds the total:
+    // the buffers are sized by it, the pass can be enqueued before getSurf's
+    // result block is read, and the count itself is read later, with the
+    // next read the host makes.
+    const int64_t cap = std::max<int64_t>(3 * (int64_t)R.N * R.N * R.K, 1);
     HIP_TRY(c, c->verts.ensure((size_t)cap * 24));
     HIP_TRY(c, c->vslot.ensure((size_t)cap * 4));
     HIP_TRY(c, c->vhit.ensure((size_t)cap * 8));
@@ -924,6 +926,8 @@ int nsgpu_get_surf(nsgpu_ctx *c, const nsgpu_surf_opts *o, nsgpu_surf_result *re
     HIP_TRY(c, hipSetDevice(c->device));
     // the edge rays' depths keep their patch ids for the vertex normals
     c->record_pid = o->accurate_triangulation != 0 && c->want_normals;
+    const FillOpts fo = fill_opts(c);
+    const int es[3] = {fo.eslot_x, fo.eslot_z, fo.eslot_y};
     for (int attempt = 0;; ++attempt) {
         if (attempt == 4) {
             c->record_pid = false;
@@ -934,6 +938,17 @@ int nsgpu_get_surf(nsgpu_ctx *c, const nsgpu_surf_opts *o, nsgpu_surf_result *re
             c->record_pid = false;
             return rc;
         }
+        // the vertex pass goes into the same stream before the result block
+        // is read: a repeated attempt clears the slots it wrote
+        mark(c, ST_VERTS, 0, c->stream);
+        if (o->accurate_triangulation) {
+            rc = assemble(c, es);
+            if (rc) {
+                c->record_pid = false;
+                return rc;
+            }
+        }
+        mark(c, ST_VERTS, 1, c->stream);
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         if (!regrow(c)) break;
     }
@@ -941,15 +956,15 @@ int nsgpu_get_surf(nsgpu_ctx *c, const nsgpu_surf_opts *o, nsgpu_surf_result *re
     c->record_pid = false;
     rc = finish_surf(c);
     if (rc) return rc;
-    mark(c, ST_VERTS, 0, c->stream);
     if (o->accurate_triangulation) {
-        const FillOpts fo = fill_opts(c);
-        const int es[3] = {fo.eslot_x, fo.eslot_z, fo.eslot_y};
-        rc = assemble(c, es);
-        if (rc) return rc;
+        // cavities numbered on the host (more components than the device
+        // replay holds) changed the corner bitmap after the vertex pass ran
+        if (o->fill_cavities && o->build_status && c->hres->cav_info()[0] == 0) {
+            rc = assemble(c, es);
+            if (rc) return rc;
+        }
         c->have_vertices = true;
     }
-    mark(c, ST_VERTS, 1, c->stream);
     c->have_surf = true;
     if (res) *res = c->res;
     return NSGPU_OK;

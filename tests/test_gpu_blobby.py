@@ -32,7 +32,7 @@ def _blobby(n, scale, seed, perfil=80.0, cutoff=6.0, B=-2.5):
     return grid, dev, info, ref
 
 
-def _compare_surface(grid, dev, orc, opts):
+def _compare_surface(grid, dev, orc, opts, area_rel=1e-12):
     r = dev.get_surf(**opts)
     ro = orc.get_surf(**opts)
     for k in ("total_rays", "odd_rays", "total_hits", "candidates", "num_cavities", "max_hits_per_ray"):
@@ -52,7 +52,7 @@ def _compare_surface(grid, dev, orc, opts):
     assert np.array_equal(dev.corner_bits(), orc.corner_bits())
     m, mo = dev.triangulate(), orc.triangulate()
     assert (m["num_vertices"], m["num_triangles"]) == (mo["num_vertices"], mo["num_triangles"])
-    assert m["area"] == pytest.approx(mo["area"], rel=1e-12)
+    assert m["area"] == pytest.approx(mo["area"], rel=area_rel)
     va, ta = dev.mesh()
     vb, tb = orc.mesh()
     assert np.array_equal(ta, tb) and np.array_equal(va, vb)
