@@ -465,8 +465,14 @@ static int enqueue_trace(nsgpu_ctx *c, hipStream_t st) {
     HIP_TRY(c, c->cand.ensure((size_t)irows * 8));
     HIP_TRY(c, c->stats.ensure((size_t)srows * 3 * 8));
     unsigned long long *spart = c->stats.as<unsigned long long>(), *sout = c->dres.as<unsigned long long>();
-    launch_fill32(st, c->rcnt.p, 0u, nrays);
-    launch_fill32(st, c->cand.p, 0u, irows * 2);
+    // one launch clears the ray counts, the candidate partials and the result
+    // words the column reductions add into (hits, odd rays, longest, candidates)
+    FillList fl{};
+    fl.s[0] = FillSeg{c->rcnt.as<uint32_t>(), 0u, nrays};
+    fl.s[1] = FillSeg{c->cand.as<uint32_t>(), 0u, irows * 2};
+    fl.s[2] = FillSeg{reinterpret_cast<uint32_t *>(sout + RES_HITS), 0u, 2 * (RES_CAND + 1 - RES_HITS)};
+    fl.k = 3;
+    launch_fill_multi(st, fl);
     mark(c, ST_INTERSECT, 0, st);
     if (c->surface_is_mesh)
         launch_intersect_mesh(st, g, R, mesh_args(c), c->cand.as<unsigned long long>());
@@ -479,8 +485,8 @@ static int enqueue_trace(nsgpu_ctx *c, hipStream_t st) {
     mark(c, ST_INTERSECT, 1, st);
     mark(c, ST_SORT, 0, st);
     launch_sort(st, g, R, spart);
-    launch_reduce_cols(st, spart, srows, 3, 1u << 2, sout + RES_HITS);
-    launch_reduce_cols(st, c->cand.as<unsigned long long>(), irows, 1, 0u, sout + RES_CAND);
+    launch_reduce_cols(st, spart, srows, 3, 1u << 2, sout + RES_HITS, false);
+    launch_reduce_cols(st, c->cand.as<unsigned long long>(), irows, 1, 0u, sout + RES_CAND, false);
     HIP_TRY(c, hipGetLastError());
     mark(c, ST_SORT, 1, st);
     return NSGPU_OK;
@@ -551,10 +557,9 @@ static int enqueue_cavities(nsgpu_ctx *c, hipStream_t st) {
     HIP_TRY(c, c->cfilled.ensure((size_t)kReplayMax));
     HIP_TRY(c, c->dcavsize.ensure((size_t)kReplayMax * 8));
     HIP_TRY(c, c->dcavfilled.ensure((size_t)kReplayMax));
-    launch_fill32(st, c->nroots.p, 0u, 1);
     const CclRuns runs = ccl_runs(c);
-    launch_ccl(st, g, dev_rays(c), runs, c->scan_scratch.as<int>(), c->nroots.as<int>(), c->rootlist.as<int>());
-    launch_ccl_clear_keys(st, c->nroots.as<int>(), c->bkey.as<unsigned int>(), c->ccount.as<int>(), kReplayMax);
+    launch_ccl(st, g, dev_rays(c), runs, c->scan_scratch.as<int>(), c->nroots.as<int>(), c->rootlist.as<int>(),
+               c->bkey.as<unsigned int>(), c->ccount.as<int>(), kReplayMax);
     launch_ccl_keys(st, g, runs, c->bkey.as<unsigned int>(), c->ccount.as<int>(), kReplayMax);
     CavityReplay r;
     r.nroots = c->nroots.as<int>();
@@ -680,6 +685,7 @@ static FillOpts fill_opts(const nsgpu_ctx *c) {
     fo.eslot_z = c->panels[0] + 1;
     fo.eslot_y = c->panels[0] + 2;
     fo.out_code = (o->fill_cavities && o->build_status) ? NSGPU_STATUS_OUT : NSGPU_STATUS_TEMP_OUT;
+    fo.init_bits = nullptr;
     return fo;
 }
 
@@ -718,12 +724,14 @@ static int enqueue_surf(nsgpu_ctx *c, hipStream_t st) {
     const nsgpu_surf_opts *o = &c->opts;
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
     const int64_t nwords = (nvox + 31) / 32;
-    const FillOpts fo = fill_opts(c);
+    FillOpts fo = fill_opts(c);
     // maps (buffers always exist so that later stages can index them)
     HIP_TRY(c, c->eps.ensure((size_t)nvox * 12));
     HIP_TRY(c, c->ideb.ensure((size_t)nvox));
     HIP_TRY(c, c->status.ensure((size_t)nvox * 4));
     HIP_TRY(c, c->bits.ensure((size_t)nwords * 4));
+    // the fill's lanes set the corner bitmap to all outside on their way
+    if (o->accurate_triangulation && c->panels[0] > 0) fo.init_bits = c->bits.as<uint32_t>();
     mark(c, ST_FILL, 0, st);
     if (c->panels[0] > 0)
         launch_fill_bits(st, g, R, fo, c->eps.as<int>(), c->ideb.as<uint8_t>(), c->status.as<int>());
@@ -731,7 +739,7 @@ static int enqueue_surf(nsgpu_ctx *c, hipStream_t st) {
     mark(c, ST_FILL, 1, st);
     mark(c, ST_CORNERS, 0, st);
     if (o->accurate_triangulation) {
-        launch_fill32(st, c->bits.p, 0xffffffffu, nwords);
+        if (!fo.init_bits) launch_fill32(st, c->bits.p, 0xffffffffu, nwords);
         launch_corners_bits(st, g, R, fo, c->bits.as<uint32_t>());
     }
     HIP_TRY(c, hipGetLastError());
@@ -897,9 +905,9 @@ static int assemble(nsgpu_ctx *c, const int eslots[3]) {
     c->idx_nvox = nvox;
     c->idx_dirty = 0;
     launch_vtx_count(c->stream, g, R, eslots, c->bits.as<uint32_t>(), c->vcount.as<int>());
-    launch_scan(c->stream, c->vcount.as<int>(), nrays, c->voffs.as<int>(), c->scan_scratch.as<int>());
     HIP_TRY(c, c->dnv.ensure(8));
-    launch_scan_total(c->stream, c->vcount.as<int>(), c->voffs.as<int>(), nrays, c->dnv.as<int>());
+    launch_scan(c->stream, c->vcount.as<int>(), nrays, c->voffs.as<int>(), c->scan_scratch.as<int>(),
+                c->dnv.as<int>());
     HIP_TRY(c, hipGetLastError());
     // Every vertex comes from a depth of an edge ray (at most one per depth,
     // at most K per ray), so the edge rays' list capacity bounds the total:

@@ -61,9 +61,11 @@ __device__ void unite(int *P, int a, int b) {
 // plane is set), so run starts and ends come from 64-node words with bit
 // operations and the status map itself is not read.  The k-th start and the
 // k-th end of a row belong to the same run.
+// Also zeroes *nroots for k_ccl_roots.
 __global__ void __launch_bounds__(256) k_ccl_runs(DevGrid g, const unsigned long long *inside, int N, int64_t rs,
-                                                  int *rx0, int *rx1, int *rowcnt) {
+                                                  int *rx0, int *rx1, int *rowcnt, int *nroots) {
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row == 0) *nroots = 0;
     if (row >= (int64_t)g.ny * g.nz) return;
     const int j = (int)(row % g.ny), k = (int)(row / g.ny);
     const int64_t NN = (int64_t)N * N, ray = (int64_t)k * N + j;
@@ -242,8 +244,10 @@ __global__ void __launch_bounds__(256) k_ccl_flatten(int64_t rows, const int *ro
 
 // roots get a compact slot, stored in place as -(slot+2); rootlist holds the
 // root's first node (the component's smallest flat index)
+// and the keys of slots below keycap start cleared (bkey all ones, count 0)
 __global__ void __launch_bounds__(256) k_ccl_roots(DevGrid g, int64_t rs, const int *rx0, const int *rowcnt,
-                                                   const int *roff, int *R, int *nroots, int *rootlist) {
+                                                   const int *roff, int *R, int *nroots, int *rootlist,
+                                                   unsigned int *bkey, int *count, int keycap) {
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= (int64_t)g.ny * g.nz) return;
     const int n = rowcnt[row], c0 = roff[row];
@@ -252,6 +256,10 @@ __global__ void __launch_bounds__(256) k_ccl_roots(DevGrid g, int64_t rs, const 
         const int s = atomicAdd(nroots, 1);
         rootlist[s] = (int)(row * g.nx + rx0[row + rs * q]);
         R[c0 + q] = -(s + 2);
+        if (s < keycap) {
+            bkey[s] = 0xffffffffu;
+            count[s] = 0;
+        }
     }
 }
 
@@ -339,9 +347,10 @@ static inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 int ccl_row_capacity(const DevGrid &g) { return (g.nx + 1) / 2; }
 
 void launch_ccl(hipStream_t s, const DevGrid &g, const DevRays &R, const CclRuns &c, int *scan_scratch, int *nroots,
-                int *rootlist) {
+                int *rootlist, unsigned int *bkey, int *count, int keycap) {
     const int64_t rows = (int64_t)g.ny * g.nz;
-    hipLaunchKernelGGL(k_ccl_runs, dim3(nblk(rows)), dim3(256), 0, s, g, R.cov_st, R.N, rows, c.rx0, c.rx1, c.rowcnt);
+    hipLaunchKernelGGL(k_ccl_runs, dim3(nblk(rows)), dim3(256), 0, s, g, R.cov_st, R.N, rows, c.rx0, c.rx1, c.rowcnt,
+                       nroots);
     launch_scan(s, c.rowcnt, rows, c.roff, scan_scratch);
     hipLaunchKernelGGL(k_ccl_plane, dim3((unsigned)g.nz), dim3(1024), 0, s, g, rows, c.rx0, c.rx1, c.rowcnt, c.roff,
                        c.R);
@@ -350,7 +359,7 @@ void launch_ccl(hipStream_t s, const DevGrid &g, const DevRays &R, const CclRuns
                            c.rowcnt, c.roff, c.R);
     hipLaunchKernelGGL(k_ccl_flatten, dim3(nblk(rows)), dim3(256), 0, s, rows, c.rowcnt, c.roff, c.R);
     hipLaunchKernelGGL(k_ccl_roots, dim3(nblk(rows)), dim3(256), 0, s, g, rows, c.rx0, c.rowcnt, c.roff, c.R, nroots,
-                       rootlist);
+                       rootlist, bkey, count, keycap);
 }
 
 void launch_ccl_keys(hipStream_t s, const DevGrid &g, const CclRuns &c, unsigned int *bkey, int *count,

@@ -27,6 +27,7 @@ __global__ void __launch_bounds__(256) k_fill_bits(DevGrid g, DevRays R, FillOpt
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
     const int64_t f0 = kFillNodes * ((int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
     if (f0 >= nvox) return;
+    if (o.init_bits && (f0 & 31) == 0) o.init_bits[f0 >> 5] = 0xffffffffu;  // word f0 / 32, < ceil(nvox / 32)
     const int64_t NN = (int64_t)R.N * R.N, SW = (int64_t)R.W * NN;
     const int64_t plane = (int64_t)g.nx * g.ny;
     int k = (int)(f0 / plane);

@@ -13,6 +13,9 @@ struct FillOpts {
     // index, is numbered NSGPU_STATUS_OUT, and only the other components'
     // nodes are relabelled), else NSGPU_STATUS_TEMP_OUT
     int out_code;
+    // when set, the fill also sets every word of this corner bitmap to all
+    // ones (outside), ahead of k_corners_bits
+    uint32_t *init_bits;
 };
 
 constexpr int kPathCounters = 8;
@@ -27,8 +30,9 @@ void launch_intersect_count(hipStream_t, const DevGrid &, const DevPatches &, co
 // skin cells: order[n - n_skin, n), same per-workgroup candidate rows
 void launch_intersect_skin(hipStream_t, const DevGrid &, const DevPatches &, const DevRays &, unsigned long long *part);
 void launch_sort(hipStream_t, const DevGrid &, const DevRays &, unsigned long long *part);
+// zero_out = false: the caller cleared out[0 .. ncols) already
 void launch_reduce_cols(hipStream_t, const unsigned long long *in, int64_t nrows, int ncols, unsigned int max_mask,
-                        unsigned long long *out);
+                        unsigned long long *out, bool zero_out = true);
 void launch_fill_bits(hipStream_t, const DevGrid &, const DevRays &, const FillOpts &, int *, uint8_t *, int *);
 // bits must start at all ones
 void launch_corners_bits(hipStream_t, const DevGrid &, const DevRays &, const FillOpts &, uint32_t *);
@@ -43,9 +47,10 @@ struct CclRuns {
     int *rx0, *rx1, *R, *rowcnt, *roff;
 };
 int ccl_row_capacity(const DevGrid &);
-// runs from the x grid rays' status plane (R.cov_st); scan_scratch: scan_scratch_ints(rows) ints
+// runs from the x grid rays' status plane (R.cov_st); scan_scratch: scan_scratch_ints(rows) ints;
+// nroots is zeroed here; the keys of root slots below keycap start cleared
 void launch_ccl(hipStream_t, const DevGrid &, const DevRays &R, const CclRuns &, int *scan_scratch, int *nroots,
-                int *rootlist);
+                int *rootlist, unsigned int *bkey, int *count, int keycap);
 // components with slot >= capacity are skipped
 void launch_ccl_keys(hipStream_t, const DevGrid &, const CclRuns &, unsigned int *bkey, int *count, int capacity);
 // done: the replay's info[0]; the pass does nothing when it is 0
@@ -64,6 +69,17 @@ int64_t adjoint_blocks(const DevRays &);
 void launch_adjoint(hipStream_t, const DevGrid &, const DevPatches &, const DevRays &, const AdjointArgs &);
 // fills (graph-safe clears) and the volume reduction
 void launch_fill32(hipStream_t, void *p, uint32_t v, int64_t n_words);
+// up to four fills (32-bit words) in one launch
+struct FillSeg {
+    uint32_t *p;
+    uint32_t v;
+    int64_t n;
+};
+struct FillList {
+    FillSeg s[4];
+    int k;
+};
+void launch_fill_multi(hipStream_t, const FillList &);
 struct VolumeSlots {
     int n, flag_sum;
     int slot[6];
@@ -90,8 +106,8 @@ struct CavityReplay {
 void launch_ccl_clear_keys(hipStream_t, const int *nroots, unsigned int *bkey, int *count, int64_t capacity);
 void launch_ccl_replay(hipStream_t, const CavityReplay &);
 int64_t scan_scratch_ints(int64_t n);
-void launch_scan(hipStream_t, const int *, int64_t, int *, int *);
-void launch_scan_total(hipStream_t, const int *counts, const int *offs, int64_t n, int *out);
+// total (optional): total[0] = sum of the n inputs, total[1] = 0
+void launch_scan(hipStream_t, const int *, int64_t, int *, int *, int *total = nullptr);
 // up to 8 device ints and nd doubles gathered into one device buffer
 // (ints at dst[0..8), doubles from dst + 8), for a single copy to the host
 struct GatherSrc {

@@ -1100,8 +1100,8 @@ void launch_sort(hipStream_t s, const DevGrid &g, const DevRays &R, unsigned lon
 }
 
 void launch_reduce_cols(hipStream_t s, const unsigned long long *in, int64_t nrows, int ncols, unsigned int max_mask,
-                        unsigned long long *out) {
-    launch_fill32(s, out, 0u, 2 * (int64_t)ncols);
+                        unsigned long long *out, bool zero_out) {
+    if (zero_out) launch_fill32(s, out, 0u, 2 * (int64_t)ncols);
     const int64_t parts = std::min<int64_t>(std::max<int64_t>((nrows + 16383) / 16384, 1), 256);
     hipLaunchKernelGGL(k_reduce_cols, dim3((unsigned)ncols, (unsigned)parts), dim3(1024), 0, s, in, nrows, ncols,
                        max_mask, out);

@@ -1,6 +1,7 @@
-// Exclusive prefix sum of int32 counts (reduce, scan of block sums, add),
-// used for the ordered compaction of vertices and triangles.  Blocks of 256
-// lanes cover 1024 items; block sums are scanned recursively.
+// Exclusive prefix sum of int32 counts (block sums, then each block adds the
+// sums before it), used for the ordered compaction of vertices, cubes and
+// triangles.  Blocks of 256 lanes cover 1024 items; beyond 4096 blocks the
+// block sums are scanned recursively.
 #include "launch.hpp"
 
 namespace nsg {
@@ -35,7 +36,9 @@ __global__ void __launch_bounds__(256) k_scan_reduce(const int *in, int64_t n, i
     if (threadIdx.x == 0) bsum[blockIdx.x] = total;
 }
 
-__global__ void __launch_bounds__(256) k_scan_down(const int *in, int64_t n, const int *boff, int *out) {
+// total (optional): total[0] = the sum of all n inputs, total[1] = 0, written
+// by the thread of the last element
+__global__ void __launch_bounds__(256) k_scan_down(const int *in, int64_t n, const int *boff, int *out, int *total) {
     __shared__ int sh[4];
     const int64_t base = (int64_t)blockIdx.x * kScanItems + threadIdx.x * 4;
     int v[4], s = 0;
@@ -43,11 +46,15 @@ __global__ void __launch_bounds__(256) k_scan_down(const int *in, int64_t n, con
         v[q] = (base + q < n) ? in[base + q] : 0;
         s += v[q];
     }
-    int total;
-    int run = block_excl_scan(s, sh, total) + (boff ? boff[blockIdx.x] : 0);
+    int btotal;
+    int run = block_excl_scan(s, sh, btotal) + (boff ? boff[blockIdx.x] : 0);
     for (int q = 0; q < 4; ++q) {
         if (base + q < n) out[base + q] = run;
         run += v[q];
+        if (total && base + q == n - 1) {
+            total[0] = run;
+            total[1] = 0;
+        }
     }
 }
 
@@ -56,7 +63,8 @@ __global__ void __launch_bounds__(256) k_scan_down(const int *in, int64_t n, con
 // sums as a launch of its own.
 constexpr int64_t kFusedBlocks = 4096;
 
-__global__ void __launch_bounds__(256) k_scan_down_fused(const int *in, int64_t n, const int *bsum, int *out) {
+__global__ void __launch_bounds__(256) k_scan_down_fused(const int *in, int64_t n, const int *bsum, int *out,
+                                                         int *total) {
     __shared__ int sh[4], red[4];
     int pre = 0;
     for (int q = threadIdx.x; q < (int)blockIdx.x; q += 256) pre += bsum[q];
@@ -70,16 +78,19 @@ __global__ void __launch_bounds__(256) k_scan_down_fused(const int *in, int64_t 
         v[q] = (base + q < n) ? in[base + q] : 0;
         sum += v[q];
     }
-    int total;
-    int run = block_excl_scan(sum, sh, total) + boff;
+    int btotal;
+    int run = block_excl_scan(sum, sh, btotal) + boff;
     for (int q = 0; q < 4; ++q) {
         if (base + q < n) out[base + q] = run;
         run += v[q];
+        if (total && base + q == n - 1) {
+            total[0] = run;
+            total[1] = 0;
+        }
     }
 }
 
-// scratch must hold scan_scratch_ints(n) ints; out[n] receives the total when
-// out has n+1 entries (with_total)
+// scratch must hold scan_scratch_ints(n) ints
 int64_t scan_scratch_ints(int64_t n) {
     int64_t s = 0;
     while (n > kScanItems) {
@@ -89,33 +100,25 @@ int64_t scan_scratch_ints(int64_t n) {
     return s + 2;
 }
 
-void launch_scan(hipStream_t st, const int *in, int64_t n, int *out, int *scratch) {
-    if (n <= 0) return;
+void launch_scan(hipStream_t st, const int *in, int64_t n, int *out, int *scratch, int *total) {
+    if (n <= 0) {
+        if (total) launch_fill32(st, total, 0u, 2);
+        return;
+    }
     const int64_t nb = (n + kScanItems - 1) / kScanItems;
     if (nb == 1) {
-        hipLaunchKernelGGL(k_scan_down, dim3(1), dim3(256), 0, st, in, n, (const int *)nullptr, out);
+        hipLaunchKernelGGL(k_scan_down, dim3(1), dim3(256), 0, st, in, n, (const int *)nullptr, out, total);
         return;
     }
     int *bsum = scratch, *boff = scratch + nb;
     hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(256), 0, st, in, n, bsum);
     if (nb <= kFusedBlocks) {
-        hipLaunchKernelGGL(k_scan_down_fused, dim3((unsigned)nb), dim3(256), 0, st, in, n, (const int *)bsum, out);
+        hipLaunchKernelGGL(k_scan_down_fused, dim3((unsigned)nb), dim3(256), 0, st, in, n, (const int *)bsum, out,
+                           total);
         return;
     }
     launch_scan(st, bsum, nb, boff, scratch + 2 * nb);
-    hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(256), 0, st, in, n, boff, out);
-}
-
-// out[0] = counts[n-1] + offs[n-1] (the total of an exclusive scan), out[1] = 0:
-// the total stays on the device for a later batched read
-__global__ void k_scan_total(const int *counts, const int *offs, int64_t n, int *out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    out[0] = n > 0 ? counts[n - 1] + offs[n - 1] : 0;
-    out[1] = 0;
-}
-
-void launch_scan_total(hipStream_t st, const int *counts, const int *offs, int64_t n, int *out) {
-    hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, st, counts, offs, n, out);
+    hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(256), 0, st, in, n, boff, out, total);
 }
 
 // dst[0..n) = *src[q] (the scalars the host reads between stages), then

@@ -12,6 +12,22 @@ __global__ void __launch_bounds__(256) k_fill32(uint32_t *p, uint32_t v, int64_t
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
+// up to four fills in one launch: the words of the segments in turn
+__global__ void __launch_bounds__(256) k_fill_multi(FillList f) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int q = 0; q < f.k; ++q)
+        for (int64_t i = t0; i < f.s[q].n; i += stride) f.s[q].p[i] = f.s[q].v;
+}
+
+void launch_fill_multi(hipStream_t s, const FillList &f) {
+    int64_t n = 0;
+    for (int q = 0; q < f.k; ++q) n = std::max(n, f.s[q].n);
+    if (n <= 0) return;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_multi, dim3((unsigned)blocks), dim3(256), 0, s, f);
+}
+
 void launch_fill32(hipStream_t s, void *p, uint32_t v, int64_t n) {
     if (n <= 0) return;
     const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
