@@ -211,7 +211,9 @@ int nsgpu_set_patches(nsgpu_ctx *ctx, const nsgpu_patches *p, double probe_radiu
  * corner bitmap and vertex records, cavity labelling and conditional filling
  * (getCavitiesWithBilevelStatusMap Surface.cpp:1994-2126, fillCavities
  * :2355-2487), analytic vertex assembly (assembleVerticesList :7415-7660).
- * Maps stay on the device until read back. */
+ * Maps stay on the device until read back.  Returns once *res is complete;
+ * the vertex assembly may still be running on the context's stream, and every
+ * later call on the context is ordered after it. */
 int nsgpu_get_surf(nsgpu_ctx *ctx, const nsgpu_surf_opts *opts, nsgpu_surf_result *res);
 
 /* Surface::triangulateSurface(outputMesh, buildAtomsMap, iso=0) (Surface.cpp:9909-10520):
