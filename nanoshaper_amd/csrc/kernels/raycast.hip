@@ -167,22 +167,35 @@ __device__ __forceinline__ bool in_clip_sphere(const double *q, const double p[3
     return dd < 0;
 }
 
-__device__ bool feasible_plain(int kind, const double *d, int cnt, const double p[3]) {
+// Point cells (outside every clipping sphere) and facets (inside every
+// plane), for both depths of a sphere or facet ray at once: one pass over the
+// items tests the two points, so each item is loaded once, and a lane stops
+// only when both points have failed.  Each result is the AND of its own
+// tests.
+__device__ __forceinline__ void feasible_plain2(int kind, const double *d, int cnt, const double p[3],
+                                                const double q[3], bool &ok_p, bool &ok_q) {
+    bool out_p = false, out_q = false;
     if (kind == NSGPU_PATCH_POINT) {
         for (int i = 0; i < cnt; i += 4) {
             const int j1 = min(i + 1, cnt - 1), j2 = min(i + 2, cnt - 1), j3 = min(i + 3, cnt - 1);
-            if (in_clip_sphere(d + 4 * i, p) | in_clip_sphere(d + 4 * j1, p) | in_clip_sphere(d + 4 * j2, p) |
-                in_clip_sphere(d + 4 * j3, p))
-                return false;
+            out_p |= in_clip_sphere(d + 4 * i, p) | in_clip_sphere(d + 4 * j1, p) | in_clip_sphere(d + 4 * j2, p) |
+                     in_clip_sphere(d + 4 * j3, p);
+            out_q |= in_clip_sphere(d + 4 * i, q) | in_clip_sphere(d + 4 * j1, q) | in_clip_sphere(d + 4 * j2, q) |
+                     in_clip_sphere(d + 4 * j3, q);
+            if (out_p & out_q) break;
         }
-        return true;
+    } else {
+        for (int i = 0; i < cnt; i += 4) {
+            const int j1 = min(i + 1, cnt - 1), j2 = min(i + 2, cnt - 1), j3 = min(i + 3, cnt - 1);
+            out_p |= plane_out(d + 4 * i, p) | plane_out(d + 4 * j1, p) | plane_out(d + 4 * j2, p) |
+                     plane_out(d + 4 * j3, p);
+            out_q |= plane_out(d + 4 * i, q) | plane_out(d + 4 * j1, q) | plane_out(d + 4 * j2, q) |
+                     plane_out(d + 4 * j3, q);
+            if (out_p & out_q) break;
+        }
     }
-    for (int i = 0; i < cnt; i += 4) {
-        const int j1 = min(i + 1, cnt - 1), j2 = min(i + 2, cnt - 1), j3 = min(i + 3, cnt - 1);
-        if (plane_out(d + 4 * i, p) | plane_out(d + 4 * j1, p) | plane_out(d + 4 * j2, p) | plane_out(d + 4 * j3, p))
-            return false;
-    }
-    return true;
+    ok_p = !out_p;
+    ok_q = !out_q;
 }
 
 // torus: outside the self-intersection sphere, then any visible plane pair
@@ -736,15 +749,32 @@ __global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRa
                     ni = ray_torus(pd, sc, probe, radius, vc, pa, ray_dir, t);
                 if (kCount) ++pc[PC_TORUS];
             }
+            if (!kTorus) {
+                // the sphere's two depths, tested together
+                double ip0[3] = {pa[0], pa[1], pa[2]}, ip1[3] = {pa[0], pa[1], pa[2]};
+                ip0[vc] += t[0] * ray_dir;
+                ip1[vc] += t[1] * ray_dir;
+                bool ok[2];
+                feasible_plain2(kind, pd, pcnt, ip0, ip1, ok[0], ok[1]);
+                for (int i = 0; i < 2; i++) {
+                    if (kCount) {
+                        ++pc[PC_ROOTS];
+                        ++pc[PC_FEAS_CALLS];
+                        pc[PC_FEAS_ITEMS] += pcnt;
+                        pc[PC_HITS] += ok[i];
+                    }
+                    if (!kCount && ok[i]) push_hit(R, slot, (int64_t)n * R.N + m, t[i], it);
+                }
+                return;
+            }
             for (int i = 0; i < ni; i++) {
                 double ip[3] = {pa[0], pa[1], pa[2]};
                 ip[vc] += t[i] * ray_dir;
-                const bool ok =
-                    kTorus ? feasible_torus(kind, pd, pcnt, pflags, ip) : feasible_plain(kind, pd, pcnt, ip);
+                const bool ok = feasible_torus(kind, pd, pcnt, pflags, ip);
                 if (kCount) {
                     ++pc[PC_ROOTS];
                     ++pc[PC_FEAS_CALLS];
-                    pc[PC_FEAS_ITEMS] += kTorus ? 2 * pcnt + 1 : pcnt;
+                    pc[PC_FEAS_ITEMS] += 2 * pcnt + 1;
                     pc[PC_HITS] += ok;
                 }
                 if (!ok) continue;
