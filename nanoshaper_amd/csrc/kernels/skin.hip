@@ -40,12 +40,18 @@ __device__ __forceinline__ bool ray_quadric(const double *Q, const double o[3], 
     return true;
 }
 
-__device__ __forceinline__ bool skin_feasible(const double *pl, int cnt, const double p[3]) {
-    for (int q = 0; q < cnt; ++q) {
+// both roots of a ray against the cell's planes (isFeasible for each): one
+// pass loads each plane once; a lane stops when both points have failed
+__device__ __forceinline__ void skin_feasible2(const double *pl, int cnt, const double p[3], const double r[3],
+                                               bool &ok_p, bool &ok_r) {
+    bool out_p = false, out_r = false;
+    for (int q = 0; q < cnt && !(out_p && out_r); ++q) {
         const double *w = pl + 4 * q;
-        if (w[0] * p[0] + w[1] * p[1] + w[2] * p[2] + w[3] > 0) return false;
+        out_p |= w[0] * p[0] + w[1] * p[1] + w[2] * p[2] + w[3] > 0;
+        out_r |= w[0] * r[0] + w[1] * r[1] + w[2] * r[2] + w[3] > 0;
     }
-    return true;
+    ok_p = !out_p;
+    ok_r = !out_r;
 }
 
 // A cell's rectangle for one panel slot.
@@ -118,12 +124,13 @@ __device__ __forceinline__ unsigned int skin_rect_rays(const DevGrid &g, const D
         }
         double t[2];
         if (!ray_quadric(pd + 6, pa, A::vc, t)) continue;
-        for (int i = 0; i < 2; ++i) {
-            double ip[3] = {pa[0], pa[1], pa[2]};
-            ip[A::vc] += t[i];
-            if (!skin_feasible(pd + NSGPU_SKIN_HDR, c.cnt, ip)) continue;
-            push_hit(R, slot, (int64_t)n * R.N + m, t[i] / ray_dir, c.it);
-        }
+        double ip0[3] = {pa[0], pa[1], pa[2]}, ip1[3] = {pa[0], pa[1], pa[2]};
+        ip0[A::vc] += t[0];
+        ip1[A::vc] += t[1];
+        bool ok0, ok1;
+        skin_feasible2(pd + NSGPU_SKIN_HDR, c.cnt, ip0, ip1, ok0, ok1);
+        if (ok0) push_hit(R, slot, (int64_t)n * R.N + m, t[0] / ray_dir, c.it);
+        if (ok1) push_hit(R, slot, (int64_t)n * R.N + m, t[1] / ray_dir, c.it);
     }
     return cand;
 }
