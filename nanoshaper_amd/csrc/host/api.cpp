@@ -87,7 +87,7 @@ struct nsgpu_ctx {
     DBuf eps, ideb, status, bits;
     // cavities
     DBuf crx0, crx1, cpar, crowcnt, croff, nroots, rootlist, bkey, ccount, clabel, cfilled, dcavsize, dcavfilled,
-        hostdone;
+        hostdone, clist, clrow;
     // result block (device) and its pinned host mirror
     DBuf dres;
     HostResults *hres = nullptr;
@@ -233,7 +233,7 @@ void nsgpu_destroy(nsgpu_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     DBuf *all[] = {&c->gx, &c->gy, &c->gz, &c->pkind, &c->poff, &c->pcnt, &c->pflags, &c->pbsph, &c->pdata, &c->porder, &c->patom, &c->atoms, &c->rpid, &c->gradv, &c->grada, &c->asums,
                    &c->rcnt, &c->rt, &c->rvol, &c->partial, &c->stats, &c->cand, &c->cov, &c->covst, &c->eps, &c->ideb, &c->status,
-                   &c->bits, &c->crx0, &c->crx1, &c->cpar, &c->crowcnt, &c->croff, &c->nroots, &c->rootlist, &c->bkey, &c->ccount, &c->clabel, &c->cfilled, &c->dcavsize, &c->dcavfilled, &c->hostdone, &c->dres, &c->dnv, &c->dgather, &c->aedge,
+                   &c->bits, &c->crx0, &c->crx1, &c->cpar, &c->crowcnt, &c->croff, &c->nroots, &c->rootlist, &c->bkey, &c->ccount, &c->clabel, &c->cfilled, &c->dcavsize, &c->dcavfilled, &c->hostdone, &c->dres, &c->dnv, &c->dgather, &c->aedge, &c->clist, &c->clrow,
                    &c->idx, &c->vslot, &c->vhit, &c->vorig, &c->normals, &c->vcount, &c->voffs, &c->scan_scratch, &c->verts, &c->tris, &c->tarea,
                    &c->segmask, &c->anode, &c->acase, &c->anv, &c->ant, &c->avoff, &c->atoff, &c->mverts, &c->mtris,
                    &c->mrr, &c->mraw, &c->field, &c->fbits, &c->batoms, &c->bbox, &c->bcount, &c->boff, &c->bcur,
@@ -533,7 +533,7 @@ static int trace(nsgpu_ctx *c) {
 
 static CclRuns ccl_runs(nsgpu_ctx *c) {
     return CclRuns{c->crx0.as<int>(), c->crx1.as<int>(), c->cpar.as<int>(), c->crowcnt.as<int>(),
-                   c->croff.as<int>()};
+                   c->croff.as<int>(), c->clist.as<int>(), c->clrow.as<int>(), c->nroots.as<int>() + 1};
 }
 
 // cavity detection, numbering and filling (getCavitiesWithBilevelStatusMap +
@@ -549,8 +549,10 @@ static int enqueue_cavities(nsgpu_ctx *c, hipStream_t st) {
     HIP_TRY(c, c->crowcnt.ensure(rows * 4));
     HIP_TRY(c, c->croff.ensure(rows * 4));
     HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(rows) * 4));
-    HIP_TRY(c, c->nroots.ensure(4));
+    HIP_TRY(c, c->nroots.ensure(8));  // root count, run list length
     HIP_TRY(c, c->rootlist.ensure(slots * 4));
+    HIP_TRY(c, c->clist.ensure(slots * 4));
+    HIP_TRY(c, c->clrow.ensure(slots * 4));
     HIP_TRY(c, c->bkey.ensure((size_t)kReplayMax * 4));
     HIP_TRY(c, c->ccount.ensure((size_t)kReplayMax * 4));
     HIP_TRY(c, c->clabel.ensure((size_t)kReplayMax * 4));

@@ -61,11 +61,14 @@ __device__ void unite(int *P, int a, int b) {
 // plane is set), so run starts and ends come from 64-node words with bit
 // operations and the status map itself is not read.  The k-th start and the
 // k-th end of a row belong to the same run.
-// Also zeroes *nroots for k_ccl_roots.
+// Also zeroes *nroots for k_ccl_roots and *nlist for k_ccl_flatten.
 __global__ void __launch_bounds__(256) k_ccl_runs(DevGrid g, const unsigned long long *inside, int N, int64_t rs,
-                                                  int *rx0, int *rx1, int *rowcnt, int *nroots) {
+                                                  int *rx0, int *rx1, int *rowcnt, int *nroots, int *nlist) {
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (row == 0) *nroots = 0;
+    if (row == 0) {
+        *nroots = 0;
+        *nlist = 0;
+    }
     if (row >= (int64_t)g.ny * g.nz) return;
     const int j = (int)(row % g.ny), k = (int)(row / g.ny);
     const int64_t NN = (int64_t)N * N, ray = (int64_t)k * N + j;
@@ -230,8 +233,12 @@ __global__ void __launch_bounds__(256) k_ccl_zlink(DevGrid g, int64_t rs, const 
 }
 
 // Every run is pointed at its root: read-only walks, each thread writes its
-// own row's entries, always with a root.
-__global__ void __launch_bounds__(256) k_ccl_flatten(int64_t rows, const int *rowcnt, const int *roff, int *R) {
+// own row's entries, always with a root.  The runs outside the component of
+// run 0 go to a compact list: that component has the smallest root, so the
+// replay always numbers it first (NSGPU_STATUS_OUT, never a cavity), and the
+// key and label passes need only the others, a small fraction of the runs.
+__global__ void __launch_bounds__(256) k_ccl_flatten(int64_t rows, const int *rowcnt, const int *roff, int *R,
+                                                     int *list_c, int *list_row, int *nlist) {
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= rows) return;
     const int n = rowcnt[row], c0 = roff[row];
@@ -239,6 +246,11 @@ __global__ void __launch_bounds__(256) k_ccl_flatten(int64_t rows, const int *ro
         int x = ld(R + c0 + q);
         for (int p = ld(R + x); p != x; p = ld(R + x)) x = p;
         st(R + c0 + q, x);
+        if (x != 0) {
+            const int at = atomicAdd(nlist, 1);
+            list_c[at] = c0 + q;
+            list_row[at] = (int)row;
+        }
     }
 }
 
@@ -270,22 +282,20 @@ __device__ __forceinline__ int slot_of(const int *R, int c) {
 
 // Size and backward-scan key of every component, per run: the run's first
 // node has its smallest backward rank (k reversed, then j, then i).  The
-// component of node 0 (when node 0 is TEMP_OUT) is the first one any scan
-// meets, so it is always labelled first and neither key is needed for it: it
-// is skipped, which removes the only heavily contended address.
+// component of run 0 is numbered first by every replay, so neither key is
+// needed for it: only the listed runs are visited (grid-stride over the list).
 __global__ void __launch_bounds__(256) k_ccl_keys(DevGrid g, int64_t rs, const int *rx0, const int *rx1,
-                                                  const int *rowcnt, const int *roff, const int *R, unsigned int *bkey,
+                                                  const int *roff, const int *R, const int *list_c,
+                                                  const int *list_row, const int *nlist, unsigned int *bkey,
                                                   int *count, int capacity) {
-    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= (int64_t)g.ny * g.nz) return;
-    const int skip = (rowcnt[0] > 0 && rx0[0] == 0) ? slot_of(R, 0) : -1;
-    const int n = rowcnt[row], c0 = roff[row];
-    const int j = (int)(row % g.ny), k = (int)(row / g.ny);
+    const int nl = *nlist;
     const int64_t plane = (int64_t)g.nx * g.ny;
-    for (int q = 0; q < n; ++q) {
-        const int s = slot_of(R, c0 + q);
-        if (s == skip || s >= capacity) continue;  // beyond capacity: the replay falls back to the host
-        const int64_t id = row + rs * q;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nl; e += gridDim.x * blockDim.x) {
+        const int c = list_c[e], row = list_row[e];
+        const int s = slot_of(R, c);
+        if (s >= capacity) continue;  // beyond capacity: the replay falls back to the host
+        const int j = row % g.ny, k = row / g.ny;
+        const int64_t id = row + rs * (c - roff[row]);
         const unsigned int rank = (unsigned int)((g.nz - 1 - k) * plane + (int64_t)j * g.nx + rx0[id]);
         atomicMin(bkey + s, rank);
         atomicAdd(count + s, rx1[id] - rx0[id] + 1);
@@ -293,28 +303,29 @@ __global__ void __launch_bounds__(256) k_ccl_keys(DevGrid g, int64_t rs, const i
 }
 
 // Final labels and fillCavities (Surface.cpp:2355-2487) for the filled
-// components: one wave per row, lanes stride over each run's nodes.  The
-// map fill already wrote NSGPU_STATUS_OUT at every TEMP_OUT node, the label
-// of the first component (the outside region whenever node 0 is outside it),
-// so only the nodes of the other components are written here.
+// components: one wave per listed run (grid-stride), lanes stride over the
+// run's nodes.  The map fill already wrote NSGPU_STATUS_OUT at every TEMP_OUT
+// node, the label of the component of run 0 (the first one numbered), so only
+// the listed runs of the other components are written here.
 __global__ void __launch_bounds__(256) k_ccl_label(DevGrid g, int64_t rs, const int *rx0, const int *rx1,
-                                                   const int *rowcnt, const int *roff, const int *R, const int *label,
+                                                   const int *roff, const int *R, const int *list_c,
+                                                   const int *list_row, const int *nlist, const int *label,
                                                    const uint8_t *filled, const int *done, int build_eps, int accurate,
                                                    int inside, int *status, int *epsmap, uint8_t *idebmap,
                                                    uint32_t *bits) {
-    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (row >= (int64_t)g.ny * g.nz || *done == 0) return;
+    if (*done == 0) return;
+    const int nl = *nlist;
     const int lane = threadIdx.x & 63;
-    const int n = rowcnt[row], c0 = roff[row];
-    const int j = (int)(row % g.ny), k = (int)(row / g.ny);
-    const int64_t base = row * g.nx;
     const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
-    for (int q = 0; q < n; ++q) {
-        const int s = slot_of(R, c0 + q);
+    for (int e = blockIdx.x * 4 + (threadIdx.x >> 6); e < nl; e += gridDim.x * 4) {
+        const int c = list_c[e], row = list_row[e];
+        const int j = row % g.ny, k = row / g.ny;
+        const int64_t base = (int64_t)row * g.nx;
+        const int s = slot_of(R, c);
         const int lab = label[s];
         if (lab == NSGPU_STATUS_OUT) continue;  // never filled: not a cavity
         const bool fill = filled[s] != 0;
-        const int64_t id = row + rs * q;
+        const int64_t id = row + rs * (c - roff[row]);
         const int x1 = rx1[id];
         for (int i = rx0[id] + lane; i <= x1; i += 64) {
             const int64_t f = base + i;
@@ -344,20 +355,24 @@ __global__ void __launch_bounds__(256) k_ccl_label(DevGrid g, int64_t rs, const 
 
 static inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 
+// grid of the passes over the run list (its length is known on the device only)
+constexpr unsigned kListBlocks = 1024;
+
 int ccl_row_capacity(const DevGrid &g) { return (g.nx + 1) / 2; }
 
 void launch_ccl(hipStream_t s, const DevGrid &g, const DevRays &R, const CclRuns &c, int *scan_scratch, int *nroots,
                 int *rootlist, unsigned int *bkey, int *count, int keycap) {
     const int64_t rows = (int64_t)g.ny * g.nz;
     hipLaunchKernelGGL(k_ccl_runs, dim3(nblk(rows)), dim3(256), 0, s, g, R.cov_st, R.N, rows, c.rx0, c.rx1, c.rowcnt,
-                       nroots);
+                       nroots, c.nlist);
     launch_scan(s, c.rowcnt, rows, c.roff, scan_scratch);
     hipLaunchKernelGGL(k_ccl_plane, dim3((unsigned)g.nz), dim3(1024), 0, s, g, rows, c.rx0, c.rx1, c.rowcnt, c.roff,
                        c.R);
     if (g.nz > 1)
         hipLaunchKernelGGL(k_ccl_zlink, dim3(nblk((int64_t)g.ny * (g.nz - 1))), dim3(256), 0, s, g, rows, c.rx0, c.rx1,
                            c.rowcnt, c.roff, c.R);
-    hipLaunchKernelGGL(k_ccl_flatten, dim3(nblk(rows)), dim3(256), 0, s, rows, c.rowcnt, c.roff, c.R);
+    hipLaunchKernelGGL(k_ccl_flatten, dim3(nblk(rows)), dim3(256), 0, s, rows, c.rowcnt, c.roff, c.R, c.list_c,
+                       c.list_row, c.nlist);
     hipLaunchKernelGGL(k_ccl_roots, dim3(nblk(rows)), dim3(256), 0, s, g, rows, c.rx0, c.rowcnt, c.roff, c.R, nroots,
                        rootlist, bkey, count, keycap);
 }
@@ -365,16 +380,16 @@ void launch_ccl(hipStream_t s, const DevGrid &g, const DevRays &R, const CclRuns
 void launch_ccl_keys(hipStream_t s, const DevGrid &g, const CclRuns &c, unsigned int *bkey, int *count,
                      int capacity) {
     const int64_t rows = (int64_t)g.ny * g.nz;
-    hipLaunchKernelGGL(k_ccl_keys, dim3(nblk(rows)), dim3(256), 0, s, g, rows, c.rx0, c.rx1, c.rowcnt,
-                       c.roff, c.R, bkey, count, capacity);
+    hipLaunchKernelGGL(k_ccl_keys, dim3(kListBlocks), dim3(256), 0, s, g, rows, c.rx0, c.rx1, c.roff, c.R,
+                       c.list_c, c.list_row, c.nlist, bkey, count, capacity);
 }
 
 void launch_ccl_label(hipStream_t s, const DevGrid &g, const CclRuns &c, const int *label, const uint8_t *filled,
                       const int *done, int build_eps, int accurate, int inside, int *status, int *eps, uint8_t *ideb,
                       uint32_t *bits) {
     const int64_t rows = (int64_t)g.ny * g.nz;
-    hipLaunchKernelGGL(k_ccl_label, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, s, g, rows, c.rx0,
-                       c.rx1, c.rowcnt, c.roff, c.R, label, filled, done, build_eps, accurate, inside, status, eps,
+    hipLaunchKernelGGL(k_ccl_label, dim3(kListBlocks), dim3(256), 0, s, g, rows, c.rx0, c.rx1, c.roff, c.R,
+                       c.list_c, c.list_row, c.nlist, label, filled, done, build_eps, accurate, inside, status, eps,
                        ideb, bits);
 }
 

@@ -45,6 +45,8 @@ void launch_sum_blocks_scan(hipStream_t, const double *v, int64_t ncap, const in
 // and R (R is indexed by compact run id), one count and one offset per row
 struct CclRuns {
     int *rx0, *rx1, *R, *rowcnt, *roff;
+    // the runs outside the component of run 0 (run id and row), *nlist of them
+    int *list_c, *list_row, *nlist;
 };
 int ccl_row_capacity(const DevGrid &);
 // runs from the x grid rays' status plane (R.cov_st); scan_scratch: scan_scratch_ints(rows) ints;
