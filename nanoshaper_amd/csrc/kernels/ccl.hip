@@ -284,21 +284,50 @@ __device__ __forceinline__ int slot_of(const int *R, int c) {
 // node has its smallest backward rank (k reversed, then j, then i).  The
 // component of run 0 is numbered first by every replay, so neither key is
 // needed for it: only the listed runs are visited (grid-stride over the list).
+// A cavity's hundreds of runs would all hit its two key words, so a wave
+// first combines its lanes per component (min of the ranks, sum of the
+// sizes) and one lane per component does the two atomics.
 __global__ void __launch_bounds__(256) k_ccl_keys(DevGrid g, int64_t rs, const int *rx0, const int *rx1,
                                                   const int *roff, const int *R, const int *list_c,
                                                   const int *list_row, const int *nlist, unsigned int *bkey,
                                                   int *count, int capacity) {
     const int nl = *nlist;
+    const int lane = threadIdx.x & 63;
     const int64_t plane = (int64_t)g.nx * g.ny;
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nl; e += gridDim.x * blockDim.x) {
-        const int c = list_c[e], row = list_row[e];
-        const int s = slot_of(R, c);
-        if (s >= capacity) continue;  // beyond capacity: the replay falls back to the host
-        const int j = row % g.ny, k = row / g.ny;
-        const int64_t id = row + rs * (c - roff[row]);
-        const unsigned int rank = (unsigned int)((g.nz - 1 - k) * plane + (int64_t)j * g.nx + rx0[id]);
-        atomicMin(bkey + s, rank);
-        atomicAdd(count + s, rx1[id] - rx0[id] + 1);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < nl; base += stride) {
+        const int64_t e = base + lane;
+        int s = -1, size = 0;
+        unsigned int rank = 0xffffffffu;
+        if (e < nl) {
+            const int c = list_c[e], row = list_row[e];
+            s = slot_of(R, c);
+            if (s < capacity) {  // beyond capacity: the replay falls back to the host
+                const int j = row % g.ny, k = row / g.ny;
+                const int64_t id = row + rs * (c - roff[row]);
+                rank = (unsigned int)((g.nz - 1 - k) * plane + (int64_t)j * g.nx + rx0[id]);
+                size = rx1[id] - rx0[id] + 1;
+            } else {
+                s = -1;
+            }
+        }
+        unsigned long long pending = __ballot(s >= 0);
+        while (pending) {
+            const int leader = __ffsll((unsigned long long)pending) - 1;
+            const int key = __shfl(s, leader, 64);
+            const bool mine = s == key;
+            unsigned int r = mine ? rank : 0xffffffffu;
+            int z = mine ? size : 0;
+            for (int off = 32; off > 0; off >>= 1) {
+                r = min(r, (unsigned int)__shfl_xor((int)r, off, 64));
+                z += __shfl_xor(z, off, 64);
+            }
+            if (lane == leader) {
+                atomicMin(bkey + key, r);
+                atomicAdd(count + key, z);
+            }
+            pending &= ~__ballot(mine);
+        }
     }
 }
 
