@@ -63,11 +63,29 @@ __global__ void __launch_bounds__(256) k_fill_bits(DevGrid g, DevRays R, FillOpt
             }
         }
     }
+    // A full wave stages its 256 nodes' epsmap (3 KB) in LDS, so that each of
+    // its three 16-byte stores writes one contiguous 1 KB: stored from the
+    // lanes' own 48-byte runs, every store instruction left partial lines at a
+    // 48-byte stride and the map fill wrote at half the bandwidth of a plain
+    // fill.  A wave's LDS accesses are performed in order, so the reads see
+    // the other lanes' writes; the wave barrier keeps the compiler from
+    // moving them.
+    __shared__ int4 stage[4][3 * 64];
+    const bool full = __ballot(nq == kFillNodes) == ~0ull;
     if (nq == kFillNodes) {
-        if (o.build_eps) {
+        if (o.build_eps && full) {
+            const int lane = threadIdx.x & 63;
+            int4 *w = stage[threadIdx.x >> 6];
+            for (int v = 0; v < 3; ++v) w[3 * lane + v] = make_int4(eps[4 * v], eps[4 * v + 1], eps[4 * v + 2], eps[4 * v + 3]);
+            __builtin_amdgcn_wave_barrier();
+            int4 *dst = reinterpret_cast<int4 *>(epsmap + 3 * (f0 - kFillNodes * lane));
+            for (int v = 0; v < 3; ++v) dst[lane + 64 * v] = w[lane + 64 * v];
+        } else if (o.build_eps) {
             int4 *e = reinterpret_cast<int4 *>(epsmap + 3 * f0);
             for (int v = 0; v < 3 * kFillNodes / 4; ++v)
                 e[v] = make_int4(eps[4 * v], eps[4 * v + 1], eps[4 * v + 2], eps[4 * v + 3]);
+        }
+        if (o.build_eps) {
             uint32_t *iw = reinterpret_cast<uint32_t *>(idebmap + f0);
             for (int v = 0; v < kFillNodes / 4; ++v)
                 iw[v] = (uint32_t)ideb[4 * v] | ((uint32_t)ideb[4 * v + 1] << 8) | ((uint32_t)ideb[4 * v + 2] << 16) |
