@@ -112,6 +112,10 @@ struct nsgpu_ctx {
     DBuf dnv;
     bool nv_pending = false;
     int64_t nv_cap = 0;
+    // getSurf already classified the cubes for triangulate (segmask, vcount,
+    // voffs hold the result) and read the active-cube count
+    bool pre_mc = false;
+    int64_t pre_nact = 0;
     // vertex provenance for the normals: depth of each analytic vertex, cube
     // and edge of each mid point
     DBuf vhit, vorig, normals;
@@ -929,6 +933,38 @@ static int assemble(nsgpu_ctx *c, const int eslots[3]) {
     return NSGPU_OK;
 }
 
+// Cube classification of triangulate (row-segment masks, their scan) and a
+// copy of the active-cube and analytic-vertex totals into pinned memory:
+// enqueued by getSurf behind the vertex pass, read after its one
+// synchronisation.
+static int enqueue_classify(nsgpu_ctx *c) {
+    const DevGrid g = dev_grid(c);
+    const int64_t nseg = mc_num_segments(g);
+    HIP_TRY(c, c->segmask.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
+    HIP_TRY(c, c->vcount.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
+    HIP_TRY(c, c->voffs.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
+    HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(std::max<int64_t>(nseg, 1)) * 4));
+    void *hp = nullptr;
+    int rc = pinned(c, 32, &hp);
+    if (rc) return rc;
+    HIP_TRY(c, c->dgather.ensure(32));
+    launch_mc_classify(c->stream, g, c->bits.as<uint32_t>(), c->segmask.as<uint32_t>(), c->vcount.as<int>());
+    launch_scan(c->stream, c->vcount.as<int>(), nseg, c->voffs.as<int>(), c->scan_scratch.as<int>());
+    GatherSrc gs{};
+    gs.src[0] = c->dnv.as<int>();
+    gs.src[1] = c->dnv.as<int>() + 1;
+    gs.n = 2;
+    if (nseg > 0) {
+        gs.src[2] = c->vcount.as<int>() + nseg - 1;
+        gs.src[3] = c->voffs.as<int>() + nseg - 1;
+        gs.n = 4;
+    }
+    launch_gather(c->stream, gs, c->dgather.as<int>());
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipMemcpyAsync(hp, c->dgather.p, 32, hipMemcpyDeviceToHost, c->stream));
+    return NSGPU_OK;
+}
+
 int nsgpu_get_surf(nsgpu_ctx *c, const nsgpu_surf_opts *o, nsgpu_surf_result *res) {
     if (!c || !o) return NSGPU_E_ARG;
     int rc = begin_surf(c, o);
@@ -951,8 +987,10 @@ int nsgpu_get_surf(nsgpu_ctx *c, const nsgpu_surf_opts *o, nsgpu_surf_result *re
         // the vertex pass goes into the same stream before the result block
         // is read: a repeated attempt clears the slots it wrote
         mark(c, ST_VERTS, 0, c->stream);
+        c->pre_mc = false;
         if (o->accurate_triangulation) {
             rc = assemble(c, es);
+            if (rc == NSGPU_OK) rc = enqueue_classify(c);
             if (rc) {
                 c->record_pid = false;
                 return rc;
@@ -967,9 +1005,16 @@ int nsgpu_get_surf(nsgpu_ctx *c, const nsgpu_surf_opts *o, nsgpu_surf_result *re
     rc = finish_surf(c);
     if (rc) return rc;
     if (o->accurate_triangulation) {
+        // the totals read with the result block (enqueue_classify)
+        const int *h = static_cast<const int *>(c->hpin);
+        rc = set_nv(c, (int64_t)h[0] + h[1]);
+        if (rc) return rc;
+        c->pre_nact = mc_num_segments(dev_grid(c)) > 0 ? (int64_t)h[2] + h[3] : 0;
+        c->pre_mc = true;
         // cavities numbered on the host (more components than the device
         // replay holds) changed the corner bitmap after the vertex pass ran
         if (o->fill_cavities && o->build_status && c->hres->cav_info()[0] == 0) {
+            c->pre_mc = false;
             rc = assemble(c, es);
             if (rc) return rc;
         }
@@ -988,6 +1033,11 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
     const int64_t nseg = mc_num_segments(g);
     mark(c, ST_MC, 0, c->stream);
     // 1. classify 32-cube row segments, compact the active cubes in cube order
+    // (getSurf did the classification and read its count when the bitmap has
+    // not changed since)
+    int64_t tot[2] = {c->pre_nact, 0};
+    int rc = NSGPU_OK;
+    if (!c->pre_mc) {
     HIP_TRY(c, c->segmask.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
     HIP_TRY(c, c->vcount.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
     HIP_TRY(c, c->voffs.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
@@ -996,8 +1046,6 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
     launch_scan(c->stream, c->vcount.as<int>(), nseg, c->voffs.as<int>(), c->scan_scratch.as<int>());
     HIP_TRY(c, hipGetLastError());
     // the active-cube count, with the analytic vertex count if still pending
-    int64_t tot[2] = {0, 0};
-    int rc = NSGPU_OK;
     {
         const int *cnts[2] = {c->vcount.as<int>(), c->dnv.as<int>()};
         const int *offs[2] = {c->voffs.as<int>(), c->dnv.as<int>() + 1};
@@ -1009,6 +1057,8 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
             if (rc) return rc;
         }
     }
+    }
+    c->pre_mc = false;
     const int64_t nact = tot[0];
     c->nverts = c->n_analytic;
     const size_t na = (size_t)std::max<int64_t>(nact, 1);
@@ -1222,6 +1272,7 @@ int nsgpu_blobby_build(nsgpu_ctx *c, const double *xyzr, int32_t n, double B, do
     if (!c->have_grid) return fail(c, NSGPU_E_STATE, "set_grid first");
     if (!(B < 0)) B = -2.5;  // BlobbySurface::setBlobbyness (BlobbySurface.cpp:46-54)
     HIP_TRY(c, hipSetDevice(c->device));
+    c->pre_mc = false;  // the classification buffers are reused below
     const DevGrid g = dev_grid(c);
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
     hipStream_t st = c->stream;
