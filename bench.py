@@ -386,7 +386,11 @@ def main():
         parts = [kern.get(k) for k in INTERSECT_KERNELS]
         if all(parts):
             traffic = sum(t["fetch_bytes"] + t["write_bytes"] for t in parts)
-    roofline = {"kernel": "k_intersect (plain and torus launches, one stage)", "bound": "fp64",
+    # compute-bound ("mfma" is the contract's name for the compute roof): FP64
+    # vector arithmetic, priced against MI355X's FP64 peak, the same 78.6
+    # TFLOP/s for vector and matrix FP64; the kernel has no matrix form
+    roofline = {"kernel": "k_intersect (plain and torus launches, one stage)", "bound": "mfma",
+                "compute": "fp64 vector (no MFMA form; peak = MI355X FP64 rate)",
                 "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                 "flops_per_launch": flops, "path_counts": pc}
