@@ -126,6 +126,30 @@ def test_repeat_calls_are_identical(device_ok):
     assert np.array_equal(e1, dev.epsmap()) and np.array_equal(s1, dev.status())
 
 
+def test_call_sequences_match_a_fresh_context(device_ok):
+    # getSurf also classifies the cubes for triangulate and reads the vertex
+    # total with its result block; whatever ran before on the context, the
+    # next triangulate must give what a fresh context gives
+    grid, ps, dev, orc = _surfaces(330, 12.0, 2.0, 90.0, seed=5)
+    fill = dict(fill_cavities=True, cav_vol=1000.0)  # fills every cavity: the corner bitmap changes
+    keep = dict(fill_cavities=False)
+    dev.get_surf(**fill)
+    dev.triangulate()
+    dev.get_surf(accurate_triangulation=False)
+    with pytest.raises(ns.NsgpuError):
+        dev.triangulate()  # no vertices after that call
+    seq = []
+    for opts in (keep, fill, fill):
+        dev.get_surf(**opts)
+        seq.append((dev.triangulate(), dev.mesh()))
+    _, _, fresh, _ = _surfaces(330, 12.0, 2.0, 90.0, seed=5)
+    for opts, (m, (v, t)) in zip((keep, fill, fill), seq):
+        fresh.get_surf(**opts)
+        assert fresh.triangulate() == m
+        vf, tf = fresh.mesh()
+        assert np.array_equal(vf, v) and np.array_equal(tf, t)
+
+
 def test_c2_scale_parity(device_ok):
     # the benchmark workload, 5,000 atoms at 4 points/A
     grid, ps, dev, orc = _surfaces(5000, 28.0, 4.0, 80.0, seed=0)
