@@ -715,6 +715,7 @@ __global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRa
     // run on full waves: the disc cull and the rectangle's last partial row of
     // lanes would otherwise leave a fifth to a half of the lanes idle there.
     __shared__ int qm[128], qn[128];
+    __shared__ double qt0[128], qt1[128];  // the entries' bounding-sphere depths
     // One instance per panel, so that every axis index is a constant and the
     // ray origin, hit point and frame rows stay in registers.
     auto walk = [&](auto panel_c) {
@@ -735,10 +736,11 @@ __global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRa
             pa[fd] = gf[m] + delta;
             pa[ld] = gl[n] + delta;
         };
-        auto roots = [&](int m, int n) {
+        auto roots = [&](int m, int n, double t0, double t1) {
             double pa[3], t[4];
             origin(m, n, pa);
-            ray_sphere(pa, dir, sc, radius, t[0], t[1]);  // true: it passed in the cull
+            t[0] = t0;  // ray_sphere's roots from the cull pass
+            t[1] = t1;
             int ni = 2;
             if (kTorus) {
                 if (kRoots == kRootsMixed)
@@ -789,6 +791,7 @@ __global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRa
         int queued = 0;
         for (int64_t base = 0; base < total; base += 64) {
             bool act = false;
+            double t0 = 0., t1 = 0.;
             const int m = col + (int)is[fd], n = row + (int)is[ld];
             if (base + threadIdx.x < total) {
                 double pa[3];
@@ -799,7 +802,6 @@ __global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRa
                     const double dx = sc[fd] - pa[fd], dy = sc[ld] - pa[ld];
                     if (!(dx * dx + dy * dy > radius * radius)) {
                         if (kCount) ++pc[PC_DISC];
-                        double t0, t1;
                         act = ray_sphere(pa, dir, sc, radius, t0, t1);
                         if (kCount && act) ++pc[PC_SPHERE];
                     }
@@ -816,18 +818,21 @@ __global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRa
                 const int at = queued + __popcll(mask & ((1ull << threadIdx.x) - 1ull));
                 qm[at] = m;
                 qn[at] = n;
+                qt0[at] = t0;
+                qt1[at] = t1;
             }
             queued += __popcll(mask);
             if (queued >= 64) {
                 __syncthreads();
                 queued -= 64;
-                roots(qm[queued + threadIdx.x], qn[queued + threadIdx.x]);
+                const int e = queued + threadIdx.x;
+                roots(qm[e], qn[e], qt0[e], qt1[e]);
                 __syncthreads();
             }
         }
         if (queued > 0) {
             __syncthreads();
-            if ((int)threadIdx.x < queued) roots(qm[threadIdx.x], qn[threadIdx.x]);
+            if ((int)threadIdx.x < queued) roots(qm[threadIdx.x], qn[threadIdx.x], qt0[threadIdx.x], qt1[threadIdx.x]);
         }
     };
     if (panel == 0) walk(std::integral_constant<int, 0>());
