@@ -97,6 +97,7 @@ struct nsgpu_ctx {
     size_t hpin_cap = 0;
     DBuf dgather;  // device side of those reads (launch_gather)
     DBuf aedge;    // per active cube: its twelve edge vertex indices
+    DBuf cubeidx;  // smoothing: node -> active cube index (validated, never cleared)
     bool capturing = false;  // enqueueing into a graph: no stage events
     std::vector<double> cav_size;
     std::vector<uint8_t> cav_filled;
@@ -237,7 +238,7 @@ void nsgpu_destroy(nsgpu_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     DBuf *all[] = {&c->gx, &c->gy, &c->gz, &c->pkind, &c->poff, &c->pcnt, &c->pflags, &c->pbsph, &c->pdata, &c->porder, &c->patom, &c->atoms, &c->rpid, &c->gradv, &c->grada, &c->asums,
                    &c->rcnt, &c->rt, &c->rvol, &c->partial, &c->stats, &c->cand, &c->cov, &c->covst, &c->eps, &c->ideb, &c->status,
-                   &c->bits, &c->crx0, &c->crx1, &c->cpar, &c->crowcnt, &c->croff, &c->nroots, &c->rootlist, &c->bkey, &c->ccount, &c->clabel, &c->cfilled, &c->dcavsize, &c->dcavfilled, &c->hostdone, &c->dres, &c->dnv, &c->dgather, &c->aedge, &c->clist, &c->clrow,
+                   &c->bits, &c->crx0, &c->crx1, &c->cpar, &c->crowcnt, &c->croff, &c->nroots, &c->rootlist, &c->bkey, &c->ccount, &c->clabel, &c->cfilled, &c->dcavsize, &c->dcavfilled, &c->hostdone, &c->dres, &c->dnv, &c->dgather, &c->aedge, &c->clist, &c->clrow, &c->cubeidx,
                    &c->idx, &c->vslot, &c->vhit, &c->vorig, &c->normals, &c->vcount, &c->voffs, &c->scan_scratch, &c->verts, &c->tris, &c->tarea,
                    &c->segmask, &c->anode, &c->acase, &c->anv, &c->ant, &c->avoff, &c->atoff, &c->mverts, &c->mtris,
                    &c->mrr, &c->mraw, &c->field, &c->fbits, &c->batoms, &c->bbox, &c->bcount, &c->boff, &c->bcur,
@@ -1398,6 +1399,8 @@ int nsgpu_blobby_build(nsgpu_ctx *c, const double *xyzr, int32_t n, double B, do
     sm.tris = c->tris.as<int>();
     sm.nact = (int)nact;
     sm.vout = c->mraw.as<double>();
+    HIP_TRY(c, c->cubeidx.ensure((size_t)std::max<int64_t>(nvox, 1) * 4));
+    sm.cube_index = c->cubeidx.as<int>();
     launch_smooth(st, g, sm);
     // 4. the OFF text round trip ("%.3f" written, "%f" read into floats) and the mesh
     launch_quantize(st, c->mraw.as<double>(), 3 * nv, c->mverts.as<double>());

@@ -186,6 +186,9 @@ struct SmoothArgs {
     const int *anode, *ant, *atoff, *tris;  // active cubes, their triangle counts and offsets, triangles
     int nact;
     double *vout;
+    // grid-sized cube -> active index map (launch_smooth fills it; an entry is
+    // trusted only where anode confirms it, so the map is never cleared)
+    int *cube_index;
 };
 void launch_smooth(hipStream_t, const DevGrid &, const SmoothArgs &);
 void launch_quantize(hipStream_t, const double *in, int64_t n, double *out);

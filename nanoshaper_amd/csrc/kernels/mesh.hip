@@ -184,15 +184,17 @@ void launch_field_bits(hipStream_t s, const double *field, int64_t nvox, double 
 // triangle list in cube order, so visiting the cubes in increasing node order
 // visits the vertex's triangles in triangle order, with no incidence lists.
 
-// index of active cube `key` in the sorted active-cube list, or -1
-__device__ __forceinline__ int active_index(const int *anode, int nact, int key) {
-    int lo = 0, hi = nact;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (anode[mid] < key) lo = mid + 1;
-        else hi = mid;
-    }
-    return (lo < nact && anode[lo] == key) ? lo : -1;
+// index of active cube `key` in the active-cube list, or -1: the map entry
+// of the cube, confirmed by the list (entries of earlier calls, or never
+// written, fail the check)
+__device__ __forceinline__ int active_index(const int *anode, int nact, const int *cube_index, int key) {
+    const int ci = cube_index[key];
+    return (ci >= 0 && ci < nact && anode[ci] == key) ? ci : -1;
+}
+
+__global__ void __launch_bounds__(256) k_cube_index(const int *anode, int nact, int *cube_index) {
+    const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci < nact) cube_index[anode[ci]] = ci;
 }
 
 __global__ void __launch_bounds__(256) k_smooth(DevGrid g, SmoothArgs a) {
@@ -210,7 +212,7 @@ __global__ void __launch_bounds__(256) k_smooth(DevGrid g, SmoothArgs a) {
     int c = 0;
     for (int q = 0; q < 4; ++q) {
         if (cubes[q] < 0) continue;
-        const int ci = active_index(a.anode, a.nact, cubes[q]);
+        const int ci = active_index(a.anode, a.nact, a.cube_index, cubes[q]);
         if (ci < 0) continue;
         for (int t = a.atoff[ci]; t < a.atoff[ci] + a.ant[ci]; ++t) {
             const int *tv = a.tris + 3 * (int64_t)t;
@@ -271,6 +273,7 @@ __global__ void __launch_bounds__(256) k_quantize(const double *in, int64_t n, d
 }
 
 void launch_smooth(hipStream_t s, const DevGrid &g, const SmoothArgs &a) {
+    if (a.nact > 0) hipLaunchKernelGGL(k_cube_index, dim3(nblk(a.nact)), dim3(256), 0, s, a.anode, a.nact, a.cube_index);
     if (a.nv > 0) hipLaunchKernelGGL(k_smooth, dim3(nblk(a.nv)), dim3(256), 0, s, g, a);
 }
 void launch_quantize(hipStream_t s, const double *in, int64_t n, double *out) {
