@@ -1563,8 +1563,11 @@ int nsgpu_surface_gradient(nsgpu_ctx *c, double *volume, double *area, double *d
     HIP_TRY(c, c->gradv.ensure(ng * 8));
     HIP_TRY(c, c->grada.ensure(ng * 8));
     HIP_TRY(c, c->asums.ensure((size_t)std::max<int64_t>(nb, 1) * 16));
-    HIP_TRY(c, hipMemsetAsync(c->gradv.p, 0, ng * 8, c->stream));
-    HIP_TRY(c, hipMemsetAsync(c->grada.p, 0, ng * 8, c->stream));
+    FillList fl{};
+    fl.s[0] = FillSeg{static_cast<uint32_t *>(c->gradv.p), 0u, (int64_t)ng * 2};
+    fl.s[1] = FillSeg{static_cast<uint32_t *>(c->grada.p), 0u, (int64_t)ng * 2};
+    fl.k = 2;
+    launch_fill_multi(c->stream, fl);
     AdjointArgs a;
     a.atoms = c->atoms.as<double>();
     a.probe = c->probe;
@@ -1577,14 +1580,21 @@ int nsgpu_surface_gradient(nsgpu_ctx *c, double *volume, double *area, double *d
     launch_adjoint(c->stream, g, P, R, a);
     HIP_TRY(c, hipGetLastError());
     mark(c, ST_ADJOINT, 1, c->stream);
-    std::vector<double> sums((size_t)nb * 2);
-    if (nb)
-        HIP_TRY(c, hipMemcpyAsync(sums.data(), c->asums.p, sums.size() * 8, hipMemcpyDeviceToHost, c->stream));
-    if (dvolume && c->natoms)
-        HIP_TRY(c, hipMemcpyAsync(dvolume, c->gradv.p, (size_t)c->natoms * 32, hipMemcpyDeviceToHost, c->stream));
-    if (darea && c->natoms)
-        HIP_TRY(c, hipMemcpyAsync(darea, c->grada.p, (size_t)c->natoms * 32, hipMemcpyDeviceToHost, c->stream));
+    // the block partials and both gradients go through the pinned staging
+    // buffer (a pageable copy is staged and synchronised by the runtime on
+    // its own): three queued copies, one synchronisation
+    const size_t gbytes = (size_t)c->natoms * 32, sbytes = (size_t)nb * 16;
+    void *hp = nullptr;
+    const int prc = pinned(c, sbytes + 2 * gbytes, &hp);
+    if (prc) return prc;
+    char *hsum = static_cast<char *>(hp), *hdv = hsum + sbytes, *hda = hdv + gbytes;
+    if (nb) HIP_TRY(c, hipMemcpyAsync(hsum, c->asums.p, sbytes, hipMemcpyDeviceToHost, c->stream));
+    if (dvolume && gbytes) HIP_TRY(c, hipMemcpyAsync(hdv, c->gradv.p, gbytes, hipMemcpyDeviceToHost, c->stream));
+    if (darea && gbytes) HIP_TRY(c, hipMemcpyAsync(hda, c->grada.p, gbytes, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (dvolume && gbytes) std::memcpy(dvolume, hdv, gbytes);
+    if (darea && gbytes) std::memcpy(darea, hda, gbytes);
+    const double *sums = static_cast<const double *>(hp);
     double v = 0., s = 0.;
     for (int64_t b = 0; b < nb; ++b) {
         v += sums[2 * b];
