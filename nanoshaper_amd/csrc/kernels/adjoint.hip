@@ -11,9 +11,10 @@
 // Every hit lies on the sphere or torus that its patch is cut from; the
 // implicit function f(x; atoms) of that surface gives the depth derivative
 // dt/dtheta = -(df/dtheta) / (grad f . ray) (implicit differentiation) and the
-// normal derivative follows from grad f at the moved hit point.  Both are
-// evaluated in forward mode, one parameter at a time, with first-order dual
-// numbers:
+// normal derivative follows from grad f at the moved hit point:
+// d(grad f)/dtheta + (dgrad f/dx_ray) L dt/dtheta.  Both are evaluated in
+// forward mode with first-order dual numbers, one pass per parameter plus one
+// pass seeded along the ray axis for the second term:
 //   point cell: f = |x - c|^2 - r^2 of its atom;
 //   probe cap (facet): f = |x - p|^2 - rp^2, the probe centre p tangent to its
 //     three atoms (|p - c_m| = r_m + rp); dp follows from the three
@@ -49,8 +50,8 @@ struct HitSurface {
 };
 
 // f and grad_x f at x; parameter (atom ks, component kq: x, y, z, radius)
-// carries the derivative seed.  Inlined into loops over constant ks, kq, so
-// every array index below is a constant.
+// carries the derivative seed (ks = 3: no parameter seed, only x's).  Inlined
+// into loops over constant ks, kq, so every array index below is a constant.
 __device__ __forceinline__ void surface_field(const HitSurface &h, double rp, int ks, int kq, const D1 x[3], D1 &f,
                                              D1 grad[3]) {
     D1 c[3][3], r[3];
@@ -67,9 +68,9 @@ __device__ __forceinline__ void surface_field(const HitSurface &h, double rp, in
     }
     if (h.kind == NSGPU_PATCH_FACET_REG || h.kind == NSGPU_PATCH_FACET_SING) {
         // (p - c_m) . (dp - dc_m) = (r_m + rp) dr_m  =>  dp = Minv b, b only in row ks
-        const double gk = (kq < 3) ? (h.p[kq] - h.c[ks][kq]) : (h.r[ks] + rp);
+        const double gk = ks >= 3 ? 0. : ((kq < 3) ? (h.p[kq] - h.c[ks][kq]) : (h.r[ks] + rp));
         D1 y[3];
-        for (int q = 0; q < 3; ++q) y[q] = x[q] - D1{h.p[q], h.minv[q][ks] * gk};
+        for (int q = 0; q < 3; ++q) y[q] = x[q] - D1{h.p[q], ks >= 3 ? 0. : h.minv[q][ks] * gk};
         f = dot3(y, y) - D1{rp * rp, 0.};
         for (int q = 0; q < 3; ++q) grad[q] = 2. * y[q];
         return;
@@ -105,11 +106,17 @@ __device__ __forceinline__ void inv3(const double m[3][3], double o[3][3]) {
     o[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * id;
 }
 
-// One hit: adds its volume and area derivatives to the atoms' gradients and
-// returns its area term 1/|n|_1.  VC is the ray axis.
+// One hit: its volume and area derivatives for parameter (atom s, component
+// q) in dv, da[4 s + q] (zero past the patch's atoms, whose ids go to atom[]),
+// and its area term 1/|n|_1 as the result.  VC is the ray axis.
+struct HitGrad {
+    double dv[12], da[12];
+    int atom[3];
+    int nat;
+};
 template <int VC>
 __device__ __forceinline__ double hit_adjoint(const DevPatches &P, const AdjointArgs &a, int pid, const double x0[3],
-                                              double L, double sgn) {
+                                              double L, double sgn, HitGrad &out) {
     HitSurface h;
     h.kind = P.kind[pid];
     h.nat = h.kind == NSGPU_PATCH_POINT ? 1 : (h.kind <= NSGPU_PATCH_FACET_SING ? 3 : 2);
@@ -117,6 +124,7 @@ __device__ __forceinline__ double hit_adjoint(const DevPatches &P, const Adjoint
     for (int s = 0; s < 3; ++s) {
         const int id = s < h.nat ? P.atom[3 * pid + s] : 0;
         h.atom[s] = id;
+        out.atom[s] = id;
         for (int q = 0; q < 3; ++q) h.c[s][q] = s < h.nat ? a.atoms[4 * id + q] : 0.;
         h.r[s] = s < h.nat ? a.atoms[4 * id + 3] : 0.;
     }
@@ -127,39 +135,52 @@ __device__ __forceinline__ double hit_adjoint(const DevPatches &P, const Adjoint
             for (int q = 0; q < 3; ++q) m[s][q] = h.p[q] - h.c[s][q];
         inv3(m, h.minv);
     }
+    out.nat = h.nat;
     const double vscale = sgn * a.area_px * L / a.flag;
+    // d(grad f)/dx along the ray axis (no parameter seed)
+    double hx[3];
+    {
+        D1 x[3] = {{x0[0], 0.}, {x0[1], 0.}, {x0[2], 0.}};
+        x[VC].d = 1.;
+        D1 f, gr[3];
+        surface_field(h, a.probe, 3, 0, x, f, gr);
+        for (int q = 0; q < 3; ++q) hx[q] = gr[q].d;
+    }
     double inv_v = 0.;
 #pragma unroll
     for (int ks = 0; ks < 3; ++ks) {
-        if (ks >= h.nat) break;
 #pragma unroll
         for (int kq = 0; kq < 4; ++kq) {
+            out.dv[4 * ks + kq] = 0.;
+            out.da[4 * ks + kq] = 0.;
+            if (ks >= h.nat) continue;
             // depth derivative at the fixed point: dt = -(df/dk) / (grad f . e L)
-            D1 x[3] = {{x0[0], 0.}, {x0[1], 0.}, {x0[2], 0.}};
+            const D1 x[3] = {{x0[0], 0.}, {x0[1], 0.}, {x0[2], 0.}};
             D1 f, gr[3];
             surface_field(h, a.probe, ks, kq, x, f, gr);
             const double ft = gr[VC].v * L;
             const double dt = (ft != 0.) ? -f.d / ft : 0.;
-            // normal at the moved point
-            x[VC].d = L * dt;
-            surface_field(h, a.probe, ks, kq, x, f, gr);
-            const D1 gn = dsqrt(dot3(gr, gr));
-            const D1 l1 = dabs(gr[0] / gn) + dabs(gr[1] / gn) + dabs(gr[2] / gn);
+            // normal at the moved point: the parameter's own term plus the
+            // ray-axis term at the depth change
+            D1 gm[3];
+            for (int q = 0; q < 3; ++q) gm[q] = D1{gr[q].v, gr[q].d + (L * dt) * hx[q]};
+            const D1 gn = dsqrt(dot3(gm, gm));
+            const D1 l1 = dabs(gm[0] / gn) + dabs(gm[1] / gn) + dabs(gm[2] / gn);
             const D1 inv = D1{1., 0.} / l1;
             inv_v = inv.v;
-            const int idx = 4 * h.atom[ks] + kq;
-            atomicAdd(a.grad_v + idx, vscale * dt);
-            atomicAdd(a.grad_a + idx, a.area_px * inv.d);
+            out.dv[4 * ks + kq] = vscale * dt;
+            out.da[4 * ks + kq] = a.area_px * inv.d;
         }
     }
     return inv_v;
 }
 
-// the kept segments of one grid ray of panel PANEL: volume and area terms
+// depth e of grid ray r (panel PANEL) when its segment is kept: the
+// segment's length (from its entry depth) and the hit's area term
 template <int PANEL>
-__device__ __forceinline__ void ray_adjoint(const DevGrid &g, const DevPatches &P, const DevRays &R,
-                                            const AdjointArgs &a, int64_t r, int64_t ray, int c, double &vsum,
-                                            double &asum) {
+__device__ __forceinline__ void hit_slot(const DevGrid &g, const DevPatches &P, const DevRays &R, const AdjointArgs &a,
+                                         int64_t r, int64_t ray, int e, double &vsum, double &asum, int &key,
+                                         HitGrad &hg) {
     constexpr int fd = PanelAxes<PANEL>::fd, ld = PanelAxes<PANEL>::ld, vc = PanelAxes<PANEL>::vc;
     const double *ga = vc == 0 ? g.x : (vc == 1 ? g.y : g.z);
     const double *gf = fd == 0 ? g.x : g.y;
@@ -172,31 +193,74 @@ __device__ __forceinline__ void ray_adjoint(const DevGrid &g, const DevPatches &
     pa[fd] = gf[m];
     pa[ld] = gl[n];
     const double *T = R.t + r * R.K;
-    const int *I = R.pid + r * R.K;
-    for (int i = 0; i < c; i += 2) {
-        if (fabs(T[i + 1] - T[i]) < kEpsInt) continue;
-        vsum += (pa[vc] + L * T[i + 1]) - (pa[vc] + L * T[i]);
-        for (int e = 0; e < 2; ++e) {
-            double x[3] = {pa[0], pa[1], pa[2]};
-            x[vc] += L * T[i + e];
-            asum += hit_adjoint<vc>(P, a, I[i + e], x, L, e ? 1. : -1.);
-        }
-    }
+    const int i = e & ~1;
+    if (fabs(T[i + 1] - T[i]) < kEpsInt) return;
+    if (e == i) vsum += (pa[vc] + L * T[i + 1]) - (pa[vc] + L * T[i]);
+    double x[3] = {pa[0], pa[1], pa[2]};
+    x[vc] += L * T[e];
+    key = R.pid[r * R.K + e];
+    asum += hit_adjoint<vc>(P, a, key, x, L, (e & 1) ? 1. : -1., hg);
 }
 
+// One thread per depth slot, slot-major: thread t takes depth e = t / nrays of
+// ray t % nrays, so a wave covers 64 neighbouring rays at the same depth index
+// (mostly the same patches, so its lanes take the same branches), and the
+// launch has K times the rays' parallelism.  Slots past a ray's count exit.
 __global__ void __launch_bounds__(256) k_adjoint(DevGrid g, DevPatches P, DevRays R, AdjointArgs a) {
     __shared__ double sh[2][4];
-    const int64_t nper = (int64_t)R.N * R.N;
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nper = (int64_t)R.N * R.N, nrays = nper * R.panels0;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double vsum = 0., asum = 0.;
-    if (r < nper * R.panels0) {
+    int key = -1;  // patch of the lane's hit (-1: none)
+    HitGrad hg;
+    if (t < nrays * R.K) {
+        const int e = (int)(t / nrays);
+        const int64_t r = t - (int64_t)e * nrays;
         const int c = R.cnt[r];
-        if (c > 0 && c <= R.K && (c & 1) == 0) {
+        if (e < c && c <= R.K && (c & 1) == 0) {
             const int panel = (int)(r / nper);
             const int64_t ray = r - panel * nper;
-            if (panel == 0) ray_adjoint<0>(g, P, R, a, r, ray, c, vsum, asum);
-            else if (panel == 1) ray_adjoint<1>(g, P, R, a, r, ray, c, vsum, asum);
-            else ray_adjoint<2>(g, P, R, a, r, ray, c, vsum, asum);
+            if (panel == 0) hit_slot<0>(g, P, R, a, r, ray, e, vsum, asum, key, hg);
+            else if (panel == 1) hit_slot<1>(g, P, R, a, r, ray, e, vsum, asum, key, hg);
+            else hit_slot<2>(g, P, R, a, r, ray, e, vsum, asum, key, hg);
+        }
+    }
+    // Neighbouring lanes often hit the same patch, so the same atoms: the
+    // derivatives are summed over each run of lanes with one patch (a
+    // segmented scan across the wave) and the run's last lane adds them to
+    // the gradients, one atomic per parameter instead of one per hit.
+    {
+        const int lane = threadIdx.x & 63;
+        if (key < 0) {
+#pragma unroll
+            for (int q = 0; q < 12; ++q) hg.dv[q] = hg.da[q] = 0.;
+        }
+        const int prev = __shfl_up(key, 1, 64);
+        const unsigned long long heads = __ballot(lane == 0 || prev != key);
+        const int start = 63 - __builtin_clzll(heads & ((2ull << lane) - 1ull));
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const bool take = lane - off >= start;
+#pragma unroll
+            for (int q = 0; q < 12; ++q) {
+                const double yv = __shfl_up(hg.dv[q], off, 64), ya = __shfl_up(hg.da[q], off, 64);
+                if (take) {
+                    hg.dv[q] += yv;
+                    hg.da[q] += ya;
+                }
+            }
+        }
+        const int next = __shfl_down(key, 1, 64);
+        if (key >= 0 && (lane == 63 || next != key)) {
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                if (s >= hg.nat) break;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    atomicAdd(a.grad_v + 4 * hg.atom[s] + q, hg.dv[4 * s + q]);
+                    atomicAdd(a.grad_a + 4 * hg.atom[s] + q, hg.da[4 * s + q]);
+                }
+            }
         }
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -215,7 +279,7 @@ __global__ void __launch_bounds__(256) k_adjoint(DevGrid g, DevPatches P, DevRay
     }
 }
 
-int64_t adjoint_blocks(const DevRays &R) { return ((int64_t)R.N * R.N * R.panels0 + 255) / 256; }
+int64_t adjoint_blocks(const DevRays &R) { return ((int64_t)R.N * R.N * R.panels0 * R.K + 255) / 256; }
 
 void launch_adjoint(hipStream_t s, const DevGrid &g, const DevPatches &P, const DevRays &R, const AdjointArgs &a) {
     const int64_t nb = adjoint_blocks(R);
