@@ -343,8 +343,24 @@ int nsgpu_set_patches(nsgpu_ctx *c, const nsgpu_patches *p, double probe_radius)
     for (int i = 0; i < p->n; ++i)
         if (p->kind[i] == NSGPU_PATCH_TORUS_REG || p->kind[i] == NSGPU_PATCH_TORUS_SING) order.push_back(i);
     const int n_skin = p->n - (int)order.size();
+    // skin cells largest first (by the summed faces of their bounding box, the
+    // pixels their rectangles cover over the three panels), so that the waves
+    // with the largest cells start first and the small ones fill in after
+    const size_t skin0 = order.size();
     for (int i = 0; i < p->n; ++i)
         if (p->kind[i] >= NSGPU_PATCH_SKIN_POINT) order.push_back(i);
+    {
+        std::vector<std::pair<double, int>> key;
+        key.reserve(order.size() - skin0);
+        for (size_t q = skin0; q < order.size(); ++q) {
+            const double *b = p->data + p->off[order[q]];
+            const double dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+            key.emplace_back(-(dx * dy + dy * dz + dz * dx), order[q]);
+        }
+        std::stable_sort(key.begin(), key.end(),
+                         [](const std::pair<double, int> &u, const std::pair<double, int> &v) { return u.first < v.first; });
+        for (size_t q = 0; q < key.size(); ++q) order[skin0 + q] = key[q].second;
+    }
     HIP_TRY(c, c->porder.ensure(std::max<size_t>(n, 1) * 4));
     if (n) HIP_TRY(c, hipMemcpyAsync(c->porder.p, order.data(), n * 4, hipMemcpyHostToDevice, c->stream));
     // atom ids (optional: only the adjoint needs them)
