@@ -334,14 +334,22 @@ int nsgpu_set_patches(nsgpu_ctx *c, const nsgpu_patches *p, double probe_radius)
     }
     if (p->n_data)
         HIP_TRY(c, hipMemcpyAsync(c->pdata.p, p->data, (size_t)p->n_data * 8, hipMemcpyHostToDevice, c->stream));
-    // patch ids by class: spheres and facets first, then tori, then skin cells
+    // patch ids by class: spheres and facets first, then tori, then skin
+    // cells; within the SES classes largest bounding sphere first, so that the
+    // waves of the largest rectangles start first
     std::vector<int> order;
     order.reserve(n);
+    auto by_radius = [&](size_t from) {
+        std::stable_sort(order.begin() + (std::ptrdiff_t)from, order.end(),
+                         [&](int u, int v) { return p->bsph[4 * u + 3] > p->bsph[4 * v + 3]; });
+    };
     for (int i = 0; i < p->n; ++i)
         if (p->kind[i] < NSGPU_PATCH_TORUS_REG) order.push_back(i);
+    by_radius(0);
     const int n_plain = (int)order.size();
     for (int i = 0; i < p->n; ++i)
         if (p->kind[i] == NSGPU_PATCH_TORUS_REG || p->kind[i] == NSGPU_PATCH_TORUS_SING) order.push_back(i);
+    by_radius((size_t)n_plain);
     const int n_skin = p->n - (int)order.size();
     // skin cells largest first (by the summed faces of their bounding box, the
     // pixels their rectangles cover over the three panels), so that the waves

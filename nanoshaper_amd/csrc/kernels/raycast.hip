@@ -681,10 +681,13 @@ enum { PC_CAND, PC_DISC, PC_SPHERE, PC_TORUS, PC_ROOTS, PC_HITS, PC_FEAS_ITEMS, 
 template <bool kCount, int kRoots, bool kTorus>
 __global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRays R, double probe,
                                                   unsigned long long *part) {
+    // workgroup = patch rank * nslots + slot: the patches of a class come
+    // largest first, and every slot starts on them together
     const int first = kTorus ? P.n_plain : 0;
-    const int64_t wg = (int64_t)blockIdx.y * P.n + first + blockIdx.x;
-    const int it = P.order[first + blockIdx.x];
-    const int slot = blockIdx.y;
+    const int rank = (int)(blockIdx.x / R.nslots);
+    const int slot = (int)(blockIdx.x % R.nslots);
+    const int64_t wg = (int64_t)slot * P.n + first + rank;
+    const int it = P.order[first + rank];
     const int phase = slot < R.panels0 ? 0 : 1;
     const int panel = phase == 0 ? slot : slot - R.panels0;
     const int kind = P.kind[it];
@@ -1104,12 +1107,12 @@ static void launch_classes(hipStream_t s, const DevGrid &g, const DevPatches &P,
                            unsigned long long *part) {
     if (P.n == 0 || R.nslots == 0) return;
     if (P.n_plain > 0)
-        hipLaunchKernelGGL((k_intersect<kCount, kRoots, false>), dim3((unsigned)P.n_plain, (unsigned)R.nslots),
+        hipLaunchKernelGGL((k_intersect<kCount, kRoots, false>), dim3((unsigned)P.n_plain * (unsigned)R.nslots),
                            dim3(64), 0, s, g, P, R, probe, part);
     if (P.n - P.n_skin > P.n_plain)
         hipLaunchKernelGGL((k_intersect<kCount, kRoots, true>),
-                           dim3((unsigned)(P.n - P.n_skin - P.n_plain), (unsigned)R.nslots), dim3(64), 0, s, g, P, R,
-                           probe, part);
+                           dim3((unsigned)(P.n - P.n_skin - P.n_plain) * (unsigned)R.nslots), dim3(64), 0, s, g,
+                           P, R, probe, part);
     if (!kCount && P.n_skin > 0) launch_intersect_skin(s, g, P, R, part);
 }
 void launch_intersect(hipStream_t s, const DevGrid &g, const DevPatches &P, const DevRays &R, double probe,
