@@ -290,6 +290,11 @@ __global__ void __launch_bounds__(256) k_mc_compact(DevGrid g, SegIter si, const
 __global__ void __launch_bounds__(256) k_mc_counts(DevGrid g, int nact, const int *anode, const uint8_t *acase,
                                                    const int *idx, int *nv, int *nt, int *aedge) {
     const int q = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    // a full wave's records (3 KB, contiguous) go out through LDS as three
+    // contiguous 1 KB stores: stored per lane, each store instruction left
+    // partial lines at a 48-byte stride
+    __shared__ int4 stage[4][3 * 64];
+    const bool full = __ballot(q < nact) == ~0ull;
     if (q >= nact) return;
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (int64_t)g.nx * g.ny;
     const int64_t f = anode[q];
@@ -303,10 +308,23 @@ __global__ void __launch_bounds__(256) k_mc_counts(DevGrid g, int nact, const in
         ids[e] = idx[edge_slot(e, f, sy, sz, nvox)];
         have |= (ids[e] != -1 ? 1u : 0u) << e;
     }
-    int4 *rec = reinterpret_cast<int4 *>(aedge + 12 * (int64_t)q);
-    rec[0] = make_int4(ids[0], ids[1], ids[2], ids[3]);
-    rec[1] = make_int4(ids[4], ids[5], ids[6], ids[7]);
-    rec[2] = make_int4(ids[8], ids[9], ids[10], ids[11]);
+    if (full) {
+        // a wave's LDS accesses are performed in order, so the reads see the
+        // other lanes' writes; the wave barrier keeps the compiler from moving them
+        const int lane = threadIdx.x & 63;
+        int4 *w = stage[threadIdx.x >> 6];
+        w[3 * lane + 0] = make_int4(ids[0], ids[1], ids[2], ids[3]);
+        w[3 * lane + 1] = make_int4(ids[4], ids[5], ids[6], ids[7]);
+        w[3 * lane + 2] = make_int4(ids[8], ids[9], ids[10], ids[11]);
+        __builtin_amdgcn_wave_barrier();
+        int4 *dst = reinterpret_cast<int4 *>(aedge + 12 * (int64_t)(q - lane));
+        for (int v = 0; v < 3; ++v) dst[lane + 64 * v] = w[lane + 64 * v];
+    } else {
+        int4 *rec = reinterpret_cast<int4 *>(aedge + 12 * (int64_t)q);
+        rec[0] = make_int4(ids[0], ids[1], ids[2], ids[3]);
+        rec[1] = make_int4(ids[4], ids[5], ids[6], ids[7]);
+        rec[2] = make_int4(ids[8], ids[9], ids[10], ids[11]);
+    }
     int v = 0;
     v += edge_on(c, 5) && ids[5] == -1;
     v += edge_on(c, 6) && ids[6] == -1;
@@ -421,11 +439,21 @@ __global__ void __launch_bounds__(256) k_mc_twrite(DevGrid g, int nact, const in
                                                    const int *aedge, const int *idx, const int *toff,
                                                    const double *verts, int *tris, double *tarea, int reverse) {
     const int q = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (q >= nact) return;
+    // The valid lanes of a wave are a prefix and their triangles one
+    // contiguous range from lane 0's offset (at most 5 per cube): they are
+    // staged in LDS and the wave stores the range with consecutive lanes.
+    // Stored per lane at its own offset, every store left partial lines.
+    __shared__ int stri[4][3 * 5 * 64];
+    __shared__ double sarea[4][5 * 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const bool valid = q < nact;
+    const unsigned long long vm = __ballot(valid);
+    if (vm == 0) return;
+    const int base = __shfl(valid ? toff[q] : 0, 0, 64);
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (int64_t)g.nx * g.ny;
-    const int64_t f = anode[q];
-    const uint64_t w = c_tri_cases[acase[q]];
-    int o = toff[q];
+    const int64_t f = valid ? anode[q] : 0;
+    const uint64_t w = valid ? c_tri_cases[acase[q]] : ~0ull;
+    int o = valid ? toff[q] : base;
     for (int t = 0; t < 5; ++t) {
         const int e0 = (int)((w >> (12 * t)) & 15), e1 = (int)((w >> (12 * t + 4)) & 15),
                   e2 = (int)((w >> (12 * t + 8)) & 15);
@@ -435,9 +463,10 @@ __global__ void __launch_bounds__(256) k_mc_twrite(DevGrid g, int nact, const in
         const int v1 = edge_vertex(rec, idx, e1, f, sy, sz, nvox);
         const int v2 = edge_vertex(rec, idx, e2, f, sy, sz, nvox);
         if (v0 == -1 || v1 == -1 || v2 == -1) continue;
-        tris[3 * (int64_t)o + 0] = reverse ? v2 : v0;
-        tris[3 * (int64_t)o + 1] = v1;
-        tris[3 * (int64_t)o + 2] = reverse ? v0 : v2;
+        const int so = o - base;  // < 5 * 64: at most five triangles per lane before it
+        stri[wv][3 * so + 0] = reverse ? v2 : v0;
+        stri[wv][3 * so + 1] = v1;
+        stri[wv][3 * so + 2] = reverse ? v0 : v2;
         const double *a0 = verts + 3 * (int64_t)v0, *a1 = verts + 3 * (int64_t)v1, *a2 = verts + 3 * (int64_t)v2;
         const double a = sqrt((a0[0] - a1[0]) * (a0[0] - a1[0]) + (a0[1] - a1[1]) * (a0[1] - a1[1]) +
                               (a0[2] - a1[2]) * (a0[2] - a1[2]));
@@ -446,9 +475,14 @@ __global__ void __launch_bounds__(256) k_mc_twrite(DevGrid g, int nact, const in
         const double cc = sqrt((a1[0] - a2[0]) * (a1[0] - a2[0]) + (a1[1] - a2[1]) * (a1[1] - a2[1]) +
                                (a1[2] - a2[2]) * (a1[2] - a2[2]));
         const double ttt = (a + b + cc) * (b + cc - a) * (cc + a - b) * (a + b - cc);
-        tarea[o] = ttt > 0 ? 0.25 * sqrt(ttt) : 0.0;
+        sarea[wv][so] = ttt > 0 ? 0.25 * sqrt(ttt) : 0.0;
         ++o;
     }
+    // the wave's end: the last valid lane's
+    const int n = __shfl(o, 63 - __builtin_clzll(vm), 64) - base;
+    __builtin_amdgcn_wave_barrier();
+    for (int x = lane; x < 3 * n; x += 64) tris[3 * (int64_t)base + x] = stri[wv][x];
+    for (int x = lane; x < n; x += 64) tarea[base + x] = sarea[wv][x];
 }
 
 static inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
