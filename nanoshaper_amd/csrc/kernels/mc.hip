@@ -118,27 +118,62 @@ __global__ void __launch_bounds__(256) k_vtx_count(DevGrid g, DevRays R, EdgeSlo
                               [](int, int, int64_t, double, double, double, int) {});
 }
 
+// The vertices of a wave's rays are one contiguous range from lane 0's
+// offset (the offsets are the scanned counts).  Its first kVtxStage vertices
+// are staged in LDS and stored by consecutive lanes after the walk; any beyond
+// go straight out.  Stored per lane at its own offset, every store of a
+// vertex's 24 bytes, slot and depth id left partial lines.
+constexpr int kVtxStage = 192;
+
 __global__ void __launch_bounds__(256) k_vtx_write(DevGrid g, DevRays R, EdgeSlots es, const uint32_t *bits,
                                                    const int *offs, double *verts, int *idx, int *vslot,
                                                    int64_t *vhit, int64_t vcap) {
+    __shared__ double sv[4][3 * kVtxStage];
+    __shared__ int ss[4][kVtxStage];
+    __shared__ int64_t sh[4][kVtxStage];
     const int64_t NN = (int64_t)R.N * R.N;
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= 3 * NN) return;
-    const int panel = (int)(r / NN);
-    const int base = offs[r];
-    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
-    walk_edge_ray(g, R, bits, panel, es.s[panel], r - panel * NN,
-                  [&](int q, int d, int64_t f, double px, double py, double pz, int hit) {
-                      const int v = base + q;
-                      if (v >= vcap) return;
-                      verts[3 * (int64_t)v + 0] = px;
-                      verts[3 * (int64_t)v + 1] = py;
-                      verts[3 * (int64_t)v + 2] = pz;
-                      idx[3 * f + d] = v;
-                      vslot[v] = (int)(3 * f + d);
-                      // the depth this vertex comes from, for its analytic normal
-                      vhit[v] = ((int64_t)es.s[panel] * NN + (r - panel * NN)) * R.K + hit;
-                  });
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const bool valid = r < 3 * NN;
+    const unsigned long long vm = __ballot(valid);
+    if (vm == 0) return;
+    const int base = valid ? offs[r] : 0;
+    const int wbase = __shfl(base, 0, 64);
+    int cnt = 0;
+    if (valid) {
+        const int panel = (int)(r / NN);
+        cnt = walk_edge_ray(g, R, bits, panel, es.s[panel], r - panel * NN,
+                            [&](int q, int d, int64_t f, double px, double py, double pz, int hit) {
+                                const int v = base + q;
+                                if (v >= vcap) return;
+                                idx[3 * f + d] = v;
+                                // the depth this vertex comes from, for its analytic normal
+                                const int64_t h = ((int64_t)es.s[panel] * NN + (r - panel * NN)) * R.K + hit;
+                                const int so = v - wbase;
+                                if (so < kVtxStage) {
+                                    sv[wv][3 * so + 0] = px;
+                                    sv[wv][3 * so + 1] = py;
+                                    sv[wv][3 * so + 2] = pz;
+                                    ss[wv][so] = (int)(3 * f + d);
+                                    sh[wv][so] = h;
+                                } else {
+                                    verts[3 * (int64_t)v + 0] = px;
+                                    verts[3 * (int64_t)v + 1] = py;
+                                    verts[3 * (int64_t)v + 2] = pz;
+                                    vslot[v] = (int)(3 * f + d);
+                                    vhit[v] = h;
+                                }
+                            });
+    }
+    // staged: v in [wbase, wbase + n), below vcap and the stage's size
+    const int64_t wend = (int64_t)__shfl(base + cnt, 63 - __builtin_clzll(vm), 64);
+    const int n = (int)max((int64_t)0, min(min(wend, vcap) - wbase, (int64_t)kVtxStage));
+    __builtin_amdgcn_wave_barrier();
+    for (int x = lane; x < 3 * n; x += 64) verts[3 * (int64_t)wbase + x] = sv[wv][x];
+    for (int x = lane; x < n; x += 64) {
+        vslot[wbase + x] = ss[wv][x];
+        vhit[wbase + x] = sh[wv][x];
+    }
 }
 
 // ---------------------------------------------------------------- marching cubes
