@@ -16,39 +16,30 @@ __device__ __forceinline__ bool cov_bit(const unsigned long long *plane, int64_t
     return (plane[(int64_t)(n >> 6) * NN + ray] >> (n & 63)) & 1ull;
 }
 
-// A wave takes kWaveNodes consecutive nodes of the flat order, lane l the
-// nodes l, l + 64, l + 128, ... of them: every coverage-word load reads 64
-// consecutive words (the z- and y-ray words of consecutive i are consecutive
-// words of their planes; the x-ray words are shared by a row), where four
-// consecutive nodes per lane spread each load over 16 lines.  The epsmap faces
-// (12 bytes a node) and idebmap bytes are staged in LDS and stored as
-// contiguous 16- and 4-byte lane stores; the status words go out as 256-byte
-// rows.
-constexpr int kFillNodes = 4, kWaveNodes = 64 * kFillNodes;
+// A lane takes kFillNodes consecutive nodes of the flat order, so its epsmap
+// faces (12 bytes a node), idebmap bytes and status words are aligned 16-byte
+// stores; the x-ray words are shared by the lanes of a row and the z- and
+// y-ray words of consecutive i are consecutive words of their planes.
+constexpr int kFillNodes = 4;  // 8 was slower (0.25 vs 0.22 ms at 325^3)
 
 __global__ void __launch_bounds__(256) k_fill_bits(DevGrid g, DevRays R, FillOpts o, int *epsmap, uint8_t *idebmap,
                                                    int *status) {
-    __shared__ int se[4][3 * kWaveNodes];
-    __shared__ uint32_t sd[4][kWaveNodes / 4];
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int64_t w0 = kWaveNodes * ((int64_t)xcd_block(blockIdx.x, gridDim.x) * 4 + wv);
-    if (w0 >= nvox) return;
-    // corner words of nodes w0 .. w0 + kWaveNodes - 1 (w0 is a multiple of 32), below ceil(nvox / 32)
-    if (o.init_bits && lane < kWaveNodes / 32 && w0 + 32 * lane < nvox) o.init_bits[(w0 >> 5) + lane] = 0xffffffffu;
+    const int64_t f0 = kFillNodes * ((int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
+    if (f0 >= nvox) return;
+    if (o.init_bits && (f0 & 31) == 0) o.init_bits[f0 >> 5] = 0xffffffffu;  // word f0 / 32, < ceil(nvox / 32)
     const int64_t NN = (int64_t)R.N * R.N, SW = (int64_t)R.W * NN;
     const int64_t plane = (int64_t)g.nx * g.ny;
-    const int64_t f0 = w0 + lane;
     int k = (int)(f0 / plane);
     const int64_t rem = f0 - (int64_t)k * plane;
     int j = (int)(rem / g.nx);
     int i = (int)(rem - (int64_t)j * g.nx);
-    const bool full = w0 + kWaveNodes <= nvox;
-    uint8_t *sdb = reinterpret_cast<uint8_t *>(sd[wv]);
+    int eps[3 * kFillNodes], st[kFillNodes];
+    uint8_t ideb[kFillNodes];
+    const int nq = (int)min((int64_t)kFillNodes, nvox - f0);
     for (int q = 0; q < kFillNodes; ++q) {
-        const int64_t f = f0 + 64 * q;
-        if (f < nvox) {
-            bool e0 = false, in = false, e1 = false, e2 = false;
+        bool e0 = false, in = false, e1 = false, e2 = false;
+        if (q < nq) {
             if (o.slot_x >= 0) {
                 const int64_t ray = (int64_t)k * R.N + j;
                 e0 = cov_bit(R.cov + o.slot_x * SW, NN, ray, i);
@@ -58,41 +49,60 @@ __global__ void __launch_bounds__(256) k_fill_bits(DevGrid g, DevRays R, FillOpt
                 e2 = o.slot_z >= 0 && cov_bit(R.cov + o.slot_z * SW, NN, (int64_t)j * R.N + i, k);
                 e1 = o.slot_y >= 0 && cov_bit(R.cov + o.slot_y * SW, NN, (int64_t)k * R.N + i, j);
             }
-            const int x0 = e0 ? o.inside : 0, x1 = e1 ? o.inside : 0, x2 = e2 ? o.inside : 0;
-            const uint8_t db = in ? 0 : 1;
-            if (o.build_status) status[f] = in ? NSGPU_STATUS_INSIDE : o.out_code;
-            if (full) {
-                const int n = lane + 64 * q;
-                se[wv][3 * n + 0] = x0;
-                se[wv][3 * n + 1] = x1;
-                se[wv][3 * n + 2] = x2;
-                sdb[n] = db;
-            } else if (o.build_eps) {
-                epsmap[3 * f + 0] = x0;
-                epsmap[3 * f + 1] = x1;
-                epsmap[3 * f + 2] = x2;
-                idebmap[f] = db;
-            }
         }
-        // next node: 64 further in the flat order
-        i += 64;
-        while (i >= g.nx) {
-            i -= g.nx;
+        eps[3 * q + 0] = e0 ? o.inside : 0;
+        eps[3 * q + 1] = e1 ? o.inside : 0;
+        eps[3 * q + 2] = e2 ? o.inside : 0;
+        ideb[q] = in ? 0 : 1;
+        st[q] = in ? NSGPU_STATUS_INSIDE : o.out_code;
+        if (++i == g.nx) {
+            i = 0;
             if (++j == g.ny) {
                 j = 0;
                 ++k;
             }
         }
     }
-    if (full && o.build_eps) {
-        // a wave's LDS accesses are performed in order, so the reads see the
-        // other lanes' writes; the wave barrier keeps the compiler from moving them
-        __builtin_amdgcn_wave_barrier();
-        const int4 *src = reinterpret_cast<const int4 *>(se[wv]);
-        int4 *dst = reinterpret_cast<int4 *>(epsmap + 3 * w0);
-        for (int v = 0; v < 3 * kFillNodes / 4; ++v) dst[lane + 64 * v] = src[lane + 64 * v];
-        for (int v = 0; v < kFillNodes / 4; ++v)
-            reinterpret_cast<uint32_t *>(idebmap + w0)[lane + 64 * v] = sd[wv][lane + 64 * v];
+    // A full wave stages its 256 nodes' epsmap (3 KB) in LDS, so that each of
+    // its three 16-byte stores writes one contiguous 1 KB: stored from the
+    // lanes' own 48-byte runs, every store instruction left partial lines at a
+    // 48-byte stride and the map fill wrote at half the bandwidth of a plain
+    // fill.  A wave's LDS accesses are performed in order, so the reads see
+    // the other lanes' writes; the wave barrier keeps the compiler from
+    // moving them.
+    __shared__ int4 stage[4][3 * 64];
+    const bool full = __ballot(nq == kFillNodes) == ~0ull;
+    if (nq == kFillNodes) {
+        if (o.build_eps && full) {
+            const int lane = threadIdx.x & 63;
+            int4 *w = stage[threadIdx.x >> 6];
+            for (int v = 0; v < 3; ++v) w[3 * lane + v] = make_int4(eps[4 * v], eps[4 * v + 1], eps[4 * v + 2], eps[4 * v + 3]);
+            __builtin_amdgcn_wave_barrier();
+            int4 *dst = reinterpret_cast<int4 *>(epsmap + 3 * (f0 - kFillNodes * lane));
+            for (int v = 0; v < 3; ++v) dst[lane + 64 * v] = w[lane + 64 * v];
+        } else if (o.build_eps) {
+            int4 *e = reinterpret_cast<int4 *>(epsmap + 3 * f0);
+            for (int v = 0; v < 3 * kFillNodes / 4; ++v)
+                e[v] = make_int4(eps[4 * v], eps[4 * v + 1], eps[4 * v + 2], eps[4 * v + 3]);
+        }
+        if (o.build_eps) {
+            uint32_t *iw = reinterpret_cast<uint32_t *>(idebmap + f0);
+            for (int v = 0; v < kFillNodes / 4; ++v)
+                iw[v] = (uint32_t)ideb[4 * v] | ((uint32_t)ideb[4 * v + 1] << 8) | ((uint32_t)ideb[4 * v + 2] << 16) |
+                        ((uint32_t)ideb[4 * v + 3] << 24);
+        }
+        if (o.build_status) {
+            int4 *sv = reinterpret_cast<int4 *>(status + f0);
+            for (int v = 0; v < kFillNodes / 4; ++v) sv[v] = make_int4(st[4 * v], st[4 * v + 1], st[4 * v + 2], st[4 * v + 3]);
+        }
+    } else {
+        for (int q = 0; q < nq; ++q) {
+            if (o.build_eps) {
+                for (int l = 0; l < 3; ++l) epsmap[3 * (f0 + q) + l] = eps[3 * q + l];
+                idebmap[f0 + q] = ideb[q];
+            }
+            if (o.build_status) status[f0 + q] = st[q];
+        }
     }
 }
 
@@ -160,10 +170,9 @@ __global__ void __launch_bounds__(256) k_corners_bits(DevGrid g, DevRays R, Fill
 
 void launch_fill_bits(hipStream_t s, const DevGrid &g, const DevRays &R, const FillOpts &o, int *eps, uint8_t *ideb,
                       int *status) {
-    // four waves of kWaveNodes nodes per workgroup
-    const int64_t nodes = (int64_t)g.nx * g.ny * g.nz;
-    hipLaunchKernelGGL(k_fill_bits, dim3((unsigned)((nodes + 4 * kWaveNodes - 1) / (4 * kWaveNodes))), dim3(256), 0,
-                       s, g, R, o, eps, ideb, status);
+    const int64_t groups = ((int64_t)g.nx * g.ny * g.nz + kFillNodes - 1) / kFillNodes;
+    hipLaunchKernelGGL(k_fill_bits, dim3((unsigned)((groups + 255) / 256)), dim3(256), 0, s, g, R, o, eps, ideb,
+                       status);
 }
 
 void launch_corners_bits(hipStream_t s, const DevGrid &g, const DevRays &R, const FillOpts &o, uint32_t *bits) {
