@@ -174,7 +174,8 @@ void nsgpu_jitter(double *xyzr, int32_t n, double amplitude, int32_t *seed);
 /* SES patch construction, the host stage ConnollySurface::build
  * (ConnollySurface.cpp:2326-3316) emits: point, facet and edge cells with
  * their clipping geometry.  Built here without CGAL by free-probe enumeration
- * (see DESIGN.md).  The patch set is owned by the library. */
+ * (every probe position tangent to three atoms that no atom overlaps).  The
+ * patch set is owned by the library. */
 typedef struct nsgpu_patch_set nsgpu_patch_set;
 int nsgpu_ses_build(const double *xyzr, int32_t n, double probe_radius,
                     nsgpu_patch_set **out);

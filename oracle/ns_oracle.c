@@ -10,8 +10,8 @@
  * map writes are those of the reference's FP64 code.  It is single threaded:
  * the reference's thread partition only changes the order of equal work.
  *
- * Parity status of each part is recorded in DESIGN.md; the checks that pin
- * each function are in tests/test_oracle.py.
+ * The checks that pin each function (golden vectors, closed forms,
+ * invariants, brute force on small inputs) are in tests/test_oracle.py.
  */
 #include <math.h>
 #include <stdint.h>
