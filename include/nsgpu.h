@@ -159,6 +159,21 @@ const char *nsgpu_last_error(const nsgpu_ctx *ctx);
 int nsgpu_abi_version(void);
 
 /* -------------------- host stages that stay on the CPU --------------------- */
+/* DelPhiShared::loadAtoms (DelphiShared.cpp:309-702): atoms of an .xyzr file
+ * ("x y z r" per line) or, when the path contains ".pqr", of the ATOM/HETATM
+ * records of a PQR file (10 or 11 fields, radius below 1e-20 skipped).
+ * max_atoms > 0 stops after that many (Max_Num_Atoms); domain_shrinkage in
+ * (0, 1) keeps the atoms inside the centred bounding box shrunk by that factor
+ * (Domain_Shrinkage).  Fewer than 4 atoms or all radii null fail, as in the
+ * reference.  The set is library-owned; on failure *out still holds the
+ * message (nsgpu_atom_set_error) and must be freed. */
+typedef struct nsgpu_atom_set nsgpu_atom_set;
+int nsgpu_load_atoms(const char *path, int32_t max_atoms, double domain_shrinkage, nsgpu_atom_set **out);
+/* number of atoms; *xyzr (n*4 doubles) and *charge (n, zeros for xyzr files) */
+int32_t nsgpu_atom_set_view(const nsgpu_atom_set *s, const double **xyzr, const double **charge);
+const char *nsgpu_atom_set_error(const nsgpu_atom_set *s);
+void nsgpu_atom_set_free(nsgpu_atom_set *s);
+
 /* DelPhiShared::buildGrid(scale, perfil) (DelphiShared.cpp:1038-1201),
  * bounds from DelPhiShared::getBounds (DelphiShared.cpp:775-797).
  * xyzr: n*4 doubles (x, y, z, radius). */
@@ -332,7 +347,8 @@ int nsgpu_batch_create(int device, int32_t n_frames, nsgpu_batch **out);
 void nsgpu_batch_destroy(nsgpu_batch *batch);
 const char *nsgpu_batch_last_error(const nsgpu_batch *batch);
 nsgpu_ctx *nsgpu_batch_frame(nsgpu_batch *batch, int32_t frame);
-/* nsgpu_set_grid + nsgpu_set_patches of one frame (drops the captured graph). */
+/* nsgpu_set_grid + nsgpu_set_patches of one frame (the graph is captured
+ * again at the next nsgpu_batch_run). */
 int nsgpu_batch_set_frame(nsgpu_batch *batch, int32_t frame, const nsgpu_grid *grid,
                           const nsgpu_patches *patches, double probe_radius);
 /* Runs every frame once eagerly (sizing its buffers and ray lists), then
@@ -345,8 +361,11 @@ int32_t nsgpu_batch_chains(const nsgpu_batch *batch);
 /* One launch of the graph.  latency_ms[n_frames] (may be NULL) receives each
  * frame's completion time after the launch (device clock); *redone (may be
  * NULL) the frames that were finished on the host path (ray lists outgrown,
- * which also drops the graph, or more cavity components than the device
- * numbering holds). */
+ * or more cavity components than the device numbering holds).  A frame whose
+ * context changed since the capture (new inputs or options through
+ * nsgpu_batch_frame, a device buffer that grew, a redone frame) is first run
+ * eagerly and the graph captured again, so the launch never reads a freed
+ * buffer. */
 int nsgpu_batch_run(nsgpu_batch *batch, double *latency_ms, int32_t *redone);
 /* The per-frame launch loop over the same frames: nsgpu_get_surf on each in
  * turn; latency_ms[n_frames] (may be NULL) receives each call's wall time. */

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <fstream>
 #include <map>
 #include <sstream>
@@ -92,6 +93,23 @@ class DelPhiShared {
         if (xyzr.empty() || xyzr.size() % 4) throw std::runtime_error("atoms must be x,y,z,r quadruples");
         atoms = xyzr;
     }
+    // DelPhiShared::loadAtoms(fn) (DelphiShared.cpp:309-702): an .xyzr or
+    // .pqr file; false (with the message in loadError) when it cannot be read
+    bool loadAtoms(const std::string &fn, int maxNumAtoms = -1, double domainShrinkage = 0) {
+        nsgpu_atom_set *set = nullptr;
+        const int rc = nsgpu_load_atoms(fn.c_str(), maxNumAtoms, domainShrinkage, &set);
+        if (rc != NSGPU_OK) {
+            loadError = set ? nsgpu_atom_set_error(set) : "cannot load atoms";
+            nsgpu_atom_set_free(set);
+            return false;
+        }
+        const double *x = nullptr, *q = nullptr;
+        const int32_t n = nsgpu_atom_set_view(set, &x, &q);
+        atoms.assign(x, x + 4 * (size_t)n);
+        charges.assign(q, q + n);
+        nsgpu_atom_set_free(set);
+        return true;
+    }
     bool buildGrid(double scale_, double perfil_) {
         return nsgpu_build_grid(atoms.data(), (int32_t)(atoms.size() / 4), scale_, perfil_, &grid) == NSGPU_OK;
     }
@@ -99,6 +117,8 @@ class DelPhiShared {
 
     bool buildEpsMap, buildStatus;
     std::vector<double> atoms;
+    std::vector<double> charges;  // pqr charges (zeros for xyzr)
+    std::string loadError;
     nsgpu_grid grid{};
     std::vector<int32_t> epsmap;   // [((k*ny+j)*nx+i)*3+l]
     std::vector<uint8_t> idebmap;  // 1 = out
@@ -148,7 +168,8 @@ class Surface {
         if (nsgpu_set_vertex_normals(ctx_, computeNormals) != NSGPU_OK) return false;
         if (nsgpu_get_surf(ctx_, &o, &last_) != NSGPU_OK) return false;
         if (surf_volume) *surf_volume = last_.volume;
-        const int64_t nv = delphi->numNodes();
+        // the maps are those of the grid on the device (delphi->grid, uploaded by ensure_device)
+        const int64_t nv = (int64_t)grid_.nx * grid_.ny * grid_.nz;
         if (delphi->buildEpsMap) {
             delphi->epsmap.resize(nv * 3);
             delphi->idebmap.resize(nv);
@@ -160,7 +181,9 @@ class Surface {
             if (nsgpu_read_status(ctx_, delphi->status.data())) return false;
             delphi->cavitiesSize.resize(last_.num_cavities);
             delphi->cavitiesFlag.resize(last_.num_cavities);
-            nsgpu_read_cavities(ctx_, delphi->cavitiesSize.data(), delphi->cavitiesFlag.data());
+            if (last_.num_cavities > 0 &&
+                nsgpu_read_cavities(ctx_, delphi->cavitiesSize.data(), delphi->cavitiesFlag.data()) != NSGPU_OK)
+                return false;
         }
         return true;
     }
@@ -206,10 +229,14 @@ class Surface {
         const nsgpu_patches *p = patches();
         return p && nsgpu_set_patches(ctx_, p, probe_radius) == NSGPU_OK;
     }
+    // the device holds delphi's current grid and the surface: a grid rebuilt
+    // since the last upload (another scale, other atoms) is uploaded again
     bool ensure_device() {
         if (!ctx_ && nsgpu_create(device, &ctx_) != NSGPU_OK) return false;
+        if (uploaded_ && std::memcmp(&grid_, &delphi->grid, sizeof(nsgpu_grid)) != 0) uploaded_ = false;
         if (!uploaded_) {
             if (nsgpu_set_grid(ctx_, &delphi->grid) != NSGPU_OK) return false;
+            grid_ = delphi->grid;
             if (!upload()) return false;
             uploaded_ = true;
         }
@@ -219,6 +246,7 @@ class Surface {
     std::string err_;
     nsgpu_ctx *ctx_ = nullptr;
     bool uploaded_ = false;
+    nsgpu_grid grid_{};  // the grid on the device
     nsgpu_surf_result last_{};
 };
 
@@ -337,19 +365,21 @@ class BlobbySurface : public Surface {
             return false;
         }
         if (!delphi || delphi->atoms.empty()) return false;
-        if (!ctx_ && nsgpu_create(device, &ctx_) != NSGPU_OK) return false;
-        if (nsgpu_set_grid(ctx_, &delphi->grid) != NSGPU_OK) return false;
-        if (nsgpu_blobby_build(ctx_, delphi->atoms.data(), (int32_t)(delphi->atoms.size() / 4), B, cutoff,
-                               auxGrid2D, &info) != NSGPU_OK)
-            return false;
-        uploaded_ = true;
-        return true;
+        uploaded_ = false;
+        return ensure_device();
     }
     int getNumPatches() override { return 0; }  // BlobbySurface.cpp:71-74
     const nsgpu_patches *patches() override { return nullptr; }
     double B = -2.5;
     double cutoff = 6.0;  // DEFAULT_CUTOFF, BlobbySurface.h:19
     nsgpu_mesh_info info{};
+
+   protected:
+    // the blobby mesh lives on the grid: built again whenever the grid is
+    bool upload() override {
+        return nsgpu_blobby_build(ctx_, delphi->atoms.data(), (int32_t)(delphi->atoms.size() / 4), B, cutoff,
+                                  auxGrid2D, &info) == NSGPU_OK;
+    }
 };
 
 typedef Surface *(*surface_instantiator)(ConfigFile *, DelPhiShared *);

@@ -54,6 +54,10 @@ SIGNATURES = {
     "nsgpu_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(VP)]),
     "nsgpu_destroy": (None, [VP]),
     "nsgpu_last_error": (ctypes.c_char_p, [VP]),
+    "nsgpu_load_atoms": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_double, ctypes.POINTER(VP)]),
+    "nsgpu_atom_set_view": (ctypes.c_int32, [VP, ctypes.POINTER(c_double_p), ctypes.POINTER(c_double_p)]),
+    "nsgpu_atom_set_error": (ctypes.c_char_p, [VP]),
+    "nsgpu_atom_set_free": (None, [VP]),
     "nsgpu_build_grid": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.POINTER(Grid)]),
     "nsgpu_jitter": (None, [VP, ctypes.c_int32, ctypes.c_double, c_int32_p]),
     "nsgpu_ses_build": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_double, ctypes.POINTER(VP)]),

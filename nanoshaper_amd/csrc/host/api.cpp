@@ -21,16 +21,19 @@ namespace {
 // set while a graph is being captured: buffers must already be large enough
 thread_local bool g_no_alloc = false;
 
-// grow-only device buffer
+// grow-only device buffer; serial counts its (re)allocations, so that a
+// captured graph holding its address can tell that it moved
 struct DBuf {
     void *p = nullptr;
     size_t cap = 0;
+    uint32_t serial = 0;
     hipError_t ensure(size_t bytes) {
         if (bytes <= cap && p) return hipSuccess;
         if (g_no_alloc) return hipErrorOutOfMemory;
         if (p) (void)hipFree(p);
         p = nullptr;
         cap = 0;
+        ++serial;
         size_t b = std::max<size_t>(bytes, 256);
         hipError_t e = hipMalloc(&p, b);
         if (e == hipSuccess) cap = b;
@@ -40,6 +43,7 @@ struct DBuf {
         if (p) (void)hipFree(p);
         p = nullptr;
         cap = 0;
+        ++serial;
     }
     template <class T>
     T *as() const { return (T *)p; }
@@ -140,6 +144,10 @@ struct nsgpu_ctx {
     bool have_surf = false, have_vertices = false, have_mesh = false;
     nsgpu_surf_opts opts{};
     nsgpu_surf_result res{};
+    // bumped by every call that replaces the context's inputs (grid,
+    // patches, mesh, atoms); with the buffers' serials it tells a batch
+    // whether its captured graph still matches this context
+    uint64_t setgen = 0;
     // stage timing
     bool profile = false;
     hipEvent_t ev[2 * ST_COUNT] = {};
@@ -232,10 +240,11 @@ int nsgpu_create(int device, nsgpu_ctx **out) {
     return NSGPU_OK;
 }
 
-void nsgpu_destroy(nsgpu_ctx *c) {
-    if (!c) return;
-    (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
+}  // extern "C"
+
+// every device buffer of a context
+template <class F>
+static void for_each_buf(nsgpu_ctx *c, F fn) {
     DBuf *all[] = {&c->gx, &c->gy, &c->gz, &c->pkind, &c->poff, &c->pcnt, &c->pflags, &c->pbsph, &c->pdata, &c->porder, &c->patom, &c->atoms, &c->rpid, &c->gradv, &c->grada, &c->asums,
                    &c->rcnt, &c->rt, &c->rvol, &c->partial, &c->stats, &c->cand, &c->cov, &c->covst, &c->eps, &c->ideb, &c->status,
                    &c->bits, &c->crx0, &c->crx1, &c->cpar, &c->crowcnt, &c->croff, &c->nroots, &c->rootlist, &c->bkey, &c->ccount, &c->clabel, &c->cfilled, &c->dcavsize, &c->dcavfilled, &c->hostdone, &c->dres, &c->dnv, &c->dgather, &c->aedge, &c->clist, &c->clrow, &c->cubeidx,
@@ -243,7 +252,24 @@ void nsgpu_destroy(nsgpu_ctx *c) {
                    &c->segmask, &c->anode, &c->acase, &c->anv, &c->ant, &c->avoff, &c->atoff, &c->mverts, &c->mtris,
                    &c->mrr, &c->mraw, &c->field, &c->fbits, &c->batoms, &c->bbox, &c->bcount, &c->boff, &c->bcur,
                    &c->blist, &c->flag};
-    for (DBuf *b : all) b->release();
+    for (DBuf *b : all) fn(b);
+}
+
+// state a captured graph depends on: the inputs, every buffer address (its
+// allocation serial) and the ray-list capacity
+static uint64_t ctx_generation(nsgpu_ctx *c) {
+    uint64_t h = c->setgen * 0x9e3779b97f4a7c15ull + (uint64_t)c->K;
+    for_each_buf(c, [&](DBuf *b) { h = h * 1000003ull + b->serial; });
+    return h;
+}
+
+extern "C" {
+
+void nsgpu_destroy(nsgpu_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for_each_buf(c, [](DBuf *b) { b->release(); });
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->hres) (void)hipHostFree(c->hres);
@@ -281,6 +307,7 @@ int nsgpu_set_grid(nsgpu_ctx *c, const nsgpu_grid *g) {
     if (!c || !g || g->nx <= 1 || g->ny <= 1 || g->nz <= 1) return NSGPU_E_ARG;
     if ((int64_t)g->nx * g->ny * g->nz >= (int64_t)1 << 31)
         return fail(c, NSGPU_E_ARG, "grid exceeds 2^31 nodes");
+    ++c->setgen;
     HIP_TRY(c, hipSetDevice(c->device));
     c->grid = *g;
     std::vector<double> x(g->nx), y(g->ny), z(g->nz);
@@ -306,6 +333,7 @@ int nsgpu_set_patches(nsgpu_ctx *c, const nsgpu_patches *p, double probe_radius)
     if (!c || !p || p->n < 0 || p->n_data < 0 || !(probe_radius > 0)) return NSGPU_E_ARG;
     if (p->n > 0 && (!p->kind || !p->off || !p->cnt || !p->flags || !p->bsph || (!p->data && p->n_data)))
         return NSGPU_E_ARG;
+    ++c->setgen;
     for (int i = 0; i < p->n; ++i) {
         const int k = p->kind[i];
         if (k < NSGPU_PATCH_POINT || k > NSGPU_PATCH_SKIN_TETRA) return fail(c, NSGPU_E_ARG, "bad patch kind");
@@ -928,11 +956,11 @@ static int assemble(nsgpu_ctx *c, const int eslots[3]) {
     HIP_TRY(c, c->voffs.ensure((size_t)nmax * 4));
     HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(nmax) * 4));
     // vertex index slots back to -1: only the ones the last vertices used,
-    // unless the buffer is new, the grid changed or slots exceed int32
-    if (c->idx.p != old_idx || c->idx_nvox != nvox || 3 * nvox > INT32_MAX)
+    // unless the buffer is new or the grid changed
+    if (c->idx.p != old_idx || c->idx_nvox != nvox)
         launch_fill32(c->stream, c->idx.p, 0xffffffffu, nvox * 3);
     else
-        launch_idx_reset(c->stream, c->idx.as<int>(), c->vslot.as<int>(), c->idx_dirty);
+        launch_idx_reset(c->stream, c->idx.as<int>(), c->vslot.as<int64_t>(), c->idx_dirty);
     c->idx_nvox = nvox;
     c->idx_dirty = 0;
     launch_vtx_count(c->stream, g, R, eslots, c->bits.as<uint32_t>(), c->vcount.as<int>());
@@ -947,10 +975,10 @@ static int assemble(nsgpu_ctx *c, const int eslots[3]) {
     // next read the host makes.
     const int64_t cap = std::max<int64_t>(3 * (int64_t)R.N * R.N * R.K, 1);
     HIP_TRY(c, c->verts.ensure((size_t)cap * 24));
-    HIP_TRY(c, c->vslot.ensure((size_t)cap * 4));
+    HIP_TRY(c, c->vslot.ensure((size_t)cap * 8));
     HIP_TRY(c, c->vhit.ensure((size_t)cap * 8));
     launch_vtx_write(c->stream, g, R, eslots, c->bits.as<uint32_t>(), c->voffs.as<int>(), c->verts.as<double>(),
-                     c->idx.as<int>(), c->vslot.as<int>(), c->vhit.as<int64_t>(), cap);
+                     c->idx.as<int>(), c->vslot.as<int64_t>(), c->vhit.as<int64_t>(), cap);
     HIP_TRY(c, hipGetLastError());
     c->idx_dirty = 0;
     c->nv_cap = cap;
@@ -1119,11 +1147,11 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
         c->verts.release();
         c->verts = nb;
     }
-    if ((size_t)nv_cap * 4 > c->vslot.cap) {
+    if ((size_t)nv_cap * 8 > c->vslot.cap) {
         DBuf nb;
-        HIP_TRY(c, nb.ensure((size_t)nv_cap * 4));
+        HIP_TRY(c, nb.ensure((size_t)nv_cap * 8));
         if (c->n_analytic)
-            HIP_TRY(c, hipMemcpyAsync(nb.p, c->vslot.p, (size_t)c->n_analytic * 4, hipMemcpyDeviceToDevice, c->stream));
+            HIP_TRY(c, hipMemcpyAsync(nb.p, c->vslot.p, (size_t)c->n_analytic * 8, hipMemcpyDeviceToDevice, c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         c->vslot.release();
         c->vslot = nb;
@@ -1132,7 +1160,7 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
     HIP_TRY(c, c->vorig.ensure((size_t)std::max<int64_t>(nadd_cap, 1) * 4));
     launch_mc_vwrite(c->stream, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->aedge.as<int>(),
                      c->idx.as<int>(), c->avoff.as<int>(), (int)c->n_analytic, c->verts.as<double>(),
-                     c->vslot.as<int>(), c->vorig.as<int>());
+                     c->vslot.as<int64_t>(), c->vorig.as<int>());
     c->nact = nact;
     HIP_TRY(c, c->tris.ensure((size_t)std::max<int64_t>(nt_cap, 1) * 12));
     HIP_TRY(c, c->tarea.ensure((size_t)std::max<int64_t>(nt_cap, 1) * 8));
@@ -1275,6 +1303,7 @@ int nsgpu_set_mesh(nsgpu_ctx *c, const double *verts, int32_t nv, const int32_t 
                    int32_t aux_grid_dim) {
     if (!c || nv < 0 || nt < 0 || (nv && !verts) || (nt && !tris) || aux_grid_dim < 1) return NSGPU_E_ARG;
     if (!c->have_grid) return fail(c, NSGPU_E_STATE, "set_grid first");
+    ++c->setgen;
     for (int64_t q = 0; q < 3 * (int64_t)nt; ++q)
         if (tris[q] < 0 || tris[q] >= nv) return fail(c, NSGPU_E_ARG, "triangle vertex index out of range");
     for (int64_t q = 0; q < 3 * (int64_t)nv; ++q)
@@ -1296,6 +1325,7 @@ int nsgpu_blobby_build(nsgpu_ctx *c, const double *xyzr, int32_t n, double B, do
     if (!c || n < 0 || (n && !xyzr) || !(cutoff > 0) || aux_grid_dim < 1) return NSGPU_E_ARG;
     if (!c->have_grid) return fail(c, NSGPU_E_STATE, "set_grid first");
     if (!(B < 0)) B = -2.5;  // BlobbySurface::setBlobbyness (BlobbySurface.cpp:46-54)
+    ++c->setgen;
     HIP_TRY(c, hipSetDevice(c->device));
     c->pre_mc = false;  // the classification buffers are reused below
     const DevGrid g = dev_grid(c);
@@ -1354,7 +1384,7 @@ int nsgpu_blobby_build(nsgpu_ctx *c, const double *xyzr, int32_t n, double B, do
     if (c->idx.p != old_idx || c->idx_nvox != nvox || 3 * nvox > INT32_MAX)
         launch_fill32(st, c->idx.p, 0xffffffffu, nvox * 3);
     else
-        launch_idx_reset(st, c->idx.as<int>(), c->vslot.as<int>(), c->idx_dirty);
+        launch_idx_reset(st, c->idx.as<int>(), c->vslot.as<int64_t>(), c->idx_dirty);
     c->idx_nvox = nvox;
     c->idx_dirty = 0;
     c->have_trace = c->have_surf = c->have_vertices = c->have_mesh = false;
@@ -1396,10 +1426,10 @@ int nsgpu_blobby_build(nsgpu_ctx *c, const double *xyzr, int32_t n, double B, do
     const int64_t nv = tot[0], nt = tot[1];
     if (nv >= INT32_MAX / 3 || nt >= INT32_MAX / 3) return fail(c, NSGPU_E_NOMEM, "blobby mesh too large");
     HIP_TRY(c, c->verts.ensure((size_t)std::max<int64_t>(nv, 1) * 24));
-    HIP_TRY(c, c->vslot.ensure((size_t)std::max<int64_t>(nv, 1) * 4));
+    HIP_TRY(c, c->vslot.ensure((size_t)std::max<int64_t>(nv, 1) * 8));
     HIP_TRY(c, c->vorig.ensure((size_t)std::max<int64_t>(nv, 1) * 4));
     launch_mc_vwrite(st, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->aedge.as<int>(), c->idx.as<int>(),
-                     c->avoff.as<int>(), 0, c->verts.as<double>(), c->vslot.as<int>(), c->vorig.as<int>(), a.field, 1.0);
+                     c->avoff.as<int>(), 0, c->verts.as<double>(), c->vslot.as<int64_t>(), c->vorig.as<int>(), a.field, 1.0);
     c->idx_dirty = nv;
     HIP_TRY(c, c->tris.ensure((size_t)std::max<int64_t>(nt, 1) * 12));
     HIP_TRY(c, c->tarea.ensure((size_t)std::max<int64_t>(nt, 1) * 8));
@@ -1415,7 +1445,7 @@ int nsgpu_blobby_build(nsgpu_ctx *c, const double *xyzr, int32_t n, double B, do
     HIP_TRY(c, c->mtris.ensure((size_t)std::max<int64_t>(nt, 1) * 12));
     SmoothArgs sm;
     sm.vin = c->verts.as<double>();
-    sm.vslot = c->vslot.as<int>();
+    sm.vslot = c->vslot.as<int64_t>();
     sm.nv = nv;
     sm.anode = c->anode.as<int>();
     sm.ant = c->ant.as<int>();
@@ -1540,6 +1570,7 @@ int nsgpu_read_ray_hits(nsgpu_ctx *c, int32_t slot, int32_t max_hits, int32_t *c
 
 int nsgpu_set_atoms(nsgpu_ctx *c, const double *xyzr, int32_t n) {
     if (!c || n < 0 || (n > 0 && !xyzr)) return NSGPU_E_ARG;
+    ++c->setgen;
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, c->atoms.ensure(std::max<size_t>((size_t)n, 1) * 32));
     if (n) HIP_TRY(c, hipMemcpyAsync(c->atoms.p, xyzr, (size_t)n * 32, hipMemcpyHostToDevice, c->stream));
@@ -1634,6 +1665,7 @@ int nsgpu_surface_gradient(nsgpu_ctx *c, double *volume, double *area, double *d
 
 int nsgpu_set_vertex_normals(nsgpu_ctx *c, int on) {
     if (!c) return NSGPU_E_ARG;
+    if (c->want_normals != (on != 0)) ++c->setgen;
     c->want_normals = on != 0;
     return NSGPU_OK;
 }
@@ -1702,6 +1734,10 @@ struct nsgpu_batch {
     bool prepared = false;
     DBuf stamps;  // frame completion clocks, then the start clock
     double clock_khz = 0;
+    // each frame's ctx_generation when the graph was captured: a frame
+    // changed since (new inputs, a buffer moved, other options through its
+    // nsgpu_batch_frame context) makes nsgpu_batch_run capture again
+    std::vector<uint64_t> gen;
     std::string err;
 };
 
@@ -1781,7 +1817,7 @@ int nsgpu_batch_set_frame(nsgpu_batch *b, int32_t f, const nsgpu_grid *grid, con
     int rc = nsgpu_set_grid(c, grid);
     if (!rc) rc = nsgpu_set_patches(c, p, probe);
     if (rc) return bfail(b, rc, std::string("frame ") + std::to_string(f) + ": " + c->err);
-    drop_graph(b);
+    // the frame's generation changed: a prepared batch captures again at its next run
     return NSGPU_OK;
 }
 
@@ -1834,7 +1870,21 @@ static int capture(nsgpu_batch *b, int chains) {
     BHIP_TRY(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     b->chains = chains;
     b->prepared = true;
+    b->gen.resize(n);
+    for (int f = 0; f < n; ++f) b->gen[f] = ctx_generation(b->frames[f]);
     return NSGPU_OK;
+}
+
+// frames that no longer match the captured graph
+static bool graph_stale(nsgpu_batch *b, std::vector<int> *stale) {
+    stale->clear();
+    for (size_t f = 0; f < b->frames.size(); ++f) {
+        nsgpu_ctx *c = b->frames[f];
+        if (f >= b->gen.size() || ctx_generation(c) != b->gen[f] ||
+            std::memcmp(&c->opts, &b->opts, sizeof(nsgpu_surf_opts)) != 0)
+            stale->push_back((int)f);
+    }
+    return !stale->empty();
 }
 
 static int launch_graph(nsgpu_batch *b, double *ms) {
@@ -1889,19 +1939,28 @@ int nsgpu_batch_run(nsgpu_batch *b, double *latency_ms, int32_t *redone) {
     if (!b->prepared) return bfail(b, NSGPU_E_STATE, "nsgpu_batch_prepare first");
     BHIP_TRY(b, hipSetDevice(b->device));
     const int n = (int)b->frames.size();
-    int rc = launch_graph(b, nullptr);
+    int rc;
+    std::vector<int> stale;
+    if (graph_stale(b, &stale)) {
+        // size the changed frames' buffers eagerly, then capture again
+        for (int f : stale) {
+            rc = nsgpu_get_surf(b->frames[f], &b->opts, nullptr);
+            if (rc) return bfail(b, rc, std::string("frame ") + std::to_string(f) + ": " + b->frames[f]->err);
+        }
+        rc = capture(b, b->chains);
+        if (rc) return rc;
+    }
+    rc = launch_graph(b, nullptr);
     if (rc) return rc;
     std::vector<unsigned long long> st(n + 1);
     BHIP_TRY(b, hipMemcpy(st.data(), b->stamps.p, (size_t)(n + 1) * 8, hipMemcpyDeviceToHost));
     if (latency_ms)
         for (int f = 0; f < n; ++f) latency_ms[f] = (double)(st[f] - st[n]) / b->clock_khz;
     int again = 0;
-    bool regrown = false;
     for (int f = 0; f < n; ++f) {
         nsgpu_ctx *c = b->frames[f];
         if (regrow(c)) {
-            // the frame's rays outgrew their lists: redo it eagerly, capture again next time
-            regrown = true;
+            // the frame's rays outgrew their lists: redo it eagerly
             ++again;
             rc = nsgpu_get_surf(c, &b->opts, nullptr);
         } else {
@@ -1911,7 +1970,8 @@ int nsgpu_batch_run(nsgpu_batch *b, double *latency_ms, int32_t *redone) {
         }
         if (rc) return bfail(b, rc, std::string("frame ") + std::to_string(f) + ": " + c->err);
     }
-    if (regrown) drop_graph(b);
+    // a frame redone with grown ray lists (or numbered on the host, which may
+    // grow its key buffers) has a new generation: the next run captures again
     if (redone) *redone = again;
     return NSGPU_OK;
 }

@@ -125,8 +125,8 @@ void launch_vtx_count(hipStream_t, const DevGrid &, const DevRays &, const int[3
 // the index of its depth in R.t / R.pid
 // vertices v >= vcap are dropped (vcap bounds the total; the host checks it)
 void launch_vtx_write(hipStream_t, const DevGrid &, const DevRays &, const int[3], const uint32_t *, const int *,
-                      double *, int *, int *vslot, int64_t *vhit, int64_t vcap);
-void launch_idx_reset(hipStream_t, int *idx, const int *vslot, int64_t n);
+                      double *, int *, int64_t *vslot, int64_t *vhit, int64_t vcap);
+void launch_idx_reset(hipStream_t, int *idx, const int64_t *vslot, int64_t n);
 int64_t mc_num_segments(const DevGrid &);
 void launch_mc_classify(hipStream_t, const DevGrid &, const uint32_t *bits, uint32_t *segmask, int *segcnt);
 void launch_mc_compact(hipStream_t, const DevGrid &, const uint32_t *bits, const uint32_t *segmask, const int *segoff,
@@ -139,7 +139,7 @@ void launch_mc_counts(hipStream_t, const DevGrid &, int nact, const int *anode, 
 // field == nullptr: edge mid points; else vertexInterp at iso on the scalar
 // field (one double per node)
 void launch_mc_vwrite(hipStream_t, const DevGrid &, int nact, const int *anode, const uint8_t *acase,
-                      const int *aedge, int *idx, const int *voff, int base, double *verts, int *vslot, int *vorig,
+                      const int *aedge, int *idx, const int *voff, int base, double *verts, int64_t *vslot, int *vorig,
                       const double *field = nullptr, double iso = 0.);
 // vertex normals: analytic at the depths the vertices come from (the ray
 // lists must carry patch ids), mid points from their adjacent triangles
@@ -181,7 +181,7 @@ void launch_field_bits(hipStream_t, const double *field, int64_t nvox, double is
 // smoothing pass and the OFF text round trip of the blobby mesh
 struct SmoothArgs {
     const double *vin;  // marching-cubes vertices
-    const int *vslot;   // edge slot 3 * node + direction of each vertex
+    const int64_t *vslot;  // edge slot 3 * node + direction of each vertex
     int64_t nv;
     const int *anode, *ant, *atoff, *tris;  // active cubes, their triangle counts and offsets, triangles
     int nact;

@@ -126,10 +126,10 @@ __global__ void __launch_bounds__(256) k_vtx_count(DevGrid g, DevRays R, EdgeSlo
 constexpr int kVtxStage = 192;
 
 __global__ void __launch_bounds__(256) k_vtx_write(DevGrid g, DevRays R, EdgeSlots es, const uint32_t *bits,
-                                                   const int *offs, double *verts, int *idx, int *vslot,
+                                                   const int *offs, double *verts, int *idx, int64_t *vslot,
                                                    int64_t *vhit, int64_t vcap) {
     __shared__ double sv[4][3 * kVtxStage];
-    __shared__ int ss[4][kVtxStage];
+    __shared__ int64_t ss[4][kVtxStage];
     __shared__ int64_t sh[4][kVtxStage];
     const int64_t NN = (int64_t)R.N * R.N;
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -154,13 +154,13 @@ __global__ void __launch_bounds__(256) k_vtx_write(DevGrid g, DevRays R, EdgeSlo
                                     sv[wv][3 * so + 0] = px;
                                     sv[wv][3 * so + 1] = py;
                                     sv[wv][3 * so + 2] = pz;
-                                    ss[wv][so] = (int)(3 * f + d);
+                                    ss[wv][so] = 3 * f + d;
                                     sh[wv][so] = h;
                                 } else {
                                     verts[3 * (int64_t)v + 0] = px;
                                     verts[3 * (int64_t)v + 1] = py;
                                     verts[3 * (int64_t)v + 2] = pz;
-                                    vslot[v] = (int)(3 * f + d);
+                                    vslot[v] = 3 * f + d;
                                     vhit[v] = h;
                                 }
                             });
@@ -391,7 +391,7 @@ __device__ __forceinline__ void vertex_interp(double iso, const double p1[3], co
 
 __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const int *anode, const uint8_t *acase,
                                                    const int *aedge, int *idx, const int *voff, int base,
-                                                   double *verts, int *vslot, int *vorig, const double *field,
+                                                   double *verts, int64_t *vslot, int *vorig, const double *field,
                                                    double iso) {
     const int q = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (q >= nact) return;
@@ -417,19 +417,19 @@ __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const in
                      f7 = field[f + sy + sz];
         if (m5) {
             vertex_interp(iso, c5, c6, f5, f6, verts + 3 * (int64_t)v);
-            vslot[v] = (int)s5;
+            vslot[v] = s5;
             vorig[v - base] = 4 * q + 0;
             idx[s5] = v++;
         }
         if (m6) {
             vertex_interp(iso, c6, c7, f6, f7, verts + 3 * (int64_t)v);
-            vslot[v] = (int)s6;
+            vslot[v] = s6;
             vorig[v - base] = 4 * q + 1;
             idx[s6] = v++;
         }
         if (m10) {
             vertex_interp(iso, c2, c6, f2, f6, verts + 3 * (int64_t)v);
-            vslot[v] = (int)s10;
+            vslot[v] = s10;
             vorig[v - base] = 4 * q + 2;
             idx[s10] = v++;
         }
@@ -439,7 +439,7 @@ __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const in
         verts[3 * (int64_t)v] = g.x[i + 1] - g.hside;
         verts[3 * (int64_t)v + 1] = g.y[j];
         verts[3 * (int64_t)v + 2] = g.z[k + 1] - g.hside;
-        vslot[v] = (int)s5;
+        vslot[v] = s5;
         vorig[v - base] = 4 * q + 0;
         idx[s5] = v++;
     }
@@ -447,7 +447,7 @@ __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const in
         verts[3 * (int64_t)v] = g.x[i];
         verts[3 * (int64_t)v + 1] = g.y[j + 1] - g.hside;
         verts[3 * (int64_t)v + 2] = g.z[k + 1] - g.hside;
-        vslot[v] = (int)s6;
+        vslot[v] = s6;
         vorig[v - base] = 4 * q + 1;
         idx[s6] = v++;
     }
@@ -455,7 +455,7 @@ __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const in
         verts[3 * (int64_t)v] = g.x[i + 1] - g.hside;
         verts[3 * (int64_t)v + 1] = g.y[j + 1] - g.hside;
         verts[3 * (int64_t)v + 2] = g.z[k];
-        vslot[v] = (int)s10;
+        vslot[v] = s10;
         vorig[v - base] = 4 * q + 2;
         idx[s10] = v++;
     }
@@ -534,7 +534,7 @@ void launch_vtx_count(hipStream_t s, const DevGrid &g, const DevRays &R, const i
 }
 
 void launch_vtx_write(hipStream_t s, const DevGrid &g, const DevRays &R, const int eslots[3], const uint32_t *bits,
-                      const int *offs, double *verts, int *idx, int *vslot, int64_t *vhit, int64_t vcap) {
+                      const int *offs, double *verts, int *idx, int64_t *vslot, int64_t *vhit, int64_t vcap) {
     EdgeSlots es{{eslots[0], eslots[1], eslots[2]}};
     const int64_t n = 3 * (int64_t)R.N * R.N;
     hipLaunchKernelGGL(k_vtx_write, dim3(nblk(n)), dim3(256), 0, s, g, R, es, bits, offs, verts, idx, vslot, vhit,
@@ -542,12 +542,12 @@ void launch_vtx_write(hipStream_t s, const DevGrid &g, const DevRays &R, const i
 }
 
 // back to "no vertex" for the slots the last vertices were written to
-__global__ void __launch_bounds__(256) k_idx_reset(int *idx, const int *vslot, int64_t n) {
+__global__ void __launch_bounds__(256) k_idx_reset(int *idx, const int64_t *vslot, int64_t n) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v < n) idx[vslot[v]] = -1;
 }
 
-void launch_idx_reset(hipStream_t s, int *idx, const int *vslot, int64_t n) {
+void launch_idx_reset(hipStream_t s, int *idx, const int64_t *vslot, int64_t n) {
     if (n > 0) hipLaunchKernelGGL(k_idx_reset, dim3(nblk(n)), dim3(256), 0, s, idx, vslot, n);
 }
 
@@ -583,7 +583,7 @@ void launch_mc_counts(hipStream_t s, const DevGrid &g, int nact, const int *anod
 }
 
 void launch_mc_vwrite(hipStream_t s, const DevGrid &g, int nact, const int *anode, const uint8_t *acase,
-                      const int *aedge, int *idx, const int *voff, int base, double *verts, int *vslot, int *vorig,
+                      const int *aedge, int *idx, const int *voff, int base, double *verts, int64_t *vslot, int *vorig,
                       const double *field, double iso) {
     if (nact)
         hipLaunchKernelGGL(k_mc_vwrite, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, aedge, idx, voff,

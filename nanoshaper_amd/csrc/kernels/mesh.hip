@@ -200,11 +200,12 @@ __global__ void __launch_bounds__(256) k_cube_index(const int *anode, int nact, 
 __global__ void __launch_bounds__(256) k_smooth(DevGrid g, SmoothArgs a) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= a.nv) return;
-    const int slot = a.vslot[v];
-    const int f = slot / 3, dir = slot - 3 * f;
-    const int sy = g.nx, sz = g.nx * g.ny;
+    const int64_t slot = a.vslot[v];
+    const int64_t f = slot / 3;
+    const int dir = (int)(slot - 3 * f);
+    const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
     // the cubes around the edge (node f, direction dir), increasing node index
-    int cubes[4];
+    int64_t cubes[4];
     if (dir == 0) { cubes[0] = f - sy - sz; cubes[1] = f - sz; cubes[2] = f - sy; cubes[3] = f; }
     else if (dir == 1) { cubes[0] = f - 1 - sz; cubes[1] = f - sz; cubes[2] = f - 1; cubes[3] = f; }
     else { cubes[0] = f - 1 - sy; cubes[1] = f - sy; cubes[2] = f - 1; cubes[3] = f; }
@@ -212,7 +213,7 @@ __global__ void __launch_bounds__(256) k_smooth(DevGrid g, SmoothArgs a) {
     int c = 0;
     for (int q = 0; q < 4; ++q) {
         if (cubes[q] < 0) continue;
-        const int ci = active_index(a.anode, a.nact, a.cube_index, cubes[q]);
+        const int ci = active_index(a.anode, a.nact, a.cube_index, (int)cubes[q]);
         if (ci < 0) continue;
         for (int t = a.atoff[ci]; t < a.atoff[ci] + a.ant[ci]; ++t) {
             const int *tv = a.tris + 3 * (int64_t)t;
@@ -231,8 +232,15 @@ __global__ void __launch_bounds__(256) k_smooth(DevGrid g, SmoothArgs a) {
             ++c;
         }
     }
-    const double scale = 1.0 / (double)(2 * c);
     const double *V = a.vin + 3 * v;
+    if (c == 0) {
+        // no triangle names the vertex: it keeps its position
+        a.vout[3 * v + 0] = V[0];
+        a.vout[3 * v + 1] = V[1];
+        a.vout[3 * v + 2] = V[2];
+        return;
+    }
+    const double scale = 1.0 / (double)(2 * c);
     a.vout[3 * v + 0] = 0.5 * (V[0] + s0 * scale);
     a.vout[3 * v + 1] = 0.5 * (V[1] + s1 * scale);
     a.vout[3 * v + 2] = 0.5 * (V[2] + s2 * scale);

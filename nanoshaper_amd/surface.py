@@ -42,6 +42,26 @@ def as_xyzr(atoms):
     return a
 
 
+def load_atoms(path, max_atoms=-1, domain_shrinkage=0.0, charges=False):
+    """DelPhiShared::loadAtoms: (n, 4) x, y, z, radius of an .xyzr or .pqr
+    file (and the (n,) charges when charges=True)."""
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    rc = lib.nsgpu_load_atoms(str(path).encode(), int(max_atoms), float(domain_shrinkage), ctypes.byref(h))
+    try:
+        if rc != 0:
+            msg = lib.nsgpu_atom_set_error(h).decode() if h else ""
+            raise NsgpuError("load_atoms %s failed (%d): %s" % (path, rc, msg))
+        px, pq = ctypes.POINTER(ctypes.c_double)(), ctypes.POINTER(ctypes.c_double)()
+        n = lib.nsgpu_atom_set_view(h, ctypes.byref(px), ctypes.byref(pq))
+        xyzr = np.ctypeslib.as_array(px, (n, 4)).copy()
+        q = np.ctypeslib.as_array(pq, (n,)).copy()
+    finally:
+        if h:
+            lib.nsgpu_atom_set_free(h)
+    return (xyzr, q) if charges else xyzr
+
+
 def build_grid(xyzr, scale=2.0, perfil=80.0):
     """DelPhiShared::buildGrid: cubic odd grid centred on the atom bounding box."""
     lib = _lib.load()

@@ -58,7 +58,8 @@ def load():
         "orc_blobby_build": (ctypes.c_int, [ctypes.POINTER(Grid), VP, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                             ctypes.POINTER(VP), ctypes.POINTER(VP), ctypes.POINTER(VP),
                                             ctypes.POINTER(VP), ctypes.POINTER(ctypes.c_int64),
-                                            ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]),
+                                            ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(VP)]),
         "orc_free_buf": (None, [VP]),
     }
     for name, (res, args) in sig.items():
@@ -74,16 +75,17 @@ def _ptr(a):
 
 
 def blobby_build(grid, xyzr, blobbyness=-2.5, cutoff=6.0):
-    """BlobbySurface::build restated: {'field' (nz, ny, nx), 'raw' and 'verts'
-    (nv, 3): smoothed vertices before and after the OFF text round trip,
+    """BlobbySurface::build restated: {'field' (nz, ny, nx), 'mc' (nv, 3): the
+    marching-cubes vertices, 'raw' and 'verts' (nv, 3): smoothed vertices
+    before and after the OFF text round trip,
     'tris' (nt, 3), 'area' of the marching cubes before smoothing}."""
     lib = load()
     a = np.ascontiguousarray(xyzr, dtype=np.float64).reshape(-1, 4)
-    f, raw, v, t = VP(), VP(), VP(), VP()
+    f, raw, v, t, mc = VP(), VP(), VP(), VP(), VP()
     nv, nt, area = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_double()
     lib.orc_blobby_build(ctypes.byref(grid), _ptr(a), a.shape[0], float(blobbyness), float(cutoff), ctypes.byref(f),
                          ctypes.byref(raw), ctypes.byref(v), ctypes.byref(t), ctypes.byref(nv), ctypes.byref(nt),
-                         ctypes.byref(area))
+                         ctypes.byref(area), ctypes.byref(mc))
 
     def take(p, n, dt):
         out = np.empty(n, dt)
@@ -93,7 +95,8 @@ def blobby_build(grid, xyzr, blobbyness=-2.5, cutoff=6.0):
         return out
 
     field = take(f, grid.nx * grid.ny * grid.nz, np.float64).reshape(grid.nz, grid.ny, grid.nx)
-    return {"field": field, "raw": take(raw, 3 * nv.value, np.float64).reshape(-1, 3),
+    return {"field": field, "mc": take(mc, 3 * nv.value, np.float64).reshape(-1, 3),
+            "raw": take(raw, 3 * nv.value, np.float64).reshape(-1, 3),
             "verts": take(v, 3 * nv.value, np.float64).reshape(-1, 3),
             "tris": take(t, 3 * nt.value, np.int32).reshape(-1, 3), "area": area.value}
 

@@ -1793,10 +1793,11 @@ static const int kMulTable[8][3] = {{0, 2, 4}, {1, 2, 4}, {1, 3, 4}, {0, 2, 4},
  * reversed), one smoothing pass (smoothSurface :13089-13240), then the OFF
  * text round trip ("%.3f" written, Surface.cpp:10961; "%f" read into
  * floats, MeshSurface.cpp:2436-2446).  Outputs are malloc'd: field [nvox],
- * raw (smoothed) and verts (reloaded) [nv*3], tris [nt*3]. */
+ * raw (smoothed) and verts (reloaded) [nv*3], tris [nt*3], and (mc_out may
+ * be NULL) the marching-cubes vertices before smoothing [nv*3]. */
 int orc_blobby_build(const nsgpu_grid *g, const double *xyzr, int n, double B, double cutoff, double **field_out,
                      double **raw_out, double **verts_out, int32_t **tris_out, int64_t *nv_out, int64_t *nt_out,
-                     double *area_out) {
+                     double *area_out, double **mc_out) {
     const int64_t NX = g->nx, NY = g->ny, NZ = g->nz, nvox = NX * NY * NZ;
     double *x = (double *)xcalloc(NX, sizeof(double)), *y = (double *)xcalloc(NY, sizeof(double)),
            *z = (double *)xcalloc(NZ, sizeof(double));
@@ -1940,7 +1941,7 @@ int orc_blobby_build(const nsgpu_grid *g, const double *xyzr, int n, double B, d
     for (int d = 0; d < 3; d++) free(idx[d]);
     free(temp);
     free(num);
-    free(V);
+    if (mc_out) *mc_out = V; else free(V);
     free(x);
     free(y);
     free(z);

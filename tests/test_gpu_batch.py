@@ -78,3 +78,34 @@ def test_batch_frame_with_cavities(device_ok):
     b.run()
     _check(b, ref)
     b.close()
+
+
+def test_batch_captures_again_after_a_frame_changes(device_ok):
+    """Calls through a frame's own context that move its buffers or change
+    its options (a larger grid, get_surf with edge rays) make the next run
+    capture the graph again instead of launching it onto freed memory."""
+    frames = ns.perturbed_frames(3, n_min=150, n_max=600, seed=5)
+    inputs = _inputs(frames)
+    b = ns.Batch(len(inputs))
+    for f, (grid, ps) in enumerate(inputs):
+        b.set_frame(f, grid, ps)
+    b.prepare(chains=2, **OPTS)
+    b.run()
+    _check(b, _reference(inputs))
+    # frame 0 on a finer grid: every map buffer of that context grows
+    big = ns.build_grid(frames[0], 3.0, 80.0)
+    v0 = b.frame(0)
+    v0.set_grid(big)
+    v0.set_patches(inputs[0][1])
+    inputs[0] = (big, inputs[0][1])
+    b.grids[0] = big
+    # frame 1 with other options (edge rays, vertex pass): its host state changes
+    b.frame(1).get_surf(build_epsmap=True, build_status=True, accurate_triangulation=True, fill_cavities=True)
+    print("run after the frames changed", flush=True)
+    lat, redone = b.run()
+    assert redone == 0
+    _check(b, _reference(inputs))
+    # and again with nothing changed: the recaptured graph is reused
+    b.run()
+    _check(b, _reference(inputs))
+    b.close()

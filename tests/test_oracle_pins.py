@@ -1,0 +1,368 @@
+"""Pins the oracle's integer rules against the surfaces' own definitions.
+
+* The map fill (status, idebmap, the three epsmap faces) and the corner
+  bitmap equal the exact SES classifier at every point they stand for, except
+  within 1e-6 A of the surface: status and idebmap the node, epsmap face l
+  the node moved by half a cell along axis l, corner bits the node moved by
+  half a cell back along every axis.
+* fillCavities: every node of a non-outside TEMP_OUT component (labelled here
+  with scipy) gets its six epsmap faces, idebmap and eight corner bits set to
+  inside; nothing else changes.
+* Analytic marching cubes: closed, consistently oriented 2-manifolds (V - E +
+  F even and at most 2 per piece; 2 for one atom), area -> 4 pi r^2 for one atom, the
+  enclosed volume equal to the ray volume, every analytic vertex on the
+  surface and every triangle inside its cube.
+* Skin: the status map equals the skin's definition (a convex program over
+  the weights of the atoms), including the hyperboloid cells of two atoms.
+* Blobby: every marching-cubes vertex sits where the field's linear
+  interpolant along its edge crosses the iso value, and the smoothing pass
+  moves each vertex to the average of itself and its triangles' other
+  vertices.
+"""
+import numpy as np
+import pytest
+from scipy import ndimage
+
+import nanoshaper_amd as ns
+from oracle import binding
+from tests.exact_surfaces import ExactSES, ExactSkin, ses_near_ray
+
+PROBE = 1.4
+MARGIN = 1e-6
+
+
+def _ses(n, radius, scale, perfil, seed):
+    xyzr, _ = ns.generate_chain(n, radius, seed=seed)
+    g = ns.build_grid(xyzr, scale, perfil)
+    jit, _ = ns.jitter(xyzr)
+    return g, jit, ns.ses_patches(jit, PROBE)
+
+
+def _node_points(g, shift):
+    xs, ys, zs = ns.grid_coords(g)
+    Z, Y, X = np.meshgrid(zs + shift[2], ys + shift[1], xs + shift[0], indexing="ij")
+    return np.column_stack([X.ravel(), Y.ravel(), Z.ravel()])
+
+
+def _unpack_bits(words, nvox):
+    b = np.unpackbits(words.view(np.uint8), bitorder="little")
+    return b[:nvox].astype(bool)
+
+
+def _check_points(exact, pts, got_outside, what):
+    f = exact.signed(pts)
+    sure = np.abs(f) > MARGIN
+    want = f > 0
+    bad = np.nonzero(sure & (want != got_outside))[0]
+    assert bad.size == 0, (what, bad.size, pts[bad[:5]], f[bad[:5]])
+    return int(sure.sum())
+
+
+@pytest.mark.parametrize("n,radius,scale,perfil,seed", [(3, 4.0, 2.0, 60.0, 21), (12, 6.0, 2.0, 70.0, 22),
+                                                        (30, 8.0, 1.5, 70.0, 23), (8, 5.0, 3.0, 70.0, 24)])
+def test_fill_and_corners_match_exact_ses(n, radius, scale, perfil, seed):
+    g, jit, ps = _ses(n, radius, scale, perfil, seed)
+    o = binding.OracleSurface(g, ps)
+    r = o.get_surf(build_epsmap=True, build_status=True, accurate_triangulation=True, fill_cavities=False)
+    assert r["odd_rays"] == 0
+    ex = ExactSES(jit[:, :3], jit[:, 3], PROBE)
+    h = g.hside
+    nvox = g.nx * g.ny * g.nz
+    st, ideb, eps = o.status().ravel(), o.idebmap().ravel(), o.epsmap().reshape(-1, 3)
+    assert set(np.unique(st)) <= {ns.STATUS_INSIDE, ns.STATUS_TEMP_OUT}
+    checked = _check_points(ex, _node_points(g, (0, 0, 0)), st != ns.STATUS_INSIDE, "status")
+    assert np.array_equal(ideb.astype(bool), st != ns.STATUS_INSIDE)
+    for face in range(3):
+        sh = [0.0, 0.0, 0.0]
+        sh[face] = h
+        checked += _check_points(ex, _node_points(g, sh), eps[:, face] == 0, "epsmap face %d" % face)
+    bits = _unpack_bits(o.corner_bits(), nvox)
+    checked += _check_points(ex, _node_points(g, (-h, -h, -h)), bits, "corner bits")
+    assert checked > 3 * nvox
+
+
+def test_fill_cavities_rule():
+    # a compact chain with buried cavities
+    xyzr, _ = ns.generate_chain(800, 16.0, seed=5)
+    g = ns.build_grid(xyzr, 3.0, 80.0)
+    jit, _ = ns.jitter(xyzr)
+    ps = ns.ses_patches(jit, PROBE)
+    o = binding.OracleSurface(g, ps)
+    opts = dict(build_epsmap=True, build_status=True, accurate_triangulation=True)
+    o.get_surf(fill_cavities=False, **opts)
+    st0, eps0, ideb0, bits0 = o.status(), o.epsmap(), o.idebmap(), o.corner_bits()
+    o.get_surf(fill_cavities=True, cav_vol=1e30, **opts)
+    st1, eps1, ideb1, bits1 = o.status(), o.epsmap(), o.idebmap(), o.corner_bits()
+    sizes, filled = o.cavities()
+    lab, nlab = ndimage.label(st0 == ns.STATUS_TEMP_OUT, structure=ndimage.generate_binary_structure(3, 1))
+    outside = lab.ravel()[0]
+    assert outside > 0 and nlab >= 3
+    cav = (lab > 0) & (lab != outside)
+    # status: the outside component and the cavity labels; cavities all filled
+    assert np.array_equal(st1 == ns.STATUS_OUT, lab == outside)
+    assert np.all(st1[cav] >= ns.STATUS_FIRST_CAV)
+    assert len(sizes) == nlab - 1 and filled.all()
+    side3 = g.side ** 3
+    counts = np.bincount(st1[cav] - ns.STATUS_FIRST_CAV, minlength=len(sizes))
+    assert np.allclose(sizes, counts * side3, rtol=1e-12)
+    # epsmap: faces 0, 1, 2 of every cavity node and face l of its -l neighbour
+    exp_eps = eps0.copy()
+    k, j, i = np.nonzero(cav)
+    for face, (dk, dj, di) in enumerate([(0, 0, 1), (0, 1, 0), (1, 0, 0)]):
+        exp_eps[k, j, i, face] = 5
+        exp_eps[k - dk, j - dj, i - di, face] = 5
+    assert np.array_equal(exp_eps, eps1)
+    exp_ideb = ideb0.copy()
+    exp_ideb[cav] = 0
+    assert np.array_equal(exp_ideb, ideb1)
+    # corner bits: the eight corners (i..i+1, j..j+1, k..k+1) of each cavity node
+    nz, ny, nx = st0.shape
+    b0 = _unpack_bits(bits0, nx * ny * nz).reshape(nz, ny, nx)
+    exp_b = b0.copy()
+    for dk in (0, 1):
+        for dj in (0, 1):
+            for di in (0, 1):
+                exp_b[k + dk, j + dj, i + di] = False
+    assert np.array_equal(exp_b.ravel(), _unpack_bits(bits1, nx * ny * nz))
+    # a threshold below every cavity changes no map
+    o.get_surf(fill_cavities=True, cav_vol=0.5 * side3, **opts)
+    assert np.array_equal(o.epsmap(), eps0) and np.array_equal(o.corner_bits(), bits0)
+    assert not o.cavities()[1].any()
+
+
+# ------------------------------------------------------------------ meshes
+def _edges(t):
+    e = np.concatenate([t[:, [0, 1]], t[:, [1, 2]], t[:, [2, 0]]])
+    return e
+
+
+def _closed_oriented(t):
+    """Every directed edge appears once and its reverse once."""
+    e = _edges(t)
+    key = e[:, 0].astype(np.int64) * (1 << 32) + e[:, 1]
+    rkey = e[:, 1].astype(np.int64) * (1 << 32) + e[:, 0]
+    return len(np.unique(key)) == len(key) and np.array_equal(np.sort(key), np.sort(rkey))
+
+
+def _euler_per_component(nv, t):
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    e = _edges(t)
+    a = coo_matrix((np.ones(len(e)), (e[:, 0], e[:, 1])), shape=(nv, nv))
+    ncomp, lab = connected_components(a, directed=False)
+    used = np.zeros(nv, bool)
+    used[t.ravel()] = True
+    chis = []
+    for c in np.unique(lab[used]):
+        tc = t[lab[t[:, 0]] == c]
+        V = len(np.unique(tc))
+        E = len(np.unique(np.sort(_edges(tc), axis=1), axis=0))
+        chis.append(V - E + len(tc))
+    return chis
+
+
+def _signed_volume(v, t):
+    a, b, c = v[t[:, 0]], v[t[:, 1]], v[t[:, 2]]
+    return np.einsum("ij,ij->i", a, np.cross(b, c)).sum() / 6.0
+
+
+def _mesh(g, ps):
+    o = binding.OracleSurface(g, ps)
+    r = o.get_surf(build_epsmap=True, build_status=True, accurate_triangulation=True, fill_cavities=False)
+    m = o.triangulate()
+    v, t = o.mesh()
+    return r, m, v, t
+
+
+def test_one_atom_analytic_mesh_converges_to_the_sphere():
+    c = np.array([[0.3, -0.2, 0.1, 1.8]])
+    jit = np.repeat(c, 4, axis=0)  # the grid builder needs atoms; all the same sphere
+    areas = []
+    for scale in (2.0, 4.0, 8.0):
+        g = ns.build_grid(jit, scale, 50.0)
+        ps = ns.ses_patches(c, PROBE)
+        r, m, v, t = _mesh(g, ps)
+        assert _closed_oriented(t)
+        assert _euler_per_component(len(v), t) == [2]
+        # analytic vertices on the sphere, mid points within a cell of it
+        na = m["num_analytic_vertices"]
+        rad = np.linalg.norm(v - c[0, :3], axis=1)
+        assert np.max(np.abs(rad[:na] - 1.8)) < 1e-9
+        assert np.max(np.abs(rad - 1.8)) < g.side
+        areas.append(m["area"] / (4 * np.pi * 1.8 ** 2))
+        assert abs(r["volume"] / (4 / 3 * np.pi * 1.8 ** 3) - 1) < 0.02
+    # monotone convergence of the area to 4 pi r^2
+    assert areas[0] < areas[1] < areas[2] < 1.0 and areas[2] > 0.995
+
+
+@pytest.mark.parametrize("n,radius,scale,seed", [(20, 6.0, 2.0, 31), (60, 9.0, 2.0, 32), (40, 7.0, 3.0, 33)])
+def test_chain_analytic_mesh_properties(n, radius, scale, seed):
+    g, jit, ps = _ses(n, radius, scale, 80.0, seed)
+    r, m, v, t = _mesh(g, ps)
+    assert r["odd_rays"] == 0
+    assert _closed_oriented(t)
+    # closed orientable pieces: V - E + F = 2 - 2 genus (a looped chain's SES
+    # may have handles)
+    chis = _euler_per_component(len(v), t)
+    assert all(c <= 2 and c % 2 == 0 for c in chis), chis
+    # the mesh encloses the ray volume (triangles wound outwards)
+    vol = _signed_volume(v, t)
+    assert abs(vol / r["volume"] - 1) < 0.02
+    # analytic vertices lie on the SES
+    ex = ExactSES(jit[:, :3], jit[:, 3], PROBE)
+    na = m["num_analytic_vertices"]
+    assert np.max(np.abs(ex.signed(v[:na]))) < 1e-6
+    # each triangle lies in one cube of the grid shifted by -hside
+    xs, ys, zs = ns.grid_coords(g)
+    lo = np.array([xs[0], ys[0], zs[0]]) - g.hside
+    cell = np.floor((v[t] - lo) / g.side + 1e-9)  # (nt, 3 vertices, 3)
+    span = cell.max(axis=1) - cell.min(axis=1)
+    assert span.max() <= 1
+
+
+def test_1crn_rays_match_exact_ses():
+    """The reference's own input: sampled rays of all three panels of 1crn
+    (327 atoms, 2 points/A) cross the exact SES where the oracle's hits are."""
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "1crn.xyzr")
+    xyzr = ns.load_atoms(path)
+    g = ns.build_grid(xyzr, 2.0, 90.0)
+    jit, _ = ns.jitter(xyzr)
+    ps = ns.ses_patches(jit, PROBE)
+    o = binding.OracleSurface(g, ps)
+    r = o.get_surf(build_epsmap=True, build_status=True, accurate_triangulation=False, fill_cavities=False)
+    assert r["odd_rays"] == 0
+    xs, ys, zs = ns.grid_coords(g)
+    N = g.nx
+    rng = np.random.default_rng(7)
+    s = np.linspace(0, 1, 801)
+    checked = 0
+    for slot, (axis, ua, va) in enumerate([(0, ys, zs), (2, xs, ys), (1, xs, zs)]):
+        counts, T = o.ray_hits(slot)
+        L = (xs, ys, zs)[axis][-1] - (xs, ys, zs)[axis][0]
+        start = (xs, ys, zs)[axis][0]
+        busy = np.nonzero(counts > 0)[0]
+        for ray in rng.choice(busy, size=5, replace=False):
+            n_, m_ = divmod(int(ray), N)
+            u, w = ua[m_], va[n_]
+            ex = ses_near_ray(jit[:, :3], jit[:, 3], PROBE, axis, u, w)
+
+            def pt(tt):
+                p = np.empty((len(tt), 3))
+                o_ = [d for d in range(3) if d != axis]
+                p[:, axis] = start + tt * L
+                p[:, o_[0]] = u
+                p[:, o_[1]] = w
+                return p
+            f = ex.signed(pt(s))
+            ch = np.nonzero((f[1:] > 0) != (f[:-1] > 0))[0]
+            # bisect every crossing of the ray at once
+            a, b, fa = s[ch], s[ch + 1], f[ch]
+            for _ in range(40):
+                mid = 0.5 * (a + b)
+                fm = ex.signed(pt(mid))
+                same = (fm > 0) == (fa > 0)
+                a, fa, b = np.where(same, mid, a), np.where(same, fm, fa), np.where(same, b, mid)
+            exact = 0.5 * (a + b)
+            got = T[ray, :counts[ray]]
+            assert len(got) == len(exact), (slot, ray, got, exact)
+            assert np.max(np.abs(np.array(got) - np.array(exact))) * L < 1e-6
+            checked += 1
+    assert checked == 15
+
+
+# ------------------------------------------------------------------ skin
+def _skin_case(n, R, s, scale, seed):
+    xyzr, _ = ns.generate_chain(n, R, seed=seed)
+    jit, _ = ns.jitter(xyzr)
+    g = ns.build_grid(xyzr, scale, 60.0)
+    o = binding.OracleSurface(g, ns.skin_patches(jit, s))
+    r = o.get_surf(build_epsmap=True, build_status=True, accurate_triangulation=False, fill_cavities=False)
+    return g, jit, o, r
+
+
+@pytest.mark.parametrize("n,R,s,scale,seed,faces", [(5, 2.8, 0.45, 2.0, 3, True), (8, 3.3, 0.45, 3.0, 44, True),
+                                                    (12, 3.8, 0.6, 2.0, 45, True), (20, 4.5, 0.3, 2.0, 6, False)])
+def test_skin_status_matches_definition(n, R, s, scale, seed, faces):
+    g, jit, o, r = _skin_case(n, R, s, scale, seed)
+    assert r["odd_rays"] == 0
+    ex = ExactSkin(jit[:, :3], jit[:, 3], s)
+    st = o.status().ravel()
+    eps = o.epsmap().reshape(-1, 3)
+    checked = 0
+    pts = [((0, 0, 0), st != ns.STATUS_INSIDE)]
+    if faces:
+        pts += [((g.hside, 0, 0), eps[:, 0] == 0), ((0, g.hside, 0), eps[:, 1] == 0), ((0, 0, g.hside), eps[:, 2] == 0)]
+    for shift, got_out in pts:
+        gm = ex.gmin(_node_points(g, shift))
+        sure = np.abs(gm) > 1e-5
+        bad = np.nonzero(sure & ((gm > 0) != got_out))[0]
+        assert bad.size == 0, (shift, bad.size, gm[bad[:5]])
+        checked += int(sure.sum())
+    assert checked > 0 and (st == ns.STATUS_INSIDE).sum() > 0
+
+
+def test_skin_of_three_atoms_is_cut_by_the_dummy_cells():
+    """Three atoms: the six dummy points the reference adds (two half-extents
+    outside the bounding box, SkinSurface.cpp:1290-1345) are close enough
+    that the culled dummy cells cut the atoms' mixed cells, so some rays
+    leave the skin where no kept cell lies (odd rays).  Every ray with an even
+    count still crosses the skin exactly where the definition says."""
+    g, jit, o, r = _skin_case(3, 2.4, 0.45, 3.0, 42)
+    assert r["odd_rays"] > 0
+    ex = ExactSkin(jit[:, :3], jit[:, 3], 0.45)
+    xs, ys, zs = ns.grid_coords(g)
+    cnt, T = o.ray_hits(0)
+    N = g.nx
+    L = xs[-1] - xs[0]
+    s_ = np.linspace(0, 1, 4001)
+    even = np.nonzero((cnt > 0) & (cnt % 2 == 0))[0]
+    assert len(even) > 20
+    for ray in even:
+        n_, m_ = divmod(int(ray), N)
+        p = np.column_stack([xs[0] + s_ * L, np.full_like(s_, ys[m_]), np.full_like(s_, zs[n_])])
+        gm = ex.gmin(p)
+        ch = np.nonzero((gm[1:] > 0) != (gm[:-1] > 0))[0]
+        assert len(ch) == cnt[ray]
+        assert np.max(np.abs(s_[ch] + 0.5 / 4000 - T[ray, :cnt[ray]])) <= 0.5 / 4000 + 1e-12
+
+
+# ------------------------------------------------------------------ blobby
+def test_blobby_vertices_and_smoothing():
+    xyzr, _ = ns.generate_chain(25, 6.0, seed=51)
+    g = ns.build_grid(xyzr, 2.0, 70.0)
+    B = -2.5
+    res = binding.blobby_build(g, xyzr, B)
+    mc, raw, t, field = res["mc"], res["raw"], res["tris"], res["field"]
+    assert len(mc) == len(raw) and len(t) > 0
+    # each marching-cubes vertex lies on a grid edge between corners x -+ hside
+    # where the field's linear interpolant is 1
+    xs, ys, zs = ns.grid_coords(g)
+    lo = np.array([xs[0], ys[0], zs[0]]) - g.hside
+    q = (mc - lo) / g.side
+    on = np.abs(q - np.round(q)) < 1e-9
+    assert np.all(on.sum(axis=1) >= 2)
+    checked = 0
+    for vi in range(len(mc)):
+        d = int(np.argmin(on[vi]))  # the coordinate that varies along the edge
+        if on[vi].all():
+            continue  # exactly at a corner
+        idx = np.round(q[vi]).astype(int)
+        a = idx.copy()
+        a[d] = int(np.floor(q[vi, d]))
+        b = a.copy()
+        b[d] += 1
+        fa, fb = field[a[2], a[1], a[0]], field[b[2], b[1], b[0]]
+        mu = q[vi, d] - a[d]
+        assert (fa - 1.0) * (fb - 1.0) <= 0
+        assert abs(fa + mu * (fb - fa) - 1.0) < 1e-9
+        checked += 1
+    assert checked > 0.9 * len(mc)
+    # smoothing: v' = (v + sum over its triangles of the other two vertices / (2 n_tri)) / 2
+    acc = np.zeros_like(mc)
+    cnt = np.zeros(len(mc))
+    for a_, b_, c_ in ((0, 1, 2), (1, 0, 2), (2, 0, 1)):
+        np.add.at(acc, t[:, a_], mc[t[:, b_]] + mc[t[:, c_]])
+        np.add.at(cnt, t[:, a_], 2)
+    exp = 0.5 * (mc + acc / cnt[:, None])
+    assert np.max(np.abs(exp - raw)) < 1e-12
