@@ -232,6 +232,26 @@ int nsgpu_set_patches(nsgpu_ctx *ctx, const nsgpu_patches *p, double probe_radiu
  * later call on the context is ordered after it. */
 int nsgpu_get_surf(nsgpu_ctx *ctx, const nsgpu_surf_opts *opts, nsgpu_surf_result *res);
 
+/* nsgpu_get_surf without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The maps
+ * are written to the caller's device arrays in the layouts of the read calls
+ * below (epsmap and idebmap when opts->build_epsmap, status when
+ * opts->build_status; the others may be NULL).  Every scratch buffer of the
+ * call comes from one device array of NSGPU_NOALLOC_SCRATCH_BYTES per GPU,
+ * held by the library until the process ends; a call whose buffers do not fit
+ * fails with NSGPU_E_NOMEM and says how many bytes it needed (grids up to
+ * about 150^3 nodes fit).  Calls on one GPU share that array: calls that
+ * overlap on one GPU race.  Edge-ray panels are refused
+ * (opts->accurate_triangulation must be 0), as are more cavity components
+ * than the device numbering holds (4,096).  Returns when the maps are
+ * written; *res as nsgpu_get_surf.  The context's own map and ray buffers do
+ * not receive this call's results (the read calls refuse until the next
+ * nsgpu_get_surf), and no cavity sizes are kept. */
+#define NSGPU_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int nsgpu_get_surf_noalloc(nsgpu_ctx *ctx, const nsgpu_surf_opts *opts, nsgpu_surf_result *res, int32_t *epsmap,
+                           uint8_t *idebmap, int32_t *status);
+
 /* Surface::triangulateSurface(outputMesh, buildAtomsMap, iso=0) (Surface.cpp:9909-10520):
  * classify cubes from the corner bitmap, add the missing edge midpoints,
  * emit triangles (reversed winding, as the reference), sum the area. */

@@ -70,6 +70,7 @@ SIGNATURES = {
     "nsgpu_set_grid": (ctypes.c_int, [VP, ctypes.POINTER(Grid)]),
     "nsgpu_set_patches": (ctypes.c_int, [VP, ctypes.POINTER(Patches), ctypes.c_double]),
     "nsgpu_get_surf": (ctypes.c_int, [VP, ctypes.POINTER(SurfOpts), ctypes.POINTER(SurfResult)]),
+    "nsgpu_get_surf_noalloc": (ctypes.c_int, [VP, ctypes.POINTER(SurfOpts), ctypes.POINTER(SurfResult), VP, VP, VP]),
     "nsgpu_triangulate": (ctypes.c_int, [VP, ctypes.POINTER(MeshInfo)]),
     "nsgpu_synchronize": (ctypes.c_int, [VP]),
     "nsgpu_set_profiling": (ctypes.c_int, [VP, ctypes.c_int]),

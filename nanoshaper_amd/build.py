@@ -31,7 +31,9 @@ def sources():
                   if f.endswith(".hip"))
     host = sorted(os.path.join(CSRC, "host", f) for f in os.listdir(os.path.join(CSRC, "host"))
                   if f.endswith(".cpp"))
-    return kern, host
+    host_hip = sorted(os.path.join(CSRC, "host", f) for f in os.listdir(os.path.join(CSRC, "host"))
+                      if f.endswith(".hip"))
+    return kern + host_hip, host
 
 
 def _compile(src, is_hip):

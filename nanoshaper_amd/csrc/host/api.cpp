@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../kernels/launch.hpp"
+#include "internal.hpp"
 #include "nsgpu.h"
 
 
@@ -21,6 +22,14 @@ namespace {
 // set while a graph is being captured: buffers must already be large enough
 thread_local bool g_no_alloc = false;
 
+// set while nsgpu_get_surf_noalloc runs: buffers are carved from this
+// caller-provided device region (bump allocation) instead of hipMalloc
+struct Arena {
+    char *base;
+    size_t cap, used, need;
+};
+thread_local Arena *g_arena = nullptr;
+
 // grow-only device buffer; serial counts its (re)allocations, so that a
 // captured graph holding its address can tell that it moved
 struct DBuf {
@@ -29,6 +38,7 @@ struct DBuf {
     uint32_t serial = 0;
     hipError_t ensure(size_t bytes) {
         if (bytes <= cap && p) return hipSuccess;
+        if (g_arena) return carve(bytes);
         if (g_no_alloc) return hipErrorOutOfMemory;
         if (p) (void)hipFree(p);
         p = nullptr;
@@ -39,8 +49,23 @@ struct DBuf {
         if (e == hipSuccess) cap = b;
         return e;
     }
+    // the next 256-byte aligned bytes of the arena; the region it had is
+    // left behind (the arena restarts at every attempt of the call)
+    hipError_t carve(size_t bytes) {
+        const size_t off = (g_arena->used + 255) & ~(size_t)255;
+        if (off > g_arena->cap || bytes > g_arena->cap - off) {
+            g_arena->need = std::max(g_arena->need, off + bytes);
+            return hipErrorOutOfMemory;
+        }
+        p = g_arena->base + off;
+        cap = bytes;
+        g_arena->used = off + bytes;
+        ++serial;
+        return hipSuccess;
+    }
     void release() {
-        if (p) (void)hipFree(p);
+        const bool in_arena = g_arena && (char *)p >= g_arena->base && (char *)p < g_arena->base + g_arena->cap;
+        if (p && !in_arena) (void)hipFree(p);
         p = nullptr;
         cap = 0;
         ++serial;
@@ -236,6 +261,7 @@ int nsgpu_create(int device, nsgpu_ctx **out) {
     }
     std::memset(c->hres, 0, sizeof(HostResults));
     mc_upload_tables(c->stream);
+    nsg_scratch_setup(device);
     *out = c;
     return NSGPU_OK;
 }
@@ -1077,6 +1103,111 @@ int nsgpu_get_surf(nsgpu_ctx *c, const nsgpu_surf_opts *o, nsgpu_surf_result *re
     if (res) *res = c->res;
     return NSGPU_OK;
 }
+
+}  // extern "C"
+
+int nsg_ctx_device(const nsgpu_ctx *c) { return c->device; }
+
+// The input buffers (grid, patches, mesh, atoms) and the result block stay;
+// every other buffer of the context is set aside for an allocation-free call.
+static bool kept_buffer(nsgpu_ctx *c, const DBuf *b) {
+    const DBuf *keep[] = {&c->gx, &c->gy, &c->gz, &c->pkind, &c->poff, &c->pcnt, &c->pflags, &c->pbsph,
+                          &c->pdata, &c->porder, &c->patom, &c->atoms, &c->mverts, &c->mtris, &c->mrr, &c->dres};
+    for (const DBuf *k : keep)
+        if (k == b) return true;
+    return false;
+}
+
+int nsg_get_surf_in(nsgpu_ctx *c, const nsgpu_surf_opts *o, nsgpu_surf_result *res, int32_t *epsmap,
+                    uint8_t *idebmap, int32_t *status, void *scratch, size_t bytes) {
+    if (!c || !o) return NSGPU_E_ARG;
+    if (!scratch) return fail(c, NSGPU_E_DEVICE, "no scratch array on this device");
+    if (o->accurate_triangulation)
+        return fail(c, NSGPU_E_ARG, "the allocation-free getSurf has no edge-ray panels (accurate_triangulation = 0)");
+    if ((o->build_epsmap && (!epsmap || !idebmap)) || (o->build_status && !status))
+        return fail(c, NSGPU_E_ARG, "a map to build has no output array");
+    int rc = begin_surf(c, o);
+    if (rc) return rc;
+    HIP_TRY(c, hipSetDevice(c->device));
+    const int64_t nvox = (int64_t)c->grid.nx * c->grid.ny * c->grid.nz;
+    // set the context's own buffers aside (a fixed array: no host allocation)
+    constexpr int kMaxBufs = 128;
+    DBuf saved[kMaxBufs];
+    DBuf *which[kMaxBufs];
+    int ns = 0;
+    for_each_buf(c, [&](DBuf *b) {
+        if (kept_buffer(c, b) || ns == kMaxBufs) return;
+        which[ns] = b;
+        saved[ns++] = *b;
+    });
+    Arena a{static_cast<char *>(scratch), bytes, 0, 0};
+    c->record_pid = false;
+    for (int attempt = 0;; ++attempt) {
+        if (attempt == 4) {
+            rc = fail(c, NSGPU_E_STATE, "ray hit capacity did not converge");
+            break;
+        }
+        a.used = 0;
+        for (int q = 0; q < ns; ++q) {
+            which[q]->p = nullptr;
+            which[q]->cap = 0;
+        }
+        // the maps are the caller's arrays; a map not built is never read or
+        // written by the kernels (the fill and the cavity labels test
+        // build_eps / build_status), so it only needs a non-null address
+        c->eps.p = o->build_epsmap ? (void *)epsmap : scratch;
+        c->ideb.p = o->build_epsmap ? (void *)idebmap : scratch;
+        c->status.p = o->build_status ? (void *)status : scratch;
+        c->eps.cap = o->build_epsmap ? (size_t)nvox * 12 : SIZE_MAX;
+        c->ideb.cap = o->build_epsmap ? (size_t)nvox : SIZE_MAX;
+        c->status.cap = o->build_status ? (size_t)nvox * 4 : SIZE_MAX;
+        g_arena = &a;
+        rc = enqueue_surf(c, c->stream);
+        g_arena = nullptr;
+        if (rc) {
+            // kernels already queued may still write the caller's arrays
+            (void)hipStreamSynchronize(c->stream);
+            if (a.need > a.cap) {
+                char msg[160];
+                std::snprintf(msg, sizeof msg, "the call needs %zu bytes of scratch; the allocation-free path holds %zu",
+                              a.need, a.cap);
+                rc = fail(c, NSGPU_E_NOMEM, msg);
+            }
+            break;
+        }
+        if (hipStreamSynchronize(c->stream) != hipSuccess) {
+            rc = fail(c, NSGPU_E_DEVICE, "stream synchronisation failed");
+            break;
+        }
+        if (!regrow(c)) break;  // longer ray lists: the next attempt carves them
+    }
+    if (!rc) {
+        take_trace_stats(c);
+        int64_t traced = 0;
+        (void)volume_slots(c, &traced);
+        c->res.total_rays = traced;
+        c->res.volume = c->hres->volume();
+        c->cav_size.clear();
+        c->cav_filled.clear();
+        c->cav_on_device = false;  // the sizes stayed in the scratch array
+        c->res.num_cavities = 0;
+        if (c->opts.fill_cavities && c->opts.build_status) {
+            const int *info = c->hres->cav_info();
+            if (info[0] == 0)
+                rc = fail(c, NSGPU_E_STATE,
+                          "more cavity components than the device numbering holds: use nsgpu_get_surf");
+            else
+                c->res.num_cavities = info[1];
+        }
+    }
+    for (int q = 0; q < ns; ++q) *which[q] = saved[q];
+    // the context's buffers do not hold this call's results
+    c->have_trace = c->have_surf = c->have_vertices = c->have_mesh = false;
+    if (!rc && res) *res = c->res;
+    return rc;
+}
+
+extern "C" {
 
 int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
     if (!c) return NSGPU_E_ARG;
