@@ -740,8 +740,8 @@ static int cavities_on_host(nsgpu_ctx *c) {
 // panel rule of getSurf (Surface.cpp:652-659) and the per-call state
 static int begin_surf(nsgpu_ctx *c, const nsgpu_surf_opts *o) {
     if (!c->have_grid || !c->have_patches) return fail(c, NSGPU_E_STATE, "set_grid and set_patches first");
-    if ((c->surface_is_mesh || c->npatch_skin) && c->want_normals && o->accurate_triangulation)
-        return fail(c, NSGPU_E_STATE, "vertex normals need an SES patch set");
+    if (c->surface_is_mesh && c->want_normals && o->accurate_triangulation)
+        return fail(c, NSGPU_E_STATE, "vertex normals need analytic patches (ses or skin)");
     if (o->precision != NSGPU_PREC_FP64 && o->precision != NSGPU_PREC_MIXED && o->precision != NSGPU_PREC_STURM)
         return fail(c, NSGPU_E_ARG, "unknown precision mode");
     c->opts = *o;

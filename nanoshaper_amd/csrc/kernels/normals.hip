@@ -3,7 +3,8 @@
 // Analytic vertices come from edge-ray depths: their normal is the patch's
 // analytic normal at the intersection point, as getPatchIntersectionData
 // stores it with the depth (ConnollySurface::getNormal, ConnollySurface.cpp:
-// 5736-5766; getNormalToTorus :5794-5819).  Mid-point vertices have no
+// 5736-5766; getNormalToTorus :5794-5819; the skin cells' quadric gradient,
+// SkinSurface.cpp:3680-3696).  Mid-point vertices have no
 // analytic normal: like approximateNormals (Surface.cpp:10588-10737) their
 // normal is the normalised sum of plane3points (tools.cpp:258-274, not
 // normalised) over the triangles that use them, in triangle order.
@@ -14,6 +15,16 @@ namespace nsg {
 __device__ void analytic_normal(const DevPatches &P, int pid, double probe, const double y[3], double n[3]) {
     const int kind = P.kind[pid];
     const double *c = P.bsph + 4 * pid;
+    if (kind >= NSGPU_PATCH_SKIN_POINT) {
+        // skin cell: 2 grad Q / 2 = Q (x, 1), doubled and negated in the
+        // concave tetra cells, not normalised (SkinSurface.cpp:3680-3696)
+        const double *Q = P.data + P.off[pid] + 6;
+        const double sg = kind == NSGPU_PATCH_SKIN_TETRA ? -2.0 : 2.0;
+        n[0] = sg * (Q[0] * y[0] + Q[4] * y[1] + Q[7] * y[2] + Q[9]);
+        n[1] = sg * (Q[4] * y[0] + Q[1] * y[1] + Q[5] * y[2] + Q[8]);
+        n[2] = sg * (Q[7] * y[0] + Q[5] * y[1] + Q[2] * y[2] + Q[6]);
+        return;
+    }
     if (kind == NSGPU_PATCH_FACET_REG || kind == NSGPU_PATCH_FACET_SING) {
         for (int q = 0; q < 3; ++q) n[q] = (c[q] - y[q]) / probe;
         return;

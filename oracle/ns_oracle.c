@@ -1168,7 +1168,15 @@ static void patch_normal(const orc_surface *s, int it, const double y[3], double
     const nsgpu_patches *P = &s->P;
     const int kind = P->kind[it];
     const double *c = P->bsph + 4 * (size_t)it;
-    if (kind == NSGPU_PATCH_FACET_REG || kind == NSGPU_PATCH_FACET_SING) {
+    if (kind >= NSGPU_PATCH_SKIN_POINT) {
+        /* SkinSurface::getPatchIntersectionData (SkinSurface.cpp:3680-3696):
+         * double_normal_sign * (Q (x, 1)), -2 in tetra cells, +2 elsewhere */
+        const double *Q = P->data + P->off[it] + 6;
+        const double sg = (kind == NSGPU_PATCH_SKIN_TETRA) ? -2.0 : 2.0;
+        n[0] = sg * (Q[0] * y[0] + Q[4] * y[1] + Q[7] * y[2] + Q[9]);
+        n[1] = sg * (Q[4] * y[0] + Q[1] * y[1] + Q[5] * y[2] + Q[8]);
+        n[2] = sg * (Q[7] * y[0] + Q[5] * y[1] + Q[2] * y[2] + Q[6]);
+    } else if (kind == NSGPU_PATCH_FACET_REG || kind == NSGPU_PATCH_FACET_SING) {
         for (int q = 0; q < 3; q++) n[q] = (c[q] - y[q]) / s->probe;
     } else if (kind == NSGPU_PATCH_POINT) {
         for (int q = 0; q < 3; q++) n[q] = (y[q] - c[q]) / c[3];

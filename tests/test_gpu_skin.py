@@ -81,11 +81,19 @@ def test_skin_flags(device_ok, opts):
     _compare(dev, orc, opts)
 
 
-def test_skin_refuses_normals_and_gradients(device_ok):
-    grid, ps, dev, orc = _skin(40, 2.0, 1)
+@pytest.mark.parametrize("n,scale,seed,s", [(1, 4.0, 0, 0.45), (330, 2.0, 0, 0.45), (330, 3.0, 2, 0.8)])
+def test_skin_vertex_normals_match_oracle(device_ok, n, scale, seed, s):
+    grid, ps, dev, orc = _skin(n, scale, seed, s)
     dev.set_vertex_normals(True)
-    with pytest.raises(ns.NsgpuError):
-        dev.get_surf(**OPTS)
+    _compare(dev, orc, OPTS)
+    na, no = dev.normals(), orc.normals()
+    assert na.shape == no.shape
+    # analytic normals: the same quadric expression at points equal to 1e-11 A
+    assert np.abs(na - no).max() < 1e-9
+
+
+def test_skin_refuses_gradients(device_ok):
+    grid, ps, dev, orc = _skin(40, 2.0, 1)
     with pytest.raises(ns.NsgpuError):
         dev.surface_gradient()
 

@@ -327,6 +327,31 @@ def test_skin_of_three_atoms_is_cut_by_the_dummy_cells():
         assert np.max(np.abs(s_[ch] + 0.5 / 4000 - T[ray, :cnt[ray]])) <= 0.5 / 4000 + 1e-12
 
 
+def test_skin_vertex_normals_are_the_outward_gradient():
+    """Analytic skin vertex normals (2 grad Q of the vertex's cell, negated in
+    the concave tetra cells, SkinSurface.cpp:3680-3696) point along the
+    outward gradient of the skin's defining function; one atom's are 2 (x - c)."""
+    a = np.array([[0.3, -0.2, 0.1, 1.7]])
+    o = binding.OracleSurface(ns.build_grid(np.repeat(a, 4, 0), 3.0, 60.0), ns.skin_patches(a, 0.45))
+    o.get_surf(build_epsmap=True, build_status=True, accurate_triangulation=True, fill_cavities=False)
+    m = o.triangulate()
+    v, _ = o.mesh()
+    na = m["num_analytic_vertices"]
+    assert np.abs(o.normals()[:na] - 2 * (v[:na] - a[0, :3])).max() < 1e-12
+    g, jit, o, r = _skin_case(12, 3.8, 0.6, 2.0, 45)
+    o.get_surf(build_epsmap=True, build_status=True, accurate_triangulation=True, fill_cavities=False)
+    m = o.triangulate()
+    v, _ = o.mesh()
+    na = m["num_analytic_vertices"]
+    n = o.normals()[:na]
+    ex = ExactSkin(jit[:, :3], jit[:, 3], 0.6)
+    h = 1e-5
+    grad = np.column_stack([(ex.gmin(v[:na] + h * e) - ex.gmin(v[:na] - h * e)) / (2 * h) for e in np.eye(3)])
+    cosang = (n * grad).sum(1) / np.linalg.norm(n, axis=1) / np.linalg.norm(grad, axis=1)
+    assert cosang.min() > 1 - 1e-6, cosang.min()
+    assert na > 100
+
+
 # ------------------------------------------------------------------ blobby
 def test_blobby_vertices_and_smoothing():
     xyzr, _ = ns.generate_chain(25, 6.0, seed=51)
