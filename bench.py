@@ -8,7 +8,8 @@ jitter, patch construction) run once before timing.
 
 Prints one JSON line: rays/s (all six panels) as the value, with voxels/s,
 stage times, the roofline of the dominant kernel and the CPU baseline (the
-oracle, single threaded, on the same workload) beside it.  Three extensions
+oracle on the same workload, on all available host cores with the
+reference's thread decomposition and on one core) beside it.  Three extensions
 of the plain path are measured alongside:
 * "mixed_precision": the same step with FP32-seeded, FP64-refined torus roots;
 * "batched": 256 frames of 500-8000 atoms (log-uniform, sigma 0.3 A
@@ -69,25 +70,62 @@ def build_inputs(cfg):
     return xyzr, r_used, grid, ps, time.perf_counter() - t0
 
 
+def host_info():
+    """The host cores the CPU legs may use: the process's CPU set, capped at
+    16 (one GPU's share of the box), and the CPU model."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpus_available": avail, "threads": max(1, min(16, avail)), "cpu_model": model}
+
+
+def _time_oracle(grid, ps, threads, budget_s, min_runs=1):
+    """Median seconds of getSurf + triangulate of the oracle with `threads`
+    threads (the reference's decomposition), repeated while the budget lasts."""
+    from oracle import binding
+    binding.set_threads(threads)
+    try:
+        orc = binding.OracleSurface(grid, ps)
+        times, rays = [], 0
+        t_all = time.perf_counter()
+        while True:
+            t = time.perf_counter()
+            r = orc.get_surf()
+            orc.triangulate()
+            times.append(time.perf_counter() - t)
+            rays = r["total_rays"]
+            if len(times) >= min_runs and time.perf_counter() - t_all + times[-1] > budget_s:
+                break
+        orc.close()
+    finally:
+        binding.set_threads(1)
+    return float(np.median(times)), len(times), rays
+
+
 def cpu_baseline(grid, ps, budget_s):
-    """Oracle (single-threaded C restatement) on the full workload, repeated
-    while the budget lasts (at least once)."""
-    from oracle.binding import OracleSurface
-    orc = OracleSurface(grid, ps)
-    times, rays = [], 0
-    t_all = time.perf_counter()
-    while True:
-        t = time.perf_counter()
-        r = orc.get_surf()
-        orc.triangulate()
-        times.append(time.perf_counter() - t)
-        rays = r["total_rays"]
-        if time.perf_counter() - t_all + times[-1] > budget_s:
-            break
-    orc.close()
-    sec = float(np.median(times))
-    return {"value": rays / sec, "unit": "rays/s", "cores": 1, "kind": "port",
-            "sample": "full workload, %d run(s), median %.3f s per step" % (len(times), sec)}
+    """The oracle (C restatement of the reference's path) on the full
+    workload: on all available host cores with the reference's thread
+    decomposition (patches strided over threads, 4-row ray blocks, z-strided
+    marching cubes), and on one core."""
+    info = host_info()
+    T = info["threads"]
+    sec_t, runs_t, rays = _time_oracle(grid, ps, T, 0.5 * budget_s)
+    sec_1, runs_1, _ = _time_oracle(grid, ps, 1, 0.5 * budget_s)
+    return {"value": rays / sec_t, "unit": "rays/s", "cores": T, "kind": "port",
+            "sample": "full workload: %d thread(s) median %.3f s per step over %d run(s); 1 thread median %.3f s "
+                      "over %d run(s)" % (T, sec_t, runs_t, sec_1, runs_1),
+            "single_core": {"value": rays / sec_1, "unit": "rays/s", "cores": 1, "seconds_per_step": sec_1},
+            "seconds_per_step": sec_t, **info}
 
 
 def bench_mixed(dev, steps, warmup):
@@ -238,7 +276,8 @@ def bench_blobby(steps=5, cpu=True):
         cpu_s = time.perf_counter() - t0
         o.close()
         res["cpu_baseline"] = {"seconds_per_step": cpu_s, "cores": 1, "kind": "port",
-                               "sample": "full workload, 1 run"}
+                               "sample": "full workload, 1 run (the blobby build and the mesh cast of the oracle "
+                                         "are single threaded)"}
         res["speedup_vs_cpu"] = cpu_s / step
     return res
 
@@ -282,15 +321,28 @@ def bench_skin(steps=10, cpu=True):
            "result": {"volume": r["volume"], "hits": r["total_hits"], "odd_rays": r["odd_rays"],
                       "cavities": r["num_cavities"], "triangles": m["num_triangles"], "area": m["area"]}}
     if cpu:
+        # the oracle on all available cores (skin cells strided over threads,
+        # ray rows in 4-row blocks, z-strided marching cubes) and on one core
         from oracle import binding
-        o = binding.OracleSurface(grid, ps)
-        t0 = time.perf_counter()
-        o.get_surf(**opts)
-        o.triangulate()
-        cpu_s = time.perf_counter() - t0
-        o.close()
-        res["cpu_baseline"] = {"seconds_per_step": cpu_s, "cores": 1, "kind": "port", "sample": "full workload, 1 run"}
-        res["speedup_vs_cpu"] = cpu_s / step
+        info = host_info()
+        secs = {}
+        for threads in (info["threads"], 1):
+            binding.set_threads(threads)
+            try:
+                o = binding.OracleSurface(grid, ps)
+                t0 = time.perf_counter()
+                o.get_surf(**opts)
+                o.triangulate()
+                secs[threads] = time.perf_counter() - t0
+                o.close()
+            finally:
+                binding.set_threads(1)
+        T = info["threads"]
+        res["cpu_baseline"] = {"seconds_per_step": secs[T], "cores": T, "kind": "port",
+                               "sample": "full workload, 1 run per thread count",
+                               "single_core_seconds_per_step": secs[1], **info}
+        res["speedup_vs_cpu"] = secs[T] / step
+        res["speedup_vs_cpu_1core"] = secs[1] / step
     return res
 
 
@@ -453,6 +505,7 @@ def main():
         "roofline_fill": fill_roof,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "speedup_vs_cpu_1core": (value / cpu["single_core"]["value"]) if cpu else None,
         "result": {"volume": r["volume"], "area": m["area"], "cavities": r["num_cavities"],
                    "triangles": m["num_triangles"], "hits": r["total_hits"], "odd_rays": r["odd_rays"]},
     }

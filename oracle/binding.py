@@ -51,6 +51,7 @@ def load():
                                          VP, VP, VP, VP]),
         "orc_is_feasible": (ctypes.c_int, [ctypes.POINTER(Patches), ctypes.c_int, VP]),
         "orc_set_root_refinement": (None, [ctypes.c_int]),
+        "orc_set_threads": (None, [ctypes.c_int]),
         "orc_sturm_quartic": (ctypes.c_int, [VP, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
         "orc_create_mesh": (VP, [ctypes.POINTER(Grid), VP, ctypes.c_int64, VP, ctypes.c_int64, ctypes.c_int,
                                  ctypes.c_int]),
@@ -72,6 +73,12 @@ def load():
 
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def set_threads(n):
+    """Threads of the oracle's patch loop, ray rows and marching-cubes slices
+    (the reference's decomposition); results do not depend on the count."""
+    load().orc_set_threads(int(n))
 
 
 def blobby_build(grid, xyzr, blobbyness=-2.5, cutoff=6.0):

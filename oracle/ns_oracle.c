@@ -14,6 +14,7 @@
  * invariants, brute force on small inputs) are in tests/test_oracle.py.
  */
 #include <math.h>
+#include <pthread.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -544,6 +545,43 @@ typedef struct orc_surface {
 
 static void *xcalloc(size_t n, size_t s) { return calloc(n ? n : 1, s); }
 
+/* Threads of the timing runs (orc_set_threads): the patch loop, the ray rows
+ * and the marching-cubes slices are dealt to them as the reference deals its
+ * boost threads (patches strided, Surface.cpp:692-770; 4-row blocks,
+ * :1020-1066; z slices, :10002, :10207).  Every stage merges its per-thread
+ * output in the single-thread order, so the results do not depend on the
+ * thread count. */
+static int g_threads = 1;
+void orc_set_threads(int t) { g_threads = t < 1 ? 1 : (t > 256 ? 256 : t); }
+
+typedef struct {
+    void (*fn)(void *, int, int);
+    void *arg;
+    int tid, n;
+} orc_thread_arg;
+
+static void *orc_thread_main(void *p) {
+    orc_thread_arg *a = (orc_thread_arg *)p;
+    a->fn(a->arg, a->tid, a->n);
+    return NULL;
+}
+
+static void run_threads(void (*fn)(void *, int, int), void *arg) {
+    const int T = g_threads;
+    if (T == 1) {
+        fn(arg, 0, 1);
+        return;
+    }
+    pthread_t th[256];
+    orc_thread_arg ta[256];
+    for (int t = 1; t < T; t++) {
+        ta[t] = (orc_thread_arg){fn, arg, t, T};
+        pthread_create(&th[t], NULL, orc_thread_main, &ta[t]);
+    }
+    fn(arg, 0, T);
+    for (int t = 1; t < T; t++) pthread_join(th[t], NULL);
+}
+
 orc_surface *orc_create(const nsgpu_grid *g, const nsgpu_patches *p, double probe) {
     orc_surface *s = (orc_surface *)xcalloc(1, sizeof(orc_surface));
     s->g = *g;
@@ -1003,21 +1041,23 @@ orc_surface *orc_create_mesh(const nsgpu_grid *g, const double *verts, int64_t n
     return s;
 }
 
-static void patch_intersections(orc_surface *s) {
+/* ConnollySurface::getPatchIntersectionData (ConnollySurface.cpp:4319-4591)
+ * for patch it: its candidate pixels, ray/primitive roots and feasibility;
+ * hits appended to the caller's buffer */
+static void patch_hits(orc_surface *s, int it, orc_hit **hitsp, int64_t *nhp, int64_t *capp, int64_t *candp) {
     const nsgpu_grid *g = &s->g;
     const nsgpu_patches *P = &s->P;
     const int64_t NX = s->NX, NY = s->NY, NZ = s->NZ, N_MAX = s->N;
     const int numPanels = s->numPanels;
     const double hside = g->hside, scale = g->scale;
-    orc_hit *hits = NULL;
-    int64_t nh = 0, cap = 0, cand = 0;
+    orc_hit *hits = *hitsp;
+    int64_t nh = *nhp, cap = *capp, cand = *candp;
     double pa[3] = {0, 0, 0};
-    if (s->is_mesh) mesh_intersections(s, &hits, &nh, &cap, &cand);
-    for (int it = 0; it < (s->is_mesh ? 0 : P->n); it++) {
+    {
         const int kind = P->kind[it];
         if (kind >= NSGPU_PATCH_SKIN_POINT) {
             skin_cell_hits(s, it, &hits, &nh, &cap, &cand);
-            continue;
+            goto done;
         }
         const double *sphere_center = P->bsph + 4 * (size_t)it;
         const double radius = P->bsph[4 * (size_t)it + 3];
@@ -1027,12 +1067,12 @@ static void patch_intersections(orc_surface *s) {
                downz = sphere_center[2] - radius;
         double upx = sphere_center[0] + radius, upy = sphere_center[1] + radius, upz = sphere_center[2] + radius;
         /* CELL_CULLING */
-        if (downx > g->xmax + ORC_EPS_INT) continue;
-        if (downy > g->ymax + ORC_EPS_INT) continue;
-        if (downz > g->zmax + ORC_EPS_INT) continue;
-        if (upx < g->xmin - (hside - ORC_DELTA + ORC_EPS_INT)) continue;
-        if (upy < g->ymin - (hside - ORC_DELTA + ORC_EPS_INT)) continue;
-        if (upz < g->zmin - (hside - ORC_DELTA + ORC_EPS_INT)) continue;
+        if (downx > g->xmax + ORC_EPS_INT) goto done;
+        if (downy > g->ymax + ORC_EPS_INT) goto done;
+        if (downz > g->zmax + ORC_EPS_INT) goto done;
+        if (upx < g->xmin - (hside - ORC_DELTA + ORC_EPS_INT)) goto done;
+        if (upy < g->ymin - (hside - ORC_DELTA + ORC_EPS_INT)) goto done;
+        if (upz < g->zmin - (hside - ORC_DELTA + ORC_EPS_INT)) goto done;
         int64_t i_start[3], i_end[3];
         i_start[0] = (int64_t)rintp(fmax(0., downx - g->xmin) * scale);
         i_start[1] = (int64_t)rintp(fmax(0., downy - g->ymin) * scale);
@@ -1118,6 +1158,95 @@ static void patch_intersections(orc_surface *s) {
             }
         }
     }
+done:
+    *hitsp = hits;
+    *nhp = nh;
+    *capp = cap;
+    *candp = cand;
+}
+
+typedef struct {
+    orc_surface *s;
+    orc_hit *hits[256];
+    int64_t nh[256], cap[256], cand[256];
+} orc_patch_job;
+
+static void patch_worker(void *arg, int tid, int nthreads) {
+    orc_patch_job *j = (orc_patch_job *)arg;
+    for (int it = tid; it < j->s->P.n; it += nthreads) patch_hits(j->s, it, &j->hits[tid], &j->nh[tid], &j->cap[tid], &j->cand[tid]);
+}
+
+typedef struct {
+    orc_hit *hits;
+    const int64_t *off;
+    int64_t npix;
+    int maxh[256];
+} orc_sort_job;
+
+static void sort_worker(void *arg, int tid, int nthreads) {
+    orc_sort_job *j = (orc_sort_job *)arg;
+    int maxh = 0;
+    for (int64_t p = tid; p < j->npix; p += nthreads) {
+        int64_t c = j->off[p + 1] - j->off[p];
+        if (c > 1) qsort(j->hits + j->off[p], c, sizeof(orc_hit), cmp_hit_t);
+        if (c > maxh) maxh = (int)c;
+    }
+    j->maxh[tid] = maxh;
+}
+
+static void patch_intersections(orc_surface *s) {
+    const int64_t N_MAX = s->N;
+    const int numPanels = s->numPanels;
+    orc_hit *hits = NULL;
+    int64_t nh = 0, cap = 0, cand = 0;
+    if (s->is_mesh) mesh_intersections(s, &hits, &nh, &cap, &cand);
+    if (!s->is_mesh && g_threads == 1) {
+        for (int it = 0; it < s->P.n; it++) patch_hits(s, it, &hits, &nh, &cap, &cand);
+    } else if (!s->is_mesh) {
+        /* patches strided over the threads; the hits are then sorted by
+         * pixel (counting sort) and by depth inside each pixel (pixels strided
+         * over the threads), which orders them as the single-thread path */
+        orc_patch_job *job = (orc_patch_job *)xcalloc(1, sizeof(orc_patch_job));
+        job->s = s;
+        run_threads(patch_worker, job);
+        for (int t = 0; t < g_threads; t++) {
+            cand += job->cand[t];
+            nh += job->nh[t];
+        }
+        hits = (orc_hit *)xcalloc(nh, sizeof(orc_hit));
+        s->npix = N_MAX * N_MAX * numPanels;
+        s->pix_off = (int64_t *)xcalloc(s->npix + 1, sizeof(int64_t));
+        for (int t = 0; t < g_threads; t++)
+            for (int64_t h = 0; h < job->nh[t]; h++) s->pix_off[job->hits[t][h].pix + 1]++;
+        for (int64_t p = 0; p < s->npix; p++) s->pix_off[p + 1] += s->pix_off[p];
+        int64_t *at = (int64_t *)xcalloc(s->npix, sizeof(int64_t));
+        memcpy(at, s->pix_off, s->npix * sizeof(int64_t));
+        for (int t = 0; t < g_threads; t++) {
+            for (int64_t h = 0; h < job->nh[t]; h++) hits[at[job->hits[t][h].pix]++] = job->hits[t][h];
+            free(job->hits[t]);
+        }
+        free(at);
+        free(job);
+        orc_sort_job sj;
+        memset(&sj, 0, sizeof sj);
+        sj.hits = hits;
+        sj.off = s->pix_off;
+        sj.npix = s->npix;
+        run_threads(sort_worker, &sj);
+        int maxh = 0;
+        for (int t = 0; t < g_threads; t++) maxh = sj.maxh[t] > maxh ? sj.maxh[t] : maxh;
+        s->pix_t = (double *)xcalloc(nh, sizeof(double));
+        s->pix_patch = (int32_t *)xcalloc(nh, sizeof(int32_t));
+        for (int64_t h = 0; h < nh; h++) {
+            s->pix_t[h] = hits[h].t;
+            s->pix_patch[h] = hits[h].patch;
+        }
+        free(hits);
+        s->res.total_hits = nh;
+        s->res.candidates = cand;
+        s->res.max_hits_per_ray = maxh;
+        return;
+    }
     /* counting sort by pixel id, then ascending t inside each pixel */
     s->npix = N_MAX * N_MAX * numPanels;
     s->pix_off = (int64_t *)xcalloc(s->npix + 1, sizeof(int64_t));
@@ -1151,7 +1280,7 @@ static void patch_intersections(orc_surface *s) {
 
 static void bit_clear(orc_surface *s, int64_t i, int64_t j, int64_t k) {
     int64_t f = FLAT(i, j, k);
-    s->bits[f >> 5] &= ~(1u << (uint32_t)(f & 31));
+    __atomic_fetch_and(&s->bits[f >> 5], ~(1u << (uint32_t)(f & 31)), __ATOMIC_RELAXED);
 }
 /* read32xCompressedGrid (tools.h:1369-1390) addresses by flat index, so
  * i = NX reads the first node of the next row; past the end reads "outside" */
@@ -1244,179 +1373,247 @@ static int ray_pairs(const double *t, int64_t vsize, int *pairs) {
     return np;
 }
 
-static void fill_panel(orc_surface *s, int panel, double *volPanel, int *flags) {
+/* Output of one ray row of the fill: counters, each ray's segment volume and
+ * the vertex records of the edge rays, merged into the surface in row order
+ * so that any row partition gives the single-thread result. */
+typedef struct {
+    int64_t rays, odd;
+    double *vol; /* [nb] per ray, 0 for rays without a segment pair */
+    orc_vrec *v;
+    int64_t nv, cap;
+} orc_rowout;
+
+static void row_vrec(orc_rowout *o, int ix, int iy, int iz, int dir, double px, double py, double pz,
+                     const double nrm[3]) {
+    if (o->nv == o->cap) {
+        o->cap = o->cap ? 2 * o->cap : 64;
+        o->v = (orc_vrec *)realloc(o->v, o->cap * sizeof(orc_vrec));
+    }
+    orc_vrec *v = &o->v[o->nv++];
+    v->ix = ix;
+    v->iy = iy;
+    v->iz = iz;
+    v->dir = dir;
+    v->p[0] = px;
+    v->p[1] = py;
+    v->p[2] = pz;
+    memcpy(v->n, nrm, sizeof v->n);
+}
+
+/* the rays of row n of panel `panel`, grid rays (phase 0) or edge rays
+ * (phase 1); pairs: scratch of max_hits_per_ray + 2 ints */
+static void fill_row(orc_surface *s, int panel, int phase, int n, int *pairs, orc_rowout *out) {
     const int64_t NX = s->NX, NY = s->NY, NZ = s->NZ, N_MAX = s->N;
     const double hside = s->g.hside, scale = s->g.scale;
     const int inside = s->o.inside;
     const int buildEps = s->o.build_epsmap, buildStatus = s->o.build_status;
-    int na, nb;
-    if (panel == 0) { na = (int)NZ; nb = (int)NY; }
-    else if (panel == 1) { na = (int)NY; nb = (int)NX; }
-    else { na = (int)NZ; nb = (int)NX; }
-    int *pairs = (int *)xcalloc(s->res.max_hits_per_ray + 2, sizeof(int));
+    const int nb = (int)(panel == 0 ? NY : NX);
     double pa[3] = {0, 0, 0}, pb[3] = {0, 0, 0}, dir;
-    /* phase 1: grid rays (Surface.cpp:5166-5757) */
-    if ((buildStatus && panel == 0) || buildEps) {
-        flags[2 * panel + 0] = 1;
+    if (phase == 0) {
+        /* grid rays (Surface.cpp:5166-5757) */
         if (panel == 0) { pa[0] = s->x[0]; pb[0] = s->x[NX - 1]; }
         else if (panel == 1) { pa[2] = s->z[0]; pb[2] = s->z[NZ - 1]; }
         else { pa[1] = s->y[0]; pb[1] = s->y[NY - 1]; }
-        for (int n = 0; n < na; n++) {
-            if (panel == 0) { pa[2] = s->z[n]; pb[2] = pa[2]; }
-            else if (panel == 1) { pa[1] = s->y[n]; pb[1] = pa[1]; }
-            else { pa[2] = s->z[n]; pb[2] = pa[2]; }
-            for (int m = 0; m < nb; m++) {
-                s->res.total_rays++;
-                int64_t id = ((int64_t)n * N_MAX + m) * s->numPanels + panel;
-                int64_t vsize = s->pix_off[id + 1] - s->pix_off[id];
-                if (vsize == 0) continue;
-                if (vsize & 1) s->res.odd_rays++;
-                const double *T = s->pix_t + s->pix_off[id];
-                int np = ray_pairs(T, vsize, pairs);
-                if (np == 0) continue;
-                if (panel == 0) { pa[1] = s->y[m]; pb[1] = pa[1]; }
-                else { pa[0] = s->x[m]; pb[0] = pa[0]; }
-                double vol = 0.0;
-                if (panel == 0) {
-                    dir = pb[0] - pa[0];
-                    for (int q = 0; q < np; q++) {
-                        double lim1 = pa[0] + dir * T[pairs[q]];
-                        double lim2 = pa[0] + dir * T[pairs[q] + 1];
-                        vol += lim2 - lim1;
-                        int i1 = (int)rintp((lim1 - s->g.xmin) * scale);
-                        int i2 = (int)rintp((lim2 - s->g.xmin) * scale);
-                        i1 = (i1 < 0) ? 0 : i1;
-                        i1 = (i1 >= NX) ? (int)NX - 1 : i1;
-                        i2 = (i2 < 0) ? 0 : i2;
-                        i2 = (i2 >= NX) ? (int)NX - 1 : i2;
-                        if (i2 < i1) continue;
-                        for (int ix = i1; ix <= i2; ix++) {
-                            if (buildEps && s->x[ix] <= lim2 - hside) EPS(ix, m, n, 0) = inside;
-                            if (s->x[ix] <= lim2 && s->x[ix] >= lim1) {
-                                if (buildStatus) s->status[FLAT(ix, m, n)] = NSGPU_STATUS_INSIDE;
-                                if (buildEps) s->idebmap[FLAT(ix, m, n)] = 0;
-                            }
+        if (panel == 0) { pa[2] = s->z[n]; pb[2] = pa[2]; }
+        else if (panel == 1) { pa[1] = s->y[n]; pb[1] = pa[1]; }
+        else { pa[2] = s->z[n]; pb[2] = pa[2]; }
+        for (int m = 0; m < nb; m++) {
+            out->rays++;
+            out->vol[m] = 0.0;
+            int64_t id = ((int64_t)n * N_MAX + m) * s->numPanels + panel;
+            int64_t vsize = s->pix_off[id + 1] - s->pix_off[id];
+            if (vsize == 0) continue;
+            if (vsize & 1) out->odd++;
+            const double *T = s->pix_t + s->pix_off[id];
+            int np = ray_pairs(T, vsize, pairs);
+            if (np == 0) continue;
+            if (panel == 0) { pa[1] = s->y[m]; pb[1] = pa[1]; }
+            else { pa[0] = s->x[m]; pb[0] = pa[0]; }
+            double vol = 0.0;
+            if (panel == 0) {
+                dir = pb[0] - pa[0];
+                for (int q = 0; q < np; q++) {
+                    double lim1 = pa[0] + dir * T[pairs[q]];
+                    double lim2 = pa[0] + dir * T[pairs[q] + 1];
+                    vol += lim2 - lim1;
+                    int i1 = (int)rintp((lim1 - s->g.xmin) * scale);
+                    int i2 = (int)rintp((lim2 - s->g.xmin) * scale);
+                    i1 = (i1 < 0) ? 0 : i1;
+                    i1 = (i1 >= NX) ? (int)NX - 1 : i1;
+                    i2 = (i2 < 0) ? 0 : i2;
+                    i2 = (i2 >= NX) ? (int)NX - 1 : i2;
+                    if (i2 < i1) continue;
+                    for (int ix = i1; ix <= i2; ix++) {
+                        if (buildEps && s->x[ix] <= lim2 - hside) EPS(ix, m, n, 0) = inside;
+                        if (s->x[ix] <= lim2 && s->x[ix] >= lim1) {
+                            if (buildStatus) s->status[FLAT(ix, m, n)] = NSGPU_STATUS_INSIDE;
+                            if (buildEps) s->idebmap[FLAT(ix, m, n)] = 0;
                         }
                     }
-                } else if (panel == 1) {
-                    dir = pb[2] - pa[2];
-                    for (int q = 0; q < np; q++) {
-                        double lim1 = pa[2] + dir * T[pairs[q]];
-                        double lim2 = pa[2] + dir * T[pairs[q] + 1];
-                        vol += lim2 - lim1;
-                        int k1 = (int)rintp((lim1 - s->g.zmin) * scale);
-                        int k2 = (int)rintp((lim2 - s->g.zmin) * scale);
-                        k1 = (k1 < 0) ? 0 : k1;
-                        k1 = (k1 >= NZ) ? (int)NZ - 1 : k1;
-                        k2 = (k2 < 0) ? 0 : k2;
-                        k2 = (k2 >= NZ) ? (int)NZ - 1 : k2;
-                        if (k2 < k1) continue;
-                        if (buildEps)
-                            for (int iz = k1; iz < k2; iz++)
-                                if (s->z[iz] <= lim2 - hside) EPS(m, n, iz, 2) = inside;
-                    }
-                } else {
-                    dir = pb[1] - pa[1];
-                    for (int q = 0; q < np; q++) {
-                        double lim1 = pa[1] + dir * T[pairs[q]];
-                        double lim2 = pa[1] + dir * T[pairs[q] + 1];
-                        vol += lim2 - lim1;
-                        int j1 = (int)rintp((lim1 - s->g.ymin) * scale);
-                        int j2 = (int)rintp((lim2 - s->g.ymin) * scale);
-                        j1 = (j1 < 0) ? 0 : j1;
-                        j1 = (j1 >= NY) ? (int)NY - 1 : j1;
-                        j2 = (j2 < 0) ? 0 : j2;
-                        j2 = (j2 >= NY) ? (int)NY - 1 : j2;
-                        if (j2 < j1) continue;
-                        if (buildEps)
-                            for (int iy = j1; iy < j2; iy++)
-                                if (s->y[iy] <= lim2 - hside) EPS(m, iy, n, 1) = inside;
-                    }
                 }
-                volPanel[panel] += vol;
+            } else if (panel == 1) {
+                dir = pb[2] - pa[2];
+                for (int q = 0; q < np; q++) {
+                    double lim1 = pa[2] + dir * T[pairs[q]];
+                    double lim2 = pa[2] + dir * T[pairs[q] + 1];
+                    vol += lim2 - lim1;
+                    int k1 = (int)rintp((lim1 - s->g.zmin) * scale);
+                    int k2 = (int)rintp((lim2 - s->g.zmin) * scale);
+                    k1 = (k1 < 0) ? 0 : k1;
+                    k1 = (k1 >= NZ) ? (int)NZ - 1 : k1;
+                    k2 = (k2 < 0) ? 0 : k2;
+                    k2 = (k2 >= NZ) ? (int)NZ - 1 : k2;
+                    if (k2 < k1) continue;
+                    if (buildEps)
+                        for (int iz = k1; iz < k2; iz++)
+                            if (s->z[iz] <= lim2 - hside) EPS(m, n, iz, 2) = inside;
+                }
+            } else {
+                dir = pb[1] - pa[1];
+                for (int q = 0; q < np; q++) {
+                    double lim1 = pa[1] + dir * T[pairs[q]];
+                    double lim2 = pa[1] + dir * T[pairs[q] + 1];
+                    vol += lim2 - lim1;
+                    int j1 = (int)rintp((lim1 - s->g.ymin) * scale);
+                    int j2 = (int)rintp((lim2 - s->g.ymin) * scale);
+                    j1 = (j1 < 0) ? 0 : j1;
+                    j1 = (j1 >= NY) ? (int)NY - 1 : j1;
+                    j2 = (j2 < 0) ? 0 : j2;
+                    j2 = (j2 >= NY) ? (int)NY - 1 : j2;
+                    if (j2 < j1) continue;
+                    if (buildEps)
+                        for (int iy = j1; iy < j2; iy++)
+                            if (s->y[iy] <= lim2 - hside) EPS(m, iy, n, 1) = inside;
+                }
+            }
+            out->vol[m] = vol;
+        }
+        return;
+    }
+    /* edge rays (Surface.cpp:5761-6250) */
+    const double delta = ORC_DELTA - hside;
+    if (panel == 0) { pa[0] = s->x[0] + delta; pb[0] = s->x[NX - 1] + delta; }
+    else if (panel == 1) { pa[2] = s->z[0] + delta; pb[2] = s->z[NZ - 1] + delta; }
+    else { pa[1] = s->y[0] + delta; pb[1] = s->y[NY - 1] + delta; }
+    if (panel == 0) { pa[2] = s->z[n] + delta; pb[2] = pa[2]; }
+    else if (panel == 1) { pa[1] = s->y[n] + delta; pb[1] = pa[1]; }
+    else { pa[2] = s->z[n] + delta; pb[2] = pa[2]; }
+    for (int m = 0; m < nb; m++) {
+        out->rays++;
+        out->vol[m] = 0.0;
+        int64_t id = ((int64_t)n * N_MAX + m) * s->numPanels + (s->panels[0] + panel);
+        int64_t vsize = s->pix_off[id + 1] - s->pix_off[id];
+        if (vsize == 0) continue;
+        if (vsize & 1) out->odd++;
+        const double *T = s->pix_t + s->pix_off[id];
+        int np = ray_pairs(T, vsize, pairs);
+        if (np == 0) continue;
+        if (panel == 0) { pa[1] = s->y[m] + delta; pb[1] = pa[1]; }
+        else { pa[0] = s->x[m] + delta; pb[0] = pa[0]; }
+        double vol = 0.0;
+        if (panel == 0) {
+            dir = pb[0] - pa[0];
+            for (int q = 0; q < np; q++) {
+                double lim1 = pa[0] + dir * T[pairs[q]];
+                double lim2 = pa[0] + dir * T[pairs[q] + 1];
+                vol += lim2 - lim1;
+                int xa = (int)rintp((lim1 - s->g.xmin) * scale);
+                int xb = (int)rintp((lim2 - s->g.xmin) * scale);
+                if (xb <= xa) continue;
+                for (int i = xa + 1; i <= xb; i++)
+                    if (i >= 0 && i < NX) bit_clear(s, i, m, n);
+                double n1[3], n2[3];
+                depth_normal(s, 0, m, n, s->pix_off[id] + pairs[q], n1);
+                depth_normal(s, 0, m, n, s->pix_off[id] + pairs[q] + 1, n2);
+                row_vrec(out, xa, m, n, 0, lim1, pa[1], pa[2], n1);
+                row_vrec(out, xb, m, n, 0, lim2, pa[1], pa[2], n2);
+            }
+        } else if (panel == 1) {
+            dir = pb[2] - pa[2];
+            for (int q = 0; q < np; q++) {
+                double lim1 = pa[2] + dir * T[pairs[q]];
+                double lim2 = pa[2] + dir * T[pairs[q] + 1];
+                vol += lim2 - lim1;
+                int za = (int)rintp((lim1 - s->g.zmin) * scale);
+                int zb = (int)rintp((lim2 - s->g.zmin) * scale);
+                if (zb <= za) continue;
+                for (int k = za + 1; k <= zb; k++)
+                    if (k >= 0 && k < NZ) bit_clear(s, m, n, k);
+                double n1[3], n2[3];
+                depth_normal(s, 1, m, n, s->pix_off[id] + pairs[q], n1);
+                depth_normal(s, 1, m, n, s->pix_off[id] + pairs[q] + 1, n2);
+                row_vrec(out, m, n, za, 2, pa[0], pa[1], lim1, n1);
+                row_vrec(out, m, n, zb, 2, pa[0], pa[1], lim2, n2);
+            }
+        } else {
+            dir = pb[1] - pa[1];
+            for (int q = 0; q < np; q++) {
+                double lim1 = pa[1] + dir * T[pairs[q]];
+                double lim2 = pa[1] + dir * T[pairs[q] + 1];
+                vol += lim2 - lim1;
+                int ya = (int)rintp((lim1 - s->g.ymin) * scale);
+                int yb = (int)rintp((lim2 - s->g.ymin) * scale);
+                if (yb <= ya) continue;
+                for (int j = ya + 1; j <= yb; j++)
+                    if (j >= 0 && j < NY) bit_clear(s, m, j, n);
+                double n1[3], n2[3];
+                depth_normal(s, 2, m, n, s->pix_off[id] + pairs[q], n1);
+                depth_normal(s, 2, m, n, s->pix_off[id] + pairs[q] + 1, n2);
+                row_vrec(out, m, ya, n, 1, pa[0], lim1, pa[2], n1);
+                row_vrec(out, m, yb, n, 1, pa[0], lim2, pa[2], n2);
             }
         }
+        out->vol[m] = vol;
     }
-    /* phase 2: edge rays (Surface.cpp:5761-6250) */
+}
+
+/* the rows of one panel and phase, in 4-row blocks dealt to the threads
+ * (Surface.cpp:1020-1066), then merged in row order */
+typedef struct {
+    orc_surface *s;
+    int panel, phase, na;
+    orc_rowout *rows;
+} orc_fill_job;
+
+static void fill_rows_worker(void *arg, int tid, int nthreads) {
+    orc_fill_job *j = (orc_fill_job *)arg;
+    int *pairs = (int *)xcalloc(j->s->res.max_hits_per_ray + 2, sizeof(int));
+    for (int b = 4 * tid; b < j->na; b += 4 * nthreads)
+        for (int n = b; n < b + 4 && n < j->na; n++) fill_row(j->s, j->panel, j->phase, n, pairs, &j->rows[n]);
+    free(pairs);
+}
+
+static void fill_phase(orc_surface *s, int panel, int phase, double *volPanel) {
+    const int na = (int)(panel == 0 ? s->NZ : (panel == 1 ? s->NY : s->NZ));
+    const int nb = (int)(panel == 0 ? s->NY : s->NX);
+    orc_fill_job job = {s, panel, phase, na, (orc_rowout *)xcalloc(na, sizeof(orc_rowout))};
+    double *vol = (double *)xcalloc((size_t)na * nb, sizeof(double));
+    for (int n = 0; n < na; n++) job.rows[n].vol = vol + (size_t)n * nb;
+    run_threads(fill_rows_worker, &job);
+    for (int n = 0; n < na; n++) {
+        orc_rowout *o = &job.rows[n];
+        s->res.total_rays += o->rays;
+        s->res.odd_rays += o->odd;
+        for (int m = 0; m < nb; m++)
+            if (o->vol[m] != 0.0) volPanel[panel] += o->vol[m];
+        for (int64_t q = 0; q < o->nv; q++)
+            push_vrec(s, o->v[q].ix, o->v[q].iy, o->v[q].iz, o->v[q].dir, o->v[q].p[0], o->v[q].p[1], o->v[q].p[2],
+                      o->v[q].n);
+        free(o->v);
+    }
+    free(vol);
+    free(job.rows);
+}
+
+static void fill_panel(orc_surface *s, int panel, double *volPanel, int *flags) {
+    if ((s->o.build_status && panel == 0) || s->o.build_epsmap) {
+        flags[2 * panel + 0] = 1;
+        fill_phase(s, panel, 0, volPanel);
+    }
     if (s->o.accurate_triangulation) {
         flags[2 * panel + 1] = 1;
-        const double delta = ORC_DELTA - hside;
-        if (panel == 0) { pa[0] = s->x[0] + delta; pb[0] = s->x[NX - 1] + delta; }
-        else if (panel == 1) { pa[2] = s->z[0] + delta; pb[2] = s->z[NZ - 1] + delta; }
-        else { pa[1] = s->y[0] + delta; pb[1] = s->y[NY - 1] + delta; }
-        for (int n = 0; n < na; n++) {
-            if (panel == 0) { pa[2] = s->z[n] + delta; pb[2] = pa[2]; }
-            else if (panel == 1) { pa[1] = s->y[n] + delta; pb[1] = pa[1]; }
-            else { pa[2] = s->z[n] + delta; pb[2] = pa[2]; }
-            for (int m = 0; m < nb; m++) {
-                s->res.total_rays++;
-                int64_t id = ((int64_t)n * N_MAX + m) * s->numPanels + (s->panels[0] + panel);
-                int64_t vsize = s->pix_off[id + 1] - s->pix_off[id];
-                if (vsize == 0) continue;
-                if (vsize & 1) s->res.odd_rays++;
-                const double *T = s->pix_t + s->pix_off[id];
-                int np = ray_pairs(T, vsize, pairs);
-                if (np == 0) continue;
-                if (panel == 0) { pa[1] = s->y[m] + delta; pb[1] = pa[1]; }
-                else { pa[0] = s->x[m] + delta; pb[0] = pa[0]; }
-                double vol = 0.0;
-                if (panel == 0) {
-                    dir = pb[0] - pa[0];
-                    for (int q = 0; q < np; q++) {
-                        double lim1 = pa[0] + dir * T[pairs[q]];
-                        double lim2 = pa[0] + dir * T[pairs[q] + 1];
-                        vol += lim2 - lim1;
-                        int xa = (int)rintp((lim1 - s->g.xmin) * scale);
-                        int xb = (int)rintp((lim2 - s->g.xmin) * scale);
-                        if (xb <= xa) continue;
-                        for (int i = xa + 1; i <= xb; i++)
-                            if (i >= 0 && i < NX) bit_clear(s, i, m, n);
-                        double n1[3], n2[3];
-                        depth_normal(s, 0, m, n, s->pix_off[id] + pairs[q], n1);
-                        depth_normal(s, 0, m, n, s->pix_off[id] + pairs[q] + 1, n2);
-                        push_vrec(s, xa, m, n, 0, lim1, pa[1], pa[2], n1);
-                        push_vrec(s, xb, m, n, 0, lim2, pa[1], pa[2], n2);
-                    }
-                } else if (panel == 1) {
-                    dir = pb[2] - pa[2];
-                    for (int q = 0; q < np; q++) {
-                        double lim1 = pa[2] + dir * T[pairs[q]];
-                        double lim2 = pa[2] + dir * T[pairs[q] + 1];
-                        vol += lim2 - lim1;
-                        int za = (int)rintp((lim1 - s->g.zmin) * scale);
-                        int zb = (int)rintp((lim2 - s->g.zmin) * scale);
-                        if (zb <= za) continue;
-                        for (int k = za + 1; k <= zb; k++)
-                            if (k >= 0 && k < NZ) bit_clear(s, m, n, k);
-                        double n1[3], n2[3];
-                        depth_normal(s, 1, m, n, s->pix_off[id] + pairs[q], n1);
-                        depth_normal(s, 1, m, n, s->pix_off[id] + pairs[q] + 1, n2);
-                        push_vrec(s, m, n, za, 2, pa[0], pa[1], lim1, n1);
-                        push_vrec(s, m, n, zb, 2, pa[0], pa[1], lim2, n2);
-                    }
-                } else {
-                    dir = pb[1] - pa[1];
-                    for (int q = 0; q < np; q++) {
-                        double lim1 = pa[1] + dir * T[pairs[q]];
-                        double lim2 = pa[1] + dir * T[pairs[q] + 1];
-                        vol += lim2 - lim1;
-                        int ya = (int)rintp((lim1 - s->g.ymin) * scale);
-                        int yb = (int)rintp((lim2 - s->g.ymin) * scale);
-                        if (yb <= ya) continue;
-                        for (int j = ya + 1; j <= yb; j++)
-                            if (j >= 0 && j < NY) bit_clear(s, m, j, n);
-                        double n1[3], n2[3];
-                        depth_normal(s, 2, m, n, s->pix_off[id] + pairs[q], n1);
-                        depth_normal(s, 2, m, n, s->pix_off[id] + pairs[q] + 1, n2);
-                        push_vrec(s, m, ya, n, 1, pa[0], lim1, pa[2], n1);
-                        push_vrec(s, m, yb, n, 1, pa[0], lim2, pa[2], n2);
-                    }
-                }
-                volPanel[panel] += vol;
-            }
-        }
+        fill_phase(s, panel, 1, volPanel);
     }
-    free(pairs);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1675,70 +1872,132 @@ static int edge_active(int c, int e) {
     return ((c >> kMcEdgeCorners[e][0]) & 1) != ((c >> kMcEdgeCorners[e][1]) & 1);
 }
 
-int orc_triangulate(orc_surface *s, nsgpu_mesh_info *info) {
-    if (!s->bits || !s->idx[0]) return -1;
-    const int64_t NX = s->NX, NY = s->NY, NZ = s->NZ;
-    s->nverts = s->n_analytic;
-    s->ntris = 0;
-    /* getVertices */
-    for (int64_t k = 1; k < NZ - 1; k++)
-        for (int64_t j = 1; j < NY - 1; j++)
-            for (int64_t i = 1; i < NX - 1; i++) {
-                int c = cube_index(s, i, j, k);
-                if (c == 0 || c == 255) continue;
-                if (edge_active(c, 5) && s->idx[1][FLAT(i + 1, j, k + 1)] == -1) {
-                    double p[3] = {s->x[i + 1] - s->g.hside, s->y[j], s->z[k + 1] - s->g.hside};
-                    s->idx[1][FLAT(i + 1, j, k + 1)] = (int32_t)s->nverts;
-                    push_vert(s, p, NULL);
-                }
-                if (edge_active(c, 6) && s->idx[0][FLAT(i, j + 1, k + 1)] == -1) {
-                    double p[3] = {s->x[i], s->y[j + 1] - s->g.hside, s->z[k + 1] - s->g.hside};
-                    s->idx[0][FLAT(i, j + 1, k + 1)] = (int32_t)s->nverts;
-                    push_vert(s, p, NULL);
-                }
-                if (edge_active(c, 10) && s->idx[2][FLAT(i + 1, j + 1, k)] == -1) {
-                    double p[3] = {s->x[i + 1] - s->g.hside, s->y[j + 1] - s->g.hside, s->z[k]};
-                    s->idx[2][FLAT(i + 1, j + 1, k)] = (int32_t)s->nverts;
-                    push_vert(s, p, NULL);
-                }
-            }
+/* the cubes of one z slice: its mid points (getVertices) or its triangles
+ * with their Heron areas (triangulationKernel), merged in slice order */
+typedef struct {
+    int32_t dir;
+    int64_t flat;
+    double p[3];
+} orc_mid;
+
+typedef struct {
+    orc_mid *mid;
+    int64_t nmid, capmid;
+    int32_t *tri;
+    double *tarea; /* Heron area of each triangle, 0 where the reference adds none */
+    int64_t ntri, captri;
+} orc_slice;
+
+typedef struct {
+    orc_surface *s;
+    orc_slice *sl; /* [NZ] */
+    int pass;      /* 0 mid points, 1 triangles */
+} orc_mc_job;
+
+static void slice_mid(orc_slice *o, int dir, int64_t flat, double x, double y, double z) {
+    if (o->nmid == o->capmid) {
+        o->capmid = o->capmid ? 2 * o->capmid : 256;
+        o->mid = (orc_mid *)realloc(o->mid, o->capmid * sizeof(orc_mid));
+    }
+    orc_mid *m = &o->mid[o->nmid++];
+    m->dir = dir;
+    m->flat = flat;
+    m->p[0] = x;
+    m->p[1] = y;
+    m->p[2] = z;
+}
+
+static void mc_slice(orc_surface *s, int64_t k, int pass, orc_slice *o) {
+    const int64_t NX = s->NX, NY = s->NY;
     /* edge -> (direction, node offset) of getTriangles */
     static const int edir[12] = {0, 1, 0, 1, 0, 1, 0, 1, 2, 2, 2, 2};
     static const int eoff[12][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 1}, {1, 0, 1},
                                     {0, 1, 1}, {0, 0, 1}, {0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}};
-    double area = 0;
-    for (int64_t k = 1; k < NZ - 1; k++)
-        for (int64_t j = 1; j < NY - 1; j++)
-            for (int64_t i = 1; i < NX - 1; i++) {
-                int c = cube_index(s, i, j, k);
-                if (c == 0 || c == 255) continue;
-                int vl[12];
-                for (int e = 0; e < 12; e++)
-                    vl[e] = edge_active(c, e) ? s->idx[edir[e]][FLAT(i + eoff[e][0], j + eoff[e][1], k + eoff[e][2])] : -1;
-                uint64_t w = kMcTriCases[c];
-                for (int q = 0; q < 5; q++) {
-                    int e0 = (int)((w >> (12 * q)) & 15), e1 = (int)((w >> (12 * q + 4)) & 15),
-                        e2 = (int)((w >> (12 * q + 8)) & 15);
-                    if (e0 == 15) break;
-                    if (vl[e0] == -1 || vl[e1] == -1 || vl[e2] == -1) continue;
-                    int tv[3] = {vl[e0], vl[e1], vl[e2]};
-                    const double *v0 = s->verts + 3 * (int64_t)tv[0], *v1 = s->verts + 3 * (int64_t)tv[1],
-                                 *v2 = s->verts + 3 * (int64_t)tv[2];
-                    double a = sqrt((v0[0] - v1[0]) * (v0[0] - v1[0]) + (v0[1] - v1[1]) * (v0[1] - v1[1]) + (v0[2] - v1[2]) * (v0[2] - v1[2]));
-                    double b = sqrt((v0[0] - v2[0]) * (v0[0] - v2[0]) + (v0[1] - v2[1]) * (v0[1] - v2[1]) + (v0[2] - v2[2]) * (v0[2] - v2[2]));
-                    double cc = sqrt((v1[0] - v2[0]) * (v1[0] - v2[0]) + (v1[1] - v2[1]) * (v1[1] - v2[1]) + (v1[2] - v2[2]) * (v1[2] - v2[2]));
-                    double ttt = (a + b + cc) * (b + cc - a) * (cc + a - b) * (a + b - cc);
-                    if (ttt > 0) area += 0.25 * sqrt(ttt);
-                    if (s->ntris == s->cap_tris) {
-                        s->cap_tris = s->cap_tris ? 2 * s->cap_tris : 4096;
-                        s->tris = (int32_t *)realloc(s->tris, s->cap_tris * 3 * sizeof(int32_t));
-                    }
-                    s->tris[3 * s->ntris + 0] = tv[2];
-                    s->tris[3 * s->ntris + 1] = tv[1];
-                    s->tris[3 * s->ntris + 2] = tv[0];
-                    s->ntris++;
-                }
+    for (int64_t j = 1; j < NY - 1; j++)
+        for (int64_t i = 1; i < NX - 1; i++) {
+            int c = cube_index(s, i, j, k);
+            if (c == 0 || c == 255) continue;
+            if (pass == 0) {
+                /* edges 5, 6 and 10 belong to this cube alone: their slots
+                 * hold -1 unless an analytic vertex is there */
+                if (edge_active(c, 5) && s->idx[1][FLAT(i + 1, j, k + 1)] == -1)
+                    slice_mid(o, 1, FLAT(i + 1, j, k + 1), s->x[i + 1] - s->g.hside, s->y[j], s->z[k + 1] - s->g.hside);
+                if (edge_active(c, 6) && s->idx[0][FLAT(i, j + 1, k + 1)] == -1)
+                    slice_mid(o, 0, FLAT(i, j + 1, k + 1), s->x[i], s->y[j + 1] - s->g.hside, s->z[k + 1] - s->g.hside);
+                if (edge_active(c, 10) && s->idx[2][FLAT(i + 1, j + 1, k)] == -1)
+                    slice_mid(o, 2, FLAT(i + 1, j + 1, k), s->x[i + 1] - s->g.hside, s->y[j + 1] - s->g.hside, s->z[k]);
+                continue;
             }
+            int vl[12];
+            for (int e = 0; e < 12; e++)
+                vl[e] = edge_active(c, e) ? s->idx[edir[e]][FLAT(i + eoff[e][0], j + eoff[e][1], k + eoff[e][2])] : -1;
+            uint64_t w = kMcTriCases[c];
+            for (int q = 0; q < 5; q++) {
+                int e0 = (int)((w >> (12 * q)) & 15), e1 = (int)((w >> (12 * q + 4)) & 15),
+                    e2 = (int)((w >> (12 * q + 8)) & 15);
+                if (e0 == 15) break;
+                if (vl[e0] == -1 || vl[e1] == -1 || vl[e2] == -1) continue;
+                int tv[3] = {vl[e0], vl[e1], vl[e2]};
+                const double *v0 = s->verts + 3 * (int64_t)tv[0], *v1 = s->verts + 3 * (int64_t)tv[1],
+                             *v2 = s->verts + 3 * (int64_t)tv[2];
+                double a = sqrt((v0[0] - v1[0]) * (v0[0] - v1[0]) + (v0[1] - v1[1]) * (v0[1] - v1[1]) + (v0[2] - v1[2]) * (v0[2] - v1[2]));
+                double b = sqrt((v0[0] - v2[0]) * (v0[0] - v2[0]) + (v0[1] - v2[1]) * (v0[1] - v2[1]) + (v0[2] - v2[2]) * (v0[2] - v2[2]));
+                double cc = sqrt((v1[0] - v2[0]) * (v1[0] - v2[0]) + (v1[1] - v2[1]) * (v1[1] - v2[1]) + (v1[2] - v2[2]) * (v1[2] - v2[2]));
+                double ttt = (a + b + cc) * (b + cc - a) * (cc + a - b) * (a + b - cc);
+                if (o->ntri == o->captri) {
+                    o->captri = o->captri ? 2 * o->captri : 256;
+                    o->tri = (int32_t *)realloc(o->tri, o->captri * 3 * sizeof(int32_t));
+                    o->tarea = (double *)realloc(o->tarea, o->captri * sizeof(double));
+                }
+                o->tarea[o->ntri] = ttt > 0 ? 0.25 * sqrt(ttt) : 0.0;
+                o->tri[3 * o->ntri + 0] = tv[2];
+                o->tri[3 * o->ntri + 1] = tv[1];
+                o->tri[3 * o->ntri + 2] = tv[0];
+                o->ntri++;
+            }
+        }
+}
+
+static void mc_worker(void *arg, int tid, int nthreads) {
+    orc_mc_job *j = (orc_mc_job *)arg;
+    for (int64_t k = 1 + tid; k < j->s->NZ - 1; k += nthreads) mc_slice(j->s, k, j->pass, &j->sl[k]);
+}
+
+int orc_triangulate(orc_surface *s, nsgpu_mesh_info *info) {
+    if (!s->bits || !s->idx[0]) return -1;
+    const int64_t NZ = s->NZ;
+    s->nverts = s->n_analytic;
+    s->ntris = 0;
+    orc_mc_job job = {s, (orc_slice *)xcalloc(NZ, sizeof(orc_slice)), 0};
+    /* getVertices: the mid points of each slice, numbered in cube order */
+    run_threads(mc_worker, &job);
+    for (int64_t k = 1; k < NZ - 1; k++) {
+        orc_slice *o = &job.sl[k];
+        for (int64_t q = 0; q < o->nmid; q++) {
+            s->idx[o->mid[q].dir][o->mid[q].flat] = (int32_t)s->nverts;
+            push_vert(s, o->mid[q].p, NULL);
+        }
+        free(o->mid);
+    }
+    /* triangulationKernel: triangles in cube order, the area summed in that order */
+    job.pass = 1;
+    run_threads(mc_worker, &job);
+    double area = 0;
+    for (int64_t k = 1; k < NZ - 1; k++) {
+        orc_slice *o = &job.sl[k];
+        for (int64_t t = 0; t < o->ntri; t++) {
+            if (o->tarea[t] != 0.0) area += o->tarea[t];
+            if (s->ntris == s->cap_tris) {
+                s->cap_tris = s->cap_tris ? 2 * s->cap_tris : 4096;
+                s->tris = (int32_t *)realloc(s->tris, s->cap_tris * 3 * sizeof(int32_t));
+            }
+            memcpy(s->tris + 3 * s->ntris, o->tri + 3 * t, 3 * sizeof(int32_t));
+            s->ntris++;
+        }
+        free(o->tri);
+        free(o->tarea);
+    }
+    free(job.sl);
     s->area = area;
     /* approximateNormals(appNormals, true) (Surface.cpp:10588-10737): every
      * vertex with a zero normal (the mid points) gets the normalised sum of

@@ -465,3 +465,35 @@ def test_oracle_hits_match_exact_ses(seed, n):
             checked += 1
     assert r["odd_rays"] == 0
     assert checked > 0
+
+
+# --------------------------------------------------------------- threaded oracle
+@pytest.mark.parametrize("kind", ["ses", "skin"])
+def test_threaded_oracle_equals_single_thread(kind):
+    """The CPU baseline runs the oracle on all host cores with the reference's
+    decomposition; every output equals the single-thread run's."""
+    xyzr, _ = ns.generate_chain(300, 9.0, seed=4)
+    g = ns.build_grid(xyzr, 3.0, 80.0)
+    jit, _ = ns.jitter(xyzr)
+    ps = ns.ses_patches(jit, 1.4) if kind == "ses" else ns.skin_patches(jit, 0.45)
+    out = []
+    for threads in (1, 7):
+        binding.set_threads(threads)
+        try:
+            o = binding.OracleSurface(g, ps)
+            r = o.get_surf()
+            m = o.triangulate()
+            v, t = o.mesh()
+            hits = [o.ray_hits(s_) for s_ in range(6)]
+            out.append((r, m, o.epsmap(), o.idebmap(), o.status(), o.corner_bits(), v, t, o.normals(), hits,
+                        o.cavities()))
+        finally:
+            binding.set_threads(1)
+    a, b = out
+    assert a[0] == b[0] and a[1] == b[1]
+    for x, y in zip(a[2:8], b[2:8]):
+        assert np.array_equal(x, y)
+    assert np.abs(a[8] - b[8]).max() < 1e-9  # equal depths of two patches may swap their normals
+    for (ca, ta), (cb, tb) in zip(a[9], b[9]):
+        assert np.array_equal(ca, cb) and np.array_equal(ta, tb)
+    assert all(np.array_equal(p, q) for p, q in zip(a[10], b[10]))
