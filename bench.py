@@ -142,6 +142,36 @@ def bench_mixed(dev, steps, warmup):
     return (time.perf_counter() - t0) * 1e3 / steps
 
 
+def bench_cast_tiles(dev, steps, warmup):
+    """The C2 step with the ray-tile cast over the cell grid instead of the
+    patch-centric cast: step time, and the intersection stage time per mode."""
+    out = {}
+    for name, mode in (("patches", dev.CAST_PATCHES), ("tiles", dev.CAST_TILES)):
+        dev.set_cast_mode(mode)
+        dev.get_surf()  # builds the cell grid (tiles) outside the timed region
+        dev.triangulate()
+        for _ in range(warmup):
+            dev.get_surf()
+            dev.triangulate()
+        dev.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            dev.get_surf()
+            dev.triangulate()
+        dev.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        dev.set_profiling(True)
+        inter = 0.0
+        for _ in range(steps):
+            dev.get_surf()
+            dev.triangulate()
+            inter += dev.stage_times()["intersect"] / steps
+        dev.set_profiling(False)
+        out[name] = {"ms_per_step": ms, "intersect_ms": inter}
+    dev.set_cast_mode(dev.CAST_PATCHES)
+    return out
+
+
 def bench_batched(n_frames=256, repeats=3):
     """Uneven frames, one captured graph launch vs the per-frame loop."""
     t0 = time.perf_counter()
@@ -464,6 +494,8 @@ def main():
                                      "speedup_vs_fp64": ms_per_step / mixed_ms,
                                      "hit_error": "<= 1e-6 A against the true roots of the same quartics "
                                                   "(tests/test_gpu_parity.py::test_mixed_precision_matches_oracle)"}
+        print("cell grid ray tiles ...", file=sys.stderr, flush=True)
+        extras["cast_modes"] = bench_cast_tiles(dev, args.steps, args.warmup)
     dev.close()
     if not args.no_extras:
         print("batched frames ...", file=sys.stderr, flush=True)

@@ -257,6 +257,34 @@ int nsgpu_get_surf_noalloc(nsgpu_ctx *ctx, const nsgpu_surf_opts *opts, nsgpu_su
  * emit triangles (reversed winding, as the reference), sum the area. */
 int nsgpu_triangulate(nsgpu_ctx *ctx, nsgpu_mesh_info *info);
 
+/* The ray/patch intersection of get_surf and surface_gradient.
+ * NSGPU_CAST_PATCHES (default): one workgroup per (patch, panel slot) walks
+ * the pixel rectangle of the patch, as getPatchIntersectionData does
+ * (ConnollySurface.cpp:4319-4591).  NSGPU_CAST_TILES: a uniform grid of 8x8
+ * pixel cells lists the patches meeting each cell (the ray-centric auxiliary
+ * grid of ConnollySurface::preProcessPanel, ConnollySurface.cpp:3318-3560, on
+ * the grid's pixels, built on the device at the first call after set_grid or
+ * set_patches); one wave per cell and slot casts the cell's 64 rays against
+ * its list.  Both give the same depths, maps and meshes. */
+#define NSGPU_CAST_PATCHES 0
+#define NSGPU_CAST_TILES 1
+int nsgpu_set_cast_mode(nsgpu_ctx *ctx, int mode);
+
+/* NS::NanoShaper::castAxisOrientedRay (nanoshaper.cpp:220-297) for a batch of
+ * rays, on the device: ray r starts at origins[3r..3r+2] and runs along
+ * direction (0 = x, 1 = y, 2 = z) to the coordinate coords[r] on that axis.
+ * Its intersections with the patches of its cell of the auxiliary grid
+ * (ConnollySurface::getRayIntersection, ConnollySurface.cpp:5494-5608, the
+ * roots of the whole line as the reference keeps them) are returned as
+ * coordinates along the axis in ascending order, hits[r*max_hits ...], with a
+ * depth within EPS_INT (1e-8) of the last kept one dropped; normals (may be
+ * NULL) receives their analytic normals [r*max_hits*3 ...].  counts[r]: the
+ * hits kept, more than max_hits when the list was cut (call again with a
+ * larger max_hits), -1 when coords[r] <= the origin's coordinate, -2 when the
+ * ray lies outside the grid.  Host arrays; returns when they are written. */
+int nsgpu_cast_rays(nsgpu_ctx *ctx, int32_t direction, int32_t n, const double *origins, const double *coords,
+                    int32_t max_hits, int32_t *counts, double *hits, double *normals);
+
 /* Wait for all work queued on the context's stream. */
 int nsgpu_synchronize(nsgpu_ctx *ctx);
 

@@ -73,6 +73,8 @@ SIGNATURES = {
     "nsgpu_get_surf_noalloc": (ctypes.c_int, [VP, ctypes.POINTER(SurfOpts), ctypes.POINTER(SurfResult), VP, VP, VP]),
     "nsgpu_triangulate": (ctypes.c_int, [VP, ctypes.POINTER(MeshInfo)]),
     "nsgpu_synchronize": (ctypes.c_int, [VP]),
+    "nsgpu_set_cast_mode": (ctypes.c_int, [VP, ctypes.c_int]),
+    "nsgpu_cast_rays": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_int32, VP, VP, ctypes.c_int32, VP, VP, VP]),
     "nsgpu_set_profiling": (ctypes.c_int, [VP, ctypes.c_int]),
     "nsgpu_stage_times": (ctypes.c_int, [VP, VP, ctypes.c_int]),
     "nsgpu_intersect_path_counts": (ctypes.c_int, [VP, ctypes.POINTER(SurfOpts), VP, ctypes.c_int]),

@@ -169,6 +169,12 @@ struct nsgpu_ctx {
     bool have_surf = false, have_vertices = false, have_mesh = false;
     nsgpu_surf_opts opts{};
     nsgpu_surf_result res{};
+    // ray/patch intersection: patch-centric workgroups or ray tiles over a
+    // cell grid of the patches (built for the current grid and patch set)
+    int cast_mode = NSGPU_CAST_PATCHES;
+    DBuf cgcount, cgoff, cgcursor, cglist, crorig, crcoord, crcount, crhits, crnorm;
+    int cg_nc = 0;
+    bool cg_valid = false;
     // bumped by every call that replaces the context's inputs (grid,
     // patches, mesh, atoms); with the buffers' serials it tells a batch
     // whether its captured graph still matches this context
@@ -277,7 +283,8 @@ static void for_each_buf(nsgpu_ctx *c, F fn) {
                    &c->idx, &c->vslot, &c->vhit, &c->vorig, &c->normals, &c->vcount, &c->voffs, &c->scan_scratch, &c->verts, &c->tris, &c->tarea,
                    &c->segmask, &c->anode, &c->acase, &c->anv, &c->ant, &c->avoff, &c->atoff, &c->mverts, &c->mtris,
                    &c->mrr, &c->mraw, &c->field, &c->fbits, &c->batoms, &c->bbox, &c->bcount, &c->boff, &c->bcur,
-                   &c->blist, &c->flag};
+                   &c->blist, &c->flag, &c->cgcount, &c->cgoff, &c->cgcursor, &c->cglist, &c->crorig,
+                   &c->crcoord, &c->crcount, &c->crhits, &c->crnorm};
     for (DBuf *b : all) fn(b);
 }
 
@@ -334,6 +341,7 @@ int nsgpu_set_grid(nsgpu_ctx *c, const nsgpu_grid *g) {
     if ((int64_t)g->nx * g->ny * g->nz >= (int64_t)1 << 31)
         return fail(c, NSGPU_E_ARG, "grid exceeds 2^31 nodes");
     ++c->setgen;
+    c->cg_valid = false;
     HIP_TRY(c, hipSetDevice(c->device));
     c->grid = *g;
     std::vector<double> x(g->nx), y(g->ny), z(g->nz);
@@ -360,6 +368,7 @@ int nsgpu_set_patches(nsgpu_ctx *c, const nsgpu_patches *p, double probe_radius)
     if (p->n > 0 && (!p->kind || !p->off || !p->cnt || !p->flags || !p->bsph || (!p->data && p->n_data)))
         return NSGPU_E_ARG;
     ++c->setgen;
+    c->cg_valid = false;
     for (int i = 0; i < p->n; ++i) {
         const int k = p->kind[i];
         if (k < NSGPU_PATCH_POINT || k > NSGPU_PATCH_SKIN_TETRA) return fail(c, NSGPU_E_ARG, "bad patch kind");
@@ -518,10 +527,7 @@ static MeshArgs mesh_args(const nsgpu_ctx *c) {
     return a;
 }
 
-static int enqueue_trace(nsgpu_ctx *c, hipStream_t st) {
-    const DevGrid g = dev_grid(c);
-    const int64_t N = g.N;
-    const int64_t nrays = N * N * c->nslots;
+static DevPatches dev_patches(const nsgpu_ctx *c) {
     DevPatches P;
     P.n = c->npatch;
     P.kind = c->pkind.as<int>();
@@ -534,6 +540,52 @@ static int enqueue_trace(nsgpu_ctx *c, hipStream_t st) {
     P.n_plain = c->npatch_plain;
     P.n_skin = c->npatch_skin;
     P.atom = c->patom.as<int>();
+    return P;
+}
+
+static CellGrid cell_grid(const nsgpu_ctx *c) {
+    return CellGrid{c->cg_nc, c->cgcount.as<int>(), c->cgoff.as<int>(), c->cgcursor.as<int>(), c->cglist.as<int>()};
+}
+
+static int read_total(nsgpu_ctx *c, const int *counts, const int *offs, int64_t n, int64_t *total);
+
+// The cell grid of the current grid and patch set: count, scan, one read of
+// the total (the lists' size), fill.  Built eagerly, never inside a capture.
+static int ensure_cells(nsgpu_ctx *c) {
+    if (c->cg_valid) return NSGPU_OK;
+    if (c->capturing) return fail(c, NSGPU_E_STATE, "the cell grid is built before a capture");
+    const DevGrid g = dev_grid(c);
+    const DevPatches P = dev_patches(c);
+    const int nc = cell_grid_side(dev_rays(c));
+    const int64_t cells = 3 * (int64_t)nc * nc;
+    HIP_TRY(c, c->cgcount.ensure((size_t)cells * 4));
+    HIP_TRY(c, c->cgoff.ensure((size_t)cells * 4));
+    HIP_TRY(c, c->cgcursor.ensure((size_t)cells * 4));
+    HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(cells) * 4));
+    c->cg_nc = nc;
+    CellGrid C = cell_grid(c);
+    launch_cells_count(c->stream, g, P, C);
+    launch_scan(c->stream, C.count, cells, C.off, c->scan_scratch.as<int>());
+    HIP_TRY(c, hipGetLastError());
+    int64_t total = 0;
+    int rc = read_total(c, C.count, C.off, cells, &total);
+    if (rc) return rc;
+    if (total > INT32_MAX) return fail(c, NSGPU_E_NOMEM, "cell grid lists exceed 2^31 entries");
+    HIP_TRY(c, c->cglist.ensure((size_t)std::max<int64_t>(total, 1) * 4));
+    C = cell_grid(c);
+    launch_cells_fill(c->stream, g, P, C);
+    HIP_TRY(c, hipGetLastError());
+    c->cg_valid = true;
+    return NSGPU_OK;
+}
+
+static int enqueue_trace(nsgpu_ctx *c, hipStream_t st) {
+    const DevGrid g = dev_grid(c);
+    const int64_t N = g.N;
+    const int64_t nrays = N * N * c->nslots;
+    const DevPatches P = dev_patches(c);
+    const bool tiles = c->cast_mode == NSGPU_CAST_TILES && !c->surface_is_mesh;
+    if (tiles && !c->cg_valid) return fail(c, NSGPU_E_STATE, "the cell grid is not built");
     HIP_TRY(c, c->rcnt.ensure(nrays * 4));
     HIP_TRY(c, c->rvol.ensure(nrays * 8));
     const int64_t W = (N + 63) / 64;
@@ -543,7 +595,10 @@ static int enqueue_trace(nsgpu_ctx *c, hipStream_t st) {
     if (c->record_pid) HIP_TRY(c, c->rpid.ensure((size_t)nrays * c->K * 4));
     const DevRays R = dev_rays(c);
     // per-workgroup partials: candidates of the intersection, (hits, odd, longest) of the sort
-    const int64_t irows = std::max<int64_t>(c->surface_is_mesh ? mesh_workgroups(c->mnt, R) : intersect_workgroups(P, R), 1),
+    const int64_t irows = std::max<int64_t>(c->surface_is_mesh ? mesh_workgroups(c->mnt, R)
+                                            : tiles             ? tile_workgroups(R, c->cg_nc)
+                                                                : intersect_workgroups(P, R),
+                                            1),
                   srows = sort_blocks(R);
     HIP_TRY(c, c->cand.ensure((size_t)irows * 8));
     HIP_TRY(c, c->stats.ensure((size_t)srows * 3 * 8));
@@ -557,13 +612,15 @@ static int enqueue_trace(nsgpu_ctx *c, hipStream_t st) {
     fl.k = 3;
     launch_fill_multi(st, fl);
     mark(c, ST_INTERSECT, 0, st);
+    const int roots = c->opts.precision == NSGPU_PREC_MIXED   ? kRootsMixed
+                      : c->opts.precision == NSGPU_PREC_STURM ? kRootsSturm
+                                                              : kRootsClosed;
     if (c->surface_is_mesh)
         launch_intersect_mesh(st, g, R, mesh_args(c), c->cand.as<unsigned long long>());
+    else if (tiles)
+        launch_cast_tiles(st, g, P, R, cell_grid(c), c->probe, c->cand.as<unsigned long long>(), roots);
     else
-        launch_intersect(st, g, P, R, c->probe, c->cand.as<unsigned long long>(),
-                         c->opts.precision == NSGPU_PREC_MIXED   ? kRootsMixed
-                         : c->opts.precision == NSGPU_PREC_STURM ? kRootsSturm
-                                                                 : kRootsClosed);
+        launch_intersect(st, g, P, R, c->probe, c->cand.as<unsigned long long>(), roots);
     HIP_TRY(c, hipGetLastError());
     mark(c, ST_INTERSECT, 1, st);
     mark(c, ST_SORT, 0, st);
@@ -600,6 +657,10 @@ static bool regrow(nsgpu_ctx *c) {
 
 // eager trace (adjoint path): enqueue, wait, grow the ray lists until they hold every hit
 static int trace(nsgpu_ctx *c) {
+    if (c->cast_mode == NSGPU_CAST_TILES && !c->surface_is_mesh) {
+        const int rc = ensure_cells(c);
+        if (rc) return rc;
+    }
     for (int attempt = 0; attempt < 4; ++attempt) {
         int rc = enqueue_trace(c, c->stream);
         if (rc) return rc;
@@ -1049,6 +1110,10 @@ int nsgpu_get_surf(nsgpu_ctx *c, const nsgpu_surf_opts *o, nsgpu_surf_result *re
     int rc = begin_surf(c, o);
     if (rc) return rc;
     HIP_TRY(c, hipSetDevice(c->device));
+    if (c->cast_mode == NSGPU_CAST_TILES && !c->surface_is_mesh) {
+        rc = ensure_cells(c);
+        if (rc) return rc;
+    }
     // the edge rays' depths keep their patch ids for the vertex normals
     c->record_pid = o->accurate_triangulation != 0 && c->want_normals;
     const FillOpts fo = fill_opts(c);
@@ -1104,6 +1169,44 @@ int nsgpu_get_surf(nsgpu_ctx *c, const nsgpu_surf_opts *o, nsgpu_surf_result *re
     return NSGPU_OK;
 }
 
+int nsgpu_set_cast_mode(nsgpu_ctx *c, int mode) {
+    if (!c || (mode != NSGPU_CAST_PATCHES && mode != NSGPU_CAST_TILES)) return NSGPU_E_ARG;
+    if (c->cast_mode != mode) ++c->setgen;
+    c->cast_mode = mode;
+    return NSGPU_OK;
+}
+
+int nsgpu_cast_rays(nsgpu_ctx *c, int32_t direction, int32_t n, const double *origins, const double *coords,
+                    int32_t max_hits, int32_t *counts, double *hits, double *normals) {
+    if (!c || n < 0 || direction < 0 || direction > 2 || max_hits < 1) return NSGPU_E_ARG;
+    if (n > 0 && (!origins || !coords || !counts || !hits)) return NSGPU_E_ARG;
+    if (!c->have_grid || !c->have_patches) return fail(c, NSGPU_E_STATE, "set_grid and set_patches first");
+    if (c->surface_is_mesh) return fail(c, NSGPU_E_STATE, "single rays need an SES or skin patch set");
+    if (n == 0) return NSGPU_OK;
+    HIP_TRY(c, hipSetDevice(c->device));
+    int rc = ensure_cells(c);
+    if (rc) return rc;
+    // direction 0 = x, 1 = y, 2 = z; panels 0 (x), 1 (z), 2 (y) (NanoShaper::panelResolver)
+    const int panel = direction == 0 ? 0 : (direction == 2 ? 1 : 2);
+    const size_t slots = (size_t)n * max_hits;
+    HIP_TRY(c, c->crorig.ensure((size_t)n * 24));
+    HIP_TRY(c, c->crcoord.ensure((size_t)n * 8));
+    HIP_TRY(c, c->crcount.ensure((size_t)n * 4));
+    HIP_TRY(c, c->crhits.ensure(slots * 8));
+    if (normals) HIP_TRY(c, c->crnorm.ensure(slots * 24));
+    HIP_TRY(c, hipMemcpyAsync(c->crorig.p, origins, (size_t)n * 24, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->crcoord.p, coords, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+    launch_cast_rays(c->stream, dev_grid(c), dev_patches(c), cell_grid(c), c->probe, panel, n, c->crorig.as<double>(),
+                     c->crcoord.as<double>(), max_hits, c->crcount.as<int>(), c->crhits.as<double>(),
+                     normals ? c->crnorm.as<double>() : nullptr);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipMemcpyAsync(counts, c->crcount.p, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(hits, c->crhits.p, slots * 8, hipMemcpyDeviceToHost, c->stream));
+    if (normals) HIP_TRY(c, hipMemcpyAsync(normals, c->crnorm.p, slots * 24, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return NSGPU_OK;
+}
+
 }  // extern "C"
 
 int nsg_ctx_device(const nsgpu_ctx *c) { return c->device; }
@@ -1142,6 +1245,9 @@ int nsg_get_surf_in(nsgpu_ctx *c, const nsgpu_surf_opts *o, nsgpu_surf_result *r
     });
     Arena a{static_cast<char *>(scratch), bytes, 0, 0};
     c->record_pid = false;
+    // the patch-centric cast: the cell grid's lists would need allocating
+    const int cast_mode = c->cast_mode;
+    c->cast_mode = NSGPU_CAST_PATCHES;
     for (int attempt = 0;; ++attempt) {
         if (attempt == 4) {
             rc = fail(c, NSGPU_E_STATE, "ray hit capacity did not converge");
@@ -1201,6 +1307,7 @@ int nsg_get_surf_in(nsgpu_ctx *c, const nsgpu_surf_opts *o, nsgpu_surf_result *r
         }
     }
     for (int q = 0; q < ns; ++q) *which[q] = saved[q];
+    c->cast_mode = cast_mode;
     // the context's buffers do not hold this call's results
     c->have_trace = c->have_surf = c->have_vertices = c->have_mesh = false;
     if (!rc && res) *res = c->res;
@@ -1435,6 +1542,7 @@ int nsgpu_set_mesh(nsgpu_ctx *c, const double *verts, int32_t nv, const int32_t 
     if (!c || nv < 0 || nt < 0 || (nv && !verts) || (nt && !tris) || aux_grid_dim < 1) return NSGPU_E_ARG;
     if (!c->have_grid) return fail(c, NSGPU_E_STATE, "set_grid first");
     ++c->setgen;
+    c->cg_valid = false;
     for (int64_t q = 0; q < 3 * (int64_t)nt; ++q)
         if (tris[q] < 0 || tris[q] >= nv) return fail(c, NSGPU_E_ARG, "triangle vertex index out of range");
     for (int64_t q = 0; q < 3 * (int64_t)nv; ++q)
@@ -1457,6 +1565,7 @@ int nsgpu_blobby_build(nsgpu_ctx *c, const double *xyzr, int32_t n, double B, do
     if (!c->have_grid) return fail(c, NSGPU_E_STATE, "set_grid first");
     if (!(B < 0)) B = -2.5;  // BlobbySurface::setBlobbyness (BlobbySurface.cpp:46-54)
     ++c->setgen;
+    c->cg_valid = false;
     HIP_TRY(c, hipSetDevice(c->device));
     c->pre_mc = false;  // the classification buffers are reused below
     const DevGrid g = dev_grid(c);
@@ -1702,6 +1811,7 @@ int nsgpu_read_ray_hits(nsgpu_ctx *c, int32_t slot, int32_t max_hits, int32_t *c
 int nsgpu_set_atoms(nsgpu_ctx *c, const double *xyzr, int32_t n) {
     if (!c || n < 0 || (n > 0 && !xyzr)) return NSGPU_E_ARG;
     ++c->setgen;
+    c->cg_valid = false;
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, c->atoms.ensure(std::max<size_t>((size_t)n, 1) * 32));
     if (n) HIP_TRY(c, hipMemcpyAsync(c->atoms.p, xyzr, (size_t)n * 32, hipMemcpyHostToDevice, c->stream));

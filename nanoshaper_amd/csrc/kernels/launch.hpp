@@ -18,6 +18,27 @@ struct FillOpts {
     uint32_t *init_bits;
 };
 
+// Uniform cell grid over each panel's plane: kCell x kCell pixel cells, per
+// (panel, cell) a count, an offset into list and the patch ids in ascending
+// order; cells of panel p at [p * nc * nc, (p + 1) * nc * nc).
+constexpr int kCell = 8;
+struct CellGrid {
+    int nc;  // cells per side
+    int *count, *off, *cursor, *list;
+};
+int cell_grid_side(const DevRays &);
+// count the (panel, cell) entries; scan count into off (host); then fill the lists
+void launch_cells_count(hipStream_t, const DevGrid &, const DevPatches &, const CellGrid &);
+void launch_cells_fill(hipStream_t, const DevGrid &, const DevPatches &, const CellGrid &);
+// ray tiles: one wave per (cell, slot); candidate partials part[slot * nc * nc + cell]
+int64_t tile_workgroups(const DevRays &, int nc);
+void launch_cast_tiles(hipStream_t, const DevGrid &, const DevPatches &, const DevRays &, const CellGrid &, double probe,
+                       unsigned long long *part, int roots);
+// arbitrary axis-aligned rays of one panel (device arrays)
+void launch_cast_rays(hipStream_t, const DevGrid &, const DevPatches &, const CellGrid &, double probe, int panel,
+                      int n, const double *orig, const double *coord, int maxh, int *count, double *hits,
+                      double *normals);
+
 constexpr int kPathCounters = 8;
 int64_t intersect_workgroups(const DevPatches &, const DevRays &);
 int64_t sort_blocks(const DevRays &);

@@ -12,43 +12,6 @@
 
 namespace nsg {
 
-__device__ void analytic_normal(const DevPatches &P, int pid, double probe, const double y[3], double n[3]) {
-    const int kind = P.kind[pid];
-    const double *c = P.bsph + 4 * pid;
-    if (kind >= NSGPU_PATCH_SKIN_POINT) {
-        // skin cell: 2 grad Q / 2 = Q (x, 1), doubled and negated in the
-        // concave tetra cells, not normalised (SkinSurface.cpp:3680-3696)
-        const double *Q = P.data + P.off[pid] + 6;
-        const double sg = kind == NSGPU_PATCH_SKIN_TETRA ? -2.0 : 2.0;
-        n[0] = sg * (Q[0] * y[0] + Q[4] * y[1] + Q[7] * y[2] + Q[9]);
-        n[1] = sg * (Q[4] * y[0] + Q[1] * y[1] + Q[5] * y[2] + Q[8]);
-        n[2] = sg * (Q[7] * y[0] + Q[5] * y[1] + Q[2] * y[2] + Q[6]);
-        return;
-    }
-    if (kind == NSGPU_PATCH_FACET_REG || kind == NSGPU_PATCH_FACET_SING) {
-        for (int q = 0; q < 3; ++q) n[q] = (c[q] - y[q]) / probe;
-        return;
-    }
-    if (kind == NSGPU_PATCH_POINT) {
-        for (int q = 0; q < 3; ++q) n[q] = (y[q] - c[q]) / c[3];
-        return;
-    }
-    // torus: into the torus frame, direction from the point to the circle,
-    // back to the world frame (the frame is orthonormal: Rot = invrot^T)
-    const double *td = P.data + P.off[pid];
-    const double *iv = td + 3;
-    const double t[3] = {y[0] - td[0], y[1] - td[1], y[2] - td[2]};
-    double C[3];
-    for (int i = 0; i < 3; ++i) C[i] = iv[3 * i + 0] * t[0] + iv[3 * i + 1] * t[1] + iv[3 * i + 2] * t[2];
-    double d = C[0] * C[0] + C[1] * C[1];
-    const double rm1 = td[12] / sqrt(d) - 1.;
-    d = sqrt(d * rm1 * rm1 + C[2] * C[2]);
-    C[0] *= rm1 / d;
-    C[1] *= rm1 / d;
-    C[2] /= -d;
-    for (int i = 0; i < 3; ++i) n[i] = iv[0 + i] * C[0] + iv[3 + i] * C[1] + iv[6 + i] * C[2];
-}
-
 __global__ void __launch_bounds__(256) k_normals_analytic(DevGrid g, DevPatches P, DevRays R, NormalArgs a) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= a.n_analytic) return;

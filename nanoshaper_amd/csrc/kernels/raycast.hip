@@ -1097,6 +1097,331 @@ __global__ void __launch_bounds__(1024) k_reduce_cols(const unsigned long long *
     }
 }
 
+// ---------------------------------------------------------------- cell grid
+// A uniform grid of kCell x kCell pixel cells over each panel's plane; a cell
+// lists the patches whose pixel rectangle (the rectangle the patch-centric
+// cast walks: the bounding box rounded to pixels, ConnollySurface.cpp:
+// 4378-4410) meets it.  It is the auxiliary 2D grid of the ray-centric path
+// (ConnollySurface::preProcessPanel, ConnollySurface.cpp:3318-3560, which bins
+// the same bounding boxes) laid on the grid's own pixels.  Built on the device:
+// count, scan, scatter, then each cell's list in ascending patch id.
+
+// the pixel rectangle [is0, ie0] x [is1, ie1] of patch it on panel PANEL;
+// false when the patch is culled (CELL_CULLING) or the rectangle is empty
+template <int PANEL>
+__device__ __forceinline__ bool patch_rect(const DevGrid &g, const DevPatches &P, int it, int &is0, int &ie0, int &is1,
+                                           int &ie1) {
+    using A = PanelAxes<PANEL>;
+    double down[3], up[3];
+    if (P.kind[it] >= NSGPU_PATCH_SKIN_POINT) {
+        const double *pd = P.data + P.off[it];
+        for (int q = 0; q < 3; ++q) {
+            down[q] = pd[q];
+            up[q] = pd[3 + q];
+        }
+    } else {
+        const double *c = P.bsph + 4 * it;
+        for (int q = 0; q < 3; ++q) {
+            down[q] = c[q] - c[3];
+            up[q] = c[q] + c[3];
+        }
+    }
+    if (down[0] > g.xmax + kEpsInt || down[1] > g.ymax + kEpsInt || down[2] > g.zmax + kEpsInt) return false;
+    const double lowpad = g.hside - kDelta + kEpsInt;
+    if (up[0] < g.xmin - lowpad || up[1] < g.ymin - lowpad || up[2] < g.zmin - lowpad) return false;
+    const double gmin[3] = {g.xmin, g.ymin, g.zmin};
+    const int gn[3] = {g.nx, g.ny, g.nz};
+    int64_t a0 = (int64_t)rintp(fmax(0., down[A::fd] - gmin[A::fd]) * g.scale);
+    int64_t b0 = (int64_t)rintp(fmax(0., up[A::fd] - gmin[A::fd]) * g.scale);
+    int64_t a1 = (int64_t)rintp(fmax(0., down[A::ld] - gmin[A::ld]) * g.scale);
+    int64_t b1 = (int64_t)rintp(fmax(0., up[A::ld] - gmin[A::ld]) * g.scale);
+    if (b0 >= gn[A::fd]) b0 = gn[A::fd] - 1;
+    if (b1 >= gn[A::ld]) b1 = gn[A::ld] - 1;
+    if (b0 < a0 || b1 < a1) return false;
+    is0 = (int)a0;
+    ie0 = (int)b0;
+    is1 = (int)a1;
+    ie1 = (int)b1;
+    return true;
+}
+
+// every (panel, cell) the patch's rectangles meet: count (fill = false) or list
+__global__ void __launch_bounds__(256) k_cells_bin(DevGrid g, DevPatches P, CellGrid C, bool fill) {
+    const int it = blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= P.n) return;
+    const int nc = C.nc, nn = nc * nc;
+    for (int panel = 0; panel < 3; ++panel) {
+        int is0, ie0, is1, ie1;
+        bool ok;
+        if (panel == 0) ok = patch_rect<0>(g, P, it, is0, ie0, is1, ie1);
+        else if (panel == 1) ok = patch_rect<1>(g, P, it, is0, ie0, is1, ie1);
+        else ok = patch_rect<2>(g, P, it, is0, ie0, is1, ie1);
+        if (!ok) continue;
+        for (int cv = is1 / kCell; cv <= ie1 / kCell; ++cv)
+            for (int cu = is0 / kCell; cu <= ie0 / kCell; ++cu) {
+                const int c = panel * nn + cv * nc + cu;
+                if (!fill) {
+                    atomicAdd(C.count + c, 1);
+                } else {
+                    const int at = atomicAdd(C.cursor + c, 1);
+                    C.list[C.off[c] + at] = it;
+                }
+            }
+    }
+}
+
+// each cell's list in ascending patch id (insertion sort: lists are short)
+__global__ void __launch_bounds__(256) k_cells_sort(CellGrid C) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= 3 * C.nc * C.nc) return;
+    int *l = C.list + C.off[c];
+    const int n = C.count[c];
+    for (int i = 1; i < n; ++i) {
+        const int v = l[i];
+        int j = i - 1;
+        while (j >= 0 && l[j] > v) {
+            l[j + 1] = l[j];
+            --j;
+        }
+        l[j + 1] = v;
+    }
+}
+
+int cell_grid_side(const DevRays &R) { return (R.N + kCell - 1) / kCell; }
+
+void launch_cells_count(hipStream_t s, const DevGrid &g, const DevPatches &P, const CellGrid &C) {
+    launch_fill32(s, C.count, 0u, 3 * (int64_t)C.nc * C.nc);
+    if (P.n > 0) hipLaunchKernelGGL(k_cells_bin, dim3((P.n + 255) / 256), dim3(256), 0, s, g, P, C, false);
+}
+
+void launch_cells_fill(hipStream_t s, const DevGrid &g, const DevPatches &P, const CellGrid &C) {
+    const int64_t cells = 3 * (int64_t)C.nc * C.nc;
+    launch_fill32(s, C.cursor, 0u, cells);
+    if (P.n > 0) hipLaunchKernelGGL(k_cells_bin, dim3((P.n + 255) / 256), dim3(256), 0, s, g, P, C, true);
+    hipLaunchKernelGGL(k_cells_sort, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, C);
+}
+
+// One patch of a cell against the lane's ray (pa, ray_dir along vc): the
+// bounding box, the disc cull and the roots of getPatchIntersectionData
+// (ConnollySurface.cpp:4496-4586; SkinSurface.cpp:3602-3700), appending
+// feasible depths (ray parameter in [0, 1] units of ray_dir) via emit.
+template <int PANEL, int kRoots, class Emit>
+__device__ __forceinline__ bool tile_patch(const DevPatches &P, int it, double probe, const double pa[3],
+                                           double ray_dir, double skin_dir, Emit emit) {
+    using A = PanelAxes<PANEL>;
+    constexpr int fd = A::fd, ld = A::ld, vc = A::vc;
+    const int kind = P.kind[it];
+    const double *pd = P.data + P.off[it];
+    if (kind >= NSGPU_PATCH_SKIN_POINT) {
+        if (pa[fd] < pd[fd] || pa[fd] > pd[3 + fd] || pa[ld] < pd[ld] || pa[ld] > pd[3 + ld]) return false;
+        if (kind == NSGPU_PATCH_SKIN_POINT) {  // EQ_CULLING (SkinSurface.cpp:3602-3608)
+            const double *sp = P.bsph + 4 * it;
+            const double dx = sp[fd] - pa[fd], dy = sp[ld] - pa[ld];
+            if (dx * dx + dy * dy > sp[3] * sp[3]) return true;
+        }
+        double t[2];
+        if (!ray_quadric(pd + 6, pa, vc, t)) return true;
+        double ip0[3] = {pa[0], pa[1], pa[2]}, ip1[3] = {pa[0], pa[1], pa[2]};
+        ip0[vc] += t[0];
+        ip1[vc] += t[1];
+        bool ok0, ok1;
+        skin_feasible2(pd + NSGPU_SKIN_HDR, P.cnt[it], ip0, ip1, ok0, ok1);
+        if (ok0) emit(t[0] / skin_dir);
+        if (ok1) emit(t[1] / skin_dir);
+        return true;
+    }
+    const double sc[3] = {P.bsph[4 * it], P.bsph[4 * it + 1], P.bsph[4 * it + 2]};
+    const double radius = P.bsph[4 * it + 3];
+    if (pa[fd] < sc[fd] - radius || pa[fd] > sc[fd] + radius || pa[ld] < sc[ld] - radius || pa[ld] > sc[ld] + radius)
+        return false;
+    const double dx = sc[fd] - pa[fd], dy = sc[ld] - pa[ld];
+    if (dx * dx + dy * dy > radius * radius) return true;
+    double dir[3] = {0., 0., 0.};
+    dir[vc] = ray_dir;
+    double t[4];
+    if (!ray_sphere(pa, dir, sc, radius, t[0], t[1])) return true;
+    const int pcnt = P.cnt[it];
+    if (kind == NSGPU_PATCH_TORUS_REG || kind == NSGPU_PATCH_TORUS_SING) {
+        int ni;
+        if (kRoots == kRootsMixed) ni = ray_torus_mixed(pd, sc, probe, radius, vc, pa, ray_dir, t);
+        else if (kRoots == kRootsSturm) ni = ray_torus_sturm(pd, sc, probe, radius, vc, pa, ray_dir, t);
+        else ni = ray_torus(pd, sc, probe, radius, vc, pa, ray_dir, t);
+        const int pflags = P.flags[it];
+        for (int i = 0; i < ni; i++) {
+            double ip[3] = {pa[0], pa[1], pa[2]};
+            ip[vc] += t[i] * ray_dir;
+            if (feasible_torus(kind, pd, pcnt, pflags, ip)) emit(t[i]);
+        }
+        return true;
+    }
+    double ip0[3] = {pa[0], pa[1], pa[2]}, ip1[3] = {pa[0], pa[1], pa[2]};
+    ip0[vc] += t[0] * ray_dir;
+    ip1[vc] += t[1] * ray_dir;
+    bool ok0, ok1;
+    feasible_plain2(kind, pd, pcnt, ip0, ip1, ok0, ok1);
+    if (ok0) emit(t[0]);
+    if (ok1) emit(t[1]);
+    return true;
+}
+
+// One wave per (cell, slot): lane l takes the ray of pixel (8 cu + l % 8,
+// 8 cv + l / 8) and walks the cell's patch list, which every lane reads in
+// the same order.  The lane owns its ray, so depths are written without
+// atomics; the per-ray sort that follows (k_sort) orders them.
+template <int PANEL, int kRoots>
+__global__ void __launch_bounds__(64) k_cast_tiles(DevGrid g, DevPatches P, DevRays R, CellGrid C, double probe,
+                                                   int slot, unsigned long long *part) {
+    using A = PanelAxes<PANEL>;
+    const int nc = C.nc, cell = blockIdx.x;
+    const int cu = cell % nc, cv = cell / nc;
+    const int lane = threadIdx.x;
+    const int m = cu * kCell + (lane % kCell), n = cv * kCell + lane / kCell;
+    const int gn[3] = {g.nx, g.ny, g.nz};
+    const bool valid = m < gn[A::fd] && n < gn[A::ld];
+    const double delta = slot < R.panels0 ? 0. : kDelta - g.hside;
+    const double *ga = A::vc == 0 ? g.x : (A::vc == 1 ? g.y : g.z);
+    const double *gf = A::fd == 0 ? g.x : g.y;
+    const double *gl = A::ld == 1 ? g.y : g.z;
+    const double gmin[3] = {g.xmin, g.ymin, g.zmin}, gmax[3] = {g.xmax, g.ymax, g.zmax};
+    const double ray_dir = ga[gn[A::vc] - 1] - ga[0], skin_dir = gmax[A::vc] - gmin[A::vc];
+    double pa[3];
+    pa[A::vc] = ga[0] + delta;
+    pa[A::fd] = (valid ? gf[m] : gf[0]) + delta;
+    pa[A::ld] = (valid ? gl[n] : gl[0]) + delta;
+    const int64_t NN = (int64_t)R.N * R.N, r = (int64_t)slot * NN + (int64_t)n * R.N + m;
+    double *tout = R.t + r * R.K;
+    int *pout = R.pid ? R.pid + r * R.K : nullptr;
+    int cnt = 0;
+    unsigned int cand = 0;
+    const int pc = PANEL * nc * nc + cell;
+    const int b = C.off[pc], e = b + C.count[pc];
+    for (int q = b; q < e; ++q) {
+        const int it = C.list[q];
+        if (!valid) continue;
+        const bool in_box = tile_patch<PANEL, kRoots>(P, it, probe, pa, ray_dir, skin_dir, [&](double t) {
+            if (cnt < R.K) {
+                tout[cnt] = t;
+                if (pout) pout[cnt] = it;
+            }
+            ++cnt;
+        });
+        cand += in_box;
+    }
+    if (valid) R.cnt[r] = cnt;
+    for (int off = 32; off > 0; off >>= 1) cand += __shfl_down(cand, off, 64);
+    if (lane == 0) part[(int64_t)slot * nc * nc + cell] = cand;
+}
+
+int64_t tile_workgroups(const DevRays &R, int nc) { return (int64_t)nc * nc * R.nslots; }
+
+void launch_cast_tiles(hipStream_t s, const DevGrid &g, const DevPatches &P, const DevRays &R, const CellGrid &C,
+                       double probe, unsigned long long *part, int roots) {
+    const dim3 grid((unsigned)((int64_t)C.nc * C.nc)), block(64);
+    for (int slot = 0; slot < R.nslots; ++slot) {
+        const int panel = slot < R.panels0 ? slot : slot - R.panels0;
+#define NSG_TILES(PN, RT) hipLaunchKernelGGL((k_cast_tiles<PN, RT>), grid, block, 0, s, g, P, R, C, probe, slot, part)
+#define NSG_TILES_P(RT)               \
+    if (panel == 0) NSG_TILES(0, RT); \
+    else if (panel == 1) NSG_TILES(1, RT); \
+    else NSG_TILES(2, RT)
+        if (roots == kRootsMixed) { NSG_TILES_P(kRootsMixed); }
+        else if (roots == kRootsSturm) { NSG_TILES_P(kRootsSturm); }
+        else { NSG_TILES_P(kRootsClosed); }
+#undef NSG_TILES_P
+#undef NSG_TILES
+    }
+}
+
+// getRayIntersection for arbitrary axis-aligned rays (castAxisOrientedRay,
+// nanoshaper.cpp:220-297; ConnollySurface::getRayIntersection,
+// ConnollySurface.cpp:5494-5608): one thread per ray o -> o + len e_vc, the
+// patches of the ray's cell, their feasible roots as coordinates along the
+// axis, sorted, with depths closer than EPS_INT to the last kept one dropped.
+// count[r] = hits kept (-1: len <= 0; -2: the ray misses the grid's cells);
+// more than maxh hits keep the first maxh in depth order and report all.
+template <int PANEL>
+__global__ void __launch_bounds__(64) k_cast_rays(DevGrid g, DevPatches P, CellGrid C, double probe, int n,
+                                                  const double *orig, const double *coord, int maxh, int *count,
+                                                  double *hits, double *normals) {
+    using A = PanelAxes<PANEL>;
+    constexpr int vc = A::vc, fd = A::fd, ld = A::ld;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const double pa[3] = {orig[3 * r], orig[3 * r + 1], orig[3 * r + 2]};
+    const double len = coord[r] - pa[vc];
+    if (!(len > 0)) {
+        count[r] = -1;
+        return;
+    }
+    const double gmin[3] = {g.xmin, g.ymin, g.zmin};
+    const int gn[3] = {g.nx, g.ny, g.nz};
+    const double pf = rintp((pa[fd] - gmin[fd]) * g.scale), pl = rintp((pa[ld] - gmin[ld]) * g.scale);
+    if (!(pf >= 0 && pl >= 0 && pf < gn[fd] && pl < gn[ld])) {
+        count[r] = -2;
+        return;
+    }
+    const int cell = PANEL * C.nc * C.nc + ((int)pl / kCell) * C.nc + (int)pf / kCell;
+    double *H = hits + (int64_t)r * maxh;
+    double *Nm = normals ? normals + (int64_t)r * maxh * 3 : nullptr;
+    int total = 0;
+    const int b = C.off[cell], e = b + C.count[cell];
+    for (int q = b; q < e; ++q) {
+        const int it = C.list[q];
+        tile_patch<PANEL, kRootsClosed>(P, it, probe, pa, len, 1.0, [&](double t) {
+            // the skin form returns a distance along the axis, the others a
+            // fraction of len; both become a coordinate
+            const double a = P.kind[it] >= NSGPU_PATCH_SKIN_POINT ? pa[vc] + t : pa[vc] + len * t;
+            double nrm[3] = {0., 0., 0.};
+            if (Nm) {
+                double y[3] = {pa[0], pa[1], pa[2]};
+                y[vc] = a;
+                analytic_normal(P, it, probe, y, nrm);
+            }
+            // insertion into the ray's sorted list (ties keep arrival order)
+            int pos = total < maxh ? total : maxh;
+            while (pos > 0 && H[pos - 1] > a) {
+                if (pos < maxh) {
+                    H[pos] = H[pos - 1];
+                    if (Nm)
+                        for (int k = 0; k < 3; ++k) Nm[3 * pos + k] = Nm[3 * (pos - 1) + k];
+                }
+                --pos;
+            }
+            if (pos < maxh) {
+                H[pos] = a;
+                if (Nm)
+                    for (int k = 0; k < 3; ++k) Nm[3 * pos + k] = nrm[k];
+            }
+            ++total;
+        });
+    }
+    // near-coincident depths (castAxisOrientedRay's clean-up): keep a depth
+    // only when it is at least EPS_INT from the last kept one
+    const int kept_in = total < maxh ? total : maxh;
+    int kept = 0;
+    double last = INFINITY;
+    for (int q = 0; q < kept_in; ++q) {
+        if (fabs(H[q] - last) >= kEpsInt) {
+            H[kept] = H[q];
+            if (Nm)
+                for (int k = 0; k < 3; ++k) Nm[3 * kept + k] = Nm[3 * q + k];
+            ++kept;
+            last = H[q];
+        }
+    }
+    count[r] = kept + (total - kept_in);
+}
+
+void launch_cast_rays(hipStream_t s, const DevGrid &g, const DevPatches &P, const CellGrid &C, double probe,
+                      int panel, int n, const double *orig, const double *coord, int maxh, int *count, double *hits,
+                      double *normals) {
+    if (n <= 0) return;
+    const dim3 grid((unsigned)((n + 63) / 64)), block(64);
+    if (panel == 0) hipLaunchKernelGGL(k_cast_rays<0>, grid, block, 0, s, g, P, C, probe, n, orig, coord, maxh, count, hits, normals);
+    else if (panel == 1) hipLaunchKernelGGL(k_cast_rays<1>, grid, block, 0, s, g, P, C, probe, n, orig, coord, maxh, count, hits, normals);
+    else hipLaunchKernelGGL(k_cast_rays<2>, grid, block, 0, s, g, P, C, probe, n, orig, coord, maxh, count, hits, normals);
+}
+
 int64_t intersect_workgroups(const DevPatches &P, const DevRays &R) { return (int64_t)P.n * R.nslots; }
 int64_t sort_blocks(const DevRays &R) { return ((int64_t)R.N * R.N * R.nslots + 255) / 256; }
 

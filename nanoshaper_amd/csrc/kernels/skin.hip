@@ -12,48 +12,6 @@
 
 namespace nsg {
 
-// unit-axis ray o + t e_axis against the quadric; the coefficient order and
-// the expressions are the reference's
-__device__ __forceinline__ bool ray_quadric(const double *Q, const double o[3], const int axis, double t[2]) {
-    const double q44 = Q[3], q33 = Q[2], q22 = Q[1], q11 = Q[0], q34 = Q[6], q24 = Q[8], q23 = Q[5], q14 = Q[9],
-                 q13 = Q[7], q12 = Q[4];
-    double qa, b;
-    if (axis == 0) {
-        qa = q11;
-        b = q14 + o[2] * q13 + o[1] * q12 + o[0] * q11;
-    } else if (axis == 1) {
-        qa = q22;
-        b = q24 + o[2] * q23 + o[1] * q22 + o[0] * q12;
-    } else {
-        qa = q33;
-        b = q34 + o[2] * q33 + o[1] * q23 + o[0] * q13;
-    }
-    if (qa == 0) return false;
-    const double c = q44 + (o[2] + o[2]) * q34 + (o[2] * o[2]) * q33 + (o[1] + o[1]) * q24 + (2 * o[1] * o[2]) * q23 +
-                     (o[1] * o[1]) * q22 + (o[0] + o[0]) * q14 + (2 * o[0] * o[2]) * q13 + (2 * o[0] * o[1]) * q12 +
-                     (o[0] * o[0]) * q11;
-    double delta = b * b - qa * c;
-    if (delta < 0) return false;
-    delta = sqrt(delta);
-    t[0] = (-b - delta) / qa;
-    t[1] = (-b + delta) / qa;
-    return true;
-}
-
-// both roots of a ray against the cell's planes (isFeasible for each): one
-// pass loads each plane once; a lane stops when both points have failed
-__device__ __forceinline__ void skin_feasible2(const double *pl, int cnt, const double p[3], const double r[3],
-                                               bool &ok_p, bool &ok_r) {
-    bool out_p = false, out_r = false;
-    for (int q = 0; q < cnt && !(out_p && out_r); ++q) {
-        const double *w = pl + 4 * q;
-        out_p |= w[0] * p[0] + w[1] * p[1] + w[2] * p[2] + w[3] > 0;
-        out_r |= w[0] * r[0] + w[1] * r[1] + w[2] * r[2] + w[3] > 0;
-    }
-    ok_p = !out_p;
-    ok_r = !out_r;
-}
-
 // A cell's rectangle for one panel slot.
 struct SkinRect {
     int it, kind, cnt;

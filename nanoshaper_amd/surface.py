@@ -291,6 +291,28 @@ class DeviceSurface:
         self.result = _result_dict(r)
         return {"volume": v.value, "area": s.value, "dvolume": dv[:n], "darea": da[:n]}
 
+    CAST_PATCHES, CAST_TILES = 0, 1
+
+    def set_cast_mode(self, mode):
+        """Intersection of get_surf: CAST_PATCHES (one workgroup per patch and
+        panel) or CAST_TILES (ray tiles over a cell grid of the patches)."""
+        self._chk(self._lib.nsgpu_set_cast_mode(self._h, int(mode)), "set_cast_mode")
+
+    def cast_rays(self, direction, origins, coords, max_hits=32, normals=False):
+        """castAxisOrientedRay for a batch: rays from origins (n, 3) along
+        direction (0 x, 1 y, 2 z) to coordinate coords (n,).  Returns
+        (counts (n,), hits (n, max_hits) axis coordinates ascending[, normals
+        (n, max_hits, 3)]); counts -1: coord <= origin, -2: outside the grid."""
+        o = np.ascontiguousarray(origins, dtype=np.float64).reshape(-1, 3)
+        n = o.shape[0]
+        c = np.ascontiguousarray(np.broadcast_to(np.asarray(coords, dtype=np.float64), (n,)))
+        counts = np.zeros(n, np.int32)
+        hits = np.zeros((n, max_hits))
+        nrm = np.zeros((n, max_hits, 3)) if normals else None
+        self._chk(self._lib.nsgpu_cast_rays(self._h, int(direction), n, _ptr(o), _ptr(c), int(max_hits), _ptr(counts),
+                                            _ptr(hits), _ptr(nrm) if normals else None), "cast_rays")
+        return (counts, hits, nrm) if normals else (counts, hits)
+
     def set_profiling(self, on=True):
         self._chk(self._lib.nsgpu_set_profiling(self._h, int(on)), "set_profiling")
 
@@ -513,7 +535,9 @@ class Batch:
         """DeviceSurface view of frame f (maps, status, cavities)."""
         h = self._lib.nsgpu_batch_frame(self._h, int(f))
         d = DeviceSurface._view(h, self.grids[f])
-        d.result = self.result(f)
+        r = SurfResult()
+        if self._lib.nsgpu_batch_result(self._h, int(f), ctypes.byref(r)) == 0:
+            d.result = _result_dict(r)  # None until the frame has been computed
         return d
 
     def close(self):
