@@ -374,6 +374,12 @@ int nsgpu_read_mesh(nsgpu_ctx *ctx, double *verts, int32_t *tris);
  * plane3points normals in triangle order (approximateNormals, Surface.cpp:
  * 10588-10737). */
 int nsgpu_read_normals(nsgpu_ctx *ctx, double *normals);
+/* Surface::smoothSurface (Surface.cpp:13089-13240), the Smooth_Mesh pass of
+ * triangulate: every vertex of the last mesh moves to the mean of itself and
+ * the average of its triangles' other vertices; with vertex normals on, each
+ * normal becomes the normalised sum of itself and its neighbours' average
+ * (nsgpu_read_normals then returns the smoothed normals). */
+int nsgpu_smooth_surface(nsgpu_ctx *ctx);
 /* Compute_Vertex_Normals: when on, get_surf keeps the patch of every depth so
  * that nsgpu_read_normals can follow the mesh (off by default, as in the
  * reference; it costs the sort a second permuted array). */

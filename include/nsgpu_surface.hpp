@@ -8,7 +8,11 @@
 //   ConnollySurface "ses" ConnollySurface.h:682-692 (factory registration)
 //   MeshSurface "mesh"    MeshSurface.h:211-221, MeshSurface.cpp:176-248 (OFF input)
 //   BlobbySurface "blobby" BlobbySurface.h:68-79, BlobbySurface.cpp:83-229
+//   SkinSurface "skin"    SkinSurface.h:516-526 (mixed cells from a regular
+//                         triangulation the caller plugs in)
 //   SurfaceFactory        SurfaceFactory.h:15-65 (name -> instantiator, key "Surface")
+//   normalMode            main.cpp:1032-1204 (init's keys and checks), :1253-1320
+//   NanoShaper            nanoshaper.h:294-424, nanoshaper.cpp (the C++ API)
 // Everything on the device goes through the extern "C" entry points; this
 // header adds no symbols to libnsgpu.so.
 #pragma once
@@ -16,9 +20,12 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <array>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <map>
+#include <memory>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -192,6 +199,7 @@ class Surface {
     double triangulateSurface(bool outputMesh = true, bool /*buildAtomsMapHere*/ = false, double /*iso*/ = 0.0) {
         nsgpu_mesh_info m{};
         if (!ctx_ || nsgpu_triangulate(ctx_, &m) != NSGPU_OK) return 0.0;
+        mesh_ = m;
         if (outputMesh) {
             vertList.resize(m.num_vertices * 3);
             triList.resize(m.num_triangles * 3);
@@ -203,6 +211,30 @@ class Surface {
         }
         return m.area;
     }
+
+    // Surface::smoothSurface (Surface.cpp:13089-13240) on the last mesh, and
+    // on its normals when computeNormals is on
+    bool smoothSurface() {
+        nsgpu_mesh_info m{};
+        if (!ctx_ || nsgpu_smooth_surface(ctx_) != NSGPU_OK) return false;
+        if (!last_mesh(m)) return false;
+        vertList.resize(m.num_vertices * 3);
+        triList.resize(m.num_triangles * 3);
+        if (nsgpu_read_mesh(ctx_, vertList.data(), triList.data()) != NSGPU_OK) return false;
+        if (computeNormals) {
+            normList.resize(m.num_vertices * 3);
+            if (nsgpu_read_normals(ctx_, normList.data()) != NSGPU_OK) return false;
+        }
+        return true;
+    }
+    void setComputeNormals(bool on) { computeNormals = on; }
+    bool getComputeNormals() const { return computeNormals; }
+    void setRandDisplacement(double r) { randDisplacement = r; }
+    int64_t getNumVertices() const { return (int64_t)(vertList.size() / 3); }
+    int64_t getNumTriangles() const { return (int64_t)(triList.size() / 3); }
+    // the device context holding the surface (created and filled on first use)
+    nsgpu_ctx *context() { return ensure_device() ? ctx_ : nullptr; }
+    const nsgpu_grid &deviceGrid() const { return grid_; }
 
     const char *lastError() const {
         if (!err_.empty()) return err_.c_str();
@@ -247,6 +279,11 @@ class Surface {
     nsgpu_ctx *ctx_ = nullptr;
     bool uploaded_ = false;
     nsgpu_grid grid_{};  // the grid on the device
+    nsgpu_mesh_info mesh_{};  // the last triangulation
+    bool last_mesh(nsgpu_mesh_info &m) const {
+        m = mesh_;
+        return mesh_.num_vertices || mesh_.num_triangles;
+    }
     nsgpu_surf_result last_{};
 };
 
@@ -382,6 +419,63 @@ class BlobbySurface : public Surface {
     }
 };
 
+// "skin": the skin of the jittered atoms (Skin_Surface_Parameter s) from
+// the mixed cells the host builds (nsgpu_skin_build) out of the regular
+// triangulation of the weighted points nsgpu_skin_points gives.  The
+// reference takes that triangulation from CGAL's Regular_triangulation_3
+// (SkinSurface.cpp:1271-1811); here the caller plugs one in: it receives the
+// (n + 6) weighted points (x, y, z, w) and returns the finite tetrahedra and
+// their neighbours (-1 on the hull), as nsgpu_skin_build takes them.
+class SkinSurface : public Surface {
+   public:
+    using Triangulator = std::function<bool(const std::vector<double> &wpts, std::vector<int32_t> &tets,
+                                            std::vector<int32_t> &nbrs)>;
+    static Triangulator &triangulator() {
+        static Triangulator t;
+        return t;
+    }
+    SkinSurface(ConfigFile *cf, DelPhiShared *ds) : Surface(cf, ds) {
+        if (cf) s = cf->read<double>("Skin_Surface_Parameter", s);
+    }
+    ~SkinSurface() override {
+        if (ps_) nsgpu_patch_set_free(ps_);
+    }
+    bool build() override {
+        if (!delphi || delphi->atoms.empty()) return false;
+        if (!triangulator()) {
+            err_ = "skin surfaces need a regular triangulator (SkinSurface::triangulator())";
+            return false;
+        }
+        jittered_ = delphi->atoms;
+        int32_t seed = 1234;  // SEED, tools.h:1827
+        const int32_t n = (int32_t)(jittered_.size() / 4);
+        nsgpu_jitter(jittered_.data(), n, randDisplacement, &seed);
+        std::vector<double> w((size_t)(n + 6) * 4);
+        if (nsgpu_skin_points(jittered_.data(), n, s, w.data()) != NSGPU_OK) return false;
+        std::vector<int32_t> tets, nbrs;
+        if (!triangulator()(w, tets, nbrs) || tets.size() % 4 || nbrs.size() != tets.size()) {
+            err_ = "the regular triangulation failed";
+            return false;
+        }
+        if (ps_) nsgpu_patch_set_free(ps_);
+        ps_ = nullptr;
+        if (nsgpu_skin_build(jittered_.data(), n, s, tets.data(), nbrs.data(), (int32_t)(tets.size() / 4), &ps_) !=
+            NSGPU_OK)
+            return false;
+        nsgpu_patch_set_view(ps_, &view_);
+        uploaded_ = false;
+        return true;
+    }
+    int getNumPatches() override { return ps_ ? view_.n : 0; }
+    const nsgpu_patches *patches() override { return ps_ ? &view_ : nullptr; }
+    double s = 0.45;
+
+   private:
+    std::vector<double> jittered_;
+    nsgpu_patch_set *ps_ = nullptr;
+    nsgpu_patches view_{};
+};
+
 typedef Surface *(*surface_instantiator)(ConfigFile *, DelPhiShared *);
 
 class SurfaceFactory {
@@ -405,10 +499,431 @@ inline SurfaceFactory &surfaceFactory() {
         f.add("ses", [](ConfigFile *c, DelPhiShared *d) -> Surface * { return new ConnollySurface(c, d); });
         f.add("mesh", [](ConfigFile *c, DelPhiShared *d) -> Surface * { return new MeshSurface(c, d); });
         f.add("blobby", [](ConfigFile *c, DelPhiShared *d) -> Surface * { return new BlobbySurface(c, d); });
+        f.add("skin", [](ConfigFile *c, DelPhiShared *d) -> Surface * { return new SkinSurface(c, d); });
         return true;
     }();
     (void)init;
     return f;
 }
+
+// The normal-mode keys of the CLI (init, main.cpp:1032-1204) with the
+// reference's defaults; error is set for the flag combinations it refuses.
+struct Configuration {
+    bool buildEpsmaps, buildStatus, fillCavities, accTri, tri, optimizeGrids, smoothing, patchBased, projBGP;
+    double cavVol, scale, perfill, domainShrinkage;
+    int maxNumAtoms;
+    std::string molFile, surface, operativeMode, error;
+    explicit Configuration(const ConfigFile &cf) {
+        buildEpsmaps = cf.read<bool>("Build_epsilon_maps", false);
+        const bool saveEps = cf.read<bool>("Save_eps_maps", false);
+        buildStatus = cf.read<bool>("Build_status_map", false);
+        const bool saveIdeb = cf.read<bool>("Save_ideb_map", false);
+        fillCavities = cf.read<bool>("Cavity_Detection_Filling", false);
+        projBGP = cf.read<bool>("Project_boundary_grid_points", false);
+        accTri = cf.read<bool>("Accurate_Triangulation", false);
+        tri = cf.read<bool>("Triangulation", false);
+        operativeMode = cf.read<std::string>("Operative_Mode", "normal");
+        patchBased = cf.read<bool>("Patch_Based_Algorithm", true);
+        maxNumAtoms = cf.read<int>("Max_Num_Atoms", -1);
+        domainShrinkage = cf.read<double>("Domain_Shrinkage", 0.0);
+        optimizeGrids = cf.read<bool>("Optimize_Grids", true);
+        surface = cf.read<std::string>("Surface", "ses");
+        cavVol = cf.read<double>("Conditional_Volume_Filling_Value", 11.4);
+        scale = cf.read<double>("Grid_scale", 2.0);
+        perfill = cf.read<double>("Grid_perfil", 80.0);
+        molFile = cf.read<std::string>("XYZR_FileName", "temp.xyzr");
+        smoothing = cf.read<bool>("Smooth_Mesh", false);
+        if (!buildEpsmaps && saveEps) error = "Asked to save epsmap without builiding it";
+        else if (!buildEpsmaps && projBGP) error = "Cannot project boundary grid points without an epsilon map";
+        else if (!accTri && !buildStatus && tri) error = "If non analytical triangulation is enabled status map is needed";
+        else if (fillCavities && !buildStatus) error = "If cavity detection is enabled status map is needed";
+        else if (saveIdeb && !buildEpsmaps) error = "Idebmap is computed only if epsilon map is enabled";
+        else if (patchBased && surface != "ses" && surface != "skin" && surface != "mesh")
+            error = "This surface type does not support patch based ray-tracing";
+        else if (operativeMode != "normal") error = "only the normal operative mode is built";
+    }
+};
+
+// normalMode (main.cpp:1253-1320): build, getSurf, and triangulateSurface
+// (with the Smooth_Mesh pass) when Triangulation is on.
+struct NormalModeResult {
+    bool ok = false;
+    double volume = 0, area = 0;
+    std::string error;
+};
+inline NormalModeResult normalMode(Surface *surf, DelPhiShared *dg, const Configuration &conf) {
+    NormalModeResult r;
+    if (!surf || !dg) {
+        r.error = "no surface";
+        return r;
+    }
+    if (!surf->build()) {
+        r.error = std::string("Surface construction failed: ") + surf->lastError();
+        return r;
+    }
+    if (!surf->getSurf(&r.volume, conf.optimizeGrids, conf.fillCavities, conf.cavVol)) {
+        r.error = std::string("getSurf failed: ") + surf->lastError();
+        return r;
+    }
+    if (conf.tri) {
+        r.area = surf->triangulateSurface(!conf.smoothing, !conf.smoothing);
+        if (conf.smoothing && !surf->smoothSurface()) {
+            r.error = std::string("smoothSurface failed: ") + surf->lastError();
+            return r;
+        }
+    }
+    r.ok = true;
+    return r;
+}
+
+// A configuration file run as the CLI runs it: the atom file relative to
+// the configuration's directory.
+inline NormalModeResult runConfiguration(const std::string &prm, DelPhiShared &dg, std::unique_ptr<Surface> &surf) {
+    NormalModeResult r;
+    ConfigFile cf(prm);
+    Configuration conf(cf);
+    if (!conf.error.empty()) {
+        r.error = conf.error;
+        return r;
+    }
+    std::string mol = conf.molFile;
+    const size_t slash = prm.find_last_of('/');
+    if (!mol.empty() && mol[0] != '/' && slash != std::string::npos) mol = prm.substr(0, slash + 1) + mol;
+    dg.buildEpsMap = conf.buildEpsmaps;
+    dg.buildStatus = conf.buildStatus;
+    if (!dg.loadAtoms(mol, conf.maxNumAtoms, conf.domainShrinkage)) {
+        r.error = dg.loadError;
+        return r;
+    }
+    if (!dg.buildGrid(conf.scale, conf.perfill)) {
+        r.error = "buildGrid failed";
+        return r;
+    }
+    surf.reset(surfaceFactory().create(&cf, &dg));
+    if (!surf) {
+        r.error = "unknown surface " + conf.surface;
+        return r;
+    }
+    return normalMode(surf.get(), &dg, conf);
+}
+
+// ------------------------------------------------------------------ NanoShaper
+// Atom of the C++ API (nanoshaper.h:227-292): centre, radius, charge.
+struct Atom {
+    double pos[3] = {0, 0, 0};
+    double radius = 0, charge = 0;
+    int dielectric = 0;
+    Atom() = default;
+    Atom(double x, double y, double z, double r, double q = 0) : pos{x, y, z}, radius(r), charge(q) {}
+};
+
+enum surface_type { ses, skin, blobby };
+
+// Grid-ray crossings of colourGrid with collectGridRays (raytracer_datatype.h:9-43):
+// per direction (0 x, 1 y, 2 z) and ray (its two other coordinates), the
+// kept segment ends along the ray and their normals.
+struct crossings_t {
+    double point[2] = {0, 0};
+    unsigned dir = 0;
+    bool init = false;
+    std::vector<double> inters, normals;
+};
+struct map_compare {
+    bool operator()(const std::array<double, 2> &a, const std::array<double, 2> &b) const {
+        const double tol = 1.e-3;
+        if (a[0] - b[0] < -tol) return true;
+        if (std::fabs(a[0] - b[0]) <= tol) return a[1] - b[1] < -tol;
+        return false;
+    }
+};
+using rays_t = std::array<std::map<std::array<double, 2>, crossings_t, map_compare>, 3>;
+
+// NS::NanoShaper (nanoshaper.h:294-424): atoms in, surface built on the host
+// and cast on the device, grid queries and single axis-aligned rays.
+class NanoShaper {
+   public:
+    NanoShaper(const std::vector<Atom> &ats, surface_type flag, double parameter, double stern_layer_thickness = 0.,
+               unsigned nt = 1, const std::string *configFile = nullptr) {
+        initConstructor(ats, flag, parameter, stern_layer_thickness, nt, configFile);
+    }
+    NanoShaper() = default;
+
+    // nanoshaper.cpp:30-91: defaults, then the file, then the arguments
+    void initConstructor(const std::vector<Atom> &ats, surface_type flag, double parameter, double /*stern*/,
+                         unsigned num_cores = 1, const std::string *configFile = nullptr) {
+        initConfig(configFile);
+        atoms_ = ats;
+        currentPanel_ = -1;
+        cf_.add<int>("Number_thread", (int)num_cores);
+        if (flag == skin) {
+            cf_.add<std::string>("Surface", "skin");
+            cf_.add<double>("Skin_Surface_Parameter", parameter);
+        } else if (flag == ses) {
+            cf_.add<std::string>("Surface", "ses");
+            cf_.add<double>("Probe_Radius", parameter);
+        } else {
+            cf_.add<std::string>("Surface", "blobby");
+            cf_.add<double>("Blobbyness", parameter);
+            cf_.add<bool>("Patch_Based_Algorithm", false);
+        }
+    }
+
+    // initConfig (nanoshaper.cpp:707-805): the defaults of the API
+    void initConfig(const std::string *filePath = nullptr) {
+        cf_ = filePath ? ConfigFile(*filePath) : ConfigFile();
+        auto def = [&](const char *k, const std::string &v) {
+            if (!cf_.keyExists(k)) cf_.parse_line(std::string(k) + " = " + v);
+        };
+        def("Build_epsilon_maps", "false");
+        def("Build_status_map", "true");
+        def("Cavity_Detection_Filling", "false");
+        def("Conditional_Volume_Filling_Value", "11.4");
+        def("Accurate_Triangulation", "false");
+        def("Triangulation", "false");
+        def("Smooth_Mesh", "true");
+        def("Optimize_Grids", "true");
+        def("Grid_scale", "1.0");
+        def("Grid_perfil", "80.0");
+        def("Probe_Radius", "1.4");
+        def("Skin_Surface_Parameter", "0.45");
+        def("Blobbyness", "-2.5");
+        def("Compute_Vertex_Normals", "false");
+        def("Patch_Based_Algorithm", "true");
+        def("Analytical_Ray_Vs_Torus_Intersection", "true");
+        def("Max_Num_Atoms", "-1");
+        def("Domain_Shrinkage", "0");
+    }
+
+    template <class T>
+    T readConfig(const std::string &k) { return cf_.read<T>(k, T()); }
+    template <class T>
+    void setConfig(const std::string &k, const T &v) { cf_.add<T>(k, v); }
+
+    // buildAnalyticalSurface (nanoshaper.cpp:300-368): grid by perfil, the
+    // surface of key "Surface", its host build
+    bool buildAnalyticalSurface(double randDisplacement = -1) {
+        clean();
+        if (cf_.read<bool>("PB_grid_mode", false)) {
+            err_ = "PB_grid_mode (explicit grid bounds) is not built";
+            return false;
+        }
+        ds_.reset(new DelPhiShared(cf_.read<bool>("Build_epsilon_maps", false), cf_.read<bool>("Build_status_map", true)));
+        std::vector<double> xyzr;
+        for (const Atom &a : atoms_) xyzr.insert(xyzr.end(), {a.pos[0], a.pos[1], a.pos[2], a.radius});
+        ds_->loadAtoms(xyzr);
+        if (!ds_->buildGrid(cf_.read<double>("Grid_scale", 1.0), cf_.read<double>("Grid_perfil", 80.0))) return false;
+        surf_.reset(surfaceFactory().create(&cf_, ds_.get()));
+        if (!surf_) return false;
+        if (randDisplacement >= 0) surf_->setRandDisplacement(randDisplacement);
+        return surf_->build();
+    }
+
+    bool colourGrid() {
+        double v = 0;
+        return colourGrid(&v);
+    }
+    // colourGrid (nanoshaper.cpp:498-690), with the grid rays' crossings
+    // collected when setCollectGridRays(true)
+    bool colourGrid(double *surf_volume) {
+        if (!surf_) return false;
+        if (!surf_->getSurf(surf_volume, cf_.read<bool>("Optimize_Grids", true),
+                            cf_.read<bool>("Cavity_Detection_Filling", false),
+                            cf_.read<double>("Conditional_Volume_Filling_Value", 11.4)))
+            return false;
+        return collectGridRays_ ? collectRays() : true;
+    }
+
+    // triangulate (nanoshaper.cpp:371-388)
+    bool triangulate(double *surf_area) {
+        if (!surf_ || !ds_) return false;
+        const bool smooth = cf_.read<bool>("Smooth_Mesh", true);
+        *surf_area = surf_->triangulateSurface(!smooth, !smooth);
+        if (*surf_area == 0) return false;
+        if (smooth) return surf_->smoothSurface();
+        return true;
+    }
+
+    void clean() {
+        surf_.reset();
+        ds_.reset();
+        currentPanel_ = -1;
+        for (auto &m : rays_) m.clear();
+    }
+
+    // panelResolver (nanoshaper.cpp:190-206): x -> panel 0, z -> panel 1, y -> panel 2
+    static int panelResolver(unsigned direction) { return direction == 0 ? 0 : direction == 2 ? 1 : direction == 1 ? 2 : -1; }
+
+    bool setDirection(unsigned direction) {
+        if (direction > 2 || !surf_) return false;
+        currentPanel_ = panelResolver(direction);
+        return true;
+    }
+
+    // castAxisOrientedRay (nanoshaper.cpp:220-297): the ray from pa to the
+    // coordinate coord along the current direction; hits as (coordinate,
+    // normal or nullptr), near-coincident ones dropped.  -1 on a wrong
+    // direction, a non-progressive ray, no surface or no intersection.
+    short castAxisOrientedRay(double *pa, double coord, std::vector<std::pair<double, double *>> &intersections,
+                              unsigned direction, bool computeNormals = false) {
+        const int desired = panelResolver(direction);
+        if (desired < 0 || currentPanel_ != desired || pa[direction] >= coord || !surf_) return -1;
+        nsgpu_ctx *ctx = surf_->context();
+        if (!ctx) return -1;
+        int32_t maxh = 64;
+        for (;;) {
+            int32_t count = 0;
+            hitbuf_.assign(maxh, 0.0);
+            normbuf_.assign(computeNormals ? 3 * maxh : 0, 0.0);
+            if (nsgpu_cast_rays(ctx, (int32_t)direction, 1, pa, &coord, maxh, &count, hitbuf_.data(),
+                                computeNormals ? normbuf_.data() : nullptr) != NSGPU_OK)
+                return -1;
+            if (count < 0) return -1;
+            if (count > maxh) {
+                maxh = count;
+                continue;
+            }
+            if (count == 0) return -1;
+            for (int32_t q = 0; q < count; ++q)
+                intersections.push_back({hitbuf_[q], computeNormals ? &normbuf_[3 * q] : nullptr});
+            return 1;
+        }
+    }
+
+    // getGridPointStatus: +1 outside, -1 inside, 0 unknown
+    int getGridPointStatus(unsigned ix, unsigned iy, unsigned iz) const {
+        if (!ds_ || ds_->status.empty()) return 0;
+        const nsgpu_grid &g = ds_->grid;
+        if (ix >= (unsigned)g.nx || iy >= (unsigned)g.ny || iz >= (unsigned)g.nz) return 0;
+        return ds_->status[((size_t)iz * g.ny + iy) * g.nx + ix] < 0 ? -1 : +1;
+    }
+    bool getGridSize(std::vector<unsigned> &p) const {
+        if (!ds_ || ds_->status.empty()) return false;
+        p = {(unsigned)ds_->grid.nx, (unsigned)ds_->grid.ny, (unsigned)ds_->grid.nz};
+        return true;
+    }
+    bool getGridPointCoordinates(unsigned ix, unsigned iy, unsigned iz, std::vector<double> &p) const {
+        if (!ds_ || ds_->status.empty()) return false;
+        const nsgpu_grid &g = ds_->grid;
+        if (ix >= (unsigned)g.nx || iy >= (unsigned)g.ny || iz >= (unsigned)g.nz) return false;
+        p = {g.xmin + ix * g.side, g.ymin + iy * g.side, g.zmin + iz * g.side};
+        return true;
+    }
+    // getNearestGridPoint (nanoshaper.cpp:391-449): indices saturated to the
+    // grid; true when p lies outside it (saturation), as the reference
+    bool getNearestGridPoint(double p[3], std::vector<unsigned> &indices) const {
+        if (!ds_ || ds_->status.empty()) return false;
+        const nsgpu_grid &g = ds_->grid;
+        const double lo[3] = {g.xmin, g.ymin, g.zmin};
+        const int n[3] = {g.nx, g.ny, g.nz};
+        bool outside = false;
+        indices.clear();
+        for (int d = 0; d < 3; ++d) {
+            int i = (int)std::floor((p[d] - lo[d]) / g.side + 0.5);
+            if (i < 0) {
+                i = 0;
+                outside = true;
+            }
+            if (i >= n[d]) {
+                i = n[d] - 1;
+                outside = true;
+            }
+            indices.push_back((unsigned)i);
+        }
+        return outside;
+    }
+
+    void setCollectGridRays(bool c) { collectGridRays_ = c; }
+    void setGridTol(double t) { gridTol_ = t; }
+    void setOptimizeGrids(bool o) { cf_.add<bool>("Optimize_Grids", o); }
+    rays_t &getRaysMap() { return rays_; }
+
+    Surface *surface() { return surf_.get(); }
+    DelPhiShared *delphi() { return ds_.get(); }
+    const char *lastError() const { return !err_.empty() ? err_.c_str() : surf_ ? surf_->lastError() : "no surface"; }
+
+   private:
+    // the kept segment ends of every traced grid ray (the pairing rule of the
+    // fill) and their analytic normals, cast again along the same grid lines
+    bool collectRays() {
+        nsgpu_ctx *ctx = surf_->context();
+        if (!ctx) return false;
+        for (auto &m : rays_) m.clear();
+        const nsgpu_grid &g = surf_->deviceGrid();
+        const int N = std::max(g.nx, std::max(g.ny, g.nz));
+        const int panels = ds_->buildEpsMap ? 3 : (ds_->buildStatus ? 1 : 0);
+        const double lo[3] = {g.xmin, g.ymin, g.zmin};
+        const int n[3] = {g.nx, g.ny, g.nz};
+        for (int slot = 0; slot < 3; ++slot) {
+            // slot 0: x rays over (y, z); slot 1: z rays over (x, y); slot 2: y rays over (x, z)
+            const int dir = slot == 0 ? 0 : (slot == 1 ? 2 : 1);
+            const int a0 = slot == 0 ? 1 : 0, a1 = slot == 1 ? 1 : 2;
+            for (int j = 0; j < n[a1]; ++j)
+                for (int i = 0; i < n[a0]; ++i) {
+                    std::array<double, 2> pt = {lo[a0] + i * g.side, lo[a1] + j * g.side};
+                    crossings_t &c = rays_[dir][pt];
+                    c.point[0] = pt[0];
+                    c.point[1] = pt[1];
+                    c.dir = (unsigned)dir;
+                    c.init = true;
+                }
+            if (slot >= panels) continue;
+            const int maxh = std::max(1, surf_->lastResult().max_hits_per_ray);
+            std::vector<int32_t> cnt((size_t)N * N);
+            std::vector<double> t((size_t)N * N * maxh);
+            if (nsgpu_read_ray_hits(ctx, slot, maxh, cnt.data(), t.data()) != NSGPU_OK) return false;
+            const double start = lo[dir], len = (n[dir] - 1) * g.side;
+            std::vector<double> orig, coord;
+            std::vector<int> ray;
+            for (int r = 0; r < N * N; ++r) {
+                if (cnt[r] == 0 || (cnt[r] & 1)) continue;
+                const int m = r % N, nn = r / N;
+                double o[3];
+                o[dir] = start;
+                o[a0] = lo[a0] + m * g.side;
+                o[a1] = lo[a1] + nn * g.side;
+                orig.insert(orig.end(), {o[0], o[1], o[2]});
+                coord.push_back(start + len);
+                ray.push_back(r);
+            }
+            const int nr = (int)ray.size();
+            std::vector<int32_t> hc(nr);
+            std::vector<double> hh((size_t)nr * maxh), hn((size_t)nr * maxh * 3);
+            if (nr && nsgpu_cast_rays(ctx, dir, nr, orig.data(), coord.data(), maxh, hc.data(), hh.data(),
+                                      hn.data()) != NSGPU_OK)
+                return false;
+            for (int q = 0; q < nr; ++q) {
+                const int r = ray[q];
+                crossings_t &c = rays_[dir][{orig[3 * q + a0], orig[3 * q + a1]}];
+                const double *T = &t[(size_t)r * maxh];
+                for (int h = 0; h + 1 < cnt[r]; h += 2) {
+                    if (std::fabs(T[h + 1] - T[h]) < 1e-8) continue;  // EPS_INT pairing
+                    for (int e = 0; e < 2; ++e) {
+                        const double x = start + len * T[h + e];
+                        c.inters.push_back(x);
+                        // the normal of the cast depth nearest this one
+                        int best = 0;
+                        for (int k = 1; k < std::min(hc[q], maxh); ++k)
+                            if (std::fabs(hh[(size_t)q * maxh + k] - x) < std::fabs(hh[(size_t)q * maxh + best] - x))
+                                best = k;
+                        const double *nv = &hn[((size_t)q * maxh + best) * 3];
+                        c.normals.insert(c.normals.end(), {nv[0], nv[1], nv[2]});
+                    }
+                }
+            }
+        }
+        return true;
+    }
+
+    ConfigFile cf_;
+    std::vector<Atom> atoms_;
+    std::unique_ptr<DelPhiShared> ds_;
+    std::unique_ptr<Surface> surf_;
+    int currentPanel_ = -1;
+    bool collectGridRays_ = false;
+    double gridTol_ = 1e-2;
+    rays_t rays_;
+    std::vector<double> hitbuf_, normbuf_;
+    std::string err_;
+};
 
 }  // namespace nsgpu

@@ -150,6 +150,7 @@ struct nsgpu_ctx {
     // and edge of each mid point
     DBuf vhit, vorig, normals;
     bool hits_have_patches = false;  // the ray lists of the last trace carry patch ids
+    bool normals_smoothed = false;   // normals holds the smoothed normals of the current mesh
     bool want_normals = false;       // keep patch ids for the vertex normals (Compute_Vertex_Normals)
     DBuf segmask, anode, acase, anv, ant, avoff, atoff;
     // triangle-mesh surface (MeshSurface, BlobbySurface): vertices as loaded,
@@ -1441,6 +1442,7 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
     c->ntris = nt;
     c->area = area;
     c->have_mesh = true;
+    c->normals_smoothed = false;
     if (info) {
         info->num_vertices = c->nverts;
         info->num_triangles = c->ntris;
@@ -1693,6 +1695,8 @@ int nsgpu_blobby_build(nsgpu_ctx *c, const double *xyzr, int32_t n, double B, do
     sm.tris = c->tris.as<int>();
     sm.nact = (int)nact;
     sm.vout = c->mraw.as<double>();
+    sm.nin = nullptr;
+    sm.nout = nullptr;
     HIP_TRY(c, c->cubeidx.ensure((size_t)std::max<int64_t>(nvox, 1) * 4));
     sm.cube_index = c->cubeidx.as<int>();
     launch_smooth(st, g, sm);
@@ -1911,12 +1915,12 @@ int nsgpu_set_vertex_normals(nsgpu_ctx *c, int on) {
     return NSGPU_OK;
 }
 
-int nsgpu_read_normals(nsgpu_ctx *c, double *normals) {
-    if (!c || !normals) return NSGPU_E_ARG;
-    if (!c->have_mesh) return fail(c, NSGPU_E_STATE, "no mesh computed");
+}  // extern "C"
+
+// the vertex normals of the current mesh into c->normals
+static int compute_normals(nsgpu_ctx *c) {
     if (!c->hits_have_patches)
         return fail(c, NSGPU_E_STATE, "vertex normals need nsgpu_set_vertex_normals(ctx, 1) before get_surf");
-    HIP_TRY(c, hipSetDevice(c->device));
     const DevGrid g = dev_grid(c);
     DevRays R = dev_rays(c);
     R.pid = c->rpid.as<int>();
@@ -1949,9 +1953,62 @@ int nsgpu_read_normals(nsgpu_ctx *c, double *normals) {
     a.normals = c->normals.as<double>();
     launch_normals(c->stream, g, P, R, a);
     HIP_TRY(c, hipGetLastError());
+    return NSGPU_OK;
+}
+
+extern "C" {
+
+int nsgpu_read_normals(nsgpu_ctx *c, double *normals) {
+    if (!c || !normals) return NSGPU_E_ARG;
+    if (!c->have_mesh) return fail(c, NSGPU_E_STATE, "no mesh computed");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (!c->normals_smoothed) {
+        const int rc = compute_normals(c);
+        if (rc) return rc;
+    }
     if (c->nverts)
         HIP_TRY(c, hipMemcpyAsync(normals, c->normals.p, (size_t)c->nverts * 24, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return NSGPU_OK;
+}
+
+int nsgpu_smooth_surface(nsgpu_ctx *c) {
+    if (!c) return NSGPU_E_ARG;
+    if (!c->have_mesh) return fail(c, NSGPU_E_STATE, "no mesh computed");
+    HIP_TRY(c, hipSetDevice(c->device));
+    // normals are smoothed with the vertices when the context keeps them
+    const bool smooth_normals = c->hits_have_patches;
+    const int64_t nv = c->nverts, nvox = nvox_of(c);
+    if (smooth_normals && !c->normals_smoothed) {
+        const int rc = compute_normals(c);
+        if (rc) return rc;
+    }
+    HIP_TRY(c, c->mraw.ensure((size_t)std::max<int64_t>(nv, 1) * 24));
+    HIP_TRY(c, c->cubeidx.ensure((size_t)std::max<int64_t>(nvox, 1) * 4));
+    if (smooth_normals) HIP_TRY(c, c->vorig.ensure((size_t)std::max<int64_t>(nv, 1) * 24));
+    SmoothArgs sm;
+    sm.vin = c->verts.as<double>();
+    sm.vslot = c->vslot.as<int64_t>();
+    sm.nv = nv;
+    sm.anode = c->anode.as<int>();
+    sm.ant = c->ant.as<int>();
+    sm.atoff = c->atoff.as<int>();
+    sm.tris = c->tris.as<int>();
+    sm.nact = (int)c->nact;
+    sm.vout = c->mraw.as<double>();
+    sm.nin = smooth_normals ? c->normals.as<double>() : nullptr;
+    sm.nout = smooth_normals ? c->vorig.as<double>() : nullptr;
+    sm.cube_index = c->cubeidx.as<int>();
+    launch_smooth(c->stream, dev_grid(c), sm);
+    HIP_TRY(c, hipGetLastError());
+    if (nv) {
+        HIP_TRY(c, hipMemcpyAsync(c->verts.p, c->mraw.p, (size_t)nv * 24, hipMemcpyDeviceToDevice, c->stream));
+        if (smooth_normals)
+            HIP_TRY(c, hipMemcpyAsync(c->normals.p, c->vorig.p, (size_t)nv * 24, hipMemcpyDeviceToDevice, c->stream));
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    // the mid-point provenance in vorig is spent: normals are final from here
+    if (smooth_normals) c->normals_smoothed = true;
     return NSGPU_OK;
 }
 

@@ -207,6 +207,9 @@ struct SmoothArgs {
     const int *anode, *ant, *atoff, *tris;  // active cubes, their triangle counts and offsets, triangles
     int nact;
     double *vout;
+    // vertex normals smoothed with the vertices (null: none)
+    const double *nin;
+    double *nout;
     // grid-sized cube -> active index map (launch_smooth fills it; an entry is
     // trusted only where anode confirms it, so the map is never cleared)
     int *cube_index;

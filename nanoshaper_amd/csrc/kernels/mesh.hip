@@ -209,7 +209,7 @@ __global__ void __launch_bounds__(256) k_smooth(DevGrid g, SmoothArgs a) {
     if (dir == 0) { cubes[0] = f - sy - sz; cubes[1] = f - sz; cubes[2] = f - sy; cubes[3] = f; }
     else if (dir == 1) { cubes[0] = f - 1 - sz; cubes[1] = f - sz; cubes[2] = f - 1; cubes[3] = f; }
     else { cubes[0] = f - 1 - sy; cubes[1] = f - sy; cubes[2] = f - 1; cubes[3] = f; }
-    double s0 = 0., s1 = 0., s2 = 0.;
+    double s0 = 0., s1 = 0., s2 = 0., n0 = 0., n1 = 0., n2 = 0.;
     int c = 0;
     for (int q = 0; q < 4; ++q) {
         if (cubes[q] < 0) continue;
@@ -229,21 +229,39 @@ __global__ void __launch_bounds__(256) k_smooth(DevGrid g, SmoothArgs a) {
             s0 += A[0] + B[0];
             s1 += A[1] + B[1];
             s2 += A[2] + B[2];
+            if (a.nin) {
+                const double *NA = a.nin + 3 * (int64_t)oa, *NB = a.nin + 3 * (int64_t)ob;
+                n0 += NA[0] + NB[0];
+                n1 += NA[1] + NB[1];
+                n2 += NA[2] + NB[2];
+            }
             ++c;
         }
     }
     const double *V = a.vin + 3 * v;
     if (c == 0) {
-        // no triangle names the vertex: it keeps its position
+        // no triangle names the vertex: it keeps its position and normal
         a.vout[3 * v + 0] = V[0];
         a.vout[3 * v + 1] = V[1];
         a.vout[3 * v + 2] = V[2];
+        if (a.nin)
+            for (int k = 0; k < 3; ++k) a.nout[3 * v + k] = a.nin[3 * v + k];
         return;
     }
     const double scale = 1.0 / (double)(2 * c);
     a.vout[3 * v + 0] = 0.5 * (V[0] + s0 * scale);
     a.vout[3 * v + 1] = 0.5 * (V[1] + s1 * scale);
     a.vout[3 * v + 2] = 0.5 * (V[2] + s2 * scale);
+    if (a.nin) {
+        // the normal plus the neighbours' average, normalised
+        // (Surface.cpp:13156-13168)
+        const double *Nv = a.nin + 3 * v;
+        const double t0 = Nv[0] + n0 * scale, t1 = Nv[1] + n1 * scale, t2 = Nv[2] + n2 * scale;
+        const double tt = 1.0 / sqrt(t0 * t0 + t1 * t1 + t2 * t2);
+        a.nout[3 * v + 0] = t0 * tt;
+        a.nout[3 * v + 1] = t1 * tt;
+        a.nout[3 * v + 2] = t2 * tt;
+    }
 }
 
 // "%.3f" (round half to even on the exact binary value, as printf) read back

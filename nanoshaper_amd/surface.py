@@ -384,6 +384,11 @@ class DeviceSurface:
         self._chk(self._lib.nsgpu_read_mesh(self._h, _ptr(v), _ptr(t)), "read_mesh")
         return v, t
 
+    def smooth_surface(self):
+        """Smooth_Mesh: one smoothing pass over the last mesh (and its normals
+        when vertex normals are on)."""
+        self._chk(self._lib.nsgpu_smooth_surface(self._h), "smooth_surface")
+
     def set_vertex_normals(self, on=True):
         """Compute_Vertex_Normals: keep what normals() needs from the next get_surf on."""
         self._chk(self._lib.nsgpu_set_vertex_normals(self._h, int(on)), "set_vertex_normals")
@@ -478,6 +483,50 @@ class NanoShaper:
     def get_grid_size(self):
         g = self.grid
         return g.nx, g.ny, g.nz
+
+    def get_grid_point_coordinates(self, ix, iy, iz):
+        """(x, y, z) of node (ix, iy, iz) (NS::NanoShaper::getGridPointCoordinates)."""
+        g = self.grid
+        return g.xmin + ix * g.side, g.ymin + iy * g.side, g.zmin + iz * g.side
+
+    def get_nearest_grid_point(self, p):
+        """(indices, saturated): the nearest node, clamped to the grid; saturated
+        when p lies outside it (NS::NanoShaper::getNearestGridPoint)."""
+        g = self.grid
+        out = False
+        idx = []
+        for d, (lo, n) in enumerate(((g.xmin, g.nx), (g.ymin, g.ny), (g.zmin, g.nz))):
+            i = int(np.floor((p[d] - lo) / g.side + 0.5))
+            if i < 0 or i >= n:
+                out = True
+                i = min(max(i, 0), n - 1)
+            idx.append(i)
+        return tuple(idx), out
+
+    def set_direction(self, direction):
+        """NS::NanoShaper::setDirection: 0 x, 1 y, 2 z."""
+        if direction not in (0, 1, 2) or self.surface is None:
+            return False
+        self.direction = direction
+        return True
+
+    def cast_axis_oriented_ray(self, pa, coord, direction, compute_normals=False):
+        """NS::NanoShaper::castAxisOrientedRay: (status, [(coordinate, normal or
+        None), ...]); status -1 for a wrong direction, a non-progressive ray or
+        no intersection, +1 otherwise."""
+        if getattr(self, "direction", None) != direction or pa[direction] >= coord or self.surface is None:
+            return -1, []
+        maxh = 64
+        while True:
+            res = self.surface.cast_rays(direction, [pa], coord, max_hits=maxh, normals=compute_normals)
+            cnt = int(res[0][0])
+            if cnt > maxh:
+                maxh = cnt
+                continue
+            if cnt <= 0:
+                return -1, []
+            nrm = res[2][0] if compute_normals else [None] * cnt
+            return 1, [(float(res[1][0, q]), nrm[q]) for q in range(cnt)]
 
 
 class Batch:

@@ -52,6 +52,7 @@ def load():
         "orc_is_feasible": (ctypes.c_int, [ctypes.POINTER(Patches), ctypes.c_int, VP]),
         "orc_set_root_refinement": (None, [ctypes.c_int]),
         "orc_set_threads": (None, [ctypes.c_int]),
+        "orc_smooth_surface": (ctypes.c_int, [VP, ctypes.c_int]),
         "orc_sturm_quartic": (ctypes.c_int, [VP, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
         "orc_create_mesh": (VP, [ctypes.POINTER(Grid), VP, ctypes.c_int64, VP, ctypes.c_int64, ctypes.c_int,
                                  ctypes.c_int]),
@@ -209,6 +210,9 @@ class OracleSurface:
         t = np.empty((m["num_triangles"], 3), np.int32)
         self._lib.orc_read_mesh(self._h, _ptr(v), _ptr(t))
         return v, t
+
+    def smooth_surface(self, normals=False):
+        self._lib.orc_smooth_surface(self._h, int(bool(normals)))
 
     def normals(self):
         out = np.empty((self.mesh_info["num_vertices"], 3))

@@ -2222,6 +2222,50 @@ int orc_blobby_build(const nsgpu_grid *g, const double *xyzr, int n, double B, d
     return 0;
 }
 
+/* Surface::smoothSurface (USE_OPTIMIZED_VERTICES_BUFFERING branch,
+ * Surface.cpp:13172-13215) on the last mesh: every vertex moves to half of
+ * itself plus the average of its triangles' other vertices; with normals,
+ * each normal becomes the normalised sum of itself and its neighbours' average. */
+int orc_smooth_surface(orc_surface *s, int normals) {
+    const int64_t nv = s->nverts, nt = s->ntris;
+    if (!s->verts && nv) return -1;
+    double *temp = (double *)xcalloc(nv * 3, sizeof(double));
+    double *tempN = normals ? (double *)xcalloc(nv * 3, sizeof(double)) : NULL;
+    int *num = (int *)xcalloc(nv, sizeof(int));
+    const double *V = s->verts, *Nm = s->normals;
+    for (int64_t i = 0; i < nt; i++) {
+        int v1 = s->tris[i * 3 + 0], v2 = s->tris[i * 3 + 1], v3 = s->tris[i * 3 + 2];
+        for (int q = 0; q < 3; q++) {
+            temp[v1 * 3 + q] += V[v2 * 3 + q] + V[v3 * 3 + q];
+            temp[v2 * 3 + q] += V[v1 * 3 + q] + V[v3 * 3 + q];
+            temp[v3 * 3 + q] += V[v1 * 3 + q] + V[v2 * 3 + q];
+            if (tempN) {
+                tempN[v1 * 3 + q] += Nm[v2 * 3 + q] + Nm[v3 * 3 + q];
+                tempN[v2 * 3 + q] += Nm[v1 * 3 + q] + Nm[v3 * 3 + q];
+                tempN[v3 * 3 + q] += Nm[v1 * 3 + q] + Nm[v2 * 3 + q];
+            }
+        }
+        num[v1] += 2;
+        num[v2] += 2;
+        num[v3] += 2;
+    }
+    for (int64_t i = 0; i < nv; i++) {
+        if (num[i] == 0) continue; /* no triangle names it: it keeps its position */
+        double scale = 1.0 / (double)num[i];
+        for (int q = 0; q < 3; q++) s->verts[i * 3 + q] = 0.5 * (s->verts[i * 3 + q] + temp[i * 3 + q] * scale);
+        if (tempN) {
+            for (int q = 0; q < 3; q++) tempN[i * 3 + q] = s->normals[i * 3 + q] + tempN[i * 3 + q] * scale;
+            double tt = 1.0 / sqrt(tempN[i * 3 + 0] * tempN[i * 3 + 0] + tempN[i * 3 + 1] * tempN[i * 3 + 1] +
+                                   tempN[i * 3 + 2] * tempN[i * 3 + 2]);
+            for (int q = 0; q < 3; q++) s->normals[i * 3 + q] = tempN[i * 3 + q] * tt;
+        }
+    }
+    free(temp);
+    free(tempN);
+    free(num);
+    return 0;
+}
+
 void orc_free_buf(void *p) { free(p); }
 /* ------------------------------------------------------------------------- */
 /* read back                                                                 */

@@ -48,9 +48,108 @@ static int mesh_modes(int argc, char **argv) {
     return 0;
 }
 
+// "facade": the NS::NanoShaper API (nanoshaper.h) over the device path
+static int facade_mode() {
+    std::vector<double> xyzr(330 * 4);
+    double radius_used = 0;
+    const int n = nsgpu_generate_chain(330, 12.0, 0, xyzr.data(), &radius_used);
+    std::vector<nsgpu::Atom> atoms;
+    for (int i = 0; i < n; ++i) atoms.emplace_back(xyzr[4 * i], xyzr[4 * i + 1], xyzr[4 * i + 2], xyzr[4 * i + 3]);
+    nsgpu::NanoShaper ns(atoms, nsgpu::ses, 1.4, 0.0, 1);
+    ns.setConfig<double>("Grid_scale", 2.0);
+    ns.setConfig<double>("Grid_perfil", 90.0);
+    ns.setConfig<bool>("Build_epsilon_maps", true);
+    ns.setConfig<bool>("Accurate_Triangulation", true);
+    ns.setConfig<bool>("Compute_Vertex_Normals", true);
+    ns.setConfig<bool>("Smooth_Mesh", false);
+    if (ns.readConfig<double>("Probe_Radius") != 1.4) return 20;
+    if (!ns.buildAnalyticalSurface()) return 21;
+    double vol = 0, area = 0;
+    ns.setCollectGridRays(true);
+    if (!ns.colourGrid(&vol)) return 22;
+    if (!ns.triangulate(&area)) return 23;
+    std::vector<unsigned> size, idx;
+    if (!ns.getGridSize(size) || size.size() != 3) return 24;
+    std::vector<double> p0;
+    if (!ns.getGridPointCoordinates(size[0] / 2, size[1] / 2, size[2] / 2, p0)) return 25;
+    double q[3] = {p0[0] + 0.1, p0[1] - 0.1, p0[2]};
+    if (ns.getNearestGridPoint(q, idx) || idx[0] != size[0] / 2 || idx[1] != size[1] / 2) return 26;
+    double far[3] = {-1e4, p0[1], p0[2]};
+    if (!ns.getNearestGridPoint(far, idx) || idx[0] != 0) return 27;  // saturated
+    std::vector<double> c0;
+    ns.getGridPointCoordinates(0, size[1] / 2, size[2] / 2, c0);
+    std::vector<double> cN;
+    ns.getGridPointCoordinates(size[0] - 1, 0, 0, cN);
+    // a ray along x through the middle of the grid, before and after setDirection
+    double pa[3] = {c0[0], c0[1], c0[2]};
+    std::vector<std::pair<double, double *>> hits;
+    if (ns.castAxisOrientedRay(pa, cN[0], hits, 0, true) != -1) return 28;  // no direction set yet
+    if (!ns.setDirection(0)) return 29;
+    if (ns.castAxisOrientedRay(pa, pa[0] - 1.0, hits, 0, true) != -1) return 30;  // not progressive
+    if (ns.castAxisOrientedRay(pa, cN[0], hits, 0, true) != 1 || hits.empty()) return 31;
+    for (auto &h : hits) {
+        const double *nv = h.second;
+        if (!nv || std::fabs(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2] - 1.0) > 1e-9) return 32;
+    }
+    const int status = ns.getGridPointStatus(size[0] / 2, size[1] / 2, size[2] / 2);
+    size_t crossings = 0;
+    for (auto &kv : ns.getRaysMap()[0]) crossings += kv.second.inters.size();
+    for (auto &kv : ns.getRaysMap()[1]) crossings += kv.second.inters.size();
+    for (auto &kv : ns.getRaysMap()[2]) crossings += kv.second.inters.size();
+    std::printf("facade volume %.9f area %.9f hits %zu first %.12f status %d rays %zu crossings %zu\n", vol, area,
+                hits.size(), hits[0].first, status,
+                ns.getRaysMap()[0].size() + ns.getRaysMap()[1].size() + ns.getRaysMap()[2].size(), crossings);
+    return 0;
+}
+
+// "prm FILE": a configuration file as the CLI's normal mode runs it
+static int prm_mode(const char *prm) {
+    nsgpu::DelPhiShared dg;
+    std::unique_ptr<nsgpu::Surface> surf;
+    const nsgpu::NormalModeResult r = nsgpu::runConfiguration(prm, dg, surf);
+    if (!r.ok) {
+        std::printf("error %s\n", r.error.c_str());
+        return 40;
+    }
+    long inside = 0;
+    for (int v : dg.status) inside += (v == -1);
+    std::printf("prm volume %.9f area %.9f triangles %lld inside %ld atoms %zu\n", r.volume, r.area,
+                (long long)surf->getNumTriangles(), inside, dg.atoms.size() / 4);
+    return 0;
+}
+
+// "rescale": getSurf, then a grid built at another scale and getSurf again
+// on the same surface (the maps follow the new grid)
+static int rescale_mode() {
+    std::vector<double> xyzr(200 * 4);
+    double radius_used = 0;
+    const int n = nsgpu_generate_chain(200, 10.0, 1, xyzr.data(), &radius_used);
+    xyzr.resize(n * 4);
+    nsgpu::ConfigFile cf;
+    cf.parse_line("Surface = ses");
+    cf.parse_line("Accurate_Triangulation = false");
+    nsgpu::DelPhiShared ds(true, true);
+    ds.loadAtoms(xyzr);
+    std::unique_ptr<nsgpu::Surface> surf(nsgpu::surfaceFactory().create(&cf, &ds));
+    if (!surf->build()) return 50;
+    double v1 = 0, v2 = 0, v3 = 0;
+    if (!ds.buildGrid(2.0, 80.0) || !surf->getSurf(&v1, true, true, 11.4)) return 51;
+    const size_t n1 = ds.status.size();
+    if (!ds.buildGrid(3.0, 80.0) || !surf->getSurf(&v2, true, true, 11.4)) return 52;
+    if (ds.status.size() != (size_t)ds.numNodes() || ds.status.size() <= n1) return 53;
+    // a fresh surface on the finer grid gives the same volume
+    std::unique_ptr<nsgpu::Surface> fresh(nsgpu::surfaceFactory().create(&cf, &ds));
+    if (!fresh->build() || !fresh->getSurf(&v3, true, true, 11.4)) return 54;
+    std::printf("rescale %.9f %.9f %.9f nodes %zu %zu\n", v1, v2, v3, n1, ds.status.size());
+    return v2 == v3 ? 0 : 55;
+}
+
 int main(int argc, char **argv) {
     if (argc > 1 && (std::strcmp(argv[1], "blobby") == 0 || std::strcmp(argv[1], "mesh") == 0))
         return mesh_modes(argc, argv);
+    if (argc > 1 && std::strcmp(argv[1], "facade") == 0) return facade_mode();
+    if (argc > 2 && std::strcmp(argv[1], "prm") == 0) return prm_mode(argv[2]);
+    if (argc > 1 && std::strcmp(argv[1], "rescale") == 0) return rescale_mode();
     const bool gpu = argc > 1 && std::strcmp(argv[1], "gpu") == 0;
     nsgpu::ConfigFile cf;
     cf.parse_line("Surface = ses   # surface type");

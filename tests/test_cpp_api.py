@@ -73,3 +73,55 @@ def test_cpp_client_blobby_and_mesh(client, device_ok, tmp_path):
     out = subprocess.run([client, "mesh", off], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout + out.stderr
     assert float(out.stdout.split()[1]) == pytest.approx(r["volume"], rel=1e-12)
+
+
+@pytest.mark.gpu
+def test_cpp_facade(client, device_ok):
+    """NS::NanoShaper over the device: volume and area equal the device path's,
+    castAxisOrientedRay returns the grid line's depths with unit normals, and
+    the collected grid-ray crossings are the kept segment ends of all panels."""
+    import numpy as np
+
+    import nanoshaper_amd as ns
+    out = subprocess.run([client, "facade"], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    f = out.stdout.split()
+    vol, area, nhits, first, rays, crossings = float(f[2]), float(f[4]), int(f[6]), float(f[8]), int(f[12]), int(f[14])
+    xyzr, _ = ns.generate_chain(330, 12.0, seed=0)
+    g = ns.build_grid(xyzr, 2.0, 90.0)
+    jit, _ = ns.jitter(xyzr)
+    dev = ns.DeviceSurface(0)
+    dev.set_grid(g)
+    dev.set_patches(ns.ses_patches(jit, 1.4))
+    r = dev.get_surf(build_epsmap=True, build_status=True, accurate_triangulation=True, fill_cavities=False)
+    m = dev.triangulate()
+    assert vol == pytest.approx(r["volume"], rel=1e-12) and area == pytest.approx(m["area"], rel=1e-12)
+    xs, ys, zs = ns.grid_coords(g)
+    counts, hits = dev.cast_rays(0, [[xs[0], ys[g.ny // 2], zs[g.nz // 2]]], xs[-1])
+    assert counts[0] == nhits and float("%.12f" % hits[0, 0]) == pytest.approx(first, abs=1e-12)
+    kept = 0
+    for slot in range(3):
+        cnt, t = dev.ray_hits(slot)
+        for ray in np.nonzero((cnt > 0) & (cnt % 2 == 0))[0]:
+            T = t[ray, :cnt[ray]]
+            kept += 2 * int((np.abs(T[1::2] - T[0::2]) >= 1e-8).sum())
+    assert crossings == kept and rays == 3 * g.nx * g.ny
+
+
+@pytest.mark.gpu
+def test_cpp_configuration_run(client, device_ok):
+    from nanoshaper_amd import config
+    prm = os.path.join(ROOT, "tests", "golden", "1crn_ses_conf.prm")
+    out = subprocess.run([client, "prm", prm], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    f = out.stdout.split()
+    run = config.normal_mode(prm)
+    assert float(f[2]) == pytest.approx(run["result"]["volume"], rel=1e-12)
+    assert float(f[4]) == pytest.approx(run["mesh"]["area"], rel=1e-12)
+    assert int(f[6]) == run["mesh"]["num_triangles"] and int(f[10]) == 327
+
+
+@pytest.mark.gpu
+def test_cpp_surface_follows_a_rebuilt_grid(client, device_ok):
+    out = subprocess.run([client, "rescale"], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr

@@ -198,3 +198,21 @@ def test_cast_rays_off_grid_cross_the_exact_ses(device_ok):
                 found += 1
     assert found > 20
     dev.close()
+
+
+def test_python_facade_cast_and_grid_queries(device_ok):
+    xyzr, _ = ns.generate_chain(200, 10.0, seed=6)
+    shaper = ns.NanoShaper(xyzr, "ses", scale=2.0, perfil=80.0)
+    shaper.build_analytical_surface()
+    shaper.colour_grid()
+    g = shaper.grid
+    pa = list(shaper.get_grid_point_coordinates(0, g.ny // 2, g.nz // 2))
+    assert shaper.cast_axis_oriented_ray(pa, pa[0] + 40.0, 0)[0] == -1  # no direction set
+    assert shaper.set_direction(0)
+    assert shaper.cast_axis_oriented_ray(pa, pa[0] - 1.0, 0)[0] == -1  # not progressive
+    status, hits = shaper.cast_axis_oriented_ray(pa, pa[0] + 40.0, 0, compute_normals=True)
+    assert status == 1 and len(hits) % 2 == 0
+    assert all(abs(np.linalg.norm(n) - 1) < 1e-9 for _, n in hits)
+    idx, sat = shaper.get_nearest_grid_point([pa[0] + 0.1, pa[1] - 0.2, pa[2]])
+    assert idx == (0, g.ny // 2, g.nz // 2) and not sat
+    assert shaper.get_nearest_grid_point([-1e4, 0, 0])[1]

@@ -252,3 +252,24 @@ def test_sturm_roots_match_oracle(device_ok, n, radius, scale, perfil, seed):
     # bisection, restated in the kernel and in the oracle
     grid, ps, dev, orc = _surfaces(n, radius, scale, perfil, seed)
     _compare(grid, dev, orc, dict(precision=2))
+
+
+@pytest.mark.parametrize("normals", [False, True])
+def test_smooth_surface_matches_oracle(device_ok, normals):
+    """Smooth_Mesh on the analytic mesh (smoothSurface): the device's pass
+    equals the oracle's restatement; with normals, the smoothed normals too."""
+    from oracle.binding import OracleSurface
+    grid, ps, dev, orc = _surfaces(800, 14.0, 3.0, 80.0, seed=4)
+    dev.set_vertex_normals(normals)
+    dev.get_surf()
+    dev.triangulate()
+    orc.get_surf()
+    orc.triangulate()
+    dev.smooth_surface()
+    orc.smooth_surface(normals)
+    va, ta = dev.mesh()
+    vb, tb = orc.mesh()
+    assert np.array_equal(ta, tb)
+    assert np.abs(va - vb).max() < 1e-9
+    if normals:
+        assert np.abs(dev.normals() - orc.normals()).max() < 1e-9
