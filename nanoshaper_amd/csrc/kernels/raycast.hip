@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <type_traits>
 
+#include "fastmath.hpp"
 #include "launch.hpp"
 
 namespace nsg {
@@ -35,7 +36,9 @@ __device__ __forceinline__ bool ray_sphere(const double o[3], const double dir[3
     return true;
 }
 
-// realCubicSolution, tools.cpp:350-393
+// realCubicSolution, tools.cpp:350-393.  The cube roots and cos(acos(.)/3)
+// come from fastmath.hpp (FP32 seeds refined in FP64 to within about one unit
+// in the last place, as the FP64 library functions are).
 __device__ double cubic_root(double b, double c, double d) {
     const double p = -(1.0 / 3.0) * b * b + c;
     const double q = (2. / 27.) * b * b * b - (1. / 3.) * b * c + d;
@@ -50,12 +53,11 @@ __device__ double cubic_root(double b, double c, double d) {
     }
     if (D3 >= 0.) {
         const double radD3 = sqrt(D3);
-        const double A = cbrt(radD3 - 0.5 * q);
-        const double psu3A = cbrt(radD3 + 0.5 * q);
+        const double A = cbrt_fast(radD3 - 0.5 * q);
+        const double psu3A = cbrt_fast(radD3 + 0.5 * q);
         return A - psu3A - (1. / 3.) * b;
     }
-    const double Aarg = acos(-0.5 * q / sqrt(-D3p));
-    return 2. * sqrt(-(1. / 3.) * p) * cos((1. / 3.) * Aarg) - (1. / 3.) * b;
+    return 2. * sqrt(-(1. / 3.) * p) * cos_third_acos(-0.5 * q / sqrt(-D3p)) - (1. / 3.) * b;
 }
 
 // quarticEqSolutions, tools.cpp:525-641: Ferrari's depressed quartic with the
@@ -150,6 +152,38 @@ __device__ __forceinline__ int ray_torus(const double *td, const double sc[3], d
         if (dd < radius * radius) t[ni++] = t0 + roots[i] / ray_dir;
     }
     return ni;
+}
+
+// A chord of the clipping sphere that stays farther than R + r from the torus
+// axis, or (ring torus, R > r) nearer than R - r, lies outside the solid torus,
+// so the torus quartic has no root on it and rayTorus finds nothing there; the
+// walk drops such rays before the solve.  Along the ray (x = pa + s e_vc) the
+// squared distance to the axis is the convex quadratic A s^2 + B s + C; the
+// chord is widened and the radii padded by kTorusCullPad (A), far beyond the
+// error of any computed root, and the comparisons are made without division.
+constexpr double kTorusCullPad = 1e-3;
+
+template <int VC>
+__device__ __forceinline__ bool torus_far(const double *td, const double pa[3], double s0, double s1, double probe) {
+    const double u[3] = {pa[0] - td[0], pa[1] - td[1], pa[2] - td[2]};
+    const double w[3] = {td[9], td[10], td[11]};  // third row of the inverse frame: the axis
+    const double aw = u[0] * w[0] + u[1] * w[1] + u[2] * w[2];
+    const double A = 1. - w[VC] * w[VC];
+    const double B = 2. * (u[VC] - aw * w[VC]);
+    const double C = u[0] * u[0] + u[1] * u[1] + u[2] * u[2] - aw * aw;
+    const double lo = s0 - kTorusCullPad, hi = s1 + kTorusCullPad;
+    const double rlo = (A * lo + B) * lo + C, rhi = (A * hi + B) * hi + C;
+    const double R = td[12];
+    const double Ro = R + probe + kTorusCullPad;
+    double rmin = fmin(rlo, rhi);
+    bool far_out;
+    if (A > 1e-9 && -B >= 2. * A * lo && -B <= 2. * A * hi)  // vertex inside the chord
+        far_out = 4. * A * C - B * B > 4. * A * (Ro * Ro);
+    else
+        far_out = rmin > Ro * Ro;
+    if (far_out) return true;
+    const double Ri = R - probe - kTorusCullPad;
+    return Ri > 0. && fmax(rlo, rhi) < Ri * Ri;
 }
 
 // ConnollySurface::isFeasible, ConnollySurface.cpp:4769-5021, on the SoA form
@@ -807,6 +841,7 @@ __global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRa
                         if (kCount) ++pc[PC_DISC];
                         act = ray_sphere(pa, dir, sc, radius, t0, t1);
                         if (kCount && act) ++pc[PC_SPHERE];
+                        if (kTorus && act) act = !torus_far<vc>(pd, pa, t0 * ray_dir, t1 * ray_dir, probe);
                     }
                 }
             }
