@@ -50,7 +50,7 @@ OPS = {"candidates": 9, "disc_pass": 23, "sphere_hits": 5, "torus_quartics": 160
        "feas_items": 6}
 
 # the two launches of the intersection stage (spheres and facets; tori)
-INTERSECT_KERNELS = ("nsg::k_intersect<false, 0, false>", "nsg::k_intersect<false, 0, true>")
+INTERSECT_KERNELS = ("nsg::k_intersect_plain<false>", "nsg::k_intersect_torus<false, 0>")
 
 CONFIGS = {
     # name: atoms, confinement radius, grid scale, perfil

@@ -708,16 +708,16 @@ enum { PC_CAND, PC_DISC, PC_SPHERE, PC_TORUS, PC_ROOTS, PC_HITS, PC_FEAS_ITEMS, 
 // Per-workgroup counts go to the workgroup's own row of 'part' (zeroed before
 // the launch) and are summed by k_reduce_cols: one atomic per workgroup on a
 // shared address would serialise 10^5 workgroups at the L2.
-// kTorus selects the class of patches (P.order): the plain launch covers
-// spheres and facets, the torus launch the tori.  kRoots: torus roots from the
-// closed form (kRootsClosed), the mixed-precision solver (kRootsMixed) or Sturm
-// bisection (kRootsSturm).  kCount: instead of hits, per-workgroup path counts.
-template <bool kCount, int kRoots, bool kTorus>
-__global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRays R, double probe,
+// The tori of P.order (spheres and facets run in k_intersect_plain).  kRoots:
+// torus roots from the closed form (kRootsClosed), the mixed-precision solver
+// (kRootsMixed) or Sturm bisection (kRootsSturm).  kCount: instead of hits,
+// per-workgroup path counts.
+template <bool kCount, int kRoots>
+__global__ void __launch_bounds__(64) k_intersect_torus(DevGrid g, DevPatches P, DevRays R, double probe,
                                                   unsigned long long *part) {
     // workgroup = patch rank * nslots + slot: the patches of a class come
     // largest first, and every slot starts on them together
-    const int first = kTorus ? P.n_plain : 0;
+    const int first = P.n_plain;
     const int rank = (int)(blockIdx.x / R.nslots);
     const int slot = (int)(blockIdx.x % R.nslots);
     const int64_t wg = (int64_t)slot * P.n + first + rank;
@@ -778,34 +778,14 @@ __global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRa
             origin(m, n, pa);
             t[0] = t0;  // ray_sphere's roots from the cull pass
             t[1] = t1;
-            int ni = 2;
-            if (kTorus) {
-                if (kRoots == kRootsMixed)
-                    ni = ray_torus_mixed(pd, sc, probe, radius, vc, pa, ray_dir, t);
-                else if (kRoots == kRootsSturm)
-                    ni = ray_torus_sturm(pd, sc, probe, radius, vc, pa, ray_dir, t);
-                else
-                    ni = ray_torus(pd, sc, probe, radius, vc, pa, ray_dir, t);
-                if (kCount) ++pc[PC_TORUS];
-            }
-            if (!kTorus) {
-                // the sphere's two depths, tested together
-                double ip0[3] = {pa[0], pa[1], pa[2]}, ip1[3] = {pa[0], pa[1], pa[2]};
-                ip0[vc] += t[0] * ray_dir;
-                ip1[vc] += t[1] * ray_dir;
-                bool ok[2];
-                feasible_plain2(kind, pd, pcnt, ip0, ip1, ok[0], ok[1]);
-                for (int i = 0; i < 2; i++) {
-                    if (kCount) {
-                        ++pc[PC_ROOTS];
-                        ++pc[PC_FEAS_CALLS];
-                        pc[PC_FEAS_ITEMS] += pcnt;
-                        pc[PC_HITS] += ok[i];
-                    }
-                    if (!kCount && ok[i]) push_hit(R, slot, (int64_t)n * R.N + m, t[i], it);
-                }
-                return;
-            }
+            int ni;
+            if (kRoots == kRootsMixed)
+                ni = ray_torus_mixed(pd, sc, probe, radius, vc, pa, ray_dir, t);
+            else if (kRoots == kRootsSturm)
+                ni = ray_torus_sturm(pd, sc, probe, radius, vc, pa, ray_dir, t);
+            else
+                ni = ray_torus(pd, sc, probe, radius, vc, pa, ray_dir, t);
+            if (kCount) ++pc[PC_TORUS];
             for (int i = 0; i < ni; i++) {
                 double ip[3] = {pa[0], pa[1], pa[2]};
                 ip[vc] += t[i] * ray_dir;
@@ -841,7 +821,7 @@ __global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRa
                         if (kCount) ++pc[PC_DISC];
                         act = ray_sphere(pa, dir, sc, radius, t0, t1);
                         if (kCount && act) ++pc[PC_SPHERE];
-                        if (kTorus && act) act = !torus_far<vc>(pd, pa, t0 * ray_dir, t1 * ray_dir, probe);
+                        if (act) act = !torus_far<vc>(pd, pa, t0 * ray_dir, t1 * ray_dir, probe);
                     }
                 }
             }
@@ -890,6 +870,246 @@ __global__ void __launch_bounds__(64) k_intersect(DevGrid g, DevPatches P, DevRa
     if (threadIdx.x == 0 && cand) part[wg] = cand;
 }
 
+
+// Sphere and facet patches (P.order [0, n_plain)), one workgroup per patch
+// over every slot.  The walk over each slot's pixel rectangle (the culls of
+// k_intersect) appends the pixels that reach the sphere test to a ring queue
+// in LDS; a lane takes the next queued pixel as soon as its own is decided
+// and tests its two depths against the patch's items in order, eight at a
+// time (isFeasible, ConnollySurface.cpp:4769-5021, the same FP64
+// expressions); the walk refills the queue whenever it holds less than a
+// wave's worth.  So the lanes stay busy until the patch's last pixel: held
+// until the slowest of a batch of 64 was decided (a pixel with a feasible
+// depth tests every item and nearly every batch holds one), and drained at
+// the end of every slot, the waves tested about 2.8 times the items needed.
+// Items are staged in LDS when a patch has at most kStageItems of them.
+constexpr int kRing = 128;  // a power of two, >= 2 * 64
+constexpr int kStageItems = 64;
+
+template <bool kCount>
+__global__ void __launch_bounds__(64) k_intersect_plain(DevGrid g, DevPatches P, DevRays R,
+                                                        unsigned long long *part) {
+    const int rank = (int)blockIdx.x;
+    const int it = P.order[rank];
+    const int kind = P.kind[it];
+    const double sc[3] = {P.bsph[4 * it], P.bsph[4 * it + 1], P.bsph[4 * it + 2]};
+    const double radius = P.bsph[4 * it + 3];
+    const double *pd = P.data + P.off[it];
+    const int pcnt = P.cnt[it];
+    const double down[3] = {sc[0] - radius, sc[1] - radius, sc[2] - radius};
+    const double up[3] = {sc[0] + radius, sc[1] + radius, sc[2] + radius};
+    if (R.nslots == 0) return;
+    // CELL_CULLING (ConnollySurface.cpp:4378-4391)
+    if (down[0] > g.xmax + kEpsInt || down[1] > g.ymax + kEpsInt || down[2] > g.zmax + kEpsInt) return;
+    const double lowpad = g.hside - kDelta + kEpsInt;
+    if (up[0] < g.xmin - lowpad || up[1] < g.ymin - lowpad || up[2] < g.zmin - lowpad) return;
+    const double gmin[3] = {g.xmin, g.ymin, g.zmin};
+    const int gn[3] = {g.nx, g.ny, g.nz};
+    int64_t is[3], ie[3];
+    for (int d = 0; d < 3; ++d) {
+        is[d] = (int64_t)rintp(fmax(0., down[d] - gmin[d]) * g.scale);
+        ie[d] = (int64_t)rintp(fmax(0., up[d] - gmin[d]) * g.scale);
+        if (ie[d] >= gn[d]) ie[d] = gn[d] - 1;
+    }
+    const int64_t pix[3] = {1 + ie[0] - is[0], 1 + ie[1] - is[1], 1 + ie[2] - is[2]};
+    const int lane = (int)threadIdx.x;
+    const unsigned long long below = (1ull << lane) - 1ull;
+
+    __shared__ double sitem[4 * kStageItems];
+    const bool staged = pcnt <= kStageItems;
+    if (staged) {
+        for (int x = lane; x < 4 * pcnt; x += 64) sitem[x] = pd[x];
+        __syncthreads();
+    }
+    __shared__ int qm[kRing], qn[kRing], qs[kRing];
+    __shared__ double qt0[kRing], qt1[kRing];  // the pixels' bounding-sphere depths
+    unsigned int cand = 0;
+    unsigned int pc[PC_N] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+    // walk cursor: slot, the slot's rectangle, and each lane's pixel in it
+    // (pixel rp = row * pf + col advances by 64 = step_r rows + step_c columns)
+    int wslot = 0, wpf = 1, step_r = 0, step_c = 0, col = 0, row = 0;
+    int64_t wbase = 0, wtotal = 0;
+    auto start_slot = [&](int s) {
+        const int panel = s < R.panels0 ? s : s - R.panels0;
+        const int fd = panel == 0 ? 1 : 0, ld = panel == 1 ? 1 : 2;
+        wpf = (int)pix[fd];
+        wtotal = pix[fd] * pix[ld];
+        wbase = 0;
+        step_r = 64 / wpf;
+        step_c = 64 % wpf;
+        col = lane % wpf;
+        row = lane / wpf;
+    };
+    bool walking = true;
+    start_slot(0);
+    int head = 0, tail = 0;  // ring positions (uniform); entries [head, tail) wait
+    // One walk step of 64 pixels; one instance per panel, so that every axis
+    // index is a constant.
+    auto step = [&](auto panel_c) {
+        constexpr int fd = PanelAxes<decltype(panel_c)::value>::fd;
+        constexpr int ld = PanelAxes<decltype(panel_c)::value>::ld;
+        constexpr int vc = PanelAxes<decltype(panel_c)::value>::vc;
+        const double *ga = vc == 0 ? g.x : (vc == 1 ? g.y : g.z);
+        const double *gf = fd == 0 ? g.x : g.y;
+        const double *gl = ld == 1 ? g.y : g.z;
+        const double delta = wslot < R.panels0 ? 0. : kDelta - g.hside;
+        double dir[3] = {0., 0., 0.};
+        dir[vc] = ga[gn[vc] - 1] - ga[0];
+        bool act = false;
+        double t0 = 0., t1 = 0.;
+        const int m = col + (int)is[fd], n = row + (int)is[ld];
+        if (wbase + lane < wtotal) {
+            double pa[3];
+            pa[vc] = ga[0] + delta;
+            pa[fd] = gf[m] + delta;
+            pa[ld] = gl[n] + delta;
+            // bounding box, EQ_CULLING (ConnollySurface.cpp:4496-4502), bounding sphere
+            if (!(pa[fd] < down[fd] || pa[fd] > up[fd] || pa[ld] < down[ld] || pa[ld] > up[ld])) {
+                ++cand;
+                const double dx = sc[fd] - pa[fd], dy = sc[ld] - pa[ld];
+                if (!(dx * dx + dy * dy > radius * radius)) {
+                    if (kCount) ++pc[PC_DISC];
+                    act = ray_sphere(pa, dir, sc, radius, t0, t1);
+                    if (kCount && act) ++pc[PC_SPHERE];
+                }
+            }
+        }
+        col += step_c;
+        row += step_r;
+        if (col >= wpf) {
+            col -= wpf;
+            ++row;
+        }
+        const unsigned long long mask = __ballot(act);
+        if (act) {
+            const int at = (tail + __popcll(mask & below)) & (kRing - 1);
+            qm[at] = m;
+            qn[at] = n;
+            qs[at] = wslot;
+            qt0[at] = t0;
+            qt1[at] = t1;
+        }
+        tail += __popcll(mask);
+        wbase += 64;
+        if (wbase >= wtotal) {
+            if (++wslot < R.nslots) start_slot(wslot);
+            else walking = false;
+        }
+    };
+    // four items against both depths; indices past the last item repeat it
+    // (a repeated test does not change an AND)
+    bool op = false, oq = false;
+    double ip0[3] = {0., 0., 0.}, ip1[3] = {0., 0., 0.};
+    auto test4 = [&](const double *items, int i) {
+        const double *a = items + 4 * i, *b = items + 4 * min(i + 1, pcnt - 1), *c = items + 4 * min(i + 2, pcnt - 1),
+                     *d = items + 4 * min(i + 3, pcnt - 1);
+        if (kind == NSGPU_PATCH_POINT) {
+            op |= in_clip_sphere(a, ip0) | in_clip_sphere(b, ip0) | in_clip_sphere(c, ip0) | in_clip_sphere(d, ip0);
+            oq |= in_clip_sphere(a, ip1) | in_clip_sphere(b, ip1) | in_clip_sphere(c, ip1) | in_clip_sphere(d, ip1);
+        } else {
+            op |= plane_out(a, ip0) | plane_out(b, ip0) | plane_out(c, ip0) | plane_out(d, ip0);
+            oq |= plane_out(a, ip1) | plane_out(b, ip1) | plane_out(c, ip1) | plane_out(d, ip1);
+        }
+    };
+    bool busy = false;
+    int item = 0, m = 0, n = 0, es = 0;
+    double t0 = 0., t1 = 0.;
+    for (;;) {
+        if (walking && tail - head < 64) {
+            const int panel = wslot < R.panels0 ? wslot : wslot - R.panels0;
+            if (panel == 0) step(std::integral_constant<int, 0>());
+            else if (panel == 1) step(std::integral_constant<int, 1>());
+            else step(std::integral_constant<int, 2>());
+            __syncthreads();
+            continue;
+        }
+        const unsigned long long fm = __ballot(!busy);
+        if (!busy) {
+            const int at = head + __popcll(fm & below);
+            if (at < tail) {
+                const int x = at & (kRing - 1);
+                m = qm[x];
+                n = qn[x];
+                es = qs[x];
+                t0 = qt0[x];
+                t1 = qt1[x];
+                // the pixel's origin and the two depths' points (as the walk's origin)
+                const int panel = es < R.panels0 ? es : es - R.panels0;
+                const double delta = es < R.panels0 ? 0. : kDelta - g.hside;
+                double pa[3], rd;
+                int vc;
+                if (panel == 0) {
+                    pa[0] = g.x[0] + delta;
+                    pa[1] = g.y[m] + delta;
+                    pa[2] = g.z[n] + delta;
+                    rd = g.x[g.nx - 1] - g.x[0];
+                    vc = 0;
+                } else if (panel == 1) {
+                    pa[0] = g.x[m] + delta;
+                    pa[1] = g.y[n] + delta;
+                    pa[2] = g.z[0] + delta;
+                    rd = g.z[g.nz - 1] - g.z[0];
+                    vc = 2;
+                } else {
+                    pa[0] = g.x[m] + delta;
+                    pa[1] = g.y[0] + delta;
+                    pa[2] = g.z[n] + delta;
+                    rd = g.y[g.ny - 1] - g.y[0];
+                    vc = 1;
+                }
+                for (int d = 0; d < 3; ++d) {
+                    ip0[d] = d == vc ? pa[d] + t0 * rd : pa[d];
+                    ip1[d] = d == vc ? pa[d] + t1 * rd : pa[d];
+                }
+                busy = true;
+                item = 0;
+                op = oq = false;
+            }
+        }
+        head = min(head + __popcll(fm), tail);
+        if (__ballot(busy) == 0) {
+            if (!walking) break;
+            continue;
+        }
+        if (!busy) continue;
+        if (item < pcnt) {  // eight items a step (278 vs 290 us with four at 335^3)
+            if (staged) {
+                test4(sitem, item);
+                if (item + 4 < pcnt) test4(sitem, item + 4);
+            } else {
+                test4(pd, item);
+                if (item + 4 < pcnt) test4(pd, item + 4);
+            }
+            item += 8;
+        }
+        if ((op & oq) || item >= pcnt) {
+            if (kCount) {
+                pc[PC_ROOTS] += 2;
+                pc[PC_FEAS_CALLS] += 2;
+                pc[PC_FEAS_ITEMS] += 2 * pcnt;
+                pc[PC_HITS] += (unsigned)!op + (unsigned)!oq;
+            } else {
+                const int64_t ray = (int64_t)n * R.N + m;
+                if (!op) push_hit(R, es, ray, t0, it);
+                if (!oq) push_hit(R, es, ray, t1, it);
+            }
+            busy = false;
+        }
+    }
+    // per-workgroup partials in the rows of slot 0
+    if (kCount) {
+        pc[PC_CAND] = cand;
+        for (int q = 0; q < PC_N; ++q) {
+            unsigned long long v = pc[q];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+            if (lane == 0 && v) part[(int64_t)rank * PC_N + q] = v;
+        }
+        return;
+    }
+    for (int off = 32; off > 0; off >>= 1) cand += __shfl_down(cand, off, 64);
+    if (lane == 0 && cand) part[rank] = cand;
+}
 
 // Writes one ray's coverage plane: node intervals arrive in ascending,
 // disjoint order (the segments of a sorted ray), and every one of the W words
@@ -1467,10 +1687,9 @@ static void launch_classes(hipStream_t s, const DevGrid &g, const DevPatches &P,
                            unsigned long long *part) {
     if (P.n == 0 || R.nslots == 0) return;
     if (P.n_plain > 0)
-        hipLaunchKernelGGL((k_intersect<kCount, kRoots, false>), dim3((unsigned)P.n_plain * (unsigned)R.nslots),
-                           dim3(64), 0, s, g, P, R, probe, part);
+        hipLaunchKernelGGL((k_intersect_plain<kCount>), dim3((unsigned)P.n_plain), dim3(64), 0, s, g, P, R, part);
     if (P.n - P.n_skin > P.n_plain)
-        hipLaunchKernelGGL((k_intersect<kCount, kRoots, true>),
+        hipLaunchKernelGGL((k_intersect_torus<kCount, kRoots>),
                            dim3((unsigned)(P.n - P.n_skin - P.n_plain) * (unsigned)R.nslots), dim3(64), 0, s, g,
                            P, R, probe, part);
     if (!kCount && P.n_skin > 0) launch_intersect_skin(s, g, P, R, part);
