@@ -133,10 +133,11 @@ struct nsgpu_ctx {
     bool cav_on_device = false;  // cav_size/cav_filled not yet copied from the device
     // vertices and mesh
     DBuf idx, vcount, voffs, scan_scratch, verts, tris, tarea;
-    // idx holds -1 everywhere except at the vslot[0 .. idx_dirty) slots written
-    // last, for grids of idx_nvox nodes (-1: state unknown, fill it)
+    // idx: vertex index slots for grids of idx_nvox nodes (-1: state unknown,
+    // fill it); the last pass tagged ids [0, idx_dirty) with idx_gen
     DBuf vslot;
     int64_t idx_nvox = -1, idx_dirty = 0;
+    uint32_t idx_gen = 0;
     // the analytic vertex count of the last assemble is still on the device
     // (dnv[0]); n_analytic, nverts and idx_dirty wait for resolve_nv
     DBuf dnv;
@@ -1030,6 +1031,21 @@ static int read_partial_sum(nsgpu_ctx *c, const double *partial, int64_t per, hi
     return NSGPU_OK;
 }
 
+// A vertex pass tags its ids with a generation above every id of the passes
+// before it instead of clearing their slots; the slots are refilled only when
+// the buffer or the grid changes or the generation would pass 2^31 - 1.
+static void idx_begin(nsgpu_ctx *c, hipStream_t st, int64_t nvox, const void *old_idx) {
+    const uint64_t next = (uint64_t)c->idx_gen + (uint64_t)c->idx_dirty;
+    if (c->idx.p != old_idx || c->idx_nvox != nvox || next >= 0x7fffffffull) {
+        launch_fill32(st, c->idx.p, 0xffffffffu, nvox * 3);
+        c->idx_gen = 0;
+    } else {
+        c->idx_gen = (uint32_t)next;
+    }
+    c->idx_nvox = nvox;
+    c->idx_dirty = 0;
+}
+
 static int assemble(nsgpu_ctx *c, const int eslots[3]) {
     const DevGrid g = dev_grid(c);
     const DevRays R = dev_rays(c);
@@ -1043,14 +1059,7 @@ static int assemble(nsgpu_ctx *c, const int eslots[3]) {
     HIP_TRY(c, c->vcount.ensure((size_t)nmax * 4));
     HIP_TRY(c, c->voffs.ensure((size_t)nmax * 4));
     HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(nmax) * 4));
-    // vertex index slots back to -1: only the ones the last vertices used,
-    // unless the buffer is new or the grid changed
-    if (c->idx.p != old_idx || c->idx_nvox != nvox)
-        launch_fill32(c->stream, c->idx.p, 0xffffffffu, nvox * 3);
-    else
-        launch_idx_reset(c->stream, c->idx.as<int>(), c->vslot.as<int64_t>(), c->idx_dirty);
-    c->idx_nvox = nvox;
-    c->idx_dirty = 0;
+    idx_begin(c, c->stream, nvox, old_idx);
     launch_vtx_count(c->stream, g, R, eslots, c->bits.as<uint32_t>(), c->vcount.as<int>());
     HIP_TRY(c, c->dnv.ensure(8));
     launch_scan(c->stream, c->vcount.as<int>(), nrays, c->voffs.as<int>(), c->scan_scratch.as<int>(),
@@ -1066,7 +1075,7 @@ static int assemble(nsgpu_ctx *c, const int eslots[3]) {
     HIP_TRY(c, c->vslot.ensure((size_t)cap * 8));
     HIP_TRY(c, c->vhit.ensure((size_t)cap * 8));
     launch_vtx_write(c->stream, g, R, eslots, c->bits.as<uint32_t>(), c->voffs.as<int>(), c->verts.as<double>(),
-                     c->idx.as<int>(), c->vslot.as<int64_t>(), c->vhit.as<int64_t>(), cap);
+                     c->idx.as<int>(), c->idx_gen, c->vslot.as<int64_t>(), c->vhit.as<int64_t>(), cap);
     HIP_TRY(c, hipGetLastError());
     c->idx_dirty = 0;
     c->nv_cap = cap;
@@ -1366,7 +1375,7 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
     // 2. per active cube: mid-point vertices added and triangles emitted
     HIP_TRY(c, c->aedge.ensure(na * 48));
     launch_mc_counts(c->stream, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->idx.as<int>(),
-                     c->anv.as<int>(), c->ant.as<int>(), c->aedge.as<int>());
+                     c->idx_gen, c->anv.as<int>(), c->ant.as<int>(), c->aedge.as<int>());
     launch_scan(c->stream, c->anv.as<int>(), nact, c->avoff.as<int>(), c->scan_scratch.as<int>());
     launch_scan(c->stream, c->ant.as<int>(), nact, c->atoff.as<int>(), c->scan_scratch.as<int>());
     HIP_TRY(c, hipGetLastError());
@@ -1398,13 +1407,13 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
     // 3. write the mid points, then the triangles that may reference them
     HIP_TRY(c, c->vorig.ensure((size_t)std::max<int64_t>(nadd_cap, 1) * 4));
     launch_mc_vwrite(c->stream, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->aedge.as<int>(),
-                     c->idx.as<int>(), c->avoff.as<int>(), (int)c->n_analytic, c->verts.as<double>(),
+                     c->idx.as<int>(), c->idx_gen, c->avoff.as<int>(), (int)c->n_analytic, c->verts.as<double>(),
                      c->vslot.as<int64_t>(), c->vorig.as<int>());
     c->nact = nact;
     HIP_TRY(c, c->tris.ensure((size_t)std::max<int64_t>(nt_cap, 1) * 12));
     HIP_TRY(c, c->tarea.ensure((size_t)std::max<int64_t>(nt_cap, 1) * 8));
     launch_mc_twrite(c->stream, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->aedge.as<int>(),
-                     c->idx.as<int>(), c->atoff.as<int>(), c->verts.as<double>(), c->tris.as<int>(),
+                     c->idx.as<int>(), c->idx_gen, c->atoff.as<int>(), c->verts.as<double>(), c->tris.as<int>(),
                      c->tarea.as<double>());
     HIP_TRY(c, hipGetLastError());
     const int64_t per = sum_partials_count(nt_cap);
@@ -1623,12 +1632,7 @@ int nsgpu_blobby_build(nsgpu_ctx *c, const double *xyzr, int32_t n, double B, do
     if (rc) return rc;
     const void *old_idx = c->idx.p;
     HIP_TRY(c, c->idx.ensure((size_t)nvox * 3 * 4));
-    if (c->idx.p != old_idx || c->idx_nvox != nvox || 3 * nvox > INT32_MAX)
-        launch_fill32(st, c->idx.p, 0xffffffffu, nvox * 3);
-    else
-        launch_idx_reset(st, c->idx.as<int>(), c->vslot.as<int64_t>(), c->idx_dirty);
-    c->idx_nvox = nvox;
-    c->idx_dirty = 0;
+    idx_begin(c, st, nvox, old_idx);
     c->have_trace = c->have_surf = c->have_vertices = c->have_mesh = false;
     const int64_t nseg = mc_num_segments(g);
     HIP_TRY(c, c->segmask.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
@@ -1652,8 +1656,8 @@ int nsgpu_blobby_build(nsgpu_ctx *c, const double *xyzr, int32_t n, double B, do
     launch_mc_compact(st, g, c->fbits.as<uint32_t>(), c->segmask.as<uint32_t>(), c->voffs.as<int>(), c->anode.as<int>(),
                       c->acase.as<uint8_t>());
     HIP_TRY(c, c->aedge.ensure(na * 48));
-    launch_mc_counts(st, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->idx.as<int>(), c->anv.as<int>(),
-                     c->ant.as<int>(), c->aedge.as<int>());
+    launch_mc_counts(st, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->idx.as<int>(), c->idx_gen,
+                     c->anv.as<int>(), c->ant.as<int>(), c->aedge.as<int>());
     launch_scan(st, c->anv.as<int>(), nact, c->avoff.as<int>(), c->scan_scratch.as<int>());
     launch_scan(st, c->ant.as<int>(), nact, c->atoff.as<int>(), c->scan_scratch.as<int>());
     HIP_TRY(c, hipGetLastError());
@@ -1671,12 +1675,12 @@ int nsgpu_blobby_build(nsgpu_ctx *c, const double *xyzr, int32_t n, double B, do
     HIP_TRY(c, c->vslot.ensure((size_t)std::max<int64_t>(nv, 1) * 8));
     HIP_TRY(c, c->vorig.ensure((size_t)std::max<int64_t>(nv, 1) * 4));
     launch_mc_vwrite(st, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->aedge.as<int>(), c->idx.as<int>(),
-                     c->avoff.as<int>(), 0, c->verts.as<double>(), c->vslot.as<int64_t>(), c->vorig.as<int>(), a.field, 1.0);
+                     c->idx_gen, c->avoff.as<int>(), 0, c->verts.as<double>(), c->vslot.as<int64_t>(), c->vorig.as<int>(), a.field, 1.0);
     c->idx_dirty = nv;
     HIP_TRY(c, c->tris.ensure((size_t)std::max<int64_t>(nt, 1) * 12));
     HIP_TRY(c, c->tarea.ensure((size_t)std::max<int64_t>(nt, 1) * 8));
     launch_mc_twrite(st, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->aedge.as<int>(), c->idx.as<int>(),
-                     c->atoff.as<int>(), c->verts.as<double>(), c->tris.as<int>(), c->tarea.as<double>(), 0);
+                     c->idx_gen, c->atoff.as<int>(), c->verts.as<double>(), c->tris.as<int>(), c->tarea.as<double>(), 0);
     const int64_t per = sum_partials_count(nt);
     HIP_TRY(c, c->partial.ensure((size_t)std::max<int64_t>(per, 1) * 8));
     launch_sum_blocks(st, c->tarea.as<double>(), nt, c->partial.as<double>());

@@ -142,12 +142,14 @@ struct GatherSrc {
 void launch_gather(hipStream_t, const GatherSrc &, int *dst);
 void mc_upload_tables(hipStream_t);
 void launch_vtx_count(hipStream_t, const DevGrid &, const DevRays &, const int[3], const uint32_t *, int *);
-// vslot[v] receives the index slot of vertex v (for launch_idx_reset), vhit[v]
-// the index of its depth in R.t / R.pid
-// vertices v >= vcap are dropped (vcap bounds the total; the host checks it)
+// Vertex index slots idx[3 * node + direction] hold gen + v for vertex v of
+// the current pass; anything else reads as no vertex (gen <= 2^31 - 1, and
+// every id of earlier passes is below gen).
+// vslot[v] receives the index slot of vertex v, vhit[v] the index of its
+// depth in R.t / R.pid; vertices v >= vcap are dropped (vcap bounds the
+// total; the host checks it)
 void launch_vtx_write(hipStream_t, const DevGrid &, const DevRays &, const int[3], const uint32_t *, const int *,
-                      double *, int *, int64_t *vslot, int64_t *vhit, int64_t vcap);
-void launch_idx_reset(hipStream_t, int *idx, const int64_t *vslot, int64_t n);
+                      double *, int *idx, uint32_t gen, int64_t *vslot, int64_t *vhit, int64_t vcap);
 int64_t mc_num_segments(const DevGrid &);
 void launch_mc_classify(hipStream_t, const DevGrid &, const uint32_t *bits, uint32_t *segmask, int *segcnt);
 void launch_mc_compact(hipStream_t, const DevGrid &, const uint32_t *bits, const uint32_t *segmask, const int *segoff,
@@ -155,13 +157,13 @@ void launch_mc_compact(hipStream_t, const DevGrid &, const uint32_t *bits, const
 // aedge[12 q + e]: the vertex index of edge e of active cube q (-1: none yet),
 // read once here from idx so that the later passes read it coalesced
 void launch_mc_counts(hipStream_t, const DevGrid &, int nact, const int *anode, const uint8_t *acase, const int *idx,
-                      int *nv, int *nt, int *aedge);
+                      uint32_t gen, int *nv, int *nt, int *aedge);
 // vorig[v - base] = 4 * (active cube) + (0, 1, 2 for edge 5, 6, 10) of each mid point
 // field == nullptr: edge mid points; else vertexInterp at iso on the scalar
 // field (one double per node)
 void launch_mc_vwrite(hipStream_t, const DevGrid &, int nact, const int *anode, const uint8_t *acase,
-                      const int *aedge, int *idx, const int *voff, int base, double *verts, int64_t *vslot, int *vorig,
-                      const double *field = nullptr, double iso = 0.);
+                      const int *aedge, int *idx, uint32_t gen, const int *voff, int base, double *verts,
+                      int64_t *vslot, int *vorig, const double *field = nullptr, double iso = 0.);
 // vertex normals: analytic at the depths the vertices come from (the ray
 // lists must carry patch ids), mid points from their adjacent triangles
 struct NormalArgs {
@@ -179,8 +181,8 @@ void launch_normals(hipStream_t, const DevGrid &, const DevPatches &, const DevR
 // reverse: triangles written (t2, t1, t0) as for the analytic corner bitmap;
 // the scalar-field marching cubes keeps (t0, t1, t2)
 void launch_mc_twrite(hipStream_t, const DevGrid &, int nact, const int *anode, const uint8_t *acase,
-                      const int *aedge, const int *idx, const int *toff, const double *verts, int *tris, double *tarea,
-                      int reverse = 1);
+                      const int *aedge, const int *idx, uint32_t gen, const int *toff, const double *verts,
+                      int *tris, double *tarea, int reverse = 1);
 
 // blobby scalar field (BlobbySurface::build)
 struct BlobbyArgs {

@@ -19,6 +19,16 @@
 
 namespace nsg {
 
+// Vertex index slots (idx[3 * node + direction]) hold gen + v for vertex v of
+// the current pass (v < 2^31, gen <= 2^31 - 1); any other content (the ids of
+// earlier passes, all below gen, or the initial 0xffffffff) reads as "no
+// vertex", so the slots are never cleared between passes.
+__device__ __forceinline__ int vid(int raw, uint32_t gen) {
+    const uint32_t d = (uint32_t)raw - gen;
+    return d < 0x80000000u ? (int)d : -1;
+}
+__device__ __forceinline__ int vtag(int v, uint32_t gen) { return (int)(gen + (uint32_t)v); }
+
 __device__ __forceinline__ int bit_at(const uint32_t *bits, int64_t f, int64_t nvox) {
     if (f < 0 || f >= nvox) return 1;
     return (bits[f >> 5] >> (uint32_t)(f & 31)) & 1u;
@@ -126,8 +136,8 @@ __global__ void __launch_bounds__(256) k_vtx_count(DevGrid g, DevRays R, EdgeSlo
 constexpr int kVtxStage = 192;
 
 __global__ void __launch_bounds__(256) k_vtx_write(DevGrid g, DevRays R, EdgeSlots es, const uint32_t *bits,
-                                                   const int *offs, double *verts, int *idx, int64_t *vslot,
-                                                   int64_t *vhit, int64_t vcap) {
+                                                   const int *offs, double *verts, int *idx, uint32_t gen,
+                                                   int64_t *vslot, int64_t *vhit, int64_t vcap) {
     __shared__ double sv[4][3 * kVtxStage];
     __shared__ int64_t ss[4][kVtxStage];
     __shared__ int64_t sh[4][kVtxStage];
@@ -146,7 +156,7 @@ __global__ void __launch_bounds__(256) k_vtx_write(DevGrid g, DevRays R, EdgeSlo
                             [&](int q, int d, int64_t f, double px, double py, double pz, int hit) {
                                 const int v = base + q;
                                 if (v >= vcap) return;
-                                idx[3 * f + d] = v;
+                                idx[3 * f + d] = vtag(v, gen);
                                 // the depth this vertex comes from, for its analytic normal
                                 const int64_t h = ((int64_t)es.s[panel] * NN + (r - panel * NN)) * R.K + hit;
                                 const int so = v - wbase;
@@ -323,7 +333,7 @@ __global__ void __launch_bounds__(256) k_mc_compact(DevGrid g, SegIter si, const
 // an analytic vertex) and triangles it emits (all three vertices present after
 // the mid points are added)
 __global__ void __launch_bounds__(256) k_mc_counts(DevGrid g, int nact, const int *anode, const uint8_t *acase,
-                                                   const int *idx, int *nv, int *nt, int *aedge) {
+                                                   const int *idx, uint32_t gen, int *nv, int *nt, int *aedge) {
     const int q = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     // a full wave's records (3 KB, contiguous) go out through LDS as three
     // contiguous 1 KB stores: stored per lane, each store instruction left
@@ -340,7 +350,7 @@ __global__ void __launch_bounds__(256) k_mc_counts(DevGrid g, int nact, const in
     unsigned have = 0;  // bit e: edge e has a vertex
 #pragma unroll
     for (int e = 0; e < 12; ++e) {
-        ids[e] = idx[edge_slot(e, f, sy, sz, nvox)];
+        ids[e] = vid(idx[edge_slot(e, f, sy, sz, nvox)], gen);
         have |= (ids[e] != -1 ? 1u : 0u) << e;
     }
     if (full) {
@@ -390,7 +400,7 @@ __device__ __forceinline__ void vertex_interp(double iso, const double p1[3], co
 }
 
 __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const int *anode, const uint8_t *acase,
-                                                   const int *aedge, int *idx, const int *voff, int base,
+                                                   const int *aedge, int *idx, uint32_t gen, const int *voff, int base,
                                                    double *verts, int64_t *vslot, int *vorig, const double *field,
                                                    double iso) {
     const int q = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -419,19 +429,19 @@ __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const in
             vertex_interp(iso, c5, c6, f5, f6, verts + 3 * (int64_t)v);
             vslot[v] = s5;
             vorig[v - base] = 4 * q + 0;
-            idx[s5] = v++;
+            idx[s5] = vtag(v++, gen);
         }
         if (m6) {
             vertex_interp(iso, c6, c7, f6, f7, verts + 3 * (int64_t)v);
             vslot[v] = s6;
             vorig[v - base] = 4 * q + 1;
-            idx[s6] = v++;
+            idx[s6] = vtag(v++, gen);
         }
         if (m10) {
             vertex_interp(iso, c2, c6, f2, f6, verts + 3 * (int64_t)v);
             vslot[v] = s10;
             vorig[v - base] = 4 * q + 2;
-            idx[s10] = v++;
+            idx[s10] = vtag(v++, gen);
         }
         return;
     }
@@ -441,7 +451,7 @@ __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const in
         verts[3 * (int64_t)v + 2] = g.z[k + 1] - g.hside;
         vslot[v] = s5;
         vorig[v - base] = 4 * q + 0;
-        idx[s5] = v++;
+        idx[s5] = vtag(v++, gen);
     }
     if (m6) {
         verts[3 * (int64_t)v] = g.x[i];
@@ -449,7 +459,7 @@ __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const in
         verts[3 * (int64_t)v + 2] = g.z[k + 1] - g.hside;
         vslot[v] = s6;
         vorig[v - base] = 4 * q + 1;
-        idx[s6] = v++;
+        idx[s6] = vtag(v++, gen);
     }
     if (m10) {
         verts[3 * (int64_t)v] = g.x[i + 1] - g.hside;
@@ -457,21 +467,21 @@ __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const in
         verts[3 * (int64_t)v + 2] = g.z[k];
         vslot[v] = s10;
         vorig[v - base] = 4 * q + 2;
-        idx[s10] = v++;
+        idx[s10] = vtag(v++, gen);
     }
 }
 
 // Edge vertex indices from the cube's record; an edge still at -1 there may
 // have received a mid point from the cube that owns it (k_mc_vwrite), so
 // only those are looked up in idx again.
-__device__ __forceinline__ int edge_vertex(const int *rec, const int *idx, int e, int64_t f, int64_t sy, int64_t sz,
-                                           int64_t nvox) {
+__device__ __forceinline__ int edge_vertex(const int *rec, const int *idx, uint32_t gen, int e, int64_t f, int64_t sy,
+                                           int64_t sz, int64_t nvox) {
     const int v = rec[e];
-    return v != -1 ? v : idx[edge_slot(e, f, sy, sz, nvox)];
+    return v != -1 ? v : vid(idx[edge_slot(e, f, sy, sz, nvox)], gen);
 }
 
 __global__ void __launch_bounds__(256) k_mc_twrite(DevGrid g, int nact, const int *anode, const uint8_t *acase,
-                                                   const int *aedge, const int *idx, const int *toff,
+                                                   const int *aedge, const int *idx, uint32_t gen, const int *toff,
                                                    const double *verts, int *tris, double *tarea, int reverse) {
     const int q = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     // The valid lanes of a wave are a prefix and their triangles one
@@ -494,9 +504,9 @@ __global__ void __launch_bounds__(256) k_mc_twrite(DevGrid g, int nact, const in
                   e2 = (int)((w >> (12 * t + 8)) & 15);
         if (e0 == 15) break;
         const int *rec = aedge + 12 * (int64_t)q;
-        const int v0 = edge_vertex(rec, idx, e0, f, sy, sz, nvox);
-        const int v1 = edge_vertex(rec, idx, e1, f, sy, sz, nvox);
-        const int v2 = edge_vertex(rec, idx, e2, f, sy, sz, nvox);
+        const int v0 = edge_vertex(rec, idx, gen, e0, f, sy, sz, nvox);
+        const int v1 = edge_vertex(rec, idx, gen, e1, f, sy, sz, nvox);
+        const int v2 = edge_vertex(rec, idx, gen, e2, f, sy, sz, nvox);
         if (v0 == -1 || v1 == -1 || v2 == -1) continue;
         const int so = o - base;  // < 5 * 64: at most five triangles per lane before it
         stri[wv][3 * so + 0] = reverse ? v2 : v0;
@@ -534,21 +544,12 @@ void launch_vtx_count(hipStream_t s, const DevGrid &g, const DevRays &R, const i
 }
 
 void launch_vtx_write(hipStream_t s, const DevGrid &g, const DevRays &R, const int eslots[3], const uint32_t *bits,
-                      const int *offs, double *verts, int *idx, int64_t *vslot, int64_t *vhit, int64_t vcap) {
+                      const int *offs, double *verts, int *idx, uint32_t gen, int64_t *vslot, int64_t *vhit,
+                      int64_t vcap) {
     EdgeSlots es{{eslots[0], eslots[1], eslots[2]}};
     const int64_t n = 3 * (int64_t)R.N * R.N;
-    hipLaunchKernelGGL(k_vtx_write, dim3(nblk(n)), dim3(256), 0, s, g, R, es, bits, offs, verts, idx, vslot, vhit,
-                       vcap);
-}
-
-// back to "no vertex" for the slots the last vertices were written to
-__global__ void __launch_bounds__(256) k_idx_reset(int *idx, const int64_t *vslot, int64_t n) {
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < n) idx[vslot[v]] = -1;
-}
-
-void launch_idx_reset(hipStream_t s, int *idx, const int64_t *vslot, int64_t n) {
-    if (n > 0) hipLaunchKernelGGL(k_idx_reset, dim3(nblk(n)), dim3(256), 0, s, idx, vslot, n);
+    hipLaunchKernelGGL(k_vtx_write, dim3(nblk(n)), dim3(256), 0, s, g, R, es, bits, offs, verts, idx, gen, vslot,
+                       vhit, vcap);
 }
 
 static SegIter seg_iter(const DevGrid &g) {
@@ -577,25 +578,26 @@ void launch_mc_compact(hipStream_t s, const DevGrid &g, const uint32_t *bits, co
 }
 
 void launch_mc_counts(hipStream_t s, const DevGrid &g, int nact, const int *anode, const uint8_t *acase,
-                      const int *idx, int *nv, int *nt, int *aedge) {
+                      const int *idx, uint32_t gen, int *nv, int *nt, int *aedge) {
     if (nact)
-        hipLaunchKernelGGL(k_mc_counts, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, idx, nv, nt, aedge);
+        hipLaunchKernelGGL(k_mc_counts, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, idx, gen, nv, nt,
+                           aedge);
 }
 
 void launch_mc_vwrite(hipStream_t s, const DevGrid &g, int nact, const int *anode, const uint8_t *acase,
-                      const int *aedge, int *idx, const int *voff, int base, double *verts, int64_t *vslot, int *vorig,
-                      const double *field, double iso) {
+                      const int *aedge, int *idx, uint32_t gen, const int *voff, int base, double *verts,
+                      int64_t *vslot, int *vorig, const double *field, double iso) {
     if (nact)
-        hipLaunchKernelGGL(k_mc_vwrite, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, aedge, idx, voff,
-                           base, verts, vslot, vorig, field, iso);
+        hipLaunchKernelGGL(k_mc_vwrite, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, aedge, idx, gen,
+                           voff, base, verts, vslot, vorig, field, iso);
 }
 
 void launch_mc_twrite(hipStream_t s, const DevGrid &g, int nact, const int *anode, const uint8_t *acase,
-                      const int *aedge, const int *idx, const int *toff, const double *verts, int *tris,
-                      double *tarea, int reverse) {
+                      const int *aedge, const int *idx, uint32_t gen, const int *toff, const double *verts,
+                      int *tris, double *tarea, int reverse) {
     if (nact)
-        hipLaunchKernelGGL(k_mc_twrite, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, aedge, idx, toff,
-                           verts, tris, tarea, reverse);
+        hipLaunchKernelGGL(k_mc_twrite, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, aedge, idx, gen,
+                           toff, verts, tris, tarea, reverse);
 }
 
 }  // namespace nsg
