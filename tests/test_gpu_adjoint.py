@@ -97,10 +97,82 @@ def test_adjoint_matches_central_differences(device_ok, n, radius, scale, natoms
     ev5, ea5, sk5 = stats[1e-5]
     assert len(ev5) >= 0.9 * natoms * 4
     assert np.quantile(ev5, 0.95) < 1e-4 and np.quantile(ea5, 0.95) < 1e-4
-    assert ev5.max() < 1e-2 and ea5.max() < 1e-2
+    assert ev5.max() < 2e-3 and ea5.max() < 2e-3
     ev4, ea4, sk4 = stats[1e-4]
     assert len(ev4) >= 0.75 * natoms * 4
     assert ev4.max() < 5e-2 and np.median(ev4) < 1e-4 and np.median(ea4) < 1e-4
+
+
+def _fd_error(dev, jit, base, i, q, h):
+    ap, am = jit.copy(), jit.copy()
+    ap[i, q] += h
+    am[i, q] -= h
+    p, cp = _evaluate(dev, ap)
+    m, cm = _evaluate(dev, am)
+    if not all(np.array_equal(x, y) for x, y in zip(cp, cm)):
+        return None
+    fv = (p["volume"] - m["volume"]) / (2 * h)
+    fa = (p["area"] - m["area"]) / (2 * h)
+    return (abs(fv - base["dvolume"][i, q]) / max(1.0, abs(fv)), abs(fa - base["darea"][i, q]) / max(1.0, abs(fa)))
+
+
+def _fd_volume_refined(dev, jit, base, i, q, h):
+    # the same three-panel volume from getSurf with the torus roots refined to
+    # the true roots of their quartics (precision 1), free of the closed
+    # form's root noise
+    opts = dict(build_epsmap=True, build_status=True, accurate_triangulation=False, fill_cavities=False,
+                precision=1)
+    vals = []
+    for sgn in (1, -1):
+        a = jit.copy()
+        a[i, q] += sgn * h
+        dev.set_patches(ns.ses_patches(a, 1.4))
+        vals.append(dev.get_surf(**opts)["volume"])
+    fv = (vals[0] - vals[1]) / (2 * h)
+    return abs(fv - base["dvolume"][i, q]) / max(1.0, abs(fv))
+
+
+@pytest.mark.parametrize("n,radius,scale,natoms", [
+    (200, 9.0, 3.0, 16),
+    (2000, 21.0, 3.0, 32),
+])
+def test_adjoint_fd_converges_on_the_worst_parameters(device_ok, n, radius, scale, natoms):
+    # The parameters with the largest central-difference disagreement at
+    # h = 1e-4 A (up to ~150 % in the area estimate: rays a fraction of a
+    # micrometre from tangency, whose chord is sqrt-like in the parameter)
+    # converge to the adjoint as h shrinks, the error falling about as h^2.
+    # Below h ~ 1e-5 A the differences of the FP64 forward meet the noise of
+    # the closed-form torus roots themselves (off by up to ~1e-5 A on grazing
+    # rays, as the reference's are), which grows as 1/h.  With the roots
+    # refined to the true roots of their quartics the volume's differences
+    # keep converging, to 1e-5 of the adjoint; the area estimate (no refined
+    # forward) agrees to 1e-4 at the best of h = 1e-5, 3e-6, 1e-6 A.  A
+    # truncation error of the differences, not an error of the adjoint.
+    grid, jit, dev = _setup(n, radius, scale)
+    base, _ = _evaluate(dev, jit)
+    rng = np.random.default_rng(1)
+    ids = rng.choice(len(jit), size=natoms, replace=False)
+    scored = []
+    for i in ids:
+        for q in range(4):
+            e = _fd_error(dev, jit, base, i, q, 1e-4)
+            if e is not None:
+                scored.append((max(e), i, q, e))
+    scored.sort(key=lambda x: -x[0])
+    worst = scored[:6]
+    print("worst at h=1e-4:", [(int(i), q, "%.1e" % w) for w, i, q, _ in worst], flush=True)
+    assert worst[0][0] > 1e-3  # the check has something to show
+    for w, i, q, e4 in worst:
+        hs = (1e-5, 3e-6, 1e-6)
+        errs = {h: _fd_error(dev, jit, base, i, q, h) for h in hs}
+        ref = {h: _fd_volume_refined(dev, jit, base, i, q, h) for h in hs}
+        print("atom %d param %d: fp64 %s refined volume %s" % (
+            i, q, {h: None if e is None else ("%.1e" % e[0], "%.1e" % e[1]) for h, e in errs.items()},
+            {h: "%.1e" % e for h, e in ref.items()}), flush=True)
+        best_v = min(ref.values())
+        assert best_v < 1e-5 and best_v < max(1e-6, 1e-2 * e4[0])
+        seen = [e for e in errs.values() if e is not None]
+        assert seen and min(e[1] for e in seen) < 1e-4
 
 
 def test_autograd_function_backward(device_ok):
