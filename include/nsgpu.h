@@ -174,6 +174,16 @@ int32_t nsgpu_atom_set_view(const nsgpu_atom_set *s, const double **xyzr, const 
 const char *nsgpu_atom_set_error(const nsgpu_atom_set *s);
 void nsgpu_atom_set_free(nsgpu_atom_set *s);
 
+/* Nearest atom of each point (m points, 3 doubles each): the acceleration
+ * grid of Surface::buildAtomsMap (Surface.cpp:1695-1749) over the grid's
+ * baricenter and rmaxdim, and the 27-cell search of Surface::vdwAccessible
+ * (Surface.cpp:16663-16713): the first atom of smallest |p - c|^2 - r^2 in
+ * the reference's cell order, or -1.  The vertex-atom map of the mesh files
+ * and, over the cavities' boundary nodes, Surface::getCavitiesAtoms
+ * (Surface.cpp:2129-2245). */
+int nsgpu_nearest_atoms(const nsgpu_grid *grid, const double *xyzr, int64_t n, const double *points, int64_t m,
+                        int32_t *out);
+
 /* DelPhiShared::buildGrid(scale, perfil) (DelphiShared.cpp:1038-1201),
  * bounds from DelPhiShared::getBounds (DelphiShared.cpp:775-797).
  * xyzr: n*4 doubles (x, y, z, radius). */

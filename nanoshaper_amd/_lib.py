@@ -59,6 +59,7 @@ SIGNATURES = {
     "nsgpu_atom_set_error": (ctypes.c_char_p, [VP]),
     "nsgpu_atom_set_free": (None, [VP]),
     "nsgpu_build_grid": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.POINTER(Grid)]),
+    "nsgpu_nearest_atoms": (ctypes.c_int, [ctypes.POINTER(Grid), VP, ctypes.c_int64, VP, ctypes.c_int64, VP]),
     "nsgpu_jitter": (None, [VP, ctypes.c_int32, ctypes.c_double, c_int32_p]),
     "nsgpu_ses_build": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_double, ctypes.POINTER(VP)]),
     "nsgpu_skin_points": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_double, VP]),

@@ -6,9 +6,12 @@
 * ``normal_mode``  DelPhiShared from the atom file and grid keys, the surface
   of key "Surface", build, getSurf, triangulateSurface (normalMode,
   main.cpp:1253-1320)
+* ``write_outputs`` the files normal mode saves (main.cpp:1359-1394 and the
+  mesh of triangulateSurface / smoothSurface, Surface.cpp:10478-10515)
 """
 import os
 
+from . import io as IO
 from . import surface as S
 
 
@@ -84,6 +87,11 @@ class Configuration:
         self.normals = r("Compute_Vertex_Normals", False)
         self.aux_grid_2d = r("Max_mesh_auxiliary_grid_2d_size", 100)
         self.sys_name = r("Sys_Name", "mol")
+        self.root_file = r("Root_FileName", "")
+        self.save_cavities = r("Save_Cavities", False)
+        self.save_msms = r("Save_Mesh_MSMS_Format", False)
+        self.save_ply = r("Save_Mesh_PLY_Format", False)
+        self.vertex_atom_info = r("Vertex_Atom_Info", False)
         if not self.build_epsmaps and self.save_epsmaps:
             raise ConfigError("Asked to save epsmap without builiding it: set Build_epsilon_maps = true")
         if not self.build_epsmaps and self.proj_bgp:
@@ -149,3 +157,38 @@ def run(conf, atoms, device=0):
     mesh = dev.triangulate() if (conf.tri and conf.acc_tri) else None
     return {"atoms": atoms, "jittered": jit, "grid": grid, "patches": ps, "device": dev, "result": res,
             "mesh": mesh}
+
+
+def write_outputs(out, directory="."):
+    """The files of a normal-mode run (out from normal_mode or run): the mesh
+    (triangulatedSurf in the format of Surface::deduceFormat, smoothed when
+    Smooth_Mesh, with triangleAreas.txt), then the maps and cavity files the
+    Save_* keys ask for, each named root + Sys_Name as the reference names
+    them.  Returns the paths written."""
+    conf = out["config"]
+    dev, grid = out["device"], out["grid"]
+    root = os.path.join(directory, conf.root_file)
+    name = conf.sys_name
+    paths = []
+    if out.get("mesh") is not None:
+        if conf.smoothing:
+            dev.smooth_surface()
+        v, t = dev.mesh()
+        normals = dev.normals() if conf.normals else None
+        va = IO.vertex_atoms(grid, out["jittered"], v) if conf.vertex_atom_info else None
+        fmt = IO.deduce_format(conf.save_ply, conf.save_msms, va is not None, normals is not None)
+        paths += IO.save_mesh(root, v, t, normals=normals, vertex_atoms=va, fmt=fmt)
+    if conf.save_epsmaps:
+        paths += IO.save_epsmaps(root + name, dev.epsmap())
+    if conf.save_status or conf.save_cavities:
+        st = dev.status()
+        sizes, filled = dev.cavities()
+        if conf.save_status:
+            paths.append(root + name + ".status.txt")
+            IO.save_status(paths[-1], st)
+        IO.save_cavities(root, grid, st, sizes, filled)
+        paths += [root + "cavities.txt", root + "cavitiesSize.txt", root + "cavAtomsSerials.txt"]
+    if conf.save_idebmap:
+        paths.append(root + name + ".idebmap.txt")
+        IO.save_idebmap(paths[-1], dev.idebmap())
+    return paths

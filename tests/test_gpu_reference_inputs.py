@@ -70,3 +70,55 @@ def test_6xm4_skin_configuration(device_ok):
     r, m = _compare(run)
     assert m["num_triangles"] > 1000000
     run["device"].close()
+
+
+@pytest.mark.parametrize("extra,files", [
+    ({"Vertex_Atom_Info": "true", "Save_Status_map": "true", "Cavity_Detection_Filling": "true"},
+     ["triangulatedSurf.off", "triangulatedSurftriangleAreas.txt", "1crn.status.txt", "cavities.txt",
+      "cavitiesSize.txt", "cavAtomsSerials.txt"]),
+    ({"Save_Mesh_MSMS_Format": "true", "Vertex_Atom_Info": "true", "Build_epsilon_maps": "true",
+      "Save_eps_maps": "true", "Save_ideb_map": "true"},
+     ["triangulatedSurf.face", "triangulatedSurf.vert", "1crn.epsmapx.txt", "1crn.epsmapy.txt",
+      "1crn.epsmapz.txt", "1crn.idebmap.txt"]),
+    ({"Save_Mesh_PLY_Format": "true", "Smooth_Mesh": "true"}, ["triangulatedSurf.ply"]),
+])
+def test_1crn_normal_mode_files(device_ok, tmp_path, extra, files):
+    # the 1crn SES configuration with output keys switched on: the files the
+    # reference's normal mode writes, holding the device's maps and mesh
+    from nanoshaper_amd import io
+    text = open(os.path.join(GOLDEN, "1crn_ses_conf.prm")).read()
+    lines = [l for l in text.splitlines() if l.split("=")[0].strip() not in set(extra) | {"XYZR_FileName",
+                                                                                       "Sys_Name"}]
+    lines += ["%s = %s" % kv for kv in extra.items()]
+    lines += ["XYZR_FileName = %s" % os.path.join(GOLDEN, "1crn.xyzr"), "Sys_Name = 1crn"]
+    prm = tmp_path / "run.prm"
+    prm.write_text("\n".join(lines) + "\n")
+    print("config", extra, flush=True)
+    run = config.normal_mode(str(prm))
+    out_dir = tmp_path / "out"
+    out_dir.mkdir()
+    paths = config.write_outputs(run, str(out_dir))
+    for f in files:
+        assert os.path.exists(out_dir / f), f
+    dev = run["device"]
+    v, t = dev.mesh()
+    if "Save_Mesh_PLY_Format" in extra:
+        v2, t2, n2 = io.load_ply(str(out_dir / "triangulatedSurf.ply"))
+        # Smooth_Mesh: the saved mesh is the smoothed one, with its normals
+        assert np.array_equal(v2, v) and np.array_equal(t2, t) and np.array_equal(n2, dev.normals())
+    elif "Save_Mesh_MSMS_Format" in extra:
+        vert = open(out_dir / "triangulatedSurf.vert").read().splitlines()
+        va = np.array([int(l.split()[7]) for l in vert[3:]]) - 1
+        assert np.array_equal(va, io.vertex_atoms(run["grid"], run["jittered"], v))
+        e = np.loadtxt(out_dir / "1crn.epsmapx.txt").ravel()
+        assert e.size == run["grid"].nx * run["grid"].ny * run["grid"].nz
+    else:
+        v3, t3, _ = io.load_off(str(out_dir / "triangulatedSurf.off"))
+        assert np.array_equal(t3, t) and np.abs(v3 - v).max() <= 5e-4
+        head = open(out_dir / "triangulatedSurf.off").readline().strip()
+        assert head == "OFF+N+A"
+        st = np.loadtxt(out_dir / "1crn.status.txt").ravel()
+        assert st.size == run["grid"].nx * run["grid"].ny * run["grid"].nz
+        rows = open(out_dir / "cavitiesSize.txt").read().splitlines()
+        assert float(rows[0]) == run["result"]["num_cavities"] == len(rows) - 1
+    run["device"].close()
