@@ -1407,7 +1407,7 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
     // 3. write the mid points, then the triangles that may reference them
     HIP_TRY(c, c->vorig.ensure((size_t)std::max<int64_t>(nadd_cap, 1) * 4));
     launch_mc_vwrite(c->stream, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->aedge.as<int>(),
-                     c->idx.as<int>(), c->idx_gen, c->avoff.as<int>(), (int)c->n_analytic, c->verts.as<double>(),
+                     c->anv.as<int>(), c->idx.as<int>(), c->idx_gen, c->avoff.as<int>(), (int)c->n_analytic, c->verts.as<double>(),
                      c->vslot.as<int64_t>(), c->vorig.as<int>());
     c->nact = nact;
     HIP_TRY(c, c->tris.ensure((size_t)std::max<int64_t>(nt_cap, 1) * 12));
@@ -1674,8 +1674,8 @@ int nsgpu_blobby_build(nsgpu_ctx *c, const double *xyzr, int32_t n, double B, do
     HIP_TRY(c, c->verts.ensure((size_t)std::max<int64_t>(nv, 1) * 24));
     HIP_TRY(c, c->vslot.ensure((size_t)std::max<int64_t>(nv, 1) * 8));
     HIP_TRY(c, c->vorig.ensure((size_t)std::max<int64_t>(nv, 1) * 4));
-    launch_mc_vwrite(st, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->aedge.as<int>(), c->idx.as<int>(),
-                     c->idx_gen, c->avoff.as<int>(), 0, c->verts.as<double>(), c->vslot.as<int64_t>(), c->vorig.as<int>(), a.field, 1.0);
+    launch_mc_vwrite(st, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->aedge.as<int>(), c->anv.as<int>(),
+                     c->idx.as<int>(), c->idx_gen, c->avoff.as<int>(), 0, c->verts.as<double>(), c->vslot.as<int64_t>(), c->vorig.as<int>(), a.field, 1.0);
     c->idx_dirty = nv;
     HIP_TRY(c, c->tris.ensure((size_t)std::max<int64_t>(nt, 1) * 12));
     HIP_TRY(c, c->tarea.ensure((size_t)std::max<int64_t>(nt, 1) * 8));

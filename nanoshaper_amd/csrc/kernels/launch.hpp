@@ -161,8 +161,9 @@ void launch_mc_counts(hipStream_t, const DevGrid &, int nact, const int *anode, 
 // vorig[v - base] = 4 * (active cube) + (0, 1, 2 for edge 5, 6, 10) of each mid point
 // field == nullptr: edge mid points; else vertexInterp at iso on the scalar
 // field (one double per node)
+// nadd[q]: the mid points cube q adds (k_mc_counts' nv)
 void launch_mc_vwrite(hipStream_t, const DevGrid &, int nact, const int *anode, const uint8_t *acase,
-                      const int *aedge, int *idx, uint32_t gen, const int *voff, int base, double *verts,
+                      const int *aedge, const int *nadd, int *idx, uint32_t gen, const int *voff, int base, double *verts,
                       int64_t *vslot, int *vorig, const double *field = nullptr, double iso = 0.);
 // vertex normals: analytic at the depths the vertices come from (the ray
 // lists must carry patch ids), mid points from their adjacent triangles

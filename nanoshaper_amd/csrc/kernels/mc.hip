@@ -400,11 +400,12 @@ __device__ __forceinline__ void vertex_interp(double iso, const double p1[3], co
 }
 
 __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const int *anode, const uint8_t *acase,
-                                                   const int *aedge, int *idx, uint32_t gen, const int *voff, int base,
-                                                   double *verts, int64_t *vslot, int *vorig, const double *field,
-                                                   double iso) {
+                                                   const int *aedge, const int *nadd, int *idx, uint32_t gen,
+                                                   const int *voff, int base, double *verts, int64_t *vslot,
+                                                   int *vorig, const double *field, double iso) {
     const int q = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (q >= nact) return;
+    // cubes that add no mid point (k_mc_counts' count) read nothing more
+    if (q >= nact || nadd[q] == 0) return;
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (int64_t)g.nx * g.ny;
     const int64_t f = anode[q];
     const int c = acase[q];
@@ -585,10 +586,10 @@ void launch_mc_counts(hipStream_t s, const DevGrid &g, int nact, const int *anod
 }
 
 void launch_mc_vwrite(hipStream_t s, const DevGrid &g, int nact, const int *anode, const uint8_t *acase,
-                      const int *aedge, int *idx, uint32_t gen, const int *voff, int base, double *verts,
+                      const int *aedge, const int *nadd, int *idx, uint32_t gen, const int *voff, int base, double *verts,
                       int64_t *vslot, int *vorig, const double *field, double iso) {
     if (nact)
-        hipLaunchKernelGGL(k_mc_vwrite, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, aedge, idx, gen,
+        hipLaunchKernelGGL(k_mc_vwrite, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, aedge, nadd, idx, gen,
                            voff, base, verts, vslot, vorig, field, iso);
 }
 
