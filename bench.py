@@ -282,6 +282,32 @@ def bench_differentiable(steps=10):
     return res
 
 
+def counter_roofline(profile, kernel, seconds=None):
+    """Roofline entry of a config-3 kernel from its committed counter profile
+    (profiles/*_traffic.json): FP64 lane operations issued (wave instructions
+    x 64, an FMA counting 2) over the kernel's time (the live stage time when
+    given, else the profile's rocprof average) against the FP64 peak, and its
+    calibrated HBM bytes against the HBM peak.  Issued, not algorithmic:
+    inactive lanes of an issued instruction are counted too."""
+    path = os.path.join(ROOT, "profiles", profile)
+    if not os.path.exists(path):
+        return None
+    k = json.load(open(path))["kernels"].get(kernel)
+    if not k or "fp64_lane_ops" not in k:
+        return None
+    t = seconds if seconds else k.get("avg_us", 0) * 1e-6
+    if not t:
+        return None
+    traffic = k["fetch_bytes"] + k["write_bytes"]
+    fl = k["fp64_lane_ops"] / t / 1e12
+    return {"kernel": kernel, "bound": "mfma", "compute": "FP64 VALU (no matrix form)",
+            "achieved": fl, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": fl / PEAK_FP64_TFLOPS,
+            "ops": "FP64 lane operations issued, from counters", "traffic": traffic,
+            "hbm_GBs": traffic / t / 1e9, "hbm_frac": traffic / t / 1e9 / PEAK_HBM_GBS,
+            "time_us": t * 1e6, "time": "live stage time" if seconds else "rocprof average (profile)",
+            "profile": "profiles/" + profile}
+
+
 def bench_blobby(steps=5, cpu=True):
     """Blobby surface (B = -2.5, cutoff 6 A): 20,000 atoms (compacted radius
     48 A), 2.0 points/A, perfil 80.  A step is BlobbySurface::build (field,
@@ -320,7 +346,10 @@ def bench_blobby(steps=5, cpu=True):
            "rays_per_s": r["total_rays"] / step, "voxels_per_s": nvox / step,
            "blobby_mesh": {"vertices": info["num_vertices"], "triangles": info["num_triangles"]},
            "result": {"volume": r["volume"], "hits": r["total_hits"], "odd_rays": r["odd_rays"],
-                      "cavities": r["num_cavities"], "triangles": m["num_triangles"], "area": m["area"]}}
+                      "cavities": r["num_cavities"], "triangles": m["num_triangles"], "area": m["area"]},
+           "roofline": {"field": counter_roofline("blobby_c3_20000atoms_traffic.json", "nsg::k_blobby_field"),
+                        "mesh_cast": counter_roofline("blobby_c3_20000atoms_traffic.json",
+                                                      "nsg::k_intersect_mesh<true>")}}
     if cpu:
         # the oracle (single thread) on the same workload, once
         from oracle import binding
@@ -374,6 +403,8 @@ def bench_skin(steps=10, cpu=True):
            "ms_per_step": step * 1e3, "rays_per_s": r["total_rays"] / step,
            "voxels_per_s": grid.nx * grid.ny * grid.nz / step, "stage_ms": st,
            "cells": dict(zip(ns.surface.SKIN_KINDS, kinds)), "host_build_s": host_s,
+           "roofline": counter_roofline("skin_c3_20000atoms_traffic.json", "nsg::k_intersect_skin",
+                                        st["intersect"] * 1e-3),
            "result": {"volume": r["volume"], "hits": r["total_hits"], "odd_rays": r["odd_rays"],
                       "cavities": r["num_cavities"], "triangles": m["num_triangles"], "area": m["area"]}}
     if cpu:
