@@ -69,7 +69,7 @@ struct SkinCells {
 
 template <int PANEL>
 __device__ __forceinline__ unsigned int skin_slot(const DevGrid &g, const DevPatches &P, const DevRays &R, int slot,
-                                                  double delta, SkinCells &L) {
+                                                  double delta, int group, SkinCells &L) {
     using A = PanelAxes<PANEL>;
     int *s_end = L.end, *s_it = L.it, *s_kind = L.kind, *s_cnt = L.cnt, *s_off = L.off, *s_is0 = L.is0,
         *s_is1 = L.is1, *s_pf = L.pf;
@@ -81,7 +81,7 @@ __device__ __forceinline__ unsigned int skin_slot(const DevGrid &g, const DevPat
     // cells come largest first; wave w takes cells w, w + W, w + 2 W, ... (W
     // waves per slot), one from each size band, so the waves' totals are alike
     const int nw = (P.n_skin + kSkinCells - 1) / kSkinCells;
-    const int ci = lane * nw + (int)(blockIdx.x / R.nslots);
+    const int ci = lane * nw + group;
     int total = 0;
     if (lane < kSkinCells && ci < P.n_skin) {
         const SkinRect c = skin_rect<PANEL>(g, P, P.order[first + ci]);
@@ -154,21 +154,30 @@ __device__ __forceinline__ unsigned int skin_slot(const DevGrid &g, const DevPat
     return cand;
 }
 
-// grid: ceil(n_skin / kSkinCells) * nslots waves, one per workgroup, the
-// panel slot fastest: the cells come largest first, and so do their waves in
-// every slot
+// grid: ceil(n_skin / kSkinCells) groups (padded to a multiple of 8) times
+// nslots waves, one per workgroup: the cells come largest first, and so do
+// their waves in every slot
 __global__ void __launch_bounds__(64) k_intersect_skin(DevGrid g, DevPatches P, DevRays R, unsigned long long *part) {
     const int first = P.n - P.n_skin;
-    const int slot = (int)(blockIdx.x % R.nslots);
-    const int64_t wg = (int64_t)slot * P.n + first + blockIdx.x / R.nslots;
+    // Block b runs on XCD b % 8.  XCD x takes groups x, x + 8, x + 16, ...,
+    // each with all its slots in turn, so a group's cells are fetched into one
+    // L2 (six times the HBM traffic when every slot of a group ran on another
+    // XCD), while the groups still start largest first across the chip (one
+    // contiguous range of groups per XCD left the heaviest range on one XCD
+    // and doubled the time).  The grid is padded to a multiple of 8 groups.
+    const int nw = (P.n_skin + kSkinCells - 1) / kSkinCells;
+    const int xcd = (int)(blockIdx.x & 7), r = (int)(blockIdx.x >> 3);
+    const int slot = r % R.nslots, group = (r / R.nslots) * 8 + xcd;
+    if (group >= nw) return;
+    const int64_t wg = (int64_t)slot * P.n + first + group;
     const int phase = slot < R.panels0 ? 0 : 1;
     const int panel = phase == 0 ? slot : slot - R.panels0;
     const double delta = (phase == 0) ? 0. : kDelta - g.hside;
     __shared__ SkinCells cells;
     unsigned int cand;
-    if (panel == 0) cand = skin_slot<0>(g, P, R, slot, delta, cells);
-    else if (panel == 1) cand = skin_slot<1>(g, P, R, slot, delta, cells);
-    else cand = skin_slot<2>(g, P, R, slot, delta, cells);
+    if (panel == 0) cand = skin_slot<0>(g, P, R, slot, delta, group, cells);
+    else if (panel == 1) cand = skin_slot<1>(g, P, R, slot, delta, group, cells);
+    else cand = skin_slot<2>(g, P, R, slot, delta, group, cells);
     for (int off = 32; off > 0; off >>= 1) cand += __shfl_down(cand, off, 64);
     if (threadIdx.x == 0 && cand) part[wg] = cand;
 }
@@ -176,8 +185,8 @@ __global__ void __launch_bounds__(64) k_intersect_skin(DevGrid g, DevPatches P, 
 void launch_intersect_skin(hipStream_t s, const DevGrid &g, const DevPatches &P, const DevRays &R,
                            unsigned long long *part) {
     if (P.n_skin <= 0 || R.nslots <= 0) return;
-    hipLaunchKernelGGL(k_intersect_skin, dim3((unsigned)((P.n_skin + kSkinCells - 1) / kSkinCells * R.nslots)), dim3(64), 0, s,
-                       g, P, R, part);
+    const int64_t groups = ((P.n_skin + kSkinCells - 1) / kSkinCells + 7) / 8 * 8;
+    hipLaunchKernelGGL(k_intersect_skin, dim3((unsigned)(groups * R.nslots)), dim3(64), 0, s, g, P, R, part);
 }
 
 }  // namespace nsg
