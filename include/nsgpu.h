@@ -390,6 +390,31 @@ int nsgpu_read_normals(nsgpu_ctx *ctx, double *normals);
  * normal becomes the normalised sum of itself and its neighbours' average
  * (nsgpu_read_normals then returns the smoothed normals). */
 int nsgpu_smooth_surface(nsgpu_ctx *ctx);
+/* Boundary grid points, phase 2 of Surface::getSurf with
+ * Project_boundary_grid_points (Surface.cpp:1476-1588, projector :4657-4680),
+ * from the epsmap of the last get_surf: every node with 1 <= i, j, k < n
+ * whose six epsmap faces (its own three and the +x/+y/+z faces of its -x, -y,
+ * -z neighbours) are not all equal, in z, y, x loop order.  A point with a
+ * face outside is projected on the nearest feasible SES patch of its cell of
+ * the auxiliary grid of at most aux_grid_dim cells per side
+ * (Max_ses_patches_auxiliary_grid_size, default 50; ConnollySurface::
+ * buildAuxiliaryGrid ConnollySurface.cpp:3666-3909, getProjection
+ * :5822-6106): atom and probe spheres, tori (tested against their planes and
+ * self-intersection sphere, not their clipping sphere, as the reference
+ * does); with no feasible patch, the probe positions sampled along the circle
+ * of each self-intersecting regular torus of the cell (500 samples, kept when
+ * a facet on that edge accepts the point, needs the patches' atom ids); else
+ * the node itself.  Normals are zero, as the reference leaves them for the
+ * SES.  A point with no face outside (only with several dielectrics) keeps
+ * its node.  counts (may be NULL) [4]: points, points with no face outside,
+ * points placed by the sampling, points left at their node.  SES patch sets
+ * only (no skin cells, no mesh surface). */
+int nsgpu_project_bgp(nsgpu_ctx *ctx, int32_t aux_grid_dim, int64_t *counts);
+/* The last nsgpu_project_bgp: ibgp [n*3] node indices (0-based; the
+ * reference's DelPhi binding and saveBGP add 1, DelphiShared.cpp:286-290,
+ * :1437-1441), scspos [n*3] projections, scsnor [n*3] normals; any may be
+ * NULL. */
+int nsgpu_read_bgp(nsgpu_ctx *ctx, int32_t *ibgp, double *scspos, double *scsnor);
 /* Compute_Vertex_Normals: when on, get_surf keeps the patch of every depth so
  * that nsgpu_read_normals can follow the mesh (off by default, as in the
  * reference; it costs the sort a second permuted array). */

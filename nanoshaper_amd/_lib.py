@@ -87,6 +87,8 @@ SIGNATURES = {
     "nsgpu_read_mesh": (ctypes.c_int, [VP, VP, VP]),
     "nsgpu_read_normals": (ctypes.c_int, [VP, VP]),
     "nsgpu_smooth_surface": (ctypes.c_int, [VP]),
+    "nsgpu_project_bgp": (ctypes.c_int, [VP, ctypes.c_int32, VP]),
+    "nsgpu_read_bgp": (ctypes.c_int, [VP, VP, VP, VP]),
     "nsgpu_set_vertex_normals": (ctypes.c_int, [VP, ctypes.c_int]),
     "nsgpu_read_ray_hits": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_int32, VP, VP]),
     "nsgpu_set_mesh": (ctypes.c_int, [VP, VP, ctypes.c_int32, VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),

@@ -181,6 +181,13 @@ struct nsgpu_ctx {
     // patches, mesh, atoms); with the buffers' serials it tells a batch
     // whether its captured graph still matches this context
     uint64_t setgen = 0;
+    // boundary grid points of the last nsgpu_project_bgp: bitmask and word
+    // offsets, the auxiliary grid (node ranges, patch lists), facets by atom
+    // pair, outputs and counters
+    DBuf bgbits, bgwcount, bgwoff, bgrange, bgccount, bgcoff, bgccursor, bgclist, bgfkey, bgfval, bgib, bgpos, bgnor,
+        bgcnt;
+    int64_t nbgp = 0;
+    bool have_bgp = false;
     // stage timing
     bool profile = false;
     hipEvent_t ev[2 * ST_COUNT] = {};
@@ -286,7 +293,9 @@ static void for_each_buf(nsgpu_ctx *c, F fn) {
                    &c->segmask, &c->anode, &c->acase, &c->anv, &c->ant, &c->avoff, &c->atoff, &c->mverts, &c->mtris,
                    &c->mrr, &c->mraw, &c->field, &c->fbits, &c->batoms, &c->bbox, &c->bcount, &c->boff, &c->bcur,
                    &c->blist, &c->flag, &c->cgcount, &c->cgoff, &c->cgcursor, &c->cglist, &c->crorig,
-                   &c->crcoord, &c->crcount, &c->crhits, &c->crnorm};
+                   &c->crcoord, &c->crcount, &c->crhits, &c->crnorm, &c->bgbits, &c->bgwcount, &c->bgwoff,
+                   &c->bgrange, &c->bgccount, &c->bgcoff, &c->bgccursor, &c->bgclist, &c->bgfkey, &c->bgfval,
+                   &c->bgib, &c->bgpos, &c->bgnor, &c->bgcnt};
     for (DBuf *b : all) fn(b);
 }
 
@@ -810,6 +819,7 @@ static int begin_surf(nsgpu_ctx *c, const nsgpu_surf_opts *o) {
     c->opts = *o;
     std::memset(&c->res, 0, sizeof c->res);
     c->have_trace = c->have_surf = c->have_vertices = c->have_mesh = false;
+    c->have_bgp = false;
     c->panels[0] = 3;
     c->panels[1] = 3;
     if (!o->build_status && !o->build_epsmap) c->panels[0] = 0;
@@ -2294,6 +2304,185 @@ int nsgpu_batch_result(nsgpu_batch *b, int32_t f, nsgpu_surf_result *res) {
     if (!b || !res || f < 0 || f >= (int)b->frames.size()) return NSGPU_E_ARG;
     if (!b->frames[f]->have_surf) return bfail(b, NSGPU_E_STATE, "frame not computed");
     *res = b->frames[f]->res;
+    return NSGPU_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Boundary grid points (Surface.cpp:1476-1588) projected on the SES
+// (Surface::projector :4657-4680, ConnollySurface::getProjection
+// ConnollySurface.cpp:5822-6106), from the epsmap of the last getSurf.
+
+extern "C" {
+
+int nsgpu_project_bgp(nsgpu_ctx *c, int32_t aux_grid_dim, int64_t *counts) {
+    if (!c || aux_grid_dim < 1) return NSGPU_E_ARG;
+    if (!c->have_surf || !c->opts.build_epsmap)
+        return fail(c, NSGPU_E_STATE, "boundary grid points need a getSurf with the epsilon map");
+    if (c->surface_is_mesh || c->npatch_skin > 0)
+        return fail(c, NSGPU_E_STATE, "boundary grid point projection is built for SES patches only");
+    HIP_TRY(c, hipSetDevice(c->device));
+    c->have_bgp = false;
+    const nsgpu_grid &G = c->grid;
+    const DevGrid g = dev_grid(c);
+    const DevPatches P = dev_patches(c);
+    // auxiliary grid (ConnollySurface.cpp:3683-3742)
+    int64_t igrid = G.nx, gridMul = 1;
+    while (igrid > aux_grid_dim) {
+        gridMul += 2;
+        int64_t digrid = G.nx;
+        while (true) {
+            const int64_t fixedPoint = (digrid + (gridMul - 1)) / gridMul;
+            igrid = fixedPoint * gridMul;
+            if (igrid % 2 == 0) {
+                digrid = igrid + 1;
+            } else {
+                igrid = fixedPoint;
+                break;
+            }
+        }
+    }
+    const int na = (int)igrid;
+    if ((int64_t)na * na * na > INT32_MAX) return fail(c, NSGPU_E_ARG, "auxiliary grid too large");
+    BgpArgs A{};
+    A.na = na;
+    A.ascale = G.scale / ((double)gridMul);
+    const double aside = ((double)gridMul) / G.scale;
+    for (int d = 0; d < 3; ++d) A.amin[d] = G.baricenter[d] - (igrid - 1) * 0.5 * aside;
+    // grid nodes of each auxiliary cell, per axis: the node's cell is
+    // rintp((x - amin) * scale) (ConnollySurface.cpp:5835-5837)
+    std::vector<int> range((size_t)6 * na);
+    const double mins[3] = {G.xmin, G.ymin, G.zmin};
+    const int ns[3] = {G.nx, G.ny, G.nz};
+    for (int d = 0; d < 3; ++d) {
+        int *rg = range.data() + (size_t)2 * d * na;
+        for (int a = 0; a < na; ++a) {
+            rg[2 * a] = INT32_MAX;
+            rg[2 * a + 1] = -1;
+        }
+        for (int m = 0; m < ns[d]; ++m) {
+            const double coord = mins[d] + m * G.side;
+            int64_t a = (int64_t)std::floor((coord - A.amin[d]) * A.ascale + 0.5);
+            a = std::min<int64_t>(std::max<int64_t>(a, 0), na - 1);
+            rg[2 * a] = std::min(rg[2 * a], m);
+            rg[2 * a + 1] = std::max(rg[2 * a + 1], m);
+        }
+        for (int a = 0; a < na; ++a) {
+            if (rg[2 * a + 1] >= 0 && rg[2 * a + 1] - rg[2 * a] + 1 > 64)
+                return fail(c, NSGPU_E_ARG, "auxiliary grid too coarse: a cell spans more than 64 grid nodes");
+            if (rg[2 * a + 1] < 0) rg[2 * a] = 0;
+        }
+    }
+    // facets by atom pair, for the sampled projections
+    const int n = c->npatch;
+    std::vector<int32_t> kind(n), atom(3 * (size_t)n);
+    if (n) {
+        HIP_TRY(c, hipMemcpyAsync(kind.data(), c->pkind.p, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(atom.data(), c->patom.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
+    std::vector<std::pair<long long, int>> fk;
+    if (c->patch_max_atom >= 0)
+        for (int i = 0; i < n; ++i) {
+            if (kind[i] != NSGPU_PATCH_FACET_REG && kind[i] != NSGPU_PATCH_FACET_SING) continue;
+            const int *a = &atom[3 * (size_t)i];
+            const int pr[3][2] = {{a[0], a[1]}, {a[1], a[2]}, {a[0], a[2]}};
+            for (auto &q : pr)
+                fk.emplace_back(((long long)std::min(q[0], q[1]) << 32) | (unsigned)std::max(q[0], q[1]), i);
+        }
+    std::sort(fk.begin(), fk.end());
+    std::vector<long long> fkey(fk.size());
+    std::vector<int> fval(fk.size());
+    for (size_t q = 0; q < fk.size(); ++q) {
+        fkey[q] = fk[q].first;
+        fval[q] = fk[q].second;
+    }
+    // buffers
+    const int W = (G.nx + 63) / 64;
+    const int64_t nwords = (int64_t)G.ny * G.nz * W;
+    const int64_t ncell = (int64_t)na * na * na;
+    HIP_TRY(c, c->bgbits.ensure((size_t)nwords * 8));
+    HIP_TRY(c, c->bgwcount.ensure((size_t)nwords * 4));
+    HIP_TRY(c, c->bgwoff.ensure((size_t)nwords * 4));
+    HIP_TRY(c, c->bgrange.ensure(range.size() * 4));
+    HIP_TRY(c, c->bgccount.ensure((size_t)ncell * 4));
+    HIP_TRY(c, c->bgcoff.ensure((size_t)ncell * 4));
+    HIP_TRY(c, c->bgccursor.ensure((size_t)ncell * 4));
+    HIP_TRY(c, c->bgfkey.ensure(std::max<size_t>(fkey.size(), 1) * 8));
+    HIP_TRY(c, c->bgfval.ensure(std::max<size_t>(fval.size(), 1) * 4));
+    HIP_TRY(c, c->bgcnt.ensure(4 * 4));
+    HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(std::max(nwords, ncell)) * 4));
+    HIP_TRY(c, hipMemcpyAsync(c->bgrange.p, range.data(), range.size() * 4, hipMemcpyHostToDevice, c->stream));
+    if (!fkey.empty()) {
+        HIP_TRY(c, hipMemcpyAsync(c->bgfkey.p, fkey.data(), fkey.size() * 8, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->bgfval.p, fval.data(), fval.size() * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    int *cnt = c->bgcnt.as<int>();
+    launch_fill32(c->stream, cnt, 0u, 4);
+    // detection, offsets in the reference's z, y, x order
+    launch_bgp_flags(c->stream, g, c->eps.as<int>(), W, c->bgbits.as<unsigned long long>(), c->bgwcount.as<int>(),
+                     cnt + 2);
+    launch_scan(c->stream, c->bgwcount.as<int>(), nwords, c->bgwoff.as<int>(), c->scan_scratch.as<int>());
+    HIP_TRY(c, hipGetLastError());
+    int64_t nb = 0;
+    int rc = read_total(c, c->bgwcount.as<int>(), c->bgwoff.as<int>(), nwords, &nb);
+    if (rc) return rc;
+    if (nb > INT32_MAX / 3) return fail(c, NSGPU_E_NOMEM, "too many boundary grid points");
+    // auxiliary cell lists
+    A.eps = c->eps.as<int>();
+    A.W = W;
+    A.bits = c->bgbits.as<unsigned long long>();
+    A.woff = c->bgwoff.as<int>();
+    A.range = c->bgrange.as<int>();
+    A.ccount = c->bgccount.as<int>();
+    A.coff = c->bgcoff.as<int>();
+    A.ccursor = c->bgccursor.as<int>();
+    launch_bgp_bin(c->stream, P, A, false);
+    launch_scan(c->stream, A.ccount, ncell, A.coff, c->scan_scratch.as<int>());
+    HIP_TRY(c, hipGetLastError());
+    int64_t entries = 0;
+    rc = read_total(c, A.ccount, A.coff, ncell, &entries);
+    if (rc) return rc;
+    if (entries > INT32_MAX) return fail(c, NSGPU_E_NOMEM, "auxiliary cell lists exceed 2^31 entries");
+    HIP_TRY(c, c->bgclist.ensure((size_t)std::max<int64_t>(entries, 1) * 4));
+    HIP_TRY(c, c->bgib.ensure((size_t)std::max<int64_t>(nb, 1) * 12));
+    HIP_TRY(c, c->bgpos.ensure((size_t)std::max<int64_t>(nb, 1) * 24));
+    HIP_TRY(c, c->bgnor.ensure((size_t)std::max<int64_t>(nb, 1) * 24));
+    A.clist = c->bgclist.as<int>();
+    launch_bgp_bin(c->stream, P, A, true);
+    A.fkey = c->bgfkey.as<long long>();
+    A.fval = c->bgfval.as<int>();
+    A.nf = (int)fkey.size();
+    A.ibgp = c->bgib.as<int>();
+    A.pos = c->bgpos.as<double>();
+    A.nor = c->bgnor.as<double>();
+    A.counters = cnt;
+    if (nb > 0) launch_bgp_project(c->stream, g, P, A, c->probe);
+    HIP_TRY(c, hipGetLastError());
+    int hc[4] = {0, 0, 0, 0};
+    HIP_TRY(c, hipMemcpyAsync(hc, cnt, sizeof hc, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->nbgp = nb;
+    c->have_bgp = true;
+    if (counts) {
+        counts[0] = nb;
+        counts[1] = hc[2];
+        counts[2] = hc[0];
+        counts[3] = hc[1];
+    }
+    return NSGPU_OK;
+}
+
+int nsgpu_read_bgp(nsgpu_ctx *c, int32_t *ibgp, double *scspos, double *scsnor) {
+    if (!c) return NSGPU_E_ARG;
+    if (!c->have_bgp || !c->have_surf) return fail(c, NSGPU_E_STATE, "no boundary grid points computed");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const size_t n = (size_t)c->nbgp;
+    if (n && ibgp) HIP_TRY(c, hipMemcpyAsync(ibgp, c->bgib.p, n * 12, hipMemcpyDeviceToHost, c->stream));
+    if (n && scspos) HIP_TRY(c, hipMemcpyAsync(scspos, c->bgpos.p, n * 24, hipMemcpyDeviceToHost, c->stream));
+    if (n && scsnor) HIP_TRY(c, hipMemcpyAsync(scsnor, c->bgnor.p, n * 24, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
     return NSGPU_OK;
 }
 

@@ -235,4 +235,33 @@ struct MeshArgs {
 int64_t mesh_workgroups(int64_t nt, const DevRays &);
 void launch_intersect_mesh(hipStream_t, const DevGrid &, const DevRays &, const MeshArgs &, unsigned long long *part);
 
+
+// Boundary grid points (Surface.cpp:1476-1588) and their SES projection
+// (ConnollySurface::getProjection, ConnollySurface.cpp:5822-6106).
+struct BgpArgs {
+    const int *eps;                  // epsmap
+    int W;                           // bitmask words per grid row (ceil(nx / 64))
+    const unsigned long long *bits;  // [nz * ny * W] boundary-point bitmask
+    const int *woff;                 // exclusive scan of the words' popcounts
+    // auxiliary grid of na^3 cells (buildAuxiliaryGrid): node ranges of each
+    // cell per axis at range[2 * (axis * na + a)], [lo, hi]; patch lists
+    int na;
+    double amin[3], ascale;
+    const int *range;
+    int *ccount, *coff, *ccursor, *clist;
+    // facets by atom pair (key min << 32 | max, ascending) for the sampled
+    // projection near self-intersecting tori
+    const long long *fkey;
+    const int *fval;
+    int nf;
+    // outputs: node indices, projections, normals (zero, as the reference's)
+    int *ibgp;
+    double *pos, *nor;
+    int *counters;  // [0] sampled projections, [1] points left at their node
+};
+void launch_bgp_flags(hipStream_t, const DevGrid &, const int *eps, int W, unsigned long long *bits, int *wcount,
+                      int *n_internal);
+void launch_bgp_bin(hipStream_t, const DevPatches &, const BgpArgs &, bool fill);
+void launch_bgp_project(hipStream_t, const DevGrid &, const DevPatches &, const BgpArgs &, double probe);
+
 }  // namespace nsg

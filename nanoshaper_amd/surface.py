@@ -389,6 +389,22 @@ class DeviceSurface:
         when vertex normals are on)."""
         self._chk(self._lib.nsgpu_smooth_surface(self._h), "smooth_surface")
 
+    def project_bgp(self, aux_grid_dim=50):
+        """Project_boundary_grid_points on the maps of the last get_surf (with
+        the epsilon map; SES patches): {'ibgp' (n, 3) 0-based node indices in
+        z, y, x order, 'scspos' (n, 3) projections on the surface, 'scsnor'
+        (n, 3) normals (zero, as the reference's), 'counts' [points, points
+        with no face outside, points placed by the singular-edge sampling,
+        points left at their node]}."""
+        counts = np.zeros(4, np.int64)
+        self._chk(self._lib.nsgpu_project_bgp(self._h, int(aux_grid_dim), _ptr(counts)), "project_bgp")
+        n = int(counts[0])
+        ib = np.empty((n, 3), np.int32)
+        pos = np.empty((n, 3))
+        nor = np.empty((n, 3))
+        self._chk(self._lib.nsgpu_read_bgp(self._h, _ptr(ib), _ptr(pos), _ptr(nor)), "read_bgp")
+        return {"ibgp": ib, "scspos": pos, "scsnor": nor, "counts": counts}
+
     def set_vertex_normals(self, on=True):
         """Compute_Vertex_Normals: keep what normals() needs from the next get_surf on."""
         self._chk(self._lib.nsgpu_set_vertex_normals(self._h, int(on)), "set_vertex_normals")

@@ -63,6 +63,8 @@ def load():
                                             ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(VP)]),
         "orc_free_buf": (None, [VP]),
+        "orc_project_bgp": (ctypes.c_int, [VP, ctypes.c_int, VP]),
+        "orc_read_bgp": (None, [VP, VP, VP, VP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -218,6 +220,20 @@ class OracleSurface:
         out = np.empty((self.mesh_info["num_vertices"], 3))
         self._lib.orc_read_normals(self._h, _ptr(out))
         return out
+
+    def project_bgp(self, aux_grid_dim=50):
+        """Boundary grid points and their projections on the SES:
+        {'ibgp' (n, 3) node indices, 'scspos' (n, 3), 'scsnor' (n, 3),
+        'counts' [points, internal, sampled fallback, left at the node]}."""
+        counts = np.zeros(4, np.int64)
+        if self._lib.orc_project_bgp(self._h, int(aux_grid_dim), _ptr(counts)) != 0:
+            raise RuntimeError("oracle BGP projection needs an SES get_surf with the epsilon map")
+        n = int(counts[0])
+        ib = np.empty((n, 3), np.int32)
+        pos = np.empty((n, 3))
+        nor = np.empty((n, 3))
+        self._lib.orc_read_bgp(self._h, _ptr(ib), _ptr(pos), _ptr(nor))
+        return {"ibgp": ib, "scspos": pos, "scsnor": nor, "counts": counts}
 
     def ray_hits(self, slot):
         g = self.grid

@@ -541,6 +541,11 @@ typedef struct orc_surface {
     double *mv;  /* 3 per vertex */
     int32_t *mt; /* 3 per triangle */
     int64_t mnv, mnt;
+    /* boundary grid points of the last orc_project_bgp */
+    int64_t nbgp;
+    int32_t *ibgp;
+    double *scspos, *scsnor;
+    int64_t bgp_counts[4];
 } orc_surface;
 
 static void *xcalloc(size_t n, size_t s) { return calloc(n ? n : 1, s); }
@@ -638,6 +643,13 @@ static void free_results(orc_surface *s) {
     for (int d = 0; d < 3; d++) free(s->idx[d]);
     free(s->verts);
     free(s->tris);
+    free(s->ibgp);
+    free(s->scspos);
+    free(s->scsnor);
+    s->ibgp = NULL;
+    s->scspos = s->scsnor = NULL;
+    s->nbgp = 0;
+    memset(s->bgp_counts, 0, sizeof(s->bgp_counts));
     s->pix_off = NULL;
     s->pix_t = NULL;
     s->pix_patch = NULL;
@@ -2264,6 +2276,312 @@ int orc_smooth_surface(orc_surface *s, int normals) {
     free(tempN);
     free(num);
     return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Boundary grid points: phase 2 of Surface::getSurf (Surface.cpp:1476-1588)   */
+/* and Surface::projector (:4657-4680) over the SES projection of              */
+/* ConnollySurface: preBoundaryProjection/buildAuxiliaryGrid (:366-380,        */
+/* :3666-3909), getProjection (:5822-6106), projectToCircle/projectToTorus     */
+/* (:5693-5734), getCoi (:6175-6195), and tools.cpp's projectToSphere          */
+/* (:1112-1120), point2plane (:277-287) and plane3points (:258-274).           */
+/* The torus frame Rot is taken as invrot transposed (the patch SoA holds the */
+/* inverse rotation only; the reference's Rot equals it up to rounding), and  */
+/* the circle of the probe centres has the torus major radius as its radius.  */
+/* ------------------------------------------------------------------------- */
+
+/* tools.cpp:1112-1120 */
+static void orc_project_sphere(const double *y, const double *c, double r, double *proj, double *dist) {
+    double d = sqrt((y[0] - c[0]) * (y[0] - c[0]) + (y[1] - c[1]) * (y[1] - c[1]) + (y[2] - c[2]) * (y[2] - c[2]));
+    d = r / d;
+    proj[0] = y[0] - c[0];
+    proj[1] = y[1] - c[1];
+    proj[2] = y[2] - c[2];
+    proj[0] = c[0] + d * proj[0];
+    proj[1] = c[1] + d * proj[1];
+    proj[2] = c[2] + d * proj[2];
+    *dist = sqrt((proj[0] - y[0]) * (proj[0] - y[0]) + (proj[1] - y[1]) * (proj[1] - y[1]) +
+                 (proj[2] - y[2]) * (proj[2] - y[2]));
+}
+
+/* tools.cpp:277-287 (the distance is not needed here) */
+static void orc_point2plane(const double *p, const double *w, double *proj) {
+    double den = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    double val = (w[0] * p[0] + w[1] * p[1] + w[2] * p[2]) + w[3];
+    double c = val / den;
+    proj[0] = p[0] - w[0] * c;
+    proj[1] = p[1] - w[1] * c;
+    proj[2] = p[2] - w[2] * c;
+}
+
+/* ConnollySurface.cpp:5693-5702 */
+static void orc_project_circle(const double *point, double radius, const double *center, const double *plane,
+                               double *proj, double *dist) {
+    orc_point2plane(point, plane, proj);
+    proj[0] = proj[0] - center[0];
+    proj[1] = proj[1] - center[1];
+    proj[2] = proj[2] - center[2];
+    radius /= sqrt(proj[0] * proj[0] + proj[1] * proj[1] + proj[2] * proj[2]);
+    proj[0] = center[0] + radius * proj[0];
+    proj[1] = center[1] + radius * proj[1];
+    proj[2] = center[2] + radius * proj[2];
+    *dist = sqrt((point[0] - proj[0]) * (point[0] - proj[0]) + (point[1] - proj[1]) * (point[1] - proj[1]) +
+                 (point[2] - proj[2]) * (point[2] - proj[2]));
+}
+
+/* tools.cpp:258-274 with normalize = true */
+static void orc_plane3points(const double *p1, const double *p2, const double *p3, double *w) {
+    w[0] = p1[1] * (p2[2] - p3[2]) + p2[1] * (p3[2] - p1[2]) + p3[1] * (p1[2] - p2[2]);
+    w[1] = p1[2] * (p2[0] - p3[0]) + p2[2] * (p3[0] - p1[0]) + p3[2] * (p1[0] - p2[0]);
+    w[2] = p1[0] * (p2[1] - p3[1]) + p2[0] * (p3[1] - p1[1]) + p3[0] * (p1[1] - p2[1]);
+    w[3] = -(w[0] * p1[0] + w[1] * p1[1] + w[2] * p1[2]);
+    double norm = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    w[0] /= norm;
+    w[1] /= norm;
+    w[2] /= norm;
+    w[3] /= norm;
+}
+
+/* ConnollySurface.cpp:5705-5734; td is the torus record (centre, invrot, R) */
+static void orc_project_torus(const double *y, const double *td, double probe, double *proj, double *dist) {
+    const double *iv = td + 3;
+    double temp[3], pp[3], proj_major[3], minor_plane[4];
+    const double torus_plane[4] = {0, 0, +1, 0}, torus_center[3] = {0, 0, 0};
+    temp[0] = y[0] - td[0];
+    temp[1] = y[1] - td[1];
+    temp[2] = y[2] - td[2];
+    for (int i = 0; i < 3; i++) pp[i] = iv[3 * i] * temp[0] + iv[3 * i + 1] * temp[1] + iv[3 * i + 2] * temp[2];
+    orc_project_circle(pp, td[12], torus_center, torus_plane, proj_major, dist);
+    const double p2[3] = {0, 0, +1}, p3[3] = {0, 0, -1};
+    orc_plane3points(proj_major, p2, p3, minor_plane);
+    orc_project_circle(pp, probe, proj_major, minor_plane, proj, dist);
+    /* Rot[i][j] = invrot[j][i] */
+    for (int i = 0; i < 3; i++) pp[i] = iv[i] * proj[0] + iv[3 + i] * proj[1] + iv[6 + i] * proj[2];
+    proj[0] = pp[0] + td[0];
+    proj[1] = pp[1] + td[1];
+    proj[2] = pp[2] + td[2];
+    *dist = sqrt((proj[0] - y[0]) * (proj[0] - y[0]) + (proj[1] - y[1]) * (proj[1] - y[1]) +
+                 (proj[2] - y[2]) * (proj[2] - y[2]));
+}
+
+/* the auxiliary grid of buildAuxiliaryGrid (ConnollySurface.cpp:3683-3742) */
+static void orc_bgp_aux_grid(const nsgpu_grid *g, int aux_dim, int64_t *n_aux, double *scale, double *side,
+                             double amin[3]) {
+    int64_t igrid = g->nx;
+    int64_t gridMul = 1;
+    while (igrid > aux_dim) {
+        gridMul += 2;
+        int64_t digrid = g->nx;
+        while (1) {
+            int64_t fixedPoint = (digrid + (gridMul - 1)) / gridMul;
+            igrid = fixedPoint * gridMul;
+            if (igrid % 2 == 0)
+                digrid = igrid + 1;
+            else {
+                igrid = fixedPoint;
+                break;
+            }
+        }
+    }
+    *scale = g->scale / ((double)gridMul);
+    *side = ((double)gridMul) / g->scale;
+    for (int d = 0; d < 3; d++) amin[d] = g->baricenter[d] - (igrid - 1) * 0.5 * (*side);
+    *n_aux = igrid;
+}
+
+/* counts: [0] boundary grid points, [1] internal ones (no face outside: only
+ * with several dielectrics, left at the grid point), [2] projected by the
+ * singular-edge sampling, [3] left at the grid point (no feasible projection,
+ * or an empty auxiliary cell, where the reference writes nothing).
+ * Returns -1 for a surface without SES patches. */
+int orc_project_bgp(orc_surface *s, int aux_dim, int64_t *counts) {
+    if (!s->epsmap || s->is_mesh) return -1;
+    const nsgpu_patches *P = &s->P;
+    for (int it = 0; it < P->n; it++)
+        if (P->kind[it] > NSGPU_PATCH_TORUS_SING) return -1;
+    free(s->ibgp);
+    free(s->scspos);
+    free(s->scsnor);
+    s->ibgp = NULL;
+    s->scspos = s->scsnor = NULL;
+    memset(s->bgp_counts, 0, sizeof(s->bgp_counts));
+    const int64_t NX = s->NX, NY = s->NY, NZ = s->NZ;
+    const int32_t *E = s->epsmap;
+#define EPS_AT(ix, iy, iz, l) E[(((iz) * NY + (iy)) * NX + (ix)) * 3 + (l)]
+    /* detection, Surface.cpp:1484-1542 */
+    int64_t cap = 1024, nb = 0;
+    int32_t *ib = (int32_t *)malloc(cap * 3 * sizeof(int32_t));
+    uint8_t *ext = (uint8_t *)malloc(cap);
+    for (int64_t iz = 1; iz < NZ; iz++)
+        for (int64_t iy = 1; iy < NY; iy++)
+            for (int64_t ix = 1; ix < NX; ix++) {
+                const int konst = EPS_AT(ix, iy, iz, 0);
+                const int f1 = EPS_AT(ix, iy, iz, 1), f2 = EPS_AT(ix, iy, iz, 2);
+                const int b0 = EPS_AT(ix - 1, iy, iz, 0), b1 = EPS_AT(ix, iy - 1, iz, 1), b2 = EPS_AT(ix, iy, iz - 1, 2);
+                if (konst != f1 || konst != f2 || konst != b0 || konst != b1 || konst != b2) {
+                    if (nb == cap) {
+                        cap *= 2;
+                        ib = (int32_t *)realloc(ib, cap * 3 * sizeof(int32_t));
+                        ext = (uint8_t *)realloc(ext, cap);
+                    }
+                    ib[3 * nb] = (int32_t)ix;
+                    ib[3 * nb + 1] = (int32_t)iy;
+                    ib[3 * nb + 2] = (int32_t)iz;
+                    ext[nb] = (konst == 0 || f1 == 0 || f2 == 0 || b0 == 0 || b1 == 0 || b2 == 0);
+                    nb++;
+                }
+            }
+#undef EPS_AT
+    /* auxiliary grid: every patch in the cells its bounding box covers
+     * (ConnollySurface.cpp:3792-3897), lists in ascending patch order */
+    int64_t na;
+    double ascale, aside, amin[3];
+    orc_bgp_aux_grid(&s->g, aux_dim, &na, &ascale, &aside, amin);
+    const int64_t ncell = na * na * na;
+    int64_t *coff = (int64_t *)xcalloc(ncell + 1, sizeof(int64_t));
+    int64_t(*box)[6] = xcalloc((size_t)P->n, sizeof(*box));
+    for (int it = 0; it < P->n; it++) {
+        const double *c = P->bsph + 4 * it;
+        const double radius = c[3];
+        double down[3] = {c[0] - radius, c[1] - radius, c[2] - radius};
+        double up[3] = {c[0] + radius, c[1] + radius, c[2] + radius};
+        for (int d = 0; d < 3; d++) {
+            int64_t a = (int64_t)rintp(fmax(0., down[d] - amin[d]) * ascale);
+            int64_t b = (int64_t)rintp(fmax(0., up[d] - amin[d]) * ascale);
+            if (a >= na) a = na - 1;
+            if (b >= na) b = na - 1;
+            box[it][d] = a;
+            box[it][3 + d] = b;
+        }
+        for (int64_t iz = box[it][2]; iz <= box[it][5]; iz++)
+            for (int64_t iy = box[it][1]; iy <= box[it][4]; iy++)
+                for (int64_t ix = box[it][0]; ix <= box[it][3]; ix++) coff[(iz * na + iy) * na + ix + 1]++;
+    }
+    for (int64_t q = 0; q < ncell; q++) coff[q + 1] += coff[q];
+    int32_t *clist = (int32_t *)xcalloc((size_t)coff[ncell], sizeof(int32_t));
+    int64_t *cur = (int64_t *)xcalloc(ncell, sizeof(int64_t));
+    for (int it = 0; it < P->n; it++)
+        for (int64_t iz = box[it][2]; iz <= box[it][5]; iz++)
+            for (int64_t iy = box[it][1]; iy <= box[it][4]; iy++)
+                for (int64_t ix = box[it][0]; ix <= box[it][3]; ix++) {
+                    const int64_t q = (iz * na + iy) * na + ix;
+                    clist[coff[q] + cur[q]++] = it;
+                }
+    free(cur);
+    free(box);
+    /* projection of every boundary grid point (Surface.cpp:4657-4680,
+     * ConnollySurface.cpp:5822-6106) */
+    double *pos = (double *)xcalloc((size_t)nb * 3, sizeof(double));
+    double *nor = (double *)xcalloc((size_t)nb * 3, sizeof(double));
+    int64_t n_int = 0, n_fixed = 0, n_approx = 0;
+    const double probe = s->probe;
+    const double step = (2.0 * ORC_PI) / 500;
+    for (int64_t i = 0; i < nb; i++) {
+        const double p[3] = {s->x[ib[3 * i]], s->y[ib[3 * i + 1]], s->z[ib[3 * i + 2]]};
+        double *proj = pos + 3 * i;
+        proj[0] = p[0];
+        proj[1] = p[1];
+        proj[2] = p[2];
+        if (!ext[i]) {
+            n_int++;
+            continue;
+        }
+        int64_t ir[3];
+        for (int d = 0; d < 3; d++) {
+            ir[d] = (int64_t)rintp((p[d] - amin[d]) * ascale);
+            if (ir[d] < 0) ir[d] = 0;
+            if (ir[d] >= na) ir[d] = na - 1;
+        }
+        const int64_t q = (ir[2] * na + ir[1]) * na + ir[0];
+        const int32_t *cells = clist + coff[q];
+        const int64_t ncl = coff[q + 1] - coff[q];
+        if (ncl == 0) {
+            n_approx++;
+            continue;
+        }
+        double minDist = INFINITY, loc[3], dist;
+        for (int64_t k = 0; k < ncl; k++) {
+            const int it = cells[k];
+            const int kind = P->kind[it];
+            if (kind == NSGPU_PATCH_POINT || kind == NSGPU_PATCH_FACET_REG || kind == NSGPU_PATCH_FACET_SING)
+                orc_project_sphere(p, P->bsph + 4 * it, kind == NSGPU_PATCH_POINT ? P->bsph[4 * it + 3] : probe, loc,
+                                   &dist);
+            else
+                orc_project_torus(p, P->data + P->off[it], probe, loc, &dist);
+            if (!orc_is_feasible(P, it, loc)) continue;
+            if (dist < minDist) {
+                minDist = dist;
+                proj[0] = loc[0];
+                proj[1] = loc[1];
+                proj[2] = loc[2];
+            }
+        }
+        if (minDist != INFINITY) continue;
+        /* no feasible projection: sample the probe positions along the circle
+         * of every self-intersecting regular torus of the cell and keep the
+         * nearest probe-sphere projection feasible for a facet on that edge
+         * (ConnollySurface.cpp:5916-6086) */
+        int fixed = 0;
+        for (int64_t k = 0; k < ncl; k++) {
+            const int it = cells[k];
+            if (P->kind[it] != NSGPU_PATCH_TORUS_REG) continue;
+            const double *td = P->data + P->off[it];
+            if (!(td[13] >= 0.)) continue;
+            const int ref1 = P->atom[3 * it], ref2 = P->atom[3 * it + 1];
+            /* the facets incident to the edge (both atoms among theirs) */
+            int fcv[64], nfc = 0;
+            for (int f = 0; f < P->n && nfc < 64; f++) {
+                const int fk = P->kind[f];
+                if (fk != NSGPU_PATCH_FACET_REG && fk != NSGPU_PATCH_FACET_SING) continue;
+                const int32_t *a = P->atom + 3 * f;
+                const int h1 = a[0] == ref1 || a[1] == ref1 || a[2] == ref1;
+                const int h2 = a[0] == ref2 || a[1] == ref2 || a[2] == ref2;
+                if (h1 && h2) fcv[nfc++] = f;
+            }
+            const double *iv = td + 3;
+            const double v[3] = {iv[0], iv[1], iv[2]}, u[3] = {iv[3], iv[4], iv[5]};
+            double tcoi = 0.;
+            for (int kk = 0; kk < 500; kk++) {
+                double sp[3];
+                for (int d = 0; d < 3; d++) sp[d] = td[d] + td[12] * (cos(tcoi) * u[d] + sin(tcoi) * v[d]);
+                tcoi += step;
+                orc_project_sphere(p, sp, probe, loc, &dist);
+                for (int ll = 0; ll < nfc; ll++) {
+                    if (!orc_is_feasible(P, fcv[ll], loc)) continue;
+                    if (dist < minDist) {
+                        minDist = dist;
+                        proj[0] = loc[0];
+                        proj[1] = loc[1];
+                        proj[2] = loc[2];
+                        fixed = 1;
+                    }
+                }
+            }
+        }
+        if (fixed)
+            n_fixed++;
+        else
+            n_approx++;
+    }
+    free(coff);
+    free(clist);
+    free(ext);
+    s->nbgp = nb;
+    s->ibgp = ib;
+    s->scspos = pos;
+    s->scsnor = nor;
+    s->bgp_counts[0] = nb;
+    s->bgp_counts[1] = n_int;
+    s->bgp_counts[2] = n_fixed;
+    s->bgp_counts[3] = n_approx;
+    if (counts) memcpy(counts, s->bgp_counts, sizeof(s->bgp_counts));
+    return 0;
+}
+
+void orc_read_bgp(const orc_surface *s, int32_t *ibgp, double *scspos, double *scsnor) {
+    if (ibgp) memcpy(ibgp, s->ibgp, (size_t)s->nbgp * 3 * sizeof(int32_t));
+    if (scspos) memcpy(scspos, s->scspos, (size_t)s->nbgp * 3 * sizeof(double));
+    if (scsnor) memcpy(scsnor, s->scsnor, (size_t)s->nbgp * 3 * sizeof(double));
 }
 
 void orc_free_buf(void *p) { free(p); }
