@@ -20,7 +20,9 @@ of the plain path are measured alongside:
 * "blobby": the blobby surface of 20,000 atoms at 2 points/A, built and ray
   cast as a mesh on the device, with the oracle's time for the same work;
 * "skin": the skin surface (s = 0.45) of the same atoms, its mixed cells
-  built on the host and cast on the device, with the oracle's time.
+  built on the host and cast on the device, with the oracle's time;
+* "bgp": the boundary grid points of the benchmark surface projected on the
+  SES (Project_boundary_grid_points), with the oracle's time.
 """
 import argparse
 import json
@@ -433,6 +435,38 @@ def bench_skin(steps=10, cpu=True):
     return res
 
 
+def bench_bgp(dev, grid, ps, steps=10, cpu=True):
+    """Project_boundary_grid_points on the benchmark surface: detection from
+    the epsmap, the auxiliary cell lists and the projections, timed per call
+    of the device stage (results left on the device), beside the oracle's
+    single-threaded restatement on the same maps."""
+    r = dev.get_surf(build_epsmap=True, build_status=True, accurate_triangulation=False, fill_cavities=True)
+    counts = dev.project_bgp_device()
+    dev.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        counts = dev.project_bgp_device()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    n = int(counts[0])
+    out = {"workload": "the benchmark SES (%d^3), maps + cavities, auxiliary grid <= 50 cells per side" % grid.nx,
+           "ms_per_call": ms, "points": n, "points_per_s": n / (ms * 1e-3),
+           "counts": {"points": n, "no_face_outside": int(counts[1]), "sampled": int(counts[2]),
+                      "left_at_node": int(counts[3])}}
+    if cpu:
+        from oracle.binding import OracleSurface
+        orc = OracleSurface(grid, ps)
+        orc.get_surf(build_epsmap=True, build_status=True, accurate_triangulation=False, fill_cavities=True)
+        t0 = time.perf_counter()
+        bo = orc.project_bgp()
+        cs = time.perf_counter() - t0
+        orc.close()
+        out["cpu_baseline"] = {"seconds_per_call": cs, "cores": 1, "kind": "port",
+                               "sample": "full workload, 1 run (the oracle's projection is single threaded)",
+                               "points_equal": bool(int(bo["counts"][0]) == n)}
+        out["speedup_vs_cpu"] = cs / (ms * 1e-3)
+    return out
+
+
 def copy_reference(nbytes, repeats=5):
     """Plain device-to-device copy of known size: the calibration kernel for
     the FETCH_SIZE / WRITE_SIZE counters (a copy reads and writes nbytes)."""
@@ -464,6 +498,7 @@ def main():
                     help="skip the mixed, batched, differentiable and blobby lines")
     ap.add_argument("--blobby-only", action="store_true", help="only the blobby line (profiling)")
     ap.add_argument("--skin-only", action="store_true", help="only the skin line (profiling)")
+    ap.add_argument("--bgp-only", action="store_true", help="only the boundary-grid-point line (profiling)")
     args = ap.parse_args()
     if args.copy_ref:
         copy_reference(args.copy_ref)
@@ -473,6 +508,14 @@ def main():
         return
     if args.skin_only:
         print(json.dumps({"skin": bench_skin(args.steps, cpu=not args.no_cpu)}))
+        return
+    if args.bgp_only:
+        xyzr, r_used, grid, ps, host_s = build_inputs(args.config)
+        dev = ns.DeviceSurface(0)
+        dev.set_grid(grid)
+        dev.set_patches(ps)
+        print(json.dumps({"bgp": bench_bgp(dev, grid, ps, args.steps, cpu=not args.no_cpu)}))
+        dev.close()
         return
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -556,6 +599,8 @@ def main():
                                                   "(tests/test_gpu_parity.py::test_mixed_precision_matches_oracle)"}
         print("cell grid ray tiles ...", file=sys.stderr, flush=True)
         extras["cast_modes"] = bench_cast_tiles(dev, args.steps, args.warmup)
+        print("boundary grid points ...", file=sys.stderr, flush=True)
+        extras["bgp"] = bench_bgp(dev, grid, ps, cpu=not args.no_cpu)
     dev.close()
     if not args.no_extras:
         print("batched frames ...", file=sys.stderr, flush=True)

@@ -132,6 +132,39 @@ class DelPhiShared {
     std::vector<int32_t> status;   // -1 in, 3 out, >=4 cavity
     std::vector<double> cavitiesSize;
     std::vector<uint8_t> cavitiesFlag;
+    // boundary grid points of getSurf with projBGP (Surface.cpp:1544-1575):
+    // ibgp [nbgp*3] 0-based node indices, scspos/scsnor [nbgp*3]
+    int nbgp = 0;
+    std::vector<int32_t> ibgp;
+    std::vector<double> scspos, scsnor;
+
+    // DelPhiShared::saveBGP (DelphiShared.cpp:1421-1455): rootFile + fname +
+    // ".projections.txt" (count, then 1-based indices, projection and normal
+    // per line) and rootFile + "surf.txt" (projection and normal per line)
+    bool saveBGP(const std::string &rootFile, const std::string &fname) const {
+        if (scspos.size() < (size_t)nbgp * 3 || scsnor.size() < (size_t)nbgp * 3) return false;
+        FILE *fp = std::fopen((rootFile + fname + ".projections.txt").c_str(), "w");
+        FILE *fp2 = std::fopen((rootFile + "surf.txt").c_str(), "w");
+        if (!fp || !fp2) {
+            if (fp) std::fclose(fp);
+            if (fp2) std::fclose(fp2);
+            return false;
+        }
+        std::fprintf(fp, "%d\n", nbgp);
+        for (int i = 0; i < nbgp; ++i) {
+            const double *p = &scspos[3 * (size_t)i], *q = &scsnor[3 * (size_t)i];
+            std::fprintf(fp, "%d %d %d %lf %lf %lf %lf %lf %lf", ibgp[3 * (size_t)i] + 1, ibgp[3 * (size_t)i + 1] + 1,
+                         ibgp[3 * (size_t)i + 2] + 1, p[0], p[1], p[2], q[0], q[1], q[2]);
+            std::fprintf(fp2, "%lf %lf %lf %lf %lf %lf", p[0], p[1], p[2], q[0], q[1], q[2]);
+            if (i < nbgp - 1) {
+                std::fprintf(fp, "\n");
+                std::fprintf(fp2, "\n");
+            }
+        }
+        std::fclose(fp);
+        std::fclose(fp2);
+        return true;
+    }
 };
 
 // Base class of the surfaces; the ray caster is the device path.
@@ -147,6 +180,8 @@ class Surface {
             device = cf->read<int>("Device", device);
             patchBased = cf->read<bool>("Patch_Based_Algorithm", patchBased);
             auxGrid2D = cf->read<int>("Max_mesh_auxiliary_grid_2d_size", auxGrid2D);
+            projBGP = cf->read<bool>("Project_boundary_grid_points", projBGP);
+            auxGrid3D = cf->read<int>("Max_ses_patches_auxiliary_grid_size", auxGrid3D);
         }
     }
     virtual ~Surface() {
@@ -192,6 +227,18 @@ class Surface {
                 nsgpu_read_cavities(ctx_, delphi->cavitiesSize.data(), delphi->cavitiesFlag.data()) != NSGPU_OK)
                 return false;
         }
+        // phase 2: boundary grid points projected on the surface (SES)
+        delphi->nbgp = 0;
+        if (projBGP && delphi->buildEpsMap) {
+            int64_t counts[4] = {0, 0, 0, 0};
+            if (nsgpu_project_bgp(ctx_, auxGrid3D, counts) != NSGPU_OK) return false;
+            delphi->nbgp = (int)counts[0];
+            delphi->ibgp.resize((size_t)counts[0] * 3);
+            delphi->scspos.resize((size_t)counts[0] * 3);
+            delphi->scsnor.resize((size_t)counts[0] * 3);
+            if (nsgpu_read_bgp(ctx_, delphi->ibgp.data(), delphi->scspos.data(), delphi->scsnor.data()) != NSGPU_OK)
+                return false;
+        }
         return true;
     }
 
@@ -228,6 +275,9 @@ class Surface {
         return true;
     }
     void setComputeNormals(bool on) { computeNormals = on; }
+    // Surface::setProjBGP / getProjBGP (Surface.h:1054-1062)
+    void setProjBGP(bool on) { projBGP = on; }
+    bool getProjBGP() const { return projBGP; }
     bool getComputeNormals() const { return computeNormals; }
     void setRandDisplacement(double r) { randDisplacement = r; }
     int64_t getNumVertices() const { return (int64_t)(vertList.size() / 3); }
@@ -251,6 +301,8 @@ class Surface {
     int device = 0;
     bool patchBased = true;  // Patch_Based_Algorithm (triangle meshes; ses and skin are always patch based)
     int auxGrid2D = 100;     // Max_mesh_auxiliary_grid_2d_size
+    bool projBGP = false;    // Project_boundary_grid_points (SES patch sets)
+    int auxGrid3D = 50;      // Max_ses_patches_auxiliary_grid_size
     std::vector<double> vertList;
     std::vector<int32_t> triList;
     std::vector<double> normList;  // filled when computeNormals

@@ -92,6 +92,8 @@ class Configuration:
         self.save_msms = r("Save_Mesh_MSMS_Format", False)
         self.save_ply = r("Save_Mesh_PLY_Format", False)
         self.vertex_atom_info = r("Vertex_Atom_Info", False)
+        self.save_bgps = r("Save_bgps", False)
+        self.aux_grid_3d = r("Max_ses_patches_auxiliary_grid_size", 50)
         if not self.build_epsmaps and self.save_epsmaps:
             raise ConfigError("Asked to save epsmap without builiding it: set Build_epsilon_maps = true")
         if not self.build_epsmaps and self.proj_bgp:
@@ -154,9 +156,11 @@ def run(conf, atoms, device=0):
         dev.set_patches(ps)
     dev.set_vertex_normals(conf.normals and conf.acc_tri and conf.tri)
     res = dev.get_surf(**conf.surf_opts())
+    # phase 2 of getSurf: boundary grid points (Surface.cpp:1476-1588)
+    bgp = dev.project_bgp(conf.aux_grid_3d) if conf.proj_bgp else None
     mesh = dev.triangulate() if (conf.tri and conf.acc_tri) else None
     return {"atoms": atoms, "jittered": jit, "grid": grid, "patches": ps, "device": dev, "result": res,
-            "mesh": mesh}
+            "mesh": mesh, "bgp": bgp}
 
 
 def write_outputs(out, directory="."):
@@ -188,6 +192,9 @@ def write_outputs(out, directory="."):
             IO.save_status(paths[-1], st)
         IO.save_cavities(root, grid, st, sizes, filled)
         paths += [root + "cavities.txt", root + "cavitiesSize.txt", root + "cavAtomsSerials.txt"]
+    if conf.save_bgps and out.get("bgp") is not None:
+        b = out["bgp"]
+        paths += IO.save_bgp(root, name, b["ibgp"], b["scspos"], b["scsnor"])
     if conf.save_idebmap:
         paths.append(root + name + ".idebmap.txt")
         IO.save_idebmap(paths[-1], dev.idebmap())

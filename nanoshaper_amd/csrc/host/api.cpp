@@ -185,9 +185,14 @@ struct nsgpu_ctx {
     // offsets, the auxiliary grid (node ranges, patch lists), facets by atom
     // pair, outputs and counters
     DBuf bgbits, bgwcount, bgwoff, bgrange, bgccount, bgcoff, bgccursor, bgclist, bgfkey, bgfval, bgib, bgpos, bgnor,
-        bgcnt;
+        bgcnt, bgcoi, bgnaux, bgcflag, bgcoffa, bgactive, bgfq;
     int64_t nbgp = 0;
     bool have_bgp = false;
+    // the cached tables of bgp_tables and what they were built for
+    BgpArgs bgA{};
+    uint64_t bg_setgen = 0;
+    int bg_aux = -1;
+    bool bg_valid = false;
     // stage timing
     bool profile = false;
     hipEvent_t ev[2 * ST_COUNT] = {};
@@ -295,7 +300,8 @@ static void for_each_buf(nsgpu_ctx *c, F fn) {
                    &c->blist, &c->flag, &c->cgcount, &c->cgoff, &c->cgcursor, &c->cglist, &c->crorig,
                    &c->crcoord, &c->crcount, &c->crhits, &c->crnorm, &c->bgbits, &c->bgwcount, &c->bgwoff,
                    &c->bgrange, &c->bgccount, &c->bgcoff, &c->bgccursor, &c->bgclist, &c->bgfkey, &c->bgfval,
-                   &c->bgib, &c->bgpos, &c->bgnor, &c->bgcnt};
+                   &c->bgib, &c->bgpos, &c->bgnor, &c->bgcnt, &c->bgcoi, &c->bgnaux, &c->bgcflag,
+                   &c->bgcoffa, &c->bgactive, &c->bgfq};
     for (DBuf *b : all) fn(b);
 }
 
@@ -2314,20 +2320,15 @@ int nsgpu_batch_result(nsgpu_batch *b, int32_t f, nsgpu_surf_result *res) {
 // (Surface::projector :4657-4680, ConnollySurface::getProjection
 // ConnollySurface.cpp:5822-6106), from the epsmap of the last getSurf.
 
-extern "C" {
-
-int nsgpu_project_bgp(nsgpu_ctx *c, int32_t aux_grid_dim, int64_t *counts) {
-    if (!c || aux_grid_dim < 1) return NSGPU_E_ARG;
-    if (!c->have_surf || !c->opts.build_epsmap)
-        return fail(c, NSGPU_E_STATE, "boundary grid points need a getSurf with the epsilon map");
-    if (c->surface_is_mesh || c->npatch_skin > 0)
-        return fail(c, NSGPU_E_STATE, "boundary grid point projection is built for SES patches only");
-    HIP_TRY(c, hipSetDevice(c->device));
-    c->have_bgp = false;
+// The tables of the projection that depend only on the grid, the patch set
+// and the auxiliary grid size: the auxiliary grid (ConnollySurface.cpp:
+// 3683-3742) with each cell's grid-node ranges and patch lists, the facets by
+// atom pair and the sample angles; built once per (inputs, aux_grid_dim).
+static int bgp_tables(nsgpu_ctx *c, int32_t aux_grid_dim) {
+    if (c->bg_valid && c->bg_setgen == c->setgen && c->bg_aux == aux_grid_dim) return NSGPU_OK;
+    c->bg_valid = false;
     const nsgpu_grid &G = c->grid;
-    const DevGrid g = dev_grid(c);
     const DevPatches P = dev_patches(c);
-    // auxiliary grid (ConnollySurface.cpp:3683-3742)
     int64_t igrid = G.nx, gridMul = 1;
     while (igrid > aux_grid_dim) {
         gridMul += 2;
@@ -2355,6 +2356,8 @@ int nsgpu_project_bgp(nsgpu_ctx *c, int32_t aux_grid_dim, int64_t *counts) {
     std::vector<int> range((size_t)6 * na);
     const double mins[3] = {G.xmin, G.ymin, G.zmin};
     const int ns[3] = {G.nx, G.ny, G.nz};
+    const int NN = std::max(G.nx, std::max(G.ny, G.nz));
+    std::vector<int> node_aux((size_t)3 * NN, 0);
     for (int d = 0; d < 3; ++d) {
         int *rg = range.data() + (size_t)2 * d * na;
         for (int a = 0; a < na; ++a) {
@@ -2365,6 +2368,7 @@ int nsgpu_project_bgp(nsgpu_ctx *c, int32_t aux_grid_dim, int64_t *counts) {
             const double coord = mins[d] + m * G.side;
             int64_t a = (int64_t)std::floor((coord - A.amin[d]) * A.ascale + 0.5);
             a = std::min<int64_t>(std::max<int64_t>(a, 0), na - 1);
+            node_aux[(size_t)d * NN + m] = (int)a;
             rg[2 * a] = std::min(rg[2 * a], m);
             rg[2 * a + 1] = std::max(rg[2 * a + 1], m);
         }
@@ -2398,42 +2402,47 @@ int nsgpu_project_bgp(nsgpu_ctx *c, int32_t aux_grid_dim, int64_t *counts) {
         fkey[q] = fk[q].first;
         fval[q] = fk[q].second;
     }
-    // buffers
-    const int W = (G.nx + 63) / 64;
-    const int64_t nwords = (int64_t)G.ny * G.nz * W;
+    // each torus patch's facets: the run of its atom pair's key
+    std::vector<int> efac(2 * (size_t)std::max(n, 1), 0);
+    if (c->patch_max_atom >= 0)
+        for (int i = 0; i < n; ++i) {
+            if (kind[i] != NSGPU_PATCH_TORUS_REG && kind[i] != NSGPU_PATCH_TORUS_SING) continue;
+            const int a = atom[3 * (size_t)i], b = atom[3 * (size_t)i + 1];
+            const long long key = ((long long)std::min(a, b) << 32) | (unsigned)std::max(a, b);
+            const auto r = std::equal_range(fkey.begin(), fkey.end(), key);
+            efac[2 * (size_t)i] = (int)(r.first - fkey.begin());
+            efac[2 * (size_t)i + 1] = (int)(r.second - r.first);
+        }
+    // the sample angles of getCoi (ConnollySurface.cpp:6175-6195): the angle
+    // accumulated step by step, cos and sin of each
+    std::vector<double> coi(2 * kCoiSamples);
+    const double step = (2.0 * 3.14159265358979323846264338327950288) / kCoiSamples;
+    double tcoi = 0.;
+    for (int k = 0; k < kCoiSamples; ++k) {
+        coi[2 * k] = std::cos(tcoi);
+        coi[2 * k + 1] = std::sin(tcoi);
+        tcoi += step;
+    }
     const int64_t ncell = (int64_t)na * na * na;
-    HIP_TRY(c, c->bgbits.ensure((size_t)nwords * 8));
-    HIP_TRY(c, c->bgwcount.ensure((size_t)nwords * 4));
-    HIP_TRY(c, c->bgwoff.ensure((size_t)nwords * 4));
     HIP_TRY(c, c->bgrange.ensure(range.size() * 4));
     HIP_TRY(c, c->bgccount.ensure((size_t)ncell * 4));
     HIP_TRY(c, c->bgcoff.ensure((size_t)ncell * 4));
     HIP_TRY(c, c->bgccursor.ensure((size_t)ncell * 4));
-    HIP_TRY(c, c->bgfkey.ensure(std::max<size_t>(fkey.size(), 1) * 8));
+    HIP_TRY(c, c->bgfkey.ensure(efac.size() * 4));
     HIP_TRY(c, c->bgfval.ensure(std::max<size_t>(fval.size(), 1) * 4));
-    HIP_TRY(c, c->bgcnt.ensure(4 * 4));
-    HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(std::max(nwords, ncell)) * 4));
+    HIP_TRY(c, c->bgcoi.ensure(coi.size() * 8));
+    HIP_TRY(c, c->bgnaux.ensure(node_aux.size() * 4));
+    HIP_TRY(c, c->bgcflag.ensure((size_t)ncell * 4));
+    HIP_TRY(c, c->bgcoffa.ensure((size_t)ncell * 4));
+    HIP_TRY(c, c->bgactive.ensure((size_t)ncell * 4));
+    HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(ncell) * 4));
+    HIP_TRY(c, hipMemcpyAsync(c->bgnaux.p, node_aux.data(), node_aux.size() * 4, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->bgrange.p, range.data(), range.size() * 4, hipMemcpyHostToDevice, c->stream));
-    if (!fkey.empty()) {
-        HIP_TRY(c, hipMemcpyAsync(c->bgfkey.p, fkey.data(), fkey.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->bgcoi.p, coi.data(), coi.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->bgfkey.p, efac.data(), efac.size() * 4, hipMemcpyHostToDevice, c->stream));
+    if (!fval.empty())
         HIP_TRY(c, hipMemcpyAsync(c->bgfval.p, fval.data(), fval.size() * 4, hipMemcpyHostToDevice, c->stream));
-    }
-    int *cnt = c->bgcnt.as<int>();
-    launch_fill32(c->stream, cnt, 0u, 4);
-    // detection, offsets in the reference's z, y, x order
-    launch_bgp_flags(c->stream, g, c->eps.as<int>(), W, c->bgbits.as<unsigned long long>(), c->bgwcount.as<int>(),
-                     cnt + 2);
-    launch_scan(c->stream, c->bgwcount.as<int>(), nwords, c->bgwoff.as<int>(), c->scan_scratch.as<int>());
-    HIP_TRY(c, hipGetLastError());
-    int64_t nb = 0;
-    int rc = read_total(c, c->bgwcount.as<int>(), c->bgwoff.as<int>(), nwords, &nb);
-    if (rc) return rc;
-    if (nb > INT32_MAX / 3) return fail(c, NSGPU_E_NOMEM, "too many boundary grid points");
-    // auxiliary cell lists
-    A.eps = c->eps.as<int>();
-    A.W = W;
-    A.bits = c->bgbits.as<unsigned long long>();
-    A.woff = c->bgwoff.as<int>();
+    // auxiliary cell lists: count, scan, fill
     A.range = c->bgrange.as<int>();
     A.ccount = c->bgccount.as<int>();
     A.coff = c->bgcoff.as<int>();
@@ -2442,23 +2451,89 @@ int nsgpu_project_bgp(nsgpu_ctx *c, int32_t aux_grid_dim, int64_t *counts) {
     launch_scan(c->stream, A.ccount, ncell, A.coff, c->scan_scratch.as<int>());
     HIP_TRY(c, hipGetLastError());
     int64_t entries = 0;
-    rc = read_total(c, A.ccount, A.coff, ncell, &entries);
+    int rc = read_total(c, A.ccount, A.coff, ncell, &entries);
     if (rc) return rc;
     if (entries > INT32_MAX) return fail(c, NSGPU_E_NOMEM, "auxiliary cell lists exceed 2^31 entries");
     HIP_TRY(c, c->bgclist.ensure((size_t)std::max<int64_t>(entries, 1) * 4));
+    A.clist = c->bgclist.as<int>();
+    launch_bgp_bin(c->stream, P, A, true);
+    HIP_TRY(c, hipGetLastError());
+    A.efac = c->bgfkey.as<int>();
+    A.fval = c->bgfval.as<int>();
+    A.coi = c->bgcoi.as<double>();
+    A.node_aux = c->bgnaux.as<int>();
+    A.cflag = c->bgcflag.as<int>();
+    A.coff_active = c->bgcoffa.as<int>();
+    A.active = c->bgactive.as<int>();
+    // the host vectors above are read by the copies before they go
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->bgA = A;
+    c->bg_setgen = c->setgen;
+    c->bg_aux = aux_grid_dim;
+    c->bg_valid = true;
+    return NSGPU_OK;
+}
+
+extern "C" {
+
+int nsgpu_project_bgp(nsgpu_ctx *c, int32_t aux_grid_dim, int64_t *counts) {
+    if (!c || aux_grid_dim < 1) return NSGPU_E_ARG;
+    if (!c->have_surf || !c->opts.build_epsmap)
+        return fail(c, NSGPU_E_STATE, "boundary grid points need a getSurf with the epsilon map");
+    if (c->surface_is_mesh || c->npatch_skin > 0)
+        return fail(c, NSGPU_E_STATE, "boundary grid point projection is built for SES patches only");
+    HIP_TRY(c, hipSetDevice(c->device));
+    c->have_bgp = false;
+    int rc = bgp_tables(c, aux_grid_dim);
+    if (rc) return rc;
+    const nsgpu_grid &G = c->grid;
+    const DevGrid g = dev_grid(c);
+    const DevPatches P = dev_patches(c);
+    const int W = (G.nx + 63) / 64;
+    const int64_t nwords = (int64_t)G.ny * G.nz * W;
+    HIP_TRY(c, c->bgbits.ensure((size_t)nwords * 8));
+    HIP_TRY(c, c->bgwcount.ensure((size_t)nwords * 4));
+    HIP_TRY(c, c->bgwoff.ensure((size_t)nwords * 4));
+    HIP_TRY(c, c->bgcnt.ensure(4 * 4));
+    const int64_t ncell = (int64_t)c->bgA.na * c->bgA.na * c->bgA.na;
+    HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(std::max(nwords, ncell)) * 4));
+    int *cnt = c->bgcnt.as<int>();
+    BgpArgs A = c->bgA;
+    A.eps = c->eps.as<int>();
+    A.W = W;
+    A.bits = c->bgbits.as<unsigned long long>();
+    A.woff = c->bgwoff.as<int>();
+    A.counters = cnt;
+    launch_fill32(c->stream, cnt, 0u, 4);
+    launch_fill32(c->stream, A.cflag, 0u, ncell);
+    // detection (offsets in the reference's z, y, x order) and the auxiliary
+    // cells holding boundary points
+    launch_bgp_flags(c->stream, g, A, c->bgbits.as<unsigned long long>(), c->bgwcount.as<int>(), cnt + 2);
+    launch_scan(c->stream, c->bgwcount.as<int>(), nwords, c->bgwoff.as<int>(), c->scan_scratch.as<int>());
+    launch_scan(c->stream, A.cflag, ncell, A.coff_active, c->scan_scratch.as<int>());
+    HIP_TRY(c, hipGetLastError());
+    int64_t tot[2] = {0, 0};
+    {
+        const int *cs[2] = {c->bgwcount.as<int>(), A.cflag};
+        const int *os[2] = {c->bgwoff.as<int>(), A.coff_active};
+        const int64_t nn[2] = {nwords, ncell};
+        rc = read_totals(c, 2, cs, os, nn, tot);
+        if (rc) return rc;
+    }
+    const int64_t nb = tot[0], n_active = tot[1];
+    if (nb > INT32_MAX / 3) return fail(c, NSGPU_E_NOMEM, "too many boundary grid points");
     HIP_TRY(c, c->bgib.ensure((size_t)std::max<int64_t>(nb, 1) * 12));
     HIP_TRY(c, c->bgpos.ensure((size_t)std::max<int64_t>(nb, 1) * 24));
     HIP_TRY(c, c->bgnor.ensure((size_t)std::max<int64_t>(nb, 1) * 24));
-    A.clist = c->bgclist.as<int>();
-    launch_bgp_bin(c->stream, P, A, true);
-    A.fkey = c->bgfkey.as<long long>();
-    A.fval = c->bgfval.as<int>();
-    A.nf = (int)fkey.size();
+    HIP_TRY(c, c->bgfq.ensure((size_t)std::max<int64_t>(nb, 1) * 8));
     A.ibgp = c->bgib.as<int>();
     A.pos = c->bgpos.as<double>();
     A.nor = c->bgnor.as<double>();
-    A.counters = cnt;
-    if (nb > 0) launch_bgp_project(c->stream, g, P, A, c->probe);
+    A.fq = c->bgfq.as<int>();
+    if (nb > 0) {
+        launch_bgp_cells(c->stream, A);
+        launch_bgp_project(c->stream, g, P, A, c->probe, n_active);
+    }
     HIP_TRY(c, hipGetLastError());
     int hc[4] = {0, 0, 0, 0};
     HIP_TRY(c, hipMemcpyAsync(hc, cnt, sizeof hc, hipMemcpyDeviceToHost, c->stream));

@@ -11,11 +11,14 @@
 //                  so points keep the reference's z, y, x order)
 //   k_bgp_bin      the auxiliary grid: every patch in the cells its bounding
 //                  box covers (count, then fill)
-//   k_bgp_project  one wave per auxiliary cell: the cell's boundary points are
-//                  gathered from the bitmask into LDS, then each lane projects
-//                  one point on every patch of the cell's list; the patch is
-//                  the same for all lanes, so its kind, data and feasibility
-//                  loop are uniform across the wave
+//   k_bgp_project  one wave per auxiliary cell that holds boundary points
+//                  (flagged by k_bgp_flags, listed by k_bgp_cells): the
+//                  cell's points are gathered from the bitmask into LDS, then
+//                  each lane projects one point on every patch of the cell's
+//                  list; the patch is the same for all lanes, so its kind,
+//                  data and feasibility loop are uniform across the wave
+//   k_bgp_fallback one workgroup per point with no feasible patch projection,
+//                  sampling probe positions along self-intersecting tori
 #include <hip/hip_runtime.h>
 
 #include "launch.hpp"
@@ -24,7 +27,9 @@ namespace nsg {
 
 namespace {
 
-constexpr int kBgpCap = 512;  // LDS entries per gathered batch of one cell
+constexpr int kBgpCap = 256;  // LDS entries per gathered batch of one cell
+constexpr int kFallbackThreads = 512;  // one thread per sample angle (kCoiSamples)
+static_assert(kFallbackThreads >= kCoiSamples, "a sample per thread");
 
 // the six epsmap faces around node n: +x, +y, +z faces of the node, the +x
 // face of node - x, +y of node - y, +z of node - z (Surface.cpp:1490-1525)
@@ -43,8 +48,10 @@ __device__ __forceinline__ Faces faces_at(const DevGrid &g, const int *eps, int6
     return f;
 }
 
-__global__ void __launch_bounds__(256) k_bgp_flags(DevGrid g, const int *eps, int W, unsigned long long *bits,
-                                                   int *wcount, int *n_internal) {
+__global__ void __launch_bounds__(256) k_bgp_flags(DevGrid g, BgpArgs A, unsigned long long *bits, int *wcount,
+                                                   int *n_internal) {
+    const int *eps = A.eps;
+    const int W = A.W;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     const int64_t nwords = (int64_t)g.ny * g.nz * W;
@@ -62,6 +69,8 @@ __global__ void __launch_bounds__(256) k_bgp_flags(DevGrid g, const int *eps, in
         for (int q = 0; q < 6; ++q) any_out |= f.v[q] == 0;
         internal = is_bgp && !any_out;
     }
+    // the auxiliary cell of a boundary point has work for k_bgp_project
+    if (is_bgp) A.cflag[((int64_t)A.node_aux[2 * g.N + iz] * A.na + A.node_aux[g.N + iy]) * A.na + A.node_aux[ix]] = 1;
     const unsigned long long m = __ballot(is_bgp);
     const unsigned long long mi = __ballot(internal);
     if (lane == 0) {
@@ -147,10 +156,8 @@ __device__ __forceinline__ double project_torus(const double y[3], const double 
 }
 
 // ConnollySurface::isFeasible (ConnollySurface.cpp:4769-5021) on the patch
-// SoA's normalised tests, for one point
-__device__ __forceinline__ bool feasible1(const DevPatches &P, int it, const double p[3]) {
-    const int kind = P.kind[it], cnt = P.cnt[it];
-    const double *d = P.data + P.off[it];
+// SoA's normalised tests, for one point; d = the patch's data
+__device__ __forceinline__ bool feasible_hdr(int kind, int cnt, int flags, const double *d, const double p[3]) {
     if (kind == NSGPU_PATCH_POINT) {
         for (int i = 0; i < cnt; ++i) {
             const double *s = d + 4 * i;
@@ -174,7 +181,6 @@ __device__ __forceinline__ bool feasible1(const DevPatches &P, int it, const dou
     }
     if (kind == NSGPU_PATCH_TORUS_SING) return true;
     const double *pl = d + NSGPU_TORUS_HDR;
-    const int flags = P.flags[it];
     for (int i = 0; i < cnt; ++i, pl += 8) {
         const bool le1 = !((pl[0] * p[0] + pl[1] * p[1] + pl[2] * p[2]) + pl[3] > 0);
         const bool le2 = !((pl[4] * p[0] + pl[5] * p[1] + pl[6] * p[2]) + pl[7] > 0);
@@ -182,82 +188,151 @@ __device__ __forceinline__ bool feasible1(const DevPatches &P, int it, const dou
     }
     return false;
 }
+__device__ __forceinline__ bool feasible1(const DevPatches &P, int it, const double p[3]) {
+    return feasible_hdr(P.kind[it], P.cnt[it], P.flags[it], P.data + P.off[it], p);
+}
 
 __device__ __forceinline__ bool lex_less(double d, int it, int kk, double bd, int bit, int bkk) {
     return d < bd || (d == bd && (it < bit || (it == bit && kk < bkk)));
 }
 
-// no patch projection of p is feasible: sample the probe positions along the
+// No patch projection of p is feasible: sample the probe positions along the
 // circle of every self-intersecting regular torus of the cell and keep the
 // nearest probe-sphere projection that a facet on that edge accepts
-// (ConnollySurface.cpp:5916-6086); (distance, patch, sample) decide ties as
-// the reference's ascending loops do
-__device__ bool sampled_projection(const DevPatches &P, const BgpArgs &A, const int *cl, int nc, double probe,
-                                   const double p[3], double proj[3]) {
+// (ConnollySurface.cpp:5916-6086).  The whole wave searches for one point:
+// lane l takes samples l, l + 64, ... of each torus, whose cos and sin come
+// from the host's table (the reference's accumulated angle, glibc's cos and
+// sin); the lanes' bests are reduced in (distance, patch, sample) order,
+// which decides ties as the reference's ascending loops do.  Called by the
+// whole wave with p uniform; returns whether a sample was kept.
+__device__ bool sampled_projection_block(const DevPatches &P, const BgpArgs &A, const int *cl, int nc, double probe,
+                                         const double p[3], double proj[3]) {
+    const int lane = threadIdx.x & 63, tid = threadIdx.x, nt = blockDim.x;
     double best = INFINITY;
     int bit = 0x7fffffff, bkk = 0x7fffffff;
-    const double step = (2.0 * 3.14159265358979323846264338327950288) / 500;
-    for (int k = 0; k < nc; ++k) {
-        const int it = cl[k];
-        if (P.kind[it] != NSGPU_PATCH_TORUS_REG) continue;
-        const double *td = P.data + P.off[it];
-        if (!(td[13] >= 0.)) continue;
-        const int a = P.atom[3 * it], b = P.atom[3 * it + 1];
-        const long long key = ((long long)min(a, b) << 32) | (unsigned)max(a, b);
-        int lo = 0, hi = A.nf;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (A.fkey[mid] < key) lo = mid + 1;
-            else hi = mid;
+    double bp[3] = {0, 0, 0};
+    // the cell's eligible patches (self-intersecting regular tori with facets
+    // on their edge), found by the lanes in parallel, 64 patches at a time
+    for (int k0 = 0; k0 < nc; k0 += 64) {
+        bool elig = false;
+        int my_it = 0;
+        if (k0 + lane < nc) {
+            my_it = cl[k0 + lane];
+            elig = P.kind[my_it] == NSGPU_PATCH_TORUS_REG && P.data[P.off[my_it] + 13] >= 0. &&
+                   A.efac[2 * my_it + 1] > 0;
         }
-        if (lo >= A.nf || A.fkey[lo] != key) continue;
-        const double *iv = td + 3;
-        double tcoi = 0.;
-        for (int kk = 0; kk < 500; ++kk) {
-            double sp[3], loc[3];
-            const double ct = cos(tcoi), st = sin(tcoi);
-            for (int d = 0; d < 3; ++d) sp[d] = td[d] + td[12] * (ct * iv[3 + d] + st * iv[d]);
-            tcoi += step;
-            const double dist = project_sphere(p, sp, probe, loc);
-            if (!lex_less(dist, it, kk, best, bit, bkk)) continue;
-            for (int f = lo; f < A.nf && A.fkey[f] == key; ++f)
-                if (feasible1(P, A.fval[f], loc)) {
+        unsigned long long em = __ballot(elig);
+        while (em) {
+            const int L = __ffsll((long long)em) - 1;
+            em &= em - 1;
+            const int it = __shfl(my_it, L, 64);
+            const double *td = P.data + P.off[it];
+            // the edge's facets (usually two): entries [lo, lo + n_all) of
+            // fval, the headers of the first kMaxEdgeFacets loaded once
+            const int lo = A.efac[2 * it], n_all = A.efac[2 * it + 1];
+            constexpr int kMaxEdgeFacets = 8;
+            int fkind[kMaxEdgeFacets], fcnt[kMaxEdgeFacets];
+            const double *fdat[kMaxEdgeFacets];
+            const int nfc = min(n_all, kMaxEdgeFacets);
+            for (int f = 0; f < nfc; ++f) {
+                const int fi = A.fval[lo + f];
+                fkind[f] = P.kind[fi];
+                fcnt[f] = P.cnt[fi];
+                fdat[f] = P.data + P.off[fi];
+            }
+            const double *iv = td + 3;
+            for (int kk = tid; kk < kCoiSamples; kk += nt) {
+                double sp[3], loc[3];
+                const double ct = A.coi[2 * kk], st = A.coi[2 * kk + 1];
+                for (int d = 0; d < 3; ++d) sp[d] = td[d] + td[12] * (ct * iv[3 + d] + st * iv[d]);
+                const double dist = project_sphere(p, sp, probe, loc);
+                if (!lex_less(dist, it, kk, best, bit, bkk)) continue;
+                bool ok = false;
+                for (int f = 0; f < nfc && !ok; ++f) ok = feasible_hdr(fkind[f], fcnt[f], 0, fdat[f], loc);
+                // facets past the first kMaxEdgeFacets (a degenerate edge)
+                for (int f = kMaxEdgeFacets; !ok && f < n_all; ++f) ok = feasible1(P, A.fval[lo + f], loc);
+                if (ok) {
                     best = dist;
                     bit = it;
                     bkk = kk;
-                    proj[0] = loc[0];
-                    proj[1] = loc[1];
-                    proj[2] = loc[2];
-                    break;
+                    bp[0] = loc[0];
+                    bp[1] = loc[1];
+                    bp[2] = loc[2];
                 }
+            }
         }
     }
-    return best != INFINITY;
+    // wave reduction in (distance, patch, sample) order
+    for (int o = 32; o >= 1; o >>= 1) {
+        const double ob = __shfl_xor(best, o, 64);
+        const int oit = __shfl_xor(bit, o, 64), okk = __shfl_xor(bkk, o, 64);
+        const double o0 = __shfl_xor(bp[0], o, 64), o1 = __shfl_xor(bp[1], o, 64), o2 = __shfl_xor(bp[2], o, 64);
+        if (lex_less(ob, oit, okk, best, bit, bkk)) {
+            best = ob;
+            bit = oit;
+            bkk = okk;
+            bp[0] = o0;
+            bp[1] = o1;
+            bp[2] = o2;
+        }
+    }
+    // then across the block's waves
+    __shared__ double r_best[kFallbackThreads / 64], r_p[kFallbackThreads / 64][3];
+    __shared__ int r_it[kFallbackThreads / 64], r_kk[kFallbackThreads / 64];
+    if (lane == 0) {
+        r_best[tid >> 6] = best;
+        r_it[tid >> 6] = bit;
+        r_kk[tid >> 6] = bkk;
+        for (int d = 0; d < 3; ++d) r_p[tid >> 6][d] = bp[d];
+    }
+    __syncthreads();
+    best = r_best[0];
+    bit = r_it[0];
+    bkk = r_kk[0];
+    for (int d = 0; d < 3; ++d) bp[d] = r_p[0][d];
+    for (int w = 1; w < nt / 64; ++w)
+        if (lex_less(r_best[w], r_it[w], r_kk[w], best, bit, bkk)) {
+            best = r_best[w];
+            bit = r_it[w];
+            bkk = r_kk[w];
+            for (int d = 0; d < 3; ++d) bp[d] = r_p[w][d];
+        }
+    __syncthreads();
+    if (best == INFINITY) return false;
+    proj[0] = bp[0];
+    proj[1] = bp[1];
+    proj[2] = bp[2];
+    return true;
 }
 
 __global__ void __launch_bounds__(64) k_bgp_project(DevGrid g, DevPatches P, BgpArgs A, double probe) {
-    __shared__ int sx[kBgpCap], sy[kBgpCap], sz[kBgpCap], spos[kBgpCap];
+    // gathered points: node offsets within the cell's ranges (6 bits each,
+    // a cell spans at most 64 nodes per axis) and output positions
+    __shared__ int sloc[kBgpCap], spos[kBgpCap];
+    __shared__ int h_it[64], h_kind[64], h_cnt[64], h_flags[64], h_off[64];
+    __shared__ double h_bs[64][4];
+    __shared__ double h_td[64][NSGPU_TORUS_HDR];
     __shared__ int s_total;
     const int lane = threadIdx.x;
-    const int64_t cell = blockIdx.x;
+    const int64_t cell = A.active[blockIdx.x];
     const int ax = (int)(cell % A.na), ay = (int)((cell / A.na) % A.na), az = (int)(cell / ((int64_t)A.na * A.na));
     const int xlo = A.range[2 * ax], xhi = A.range[2 * ax + 1];
     const int ylo = A.range[2 * (A.na + ay)], yhi = A.range[2 * (A.na + ay) + 1];
     const int zlo = A.range[2 * (2 * A.na + az)], zhi = A.range[2 * (2 * A.na + az) + 1];
     if (xhi < xlo || yhi < ylo || zhi < zlo) return;
     const int ny_c = yhi - ylo + 1, nrows = ny_c * (zhi - zlo + 1);
-    const int xspan = xhi - xlo + 1;
-    const int rows_per_batch = min(64, max(1, kBgpCap / xspan));
     const int nc = A.ccount[cell];
     const int *cl = A.clist + A.coff[cell];
-    for (int r0 = 0; r0 < nrows; r0 += rows_per_batch) {
-        // gather: lane l < rows_per_batch reads its row's words over [xlo, xhi]
+    for (int r0 = 0; r0 < nrows;) {
+        // gather: lane l reads row r0 + l's words over [xlo, xhi]; the batch
+        // takes the leading rows whose points fit the LDS lists (a row holds
+        // at most 64 points, so at least kBgpCap / 64 rows are taken)
         int cnt = 0;
         unsigned long long wv[2] = {0, 0};
         int iy = 0, iz = 0, w0 = 0;
         int64_t wbase = 0;
         const int r = r0 + lane;
-        if (lane < rows_per_batch && r < nrows) {
+        if (r < nrows) {
             iy = ylo + r % ny_c;
             iz = zlo + r / ny_c;
             w0 = xlo >> 6;
@@ -272,15 +347,19 @@ __global__ void __launch_bounds__(64) k_bgp_project(DevGrid g, DevPatches P, Bgp
                 cnt += __popcll(m);
             }
         }
-        // wave exclusive prefix of cnt
+        // wave inclusive prefix of cnt; rows up to the last lane whose prefix
+        // fits are taken
         int pre = cnt;
         for (int o = 1; o < 64; o <<= 1) {
             const int v = __shfl_up(pre, o, 64);
             if (lane >= o) pre += v;
         }
-        const int total = __shfl(pre, 63, 64);
+        const unsigned long long fits = __ballot(r < nrows && pre <= kBgpCap);
+        const int take = __popcll(fits);  // fits is a prefix of the lanes
+        const int total = take ? __shfl(pre, take - 1, 64) : 0;
         pre -= cnt;
-        if (cnt) {
+        r0 += take;
+        if (cnt && lane < take) {
             int at = pre;
             for (int q = 0; q < 2; ++q) {
                 unsigned long long m = wv[q];
@@ -292,9 +371,7 @@ __global__ void __launch_bounds__(64) k_bgp_project(DevGrid g, DevPatches P, Bgp
                     // bit b in it
                     const unsigned long long full = A.bits[wbase + w];
                     const int pos = A.woff[wbase + w] + __popcll(b ? (full & (~0ull >> (64 - b))) : 0ull);
-                    sx[at] = w * 64 + b;
-                    sy[at] = iy;
-                    sz[at] = iz;
+                    sloc[at] = (w * 64 + b - xlo) | ((iy - ylo) << 6) | ((iz - zlo) << 12);
                     spos[at] = pos;
                     ++at;
                 }
@@ -309,9 +386,10 @@ __global__ void __launch_bounds__(64) k_bgp_project(DevGrid g, DevPatches P, Bgp
             double p[3] = {0, 0, 0}, proj[3] = {0, 0, 0};
             bool internal = false;
             if (active) {
-                ix = sx[e];
-                jy = sy[e];
-                kz = sz[e];
+                const int lc = sloc[e];
+                ix = xlo + (lc & 63);
+                jy = ylo + ((lc >> 6) & 63);
+                kz = zlo + (lc >> 12);
                 pos = spos[e];
                 p[0] = g.x[ix];
                 p[1] = g.y[jy];
@@ -327,28 +405,53 @@ __global__ void __launch_bounds__(64) k_bgp_project(DevGrid g, DevPatches P, Bgp
             double best = INFINITY;
             int bit = 0x7fffffff;
             const bool work = active && !internal;
-            for (int k = 0; k < nc; ++k) {
-                const int it = cl[k];
-                const int kind = P.kind[it];
-                double loc[3], dist;
-                if (kind == NSGPU_PATCH_TORUS_REG || kind == NSGPU_PATCH_TORUS_SING)
-                    dist = project_torus(p, P.data + P.off[it], probe, loc);
-                else
-                    dist = project_sphere(p, P.bsph + 4 * it, kind == NSGPU_PATCH_POINT ? P.bsph[4 * it + 3] : probe,
-                                          loc);
-                if (!work || !lex_less(dist, it, 0, best, bit, 0)) continue;
-                if (!feasible1(P, it, loc)) continue;
-                best = dist;
-                bit = it;
-                proj[0] = loc[0];
-                proj[1] = loc[1];
-                proj[2] = loc[2];
+            // the cell's patches, 64 at a time: their headers staged in LDS by
+            // one load per lane, then every lane tests its point on each
+            for (int c0 = 0; c0 < nc; c0 += 64) {
+                const int nk = min(64, nc - c0);
+                __syncthreads();
+                if (lane < nk) {
+                    const int it = cl[c0 + lane];
+                    h_it[lane] = it;
+                    h_kind[lane] = P.kind[it];
+                    h_cnt[lane] = P.cnt[it];
+                    h_flags[lane] = P.flags[it];
+                    h_off[lane] = P.off[it];
+                    for (int q = 0; q < 4; ++q) h_bs[lane][q] = P.bsph[4 * it + q];
+                    // a torus's centre, frame, radii: read by every lane
+                    if (h_kind[lane] == NSGPU_PATCH_TORUS_REG || h_kind[lane] == NSGPU_PATCH_TORUS_SING) {
+                        const double *td = P.data + h_off[lane];
+                        for (int q = 0; q < NSGPU_TORUS_HDR; ++q) h_td[lane][q] = td[q];
+                    }
+                }
+                __syncthreads();
+                for (int k = 0; k < nk; ++k) {
+                    const int it = h_it[k], kind = h_kind[k];
+                    const double *d = P.data + h_off[k];
+                    double loc[3], dist;
+                    if (kind == NSGPU_PATCH_TORUS_REG || kind == NSGPU_PATCH_TORUS_SING)
+                        dist = project_torus(p, h_td[k], probe, loc);
+                    else
+                        dist = project_sphere(p, h_bs[k], kind == NSGPU_PATCH_POINT ? h_bs[k][3] : probe, loc);
+                    if (!work || !lex_less(dist, it, 0, best, bit, 0)) continue;
+                    if (!feasible_hdr(kind, h_cnt[k], h_flags[k], d, loc)) continue;
+                    best = dist;
+                    bit = it;
+                    proj[0] = loc[0];
+                    proj[1] = loc[1];
+                    proj[2] = loc[2];
+                }
             }
+            // points with no feasible patch projection: queued for
+            // k_bgp_fallback, which gives each its own wave
             if (work && best == INFINITY) {
-                if (nc > 0 && sampled_projection(P, A, cl, nc, probe, p, proj))
-                    atomicAdd(A.counters + 0, 1);
-                else
+                if (nc > 0) {
+                    const int q = atomicAdd(A.counters + 3, 1);
+                    A.fq[2 * q] = pos;
+                    A.fq[2 * q + 1] = (int)cell;
+                } else {
                     atomicAdd(A.counters + 1, 1);
+                }
             }
             if (active) {
                 A.ibgp[3 * (int64_t)pos] = ix;
@@ -364,14 +467,44 @@ __global__ void __launch_bounds__(64) k_bgp_project(DevGrid g, DevPatches P, Bgp
     }
 }
 
+// one workgroup per queued point (grid-stride over the queue): the sampled
+// projection of the point's cell
+__global__ void __launch_bounds__(kFallbackThreads) k_bgp_fallback(DevGrid g, DevPatches P, BgpArgs A, double probe) {
+    const int nq = A.counters[3];
+    for (int q = blockIdx.x; q < nq; q += gridDim.x) {
+        const int pos = A.fq[2 * q];
+        const int64_t cell = A.fq[2 * q + 1];
+        const int *ib = A.ibgp + 3 * (int64_t)pos;
+        const double p[3] = {g.x[ib[0]], g.y[ib[1]], g.z[ib[2]]};
+        double fp[3];
+        const bool kept = sampled_projection_block(P, A, A.clist + A.coff[cell], A.ccount[cell], probe, p, fp);
+        if (threadIdx.x == 0) {
+            if (kept)
+                for (int d = 0; d < 3; ++d) A.pos[3 * (int64_t)pos + d] = fp[d];
+            atomicAdd(A.counters + (kept ? 0 : 1), 1);
+        }
+    }
+}
+
+// list of the auxiliary cells flagged by k_bgp_flags
+__global__ void __launch_bounds__(256) k_bgp_cells(BgpArgs A, int64_t ncell) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < ncell && A.cflag[c]) A.active[A.coff_active[c]] = (int)c;
+}
+
 }  // namespace
 
-void launch_bgp_flags(hipStream_t s, const DevGrid &g, const int *eps, int W, unsigned long long *bits, int *wcount,
+void launch_bgp_flags(hipStream_t s, const DevGrid &g, const BgpArgs &A, unsigned long long *bits, int *wcount,
                       int *n_internal) {
-    const int64_t waves = (int64_t)g.ny * g.nz * W;
+    const int64_t waves = (int64_t)g.ny * g.nz * A.W;
     if (waves <= 0) return;
-    hipLaunchKernelGGL(k_bgp_flags, dim3((unsigned)((waves * 64 + 255) / 256)), dim3(256), 0, s, g, eps, W, bits,
-                       wcount, n_internal);
+    hipLaunchKernelGGL(k_bgp_flags, dim3((unsigned)((waves * 64 + 255) / 256)), dim3(256), 0, s, g, A, bits, wcount,
+                       n_internal);
+}
+
+void launch_bgp_cells(hipStream_t s, const BgpArgs &A) {
+    const int64_t cells = (int64_t)A.na * A.na * A.na;
+    hipLaunchKernelGGL(k_bgp_cells, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, A, cells);
 }
 
 void launch_bgp_bin(hipStream_t s, const DevPatches &P, const BgpArgs &A, bool fill) {
@@ -380,10 +513,11 @@ void launch_bgp_bin(hipStream_t s, const DevPatches &P, const BgpArgs &A, bool f
     if (P.n > 0) hipLaunchKernelGGL(k_bgp_bin, dim3((P.n + 255) / 256), dim3(256), 0, s, P, A, fill);
 }
 
-void launch_bgp_project(hipStream_t s, const DevGrid &g, const DevPatches &P, const BgpArgs &A, double probe) {
-    const int64_t cells = (int64_t)A.na * A.na * A.na;
-    if (cells <= 0) return;
-    hipLaunchKernelGGL(k_bgp_project, dim3((unsigned)cells), dim3(64), 0, s, g, P, A, probe);
+void launch_bgp_project(hipStream_t s, const DevGrid &g, const DevPatches &P, const BgpArgs &A, double probe,
+                        int64_t n_active) {
+    if (n_active <= 0) return;
+    hipLaunchKernelGGL(k_bgp_project, dim3((unsigned)n_active), dim3(64), 0, s, g, P, A, probe);
+    hipLaunchKernelGGL(k_bgp_fallback, dim3(1024), dim3(kFallbackThreads), 0, s, g, P, A, probe);
 }
 
 }  // namespace nsg

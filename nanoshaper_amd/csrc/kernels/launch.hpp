@@ -236,6 +236,8 @@ int64_t mesh_workgroups(int64_t nt, const DevRays &);
 void launch_intersect_mesh(hipStream_t, const DevGrid &, const DevRays &, const MeshArgs &, unsigned long long *part);
 
 
+// probe positions sampled along a torus circle (coiNum, ConnollySurface.cpp:5927)
+constexpr int kCoiSamples = 500;
 // Boundary grid points (Surface.cpp:1476-1588) and their SES projection
 // (ConnollySurface::getProjection, ConnollySurface.cpp:5822-6106).
 struct BgpArgs {
@@ -249,19 +251,29 @@ struct BgpArgs {
     double amin[3], ascale;
     const int *range;
     int *ccount, *coff, *ccursor, *clist;
-    // facets by atom pair (key min << 32 | max, ascending) for the sampled
-    // projection near self-intersecting tori
-    const long long *fkey;
+    // facets by atom pair (ascending pair key), and per patch the range
+    // [efac[2p], efac[2p] + efac[2p+1]) of fval holding the facets on a torus
+    // patch's atom pair, for the sampled projection near self-intersecting tori
     const int *fval;
-    int nf;
+    const int *efac;
+    const double *coi;  // cos, sin of the kCoiSamples sample angles (getCoi)
     // outputs: node indices, projections, normals (zero, as the reference's)
     int *ibgp;
     double *pos, *nor;
-    int *counters;  // [0] sampled projections, [1] points left at their node
+    // [0] sampled projections, [1] points left at their node, [2] points
+    // with no face outside, [3] points queued for the sampled projection
+    int *counters;
+    // node -> auxiliary cell index per axis (x at [0, N), y at [N, 2N), z at
+    // [2N, 3N), N = the grid's longest side); cells holding boundary points
+    // (flag, exclusive scan, list) and the queue of (point, cell) pairs
+    const int *node_aux;
+    int *cflag, *coff_active, *active, *fq;
 };
-void launch_bgp_flags(hipStream_t, const DevGrid &, const int *eps, int W, unsigned long long *bits, int *wcount,
+void launch_bgp_flags(hipStream_t, const DevGrid &, const BgpArgs &, unsigned long long *bits, int *wcount,
                       int *n_internal);
+void launch_bgp_cells(hipStream_t, const BgpArgs &);
 void launch_bgp_bin(hipStream_t, const DevPatches &, const BgpArgs &, bool fill);
-void launch_bgp_project(hipStream_t, const DevGrid &, const DevPatches &, const BgpArgs &, double probe);
+void launch_bgp_project(hipStream_t, const DevGrid &, const DevPatches &, const BgpArgs &, double probe,
+                        int64_t n_active);
 
 }  // namespace nsg

@@ -14,6 +14,9 @@
 * save_status (DelPhiShared::saveStatus, DelphiShared.cpp:1458-1503) and
   save_cavities (DelPhiShared::saveCavities, DelphiShared.cpp:1889-1990):
   cavities.txt, cavitiesSize.txt and cavAtomsSerials.txt.
+* save_bgp (DelPhiShared::saveBGP, DelphiShared.cpp:1421-1455): the
+  boundary grid points as root + name + ".projections.txt" and root +
+  "surf.txt".
 * vertex_atoms / cavity_atoms: the nearest atom of every mesh vertex
   (vertexAtomsMap) and the atoms of every cavity (Surface::getCavitiesAtoms,
   Surface.cpp:2129-2245), through nsgpu_nearest_atoms.
@@ -107,6 +110,24 @@ def save_status(path, status):
     """status: (nz, ny, nx) in the DelPhi layout (DelPhiShared::saveStatus:
     i outermost, a value and a space per node, k fastest)."""
     _write_grid(path, np.asarray(status))
+
+
+def save_bgp(root, name, ibgp, scspos, scsnor):
+    """root + name + '.projections.txt': the count, then per point its 1-based
+    node indices, projection and normal; root + 'surf.txt': projection and
+    normal per point.  "%lf" fields, no newline after the last line.
+    Returns the two paths."""
+    n = len(ibgp)
+    ib = np.asarray(ibgp, dtype=np.int64) + 1
+    pos, nor = np.asarray(scspos, float), np.asarray(scsnor, float)
+    p1, p2 = root + name + ".projections.txt", root + "surf.txt"
+    six = ["%f %f %f %f %f %f" % (tuple(pos[i]) + tuple(nor[i])) for i in range(n)]
+    with open(p1, "w") as f:
+        f.write("%d\n" % n)
+        f.write("\n".join("%d %d %d " % tuple(ib[i]) + six[i] for i in range(n)))
+    with open(p2, "w") as f:
+        f.write("\n".join(six))
+    return [p1, p2]
 
 
 def _grid_axes(grid):

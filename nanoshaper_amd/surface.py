@@ -389,6 +389,13 @@ class DeviceSurface:
         when vertex normals are on)."""
         self._chk(self._lib.nsgpu_smooth_surface(self._h), "smooth_surface")
 
+    def project_bgp_device(self, aux_grid_dim=50):
+        """The device stage of project_bgp alone (results stay on the device
+        for read_bgp); returns the counts."""
+        counts = np.zeros(4, np.int64)
+        self._chk(self._lib.nsgpu_project_bgp(self._h, int(aux_grid_dim), _ptr(counts)), "project_bgp")
+        return counts
+
     def project_bgp(self, aux_grid_dim=50):
         """Project_boundary_grid_points on the maps of the last get_surf (with
         the epsilon map; SES patches): {'ibgp' (n, 3) 0-based node indices in
@@ -396,8 +403,7 @@ class DeviceSurface:
         (n, 3) normals (zero, as the reference's), 'counts' [points, points
         with no face outside, points placed by the singular-edge sampling,
         points left at their node]}."""
-        counts = np.zeros(4, np.int64)
-        self._chk(self._lib.nsgpu_project_bgp(self._h, int(aux_grid_dim), _ptr(counts)), "project_bgp")
+        counts = self.project_bgp_device(aux_grid_dim)
         n = int(counts[0])
         ib = np.empty((n, 3), np.int32)
         pos = np.empty((n, 3))

@@ -2529,14 +2529,22 @@ int orc_project_bgp(orc_surface *s, int aux_dim, int64_t *counts) {
             if (!(td[13] >= 0.)) continue;
             const int ref1 = P->atom[3 * it], ref2 = P->atom[3 * it + 1];
             /* the facets incident to the edge (both atoms among theirs) */
-            int fcv[64], nfc = 0;
-            for (int f = 0; f < P->n && nfc < 64; f++) {
+            int fcv_local[64], *fcv = fcv_local, nfc = 0, cap = 64;
+            for (int f = 0; f < P->n; f++) {
                 const int fk = P->kind[f];
                 if (fk != NSGPU_PATCH_FACET_REG && fk != NSGPU_PATCH_FACET_SING) continue;
                 const int32_t *a = P->atom + 3 * f;
                 const int h1 = a[0] == ref1 || a[1] == ref1 || a[2] == ref1;
                 const int h2 = a[0] == ref2 || a[1] == ref2 || a[2] == ref2;
-                if (h1 && h2) fcv[nfc++] = f;
+                if (!h1 || !h2) continue;
+                if (nfc == cap) {
+                    int *grown = (int *)malloc(2 * cap * sizeof(int));
+                    memcpy(grown, fcv, cap * sizeof(int));
+                    if (fcv != fcv_local) free(fcv);
+                    fcv = grown;
+                    cap *= 2;
+                }
+                fcv[nfc++] = f;
             }
             const double *iv = td + 3;
             const double v[3] = {iv[0], iv[1], iv[2]}, u[3] = {iv[3], iv[4], iv[5]};
@@ -2557,6 +2565,7 @@ int orc_project_bgp(orc_surface *s, int aux_dim, int64_t *counts) {
                     }
                 }
             }
+            if (fcv != fcv_local) free(fcv);
         }
         if (fixed)
             n_fixed++;

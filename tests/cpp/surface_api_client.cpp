@@ -144,11 +144,36 @@ static int rescale_mode() {
     return v2 == v3 ? 0 : 55;
 }
 
+// "bgp DIR": getSurf with Project_boundary_grid_points on the 330-atom chain,
+// the projections saved with saveBGP into DIR
+static int bgp_mode(const char *dir) {
+    std::vector<double> xyzr(330 * 4);
+    double radius_used = 0;
+    const int n = nsgpu_generate_chain(330, 12.0, 0, xyzr.data(), &radius_used);
+    xyzr.resize(n * 4);
+    nsgpu::ConfigFile cf;
+    cf.parse_line("Surface = ses");
+    cf.parse_line("Accurate_Triangulation = false");
+    cf.parse_line("Project_boundary_grid_points = true");
+    nsgpu::DelPhiShared ds(true, true);
+    ds.loadAtoms(xyzr);
+    if (!ds.buildGrid(2.0, 90.0)) return 60;
+    std::unique_ptr<nsgpu::Surface> surf(nsgpu::surfaceFactory().create(&cf, &ds));
+    double vol = 0;
+    if (!surf->build() || !surf->getSurf(&vol, true, true, 11.4)) return 61;
+    if (!ds.saveBGP(std::string(dir) + "/", "mol")) return 62;
+    double sum = 0;
+    for (double v : ds.scspos) sum += v;
+    std::printf("bgp %d sum %.9f\n", ds.nbgp, sum);
+    return ds.nbgp > 0 ? 0 : 63;
+}
+
 int main(int argc, char **argv) {
     if (argc > 1 && (std::strcmp(argv[1], "blobby") == 0 || std::strcmp(argv[1], "mesh") == 0))
         return mesh_modes(argc, argv);
     if (argc > 1 && std::strcmp(argv[1], "facade") == 0) return facade_mode();
     if (argc > 2 && std::strcmp(argv[1], "prm") == 0) return prm_mode(argv[2]);
+    if (argc > 2 && std::strcmp(argv[1], "bgp") == 0) return bgp_mode(argv[2]);
     if (argc > 1 && std::strcmp(argv[1], "rescale") == 0) return rescale_mode();
     const bool gpu = argc > 1 && std::strcmp(argv[1], "gpu") == 0;
     nsgpu::ConfigFile cf;

@@ -125,3 +125,30 @@ def test_cpp_configuration_run(client, device_ok):
 def test_cpp_surface_follows_a_rebuilt_grid(client, device_ok):
     out = subprocess.run([client, "rescale"], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout + out.stderr
+
+
+@pytest.mark.gpu
+def test_cpp_bgp_files_match_python(client, device_ok, tmp_path):
+    """getSurf with Project_boundary_grid_points through the C++ layer, saved
+    by DelPhiShared::saveBGP: the same files as the Python path writes."""
+    import nanoshaper_amd as ns
+    from nanoshaper_amd import io as IO
+    cdir = tmp_path / "cpp"
+    cdir.mkdir()
+    out = subprocess.run([client, "bgp", str(cdir)], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    nb = int(out.stdout.split()[1])
+    xyzr, _ = ns.generate_chain(330, 12.0, seed=0)
+    g = ns.build_grid(xyzr, 2.0, 90.0)
+    jit, _ = ns.jitter(xyzr)
+    dev = ns.DeviceSurface(0)
+    dev.set_grid(g)
+    dev.set_patches(ns.ses_patches(jit, 1.4))
+    dev.get_surf(build_epsmap=True, build_status=True, accurate_triangulation=False, fill_cavities=True)
+    b = dev.project_bgp()
+    assert b["counts"][0] == nb
+    pdir = tmp_path / "py"
+    pdir.mkdir()
+    IO.save_bgp(str(pdir) + "/", "mol", b["ibgp"], b["scspos"], b["scsnor"])
+    for f in ("mol.projections.txt", "surf.txt"):
+        assert (cdir / f).read_bytes() == (pdir / f).read_bytes()

@@ -71,6 +71,31 @@ def test_bgp_benchmark_size():
     assert a["counts"][0] > 500000
 
 
+def test_bgp_tables_follow_the_inputs():
+    # one context: the cached auxiliary grid follows the aux size and a new
+    # patch set on the same grid
+    from oracle.binding import OracleSurface
+    xyzr, _ = ns.generate_chain(150, 9.0, seed=11)
+    grid = ns.build_grid(xyzr, 3.0, 80.0)
+    dev = ns.DeviceSurface(0)
+    dev.set_grid(grid)
+    opts = dict(build_epsmap=True, build_status=True, fill_cavities=True)
+    for seed, aux in ((11, 50), (11, 100), (12, 100)):
+        print("bgp tables seed=%d aux=%d" % (seed, aux), flush=True)
+        x, _ = ns.generate_chain(150, 9.0, seed=seed)
+        jit, _ = ns.jitter(x)
+        ps = ns.ses_patches(jit, 1.4)
+        dev.set_patches(ps)
+        dev.get_surf(**opts)
+        orc = OracleSurface(grid, ps)
+        orc.get_surf(**opts)
+        a, b = dev.project_bgp(aux), orc.project_bgp(aux)
+        assert np.array_equal(a["ibgp"], b["ibgp"]) and list(a["counts"]) == list(b["counts"])
+        assert np.abs(a["scspos"] - b["scspos"]).max() <= POS_TOL_A
+        orc.close()
+    dev.close()
+
+
 def test_bgp_refusals():
     xyzr, _ = ns.generate_chain(40, 6.0, seed=2)
     grid = ns.build_grid(xyzr, 2.0, 80.0)

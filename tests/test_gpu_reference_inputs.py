@@ -81,6 +81,8 @@ def test_6xm4_skin_configuration(device_ok):
      ["triangulatedSurf.face", "triangulatedSurf.vert", "1crn.epsmapx.txt", "1crn.epsmapy.txt",
       "1crn.epsmapz.txt", "1crn.idebmap.txt"]),
     ({"Save_Mesh_PLY_Format": "true", "Smooth_Mesh": "true"}, ["triangulatedSurf.ply"]),
+    ({"Build_epsilon_maps": "true", "Project_boundary_grid_points": "true", "Save_bgps": "true"},
+     ["1crn.projections.txt", "surf.txt"]),
 ])
 def test_1crn_normal_mode_files(device_ok, tmp_path, extra, files):
     # the 1crn SES configuration with output keys switched on: the files the
@@ -106,6 +108,14 @@ def test_1crn_normal_mode_files(device_ok, tmp_path, extra, files):
         v2, t2, n2 = io.load_ply(str(out_dir / "triangulatedSurf.ply"))
         # Smooth_Mesh: the saved mesh is the smoothed one, with its normals
         assert np.array_equal(v2, v) and np.array_equal(t2, t) and np.array_equal(n2, dev.normals())
+    elif "Save_bgps" in extra:
+        b = run["bgp"]
+        rows = open(out_dir / "1crn.projections.txt").read().splitlines()
+        assert int(rows[0]) == len(b["ibgp"]) == len(rows) - 1 > 1000
+        first = rows[1].split()
+        assert [int(x) - 1 for x in first[:3]] == list(b["ibgp"][0])
+        assert np.allclose([float(x) for x in first[3:6]], b["scspos"][0], atol=5e-7)
+        assert len(open(out_dir / "surf.txt").read().splitlines()) == len(b["ibgp"])
     elif "Save_Mesh_MSMS_Format" in extra:
         vert = open(out_dir / "triangulatedSurf.vert").read().splitlines()
         va = np.array([int(l.split()[7]) for l in vert[3:]]) - 1

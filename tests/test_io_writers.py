@@ -164,3 +164,16 @@ def test_deduce_format():
     assert io.deduce_format(save_msms=True) == "MSMS_NO_A"
     assert io.deduce_format(save_msms=True, vertex_atoms=True) == "MSMS"
     assert io.deduce_format(save_ply=True, save_msms=True) == "PLY"
+
+
+def test_save_bgp_format(tmp_path):
+    # DelPhiShared::saveBGP: count line, 1-based indices and "%lf" fields, no
+    # newline after the last line; surf.txt without the indices
+    ib = np.array([[1, 2, 3], [4, 5, 6]], np.int32)
+    pos = np.array([[0.5, -1.25, 2.0], [1e-7, 3.0, -0.0]])
+    nor = np.zeros((2, 3))
+    p1, p2 = io.save_bgp(str(tmp_path) + "/", "mol", ib, pos, nor)
+    assert open(p1).read() == ("2\n2 3 4 0.500000 -1.250000 2.000000 0.000000 0.000000 0.000000\n"
+                               "5 6 7 0.000000 3.000000 -0.000000 0.000000 0.000000 0.000000")
+    assert open(p2).read() == ("0.500000 -1.250000 2.000000 0.000000 0.000000 0.000000\n"
+                               "0.000000 3.000000 -0.000000 0.000000 0.000000 0.000000")
