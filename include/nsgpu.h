@@ -415,6 +415,49 @@ int nsgpu_project_bgp(nsgpu_ctx *ctx, int32_t aux_grid_dim, int64_t *counts);
  * :1437-1441), scspos [n*3] projections, scsnor [n*3] normals; any may be
  * NULL. */
 int nsgpu_read_bgp(nsgpu_ctx *ctx, int32_t *ibgp, double *scspos, double *scsnor);
+/* Pocket detection: the grid part of the CLI's pocket mode (pocketMode,
+ * main.cpp:1411-1660) on two surfaces of the same atoms on one grid, both
+ * after get_surf with the status map:
+ *   fat    probe Pocket_Radius_Big (3.0), cavities detected and all filled
+ *          (fill_cavities = 1, cav_vol = INFINITY);
+ *   small  probe Pocket_Radius_Small (1.4), cavities detected (fill_cavities
+ *          = 1; the reference detects them after the difference when
+ *          cavities and pockets are kept together, main.cpp:1494, :1629-1646).
+ * The difference map (not out in fat, out in small; small's cavities count as
+ * out when cav_and_pockets, Surface::difference Surface.cpp:11471-11718) is
+ * filtered by the filter_probe ball (operator-= sets 1.4: a node stays when it
+ * lies in a ball of that radius holding only difference nodes,
+ * filterCavities :2751-3010, cav2out :2649-2688), split into 6-connected
+ * components numbered from 4 in getCavities' alternating scan order (:1877-
+ * 1991), components of at most 11.4 * num_mol A^3 marked filled
+ * (fillCavities), the filter applied again to the rest (main.cpp:1616; a
+ * component it empties is filled too), and a kept component is a pocket when
+ * one of its nodes lies in small's outside component (markPockets,
+ * DelphiShared.cpp:1530-1584).  fat's status map (nsgpu_read_status) then
+ * holds the reference's labels: -id at the support points (the centres of the
+ * balls that fit), id at the other kept nodes and in filled components, -1
+ * elsewhere.  Both contexts on one device; fat's device buffers hold the work
+ * arrays. */
+typedef struct {
+    double filter_probe;     /* probe of the shape filter, 1.4 in the reference     */
+    double num_mol;          /* Num_Wat_Pocket (2): volume threshold 11.4 * num_mol */
+    int32_t cav_and_pockets; /* Pockets_And_Cavities (1)                            */
+    int32_t pad_;
+} nsgpu_pocket_opts;
+typedef struct {
+    int32_t num_components;  /* components of the filtered difference map          */
+    int32_t num_kept;        /* not filled: the cav%d.txt files of cavitiesToAtoms  */
+    int64_t support_points;  /* support points of the kept components               */
+} nsgpu_pocket_result;
+int nsgpu_pockets(nsgpu_ctx *fat, nsgpu_ctx *small, const nsgpu_pocket_opts *opts, nsgpu_pocket_result *res);
+/* The last nsgpu_pockets, per component in id order: volumes (node count *
+ * side^3, DelPhiShared::cavitiesSize), filled flags (cavitiesFlag), pocket
+ * flags (meaningful for kept components); offsets[num_kept + 1] and support
+ * [support_points]: the flat node indices of each kept component's support
+ * points (the points cavitiesToAtoms writes, DelphiShared.cpp:1669-1745), in
+ * flat order.  Any pointer may be NULL. */
+int nsgpu_read_pockets(nsgpu_ctx *ctx, double *volumes, uint8_t *filled, uint8_t *is_pocket, int64_t *offsets,
+                       int64_t *support);
 /* Compute_Vertex_Normals: when on, get_surf keeps the patch of every depth so
  * that nsgpu_read_normals can follow the mesh (off by default, as in the
  * reference; it costs the sort a second permuted array). */

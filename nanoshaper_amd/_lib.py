@@ -49,6 +49,16 @@ class MeshInfo(ctypes.Structure):
 VP = ctypes.c_void_p
 
 # name -> (restype, argtypes); every symbol declared in include/nsgpu.h
+class PocketOpts(ctypes.Structure):
+    _fields_ = [("filter_probe", ctypes.c_double), ("num_mol", ctypes.c_double),
+                ("cav_and_pockets", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+
+
+class PocketResult(ctypes.Structure):
+    _fields_ = [("num_components", ctypes.c_int32), ("num_kept", ctypes.c_int32),
+                ("support_points", ctypes.c_int64)]
+
+
 SIGNATURES = {
     "nsgpu_abi_version": (ctypes.c_int, []),
     "nsgpu_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(VP)]),
@@ -89,6 +99,8 @@ SIGNATURES = {
     "nsgpu_smooth_surface": (ctypes.c_int, [VP]),
     "nsgpu_project_bgp": (ctypes.c_int, [VP, ctypes.c_int32, VP]),
     "nsgpu_read_bgp": (ctypes.c_int, [VP, VP, VP, VP]),
+    "nsgpu_pockets": (ctypes.c_int, [VP, VP, ctypes.POINTER(PocketOpts), ctypes.POINTER(PocketResult)]),
+    "nsgpu_read_pockets": (ctypes.c_int, [VP, VP, VP, VP, VP, VP]),
     "nsgpu_set_vertex_normals": (ctypes.c_int, [VP, ctypes.c_int]),
     "nsgpu_read_ray_hits": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_int32, VP, VP]),
     "nsgpu_set_mesh": (ctypes.c_int, [VP, VP, ctypes.c_int32, VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),

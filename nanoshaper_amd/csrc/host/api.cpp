@@ -188,6 +188,14 @@ struct nsgpu_ctx {
         bgcnt, bgcoi, bgnaux, bgcflag, bgcoffa, bgactive, bgfq;
     int64_t nbgp = 0;
     bool have_bgp = false;
+    // pockets (nsgpu_pockets, on the fat-probe context): bit planes, union-find,
+    // per-component arrays, support lists; host results in id order
+    DBuf pkD, pkS1, pkK, pkU, pkS2, pkK2, pkroots, pkparent, pkrcount, pkroff, pkball, pkccount, pkcroot, pkcfilled,
+        pkcid, pkckept, pkcpocket, pkcbkey, pkscount, pksoff, pksupn, pksupc;
+    std::vector<double> pk_vol;
+    std::vector<uint8_t> pk_filled, pk_pocket;
+    std::vector<int64_t> pk_off, pk_sup;
+    bool have_pockets = false;
     // the cached tables of bgp_tables and what they were built for
     BgpArgs bgA{};
     uint64_t bg_setgen = 0;
@@ -301,7 +309,10 @@ static void for_each_buf(nsgpu_ctx *c, F fn) {
                    &c->crcoord, &c->crcount, &c->crhits, &c->crnorm, &c->bgbits, &c->bgwcount, &c->bgwoff,
                    &c->bgrange, &c->bgccount, &c->bgcoff, &c->bgccursor, &c->bgclist, &c->bgfkey, &c->bgfval,
                    &c->bgib, &c->bgpos, &c->bgnor, &c->bgcnt, &c->bgcoi, &c->bgnaux, &c->bgcflag,
-                   &c->bgcoffa, &c->bgactive, &c->bgfq};
+                   &c->bgcoffa, &c->bgactive, &c->bgfq, &c->pkD, &c->pkS1, &c->pkK, &c->pkU, &c->pkS2, &c->pkK2,
+                   &c->pkroots, &c->pkparent, &c->pkrcount, &c->pkroff, &c->pkball, &c->pkccount, &c->pkcroot,
+                   &c->pkcfilled, &c->pkcid, &c->pkckept, &c->pkcpocket, &c->pkcbkey, &c->pkscount, &c->pksoff,
+                   &c->pksupn, &c->pksupc};
     for (DBuf *b : all) fn(b);
 }
 
@@ -826,6 +837,7 @@ static int begin_surf(nsgpu_ctx *c, const nsgpu_surf_opts *o) {
     std::memset(&c->res, 0, sizeof c->res);
     c->have_trace = c->have_surf = c->have_vertices = c->have_mesh = false;
     c->have_bgp = false;
+    c->have_pockets = false;
     c->panels[0] = 3;
     c->panels[1] = 3;
     if (!o->build_status && !o->build_epsmap) c->panels[0] = 0;
@@ -2558,6 +2570,222 @@ int nsgpu_read_bgp(nsgpu_ctx *c, int32_t *ibgp, double *scspos, double *scsnor) 
     if (n && scspos) HIP_TRY(c, hipMemcpyAsync(scspos, c->bgpos.p, n * 24, hipMemcpyDeviceToHost, c->stream));
     if (n && scsnor) HIP_TRY(c, hipMemcpyAsync(scsnor, c->bgnor.p, n * 24, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return NSGPU_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Pocket detection (the grid part of pocketMode, main.cpp:1411-1660) on the
+// status maps of a fat-probe and a small-probe surface.
+
+extern "C" {
+
+int nsgpu_pockets(nsgpu_ctx *fat, nsgpu_ctx *small, const nsgpu_pocket_opts *o, nsgpu_pocket_result *res) {
+    if (!fat || !small || !o || fat == small) return NSGPU_E_ARG;
+    nsgpu_ctx *c = fat;
+    if (!fat->have_surf || !small->have_surf || !fat->opts.build_status || !small->opts.build_status)
+        return fail(c, NSGPU_E_STATE, "pockets need a getSurf with the status map on both surfaces");
+    if (fat->device != small->device) return fail(c, NSGPU_E_ARG, "both surfaces must be on one device");
+    const nsgpu_grid &G = fat->grid;
+    if (G.nx != small->grid.nx || G.ny != small->grid.ny || G.nz != small->grid.nz)
+        return fail(c, NSGPU_E_ARG, "the two surfaces are on different grids");
+    if (!(o->filter_probe > 0) || !(o->num_mol >= 0)) return fail(c, NSGPU_E_ARG, "bad pocket options");
+    HIP_TRY(c, hipSetDevice(c->device));
+    c->have_pockets = false;
+    const DevGrid g = dev_grid(c);
+    const int64_t nvox = (int64_t)G.nx * G.ny * G.nz;
+    const int W = (G.nx + 63) / 64;
+    const int64_t nwords = (int64_t)G.ny * G.nz * W;
+    // the probe's digital ball (filterCavities' box and dist2 test,
+    // Surface.cpp:2775-2830): per (dy, dz) the largest |dx| of a node within
+    // the probe radius
+    std::vector<int> ball;
+    {
+        const double r = o->filter_probe, side = G.side, r2 = r * r;
+        const int m = (int)std::floor(r / side) + 1;
+        for (int dk = -m; dk <= m; ++dk)
+            for (int dj = -m; dj <= m; ++dj) {
+                int hw = -1;
+                for (int di = 0; di <= m; ++di) {
+                    const double a = di * side, b = dj * side, cc = dk * side;
+                    if (a * a + b * b + cc * cc <= r2) hw = di;
+                }
+                if (hw < 0) continue;
+                if (hw > 63) return fail(c, NSGPU_E_ARG, "the filter probe spans more than 63 grid nodes");
+                ball.insert(ball.end(), {dj, dk, hw});
+            }
+    }
+    PocketArgs A{};
+    A.W = W;
+    A.nwords = nwords;
+    A.cav_and_pockets = o->cav_and_pockets != 0;
+    A.nball = (int)(ball.size() / 3);
+    HIP_TRY(c, c->pkball.ensure(ball.size() * 4));
+    HIP_TRY(c, hipMemcpyAsync(c->pkball.p, ball.data(), ball.size() * 4, hipMemcpyHostToDevice, c->stream));
+    for (DBuf *b : {&c->pkD, &c->pkS1, &c->pkK, &c->pkU, &c->pkS2, &c->pkK2, &c->pkroots})
+        HIP_TRY(c, b->ensure((size_t)nwords * 8));
+    for (DBuf *b : {&c->pkrcount, &c->pkroff, &c->pkscount, &c->pksoff}) HIP_TRY(c, b->ensure((size_t)nwords * 4));
+    HIP_TRY(c, c->pkparent.ensure((size_t)nvox * 4));
+    HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(nwords) * 4));
+    A.ball = c->pkball.as<int>();
+    A.st1 = c->status.as<int>();
+    A.st2 = small->status.as<int>();
+    A.D = c->pkD.as<unsigned long long>();
+    A.U = c->pkU.as<unsigned long long>();
+    A.S2 = c->pkS2.as<unsigned long long>();
+    A.K2 = c->pkK2.as<unsigned long long>();
+    A.roots = c->pkroots.as<unsigned long long>();
+    A.parent = c->pkparent.as<int>();
+    A.rcount = c->pkrcount.as<int>();
+    A.roff = c->pkroff.as<int>();
+    A.scount = c->pkscount.as<int>();
+    A.soff = c->pksoff.as<int>();
+    unsigned long long *S1 = c->pkS1.as<unsigned long long>(), *K = c->pkK.as<unsigned long long>();
+    // the small surface's map is complete: its getSurf synchronised its stream
+    // difference, the shape filter on it (operator-=, Surface.cpp:15556-15616),
+    // the components of what it keeps
+    launch_pk_diff(c->stream, g, A);
+    launch_pk_morph(c->stream, g, A, A.D, nullptr, S1, false);
+    launch_pk_morph(c->stream, g, A, S1, A.D, K, true);
+    launch_pk_components(c->stream, g, A, K);
+    launch_scan(c->stream, A.rcount, nwords, A.roff, c->scan_scratch.as<int>());
+    HIP_TRY(c, hipGetLastError());
+    int64_t ncomp = 0;
+    int rc = read_total(c, A.rcount, A.roff, nwords, &ncomp);
+    if (rc) return rc;
+    const size_t nc = (size_t)std::max<int64_t>(ncomp, 1);
+    for (DBuf *b : {&c->pkccount, &c->pkcroot, &c->pkcfilled, &c->pkcid, &c->pkckept, &c->pkcpocket})
+        HIP_TRY(c, b->ensure(nc * 4));
+    HIP_TRY(c, c->pkcbkey.ensure(nc * 8));
+    A.ccount = c->pkccount.as<int>();
+    A.croot = c->pkcroot.as<int>();
+    A.cfilled = c->pkcfilled.as<int>();
+    A.cid = c->pkcid.as<int>();
+    A.ckept = c->pkckept.as<int>();
+    A.cpocket = c->pkcpocket.as<int>();
+    A.cbkey = c->pkcbkey.as<long long>();
+    std::vector<int> cnt(nc), root(nc);
+    std::vector<long long> bkey(nc);
+    if (ncomp > 0) {
+        launch_fill32(c->stream, A.ccount, 0u, ncomp);
+        launch_fill32(c->stream, A.cbkey, 0x7fffffffu, 2 * ncomp);
+        launch_pk_stats(c->stream, g, A);
+        HIP_TRY(c, hipGetLastError());
+        HIP_TRY(c, hipMemcpyAsync(cnt.data(), A.ccount, (size_t)ncomp * 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(root.data(), A.croot, (size_t)ncomp * 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(bkey.data(), A.cbkey, (size_t)ncomp * 8, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
+    // ids in the alternating scan order of getCavities(STATUS_POINT_OUT)
+    // (Surface.cpp:1920-1981): even passes take the component met first
+    // scanning k upwards (smallest root), odd passes scanning k downwards
+    std::vector<int> by_f(ncomp), by_b(ncomp), id((size_t)ncomp), comp_of_id((size_t)ncomp);
+    for (int q = 0; q < ncomp; ++q) by_f[q] = by_b[q] = q;
+    std::sort(by_f.begin(), by_f.end(), [&](int a, int b) { return root[a] < root[b]; });
+    std::sort(by_b.begin(), by_b.end(), [&](int a, int b) { return bkey[a] < bkey[b]; });
+    {
+        std::vector<char> done((size_t)ncomp, 0);
+        size_t pf = 0, pb = 0;
+        for (int t = 0; t < ncomp; ++t) {
+            int q;
+            if (t % 2 == 0) {
+                while (done[by_f[pf]]) ++pf;
+                q = by_f[pf];
+            } else {
+                while (done[by_b[pb]]) ++pb;
+                q = by_b[pb];
+            }
+            done[q] = 1;
+            id[q] = NSGPU_STATUS_FIRST_CAV + t;
+            comp_of_id[t] = q;
+        }
+    }
+    // volume threshold (fillCavities(11.4 * Num_Wat_Pocket), :2355-2487)
+    const double cubeVol = G.side * G.side * G.side;
+    const double vmax = 11.4 * o->num_mol;
+    std::vector<int> filled((size_t)ncomp);
+    c->pk_vol.assign((size_t)ncomp, 0.0);
+    for (int t = 0; t < ncomp; ++t) {
+        const int q = comp_of_id[t];
+        c->pk_vol[t] = cnt[q] * cubeVol;
+        filled[q] = c->pk_vol[t] <= vmax;
+    }
+    if (ncomp > 0) {
+        HIP_TRY(c, hipMemcpyAsync(A.cid, id.data(), (size_t)ncomp * 4, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(A.cfilled, filled.data(), (size_t)ncomp * 4, hipMemcpyHostToDevice, c->stream));
+        launch_fill32(c->stream, A.ckept, 0u, ncomp);
+        launch_fill32(c->stream, A.cpocket, 0u, ncomp);
+    }
+    // the filter on the kept components (main.cpp:1616), the final labels
+    launch_pk_union_kept(c->stream, g, A);
+    launch_pk_morph(c->stream, g, A, A.U, nullptr, A.S2, false);
+    launch_pk_morph(c->stream, g, A, A.S2, A.U, A.K2, true);
+    launch_pk_label(c->stream, g, A);
+    launch_scan(c->stream, A.scount, nwords, A.soff, c->scan_scratch.as<int>());
+    HIP_TRY(c, hipGetLastError());
+    int64_t nsup = 0;
+    rc = read_total(c, A.scount, A.soff, nwords, &nsup);
+    if (rc) return rc;
+    HIP_TRY(c, c->pksupn.ensure((size_t)std::max<int64_t>(nsup, 1) * 8));
+    HIP_TRY(c, c->pksupc.ensure((size_t)std::max<int64_t>(nsup, 1) * 4));
+    A.sup_node = c->pksupn.as<long long>();
+    A.sup_comp = c->pksupc.as<int>();
+    launch_pk_support_list(c->stream, g, A);
+    HIP_TRY(c, hipGetLastError());
+    std::vector<int> kept((size_t)ncomp), pocket((size_t)ncomp), supc((size_t)nsup);
+    std::vector<long long> supn((size_t)nsup);
+    if (ncomp > 0) {
+        HIP_TRY(c, hipMemcpyAsync(kept.data(), A.ckept, (size_t)ncomp * 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(pocket.data(), A.cpocket, (size_t)ncomp * 4, hipMemcpyDeviceToHost, c->stream));
+    }
+    if (nsup > 0) {
+        HIP_TRY(c, hipMemcpyAsync(supn.data(), A.sup_node, (size_t)nsup * 8, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(supc.data(), A.sup_comp, (size_t)nsup * 4, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    // a component the filter removed entirely counts as filled
+    // (filterCavities, Surface.cpp:2996-3006); results in id order, support
+    // points of each kept component in flat order
+    c->pk_filled.assign((size_t)ncomp, 0);
+    c->pk_pocket.assign((size_t)ncomp, 0);
+    std::vector<int> kept_rank((size_t)ncomp, -1);
+    int nk = 0;
+    for (int t = 0; t < ncomp; ++t) {
+        const int q = comp_of_id[t];
+        c->pk_filled[t] = filled[q] || kept[q] == 0;
+        c->pk_pocket[t] = pocket[q] != 0;
+        if (!c->pk_filled[t]) kept_rank[q] = nk++;
+    }
+    c->pk_off.assign((size_t)nk + 1, 0);
+    for (int64_t q = 0; q < nsup; ++q)
+        if (kept_rank[supc[q]] >= 0) c->pk_off[kept_rank[supc[q]] + 1]++;
+    for (int k = 0; k < nk; ++k) c->pk_off[k + 1] += c->pk_off[k];
+    c->pk_sup.assign((size_t)c->pk_off[nk], 0);
+    {
+        std::vector<int64_t> cur(c->pk_off.begin(), c->pk_off.end() - 1);
+        for (int64_t q = 0; q < nsup; ++q)
+            if (kept_rank[supc[q]] >= 0) c->pk_sup[cur[kept_rank[supc[q]]]++] = supn[q];
+    }
+    c->have_pockets = true;
+    if (res) {
+        res->num_components = (int32_t)ncomp;
+        res->num_kept = nk;
+        res->support_points = c->pk_off[nk];
+    }
+    return NSGPU_OK;
+}
+
+int nsgpu_read_pockets(nsgpu_ctx *c, double *volumes, uint8_t *filled, uint8_t *is_pocket, int64_t *offsets,
+                       int64_t *support) {
+    if (!c) return NSGPU_E_ARG;
+    if (!c->have_pockets) return fail(c, NSGPU_E_STATE, "no pockets computed");
+    const size_t n = c->pk_vol.size();
+    if (volumes && n) std::memcpy(volumes, c->pk_vol.data(), n * sizeof(double));
+    if (filled && n) std::memcpy(filled, c->pk_filled.data(), n);
+    if (is_pocket && n) std::memcpy(is_pocket, c->pk_pocket.data(), n);
+    if (offsets) std::memcpy(offsets, c->pk_off.data(), c->pk_off.size() * sizeof(int64_t));
+    if (support && !c->pk_sup.empty()) std::memcpy(support, c->pk_sup.data(), c->pk_sup.size() * sizeof(int64_t));
     return NSGPU_OK;
 }
 

@@ -276,4 +276,33 @@ void launch_bgp_bin(hipStream_t, const DevPatches &, const BgpArgs &, bool fill)
 void launch_bgp_project(hipStream_t, const DevGrid &, const DevPatches &, const BgpArgs &, double probe,
                         int64_t n_active);
 
+
+// Pocket detection on bit planes (rows of W words, word (row, w) = nodes
+// x = 64 w .. 64 w + 63 of row k * ny + j); see kernels/pocket.hip.
+struct PocketArgs {
+    int W;
+    int64_t nwords;
+    int *st1;         // fat-probe status, rewritten to the pocket labels
+    const int *st2;   // small-probe status (outside component = NSGPU_STATUS_OUT)
+    int cav_and_pockets;
+    const int *ball;  // (dy, dz, half width along x) per row of the probe's digital ball
+    int nball;
+    unsigned long long *D, *U, *S2, *K2, *roots;
+    int *parent, *rcount, *roff;
+    // per component (index = rank of its root, the smallest flat index)
+    int *ccount, *croot, *cfilled, *cid, *ckept, *cpocket;
+    long long *cbkey;
+    // support points in flat order
+    int *scount, *soff, *sup_comp;
+    long long *sup_node;
+};
+void launch_pk_diff(hipStream_t, const DevGrid &, const PocketArgs &);
+void launch_pk_morph(hipStream_t, const DevGrid &, const PocketArgs &, const unsigned long long *in,
+                     const unsigned long long *and_mask, unsigned long long *out, bool dilate);
+void launch_pk_components(hipStream_t, const DevGrid &, const PocketArgs &, const unsigned long long *m);
+void launch_pk_stats(hipStream_t, const DevGrid &, const PocketArgs &);
+void launch_pk_union_kept(hipStream_t, const DevGrid &, const PocketArgs &);
+void launch_pk_label(hipStream_t, const DevGrid &, const PocketArgs &);
+void launch_pk_support_list(hipStream_t, const DevGrid &, const PocketArgs &);
+
 }  // namespace nsg

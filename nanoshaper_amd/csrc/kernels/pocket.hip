@@ -1,0 +1,310 @@
+// Pocket detection, the grid part of pocket mode (main.cpp:1411-1660), on
+// bit planes of the grid: one bit per node, rows of W = ceil(nx / 64) words,
+// word (row, w) holding nodes x = 64 w .. 64 w + 63 of row k * ny + j.
+//
+//   k_pk_diff     the difference map (Surface::difference,
+//                 Surface.cpp:11471-11718): not out in the fat-probe status,
+//                 out in the small-probe one
+//   k_pk_morph    the reference's shape filter (filterCavities,
+//                 Surface.cpp:2751-3010) is a morphological opening by the
+//                 probe's digital ball: a support point is a node whose ball
+//                 holds only set nodes (erosion; nodes beyond the grid are
+//                 not tested), a kept node one within the ball of a support
+//                 point (dilation).  The ball is a table of (dy, dz, half
+//                 width in x); a word of the result combines the words of
+//                 the table's rows, each widened or narrowed along x by word
+//                 shifts
+//   k_pk_uf_*     6-connected components of the kept nodes by union-find over
+//                 node indices (each root the component's smallest flat
+//                 index), the component keys of the reference's alternating
+//                 scan, counts; the host numbers the components
+//   k_pk_label    the final status map and per-component tallies
+#include "launch.hpp"
+
+namespace nsg {
+
+namespace {
+
+__device__ __forceinline__ int pk_ld(const int *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void pk_st(int *p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ int pk_find(int *P, int x) {
+    int p = pk_ld(P + x);
+    while (p != x) {
+        const int gp = pk_ld(P + p);
+        if (gp != p) pk_st(P + x, gp);
+        x = p;
+        p = gp;
+    }
+    return x;
+}
+
+__device__ void pk_unite(int *P, int a, int b) {
+    for (;;) {
+        a = pk_find(P, a);
+        b = pk_find(P, b);
+        if (a == b) return;
+        if (a < b) {
+            const int t = a;
+            a = b;
+            b = t;
+        }
+        const int old = atomicCAS(P + a, a, b);
+        if (old == a) return;
+        a = old;
+    }
+}
+
+__device__ __forceinline__ bool bit_at(const unsigned long long *m, int W, int64_t row, int x) {
+    return (m[row * W + (x >> 6)] >> (x & 63)) & 1ull;
+}
+
+// one wave per word: lane = node
+__global__ void __launch_bounds__(256) k_pk_diff(DevGrid g, PocketArgs A) {
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wave >= A.nwords) return;
+    const int64_t row = wave / A.W;
+    const int x = (int)(wave - row * A.W) * 64 + lane;
+    bool d = false;
+    if (x < g.nx) {
+        const int64_t n = row * g.nx + x;
+        const int s1 = A.st1[n], s2 = A.st2[n];
+        const bool out2 = s2 == NSGPU_STATUS_TEMP_OUT || s2 == NSGPU_STATUS_OUT ||
+                          (A.cav_and_pockets && s2 >= NSGPU_STATUS_FIRST_CAV);
+        d = s1 != NSGPU_STATUS_TEMP_OUT && s1 != NSGPU_STATUS_OUT && out2;
+    }
+    const unsigned long long m = __ballot(d);
+    if (lane == 0) A.D[wave] = m;
+}
+
+// erosion (dilate = false) or dilation of `in` by the ball; one thread per
+// word.  A row's words outside the grid read as all ones for the erosion and
+// zeros for the dilation, as do the bits past the row's last node.
+__global__ void __launch_bounds__(256) k_pk_morph(DevGrid g, PocketArgs A, const unsigned long long *in,
+                                                  const unsigned long long *and_mask, unsigned long long *out,
+                                                  bool dilate) {
+    const int64_t wq = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (wq >= A.nwords) return;
+    const int64_t row = wq / A.W;
+    const int w = (int)(wq - row * A.W);
+    const int j = (int)(row % g.ny), k = (int)(row / g.ny);
+    const unsigned long long fill = dilate ? 0ull : ~0ull;
+    // bits of the last word past nx
+    const int tail = g.nx - (A.W - 1) * 64;
+    const unsigned long long past = tail >= 64 ? 0ull : (~0ull << tail);
+    unsigned long long acc = dilate ? 0ull : ~0ull;
+    for (int e = 0; e < A.nball; ++e) {
+        const int dj = A.ball[3 * e], dk = A.ball[3 * e + 1], hw = A.ball[3 * e + 2];
+        const int jj = j + dj, kk = k + dk;
+        if (jj < 0 || jj >= g.ny || kk < 0 || kk >= g.nz) continue;  // erosion: ones; dilation: zeros
+        const int64_t r2 = (int64_t)kk * g.ny + jj;
+        auto word = [&](int ww) -> unsigned long long {
+            if (ww < 0 || ww >= A.W) return fill;
+            unsigned long long v = in[r2 * A.W + ww];
+            if (ww == A.W - 1) v = dilate ? (v & ~past) : (v | past);
+            return v;
+        };
+        const unsigned long long cur = word(w), lo = word(w - 1), hi = word(w + 1);
+        unsigned long long r = cur;
+        for (int d = 1; d <= hw; ++d) {
+            const unsigned long long up = (cur >> d) | (hi << (64 - d));
+            const unsigned long long dn = (cur << d) | (lo >> (64 - d));
+            r = dilate ? (r | up | dn) : (r & up & dn);
+        }
+        acc = dilate ? (acc | r) : (acc & r);
+    }
+    if (w == A.W - 1) acc &= ~past;
+    if (and_mask) acc &= and_mask[wq];
+    out[wq] = acc;
+}
+
+// union-find over the set nodes of `m` (parent[n] = n for a set node)
+__global__ void __launch_bounds__(256) k_pk_uf_init(DevGrid g, PocketArgs A, const unsigned long long *m) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (int64_t)g.nx * g.ny * g.nz) return;
+    const int64_t row = n / g.nx;
+    const int x = (int)(n - row * g.nx);
+    A.parent[n] = bit_at(m, A.W, row, x) ? (int)n : -1;
+}
+
+__global__ void __launch_bounds__(256) k_pk_uf_union(DevGrid g, PocketArgs A, const unsigned long long *m) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (int64_t)g.nx * g.ny * g.nz) return;
+    const int64_t row = n / g.nx;
+    const int x = (int)(n - row * g.nx);
+    if (!bit_at(m, A.W, row, x)) return;
+    const int j = (int)(row % g.ny), k = (int)(row / g.ny);
+    if (x > 0 && bit_at(m, A.W, row, x - 1)) pk_unite(A.parent, (int)n, (int)n - 1);
+    if (j > 0 && bit_at(m, A.W, row - 1, x)) pk_unite(A.parent, (int)n, (int)(n - g.nx));
+    if (k > 0 && bit_at(m, A.W, row - g.ny, x)) pk_unite(A.parent, (int)n, (int)(n - (int64_t)g.nx * g.ny));
+}
+
+// flatten; roots as a bit plane (their rank = the component's index)
+__global__ void __launch_bounds__(256) k_pk_uf_roots(DevGrid g, PocketArgs A) {
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wave >= A.nwords) return;
+    const int64_t row = wave / A.W;
+    const int x = (int)(wave - row * A.W) * 64 + lane;
+    bool root = false;
+    if (x < g.nx) {
+        const int64_t n = row * g.nx + x;
+        if (pk_ld(A.parent + n) >= 0) {
+            // a find that writes only this node's own entry: another thread
+            // reading it sees its old parent or its root, both ancestors
+            int r = (int)n, p = pk_ld(A.parent + n);
+            while (p != r) {
+                r = p;
+                p = pk_ld(A.parent + p);
+            }
+            pk_st(A.parent + n, r);
+            root = r == (int)n;
+        }
+    }
+    const unsigned long long m = __ballot(root);
+    if (lane == 0) {
+        A.roots[wave] = m;
+        A.rcount[wave] = __popcll(m);
+    }
+}
+
+// the component index of root flat index r: its rank among the roots
+__device__ __forceinline__ int root_rank(const DevGrid &g, const PocketArgs &A, int r) {
+    const int64_t row = r / g.nx;
+    const int x = (int)(r - row * g.nx);
+    const int64_t wq = row * A.W + (x >> 6);
+    const int b = x & 63;
+    return A.roff[wq] + __popcll(b ? (A.roots[wq] & (~0ull >> (64 - b))) : 0ull);
+}
+
+// per component: node count and the backward-scan key (k descending, then j
+// and i ascending: ((nz - 1 - k) ny + j) nx + i); the forward key is the root
+__global__ void __launch_bounds__(256) k_pk_stats(DevGrid g, PocketArgs A) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (int64_t)g.nx * g.ny * g.nz) return;
+    const int r = A.parent[n];
+    if (r < 0) return;
+    const int c = root_rank(g, A, r);
+    const int64_t plane = (int64_t)g.nx * g.ny;
+    const int64_t k = n / plane;
+    const long long bkey = (long long)(g.nz - 1 - k) * plane + (n - k * plane);
+    atomicAdd(A.ccount + c, 1);
+    atomicMin(A.cbkey + c, bkey);
+    if (r == (int)n) A.croot[c] = r;
+}
+
+// the kept components' nodes (components not filled by the volume threshold)
+__global__ void __launch_bounds__(256) k_pk_union_kept(DevGrid g, PocketArgs A) {
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wave >= A.nwords) return;
+    const int64_t row = wave / A.W;
+    const int x = (int)(wave - row * A.W) * 64 + lane;
+    bool u = false;
+    if (x < g.nx) {
+        const int r = A.parent[row * g.nx + x];
+        if (r >= 0) u = !A.cfilled[root_rank(g, A, r)];
+    }
+    const unsigned long long m = __ballot(u);
+    if (lane == 0) A.U[wave] = m;
+}
+
+// final status map (support -id, kept id, a filled component id, else
+// inside), per component the kept nodes and whether a node lies in the small
+// surface's outside; support points as a bit plane for the ordered lists
+__global__ void __launch_bounds__(256) k_pk_label(DevGrid g, PocketArgs A) {
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wave >= A.nwords) return;
+    const int64_t row = wave / A.W;
+    const int x = (int)(wave - row * A.W) * 64 + lane;
+    bool sup = false;
+    if (x < g.nx) {
+        const int64_t n = row * g.nx + x;
+        const int r = A.parent[n];
+        int st = NSGPU_STATUS_INSIDE;
+        if (r >= 0) {
+            const int c = root_rank(g, A, r);
+            const int id = A.cid[c];
+            const unsigned long long bit = 1ull << (x & 63);
+            const int64_t wq = row * A.W + (x >> 6);
+            if (A.cfilled[c]) {
+                st = id;
+            } else if (A.S2[wq] & bit) {
+                st = -id;
+                sup = true;
+                atomicAdd(A.ckept + c, 1);
+            } else if (A.K2[wq] & bit) {
+                st = id;
+                atomicAdd(A.ckept + c, 1);
+            }
+            if (A.st2[n] == NSGPU_STATUS_TEMP_OUT || A.st2[n] == NSGPU_STATUS_OUT) A.cpocket[c] = 1;
+        }
+        A.st1[n] = st;
+    }
+    const unsigned long long m = __ballot(sup);
+    if (lane == 0) A.scount[wave] = __popcll(m);
+}
+
+// support nodes in flat order with their component index
+__global__ void __launch_bounds__(256) k_pk_support_list(DevGrid g, PocketArgs A) {
+    const int64_t wq = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (wq >= A.nwords) return;
+    unsigned long long m = A.S2[wq];
+    if (!m) return;
+    const int64_t row = wq / A.W;
+    const int x0 = (int)(wq - row * A.W) * 64;
+    int at = A.soff[wq];
+    while (m) {
+        const int b = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        const int64_t n = row * g.nx + x0 + b;
+        A.sup_node[at] = (long long)n;
+        A.sup_comp[at] = root_rank(g, A, A.parent[n]);
+        ++at;
+    }
+}
+
+}  // namespace
+
+static unsigned blocks_of(int64_t threads) { return (unsigned)((threads + 255) / 256); }
+
+void launch_pk_diff(hipStream_t s, const DevGrid &g, const PocketArgs &A) {
+    hipLaunchKernelGGL(k_pk_diff, dim3(blocks_of(A.nwords * 64)), dim3(256), 0, s, g, A);
+}
+
+void launch_pk_morph(hipStream_t s, const DevGrid &g, const PocketArgs &A, const unsigned long long *in,
+                     const unsigned long long *and_mask, unsigned long long *out, bool dilate) {
+    hipLaunchKernelGGL(k_pk_morph, dim3(blocks_of(A.nwords)), dim3(256), 0, s, g, A, in, and_mask, out, dilate);
+}
+
+void launch_pk_components(hipStream_t s, const DevGrid &g, const PocketArgs &A, const unsigned long long *m) {
+    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
+    hipLaunchKernelGGL(k_pk_uf_init, dim3(blocks_of(nvox)), dim3(256), 0, s, g, A, m);
+    hipLaunchKernelGGL(k_pk_uf_union, dim3(blocks_of(nvox)), dim3(256), 0, s, g, A, m);
+    hipLaunchKernelGGL(k_pk_uf_roots, dim3(blocks_of(A.nwords * 64)), dim3(256), 0, s, g, A);
+}
+
+void launch_pk_stats(hipStream_t s, const DevGrid &g, const PocketArgs &A) {
+    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
+    hipLaunchKernelGGL(k_pk_stats, dim3(blocks_of(nvox)), dim3(256), 0, s, g, A);
+}
+
+void launch_pk_union_kept(hipStream_t s, const DevGrid &g, const PocketArgs &A) {
+    hipLaunchKernelGGL(k_pk_union_kept, dim3(blocks_of(A.nwords * 64)), dim3(256), 0, s, g, A);
+}
+
+void launch_pk_label(hipStream_t s, const DevGrid &g, const PocketArgs &A) {
+    hipLaunchKernelGGL(k_pk_label, dim3(blocks_of(A.nwords * 64)), dim3(256), 0, s, g, A);
+}
+
+void launch_pk_support_list(hipStream_t s, const DevGrid &g, const PocketArgs &A) {
+    hipLaunchKernelGGL(k_pk_support_list, dim3(blocks_of(A.nwords)), dim3(256), 0, s, g, A);
+}
+
+}  // namespace nsg

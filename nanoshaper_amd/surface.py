@@ -411,6 +411,27 @@ class DeviceSurface:
         self._chk(self._lib.nsgpu_read_bgp(self._h, _ptr(ib), _ptr(pos), _ptr(nor)), "read_bgp")
         return {"ibgp": ib, "scspos": pos, "scsnor": nor, "counts": counts}
 
+    def pockets(self, small, filter_probe=1.4, num_mol=2, cav_and_pockets=True):
+        """The grid part of pocket mode with this surface as the fat-probe one
+        and `small` the small-probe one (both after get_surf with the status
+        map on the same grid; see nsgpu_pockets): this surface's status map
+        becomes the pocket labels.  Returns {'volumes', 'filled', 'is_pocket'}
+        per component (ids 4, 5, ...) and 'support': the flat node indices of
+        each kept component's support points."""
+        o = _lib.PocketOpts(float(filter_probe), float(num_mol), int(bool(cav_and_pockets)), 0)
+        r = _lib.PocketResult()
+        self._chk(self._lib.nsgpu_pockets(self._h, small._h, ctypes.byref(o), ctypes.byref(r)), "pockets")
+        nc, nk, ns_ = r.num_components, r.num_kept, r.support_points
+        vol = np.zeros(max(nc, 1))
+        filled = np.zeros(max(nc, 1), np.uint8)
+        pocket = np.zeros(max(nc, 1), np.uint8)
+        off = np.zeros(nk + 1, np.int64)
+        sup = np.zeros(max(ns_, 1), np.int64)
+        self._chk(self._lib.nsgpu_read_pockets(self._h, _ptr(vol), _ptr(filled), _ptr(pocket), _ptr(off), _ptr(sup)),
+                  "read_pockets")
+        return {"volumes": vol[:nc], "filled": filled[:nc].astype(bool), "is_pocket": pocket[:nc].astype(bool),
+                "support": [sup[off[q]:off[q + 1]] for q in range(nk)]}
+
     def set_vertex_normals(self, on=True):
         """Compute_Vertex_Normals: keep what normals() needs from the next get_surf on."""
         self._chk(self._lib.nsgpu_set_vertex_normals(self._h, int(on)), "set_vertex_normals")

@@ -65,6 +65,8 @@ def load():
         "orc_free_buf": (None, [VP]),
         "orc_project_bgp": (ctypes.c_int, [VP, ctypes.c_int, VP]),
         "orc_read_bgp": (None, [VP, VP, VP, VP]),
+        "orc_pockets": (ctypes.c_int, [VP, VP, ctypes.c_double, ctypes.c_double, ctypes.c_int, VP]),
+        "orc_read_pockets": (None, [VP, VP, VP, VP, VP, VP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -234,6 +236,26 @@ class OracleSurface:
         nor = np.empty((n, 3))
         self._lib.orc_read_bgp(self._h, _ptr(ib), _ptr(pos), _ptr(nor))
         return {"ibgp": ib, "scspos": pos, "scsnor": nor, "counts": counts}
+
+    def pockets(self, small, filter_probe=1.4, num_mol=2, cav_and_pockets=True):
+        """The grid part of pocket mode on this (fat-probe) surface and the
+        small-probe surface `small` (both after get_surf on the same grid):
+        rewrites this surface's status to the pocket labels and returns
+        {'volumes', 'filled', 'is_pocket'} per component (ids 4, 5, ...) and
+        'support': for each kept component, its support nodes (flat indices)."""
+        counts = np.zeros(3, np.int64)
+        if self._lib.orc_pockets(self._h, small._h, float(filter_probe), float(num_mol), int(bool(cav_and_pockets)),
+                                 _ptr(counts)) != 0:
+            raise RuntimeError("oracle pockets need two status maps on one grid")
+        nc, nk, ns_ = (int(c) for c in counts)
+        vol = np.zeros(max(nc, 1))
+        filled = np.zeros(max(nc, 1), np.uint8)
+        pocket = np.zeros(max(nc, 1), np.uint8)
+        off = np.zeros(nk + 1, np.int64)
+        sup = np.zeros(max(ns_, 1), np.int64)
+        self._lib.orc_read_pockets(self._h, _ptr(vol), _ptr(filled), _ptr(pocket), _ptr(off), _ptr(sup))
+        return {"volumes": vol[:nc], "filled": filled[:nc].astype(bool), "is_pocket": pocket[:nc].astype(bool),
+                "support": [sup[off[q]:off[q + 1]] for q in range(nk)]}
 
     def ray_hits(self, slot):
         g = self.grid

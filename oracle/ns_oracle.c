@@ -541,6 +541,11 @@ typedef struct orc_surface {
     double *mv;  /* 3 per vertex */
     int32_t *mt; /* 3 per triangle */
     int64_t mnv, mnt;
+    /* pockets of the last orc_pockets: per component the pocket flag; the
+     * support nodes of the kept components, CSR */
+    uint8_t *pk_is_pocket;
+    int64_t *pk_off, *pk_support;
+    int pk_kept;
     /* boundary grid points of the last orc_project_bgp */
     int64_t nbgp;
     int32_t *ibgp;
@@ -650,6 +655,12 @@ static void free_results(orc_surface *s) {
     s->scspos = s->scsnor = NULL;
     s->nbgp = 0;
     memset(s->bgp_counts, 0, sizeof(s->bgp_counts));
+    free(s->pk_is_pocket);
+    free(s->pk_off);
+    free(s->pk_support);
+    s->pk_is_pocket = NULL;
+    s->pk_off = s->pk_support = NULL;
+    s->pk_kept = 0;
     s->pix_off = NULL;
     s->pix_t = NULL;
     s->pix_patch = NULL;
@@ -1686,9 +1697,11 @@ static void flood_fill(orc_surface *s, int i0, int j0, int k0, int idold, int id
     free(stk);
 }
 
-static int find_cavities(orc_surface *s) {
+/* getCavities(idStart) (Surface.cpp:1877-1991): the first component gets
+ * idStart + 1; idStart = TEMP_OUT makes it the outside (3). */
+static int find_cavities_from(orc_surface *s, int id_start) {
     const int64_t NX = s->NX, NY = s->NY, NZ = s->NZ;
-    int id = NSGPU_STATUS_TEMP_OUT;
+    int id = id_start;
     int times = 0;
     int64_t cap = 64;
     s->cav_count = (int64_t *)xcalloc(cap, sizeof(int64_t));
@@ -1739,6 +1752,8 @@ static int find_cavities(orc_surface *s) {
     if (s->ncav < 0) s->ncav = 0;
     return s->ncav;
 }
+
+static int find_cavities(orc_surface *s) { return find_cavities_from(s, NSGPU_STATUS_TEMP_OUT); }
 
 static void fill_cavities(orc_surface *s, double vol) {
     const int64_t NX = s->NX, NY = s->NY, NZ = s->NZ;
@@ -2591,6 +2606,231 @@ void orc_read_bgp(const orc_surface *s, int32_t *ibgp, double *scspos, double *s
     if (ibgp) memcpy(ibgp, s->ibgp, (size_t)s->nbgp * 3 * sizeof(int32_t));
     if (scspos) memcpy(scspos, s->scspos, (size_t)s->nbgp * 3 * sizeof(double));
     if (scsnor) memcpy(scsnor, s->scsnor, (size_t)s->nbgp * 3 * sizeof(double));
+}
+
+/* ------------------------------------------------------------------------- */
+/* Pocket detection, the grid part of pocketMode (main.cpp:1411-1660): the   */
+/* difference of the fat-probe and small-probe status maps                    */
+/* (Surface::difference, Surface.cpp:11471-11718), the shape filter and       */
+/* cav2out of operator-= (:15556-15616, filterCavities :2751-3010, cav2out    */
+/* :2649-2688), the components (getCavities(STATUS_POINT_OUT), :1877-1991),   */
+/* the volume threshold (fillCavities, :2355-2487), the filter again to mark  */
+/* each component's support points, cavitiesToAtoms (DelphiShared.cpp:       */
+/* 1669-1745) and markPockets (:1530-1584).  Status codes of DelphiShared.h.  */
+/* ------------------------------------------------------------------------- */
+#define ORC_UNDER_CHECK (-3)
+#define ORC_SHAPE_OK (-2)
+
+/* filterCavities(true) on one component: its nodes `list` (flat indices),
+ * probe radius `probe`; returns the nodes removed (set inside) */
+static int64_t orc_filter_cavity(orc_surface *s, const int64_t *list, int64_t n, int cavityId, double probe) {
+    const int64_t NX = s->NX, NY = s->NY, NZ = s->NZ;
+    int32_t *status = s->status;
+    const double dist2ref = probe * probe;
+    for (int64_t q = 0; q < n; q++) status[list[q]] = ORC_UNDER_CHECK;
+    for (int64_t q = 0; q < n; q++) {
+        const int64_t f = list[q];
+        const int64_t v0 = f % NX, v1 = (f / NX) % NY, v2 = f / (NX * NY);
+        double downx = s->x[v0] - probe, downy = s->y[v1] - probe, downz = s->z[v2] - probe;
+        double upx = s->x[v0] + probe, upy = s->y[v1] + probe, upz = s->z[v2] + probe;
+        int64_t ix_start = (int64_t)rintp((downx - s->g.xmin) * s->g.scale);
+        int64_t iy_start = (int64_t)rintp((downy - s->g.ymin) * s->g.scale);
+        int64_t iz_start = (int64_t)rintp((downz - s->g.zmin) * s->g.scale);
+        int64_t ix_end = (int64_t)rintp((upx - s->g.xmin) * s->g.scale);
+        int64_t iy_end = (int64_t)rintp((upy - s->g.ymin) * s->g.scale);
+        int64_t iz_end = (int64_t)rintp((upz - s->g.zmin) * s->g.scale);
+        if (ix_start < 0) ix_start = 0;
+        if (iy_start < 0) iy_start = 0;
+        if (iz_start < 0) iz_start = 0;
+        if (ix_end < 0) ix_end = 0;
+        if (iy_end < 0) iy_end = 0;
+        if (iz_end < 0) iz_end = 0;
+        if (ix_end >= NX) ix_end = NX - 1;
+        if (iy_end >= NY) iy_end = NY - 1;
+        if (iz_end >= NZ) iz_end = NZ - 1;
+        if (ix_start >= NX) ix_start = NX - 1;
+        if (iy_start >= NY) iy_start = NY - 1;
+        if (iz_start >= NZ) iz_start = NZ - 1;
+        int go_out = 0;
+        for (int64_t iz = iz_start; iz <= iz_end && !go_out; iz++)
+            for (int64_t iy = iy_start; iy <= iy_end && !go_out; iy++)
+                for (int64_t ix = ix_start; ix <= ix_end; ix++) {
+                    double a = (s->x[ix] - s->x[v0]);
+                    double b = (s->y[iy] - s->y[v1]);
+                    double c = (s->z[iz] - s->z[v2]);
+                    double dist2 = a * a + b * b + c * c;
+                    if (dist2 > dist2ref) continue;
+                    int val = status[FLAT(ix, iy, iz)];
+                    if (val != ORC_UNDER_CHECK && val != ORC_SHAPE_OK && val != -cavityId) {
+                        go_out = 1;
+                        break;
+                    }
+                }
+        if (!go_out) {
+            for (int64_t iz = iz_start; iz <= iz_end; iz++)
+                for (int64_t iy = iy_start; iy <= iy_end; iy++)
+                    for (int64_t ix = ix_start; ix <= ix_end; ix++) {
+                        double a = (s->x[ix] - s->x[v0]);
+                        double b = (s->y[iy] - s->y[v1]);
+                        double c = (s->z[iz] - s->z[v2]);
+                        double dist2 = a * a + b * b + c * c;
+                        if (dist2 > dist2ref) continue;
+                        if (status[FLAT(ix, iy, iz)] != -cavityId) status[FLAT(ix, iy, iz)] = ORC_SHAPE_OK;
+                    }
+            status[f] = -cavityId;
+        }
+    }
+    int64_t removed = 0;
+    for (int64_t q = 0; q < n; q++) {
+        const int64_t f = list[q];
+        if (status[f] == ORC_UNDER_CHECK) {
+            removed++;
+            status[f] = NSGPU_STATUS_INSIDE;
+        } else if (status[f] == ORC_SHAPE_OK) {
+            status[f] = cavityId;
+        }
+    }
+    return removed;
+}
+
+/* every component's nodes in flat order: CSR (off[c]..off[c+1]) over ids
+ * first_id + c */
+static void orc_component_lists(const orc_surface *s, int first_id, int ncomp, const int32_t *labels, int64_t **off_out,
+                                int64_t **list_out) {
+    int64_t *off = (int64_t *)xcalloc(ncomp + 1, sizeof(int64_t));
+    for (int64_t f = 0; f < s->nvox; f++) {
+        int c = labels[f] - first_id;
+        if (c >= 0 && c < ncomp) off[c + 1]++;
+    }
+    for (int c = 0; c < ncomp; c++) off[c + 1] += off[c];
+    int64_t *list = (int64_t *)xcalloc(off[ncomp] + 1, sizeof(int64_t));
+    int64_t *cur = (int64_t *)xcalloc(ncomp + 1, sizeof(int64_t));
+    for (int64_t f = 0; f < s->nvox; f++) {
+        int c = labels[f] - first_id;
+        if (c >= 0 && c < ncomp) list[off[c] + cur[c]++] = f;
+    }
+    free(cur);
+    *off_out = off;
+    *list_out = list;
+}
+
+/* fat: the fat-probe surface after get_surf (status; cavities filled),
+ * small: the small-probe surface after get_surf with cavity detection (its
+ * outside component labelled STATUS_POINT_OUT).  Rewrites fat's status to the
+ * pocket labels.  Per component c (id 4 + c): volumes[c], filled[c],
+ * is_pocket[c] (for the kept ones); support nodes of the kept components in
+ * flat order, CSR over the kept ones.  counts: [0] components, [1] kept,
+ * [2] support nodes. */
+int orc_pockets(orc_surface *fat, const orc_surface *small, double filter_probe, double num_mol, int cav_and_pockets,
+                int64_t *counts) {
+    if (!fat->status || !small->status || fat->nvox != small->nvox) return -1;
+    orc_surface *s = fat;
+    const int64_t nvox = s->nvox;
+    int32_t *st = s->status;
+    const int32_t *st2 = small->status;
+    /* difference (Surface.cpp:11560-11585): not out in the fat map and out in
+     * the small one.  The small surface's cavities count as out when cavities
+     * and pockets are kept together (its getSurf ran without cavity
+     * detection, main.cpp:1494) */
+    int64_t nd = 0;
+    for (int64_t f = 0; f < nvox; f++) {
+        const int s1 = st[f], s2 = st2[f];
+        const int out2 = s2 == NSGPU_STATUS_TEMP_OUT || s2 == NSGPU_STATUS_OUT ||
+                         (cav_and_pockets && s2 >= NSGPU_STATUS_FIRST_CAV);
+        if (s1 != NSGPU_STATUS_TEMP_OUT && s1 != NSGPU_STATUS_OUT && out2) {
+            st[f] = NSGPU_STATUS_FIRST_CAV;
+            nd++;
+        } else {
+            st[f] = NSGPU_STATUS_INSIDE;
+        }
+    }
+    int64_t *dlist = (int64_t *)xcalloc(nd + 1, sizeof(int64_t));
+    nd = 0;
+    for (int64_t f = 0; f < nvox; f++)
+        if (st[f] == NSGPU_STATUS_FIRST_CAV) dlist[nd++] = f;
+    /* the shape filter on the one difference 'cavity', then cav2out */
+    orc_filter_cavity(s, dlist, nd, NSGPU_STATUS_FIRST_CAV, filter_probe);
+    for (int64_t q = 0; q < nd; q++) {
+        const int v = st[dlist[q]];
+        if (v >= NSGPU_STATUS_FIRST_CAV || v <= -NSGPU_STATUS_FIRST_CAV) st[dlist[q]] = NSGPU_STATUS_TEMP_OUT;
+    }
+    free(dlist);
+    /* components, numbered from 4 in the alternating scan order */
+    free(s->cav_count);
+    s->cav_count = NULL;
+    const int ncomp = find_cavities_from(s, NSGPU_STATUS_OUT);
+    /* volume threshold */
+    const double cubeVol = s->g.side * s->g.side * s->g.side;
+    free(s->cav_size);
+    free(s->cav_filled);
+    s->cav_size = (double *)xcalloc(ncomp + 1, sizeof(double));
+    s->cav_filled = (uint8_t *)xcalloc(ncomp + 1, 1);
+    const double vol = 11.4 * num_mol;
+    for (int c = 0; c < ncomp; c++) {
+        s->cav_size[c] = s->cav_count[c] * cubeVol;
+        s->cav_filled[c] = s->cav_size[c] <= vol;
+    }
+    /* the filter on every kept component: support points marked -id */
+    int64_t *off, *list;
+    orc_component_lists(s, NSGPU_STATUS_FIRST_CAV, ncomp, st, &off, &list);
+    for (int c = 0; c < ncomp; c++) {
+        if (s->cav_filled[c]) continue;
+        const int64_t n = off[c + 1] - off[c];
+        const int64_t removed = orc_filter_cavity(s, list + off[c], n, NSGPU_STATUS_FIRST_CAV + c, filter_probe);
+        if (removed && removed == n) s->cav_filled[c] = 1;
+    }
+    /* pockets: a kept component with a node in the small surface's outside */
+    free(s->pk_is_pocket);
+    free(s->pk_off);
+    free(s->pk_support);
+    s->pk_is_pocket = (uint8_t *)xcalloc(ncomp + 1, 1);
+    s->pk_off = (int64_t *)xcalloc(ncomp + 2, sizeof(int64_t));
+    int nk = 0;
+    int64_t nsup = 0;
+    for (int c = 0; c < ncomp; c++) {
+        if (s->cav_filled[c]) continue;
+        for (int64_t q = off[c]; q < off[c + 1]; q++) {
+            const int v = st2[list[q]];
+            if (v == NSGPU_STATUS_TEMP_OUT || v == NSGPU_STATUS_OUT) {
+                s->pk_is_pocket[c] = 1;
+                break;
+            }
+        }
+        for (int64_t q = off[c]; q < off[c + 1]; q++) nsup += st[list[q]] == -(NSGPU_STATUS_FIRST_CAV + c);
+        nk++;
+    }
+    s->pk_support = (int64_t *)xcalloc(nsup + 1, sizeof(int64_t));
+    int k = 0;
+    nsup = 0;
+    for (int c = 0; c < ncomp; c++) {
+        if (s->cav_filled[c]) continue;
+        s->pk_off[k] = nsup;
+        for (int64_t q = off[c]; q < off[c + 1]; q++)
+            if (st[list[q]] == -(NSGPU_STATUS_FIRST_CAV + c)) s->pk_support[nsup++] = list[q];
+        k++;
+    }
+    s->pk_off[k] = nsup;
+    free(off);
+    free(list);
+    s->ncav = ncomp;
+    s->pk_kept = nk;
+    if (counts) {
+        counts[0] = ncomp;
+        counts[1] = nk;
+        counts[2] = nsup;
+    }
+    return 0;
+}
+
+void orc_read_pockets(const orc_surface *s, double *volumes, uint8_t *filled, uint8_t *is_pocket, int64_t *offsets,
+                      int64_t *support) {
+    for (int c = 0; c < s->ncav; c++) {
+        if (volumes) volumes[c] = s->cav_size[c];
+        if (filled) filled[c] = s->cav_filled[c];
+        if (is_pocket) is_pocket[c] = s->pk_is_pocket[c];
+    }
+    if (offsets) memcpy(offsets, s->pk_off, (size_t)(s->pk_kept + 1) * sizeof(int64_t));
+    if (support) memcpy(support, s->pk_support, (size_t)s->pk_off[s->pk_kept] * sizeof(int64_t));
 }
 
 void orc_free_buf(void *p) { free(p); }
