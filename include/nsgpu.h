@@ -458,6 +458,11 @@ int nsgpu_pockets(nsgpu_ctx *fat, nsgpu_ctx *small, const nsgpu_pocket_opts *opt
  * flat order.  Any pointer may be NULL. */
 int nsgpu_read_pockets(nsgpu_ctx *ctx, double *volumes, uint8_t *filled, uint8_t *is_pocket, int64_t *offsets,
                        int64_t *support);
+/* The components' ids of the last nsgpu_pockets before the second filter
+ * (the status map DelPhiShared::saveCavities and getCavitiesAtoms read,
+ * main.cpp:1590-1610): labels [nx*ny*nz], flat index (k*ny+j)*nx+i, -1 off
+ * the components. */
+int nsgpu_read_pocket_labels(nsgpu_ctx *ctx, int32_t *labels);
 /* Compute_Vertex_Normals: when on, get_surf keeps the patch of every depth so
  * that nsgpu_read_normals can follow the mesh (off by default, as in the
  * reference; it costs the sort a second permuted array). */

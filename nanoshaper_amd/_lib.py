@@ -101,6 +101,7 @@ SIGNATURES = {
     "nsgpu_read_bgp": (ctypes.c_int, [VP, VP, VP, VP]),
     "nsgpu_pockets": (ctypes.c_int, [VP, VP, ctypes.POINTER(PocketOpts), ctypes.POINTER(PocketResult)]),
     "nsgpu_read_pockets": (ctypes.c_int, [VP, VP, VP, VP, VP, VP]),
+    "nsgpu_read_pocket_labels": (ctypes.c_int, [VP, VP]),
     "nsgpu_set_vertex_normals": (ctypes.c_int, [VP, ctypes.c_int]),
     "nsgpu_read_ray_hits": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_int32, VP, VP]),
     "nsgpu_set_mesh": (ctypes.c_int, [VP, VP, ctypes.c_int32, VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),

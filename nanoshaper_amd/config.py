@@ -8,8 +8,15 @@
   main.cpp:1253-1320)
 * ``write_outputs`` the files normal mode saves (main.cpp:1359-1394 and the
   mesh of triangulateSurface / smoothSurface, Surface.cpp:10478-10515)
+* ``pocket_mode`` / ``run_pockets`` / ``write_pocket_outputs`` the CLI's
+  pocket mode (pocketMode, main.cpp:1411-1873): the fat- and small-probe
+  surfaces, the device's pocket detection (nsgpu_pockets) and the skin
+  envelope of each pocket or cavity
 """
+import math
 import os
+
+import numpy as np
 
 from . import io as IO
 from . import surface as S
@@ -93,6 +100,12 @@ class Configuration:
         self.save_ply = r("Save_Mesh_PLY_Format", False)
         self.vertex_atom_info = r("Vertex_Atom_Info", False)
         self.save_bgps = r("Save_bgps", False)
+        # pocket mode (main.cpp:1166, :1187-1194)
+        self.num_mol = r("Num_Wat_Pocket", 2)
+        self.cav_and_pockets = r("Pockets_And_Cavities", True)
+        self.link_pockets = r("Link_Pockets", False)
+        self.pocket_big = r("Pocket_Radius_Big", 3.0)
+        self.pocket_small = r("Pocket_Radius_Small", 1.4)
         self.aux_grid_3d = r("Max_ses_patches_auxiliary_grid_size", 50)
         if not self.build_epsmaps and self.save_epsmaps:
             raise ConfigError("Asked to save epsmap without builiding it: set Build_epsilon_maps = true")
@@ -107,8 +120,10 @@ class Configuration:
         if self.patch_based and self.surface not in ("ses", "skin", "mesh"):
             raise ConfigError("This surface type does not support patch based ray-tracing: "
                               "set Patch_Based_Algorithm = false")
-        if self.operative_mode != "normal":
-            raise ConfigError("operative mode %r is not built (normal only)" % self.operative_mode)
+        if self.operative_mode not in ("normal", "pocket"):
+            raise ConfigError("operative mode %r is not built (normal and pocket only)" % self.operative_mode)
+        if self.operative_mode == "pocket" and self.link_pockets:
+            raise ConfigError("Link_Pockets is not built")
         if self.surface not in ("ses", "skin", "blobby"):
             raise ConfigError("Unknown surface type %r" % self.surface)
 
@@ -199,3 +214,132 @@ def write_outputs(out, directory="."):
         paths.append(root + name + ".idebmap.txt")
         IO.save_idebmap(paths[-1], dev.idebmap())
     return paths
+
+
+# ------------------------------------------------------------------ pocket mode
+def pocket_mode(prm_path, device=0):
+    """Run a configuration file's pocket mode on the device (the atom file
+    relative to the file's directory); see run_pockets."""
+    cf = ConfigFile(prm_path)
+    conf = Configuration(cf)
+    mol = conf.mol_file
+    if not os.path.isabs(mol):
+        mol = os.path.join(os.path.dirname(os.path.abspath(prm_path)), mol)
+    out = run_pockets(conf, S.load_atoms(mol, conf.max_num_atoms, conf.domain_shrinkage), device)
+    out["config"] = conf
+    return out
+
+
+def _envelope(conf, pts, seed, device):
+    """The skin envelope of one pocket or cavity (pocketWrapper::
+    buildPocketCavityEnvelope, main.cpp:398-430): its support points as
+    balls of 1.4 A read back from their "%f" text (cavitiesToAtoms), at least
+    four (the last repeated), grid at perfil 50, jitter 0.1 A continuing the
+    run's random sequence, getSurf with the edge-ray panels only (no maps),
+    the volume and, with Triangulation, the mesh and its area."""
+    a = np.array([[float("%f" % v) for v in p] + [1.4] for p in pts])
+    if len(a) < 4:
+        a = np.vstack([a, np.repeat(a[-1:], 4 - len(a), axis=0)])
+    g = S.build_grid(a, conf.scale, 50.0)
+    jit, seed = S.jitter(a, amplitude=0.1, seed=seed)
+    ps = S.skin_patches(jit, conf.skin_s)
+    dev = S.DeviceSurface(device)
+    dev.set_grid(g)
+    dev.set_patches(ps)
+    r = dev.get_surf(build_epsmap=False, build_status=False, accurate_triangulation=True, fill_cavities=False)
+    mesh = dev.mesh() if conf.tri and dev.triangulate() else None
+    area = dev.mesh_info["area"] if conf.tri else 0.0
+    dev.close()
+    return r["volume"], area, mesh, seed
+
+
+def run_pockets(conf, atoms, device=0, envelopes=True):
+    """pocketMode (main.cpp:1411-1660): the fat-probe surface (probe
+    Pocket_Radius_Big, cavities filled) and the small-probe one (probe
+    Pocket_Radius_Small; its volume, and its area with Triangulation), both
+    jittered from one random sequence as the reference's builds draw from
+    one generator, the device's pocket detection, then the skin envelope of
+    every kept pocket or cavity in turn (the reference runs them on parallel
+    threads that share that generator).  Returns the grid, the two device
+    surfaces, the pockets (nsgpu_pockets), the components' labels before the
+    second filter, and per kept component its envelope volume and area."""
+    grid = S.build_grid(atoms, conf.scale, conf.perfil)
+    jit1, seed = S.jitter(atoms)
+    ps1 = S.ses_patches(jit1, conf.pocket_big)
+    jit2, seed = S.jitter(atoms, seed=seed)
+    ps2 = S.ses_patches(jit2, conf.pocket_small)
+    fat = S.DeviceSurface(device)
+    fat.set_grid(grid)
+    fat.set_patches(ps1)
+    fat.get_surf(build_epsmap=False, build_status=True, accurate_triangulation=False, fill_cavities=True,
+                 cav_vol=math.inf)
+    small = S.DeviceSurface(device)
+    small.set_grid(grid)
+    small.set_patches(ps2)
+    # the small surface as the reference computes it (cavities detected and
+    # filled only when pockets are wanted alone, main.cpp:1494), triangulated
+    # and smoothed with Triangulation
+    r2 = small.get_surf(build_epsmap=False, build_status=True, accurate_triangulation=conf.tri,
+                        fill_cavities=not conf.cav_and_pockets, cav_vol=math.inf)
+    area2 = 0.0
+    if conf.tri:
+        area2 = small.triangulate()["area"]
+        small.smooth_surface()
+    if conf.cav_and_pockets:
+        # its outside component, for markPockets (getCavities, main.cpp:1629-1646)
+        small.get_surf(build_epsmap=False, build_status=True, accurate_triangulation=False, fill_cavities=True,
+                       cav_vol=0.0)
+    pk = fat.pockets(small, filter_probe=1.4, num_mol=conf.num_mol, cav_and_pockets=conf.cav_and_pockets)
+    labels = fat.pocket_labels()
+    x, y, z = IO._grid_axes(grid)
+    nz, ny, nx = labels.shape
+    kept = np.nonzero(~pk["filled"])[0]
+    vols, areas, meshes = [], [], []
+    if envelopes:
+        for q, c in enumerate(kept):
+            f = pk["support"][q]
+            pts = np.stack([x[f % nx], y[(f // nx) % ny], z[f // (nx * ny)]], 1)
+            v, a, m, seed = _envelope(conf, pts, seed, device)
+            vols.append(v)
+            areas.append(a)
+            meshes.append(m)
+    return {"atoms": atoms, "jittered": jit1, "grid": grid, "fat": fat, "small": small, "pockets": pk,
+            "labels": labels, "kept": kept, "small_volume": r2["volume"], "small_area": area2,
+            "envelope_volumes": vols, "envelope_areas": areas, "envelope_meshes": meshes}
+
+
+def write_pocket_outputs(out, directory="."):
+    """The files of a pocket-mode run without atom information
+    (main.cpp:1590-1660, 1752-1800): cavities.txt, cavitiesSize.txt and
+    cavAtomsSerials.txt of the components after the volume threshold (only
+    the kept ones' atoms, DelPhiShared::saveCavities(true)), numcav.txt, and
+    with Triangulation each envelope's mesh as cav_tri<i>.off.  The
+    cav<i>.txt / all_cav<i>.txt inputs of the envelopes are deleted after use
+    by the reference and are not written.  Returns (paths, summary lines)."""
+    conf = out["config"]
+    grid, pk, labels = out["grid"], out["pockets"], out["labels"]
+    root = os.path.join(directory, conf.root_file)
+    n = len(pk["volumes"])
+    by_volume = pk["volumes"] <= 11.4 * conf.num_mol
+    cav_atoms = IO.cavity_atoms(grid, out["jittered"], labels, n)
+    IO.save_cavities(root, grid, labels, pk["volumes"], by_volume, cav_atoms=cav_atoms, only_non_filled=True)
+    paths = [root + "cavities.txt", root + "cavitiesSize.txt", root + "cavAtomsSerials.txt"]
+    nc = len(out["kept"])
+    with open(root + "numcav.txt", "w") as f:
+        f.write("%d" % nc)
+    paths.append(root + "numcav.txt")
+    lines = ["Detected a total of %d pockets/cavities having at least the volume of %d water molecules" %
+             (nc, conf.num_mol)]
+    for q, c in enumerate(out["kept"]):
+        v = out["envelope_volumes"][q] if out["envelope_volumes"] else 0.0
+        kind = "Pocket" if (pk["is_pocket"][c] or not conf.cav_and_pockets) else "Cavity"
+        if conf.tri and out["envelope_meshes"] and out["envelope_meshes"][q] is not None:
+            a = out["envelope_areas"][q]
+            lines.append("%s %d vol %s area %s" % (kind, q, repr(v), repr(a)))
+            mv, mt = out["envelope_meshes"][q]
+            p = root + "cav_tri%d.off" % q
+            IO.save_off(p, mv, mt)
+            paths.append(p)
+        else:
+            lines.append("%s %d vol %s" % (kind, q, repr(v)))
+    return paths, lines

@@ -191,11 +191,12 @@ struct nsgpu_ctx {
     // pockets (nsgpu_pockets, on the fat-probe context): bit planes, union-find,
     // per-component arrays, support lists; host results in id order
     DBuf pkD, pkS1, pkK, pkU, pkS2, pkK2, pkroots, pkparent, pkrcount, pkroff, pkball, pkccount, pkcroot, pkcfilled,
-        pkcid, pkckept, pkcpocket, pkcbkey, pkscount, pksoff, pksupn, pksupc;
+        pkcid, pkckept, pkcpocket, pkcbkey, pkscount, pksoff, pksupn, pksupc, pklab;
     std::vector<double> pk_vol;
     std::vector<uint8_t> pk_filled, pk_pocket;
     std::vector<int64_t> pk_off, pk_sup;
     bool have_pockets = false;
+    PocketArgs pkA{};  // the device arrays of the last nsgpu_pockets
     // the cached tables of bgp_tables and what they were built for
     BgpArgs bgA{};
     uint64_t bg_setgen = 0;
@@ -312,7 +313,7 @@ static void for_each_buf(nsgpu_ctx *c, F fn) {
                    &c->bgcoffa, &c->bgactive, &c->bgfq, &c->pkD, &c->pkS1, &c->pkK, &c->pkU, &c->pkS2, &c->pkK2,
                    &c->pkroots, &c->pkparent, &c->pkrcount, &c->pkroff, &c->pkball, &c->pkccount, &c->pkcroot,
                    &c->pkcfilled, &c->pkcid, &c->pkckept, &c->pkcpocket, &c->pkcbkey, &c->pkscount, &c->pksoff,
-                   &c->pksupn, &c->pksupc};
+                   &c->pksupn, &c->pksupc, &c->pklab};
     for (DBuf *b : all) fn(b);
 }
 
@@ -2767,12 +2768,26 @@ int nsgpu_pockets(nsgpu_ctx *fat, nsgpu_ctx *small, const nsgpu_pocket_opts *o, 
         for (int64_t q = 0; q < nsup; ++q)
             if (kept_rank[supc[q]] >= 0) c->pk_sup[cur[kept_rank[supc[q]]]++] = supn[q];
     }
+    c->pkA = A;
     c->have_pockets = true;
     if (res) {
         res->num_components = (int32_t)ncomp;
         res->num_kept = nk;
         res->support_points = c->pk_off[nk];
     }
+    return NSGPU_OK;
+}
+
+int nsgpu_read_pocket_labels(nsgpu_ctx *c, int32_t *labels) {
+    if (!c || !labels) return NSGPU_E_ARG;
+    if (!c->have_pockets) return fail(c, NSGPU_E_STATE, "no pockets computed");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const int64_t nvox = (int64_t)c->grid.nx * c->grid.ny * c->grid.nz;
+    HIP_TRY(c, c->pklab.ensure((size_t)nvox * 4));
+    launch_pk_comp_labels(c->stream, dev_grid(c), c->pkA, c->pklab.as<int>());
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipMemcpyAsync(labels, c->pklab.p, (size_t)nvox * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
     return NSGPU_OK;
 }
 

@@ -304,5 +304,6 @@ void launch_pk_stats(hipStream_t, const DevGrid &, const PocketArgs &);
 void launch_pk_union_kept(hipStream_t, const DevGrid &, const PocketArgs &);
 void launch_pk_label(hipStream_t, const DevGrid &, const PocketArgs &);
 void launch_pk_support_list(hipStream_t, const DevGrid &, const PocketArgs &);
+void launch_pk_comp_labels(hipStream_t, const DevGrid &, const PocketArgs &, int *out);
 
 }  // namespace nsg

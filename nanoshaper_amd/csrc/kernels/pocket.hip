@@ -270,7 +270,21 @@ __global__ void __launch_bounds__(256) k_pk_support_list(DevGrid g, PocketArgs A
     }
 }
 
+// the components' ids before the second filter (the status map that
+// saveCavities and getCavitiesAtoms read, main.cpp:1590-1610), -1 elsewhere
+__global__ void __launch_bounds__(256) k_pk_comp_labels(DevGrid g, PocketArgs A, int *out) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (int64_t)g.nx * g.ny * g.nz) return;
+    const int r = A.parent[n];
+    out[n] = r >= 0 ? A.cid[root_rank(g, A, r)] : NSGPU_STATUS_INSIDE;
+}
+
 }  // namespace
+
+void launch_pk_comp_labels(hipStream_t s, const DevGrid &g, const PocketArgs &A, int *out) {
+    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
+    hipLaunchKernelGGL(k_pk_comp_labels, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, s, g, A, out);
+}
 
 static unsigned blocks_of(int64_t threads) { return (unsigned)((threads + 255) / 256); }
 

@@ -432,6 +432,13 @@ class DeviceSurface:
         return {"volumes": vol[:nc], "filled": filled[:nc].astype(bool), "is_pocket": pocket[:nc].astype(bool),
                 "support": [sup[off[q]:off[q + 1]] for q in range(nk)]}
 
+    def pocket_labels(self):
+        """(nz, ny, nx) component ids of the last pockets() before the second
+        filter (-1 off the components)."""
+        out = np.empty(self._shape(), np.int32)
+        self._chk(self._lib.nsgpu_read_pocket_labels(self._h, _ptr(out)), "read_pocket_labels")
+        return out
+
     def set_vertex_normals(self, on=True):
         """Compute_Vertex_Normals: keep what normals() needs from the next get_surf on."""
         self._chk(self._lib.nsgpu_set_vertex_normals(self._h, int(on)), "set_vertex_normals")

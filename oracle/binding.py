@@ -67,6 +67,7 @@ def load():
         "orc_read_bgp": (None, [VP, VP, VP, VP]),
         "orc_pockets": (ctypes.c_int, [VP, VP, ctypes.c_double, ctypes.c_double, ctypes.c_int, VP]),
         "orc_read_pockets": (None, [VP, VP, VP, VP, VP, VP]),
+        "orc_read_pocket_labels": (None, [VP, VP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -256,6 +257,11 @@ class OracleSurface:
         self._lib.orc_read_pockets(self._h, _ptr(vol), _ptr(filled), _ptr(pocket), _ptr(off), _ptr(sup))
         return {"volumes": vol[:nc], "filled": filled[:nc].astype(bool), "is_pocket": pocket[:nc].astype(bool),
                 "support": [sup[off[q]:off[q + 1]] for q in range(nk)]}
+
+    def pocket_labels(self):
+        out = np.empty(self._shape(), np.int32)
+        self._lib.orc_read_pocket_labels(self._h, _ptr(out))
+        return out
 
     def ray_hits(self, slot):
         g = self.grid

@@ -546,6 +546,7 @@ typedef struct orc_surface {
     uint8_t *pk_is_pocket;
     int64_t *pk_off, *pk_support;
     int pk_kept;
+    int32_t *pk_labels; /* component ids before the second filter */
     /* boundary grid points of the last orc_project_bgp */
     int64_t nbgp;
     int32_t *ibgp;
@@ -658,8 +659,10 @@ static void free_results(orc_surface *s) {
     free(s->pk_is_pocket);
     free(s->pk_off);
     free(s->pk_support);
+    free(s->pk_labels);
     s->pk_is_pocket = NULL;
     s->pk_off = s->pk_support = NULL;
+    s->pk_labels = NULL;
     s->pk_kept = 0;
     s->pix_off = NULL;
     s->pix_t = NULL;
@@ -2770,6 +2773,9 @@ int orc_pockets(orc_surface *fat, const orc_surface *small, double filter_probe,
         s->cav_size[c] = s->cav_count[c] * cubeVol;
         s->cav_filled[c] = s->cav_size[c] <= vol;
     }
+    free(s->pk_labels);
+    s->pk_labels = (int32_t *)malloc(nvox * sizeof(int32_t));
+    memcpy(s->pk_labels, st, nvox * sizeof(int32_t));
     /* the filter on every kept component: support points marked -id */
     int64_t *off, *list;
     orc_component_lists(s, NSGPU_STATUS_FIRST_CAV, ncomp, st, &off, &list);
@@ -2820,6 +2826,10 @@ int orc_pockets(orc_surface *fat, const orc_surface *small, double filter_probe,
         counts[2] = nsup;
     }
     return 0;
+}
+
+void orc_read_pocket_labels(const orc_surface *s, int32_t *labels) {
+    memcpy(labels, s->pk_labels, s->nvox * sizeof(int32_t));
 }
 
 void orc_read_pockets(const orc_surface *s, double *volumes, uint8_t *filled, uint8_t *is_pocket, int64_t *offsets,

@@ -35,6 +35,7 @@ def _run(g, ps_fat, ps_small, cap, num_mol=2):
     a = df.pockets(ds, num_mol=num_mol, cav_and_pockets=cap)
     b = of.pockets(os_, num_mol=num_mol, cav_and_pockets=cap)
     assert np.array_equal(df.status(), of.status())
+    assert np.array_equal(df.pocket_labels(), of.pocket_labels())
     assert np.array_equal(a["volumes"], b["volumes"])
     assert np.array_equal(a["filled"], b["filled"])
     kept = ~b["filled"]
@@ -75,3 +76,46 @@ def test_pockets_refusals():
         df.pockets(ds)  # no status maps yet
     df.close()
     ds.close()
+
+
+def test_pocket_mode_envelopes(tmp_path):
+    """Pocket mode end to end (config.run_pockets): the device pockets, the
+    files, and each envelope's volume and area against the oracle's skin
+    surface of the same jittered support balls."""
+    from nanoshaper_amd import config
+    from oracle.binding import OracleSurface
+    conf = config.Configuration(config.ConfigFile(text="Operative_Mode = pocket\nBuild_status_map = true\n"
+                                                       "Triangulation = true\nAccurate_Triangulation = true\n"
+                                                       "Grid_scale = 2.0\nGrid_perfil = 80\n"))
+    xyzr, _ = ns.generate_chain(300, 11.0, seed=3)
+    print("pocket mode 300 atoms", flush=True)
+    run = config.run_pockets(conf, xyzr)
+    run["config"] = conf
+    pk, kept = run["pockets"], run["kept"]
+    assert len(kept) >= 3 and pk["is_pocket"][kept].any() and not pk["is_pocket"][kept].all()
+    paths, lines = config.write_pocket_outputs(run, str(tmp_path))
+    assert open(tmp_path / "numcav.txt").read() == str(len(kept))
+    assert len(lines) == len(kept) + 1 and sum(l.startswith("Pocket") for l in lines[1:]) == int(pk["is_pocket"][kept].sum())
+    rows = open(tmp_path / "cavitiesSize.txt").read().splitlines()
+    assert len(rows) == len(pk["volumes"]) + 1
+    # the envelopes, rebuilt on the host with the oracle: same random sequence
+    _, seed = ns.jitter(xyzr)
+    _, seed = ns.jitter(xyzr, seed=seed)
+    g = run["grid"]
+    x, y, z = ns.grid_coords(g)
+    for q, c in enumerate(kept):
+        f = pk["support"][q]
+        pts = np.stack([x[f % g.nx], y[(f // g.nx) % g.ny], z[f // (g.nx * g.ny)]], 1)
+        a = np.array([[float("%f" % v) for v in p] + [1.4] for p in pts])
+        if len(a) < 4:
+            a = np.vstack([a, np.repeat(a[-1:], 4 - len(a), axis=0)])
+        ge = ns.build_grid(a, conf.scale, 50.0)
+        jit, seed = ns.jitter(a, amplitude=0.1, seed=seed)
+        orc = OracleSurface(ge, ns.skin_patches(jit, conf.skin_s))
+        r = orc.get_surf(build_epsmap=False, build_status=False, accurate_triangulation=True, fill_cavities=False)
+        m = orc.triangulate()
+        assert run["envelope_volumes"][q] == pytest.approx(r["volume"], rel=1e-12)
+        assert run["envelope_areas"][q] == pytest.approx(m["area"], rel=1e-12)
+        orc.close()
+    run["fat"].close()
+    run["small"].close()

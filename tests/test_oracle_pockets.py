@@ -81,6 +81,7 @@ def test_pockets_match_morphology(n, radius, scale, seed, cap):
     lab, ids = _replay_ids(K)
     r = fat.pockets(small, cav_and_pockets=cap)
     st = fat.status()
+    lab0 = fat.pocket_labels()
     assert len(ids) == len(r["volumes"]) > 0
     counts = np.bincount(lab.ravel(), minlength=len(ids) + 1)[1:]
     order = np.argsort(ids)
@@ -90,6 +91,7 @@ def test_pockets_match_morphology(n, radius, scale, seed, cap):
     id_of = np.zeros(len(ids) + 1, int)
     id_of[1:] = ids
     L = id_of[lab]                      # component id per node (0 outside)
+    assert np.array_equal(lab0, np.where(L > 0, L, -1))
     U = (L > 0) & ~filled[np.maximum(L - 4, 0)]
     sup, K2 = _opening(U, ball)
     exp = np.full(st.shape, -1, np.int32)
