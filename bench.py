@@ -22,7 +22,9 @@ of the plain path are measured alongside:
 * "skin": the skin surface (s = 0.45) of the same atoms, its mixed cells
   built on the host and cast on the device, with the oracle's time;
 * "bgp": the boundary grid points of the benchmark surface projected on the
-  SES (Project_boundary_grid_points), with the oracle's time.
+  SES (Project_boundary_grid_points), with the oracle's time;
+* "pockets": pocket detection (the grid part of pocket mode) on the same
+  atoms at 2 points/A, with the oracle's time.
 """
 import argparse
 import json
@@ -467,6 +469,60 @@ def bench_bgp(dev, grid, ps, steps=10, cpu=True):
     return out
 
 
+def bench_pockets(steps=10, cpu=True, n_atoms=5000, scale=2.0):
+    """Pocket detection (the grid part of pocket mode) on the 5,000-atom
+    chain at 2 points/A: the fat-probe (3.0 A) and small-probe (1.4 A)
+    status maps computed once, then nsgpu_pockets timed per call (difference,
+    two probe-ball filters, components, labels, support lists), beside the
+    oracle's restatement of the reference's loops on the same maps."""
+    import math
+    xyzr, _ = ns.generate_chain(n_atoms, CONFIGS["c2"][1], seed=0)
+    grid = ns.build_grid(xyzr, scale, 80.0)
+    j1, st = ns.jitter(xyzr)
+    j2, _ = ns.jitter(xyzr, seed=st)
+    p1, p2 = ns.ses_patches(j1, 3.0), ns.ses_patches(j2, 1.4)
+    fat_o = dict(build_epsmap=False, build_status=True, accurate_triangulation=False, fill_cavities=True,
+                 cav_vol=math.inf)
+    small_o = dict(build_epsmap=False, build_status=True, accurate_triangulation=False, fill_cavities=True,
+                   cav_vol=0.0)
+    fat, small = ns.DeviceSurface(0), ns.DeviceSurface(0)
+    for d, ps, o in ((fat, p1, fat_o), (small, p2, small_o)):
+        d.set_grid(grid)
+        d.set_patches(ps)
+    small.get_surf(**small_o)
+    ms = []
+    for it in range(steps + 1):
+        fat.get_surf(**fat_o)  # the pocket labels replace its status map
+        fat.synchronize()
+        t0 = time.perf_counter()
+        pk = fat.pockets(small)
+        ms.append((time.perf_counter() - t0) * 1e3)
+    ms = float(np.median(ms[1:]))
+    nvox = grid.nx * grid.ny * grid.nz
+    out = {"workload": "pocket detection: %d-atom chain, probes 3.0 / 1.4 A, 2.0 points/A perfil 80 -> %d^3"
+                       % (n_atoms, grid.nx),
+           "ms_per_call": ms, "voxels_per_s": nvox / (ms * 1e-3), "components": len(pk["volumes"]),
+           "kept": len(pk["support"]), "pockets": int(pk["is_pocket"][~pk["filled"]].sum()),
+           "support_points": int(sum(len(x) for x in pk["support"]))}
+    if cpu:
+        from oracle.binding import OracleSurface
+        of, os_ = OracleSurface(grid, p1), OracleSurface(grid, p2)
+        of.get_surf(**fat_o)
+        os_.get_surf(**small_o)
+        t0 = time.perf_counter()
+        po = of.pockets(os_)
+        cs = time.perf_counter() - t0
+        out["cpu_baseline"] = {"seconds_per_call": cs, "cores": 1, "kind": "port",
+                               "sample": "full workload, 1 run (the oracle's filter loops are single threaded)",
+                               "components_equal": bool(len(po["volumes"]) == len(pk["volumes"]))}
+        out["speedup_vs_cpu"] = cs / (ms * 1e-3)
+        of.close()
+        os_.close()
+    fat.close()
+    small.close()
+    return out
+
+
 def copy_reference(nbytes, repeats=5):
     """Plain device-to-device copy of known size: the calibration kernel for
     the FETCH_SIZE / WRITE_SIZE counters (a copy reads and writes nbytes)."""
@@ -499,6 +555,7 @@ def main():
     ap.add_argument("--blobby-only", action="store_true", help="only the blobby line (profiling)")
     ap.add_argument("--skin-only", action="store_true", help="only the skin line (profiling)")
     ap.add_argument("--bgp-only", action="store_true", help="only the boundary-grid-point line (profiling)")
+    ap.add_argument("--pockets-only", action="store_true", help="only the pocket-detection line (profiling)")
     args = ap.parse_args()
     if args.copy_ref:
         copy_reference(args.copy_ref)
@@ -508,6 +565,9 @@ def main():
         return
     if args.skin_only:
         print(json.dumps({"skin": bench_skin(args.steps, cpu=not args.no_cpu)}))
+        return
+    if args.pockets_only:
+        print(json.dumps({"pockets": bench_pockets(args.steps, cpu=not args.no_cpu)}))
         return
     if args.bgp_only:
         xyzr, r_used, grid, ps, host_s = build_inputs(args.config)
@@ -611,6 +671,8 @@ def main():
         extras["blobby"] = bench_blobby(cpu=not args.no_cpu)
         print("skin ...", file=sys.stderr, flush=True)
         extras["skin"] = bench_skin(cpu=not args.no_cpu)
+        print("pockets ...", file=sys.stderr, flush=True)
+        extras["pockets"] = bench_pockets(cpu=not args.no_cpu)
 
     cpu = None
     if not args.no_cpu:

@@ -123,25 +123,55 @@ __global__ void __launch_bounds__(256) k_pk_morph(DevGrid g, PocketArgs A, const
     out[wq] = acc;
 }
 
-// union-find over the set nodes of `m` (parent[n] = n for a set node)
+// union-find over the set nodes of `m`, one wave per word.  A node's parent
+// starts as the first node of its x-run inside the word (no atomics); runs
+// continuing from the previous word are joined at bit 0, and each maximal
+// stretch where a row and the row below (or the plane below) are both set is
+// joined once, at its first node.
 __global__ void __launch_bounds__(256) k_pk_uf_init(DevGrid g, PocketArgs A, const unsigned long long *m) {
-    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= (int64_t)g.nx * g.ny * g.nz) return;
-    const int64_t row = n / g.nx;
-    const int x = (int)(n - row * g.nx);
-    A.parent[n] = bit_at(m, A.W, row, x) ? (int)n : -1;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wave >= A.nwords) return;
+    const int64_t row = wave / A.W;
+    const int x0 = (int)(wave - row * A.W) * 64, x = x0 + lane;
+    if (x >= g.nx) return;
+    const unsigned long long w = m[wave];
+    const int64_t n = row * g.nx + x;
+    if (!((w >> lane) & 1ull)) {
+        A.parent[n] = -1;
+        return;
+    }
+    // the highest clear bit below this one, + 1
+    const unsigned long long below_clear = ~w & ((lane ? (~0ull >> (64 - lane)) : 0ull));
+    const int start = below_clear ? 64 - __clzll((long long)below_clear) : 0;
+    A.parent[n] = (int)(row * g.nx + x0 + start);
 }
 
 __global__ void __launch_bounds__(256) k_pk_uf_union(DevGrid g, PocketArgs A, const unsigned long long *m) {
-    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= (int64_t)g.nx * g.ny * g.nz) return;
-    const int64_t row = n / g.nx;
-    const int x = (int)(n - row * g.nx);
-    if (!bit_at(m, A.W, row, x)) return;
+    const int64_t wq = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (wq >= A.nwords) return;
+    const int64_t row = wq / A.W;
+    const int wi = (int)(wq - row * A.W);
     const int j = (int)(row % g.ny), k = (int)(row / g.ny);
-    if (x > 0 && bit_at(m, A.W, row, x - 1)) pk_unite(A.parent, (int)n, (int)n - 1);
-    if (j > 0 && bit_at(m, A.W, row - 1, x)) pk_unite(A.parent, (int)n, (int)(n - g.nx));
-    if (k > 0 && bit_at(m, A.W, row - g.ny, x)) pk_unite(A.parent, (int)n, (int)(n - (int64_t)g.nx * g.ny));
+    const unsigned long long w = m[wq];
+    if (!w) return;
+    const int64_t base = row * g.nx + wi * 64;
+    // x: a run that continues from the previous word
+    if (wi > 0 && (w & 1ull) && (m[wq - 1] >> 63)) pk_unite(A.parent, (int)base, (int)base - 1);
+    // y and z: the first node of each stretch set in both rows
+    for (int d = 0; d < 2; ++d) {
+        if (d == 0 ? j == 0 : k == 0) continue;
+        const int64_t oq = d == 0 ? wq - A.W : wq - (int64_t)g.ny * A.W;
+        const int64_t step = d == 0 ? g.nx : (int64_t)g.nx * g.ny;
+        const unsigned long long both = w & m[oq];
+        const unsigned long long prev = wi > 0 ? ((m[wq - 1] & m[oq - 1]) >> 63) : 0ull;
+        unsigned long long starts = both & ~((both << 1) | prev);
+        while (starts) {
+            const int b = __ffsll((long long)starts) - 1;
+            starts &= starts - 1;
+            pk_unite(A.parent, (int)(base + b), (int)(base + b - step));
+        }
+    }
 }
 
 // flatten; roots as a bit plane (their rank = the component's index)
@@ -182,20 +212,80 @@ __device__ __forceinline__ int root_rank(const DevGrid &g, const PocketArgs &A, 
     return A.roff[wq] + __popcll(b ? (A.roots[wq] & (~0ull >> (64 - b))) : 0ull);
 }
 
+// Lanes of one wave holding nodes of one component combine their updates:
+// the first pending lane leads the lanes with its component (whose mask is
+// returned) and the pending set shrinks by them.  A wave's nodes lie in one
+// grid row, usually in one or two components.
+__device__ __forceinline__ unsigned long long next_group(unsigned long long &pending, int c, int &lead, int &cl) {
+    lead = __ffsll((long long)pending) - 1;
+    cl = __shfl(c, lead, 64);
+    const unsigned long long m = __ballot(c == cl) & pending;
+    pending &= ~m;
+    return m;
+}
+
+// Per-component tallies of a wave's run of kWordsPerWave consecutive words,
+// held by lane 0 while the groups keep naming the same component (they mostly
+// do: consecutive words are consecutive nodes of a row) and added to the
+// component's entries when it changes.
+constexpr int kWordsPerWave = 16;
+struct Tally {
+    int c = -1, n = 0;
+    long long key = 0x7fffffffffffffffll;
+    int flag = 0;
+    __device__ void flush(int *count, long long *keymin, int *flagv) {
+        if (c < 0) return;
+        if (count && n) atomicAdd(count + c, n);
+        if (keymin) atomicMin(keymin + c, key);
+        if (flagv && flag) flagv[c] = 1;
+    }
+    __device__ void add(int cl, int cnt, long long k, int f, int *count, long long *keymin, int *flagv) {
+        if (cl != c) {
+            flush(count, keymin, flagv);
+            c = cl;
+            n = 0;
+            key = 0x7fffffffffffffffll;
+            flag = 0;
+        }
+        n += cnt;
+        key = min(key, k);
+        flag |= f;
+    }
+};
+
 // per component: node count and the backward-scan key (k descending, then j
-// and i ascending: ((nz - 1 - k) ny + j) nx + i); the forward key is the root
+// and i ascending: ((nz - 1 - k) ny + j) nx + i; along a row it grows with i,
+// so a group's smallest is its leader's); the forward key is the root
 __global__ void __launch_bounds__(256) k_pk_stats(DevGrid g, PocketArgs A) {
-    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= (int64_t)g.nx * g.ny * g.nz) return;
-    const int r = A.parent[n];
-    if (r < 0) return;
-    const int c = root_rank(g, A, r);
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    const int64_t q0 = wave * kWordsPerWave;
+    if (q0 >= A.nwords) return;
     const int64_t plane = (int64_t)g.nx * g.ny;
-    const int64_t k = n / plane;
-    const long long bkey = (long long)(g.nz - 1 - k) * plane + (n - k * plane);
-    atomicAdd(A.ccount + c, 1);
-    atomicMin(A.cbkey + c, bkey);
-    if (r == (int)n) A.croot[c] = r;
+    Tally t;
+    for (int64_t wq = q0; wq < min(q0 + kWordsPerWave, A.nwords); ++wq) {
+        const int64_t row = wq / A.W;
+        const int x = (int)(wq - row * A.W) * 64 + lane;
+        const int64_t n = row * g.nx + x;
+        int c = -1;
+        if (x < g.nx) {
+            const int r = A.parent[n];
+            if (r >= 0) {
+                c = root_rank(g, A, r);
+                if (r == (int)n) A.croot[c] = r;
+            }
+        }
+        unsigned long long pending = __ballot(c >= 0);
+        const int64_t k = row / g.ny;
+        const long long bkey = (long long)(g.nz - 1 - k) * plane + (n - k * plane);
+        while (pending) {
+            int lead, cl;
+            const unsigned long long m = next_group(pending, c, lead, cl);
+            const long long lk = __shfl(bkey, lead, 64);
+            if (lane == 0) t.add(cl, __popcll(m), lk, 0, A.ccount, A.cbkey, nullptr);
+        }
+    }
+    if (lane == 0) t.flush(A.ccount, A.cbkey, nullptr);
 }
 
 // the kept components' nodes (components not filled by the volume threshold)
@@ -220,35 +310,46 @@ __global__ void __launch_bounds__(256) k_pk_union_kept(DevGrid g, PocketArgs A) 
 __global__ void __launch_bounds__(256) k_pk_label(DevGrid g, PocketArgs A) {
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
-    if (wave >= A.nwords) return;
-    const int64_t row = wave / A.W;
-    const int x = (int)(wave - row * A.W) * 64 + lane;
-    bool sup = false;
-    if (x < g.nx) {
-        const int64_t n = row * g.nx + x;
-        const int r = A.parent[n];
-        int st = NSGPU_STATUS_INSIDE;
-        if (r >= 0) {
-            const int c = root_rank(g, A, r);
-            const int id = A.cid[c];
-            const unsigned long long bit = 1ull << (x & 63);
-            const int64_t wq = row * A.W + (x >> 6);
-            if (A.cfilled[c]) {
-                st = id;
-            } else if (A.S2[wq] & bit) {
-                st = -id;
-                sup = true;
-                atomicAdd(A.ckept + c, 1);
-            } else if (A.K2[wq] & bit) {
-                st = id;
-                atomicAdd(A.ckept + c, 1);
+    const int64_t q0 = wave * kWordsPerWave;
+    if (q0 >= A.nwords) return;
+    Tally t;
+    for (int64_t wq = q0; wq < min(q0 + kWordsPerWave, A.nwords); ++wq) {
+        const int64_t row = wq / A.W;
+        const int x = (int)(wq - row * A.W) * 64 + lane;
+        bool sup = false, kept = false, out2 = false;
+        int c = -1;
+        if (x < g.nx) {
+            const int64_t n = row * g.nx + x;
+            const int r = A.parent[n];
+            int st = NSGPU_STATUS_INSIDE;
+            if (r >= 0) {
+                c = root_rank(g, A, r);
+                const int id = A.cid[c];
+                const unsigned long long bit = 1ull << (x & 63);
+                if (A.cfilled[c]) {
+                    st = id;
+                } else if (A.S2[wq] & bit) {
+                    st = -id;
+                    sup = kept = true;
+                } else if (A.K2[wq] & bit) {
+                    st = id;
+                    kept = true;
+                }
+                out2 = A.st2[n] == NSGPU_STATUS_TEMP_OUT || A.st2[n] == NSGPU_STATUS_OUT;
             }
-            if (A.st2[n] == NSGPU_STATUS_TEMP_OUT || A.st2[n] == NSGPU_STATUS_OUT) A.cpocket[c] = 1;
+            A.st1[n] = st;
         }
-        A.st1[n] = st;
+        unsigned long long pending = __ballot(c >= 0);
+        const unsigned long long km = __ballot(kept), om = __ballot(out2);
+        while (pending) {
+            int lead, cl;
+            const unsigned long long m = next_group(pending, c, lead, cl);
+            if (lane == 0) t.add(cl, __popcll(m & km), 0, (m & om) != 0, A.ckept, nullptr, A.cpocket);
+        }
+        const unsigned long long sm = __ballot(sup);
+        if (lane == 0) A.scount[wq] = __popcll(sm);
     }
-    const unsigned long long m = __ballot(sup);
-    if (lane == 0) A.scount[wave] = __popcll(m);
+    if (lane == 0) t.flush(A.ckept, nullptr, A.cpocket);
 }
 
 // support nodes in flat order with their component index
@@ -298,15 +399,14 @@ void launch_pk_morph(hipStream_t s, const DevGrid &g, const PocketArgs &A, const
 }
 
 void launch_pk_components(hipStream_t s, const DevGrid &g, const PocketArgs &A, const unsigned long long *m) {
-    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
-    hipLaunchKernelGGL(k_pk_uf_init, dim3(blocks_of(nvox)), dim3(256), 0, s, g, A, m);
-    hipLaunchKernelGGL(k_pk_uf_union, dim3(blocks_of(nvox)), dim3(256), 0, s, g, A, m);
+    hipLaunchKernelGGL(k_pk_uf_init, dim3(blocks_of(A.nwords * 64)), dim3(256), 0, s, g, A, m);
+    hipLaunchKernelGGL(k_pk_uf_union, dim3(blocks_of(A.nwords)), dim3(256), 0, s, g, A, m);
     hipLaunchKernelGGL(k_pk_uf_roots, dim3(blocks_of(A.nwords * 64)), dim3(256), 0, s, g, A);
 }
 
 void launch_pk_stats(hipStream_t s, const DevGrid &g, const PocketArgs &A) {
-    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
-    hipLaunchKernelGGL(k_pk_stats, dim3(blocks_of(nvox)), dim3(256), 0, s, g, A);
+    const int64_t waves = (A.nwords + kWordsPerWave - 1) / kWordsPerWave;
+    hipLaunchKernelGGL(k_pk_stats, dim3(blocks_of(waves * 64)), dim3(256), 0, s, g, A);
 }
 
 void launch_pk_union_kept(hipStream_t s, const DevGrid &g, const PocketArgs &A) {
@@ -314,7 +414,8 @@ void launch_pk_union_kept(hipStream_t s, const DevGrid &g, const PocketArgs &A) 
 }
 
 void launch_pk_label(hipStream_t s, const DevGrid &g, const PocketArgs &A) {
-    hipLaunchKernelGGL(k_pk_label, dim3(blocks_of(A.nwords * 64)), dim3(256), 0, s, g, A);
+    const int64_t waves = (A.nwords + kWordsPerWave - 1) / kWordsPerWave;
+    hipLaunchKernelGGL(k_pk_label, dim3(blocks_of(waves * 64)), dim3(256), 0, s, g, A);
 }
 
 void launch_pk_support_list(hipStream_t s, const DevGrid &g, const PocketArgs &A) {
