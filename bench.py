@@ -219,6 +219,7 @@ def bench_batched(n_frames=256, repeats=3):
         t = time.perf_counter() - t
         if t < best:
             best, lat_best = t, lat
+    shape = b.shape()
     loop_best, loop_lat = 1e30, None
     for _ in range(repeats):
         t = time.perf_counter()
@@ -226,6 +227,16 @@ def bench_batched(n_frames=256, repeats=3):
         t = time.perf_counter() - t
         if t < loop_best:
             loop_best, loop_lat = t, lat
+    # each cast form at 64 chains, for comparison with the autotuned shape
+    by_cast = {}
+    for cast in ("frames", "joint", "persistent"):
+        b.prepare(chains=64, cast=cast)
+        t_best = 1e30
+        for _ in range(repeats):
+            t = time.perf_counter()
+            b.run()
+            t_best = min(t_best, time.perf_counter() - t)
+        by_cast[cast] = {"ms_per_batch": t_best * 1e3, "workgroups": b.shape()["workgroups"]}
     nvox = sum(g.nx * g.ny * g.nz for g, _ in inputs)
     b.close()
     return {"workload": "%d frames, %d-%d atoms (log-uniform), sigma 0.3 A, probe 1.4 A, 2.0 points/A, "
@@ -240,7 +251,8 @@ def bench_batched(n_frames=256, repeats=3):
             "loop_p99_ms": float(np.percentile(np.cumsum(loop_lat), 99)),
             "loop_call_p50_ms": float(np.percentile(loop_lat, 50)),
             "loop_call_p99_ms": float(np.percentile(loop_lat, 99)), "speedup_vs_loop": loop_best / best,
-            "chains": chains, "frames_redone": redone, "voxels": nvox, "host_build_s": host_s}
+            "chains": chains, "shape": shape, "ms_by_cast_at_64_chains": by_cast, "frames_redone": redone,
+            "voxels": nvox, "host_build_s": host_s}
 
 
 def bench_differentiable(steps=10):
@@ -556,6 +568,7 @@ def main():
     ap.add_argument("--skin-only", action="store_true", help="only the skin line (profiling)")
     ap.add_argument("--bgp-only", action="store_true", help="only the boundary-grid-point line (profiling)")
     ap.add_argument("--pockets-only", action="store_true", help="only the pocket-detection line (profiling)")
+    ap.add_argument("--batched-only", action="store_true", help="only the batched-frames line (profiling)")
     args = ap.parse_args()
     if args.copy_ref:
         copy_reference(args.copy_ref)
@@ -568,6 +581,9 @@ def main():
         return
     if args.pockets_only:
         print(json.dumps({"pockets": bench_pockets(args.steps, cpu=not args.no_cpu)}))
+        return
+    if args.batched_only:
+        print(json.dumps({"batched": bench_batched()}))
         return
     if args.bgp_only:
         xyzr, r_used, grid, ps, host_s = build_inputs(args.config)

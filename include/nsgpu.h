@@ -490,11 +490,41 @@ int nsgpu_batch_set_frame(nsgpu_batch *batch, int32_t frame, const nsgpu_grid *g
                           const nsgpu_patches *patches, double probe_radius);
 /* Runs every frame once eagerly (sizing its buffers and ray lists), then
  * captures all frames' getSurf work into one graph: frames in decreasing
- * grid size, dealt round-robin onto `chains` concurrent chains.  chains <= 0
- * picks the chain count by timing launches of 1, 2, 4, ... 64 chains.
+ * grid size, dealt round-robin onto `chains` concurrent chains, each frame's
+ * intersection its own launches.  chains <= 0 autotunes the launch shape
+ * (nsgpu_batch_prepare_shape).
  * Maps and cavities only: opts->accurate_triangulation must be 0. */
 int nsgpu_batch_prepare(nsgpu_batch *batch, const nsgpu_surf_opts *opts, int32_t chains);
 int32_t nsgpu_batch_chains(const nsgpu_batch *batch);
+/* Launch shape of a batch graph.  cast:
+ *   NSGPU_BATCH_CAST_FRAMES      each frame's intersection is its own launches
+ *                                inside its chain;
+ *   NSGPU_BATCH_CAST_JOINT       one launch per patch class intersects every
+ *                                frame's patches (a workgroup per patch item),
+ *                                between the chains' clears and their
+ *                                sort/fill/cavity work;
+ *   NSGPU_BATCH_CAST_PERSISTENT  as JOINT, with `workgroups` persistent
+ *                                workgroups per class taking `chunk` items
+ *                                at a time from a device queue (0: the
+ *                                resident limit; 16 items).
+ * The joint casts need SES frames traced patch by patch (no skin cells, mesh
+ * surface or cell-grid cast).  chains: concurrent chains of per-frame work. */
+enum { NSGPU_BATCH_CAST_FRAMES = 0, NSGPU_BATCH_CAST_JOINT = 1, NSGPU_BATCH_CAST_PERSISTENT = 2 };
+typedef struct nsgpu_batch_shape {
+    int32_t cast;
+    int32_t chains;
+    int32_t workgroups;
+    int32_t chunk;
+} nsgpu_batch_shape;
+/* nsgpu_batch_prepare with a given shape, or (shape == NULL) the autotuned
+ * one: every candidate (per-frame casts on 1, 2, 4, ... 64 chains; joint and
+ * persistent casts on 16, 32, 64 chains, persistent at the resident limit
+ * with chunks of 16 and 4 items and at half the limit with 16) is captured
+ * and launched twice; the fastest is kept.
+ * nsgpu_batch_prepare(batch, opts, 0) autotunes the same way. */
+int nsgpu_batch_prepare_shape(nsgpu_batch *batch, const nsgpu_surf_opts *opts, const nsgpu_batch_shape *shape);
+/* The shape of the captured graph (workgroups and chunk as launched). */
+int nsgpu_batch_get_shape(const nsgpu_batch *batch, nsgpu_batch_shape *shape);
 /* One launch of the graph.  latency_ms[n_frames] (may be NULL) receives each
  * frame's completion time after the launch (device clock); *redone (may be
  * NULL) the frames that were finished on the host path (ray lists outgrown,

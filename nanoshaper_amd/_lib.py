@@ -59,6 +59,11 @@ class PocketResult(ctypes.Structure):
                 ("support_points", ctypes.c_int64)]
 
 
+class BatchShape(ctypes.Structure):
+    _fields_ = [("cast", ctypes.c_int32), ("chains", ctypes.c_int32), ("workgroups", ctypes.c_int32),
+                ("chunk", ctypes.c_int32)]
+
+
 SIGNATURES = {
     "nsgpu_abi_version": (ctypes.c_int, []),
     "nsgpu_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(VP)]),
@@ -121,6 +126,8 @@ SIGNATURES = {
                                              ctypes.c_double]),
     "nsgpu_batch_prepare": (ctypes.c_int, [VP, ctypes.POINTER(SurfOpts), ctypes.c_int32]),
     "nsgpu_batch_chains": (ctypes.c_int32, [VP]),
+    "nsgpu_batch_prepare_shape": (ctypes.c_int, [VP, ctypes.POINTER(SurfOpts), ctypes.POINTER(BatchShape)]),
+    "nsgpu_batch_get_shape": (ctypes.c_int, [VP, ctypes.POINTER(BatchShape)]),
     "nsgpu_batch_run": (ctypes.c_int, [VP, VP, VP]),
     "nsgpu_batch_run_loop": (ctypes.c_int, [VP, VP]),
     "nsgpu_batch_result": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.POINTER(SurfResult)]),

@@ -608,11 +608,24 @@ static int ensure_cells(nsgpu_ctx *c) {
     return NSGPU_OK;
 }
 
-static int enqueue_trace(nsgpu_ctx *c, hipStream_t st) {
+// Rows of the per-workgroup candidate partials of the intersection.
+static int64_t intersect_rows(const nsgpu_ctx *c) {
+    const DevPatches P = dev_patches(c);
+    const DevRays R = dev_rays(c);
+    const bool tiles = c->cast_mode == NSGPU_CAST_TILES && !c->surface_is_mesh;
+    return std::max<int64_t>(c->surface_is_mesh ? mesh_workgroups(c->mnt, R)
+                             : tiles             ? tile_workgroups(R, c->cg_nc)
+                                                 : intersect_workgroups(P, R),
+                             1);
+}
+
+// The trace in three parts (buffers and clears; the intersection; sort and
+// statistics), so that a batch can run every frame's intersection in one
+// launch between the first part and the last.
+static int trace_head(nsgpu_ctx *c, hipStream_t st) {
     const DevGrid g = dev_grid(c);
     const int64_t N = g.N;
     const int64_t nrays = N * N * c->nslots;
-    const DevPatches P = dev_patches(c);
     const bool tiles = c->cast_mode == NSGPU_CAST_TILES && !c->surface_is_mesh;
     if (tiles && !c->cg_valid) return fail(c, NSGPU_E_STATE, "the cell grid is not built");
     HIP_TRY(c, c->rcnt.ensure(nrays * 4));
@@ -624,14 +637,10 @@ static int enqueue_trace(nsgpu_ctx *c, hipStream_t st) {
     if (c->record_pid) HIP_TRY(c, c->rpid.ensure((size_t)nrays * c->K * 4));
     const DevRays R = dev_rays(c);
     // per-workgroup partials: candidates of the intersection, (hits, odd, longest) of the sort
-    const int64_t irows = std::max<int64_t>(c->surface_is_mesh ? mesh_workgroups(c->mnt, R)
-                                            : tiles             ? tile_workgroups(R, c->cg_nc)
-                                                                : intersect_workgroups(P, R),
-                                            1),
-                  srows = sort_blocks(R);
+    const int64_t irows = intersect_rows(c), srows = sort_blocks(R);
     HIP_TRY(c, c->cand.ensure((size_t)irows * 8));
     HIP_TRY(c, c->stats.ensure((size_t)srows * 3 * 8));
-    unsigned long long *spart = c->stats.as<unsigned long long>(), *sout = c->dres.as<unsigned long long>();
+    unsigned long long *sout = c->dres.as<unsigned long long>();
     // one launch clears the ray counts, the candidate partials and the result
     // words the column reductions add into (hits, odd rays, longest, candidates)
     FillList fl{};
@@ -640,18 +649,50 @@ static int enqueue_trace(nsgpu_ctx *c, hipStream_t st) {
     fl.s[2] = FillSeg{reinterpret_cast<uint32_t *>(sout + RES_HITS), 0u, 2 * (RES_CAND + 1 - RES_HITS)};
     fl.k = 3;
     launch_fill_multi(st, fl);
+    return NSGPU_OK;
+}
+
+static int torus_roots(const nsgpu_ctx *c) {
+    return c->opts.precision == NSGPU_PREC_MIXED   ? kRootsMixed
+           : c->opts.precision == NSGPU_PREC_STURM ? kRootsSturm
+                                                   : kRootsClosed;
+}
+
+// the frame as the intersection kernels see it (its items first in the lists)
+static FrameCast frame_cast(const nsgpu_ctx *c, const DevRays &R, unsigned long long *part) {
+    FrameCast F{};
+    F.g = dev_grid(c);
+    F.P = dev_patches(c);
+    F.R = R;
+    F.probe = c->probe;
+    F.part = part;
+    return F;
+}
+
+static int trace_cast(nsgpu_ctx *c, hipStream_t st) {
+    const DevGrid g = dev_grid(c);
+    const DevPatches P = dev_patches(c);
+    const DevRays R = dev_rays(c);
+    const bool tiles = c->cast_mode == NSGPU_CAST_TILES && !c->surface_is_mesh;
     mark(c, ST_INTERSECT, 0, st);
-    const int roots = c->opts.precision == NSGPU_PREC_MIXED   ? kRootsMixed
-                      : c->opts.precision == NSGPU_PREC_STURM ? kRootsSturm
-                                                              : kRootsClosed;
-    if (c->surface_is_mesh)
+    const int roots = torus_roots(c);
+    if (c->surface_is_mesh) {
         launch_intersect_mesh(st, g, R, mesh_args(c), c->cand.as<unsigned long long>());
-    else if (tiles)
+    } else if (tiles) {
         launch_cast_tiles(st, g, P, R, cell_grid(c), c->probe, c->cand.as<unsigned long long>(), roots);
-    else
+    } else {
         launch_intersect(st, g, P, R, c->probe, c->cand.as<unsigned long long>(), roots);
+    }
     HIP_TRY(c, hipGetLastError());
     mark(c, ST_INTERSECT, 1, st);
+    return NSGPU_OK;
+}
+
+static int trace_tail(nsgpu_ctx *c, hipStream_t st) {
+    const DevGrid g = dev_grid(c);
+    const DevRays R = dev_rays(c);
+    const int64_t irows = intersect_rows(c), srows = sort_blocks(R);
+    unsigned long long *spart = c->stats.as<unsigned long long>(), *sout = c->dres.as<unsigned long long>();
     mark(c, ST_SORT, 0, st);
     launch_sort(st, g, R, spart);
     launch_reduce_cols(st, spart, srows, 3, 1u << 2, sout + RES_HITS, false);
@@ -659,6 +700,13 @@ static int enqueue_trace(nsgpu_ctx *c, hipStream_t st) {
     HIP_TRY(c, hipGetLastError());
     mark(c, ST_SORT, 1, st);
     return NSGPU_OK;
+}
+
+static int enqueue_trace(nsgpu_ctx *c, hipStream_t st) {
+    int rc = trace_head(c, st);
+    if (!rc) rc = trace_cast(c, st);
+    if (!rc) rc = trace_tail(c, st);
+    return rc;
 }
 
 // copy the result block to the pinned host mirror
@@ -893,9 +941,9 @@ static VolumeSlots volume_slots(const nsgpu_ctx *c, int64_t *traced) {
 }
 
 // everything of getSurf up to the cavities, then the result block copy
-static int enqueue_surf(nsgpu_ctx *c, hipStream_t st) {
-    int rc = enqueue_trace(c, st);
-    if (rc) return rc;
+// getSurf after the trace: maps, corner bitmap, volume, cavities, results
+static int enqueue_maps(nsgpu_ctx *c, hipStream_t st) {
+    int rc;
     const DevGrid g = dev_grid(c);
     const DevRays R = dev_rays(c);
     const nsgpu_surf_opts *o = &c->opts;
@@ -939,6 +987,12 @@ static int enqueue_surf(nsgpu_ctx *c, hipStream_t st) {
     }
     mark(c, ST_CAV, 1, st);
     return enqueue_results(c, st);
+}
+
+static int enqueue_surf(nsgpu_ctx *c, hipStream_t st) {
+    int rc = enqueue_trace(c, st);
+    if (!rc) rc = enqueue_maps(c, st);
+    return rc;
 }
 
 // host side of a completed enqueue_surf: statistics, volume, cavities
@@ -1273,7 +1327,7 @@ int nsg_get_surf_in(nsgpu_ctx *c, const nsgpu_surf_opts *o, nsgpu_surf_result *r
     HIP_TRY(c, hipSetDevice(c->device));
     const int64_t nvox = (int64_t)c->grid.nx * c->grid.ny * c->grid.nz;
     // set the context's own buffers aside (a fixed array: no host allocation)
-    constexpr int kMaxBufs = 128;
+    constexpr int kMaxBufs = 192;
     DBuf saved[kMaxBufs];
     DBuf *which[kMaxBufs];
     int ns = 0;
@@ -2056,11 +2110,18 @@ struct nsgpu_batch {
     int device = 0;
     std::vector<nsgpu_ctx *> frames;
     hipStream_t s0 = nullptr;
+    hipStream_t s1 = nullptr;  // the torus launch of a joint cast, beside s0's plain one
     hipEvent_t fork = nullptr;
+    hipEvent_t cast_fork = nullptr, cast_join = nullptr, cast_done = nullptr;
     std::vector<hipEvent_t> join;
+    std::vector<hipEvent_t> head;  // each frame's clears done (joint cast)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int chains = 0;
+    nsgpu_batch_shape shape{};  // of the captured graph
+    DBuf fcast;                 // FrameCast per frame, in item order (joint cast)
+    DBuf queue;                 // the persistent launches' item counters
+    int cus = 0;
     nsgpu_surf_opts opts{};
     bool prepared = false;
     DBuf stamps;  // frame completion clocks, then the start clock
@@ -2104,15 +2165,22 @@ int nsgpu_batch_create(int device, int32_t n_frames, nsgpu_batch **out) {
             return rc;
         }
         b->frames.push_back(c);
-        hipEvent_t e;
+        hipEvent_t e = nullptr, h = nullptr;
         (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
+        (void)hipEventCreateWithFlags(&h, hipEventDisableTiming);
         b->join.push_back(e);
+        b->head.push_back(h);
     }
     int khz = 0;
     if (hipStreamCreateWithFlags(&b->s0, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->s1, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->cast_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->cast_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->cast_done, hipEventDisableTiming) != hipSuccess ||
         hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) != hipSuccess ||
-        b->stamps.ensure((size_t)(n_frames + 1) * 8) != hipSuccess) {
+        hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        b->stamps.ensure((size_t)(n_frames + 1) * 8) != hipSuccess || b->queue.ensure(2 * 4) != hipSuccess) {
         nsgpu_batch_destroy(b);
         return NSGPU_E_DEVICE;
     }
@@ -2129,9 +2197,15 @@ void nsgpu_batch_destroy(nsgpu_batch *b) {
     for (nsgpu_ctx *c : b->frames) nsgpu_destroy(c);
     for (hipEvent_t e : b->join)
         if (e) (void)hipEventDestroy(e);
-    if (b->fork) (void)hipEventDestroy(b->fork);
+    for (hipEvent_t e : b->head)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {b->fork, b->cast_fork, b->cast_join, b->cast_done})
+        if (e) (void)hipEventDestroy(e);
     if (b->s0) (void)hipStreamDestroy(b->s0);
+    if (b->s1) (void)hipStreamDestroy(b->s1);
     b->stamps.release();
+    b->fcast.release();
+    b->queue.release();
     delete b;
 }
 
@@ -2154,41 +2228,134 @@ int nsgpu_batch_set_frame(nsgpu_batch *b, int32_t f, const nsgpu_grid *grid, con
 
 static int64_t frame_nodes(const nsgpu_ctx *c) { return (int64_t)c->grid.nx * c->grid.ny * c->grid.nz; }
 
-// capture every frame's getSurf into b->graph with `chains` concurrent chains
-static int capture(nsgpu_batch *b, int chains) {
+// a joint cast covers SES frames traced patch by patch (no skin cells, mesh
+// surface, cell-grid cast or recorded patch ids)
+static bool joint_cast_ok(const nsgpu_batch *b) {
+    for (const nsgpu_ctx *c : b->frames)
+        if (c->surface_is_mesh || c->cast_mode != NSGPU_CAST_PATCHES || c->record_pid || dev_patches(c).n_skin > 0)
+            return false;
+    return true;
+}
+
+// The frames' descriptors in item order, uploaded for a joint cast.
+static int upload_frames(nsgpu_batch *b, const std::vector<int> &order, BatchCast *bc) {
+    std::vector<FrameCast> fc(order.size());
+    int64_t items[2] = {0, 0};
+    for (size_t pos = 0; pos < order.size(); ++pos) {
+        nsgpu_ctx *c = b->frames[order[pos]];
+        FrameCast &F = fc[pos];
+        F = frame_cast(c, dev_rays(c), c->cand.as<unsigned long long>());
+        F.first[0] = items[0];
+        F.first[1] = items[1];
+        if (F.R.nslots > 0) {
+            items[0] += F.P.n_plain;
+            items[1] += (int64_t)(F.P.n - F.P.n_skin - F.P.n_plain) * F.R.nslots;
+        }
+    }
+    if (items[0] >= (int64_t)1 << 31 || items[1] >= (int64_t)1 << 31)
+        return bfail(b, NSGPU_E_ARG, "a joint cast holds at most 2^31 items per class");
+    BHIP_TRY(b, b->fcast.ensure(fc.size() * sizeof(FrameCast)));
+    BHIP_TRY(b, hipMemcpy(b->fcast.p, fc.data(), fc.size() * sizeof(FrameCast), hipMemcpyHostToDevice));
+    bc->f = b->fcast.as<FrameCast>();
+    bc->nf = (int)fc.size();
+    bc->items[0] = items[0];
+    bc->items[1] = items[1];
+    bc->queue = b->queue.as<unsigned int>();
+    return NSGPU_OK;
+}
+
+// Capture every frame's getSurf into b->graph: frames dealt round-robin onto
+// `chains` concurrent chains.  With a joint cast the chains first clear each
+// frame's rays, then one launch per patch class intersects every frame's
+// patches, then the chains sort, fill and label each frame.
+static int capture(nsgpu_batch *b, const nsgpu_batch_shape &shape) {
     drop_graph(b);
     const int n = (int)b->frames.size();
+    const int chains = std::max(1, std::min((int)shape.chains, n));
+    const bool joint = shape.cast != NSGPU_BATCH_CAST_FRAMES;
+    if (joint && !joint_cast_ok(b))
+        return bfail(b, NSGPU_E_ARG, "a joint cast needs SES frames traced patch by patch");
     std::vector<int> order(n);
     for (int f = 0; f < n; ++f) order[f] = f;
     std::stable_sort(order.begin(), order.end(),
                      [&](int x, int y) { return frame_nodes(b->frames[x]) > frame_nodes(b->frames[y]); });
+    BatchCast bc{};
+    if (joint) {
+        // begin_surf decides the frames' slots, which the descriptors carry
+        for (nsgpu_ctx *c : b->frames) {
+            const int rc = begin_surf(c, &b->opts);
+            if (rc) return bfail(b, rc, c->err);
+        }
+        const int rc = upload_frames(b, order, &bc);
+        if (rc) return rc;
+        bc.persistent = shape.cast == NSGPU_BATCH_CAST_PERSISTENT;
+        bc.chunk = shape.chunk > 0 ? shape.chunk : 16;
+    }
+    const int roots = torus_roots(b->frames[0]);
+    int grid[2] = {0, 0};
+    if (bc.persistent)
+        for (int cls = 0; cls < 2; ++cls)
+            grid[cls] = shape.workgroups > 0 ? shape.workgroups
+                                             : std::max(1, intersect_frames_occupancy(cls, roots)) * b->cus;
     unsigned long long *stamps = b->stamps.as<unsigned long long>();
     BHIP_TRY(b, hipStreamBeginCapture(b->s0, hipStreamCaptureModeRelaxed));
     g_no_alloc = true;
     int rc = NSGPU_OK;
+    auto wait = [&](hipStream_t st, hipEvent_t e) {
+        if (!rc && hipStreamWaitEvent(st, e, 0) != hipSuccess) rc = bfail(b, NSGPU_E_DEVICE, "graph wait");
+    };
+    auto record = [&](hipEvent_t e, hipStream_t st) {
+        if (!rc && hipEventRecord(e, st) != hipSuccess) rc = bfail(b, NSGPU_E_DEVICE, "graph event");
+    };
+    // one pass over the frames in chain order: work(c) on each frame's stream
+    auto chain_pass = [&](hipEvent_t start, std::vector<hipEvent_t> &done, auto work) {
+        for (int pos = 0; pos < n && !rc; ++pos) {
+            const int f = order[pos];
+            nsgpu_ctx *c = b->frames[f];
+            wait(c->stream, pos < chains ? start : done[order[pos - chains]]);
+            if (rc) break;
+            c->capturing = true;
+            rc = work(c);
+            c->capturing = false;
+            if (rc) {
+                rc = bfail(b, rc, std::string("frame ") + std::to_string(f) + ": " + c->err);
+                break;
+            }
+            if (&done == &b->join) launch_stamp(c->stream, stamps + f);
+            record(done[f], c->stream);
+        }
+        for (int f = 0; f < n; ++f) wait(b->s0, done[f]);
+    };
     launch_stamp(b->s0, stamps + n);
-    if (hipEventRecord(b->fork, b->s0) != hipSuccess) rc = bfail(b, NSGPU_E_DEVICE, "fork event");
-    for (int pos = 0; pos < n && !rc; ++pos) {
-        const int f = order[pos];
-        nsgpu_ctx *c = b->frames[f];
-        const hipEvent_t dep = pos < chains ? b->fork : b->join[order[pos - chains]];
-        if (hipStreamWaitEvent(c->stream, dep, 0) != hipSuccess) {
-            rc = bfail(b, NSGPU_E_DEVICE, "chain wait");
-            break;
+    record(b->fork, b->s0);
+    if (!joint) {
+        chain_pass(b->fork, b->join, [&](nsgpu_ctx *c) {
+            int r = begin_surf(c, &b->opts);
+            if (!r) r = enqueue_surf(c, c->stream);
+            return r;
+        });
+    } else {
+        chain_pass(b->fork, b->head, [&](nsgpu_ctx *c) { return trace_head(c, c->stream); });
+        if (!rc) {
+            // plain patches on s0, tori on s1, concurrently
+            if (bc.persistent) launch_fill32(b->s0, bc.queue, 0u, 2);
+            record(b->cast_fork, b->s0);
+            wait(b->s1, b->cast_fork);
+            if (!rc) {
+                launch_intersect_frames(b->s1, bc, 1, roots, grid[1]);
+                launch_intersect_frames(b->s0, bc, 0, roots, grid[0]);
+                if (hipGetLastError() != hipSuccess) rc = bfail(b, NSGPU_E_DEVICE, "joint cast launch");
+            }
+            record(b->cast_join, b->s1);
+            wait(b->s0, b->cast_join);
+            record(b->cast_done, b->s0);
         }
-        c->capturing = true;
-        rc = begin_surf(c, &b->opts);
-        if (!rc) rc = enqueue_surf(c, c->stream);
-        c->capturing = false;
-        if (rc) {
-            rc = bfail(b, rc, std::string("frame ") + std::to_string(f) + ": " + c->err);
-            break;
-        }
-        launch_stamp(c->stream, stamps + f);
-        if (hipEventRecord(b->join[f], c->stream) != hipSuccess) rc = bfail(b, NSGPU_E_DEVICE, "join event");
+        chain_pass(b->cast_done, b->join, [&](nsgpu_ctx *c) {
+            int r = trace_tail(c, c->stream);
+            if (!r) r = enqueue_maps(c, c->stream);
+            return r;
+        });
     }
-    for (int f = 0; f < n && !rc; ++f)
-        if (hipStreamWaitEvent(b->s0, b->join[f], 0) != hipSuccess) rc = bfail(b, NSGPU_E_DEVICE, "join wait");
     g_no_alloc = false;
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(b->s0, &graph);
@@ -2200,6 +2367,10 @@ static int capture(nsgpu_batch *b, int chains) {
     b->graph = graph;
     BHIP_TRY(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     b->chains = chains;
+    b->shape = shape;
+    b->shape.chains = chains;
+    b->shape.workgroups = grid[1];
+    b->shape.chunk = bc.persistent ? bc.chunk : 0;
     b->prepared = true;
     b->gen.resize(n);
     for (int f = 0; f < n; ++f) b->gen[f] = ctx_generation(b->frames[f]);
@@ -2226,26 +2397,48 @@ static int launch_graph(nsgpu_batch *b, double *ms) {
     return NSGPU_OK;
 }
 
-int nsgpu_batch_prepare(nsgpu_batch *b, const nsgpu_surf_opts *opts, int32_t chains) {
-    if (!b || !opts) return NSGPU_E_ARG;
+// Eager pass: sizes every buffer and each frame's ray capacity.
+static int batch_eager(nsgpu_batch *b, const nsgpu_surf_opts *opts) {
+    if (!opts) return NSGPU_E_ARG;
     if (opts->accurate_triangulation)
         return bfail(b, NSGPU_E_ARG, "batches compute maps and cavities only (accurate_triangulation = 0)");
     BHIP_TRY(b, hipSetDevice(b->device));
     b->opts = *opts;
     drop_graph(b);
-    // eager pass: sizes every buffer and each frame's ray capacity
     for (size_t f = 0; f < b->frames.size(); ++f) {
         const int rc = nsgpu_get_surf(b->frames[f], opts, nullptr);
         if (rc) return bfail(b, rc, std::string("frame ") + std::to_string(f) + ": " + b->frames[f]->err);
     }
+    return NSGPU_OK;
+}
+
+// Autotuner: captures each candidate shape, keeps the one with the shortest
+// of two launches.  Candidates: per-frame casts on 1, 2, 4, ... 64 chains;
+// with SES frames also the joint cast (a workgroup per item) and the
+// persistent one (the resident limit of workgroups taking 16 or 4 items at a
+// time, and half the limit taking 16), each on 16, 32 and 64 chains.
+static int autotune(nsgpu_batch *b) {
     const int n = (int)b->frames.size();
-    if (chains > 0) return capture(b, std::min(chains, n));
-    // autotune: the chain count with the shortest of two launches
-    int best = 1;
-    double best_ms = 1e30;
+    std::vector<nsgpu_batch_shape> cands;
     for (int ch = 1;; ch *= 2) {
-        const int c = std::min(ch, n);
-        int rc = capture(b, c);
+        cands.push_back(nsgpu_batch_shape{NSGPU_BATCH_CAST_FRAMES, std::min(ch, n), 0, 0});
+        if (ch >= n || ch >= 64) break;
+    }
+    if (joint_cast_ok(b)) {
+        const int roots = torus_roots(b->frames[0]);
+        const int full = std::max(1, intersect_frames_occupancy(1, roots)) * b->cus;
+        for (int ch : {16, 32, 64}) {
+            if (ch > 2 * n) break;
+            cands.push_back(nsgpu_batch_shape{NSGPU_BATCH_CAST_JOINT, std::min(ch, n), 0, 0});
+            cands.push_back(nsgpu_batch_shape{NSGPU_BATCH_CAST_PERSISTENT, std::min(ch, n), full, 16});
+            cands.push_back(nsgpu_batch_shape{NSGPU_BATCH_CAST_PERSISTENT, std::min(ch, n), full, 4});
+            cands.push_back(nsgpu_batch_shape{NSGPU_BATCH_CAST_PERSISTENT, std::min(ch, n), full / 2, 16});
+        }
+    }
+    nsgpu_batch_shape best = cands[0];
+    double best_ms = 1e30;
+    for (const nsgpu_batch_shape &sh : cands) {
+        int rc = capture(b, sh);
         if (rc) return rc;
         double t = 1e30;
         for (int rep = 0; rep < 2; ++rep) {
@@ -2256,11 +2449,35 @@ int nsgpu_batch_prepare(nsgpu_batch *b, const nsgpu_surf_opts *opts, int32_t cha
         }
         if (t < best_ms) {
             best_ms = t;
-            best = c;
+            best = sh;
         }
-        if (c == n || ch >= 64) break;
     }
     return capture(b, best);
+}
+
+int nsgpu_batch_prepare(nsgpu_batch *b, const nsgpu_surf_opts *opts, int32_t chains) {
+    if (!b) return NSGPU_E_ARG;
+    int rc = batch_eager(b, opts);
+    if (rc) return rc;
+    if (chains > 0) return capture(b, nsgpu_batch_shape{NSGPU_BATCH_CAST_FRAMES, chains, 0, 0});
+    return autotune(b);
+}
+
+int nsgpu_batch_prepare_shape(nsgpu_batch *b, const nsgpu_surf_opts *opts, const nsgpu_batch_shape *shape) {
+    if (!b) return NSGPU_E_ARG;
+    if (shape && (shape->cast < NSGPU_BATCH_CAST_FRAMES || shape->cast > NSGPU_BATCH_CAST_PERSISTENT ||
+                  shape->chains <= 0 || shape->workgroups < 0 || shape->chunk < 0))
+        return bfail(b, NSGPU_E_ARG, "unknown batch shape");
+    int rc = batch_eager(b, opts);
+    if (rc) return rc;
+    return shape ? capture(b, *shape) : autotune(b);
+}
+
+int nsgpu_batch_get_shape(const nsgpu_batch *b, nsgpu_batch_shape *shape) {
+    if (!b || !shape) return NSGPU_E_ARG;
+    if (!b->prepared) return NSGPU_E_STATE;
+    *shape = b->shape;
+    return NSGPU_OK;
 }
 
 int32_t nsgpu_batch_chains(const nsgpu_batch *b) { return b ? b->chains : 0; }
@@ -2278,7 +2495,7 @@ int nsgpu_batch_run(nsgpu_batch *b, double *latency_ms, int32_t *redone) {
             rc = nsgpu_get_surf(b->frames[f], &b->opts, nullptr);
             if (rc) return bfail(b, rc, std::string("frame ") + std::to_string(f) + ": " + b->frames[f]->err);
         }
-        rc = capture(b, b->chains);
+        rc = capture(b, b->shape);
         if (rc) return rc;
     }
     rc = launch_graph(b, nullptr);

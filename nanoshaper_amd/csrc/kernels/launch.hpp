@@ -44,10 +44,40 @@ int64_t intersect_workgroups(const DevPatches &, const DevRays &);
 int64_t sort_blocks(const DevRays &);
 // torus root solvers of the intersection kernel
 enum { kRootsClosed = 0, kRootsMixed = 1, kRootsSturm = 2 };
+// One frame of a multi-frame cast: its grid, patches, rays, probe radius,
+// candidate partials (the rows of launch_intersect), and the first item of
+// each class (0 plain, 1 tori) in the batch's item lists.
+struct FrameCast {
+    DevGrid g;
+    DevPatches P;
+    DevRays R;
+    double probe;
+    unsigned long long *part;
+    int64_t first[2];
+};
+// The frames of a multi-frame cast in item order.  Items of class c of frame
+// f are [first[c], first[c] + count) of that class's list; a frame's plain
+// items are its plain patches, its torus items (torus rank, slot) pairs.
+struct BatchCast {
+    const FrameCast *f;
+    int nf;
+    int64_t items[2];     // items per class
+    unsigned int *queue;  // [2] next item of each class, zero before the launch
+    int persistent;       // 1: workgroups take items from the queue; 0: one item per workgroup
+    int chunk;            // items per queue take (persistent)
+};
+// One launch over every frame's items of class cls; grid = workgroups of a
+// persistent launch (ignored when B.persistent == 0).
+void launch_intersect_frames(hipStream_t, const BatchCast &B, int cls, int roots, int grid);
+// One frame's intersection, a workgroup per item of each class (part:
+// intersect_workgroups() rows, zeroed by the caller).
 void launch_intersect(hipStream_t, const DevGrid &, const DevPatches &, const DevRays &, double,
                       unsigned long long *part, int roots);
+// counting variant: PC_N path counters per row of part
 void launch_intersect_count(hipStream_t, const DevGrid &, const DevPatches &, const DevRays &, double,
                             unsigned long long *part);
+// resident workgroups per CU of the class's multi-frame kernel
+int intersect_frames_occupancy(int cls, int roots);
 // skin cells: order[n - n_skin, n), same per-workgroup candidate rows
 void launch_intersect_skin(hipStream_t, const DevGrid &, const DevPatches &, const DevRays &, unsigned long long *part);
 void launch_sort(hipStream_t, const DevGrid &, const DevRays &, unsigned long long *part);

@@ -704,175 +704,37 @@ __device__ __forceinline__ int ray_torus_sturm(const double *td, const double sc
 // Path counters of the counting variant (see intersect_path_counts).
 enum { PC_CAND, PC_DISC, PC_SPHERE, PC_TORUS, PC_ROOTS, PC_HITS, PC_FEAS_ITEMS, PC_FEAS_CALLS, PC_N };
 
-// grid: (numPatches, nslots); block: 64 lanes (one wave)
-// Per-workgroup counts go to the workgroup's own row of 'part' (zeroed before
-// the launch) and are summed by k_reduce_cols: one atomic per workgroup on a
-// shared address would serialise 10^5 workgroups at the L2.
-// The tori of P.order (spheres and facets run in k_intersect_plain).  kRoots:
-// torus roots from the closed form (kRootsClosed), the mixed-precision solver
+// The tori of P.order (spheres and facets are k_intersect_plain's), one
+// 64-lane wave per (torus, slot): as its own launch over one frame
+// (k_intersect_torus, item = blockIdx.x) or as an item of a multi-frame launch
+// (torus_item).  Per-item counts go to the item's own row of 'part' (zeroed
+// before the launch) and are summed by k_reduce_cols: one atomic per item on
+// a shared address would serialise 10^5 workgroups at the L2.  kRoots: torus
+// roots from the closed form (kRootsClosed), the mixed-precision solver
 // (kRootsMixed) or Sturm bisection (kRootsSturm).  kCount: instead of hits,
-// per-workgroup path counts.
+// per-item path counts.
 template <bool kCount, int kRoots>
 __global__ void __launch_bounds__(64) k_intersect_torus(DevGrid g, DevPatches P, DevRays R, double probe,
-                                                  unsigned long long *part) {
+                                                        unsigned long long *part) {
     // workgroup = patch rank * nslots + slot: the patches of a class come
     // largest first, and every slot starts on them together
-    const int first = P.n_plain;
     const int rank = (int)(blockIdx.x / R.nslots);
     const int slot = (int)(blockIdx.x % R.nslots);
-    const int64_t wg = (int64_t)slot * P.n + first + rank;
-    const int it = P.order[first + rank];
-    const int phase = slot < R.panels0 ? 0 : 1;
-    const int panel = phase == 0 ? slot : slot - R.panels0;
-    const int kind = P.kind[it];
-    const double sc[3] = {P.bsph[4 * it], P.bsph[4 * it + 1], P.bsph[4 * it + 2]};
-    const double radius = P.bsph[4 * it + 3];
-    const double *pd = P.data + P.off[it];
-    const int pcnt = P.cnt[it], pflags = P.flags[it];
-    const double down[3] = {sc[0] - radius, sc[1] - radius, sc[2] - radius};
-    const double up[3] = {sc[0] + radius, sc[1] + radius, sc[2] + radius};
-    // CELL_CULLING (ConnollySurface.cpp:4378-4391)
-    if (down[0] > g.xmax + kEpsInt || down[1] > g.ymax + kEpsInt || down[2] > g.zmax + kEpsInt) return;
-    const double lowpad = g.hside - kDelta + kEpsInt;
-    if (up[0] < g.xmin - lowpad || up[1] < g.ymin - lowpad || up[2] < g.zmin - lowpad) return;
-    const double gmin[3] = {g.xmin, g.ymin, g.zmin};
-    const int gn[3] = {g.nx, g.ny, g.nz};
-    int64_t is[3], ie[3];
-    for (int d = 0; d < 3; ++d) {
-        is[d] = (int64_t)rintp(fmax(0., down[d] - gmin[d]) * g.scale);
-        ie[d] = (int64_t)rintp(fmax(0., up[d] - gmin[d]) * g.scale);
-        if (ie[d] >= gn[d]) ie[d] = gn[d] - 1;
-    }
-    const int64_t pix[3] = {1 + ie[0] - is[0], 1 + ie[1] - is[1], 1 + ie[2] - is[2]};
-    const double delta = (phase == 0) ? 0. : kDelta - g.hside;
-    unsigned int cand = 0;
-    unsigned int pc[PC_N] = {0, 0, 0, 0, 0, 0, 0, 0};
-    // The pixels that reach the sphere test are queued in LDS and taken 64 at
-    // a time, so that the roots and the feasibility tests (most of the work)
-    // run on full waves: the disc cull and the rectangle's last partial row of
-    // lanes would otherwise leave a fifth to a half of the lanes idle there.
-    __shared__ int qm[128], qn[128];
-    __shared__ double qt0[128], qt1[128];  // the entries' bounding-sphere depths
-    // One instance per panel, so that every axis index is a constant and the
-    // ray origin, hit point and frame rows stay in registers.
-    auto walk = [&](auto panel_c) {
-        constexpr int fd = PanelAxes<decltype(panel_c)::value>::fd;
-        constexpr int ld = PanelAxes<decltype(panel_c)::value>::ld;
-        constexpr int vc = PanelAxes<decltype(panel_c)::value>::vc;
-        const double *ga = vc == 0 ? g.x : (vc == 1 ? g.y : g.z);  // axis of the ray
-        const double *gf = fd == 0 ? g.x : g.y;
-        const double *gl = ld == 1 ? g.y : g.z;
-        const int na = gn[vc];
-        const double ray_dir = ga[na - 1] - ga[0];
-        double dir[3] = {0., 0., 0.};
-        dir[vc] = ray_dir;
-        const int64_t total = pix[fd] * pix[ld];
-        // the origin of the ray through pixel (m, n)
-        auto origin = [&](int m, int n, double pa[3]) {
-            pa[vc] = ga[0] + delta;
-            pa[fd] = gf[m] + delta;
-            pa[ld] = gl[n] + delta;
-        };
-        auto roots = [&](int m, int n, double t0, double t1) {
-            double pa[3], t[4];
-            origin(m, n, pa);
-            t[0] = t0;  // ray_sphere's roots from the cull pass
-            t[1] = t1;
-            int ni;
-            if (kRoots == kRootsMixed)
-                ni = ray_torus_mixed(pd, sc, probe, radius, vc, pa, ray_dir, t);
-            else if (kRoots == kRootsSturm)
-                ni = ray_torus_sturm(pd, sc, probe, radius, vc, pa, ray_dir, t);
-            else
-                ni = ray_torus(pd, sc, probe, radius, vc, pa, ray_dir, t);
-            if (kCount) ++pc[PC_TORUS];
-            for (int i = 0; i < ni; i++) {
-                double ip[3] = {pa[0], pa[1], pa[2]};
-                ip[vc] += t[i] * ray_dir;
-                const bool ok = feasible_torus(kind, pd, pcnt, pflags, ip);
-                if (kCount) {
-                    ++pc[PC_ROOTS];
-                    ++pc[PC_FEAS_CALLS];
-                    pc[PC_FEAS_ITEMS] += 2 * pcnt + 1;
-                    pc[PC_HITS] += ok;
-                }
-                if (!ok) continue;
-                if (!kCount) push_hit(R, slot, (int64_t)n * R.N + m, t[i], it);
-            }
-        };
-        // lane walk over the rectangle without a division per pixel: pixel
-        // rp = row * pf + col advances by 64 = step_r rows + step_c columns
-        const int64_t pf = pix[fd];
-        const int step_r = (int)(64 / pf), step_c = (int)(64 % pf);
-        int col = (int)(threadIdx.x % pf), row = (int)(threadIdx.x / pf);
-        int queued = 0;
-        for (int64_t base = 0; base < total; base += 64) {
-            bool act = false;
-            double t0 = 0., t1 = 0.;
-            const int m = col + (int)is[fd], n = row + (int)is[ld];
-            if (base + threadIdx.x < total) {
-                double pa[3];
-                origin(m, n, pa);
-                // bounding box, EQ_CULLING (ConnollySurface.cpp:4496-4502), bounding sphere
-                if (!(pa[fd] < down[fd] || pa[fd] > up[fd] || pa[ld] < down[ld] || pa[ld] > up[ld])) {
-                    ++cand;
-                    const double dx = sc[fd] - pa[fd], dy = sc[ld] - pa[ld];
-                    if (!(dx * dx + dy * dy > radius * radius)) {
-                        if (kCount) ++pc[PC_DISC];
-                        act = ray_sphere(pa, dir, sc, radius, t0, t1);
-                        if (kCount && act) ++pc[PC_SPHERE];
-                        if (act) act = !torus_far<vc>(pd, pa, t0 * ray_dir, t1 * ray_dir, probe);
-                    }
-                }
-            }
-            col += step_c;
-            row += step_r;
-            if (col >= pf) {
-                col -= (int)pf;
-                ++row;
-            }
-            const uint64_t mask = __ballot(act);
-            if (act) {
-                const int at = queued + __popcll(mask & ((1ull << threadIdx.x) - 1ull));
-                qm[at] = m;
-                qn[at] = n;
-                qt0[at] = t0;
-                qt1[at] = t1;
-            }
-            queued += __popcll(mask);
-            if (queued >= 64) {
-                __syncthreads();
-                queued -= 64;
-                const int e = queued + threadIdx.x;
-                roots(qm[e], qn[e], qt0[e], qt1[e]);
-                __syncthreads();
-            }
-        }
-        if (queued > 0) {
-            __syncthreads();
-            if ((int)threadIdx.x < queued) roots(qm[threadIdx.x], qn[threadIdx.x], qt0[threadIdx.x], qt1[threadIdx.x]);
-        }
-    };
-    if (panel == 0) walk(std::integral_constant<int, 0>());
-    else if (panel == 1) walk(std::integral_constant<int, 1>());
-    else walk(std::integral_constant<int, 2>());
-    if (kCount) {
-        pc[PC_CAND] = cand;
-        for (int q = 0; q < PC_N; ++q) {
-            unsigned long long v = pc[q];
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-            if (threadIdx.x == 0 && v) part[wg * PC_N + q] = v;
-        }
-        return;
-    }
-    // candidate count: wave reduction, one store per workgroup
-    for (int off = 32; off > 0; off >>= 1) cand += __shfl_down(cand, off, 64);
-    if (threadIdx.x == 0 && cand) part[wg] = cand;
+#include "torus_item.inc"
+}
+
+template <bool kCount, int kRoots>
+__device__ __forceinline__ void torus_item(const DevGrid &g, const DevPatches &P, const DevRays &R, double probe,
+                                           unsigned long long *part, int bid) {
+    const int rank = bid / R.nslots;
+    const int slot = bid % R.nslots;
+#include "torus_item.inc"
 }
 
 
-// Sphere and facet patches (P.order [0, n_plain)), one workgroup per patch
-// over every slot.  The walk over each slot's pixel rectangle (the culls of
+
+// Sphere and facet patches (P.order [0, n_plain)), one item (a 64-lane
+// wave) per patch over every slot.  The walk over each slot's pixel rectangle (the culls of
 // k_intersect) appends the pixels that reach the sphere test to a ring queue
 // in LDS; a lane takes the next queued pixel as soon as its own is decided
 // and tests its two depths against the patch's items in order, eight at a
@@ -890,226 +752,15 @@ template <bool kCount>
 __global__ void __launch_bounds__(64) k_intersect_plain(DevGrid g, DevPatches P, DevRays R,
                                                         unsigned long long *part) {
     const int rank = (int)blockIdx.x;
-    const int it = P.order[rank];
-    const int kind = P.kind[it];
-    const double sc[3] = {P.bsph[4 * it], P.bsph[4 * it + 1], P.bsph[4 * it + 2]};
-    const double radius = P.bsph[4 * it + 3];
-    const double *pd = P.data + P.off[it];
-    const int pcnt = P.cnt[it];
-    const double down[3] = {sc[0] - radius, sc[1] - radius, sc[2] - radius};
-    const double up[3] = {sc[0] + radius, sc[1] + radius, sc[2] + radius};
-    if (R.nslots == 0) return;
-    // CELL_CULLING (ConnollySurface.cpp:4378-4391)
-    if (down[0] > g.xmax + kEpsInt || down[1] > g.ymax + kEpsInt || down[2] > g.zmax + kEpsInt) return;
-    const double lowpad = g.hside - kDelta + kEpsInt;
-    if (up[0] < g.xmin - lowpad || up[1] < g.ymin - lowpad || up[2] < g.zmin - lowpad) return;
-    const double gmin[3] = {g.xmin, g.ymin, g.zmin};
-    const int gn[3] = {g.nx, g.ny, g.nz};
-    int64_t is[3], ie[3];
-    for (int d = 0; d < 3; ++d) {
-        is[d] = (int64_t)rintp(fmax(0., down[d] - gmin[d]) * g.scale);
-        ie[d] = (int64_t)rintp(fmax(0., up[d] - gmin[d]) * g.scale);
-        if (ie[d] >= gn[d]) ie[d] = gn[d] - 1;
-    }
-    const int64_t pix[3] = {1 + ie[0] - is[0], 1 + ie[1] - is[1], 1 + ie[2] - is[2]};
-    const int lane = (int)threadIdx.x;
-    const unsigned long long below = (1ull << lane) - 1ull;
-
-    __shared__ double sitem[4 * kStageItems];
-    const bool staged = pcnt <= kStageItems;
-    if (staged) {
-        for (int x = lane; x < 4 * pcnt; x += 64) sitem[x] = pd[x];
-        __syncthreads();
-    }
-    __shared__ int qm[kRing], qn[kRing], qs[kRing];
-    __shared__ double qt0[kRing], qt1[kRing];  // the pixels' bounding-sphere depths
-    unsigned int cand = 0;
-    unsigned int pc[PC_N] = {0, 0, 0, 0, 0, 0, 0, 0};
-
-    // walk cursor: slot, the slot's rectangle, and each lane's pixel in it
-    // (pixel rp = row * pf + col advances by 64 = step_r rows + step_c columns)
-    int wslot = 0, wpf = 1, step_r = 0, step_c = 0, col = 0, row = 0;
-    int64_t wbase = 0, wtotal = 0;
-    auto start_slot = [&](int s) {
-        const int panel = s < R.panels0 ? s : s - R.panels0;
-        const int fd = panel == 0 ? 1 : 0, ld = panel == 1 ? 1 : 2;
-        wpf = (int)pix[fd];
-        wtotal = pix[fd] * pix[ld];
-        wbase = 0;
-        step_r = 64 / wpf;
-        step_c = 64 % wpf;
-        col = lane % wpf;
-        row = lane / wpf;
-    };
-    bool walking = true;
-    start_slot(0);
-    int head = 0, tail = 0;  // ring positions (uniform); entries [head, tail) wait
-    // One walk step of 64 pixels; one instance per panel, so that every axis
-    // index is a constant.
-    auto step = [&](auto panel_c) {
-        constexpr int fd = PanelAxes<decltype(panel_c)::value>::fd;
-        constexpr int ld = PanelAxes<decltype(panel_c)::value>::ld;
-        constexpr int vc = PanelAxes<decltype(panel_c)::value>::vc;
-        const double *ga = vc == 0 ? g.x : (vc == 1 ? g.y : g.z);
-        const double *gf = fd == 0 ? g.x : g.y;
-        const double *gl = ld == 1 ? g.y : g.z;
-        const double delta = wslot < R.panels0 ? 0. : kDelta - g.hside;
-        double dir[3] = {0., 0., 0.};
-        dir[vc] = ga[gn[vc] - 1] - ga[0];
-        bool act = false;
-        double t0 = 0., t1 = 0.;
-        const int m = col + (int)is[fd], n = row + (int)is[ld];
-        if (wbase + lane < wtotal) {
-            double pa[3];
-            pa[vc] = ga[0] + delta;
-            pa[fd] = gf[m] + delta;
-            pa[ld] = gl[n] + delta;
-            // bounding box, EQ_CULLING (ConnollySurface.cpp:4496-4502), bounding sphere
-            if (!(pa[fd] < down[fd] || pa[fd] > up[fd] || pa[ld] < down[ld] || pa[ld] > up[ld])) {
-                ++cand;
-                const double dx = sc[fd] - pa[fd], dy = sc[ld] - pa[ld];
-                if (!(dx * dx + dy * dy > radius * radius)) {
-                    if (kCount) ++pc[PC_DISC];
-                    act = ray_sphere(pa, dir, sc, radius, t0, t1);
-                    if (kCount && act) ++pc[PC_SPHERE];
-                }
-            }
-        }
-        col += step_c;
-        row += step_r;
-        if (col >= wpf) {
-            col -= wpf;
-            ++row;
-        }
-        const unsigned long long mask = __ballot(act);
-        if (act) {
-            const int at = (tail + __popcll(mask & below)) & (kRing - 1);
-            qm[at] = m;
-            qn[at] = n;
-            qs[at] = wslot;
-            qt0[at] = t0;
-            qt1[at] = t1;
-        }
-        tail += __popcll(mask);
-        wbase += 64;
-        if (wbase >= wtotal) {
-            if (++wslot < R.nslots) start_slot(wslot);
-            else walking = false;
-        }
-    };
-    // four items against both depths; indices past the last item repeat it
-    // (a repeated test does not change an AND)
-    bool op = false, oq = false;
-    double ip0[3] = {0., 0., 0.}, ip1[3] = {0., 0., 0.};
-    auto test4 = [&](const double *items, int i) {
-        const double *a = items + 4 * i, *b = items + 4 * min(i + 1, pcnt - 1), *c = items + 4 * min(i + 2, pcnt - 1),
-                     *d = items + 4 * min(i + 3, pcnt - 1);
-        if (kind == NSGPU_PATCH_POINT) {
-            op |= in_clip_sphere(a, ip0) | in_clip_sphere(b, ip0) | in_clip_sphere(c, ip0) | in_clip_sphere(d, ip0);
-            oq |= in_clip_sphere(a, ip1) | in_clip_sphere(b, ip1) | in_clip_sphere(c, ip1) | in_clip_sphere(d, ip1);
-        } else {
-            op |= plane_out(a, ip0) | plane_out(b, ip0) | plane_out(c, ip0) | plane_out(d, ip0);
-            oq |= plane_out(a, ip1) | plane_out(b, ip1) | plane_out(c, ip1) | plane_out(d, ip1);
-        }
-    };
-    bool busy = false;
-    int item = 0, m = 0, n = 0, es = 0;
-    double t0 = 0., t1 = 0.;
-    for (;;) {
-        if (walking && tail - head < 64) {
-            const int panel = wslot < R.panels0 ? wslot : wslot - R.panels0;
-            if (panel == 0) step(std::integral_constant<int, 0>());
-            else if (panel == 1) step(std::integral_constant<int, 1>());
-            else step(std::integral_constant<int, 2>());
-            __syncthreads();
-            continue;
-        }
-        const unsigned long long fm = __ballot(!busy);
-        if (!busy) {
-            const int at = head + __popcll(fm & below);
-            if (at < tail) {
-                const int x = at & (kRing - 1);
-                m = qm[x];
-                n = qn[x];
-                es = qs[x];
-                t0 = qt0[x];
-                t1 = qt1[x];
-                // the pixel's origin and the two depths' points (as the walk's origin)
-                const int panel = es < R.panels0 ? es : es - R.panels0;
-                const double delta = es < R.panels0 ? 0. : kDelta - g.hside;
-                double pa[3], rd;
-                int vc;
-                if (panel == 0) {
-                    pa[0] = g.x[0] + delta;
-                    pa[1] = g.y[m] + delta;
-                    pa[2] = g.z[n] + delta;
-                    rd = g.x[g.nx - 1] - g.x[0];
-                    vc = 0;
-                } else if (panel == 1) {
-                    pa[0] = g.x[m] + delta;
-                    pa[1] = g.y[n] + delta;
-                    pa[2] = g.z[0] + delta;
-                    rd = g.z[g.nz - 1] - g.z[0];
-                    vc = 2;
-                } else {
-                    pa[0] = g.x[m] + delta;
-                    pa[1] = g.y[0] + delta;
-                    pa[2] = g.z[n] + delta;
-                    rd = g.y[g.ny - 1] - g.y[0];
-                    vc = 1;
-                }
-                for (int d = 0; d < 3; ++d) {
-                    ip0[d] = d == vc ? pa[d] + t0 * rd : pa[d];
-                    ip1[d] = d == vc ? pa[d] + t1 * rd : pa[d];
-                }
-                busy = true;
-                item = 0;
-                op = oq = false;
-            }
-        }
-        head = min(head + __popcll(fm), tail);
-        if (__ballot(busy) == 0) {
-            if (!walking) break;
-            continue;
-        }
-        if (!busy) continue;
-        if (item < pcnt) {  // eight items a step (278 vs 290 us with four at 335^3)
-            if (staged) {
-                test4(sitem, item);
-                if (item + 4 < pcnt) test4(sitem, item + 4);
-            } else {
-                test4(pd, item);
-                if (item + 4 < pcnt) test4(pd, item + 4);
-            }
-            item += 8;
-        }
-        if ((op & oq) || item >= pcnt) {
-            if (kCount) {
-                pc[PC_ROOTS] += 2;
-                pc[PC_FEAS_CALLS] += 2;
-                pc[PC_FEAS_ITEMS] += 2 * pcnt;
-                pc[PC_HITS] += (unsigned)!op + (unsigned)!oq;
-            } else {
-                const int64_t ray = (int64_t)n * R.N + m;
-                if (!op) push_hit(R, es, ray, t0, it);
-                if (!oq) push_hit(R, es, ray, t1, it);
-            }
-            busy = false;
-        }
-    }
-    // per-workgroup partials in the rows of slot 0
-    if (kCount) {
-        pc[PC_CAND] = cand;
-        for (int q = 0; q < PC_N; ++q) {
-            unsigned long long v = pc[q];
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-            if (lane == 0 && v) part[(int64_t)rank * PC_N + q] = v;
-        }
-        return;
-    }
-    for (int off = 32; off > 0; off >>= 1) cand += __shfl_down(cand, off, 64);
-    if (lane == 0 && cand) part[rank] = cand;
+#include "plain_item.inc"
 }
+
+template <bool kCount>
+__device__ __forceinline__ void plain_item(const DevGrid &g, const DevPatches &P, const DevRays &R,
+                                           unsigned long long *part, int rank) {
+#include "plain_item.inc"
+}
+
 
 // Writes one ray's coverage plane: node intervals arrive in ascending,
 // disjoint order (the segments of a sorted ray), and every one of the W words
@@ -1675,6 +1326,108 @@ void launch_cast_rays(hipStream_t s, const DevGrid &g, const DevPatches &P, cons
     if (panel == 0) hipLaunchKernelGGL(k_cast_rays<0>, grid, block, 0, s, g, P, C, probe, n, orig, coord, maxh, count, hits, normals);
     else if (panel == 1) hipLaunchKernelGGL(k_cast_rays<1>, grid, block, 0, s, g, P, C, probe, n, orig, coord, maxh, count, hits, normals);
     else hipLaunchKernelGGL(k_cast_rays<2>, grid, block, 0, s, g, P, C, probe, n, orig, coord, maxh, count, hits, normals);
+}
+
+// The intersection of one or many frames: every frame's items of one class
+// (0: spheres and facets, one item per patch over every slot; 1: tori, one
+// item per (torus, slot)).  A single getSurf is the one-frame case.  Frames
+// come largest first and each frame's items largest patch first, so the long
+// items are taken early and the launch ends on short ones.  The frames'
+// descriptors are read from device memory through the constant address
+// space, as wave-uniform scalar loads.
+
+// the frame holding item i: the last whose first item is <= i
+template <int kClass>
+__device__ __forceinline__ int frame_of(const BatchCast &B, int64_t i) {
+    int lo = 0, hi = B.nf - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (B.f[mid].first[kClass] <= i)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int kClass, int kRoots>
+__device__ __forceinline__ void run_item(const BatchCast &B, int64_t i) {
+    typedef const __attribute__((address_space(4))) FrameCast ConstFrame;
+    ConstFrame *F = (ConstFrame *)(B.f + frame_of<kClass>(B, i));
+    if (kClass == 0)
+        plain_item<false>((const DevGrid &)F->g, (const DevPatches &)F->P, (const DevRays &)F->R, F->part,
+                          (int)(i - F->first[0]));
+    else
+        torus_item<false, kRoots>((const DevGrid &)F->g, (const DevPatches &)F->P, (const DevRays &)F->R,
+                                  F->probe, F->part, (int)(i - F->first[1]));
+}
+
+// workgroup b runs item b
+template <int kClass, int kRoots>
+__global__ void __launch_bounds__(64) k_intersect_frames(BatchCast B) {
+    if ((int64_t)blockIdx.x < B.items[kClass]) run_item<kClass, kRoots>(B, blockIdx.x);
+}
+
+// Persistent: each workgroup takes the next B.chunk items from the class's
+// queue until the list is used up (one atomic per chunk: with one per item
+// the queue's address serialises millions of returns at the L2).  The loop
+// keeps the item's invariant values (LDS addresses, lane masks, constants)
+// live across iterations, which costs a wave per SIMD against the one-item
+// kernel (3 resident instead of 4).
+template <int kClass, int kRoots>
+__global__ void __launch_bounds__(64) k_intersect_persistent(BatchCast B) {
+    const int64_t total = B.items[kClass];
+    for (;;) {
+        unsigned int v = 0;
+        if (threadIdx.x == 0) v = atomicAdd(B.queue + kClass, (unsigned)B.chunk);
+        // lane 0's value, as a wave-uniform scalar
+        const int64_t i0 = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+        if (i0 >= total) return;
+        const int64_t i1 = min(total, i0 + B.chunk);
+        for (int64_t i = i0; i < i1; ++i) {
+            run_item<kClass, kRoots>(B, i);
+            __syncthreads();  // the next item reuses the LDS queues
+        }
+    }
+}
+
+static void launch_class(hipStream_t s, const BatchCast &B, int cls, int roots, unsigned blocks) {
+    if (cls == 0)
+        hipLaunchKernelGGL((k_intersect_frames<0, kRootsClosed>), dim3(blocks), dim3(64), 0, s, B);
+    else if (roots == kRootsMixed)
+        hipLaunchKernelGGL((k_intersect_frames<1, kRootsMixed>), dim3(blocks), dim3(64), 0, s, B);
+    else if (roots == kRootsSturm)
+        hipLaunchKernelGGL((k_intersect_frames<1, kRootsSturm>), dim3(blocks), dim3(64), 0, s, B);
+    else
+        hipLaunchKernelGGL((k_intersect_frames<1, kRootsClosed>), dim3(blocks), dim3(64), 0, s, B);
+}
+
+int intersect_frames_occupancy(int cls, int roots) {
+    int n = 0;
+    const void *k = cls == 0                ? (const void *)k_intersect_persistent<0, kRootsClosed>
+                    : roots == kRootsMixed   ? (const void *)k_intersect_persistent<1, kRootsMixed>
+                    : roots == kRootsSturm   ? (const void *)k_intersect_persistent<1, kRootsSturm>
+                                             : (const void *)k_intersect_persistent<1, kRootsClosed>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, 64, 0) != hipSuccess) return 0;
+    return n;
+}
+
+void launch_intersect_frames(hipStream_t s, const BatchCast &B, int cls, int roots, int grid) {
+    const int64_t n = B.items[cls];
+    if (n <= 0) return;
+    if (!B.persistent) {
+        launch_class(s, B, cls, roots, (unsigned)n);
+        return;
+    }
+    const dim3 blocks((unsigned)std::max<int64_t>(1, std::min<int64_t>(grid, n)));
+    if (cls == 0)
+        hipLaunchKernelGGL((k_intersect_persistent<0, kRootsClosed>), blocks, dim3(64), 0, s, B);
+    else if (roots == kRootsMixed)
+        hipLaunchKernelGGL((k_intersect_persistent<1, kRootsMixed>), blocks, dim3(64), 0, s, B);
+    else if (roots == kRootsSturm)
+        hipLaunchKernelGGL((k_intersect_persistent<1, kRootsSturm>), blocks, dim3(64), 0, s, B);
+    else
+        hipLaunchKernelGGL((k_intersect_persistent<1, kRootsClosed>), blocks, dim3(64), 0, s, B);
 }
 
 int64_t intersect_workgroups(const DevPatches &P, const DevRays &R) { return (int64_t)P.n * R.nslots; }

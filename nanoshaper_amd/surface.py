@@ -604,13 +604,30 @@ class Batch:
         self.grids[f] = grid
         self._keep[f] = patchset
 
-    def prepare(self, chains=0, **kw):
-        """Size every frame and capture the graph; chains <= 0 autotunes.
-        Returns the chain count used."""
+    CASTS = {"frames": 0, "joint": 1, "persistent": 2}
+
+    def prepare(self, chains=0, cast=None, workgroups=0, chunk=0, **kw):
+        """Size every frame and capture the graph.  cast None: per-frame
+        casts on `chains` chains, or (chains <= 0) the autotuned shape;
+        cast "frames", "joint" or "persistent" with chains > 0: that shape
+        (workgroups: persistent workgroups per class, 0 the resident limit;
+        chunk: items per queue take, 0 for 16).  Returns the chain count used."""
         kw.setdefault("accurate_triangulation", False)
         o = surf_opts(**kw)
-        self._chk(self._lib.nsgpu_batch_prepare(self._h, ctypes.byref(o), int(chains)), "batch_prepare")
+        if cast is None:
+            self._chk(self._lib.nsgpu_batch_prepare(self._h, ctypes.byref(o), int(chains)), "batch_prepare")
+        else:
+            sh = _lib.BatchShape(self.CASTS[cast], int(chains), int(workgroups), int(chunk))
+            self._chk(self._lib.nsgpu_batch_prepare_shape(self._h, ctypes.byref(o), ctypes.byref(sh)),
+                      "batch_prepare_shape")
         return int(self._lib.nsgpu_batch_chains(self._h))
+
+    def shape(self):
+        """Launch shape of the captured graph: {"cast", "chains", "workgroups", "chunk"}."""
+        sh = _lib.BatchShape()
+        self._chk(self._lib.nsgpu_batch_get_shape(self._h, ctypes.byref(sh)), "batch_get_shape")
+        names = {v: k for k, v in self.CASTS.items()}
+        return {"cast": names[sh.cast], "chains": sh.chains, "workgroups": sh.workgroups, "chunk": sh.chunk}
 
     def run(self):
         """One graph launch: (per-frame completion ms after launch, frames redone on the host path)."""

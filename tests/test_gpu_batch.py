@@ -19,13 +19,13 @@ def _inputs(frames, scale=2.0):
     return out
 
 
-def _reference(inputs):
+def _reference(inputs, precision=0):
     ref = []
     for grid, ps in inputs:
         dev = ns.DeviceSurface(0)
         dev.set_grid(grid)
         dev.set_patches(ps)
-        r = dev.get_surf(**OPTS)
+        r = dev.get_surf(precision=precision, **OPTS)
         ref.append((r, dev.epsmap(), dev.idebmap(), dev.status(), dev.cavities()))
         dev.close()
     return ref
@@ -108,4 +108,78 @@ def test_batch_captures_again_after_a_frame_changes(device_ok):
     # and again with nothing changed: the recaptured graph is reused
     b.run()
     _check(b, _reference(inputs))
+    b.close()
+
+
+@pytest.mark.parametrize("cast,chains,workgroups,chunk,precision", [("joint", 2, 0, 0, 0),
+                                                                    ("persistent", 3, 0, 0, 0),
+                                                                    ("persistent", 2, 7, 1, 0),
+                                                                    ("persistent", 2, 5, 13, 0),
+                                                                    ("persistent", 4, 0, 4, 1),
+                                                                    ("joint", 1, 0, 0, 2)])
+def test_joint_cast_matches_per_frame_get_surf(device_ok, cast, chains, workgroups, chunk, precision):
+    """One intersection launch per patch class over every frame (a workgroup
+    per item, or persistent workgroups taking chunks of items from a queue;
+    5 or 7 of them take hundreds of chunks each, 13-item chunks straddle the
+    frames' item ranges) gives the per-frame results on every replay, with
+    each torus solver."""
+    frames = ns.perturbed_frames(6, n_min=150, n_max=1500, seed=4)
+    inputs = _inputs(frames)
+    ref = _reference(inputs, precision)
+    b = ns.Batch(len(inputs))
+    for f, (grid, ps) in enumerate(inputs):
+        b.set_frame(f, grid, ps)
+    print("shape", cast, chains, workgroups, chunk, "precision", precision, flush=True)
+    b.prepare(chains=chains, cast=cast, workgroups=workgroups, chunk=chunk, precision=precision, **OPTS)
+    sh = b.shape()
+    assert sh["cast"] == cast and sh["chains"] == chains
+    if cast == "persistent":
+        assert sh["workgroups"] == (workgroups or sh["workgroups"]) > 0
+        assert sh["chunk"] == (chunk or 16)
+    for rep in range(3):
+        lat, redone = b.run()
+        assert redone == 0 and (lat > 0).all()
+        _check(b, ref)
+        b.run_loop()
+        _check(b, ref)
+    b.close()
+
+
+def test_joint_cast_after_a_frame_changes(device_ok):
+    # new patches for one frame: the frame descriptors are uploaded again
+    frames = ns.perturbed_frames(4, n_min=150, n_max=600, seed=6)
+    inputs = _inputs(frames)
+    b = ns.Batch(len(inputs))
+    for f, (grid, ps) in enumerate(inputs):
+        b.set_frame(f, grid, ps)
+    b.prepare(chains=2, cast="persistent", **OPTS)
+    b.run()
+    _check(b, _reference(inputs))
+    moved = ns.perturbed_frames(1, n_min=700, n_max=700, seed=9)[0]
+    inputs[2] = _inputs([moved])[0]
+    b.set_frame(2, *inputs[2])
+    print("run after frame 2 changed", flush=True)
+    lat, redone = b.run()
+    assert redone == 0
+    _check(b, _reference(inputs))
+    b.close()
+
+
+def test_autotuned_shape_and_joint_cast_refusal(device_ok):
+    frames = ns.perturbed_frames(4, n_min=150, n_max=600, seed=8)
+    inputs = _inputs(frames)
+    b = ns.Batch(len(inputs))
+    for f, (grid, ps) in enumerate(inputs):
+        b.set_frame(f, grid, ps)
+    b.prepare(chains=0, **OPTS)
+    assert b.shape()["cast"] in ("frames", "joint", "persistent")
+    b.run()
+    _check(b, _reference(inputs))
+    # a skin frame: only per-frame casts apply
+    jit, _ = ns.jitter(frames[1])
+    b.set_frame(1, inputs[1][0], ns.skin_patches(jit, 0.45))
+    with pytest.raises(ns.NsgpuError):
+        b.prepare(chains=2, cast="joint", **OPTS)
+    b.prepare(chains=0, **OPTS)
+    assert b.shape()["cast"] == "frames"
     b.close()
