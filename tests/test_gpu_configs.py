@@ -100,25 +100,33 @@ def test_batch_of_32_uneven_frames(device_ok, oracle_threads):
         g = ns.build_grid(xyzr, 2.0, 80.0)
         jit, _ = ns.jitter(xyzr)
         inputs.append((g, ns.ses_patches(jit, 1.4)))
-    b = ns.Batch(len(inputs))
-    for f, (g, ps) in enumerate(inputs):
-        b.set_frame(f, g, ps)
-    b.prepare(chains=0, **opts)
-    lat, redone = b.run()
-    assert (lat > 0).all()
+    # the autotuned graph, and the persistent multi-frame cast (workgroups
+    # taking 16-item chunks of every frame's patches from one queue)
+    batches = []
+    for shape in (dict(chains=0), dict(chains=16, cast="persistent", chunk=16)):
+        b = ns.Batch(len(inputs))
+        for f, (g, ps) in enumerate(inputs):
+            b.set_frame(f, g, ps)
+        b.prepare(**shape, **opts)
+        print("shape", b.shape(), flush=True)
+        lat, redone = b.run()
+        assert (lat > 0).all()
+        batches.append(b)
     for f, (g, ps) in enumerate(inputs):
         orc = oracle_threads.OracleSurface(g, ps)
         ro = orc.get_surf(**opts)
-        r = b.result(f)
-        for k in ("total_rays", "odd_rays", "total_hits", "candidates", "num_cavities", "max_hits_per_ray"):
-            assert r[k] == ro[k], (f, k, r[k], ro[k])
-        assert r["volume"] == pytest.approx(ro["volume"], rel=1e-12)
-        v = b.frame(f)
-        assert np.array_equal(v.epsmap(), orc.epsmap()), f
-        assert np.array_equal(v.idebmap(), orc.idebmap()), f
-        assert np.array_equal(v.status(), orc.status()), f
-        sa, fa = v.cavities()
-        sb, fb = orc.cavities()
-        assert np.array_equal(sa, sb) and np.array_equal(fa, fb), f
+        for b in batches:
+            r = b.result(f)
+            for k in ("total_rays", "odd_rays", "total_hits", "candidates", "num_cavities", "max_hits_per_ray"):
+                assert r[k] == ro[k], (f, k, r[k], ro[k])
+            assert r["volume"] == pytest.approx(ro["volume"], rel=1e-12)
+            v = b.frame(f)
+            assert np.array_equal(v.epsmap(), orc.epsmap()), f
+            assert np.array_equal(v.idebmap(), orc.idebmap()), f
+            assert np.array_equal(v.status(), orc.status()), f
+            sa, fa = v.cavities()
+            sb, fb = orc.cavities()
+            assert np.array_equal(sa, sb) and np.array_equal(fa, fb), f
         orc.close()
-    b.close()
+    for b in batches:
+        b.close()
