@@ -221,6 +221,28 @@ void nsgpu_patch_set_counts(const nsgpu_patch_set *ps, int64_t counts[5]);
  * :348-1210) as a patch set of NSGPU_PATCH_SKIN_* kinds.  xyzr are the
  * atoms after jitter; s is Skin_Surface_Parameter (0 < s < 1). */
 int nsgpu_skin_points(const double *xyzr, int32_t n, double s, double *wpts);
+/* The regular triangulation of those weighted points, in place of CGAL's
+ * Regular_triangulation_3 (SkinSurface.cpp:1271-1345): the six dummies as an
+ * octahedron, then the atoms inserted in Morton order with exact predicates;
+ * hidden atoms are left out, as CGAL leaves them.  The handle holds the
+ * finite tetrahedra and their neighbours in nsgpu_skin_build's layout
+ * (count, then read into tets[count*4] and nbrs[count*4]).  NSGPU_E_BUILD if
+ * an atom lies outside the dummies' hull or meets an exact degeneracy the
+ * insertion cannot star. */
+typedef struct nsgpu_regular_tri nsgpu_regular_tri;
+int nsgpu_skin_triangulate(const double *xyzr, int32_t n, double s, nsgpu_regular_tri **out);
+int32_t nsgpu_regular_tri_count(const nsgpu_regular_tri *tri);
+void nsgpu_regular_tri_read(const nsgpu_regular_tri *tri, int32_t *tets, int32_t *nbrs);
+void nsgpu_regular_tri_free(nsgpu_regular_tri *tri);
+/* The whole host stage of SkinSurface::buildSkinCGAL: nsgpu_skin_points,
+ * nsgpu_skin_triangulate and nsgpu_skin_build in one call. */
+int nsgpu_skin_patches(const double *xyzr, int32_t n, double s, nsgpu_patch_set **out);
+/* The triangulation's exact predicates (exposed for their tests):
+ * orient3d = sign of det[b - a; c - a; d - a]; power_test on weighted points
+ * (x, y, z, w) = the sign that is negative when p is in conflict with the
+ * positively oriented cell (a, b, c, d). */
+int nsgpu_orient3d(const double *a, const double *b, const double *c, const double *d);
+int nsgpu_power_test(const double *a, const double *b, const double *c, const double *d, const double *p);
 int nsgpu_skin_build(const double *xyzr, int32_t n, double s, const int32_t *tets, const int32_t *nbrs,
                      int32_t ntets, nsgpu_patch_set **out);
 

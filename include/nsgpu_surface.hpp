@@ -475,7 +475,8 @@ class BlobbySurface : public Surface {
 // the mixed cells the host builds (nsgpu_skin_build) out of the regular
 // triangulation of the weighted points nsgpu_skin_points gives.  The
 // reference takes that triangulation from CGAL's Regular_triangulation_3
-// (SkinSurface.cpp:1271-1811); here the caller plugs one in: it receives the
+// (SkinSurface.cpp:1271-1811); here the library's own triangulator does it
+// (nsgpu_skin_patches), unless the caller plugs in another: it receives the
 // (n + 6) weighted points (x, y, z, w) and returns the finite tetrahedra and
 // their neighbours (-1 on the hull), as nsgpu_skin_build takes them.
 class SkinSurface : public Surface {
@@ -494,26 +495,29 @@ class SkinSurface : public Surface {
     }
     bool build() override {
         if (!delphi || delphi->atoms.empty()) return false;
-        if (!triangulator()) {
-            err_ = "skin surfaces need a regular triangulator (SkinSurface::triangulator())";
-            return false;
-        }
         jittered_ = delphi->atoms;
         int32_t seed = 1234;  // SEED, tools.h:1827
         const int32_t n = (int32_t)(jittered_.size() / 4);
         nsgpu_jitter(jittered_.data(), n, randDisplacement, &seed);
-        std::vector<double> w((size_t)(n + 6) * 4);
-        if (nsgpu_skin_points(jittered_.data(), n, s, w.data()) != NSGPU_OK) return false;
-        std::vector<int32_t> tets, nbrs;
-        if (!triangulator()(w, tets, nbrs) || tets.size() % 4 || nbrs.size() != tets.size()) {
-            err_ = "the regular triangulation failed";
-            return false;
-        }
         if (ps_) nsgpu_patch_set_free(ps_);
         ps_ = nullptr;
-        if (nsgpu_skin_build(jittered_.data(), n, s, tets.data(), nbrs.data(), (int32_t)(tets.size() / 4), &ps_) !=
-            NSGPU_OK)
-            return false;
+        if (!triangulator()) {
+            if (nsgpu_skin_patches(jittered_.data(), n, s, &ps_) != NSGPU_OK) {
+                err_ = "the skin's regular triangulation or mixed cells failed";
+                return false;
+            }
+        } else {
+            std::vector<double> w((size_t)(n + 6) * 4);
+            if (nsgpu_skin_points(jittered_.data(), n, s, w.data()) != NSGPU_OK) return false;
+            std::vector<int32_t> tets, nbrs;
+            if (!triangulator()(w, tets, nbrs) || tets.size() % 4 || nbrs.size() != tets.size()) {
+                err_ = "the regular triangulation failed";
+                return false;
+            }
+            if (nsgpu_skin_build(jittered_.data(), n, s, tets.data(), nbrs.data(), (int32_t)(tets.size() / 4), &ps_) !=
+                NSGPU_OK)
+                return false;
+        }
         nsgpu_patch_set_view(ps_, &view_);
         uploaded_ = false;
         return true;

@@ -126,6 +126,13 @@ SIGNATURES = {
                                              ctypes.c_double]),
     "nsgpu_batch_prepare": (ctypes.c_int, [VP, ctypes.POINTER(SurfOpts), ctypes.c_int32]),
     "nsgpu_batch_chains": (ctypes.c_int32, [VP]),
+    "nsgpu_skin_triangulate": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_double, ctypes.POINTER(VP)]),
+    "nsgpu_regular_tri_count": (ctypes.c_int32, [VP]),
+    "nsgpu_regular_tri_read": (None, [VP, VP, VP]),
+    "nsgpu_regular_tri_free": (None, [VP]),
+    "nsgpu_skin_patches": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_double, ctypes.POINTER(VP)]),
+    "nsgpu_orient3d": (ctypes.c_int, [VP, VP, VP, VP]),
+    "nsgpu_power_test": (ctypes.c_int, [VP, VP, VP, VP, VP]),
     "nsgpu_batch_prepare_shape": (ctypes.c_int, [VP, ctypes.POINTER(SurfOpts), ctypes.POINTER(BatchShape)]),
     "nsgpu_batch_get_shape": (ctypes.c_int, [VP, ctypes.POINTER(BatchShape)]),
     "nsgpu_batch_run": (ctypes.c_int, [VP, VP, VP]),

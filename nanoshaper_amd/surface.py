@@ -169,24 +169,55 @@ def regular_triangulation(wpts):
     return tets, nbrs
 
 
-def skin_patches(xyzr, s=0.45):
+def skin_triangulation(xyzr, s=0.45):
+    """The library's regular triangulation of the skin's weighted points
+    (nsgpu_skin_triangulate): (tets [m, 4], nbrs [m, 4]) as nsgpu_skin_build
+    takes them."""
+    lib = _lib.load()
+    a = as_xyzr(xyzr)
+    h = ctypes.c_void_p()
+    rc = lib.nsgpu_skin_triangulate(_ptr(a), a.shape[0], float(s), ctypes.byref(h))
+    if rc != 0:
+        raise NsgpuError("skin_triangulate failed (%d)" % rc)
+    m = lib.nsgpu_regular_tri_count(h)
+    tets = np.zeros((m, 4), np.int32)
+    nbrs = np.zeros((m, 4), np.int32)
+    lib.nsgpu_regular_tri_read(h, _ptr(tets), _ptr(nbrs))
+    lib.nsgpu_regular_tri_free(h)
+    return tets, nbrs
+
+
+def skin_patches(xyzr, s=0.45, triangulation=None):
     """Skin mixed cells of (already jittered) atoms with shrink factor s
     (Skin_Surface_Parameter): the weighted points and dummies of the
-    reference, their regular triangulation (Qhull), then the cells."""
+    reference, their regular triangulation, then the cells.  triangulation:
+    None for the library's own (nsgpu_skin_patches), or (tets, nbrs) from
+    elsewhere (e.g. regular_triangulation's Qhull lifting)."""
     lib = _lib.load()
     a = as_xyzr(xyzr)
     n = a.shape[0]
-    w = np.zeros(((n + 6), 4))
-    if lib.nsgpu_skin_points(_ptr(a), n, float(s), _ptr(w)) != 0:
-        raise NsgpuError("skin_points failed")
-    tets, nbrs = regular_triangulation(w)
     h = ctypes.c_void_p()
-    rc = lib.nsgpu_skin_build(_ptr(a), n, float(s), _ptr(tets), _ptr(nbrs), tets.shape[0], ctypes.byref(h))
+    if triangulation is None:
+        rc = lib.nsgpu_skin_patches(_ptr(a), n, float(s), ctypes.byref(h))
+    else:
+        tets = np.ascontiguousarray(triangulation[0], dtype=np.int32)
+        nbrs = np.ascontiguousarray(triangulation[1], dtype=np.int32)
+        rc = lib.nsgpu_skin_build(_ptr(a), n, float(s), _ptr(tets), _ptr(nbrs), tets.shape[0], ctypes.byref(h))
     if rc != 0:
         if h:
             lib.nsgpu_patch_set_free(h)
-        raise NsgpuError("skin_build failed (%d)" % rc)
+        raise NsgpuError("skin patch construction failed (%d)" % rc)
     return PatchSet(h, 1.4)
+
+
+def skin_points(xyzr, s=0.45):
+    """The (n + 6) weighted points of the skin triangulation (nsgpu_skin_points)."""
+    lib = _lib.load()
+    a = as_xyzr(xyzr)
+    w = np.zeros(((a.shape[0] + 6), 4))
+    if lib.nsgpu_skin_points(_ptr(a), a.shape[0], float(s), _ptr(w)) != 0:
+        raise NsgpuError("skin_points failed")
+    return w
 
 
 def surf_opts(build_epsmap=True, build_status=True, accurate_triangulation=True, fill_cavities=True,

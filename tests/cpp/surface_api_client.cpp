@@ -168,6 +168,28 @@ static int bgp_mode(const char *dir) {
     return ds.nbgp > 0 ? 0 : 63;
 }
 
+// skin through the factory with the library's own regular triangulation
+static int skin_host_mode() {
+    nsgpu::ConfigFile cf;
+    cf.parse_line("Surface = skin");
+    cf.parse_line("Skin_Surface_Parameter = 0.45");
+    std::vector<double> xyzr(330 * 4);
+    double radius_used = 0;
+    const int n = nsgpu_generate_chain(330, 12.0, 0, xyzr.data(), &radius_used);
+    xyzr.resize(n * 4);
+    nsgpu::DelPhiShared ds(true, true);
+    ds.loadAtoms(xyzr);
+    if (!ds.buildGrid(2.0, 80.0)) return 2;
+    std::unique_ptr<nsgpu::Surface> surf(nsgpu::surfaceFactory().create(&cf, &ds));
+    if (!surf) return 3;
+    if (!surf->build()) {
+        std::printf("build failed: %s\n", surf->lastError());
+        return 5;
+    }
+    std::printf("skin patches %d\n", surf->getNumPatches());
+    return 0;
+}
+
 int main(int argc, char **argv) {
     if (argc > 1 && (std::strcmp(argv[1], "blobby") == 0 || std::strcmp(argv[1], "mesh") == 0))
         return mesh_modes(argc, argv);
@@ -175,6 +197,7 @@ int main(int argc, char **argv) {
     if (argc > 2 && std::strcmp(argv[1], "prm") == 0) return prm_mode(argv[2]);
     if (argc > 2 && std::strcmp(argv[1], "bgp") == 0) return bgp_mode(argv[2]);
     if (argc > 1 && std::strcmp(argv[1], "rescale") == 0) return rescale_mode();
+    if (argc > 1 && std::strcmp(argv[1], "skinhost") == 0) return skin_host_mode();
     const bool gpu = argc > 1 && std::strcmp(argv[1], "gpu") == 0;
     nsgpu::ConfigFile cf;
     cf.parse_line("Surface = ses   # surface type");

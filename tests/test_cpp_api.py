@@ -24,6 +24,17 @@ def test_cpp_client_host_stages(client):
     assert "atoms 330 grid 61" in out.stdout
 
 
+def test_cpp_skin_surface_builds_its_own_triangulation(client):
+    # "skin" through the factory with no triangulator plugged in: the
+    # library's regular triangulation, the same cells as the Python path
+    import nanoshaper_amd as ns
+    out = subprocess.run([client, "skinhost"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    xyzr, _ = ns.generate_chain(330, 12.0, seed=0)
+    jit, _ = ns.jitter(xyzr)
+    assert "skin patches %d" % len(ns.skin_patches(jit, 0.45)) in out.stdout
+
+
 @pytest.mark.gpu
 def test_cpp_client_full_path_matches_python(client, device_ok):
     import numpy as np
