@@ -153,6 +153,8 @@ typedef struct nsgpu_ctx nsgpu_ctx;
 /* -------------------- context --------------------------------------------- */
 /* One context = one DelPhiShared grid + one Surface on one device.           */
 /* Calls on one context are not thread safe; distinct contexts are.           */
+/* HIP devices visible to the library (0 without one). */
+int nsgpu_device_count(void);
 int nsgpu_create(int device, nsgpu_ctx **out);
 void nsgpu_destroy(nsgpu_ctx *ctx);
 const char *nsgpu_last_error(const nsgpu_ctx *ctx);
@@ -539,10 +541,10 @@ typedef struct nsgpu_batch_shape {
     int32_t chunk;
 } nsgpu_batch_shape;
 /* nsgpu_batch_prepare with a given shape, or (shape == NULL) the autotuned
- * one: every candidate (per-frame casts on 1, 2, 4, ... 64 chains; joint and
- * persistent casts on 16, 32, 64 chains, persistent at the resident limit
- * with chunks of 16 and 4 items and at half the limit with 16) is captured
- * and launched twice; the fastest is kept.
+ * one: every candidate (per-frame casts on 1, 2, 4, ... 64 chains; with at
+ * least 8 SES frames also joint and persistent casts on 16, 32, 64 chains,
+ * persistent at the resident limit with 16-item chunks) is captured and
+ * launched twice; the fastest is kept.
  * nsgpu_batch_prepare(batch, opts, 0) autotunes the same way. */
 int nsgpu_batch_prepare_shape(nsgpu_batch *batch, const nsgpu_surf_opts *opts, const nsgpu_batch_shape *shape);
 /* The shape of the captured graph (workgroups and chunk as launched). */

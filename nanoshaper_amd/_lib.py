@@ -66,6 +66,7 @@ class BatchShape(ctypes.Structure):
 
 SIGNATURES = {
     "nsgpu_abi_version": (ctypes.c_int, []),
+    "nsgpu_device_count": (ctypes.c_int, []),
     "nsgpu_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(VP)]),
     "nsgpu_destroy": (None, [VP]),
     "nsgpu_last_error": (ctypes.c_char_p, [VP]),
@@ -145,8 +146,22 @@ SIGNATURES = {
 _lib = None
 
 
+def hip_runtime():
+    """Paths of the HIP runtime libraries mapped into this process."""
+    try:
+        with open("/proc/self/maps") as f:
+            return sorted({l.split()[-1] for l in f if "libamdhip64" in l})
+    except OSError:
+        return []
+
+
 def load():
-    """Load libnsgpu.so from the package directory (raises if it is absent)."""
+    """Load libnsgpu.so from the package directory (raises if it is absent).
+
+    The library is built against ROCm's libamdhip64.so.7.  Torch's wheel
+    bundles another libamdhip64 with the same soname; a process that loads
+    torch's first serves this library with it (the loader reuses the soname).
+    Load this package before torch: a warning says when that did not happen."""
     global _lib
     if _lib is not None:
         return _lib
@@ -160,5 +175,10 @@ def load():
         fn.argtypes = args
     if lib.nsgpu_abi_version() != 1:
         raise RuntimeError("libnsgpu ABI version mismatch")
+    rt = hip_runtime()
+    if rt and not any(p.startswith("/opt/rocm") for p in rt):
+        import warnings
+        warnings.warn("libnsgpu.so runs on the HIP runtime loaded before it (%s), not the ROCm one it is "
+                      "built for: import nanoshaper_amd before torch" % ", ".join(rt), RuntimeWarning)
     _lib = lib
     return lib

@@ -267,6 +267,15 @@ static void mark(nsgpu_ctx *c, int stage, int end, hipStream_t st) {
 
 extern "C" {
 
+int nsgpu_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
 int nsgpu_create(int device, nsgpu_ctx **out) {
     if (!out) return NSGPU_E_ARG;
     *out = nullptr;
@@ -2414,9 +2423,9 @@ static int batch_eager(nsgpu_batch *b, const nsgpu_surf_opts *opts) {
 
 // Autotuner: captures each candidate shape, keeps the one with the shortest
 // of two launches.  Candidates: per-frame casts on 1, 2, 4, ... 64 chains;
-// with SES frames also the joint cast (a workgroup per item) and the
-// persistent one (the resident limit of workgroups taking 16 or 4 items at a
-// time, and half the limit taking 16), each on 16, 32 and 64 chains.
+// with at least 8 SES frames also the joint cast (a workgroup per item) and
+// the persistent one (the resident limit of workgroups taking 16 items at a
+// time), each on 16, 32 and 64 chains.
 static int autotune(nsgpu_batch *b) {
     const int n = (int)b->frames.size();
     std::vector<nsgpu_batch_shape> cands;
@@ -2424,15 +2433,13 @@ static int autotune(nsgpu_batch *b) {
         cands.push_back(nsgpu_batch_shape{NSGPU_BATCH_CAST_FRAMES, std::min(ch, n), 0, 0});
         if (ch >= n || ch >= 64) break;
     }
-    if (joint_cast_ok(b)) {
+    if (n >= 8 && joint_cast_ok(b)) {
         const int roots = torus_roots(b->frames[0]);
         const int full = std::max(1, intersect_frames_occupancy(1, roots)) * b->cus;
         for (int ch : {16, 32, 64}) {
             if (ch > 2 * n) break;
             cands.push_back(nsgpu_batch_shape{NSGPU_BATCH_CAST_JOINT, std::min(ch, n), 0, 0});
             cands.push_back(nsgpu_batch_shape{NSGPU_BATCH_CAST_PERSISTENT, std::min(ch, n), full, 16});
-            cands.push_back(nsgpu_batch_shape{NSGPU_BATCH_CAST_PERSISTENT, std::min(ch, n), full, 4});
-            cands.push_back(nsgpu_batch_shape{NSGPU_BATCH_CAST_PERSISTENT, std::min(ch, n), full / 2, 16});
         }
     }
     nsgpu_batch_shape best = cands[0];

@@ -13,9 +13,11 @@ def pytest_configure(config):
 
 
 def gpu_available():
+    # through the library itself: importing torch here would load torch's
+    # bundled HIP runtime first, and the library would then run on it
     try:
-        import torch
-        return torch.cuda.is_available()
+        from nanoshaper_amd import _lib
+        return _lib.load().nsgpu_device_count() > 0
     except Exception:
         return False
 

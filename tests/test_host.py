@@ -30,6 +30,19 @@ def test_library_exports_every_declared_symbol():
     assert lib.nsgpu_abi_version() == 1
 
 
+def test_library_runs_on_the_rocm_runtime_it_is_built_for():
+    # loaded first (as the test session and bench.py load it), the library
+    # maps ROCm's libamdhip64, not the copy torch's wheel bundles under the
+    # same soname
+    import subprocess
+    import sys
+    code = "import nanoshaper_amd as ns; ns._lib.load(); print(ns._lib.hip_runtime())"
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120,
+                         cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    assert out.returncode == 0, out.stderr
+    assert "/opt/rocm" in out.stdout and "torch" not in out.stdout, out.stdout
+
+
 def test_argument_errors_without_device():
     lib = _lib.load()
     assert lib.nsgpu_build_grid(None, 0, 2.0, 80.0, None) == -1

@@ -22,25 +22,50 @@ def _surface(n, radius, scale, seed):
     return g, ns.ses_patches(jit, 1.4)
 
 
+class _DeviceArray:
+    """A device array of the caller, made with the HIP runtime the library
+    itself runs on (already loaded by it), filled from a host array."""
+    H2D, D2H = 1, 2
+
+    def __init__(self, host):
+        path = [p for p in _lib.hip_runtime() if "/opt/rocm" in p][0]
+        self.hip = hip = ctypes.CDLL(path)
+        hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        hip.hipFree.argtypes = [ctypes.c_void_p]
+        self.host = np.ascontiguousarray(host)
+        self.ptr = ctypes.c_void_p()
+        assert hip.hipMalloc(ctypes.byref(self.ptr), self.host.nbytes) == 0
+        assert hip.hipMemcpy(self.ptr, self.host.ctypes.data, self.host.nbytes, self.H2D) == 0
+
+    def read(self):
+        out = np.empty_like(self.host)
+        assert self.hip.hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, self.D2H) == 0
+        return out
+
+    def free(self):
+        self.hip.hipFree(self.ptr)
+
+
 def _noalloc(dev, g, **kw):
-    import torch
     lib = _lib.load()
     nvox = g.nx * g.ny * g.nz
     o = surf_opts(**kw)
-    eps = torch.full((nvox * 3,), -7, dtype=torch.int32, device="cuda")
-    ideb = torch.full((nvox,), 9, dtype=torch.uint8, device="cuda")
-    st = torch.full((nvox,), -7, dtype=torch.int32, device="cuda")
-    torch.cuda.synchronize()
+    eps = _DeviceArray(np.full(nvox * 3, -7, np.int32))
+    ideb = _DeviceArray(np.full(nvox, 9, np.uint8))
+    st = _DeviceArray(np.full(nvox, -7, np.int32))
     r = SurfResult()
     rc = lib.nsgpu_get_surf_noalloc(dev._h, ctypes.byref(o), ctypes.byref(r),
-                                    ctypes.c_void_p(eps.data_ptr() if o.build_epsmap else 0),
-                                    ctypes.c_void_p(ideb.data_ptr() if o.build_epsmap else 0),
-                                    ctypes.c_void_p(st.data_ptr() if o.build_status else 0))
+                                    eps.ptr if o.build_epsmap else ctypes.c_void_p(0),
+                                    ideb.ptr if o.build_epsmap else ctypes.c_void_p(0),
+                                    st.ptr if o.build_status else ctypes.c_void_p(0))
     msg = lib.nsgpu_last_error(dev._h).decode() if rc else ""
     res = {f: getattr(r, f) for f in FIELDS}
     shape = (g.nz, g.ny, g.nx)
-    return rc, msg, res, eps.cpu().numpy().reshape(shape + (3,)), ideb.cpu().numpy().reshape(shape), \
-        st.cpu().numpy().reshape(shape)
+    out = (eps.read().reshape(shape + (3,)), ideb.read().reshape(shape), st.read().reshape(shape))
+    for a in (eps, ideb, st):
+        a.free()
+    return (rc, msg, res) + out
 
 
 def _check(dev, g, **kw):
