@@ -185,5 +185,5 @@ def test_autograd_function_backward(device_ok):
     dev.set_patches(ns.ses_patches(jit, 1.4))
     dev.set_atoms(jit)
     out = dev.surface_gradient()
-    assert float(v) == pytest.approx(out["volume"], rel=1e-14)
+    assert float(v.detach()) == pytest.approx(out["volume"], rel=1e-14)
     assert np.allclose(x.grad.numpy(), 2.0 * out["dvolume"] + 0.5 * out["darea"], rtol=1e-12, atol=1e-12)
