@@ -233,6 +233,11 @@ int nsgpu_skin_points(const double *xyzr, int32_t n, double s, double *wpts);
  * insertion cannot star. */
 typedef struct nsgpu_regular_tri nsgpu_regular_tri;
 int nsgpu_skin_triangulate(const double *xyzr, int32_t n, double s, nsgpu_regular_tri **out);
+/* The same for any weighted points wpts[n*4] (x, y, z, w): six dummies of
+ * weight -1 around them (indices n .. n+5 in the tetrahedra) close the hull.
+ * With weights (r + probe)^2 the tetrahedra carry the weighted alpha complex
+ * of ConnollySurface (CGAL's Fixed_alpha_shape_3 at alpha 0). */
+int nsgpu_regular_triangulate(const double *wpts, int32_t n, nsgpu_regular_tri **out);
 int32_t nsgpu_regular_tri_count(const nsgpu_regular_tri *tri);
 void nsgpu_regular_tri_read(const nsgpu_regular_tri *tri, int32_t *tets, int32_t *nbrs);
 void nsgpu_regular_tri_free(nsgpu_regular_tri *tri);

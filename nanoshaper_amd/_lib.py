@@ -128,6 +128,7 @@ SIGNATURES = {
     "nsgpu_batch_prepare": (ctypes.c_int, [VP, ctypes.POINTER(SurfOpts), ctypes.c_int32]),
     "nsgpu_batch_chains": (ctypes.c_int32, [VP]),
     "nsgpu_skin_triangulate": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_double, ctypes.POINTER(VP)]),
+    "nsgpu_regular_triangulate": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.POINTER(VP)]),
     "nsgpu_regular_tri_count": (ctypes.c_int32, [VP]),
     "nsgpu_regular_tri_read": (None, [VP, VP, VP]),
     "nsgpu_regular_tri_free": (None, [VP]),

@@ -505,12 +505,13 @@ int nsgpu_power_test(const double *a, const double *b, const double *c, const do
     return power_test(a, b, c, d, p);
 }
 
-int nsgpu_skin_triangulate(const double *xyzr, int32_t n, double s, nsgpu_regular_tri **out) {
-    if (!xyzr || n <= 0 || !out || !(s > 0 && s < 1)) return NSGPU_E_ARG;
-    *out = nullptr;
-    std::vector<double> w((size_t)(n + 6) * 4);
-    int rc = nsgpu_skin_points(xyzr, n, s, w.data());
-    if (rc) return rc;
+}  // extern "C"
+
+namespace {
+
+// The regular triangulation of w[0 .. n + 6) (4 doubles per point) whose last
+// six points are the dummies -x, +x, -y, +y, -z, +z around the others.
+int triangulate(const std::vector<double> &w, int n, nsgpu_regular_tri **out) {
     for (double x : w)
         if (!std::isfinite(x)) return NSGPU_E_ARG;
     Tri T;
@@ -538,6 +539,36 @@ int nsgpu_skin_triangulate(const double *xyzr, int32_t n, double s, nsgpu_regula
     }
     *out = r;
     return NSGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nsgpu_skin_triangulate(const double *xyzr, int32_t n, double s, nsgpu_regular_tri **out) {
+    if (!xyzr || n <= 0 || !out || !(s > 0 && s < 1)) return NSGPU_E_ARG;
+    *out = nullptr;
+    std::vector<double> w((size_t)(n + 6) * 4);
+    const int rc = nsgpu_skin_points(xyzr, n, s, w.data());
+    if (rc) return rc;
+    return triangulate(w, n, out);
+}
+
+int nsgpu_regular_triangulate(const double *wpts, int32_t n, nsgpu_regular_tri **out) {
+    if (!wpts || n <= 0 || !out) return NSGPU_E_ARG;
+    *out = nullptr;
+    // six dummies of weight -1 two extents outside the box of the points'
+    // spheres (radius sqrt(w), 0 for a negative weight), as for the skin
+    std::vector<double> xyzr((size_t)n * 4);
+    for (int i = 0; i < n; ++i) {
+        for (int q = 0; q < 3; ++q) xyzr[4 * (size_t)i + q] = wpts[4 * (size_t)i + q];
+        xyzr[4 * (size_t)i + 3] = std::sqrt(std::max(0.0, wpts[4 * (size_t)i + 3]));
+    }
+    std::vector<double> w((size_t)(n + 6) * 4);
+    const int rc = nsgpu_skin_points(xyzr.data(), n, 0.5, w.data());
+    if (rc) return rc;
+    std::memcpy(w.data(), wpts, (size_t)n * 4 * sizeof(double));  // the caller's weights
+    return triangulate(w, n, out);
 }
 
 int32_t nsgpu_regular_tri_count(const nsgpu_regular_tri *t) { return t ? (int32_t)(t->tets.size() / 4) : 0; }

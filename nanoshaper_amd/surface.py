@@ -169,6 +169,15 @@ def regular_triangulation(wpts):
     return tets, nbrs
 
 
+def _read_tri(lib, h):
+    m = lib.nsgpu_regular_tri_count(h)
+    tets = np.zeros((m, 4), np.int32)
+    nbrs = np.zeros((m, 4), np.int32)
+    lib.nsgpu_regular_tri_read(h, _ptr(tets), _ptr(nbrs))
+    lib.nsgpu_regular_tri_free(h)
+    return tets, nbrs
+
+
 def skin_triangulation(xyzr, s=0.45):
     """The library's regular triangulation of the skin's weighted points
     (nsgpu_skin_triangulate): (tets [m, 4], nbrs [m, 4]) as nsgpu_skin_build
@@ -179,12 +188,20 @@ def skin_triangulation(xyzr, s=0.45):
     rc = lib.nsgpu_skin_triangulate(_ptr(a), a.shape[0], float(s), ctypes.byref(h))
     if rc != 0:
         raise NsgpuError("skin_triangulate failed (%d)" % rc)
-    m = lib.nsgpu_regular_tri_count(h)
-    tets = np.zeros((m, 4), np.int32)
-    nbrs = np.zeros((m, 4), np.int32)
-    lib.nsgpu_regular_tri_read(h, _ptr(tets), _ptr(nbrs))
-    lib.nsgpu_regular_tri_free(h)
-    return tets, nbrs
+    return _read_tri(lib, h)
+
+
+def weighted_triangulation(wpts):
+    """The library's regular triangulation of weighted points (x, y, z, w)
+    (nsgpu_regular_triangulate); indices n .. n+5 in the tetrahedra are the
+    six dummies that close the hull."""
+    lib = _lib.load()
+    p = np.ascontiguousarray(wpts, dtype=np.float64).reshape(-1, 4)
+    h = ctypes.c_void_p()
+    rc = lib.nsgpu_regular_triangulate(_ptr(p), p.shape[0], ctypes.byref(h))
+    if rc != 0:
+        raise NsgpuError("regular_triangulate failed (%d)" % rc)
+    return _read_tri(lib, h)
 
 
 def skin_patches(xyzr, s=0.45, triangulation=None):

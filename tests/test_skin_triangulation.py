@@ -174,3 +174,34 @@ def test_skin_cells_from_either_triangulation():
                       for k, ids in zip(ps.kind[keep].tolist(), ps.atom[keep].tolist()))
 
     assert cells(a) == cells(b)
+
+
+def test_weighted_points_of_any_weight():
+    # nsgpu_regular_triangulate on points whose weights include negatives
+    # and zero: its atom-only tetrahedra are tetrahedra of Qhull's lower hull
+    # (the six dummies far out may only replace some near the hull), each
+    # with an empty orthosphere, and every point not dominated by the others
+    # is a vertex
+    rng = np.random.default_rng(11)
+    p = np.column_stack([rng.uniform(-6, 6, (300, 3)), rng.uniform(-1.0, 4.0, 300)])
+    p[::17, 3] = 0.0
+    n = len(p)
+    tets, nbrs = ns.weighted_triangulation(p)
+    inner = tets[(tets < n).all(axis=1)]
+    mine = {tuple(sorted(t)) for t in inner.tolist()}
+    qt, _ = ns.regular_triangulation(p)
+    qhull = {tuple(sorted(t)) for t in qt.tolist()}
+    assert mine <= qhull and len(mine) > 0.8 * len(qhull)
+    assert set(tets.ravel().tolist()) >= set(qt.ravel().tolist())
+    P, W = p[:, :3], p[:, 3]
+    for v in inner:
+        A = 2 * (P[v[1:]] - P[v[0]])
+        b = (P[v[1:]] ** 2).sum(1) - W[v[1:]] - ((P[v[0]] ** 2).sum() - W[v[0]])
+        c = np.linalg.solve(A, b)
+        pw = ((P - c) ** 2).sum(1) - W
+        assert (pw >= pw[v[0]] - 1e-7 * max(1.0, abs(pw[v[0]]))).all()
+    for t in range(len(tets)):
+        for k in range(4):
+            u = nbrs[t, k]
+            if u >= 0:
+                assert len(set(tets[t]) & set(tets[u])) == 3
