@@ -125,26 +125,25 @@ struct nsgpu_ctx {
     void *hpin = nullptr;
     size_t hpin_cap = 0;
     DBuf dgather;  // device side of those reads (launch_gather)
-    DBuf aedge;    // per active cube: its twelve edge vertex indices
+    DBuf amask;    // per active cube: the mid points it adds (bits 0, 1, 2: edges 5, 6, 10)
     DBuf cubeidx;  // smoothing: node -> active cube index (validated, never cleared)
     bool capturing = false;  // enqueueing into a graph: no stage events
     std::vector<double> cav_size;
     std::vector<uint8_t> cav_filled;
     bool cav_on_device = false;  // cav_size/cav_filled not yet copied from the device
     // vertices and mesh
-    DBuf idx, vcount, voffs, scan_scratch, verts, tris, tarea;
-    // idx: vertex index slots for grids of idx_nvox nodes (-1: state unknown,
-    // fill it); the last pass tagged ids [0, idx_dirty) with idx_gen
+    // vcount, voffs: analytic vertices per edge ray and their exclusive scan;
+    // vbits, vpre: each edge ray's node bitmap of vertices and its per-word
+    // prefix counts (the marching cubes find an edge's vertex from voffs and these)
+    DBuf vcount, voffs, vbits, vpre, scan_scratch, verts, tris, tarea;
     DBuf vslot;
-    int64_t idx_nvox = -1, idx_dirty = 0;
-    uint32_t idx_gen = 0;
     // the analytic vertex count of the last assemble is still on the device
-    // (dnv[0]); n_analytic, nverts and idx_dirty wait for resolve_nv
+    // (dnv[0]); n_analytic and nverts wait for resolve_nv
     DBuf dnv;
     bool nv_pending = false;
     int64_t nv_cap = 0;
-    // getSurf already classified the cubes for triangulate (segmask, vcount,
-    // voffs hold the result) and read the active-cube count
+    // getSurf already classified the cubes for triangulate (segmask, segcnt,
+    // segoff hold the result) and read the active-cube count
     bool pre_mc = false;
     int64_t pre_nact = 0;
     // vertex provenance for the normals: depth of each analytic vertex, cube
@@ -153,7 +152,7 @@ struct nsgpu_ctx {
     bool hits_have_patches = false;  // the ray lists of the last trace carry patch ids
     bool normals_smoothed = false;   // normals holds the smoothed normals of the current mesh
     bool want_normals = false;       // keep patch ids for the vertex normals (Compute_Vertex_Normals)
-    DBuf segmask, anode, acase, anv, ant, avoff, atoff;
+    DBuf segmask, segcnt, segoff, anode, acase, anv, ant, avoff, atoff;
     // triangle-mesh surface (MeshSurface, BlobbySurface): vertices as loaded,
     // triangles, ray ranges of the auxiliary 2D grid; the blobby build also
     // keeps its field, its smoothed vertices before the OFF text round trip and
@@ -311,9 +310,9 @@ template <class F>
 static void for_each_buf(nsgpu_ctx *c, F fn) {
     DBuf *all[] = {&c->gx, &c->gy, &c->gz, &c->pkind, &c->poff, &c->pcnt, &c->pflags, &c->pbsph, &c->pdata, &c->porder, &c->patom, &c->atoms, &c->rpid, &c->gradv, &c->grada, &c->asums,
                    &c->rcnt, &c->rt, &c->rvol, &c->partial, &c->stats, &c->cand, &c->cov, &c->covst, &c->eps, &c->ideb, &c->status,
-                   &c->bits, &c->crx0, &c->crx1, &c->cpar, &c->crowcnt, &c->croff, &c->nroots, &c->rootlist, &c->bkey, &c->ccount, &c->clabel, &c->cfilled, &c->dcavsize, &c->dcavfilled, &c->hostdone, &c->dres, &c->dnv, &c->dgather, &c->aedge, &c->clist, &c->clrow, &c->cubeidx,
-                   &c->idx, &c->vslot, &c->vhit, &c->vorig, &c->normals, &c->vcount, &c->voffs, &c->scan_scratch, &c->verts, &c->tris, &c->tarea,
-                   &c->segmask, &c->anode, &c->acase, &c->anv, &c->ant, &c->avoff, &c->atoff, &c->mverts, &c->mtris,
+                   &c->bits, &c->crx0, &c->crx1, &c->cpar, &c->crowcnt, &c->croff, &c->nroots, &c->rootlist, &c->bkey, &c->ccount, &c->clabel, &c->cfilled, &c->dcavsize, &c->dcavfilled, &c->hostdone, &c->dres, &c->dnv, &c->dgather, &c->amask, &c->clist, &c->clrow, &c->cubeidx,
+                   &c->vslot, &c->vhit, &c->vorig, &c->normals, &c->vcount, &c->voffs, &c->vbits, &c->vpre, &c->scan_scratch, &c->verts, &c->tris, &c->tarea,
+                   &c->segmask, &c->segcnt, &c->segoff, &c->anode, &c->acase, &c->anv, &c->ant, &c->avoff, &c->atoff, &c->mverts, &c->mtris,
                    &c->mrr, &c->mraw, &c->field, &c->fbits, &c->batoms, &c->bbox, &c->bcount, &c->boff, &c->bcur,
                    &c->blist, &c->flag, &c->cgcount, &c->cgoff, &c->cgcursor, &c->cglist, &c->crorig,
                    &c->crcoord, &c->crcount, &c->crhits, &c->crnorm, &c->bgbits, &c->bgwcount, &c->bgwoff,
@@ -1094,7 +1093,6 @@ static int read_total(nsgpu_ctx *c, const int *counts, const int *offs, int64_t 
 static int set_nv(nsgpu_ctx *c, int64_t nv) {
     c->nv_pending = false;
     if (nv > c->nv_cap) return fail(c, NSGPU_E_STATE, "vertex count above its bound");
-    c->idx_dirty = nv;
     c->n_analytic = nv;
     c->nverts = nv;
     return NSGPU_OK;
@@ -1123,35 +1121,17 @@ static int read_partial_sum(nsgpu_ctx *c, const double *partial, int64_t per, hi
     return NSGPU_OK;
 }
 
-// A vertex pass tags its ids with a generation above every id of the passes
-// before it instead of clearing their slots; the slots are refilled only when
-// the buffer or the grid changes or the generation would pass 2^31 - 1.
-static void idx_begin(nsgpu_ctx *c, hipStream_t st, int64_t nvox, const void *old_idx) {
-    const uint64_t next = (uint64_t)c->idx_gen + (uint64_t)c->idx_dirty;
-    if (c->idx.p != old_idx || c->idx_nvox != nvox || next >= 0x7fffffffull) {
-        launch_fill32(st, c->idx.p, 0xffffffffu, nvox * 3);
-        c->idx_gen = 0;
-    } else {
-        c->idx_gen = (uint32_t)next;
-    }
-    c->idx_nvox = nvox;
-    c->idx_dirty = 0;
-}
-
 static int assemble(nsgpu_ctx *c, const int eslots[3]) {
     const DevGrid g = dev_grid(c);
     const DevRays R = dev_rays(c);
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
     const int64_t nrays = 3 * (int64_t)R.N * R.N;
     const int64_t nmax = nrays;
-    int rc = resolve_nv(c);  // idx_dirty of the previous vertices
+    int rc = resolve_nv(c);  // the previous pass's count, if still pending
     if (rc) return rc;
-    const void *old_idx = c->idx.p;
-    HIP_TRY(c, c->idx.ensure((size_t)nvox * 3 * 4));
     HIP_TRY(c, c->vcount.ensure((size_t)nmax * 4));
     HIP_TRY(c, c->voffs.ensure((size_t)nmax * 4));
     HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(nmax) * 4));
-    idx_begin(c, c->stream, nvox, old_idx);
     launch_vtx_count(c->stream, g, R, eslots, c->bits.as<uint32_t>(), c->vcount.as<int>());
     HIP_TRY(c, c->dnv.ensure(8));
     launch_scan(c->stream, c->vcount.as<int>(), nrays, c->voffs.as<int>(), c->scan_scratch.as<int>(),
@@ -1166,10 +1146,12 @@ static int assemble(nsgpu_ctx *c, const int eslots[3]) {
     HIP_TRY(c, c->verts.ensure((size_t)cap * 24));
     HIP_TRY(c, c->vslot.ensure((size_t)cap * 8));
     HIP_TRY(c, c->vhit.ensure((size_t)cap * 8));
+    const int64_t nwords = nrays * ((R.N + 63) / 64);
+    HIP_TRY(c, c->vbits.ensure((size_t)nwords * 8));
+    HIP_TRY(c, c->vpre.ensure((size_t)nwords * 4));
     launch_vtx_write(c->stream, g, R, eslots, c->bits.as<uint32_t>(), c->voffs.as<int>(), c->verts.as<double>(),
-                     c->idx.as<int>(), c->idx_gen, c->vslot.as<int64_t>(), c->vhit.as<int64_t>(), cap);
+                     c->vbits.as<uint64_t>(), c->vpre.as<int>(), c->vslot.as<int64_t>(), c->vhit.as<int64_t>(), cap);
     HIP_TRY(c, hipGetLastError());
-    c->idx_dirty = 0;
     c->nv_cap = cap;
     c->nv_pending = true;
     return NSGPU_OK;
@@ -1183,22 +1165,22 @@ static int enqueue_classify(nsgpu_ctx *c) {
     const DevGrid g = dev_grid(c);
     const int64_t nseg = mc_num_segments(g);
     HIP_TRY(c, c->segmask.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
-    HIP_TRY(c, c->vcount.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
-    HIP_TRY(c, c->voffs.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
+    HIP_TRY(c, c->segcnt.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
+    HIP_TRY(c, c->segoff.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
     HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(std::max<int64_t>(nseg, 1)) * 4));
     void *hp = nullptr;
     int rc = pinned(c, 32, &hp);
     if (rc) return rc;
     HIP_TRY(c, c->dgather.ensure(32));
-    launch_mc_classify(c->stream, g, c->bits.as<uint32_t>(), c->segmask.as<uint32_t>(), c->vcount.as<int>());
-    launch_scan(c->stream, c->vcount.as<int>(), nseg, c->voffs.as<int>(), c->scan_scratch.as<int>());
+    launch_mc_classify(c->stream, g, c->bits.as<uint32_t>(), c->segmask.as<uint32_t>(), c->segcnt.as<int>());
+    launch_scan(c->stream, c->segcnt.as<int>(), nseg, c->segoff.as<int>(), c->scan_scratch.as<int>());
     GatherSrc gs{};
     gs.src[0] = c->dnv.as<int>();
     gs.src[1] = c->dnv.as<int>() + 1;
     gs.n = 2;
     if (nseg > 0) {
-        gs.src[2] = c->vcount.as<int>() + nseg - 1;
-        gs.src[3] = c->voffs.as<int>() + nseg - 1;
+        gs.src[2] = c->segcnt.as<int>() + nseg - 1;
+        gs.src[3] = c->segoff.as<int>() + nseg - 1;
         gs.n = 4;
     }
     launch_gather(c->stream, gs, c->dgather.as<int>());
@@ -1432,16 +1414,16 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
     int rc = NSGPU_OK;
     if (!c->pre_mc) {
     HIP_TRY(c, c->segmask.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
-    HIP_TRY(c, c->vcount.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
-    HIP_TRY(c, c->voffs.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
+    HIP_TRY(c, c->segcnt.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
+    HIP_TRY(c, c->segoff.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
     HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(std::max<int64_t>(nseg, 1)) * 4));
-    launch_mc_classify(c->stream, g, c->bits.as<uint32_t>(), c->segmask.as<uint32_t>(), c->vcount.as<int>());
-    launch_scan(c->stream, c->vcount.as<int>(), nseg, c->voffs.as<int>(), c->scan_scratch.as<int>());
+    launch_mc_classify(c->stream, g, c->bits.as<uint32_t>(), c->segmask.as<uint32_t>(), c->segcnt.as<int>());
+    launch_scan(c->stream, c->segcnt.as<int>(), nseg, c->segoff.as<int>(), c->scan_scratch.as<int>());
     HIP_TRY(c, hipGetLastError());
     // the active-cube count, with the analytic vertex count if still pending
     {
-        const int *cnts[2] = {c->vcount.as<int>(), c->dnv.as<int>()};
-        const int *offs[2] = {c->voffs.as<int>(), c->dnv.as<int>() + 1};
+        const int *cnts[2] = {c->segcnt.as<int>(), c->dnv.as<int>()};
+        const int *offs[2] = {c->segoff.as<int>(), c->dnv.as<int>() + 1};
         const int64_t ns[2] = {nseg, c->nv_pending ? 1 : 0};
         rc = read_totals(c, 2, cnts, offs, ns, tot);
         if (rc) return rc;
@@ -1462,12 +1444,22 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
     HIP_TRY(c, c->avoff.ensure(na * 4));
     HIP_TRY(c, c->atoff.ensure(na * 4));
     HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(std::max<int64_t>(std::max<int64_t>(nseg, nact), 1)) * 4));
-    launch_mc_compact(c->stream, g, c->bits.as<uint32_t>(), c->segmask.as<uint32_t>(), c->voffs.as<int>(),
+    launch_mc_compact(c->stream, g, c->bits.as<uint32_t>(), c->segmask.as<uint32_t>(), c->segoff.as<int>(),
                       c->anode.as<int>(), c->acase.as<uint8_t>());
     // 2. per active cube: mid-point vertices added and triangles emitted
-    HIP_TRY(c, c->aedge.ensure(na * 48));
-    launch_mc_counts(c->stream, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->idx.as<int>(),
-                     c->idx_gen, c->anv.as<int>(), c->ant.as<int>(), c->aedge.as<int>());
+    HIP_TRY(c, c->amask.ensure(na));
+    McVertexMap vm{};
+    vm.offs = c->voffs.as<int>();
+    vm.vbits = c->vbits.as<uint64_t>();
+    vm.vpre = c->vpre.as<int>();
+    vm.N = dev_rays(c).N;
+    vm.segmask = c->segmask.as<uint32_t>();
+    vm.segoff = c->segoff.as<int>();
+    vm.amask = c->amask.as<uint8_t>();
+    vm.avoff = c->avoff.as<int>();
+    vm.base = (int)c->n_analytic;
+    launch_mc_counts(c->stream, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), vm, c->amask.as<uint8_t>(),
+                     c->anv.as<int>(), c->ant.as<int>());
     launch_scan(c->stream, c->anv.as<int>(), nact, c->avoff.as<int>(), c->scan_scratch.as<int>());
     launch_scan(c->stream, c->ant.as<int>(), nact, c->atoff.as<int>(), c->scan_scratch.as<int>());
     HIP_TRY(c, hipGetLastError());
@@ -1498,15 +1490,13 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
     }
     // 3. write the mid points, then the triangles that may reference them
     HIP_TRY(c, c->vorig.ensure((size_t)std::max<int64_t>(nadd_cap, 1) * 4));
-    launch_mc_vwrite(c->stream, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->aedge.as<int>(),
-                     c->anv.as<int>(), c->idx.as<int>(), c->idx_gen, c->avoff.as<int>(), (int)c->n_analytic, c->verts.as<double>(),
+    launch_mc_vwrite(c->stream, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), vm, c->verts.as<double>(),
                      c->vslot.as<int64_t>(), c->vorig.as<int>());
     c->nact = nact;
     HIP_TRY(c, c->tris.ensure((size_t)std::max<int64_t>(nt_cap, 1) * 12));
     HIP_TRY(c, c->tarea.ensure((size_t)std::max<int64_t>(nt_cap, 1) * 8));
-    launch_mc_twrite(c->stream, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->aedge.as<int>(),
-                     c->idx.as<int>(), c->idx_gen, c->atoff.as<int>(), c->verts.as<double>(), c->tris.as<int>(),
-                     c->tarea.as<double>());
+    launch_mc_twrite(c->stream, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), vm, c->atoff.as<int>(),
+                     c->verts.as<double>(), c->tris.as<int>(), c->tarea.as<double>());
     HIP_TRY(c, hipGetLastError());
     const int64_t per = sum_partials_count(nt_cap);
     HIP_TRY(c, c->partial.ensure((size_t)std::max<int64_t>(per, 1) * 8));
@@ -1538,7 +1528,6 @@ int nsgpu_triangulate(nsgpu_ctx *c, nsgpu_mesh_info *info) {
         HIP_TRY(c, hipStreamSynchronize(c->stream));
     }
     const int64_t nv = c->n_analytic + nadd;
-    c->idx_dirty = nv;
     c->nverts = nv;
     c->ntris = nt;
     c->area = area;
@@ -1720,22 +1709,19 @@ int nsgpu_blobby_build(nsgpu_ctx *c, const double *xyzr, int32_t n, double B, do
     // 2. marching cubes on the field at iso 1.0 (triangulateSurface(false, false, 1.0), BlobbySurface.cpp:186)
     HIP_TRY(c, c->fbits.ensure((size_t)((nvox + 31) / 32) * 4));
     launch_field_bits(st, a.field, nvox, 1.0, c->fbits.as<uint32_t>());
-    rc = resolve_nv(c);  // idx_dirty of the previous vertices
+    rc = resolve_nv(c);  // the previous pass's count, if still pending
     if (rc) return rc;
-    const void *old_idx = c->idx.p;
-    HIP_TRY(c, c->idx.ensure((size_t)nvox * 3 * 4));
-    idx_begin(c, st, nvox, old_idx);
     c->have_trace = c->have_surf = c->have_vertices = c->have_mesh = false;
     const int64_t nseg = mc_num_segments(g);
     HIP_TRY(c, c->segmask.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
-    HIP_TRY(c, c->vcount.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
-    HIP_TRY(c, c->voffs.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
+    HIP_TRY(c, c->segcnt.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
+    HIP_TRY(c, c->segoff.ensure((size_t)std::max<int64_t>(nseg, 1) * 4));
     HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(std::max<int64_t>(nseg, 1)) * 4));
-    launch_mc_classify(st, g, c->fbits.as<uint32_t>(), c->segmask.as<uint32_t>(), c->vcount.as<int>());
-    launch_scan(st, c->vcount.as<int>(), nseg, c->voffs.as<int>(), c->scan_scratch.as<int>());
+    launch_mc_classify(st, g, c->fbits.as<uint32_t>(), c->segmask.as<uint32_t>(), c->segcnt.as<int>());
+    launch_scan(st, c->segcnt.as<int>(), nseg, c->segoff.as<int>(), c->scan_scratch.as<int>());
     HIP_TRY(c, hipGetLastError());
     int64_t nact = 0;
-    rc = read_total(c, c->vcount.as<int>(), c->voffs.as<int>(), nseg, &nact);
+    rc = read_total(c, c->segcnt.as<int>(), c->segoff.as<int>(), nseg, &nact);
     if (rc) return rc;
     const size_t na = (size_t)std::max<int64_t>(nact, 1);
     HIP_TRY(c, c->anode.ensure(na * 4));
@@ -1745,11 +1731,18 @@ int nsgpu_blobby_build(nsgpu_ctx *c, const double *xyzr, int32_t n, double B, do
     HIP_TRY(c, c->avoff.ensure(na * 4));
     HIP_TRY(c, c->atoff.ensure(na * 4));
     HIP_TRY(c, c->scan_scratch.ensure((size_t)scan_scratch_ints(std::max<int64_t>(std::max<int64_t>(nseg, nact), 1)) * 4));
-    launch_mc_compact(st, g, c->fbits.as<uint32_t>(), c->segmask.as<uint32_t>(), c->voffs.as<int>(), c->anode.as<int>(),
+    launch_mc_compact(st, g, c->fbits.as<uint32_t>(), c->segmask.as<uint32_t>(), c->segoff.as<int>(), c->anode.as<int>(),
                       c->acase.as<uint8_t>());
-    HIP_TRY(c, c->aedge.ensure(na * 48));
-    launch_mc_counts(st, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->idx.as<int>(), c->idx_gen,
-                     c->anv.as<int>(), c->ant.as<int>(), c->aedge.as<int>());
+    HIP_TRY(c, c->amask.ensure(na));
+    // a scalar field has no analytic vertices: every vertex is a mid point
+    McVertexMap vm{};
+    vm.segmask = c->segmask.as<uint32_t>();
+    vm.segoff = c->segoff.as<int>();
+    vm.amask = c->amask.as<uint8_t>();
+    vm.avoff = c->avoff.as<int>();
+    vm.base = 0;
+    launch_mc_counts(st, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), vm, c->amask.as<uint8_t>(),
+                     c->anv.as<int>(), c->ant.as<int>());
     launch_scan(st, c->anv.as<int>(), nact, c->avoff.as<int>(), c->scan_scratch.as<int>());
     launch_scan(st, c->ant.as<int>(), nact, c->atoff.as<int>(), c->scan_scratch.as<int>());
     HIP_TRY(c, hipGetLastError());
@@ -1766,13 +1759,12 @@ int nsgpu_blobby_build(nsgpu_ctx *c, const double *xyzr, int32_t n, double B, do
     HIP_TRY(c, c->verts.ensure((size_t)std::max<int64_t>(nv, 1) * 24));
     HIP_TRY(c, c->vslot.ensure((size_t)std::max<int64_t>(nv, 1) * 8));
     HIP_TRY(c, c->vorig.ensure((size_t)std::max<int64_t>(nv, 1) * 4));
-    launch_mc_vwrite(st, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->aedge.as<int>(), c->anv.as<int>(),
-                     c->idx.as<int>(), c->idx_gen, c->avoff.as<int>(), 0, c->verts.as<double>(), c->vslot.as<int64_t>(), c->vorig.as<int>(), a.field, 1.0);
-    c->idx_dirty = nv;
+    launch_mc_vwrite(st, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), vm, c->verts.as<double>(),
+                     c->vslot.as<int64_t>(), c->vorig.as<int>(), a.field, 1.0);
     HIP_TRY(c, c->tris.ensure((size_t)std::max<int64_t>(nt, 1) * 12));
     HIP_TRY(c, c->tarea.ensure((size_t)std::max<int64_t>(nt, 1) * 8));
-    launch_mc_twrite(st, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), c->aedge.as<int>(), c->idx.as<int>(),
-                     c->idx_gen, c->atoff.as<int>(), c->verts.as<double>(), c->tris.as<int>(), c->tarea.as<double>(), 0);
+    launch_mc_twrite(st, g, (int)nact, c->anode.as<int>(), c->acase.as<uint8_t>(), vm, c->atoff.as<int>(),
+                     c->verts.as<double>(), c->tris.as<int>(), c->tarea.as<double>(), 0);
     const int64_t per = sum_partials_count(nt);
     HIP_TRY(c, c->partial.ensure((size_t)std::max<int64_t>(per, 1) * 8));
     launch_sum_blocks(st, c->tarea.as<double>(), nt, c->partial.as<double>());

@@ -172,29 +172,46 @@ struct GatherSrc {
 void launch_gather(hipStream_t, const GatherSrc &, int *dst);
 void mc_upload_tables(hipStream_t);
 void launch_vtx_count(hipStream_t, const DevGrid &, const DevRays &, const int[3], const uint32_t *, int *);
-// Vertex index slots idx[3 * node + direction] hold gen + v for vertex v of
-// the current pass; anything else reads as no vertex (gen <= 2^31 - 1, and
-// every id of earlier passes is below gen).
-// vslot[v] receives the index slot of vertex v, vhit[v] the index of its
-// depth in R.t / R.pid; vertices v >= vcap are dropped (vcap bounds the
-// total; the host checks it)
-void launch_vtx_write(hipStream_t, const DevGrid &, const DevRays &, const int[3], const uint32_t *, const int *,
-                      double *, int *idx, uint32_t gen, int64_t *vslot, int64_t *vhit, int64_t vcap);
+// The analytic vertices of edge ray r (panel * N^2 + n * N + m) are ids
+// offs[r] .. offs[r] + counts[r] - 1 (counts from launch_vtx_count, offs its
+// exclusive scan), in increasing node order along the ray.  vbits: for word
+// w = ia / 64 of the ray (W = (N + 63) / 64 words), vbits[w * 3N^2 + r] has
+// bit ia % 64 set when node ia along the ray has a vertex, and vpre[same]
+// counts the ray's vertices in the words before w.  vslot[v] receives the
+// vertex's edge (3 * node + direction), vhit[v] the index of its depth in
+// R.t / R.pid; vertices v >= vcap are dropped (vcap bounds the total; the
+// host checks it)
+void launch_vtx_write(hipStream_t, const DevGrid &, const DevRays &, const int[3], const uint32_t *, const int *offs,
+                      double *, uint64_t *vbits, int *vpre, int64_t *vslot, int64_t *vhit, int64_t vcap);
 int64_t mc_num_segments(const DevGrid &);
 void launch_mc_classify(hipStream_t, const DevGrid &, const uint32_t *bits, uint32_t *segmask, int *segcnt);
 void launch_mc_compact(hipStream_t, const DevGrid &, const uint32_t *bits, const uint32_t *segmask, const int *segoff,
                        int *anode, uint8_t *acase);
-// aedge[12 q + e]: the vertex index of edge e of active cube q (-1: none yet),
-// read once here from idx so that the later passes read it coalesced
-void launch_mc_counts(hipStream_t, const DevGrid &, int nact, const int *anode, const uint8_t *acase, const int *idx,
-                      uint32_t gen, int *nv, int *nt, int *aedge);
+// Where the marching cubes find the vertex of a cube edge: the analytic
+// vertices by edge ray (launch_vtx_write's offs, vbits and vpre; vbits ==
+// nullptr for a scalar field, which has none), and the mid points through
+// the active index of the cube that adds them (the segment masks of
+// launch_mc_classify and their exclusive scan), its mid-point mask and offset.
+struct McVertexMap {
+    const int *offs;
+    const uint64_t *vbits;
+    const int *vpre;
+    int N;
+    const uint32_t *segmask;
+    const int *segoff;
+    const uint8_t *amask;  // bit 0, 1, 2: the cube adds a mid point on edge 5, 6, 10
+    const int *avoff;      // exclusive scan of the mid points per active cube
+    int base;              // id of the first mid point
+};
+// per active cube: amask (its mid points), nv (their count) and nt (its triangles)
+void launch_mc_counts(hipStream_t, const DevGrid &, int nact, const int *anode, const uint8_t *acase,
+                      const McVertexMap &, uint8_t *amask, int *nv, int *nt);
 // vorig[v - base] = 4 * (active cube) + (0, 1, 2 for edge 5, 6, 10) of each mid point
 // field == nullptr: edge mid points; else vertexInterp at iso on the scalar
 // field (one double per node)
-// nadd[q]: the mid points cube q adds (k_mc_counts' nv)
 void launch_mc_vwrite(hipStream_t, const DevGrid &, int nact, const int *anode, const uint8_t *acase,
-                      const int *aedge, const int *nadd, int *idx, uint32_t gen, const int *voff, int base, double *verts,
-                      int64_t *vslot, int *vorig, const double *field = nullptr, double iso = 0.);
+                      const McVertexMap &, double *verts, int64_t *vslot, int *vorig, const double *field = nullptr,
+                      double iso = 0.);
 // vertex normals: analytic at the depths the vertices come from (the ray
 // lists must carry patch ids), mid points from their adjacent triangles
 struct NormalArgs {
@@ -212,8 +229,8 @@ void launch_normals(hipStream_t, const DevGrid &, const DevPatches &, const DevR
 // reverse: triangles written (t2, t1, t0) as for the analytic corner bitmap;
 // the scalar-field marching cubes keeps (t0, t1, t2)
 void launch_mc_twrite(hipStream_t, const DevGrid &, int nact, const int *anode, const uint8_t *acase,
-                      const int *aedge, const int *idx, uint32_t gen, const int *toff, const double *verts,
-                      int *tris, double *tarea, int reverse = 1);
+                      const McVertexMap &, const int *toff, const double *verts, int *tris, double *tarea,
+                      int reverse = 1);
 
 // blobby scalar field (BlobbySurface::build)
 struct BlobbyArgs {

@@ -19,15 +19,13 @@
 
 namespace nsg {
 
-// Vertex index slots (idx[3 * node + direction]) hold gen + v for vertex v of
-// the current pass (v < 2^31, gen <= 2^31 - 1); any other content (the ids of
-// earlier passes, all below gen, or the initial 0xffffffff) reads as "no
-// vertex", so the slots are never cleared between passes.
-__device__ __forceinline__ int vid(int raw, uint32_t gen) {
-    const uint32_t d = (uint32_t)raw - gen;
-    return d < 0x80000000u ? (int)d : -1;
-}
-__device__ __forceinline__ int vtag(int v, uint32_t gen) { return (int)(gen + (uint32_t)v); }
+// No node-indexed table of vertex ids: an edge's analytic vertex is found on
+// its edge ray (the ray's scanned vertex offset plus the vertex's rank among
+// the ray's vertices, which are in node order, from a bitmap of the ray's
+// nodes and per-word prefix counts: three independent loads), and a mid point
+// through the active index of the cube that adds it (segment mask and scan).
+// A table of three ids per node (12 bytes a node) was written and read at
+// random in HBM; the bitmaps take 3 N^2 (N + 63) / 64 words of 12 bytes.
 
 __device__ __forceinline__ int bit_at(const uint32_t *bits, int64_t f, int64_t nvox) {
     if (f < 0 || f >= nvox) return 1;
@@ -38,8 +36,9 @@ struct EdgeSlots {
     int s[3];  // edge-ray slots of panels 0 (x), 1 (z), 2 (y)
 };
 
-// Walk the vertex records of one edge ray.  emit(q, dir, ix, iy, iz, px, py, pz)
-// is called for every kept record in order; returns the count.  One instance
+// Walk the vertex records of one edge ray.  emit(q, dir, node, px, py, pz,
+// depth, ia) is called for every kept record in order (ia: the node's index
+// along the ray, increasing); returns the count.  One instance
 // per panel (0: x rays, 1: z rays, 2: y rays), so the grid arrays and extents
 // it reads are fixed at compile time.
 template <int PANEL, class Emit>
@@ -103,7 +102,7 @@ __device__ int walk_edge_ray_p(const DevGrid &g, const DevRays &R, const uint32_
             if (PANEL == 0) { px = lim; py = p0; pz = p1; }
             else if (PANEL == 1) { px = p0; py = p1; pz = lim; }
             else { px = p0; py = lim; pz = p1; }
-            emit(count, d, f, px, py, pz, q + h);
+            emit(count, d, f, px, py, pz, q + h, ia);
             ++count;
         }
     }
@@ -125,7 +124,7 @@ __global__ void __launch_bounds__(256) k_vtx_count(DevGrid g, DevRays R, EdgeSlo
     if (r >= 3 * NN) return;
     const int panel = (int)(r / NN);
     counts[r] = walk_edge_ray(g, R, bits, panel, es.s[panel], r - panel * NN,
-                              [](int, int, int64_t, double, double, double, int) {});
+                              [](int, int, int64_t, double, double, double, int, int) {});
 }
 
 // The vertices of a wave's rays are one contiguous range from lane 0's
@@ -136,7 +135,7 @@ __global__ void __launch_bounds__(256) k_vtx_count(DevGrid g, DevRays R, EdgeSlo
 constexpr int kVtxStage = 192;
 
 __global__ void __launch_bounds__(256) k_vtx_write(DevGrid g, DevRays R, EdgeSlots es, const uint32_t *bits,
-                                                   const int *offs, double *verts, int *idx, uint32_t gen,
+                                                   const int *offs, double *verts, uint64_t *vbits, int *vpre,
                                                    int64_t *vslot, int64_t *vhit, int64_t vcap) {
     __shared__ double sv[4][3 * kVtxStage];
     __shared__ int64_t ss[4][kVtxStage];
@@ -150,13 +149,41 @@ __global__ void __launch_bounds__(256) k_vtx_write(DevGrid g, DevRays R, EdgeSlo
     const int base = valid ? offs[r] : 0;
     const int wbase = __shfl(base, 0, 64);
     int cnt = 0;
+    // the ray's node bitmap (word-major, so that a wave's stores of one word
+    // are contiguous): up to kRegWords words are kept in registers and stored
+    // by the whole wave after the walk; longer rays store each word as the
+    // walk leaves it
+    constexpr int kRegWords = 8;
+    const int W = (R.N + 63) >> 6;
+    const bool in_regs = W <= kRegWords;
+    const int64_t NR = 3 * NN;
+    uint64_t wb[kRegWords];
+#pragma unroll
+    for (int u = 0; u < kRegWords; ++u) wb[u] = 0;
+    int cw = 0, cpre = 0;
+    uint64_t cbits = 0;
+    auto flush_to = [&](int w) {
+        for (; cw < w; ++cw) {
+            vbits[(int64_t)cw * NR + r] = cbits;
+            vpre[(int64_t)cw * NR + r] = cpre;
+            cpre += __popcll(cbits);
+            cbits = 0;
+        }
+    };
     if (valid) {
         const int panel = (int)(r / NN);
         cnt = walk_edge_ray(g, R, bits, panel, es.s[panel], r - panel * NN,
-                            [&](int q, int d, int64_t f, double px, double py, double pz, int hit) {
+                            [&](int q, int d, int64_t f, double px, double py, double pz, int hit, int ia) {
                                 const int v = base + q;
+                                if (in_regs) {
+#pragma unroll
+                                    for (int u = 0; u < kRegWords; ++u)
+                                        if (u == (ia >> 6)) wb[u] |= 1ull << (ia & 63);
+                                } else {
+                                    flush_to(ia >> 6);
+                                    cbits |= 1ull << (ia & 63);
+                                }
                                 if (v >= vcap) return;
-                                idx[3 * f + d] = vtag(v, gen);
                                 // the depth this vertex comes from, for its analytic normal
                                 const int64_t h = ((int64_t)es.s[panel] * NN + (r - panel * NN)) * R.K + hit;
                                 const int so = v - wbase;
@@ -174,6 +201,17 @@ __global__ void __launch_bounds__(256) k_vtx_write(DevGrid g, DevRays R, EdgeSlo
                                     vhit[v] = h;
                                 }
                             });
+        if (!in_regs) flush_to(W);  // the ray's last words
+    }
+    if (in_regs && valid) {
+        int pre = 0;
+#pragma unroll
+        for (int u = 0; u < kRegWords; ++u) {
+            if (u >= W) break;
+            vbits[(int64_t)u * NR + r] = wb[u];
+            vpre[(int64_t)u * NR + r] = pre;
+            pre += __popcll(wb[u]);
+        }
     }
     // staged: v in [wbase, wbase + n), below vcap and the stage's size
     const int64_t wend = (int64_t)__shfl(base + cnt, 63 - __builtin_clzll(vm), 64);
@@ -204,9 +242,8 @@ __device__ __forceinline__ bool edge_on(int c, int e) {
     return ((c >> a) & 1) != ((c >> b) & 1);
 }
 
-// edge e of the cube at node f -> vertex index slot (getTriangles): the three
-// edge directions of a node are interleaved, idx[3*node + direction], so the
-// twelve edges of a cube lie in four short runs of memory
+// edge e of the cube at node f -> the edge's key 3 * node + direction
+// (getTriangles' edge numbering), the vslot of a vertex on it
 __device__ __forceinline__ int64_t edge_slot(int e, int64_t f, int64_t sy, int64_t sz, int64_t) {
     switch (e) {
         case 0: return 3 * f + 0;
@@ -329,52 +366,101 @@ __global__ void __launch_bounds__(256) k_mc_compact(DevGrid g, SegIter si, const
     }
 }
 
-// per active cube: missing mid-point vertices it adds (edges 5, 6, 10 without
-// an analytic vertex) and triangles it emits (all three vertices present after
-// the mid points are added)
+// The analytic vertex of edge e of cube (i, j, k), or -1: the edge's ray
+// (panels 0, 1, 2 hold the x, z and y edge rays, ray n * N + m) and the
+// node's index ia along it; its rank on the ray from the ray's bitmap.
+__device__ __forceinline__ int analytic_vertex(const McVertexMap &vm, int e, int i, int j, int k) {
+    if (!vm.vbits) return -1;
+    int panel, m, n, ia;
+    switch (e) {
+        case 0: panel = 0; m = j; n = k; ia = i; break;
+        case 1: panel = 2; m = i + 1; n = k; ia = j; break;
+        case 2: panel = 0; m = j + 1; n = k; ia = i; break;
+        case 3: panel = 2; m = i; n = k; ia = j; break;
+        case 4: panel = 0; m = j; n = k + 1; ia = i; break;
+        case 5: panel = 2; m = i + 1; n = k + 1; ia = j; break;
+        case 6: panel = 0; m = j + 1; n = k + 1; ia = i; break;
+        case 7: panel = 2; m = i; n = k + 1; ia = j; break;
+        case 8: panel = 1; m = i; n = j; ia = k; break;
+        case 9: panel = 1; m = i + 1; n = j; ia = k; break;
+        case 10: panel = 1; m = i + 1; n = j + 1; ia = k; break;
+        default: panel = 1; m = i; n = j + 1; ia = k; break;
+    }
+    const int64_t r = ((int64_t)panel * vm.N + n) * vm.N + m;
+    const int64_t wi = (int64_t)(ia >> 6) * (3 * (int64_t)vm.N * vm.N) + r;
+    const uint64_t word = vm.vbits[wi];
+    const int pre = vm.vpre[wi], off = vm.offs[r];
+    const uint64_t below = (1ull << (ia & 63)) - 1ull;
+    return ((word >> (ia & 63)) & 1ull) ? off + pre + __popcll(word & below) : -1;
+}
+
+// node (i, j, k) of a flat index below 2^31 (the active list holds int32
+// nodes): 32-bit divisions, a fraction of the cost of 64-bit ones
+__device__ __forceinline__ void cube_ijk(const DevGrid &g, int f, int &i, int &j, int &k) {
+    const int row = f / g.nx;
+    i = f - row * g.nx;
+    k = row / g.ny;
+    j = row - k * g.ny;
+}
+
+// index of cube (i, j, k) in the active list, or -1 when it is not active
+__device__ __forceinline__ int active_index(const DevGrid &g, const McVertexMap &vm, int i, int j, int k) {
+    const int nseg_row = (g.nx - 2 + 31) / 32;
+    const int64_t s = ((int64_t)(k - 1) * (g.ny - 2) + (j - 1)) * nseg_row + ((i - 1) >> 5);
+    const uint32_t m = vm.segmask[s];
+    const int b = (i - 1) & 31;
+    if (!((m >> b) & 1u)) return -1;
+    return vm.segoff[s] + __popc(m & ((1u << b) - 1u));
+}
+
+// The mid point on edge e of cube (i, j, k), or -1: it is added by the cube
+// that owns the edge (where it is edge 5, 6 or 10, see owner_in_range), in
+// the order 5, 6, 10 from that cube's offset.
+__device__ __forceinline__ int midpoint_vertex(const DevGrid &g, const McVertexMap &vm, int e, int i, int j, int k) {
+    int io = i, jo = j, ko = k, bit;
+    switch (e) {
+        case 5: bit = 0; break;
+        case 6: bit = 1; break;
+        case 10: bit = 2; break;
+        case 7: io = i - 1; bit = 0; break;
+        case 1: ko = k - 1; bit = 0; break;
+        case 3: io = i - 1; ko = k - 1; bit = 0; break;
+        case 4: jo = j - 1; bit = 1; break;
+        case 2: ko = k - 1; bit = 1; break;
+        case 0: jo = j - 1; ko = k - 1; bit = 1; break;
+        case 9: jo = j - 1; bit = 2; break;
+        case 11: io = i - 1; bit = 2; break;
+        default: io = i - 1; jo = j - 1; bit = 2; break;
+    }
+    if (io < 1 || jo < 1 || ko < 1) return -1;
+    const int q = active_index(g, vm, io, jo, ko);
+    if (q < 0) return -1;
+    const int mk = vm.amask[q];
+    if (!((mk >> bit) & 1)) return -1;
+    return vm.base + vm.avoff[q] + __popc((unsigned)mk & ((1u << bit) - 1u));
+}
+
+// per active cube: the mid points it adds (edges 5, 6, 10 that are active
+// and have no analytic vertex) as a mask and a count, and the triangles it
+// emits (all three vertices present after the mid points are added)
 __global__ void __launch_bounds__(256) k_mc_counts(DevGrid g, int nact, const int *anode, const uint8_t *acase,
-                                                   const int *idx, uint32_t gen, int *nv, int *nt, int *aedge) {
+                                                   McVertexMap vm, uint8_t *amask, int *nv, int *nt) {
     const int q = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    // a full wave's records (3 KB, contiguous) go out through LDS as three
-    // contiguous 1 KB stores: stored per lane, each store instruction left
-    // partial lines at a 48-byte stride
-    __shared__ int4 stage[4][3 * 64];
-    const bool full = __ballot(q < nact) == ~0ull;
     if (q >= nact) return;
-    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (int64_t)g.nx * g.ny;
+    const int64_t sz = (int64_t)g.nx * g.ny;
     const int64_t f = anode[q];
     const int c = acase[q];
-    const int i = (int)(f % g.nx), j = (int)((f / g.nx) % g.ny), k = (int)(f / sz);
-    // the twelve edge slots of an active cube (1 <= i, j, k <= n - 2) are in the grid
-    int ids[12];
-    unsigned have = 0;  // bit e: edge e has a vertex
+    int i, j, k;
+    cube_ijk(g, (int)f, i, j, k);
+    // only active edges carry vertices, and only they appear in triangles
+    unsigned have = 0;  // bit e: edge e has an analytic vertex
 #pragma unroll
-    for (int e = 0; e < 12; ++e) {
-        ids[e] = vid(idx[edge_slot(e, f, sy, sz, nvox)], gen);
-        have |= (ids[e] != -1 ? 1u : 0u) << e;
-    }
-    if (full) {
-        // a wave's LDS accesses are performed in order, so the reads see the
-        // other lanes' writes; the wave barrier keeps the compiler from moving them
-        const int lane = threadIdx.x & 63;
-        int4 *w = stage[threadIdx.x >> 6];
-        w[3 * lane + 0] = make_int4(ids[0], ids[1], ids[2], ids[3]);
-        w[3 * lane + 1] = make_int4(ids[4], ids[5], ids[6], ids[7]);
-        w[3 * lane + 2] = make_int4(ids[8], ids[9], ids[10], ids[11]);
-        __builtin_amdgcn_wave_barrier();
-        int4 *dst = reinterpret_cast<int4 *>(aedge + 12 * (int64_t)(q - lane));
-        for (int v = 0; v < 3; ++v) dst[lane + 64 * v] = w[lane + 64 * v];
-    } else {
-        int4 *rec = reinterpret_cast<int4 *>(aedge + 12 * (int64_t)q);
-        rec[0] = make_int4(ids[0], ids[1], ids[2], ids[3]);
-        rec[1] = make_int4(ids[4], ids[5], ids[6], ids[7]);
-        rec[2] = make_int4(ids[8], ids[9], ids[10], ids[11]);
-    }
-    int v = 0;
-    v += edge_on(c, 5) && ids[5] == -1;
-    v += edge_on(c, 6) && ids[6] == -1;
-    v += edge_on(c, 10) && ids[10] == -1;
-    nv[q] = v;
+    for (int e = 0; e < 12; ++e)
+        if (edge_on(c, e) && analytic_vertex(vm, e, i, j, k) != -1) have |= 1u << e;
+    const int mk = (edge_on(c, 5) && !((have >> 5) & 1u) ? 1 : 0) | (edge_on(c, 6) && !((have >> 6) & 1u) ? 2 : 0) |
+                   (edge_on(c, 10) && !((have >> 10) & 1u) ? 4 : 0);
+    amask[q] = (uint8_t)mk;
+    nv[q] = __popc((unsigned)mk);
     const uint64_t w = c_tri_cases[c];
     int t = 0;
     for (int tr = 0; tr < 5; ++tr) {
@@ -400,24 +486,23 @@ __device__ __forceinline__ void vertex_interp(double iso, const double p1[3], co
 }
 
 __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const int *anode, const uint8_t *acase,
-                                                   const int *aedge, const int *nadd, int *idx, uint32_t gen,
-                                                   const int *voff, int base, double *verts, int64_t *vslot,
-                                                   int *vorig, const double *field, double iso) {
+                                                   McVertexMap vm, double *verts, int64_t *vslot, int *vorig,
+                                                   const double *field, double iso) {
     const int q = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    // cubes that add no mid point (k_mc_counts' count) read nothing more
-    if (q >= nact || nadd[q] == 0) return;
+    // cubes that add no mid point read nothing more
+    if (q >= nact) return;
+    const int mk = vm.amask[q];
+    if (mk == 0) return;
     const int64_t nvox = (int64_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (int64_t)g.nx * g.ny;
     const int64_t f = anode[q];
-    const int c = acase[q];
-    const int i = (int)(f % g.nx), j = (int)((f / g.nx) % g.ny), k = (int)(f / sz);
-    int v = base + voff[q];
-    // the slots of edges 5, 6, 10 belong to this cube only: no other lane writes them
+    int i, j, k;
+    cube_ijk(g, (int)f, i, j, k);
+    const int base = vm.base;
+    int v = base + vm.avoff[q];
+    // the edges 5, 6, 10 of a cube belong to it only: no other lane writes them
     const int64_t s5 = edge_slot(5, f, sy, sz, nvox), s6 = edge_slot(6, f, sy, sz, nvox),
                   s10 = edge_slot(10, f, sy, sz, nvox);
-    const int *rec = aedge + 12 * (int64_t)q;
-    const bool m5 = edge_on(c, 5) && rec[5] == -1;
-    const bool m6 = edge_on(c, 6) && rec[6] == -1;
-    const bool m10 = edge_on(c, 10) && rec[10] == -1;
+    const bool m5 = mk & 1, m6 = mk & 2, m10 = mk & 4;
     if (field) {
         // corner positions x_i +- hside (the mulTable corners, tools.h:154-162)
         // and field values of the cube (getVertices :8143-8169)
@@ -429,20 +514,17 @@ __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const in
         if (m5) {
             vertex_interp(iso, c5, c6, f5, f6, verts + 3 * (int64_t)v);
             vslot[v] = s5;
-            vorig[v - base] = 4 * q + 0;
-            idx[s5] = vtag(v++, gen);
+            vorig[v++ - base] = 4 * q + 0;
         }
         if (m6) {
             vertex_interp(iso, c6, c7, f6, f7, verts + 3 * (int64_t)v);
             vslot[v] = s6;
-            vorig[v - base] = 4 * q + 1;
-            idx[s6] = vtag(v++, gen);
+            vorig[v++ - base] = 4 * q + 1;
         }
         if (m10) {
             vertex_interp(iso, c2, c6, f2, f6, verts + 3 * (int64_t)v);
             vslot[v] = s10;
-            vorig[v - base] = 4 * q + 2;
-            idx[s10] = vtag(v++, gen);
+            vorig[v++ - base] = 4 * q + 2;
         }
         return;
     }
@@ -451,39 +533,33 @@ __global__ void __launch_bounds__(256) k_mc_vwrite(DevGrid g, int nact, const in
         verts[3 * (int64_t)v + 1] = g.y[j];
         verts[3 * (int64_t)v + 2] = g.z[k + 1] - g.hside;
         vslot[v] = s5;
-        vorig[v - base] = 4 * q + 0;
-        idx[s5] = vtag(v++, gen);
+        vorig[v++ - base] = 4 * q + 0;
     }
     if (m6) {
         verts[3 * (int64_t)v] = g.x[i];
         verts[3 * (int64_t)v + 1] = g.y[j + 1] - g.hside;
         verts[3 * (int64_t)v + 2] = g.z[k + 1] - g.hside;
         vslot[v] = s6;
-        vorig[v - base] = 4 * q + 1;
-        idx[s6] = vtag(v++, gen);
+        vorig[v++ - base] = 4 * q + 1;
     }
     if (m10) {
         verts[3 * (int64_t)v] = g.x[i + 1] - g.hside;
         verts[3 * (int64_t)v + 1] = g.y[j + 1] - g.hside;
         verts[3 * (int64_t)v + 2] = g.z[k];
         vslot[v] = s10;
-        vorig[v - base] = 4 * q + 2;
-        idx[s10] = vtag(v++, gen);
+        vorig[v++ - base] = 4 * q + 2;
     }
 }
 
-// Edge vertex indices from the cube's record; an edge still at -1 there may
-// have received a mid point from the cube that owns it (k_mc_vwrite), so
-// only those are looked up in idx again.
-__device__ __forceinline__ int edge_vertex(const int *rec, const int *idx, uint32_t gen, int e, int64_t f, int64_t sy,
-                                           int64_t sz, int64_t nvox) {
-    const int v = rec[e];
-    return v != -1 ? v : vid(idx[edge_slot(e, f, sy, sz, nvox)], gen);
+// The vertex of edge e: the analytic one, else the mid point its owner adds.
+__device__ __forceinline__ int edge_vertex(const DevGrid &g, const McVertexMap &vm, int e, int i, int j, int k) {
+    const int v = analytic_vertex(vm, e, i, j, k);
+    return v != -1 ? v : midpoint_vertex(g, vm, e, i, j, k);
 }
 
 __global__ void __launch_bounds__(256) k_mc_twrite(DevGrid g, int nact, const int *anode, const uint8_t *acase,
-                                                   const int *aedge, const int *idx, uint32_t gen, const int *toff,
-                                                   const double *verts, int *tris, double *tarea, int reverse) {
+                                                   McVertexMap vm, const int *toff, const double *verts, int *tris,
+                                                   double *tarea, int reverse) {
     const int q = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     // The valid lanes of a wave are a prefix and their triangles one
     // contiguous range from lane 0's offset (at most 5 per cube): they are
@@ -493,21 +569,22 @@ __global__ void __launch_bounds__(256) k_mc_twrite(DevGrid g, int nact, const in
     __shared__ double sarea[4][5 * 64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const bool valid = q < nact;
-    const unsigned long long vm = __ballot(valid);
-    if (vm == 0) return;
+    const unsigned long long wmask = __ballot(valid);
+    if (wmask == 0) return;
     const int base = __shfl(valid ? toff[q] : 0, 0, 64);
-    const int64_t nvox = (int64_t)g.nx * g.ny * g.nz, sy = g.nx, sz = (int64_t)g.nx * g.ny;
+    const int64_t sz = (int64_t)g.nx * g.ny;
     const int64_t f = valid ? anode[q] : 0;
+    int i, j, k;
+    cube_ijk(g, (int)f, i, j, k);
     const uint64_t w = valid ? c_tri_cases[acase[q]] : ~0ull;
     int o = valid ? toff[q] : base;
     for (int t = 0; t < 5; ++t) {
         const int e0 = (int)((w >> (12 * t)) & 15), e1 = (int)((w >> (12 * t + 4)) & 15),
                   e2 = (int)((w >> (12 * t + 8)) & 15);
         if (e0 == 15) break;
-        const int *rec = aedge + 12 * (int64_t)q;
-        const int v0 = edge_vertex(rec, idx, gen, e0, f, sy, sz, nvox);
-        const int v1 = edge_vertex(rec, idx, gen, e1, f, sy, sz, nvox);
-        const int v2 = edge_vertex(rec, idx, gen, e2, f, sy, sz, nvox);
+        const int v0 = edge_vertex(g, vm, e0, i, j, k);
+        const int v1 = edge_vertex(g, vm, e1, i, j, k);
+        const int v2 = edge_vertex(g, vm, e2, i, j, k);
         if (v0 == -1 || v1 == -1 || v2 == -1) continue;
         const int so = o - base;  // < 5 * 64: at most five triangles per lane before it
         stri[wv][3 * so + 0] = reverse ? v2 : v0;
@@ -525,7 +602,7 @@ __global__ void __launch_bounds__(256) k_mc_twrite(DevGrid g, int nact, const in
         ++o;
     }
     // the wave's end: the last valid lane's
-    const int n = __shfl(o, 63 - __builtin_clzll(vm), 64) - base;
+    const int n = __shfl(o, 63 - __builtin_clzll(wmask), 64) - base;
     __builtin_amdgcn_wave_barrier();
     for (int x = lane; x < 3 * n; x += 64) tris[3 * (int64_t)base + x] = stri[wv][x];
     for (int x = lane; x < n; x += 64) tarea[base + x] = sarea[wv][x];
@@ -545,11 +622,11 @@ void launch_vtx_count(hipStream_t s, const DevGrid &g, const DevRays &R, const i
 }
 
 void launch_vtx_write(hipStream_t s, const DevGrid &g, const DevRays &R, const int eslots[3], const uint32_t *bits,
-                      const int *offs, double *verts, int *idx, uint32_t gen, int64_t *vslot, int64_t *vhit,
+                      const int *offs, double *verts, uint64_t *vbits, int *vpre, int64_t *vslot, int64_t *vhit,
                       int64_t vcap) {
     EdgeSlots es{{eslots[0], eslots[1], eslots[2]}};
     const int64_t n = 3 * (int64_t)R.N * R.N;
-    hipLaunchKernelGGL(k_vtx_write, dim3(nblk(n)), dim3(256), 0, s, g, R, es, bits, offs, verts, idx, gen, vslot,
+    hipLaunchKernelGGL(k_vtx_write, dim3(nblk(n)), dim3(256), 0, s, g, R, es, bits, offs, verts, vbits, vpre, vslot,
                        vhit, vcap);
 }
 
@@ -579,26 +656,25 @@ void launch_mc_compact(hipStream_t s, const DevGrid &g, const uint32_t *bits, co
 }
 
 void launch_mc_counts(hipStream_t s, const DevGrid &g, int nact, const int *anode, const uint8_t *acase,
-                      const int *idx, uint32_t gen, int *nv, int *nt, int *aedge) {
+                      const McVertexMap &vm, uint8_t *amask, int *nv, int *nt) {
     if (nact)
-        hipLaunchKernelGGL(k_mc_counts, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, idx, gen, nv, nt,
-                           aedge);
+        hipLaunchKernelGGL(k_mc_counts, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, vm, amask, nv, nt);
 }
 
 void launch_mc_vwrite(hipStream_t s, const DevGrid &g, int nact, const int *anode, const uint8_t *acase,
-                      const int *aedge, const int *nadd, int *idx, uint32_t gen, const int *voff, int base, double *verts,
-                      int64_t *vslot, int *vorig, const double *field, double iso) {
+                      const McVertexMap &vm, double *verts, int64_t *vslot, int *vorig, const double *field,
+                      double iso) {
     if (nact)
-        hipLaunchKernelGGL(k_mc_vwrite, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, aedge, nadd, idx, gen,
-                           voff, base, verts, vslot, vorig, field, iso);
+        hipLaunchKernelGGL(k_mc_vwrite, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, vm, verts, vslot,
+                           vorig, field, iso);
 }
 
 void launch_mc_twrite(hipStream_t s, const DevGrid &g, int nact, const int *anode, const uint8_t *acase,
-                      const int *aedge, const int *idx, uint32_t gen, const int *toff, const double *verts,
-                      int *tris, double *tarea, int reverse) {
+                      const McVertexMap &vm, const int *toff, const double *verts, int *tris, double *tarea,
+                      int reverse) {
     if (nact)
-        hipLaunchKernelGGL(k_mc_twrite, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, aedge, idx, gen,
-                           toff, verts, tris, tarea, reverse);
+        hipLaunchKernelGGL(k_mc_twrite, dim3(nblk(nact)), dim3(256), 0, s, g, nact, anode, acase, vm, toff, verts,
+                           tris, tarea, reverse);
 }
 
 }  // namespace nsg
