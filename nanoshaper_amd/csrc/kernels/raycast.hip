@@ -713,9 +713,13 @@ enum { PC_CAND, PC_DISC, PC_SPHERE, PC_TORUS, PC_ROOTS, PC_HITS, PC_FEAS_ITEMS, 
 // roots from the closed form (kRootsClosed), the mixed-precision solver
 // (kRootsMixed) or Sturm bisection (kRootsSturm).  kCount: instead of hits,
 // per-item path counts.
+// Five waves per SIMD for the closed-form roots: capped at 96 VGPRs that item
+// needs no spills, and 5 waves hide a little more of its memory latency than
+// the 4 that its unconstrained 102 VGPRs allow (0.424 -> 0.419 ms per launch
+// at 325^3).  The mixed-precision and Sturm items would spill: no cap.
 template <bool kCount, int kRoots>
-__global__ void __launch_bounds__(64) k_intersect_torus(DevGrid g, DevPatches P, DevRays R, double probe,
-                                                        unsigned long long *part) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kRoots == kRootsClosed ? 5 : 1)))
+k_intersect_torus(DevGrid g, DevPatches P, DevRays R, double probe, unsigned long long *part) {
     // workgroup = patch rank * nslots + slot: the patches of a class come
     // largest first, and every slot starts on them together
     const int rank = (int)(blockIdx.x / R.nslots);
